@@ -1,0 +1,1932 @@
+/*
+ * agx_oracle.c -- CPU FP64 restatement of the reference algorithm for the MHD +
+ * multigrid self-gravity hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by tests/, smoke() and bench.py's
+ * cpu_baseline leg; never by the product library.  Built with -ffp-contract=off so
+ * that every expression rounds exactly as written; operand groupings follow the
+ * reference so the result is the reference's IEEE sequence.
+ *
+ * Citations are file:line in the reference's src/ tree.
+ *
+ * Parity pinning (how this restatement is checked; see tests/test_oracle_*.py):
+ *   - Riemann solvers: consistency F(U,U) = F_phys(U) for HLLD/HLLE, HLLD exact
+ *     resolution of an isolated contact, mirror symmetry of the flux.
+ *   - PLM: exact on linear data, zero slope at extrema.
+ *   - CT: div B of the face field is preserved to round-off.
+ *   - Full VL2+PLM+HLLD+CT: the reference's own regression thresholds for 3D linear
+ *     waves (tst/regression/scripts/tests/mhd/mhd_linwave.py:97-162), second-order
+ *     convergence, and bit-identical L- and R-going fast-wave errors (:159-162).
+ *   - Uniform state: zero update for any solver; conservation to round-off.
+ *   - Multigrid: closed-form Poisson solution (pgen/poisson.cpp:85-93), brute-force
+ *     FFT solve of the discrete 7-point Laplacian, linearity of every operator.
+ * No golden vectors exist in the reference for this path (tst/regression/data holds
+ * only SR-shock and MRI files), and the reference cannot be built here without its
+ * generated src/defs.hpp; the checks above are what pin this file.
+ */
+#include "agx_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NHYDRO 5
+#define NWAVE 7
+#define IDN 0
+#define IVX 1
+#define IVY 2
+#define IVZ 3
+#define IEN 4
+#define IPR 4
+#define IBY 5
+#define IBZ 6
+#define IB1 0
+#define IB2 1
+#define IB3 2
+#define SQR(x) ((x) * (x))
+#define SMALL_NUMBER 1.0e-8
+#define TINY_NUMBER 1.0e-20
+
+enum {
+  ARR_U = 0, ARR_U1, ARR_W, ARR_BCC, ARR_B1F, ARR_B2F, ARR_B3F, ARR_R1F, ARR_R2F,
+  ARR_R3F, ARR_FX1, ARR_FX2, ARR_FX3, ARR_E3X1, ARR_E2X1, ARR_E1X2, ARR_E3X2,
+  ARR_E1X3, ARR_E2X3, ARR_WX1, ARR_WX2, ARR_WX3, ARR_E1, ARR_E2, ARR_E3, ARR_PHI,
+  ARR_COUNT
+};
+
+/* ---------------------------------------------------------------- multigrid data */
+typedef struct {
+  int nlev;          /* levels, 0 = coarsest */
+  int n1[24], n2[24], n3[24];   /* active cells per level */
+  double *u[24], *src[24], *def[24];
+  double rdx;        /* cell size of the finest level */
+} mglevels;
+
+struct orc_ctx {
+  orc_desc d;
+  int nb1, nb2, nb3, nblocks;
+  int n1, n2, n3, ng;
+  int is, ie, js, je, ks, ke;
+  int64_t ncell, nf1, nf2, nf3, ne1, ne2, ne3;
+  double *arr[ARR_COUNT];
+  int64_t arrsize[ARR_COUNT];
+  double *x1f, *x1v, *dx1f, *x2f, *x2v, *dx2f, *x3f, *x3v, *dx3f;  /* per block */
+  double *cce;       /* cell-centred E scratch, [3][ncell] per block */
+  double time, dt, tlim_hint;
+  int ncycle;
+  /* multigrid */
+  mglevels *mgb;     /* one per block */
+  mglevels mgr;      /* root grid */
+  int nmblevel, nrootlevel, ntotal, current, fmglevel, coffset;
+  double last_ave, last_norm;
+  int last_iters;
+  int mg_mode, mg_niter;
+  double mg_eps;
+};
+
+/* ---------------------------------------------------------------- helpers */
+#define CIDX(c, k, j, i) ((((int64_t)(k)) * (c)->n2 + (j)) * (c)->n1 + (i))
+#define F1IDX(c, k, j, i) ((((int64_t)(k)) * (c)->n2 + (j)) * ((c)->n1 + 1) + (i))
+#define F2IDX(c, k, j, i) ((((int64_t)(k)) * ((c)->n2 + 1) + (j)) * (c)->n1 + (i))
+#define F3IDX(c, k, j, i) ((((int64_t)(k)) * (c)->n2 + (j)) * (c)->n1 + (i))
+#define E1IDX(c, k, j, i) ((((int64_t)(k)) * ((c)->n2 + 1) + (j)) * (c)->n1 + (i))
+#define E2IDX(c, k, j, i) ((((int64_t)(k)) * (c)->n2 + (j)) * ((c)->n1 + 1) + (i))
+#define E3IDX(c, k, j, i) ((((int64_t)(k)) * ((c)->n2 + 1) + (j)) * ((c)->n1 + 1) + (i))
+
+static inline double dmin(double a, double b) { return (a < b) ? a : b; }   /* std::min */
+static inline double dmax(double a, double b) { return (a < b) ? b : a; }   /* std::max */
+
+/* ComputeMeshGeneratorX with sym_interval=true, mesh/mesh.hpp:389-405 */
+static double meshgen_x(int64_t index, int64_t nrange) {
+  int64_t noffset = index - (nrange) / 2;
+  int64_t noffset_ceil = index - (nrange + 1) / 2;
+  return (double)(noffset + noffset_ceil) / (2.0 * nrange);
+}
+/* UniformMeshGeneratorX1..3, mesh/mesh.hpp:465-488 */
+static double uniform_gen(double x, double xmin, double xmax) {
+  return 0.5 * (xmin + xmax) + (x * xmax - x * xmin);
+}
+
+/* Uniform-mesh face/centre coordinates of one block along one direction:
+ * coordinates.cpp:127-145 (faces), cartesian.cpp:26-36 (centres),
+ * mesh.cpp:1668-1690 (block extents). */
+static void block_coords(int nmesh, int nblk, int nbx, int ng, int lx, double xmin,
+                         double xmax, double *xf, double *xv, double *dxf) {
+  double bmin = (lx == 0) ? xmin : uniform_gen(meshgen_x(lx, nblk), xmin, xmax);
+  double bmax = (lx == nblk - 1) ? xmax : uniform_gen(meshgen_x(lx + 1, nblk), xmin, xmax);
+  double dx = (bmax - bmin) / nbx;
+  int il = ng, iu = ng + nbx - 1;
+  for (int i = il - ng; i <= iu + ng + 1; ++i) {
+    int64_t noffset = (int64_t)(i - il) + (int64_t)lx * nbx;
+    xf[i] = uniform_gen(meshgen_x(noffset, nmesh), xmin, xmax);
+  }
+  xf[il] = bmin;
+  xf[iu + 1] = bmax;
+  for (int i = il - ng; i <= iu + ng; ++i) dxf[i] = dx;
+  for (int i = il - ng; i <= iu + ng; ++i) xv[i] = 0.5 * (xf[i + 1] + xf[i]);
+}
+
+static void *xcalloc(size_t n) {
+  void *p = calloc(n ? n : 1, 1);
+  if (!p) abort();
+  return p;
+}
+
+/* ---------------------------------------------------------------- create */
+static void mg_alloc(mglevels *m, int nx1, int nx2, int nx3, int nlev, double rdx) {
+  m->nlev = nlev;
+  m->rdx = rdx;
+  for (int l = 0; l < nlev; ++l) {
+    int ll = nlev - 1 - l;
+    m->n1[l] = nx1 >> ll;
+    m->n2[l] = nx2 >> ll;
+    m->n3[l] = nx3 >> ll;
+    size_t sz = (size_t)(m->n1[l] + 2) * (m->n2[l] + 2) * (m->n3[l] + 2);
+    m->u[l] = (double *)xcalloc(sz * sizeof(double));
+    m->src[l] = (double *)xcalloc(sz * sizeof(double));
+    m->def[l] = (double *)xcalloc(sz * sizeof(double));
+  }
+}
+
+orc_ctx *orc_create(const orc_desc *d) {
+  orc_ctx *c = (orc_ctx *)xcalloc(sizeof(orc_ctx));
+  c->d = *d;
+  c->nb1 = d->nx1 / d->bnx1;
+  c->nb2 = d->nx2 / d->bnx2;
+  c->nb3 = d->nx3 / d->bnx3;
+  c->nblocks = c->nb1 * c->nb2 * c->nb3;
+  c->ng = d->nghost;
+  c->n1 = d->bnx1 + 2 * c->ng;
+  c->n2 = d->bnx2 + 2 * c->ng;
+  c->n3 = d->bnx3 + 2 * c->ng;
+  c->is = c->ng; c->ie = c->ng + d->bnx1 - 1;
+  c->js = c->ng; c->je = c->ng + d->bnx2 - 1;
+  c->ks = c->ng; c->ke = c->ng + d->bnx3 - 1;
+  c->ncell = (int64_t)c->n1 * c->n2 * c->n3;
+  c->nf1 = (int64_t)c->n3 * c->n2 * (c->n1 + 1);
+  c->nf2 = (int64_t)c->n3 * (c->n2 + 1) * c->n1;
+  c->nf3 = (int64_t)(c->n3 + 1) * c->n2 * c->n1;
+  c->ne1 = (int64_t)(c->n3 + 1) * (c->n2 + 1) * c->n1;
+  c->ne2 = (int64_t)(c->n3 + 1) * c->n2 * (c->n1 + 1);
+  c->ne3 = (int64_t)c->n3 * (c->n2 + 1) * (c->n1 + 1);
+  int64_t per[ARR_COUNT] = {
+      NHYDRO * c->ncell, NHYDRO * c->ncell, NHYDRO * c->ncell, 3 * c->ncell,
+      c->nf1, c->nf2, c->nf3, c->nf1, c->nf2, c->nf3,
+      NHYDRO * c->nf1, NHYDRO * c->nf2, NHYDRO * c->nf3,
+      c->nf1, c->nf1, c->nf2, c->nf2, c->nf3, c->nf3,
+      c->nf1, c->nf2, c->nf3, c->ne1, c->ne2, c->ne3, c->ncell};
+  for (int a = 0; a < ARR_COUNT; ++a) {
+    c->arrsize[a] = per[a];
+    c->arr[a] = (double *)xcalloc((size_t)per[a] * c->nblocks * sizeof(double));
+  }
+  c->cce = (double *)xcalloc((size_t)3 * c->ncell * c->nblocks * sizeof(double));
+  int nb = c->nblocks;
+  c->x1f = (double *)xcalloc(sizeof(double) * nb * (c->n1 + 1));
+  c->x1v = (double *)xcalloc(sizeof(double) * nb * (c->n1 + 1));
+  c->dx1f = (double *)xcalloc(sizeof(double) * nb * (c->n1 + 1));
+  c->x2f = (double *)xcalloc(sizeof(double) * nb * (c->n2 + 1));
+  c->x2v = (double *)xcalloc(sizeof(double) * nb * (c->n2 + 1));
+  c->dx2f = (double *)xcalloc(sizeof(double) * nb * (c->n2 + 1));
+  c->x3f = (double *)xcalloc(sizeof(double) * nb * (c->n3 + 1));
+  c->x3v = (double *)xcalloc(sizeof(double) * nb * (c->n3 + 1));
+  c->dx3f = (double *)xcalloc(sizeof(double) * nb * (c->n3 + 1));
+  for (int g = 0; g < nb; ++g) {
+    int lx1 = g % c->nb1, lx2 = (g / c->nb1) % c->nb2, lx3 = g / (c->nb1 * c->nb2);
+    block_coords(d->nx1, c->nb1, d->bnx1, c->ng, lx1, d->x1min, d->x1max,
+                 c->x1f + g * (c->n1 + 1), c->x1v + g * (c->n1 + 1), c->dx1f + g * (c->n1 + 1));
+    block_coords(d->nx2, c->nb2, d->bnx2, c->ng, lx2, d->x2min, d->x2max,
+                 c->x2f + g * (c->n2 + 1), c->x2v + g * (c->n2 + 1), c->dx2f + g * (c->n2 + 1));
+    block_coords(d->nx3, c->nb3, d->bnx3, c->ng, lx3, d->x3min, d->x3max,
+                 c->x3f + g * (c->n3 + 1), c->x3v + g * (c->n3 + 1), c->dx3f + g * (c->n3 + 1));
+  }
+  c->dt = DBL_MAX;
+  /* multigrid hierarchy: multigrid.cpp:32-144, multigrid_driver.cpp:343-349 */
+  if (d->self_gravity) {
+    int nl = 0;
+    for (int l = 0; l < 20; ++l)
+      if ((1 << l) == d->bnx1) { nl = l + 1; break; }
+    c->nmblevel = nl;
+    c->mgb = (mglevels *)xcalloc(sizeof(mglevels) * nb);
+    for (int g = 0; g < nb; ++g) {
+      int lx1 = g % c->nb1, lx2 = (g / c->nb1) % c->nb2, lx3 = g / (c->nb1 * c->nb2);
+      double bmin = (lx1 == 0) ? d->x1min : uniform_gen(meshgen_x(lx1, c->nb1), d->x1min, d->x1max);
+      double bmax = (lx1 == c->nb1 - 1) ? d->x1max
+                                         : uniform_gen(meshgen_x(lx1 + 1, c->nb1), d->x1min, d->x1max);
+      (void)lx2; (void)lx3;
+      mg_alloc(&c->mgb[g], d->bnx1, d->bnx2, d->bnx3, nl, (bmax - bmin) / (double)d->bnx1);
+    }
+    int nrl = 0;
+    for (int l = 0; l < 20; ++l)
+      if (c->nb1 % (1 << l) == 0 && c->nb2 % (1 << l) == 0 && c->nb3 % (1 << l) == 0) nrl = l + 1;
+    c->nrootlevel = nrl;
+    mg_alloc(&c->mgr, c->nb1, c->nb2, c->nb3, nrl, (d->x1max - d->x1min) / (double)c->nb1);
+    c->ntotal = nrl + nl - 1;
+    c->mg_mode = d->mg_mode;
+    c->mg_niter = d->mg_niter;
+    c->mg_eps = d->mg_threshold;
+  }
+  return c;
+}
+
+static void mg_free(mglevels *m) {
+  for (int l = 0; l < m->nlev; ++l) {
+    free(m->u[l]); free(m->src[l]); free(m->def[l]);
+  }
+}
+
+void orc_destroy(orc_ctx *c) {
+  if (!c) return;
+  for (int a = 0; a < ARR_COUNT; ++a) free(c->arr[a]);
+  free(c->cce);
+  free(c->x1f); free(c->x1v); free(c->dx1f);
+  free(c->x2f); free(c->x2v); free(c->dx2f);
+  free(c->x3f); free(c->x3v); free(c->dx3f);
+  if (c->mgb) {
+    for (int g = 0; g < c->nblocks; ++g) mg_free(&c->mgb[g]);
+    free(c->mgb);
+    mg_free(&c->mgr);
+  }
+  free(c);
+}
+
+int64_t orc_size(orc_ctx *c, int what) {
+  switch (what) {
+    case 0: return c->ncell;
+    case 1: return c->nf1;
+    case 2: return c->nf2;
+    case 3: return c->nf3;
+    case 4: return c->ne1;
+    case 5: return c->ne2;
+    case 6: return c->ne3;
+    case 7: return c->nblocks;
+    default: return -1;
+  }
+}
+
+double *orc_array(orc_ctx *c, int which) {
+  if (which < 0 || which >= ARR_COUNT) return NULL;
+  return c->arr[which];
+}
+
+void orc_set_time(orc_ctx *c, double time, double dt, int ncycle) {
+  c->time = time; c->dt = dt; c->ncycle = ncycle;
+}
+void orc_get_time(orc_ctx *c, double *out3) {
+  out3[0] = c->time; out3[1] = c->dt; out3[2] = c->ncycle;
+}
+
+void orc_coords(orc_ctx *c, int gid, int dir, double *xf, double *xv, double *dxf) {
+  int n = (dir == 0) ? c->n1 : (dir == 1 ? c->n2 : c->n3);
+  double *sf = (dir == 0) ? c->x1f : (dir == 1 ? c->x2f : c->x3f);
+  double *sv = (dir == 0) ? c->x1v : (dir == 1 ? c->x2v : c->x3v);
+  double *sd = (dir == 0) ? c->dx1f : (dir == 1 ? c->dx2f : c->dx3f);
+  memcpy(xf, sf + (int64_t)gid * (n + 1), sizeof(double) * (n + 1));
+  memcpy(xv, sv + (int64_t)gid * (n + 1), sizeof(double) * n);
+  memcpy(dxf, sd + (int64_t)gid * (n + 1), sizeof(double) * n);
+}
+
+/* neighbour block id in direction (o1,o2,o3), periodic lattice */
+static int nbr_gid(const orc_ctx *c, int g, int o1, int o2, int o3) {
+  int lx1 = g % c->nb1, lx2 = (g / c->nb1) % c->nb2, lx3 = g / (c->nb1 * c->nb2);
+  lx1 = (lx1 + o1 + c->nb1) % c->nb1;
+  lx2 = (lx2 + o2 + c->nb2) % c->nb2;
+  lx3 = (lx3 + o3 + c->nb3) % c->nb3;
+  return (lx3 * c->nb2 + lx2) * c->nb1 + lx1;
+}
+
+/* ---------------------------------------------------------------- boundaries */
+/* Same-level cell-centred ghost exchange: every ghost cell receives the neighbour's
+ * interior cell at the same physical location (BufferUtility::PackData/UnpackData,
+ * utils/buffer_utils.cpp:24-89; ranges bvals/cc/bvals_cc.cpp:201-336).  All senders
+ * pack before any receiver unpacks, so reading interiors in place is equivalent. */
+void orc_bvals_cc(orc_ctx *c, int which) {
+  int nvar = (which == ARR_PHI) ? 1 : NHYDRO;
+  double *a = c->arr[which];
+  int64_t bstride = c->arrsize[which];
+  int bx1 = c->d.bnx1, bx2 = c->d.bnx2, bx3 = c->d.bnx3;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < c->nblocks; ++g) {
+    for (int k = 0; k < c->n3; ++k) {
+      int o3 = (k < c->ks) ? -1 : (k > c->ke ? 1 : 0);
+      for (int j = 0; j < c->n2; ++j) {
+        int o2 = (j < c->js) ? -1 : (j > c->je ? 1 : 0);
+        for (int i = 0; i < c->n1; ++i) {
+          int o1 = (i < c->is) ? -1 : (i > c->ie ? 1 : 0);
+          if (!o1 && !o2 && !o3) continue;
+          int s = nbr_gid(c, g, o1, o2, o3);
+          int si = i - o1 * bx1, sj = j - o2 * bx2, sk = k - o3 * bx3;
+          for (int n = 0; n < nvar; ++n)
+            a[g * bstride + n * c->ncell + CIDX(c, k, j, i)] =
+                a[s * bstride + n * c->ncell + CIDX(c, sk, sj, si)];
+        }
+      }
+    }
+  }
+}
+
+/* Same-level face-field ghost exchange, bvals/fc/bvals_fc.cpp:344-402 (pack) and
+ * :583-700 (unpack).  The face shared by two blocks (e.g. x1f(ie+1) / x1f(is) of
+ * the +x1 neighbour) is not exchanged: each block keeps its own copy. */
+void orc_bvals_fc(orc_ctx *c, int reg) {
+  int a1 = reg ? ARR_R1F : ARR_B1F, a2 = reg ? ARR_R2F : ARR_B2F, a3 = reg ? ARR_R3F : ARR_B3F;
+  int bx1 = c->d.bnx1, bx2 = c->d.bnx2, bx3 = c->d.bnx3;
+  double *b1 = c->arr[a1], *b2 = c->arr[a2], *b3 = c->arr[a3];
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < c->nblocks; ++g) {
+    /* x1f: face index range is..ie+1 is "interior" along x1 */
+    for (int k = 0; k < c->n3; ++k) {
+      int o3 = (k < c->ks) ? -1 : (k > c->ke ? 1 : 0);
+      for (int j = 0; j < c->n2; ++j) {
+        int o2 = (j < c->js) ? -1 : (j > c->je ? 1 : 0);
+        for (int i = 0; i <= c->n1; ++i) {
+          int o1 = (i < c->is) ? -1 : (i > c->ie + 1 ? 1 : 0);
+          if (!o1 && !o2 && !o3) continue;
+          int s = nbr_gid(c, g, o1, o2, o3);
+          b1[g * c->nf1 + F1IDX(c, k, j, i)] =
+              b1[s * c->nf1 + F1IDX(c, k - o3 * bx3, j - o2 * bx2, i - o1 * bx1)];
+        }
+      }
+    }
+    for (int k = 0; k < c->n3; ++k) {
+      int o3 = (k < c->ks) ? -1 : (k > c->ke ? 1 : 0);
+      for (int j = 0; j <= c->n2; ++j) {
+        int o2 = (j < c->js) ? -1 : (j > c->je + 1 ? 1 : 0);
+        for (int i = 0; i < c->n1; ++i) {
+          int o1 = (i < c->is) ? -1 : (i > c->ie ? 1 : 0);
+          if (!o1 && !o2 && !o3) continue;
+          int s = nbr_gid(c, g, o1, o2, o3);
+          b2[g * c->nf2 + F2IDX(c, k, j, i)] =
+              b2[s * c->nf2 + F2IDX(c, k - o3 * bx3, j - o2 * bx2, i - o1 * bx1)];
+        }
+      }
+    }
+    for (int k = 0; k <= c->n3; ++k) {
+      int o3 = (k < c->ks) ? -1 : (k > c->ke + 1 ? 1 : 0);
+      for (int j = 0; j < c->n2; ++j) {
+        int o2 = (j < c->js) ? -1 : (j > c->je ? 1 : 0);
+        for (int i = 0; i < c->n1; ++i) {
+          int o1 = (i < c->is) ? -1 : (i > c->ie ? 1 : 0);
+          if (!o1 && !o2 && !o3) continue;
+          int s = nbr_gid(c, g, o1, o2, o3);
+          b3[g * c->nf3 + F3IDX(c, k, j, i)] =
+              b3[s * c->nf3 + F3IDX(c, k - o3 * bx3, j - o2 * bx2, i - o1 * bx1)];
+        }
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- EOS */
+/* EquationOfState::FastMagnetosonicSpeed, eos/adiabatic_mhd.cpp:149-156 */
+static inline double fast_speed(double gamma, const double *prim, double bx) {
+  double asq = gamma * prim[IPR];
+  double vaxsq = bx * bx;
+  double ct2 = (prim[IBY] * prim[IBY] + prim[IBZ] * prim[IBZ]);
+  double qsq = vaxsq + ct2 + asq;
+  double tmp = vaxsq + ct2 - asq;
+  return sqrt(0.5 * (qsq + sqrt(tmp * tmp + 4.0 * asq * ct2)) / prim[IDN]);
+}
+
+/* Field::CalculateCellCenteredField (uniform branch, field/field.cpp:112-180) then
+ * EquationOfState::ConservedToPrimitive (eos/adiabatic_mhd.cpp:41-90), over the
+ * interior plus all ghost zones (time_integrator.cpp:1971-1977: every face has a
+ * neighbour on a periodic lattice). */
+void orc_cons2prim(orc_ctx *c) {
+  double gm1 = c->d.gamma - 1.0;
+  double dfl = c->d.dfloor, pfl = c->d.pfloor;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < c->nblocks; ++g) {
+    double *u = c->arr[ARR_U] + g * c->arrsize[ARR_U];
+    double *w = c->arr[ARR_W] + g * c->arrsize[ARR_W];
+    double *bcc = c->arr[ARR_BCC] + g * c->arrsize[ARR_BCC];
+    const double *b1 = c->arr[ARR_B1F] + g * c->nf1;
+    const double *b2 = c->arr[ARR_B2F] + g * c->nf2;
+    const double *b3 = c->arr[ARR_B3F] + g * c->nf3;
+    int64_t nc = c->ncell;
+    for (int k = 0; k < c->n3; ++k)
+      for (int j = 0; j < c->n2; ++j)
+        for (int i = 0; i < c->n1; ++i) {
+          int64_t id = CIDX(c, k, j, i);
+          bcc[IB1 * nc + id] = 0.5 * b1[F1IDX(c, k, j, i)] + 0.5 * b1[F1IDX(c, k, j, i + 1)];
+          bcc[IB2 * nc + id] = 0.5 * b2[F2IDX(c, k, j, i)] + 0.5 * b2[F2IDX(c, k, j + 1, i)];
+          bcc[IB3 * nc + id] = 0.5 * b3[F3IDX(c, k, j, i)] + 0.5 * b3[F3IDX(c, k + 1, j, i)];
+        }
+    for (int k = 0; k < c->n3; ++k)
+      for (int j = 0; j < c->n2; ++j)
+        for (int i = 0; i < c->n1; ++i) {
+          int64_t id = CIDX(c, k, j, i);
+          double u_d = u[IDN * nc + id];
+          double u_m1 = u[IVX * nc + id], u_m2 = u[IVY * nc + id], u_m3 = u[IVZ * nc + id];
+          double u_e = u[IEN * nc + id];
+          u_d = (u_d > dfl) ? u_d : dfl;
+          u[IDN * nc + id] = u_d;
+          w[IDN * nc + id] = u_d;
+          double di = 1.0 / u_d;
+          w[IVX * nc + id] = u_m1 * di;
+          w[IVY * nc + id] = u_m2 * di;
+          w[IVZ * nc + id] = u_m3 * di;
+          double bcc1 = bcc[IB1 * nc + id], bcc2 = bcc[IB2 * nc + id], bcc3 = bcc[IB3 * nc + id];
+          double pb = 0.5 * (SQR(bcc1) + SQR(bcc2) + SQR(bcc3));
+          double e_k = 0.5 * di * (SQR(u_m1) + SQR(u_m2) + SQR(u_m3));
+          double w_p = gm1 * (u_e - e_k - pb);
+          u_e = (w_p > pfl) ? u_e : ((pfl / gm1) + e_k + pb);
+          w_p = (w_p > pfl) ? w_p : pfl;
+          u[IEN * nc + id] = u_e;
+          w[IPR * nc + id] = w_p;
+        }
+  }
+}
+
+/* ---------------------------------------------------------------- Riemann solvers */
+/* Hydro::GetWeightForCT, hydro/hydro.cpp:154-158 */
+static inline double ct_weight(double dflx, double rhol, double rhor, double dx, double dt) {
+  double v_over_c = (1024.0) * dt * dflx / (dx * (rhol + rhor));
+  double tmp_min = dmin(0.5, v_over_c);
+  return 0.5 + dmax(-0.5, tmp_min);
+}
+
+typedef struct { double d, mx, my, mz, e, by, bz; } cons1d;
+
+/* HLLD for adiabatic MHD, hydro/rsolvers/mhd/hlld.cpp:38-382 (one interface). */
+static void hlld(double gamma, const double *wli, const double *wri, double bxi,
+                 double *flxi) {
+  double igm1 = 1.0 / (gamma - 1.0);
+  cons1d ul, ur, ulst, uldst, urdst, urst, fl, fr;
+  double spd[5];
+  double bxsq = bxi * bxi;
+  double pbl = 0.5 * (bxsq + (SQR(wli[IBY]) + SQR(wli[IBZ])));
+  double pbr = 0.5 * (bxsq + (SQR(wri[IBY]) + SQR(wri[IBZ])));
+  double kel = 0.5 * wli[IDN] * (SQR(wli[IVX]) + (SQR(wli[IVY]) + SQR(wli[IVZ])));
+  double ker = 0.5 * wri[IDN] * (SQR(wri[IVX]) + (SQR(wri[IVY]) + SQR(wri[IVZ])));
+  ul.d = wli[IDN];
+  ul.mx = wli[IVX] * ul.d;
+  ul.my = wli[IVY] * ul.d;
+  ul.mz = wli[IVZ] * ul.d;
+  ul.e = wli[IPR] * igm1 + kel + pbl;
+  ul.by = wli[IBY];
+  ul.bz = wli[IBZ];
+  ur.d = wri[IDN];
+  ur.mx = wri[IVX] * ur.d;
+  ur.my = wri[IVY] * ur.d;
+  ur.mz = wri[IVZ] * ur.d;
+  ur.e = wri[IPR] * igm1 + ker + pbr;
+  ur.by = wri[IBY];
+  ur.bz = wri[IBZ];
+
+  double cfl = fast_speed(gamma, wli, bxi);
+  double cfr = fast_speed(gamma, wri, bxi);
+  spd[0] = dmin(wli[IVX] - cfl, wri[IVX] - cfr);
+  spd[4] = dmax(wli[IVX] + cfl, wri[IVX] + cfr);
+
+  double ptl = wli[IPR] + pbl;
+  double ptr = wri[IPR] + pbr;
+  fl.d = ul.mx;
+  fl.mx = ul.mx * wli[IVX] + ptl - bxsq;
+  fl.my = ul.my * wli[IVX] - bxi * ul.by;
+  fl.mz = ul.mz * wli[IVX] - bxi * ul.bz;
+  fl.e = wli[IVX] * (ul.e + ptl - bxsq) - bxi * (wli[IVY] * ul.by + wli[IVZ] * ul.bz);
+  fl.by = ul.by * wli[IVX] - bxi * wli[IVY];
+  fl.bz = ul.bz * wli[IVX] - bxi * wli[IVZ];
+  fr.d = ur.mx;
+  fr.mx = ur.mx * wri[IVX] + ptr - bxsq;
+  fr.my = ur.my * wri[IVX] - bxi * ur.by;
+  fr.mz = ur.mz * wri[IVX] - bxi * ur.bz;
+  fr.e = wri[IVX] * (ur.e + ptr - bxsq) - bxi * (wri[IVY] * ur.by + wri[IVZ] * ur.bz);
+  fr.by = ur.by * wri[IVX] - bxi * wri[IVY];
+  fr.bz = ur.bz * wri[IVX] - bxi * wri[IVZ];
+
+  double sdl = spd[0] - wli[IVX];
+  double sdr = spd[4] - wri[IVX];
+  spd[2] = (sdr * ur.mx - sdl * ul.mx + (ptl - ptr)) / (sdr * ur.d - sdl * ul.d);
+  double sdml = spd[0] - spd[2];
+  double sdmr = spd[4] - spd[2];
+  double sdml_inv = 1.0 / sdml;
+  double sdmr_inv = 1.0 / sdmr;
+  ulst.d = ul.d * sdl * sdml_inv;
+  urst.d = ur.d * sdr * sdmr_inv;
+  double ulst_d_inv = 1.0 / ulst.d;
+  double urst_d_inv = 1.0 / urst.d;
+  double sqrtdl = sqrt(ulst.d);
+  double sqrtdr = sqrt(urst.d);
+  spd[1] = spd[2] - fabs(bxi) / sqrtdl;
+  spd[3] = spd[2] + fabs(bxi) / sqrtdr;
+
+  double ptstl = ptl + ul.d * sdl * (spd[2] - wli[IVX]);
+  double ptstr = ptr + ur.d * sdr * (spd[2] - wri[IVX]);
+  double ptst = 0.5 * (ptstr + ptstl);
+
+  ulst.mx = ulst.d * spd[2];
+  if (fabs(ul.d * sdl * sdml - bxsq) < (SMALL_NUMBER) * ptst) {
+    ulst.my = ulst.d * wli[IVY];
+    ulst.mz = ulst.d * wli[IVZ];
+    ulst.by = ul.by;
+    ulst.bz = ul.bz;
+  } else {
+    double tmp = bxi * (sdl - sdml) / (ul.d * sdl * sdml - bxsq);
+    ulst.my = ulst.d * (wli[IVY] - ul.by * tmp);
+    ulst.mz = ulst.d * (wli[IVZ] - ul.bz * tmp);
+    tmp = (ul.d * SQR(sdl) - bxsq) / (ul.d * sdl * sdml - bxsq);
+    ulst.by = ul.by * tmp;
+    ulst.bz = ul.bz * tmp;
+  }
+  double vbstl = (ulst.mx * bxi + (ulst.my * ulst.by + ulst.mz * ulst.bz)) * ulst_d_inv;
+  ulst.e = (sdl * ul.e - ptl * wli[IVX] + ptst * spd[2] +
+            bxi * (wli[IVX] * bxi + (wli[IVY] * ul.by + wli[IVZ] * ul.bz) - vbstl)) * sdml_inv;
+
+  urst.mx = urst.d * spd[2];
+  if (fabs(ur.d * sdr * sdmr - bxsq) < (SMALL_NUMBER) * ptst) {
+    urst.my = urst.d * wri[IVY];
+    urst.mz = urst.d * wri[IVZ];
+    urst.by = ur.by;
+    urst.bz = ur.bz;
+  } else {
+    double tmp = bxi * (sdr - sdmr) / (ur.d * sdr * sdmr - bxsq);
+    urst.my = urst.d * (wri[IVY] - ur.by * tmp);
+    urst.mz = urst.d * (wri[IVZ] - ur.bz * tmp);
+    tmp = (ur.d * SQR(sdr) - bxsq) / (ur.d * sdr * sdmr - bxsq);
+    urst.by = ur.by * tmp;
+    urst.bz = ur.bz * tmp;
+  }
+  double vbstr = (urst.mx * bxi + (urst.my * urst.by + urst.mz * urst.bz)) * urst_d_inv;
+  urst.e = (sdr * ur.e - ptr * wri[IVX] + ptst * spd[2] +
+            bxi * (wri[IVX] * bxi + (wri[IVY] * ur.by + wri[IVZ] * ur.bz) - vbstr)) * sdmr_inv;
+
+  if (0.5 * bxsq < (SMALL_NUMBER) * ptst) {
+    uldst = ulst;
+    urdst = urst;
+  } else {
+    double invsumd = 1.0 / (sqrtdl + sqrtdr);
+    double bxsig = (bxi > 0.0 ? 1.0 : -1.0);
+    uldst.d = ulst.d;
+    urdst.d = urst.d;
+    uldst.mx = ulst.mx;
+    urdst.mx = urst.mx;
+    double tmp = invsumd * (sqrtdl * (ulst.my * ulst_d_inv) + sqrtdr * (urst.my * urst_d_inv) +
+                            bxsig * (urst.by - ulst.by));
+    uldst.my = uldst.d * tmp;
+    urdst.my = urdst.d * tmp;
+    tmp = invsumd * (sqrtdl * (ulst.mz * ulst_d_inv) + sqrtdr * (urst.mz * urst_d_inv) +
+                     bxsig * (urst.bz - ulst.bz));
+    uldst.mz = uldst.d * tmp;
+    urdst.mz = urdst.d * tmp;
+    tmp = invsumd * (sqrtdl * urst.by + sqrtdr * ulst.by +
+                     bxsig * sqrtdl * sqrtdr * ((urst.my * urst_d_inv) - (ulst.my * ulst_d_inv)));
+    uldst.by = urdst.by = tmp;
+    tmp = invsumd * (sqrtdl * urst.bz + sqrtdr * ulst.bz +
+                     bxsig * sqrtdl * sqrtdr * ((urst.mz * urst_d_inv) - (ulst.mz * ulst_d_inv)));
+    uldst.bz = urdst.bz = tmp;
+    tmp = spd[2] * bxi + (uldst.my * uldst.by + uldst.mz * uldst.bz) / uldst.d;
+    uldst.e = ulst.e - sqrtdl * bxsig * (vbstl - tmp);
+    urdst.e = urst.e + sqrtdr * bxsig * (vbstr - tmp);
+  }
+
+  uldst.d = spd[1] * (uldst.d - ulst.d);
+  uldst.mx = spd[1] * (uldst.mx - ulst.mx);
+  uldst.my = spd[1] * (uldst.my - ulst.my);
+  uldst.mz = spd[1] * (uldst.mz - ulst.mz);
+  uldst.e = spd[1] * (uldst.e - ulst.e);
+  uldst.by = spd[1] * (uldst.by - ulst.by);
+  uldst.bz = spd[1] * (uldst.bz - ulst.bz);
+  ulst.d = spd[0] * (ulst.d - ul.d);
+  ulst.mx = spd[0] * (ulst.mx - ul.mx);
+  ulst.my = spd[0] * (ulst.my - ul.my);
+  ulst.mz = spd[0] * (ulst.mz - ul.mz);
+  ulst.e = spd[0] * (ulst.e - ul.e);
+  ulst.by = spd[0] * (ulst.by - ul.by);
+  ulst.bz = spd[0] * (ulst.bz - ul.bz);
+  urdst.d = spd[3] * (urdst.d - urst.d);
+  urdst.mx = spd[3] * (urdst.mx - urst.mx);
+  urdst.my = spd[3] * (urdst.my - urst.my);
+  urdst.mz = spd[3] * (urdst.mz - urst.mz);
+  urdst.e = spd[3] * (urdst.e - urst.e);
+  urdst.by = spd[3] * (urdst.by - urst.by);
+  urdst.bz = spd[3] * (urdst.bz - urst.bz);
+  urst.d = spd[4] * (urst.d - ur.d);
+  urst.mx = spd[4] * (urst.mx - ur.mx);
+  urst.my = spd[4] * (urst.my - ur.my);
+  urst.mz = spd[4] * (urst.mz - ur.mz);
+  urst.e = spd[4] * (urst.e - ur.e);
+  urst.by = spd[4] * (urst.by - ur.by);
+  urst.bz = spd[4] * (urst.bz - ur.bz);
+
+  const cons1d *f;
+  if (spd[0] >= 0.0) {
+    f = &fl;
+    flxi[IDN] = f->d; flxi[IVX] = f->mx; flxi[IVY] = f->my; flxi[IVZ] = f->mz;
+    flxi[IEN] = f->e; flxi[IBY] = f->by; flxi[IBZ] = f->bz;
+  } else if (spd[4] <= 0.0) {
+    f = &fr;
+    flxi[IDN] = f->d; flxi[IVX] = f->mx; flxi[IVY] = f->my; flxi[IVZ] = f->mz;
+    flxi[IEN] = f->e; flxi[IBY] = f->by; flxi[IBZ] = f->bz;
+  } else if (spd[1] >= 0.0) {
+    flxi[IDN] = fl.d + ulst.d;
+    flxi[IVX] = fl.mx + ulst.mx;
+    flxi[IVY] = fl.my + ulst.my;
+    flxi[IVZ] = fl.mz + ulst.mz;
+    flxi[IEN] = fl.e + ulst.e;
+    flxi[IBY] = fl.by + ulst.by;
+    flxi[IBZ] = fl.bz + ulst.bz;
+  } else if (spd[2] >= 0.0) {
+    flxi[IDN] = fl.d + ulst.d + uldst.d;
+    flxi[IVX] = fl.mx + ulst.mx + uldst.mx;
+    flxi[IVY] = fl.my + ulst.my + uldst.my;
+    flxi[IVZ] = fl.mz + ulst.mz + uldst.mz;
+    flxi[IEN] = fl.e + ulst.e + uldst.e;
+    flxi[IBY] = fl.by + ulst.by + uldst.by;
+    flxi[IBZ] = fl.bz + ulst.bz + uldst.bz;
+  } else if (spd[3] > 0.0) {
+    flxi[IDN] = fr.d + urst.d + urdst.d;
+    flxi[IVX] = fr.mx + urst.mx + urdst.mx;
+    flxi[IVY] = fr.my + urst.my + urdst.my;
+    flxi[IVZ] = fr.mz + urst.mz + urdst.mz;
+    flxi[IEN] = fr.e + urst.e + urdst.e;
+    flxi[IBY] = fr.by + urst.by + urdst.by;
+    flxi[IBZ] = fr.bz + urst.bz + urdst.bz;
+  } else {
+    flxi[IDN] = fr.d + urst.d;
+    flxi[IVX] = fr.mx + urst.mx;
+    flxi[IVY] = fr.my + urst.my;
+    flxi[IVZ] = fr.mz + urst.mz;
+    flxi[IEN] = fr.e + urst.e;
+    flxi[IBY] = fr.by + urst.by;
+    flxi[IBZ] = fr.bz + urst.bz;
+  }
+}
+
+/* HLLE for adiabatic MHD, hydro/rsolvers/mhd/hlle_mhd.cpp:25-182 (one interface). */
+static void hlle(double gamma, const double *wli, const double *wri, double bxi,
+                 double *flxi) {
+  double gm1 = gamma - 1.0;
+  double wroe[NWAVE], fl[NWAVE], fr[NWAVE];
+  double sqrtdl = sqrt(wli[IDN]);
+  double sqrtdr = sqrt(wri[IDN]);
+  double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+  wroe[IDN] = sqrtdl * sqrtdr;
+  wroe[IVX] = (sqrtdl * wli[IVX] + sqrtdr * wri[IVX]) * isdlpdr;
+  wroe[IVY] = (sqrtdl * wli[IVY] + sqrtdr * wri[IVY]) * isdlpdr;
+  wroe[IVZ] = (sqrtdl * wli[IVZ] + sqrtdr * wri[IVZ]) * isdlpdr;
+  wroe[IBY] = (sqrtdr * wli[IBY] + sqrtdl * wri[IBY]) * isdlpdr;
+  wroe[IBZ] = (sqrtdr * wli[IBZ] + sqrtdl * wri[IBZ]) * isdlpdr;
+  double x = 0.5 * (SQR(wli[IBY] - wri[IBY]) + SQR(wli[IBZ] - wri[IBZ])) / (SQR(sqrtdl + sqrtdr));
+  double y = 0.5 * (wli[IDN] + wri[IDN]) / wroe[IDN];
+  double pbl = 0.5 * (bxi * bxi + SQR(wli[IBY]) + SQR(wli[IBZ]));
+  double pbr = 0.5 * (bxi * bxi + SQR(wri[IBY]) + SQR(wri[IBZ]));
+  double el = wli[IPR] / gm1 + 0.5 * wli[IDN] * (SQR(wli[IVX]) + SQR(wli[IVY]) + SQR(wli[IVZ])) + pbl;
+  double er = wri[IPR] / gm1 + 0.5 * wri[IDN] * (SQR(wri[IVX]) + SQR(wri[IVY]) + SQR(wri[IVZ])) + pbr;
+  double hroe = ((el + wli[IPR] + pbl) / sqrtdl + (er + wri[IPR] + pbr) / sqrtdr) * isdlpdr;
+  double cl = fast_speed(gamma, wli, bxi);
+  double cr = fast_speed(gamma, wri, bxi);
+  double btsq = SQR(wroe[IBY]) + SQR(wroe[IBZ]);
+  double vaxsq = bxi * bxi / wroe[IDN];
+  double bt_starsq = (gm1 - (gm1 - 1.0) * y) * btsq;
+  double hp = hroe - (vaxsq + btsq / wroe[IDN]);
+  double vsq = SQR(wroe[IVX]) + SQR(wroe[IVY]) + SQR(wroe[IVZ]);
+  double twid_asq = dmax((gm1 * (hp - 0.5 * vsq) - (gm1 - 1.0) * x), 0.0);
+  double ct2 = bt_starsq / wroe[IDN];
+  double tsum = vaxsq + ct2 + twid_asq;
+  double tdif = vaxsq + ct2 - twid_asq;
+  double cf2_cs2 = sqrt(tdif * tdif + 4.0 * twid_asq * ct2);
+  double cfsq = 0.5 * (tsum + cf2_cs2);
+  double a = sqrt(cfsq);
+  double al = dmin((wroe[IVX] - a), (wli[IVX] - cl));
+  double ar = dmax((wroe[IVX] + a), (wri[IVX] + cr));
+  double bp = ar > 0.0 ? ar : 0.0;
+  double bm = al < 0.0 ? al : 0.0;
+  double vxl = wli[IVX] - bm;
+  double vxr = wri[IVX] - bp;
+  fl[IDN] = wli[IDN] * vxl;
+  fr[IDN] = wri[IDN] * vxr;
+  fl[IVX] = wli[IDN] * wli[IVX] * vxl + pbl - SQR(bxi);
+  fr[IVX] = wri[IDN] * wri[IVX] * vxr + pbr - SQR(bxi);
+  fl[IVY] = wli[IDN] * wli[IVY] * vxl - bxi * wli[IBY];
+  fr[IVY] = wri[IDN] * wri[IVY] * vxr - bxi * wri[IBY];
+  fl[IVZ] = wli[IDN] * wli[IVZ] * vxl - bxi * wli[IBZ];
+  fr[IVZ] = wri[IDN] * wri[IVZ] * vxr - bxi * wri[IBZ];
+  fl[IVX] += wli[IPR];
+  fr[IVX] += wri[IPR];
+  fl[IEN] = el * vxl + wli[IVX] * (wli[IPR] + pbl - bxi * bxi);
+  fr[IEN] = er * vxr + wri[IVX] * (wri[IPR] + pbr - bxi * bxi);
+  fl[IEN] -= bxi * (wli[IBY] * wli[IVY] + wli[IBZ] * wli[IVZ]);
+  fr[IEN] -= bxi * (wri[IBY] * wri[IVY] + wri[IBZ] * wri[IVZ]);
+  fl[IBY] = wli[IBY] * vxl - bxi * wli[IVY];
+  fr[IBY] = wri[IBY] * vxr - bxi * wri[IVY];
+  fl[IBZ] = wli[IBZ] * vxl - bxi * wli[IVZ];
+  fr[IBZ] = wri[IBZ] * vxr - bxi * wri[IVZ];
+  double tmp = 0.0;
+  if (bp != bm) tmp = 0.5 * (bp + bm) / (bp - bm);
+  for (int n = 0; n < NWAVE; ++n)
+    flxi[n] = 0.5 * (fl[n] + fr[n]) + (fl[n] - fr[n]) * tmp;
+}
+
+/* ---------------------------------------------------------------- fluxes */
+/* Hydro::CalculateFluxes, hydro/calculate_fluxes.cpp:36-378, orders 1 (dc.cpp:24-104)
+ * and 2 (plm.cpp:27-351, uniform Cartesian van Leer branch).  Interface states are
+ * evaluated per face directly; the pencil scratch arrays of the reference hold the
+ * same values. */
+static inline void recon_cell(const orc_ctx *c, const double *w, const double *bcc,
+                              int dir, int order, int g, int k, int j, int i,
+                              double *wm, double *wp) {
+  /* returns reconstructed state at the lower (wm: "wr" of face at this cell's lower
+   * side) and upper (wp: "wl" of face at upper side) faces of cell (k,j,i) */
+  int64_t nc = c->ncell;
+  int64_t id = CIDX(c, k, j, i);
+  int64_t st = (dir == 0) ? 1 : (dir == 1 ? c->n1 : (int64_t)c->n1 * c->n2);
+  /* variable order in the 7-slot wave array: IDN, IVX, IVY, IVZ, IPR, IBY, IBZ with
+   * (IBY,IBZ) = the two transverse bcc components (plm.cpp:46-56,160-170,273-283) */
+  int tb1 = (dir == 0) ? IB2 : (dir == 1 ? IB3 : IB1);
+  int tb2 = (dir == 0) ? IB3 : (dir == 1 ? IB1 : IB2);
+  double q[NWAVE][3];
+  for (int n = 0; n < NHYDRO; ++n) {
+    q[n][0] = w[n * nc + id - st]; q[n][1] = w[n * nc + id]; q[n][2] = w[n * nc + id + st];
+  }
+  q[IBY][0] = bcc[tb1 * nc + id - st]; q[IBY][1] = bcc[tb1 * nc + id]; q[IBY][2] = bcc[tb1 * nc + id + st];
+  q[IBZ][0] = bcc[tb2 * nc + id - st]; q[IBZ][1] = bcc[tb2 * nc + id]; q[IBZ][2] = bcc[tb2 * nc + id + st];
+  if (order == 1) {
+    for (int n = 0; n < NWAVE; ++n) { wm[n] = q[n][1]; wp[n] = q[n][1]; }
+    return;
+  }
+  double dxp, dxm;
+  if (dir == 0) {
+    const double *xf = c->x1f + g * (c->n1 + 1), *xv = c->x1v + g * (c->n1 + 1),
+                 *dx = c->dx1f + g * (c->n1 + 1);
+    dxp = (xf[i + 1] - xv[i]) / dx[i];
+    dxm = (xv[i] - xf[i]) / dx[i];
+  } else if (dir == 1) {
+    const double *xf = c->x2f + g * (c->n2 + 1), *xv = c->x2v + g * (c->n2 + 1),
+                 *dx = c->dx2f + g * (c->n2 + 1);
+    dxp = (xf[j + 1] - xv[j]) / dx[j];
+    dxm = (xv[j] - xf[j]) / dx[j];
+  } else {
+    const double *xf = c->x3f + g * (c->n3 + 1), *xv = c->x3v + g * (c->n3 + 1),
+                 *dx = c->dx3f + g * (c->n3 + 1);
+    dxp = (xf[k + 1] - xv[k]) / dx[k];
+    dxm = (xv[k] - xf[k]) / dx[k];
+  }
+  for (int n = 0; n < NWAVE; ++n) {
+    double dwl = (q[n][1] - q[n][0]);
+    double dwr = (q[n][2] - q[n][1]);
+    double dw2 = dwl * dwr;
+    double dwm = 2.0 * dw2 / (dwl + dwr);
+    if (dw2 <= 0.0) dwm = 0.0;
+    wp[n] = q[n][1] + dxp * dwm;
+    wm[n] = q[n][1] - dxm * dwm;
+  }
+}
+
+void orc_calculate_fluxes(orc_ctx *c, int order) {
+  double gamma = c->d.gamma, dt = c->dt;
+  int rs = c->d.rsolver;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < c->nblocks; ++g) {
+    const double *w = c->arr[ARR_W] + g * c->arrsize[ARR_W];
+    const double *bcc = c->arr[ARR_BCC] + g * c->arrsize[ARR_BCC];
+    for (int dir = 0; dir < 3; ++dir) {
+      const double *bn; double *flx, *ey, *ez, *wct;
+      int il, iu, jl, ju, kl, ku;
+      int64_t nf;
+      if (dir == 0) {
+        bn = c->arr[ARR_B1F] + g * c->nf1; nf = c->nf1;
+        flx = c->arr[ARR_FX1] + g * c->arrsize[ARR_FX1];
+        ey = c->arr[ARR_E3X1] + g * c->nf1; ez = c->arr[ARR_E2X1] + g * c->nf1;
+        wct = c->arr[ARR_WX1] + g * c->nf1;
+        il = c->is; iu = c->ie + 1; jl = c->js - 1; ju = c->je + 1; kl = c->ks - 1; ku = c->ke + 1;
+      } else if (dir == 1) {
+        bn = c->arr[ARR_B2F] + g * c->nf2; nf = c->nf2;
+        flx = c->arr[ARR_FX2] + g * c->arrsize[ARR_FX2];
+        ey = c->arr[ARR_E1X2] + g * c->nf2; ez = c->arr[ARR_E3X2] + g * c->nf2;
+        wct = c->arr[ARR_WX2] + g * c->nf2;
+        il = c->is - 1; iu = c->ie + 1; jl = c->js; ju = c->je + 1; kl = c->ks - 1; ku = c->ke + 1;
+      } else {
+        bn = c->arr[ARR_B3F] + g * c->nf3; nf = c->nf3;
+        flx = c->arr[ARR_FX3] + g * c->arrsize[ARR_FX3];
+        ey = c->arr[ARR_E2X3] + g * c->nf3; ez = c->arr[ARR_E1X3] + g * c->nf3;
+        wct = c->arr[ARR_WX3] + g * c->nf3;
+        il = c->is - 1; iu = c->ie + 1; jl = c->js - 1; ju = c->je + 1; kl = c->ks; ku = c->ke + 1;
+      }
+      /* ivx/ivy/ivz cyclic permutation, hlld.cpp:43-44 */
+      int ivx = IVX + dir, ivy = IVX + (dir + 1) % 3, ivz = IVX + (dir + 2) % 3;
+      for (int k = kl; k <= ku; ++k)
+        for (int j = jl; j <= ju; ++j)
+          for (int i = il; i <= iu; ++i) {
+            double lm[NWAVE], lp[NWAVE], rm[NWAVE], rp[NWAVE];
+            /* left state: upper face of the cell below; right: lower face of cell */
+            int ki = k - (dir == 2), ji = j - (dir == 1), ii = i - (dir == 0);
+            recon_cell(c, w, bcc, dir, order, g, ki, ji, ii, lm, lp);
+            recon_cell(c, w, bcc, dir, order, g, k, j, i, rm, rp);
+            double wli[NWAVE], wri[NWAVE], flxi[NWAVE];
+            wli[IDN] = lp[IDN]; wli[IVX] = lp[ivx]; wli[IVY] = lp[ivy]; wli[IVZ] = lp[ivz];
+            wli[IPR] = lp[IPR]; wli[IBY] = lp[IBY]; wli[IBZ] = lp[IBZ];
+            wri[IDN] = rm[IDN]; wri[IVX] = rm[ivx]; wri[IVY] = rm[ivy]; wri[IVZ] = rm[ivz];
+            wri[IPR] = rm[IPR]; wri[IBY] = rm[IBY]; wri[IBZ] = rm[IBZ];
+            int64_t fid = (dir == 0) ? F1IDX(c, k, j, i) : (dir == 1 ? F2IDX(c, k, j, i) : F3IDX(c, k, j, i));
+            double bxi = bn[fid];
+            if (rs == 1) hlle(gamma, wli, wri, bxi, flxi);
+            else hlld(gamma, wli, wri, bxi, flxi);
+            flx[IDN * nf + fid] = flxi[IDN];
+            flx[ivx * nf + fid] = flxi[IVX];
+            flx[ivy * nf + fid] = flxi[IVY];
+            flx[ivz * nf + fid] = flxi[IVZ];
+            flx[IEN * nf + fid] = flxi[IEN];
+            ey[fid] = -flxi[IBY];
+            ez[fid] = flxi[IBZ];
+            double dxw = (dir == 0) ? c->dx1f[g * (c->n1 + 1) + i]
+                                    : (dir == 1 ? c->dx2f[g * (c->n2 + 1) + j] : c->dx3f[g * (c->n3 + 1) + k]);
+            wct[fid] = ct_weight(flxi[IDN], wli[IDN], wri[IDN], dxw, dt);
+          }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- corner EMF */
+/* Field::ComputeCornerE, 3D Newtonian branch, field/calculate_corner_e.cpp:130-229 */
+void orc_corner_e(orc_ctx *c) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < c->nblocks; ++g) {
+    int64_t nc = c->ncell;
+    const double *w = c->arr[ARR_W] + g * c->arrsize[ARR_W];
+    const double *bcc = c->arr[ARR_BCC] + g * c->arrsize[ARR_BCC];
+    double *cce = c->cce + (int64_t)g * 3 * nc;
+    for (int k = c->ks - 1; k <= c->ke + 1; ++k)
+      for (int j = c->js - 1; j <= c->je + 1; ++j)
+        for (int i = c->is - 1; i <= c->ie + 1; ++i) {
+          int64_t id = CIDX(c, k, j, i);
+          cce[IB1 * nc + id] = w[IVZ * nc + id] * bcc[IB2 * nc + id] - w[IVY * nc + id] * bcc[IB3 * nc + id];
+          cce[IB2 * nc + id] = w[IVX * nc + id] * bcc[IB3 * nc + id] - w[IVZ * nc + id] * bcc[IB1 * nc + id];
+          cce[IB3 * nc + id] = w[IVY * nc + id] * bcc[IB1 * nc + id] - w[IVX * nc + id] * bcc[IB2 * nc + id];
+        }
+    const double *e3x1 = c->arr[ARR_E3X1] + g * c->nf1, *e2x1 = c->arr[ARR_E2X1] + g * c->nf1;
+    const double *e1x2 = c->arr[ARR_E1X2] + g * c->nf2, *e3x2 = c->arr[ARR_E3X2] + g * c->nf2;
+    const double *e1x3 = c->arr[ARR_E1X3] + g * c->nf3, *e2x3 = c->arr[ARR_E2X3] + g * c->nf3;
+    const double *wx1 = c->arr[ARR_WX1] + g * c->nf1, *wx2 = c->arr[ARR_WX2] + g * c->nf2,
+                 *wx3 = c->arr[ARR_WX3] + g * c->nf3;
+    double *e1 = c->arr[ARR_E1] + g * c->ne1, *e2 = c->arr[ARR_E2] + g * c->ne2,
+           *e3 = c->arr[ARR_E3] + g * c->ne3;
+    const double *c1 = cce + IB1 * nc, *c2 = cce + IB2 * nc, *c3 = cce + IB3 * nc;
+#define CC(a, k, j, i) a[CIDX(c, k, j, i)]
+#define A1(a, k, j, i) a[F1IDX(c, k, j, i)]
+#define A2(a, k, j, i) a[F2IDX(c, k, j, i)]
+#define A3(a, k, j, i) a[F3IDX(c, k, j, i)]
+    for (int k = c->ks; k <= c->ke + 1; ++k)
+      for (int j = c->js; j <= c->je + 1; ++j)
+        for (int i = c->is; i <= c->ie + 1; ++i) {
+          double de1_l3 = (1.0 - A2(wx2, k - 1, j, i)) * (A3(e1x3, k, j, i) - CC(c1, k - 1, j, i)) +
+                          (A2(wx2, k - 1, j, i)) * (A3(e1x3, k, j - 1, i) - CC(c1, k - 1, j - 1, i));
+          double de1_r3 = (1.0 - A2(wx2, k, j, i)) * (A3(e1x3, k, j, i) - CC(c1, k, j, i)) +
+                          (A2(wx2, k, j, i)) * (A3(e1x3, k, j - 1, i) - CC(c1, k, j - 1, i));
+          double de1_l2 = (1.0 - A3(wx3, k, j - 1, i)) * (A2(e1x2, k, j, i) - CC(c1, k, j - 1, i)) +
+                          (A3(wx3, k, j - 1, i)) * (A2(e1x2, k - 1, j, i) - CC(c1, k - 1, j - 1, i));
+          double de1_r2 = (1.0 - A3(wx3, k, j, i)) * (A2(e1x2, k, j, i) - CC(c1, k, j, i)) +
+                          (A3(wx3, k, j, i)) * (A2(e1x2, k - 1, j, i) - CC(c1, k - 1, j, i));
+          e1[E1IDX(c, k, j, i)] = 0.25 * (de1_l3 + de1_r3 + de1_l2 + de1_r2 + A2(e1x2, k - 1, j, i) +
+                                          A2(e1x2, k, j, i) + A3(e1x3, k, j - 1, i) + A3(e1x3, k, j, i));
+
+          double de2_l3 = (1.0 - A1(wx1, k - 1, j, i)) * (A3(e2x3, k, j, i) - CC(c2, k - 1, j, i)) +
+                          (A1(wx1, k - 1, j, i)) * (A3(e2x3, k, j, i - 1) - CC(c2, k - 1, j, i - 1));
+          double de2_r3 = (1.0 - A1(wx1, k, j, i)) * (A3(e2x3, k, j, i) - CC(c2, k, j, i)) +
+                          (A1(wx1, k, j, i)) * (A3(e2x3, k, j, i - 1) - CC(c2, k, j, i - 1));
+          double de2_l1 = (1.0 - A3(wx3, k, j, i - 1)) * (A1(e2x1, k, j, i) - CC(c2, k, j, i - 1)) +
+                          (A3(wx3, k, j, i - 1)) * (A1(e2x1, k - 1, j, i) - CC(c2, k - 1, j, i - 1));
+          double de2_r1 = (1.0 - A3(wx3, k, j, i)) * (A1(e2x1, k, j, i) - CC(c2, k, j, i)) +
+                          (A3(wx3, k, j, i)) * (A1(e2x1, k - 1, j, i) - CC(c2, k - 1, j, i));
+          e2[E2IDX(c, k, j, i)] = 0.25 * (de2_l3 + de2_r3 + de2_l1 + de2_r1 + A3(e2x3, k, j, i - 1) +
+                                          A3(e2x3, k, j, i) + A1(e2x1, k - 1, j, i) + A1(e2x1, k, j, i));
+
+          double de3_l2 = (1.0 - A1(wx1, k, j - 1, i)) * (A2(e3x2, k, j, i) - CC(c3, k, j - 1, i)) +
+                          (A1(wx1, k, j - 1, i)) * (A2(e3x2, k, j, i - 1) - CC(c3, k, j - 1, i - 1));
+          double de3_r2 = (1.0 - A1(wx1, k, j, i)) * (A2(e3x2, k, j, i) - CC(c3, k, j, i)) +
+                          (A1(wx1, k, j, i)) * (A2(e3x2, k, j, i - 1) - CC(c3, k, j, i - 1));
+          double de3_l1 = (1.0 - A2(wx2, k, j, i - 1)) * (A1(e3x1, k, j, i) - CC(c3, k, j, i - 1)) +
+                          (A2(wx2, k, j, i - 1)) * (A1(e3x1, k, j - 1, i) - CC(c3, k, j - 1, i - 1));
+          double de3_r1 = (1.0 - A2(wx2, k, j, i)) * (A1(e3x1, k, j, i) - CC(c3, k, j, i)) +
+                          (A2(wx2, k, j, i)) * (A1(e3x1, k, j - 1, i) - CC(c3, k, j - 1, i));
+          e3[E3IDX(c, k, j, i)] = 0.25 * (de3_l1 + de3_r1 + de3_l2 + de3_r2 + A2(e3x2, k, j, i - 1) +
+                                          A2(e3x2, k, j, i) + A1(e3x1, k, j - 1, i) + A1(e3x1, k, j, i));
+        }
+#undef CC
+#undef A1
+#undef A2
+#undef A3
+  }
+}
+
+/* ---------------------------------------------------------------- integrate */
+/* One VL2 stage of TimeIntegratorTaskList (time_integrator.cpp:1563-1660):
+ * WeightedAve + register swap (weighted_ave.cpp:33-236, :1583-1584, :1637-1640),
+ * AddFluxDivergence (add_flux_divergence.cpp:39-96), SelfGravity source
+ * (hydro/srcterms/self_gravity.cpp:35-125), Field::CT (field/ct.cpp:31-116).
+ * VL2 weights: stage 1 delta=1, beta=0.5; stage 2 delta=0, beta=1
+ * (time_integrator.cpp:199-234). */
+static void swap_ptr(double **a, double **b) { double *t = *a; *a = *b; *b = t; }
+
+void orc_integrate(orc_ctx *c, int stage) {
+  double beta = (stage == 1) ? 0.5 : 1.0;
+  double delta = (stage == 1) ? 1.0 : 0.0;
+  int nb = c->nblocks;
+  if (stage == 1) {  /* StartupTaskList, time_integrator.cpp:1389-1397 */
+    memset(c->arr[ARR_U1], 0, sizeof(double) * c->arrsize[ARR_U1] * nb);
+    memset(c->arr[ARR_R1F], 0, sizeof(double) * c->nf1 * nb);
+    memset(c->arr[ARR_R2F], 0, sizeof(double) * c->nf2 * nb);
+    memset(c->arr[ARR_R3F], 0, sizeof(double) * c->nf3 * nb);
+  }
+  /* u1 += delta*u over the interior (weighted_ave.cpp:38-97 with wght[0]=1) */
+  if (delta != 0.0) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int g = 0; g < nb; ++g) {
+      double *u = c->arr[ARR_U] + g * c->arrsize[ARR_U];
+      double *u1 = c->arr[ARR_U1] + g * c->arrsize[ARR_U1];
+      for (int n = 0; n < NHYDRO; ++n)
+        for (int k = c->ks; k <= c->ke; ++k)
+          for (int j = c->js; j <= c->je; ++j)
+            for (int i = c->is; i <= c->ie; ++i) {
+              int64_t id = n * c->ncell + CIDX(c, k, j, i);
+              u1[id] += delta * u[id];
+            }
+      double *b1 = c->arr[ARR_B1F] + g * c->nf1, *r1 = c->arr[ARR_R1F] + g * c->nf1;
+      double *b2 = c->arr[ARR_B2F] + g * c->nf2, *r2 = c->arr[ARR_R2F] + g * c->nf2;
+      double *b3 = c->arr[ARR_B3F] + g * c->nf3, *r3 = c->arr[ARR_R3F] + g * c->nf3;
+      for (int k = c->ks; k <= c->ke; ++k)
+        for (int j = c->js; j <= c->je; ++j)
+          for (int i = c->is; i <= c->ie + 1; ++i) r1[F1IDX(c, k, j, i)] += delta * b1[F1IDX(c, k, j, i)];
+      for (int k = c->ks; k <= c->ke; ++k)
+        for (int j = c->js; j <= c->je + 1; ++j)
+          for (int i = c->is; i <= c->ie; ++i) r2[F2IDX(c, k, j, i)] += delta * b2[F2IDX(c, k, j, i)];
+      for (int k = c->ks; k <= c->ke + 1; ++k)
+        for (int j = c->js; j <= c->je; ++j)
+          for (int i = c->is; i <= c->ie; ++i) r3[F3IDX(c, k, j, i)] += delta * b3[F3IDX(c, k, j, i)];
+    }
+  }
+  /* gamma = (0,1,0): swap u <-> u1, b <-> b1 */
+  swap_ptr(&c->arr[ARR_U], &c->arr[ARR_U1]);
+  swap_ptr(&c->arr[ARR_B1F], &c->arr[ARR_R1F]);
+  swap_ptr(&c->arr[ARR_B2F], &c->arr[ARR_R2F]);
+  swap_ptr(&c->arr[ARR_B3F], &c->arr[ARR_R3F]);
+
+  double wght = beta * c->dt;
+  int grav = c->d.self_gravity;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nb; ++g) {
+    int64_t nc = c->ncell;
+    double *u = c->arr[ARR_U] + g * c->arrsize[ARR_U];
+    const double *f1 = c->arr[ARR_FX1] + g * c->arrsize[ARR_FX1];
+    const double *f2 = c->arr[ARR_FX2] + g * c->arrsize[ARR_FX2];
+    const double *f3 = c->arr[ARR_FX3] + g * c->arrsize[ARR_FX3];
+    const double *dx1 = c->dx1f + g * (c->n1 + 1), *dx2 = c->dx2f + g * (c->n2 + 1),
+                 *dx3 = c->dx3f + g * (c->n3 + 1);
+    for (int k = c->ks; k <= c->ke; ++k)
+      for (int j = c->js; j <= c->je; ++j)
+        for (int i = c->is; i <= c->ie; ++i) {
+          /* Face?Area / CellVolume, coordinates.cpp:436-537 */
+          double a1 = dx2[j] * dx3[k], a1p = dx2[j] * dx3[k];
+          double a2 = dx1[i] * dx3[k], a2p = dx1[i] * dx3[k];
+          double a3 = dx1[i] * dx2[j], a3p = dx1[i] * dx2[j];
+          double vol = dx1[i] * dx2[j] * dx3[k];
+          for (int n = 0; n < NHYDRO; ++n) {
+            double dflx = (a1p * f1[n * c->nf1 + F1IDX(c, k, j, i + 1)] - a1 * f1[n * c->nf1 + F1IDX(c, k, j, i)]);
+            dflx += (a2p * f2[n * c->nf2 + F2IDX(c, k, j + 1, i)] - a2 * f2[n * c->nf2 + F2IDX(c, k, j, i)]);
+            dflx += (a3p * f3[n * c->nf3 + F3IDX(c, k + 1, j, i)] - a3 * f3[n * c->nf3 + F3IDX(c, k, j, i)]);
+            u[n * nc + CIDX(c, k, j, i)] -= wght * dflx / vol;
+          }
+        }
+    if (grav) {
+      /* SelfGravity source with dt = beta*dt (time_integrator.cpp:1650-1655) */
+      const double *w = c->arr[ARR_W] + g * c->arrsize[ARR_W];
+      const double *phi = c->arr[ARR_PHI] + g * c->arrsize[ARR_PHI];
+      double dtg = wght;
+      for (int k = c->ks; k <= c->ke; ++k)
+        for (int j = c->js; j <= c->je; ++j)
+          for (int i = c->is; i <= c->ie; ++i) {
+            int64_t id = CIDX(c, k, j, i);
+            double dtodx1 = dtg / dx1[i];   /* dx1v == dx1f on a uniform mesh */
+            double phic = phi[id];
+            double phil = 0.5 * (phi[id - 1] + phi[id]);
+            double phir = 0.5 * (phi[id] + phi[id + 1]);
+            u[IVX * nc + id] -= dtodx1 * w[IDN * nc + id] * (phir - phil);
+            double fl = f1[IDN * c->nf1 + F1IDX(c, k, j, i)], fr = f1[IDN * c->nf1 + F1IDX(c, k, j, i + 1)];
+            u[IEN * nc + id] -= dtodx1 * (fl * (phic - phil) + fr * (phir - phic));
+          }
+      for (int k = c->ks; k <= c->ke; ++k)
+        for (int j = c->js; j <= c->je; ++j)
+          for (int i = c->is; i <= c->ie; ++i) {
+            int64_t id = CIDX(c, k, j, i);
+            double dtodx2 = dtg / dx2[j];
+            double phic = phi[id];
+            double phil = 0.5 * (phi[id - c->n1] + phi[id]);
+            double phir = 0.5 * (phi[id] + phi[id + c->n1]);
+            u[IVY * nc + id] -= dtodx2 * w[IDN * nc + id] * (phir - phil);
+            u[IEN * nc + id] -= dtodx2 * (f2[IDN * c->nf2 + F2IDX(c, k, j, i)] * (phic - phil) +
+                                          f2[IDN * c->nf2 + F2IDX(c, k, j + 1, i)] * (phir - phic));
+          }
+      int64_t sk = (int64_t)c->n1 * c->n2;
+      for (int k = c->ks; k <= c->ke; ++k)
+        for (int j = c->js; j <= c->je; ++j)
+          for (int i = c->is; i <= c->ie; ++i) {
+            int64_t id = CIDX(c, k, j, i);
+            double dtodx3 = dtg / dx3[k];
+            double phic = phi[id];
+            double phil = 0.5 * (phi[id - sk] + phi[id]);
+            double phir = 0.5 * (phi[id] + phi[id + sk]);
+            u[IVZ * nc + id] -= dtodx3 * w[IDN * nc + id] * (phir - phil);
+            u[IEN * nc + id] -= dtodx3 * (f3[IDN * c->nf3 + F3IDX(c, k, j, i)] * (phic - phil) +
+                                          f3[IDN * c->nf3 + F3IDX(c, k + 1, j, i)] * (phir - phic));
+          }
+    }
+    /* Field::CT */
+    double *b1 = c->arr[ARR_B1F] + g * c->nf1, *b2 = c->arr[ARR_B2F] + g * c->nf2,
+           *b3 = c->arr[ARR_B3F] + g * c->nf3;
+    const double *e1 = c->arr[ARR_E1] + g * c->ne1, *e2 = c->arr[ARR_E2] + g * c->ne2,
+                 *e3 = c->arr[ARR_E3] + g * c->ne3;
+    for (int k = c->ks; k <= c->ke; ++k)
+      for (int j = c->js; j <= c->je; ++j)
+        for (int i = c->is; i <= c->ie + 1; ++i) {
+          double area = dx2[j] * dx3[k];
+          double len = dx3[k], len_p1 = dx3[k];
+          b1[F1IDX(c, k, j, i)] -= (wght / area) * (len_p1 * e3[E3IDX(c, k, j + 1, i)] - len * e3[E3IDX(c, k, j, i)]);
+          len = dx2[j]; len_p1 = dx2[j];
+          b1[F1IDX(c, k, j, i)] += (wght / area) * (len_p1 * e2[E2IDX(c, k + 1, j, i)] - len * e2[E2IDX(c, k, j, i)]);
+        }
+    for (int k = c->ks; k <= c->ke; ++k)
+      for (int j = c->js; j <= c->je + 1; ++j)
+        for (int i = c->is; i <= c->ie; ++i) {
+          double area = dx1[i] * dx3[k];
+          double len = dx3[k], len_ip1 = dx3[k];
+          b2[F2IDX(c, k, j, i)] += (wght / area) * (len_ip1 * e3[E3IDX(c, k, j, i + 1)] - len * e3[E3IDX(c, k, j, i)]);
+          len = dx1[i]; double len_p1 = dx1[i];
+          b2[F2IDX(c, k, j, i)] -= (wght / area) * (len_p1 * e1[E1IDX(c, k + 1, j, i)] - len * e1[E1IDX(c, k, j, i)]);
+        }
+    for (int k = c->ks; k <= c->ke + 1; ++k)
+      for (int j = c->js; j <= c->je; ++j)
+        for (int i = c->is; i <= c->ie; ++i) {
+          double area = dx1[i] * dx2[j];
+          double len = dx2[j], len_ip1 = dx2[j];
+          b3[F3IDX(c, k, j, i)] -= (wght / area) * (len_ip1 * e2[E2IDX(c, k, j, i + 1)] - len * e2[E2IDX(c, k, j, i)]);
+          len = dx1[i]; double len_p1 = dx1[i];
+          b3[F3IDX(c, k, j, i)] += (wght / area) * (len_p1 * e1[E1IDX(c, k, j + 1, i)] - len * e1[E1IDX(c, k, j, i)]);
+        }
+  }
+}
+
+/* ---------------------------------------------------------------- time step */
+/* Hydro::NewBlockTimeStep (hydro/new_blockdt.cpp:42-190), min over all blocks. */
+double orc_new_block_dt(orc_ctx *c) {
+  double gamma = c->d.gamma;
+  double mind = DBL_MAX;
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : mind)
+  for (int g = 0; g < c->nblocks; ++g) {
+    int64_t nc = c->ncell;
+    const double *w = c->arr[ARR_W] + g * c->arrsize[ARR_W];
+    const double *bcc = c->arr[ARR_BCC] + g * c->arrsize[ARR_BCC];
+    const double *b1 = c->arr[ARR_B1F] + g * c->nf1, *b2 = c->arr[ARR_B2F] + g * c->nf2,
+                 *b3 = c->arr[ARR_B3F] + g * c->nf3;
+    const double *dx1 = c->dx1f + g * (c->n1 + 1), *dx2 = c->dx2f + g * (c->n2 + 1),
+                 *dx3 = c->dx3f + g * (c->n3 + 1);
+    double m = DBL_MAX;
+    for (int k = c->ks; k <= c->ke; ++k)
+      for (int j = c->js; j <= c->je; ++j)
+        for (int i = c->is; i <= c->ie; ++i) {
+          int64_t id = CIDX(c, k, j, i);
+          double wi[NWAVE];
+          wi[IDN] = w[IDN * nc + id]; wi[IVX] = w[IVX * nc + id]; wi[IVY] = w[IVY * nc + id];
+          wi[IVZ] = w[IVZ * nc + id]; wi[IPR] = w[IPR * nc + id];
+          double bx = bcc[IB1 * nc + id] + fabs(b1[F1IDX(c, k, j, i)] - bcc[IB1 * nc + id]);
+          wi[IBY] = bcc[IB2 * nc + id]; wi[IBZ] = bcc[IB3 * nc + id];
+          double cf = fast_speed(gamma, wi, bx);
+          double dt1 = dx1[i] / (fabs(wi[IVX]) + cf);
+          wi[IBY] = bcc[IB3 * nc + id]; wi[IBZ] = bcc[IB1 * nc + id];
+          bx = bcc[IB2 * nc + id] + fabs(b2[F2IDX(c, k, j, i)] - bcc[IB2 * nc + id]);
+          cf = fast_speed(gamma, wi, bx);
+          double dt2 = dx2[j] / (fabs(wi[IVY]) + cf);
+          wi[IBY] = bcc[IB1 * nc + id]; wi[IBZ] = bcc[IB2 * nc + id];
+          bx = bcc[IB3 * nc + id] + fabs(b3[F3IDX(c, k, j, i)] - bcc[IB3 * nc + id]);
+          cf = fast_speed(gamma, wi, bx);
+          double dt3 = dx3[k] / (fabs(wi[IVZ]) + cf);
+          m = dmin(m, dt1); m = dmin(m, dt2); m = dmin(m, dt3);
+        }
+    m *= c->d.cfl;
+    mind = dmin(mind, m);
+  }
+  return mind;
+}
+
+/* Mesh::NewTimeStep, mesh/mesh.cpp:1078-1119 */
+void orc_new_time_step(orc_ctx *c, double tlim) {
+  double bdt = orc_new_block_dt(c);
+  double dt = 2.0 * c->dt;
+  dt = dmin(dt, bdt);
+  if (c->time < tlim && (tlim - c->time) < dt) dt = tlim - c->time;
+  c->dt = dt;
+}
+
+/* The blocking part of Mesh::Initialize (mesh.cpp:1401-1649): first MG solve, ghost
+ * exchange of u and b, ConservedToPrimitive on interior + ghosts, first dt. */
+void orc_initialize(orc_ctx *c) {
+  if (c->d.self_gravity) orc_mg_solve(c, 0);
+  orc_bvals_cc(c, ARR_U);
+  orc_bvals_fc(c, 0);
+  orc_cons2prim(c);
+  c->dt = DBL_MAX;
+  orc_new_time_step(c, c->tlim_hint > 0 ? c->tlim_hint : DBL_MAX);
+}
+
+/* One cycle of the main loop (main.cpp:458-489) with the VL2 task list: stage 1 uses
+ * donor-cell reconstruction (time_integrator.cpp:1466-1467), stage 2 xorder; MG solve
+ * after each stage (main.cpp:459-466); then time += dt and NewTimeStep. */
+void orc_cycle(orc_ctx *c, double tlim) {
+  for (int stage = 1; stage <= 2; ++stage) {
+    orc_calculate_fluxes(c, stage == 1 ? 1 : c->d.xorder);
+    orc_corner_e(c);
+    orc_integrate(c, stage);
+    orc_bvals_cc(c, ARR_U);
+    orc_bvals_fc(c, 0);
+    orc_cons2prim(c);
+    if (c->d.self_gravity) orc_mg_solve(c, stage);
+  }
+  c->ncycle++;
+  c->time += c->dt;
+  orc_new_time_step(c, tlim);
+}
+
+/* ================================================================ multigrid */
+#define MIDX(m, l, k, j, i) ((((int64_t)(k)) * ((m)->n2[l] + 2) + (j)) * ((m)->n1[l] + 2) + (i))
+
+/* MGGravity::Smooth, gravity/mg_gravity.cpp:220-273 (omega = 1.15, mg_gravity.hpp:43) */
+static void mg_smooth(orc_ctx *c, mglevels *m, int l, int color) {
+  int ll = m->nlev - 1 - l;
+  double dx = m->rdx * (double)(1 << ll);
+  double dx2 = SQR(dx);
+  double isix = 1.15 / 6.0;
+  color ^= c->coffset;
+  int nx = m->n1[l] + 2, ny = m->n2[l] + 2;
+  double *u = m->u[l];
+  const double *src = m->src[l];
+  int64_t sj = nx, sk = (int64_t)nx * ny;
+  for (int k = 1; k <= m->n3[l]; ++k)
+    for (int j = 1; j <= m->n2[l]; ++j) {
+      int cc = (color + k + j) & 1;
+      for (int i = 1 + cc; i <= m->n1[l]; i += 2) {
+        int64_t id = MIDX(m, l, k, j, i);
+        u[id] -= ((6.0 * u[id] - u[id + sk] - u[id + sj] - u[id + 1] - u[id - sk] - u[id - sj] - u[id - 1]) +
+                  src[id] * dx2) * isix;
+      }
+    }
+}
+
+/* MGGravity::CalculateDefect, gravity/mg_gravity.cpp:283-314 */
+static void mg_defect(mglevels *m, int l) {
+  int ll = m->nlev - 1 - l;
+  double dx = m->rdx * (double)(1 << ll);
+  double idx2 = 1.0 / SQR(dx);
+  int nx = m->n1[l] + 2, ny = m->n2[l] + 2;
+  int64_t sj = nx, sk = (int64_t)nx * ny;
+  const double *u = m->u[l], *src = m->src[l];
+  double *def = m->def[l];
+  for (int k = 1; k <= m->n3[l]; ++k)
+    for (int j = 1; j <= m->n2[l]; ++j)
+      for (int i = 1; i <= m->n1[l]; ++i) {
+        int64_t id = MIDX(m, l, k, j, i);
+        def[id] = (6.0 * u[id] - u[id + sk] - u[id + sj] - u[id + 1] - u[id - sk] - u[id - sj] - u[id - 1]) * idx2 +
+                  src[id];
+      }
+}
+
+/* Multigrid::Restrict, multigrid/multigrid.cpp:684-724 (fine level l -> l-1) */
+static void mg_restrict(mglevels *m, double *dst, const double *src, int l) {
+  int lc = l - 1;
+  for (int k = 1; k <= m->n3[lc]; ++k) {
+    int fk = 2 * k - 1;
+    for (int j = 1; j <= m->n2[lc]; ++j) {
+      int fj = 2 * j - 1;
+      for (int i = 1; i <= m->n1[lc]; ++i) {
+        int fi = 2 * i - 1;
+#define S(kk, jj, ii) src[MIDX(m, l, kk, jj, ii)]
+        dst[MIDX(m, lc, k, j, i)] = 0.125 * (S(fk, fj, fi) + S(fk, fj, fi + 1) + S(fk, fj + 1, fi) + S(fk, fj + 1, fi + 1) +
+                                             S(fk + 1, fj, fi) + S(fk + 1, fj, fi + 1) + S(fk + 1, fj + 1, fi) +
+                                             S(fk + 1, fj + 1, fi + 1));
+#undef S
+      }
+    }
+  }
+}
+
+/* Multigrid::ProlongateAndCorrect, multigrid/multigrid.cpp:733-829 (l-1 -> l) */
+static void mg_prolongate(mglevels *m, int l) {
+  int lc = l - 1;
+  const double *s = m->u[lc];
+  double *d = m->u[l];
+  for (int k = 1; k <= m->n3[lc]; ++k) {
+    int fk = 2 * (k - 1) + 1;
+    for (int j = 1; j <= m->n2[lc]; ++j) {
+      int fj = 2 * (j - 1) + 1;
+      for (int i = 1; i <= m->n1[lc]; ++i) {
+        int fi = 2 * (i - 1) + 1;
+#define S(kk, jj, ii) s[MIDX(m, lc, kk, jj, ii)]
+#define D(kk, jj, ii) d[MIDX(m, l, kk, jj, ii)]
+        for (int ok = 0; ok < 2; ++ok)
+          for (int oj = 0; oj < 2; ++oj)
+            for (int oi = 0; oi < 2; ++oi) {
+              int dk = ok ? 1 : -1, dj = oj ? 1 : -1, di = oi ? 1 : -1;
+              D(fk + ok, fj + oj, fi + oi) +=
+                  0.015625 * (27.0 * S(k, j, i) + S(k + dk, j + dj, i + di) +
+                              9.0 * (S(k, j, i + di) + S(k, j + dj, i) + S(k + dk, j, i)) +
+                              3.0 * (S(k + dk, j + dj, i) + S(k + dk, j, i + di) + S(k, j + dj, i + di)));
+            }
+#undef S
+#undef D
+      }
+    }
+  }
+}
+
+/* Multigrid::FMGProlongate, multigrid/multigrid.cpp:838-1048: tricubic-type
+ * interpolation, the tensor product of 1-D weights (5,30,-3)/32 toward the fine
+ * cell's side, summed term by term in (k,j,i) row-major order and divided by 32768. */
+static void mg_fmg_prolongate(mglevels *m, int l) {
+  int lc = l - 1;
+  const double *s = m->u[lc];
+  double *d = m->u[l];
+  static const double wlo[3] = {5.0, 30.0, -3.0}, whi[3] = {-3.0, 30.0, 5.0};
+  for (int k = 1; k <= m->n3[lc]; ++k) {
+    int fk = 2 * (k - 1) + 1;
+    for (int j = 1; j <= m->n2[lc]; ++j) {
+      int fj = 2 * (j - 1) + 1;
+      for (int i = 1; i <= m->n1[lc]; ++i) {
+        int fi = 2 * (i - 1) + 1;
+        for (int ok = 0; ok < 2; ++ok)
+          for (int oj = 0; oj < 2; ++oj)
+            for (int oi = 0; oi < 2; ++oi) {
+              const double *wk = ok ? whi : wlo, *wj = oj ? whi : wlo, *wi = oi ? whi : wlo;
+              double acc = 0.0;
+              int first = 1;
+              for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                  for (int e = 0; e < 3; ++e) {
+                    double coef = wk[a] * wj[b] * wi[e];
+                    double t = coef * s[MIDX(m, lc, k - 1 + a, j - 1 + b, i - 1 + e)];
+                    if (first) { acc = t; first = 0; }
+                    else acc = acc + t;
+                  }
+              d[MIDX(m, l, fk + ok, fj + oj, fi + oi)] = acc / 32768.0;
+            }
+      }
+    }
+  }
+}
+
+/* Same-level ghost fill of every block's level l (bvals_mg.cpp:447-520,634-747,
+ * 866-915): each ghost cell copies the neighbour block's interior cell. */
+static void mg_fill_blocks(orc_ctx *c, int l, int which) {
+  int nb = c->nblocks;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nb; ++g) {
+    mglevels *m = &c->mgb[g];
+    int n = m->n1[l];
+    double *a = (which == 0) ? m->u[l] : m->src[l];
+    for (int k = 0; k <= n + 1; ++k) {
+      int o3 = (k == 0) ? -1 : (k == n + 1 ? 1 : 0);
+      for (int j = 0; j <= n + 1; ++j) {
+        int o2 = (j == 0) ? -1 : (j == n + 1 ? 1 : 0);
+        for (int i = 0; i <= n + 1; ++i) {
+          int o1 = (i == 0) ? -1 : (i == n + 1 ? 1 : 0);
+          if (!o1 && !o2 && !o3) continue;
+          mglevels *s = &c->mgb[nbr_gid(c, g, o1, o2, o3)];
+          double *sa = (which == 0) ? s->u[l] : s->src[l];
+          a[MIDX(m, l, k, j, i)] = sa[MIDX(m, l, k - o3 * n, j - o2 * n, i - o1 * n)];
+        }
+      }
+    }
+  }
+}
+
+/* Root grid periodic physical boundaries, MGBoundaryValues::ApplyPhysicalBoundaries
+ * (bvals_mg.cpp:321-370) with MGPeriodic* (multigrid/mgbval_periodic.cpp:26-144):
+ * x1 faces over interior (j,k), then x2 over all i, then x3 over all (i,j). */
+static void mg_root_bc(mglevels *m, int l) {
+  int n1 = m->n1[l], n2 = m->n2[l], n3 = m->n3[l];
+  double *u = m->u[l];
+  for (int k = 1; k <= n3; ++k)
+    for (int j = 1; j <= n2; ++j) {
+      u[MIDX(m, l, k, j, 0)] = u[MIDX(m, l, k, j, n1)];
+      u[MIDX(m, l, k, j, n1 + 1)] = u[MIDX(m, l, k, j, 1)];
+    }
+  for (int k = 1; k <= n3; ++k)
+    for (int i = 0; i <= n1 + 1; ++i) {
+      u[MIDX(m, l, k, 0, i)] = u[MIDX(m, l, k, n2, i)];
+      u[MIDX(m, l, k, n2 + 1, i)] = u[MIDX(m, l, k, 1, i)];
+    }
+  for (int j = 0; j <= n2 + 1; ++j)
+    for (int i = 0; i <= n1 + 1; ++i) {
+      u[MIDX(m, l, 0, j, i)] = u[MIDX(m, l, n3, j, i)];
+      u[MIDX(m, l, n3 + 1, j, i)] = u[MIDX(m, l, 1, j, i)];
+    }
+}
+
+static void mg_zero(mglevels *m, int l) {
+  size_t sz = (size_t)(m->n1[l] + 2) * (m->n2[l] + 2) * (m->n3[l] + 2);
+  memset(m->u[l], 0, sz * sizeof(double));
+}
+
+/* blocks: smooth red/black with ghost fills, then RestrictBlock
+ * (mg_task_list.cpp:382-... nsmooth=1; multigrid.cpp:331-355) */
+static void mg_blocks_to_coarser(orc_ctx *c, int bl) {
+  mg_fill_blocks(c, bl, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < c->nblocks; ++g) mg_smooth(c, &c->mgb[g], bl, 0);
+  mg_fill_blocks(c, bl, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < c->nblocks; ++g) mg_smooth(c, &c->mgb[g], bl, 1);
+  mg_fill_blocks(c, bl, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < c->nblocks; ++g) {
+    mglevels *m = &c->mgb[g];
+    mg_defect(m, bl);
+    mg_restrict(m, m->src[bl - 1], m->def[bl], bl);
+    mg_zero(m, bl - 1);
+  }
+}
+
+static void mg_root_to_coarser(orc_ctx *c, int rl) {
+  mglevels *m = &c->mgr;
+  mg_root_bc(m, rl);
+  mg_smooth(c, m, rl, 0);
+  mg_root_bc(m, rl);
+  mg_smooth(c, m, rl, 1);
+  mg_root_bc(m, rl);
+  mg_defect(m, rl);
+  mg_restrict(m, m->src[rl - 1], m->def[rl], rl);
+  mg_zero(m, rl - 1);
+}
+
+/* TransferFromBlocksToRoot (multigrid_driver.cpp:467-527), non-FAS */
+static void mg_blocks_to_root(orc_ctx *c, int initflag) {
+  mglevels *r = &c->mgr;
+  int rl = c->nrootlevel - 1;
+  for (int g = 0; g < c->nblocks; ++g) {
+    int lx1 = g % c->nb1, lx2 = (g / c->nb1) % c->nb2, lx3 = g / (c->nb1 * c->nb2);
+    r->src[rl][MIDX(r, rl, lx3 + 1, lx2 + 1, lx1 + 1)] = c->mgb[g].src[0][MIDX(&c->mgb[g], 0, 1, 1, 1)];
+  }
+  (void)initflag;
+}
+
+/* TransferFromRootToBlocks / SetFromRootGrid (multigrid.cpp:480-550) */
+static void mg_root_to_blocks(orc_ctx *c) {
+  mglevels *r = &c->mgr;
+  int rl = c->nrootlevel - 1;
+  for (int g = 0; g < c->nblocks; ++g) {
+    int lx1 = g % c->nb1, lx2 = (g / c->nb1) % c->nb2, lx3 = g / (c->nb1 * c->nb2);
+    mglevels *m = &c->mgb[g];
+    for (int k = 0; k <= 2; ++k)
+      for (int j = 0; j <= 2; ++j)
+        for (int i = 0; i <= 2; ++i)
+          m->u[0][MIDX(m, 0, k, j, i)] = r->u[rl][MIDX(r, rl, lx3 + k, lx2 + j, lx1 + i)];
+  }
+}
+
+static void mg_one_step_to_coarser(orc_ctx *c) {
+  int cur = c->current;
+  if (cur >= c->nrootlevel) {
+    int bl = cur - c->nrootlevel + 1;
+    mg_blocks_to_coarser(c, bl);
+    if (cur == c->nrootlevel) {
+      mg_blocks_to_root(c, 0);
+      mg_zero(&c->mgr, c->nrootlevel - 1);
+    }
+  } else {
+    mg_root_to_coarser(c, cur);
+  }
+  c->current--;
+}
+
+/* SolveCoarsestGrid, multigrid_driver.cpp:778-819 (periodic) */
+static void mg_root_total_sub(orc_ctx *c, int l) {
+  mglevels *m = &c->mgr;
+  int ll = m->nlev - 1 - l;
+  double dx = m->rdx * (double)(1 << ll);
+  double rdy = (c->d.x2max - c->d.x2min) / (double)c->nb2;
+  double rdz = (c->d.x3max - c->d.x3min) / (double)c->nb3;
+  double dy = rdy * (double)(1 << ll), dz = rdz * (double)(1 << ll);
+  double s = 0.0;
+  for (int k = 1; k <= m->n3[l]; ++k)
+    for (int j = 1; j <= m->n2[l]; ++j)
+      for (int i = 1; i <= m->n1[l]; ++i) s += m->u[l][MIDX(m, l, k, j, i)];
+  double vol = (c->d.x1max - c->d.x1min) * (c->d.x2max - c->d.x2min) * (c->d.x3max - c->d.x3min);
+  double ave = s * dx * dy * dz / vol;
+  /* Multigrid::SubtractAverage acts on the finest level of the root object
+   * (multigrid.cpp:627-641) */
+  mglevels *mm = m;
+  int lf = m->nlev - 1;
+  size_t sz = (size_t)(mm->n1[lf] + 2) * (mm->n2[lf] + 2) * (mm->n3[lf] + 2);
+  for (size_t q = 0; q < sz; ++q) mm->u[lf][q] -= ave;
+}
+
+static void mg_solve_coarsest(orc_ctx *c) {
+  int ni = c->nb1 > c->nb2 ? c->nb1 : c->nb2;
+  ni = ni > c->nb3 ? ni : c->nb3;
+  ni >>= (c->nrootlevel - 1);
+  if (ni == 1) {
+    mg_zero(&c->mgr, 0);
+  } else {
+    mg_root_total_sub(c, 0);
+    mg_root_bc(&c->mgr, 0);
+    for (int it = 0; it < ni; ++it) {
+      mg_smooth(c, &c->mgr, 0, 0);
+      mg_root_bc(&c->mgr, 0);
+      mg_smooth(c, &c->mgr, 0, 1);
+      mg_root_bc(&c->mgr, 0);
+    }
+    mg_root_total_sub(c, 0);
+  }
+}
+
+static void mg_one_step_to_finer(orc_ctx *c) {
+  int cur = c->current;
+  if (cur == c->nrootlevel - 1) {          /* multigrid_driver.cpp:586-590 */
+    mg_root_bc(&c->mgr, cur);
+    mg_root_to_blocks(c);
+  }
+  if (cur >= c->nrootlevel - 1) {          /* block levels, SetMGTaskListToFiner */
+    int bl = cur - c->nrootlevel + 1;      /* coarse block level */
+    /* flag 1 (first step on block levels): the coarse ghosts came from the root grid;
+     * otherwise they are exchanged first (mg_task_list.cpp:287-300) */
+    if (cur != c->nrootlevel - 1) mg_fill_blocks(c, bl, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int g = 0; g < c->nblocks; ++g) mg_prolongate(&c->mgb[g], bl + 1);
+    mg_fill_blocks(c, bl + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int g = 0; g < c->nblocks; ++g) mg_smooth(c, &c->mgb[g], bl + 1, 0);
+    mg_fill_blocks(c, bl + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int g = 0; g < c->nblocks; ++g) mg_smooth(c, &c->mgb[g], bl + 1, 1);
+    if (cur == c->ntotal - 2) mg_fill_blocks(c, bl + 1, 0);   /* flag 2: last */
+  } else {                                  /* root grid */
+    mglevels *m = &c->mgr;
+    mg_root_bc(m, cur);
+    mg_prolongate(m, cur + 1);
+    mg_root_bc(m, cur + 1);
+    mg_smooth(c, m, cur + 1, 0);
+    mg_root_bc(m, cur + 1);
+    mg_smooth(c, m, cur + 1, 1);
+  }
+  c->current++;
+}
+
+static void mg_vcycle(orc_ctx *c) {   /* SolveVCycle(1,1), multigrid_driver.cpp:684-693 */
+  int start = c->current;
+  c->coffset ^= 1;
+  while (c->current > 0) mg_one_step_to_coarser(c);
+  mg_solve_coarsest(c);
+  while (c->current < start) mg_one_step_to_finer(c);
+}
+
+static void mg_fmg_prolongate_step(orc_ctx *c) {   /* multigrid_driver.cpp:553-575 */
+  int cur = c->current;
+  if (cur == c->nrootlevel - 1) {
+    mg_root_bc(&c->mgr, cur);
+    mg_root_to_blocks(c);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int g = 0; g < c->nblocks; ++g) mg_fmg_prolongate(&c->mgb[g], 1);
+  } else if (cur >= c->nrootlevel - 1) {
+    int bl = cur - c->nrootlevel + 1;
+    mg_fill_blocks(c, bl, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int g = 0; g < c->nblocks; ++g) mg_fmg_prolongate(&c->mgb[g], bl + 1);
+  } else {
+    mg_root_bc(&c->mgr, cur);
+    mg_fmg_prolongate(&c->mgr, cur + 1);
+  }
+  c->current++;
+}
+
+/* SubtractAverage (multigrid_driver.cpp:305-330) for src (type 0) or u (type 1) of
+ * the blocks' finest level; the per-block totals are CalculateTotal
+ * (multigrid.cpp:602-625). */
+static double mg_subtract_average(orc_ctx *c, int type) {
+  int l = c->nmblevel - 1;
+  double total = 0.0;
+  for (int g = 0; g < c->nblocks; ++g) {
+    mglevels *m = &c->mgb[g];
+    double *a = type ? m->u[l] : m->src[l];
+    double s = 0.0;
+    for (int k = 1; k <= m->n3[l]; ++k)
+      for (int j = 1; j <= m->n2[l]; ++j)
+        for (int i = 1; i <= m->n1[l]; ++i) s += a[MIDX(m, l, k, j, i)];
+    double dx = m->rdx;
+    int lx2 = (g / c->nb1) % c->nb2, lx3 = g / (c->nb1 * c->nb2);
+    (void)lx2; (void)lx3;
+    double dy = (c->d.x2max - c->d.x2min) / (double)c->d.nx2;   /* rdy_ of the block */
+    double dz = (c->d.x3max - c->d.x3min) / (double)c->d.nx3;
+    {
+      /* rdy_/rdz_ are block extents / block cells (multigrid.cpp:61-63) */
+      double b2min = (lx2 == 0) ? c->d.x2min : uniform_gen(meshgen_x(lx2, c->nb2), c->d.x2min, c->d.x2max);
+      double b2max = (lx2 == c->nb2 - 1) ? c->d.x2max : uniform_gen(meshgen_x(lx2 + 1, c->nb2), c->d.x2min, c->d.x2max);
+      double b3min = (lx3 == 0) ? c->d.x3min : uniform_gen(meshgen_x(lx3, c->nb3), c->d.x3min, c->d.x3max);
+      double b3max = (lx3 == c->nb3 - 1) ? c->d.x3max : uniform_gen(meshgen_x(lx3 + 1, c->nb3), c->d.x3min, c->d.x3max);
+      dy = (b2max - b2min) / (double)c->d.bnx2;
+      dz = (b3max - b3min) / (double)c->d.bnx3;
+    }
+    total += s * dx * dy * dz;
+  }
+  double vol = (c->d.x1max - c->d.x1min) * (c->d.x2max - c->d.x2min) * (c->d.x3max - c->d.x3min);
+  double ave = total / vol;
+  for (int g = 0; g < c->nblocks; ++g) {
+    mglevels *m = &c->mgb[g];
+    double *a = type ? m->u[l] : m->src[l];
+    size_t sz = (size_t)(m->n1[l] + 2) * (m->n2[l] + 2) * (m->n3[l] + 2);
+    for (size_t q = 0; q < sz; ++q) a[q] -= ave;
+  }
+  return ave;
+}
+
+/* MultigridDriver::CalculateDefectNorm (L2), multigrid_driver.cpp:826-858 with
+ * Multigrid::CalculateDefectNorm, multigrid.cpp:555-595 (current = finest) */
+static double mg_defect_norm(orc_ctx *c) {
+  int l = c->nmblevel - 1;
+  double norm = 0.0;
+  for (int g = 0; g < c->nblocks; ++g) {
+    mglevels *m = &c->mgb[g];
+    mg_defect(m, l);
+    double s = 0.0;
+    for (int k = 1; k <= m->n3[l]; ++k)
+      for (int j = 1; j <= m->n2[l]; ++j)
+        for (int i = 1; i <= m->n1[l]; ++i) s += SQR(m->def[l][MIDX(m, l, k, j, i)]);
+    int lx2 = (g / c->nb1) % c->nb2, lx3 = g / (c->nb1 * c->nb2);
+    double b2min = (lx2 == 0) ? c->d.x2min : uniform_gen(meshgen_x(lx2, c->nb2), c->d.x2min, c->d.x2max);
+    double b2max = (lx2 == c->nb2 - 1) ? c->d.x2max : uniform_gen(meshgen_x(lx2 + 1, c->nb2), c->d.x2min, c->d.x2max);
+    double b3min = (lx3 == 0) ? c->d.x3min : uniform_gen(meshgen_x(lx3, c->nb3), c->d.x3min, c->d.x3max);
+    double b3max = (lx3 == c->nb3 - 1) ? c->d.x3max : uniform_gen(meshgen_x(lx3 + 1, c->nb3), c->d.x3min, c->d.x3max);
+    double dy = (b2max - b2min) / (double)c->d.bnx2, dz = (b3max - b3min) / (double)c->d.bnx3;
+    norm += s * m->rdx * dy * dz;
+  }
+  double vol = (c->d.x1max - c->d.x1min) * (c->d.x2max - c->d.x2min) * (c->d.x3max - c->d.x3min);
+  norm /= vol;
+  return sqrt(norm);
+}
+
+void orc_mg_set_mode(orc_ctx *c, int mode, double threshold, int niter) {
+  c->mg_mode = mode; c->mg_eps = threshold; c->mg_niter = niter;
+}
+double orc_mg_last_norm(orc_ctx *c) { return c->last_norm; }
+int orc_mg_last_iters(orc_ctx *c) { return c->last_iters; }
+
+/* MGGravityDriver::Solve, gravity/mg_gravity.cpp:166-211 (periodic boundaries:
+ * fsubtract_average_ = true, no FAS, no source mask). */
+void orc_mg_solve(orc_ctx *c, int stage) {
+  (void)stage;
+  int lf = c->nmblevel - 1;
+  double fpg = c->d.four_pi_G;
+  /* LoadSource (multigrid.cpp:194-231) and, in MGI mode, LoadFinestData (:160-180) */
+  for (int g = 0; g < c->nblocks; ++g) {
+    mglevels *m = &c->mgb[g];
+    const double *u = c->arr[ARR_U] + g * c->arrsize[ARR_U];
+    const double *phi = c->arr[ARR_PHI] + g * c->arrsize[ARR_PHI];
+    for (int k = 1; k <= m->n3[lf]; ++k)
+      for (int j = 1; j <= m->n2[lf]; ++j)
+        for (int i = 1; i <= m->n1[lf]; ++i) {
+          int64_t hid = CIDX(c, k - 1 + c->ng, j - 1 + c->ng, i - 1 + c->ng);
+          double v = u[IDN * c->ncell + hid];
+          m->src[lf][MIDX(m, lf, k, j, i)] = (fpg == 1.0) ? v : v * fpg;
+          if (c->mg_mode == 1) m->u[lf][MIDX(m, lf, k, j, i)] = phi[hid];
+        }
+  }
+  c->current = c->ntotal - 1;
+  c->fmglevel = c->ntotal - 1;
+  /* SetupMultigrid, multigrid_driver.cpp:343-460 */
+  c->last_ave = mg_subtract_average(c, 0);
+  if (c->mg_mode == 0) {
+    for (int g = 0; g < c->nblocks; ++g) {   /* RestrictFMGSource, multigrid.cpp:253-262 */
+      mglevels *m = &c->mgb[g];
+      for (int l = m->nlev - 1; l > 0; --l) mg_restrict(m, m->src[l - 1], m->src[l], l);
+    }
+    mg_blocks_to_root(c, 1);
+    for (int l = c->mgr.nlev - 1; l > 0; --l) mg_restrict(&c->mgr, c->mgr.src[l - 1], c->mgr.src[l], l);
+    c->current = 0;
+    /* SolveFMGCycle, multigrid_driver.cpp:700-714 */
+    for (c->fmglevel = 0; c->fmglevel < c->ntotal; c->fmglevel++) {
+      mg_vcycle(c);
+      if (c->fmglevel != c->ntotal - 1) mg_fmg_prolongate_step(c);
+    }
+    c->fmglevel = c->ntotal - 1;
+    mg_subtract_average(c, 1);
+  }
+  int iters = 0;
+  if (c->mg_eps >= 0.0) {       /* SolveIterative, multigrid_driver.cpp:721-755 */
+    double def = mg_defect_norm(c);
+    int n = 0;
+    while (def > c->mg_eps) {
+      mg_vcycle(c);
+      iters++;
+      double olddef = def;
+      def = mg_defect_norm(c);
+      if (def / olddef > 0.8) {
+        if (c->mg_eps == 0.0) break;
+      }
+      if (n > 100) break;
+      n++;
+    }
+    c->last_norm = def;
+    mg_subtract_average(c, 1);
+  } else {                       /* SolveIterativeFixedTimes, :762-771 */
+    for (int n = 0; n < c->mg_niter; ++n) { mg_vcycle(c); iters++; }
+    mg_subtract_average(c, 1);
+    c->last_norm = mg_defect_norm(c);
+  }
+  c->last_iters = iters;
+  /* RetrieveResult with one ghost layer (multigrid.cpp:269-288) */
+  for (int g = 0; g < c->nblocks; ++g) {
+    mglevels *m = &c->mgb[g];
+    double *phi = c->arr[ARR_PHI] + g * c->arrsize[ARR_PHI];
+    int n = m->n1[lf];
+    for (int k = 0; k <= n + 1; ++k)
+      for (int j = 0; j <= n + 1; ++j)
+        for (int i = 0; i <= n + 1; ++i)
+          phi[CIDX(c, k - 1 + c->ng, j - 1 + c->ng, i - 1 + c->ng)] = m->u[lf][MIDX(m, lf, k, j, i)];
+  }
+}
+
+/* ================================================================ problem setup */
+/* Adiabatic-MHD right eigenvectors in conserved variables at a uniform state, the
+ * Roe-matrix eigensystem used by pgen/linear_wave.cpp:626-800 (x=0, y=1). */
+static void mhd_right_eigenvectors(double gamma, double d, double v1, double v2, double v3,
+                                   double h, double b1, double b2, double b3,
+                                   double ev[7], double rem[7][7]) {
+  double gm1 = gamma - 1.0, x = 0.0, y = 1.0;
+  double vsq = v1 * v1 + v2 * v2 + v3 * v3;
+  double btsq = b2 * b2 + b3 * b3;
+  double bt_starsq = (gm1 - (gm1 - 1.0) * y) * btsq;
+  double vaxsq = b1 * b1 / d;
+  double hp = h - (vaxsq + btsq / d);
+  double twid_asq = dmax((gm1 * (hp - 0.5 * vsq) - (gm1 - 1.0) * x), TINY_NUMBER);
+  double ct2 = bt_starsq / d;
+  double tsum = vaxsq + ct2 + twid_asq;
+  double tdif = vaxsq + ct2 - twid_asq;
+  double cf2_cs2 = sqrt(tdif * tdif + 4.0 * twid_asq * ct2);
+  double cfsq = 0.5 * (tsum + cf2_cs2);
+  double cf = sqrt(cfsq);
+  double cssq = twid_asq * vaxsq / cfsq;
+  double cs = sqrt(cssq);
+  double bt = sqrt(btsq);
+  double bet2, bet3;
+  if (bt == 0.0) { bet2 = 1.0; bet3 = 0.0; }
+  else { bet2 = b2 / bt; bet3 = b3 / bt; }
+  double bt_star = sqrt(bt_starsq);
+  double bet2_star = bet2 / sqrt(gm1 - (gm1 - 1.0) * y);
+  double bet3_star = bet3 / sqrt(gm1 - (gm1 - 1.0) * y);
+  double bet_starsq = bet2_star * bet2_star + bet3_star * bet3_star;
+  double vbet = v2 * bet2_star + v3 * bet3_star;
+  double alpha_f, alpha_s;
+  if ((cfsq - cssq) == 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
+  else if ((twid_asq - cssq) <= 0.0) { alpha_f = 0.0; alpha_s = 1.0; }
+  else if ((cfsq - twid_asq) <= 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
+  else {
+    alpha_f = sqrt((twid_asq - cssq) / (cfsq - cssq));
+    alpha_s = sqrt((cfsq - twid_asq) / (cfsq - cssq));
+  }
+  double sqrtd = sqrt(d), isqrtd = 1.0 / sqrtd;
+  double s = (b1 < 0.0) ? -1.0 : 1.0;
+  double twid_a = sqrt(twid_asq);
+  double qf = cf * alpha_f * s, qs = cs * alpha_s * s;
+  double af_prime = twid_a * alpha_f * isqrtd, as_prime = twid_a * alpha_s * isqrtd;
+  double afpbb = af_prime * bt_star * bet_starsq, aspbb = as_prime * bt_star * bet_starsq;
+  double vax = sqrt(vaxsq);
+  ev[0] = v1 - cf; ev[1] = v1 - vax; ev[2] = v1 - cs; ev[3] = v1;
+  ev[4] = v1 + cs; ev[5] = v1 + vax; ev[6] = v1 + cf;
+  rem[0][0] = alpha_f; rem[0][1] = 0.0; rem[0][2] = alpha_s; rem[0][3] = 1.0;
+  rem[0][4] = alpha_s; rem[0][5] = 0.0; rem[0][6] = alpha_f;
+  rem[1][0] = alpha_f * ev[0]; rem[1][1] = 0.0; rem[1][2] = alpha_s * ev[2]; rem[1][3] = v1;
+  rem[1][4] = alpha_s * ev[4]; rem[1][5] = 0.0; rem[1][6] = alpha_f * ev[6];
+  double qa = alpha_f * v2, qb = alpha_s * v2, qc = qs * bet2_star, qd = qf * bet2_star;
+  rem[2][0] = qa + qc; rem[2][1] = -bet3; rem[2][2] = qb - qd; rem[2][3] = v2;
+  rem[2][4] = qb + qd; rem[2][5] = bet3; rem[2][6] = qa - qc;
+  qa = alpha_f * v3; qb = alpha_s * v3; qc = qs * bet3_star; qd = qf * bet3_star;
+  rem[3][0] = qa + qc; rem[3][1] = bet2; rem[3][2] = qb - qd; rem[3][3] = v3;
+  rem[3][4] = qb + qd; rem[3][5] = -bet2; rem[3][6] = qa - qc;
+  rem[4][0] = alpha_f * (hp - v1 * cf) + qs * vbet + aspbb;
+  rem[4][1] = -(v2 * bet3 - v3 * bet2);
+  rem[4][2] = alpha_s * (hp - v1 * cs) - qf * vbet - afpbb;
+  rem[4][3] = 0.5 * vsq + (gm1 - 1.0) * x / gm1;
+  rem[4][4] = alpha_s * (hp + v1 * cs) + qf * vbet - afpbb;
+  rem[4][5] = -rem[4][1];
+  rem[4][6] = alpha_f * (hp + v1 * cf) - qs * vbet + aspbb;
+  rem[5][0] = as_prime * bet2_star; rem[5][1] = -bet3 * s * isqrtd;
+  rem[5][2] = -af_prime * bet2_star; rem[5][3] = 0.0; rem[5][4] = rem[5][2];
+  rem[5][5] = rem[5][1]; rem[5][6] = rem[5][0];
+  rem[6][0] = as_prime * bet3_star; rem[6][1] = bet2 * s * isqrtd;
+  rem[6][2] = -af_prime * bet3_star; rem[6][3] = 0.0; rem[6][4] = rem[6][2];
+  rem[6][5] = rem[6][1]; rem[6][6] = rem[6][0];
+}
+
+typedef struct {
+  double sin_a2, cos_a2, sin_a3, cos_a3, k_par, d0, p0, u0, bx0, by0, bz0, dby, dbz;
+  double rem[7][7], ev[7];
+} lw_params;
+
+/* Mesh::InitUserMeshData of pgen/linear_wave.cpp:78-190 (defaults: angles from the
+ * box, ang_2/3_vert false). */
+static void lw_setup(orc_ctx *c, int wave_flag, double amp, double vflow, lw_params *p) {
+  double gam = c->d.gamma, gm1 = gam - 1.0;
+  double x1size = c->d.x1max - c->d.x1min, x2size = c->d.x2max - c->d.x2min,
+         x3size = c->d.x3max - c->d.x3min;
+  double ang_3 = atan(x1size / x2size);
+  p->sin_a3 = sin(ang_3); p->cos_a3 = cos(ang_3);
+  double ang_2 = atan(0.5 * (x1size * p->cos_a3 + x2size * p->sin_a3) / x3size);
+  p->sin_a2 = sin(ang_2); p->cos_a2 = cos(ang_2);
+  double x1 = x1size * p->cos_a2 * p->cos_a3;
+  double x2 = x2size * p->cos_a2 * p->sin_a3;
+  double x3 = x3size * p->sin_a2;
+  double lambda = x1;
+  if (ang_3 != 0.0) lambda = dmin(lambda, x2);
+  if (ang_2 != 0.0) lambda = dmin(lambda, x3);
+  p->k_par = 2.0 * (M_PI) / lambda;
+  p->d0 = 1.0; p->u0 = vflow;
+  p->bx0 = 1.0; p->by0 = sqrt(2.0); p->bz0 = 0.5;
+  p->p0 = 1.0 / gam;
+  double h0 = ((p->p0 / gm1 + 0.5 * p->d0 * (p->u0 * p->u0)) + p->p0) / p->d0;
+  h0 += (p->bx0 * p->bx0 + p->by0 * p->by0 + p->bz0 * p->bz0) / p->d0;
+  mhd_right_eigenvectors(gam, p->d0, p->u0, 0.0, 0.0, h0, p->bx0, p->by0, p->bz0, p->ev, p->rem);
+  p->dby = amp * p->rem[NWAVE - 2][wave_flag];
+  p->dbz = amp * p->rem[NWAVE - 1][wave_flag];
+}
+
+static double lwA1(const lw_params *p, double x1, double x2, double x3) {
+  double x = x1 * p->cos_a2 * p->cos_a3 + x2 * p->cos_a2 * p->sin_a3 + x3 * p->sin_a2;
+  double y = -x1 * p->sin_a3 + x2 * p->cos_a3;
+  double Ay = p->bz0 * x - (p->dbz / p->k_par) * cos(p->k_par * (x));
+  double Az = -p->by0 * x + (p->dby / p->k_par) * cos(p->k_par * (x)) + p->bx0 * y;
+  return -Ay * p->sin_a3 - Az * p->sin_a2 * p->cos_a3;
+}
+static double lwA2(const lw_params *p, double x1, double x2, double x3) {
+  double x = x1 * p->cos_a2 * p->cos_a3 + x2 * p->cos_a2 * p->sin_a3 + x3 * p->sin_a2;
+  double y = -x1 * p->sin_a3 + x2 * p->cos_a3;
+  double Ay = p->bz0 * x - (p->dbz / p->k_par) * cos(p->k_par * (x));
+  double Az = -p->by0 * x + (p->dby / p->k_par) * cos(p->k_par * (x)) + p->bx0 * y;
+  return Ay * p->cos_a3 - Az * p->sin_a2 * p->sin_a3;
+}
+static double lwA3(const lw_params *p, double x1, double x2, double x3) {
+  double x = x1 * p->cos_a2 * p->cos_a3 + x2 * p->cos_a2 * p->sin_a3 + x3 * p->sin_a2;
+  double y = -x1 * p->sin_a3 + x2 * p->cos_a3;
+  double Az = -p->by0 * x + (p->dby / p->k_par) * cos(p->k_par * (x)) + p->bx0 * y;
+  return Az * p->cos_a2;
+}
+
+/* MeshBlock::ProblemGenerator, pgen/linear_wave.cpp:431-597 (uniform mesh: the
+ * vector potential is sampled at edge centres). */
+void orc_pgen_linear_wave(orc_ctx *c, int wave_flag, double amp, double vflow) {
+  lw_params p;
+  lw_setup(c, wave_flag, amp, vflow, &p);
+  double gm1 = c->d.gamma - 1.0;
+  for (int g = 0; g < c->nblocks; ++g) {
+    const double *x1f = c->x1f + g * (c->n1 + 1), *x1v = c->x1v + g * (c->n1 + 1), *dx1 = c->dx1f + g * (c->n1 + 1);
+    const double *x2f = c->x2f + g * (c->n2 + 1), *x2v = c->x2v + g * (c->n2 + 1), *dx2 = c->dx2f + g * (c->n2 + 1);
+    const double *x3f = c->x3f + g * (c->n3 + 1), *x3v = c->x3v + g * (c->n3 + 1), *dx3 = c->dx3f + g * (c->n3 + 1);
+    int64_t na = (int64_t)(c->n1 + 1) * (c->n2 + 1) * (c->n3 + 1);
+    double *a1 = (double *)xcalloc(na * sizeof(double)), *a2 = (double *)xcalloc(na * sizeof(double)),
+           *a3 = (double *)xcalloc(na * sizeof(double));
+#define AI(k, j, i) ((((int64_t)(k)) * (c->n2 + 1) + (j)) * (c->n1 + 1) + (i))
+    for (int k = c->ks; k <= c->ke + 1; k++)
+      for (int j = c->js; j <= c->je + 1; j++)
+        for (int i = c->is; i <= c->ie + 1; i++) {
+          if (i != c->ie + 1) a1[AI(k, j, i)] = lwA1(&p, x1v[i], x2f[j], x3f[k]);
+          if (j != c->je + 1) a2[AI(k, j, i)] = lwA2(&p, x1f[i], x2v[j], x3f[k]);
+          if (k != c->ke + 1) a3[AI(k, j, i)] = lwA3(&p, x1f[i], x2f[j], x3v[k]);
+        }
+    double *b1 = c->arr[ARR_B1F] + g * c->nf1, *b2 = c->arr[ARR_B2F] + g * c->nf2,
+           *b3 = c->arr[ARR_B3F] + g * c->nf3;
+    for (int k = c->ks; k <= c->ke; k++)
+      for (int j = c->js; j <= c->je; j++)
+        for (int i = c->is; i <= c->ie + 1; i++)
+          b1[F1IDX(c, k, j, i)] = (a3[AI(k, j + 1, i)] - a3[AI(k, j, i)]) / dx2[j] -
+                                  (a2[AI(k + 1, j, i)] - a2[AI(k, j, i)]) / dx3[k];
+    for (int k = c->ks; k <= c->ke; k++)
+      for (int j = c->js; j <= c->je + 1; j++)
+        for (int i = c->is; i <= c->ie; i++)
+          b2[F2IDX(c, k, j, i)] = (a1[AI(k + 1, j, i)] - a1[AI(k, j, i)]) / dx3[k] -
+                                  (a3[AI(k, j, i + 1)] - a3[AI(k, j, i)]) / dx1[i];
+    for (int k = c->ks; k <= c->ke + 1; k++)
+      for (int j = c->js; j <= c->je; j++)
+        for (int i = c->is; i <= c->ie; i++)
+          b3[F3IDX(c, k, j, i)] = (a2[AI(k, j, i + 1)] - a2[AI(k, j, i)]) / dx1[i] -
+                                  (a1[AI(k, j + 1, i)] - a1[AI(k, j, i)]) / dx2[j];
+#undef AI
+    free(a1); free(a2); free(a3);
+    double *u = c->arr[ARR_U] + g * c->arrsize[ARR_U];
+    int64_t nc = c->ncell;
+    for (int k = c->ks; k <= c->ke; k++)
+      for (int j = c->js; j <= c->je; j++)
+        for (int i = c->is; i <= c->ie; i++) {
+          int64_t id = CIDX(c, k, j, i);
+          double x = p.cos_a2 * (x1v[i] * p.cos_a3 + x2v[j] * p.sin_a3) + x3v[k] * p.sin_a2;
+          double sn = sin(p.k_par * x);
+          u[IDN * nc + id] = p.d0 + amp * sn * p.rem[0][wave_flag];
+          double mx = p.d0 * vflow + amp * sn * p.rem[1][wave_flag];
+          double my = amp * sn * p.rem[2][wave_flag];
+          double mz = amp * sn * p.rem[3][wave_flag];
+          u[IVX * nc + id] = mx * p.cos_a2 * p.cos_a3 - my * p.sin_a3 - mz * p.sin_a2 * p.cos_a3;
+          u[IVY * nc + id] = mx * p.cos_a2 * p.sin_a3 + my * p.cos_a3 - mz * p.sin_a2 * p.sin_a3;
+          u[IVZ * nc + id] = mx * p.sin_a2 + mz * p.cos_a2;
+          u[IEN * nc + id] = p.p0 / gm1 + 0.5 * p.d0 * p.u0 * p.u0 + amp * sn * p.rem[4][wave_flag];
+          u[IEN * nc + id] += 0.5 * (p.bx0 * p.bx0 + p.by0 * p.by0 + p.bz0 * p.bz0);
+        }
+  }
+}
+
+/* Mesh::UserWorkAfterLoop of pgen/linear_wave.cpp:199-425 (correct_err = false).
+ * out[0..7] = L1 errors (d, M1, M2, M3, E, B1c, B2c, B3c), out[8] = RMS-L1,
+ * out[9..16] = max errors, out[17] = largest max/L1. */
+void orc_linear_wave_errors(orc_ctx *c, int wave_flag, double amp, double vflow, double *out) {
+  lw_params p;
+  lw_setup(c, wave_flag, amp, vflow, &p);
+  double gm1 = c->d.gamma - 1.0;
+  double l1[8] = {0}, mx_[8] = {0};
+  for (int g = 0; g < c->nblocks; ++g) {
+    const double *x1v = c->x1v + g * (c->n1 + 1), *dx1 = c->dx1f + g * (c->n1 + 1);
+    const double *x2v = c->x2v + g * (c->n2 + 1), *dx2 = c->dx2f + g * (c->n2 + 1);
+    const double *x3v = c->x3v + g * (c->n3 + 1), *dx3 = c->dx3f + g * (c->n3 + 1);
+    const double *u = c->arr[ARR_U] + g * c->arrsize[ARR_U];
+    const double *bcc = c->arr[ARR_BCC] + g * c->arrsize[ARR_BCC];
+    int64_t nc = c->ncell;
+    for (int k = c->ks; k <= c->ke; k++)
+      for (int j = c->js; j <= c->je; j++)
+        for (int i = c->is; i <= c->ie; i++) {
+          int64_t id = CIDX(c, k, j, i);
+          double x = p.cos_a2 * (x1v[i] * p.cos_a3 + x2v[j] * p.sin_a3) + x3v[k] * p.sin_a2;
+          double sn = sin(p.k_par * x);
+          double ref[8];
+          ref[0] = p.d0 + amp * sn * p.rem[0][wave_flag];
+          double mx = p.d0 * vflow + amp * sn * p.rem[1][wave_flag];
+          double my = amp * sn * p.rem[2][wave_flag];
+          double mz = amp * sn * p.rem[3][wave_flag];
+          ref[1] = mx * p.cos_a2 * p.cos_a3 - my * p.sin_a3 - mz * p.sin_a2 * p.cos_a3;
+          ref[2] = mx * p.cos_a2 * p.sin_a3 + my * p.cos_a3 - mz * p.sin_a2 * p.sin_a3;
+          ref[3] = mx * p.sin_a2 + mz * p.cos_a2;
+          double e0 = p.p0 / gm1 + 0.5 * p.d0 * p.u0 * p.u0 + amp * sn * p.rem[4][wave_flag];
+          e0 += 0.5 * (p.bx0 * p.bx0 + p.by0 * p.by0 + p.bz0 * p.bz0);
+          double bx = p.bx0, by = p.by0 + amp * sn * p.rem[5][wave_flag], bz = p.bz0 + amp * sn * p.rem[6][wave_flag];
+          ref[5] = bx * p.cos_a2 * p.cos_a3 - by * p.sin_a3 - bz * p.sin_a2 * p.cos_a3;
+          ref[6] = bx * p.cos_a2 * p.sin_a3 + by * p.cos_a3 - bz * p.sin_a2 * p.sin_a3;
+          ref[7] = bx * p.sin_a2 + bz * p.cos_a2;
+          ref[4] = e0;
+          double vol = dx1[i] * dx2[j] * dx3[k];
+          for (int n = 0; n < 5; ++n) {
+            double e = fabs(ref[n] - u[n * nc + id]);
+            l1[n] += e * vol;
+            mx_[n] = dmax(e, mx_[n]);
+          }
+          for (int n = 0; n < 3; ++n) {
+            double e = fabs(ref[5 + n] - bcc[n * nc + id]);
+            l1[5 + n] += e * vol;
+            mx_[5 + n] = dmax(e, mx_[5 + n]);
+          }
+        }
+  }
+  double vol = (c->d.x1max - c->d.x1min) * (c->d.x2max - c->d.x2min) * (c->d.x3max - c->d.x3min);
+  double rms = 0.0, mm = 0.0;
+  for (int n = 0; n < 8; ++n) l1[n] /= vol;
+  for (int n = 0; n < 8; ++n) {
+    rms += SQR(l1[n]);
+    mm = dmax(mm, mx_[n] / l1[n]);
+  }
+  for (int n = 0; n < 8; ++n) { out[n] = l1[n]; out[9 + n] = mx_[n]; }
+  out[8] = sqrt(rms);
+  out[17] = mm;
+}
+
+/* max |div B| * dx over interior cells (the reference has no such diagnostic; this
+ * is the discrete divergence of the face field, normalised by the cell width). */
+double orc_max_divb(orc_ctx *c) {
+  double m = 0.0;
+  for (int g = 0; g < c->nblocks; ++g) {
+    const double *b1 = c->arr[ARR_B1F] + g * c->nf1, *b2 = c->arr[ARR_B2F] + g * c->nf2,
+                 *b3 = c->arr[ARR_B3F] + g * c->nf3;
+    const double *dx1 = c->dx1f + g * (c->n1 + 1), *dx2 = c->dx2f + g * (c->n2 + 1),
+                 *dx3 = c->dx3f + g * (c->n3 + 1);
+    for (int k = c->ks; k <= c->ke; k++)
+      for (int j = c->js; j <= c->je; j++)
+        for (int i = c->is; i <= c->ie; i++) {
+          double dv = (b1[F1IDX(c, k, j, i + 1)] - b1[F1IDX(c, k, j, i)]) / dx1[i] +
+                      (b2[F2IDX(c, k, j + 1, i)] - b2[F2IDX(c, k, j, i)]) / dx2[j] +
+                      (b3[F3IDX(c, k + 1, j, i)] - b3[F3IDX(c, k, j, i)]) / dx3[k];
+          m = dmax(m, fabs(dv));
+        }
+  }
+  return m;
+}
+
+/* volume sums of the 5 conserved variables and of the 3 face-field components */
+void orc_conserved_totals(orc_ctx *c, double *out8) {
+  for (int n = 0; n < 8; ++n) out8[n] = 0.0;
+  for (int g = 0; g < c->nblocks; ++g) {
+    const double *u = c->arr[ARR_U] + g * c->arrsize[ARR_U];
+    const double *b1 = c->arr[ARR_B1F] + g * c->nf1, *b2 = c->arr[ARR_B2F] + g * c->nf2,
+                 *b3 = c->arr[ARR_B3F] + g * c->nf3;
+    for (int k = c->ks; k <= c->ke; k++)
+      for (int j = c->js; j <= c->je; j++)
+        for (int i = c->is; i <= c->ie; i++) {
+          for (int n = 0; n < 5; ++n) out8[n] += u[n * c->ncell + CIDX(c, k, j, i)];
+          out8[5] += b1[F1IDX(c, k, j, i)];
+          out8[6] += b2[F2IDX(c, k, j, i)];
+          out8[7] += b3[F3IDX(c, k, j, i)];
+        }
+  }
+}

@@ -1,0 +1,91 @@
+/*
+ * agx_oracle.h -- CPU FP64 restatement of the reference's per-MeshBlock MHD update
+ * and multigrid self-gravity solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The product
+ * (athena-gamma_amd/) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to the
+ * reference's src/).  Array layouts are identical to the device layout of the product
+ * library so that host copies compare element by element:
+ *   cell arrays   [block][var][k][j][i]        n_d = bnx_d + 2*nghost
+ *   x1 faces      [block][k][j][i]  i in 0..n1 (n1+1 entries)
+ *   x2 faces      [block][k][j][i]  j in 0..n2
+ *   x3 faces      [block][k][j][i]  k in 0..n3
+ *   edges x1e     [block][k 0..n3][j 0..n2][i 0..n1-1]
+ *   edges x2e     [block][k 0..n3][j 0..n2-1][i 0..n1]
+ *   edges x3e     [block][k 0..n3-1][j 0..n2][i 0..n1]
+ * Blocks are numbered gid = (lx3*nb2 + lx2)*nb1 + lx1 (uniform root-level lattice).
+ */
+#ifndef AGX_ORACLE_H_
+#define AGX_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same field order and types as agx_mesh_desc in include/athena_gamma_amd.h. */
+typedef struct orc_desc {
+  int32_t nx1, nx2, nx3;        /* mesh cells */
+  int32_t bnx1, bnx2, bnx3;     /* MeshBlock cells */
+  int32_t nghost;
+  int32_t xorder;               /* 1 donor cell, 2 PLM, 3 PPM */
+  int32_t rsolver;              /* 0 HLLD, 1 HLLE */
+  int32_t eos;                  /* 0 adiabatic */
+  int32_t self_gravity;         /* 0 off, 1 multigrid */
+  int32_t mg_mode;              /* 0 FMG (+iterate), 1 MGI */
+  int32_t mg_niter;             /* fixed V-cycles when mg_threshold < 0 */
+  int32_t precision;            /* 0 FP64 (oracle ignores) */
+  double x1min, x1max, x2min, x2max, x3min, x3max;
+  double gamma, iso_cs, cfl, dfloor, pfloor;
+  double four_pi_G, mg_threshold;
+} orc_desc;
+
+typedef struct orc_ctx orc_ctx;
+
+orc_ctx *orc_create(const orc_desc *d);
+void orc_destroy(orc_ctx *c);
+
+/* sizes (per block, in doubles) */
+int64_t orc_size(orc_ctx *c, int what);   /* 0 cell, 1 x1f, 2 x2f, 3 x3f, 4 x1e, 5 x2e, 6 x3e, 7 nblocks */
+
+/* raw array access: which = see AGX_ARR_* numbering in the product header */
+double *orc_array(orc_ctx *c, int which);
+void orc_set_time(orc_ctx *c, double time, double dt, int ncycle);
+void orc_get_time(orc_ctx *c, double *out3);
+
+/* coordinates of block gid: fills x1f (n1+1), x1v (n1), etc. dir 0..2 */
+void orc_coords(orc_ctx *c, int gid, int dir, double *xf, double *xv, double *dxf);
+
+/* reference steps, all blocks */
+void orc_bvals_cc(orc_ctx *c, int which_arr);           /* bvals_cc.cpp:201-336 */
+void orc_bvals_fc(orc_ctx *c, int b_register);           /* bvals_fc.cpp:344-402,583-... */
+void orc_cons2prim(orc_ctx *c);                          /* adiabatic_mhd.cpp:41-90 */
+void orc_calculate_fluxes(orc_ctx *c, int order);        /* calculate_fluxes.cpp:36-378 */
+void orc_corner_e(orc_ctx *c);                           /* calculate_corner_e.cpp:28-236 */
+void orc_integrate(orc_ctx *c, int stage);               /* time_integrator.cpp:1563-1660 */
+double orc_new_block_dt(orc_ctx *c);                     /* new_blockdt.cpp:42-190 */
+void orc_new_time_step(orc_ctx *c, double tlim);         /* mesh.cpp:1078-1119 */
+void orc_initialize(orc_ctx *c);                         /* mesh.cpp:1367-1660 (subset) */
+void orc_cycle(orc_ctx *c, double tlim);                 /* main.cpp:430-489 */
+
+/* multigrid gravity */
+void orc_mg_solve(orc_ctx *c, int stage);                /* mg_gravity.cpp:166-211 */
+double orc_mg_last_norm(orc_ctx *c);
+int orc_mg_last_iters(orc_ctx *c);
+void orc_mg_set_mode(orc_ctx *c, int mode, double threshold, int niter);
+
+/* problem generators / diagnostics (pgen restatements) */
+void orc_pgen_linear_wave(orc_ctx *c, int wave_flag, double amp, double vflow);
+void orc_linear_wave_errors(orc_ctx *c, int wave_flag, double amp, double vflow,
+                            double *out);                /* linear_wave.cpp:199-425 */
+double orc_max_divb(orc_ctx *c);
+void orc_conserved_totals(orc_ctx *c, double *out8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,206 @@
+"""ctypes binding of the CPU oracle (oracle/liborc.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liborc.so")
+
+# array ids shared with the product C-ABI (AGX_ARR_* in include/athena_gamma_amd.h)
+ARR = dict(u=0, u1=1, w=2, bcc=3, b1f=4, b2f=5, b3f=6, r1f=7, r2f=8, r3f=9,
+           fx1=10, fx2=11, fx3=12, e3x1=13, e2x1=14, e1x2=15, e3x2=16, e1x3=17,
+           e2x3=18, wx1=19, wx2=20, wx3=21, e1=22, e2=23, e3=24, phi=25)
+
+
+class Desc(ctypes.Structure):
+    """Mesh description; identical layout to agx_mesh_desc (product) and orc_desc."""
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "nx1", "nx2", "nx3", "bnx1", "bnx2", "bnx3", "nghost", "xorder", "rsolver",
+        "eos", "self_gravity", "mg_mode", "mg_niter", "precision")] + \
+        [(n, ctypes.c_double) for n in (
+            "x1min", "x1max", "x2min", "x2max", "x3min", "x3max", "gamma", "iso_cs",
+            "cfl", "dfloor", "pfloor", "four_pi_G", "mg_threshold")]
+
+
+FLT_MIN = np.finfo(np.float32).tiny
+
+
+def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)),
+              nghost=2, xorder=2, rsolver=0, gamma=1.666666666666667, cfl=0.3,
+              self_gravity=0, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=10,
+              precision=0):
+    bnx = bnx or nx
+    d = Desc()
+    d.nx1, d.nx2, d.nx3 = nx
+    d.bnx1, d.bnx2, d.bnx3 = bnx
+    d.nghost, d.xorder, d.rsolver, d.eos = nghost, xorder, rsolver, 0
+    d.self_gravity, d.mg_mode, d.mg_niter, d.precision = self_gravity, mg_mode, mg_niter, precision
+    (d.x1min, d.x1max), (d.x2min, d.x2max), (d.x3min, d.x3max) = box
+    d.gamma, d.iso_cs, d.cfl = gamma, 1.0, cfl
+    # EquationOfState defaults, eos/adiabatic_mhd.cpp:26-31: sqrt(1024*FLT_MIN)
+    d.dfloor = d.pfloor = float(np.sqrt(1024 * np.float64(FLT_MIN)))
+    d.four_pi_G, d.mg_threshold = four_pi_G, mg_threshold
+    return d
+
+
+def _lib():
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    D = ctypes.POINTER(ctypes.c_double)
+    lib.orc_create.restype = P
+    lib.orc_create.argtypes = [ctypes.POINTER(Desc)]
+    lib.orc_destroy.argtypes = [P]
+    lib.orc_size.restype = ctypes.c_int64
+    lib.orc_size.argtypes = [P, ctypes.c_int]
+    lib.orc_array.restype = D
+    lib.orc_array.argtypes = [P, ctypes.c_int]
+    lib.orc_set_time.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    lib.orc_get_time.argtypes = [P, D]
+    lib.orc_coords.argtypes = [P, ctypes.c_int, ctypes.c_int, D, D, D]
+    for f in ("orc_bvals_cc", "orc_bvals_fc", "orc_calculate_fluxes", "orc_integrate",
+              "orc_mg_solve"):
+        getattr(lib, f).argtypes = [P, ctypes.c_int]
+    for f in ("orc_cons2prim", "orc_corner_e", "orc_initialize"):
+        getattr(lib, f).argtypes = [P]
+    lib.orc_new_block_dt.restype = ctypes.c_double
+    lib.orc_new_block_dt.argtypes = [P]
+    lib.orc_new_time_step.argtypes = [P, ctypes.c_double]
+    lib.orc_cycle.argtypes = [P, ctypes.c_double]
+    lib.orc_mg_last_norm.restype = ctypes.c_double
+    lib.orc_mg_last_norm.argtypes = [P]
+    lib.orc_mg_last_iters.restype = ctypes.c_int
+    lib.orc_mg_last_iters.argtypes = [P]
+    lib.orc_mg_set_mode.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    lib.orc_pgen_linear_wave.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    lib.orc_linear_wave_errors.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double, D]
+    lib.orc_max_divb.restype = ctypes.c_double
+    lib.orc_max_divb.argtypes = [P]
+    lib.orc_conserved_totals.argtypes = [P, D]
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _LIB = _lib()
+    return _LIB
+
+
+class Oracle:
+    """One oracle mesh (all MeshBlocks of a uniform periodic mesh)."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        self.L = lib()
+        self.h = self.L.orc_create(ctypes.byref(desc))
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+        self.nb = self.L.orc_size(self.h, 7)
+        ng = desc.nghost
+        self.n = (desc.bnx1 + 2 * ng, desc.bnx2 + 2 * ng, desc.bnx3 + 2 * ng)
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def shape(self, name):
+        n1, n2, n3 = self.n
+        nb = self.nb
+        cell = {"u": 5, "u1": 5, "w": 5, "bcc": 3, "phi": 1}
+        if name in cell:
+            return (nb, cell[name], n3, n2, n1)
+        face = {"b1f": (n3, n2, n1 + 1), "r1f": (n3, n2, n1 + 1), "wx1": (n3, n2, n1 + 1),
+                "e3x1": (n3, n2, n1 + 1), "e2x1": (n3, n2, n1 + 1),
+                "b2f": (n3, n2 + 1, n1), "r2f": (n3, n2 + 1, n1), "wx2": (n3, n2 + 1, n1),
+                "e1x2": (n3, n2 + 1, n1), "e3x2": (n3, n2 + 1, n1),
+                "b3f": (n3 + 1, n2, n1), "r3f": (n3 + 1, n2, n1), "wx3": (n3 + 1, n2, n1),
+                "e1x3": (n3 + 1, n2, n1), "e2x3": (n3 + 1, n2, n1),
+                "e1": (n3 + 1, n2 + 1, n1), "e2": (n3 + 1, n2, n1 + 1), "e3": (n3, n2 + 1, n1 + 1)}
+        if name in face:
+            return (nb,) + face[name]
+        fl = {"fx1": (5, n3, n2, n1 + 1), "fx2": (5, n3, n2 + 1, n1), "fx3": (5, n3 + 1, n2, n1)}
+        return (nb,) + fl[name]
+
+    def view(self, name):
+        """numpy view (no copy) of an oracle array; re-fetch after register swaps."""
+        shp = self.shape(name)
+        ptr = self.L.orc_array(self.h, ARR[name])
+        return np.ctypeslib.as_array(ptr, shape=shp)
+
+    def get(self, name):
+        return self.view(name).copy()
+
+    def set(self, name, arr):
+        self.view(name)[...] = arr
+
+    def coords(self, gid, d):
+        n = self.n[d]
+        xf = np.zeros(n + 1)
+        xv = np.zeros(n)
+        dx = np.zeros(n)
+        D = ctypes.POINTER(ctypes.c_double)
+        self.L.orc_coords(self.h, gid, d, xf.ctypes.data_as(D), xv.ctypes.data_as(D),
+                          dx.ctypes.data_as(D))
+        return xf, xv, dx
+
+    def time(self):
+        out = np.zeros(3)
+        self.L.orc_get_time(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out[0], out[1], int(out[2])
+
+    def set_time(self, t, dt, ncycle=0):
+        self.L.orc_set_time(self.h, t, dt, ncycle)
+
+    # reference steps
+    def initialize(self):
+        self.L.orc_initialize(self.h)
+
+    def cycle(self, tlim):
+        self.L.orc_cycle(self.h, tlim)
+
+    def run(self, tlim, nlim=-1):
+        n = 0
+        while self.time()[0] < tlim and (nlim < 0 or n < nlim):
+            self.cycle(tlim)
+            n += 1
+        return n
+
+    def pgen_linear_wave(self, wave_flag=0, amp=1e-6, vflow=0.0):
+        self.L.orc_pgen_linear_wave(self.h, wave_flag, amp, vflow)
+
+    def linear_wave_errors(self, wave_flag=0, amp=1e-6, vflow=0.0):
+        out = np.zeros(18)
+        self.L.orc_linear_wave_errors(self.h, wave_flag, amp, vflow,
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def max_divb(self):
+        return self.L.orc_max_divb(self.h)
+
+    def totals(self):
+        out = np.zeros(8)
+        self.L.orc_conserved_totals(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def mg_solve(self, stage=0):
+        self.L.orc_mg_solve(self.h, stage)
+
+    def mg_set_mode(self, mode, threshold, niter):
+        self.L.orc_mg_set_mode(self.h, mode, threshold, niter)
+
+    def mg_last(self):
+        return self.L.orc_mg_last_norm(self.h), self.L.orc_mg_last_iters(self.h)
