@@ -1,0 +1,277 @@
+"""Python binding (ctypes) of the gfx950 MHD + multigrid library's C ABI.
+
+The library is include/athena_gamma_amd.h built into lib/libathena_gamma_amd.so by
+build.py.  This module is plumbing for tests and benchmarks: every call goes through
+the C ABI, and nothing here computes physics.  Loading fails loudly when the library
+has not been built; there is no fallback.
+
+The directory name is not a Python identifier; load it with importlib, e.g.
+
+    spec = importlib.util.spec_from_file_location(
+        "athena_gamma_amd", "athena-gamma_amd/__init__.py")
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libathena_gamma_amd.so")
+
+ARR = dict(u=0, u1=1, w=2, bcc=3, b1f=4, b2f=5, b3f=6, r1f=7, r2f=8, r3f=9,
+           fx1=10, fx2=11, fx3=12, e3x1=13, e2x1=14, e1x2=15, e3x2=16, e1x3=17,
+           e2x3=18, wx1=19, wx2=20, wx3=21, e1=22, e2=23, e3=24, phi=25)
+KCLASS = dict(flux_x1=0, flux_x2=1, flux_x3=2, corner_e=3, integrate=4, bvals=5, c2p=6,
+              newdt=7, mg_smooth=8, mg_restrict=9, mg_prolong=10, mg_other=11)
+
+# every symbol include/athena_gamma_amd.h declares
+EXPORTS = [
+    "agx_mesh_create", "agx_mesh_destroy", "agx_last_error", "agx_array_dims",
+    "agx_mesh_bytes", "agx_upload", "agx_download", "agx_set_time", "agx_get_time",
+    "agx_synchronize", "agx_calculate_fluxes", "agx_compute_corner_e", "agx_integrate",
+    "agx_bvals_exchange", "agx_cons2prim", "agx_new_block_dt", "agx_new_time_step",
+    "agx_initialize", "agx_cycle", "agx_mg_solve", "agx_mg_set_mode", "agx_mg_last",
+    "agx_mg_vcycles", "agx_timing_enable", "agx_timing_reset", "agx_timing_get",
+]
+
+
+class Desc(ctypes.Structure):
+    """agx_mesh_desc (include/athena_gamma_amd.h)."""
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "nx1", "nx2", "nx3", "bnx1", "bnx2", "bnx3", "nghost", "xorder", "rsolver",
+        "eos", "self_gravity", "mg_mode", "mg_niter", "precision")] + \
+        [(n, ctypes.c_double) for n in (
+            "x1min", "x1max", "x2min", "x2max", "x3min", "x3max", "gamma", "iso_cs",
+            "cfl", "dfloor", "pfloor", "four_pi_G", "mg_threshold")]
+
+
+def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)),
+              nghost=2, xorder=2, rsolver=0, gamma=1.666666666666667, cfl=0.3,
+              self_gravity=0, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=10,
+              precision=0):
+    bnx = bnx or nx
+    d = Desc()
+    d.nx1, d.nx2, d.nx3 = nx
+    d.bnx1, d.bnx2, d.bnx3 = bnx
+    d.nghost, d.xorder, d.rsolver, d.eos = nghost, xorder, rsolver, 0
+    d.self_gravity, d.mg_mode, d.mg_niter, d.precision = self_gravity, mg_mode, mg_niter, precision
+    (d.x1min, d.x1max), (d.x2min, d.x2max), (d.x3min, d.x3max) = box
+    d.gamma, d.iso_cs, d.cfl = gamma, 1.0, cfl
+    # EquationOfState defaults sqrt(1024*FLT_MIN), eos/adiabatic_mhd.cpp:26-31
+    d.dfloor = d.pfloor = float(np.sqrt(1024 * np.float64(np.finfo(np.float32).tiny)))
+    d.four_pi_G, d.mg_threshold = four_pi_G, mg_threshold
+    return d
+
+
+_LIB = None
+
+
+def lib():
+    """Load the built library; raises if it is missing (no fallback path exists)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libathena_gamma_amd.so not built: run athena-gamma_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    D = ctypes.POINTER(ctypes.c_double)
+    I = ctypes.c_int
+    L.agx_mesh_create.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(P)]
+    L.agx_mesh_destroy.argtypes = [P]
+    L.agx_last_error.restype = ctypes.c_char_p
+    L.agx_array_dims.argtypes = [P, I, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+    L.agx_mesh_bytes.restype = ctypes.c_int64
+    L.agx_mesh_bytes.argtypes = [P]
+    L.agx_upload.argtypes = [P, I, D]
+    L.agx_download.argtypes = [P, I, D]
+    L.agx_set_time.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32]
+    L.agx_get_time.argtypes = [P, D, D, ctypes.POINTER(ctypes.c_int32)]
+    L.agx_synchronize.argtypes = [P]
+    L.agx_calculate_fluxes.argtypes = [P, I]
+    L.agx_compute_corner_e.argtypes = [P]
+    L.agx_integrate.argtypes = [P, I]
+    L.agx_bvals_exchange.argtypes = [P, I]
+    L.agx_cons2prim.argtypes = [P]
+    L.agx_new_block_dt.argtypes = [P, D]
+    L.agx_new_time_step.argtypes = [P, ctypes.c_double]
+    L.agx_initialize.argtypes = [P, ctypes.c_double]
+    L.agx_cycle.argtypes = [P, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.agx_mg_solve.argtypes = [P, I]
+    L.agx_mg_set_mode.argtypes = [P, I, ctypes.c_double, ctypes.c_int32]
+    L.agx_mg_last.argtypes = [P, D, ctypes.POINTER(ctypes.c_int32)]
+    L.agx_mg_vcycles.argtypes = [P, ctypes.c_int32]
+    L.agx_timing_enable.argtypes = [P, I]
+    L.agx_timing_reset.argtypes = [P]
+    L.agx_timing_get.argtypes = [P, I, D, ctypes.POINTER(ctypes.c_int64)]
+    _LIB = L
+    return L
+
+
+class AgxError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().agx_last_error().decode(errors="replace")
+        raise AgxError(f"{what} failed ({rc}): {msg}")
+
+
+class Mesh:
+    """One device-resident Mesh (all MeshBlocks); thin wrapper over agx_mesh."""
+
+    def __init__(self, desc):
+        self.L = lib()
+        if not isinstance(desc, Desc):   # same layout from another binding: copy fields
+            d = Desc()
+            for name, _ in Desc._fields_:
+                setattr(d, name, getattr(desc, name))
+            desc = d
+        self.desc = desc
+        h = ctypes.c_void_p()
+        _check(self.L.agx_mesh_create(ctypes.byref(desc), ctypes.byref(h)), "agx_mesh_create")
+        self.h = h
+        nb = ctypes.c_int64()
+        dims = (ctypes.c_int64 * 4)()
+        _check(self.L.agx_array_dims(self.h, 0, ctypes.byref(nb), dims), "agx_array_dims")
+        self.nb = nb.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.agx_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, name):
+        nb = ctypes.c_int64()
+        dims = (ctypes.c_int64 * 4)()
+        _check(self.L.agx_array_dims(self.h, ARR[name], ctypes.byref(nb), dims), "agx_array_dims")
+        d = list(dims)
+        if d[0] == 1 and name not in ("phi",):
+            return (nb.value, d[1], d[2], d[3])
+        return (nb.value, d[0], d[1], d[2], d[3])
+
+    def get(self, name):
+        a = np.zeros(self.shape(name))
+        _check(self.L.agx_download(self.h, ARR[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+               "agx_download")
+        return a
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape(name))
+        _check(self.L.agx_upload(self.h, ARR[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+               "agx_upload")
+
+    def time(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+        _check(self.L.agx_get_time(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)),
+               "agx_get_time")
+        return t.value, dt.value, n.value
+
+    def set_time(self, t, dt, ncycle=0):
+        _check(self.L.agx_set_time(self.h, t, dt, ncycle), "agx_set_time")
+
+    def sync(self):
+        _check(self.L.agx_synchronize(self.h), "agx_synchronize")
+
+    def calculate_fluxes(self, order):
+        _check(self.L.agx_calculate_fluxes(self.h, order), "agx_calculate_fluxes")
+
+    def corner_e(self):
+        _check(self.L.agx_compute_corner_e(self.h), "agx_compute_corner_e")
+
+    def integrate(self, stage):
+        _check(self.L.agx_integrate(self.h, stage), "agx_integrate")
+
+    def bvals(self, which=3):
+        _check(self.L.agx_bvals_exchange(self.h, which), "agx_bvals_exchange")
+
+    def cons2prim(self):
+        _check(self.L.agx_cons2prim(self.h), "agx_cons2prim")
+
+    def new_block_dt(self):
+        v = ctypes.c_double()
+        _check(self.L.agx_new_block_dt(self.h, ctypes.byref(v)), "agx_new_block_dt")
+        return v.value
+
+    def new_time_step(self, tlim):
+        _check(self.L.agx_new_time_step(self.h, tlim), "agx_new_time_step")
+
+    def initialize(self, tlim=1e300):
+        _check(self.L.agx_initialize(self.h, tlim), "agx_initialize")
+
+    def cycle(self, tlim, n=1):
+        done = ctypes.c_int32()
+        _check(self.L.agx_cycle(self.h, tlim, n, ctypes.byref(done)), "agx_cycle")
+        return done.value
+
+    def run(self, tlim, nlim=1 << 30):
+        return self.cycle(tlim, nlim)
+
+    def mg_solve(self, stage=0):
+        _check(self.L.agx_mg_solve(self.h, stage), "agx_mg_solve")
+
+    def mg_set_mode(self, mode, threshold, niter):
+        _check(self.L.agx_mg_set_mode(self.h, mode, threshold, niter), "agx_mg_set_mode")
+
+    def mg_last(self):
+        v, n = ctypes.c_double(), ctypes.c_int32()
+        _check(self.L.agx_mg_last(self.h, ctypes.byref(v), ctypes.byref(n)), "agx_mg_last")
+        return v.value, n.value
+
+    def mg_vcycles(self, n):
+        _check(self.L.agx_mg_vcycles(self.h, n), "agx_mg_vcycles")
+
+    def timing(self, enable=True):
+        _check(self.L.agx_timing_enable(self.h, 1 if enable else 0), "agx_timing_enable")
+
+    def timing_reset(self):
+        _check(self.L.agx_timing_reset(self.h), "agx_timing_reset")
+
+    def timing_get(self, kclass):
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        _check(self.L.agx_timing_get(self.h, KCLASS[kclass], ctypes.byref(ms), ctypes.byref(n)),
+               "agx_timing_get")
+        return ms.value, n.value
+
+    def device_bytes(self):
+        return self.L.agx_mesh_bytes(self.h)
+
+
+# ---------------------------------------------------------------- host utilities
+def block_lattice(desc):
+    return desc.nx1 // desc.bnx1, desc.nx2 // desc.bnx2, desc.nx3 // desc.bnx3
+
+
+def scatter_cells(desc, G):
+    """Global interior array [nv, N3, N2, N1] -> block array [nb, nv, n3, n2, n1] (ghosts 0)."""
+    ng = desc.nghost
+    nb1, nb2, nb3 = block_lattice(desc)
+    nv = G.shape[0]
+    out = np.zeros((nb1 * nb2 * nb3, nv, desc.bnx3 + 2 * ng, desc.bnx2 + 2 * ng, desc.bnx1 + 2 * ng))
+    for g in range(out.shape[0]):
+        l1, l2, l3 = g % nb1, (g // nb1) % nb2, g // (nb1 * nb2)
+        out[g, :, ng:ng + desc.bnx3, ng:ng + desc.bnx2, ng:ng + desc.bnx1] = \
+            G[:, l3 * desc.bnx3:(l3 + 1) * desc.bnx3, l2 * desc.bnx2:(l2 + 1) * desc.bnx2,
+              l1 * desc.bnx1:(l1 + 1) * desc.bnx1]
+    return out
+
+
+def gather_cells(desc, a):
+    """Block array [nb, nv, n3, n2, n1] -> global interior [nv, N3, N2, N1]."""
+    ng = desc.nghost
+    nb1, nb2, nb3 = block_lattice(desc)
+    nv = a.shape[1]
+    G = np.zeros((nv, desc.nx3, desc.nx2, desc.nx1))
+    for g in range(a.shape[0]):
+        l1, l2, l3 = g % nb1, (g // nb1) % nb2, g // (nb1 * nb2)
+        G[:, l3 * desc.bnx3:(l3 + 1) * desc.bnx3, l2 * desc.bnx2:(l2 + 1) * desc.bnx2,
+          l1 * desc.bnx1:(l1 + 1) * desc.bnx1] = a[g, :, ng:ng + desc.bnx3, ng:ng + desc.bnx2,
+                                                   ng:ng + desc.bnx1]
+    return G

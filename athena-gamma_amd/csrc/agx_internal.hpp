@@ -1,0 +1,128 @@
+// agx_internal.hpp -- shared definitions of the gfx950 MHD + multigrid library.
+//
+// Device layout: every registered array of every MeshBlock lives in one HBM buffer,
+// block-major.  A per-variable slab is a padded box of (n3+1) planes x (n2+1) rows x
+// P doubles (P = pitch, a multiple of 16 doubles = 128 B, P >= n1+1), so cell, face and
+// edge arrays share one index function IDX(k,j,i) = (k*J + j)*P + i with J = n2+1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/athena_gamma_amd.h"
+
+namespace agx {
+
+constexpr int NHYDRO = 5;
+constexpr int NWAVE = 7;
+constexpr int IDN = 0, IVX = 1, IVY = 2, IVZ = 3, IEN = 4, IPR = 4, IBY = 5, IBZ = 6;
+constexpr int IB1 = 0, IB2 = 1, IB3 = 2;
+
+// Geometry of the block-batched arrays, passed by value to every kernel.
+struct Geo {
+  int nb;                  // MeshBlocks
+  int n1, n2, n3;          // cells per block including ghosts
+  int ng;                  // ghost width
+  int is, ie, js, je, ks, ke;
+  int bx1, bx2, bx3;       // interior cells per block
+  int nb1, nb2, nb3;       // block lattice
+  int P;                   // row pitch (doubles)
+  int PL;                  // stride of the per-direction PLM coefficient rows (>= max n_d)
+  int J;                   // rows per plane = n2 + 1
+  int64_t S;               // per-variable slab = (n3+1)*J*P
+};
+
+__host__ __device__ inline int64_t IDX(const Geo &g, int k, int j, int i) {
+  return ((int64_t)k * g.J + j) * g.P + i;
+}
+
+// Per-block constant geometry on the device (uniform Cartesian mesh).
+struct BlockCoef {
+  double dx[3];            // dx?f (== dx?v) of the block, coordinates.cpp:127-145
+};
+
+// Registered device arrays (AGX_ARR_* numbering).
+struct Arrays {
+  double *a[AGX_ARR_COUNT];
+  int nvar[AGX_ARR_COUNT];
+};
+
+// Multigrid level data: one periodic grid per level covering the whole mesh.
+struct MGLevel {
+  int n1, n2, n3;
+  double dx;               // cell size used by the smoother/defect at this level
+  double *u, *src, *def;
+};
+
+struct Timer {
+  bool enabled = false;
+  std::vector<hipEvent_t> ev;        // pairs
+  std::vector<int> cls;              // class of each pair
+  size_t used = 0;
+  double ms[AGX_K_COUNT] = {0};
+  int64_t launches[AGX_K_COUNT] = {0};
+};
+
+}  // namespace agx
+
+struct agx_mesh {
+  agx_mesh_desc d;
+  agx::Geo g;
+  agx::Arrays arr;
+  hipStream_t stream = nullptr;
+  double *pool = nullptr;            // all MHD arrays
+  int64_t pool_bytes = 0;
+  agx::BlockCoef *coef = nullptr;    // [nb]
+  double *plm = nullptr;             // PLM coefficients [nb][3 dirs][2][P]
+  double *cce = nullptr;             // unused placeholder (cell-centred E computed on the fly)
+  double *scal = nullptr;            // device scalars (dt reduction etc.), 64 doubles
+  double *scal_host = nullptr;       // pinned host mirror
+  double *red = nullptr;             // reduction partials
+  int64_t red_n = 0;
+  std::vector<agx::BlockCoef> coef_h;
+  std::vector<double> plm_h;
+  double time = 0.0, dt = 0.0;
+  int32_t ncycle = 0;
+  // multigrid
+  int mg_on = 0;
+  int nrootlevel = 0, nmblevel = 0, ntotal = 0, current = 0, fmglevel = 0, coffset = 0;
+  int mg_mode = 0, mg_niter = 0;
+  double mg_eps = -1.0, last_norm = 0.0, last_ave = 0.0;
+  int32_t last_iters = 0;
+  std::vector<agx::MGLevel> lev;
+  double *mgpool = nullptr;
+  int64_t mg_bytes = 0;
+  double rdx_root = 0.0, rdy_root = 0.0, rdz_root = 0.0;
+  double rdx_blk = 0.0, rdy_blk = 0.0, rdz_blk = 0.0;
+  agx::Timer timer;
+};
+
+namespace agx {
+
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+#define AGX_HIP(call)                                   \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return agx::hip_fail(e_, #call); \
+  } while (0)
+
+// launch bracketing for the built-in timers
+void timer_begin(agx_mesh *m, int cls);
+void timer_end(agx_mesh *m);
+int timer_flush(agx_mesh *m);
+
+inline unsigned grid1d(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// multigrid entry points implemented in agx_mg.hip
+int mg_create(agx_mesh *m);
+void mg_destroy(agx_mesh *m);
+int mg_solve(agx_mesh *m, int stage);
+int mg_vcycles(agx_mesh *m, int n);
+
+}  // namespace agx

@@ -1,0 +1,249 @@
+"""Benchmark: MHD zone-cycles/s and multigrid V-cycles/s at 256^3 on one MI355X.
+
+Workload (value): 256^3 decaying MHD turbulence, 64 MeshBlocks of 64^3, VL2 + PLM +
+HLLD + CT, plain FP64, synthetic seeded initial data.  One step = one full VL2 cycle
+(both stages, ghost exchange, cons->prim, time-step reduction), inputs resident in HBM.
+Also reported: V(1,1) cycles/s of the FP64 multigrid on the 256^3 periodic
+32-blob self-gravity problem, the dominant kernel's roofline, and the CPU oracle
+timed on a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK = 8.0e12   # MI355X HBM3E, B/s
+
+
+def load_pkg():
+    name = "athena_gamma_amd"
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "athena-gamma_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    pspec = importlib.util.spec_from_file_location("agx_pgen", os.path.join(ROOT, "athena-gamma_amd", "pgen.py"))
+    pgen = importlib.util.module_from_spec(pspec)
+    pspec.loader.exec_module(pgen)
+    return mod, pgen
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# Algorithmic HBM bytes per unit for each kernel class (FP64 words x 8 B); the unit
+# counts per launch are computed from the mesh geometry below.
+#   flux_x?   per face: read 5 prim + 2 transverse bcc + 1 normal face field,
+#             write 5 fluxes + 2 face EMFs + 1 CT weight            -> 16 words
+#   corner_e  per edge point (k,j,i): read 3 v + 3 bcc + 6 face EMFs + 3 weights,
+#             write 3 edge EMFs                                      -> 18 words
+#   c2p       per cell (interior + ghosts): read 5 cons + 3 faces, write 5 prim +
+#             3 bcc + 2 cons (floors)                                -> 18 words
+#   newdt     per interior cell: read 5 prim + 3 bcc + 3 faces       -> 11 words
+#   mg_smooth per cell of one colour sweep: read u, src, write u (whole level) -> 3 words
+BYTES_PER_UNIT = {"flux_x1": 128, "flux_x2": 128, "flux_x3": 128, "corner_e": 144,
+                  "c2p": 144, "newdt": 88}
+
+
+def units_per_launch(desc, cls):
+    nb = (desc.nx1 // desc.bnx1) * (desc.nx2 // desc.bnx2) * (desc.nx3 // desc.bnx3)
+    b1, b2, b3, ng = desc.bnx1, desc.bnx2, desc.bnx3, desc.nghost
+    if cls == "flux_x1":
+        return nb * (b1 + 1) * (b2 + 2) * (b3 + 2)
+    if cls == "flux_x2":
+        return nb * (b1 + 2) * (b2 + 1) * (b3 + 2)
+    if cls == "flux_x3":
+        return nb * (b1 + 2) * (b2 + 2) * (b3 + 1)
+    if cls == "corner_e":
+        return nb * (b1 + 1) * (b2 + 1) * (b3 + 1)
+    if cls == "c2p":
+        return nb * (b1 + 2 * ng) * (b2 + 2 * ng) * (b3 + 2 * ng)
+    if cls == "newdt":
+        return nb * b1 * b2 * b3
+    raise KeyError(cls)
+
+
+def mhd_bench(agx, pgen, args):
+    N, B = args.n, args.block
+    desc = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4,
+                         gamma=5.0 / 3.0, xorder=2, rsolver=0)
+    t0 = time.time()
+    U, b1, b2, b3 = pgen.turbulence(desc, seed=2, mach=2.0, bx=0.1)
+    m = agx.Mesh(desc)
+    m.set("u", agx.scatter_cells(desc, U))
+    del U
+    f1, f2, f3 = pgen.scatter_faces(desc, b1, b2, b3)
+    m.set("b1f", f1)
+    m.set("b2f", f2)
+    m.set("b3f", f3)
+    del f1, f2, f3, b1, b2, b3
+    m.initialize()
+    log(f"[bench] MHD {N}^3 in {desc.nx1 // B}^3 blocks of {B}^3 set up in {time.time() - t0:.1f}s, "
+        f"device {m.device_bytes() / 1e9:.2f} GB")
+    m.cycle(1e30, args.warmup)
+    m.sync()
+    t0 = time.perf_counter()
+    done = m.cycle(1e30, args.steps)
+    m.sync()
+    t1 = time.perf_counter()
+    assert done == args.steps
+    sec = t1 - t0
+    zc = float(N) ** 3 * args.steps / sec
+    log(f"[bench] {args.steps} cycles in {sec * 1e3:.1f} ms -> {zc:.4e} zone-cycles/s")
+    # instrumented pass: HIP events on the mesh stream around every launch
+    m.timing(True)
+    m.timing_reset()
+    m.cycle(1e30, 2)
+    m.sync()
+    per = {}
+    for cls in ("flux_x1", "flux_x2", "flux_x3", "corner_e", "integrate", "bvals", "c2p", "newdt"):
+        ms, n = m.timing_get(cls)
+        per[cls] = (ms, n)
+    m.timing(False)
+    t_state = m.time()
+    m.close()
+    return desc, zc, sec * 1e3 / args.steps, per, t_state
+
+
+def mg_bench(agx, pgen, args):
+    N, B = args.n, args.block
+    desc = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, self_gravity=1,
+                         four_pi_G=1.0, mg_mode=0, mg_threshold=0.0)
+    rho = pgen.blobs(desc, seed=3)
+    U = np.zeros((5,) + rho.shape)
+    U[0] = rho
+    m = agx.Mesh(desc)
+    m.set("u", agx.scatter_cells(desc, U))
+    del U
+    # reference tolerance: 1e-10 relative to ||4 pi G (rho - mean)|| in the solver's norm
+    src_norm = float(np.sqrt(np.mean((rho - rho.mean()) ** 2)))
+    m.mg_set_mode(0, 1e-10 * src_norm, -1)
+    t0 = time.perf_counter()
+    m.mg_solve(0)
+    m.sync()
+    t_solve = time.perf_counter() - t0
+    norm, iters = m.mg_last()
+    log(f"[bench] MG FMG+{iters} V-cycles to {norm:.3e} (target {1e-10 * src_norm:.3e}) in {t_solve * 1e3:.1f} ms")
+    m.mg_vcycles(2)
+    m.sync()
+    nv = args.mg_cycles
+    t0 = time.perf_counter()
+    m.mg_vcycles(nv)
+    m.sync()
+    sec = time.perf_counter() - t0
+    vps = nv / sec
+    m.timing(True)
+    m.timing_reset()
+    m.mg_vcycles(3)
+    m.sync()
+    per = {c: m.timing_get(c) for c in ("mg_smooth", "mg_restrict", "mg_prolong", "mg_other")}
+    m.timing(False)
+    m.close()
+    log(f"[bench] MG {nv} V-cycles in {sec * 1e3:.1f} ms -> {vps:.1f} V-cycles/s")
+    return vps, iters, norm, t_solve, per
+
+
+def cpu_baseline(args):
+    """The CPU oracle (FP64 restatement, OpenMP over MeshBlocks) on a bounded sample of
+    the same workload: a 64^3 periodic sub-box of the turbulence problem in 8 MeshBlocks
+    of 32^3, timed over a fixed number of VL2 cycles."""
+    sys.path.insert(0, ROOT)
+    from oracle.orc import Oracle, make_desc
+    agx, pgen = load_pkg()
+    threads = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    desc = make_desc(nx=(64, 64, 64), bnx=(32, 32, 16), box=((0.0, 0.25),) * 3, cfl=0.4,
+                     gamma=5.0 / 3.0)
+    U, b1, b2, b3 = pgen.turbulence(desc, seed=2, mach=2.0, bx=0.1)
+    o = Oracle(desc)
+    o.set("u", agx.scatter_cells(desc, U))
+    f1, f2, f3 = pgen.scatter_faces(desc, b1, b2, b3)
+    o.set("b1f", f1)
+    o.set("b2f", f2)
+    o.set("b3f", f3)
+    o.initialize()
+    o.cycle(1e30)
+    ncyc = args.cpu_cycles
+    t0 = time.perf_counter()
+    for _ in range(ncyc):
+        o.cycle(1e30)
+    sec = time.perf_counter() - t0
+    zc = 64.0 ** 3 * ncyc / sec
+    log(f"[bench] CPU oracle: {ncyc} cycles of 64^3 in {sec:.1f}s -> {zc:.3e} zone-cycles/s ({threads} threads)")
+    return {"value": zc, "unit": "zone-cycles/s", "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
+            "kind": "port",
+            "sample": f"{ncyc} VL2+PLM+HLLD+CT cycles of a 64^3 periodic sub-box in 16 MeshBlocks of 32x32x16 "
+                      f"(same turbulence generator), oracle/agx_oracle.c FP64, OpenMP over MeshBlocks"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--block", type=int, default=64)
+    ap.add_argument("--mg-cycles", type=int, default=20)
+    ap.add_argument("--cpu-cycles", type=int, default=150)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mg", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per run); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    agx, pgen = load_pkg()
+    desc, zc, ms_step, per, tstate = mhd_bench(agx, pgen, args)
+    # dominant kernel among those with a defined algorithmic byte count
+    dom = max(BYTES_PER_UNIT, key=lambda c: per[c][0])
+    ms_tot, n_l = per[dom]
+    avg_s = ms_tot / max(n_l, 1) / 1e3
+    alg = BYTES_PER_UNIT[dom] * units_per_launch(desc, dom)
+    achieved = alg / avg_s
+    total_ms = sum(v[0] for v in per.values())
+    log("[bench] per-cycle kernel time: " + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in per.items())
+        + f" (sum {total_ms / 2:.2f} ms/cycle)")
+    out = {
+        "metric": "MHD zone-cycles/s + MG V-cycles/s at 256^3 on 1 MI355X; % of HBM roofline",
+        "value": zc,
+        "unit": "zone-cycles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.n}^3 decaying MHD turbulence (rms Mach 2, Bx=0.1, k^-4 on 2<=k<=8), "
+                               f"{(args.n // args.block) ** 3} MeshBlocks of {args.block}^3, "
+                               f"VL2+PLM+HLLD+CT, plain FP64, one step = one VL2 cycle",
+                   "mesh": [args.n] * 3, "meshblock": [args.block] * 3, "nghost": 2},
+        "cycle_roofline_frac": zc * 680.0 / HBM_PEAK,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": alg},
+        "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in per.items()},
+    }
+    if not args.no_mg:
+        vps, iters, norm, t_solve, mper = mg_bench(agx, pgen, args)
+        out["mg_vcycles_per_s"] = vps
+        out["mg_roofline_frac"] = vps * float(args.n) ** 3 * 148.6 / HBM_PEAK
+        out["mg_solve_ms_to_1e-10"] = t_solve * 1e3
+        out["mg_solve_vcycles_after_fmg"] = iters
+        out["mg_kernel_ms_per_vcycle"] = {k: v[0] / 3 for k, v in mper.items()}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

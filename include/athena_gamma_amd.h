@@ -1,0 +1,163 @@
+/*
+ * athena_gamma_amd.h -- C ABI of the MI355X (gfx950) MHD + multigrid-gravity library.
+ *
+ * One agx_mesh is one device-resident Mesh: every MeshBlock of a uniform periodic
+ * root-level lattice, with all registered arrays (conserved/primitive variables, face
+ * fields, fluxes, EMFs, potential, multigrid hierarchy) in HBM.  Each entry point runs
+ * one task of the reference's task lists for ALL MeshBlocks in one batch of kernel
+ * launches on the mesh's HIP stream, and replaces the reference method cited beside
+ * it (paths relative to the reference's src/).  No entry point allocates after
+ * agx_mesh_create, except agx_mesh_create itself.
+ *
+ * Conventions
+ *   - Return value: 0 on success; > 0 a HIP error code; < 0 a contract violation
+ *     (bad argument, unsupported option).  agx_last_error() returns the message of the
+ *     last failure on the calling thread.  This mirrors ATHENA_ERROR (defs.hpp.in:
+ *     119-123): a host shim turns a non-zero return into std::runtime_error.
+ *   - Host arrays are FP64, block-major, in the reference's AthenaArray order
+ *     (n,k,j,i), i fastest, including ghost zones:
+ *        cell  [nblocks][nvar][n3][n2][n1]           n_d = bnx_d + 2*nghost
+ *        x1f   [nblocks][n3][n2][n1+1]   x2f [nblocks][n3][n2+1][n1]
+ *        x3f   [nblocks][n3+1][n2][n1]
+ *        x1e   [nblocks][n3+1][n2+1][n1] x2e [nblocks][n3+1][n2][n1+1]
+ *        x3e   [nblocks][n3][n2+1][n1+1]
+ *     Block gid = (lx3*nb2 + lx2)*nb1 + lx1.  Device storage is padded (row pitch a
+ *     multiple of 16 doubles); agx_upload/agx_download convert.
+ *   - Not re-entrant per mesh: call a given agx_mesh from one host thread.  Different
+ *     meshes are independent.
+ */
+#ifndef ATHENA_GAMMA_AMD_H_
+#define ATHENA_GAMMA_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Runtime parameters of one Mesh: the <mesh>, <meshblock>, <time>, <hydro> and
+ * <gravity> input blocks the path reads (mesh.cpp:63-300, eos/adiabatic_mhd.cpp:26-31,
+ * reconstruct/reconstruction.cpp:39-100, gravity/mg_gravity.cpp:41-120). */
+typedef struct agx_mesh_desc {
+  int32_t nx1, nx2, nx3;        /* <mesh>/nx1..3 */
+  int32_t bnx1, bnx2, bnx3;     /* <meshblock>/nx1..3 */
+  int32_t nghost;               /* NGHOST (configure --nghost): >= 2, >= 3 for PPM */
+  int32_t xorder;               /* <time>/xorder: 1 donor cell, 2 PLM, 3 PPM */
+  int32_t rsolver;              /* configure --flux: AGX_RSOLVER_* */
+  int32_t eos;                  /* configure --eos: 0 adiabatic */
+  int32_t self_gravity;         /* configure --grav: 0 none, 1 multigrid */
+  int32_t mg_mode;              /* <gravity>/mgmode: 0 FMG, 1 MGI */
+  int32_t mg_niter;             /* <gravity>/niteration (used when threshold < 0) */
+  int32_t precision;            /* AGX_PREC_FP64 or AGX_PREC_MIXED */
+  double x1min, x1max, x2min, x2max, x3min, x3max;   /* <mesh>/x?min, x?max */
+  double gamma;                 /* <hydro>/gamma */
+  double iso_cs;                /* <hydro>/iso_sound_speed (isothermal EOS) */
+  double cfl;                   /* <time>/cfl_number (clamped to 0.5 in 3D) */
+  double dfloor, pfloor;        /* <hydro>/dfloor, pfloor */
+  double four_pi_G;             /* Mesh::SetFourPiG */
+  double mg_threshold;          /* <gravity>/threshold (absolute L2 defect norm) */
+} agx_mesh_desc;
+
+enum { AGX_RSOLVER_HLLD = 0, AGX_RSOLVER_HLLE = 1 };
+enum { AGX_PREC_FP64 = 0, AGX_PREC_MIXED = 1 };
+
+/* Registered arrays (MeshBlock members in the reference). */
+enum {
+  AGX_ARR_U = 0,      /* Hydro::u (NHYDRO=5)            hydro/hydro.cpp:32 */
+  AGX_ARR_U1,         /* Hydro::u1 register             hydro/hydro.cpp:34 */
+  AGX_ARR_W,          /* Hydro::w (primitive)           hydro/hydro.cpp:33 */
+  AGX_ARR_BCC,        /* Field::bcc (3)                 field/field.cpp:30 */
+  AGX_ARR_B1F, AGX_ARR_B2F, AGX_ARR_B3F,   /* Field::b.x?f  field/field.cpp:28 */
+  AGX_ARR_R1F, AGX_ARR_R2F, AGX_ARR_R3F,   /* Field::b1.x?f field/field.cpp:29 */
+  AGX_ARR_FX1, AGX_ARR_FX2, AGX_ARR_FX3,   /* Hydro::flux[3] hydro/hydro.cpp:40-42 */
+  AGX_ARR_E3X1, AGX_ARR_E2X1, AGX_ARR_E1X2, AGX_ARR_E3X2, AGX_ARR_E1X3, AGX_ARR_E2X3,
+                      /* face EMFs e?_x?f         field/field.cpp:33-38 */
+  AGX_ARR_WX1, AGX_ARR_WX2, AGX_ARR_WX3,   /* Field::wght.x?f field/field.cpp:32 */
+  AGX_ARR_E1, AGX_ARR_E2, AGX_ARR_E3,      /* Field::e.x?e   field/field.cpp:31 */
+  AGX_ARR_PHI,        /* Gravity::phi                   gravity/gravity.cpp:37 */
+  AGX_ARR_COUNT
+};
+
+/* Kernel classes for the built-in HIP-event timers. */
+enum {
+  AGX_K_FLUX_X1 = 0, AGX_K_FLUX_X2, AGX_K_FLUX_X3, AGX_K_CORNER_E, AGX_K_INTEGRATE,
+  AGX_K_BVALS, AGX_K_C2P, AGX_K_NEWDT, AGX_K_MG_SMOOTH, AGX_K_MG_RESTRICT,
+  AGX_K_MG_PROLONG, AGX_K_MG_OTHER, AGX_K_COUNT
+};
+
+typedef struct agx_mesh agx_mesh;
+
+/* ---- lifecycle (Mesh::Mesh mesh.cpp:63, MeshBlock::MeshBlock meshblock.cpp:47-185,
+ *      Coordinates coordinates.cpp:26-145, MGGravityDriver ctor mg_gravity.cpp:41-138) */
+int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out);
+int agx_mesh_destroy(agx_mesh *m);
+const char *agx_last_error(void);
+/* number of MeshBlocks and per-block array dims: dims[0..3] = nvar, n3', n2', n1' */
+int agx_array_dims(agx_mesh *m, int arr, int64_t *nblocks, int64_t dims[4]);
+/* device bytes held by the mesh */
+int64_t agx_mesh_bytes(agx_mesh *m);
+
+/* ---- host <-> device (AthenaArray copies at init/output only) */
+int agx_upload(agx_mesh *m, int arr, const double *host);
+int agx_download(agx_mesh *m, int arr, double *host);
+int agx_set_time(agx_mesh *m, double time, double dt, int32_t ncycle);
+int agx_get_time(agx_mesh *m, double *time, double *dt, int32_t *ncycle);
+int agx_synchronize(agx_mesh *m);
+
+/* ---- VL2 task-list steps, all MeshBlocks per call */
+/* Hydro::CalculateFluxes(w, b, bcc, order), hydro/calculate_fluxes.cpp:36-378, with
+ * Reconstruction::{DonorCell,PiecewiseLinear,PiecewiseParabolic}X1..3
+ * (reconstruct/dc.cpp:24-104, plm.cpp:27-351, ppm.cpp:44-923), Hydro::RiemannSolver
+ * (hydro/rsolvers/mhd/hlld.cpp:38-382, hlle_mhd.cpp:25-182) and Hydro::GetWeightForCT
+ * (hydro/hydro.cpp:154-158). */
+int agx_calculate_fluxes(agx_mesh *m, int order);
+/* Field::ComputeCornerE(w, bcc), field/calculate_corner_e.cpp:28-236 (3D). */
+int agx_compute_corner_e(agx_mesh *m);
+/* TimeIntegratorTaskList::IntegrateHydro + AddSourceTerms + IntegrateField for VL2
+ * stage `stage` (time_integrator.cpp:1563-1660): MeshBlock::WeightedAve and register
+ * swap (mesh/weighted_ave.cpp:33-236), Hydro::AddFluxDivergence
+ * (hydro/add_flux_divergence.cpp:39-96), HydroSourceTerms::SelfGravity
+ * (hydro/srcterms/self_gravity.cpp:35-125) and Field::CT (field/ct.cpp:31-116). */
+int agx_integrate(agx_mesh *m, int stage);
+/* SendHydro/ReceiveHydro/SetBoundaries for Hydro::u and Field::b, same-level
+ * neighbours (bvals/bvals_var.cpp:212-296, bvals/cc/bvals_cc.cpp:201-336,
+ * bvals/fc/bvals_fc.cpp:344-402,583-700, utils/buffer_utils.cpp:24-89). */
+int agx_bvals_exchange(agx_mesh *m, int which);   /* bit 1: u, bit 2: b, bit 4: phi */
+/* EquationOfState::ConservedToPrimitive (eos/adiabatic_mhd.cpp:41-90) with
+ * Field::CalculateCellCenteredField (field/field.cpp:112-180), interior + ghosts. */
+int agx_cons2prim(agx_mesh *m);
+/* Hydro::NewBlockTimeStep (hydro/new_blockdt.cpp:42-190), min over MeshBlocks. */
+int agx_new_block_dt(agx_mesh *m, double *min_dt);
+/* Mesh::NewTimeStep (mesh/mesh.cpp:1078-1119). */
+int agx_new_time_step(agx_mesh *m, double tlim);
+/* Mesh::Initialize, the part after the problem generator (mesh/mesh.cpp:1401-1649):
+ * first gravity solve, ghost exchange, ConservedToPrimitive, first time step. */
+int agx_initialize(agx_mesh *m, double tlim);
+/* ncycles iterations of the main loop (main.cpp:430-489): both VL2 stages with a
+ * gravity solve after each (main.cpp:459-466), time += dt, Mesh::NewTimeStep.
+ * Stops early once time >= tlim.  Returns the number of cycles run in *done. */
+int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done);
+
+/* ---- multigrid self-gravity */
+/* MGGravityDriver::Solve(stage), gravity/mg_gravity.cpp:166-211, with
+ * MultigridDriver::SetupMultigrid/SolveFMGCycle/SolveIterative(FixedTimes)
+ * (multigrid/multigrid_driver.cpp:343-460,700-771) on periodic boundaries. */
+int agx_mg_solve(agx_mesh *m, int stage);
+/* <gravity>/mgmode, threshold, niteration at run time (mg_gravity.cpp:44-53). */
+int agx_mg_set_mode(agx_mesh *m, int mode, double threshold, int32_t niter);
+/* defect norm and V-cycle count of the last solve */
+int agx_mg_last(agx_mesh *m, double *norm, int32_t *iters);
+/* n V(1,1) cycles on the current finest-level state without setup or norms
+ * (MultigridDriver::SolveVCycle, multigrid_driver.cpp:684-693): the unit of the
+ * V-cycles/s benchmark. */
+int agx_mg_vcycles(agx_mesh *m, int32_t n);
+
+/* ---- built-in timers (HIP events on the mesh stream around each launch) */
+int agx_timing_enable(agx_mesh *m, int enable);
+int agx_timing_reset(agx_mesh *m);
+int agx_timing_get(agx_mesh *m, int kclass, double *ms_total, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATHENA_GAMMA_AMD_H_ */

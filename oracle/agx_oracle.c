@@ -1930,3 +1930,27 @@ void orc_conserved_totals(orc_ctx *c, double *out8) {
         }
   }
 }
+
+/* ================================================================ unit-level probes */
+/* One Riemann-solver call in the solver's local frame (wave arrays ordered IDN, IVX
+ * (normal), IVY, IVZ, IPR, IBY, IBZ): rs 0 = HLLD, 1 = HLLE.  flx[7] as flxi. */
+void orc_riemann(int rs, double gamma, const double *wl, const double *wr, double bx,
+                 double *flx) {
+  if (rs == 1) hlle(gamma, wl, wr, bx, flx);
+  else hlld(gamma, wl, wr, bx, flx);
+}
+
+/* PLM on a 1-D pencil q[0..n-1] with face-offset factors dxp/dxm (plm.cpp:69-77,
+ * 113-119): wl[i+1] (upper face of cell i) and wr[i] (lower face), i = 1..n-2. */
+void orc_plm_pencil(int n, const double *q, const double *dxp, const double *dxm,
+                    double *wl, double *wr) {
+  for (int i = 1; i < n - 1; ++i) {
+    double dwl = (q[i] - q[i - 1]);
+    double dwr = (q[i + 1] - q[i]);
+    double dw2 = dwl * dwr;
+    double dwm = 2.0 * dw2 / (dwl + dwr);
+    if (dw2 <= 0.0) dwm = 0.0;
+    wl[i + 1] = q[i] + dxp[i] * dwm;
+    wr[i] = q[i] - dxm[i] * dwm;
+  }
+}

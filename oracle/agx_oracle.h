@@ -83,6 +83,12 @@ void orc_pgen_linear_wave(orc_ctx *c, int wave_flag, double amp, double vflow);
 void orc_linear_wave_errors(orc_ctx *c, int wave_flag, double amp, double vflow,
                             double *out);                /* linear_wave.cpp:199-425 */
 double orc_max_divb(orc_ctx *c);
+
+/* unit-level probes */
+void orc_riemann(int rs, double gamma, const double *wl, const double *wr, double bx,
+                 double *flx);
+void orc_plm_pencil(int n, const double *q, const double *dxp, const double *dxm,
+                    double *wl, double *wr);
 void orc_conserved_totals(orc_ctx *c, double *out8);
 
 #ifdef __cplusplus

@@ -81,6 +81,8 @@ def _lib():
     lib.orc_max_divb.restype = ctypes.c_double
     lib.orc_max_divb.argtypes = [P]
     lib.orc_conserved_totals.argtypes = [P, D]
+    lib.orc_riemann.argtypes = [ctypes.c_int, ctypes.c_double, D, D, ctypes.c_double, D]
+    lib.orc_plm_pencil.argtypes = [ctypes.c_int, D, D, D, D, D]
     return lib
 
 
@@ -204,3 +206,27 @@ class Oracle:
 
     def mg_last(self):
         return self.L.orc_mg_last_norm(self.h), self.L.orc_mg_last_iters(self.h)
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def riemann(rs, gamma, wl, wr, bx):
+    """One interface flux [7] from the oracle's HLLD (rs=0) or HLLE (rs=1)."""
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    out = np.zeros(7)
+    lib().orc_riemann(rs, gamma, _dp(wl), _dp(wr), bx, _dp(out))
+    return out
+
+
+def plm_pencil(q, dxp=None, dxm=None):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n = q.size
+    dxp = np.full(n, 0.5) if dxp is None else np.ascontiguousarray(dxp, dtype=np.float64)
+    dxm = np.full(n, 0.5) if dxm is None else np.ascontiguousarray(dxm, dtype=np.float64)
+    wl = np.full(n, np.nan)
+    wr = np.full(n, np.nan)
+    lib().orc_plm_pencil(n, _dp(q), _dp(dxp), _dp(dxm), _dp(wl), _dp(wr))
+    return wl, wr

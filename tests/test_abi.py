@@ -1,0 +1,67 @@
+"""The C-ABI library loads, exports every symbol include/athena_gamma_amd.h declares,
+and rejects invalid descriptors on the host (no GPU needed for these checks)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from tests.conftest import ROOT, load_agx
+
+HEADER = os.path.join(ROOT, "include", "athena_gamma_amd.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(agx_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_matches_binding_list(agx):
+    assert declared_functions() == sorted(agx.EXPORTS)
+
+
+def test_library_exports_all(agx):
+    L = agx.lib()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+
+
+def test_desc_layout(agx):
+    from oracle.orc import Desc as ODesc
+    assert ctypes.sizeof(agx.Desc) == ctypes.sizeof(ODesc)
+    assert [f[0] for f in agx.Desc._fields_] == [f[0] for f in ODesc._fields_]
+    # 14 int32 (56 B) then 13 doubles, naturally aligned
+    assert ctypes.sizeof(agx.Desc) == 56 + 13 * 8
+
+
+@pytest.mark.parametrize("bad,code", [
+    (dict(nx=(30, 16, 16), bnx=(16, 16, 16)), -3),      # mesh not a multiple of the block
+    (dict(nghost=1), -5),
+    (dict(xorder=4), -6),
+    (dict(rsolver=9), -8),
+    (dict(gamma=1.0), -11),
+])
+def test_invalid_desc_rejected(agx, bad, code):
+    kw = dict(nx=(32, 16, 16), bnx=(16, 16, 16))
+    kw.update(bad)
+    d = agx.make_desc(**kw)
+    h = ctypes.c_void_p()
+    rc = agx.lib().agx_mesh_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == code
+    assert not h.value
+    assert len(agx.lib().agx_last_error()) > 0
+
+
+def test_null_mesh_calls_fail(agx):
+    L = agx.lib()
+    assert L.agx_calculate_fluxes(None, 2) < 0
+    assert L.agx_cycle(None, 1.0, 1, None) < 0
+    assert L.agx_mg_solve(None, 0) < 0
+
+
+def test_scatter_gather_roundtrip(agx):
+    import numpy as np
+    d = agx.make_desc(nx=(8, 12, 4), bnx=(4, 4, 2))
+    G = np.random.default_rng(0).normal(size=(3, 4, 12, 8))
+    assert np.array_equal(agx.gather_cells(d, agx.scatter_cells(d, G)), G)
