@@ -329,8 +329,12 @@ int mg_create(agx_mesh *m) {
     Lv.src = p; p += n;
     Lv.def = p; p += n;
   }
-  m->red_n = (lcells(m->lev[L - 1]) + BS - 1) / BS + 1;
-  AGX_HIP(hipMalloc(&m->red, sizeof(double) * m->red_n));
+  int64_t need = (lcells(m->lev[L - 1]) + BS - 1) / BS + 1;
+  if (need > m->red_n) {
+    if (m->red) AGX_HIP(hipFree(m->red));
+    m->red_n = need;
+    AGX_HIP(hipMalloc(&m->red, sizeof(double) * m->red_n));
+  }
   AGX_HIP(hipStreamSynchronize(m->stream));
   m->mg_on = 1;
   return 0;

@@ -13,6 +13,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -361,6 +362,314 @@ __device__ __forceinline__ void hlle(double gamma, const double *wli, const doub
   for (int n = 0; n < NWAVE; ++n) flxi[n] = 0.5 * (fl[n] + fr[n]) + (fl[n] - fr[n]) * tmp;
 }
 
+// SIGN, defs.hpp.in:117
+__device__ __forceinline__ double sgn(double x) { return (x < 0.0) ? -1.0 : 1.0; }
+
+// PiecewiseParabolicX1..3 for one variable of one cell on a uniform Cartesian mesh
+// (reconstruct/ppm.cpp:58-338): CW84 fourth-order interface values with c1..c4 = 1/2,
+// c5 = 1/6, c6 = -1/6 (reconstruction.cpp:429-438), the CS08 extremum-preserving
+// limiter with C2 = 1.25 and the 1e-12 round-off guards.  q[0..4] = values at i-2..i+2.
+__device__ __forceinline__ void ppm_cell(const double *q, double &qminus, double &qplus) {
+  const double C2 = 1.25;
+  const double c1 = 0.5, c2 = 0.5, c3 = 0.5, c4 = 0.5, c5 = 1.0 / 6.0, c6 = -1.0 / 6.0;
+  const double q_im2 = q[0], q_im1 = q[1], qc = q[2], q_ip1 = q[3], q_ip2 = q[4];
+  double qa = (qc - q_im1);
+  double qb = (q_ip1 - qc);
+  double dd_im1 = c1 * qa + c2 * (q_im1 - q_im2);
+  double dd = c1 * qb + c2 * qa;
+  double dd_ip1 = c1 * (q_ip2 - q_ip1) + c2 * qb;
+  double dph = (c3 * q_im1 + c4 * qc) + (c5 * dd_im1 + c6 * dd);
+  double dph_ip1 = (c3 * qc + c4 * q_ip1) + (c5 * dd + c6 * dd_ip1);
+  double d2qc_im1 = q_im2 + qc - 2.0 * q_im1;
+  double d2qc = q_im1 + q_ip1 - 2.0 * qc;
+  double d2qc_ip1 = qc + q_ip2 - 2.0 * q_ip1;
+  {
+    double qa_tmp = dph - q_im1;
+    double qb_tmp = qc - dph;
+    double a = 3.0 * (q_im1 + qc - 2.0 * dph);
+    double qd = 0.0;
+    if (sgn(a) == sgn(d2qc_im1) && sgn(a) == sgn(d2qc))
+      qd = sgn(a) * dmin(C2 * fabs(d2qc_im1), dmin(C2 * fabs(d2qc), fabs(a)));
+    double dph_tmp = 0.5 * (q_im1 + qc) - qd / 6.0;
+    if (qa_tmp * qb_tmp < 0.0) dph = dph_tmp;
+  }
+  {
+    double qa_tmp = dph_ip1 - qc;
+    double qb_tmp = q_ip1 - dph_ip1;
+    double a = 3.0 * (qc + q_ip1 - 2.0 * dph_ip1);
+    double qd = 0.0;
+    if (sgn(a) == sgn(d2qc) && sgn(a) == sgn(d2qc_ip1))
+      qd = sgn(a) * dmin(C2 * fabs(d2qc), dmin(C2 * fabs(d2qc_ip1), fabs(a)));
+    double dphip1_tmp = 0.5 * (qc + q_ip1) - qd / 6.0;
+    if (qa_tmp * qb_tmp < 0.0) dph_ip1 = dphip1_tmp;
+  }
+  double d2qf = 6.0 * (dph + dph_ip1 - 2.0 * qc);
+  double qm = dph, qp = dph_ip1;
+  double dqf_minus = qc - qm;
+  double dqf_plus = qp - qc;
+  double qa_tmp = dqf_minus * dqf_plus;
+  double qb_tmp = (q_ip1 - qc) * (qc - q_im1);
+  double qe = 0.0;
+  if (sgn(d2qc_im1) == sgn(d2qc) && sgn(d2qc_im1) == sgn(d2qc_ip1) && sgn(d2qc_im1) == sgn(d2qf))
+    qe = sgn(d2qf) * dmin(dmin(C2 * fabs(d2qc_im1), C2 * fabs(d2qc)), dmin(C2 * fabs(d2qc_ip1), fabs(d2qf)));
+  double ma = dmax(fabs(q_im1), fabs(q_im2));
+  double mb = dmax(dmax(fabs(qc), fabs(q_ip1)), fabs(q_ip2));
+  double rho = 0.0;
+  if (fabs(d2qf) > (1.0e-12) * dmax(ma, mb)) rho = qe / d2qf;
+  double tmp_m = qc - rho * dqf_minus;
+  double tmp_p = qc + rho * dqf_plus;
+  double tmp2_m = qc - 2.0 * dqf_plus;
+  double tmp2_p = qc + 2.0 * dqf_minus;
+  if ((qa_tmp <= 0.0 || qb_tmp <= 0.0)) {
+    if (rho <= (1.0 - (1.0e-12))) {
+      qm = tmp_m;
+      qp = tmp_p;
+    }
+  } else {
+    if (fabs(dqf_minus) >= 2.0 * fabs(dqf_plus)) qm = tmp2_m;
+    if (fabs(dqf_plus) >= 2.0 * fabs(dqf_minus)) qp = tmp2_p;
+  }
+  qminus = qm;
+  qplus = qp;
+}
+
+// Roe solver for adiabatic MHD at one interface, hydro/rsolvers/mhd/roe_mhd.cpp:56-212
+// with RoeFlux (:245-498): Roe-averaged eigensystem, wave-strength dissipation, LLF
+// fallback for the hydro variables when an intermediate density is negative.
+__device__ __forceinline__ void roe(double gamma, const double *wli, const double *wri,
+                                    double bxi, double *flxi) {
+  double gm1 = gamma - 1.0;
+  double wroe[NWAVE], fl[NWAVE], fr[NWAVE], du[NWAVE], ev[NWAVE];
+  double sqrtdl = sqrt(wli[IDN]);
+  double sqrtdr = sqrt(wri[IDN]);
+  double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+  wroe[IDN] = sqrtdl * sqrtdr;
+  wroe[IVX] = (sqrtdl * wli[IVX] + sqrtdr * wri[IVX]) * isdlpdr;
+  wroe[IVY] = (sqrtdl * wli[IVY] + sqrtdr * wri[IVY]) * isdlpdr;
+  wroe[IVZ] = (sqrtdl * wli[IVZ] + sqrtdr * wri[IVZ]) * isdlpdr;
+  wroe[IBY] = (sqrtdr * wli[IBY] + sqrtdl * wri[IBY]) * isdlpdr;
+  wroe[IBZ] = (sqrtdr * wli[IBZ] + sqrtdl * wri[IBZ]) * isdlpdr;
+  double x = 0.5 * (SQR(wli[IBY] - wri[IBY]) + SQR(wli[IBZ] - wri[IBZ])) / (SQR(sqrtdl + sqrtdr));
+  double y = 0.5 * (wli[IDN] + wri[IDN]) / wroe[IDN];
+  double pbl = 0.5 * (bxi * bxi + SQR(wli[IBY]) + SQR(wli[IBZ]));
+  double pbr = 0.5 * (bxi * bxi + SQR(wri[IBY]) + SQR(wri[IBZ]));
+  double el = wli[IPR] / gm1 + 0.5 * wli[IDN] * (SQR(wli[IVX]) + SQR(wli[IVY]) + SQR(wli[IVZ])) + pbl;
+  double er = wri[IPR] / gm1 + 0.5 * wri[IDN] * (SQR(wri[IVX]) + SQR(wri[IVY]) + SQR(wri[IVZ])) + pbr;
+  wroe[IPR] = ((el + wli[IPR] + pbl) / sqrtdl + (er + wri[IPR] + pbr) / sqrtdr) * isdlpdr;
+  double mxl = wli[IDN] * wli[IVX];
+  double mxr = wri[IDN] * wri[IVX];
+  fl[IDN] = mxl;
+  fr[IDN] = mxr;
+  fl[IVX] = mxl * wli[IVX] + pbl - SQR(bxi);
+  fr[IVX] = mxr * wri[IVX] + pbr - SQR(bxi);
+  fl[IVY] = mxl * wli[IVY] - bxi * wli[IBY];
+  fr[IVY] = mxr * wri[IVY] - bxi * wri[IBY];
+  fl[IVZ] = mxl * wli[IVZ] - bxi * wli[IBZ];
+  fr[IVZ] = mxr * wri[IVZ] - bxi * wri[IBZ];
+  fl[IVX] += wli[IPR];
+  fr[IVX] += wri[IPR];
+  fl[IEN] = (el + wli[IPR] + pbl - bxi * bxi) * wli[IVX];
+  fr[IEN] = (er + wri[IPR] + pbr - bxi * bxi) * wri[IVX];
+  fl[IEN] -= bxi * (wli[IBY] * wli[IVY] + wli[IBZ] * wli[IVZ]);
+  fr[IEN] -= bxi * (wri[IBY] * wri[IVY] + wri[IBZ] * wri[IVZ]);
+  fl[IBY] = wli[IBY] * wli[IVX] - bxi * wli[IVY];
+  fr[IBY] = wri[IBY] * wri[IVX] - bxi * wri[IVY];
+  fl[IBZ] = wli[IBZ] * wli[IVX] - bxi * wli[IVZ];
+  fr[IBZ] = wri[IBZ] * wri[IVX] - bxi * wri[IVZ];
+  du[IDN] = wri[IDN] - wli[IDN];
+  du[IVX] = wri[IDN] * wri[IVX] - wli[IDN] * wli[IVX];
+  du[IVY] = wri[IDN] * wri[IVY] - wli[IDN] * wli[IVY];
+  du[IVZ] = wri[IDN] * wri[IVZ] - wli[IDN] * wli[IVZ];
+  du[IEN] = er - el;
+  du[IBY] = wri[IBY] - wli[IBY];
+  du[IBZ] = wri[IBZ] - wli[IBZ];
+#pragma unroll
+  for (int n = 0; n < NWAVE; ++n) flxi[n] = 0.5 * (fl[n] + fr[n]);
+  int llf_flag = 0;
+  {
+    /* RoeFlux, adiabatic branch (roe_mhd.cpp:245-498) */
+    const double b1 = bxi;
+    double d = wroe[IDN], v1 = wroe[IVX], v2 = wroe[IVY], v3 = wroe[IVZ];
+    double b2 = wroe[IBY], b3 = wroe[IBZ];
+    double di = 1.0 / d;
+    double btsq = b2 * b2 + b3 * b3;
+    double vaxsq = b1 * b1 * di;
+    double vsq = v1 * v1 + v2 * v2 + v3 * v3;
+    double hp = wroe[IPR] - (vaxsq + btsq * di);
+    double bt_starsq = (gm1 - (gm1 - 1.0) * y) * btsq;
+    double twid_csq = dmax((gm1 * (hp - 0.5 * vsq) - (gm1 - 1.0) * x), 1.0e-20);
+    double ct2 = bt_starsq * di;
+    double tsum = vaxsq + ct2 + twid_csq;
+    double tdif = vaxsq + ct2 - twid_csq;
+    double cf2_cs2 = sqrt(tdif * tdif + 4.0 * twid_csq * ct2);
+    double cfsq = 0.5 * (tsum + cf2_cs2);
+    double cf = sqrt(cfsq);
+    double cssq = twid_csq * vaxsq / cfsq;
+    double cs = sqrt(cssq);
+    double bt = sqrt(btsq);
+    double bt_star = sqrt(bt_starsq);
+    double bet2 = 0.0, bet3 = 0.0;
+    if (bt != 0.0) { bet2 = b2 / bt; bet3 = b3 / bt; }
+    double bet2_star = bet2 / sqrt(gm1 - (gm1 - 1.0) * y);
+    double bet3_star = bet3 / sqrt(gm1 - (gm1 - 1.0) * y);
+    double bet_starsq = bet2_star * bet2_star + bet3_star * bet3_star;
+    double vbet = v2 * bet2_star + v3 * bet3_star;
+    double q2_star = 0.0, q3_star = 0.0;
+    if (bet_starsq != 0.0) { q2_star = bet2_star / bet_starsq; q3_star = bet3_star / bet_starsq; }
+    double alpha_f, alpha_s;
+    if ((cfsq - cssq) <= 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
+    else if ((twid_csq - cssq) <= 0.0) { alpha_f = 0.0; alpha_s = 1.0; }
+    else if ((cfsq - twid_csq) <= 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
+    else {
+      alpha_f = sqrt((twid_csq - cssq) / (cfsq - cssq));
+      alpha_s = sqrt((cfsq - twid_csq) / (cfsq - cssq));
+    }
+    double sqrtd = sqrt(d);
+    double isqrtd = 1.0 / sqrtd;
+    double s = (b1 < 0.0) ? -1.0 : 1.0;
+    double twid_c = sqrt(twid_csq);
+    double qf = cf * alpha_f * s;
+    double qs = cs * alpha_s * s;
+    double af_prime = twid_c * alpha_f * isqrtd;
+    double as_prime = twid_c * alpha_s * isqrtd;
+    double afpbb = af_prime * bt_star * bet_starsq;
+    double aspbb = as_prime * bt_star * bet_starsq;
+    double vqstr = (v2 * q2_star + v3 * q3_star);
+    double vax = sqrt(vaxsq);
+    double norm = 0.5 / twid_csq;
+    double cff = norm * alpha_f * cf;
+    double css = norm * alpha_s * cs;
+    double qf_hat = qf * norm;
+    double qs_hat = qs * norm;
+    double af = norm * af_prime * d;
+    double as = norm * as_prime * d;
+    double afpb = norm * af_prime * bt_star;
+    double aspb = norm * as_prime * bt_star;
+    ev[0] = v1 - cf; ev[1] = v1 - vax; ev[2] = v1 - cs; ev[3] = v1;
+    ev[4] = v1 + cs; ev[5] = v1 + vax; ev[6] = v1 + cf;
+    double a[NWAVE];
+    double alpha_f_bar = alpha_f * gm1 * norm;
+    double alpha_s_bar = alpha_s * gm1 * norm;
+    double gm1a = gm1 / twid_csq;
+    a[0] = du[0] * (alpha_f_bar * (vsq - hp) + cff * (cf + v1) - qs_hat * vqstr - aspb);
+    a[0] -= du[1] * (alpha_f_bar * v1 + cff);
+    a[0] -= du[2] * (alpha_f_bar * v2 - qs_hat * q2_star);
+    a[0] -= du[3] * (alpha_f_bar * v3 - qs_hat * q3_star);
+    a[0] += du[4] * alpha_f_bar;
+    a[0] += du[5] * (as * q2_star - alpha_f_bar * b2);
+    a[0] += du[6] * (as * q3_star - alpha_f_bar * b3);
+    a[1] = du[0] * (v2 * bet3 - v3 * bet2);
+    a[1] -= du[2] * bet3;
+    a[1] += du[3] * bet2;
+    a[1] -= du[5] * sqrtd * bet3 * s;
+    a[1] += du[6] * sqrtd * bet2 * s;
+    a[1] *= 0.5;
+    a[2] = du[0] * (alpha_s_bar * (vsq - hp) + css * (cs + v1) + qf_hat * vqstr + afpb);
+    a[2] -= du[1] * (alpha_s_bar * v1 + css);
+    a[2] -= du[2] * (alpha_s_bar * v2 + qf_hat * q2_star);
+    a[2] -= du[3] * (alpha_s_bar * v3 + qf_hat * q3_star);
+    a[2] += du[4] * alpha_s_bar;
+    a[2] -= du[5] * (af * q2_star + alpha_s_bar * b2);
+    a[2] -= du[6] * (af * q3_star + alpha_s_bar * b3);
+    a[3] = du[0] * (1.0 - gm1a * (0.5 * vsq - (gm1 - 1.0) * x / gm1));
+    a[3] += du[1] * gm1a * v1;
+    a[3] += du[2] * gm1a * v2;
+    a[3] += du[3] * gm1a * v3;
+    a[3] -= du[4] * gm1a;
+    a[3] += du[5] * gm1a * b2;
+    a[3] += du[6] * gm1a * b3;
+    a[4] = du[0] * (alpha_s_bar * (vsq - hp) + css * (cs - v1) - qf_hat * vqstr + afpb);
+    a[4] -= du[1] * (alpha_s_bar * v1 - css);
+    a[4] -= du[2] * (alpha_s_bar * v2 - qf_hat * q2_star);
+    a[4] -= du[3] * (alpha_s_bar * v3 - qf_hat * q3_star);
+    a[4] += du[4] * alpha_s_bar;
+    a[4] -= du[5] * (af * q2_star + alpha_s_bar * b2);
+    a[4] -= du[6] * (af * q3_star + alpha_s_bar * b3);
+    a[5] = du[0] * (v3 * bet2 - v2 * bet3);
+    a[5] += du[2] * bet3;
+    a[5] -= du[3] * bet2;
+    a[5] -= du[5] * sqrtd * bet3 * s;
+    a[5] += du[6] * sqrtd * bet2 * s;
+    a[5] *= 0.5;
+    a[6] = du[0] * (alpha_f_bar * (vsq - hp) + cff * (cf - v1) + qs_hat * vqstr - aspb);
+    a[6] -= du[1] * (alpha_f_bar * v1 - cff);
+    a[6] -= du[2] * (alpha_f_bar * v2 + qs_hat * q2_star);
+    a[6] -= du[3] * (alpha_f_bar * v3 + qs_hat * q3_star);
+    a[6] += du[4] * alpha_f_bar;
+    a[6] += du[5] * (as * q2_star - alpha_f_bar * b2);
+    a[6] += du[6] * (as * q3_star - alpha_f_bar * b3);
+    double coeff[NWAVE];
+    for (int n = 0; n < NWAVE; ++n) coeff[n] = -0.5 * fabs(ev[n]) * a[n];
+    double dens = wli[IDN] + a[0] * alpha_f;
+    if (dens < 0.0) llf_flag = 1;
+    dens += a[2] * alpha_s;
+    if (dens < 0.0) llf_flag = 1;
+    dens += a[3];
+    if (dens < 0.0) llf_flag = 1;
+    dens += a[4] * alpha_s;
+    if (dens < 0.0) llf_flag = 1;
+    flxi[0] += coeff[0] * alpha_f;
+    flxi[0] += coeff[2] * alpha_s;
+    flxi[0] += coeff[3];
+    flxi[0] += coeff[4] * alpha_s;
+    flxi[0] += coeff[6] * alpha_f;
+    flxi[1] += coeff[0] * (alpha_f * (v1 - cf));
+    flxi[1] += coeff[2] * (alpha_s * (v1 - cs));
+    flxi[1] += coeff[3] * v1;
+    flxi[1] += coeff[4] * (alpha_s * (v1 + cs));
+    flxi[1] += coeff[6] * (alpha_f * (v1 + cf));
+    flxi[2] += coeff[0] * (alpha_f * v2 + qs * bet2_star);
+    flxi[2] -= coeff[1] * bet3;
+    flxi[2] += coeff[2] * (alpha_s * v2 - qf * bet2_star);
+    flxi[2] += coeff[3] * v2;
+    flxi[2] += coeff[4] * (alpha_s * v2 + qf * bet2_star);
+    flxi[2] += coeff[5] * bet3;
+    flxi[2] += coeff[6] * (alpha_f * v2 - qs * bet2_star);
+    flxi[3] += coeff[0] * (alpha_f * v3 + qs * bet3_star);
+    flxi[3] += coeff[1] * bet2;
+    flxi[3] += coeff[2] * (alpha_s * v3 - qf * bet3_star);
+    flxi[3] += coeff[3] * v3;
+    flxi[3] += coeff[4] * (alpha_s * v3 + qf * bet3_star);
+    flxi[3] -= coeff[5] * bet2;
+    flxi[3] += coeff[6] * (alpha_f * v3 - qs * bet3_star);
+    flxi[4] += coeff[0] * (alpha_f * (hp - v1 * cf) + qs * vbet + aspbb);
+    flxi[4] -= coeff[1] * (v2 * bet3 - v3 * bet2);
+    flxi[4] += coeff[2] * (alpha_s * (hp - v1 * cs) - qf * vbet - afpbb);
+    flxi[4] += coeff[3] * (0.5 * vsq + (gm1 - 1.0) * x / gm1);
+    flxi[4] += coeff[4] * (alpha_s * (hp + v1 * cs) + qf * vbet - afpbb);
+    flxi[4] += coeff[5] * (v1 * bet3 - v3 * bet2);
+    flxi[4] += coeff[6] * (alpha_f * (hp + v1 * cf) - qs * vbet + aspbb);
+    flxi[5] += coeff[0] * as_prime * bet2_star;
+    flxi[5] -= coeff[1] * bet3 * s * isqrtd;
+    flxi[5] -= coeff[2] * af_prime * bet2_star;
+    flxi[5] -= coeff[4] * af_prime * bet2_star;
+    flxi[5] -= coeff[5] * bet3 * s * isqrtd;
+    flxi[5] += coeff[6] * as_prime * bet2_star;
+    flxi[6] += coeff[0] * as_prime * bet3_star;
+    flxi[6] += coeff[1] * bet2 * s * isqrtd;
+    flxi[6] -= coeff[2] * af_prime * bet3_star;
+    flxi[6] -= coeff[4] * af_prime * bet3_star;
+    flxi[6] += coeff[5] * bet2 * s * isqrtd;
+    flxi[6] += coeff[6] * as_prime * bet3_star;
+  }
+  if (ev[0] >= 0.0) {
+    for (int n = 0; n < NWAVE; ++n) flxi[n] = fl[n];
+  }
+  if (ev[NWAVE - 1] <= 0.0) {
+    for (int n = 0; n < NWAVE; ++n) flxi[n] = fr[n];
+  }
+  if (llf_flag != 0) {
+    double cfl = fast_speed(gamma, wli, bxi);
+    double cfr = fast_speed(gamma, wri, bxi);
+    double a = 0.5 * dmax((fabs(wli[IVX]) + cfl), (fabs(wri[IVX]) + cfr));
+    flxi[IDN] = 0.5 * (fl[IDN] + fr[IDN]) - a * du[IDN];
+    flxi[IVX] = 0.5 * (fl[IVX] + fr[IVX]) - a * du[IVX];
+    flxi[IVY] = 0.5 * (fl[IVY] + fr[IVY]) - a * du[IVY];
+    flxi[IVZ] = 0.5 * (fl[IVZ] + fr[IVZ]) - a * du[IVZ];
+    flxi[IEN] = 0.5 * (fl[IEN] + fr[IEN]) - a * du[IEN];
+  }
+}
+
 // ================================================================ kernels
 // Hydro::CalculateFluxes for direction DIR (hydro/calculate_fluxes.cpp:56-321):
 // reconstruction (ORDER 1: reconstruct/dc.cpp:24-104; ORDER 2: the uniform-mesh van
@@ -371,7 +680,7 @@ __global__ void __launch_bounds__(256)
 k_flux(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
        const double *__restrict__ bn, double *__restrict__ flx, double *__restrict__ ey,
        double *__restrict__ ez, double *__restrict__ wct, const BlockCoef *__restrict__ coef,
-       const double *__restrict__ plm, double gamma, double dt) {
+       const double *__restrict__ plm, double gamma, double dt, double dfl, double pfl) {
   int il = g.is - (DIR != 0), iu = g.ie + 1;
   int jl = g.js - (DIR != 1), ju = g.je + 1;
   int kl = g.ks - (DIR != 2);
@@ -395,6 +704,15 @@ k_flux(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
     if (ORDER == 1) {
       wl[n] = q[-st];
       wr[n] = q[0];
+    } else if (ORDER == 3) {
+      double q6[6];
+#pragma unroll
+      for (int t = 0; t < 6; ++t) q6[t] = q[(t - 3) * st];
+      double m0, p0, m1, p1;
+      ppm_cell(q6, m0, p0);       // cell c-1: values at c-3..c+1
+      ppm_cell(q6 + 1, m1, p1);   // cell c:   values at c-2..c+2
+      wl[n] = p0;
+      wr[n] = m1;
     } else {
       double qm2 = q[-2 * st], qm1 = q[-st], q0 = q[0], qp1 = q[st];
       // left state: upper face of cell c-1
@@ -412,6 +730,13 @@ k_flux(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
       wr[n] = q0 - pc[g.PL + cidx] * dwm;
     }
   }
+  if (ORDER == 3) {
+    // ApplyPrimitiveFloors on both reconstructed states (ppm.cpp:333-337)
+    wl[IDN] = (wl[IDN] > dfl) ? wl[IDN] : dfl;
+    wl[IPR] = (wl[IPR] > pfl) ? wl[IPR] : pfl;
+    wr[IDN] = (wr[IDN] > dfl) ? wr[IDN] : dfl;
+    wr[IPR] = (wr[IPR] > pfl) ? wr[IPR] : pfl;
+  }
   constexpr int ivx = IVX + DIR, ivy = IVX + (DIR + 1) % 3, ivz = IVX + (DIR + 2) % 3;
   double wli[NWAVE], wri[NWAVE], flxi[NWAVE];
   wli[IDN] = wl[IDN]; wli[IVX] = wl[ivx]; wli[IVY] = wl[ivy]; wli[IVZ] = wl[ivz];
@@ -420,6 +745,7 @@ k_flux(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
   wri[IPR] = wr[IPR]; wri[IBY] = wr[IBY]; wri[IBZ] = wr[IBZ];
   double bxi = bn[(int64_t)p.b * g.S + id];
   if (RS == AGX_RSOLVER_HLLE) hlle(gamma, wli, wri, bxi, flxi);
+  else if (RS == AGX_RSOLVER_ROE) roe(gamma, wli, wri, bxi, flxi);
   else hlld(gamma, wli, wri, bxi, flxi);
   double *fb = flx + (int64_t)p.b * NHYDRO * g.S + id;
   fb[IDN * g.S] = flxi[IDN];
@@ -661,15 +987,293 @@ k_cons2prim(Geo g, double *__restrict__ u, double *__restrict__ w, double *__res
   w[c + 4 * S] = w_p;
 }
 
+// ================================================================ fused stage kernels
+// The kernels below implement the same VL2 stage as k_integrate_u + k_ct + the ghost
+// exchange + k_cons2prim with fewer HBM passes, keeping the reference's data flow
+// exactly: interior cells are updated and converted to primitives in one pass; ghost
+// cells receive the neighbour's un-floored conserved state and are converted from
+// their own face field (own shared face + exchanged ghost faces), as
+// bvals_cc.cpp:300-336 followed by adiabatic_mhd.cpp:41-90 over the ghost zones do.
+// The density/pressure floors of interior cells are written back to u only after the
+// ghost copies are taken (k_floor_fix), so ghosts see the same un-floored values.
+
+// C2P of one cell in registers (eos/adiabatic_mhd.cpp:52-86).  Returns true when a
+// floor changed the conserved state.
+__device__ __forceinline__ bool c2p_cell(double &u_d, double u_m1, double u_m2, double u_m3,
+                                         double &u_e, double bcc1, double bcc2, double bcc3,
+                                         double gm1, double dfl, double pfl, double *wv) {
+  bool fl = !(u_d > dfl);
+  u_d = (u_d > dfl) ? u_d : dfl;
+  double di = 1.0 / u_d;
+  double pb = 0.5 * (SQR(bcc1) + SQR(bcc2) + SQR(bcc3));
+  double e_k = 0.5 * di * (SQR(u_m1) + SQR(u_m2) + SQR(u_m3));
+  double w_p = gm1 * (u_e - e_k - pb);
+  fl = fl || !(w_p > pfl);
+  u_e = (w_p > pfl) ? u_e : ((pfl / gm1) + e_k + pb);
+  w_p = (w_p > pfl) ? w_p : pfl;
+  wv[0] = u_d;
+  wv[1] = u_m1 * di;
+  wv[2] = u_m2 * di;
+  wv[3] = u_m3 * di;
+  wv[4] = w_p;
+  return fl;
+}
+
+// Interior cells of one VL2 stage: flux divergence + self-gravity (as k_integrate_u),
+// then bcc from the CT-updated faces and C2P.  Writes the un-floored u, w and bcc;
+// raises *floor_flag when any floor fired.
+__global__ void __launch_bounds__(256)
+k_integrate_c2p(Geo g, const double *src, double *dst, const double *__restrict__ f1,
+                const double *__restrict__ f2, const double *__restrict__ f3, double *w,
+                double *__restrict__ bcc, const double *__restrict__ phi,
+                const double *__restrict__ b1, const double *__restrict__ b2,
+                const double *__restrict__ b3, const BlockCoef *__restrict__ coef, double wght,
+                int grav, double gm1, double dfl, double pfl, int *floor_flag) {
+  PlaneIdx p;
+  if (!plane_index(g.is, g.ie, g.js, g.je, g.ks, p)) return;
+  const int64_t S = g.S;
+  const int64_t sj = g.P, sk = (int64_t)g.J * g.P;
+  const int64_t id = IDX(g, p.k, p.j, p.i);
+  const double dx1 = coef[p.b].dx[0], dx2 = coef[p.b].dx[1], dx3 = coef[p.b].dx[2];
+  const double a1 = dx2 * dx3, a2 = dx1 * dx3, a3 = dx1 * dx2;
+  const double vol = dx1 * dx2 * dx3;
+  const int64_t cb = (int64_t)p.b * NHYDRO * S + id;
+  double un[NHYDRO];
+#pragma unroll
+  for (int n = 0; n < NHYDRO; ++n) {
+    const int64_t o = cb + n * S;
+    double dflx = (a1 * f1[o + 1] - a1 * f1[o]);
+    dflx += (a2 * f2[o + sj] - a2 * f2[o]);
+    dflx += (a3 * f3[o + sk] - a3 * f3[o]);
+    un[n] = src[o] - wght * dflx / vol;
+  }
+  if (grav) {
+    const int64_t pb = (int64_t)p.b * S + id;
+    const double rho = w[cb];
+    const double phic = phi[pb];
+    {
+      double dtodx1 = wght / dx1;
+      double phil = 0.5 * (phi[pb - 1] + phic);
+      double phir = 0.5 * (phic + phi[pb + 1]);
+      un[IVX] -= dtodx1 * rho * (phir - phil);
+      un[IEN] -= dtodx1 * (f1[cb] * (phic - phil) + f1[cb + 1] * (phir - phic));
+    }
+    {
+      double dtodx2 = wght / dx2;
+      double phil = 0.5 * (phi[pb - sj] + phic);
+      double phir = 0.5 * (phic + phi[pb + sj]);
+      un[IVY] -= dtodx2 * rho * (phir - phil);
+      un[IEN] -= dtodx2 * (f2[cb] * (phic - phil) + f2[cb + sj] * (phir - phic));
+    }
+    {
+      double dtodx3 = wght / dx3;
+      double phil = 0.5 * (phi[pb - sk] + phic);
+      double phir = 0.5 * (phic + phi[pb + sk]);
+      un[IVZ] -= dtodx3 * rho * (phir - phil);
+      un[IEN] -= dtodx3 * (f3[cb] * (phic - phil) + f3[cb + sk] * (phir - phic));
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < NHYDRO; ++n) dst[cb + n * S] = un[n];
+  const int64_t fo = (int64_t)p.b * S + id;
+  const double bcc1 = 0.5 * b1[fo] + 0.5 * b1[fo + 1];
+  const double bcc2 = 0.5 * b2[fo] + 0.5 * b2[fo + sj];
+  const double bcc3 = 0.5 * b3[fo] + 0.5 * b3[fo + sk];
+  const int64_t bo = (int64_t)p.b * 3 * S + id;
+  bcc[bo] = bcc1;
+  bcc[bo + S] = bcc2;
+  bcc[bo + 2 * S] = bcc3;
+  double u_d = un[IDN], u_e = un[IEN], wv[5];
+  bool fl = c2p_cell(u_d, un[IVX], un[IVY], un[IVZ], u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
+#pragma unroll
+  for (int n = 0; n < NHYDRO; ++n) w[cb + n * S] = wv[n];
+  if (fl) atomicOr(floor_flag, 1);
+}
+
+// Field::CT for all three face components in one launch (grid.y = component).
+__global__ void __launch_bounds__(256)
+k_ct3(Geo g, const double *s1, const double *s2, const double *s3, double *d1, double *d2,
+      double *d3, const double *__restrict__ e1, const double *__restrict__ e2,
+      const double *__restrict__ e3, const BlockCoef *__restrict__ coef, double wght) {
+  const int dir = blockIdx.y;
+  const int ni = g.ie - g.is + 1 + (dir == 0), nj = g.je - g.js + 1 + (dir == 1);
+  const int nk = g.ke - g.ks + 1 + (dir == 2);
+  int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= (int64_t)ni * nj * nk) return;
+  int i = g.is + (int)(q % ni);
+  int64_t r = q / ni;
+  int j = g.js + (int)(r % nj);
+  int k = g.ks + (int)(r / nj);
+  int b = blockIdx.z;
+  const int64_t sj = g.P, sk = (int64_t)g.J * g.P;
+  const int64_t fo = (int64_t)b * g.S + IDX(g, k, j, i);
+  const double dx1 = coef[b].dx[0], dx2 = coef[b].dx[1], dx3 = coef[b].dx[2];
+  if (dir == 0) {
+    double v = s1[fo];
+    double area = dx2 * dx3;
+    v -= (wght / area) * (dx3 * e3[fo + sj] - dx3 * e3[fo]);
+    v += (wght / area) * (dx2 * e2[fo + sk] - dx2 * e2[fo]);
+    d1[fo] = v;
+  } else if (dir == 1) {
+    double v = s2[fo];
+    double area = dx1 * dx3;
+    v += (wght / area) * (dx3 * e3[fo + 1] - dx3 * e3[fo]);
+    v -= (wght / area) * (dx1 * e1[fo + sk] - dx1 * e1[fo]);
+    d2[fo] = v;
+  } else {
+    double v = s3[fo];
+    double area = dx1 * dx2;
+    v -= (wght / area) * (dx2 * e2[fo + 1] - dx2 * e2[fo]);
+    v += (wght / area) * (dx1 * e1[fo + sj] - dx1 * e1[fo]);
+    d3[fo] = v;
+  }
+}
+
+// The q-th element of the shell of the box [0,N1)x[0,N2)x[0,N3) that lies outside the
+// inner box [a1,b1]x[a2,b2]x[a3,b3]: whole planes first, then whole rows of the inner
+// planes, then the cells of the inner rows.
+struct Shell {
+  int N1, N2, N3, a1, b1, a2, b2, a3, b3;
+  int64_t nk, nj, ni;
+};
+
+static Shell make_shell(int N1, int N2, int N3, int a1, int b1, int a2, int b2, int a3, int b3) {
+  Shell s{N1, N2, N3, a1, b1, a2, b2, a3, b3, 0, 0, 0};
+  s.nk = (int64_t)(a3 + (N3 - 1 - b3)) * N2 * N1;
+  s.nj = (int64_t)(b3 - a3 + 1) * (a2 + (N2 - 1 - b2)) * N1;
+  s.ni = (int64_t)(b3 - a3 + 1) * (b2 - a2 + 1) * (a1 + (N1 - 1 - b1));
+  return s;
+}
+
+__device__ __forceinline__ bool shell_elem(const Shell &s, int64_t q, int &k, int &j, int &i) {
+  int64_t plane = (int64_t)s.N1 * s.N2;
+  if (q < s.nk) {
+    int pidx = (int)(q / plane);
+    int64_t r = q - (int64_t)pidx * plane;
+    k = (pidx < s.a3) ? pidx : s.b3 + 1 + (pidx - s.a3);
+    j = (int)(r / s.N1);
+    i = (int)(r % s.N1);
+    return true;
+  }
+  q -= s.nk;
+  if (q < s.nj) {
+    int rows_out = s.a2 + (s.N2 - 1 - s.b2);
+    int64_t per = (int64_t)rows_out * s.N1;
+    int kk = (int)(q / per);
+    int64_t r = q - (int64_t)kk * per;
+    int rr = (int)(r / s.N1);
+    i = (int)(r % s.N1);
+    k = s.a3 + kk;
+    j = (rr < s.a2) ? rr : s.b2 + 1 + (rr - s.a2);
+    return true;
+  }
+  q -= s.nj;
+  if (q < s.ni) {
+    int cols_out = s.a1 + (s.N1 - 1 - s.b1);
+    int nrows = s.b2 - s.a2 + 1;
+    int64_t per = (int64_t)nrows * cols_out;
+    int kk = (int)(q / per);
+    int64_t r = q - (int64_t)kk * per;
+    int jj = (int)(r / cols_out);
+    int cc = (int)(r % cols_out);
+    k = s.a3 + kk;
+    j = s.a2 + jj;
+    i = (cc < s.a1) ? cc : s.b1 + 1 + (cc - s.a1);
+    return true;
+  }
+  return false;
+}
+
+// Ghost faces of all three components (bvals_fc.cpp:344-402,583-700); grid.y = comp.
+__global__ void __launch_bounds__(256)
+k_ghost_faces(Geo g, Shell s1, Shell s2, Shell s3, double *b1, double *b2, double *b3) {
+  const int dir = blockIdx.y;
+  const Shell &s = (dir == 0) ? s1 : (dir == 1 ? s2 : s3);
+  int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= s.nk + s.nj + s.ni) return;
+  int k, j, i;
+  shell_elem(s, q, k, j, i);
+  int o1 = (i < s.a1) ? -1 : (i > s.b1 ? 1 : 0);
+  int o2 = (j < s.a2) ? -1 : (j > s.b2 ? 1 : 0);
+  int o3 = (k < s.a3) ? -1 : (k > s.b3 ? 1 : 0);
+  int b = blockIdx.z;
+  int src = nbr_block(g, b, o1, o2, o3);
+  double *a = (dir == 0) ? b1 : (dir == 1 ? b2 : b3);
+  a[(int64_t)b * g.S + IDX(g, k, j, i)] =
+      a[(int64_t)src * g.S + IDX(g, k - o3 * g.bx3, j - o2 * g.bx2, i - o1 * g.bx1)];
+}
+
+// Ghost cells: copy the neighbour's (un-floored) conserved state, bcc from this block's
+// faces (ghost faces already exchanged), C2P with floors written to the ghost copy.
+__global__ void __launch_bounds__(256)
+k_ghost_cells(Geo g, Shell s, double *__restrict__ u, double *__restrict__ w,
+              double *__restrict__ bcc, const double *__restrict__ b1,
+              const double *__restrict__ b2, const double *__restrict__ b3, double gm1,
+              double dfl, double pfl) {
+  int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= s.nk + s.nj + s.ni) return;
+  int k, j, i;
+  shell_elem(s, q, k, j, i);
+  int o1 = (i < g.is) ? -1 : (i > g.ie ? 1 : 0);
+  int o2 = (j < g.js) ? -1 : (j > g.je ? 1 : 0);
+  int o3 = (k < g.ks) ? -1 : (k > g.ke ? 1 : 0);
+  int b = blockIdx.z;
+  int src = nbr_block(g, b, o1, o2, o3);
+  const int64_t S = g.S;
+  const int64_t id = IDX(g, k, j, i);
+  const int64_t sid = IDX(g, k - o3 * g.bx3, j - o2 * g.bx2, i - o1 * g.bx1);
+  const int64_t c = (int64_t)b * NHYDRO * S + id, cs = (int64_t)src * NHYDRO * S + sid;
+  double u_d = u[cs], u_m1 = u[cs + S], u_m2 = u[cs + 2 * S], u_m3 = u[cs + 3 * S], u_e = u[cs + 4 * S];
+  const int64_t fo = (int64_t)b * S + id;
+  const double bcc1 = 0.5 * b1[fo] + 0.5 * b1[fo + 1];
+  const double bcc2 = 0.5 * b2[fo] + 0.5 * b2[fo + g.P];
+  const double bcc3 = 0.5 * b3[fo] + 0.5 * b3[fo + (int64_t)g.J * g.P];
+  const int64_t bo = (int64_t)b * 3 * S + id;
+  bcc[bo] = bcc1;
+  bcc[bo + S] = bcc2;
+  bcc[bo + 2 * S] = bcc3;
+  double wv[5];
+  c2p_cell(u_d, u_m1, u_m2, u_m3, u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
+  u[c] = u_d;
+  u[c + S] = u_m1;
+  u[c + 2 * S] = u_m2;
+  u[c + 3 * S] = u_m3;
+  u[c + 4 * S] = u_e;
+#pragma unroll
+  for (int n = 0; n < NHYDRO; ++n) w[c + n * S] = wv[n];
+}
+
+// Write the interior floors back to u (ConservedToPrimitive's in-place update) once
+// the ghost copies are taken; a no-op unless k_integrate_c2p saw a floor.
+__global__ void __launch_bounds__(256)
+k_floor_fix(Geo g, double *__restrict__ u, const double *__restrict__ bcc, double gm1,
+            double dfl, double pfl, const int *__restrict__ floor_flag) {
+  if (*floor_flag == 0) return;
+  PlaneIdx p;
+  if (!plane_index(g.is, g.ie, g.js, g.je, g.ks, p)) return;
+  const int64_t S = g.S;
+  const int64_t id = IDX(g, p.k, p.j, p.i);
+  const int64_t c = (int64_t)p.b * NHYDRO * S + id;
+  const int64_t bo = (int64_t)p.b * 3 * S + id;
+  double u_d = u[c], u_e = u[c + 4 * S], wv[5];
+  c2p_cell(u_d, u[c + S], u[c + 2 * S], u[c + 3 * S], u_e, bcc[bo], bcc[bo + S], bcc[bo + 2 * S],
+           gm1, dfl, pfl, wv);
+  u[c] = u_d;
+  u[c + 4 * S] = u_e;
+}
+
+__global__ void k_reset_int(int *p) { *p = 0; }
+
 // Hydro::NewBlockTimeStep (hydro/new_blockdt.cpp:57-160): per-cell dx/(|v|+c_f) per
-// direction, min over all interior cells of all blocks (min is exact, so the reduction
-// order does not matter).  Positive doubles order like their bit patterns, so the
-// global min is an integer atomicMin.
+// direction, min over all interior cells of all blocks.  min is exact, so the reduction
+// order does not matter: wavefront shuffle, then LDS across the workgroup's 4
+// wavefronts, one partial per workgroup; k_min_final reduces the partials.
 __global__ void __launch_bounds__(256)
 k_newdt(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
         const double *__restrict__ b1, const double *__restrict__ b2,
         const double *__restrict__ b3, const BlockCoef *__restrict__ coef, double gamma,
-        unsigned long long *out) {
+        double *__restrict__ part) {
+  __shared__ double sh[4];
   PlaneIdx p;
   double m = DBL_MAX;
   if (plane_index(g.is, g.ie, g.js, g.je, g.ks, p)) {
@@ -698,15 +1302,28 @@ k_newdt(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
     m = dmin(m, dt2);
     m = dmin(m, dt3);
   }
-  // wavefront min (64 lanes), then one atomic per wavefront
-  for (int off = 32; off > 0; off >>= 1) {
-    double o = __shfl_xor(m, off, 64);
-    m = dmin(m, o);
+  for (int off = 32; off > 0; off >>= 1) m = dmin(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = dmin(dmin(sh[0], sh[1]), dmin(sh[2], sh[3]));
+    part[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
   }
-  if ((threadIdx.x & 63) == 0) atomicMin(out, (unsigned long long)__double_as_longlong(m));
 }
 
-__global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+__global__ void __launch_bounds__(256) k_min_final(const double *__restrict__ part, int64_t n,
+                                                   double *__restrict__ out) {
+  __shared__ double sh[256];
+  double m = DBL_MAX;
+  for (int64_t q = threadIdx.x; q < n; q += 256) m = dmin(m, part[q]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = dmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
 
 // ================================================================ host side
 namespace {
@@ -752,13 +1369,17 @@ int check_desc(const agx_mesh_desc *d) {
     set_error("only 3D meshes are supported (every MeshBlock dimension >= 2)"); return -4;
   }
   if (d->nghost < 2) { set_error("nghost must be >= 2"); return -5; }
-  if (d->xorder != 1 && d->xorder != 2) {
-    set_error("xorder=%d not supported (1: donor cell, 2: PLM)", d->xorder); return -6;
+  if (d->xorder < 1 || d->xorder > 3) {
+    set_error("xorder=%d not supported (1: donor cell, 2: PLM, 3: PPM)", d->xorder); return -6;
+  }
+  if (d->xorder == 3 && d->nghost < 3) {
+    set_error("xorder=3 (PPM) needs nghost >= 3 (reconstruction.cpp:90-100)"); return -13;
   }
   if (d->nghost > d->bnx1 || d->nghost > d->bnx2 || d->nghost > d->bnx3) {
     set_error("nghost larger than a MeshBlock"); return -7;
   }
-  if (d->rsolver != AGX_RSOLVER_HLLD && d->rsolver != AGX_RSOLVER_HLLE) {
+  if (d->rsolver != AGX_RSOLVER_HLLD && d->rsolver != AGX_RSOLVER_HLLE &&
+      d->rsolver != AGX_RSOLVER_ROE) {
     set_error("unknown Riemann solver %d", d->rsolver); return -8;
   }
   if (d->eos != 0) { set_error("only the adiabatic EOS is supported"); return -9; }
@@ -847,6 +1468,11 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   e = hipMalloc(&m->coef, sizeof(BlockCoef) * g.nb);
   if (e == hipSuccess) e = hipMalloc(&m->plm, sizeof(double) * m->plm_h.size());
   if (e == hipSuccess) e = hipMalloc(&m->scal, sizeof(double) * 64);
+  {
+    dim3 gr = plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb);
+    m->red_n = (int64_t)gr.x * gr.y * gr.z;
+  }
+  if (e == hipSuccess) e = hipMalloc(&m->red, sizeof(double) * m->red_n);
   if (e == hipSuccess) e = hipHostMalloc(&m->scal_host, sizeof(double) * 64, hipHostMallocDefault);
   if (e == hipSuccess)
     e = hipMemcpyAsync(m->coef, m->coef_h.data(), sizeof(BlockCoef) * g.nb, hipMemcpyHostToDevice, m->stream);
@@ -968,25 +1594,30 @@ int agx_synchronize(agx_mesh *m) {
 
 int agx_calculate_fluxes(agx_mesh *m, int order) {
   if (!m) { set_error("null mesh"); return -1; }
-  if (order != 1 && order != 2) { set_error("order %d not supported", order); return -2; }
+  if (order < 1 || order > 3) { set_error("order %d not supported", order); return -2; }
+  if (order == 3 && m->g.ng < 3) { set_error("PPM needs nghost >= 3 (reconstruction.cpp:90-100)"); return -3; }
   const Geo &g = m->g;
   double **A = m->arr.a;
   const double gamma = m->d.gamma, dt = m->dt;
-  const bool hlle_rs = m->d.rsolver == AGX_RSOLVER_HLLE;
+  const int rs = m->d.rsolver;
 #define FLUX_LAUNCH(DIR, ORD, RS, B, F, EY, EZ, WX)                                            \
   hipLaunchKernelGGL((k_flux<DIR, ORD, RS>),                                                \
                      plane_grid(g.is - (DIR != 0), g.ie + 1, g.js - (DIR != 1), g.je + 1,    \
                                 g.ks - (DIR != 2), g.ke + 1, g.nb),                          \
                      dim3(256), 0, m->stream, g, A[AGX_ARR_W], A[AGX_ARR_BCC], A[B], A[F],   \
-                     A[EY], A[EZ], A[WX], m->coef, m->plm, gamma, dt)
+                     A[EY], A[EZ], A[WX], m->coef, m->plm, gamma, dt, m->d.dfloor, m->d.pfloor)
+#define FLUX_ORD(DIR, ORD, B, F, EY, EZ, WX)                                                    \
+  if (rs == AGX_RSOLVER_HLLE) FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_HLLE, B, F, EY, EZ, WX);        \
+  else if (rs == AGX_RSOLVER_ROE) FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_ROE, B, F, EY, EZ, WX);     \
+  else FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_HLLD, B, F, EY, EZ, WX);
 #define FLUX_DIR(DIR, B, F, EY, EZ, WX)                                                         \
   timer_begin(m, AGX_K_FLUX_X1 + DIR);                                                        \
   if (order == 1) {                                                                           \
-    if (hlle_rs) FLUX_LAUNCH(DIR, 1, AGX_RSOLVER_HLLE, B, F, EY, EZ, WX);                     \
-    else FLUX_LAUNCH(DIR, 1, AGX_RSOLVER_HLLD, B, F, EY, EZ, WX);                             \
+    FLUX_ORD(DIR, 1, B, F, EY, EZ, WX)                                                        \
+  } else if (order == 2) {                                                                    \
+    FLUX_ORD(DIR, 2, B, F, EY, EZ, WX)                                                        \
   } else {                                                                                    \
-    if (hlle_rs) FLUX_LAUNCH(DIR, 2, AGX_RSOLVER_HLLE, B, F, EY, EZ, WX);                     \
-    else FLUX_LAUNCH(DIR, 2, AGX_RSOLVER_HLLD, B, F, EY, EZ, WX);                             \
+    FLUX_ORD(DIR, 3, B, F, EY, EZ, WX)                                                        \
   }                                                                                           \
   timer_end(m);                                                                               \
   LAUNCH_CHECK("k_flux");
@@ -996,6 +1627,7 @@ int agx_calculate_fluxes(agx_mesh *m, int order) {
   FLUX_DIR(1, AGX_ARR_B2F, AGX_ARR_FX2, AGX_ARR_E1X2, AGX_ARR_E3X2, AGX_ARR_WX2)
   FLUX_DIR(2, AGX_ARR_B3F, AGX_ARR_FX3, AGX_ARR_E2X3, AGX_ARR_E1X3, AGX_ARR_WX3)
 #undef FLUX_DIR
+#undef FLUX_ORD
 #undef FLUX_LAUNCH
   return 0;
 }
@@ -1093,14 +1725,12 @@ int agx_new_block_dt(agx_mesh *m, double *min_dt) {
   if (!m) { set_error("null mesh"); return -1; }
   const Geo &g = m->g;
   double **A = m->arr.a;
-  unsigned long long *slot = reinterpret_cast<unsigned long long *>(m->scal);
-  union { double d; unsigned long long u; } big;
-  big.d = DBL_MAX;
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, m->stream, slot, big.u);
+  dim3 grid = plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb);
+  int64_t nwg = (int64_t)grid.x * grid.y * grid.z;
   timer_begin(m, AGX_K_NEWDT);
-  hipLaunchKernelGGL(k_newdt, plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb), dim3(256), 0,
-                     m->stream, g, A[AGX_ARR_W], A[AGX_ARR_BCC], A[AGX_ARR_B1F], A[AGX_ARR_B2F],
-                     A[AGX_ARR_B3F], m->coef, m->d.gamma, slot);
+  hipLaunchKernelGGL(k_newdt, grid, dim3(256), 0, m->stream, g, A[AGX_ARR_W], A[AGX_ARR_BCC],
+                     A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F], m->coef, m->d.gamma, m->red);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(256), 0, m->stream, m->red, nwg, m->scal);
   timer_end(m);
   LAUNCH_CHECK("k_newdt");
   AGX_HIP(hipMemcpyAsync(m->scal_host, m->scal, sizeof(double), hipMemcpyDeviceToHost, m->stream));
@@ -1134,18 +1764,68 @@ int agx_initialize(agx_mesh *m, double tlim) {
   return agx_new_time_step(m, tlim);                    // mesh.cpp:1646-1649
 }
 
+// One VL2 stage with the fused kernels: fluxes, corner EMFs, CT into the face register,
+// interior update + C2P into the conserved register, register swap, ghost faces,
+// ghost cells, interior floors.  Same results as agx_integrate + agx_bvals_exchange +
+// agx_cons2prim (time_integrator.cpp:1460-2034).
+static int stage_fused(agx_mesh *m, int stage) {
+  const Geo &g = m->g;
+  double **A = m->arr.a;
+  int rc = agx_calculate_fluxes(m, stage == 1 ? 1 : m->d.xorder);
+  if (!rc) rc = agx_compute_corner_e(m);
+  if (rc) return rc;
+  const double wght = (stage == 1 ? 0.5 : 1.0) * m->dt;
+  const int uin = (stage == 1) ? AGX_ARR_U : AGX_ARR_U1;
+  const int bsrc = (stage == 1) ? AGX_ARR_B1F : AGX_ARR_R1F;
+  const double gm1 = m->d.gamma - 1.0, dfl = m->d.dfloor, pfl = m->d.pfloor;
+  int *flag = reinterpret_cast<int *>(m->scal + 8);
+  timer_begin(m, AGX_K_INTEGRATE);
+  {
+    int64_t n = (int64_t)(g.bx1 + 1) * (g.bx2 + 1) * (g.bx3 + 1);
+    hipLaunchKernelGGL(k_ct3, dim3(grid1d(n, 256), 3, g.nb), dim3(256), 0, m->stream, g, A[bsrc],
+                       A[bsrc + 1], A[bsrc + 2], A[AGX_ARR_R1F], A[AGX_ARR_R2F], A[AGX_ARR_R3F],
+                       A[AGX_ARR_E1], A[AGX_ARR_E2], A[AGX_ARR_E3], m->coef, wght);
+  }
+  hipLaunchKernelGGL(k_reset_int, dim3(1), dim3(1), 0, m->stream, flag);
+  hipLaunchKernelGGL(k_integrate_c2p, plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb), dim3(256), 0,
+                     m->stream, g, A[uin], A[AGX_ARR_U1], A[AGX_ARR_FX1], A[AGX_ARR_FX2], A[AGX_ARR_FX3],
+                     A[AGX_ARR_W], A[AGX_ARR_BCC], A[AGX_ARR_PHI], A[AGX_ARR_R1F], A[AGX_ARR_R2F],
+                     A[AGX_ARR_R3F], m->coef, wght, m->d.self_gravity ? 1 : 0, gm1, dfl, pfl, flag);
+  timer_end(m);
+  LAUNCH_CHECK("k_integrate_c2p");
+  std::swap(A[AGX_ARR_U], A[AGX_ARR_U1]);
+  std::swap(A[AGX_ARR_B1F], A[AGX_ARR_R1F]);
+  std::swap(A[AGX_ARR_B2F], A[AGX_ARR_R2F]);
+  std::swap(A[AGX_ARR_B3F], A[AGX_ARR_R3F]);
+  Shell f1 = make_shell(g.n1 + 1, g.n2, g.n3, g.is, g.ie + 1, g.js, g.je, g.ks, g.ke);
+  Shell f2 = make_shell(g.n1, g.n2 + 1, g.n3, g.is, g.ie, g.js, g.je + 1, g.ks, g.ke);
+  Shell f3 = make_shell(g.n1, g.n2, g.n3 + 1, g.is, g.ie, g.js, g.je, g.ks, g.ke + 1);
+  Shell cs = make_shell(g.n1, g.n2, g.n3, g.is, g.ie, g.js, g.je, g.ks, g.ke);
+  int64_t nf = std::max(f1.nk + f1.nj + f1.ni, std::max(f2.nk + f2.nj + f2.ni, f3.nk + f3.nj + f3.ni));
+  timer_begin(m, AGX_K_BVALS);
+  hipLaunchKernelGGL(k_ghost_faces, dim3(grid1d(nf, 256), 3, g.nb), dim3(256), 0, m->stream, g, f1, f2, f3,
+                     A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F]);
+  hipLaunchKernelGGL(k_ghost_cells, dim3(grid1d(cs.nk + cs.nj + cs.ni, 256), 1, g.nb), dim3(256), 0,
+                     m->stream, g, cs, A[AGX_ARR_U], A[AGX_ARR_W], A[AGX_ARR_BCC], A[AGX_ARR_B1F],
+                     A[AGX_ARR_B2F], A[AGX_ARR_B3F], gm1, dfl, pfl);
+  timer_end(m);
+  LAUNCH_CHECK("k_ghost");
+  timer_begin(m, AGX_K_C2P);
+  hipLaunchKernelGGL(k_floor_fix, plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb), dim3(256), 0,
+                     m->stream, g, A[AGX_ARR_U], A[AGX_ARR_BCC], gm1, dfl, pfl, flag);
+  timer_end(m);
+  LAUNCH_CHECK("k_floor_fix");
+  return 0;
+}
+
 int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
   if (!m) { set_error("null mesh"); return -1; }
   int32_t n = 0;
   int rc = 0;
   while (n < ncycles && m->time < tlim) {
     for (int stage = 1; stage <= 2 && !rc; ++stage) {
-      // VL2 stage 1 is donor cell, time_integrator.cpp:1466-1467
-      rc = agx_calculate_fluxes(m, stage == 1 ? 1 : m->d.xorder);
-      if (!rc) rc = agx_compute_corner_e(m);
-      if (!rc) rc = agx_integrate(m, stage);
-      if (!rc) rc = agx_bvals_exchange(m, 3);
-      if (!rc) rc = agx_cons2prim(m);
+      // VL2 stage 1 is donor cell (time_integrator.cpp:1466-1467), inside stage_fused
+      rc = stage_fused(m, stage);
       if (!rc && m->d.self_gravity) rc = mg_solve(m, stage);   // main.cpp:464-465
     }
     if (rc) break;

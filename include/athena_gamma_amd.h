@@ -58,7 +58,7 @@ typedef struct agx_mesh_desc {
   double mg_threshold;          /* <gravity>/threshold (absolute L2 defect norm) */
 } agx_mesh_desc;
 
-enum { AGX_RSOLVER_HLLD = 0, AGX_RSOLVER_HLLE = 1 };
+enum { AGX_RSOLVER_HLLD = 0, AGX_RSOLVER_HLLE = 1, AGX_RSOLVER_ROE = 2 };
 enum { AGX_PREC_FP64 = 0, AGX_PREC_MIXED = 1 };
 
 /* Registered arrays (MeshBlock members in the reference). */
@@ -108,7 +108,8 @@ int agx_synchronize(agx_mesh *m);
 /* Hydro::CalculateFluxes(w, b, bcc, order), hydro/calculate_fluxes.cpp:36-378, with
  * Reconstruction::{DonorCell,PiecewiseLinear,PiecewiseParabolic}X1..3
  * (reconstruct/dc.cpp:24-104, plm.cpp:27-351, ppm.cpp:44-923), Hydro::RiemannSolver
- * (hydro/rsolvers/mhd/hlld.cpp:38-382, hlle_mhd.cpp:25-182) and Hydro::GetWeightForCT
+ * (hydro/rsolvers/mhd/hlld.cpp:38-382, hlle_mhd.cpp:25-182, roe_mhd.cpp:43-498) and
+ * Hydro::GetWeightForCT
  * (hydro/hydro.cpp:154-158). */
 int agx_calculate_fluxes(agx_mesh *m, int order);
 /* Field::ComputeCornerE(w, bcc), field/calculate_corner_e.cpp:28-236 (3D). */

@@ -722,11 +722,322 @@ static void hlle(double gamma, const double *wli, const double *wri, double bxi,
     flxi[n] = 0.5 * (fl[n] + fr[n]) + (fl[n] - fr[n]) * tmp;
 }
 
+/* Roe solver for adiabatic MHD, hydro/rsolvers/mhd/roe_mhd.cpp:43-214 with RoeFlux
+ * (:245-498); falls back to the LLF flux for the hydro variables when an intermediate
+ * density is negative (:190-201). */
+static void roe(double gamma, const double *wli, const double *wri, double bxi, double *flxi) {
+  double gm1 = gamma - 1.0;
+  double wroe[NWAVE], fl[NWAVE], fr[NWAVE], du[NWAVE], ev[NWAVE];
+  double sqrtdl = sqrt(wli[IDN]);
+  double sqrtdr = sqrt(wri[IDN]);
+  double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+  wroe[IDN] = sqrtdl * sqrtdr;
+  wroe[IVX] = (sqrtdl * wli[IVX] + sqrtdr * wri[IVX]) * isdlpdr;
+  wroe[IVY] = (sqrtdl * wli[IVY] + sqrtdr * wri[IVY]) * isdlpdr;
+  wroe[IVZ] = (sqrtdl * wli[IVZ] + sqrtdr * wri[IVZ]) * isdlpdr;
+  wroe[IBY] = (sqrtdr * wli[IBY] + sqrtdl * wri[IBY]) * isdlpdr;
+  wroe[IBZ] = (sqrtdr * wli[IBZ] + sqrtdl * wri[IBZ]) * isdlpdr;
+  double x = 0.5 * (SQR(wli[IBY] - wri[IBY]) + SQR(wli[IBZ] - wri[IBZ])) / (SQR(sqrtdl + sqrtdr));
+  double y = 0.5 * (wli[IDN] + wri[IDN]) / wroe[IDN];
+  double pbl = 0.5 * (bxi * bxi + SQR(wli[IBY]) + SQR(wli[IBZ]));
+  double pbr = 0.5 * (bxi * bxi + SQR(wri[IBY]) + SQR(wri[IBZ]));
+  double el = wli[IPR] / gm1 + 0.5 * wli[IDN] * (SQR(wli[IVX]) + SQR(wli[IVY]) + SQR(wli[IVZ])) + pbl;
+  double er = wri[IPR] / gm1 + 0.5 * wri[IDN] * (SQR(wri[IVX]) + SQR(wri[IVY]) + SQR(wri[IVZ])) + pbr;
+  wroe[IPR] = ((el + wli[IPR] + pbl) / sqrtdl + (er + wri[IPR] + pbr) / sqrtdr) * isdlpdr;
+  double mxl = wli[IDN] * wli[IVX];
+  double mxr = wri[IDN] * wri[IVX];
+  fl[IDN] = mxl;
+  fr[IDN] = mxr;
+  fl[IVX] = mxl * wli[IVX] + pbl - SQR(bxi);
+  fr[IVX] = mxr * wri[IVX] + pbr - SQR(bxi);
+  fl[IVY] = mxl * wli[IVY] - bxi * wli[IBY];
+  fr[IVY] = mxr * wri[IVY] - bxi * wri[IBY];
+  fl[IVZ] = mxl * wli[IVZ] - bxi * wli[IBZ];
+  fr[IVZ] = mxr * wri[IVZ] - bxi * wri[IBZ];
+  fl[IVX] += wli[IPR];
+  fr[IVX] += wri[IPR];
+  fl[IEN] = (el + wli[IPR] + pbl - bxi * bxi) * wli[IVX];
+  fr[IEN] = (er + wri[IPR] + pbr - bxi * bxi) * wri[IVX];
+  fl[IEN] -= bxi * (wli[IBY] * wli[IVY] + wli[IBZ] * wli[IVZ]);
+  fr[IEN] -= bxi * (wri[IBY] * wri[IVY] + wri[IBZ] * wri[IVZ]);
+  fl[IBY] = wli[IBY] * wli[IVX] - bxi * wli[IVY];
+  fr[IBY] = wri[IBY] * wri[IVX] - bxi * wri[IVY];
+  fl[IBZ] = wli[IBZ] * wli[IVX] - bxi * wli[IVZ];
+  fr[IBZ] = wri[IBZ] * wri[IVX] - bxi * wri[IVZ];
+  du[IDN] = wri[IDN] - wli[IDN];
+  du[IVX] = wri[IDN] * wri[IVX] - wli[IDN] * wli[IVX];
+  du[IVY] = wri[IDN] * wri[IVY] - wli[IDN] * wli[IVY];
+  du[IVZ] = wri[IDN] * wri[IVZ] - wli[IDN] * wli[IVZ];
+  du[IEN] = er - el;
+  du[IBY] = wri[IBY] - wli[IBY];
+  du[IBZ] = wri[IBZ] - wli[IBZ];
+  for (int n = 0; n < NWAVE; ++n) flxi[n] = 0.5 * (fl[n] + fr[n]);
+  int llf_flag = 0;
+  {
+    /* RoeFlux, adiabatic branch (roe_mhd.cpp:245-498) */
+    const double b1 = bxi;
+    double d = wroe[IDN], v1 = wroe[IVX], v2 = wroe[IVY], v3 = wroe[IVZ];
+    double b2 = wroe[IBY], b3 = wroe[IBZ];
+    double di = 1.0 / d;
+    double btsq = b2 * b2 + b3 * b3;
+    double vaxsq = b1 * b1 * di;
+    double vsq = v1 * v1 + v2 * v2 + v3 * v3;
+    double hp = wroe[IPR] - (vaxsq + btsq * di);
+    double bt_starsq = (gm1 - (gm1 - 1.0) * y) * btsq;
+    double twid_csq = dmax((gm1 * (hp - 0.5 * vsq) - (gm1 - 1.0) * x), 1.0e-20);
+    double ct2 = bt_starsq * di;
+    double tsum = vaxsq + ct2 + twid_csq;
+    double tdif = vaxsq + ct2 - twid_csq;
+    double cf2_cs2 = sqrt(tdif * tdif + 4.0 * twid_csq * ct2);
+    double cfsq = 0.5 * (tsum + cf2_cs2);
+    double cf = sqrt(cfsq);
+    double cssq = twid_csq * vaxsq / cfsq;
+    double cs = sqrt(cssq);
+    double bt = sqrt(btsq);
+    double bt_star = sqrt(bt_starsq);
+    double bet2 = 0.0, bet3 = 0.0;
+    if (bt != 0.0) { bet2 = b2 / bt; bet3 = b3 / bt; }
+    double bet2_star = bet2 / sqrt(gm1 - (gm1 - 1.0) * y);
+    double bet3_star = bet3 / sqrt(gm1 - (gm1 - 1.0) * y);
+    double bet_starsq = bet2_star * bet2_star + bet3_star * bet3_star;
+    double vbet = v2 * bet2_star + v3 * bet3_star;
+    double q2_star = 0.0, q3_star = 0.0;
+    if (bet_starsq != 0.0) { q2_star = bet2_star / bet_starsq; q3_star = bet3_star / bet_starsq; }
+    double alpha_f, alpha_s;
+    if ((cfsq - cssq) <= 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
+    else if ((twid_csq - cssq) <= 0.0) { alpha_f = 0.0; alpha_s = 1.0; }
+    else if ((cfsq - twid_csq) <= 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
+    else {
+      alpha_f = sqrt((twid_csq - cssq) / (cfsq - cssq));
+      alpha_s = sqrt((cfsq - twid_csq) / (cfsq - cssq));
+    }
+    double sqrtd = sqrt(d);
+    double isqrtd = 1.0 / sqrtd;
+    double s = (b1 < 0.0) ? -1.0 : 1.0;
+    double twid_c = sqrt(twid_csq);
+    double qf = cf * alpha_f * s;
+    double qs = cs * alpha_s * s;
+    double af_prime = twid_c * alpha_f * isqrtd;
+    double as_prime = twid_c * alpha_s * isqrtd;
+    double afpbb = af_prime * bt_star * bet_starsq;
+    double aspbb = as_prime * bt_star * bet_starsq;
+    double vqstr = (v2 * q2_star + v3 * q3_star);
+    double vax = sqrt(vaxsq);
+    double norm = 0.5 / twid_csq;
+    double cff = norm * alpha_f * cf;
+    double css = norm * alpha_s * cs;
+    double qf_hat = qf * norm;
+    double qs_hat = qs * norm;
+    double af = norm * af_prime * d;
+    double as = norm * as_prime * d;
+    double afpb = norm * af_prime * bt_star;
+    double aspb = norm * as_prime * bt_star;
+    ev[0] = v1 - cf; ev[1] = v1 - vax; ev[2] = v1 - cs; ev[3] = v1;
+    ev[4] = v1 + cs; ev[5] = v1 + vax; ev[6] = v1 + cf;
+    double a[NWAVE];
+    double alpha_f_bar = alpha_f * gm1 * norm;
+    double alpha_s_bar = alpha_s * gm1 * norm;
+    double gm1a = gm1 / twid_csq;
+    a[0] = du[0] * (alpha_f_bar * (vsq - hp) + cff * (cf + v1) - qs_hat * vqstr - aspb);
+    a[0] -= du[1] * (alpha_f_bar * v1 + cff);
+    a[0] -= du[2] * (alpha_f_bar * v2 - qs_hat * q2_star);
+    a[0] -= du[3] * (alpha_f_bar * v3 - qs_hat * q3_star);
+    a[0] += du[4] * alpha_f_bar;
+    a[0] += du[5] * (as * q2_star - alpha_f_bar * b2);
+    a[0] += du[6] * (as * q3_star - alpha_f_bar * b3);
+    a[1] = du[0] * (v2 * bet3 - v3 * bet2);
+    a[1] -= du[2] * bet3;
+    a[1] += du[3] * bet2;
+    a[1] -= du[5] * sqrtd * bet3 * s;
+    a[1] += du[6] * sqrtd * bet2 * s;
+    a[1] *= 0.5;
+    a[2] = du[0] * (alpha_s_bar * (vsq - hp) + css * (cs + v1) + qf_hat * vqstr + afpb);
+    a[2] -= du[1] * (alpha_s_bar * v1 + css);
+    a[2] -= du[2] * (alpha_s_bar * v2 + qf_hat * q2_star);
+    a[2] -= du[3] * (alpha_s_bar * v3 + qf_hat * q3_star);
+    a[2] += du[4] * alpha_s_bar;
+    a[2] -= du[5] * (af * q2_star + alpha_s_bar * b2);
+    a[2] -= du[6] * (af * q3_star + alpha_s_bar * b3);
+    a[3] = du[0] * (1.0 - gm1a * (0.5 * vsq - (gm1 - 1.0) * x / gm1));
+    a[3] += du[1] * gm1a * v1;
+    a[3] += du[2] * gm1a * v2;
+    a[3] += du[3] * gm1a * v3;
+    a[3] -= du[4] * gm1a;
+    a[3] += du[5] * gm1a * b2;
+    a[3] += du[6] * gm1a * b3;
+    a[4] = du[0] * (alpha_s_bar * (vsq - hp) + css * (cs - v1) - qf_hat * vqstr + afpb);
+    a[4] -= du[1] * (alpha_s_bar * v1 - css);
+    a[4] -= du[2] * (alpha_s_bar * v2 - qf_hat * q2_star);
+    a[4] -= du[3] * (alpha_s_bar * v3 - qf_hat * q3_star);
+    a[4] += du[4] * alpha_s_bar;
+    a[4] -= du[5] * (af * q2_star + alpha_s_bar * b2);
+    a[4] -= du[6] * (af * q3_star + alpha_s_bar * b3);
+    a[5] = du[0] * (v3 * bet2 - v2 * bet3);
+    a[5] += du[2] * bet3;
+    a[5] -= du[3] * bet2;
+    a[5] -= du[5] * sqrtd * bet3 * s;
+    a[5] += du[6] * sqrtd * bet2 * s;
+    a[5] *= 0.5;
+    a[6] = du[0] * (alpha_f_bar * (vsq - hp) + cff * (cf - v1) + qs_hat * vqstr - aspb);
+    a[6] -= du[1] * (alpha_f_bar * v1 - cff);
+    a[6] -= du[2] * (alpha_f_bar * v2 + qs_hat * q2_star);
+    a[6] -= du[3] * (alpha_f_bar * v3 + qs_hat * q3_star);
+    a[6] += du[4] * alpha_f_bar;
+    a[6] += du[5] * (as * q2_star - alpha_f_bar * b2);
+    a[6] += du[6] * (as * q3_star - alpha_f_bar * b3);
+    double coeff[NWAVE];
+    for (int n = 0; n < NWAVE; ++n) coeff[n] = -0.5 * fabs(ev[n]) * a[n];
+    double dens = wli[IDN] + a[0] * alpha_f;
+    if (dens < 0.0) llf_flag = 1;
+    dens += a[2] * alpha_s;
+    if (dens < 0.0) llf_flag = 1;
+    dens += a[3];
+    if (dens < 0.0) llf_flag = 1;
+    dens += a[4] * alpha_s;
+    if (dens < 0.0) llf_flag = 1;
+    flxi[0] += coeff[0] * alpha_f;
+    flxi[0] += coeff[2] * alpha_s;
+    flxi[0] += coeff[3];
+    flxi[0] += coeff[4] * alpha_s;
+    flxi[0] += coeff[6] * alpha_f;
+    flxi[1] += coeff[0] * (alpha_f * (v1 - cf));
+    flxi[1] += coeff[2] * (alpha_s * (v1 - cs));
+    flxi[1] += coeff[3] * v1;
+    flxi[1] += coeff[4] * (alpha_s * (v1 + cs));
+    flxi[1] += coeff[6] * (alpha_f * (v1 + cf));
+    flxi[2] += coeff[0] * (alpha_f * v2 + qs * bet2_star);
+    flxi[2] -= coeff[1] * bet3;
+    flxi[2] += coeff[2] * (alpha_s * v2 - qf * bet2_star);
+    flxi[2] += coeff[3] * v2;
+    flxi[2] += coeff[4] * (alpha_s * v2 + qf * bet2_star);
+    flxi[2] += coeff[5] * bet3;
+    flxi[2] += coeff[6] * (alpha_f * v2 - qs * bet2_star);
+    flxi[3] += coeff[0] * (alpha_f * v3 + qs * bet3_star);
+    flxi[3] += coeff[1] * bet2;
+    flxi[3] += coeff[2] * (alpha_s * v3 - qf * bet3_star);
+    flxi[3] += coeff[3] * v3;
+    flxi[3] += coeff[4] * (alpha_s * v3 + qf * bet3_star);
+    flxi[3] -= coeff[5] * bet2;
+    flxi[3] += coeff[6] * (alpha_f * v3 - qs * bet3_star);
+    flxi[4] += coeff[0] * (alpha_f * (hp - v1 * cf) + qs * vbet + aspbb);
+    flxi[4] -= coeff[1] * (v2 * bet3 - v3 * bet2);
+    flxi[4] += coeff[2] * (alpha_s * (hp - v1 * cs) - qf * vbet - afpbb);
+    flxi[4] += coeff[3] * (0.5 * vsq + (gm1 - 1.0) * x / gm1);
+    flxi[4] += coeff[4] * (alpha_s * (hp + v1 * cs) + qf * vbet - afpbb);
+    flxi[4] += coeff[5] * (v1 * bet3 - v3 * bet2);
+    flxi[4] += coeff[6] * (alpha_f * (hp + v1 * cf) - qs * vbet + aspbb);
+    flxi[5] += coeff[0] * as_prime * bet2_star;
+    flxi[5] -= coeff[1] * bet3 * s * isqrtd;
+    flxi[5] -= coeff[2] * af_prime * bet2_star;
+    flxi[5] -= coeff[4] * af_prime * bet2_star;
+    flxi[5] -= coeff[5] * bet3 * s * isqrtd;
+    flxi[5] += coeff[6] * as_prime * bet2_star;
+    flxi[6] += coeff[0] * as_prime * bet3_star;
+    flxi[6] += coeff[1] * bet2 * s * isqrtd;
+    flxi[6] -= coeff[2] * af_prime * bet3_star;
+    flxi[6] -= coeff[4] * af_prime * bet3_star;
+    flxi[6] += coeff[5] * bet2 * s * isqrtd;
+    flxi[6] += coeff[6] * as_prime * bet3_star;
+  }
+  if (ev[0] >= 0.0) {
+    for (int n = 0; n < NWAVE; ++n) flxi[n] = fl[n];
+  }
+  if (ev[NWAVE - 1] <= 0.0) {
+    for (int n = 0; n < NWAVE; ++n) flxi[n] = fr[n];
+  }
+  if (llf_flag != 0) {
+    double cfl = fast_speed(gamma, wli, bxi);
+    double cfr = fast_speed(gamma, wri, bxi);
+    double a = 0.5 * dmax((fabs(wli[IVX]) + cfl), (fabs(wri[IVX]) + cfr));
+    flxi[IDN] = 0.5 * (fl[IDN] + fr[IDN]) - a * du[IDN];
+    flxi[IVX] = 0.5 * (fl[IVX] + fr[IVX]) - a * du[IVX];
+    flxi[IVY] = 0.5 * (fl[IVY] + fr[IVY]) - a * du[IVY];
+    flxi[IVZ] = 0.5 * (fl[IVZ] + fr[IVZ]) - a * du[IVZ];
+    flxi[IEN] = 0.5 * (fl[IEN] + fr[IEN]) - a * du[IEN];
+  }
+}
+
 /* ---------------------------------------------------------------- fluxes */
 /* Hydro::CalculateFluxes, hydro/calculate_fluxes.cpp:36-378, orders 1 (dc.cpp:24-104)
  * and 2 (plm.cpp:27-351, uniform Cartesian van Leer branch).  Interface states are
  * evaluated per face directly; the pencil scratch arrays of the reference hold the
  * same values. */
+/* SIGN, defs.hpp.in:117 */
+static inline double sgn(double x) { return (x < 0.0) ? -1.0 : 1.0; }
+
+/* PiecewiseParabolicX1..3 for one variable of one cell, uniform Cartesian mesh
+ * (reconstruct/ppm.cpp:58-338; coefficients c1..c4 = 1/2, c5 = 1/6, c6 = -1/6 from
+ * reconstruct/reconstruction.cpp:429-438, hplus/hminus ratios = 2).  q[0..4] are the
+ * values at i-2..i+2; returns qminus (state at i-1/2) and qplus (state at i+1/2). */
+static void ppm_cell(const double *q, double *qminus_out, double *qplus_out) {
+  const double C2 = 1.25;
+  const double c1 = 0.5, c2 = 0.5, c3 = 0.5, c4 = 0.5, c5 = 1.0 / 6.0, c6 = -1.0 / 6.0;
+  double q_im2 = q[0], q_im1 = q[1], qc = q[2], q_ip1 = q[3], q_ip2 = q[4];
+  double qa = (qc - q_im1);
+  double qb = (q_ip1 - qc);
+  double dd_im1 = c1 * qa + c2 * (q_im1 - q_im2);
+  double dd = c1 * qb + c2 * qa;
+  double dd_ip1 = c1 * (q_ip2 - q_ip1) + c2 * qb;
+  double dph = (c3 * q_im1 + c4 * qc) + (c5 * dd_im1 + c6 * dd);
+  double dph_ip1 = (c3 * qc + c4 * q_ip1) + (c5 * dd + c6 * dd_ip1);
+  double d2qc_im1 = q_im2 + qc - 2.0 * q_im1;
+  double d2qc = q_im1 + q_ip1 - 2.0 * qc;
+  double d2qc_ip1 = qc + q_ip2 - 2.0 * q_ip1;
+  {
+    double qa_tmp = dph - q_im1;
+    double qb_tmp = qc - dph;
+    double a = 3.0 * (q_im1 + qc - 2.0 * dph);
+    double b = d2qc_im1, cc = d2qc;
+    double qd = 0.0;
+    if (sgn(a) == sgn(b) && sgn(a) == sgn(cc))
+      qd = sgn(a) * dmin(C2 * fabs(b), dmin(C2 * fabs(cc), fabs(a)));
+    double dph_tmp = 0.5 * (q_im1 + qc) - qd / 6.0;
+    if (qa_tmp * qb_tmp < 0.0) dph = dph_tmp;
+  }
+  {
+    double qa_tmp = dph_ip1 - qc;
+    double qb_tmp = q_ip1 - dph_ip1;
+    double a = 3.0 * (qc + q_ip1 - 2.0 * dph_ip1);
+    double b = d2qc, cc = d2qc_ip1;
+    double qd = 0.0;
+    if (sgn(a) == sgn(b) && sgn(a) == sgn(cc))
+      qd = sgn(a) * dmin(C2 * fabs(b), dmin(C2 * fabs(cc), fabs(a)));
+    double dphip1_tmp = 0.5 * (qc + q_ip1) - qd / 6.0;
+    if (qa_tmp * qb_tmp < 0.0) dph_ip1 = dphip1_tmp;
+  }
+  double d2qf = 6.0 * (dph + dph_ip1 - 2.0 * qc);
+  double qminus = dph, qplus = dph_ip1;
+  double dqf_minus = qc - qminus;
+  double dqf_plus = qplus - qc;
+  {
+    double qa_tmp = dqf_minus * dqf_plus;
+    double qb_tmp = (q_ip1 - qc) * (qc - q_im1);
+    double a = d2qc_im1, b = d2qc, cc = d2qc_ip1, qd = d2qf;
+    double qe = 0.0;
+    if (sgn(a) == sgn(b) && sgn(a) == sgn(cc) && sgn(a) == sgn(qd))
+      qe = sgn(qd) * dmin(dmin(C2 * fabs(a), C2 * fabs(b)), dmin(C2 * fabs(cc), fabs(qd)));
+    a = dmax(fabs(q_im1), fabs(q_im2));
+    b = dmax(dmax(fabs(qc), fabs(q_ip1)), fabs(q_ip2));
+    double rho = 0.0;
+    if (fabs(qd) > (1.0e-12) * dmax(a, b)) rho = qe / qd;
+    double tmp_m = qc - rho * dqf_minus;
+    double tmp_p = qc + rho * dqf_plus;
+    double tmp2_m = qc - 2.0 * dqf_plus;
+    double tmp2_p = qc + 2.0 * dqf_minus;
+    if ((qa_tmp <= 0.0 || qb_tmp <= 0.0)) {
+      if (rho <= (1.0 - (1.0e-12))) {
+        qminus = tmp_m;
+        qplus = tmp_p;
+      }
+    } else {
+      if (fabs(dqf_minus) >= 2.0 * fabs(dqf_plus)) qminus = tmp2_m;
+      if (fabs(dqf_plus) >= 2.0 * fabs(dqf_minus)) qplus = tmp2_p;
+    }
+  }
+  *qminus_out = qminus;
+  *qplus_out = qplus;
+}
+
 static inline void recon_cell(const orc_ctx *c, const double *w, const double *bcc,
                               int dir, int order, int g, int k, int j, int i,
                               double *wm, double *wp) {
@@ -739,6 +1050,22 @@ static inline void recon_cell(const orc_ctx *c, const double *w, const double *b
    * (IBY,IBZ) = the two transverse bcc components (plm.cpp:46-56,160-170,273-283) */
   int tb1 = (dir == 0) ? IB2 : (dir == 1 ? IB3 : IB1);
   int tb2 = (dir == 0) ? IB3 : (dir == 1 ? IB1 : IB2);
+  if (order == 3) {
+    for (int n = 0; n < NWAVE; ++n) {
+      const double *a = (n < NHYDRO) ? w + n * nc : bcc + (n == IBY ? tb1 : tb2) * nc;
+      double q5[5];
+      for (int t = 0; t < 5; ++t) q5[t] = a[id + (t - 2) * st];
+      ppm_cell(q5, &wm[n], &wp[n]);
+    }
+    /* ApplyPrimitiveFloors on both reconstructed states (ppm.cpp:333-337,
+     * eos/adiabatic_mhd.cpp:162-172) */
+    double dfl = c->d.dfloor, pfl = c->d.pfloor;
+    wm[IDN] = (wm[IDN] > dfl) ? wm[IDN] : dfl;
+    wm[IPR] = (wm[IPR] > pfl) ? wm[IPR] : pfl;
+    wp[IDN] = (wp[IDN] > dfl) ? wp[IDN] : dfl;
+    wp[IPR] = (wp[IPR] > pfl) ? wp[IPR] : pfl;
+    return;
+  }
   double q[NWAVE][3];
   for (int n = 0; n < NHYDRO; ++n) {
     q[n][0] = w[n * nc + id - st]; q[n][1] = w[n * nc + id]; q[n][2] = w[n * nc + id + st];
@@ -825,6 +1152,7 @@ void orc_calculate_fluxes(orc_ctx *c, int order) {
             int64_t fid = (dir == 0) ? F1IDX(c, k, j, i) : (dir == 1 ? F2IDX(c, k, j, i) : F3IDX(c, k, j, i));
             double bxi = bn[fid];
             if (rs == 1) hlle(gamma, wli, wri, bxi, flxi);
+            else if (rs == 2) roe(gamma, wli, wri, bxi, flxi);
             else hlld(gamma, wli, wri, bxi, flxi);
             flx[IDN * nf + fid] = flxi[IDN];
             flx[ivx * nf + fid] = flxi[IVX];
@@ -1937,6 +2265,7 @@ void orc_conserved_totals(orc_ctx *c, double *out8) {
 void orc_riemann(int rs, double gamma, const double *wl, const double *wr, double bx,
                  double *flx) {
   if (rs == 1) hlle(gamma, wl, wr, bx, flx);
+  else if (rs == 2) roe(gamma, wl, wr, bx, flx);
   else hlld(gamma, wl, wr, bx, flx);
 }
 
@@ -1952,5 +2281,15 @@ void orc_plm_pencil(int n, const double *q, const double *dxp, const double *dxm
     if (dw2 <= 0.0) dwm = 0.0;
     wl[i + 1] = q[i] + dxp[i] * dwm;
     wr[i] = q[i] - dxm[i] * dwm;
+  }
+}
+
+/* PPM on a 1-D pencil: wl[i+1] (upper face of cell i), wr[i], i = 2..n-3. */
+void orc_ppm_pencil(int n, const double *q, double *wl, double *wr) {
+  for (int i = 2; i < n - 2; ++i) {
+    double m, p;
+    ppm_cell(q + i - 2, &m, &p);
+    wl[i + 1] = p;
+    wr[i] = m;
   }
 }

@@ -89,6 +89,7 @@ void orc_riemann(int rs, double gamma, const double *wl, const double *wr, doubl
                  double *flx);
 void orc_plm_pencil(int n, const double *q, const double *dxp, const double *dxm,
                     double *wl, double *wr);
+void orc_ppm_pencil(int n, const double *q, double *wl, double *wr);
 void orc_conserved_totals(orc_ctx *c, double *out8);
 
 #ifdef __cplusplus

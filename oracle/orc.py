@@ -83,6 +83,7 @@ def _lib():
     lib.orc_conserved_totals.argtypes = [P, D]
     lib.orc_riemann.argtypes = [ctypes.c_int, ctypes.c_double, D, D, ctypes.c_double, D]
     lib.orc_plm_pencil.argtypes = [ctypes.c_int, D, D, D, D, D]
+    lib.orc_ppm_pencil.argtypes = [ctypes.c_int, D, D, D]
     return lib
 
 
@@ -229,4 +230,13 @@ def plm_pencil(q, dxp=None, dxm=None):
     wl = np.full(n, np.nan)
     wr = np.full(n, np.nan)
     lib().orc_plm_pencil(n, _dp(q), _dp(dxp), _dp(dxm), _dp(wl), _dp(wr))
+    return wl, wr
+
+
+def ppm_pencil(q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n = q.size
+    wl = np.full(n, np.nan)
+    wr = np.full(n, np.nan)
+    lib().orc_ppm_pencil(n, _dp(q), _dp(wl), _dp(wr))
     return wl, wr

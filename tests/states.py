@@ -35,7 +35,8 @@ def scatter_faces(desc, b1, b2, b3):
     return o1, o2, o3
 
 
-def turbulent_state(desc, seed=0, mach=1.0, b0=(0.1, 0.0, 0.0), db=0.05, drho=0.2, gamma=5 / 3):
+def turbulent_state(desc, seed=0, mach=1.0, b0=(0.1, 0.0, 0.0), db=0.05, drho=0.2, gamma=5 / 3,
+                    p0=None, vamp=None):
     """Smooth random MHD state: a few random Fourier modes in v, rho, and a vector
     potential; returns block arrays (u, b1f, b2f, b3f)."""
     from tests.conftest import load_agx
@@ -58,9 +59,9 @@ def turbulent_state(desc, seed=0, mach=1.0, b0=(0.1, 0.0, 0.0), db=0.05, drho=0.
         return f
 
     rho = 1.0 + drho * np.tanh(field(1.0))
-    p = 0.6 / gamma * np.ones_like(rho)
+    p = (0.6 / gamma if p0 is None else p0) * np.ones_like(rho)
     cs = np.sqrt(gamma * p / rho)
-    v = [mach * cs * field(0.5) for _ in range(3)]
+    v = [(mach * cs if vamp is None else vamp) * field(0.5) for _ in range(3)]
     # edge-centred vector potential (sample at cell centres shifted; any smooth field works)
     A1, A2, A3 = (db * field(1.0) / (2 * np.pi) for _ in range(3))
     b1, b2, b3 = face_field_from_potential(desc, A1, A2, A3, dx)

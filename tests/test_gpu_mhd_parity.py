@@ -63,7 +63,7 @@ def test_initialize(agx, case):
 
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("order", [1, 2])
-@pytest.mark.parametrize("rsolver", [0, 1])
+@pytest.mark.parametrize("rsolver", [0, 1, 2])
 def test_fluxes_and_emf(agx, case, order, rsolver):
     o, m = build_case(agx, case, rsolver)
     o.initialize()
@@ -109,7 +109,7 @@ def test_stage_updates(agx, case):
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("rsolver", [0, 1])
+@pytest.mark.parametrize("rsolver", [0, 1, 2])
 def test_cycles(agx, case, rsolver):
     o, m = build_case(agx, case, rsolver)
     o.initialize()
@@ -157,3 +157,41 @@ def test_divb_preserved(agx):
     for n in ("b1f", "b2f", "b3f"):
         o.set(n, m.get(n))
     assert o.max_divb() < 1e-12
+
+
+def test_floors_path(agx):
+    """Near-zero pressure: density/pressure floors fire (eos/adiabatic_mhd.cpp:66-84);
+    the fused stage must reproduce the reference's floor write-back and ghost C2P."""
+    desc = make_desc(nx=(16, 16, 16), bnx=(8, 8, 8), box=((0, 1),) * 3, cfl=0.3)
+    st = turbulent_state(desc, seed=3, p0=1e-19, vamp=0.3, drho=0.5)
+    o, m = pair(agx, desc, st)
+    o.initialize()
+    m.initialize()
+    w0 = o.get("w")
+    assert (w0[:, 4] <= desc.pfloor * 1.0000001).any()   # floors active
+    for _ in range(2):
+        o.cycle(1e9)
+    m.cycle(1e9, 2)
+    for n in ("u", "w", "bcc", "b1f"):
+        close(m.get(n), o.get(n), n)
+
+
+@pytest.mark.parametrize("rsolver", [0, 1, 2])
+def test_ppm_fluxes_and_cycles(agx, rsolver):
+    """PPM reconstruction (xorder=3, nghost=3): face fluxes, then full VL2 cycles
+    (stage 1 donor cell, stage 2 PPM)."""
+    desc = make_desc(nx=(24, 24, 24), bnx=(12, 12, 12), box=((0, 1),) * 3, cfl=0.4, xorder=3,
+                     nghost=3, rsolver=rsolver)
+    st = turbulent_state(desc, seed=17, mach=2.0)
+    o, m = pair(agx, desc, st)
+    o.initialize()
+    m.initialize()
+    o.L.orc_calculate_fluxes(o.h, 3)
+    m.calculate_fluxes(3)
+    for n in ("fx1", "fx2", "fx3", "e3x1", "e2x1", "e1x2", "e3x2", "e1x3", "e2x3", "wx1", "wx2", "wx3"):
+        close(m.get(n), o.get(n), n)
+    for _ in range(3):
+        o.cycle(1e9)
+    m.cycle(1e9, 3)
+    for n in ("u", "w", "b1f", "b2f", "b3f"):
+        close(m.get(n), o.get(n), n)

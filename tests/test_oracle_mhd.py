@@ -104,3 +104,29 @@ def test_uniform_state_stays_uniform(rs, order):
     ng = 2
     sl = (slice(None), slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
     np.testing.assert_allclose(u1[sl], u0[sl], rtol=1e-13)
+
+
+def test_ppm_linear_wave_more_accurate_than_plm():
+    """PPM (xorder=3, nghost=3) converges and is more accurate than PLM on the fast
+    wave (the reference selects PPM with <time>/xorder=3, reconstruction.cpp:55-92)."""
+    errs = {}
+    for xo, ng in ((2, 2), (3, 3)):
+        for n in (32, 64):
+            o = Oracle(make_desc(nx=(n, n // 2, n // 2), bnx=(n // 2, n // 2, n // 2), cfl=0.3,
+                                 xorder=xo, nghost=ng))
+            o.pgen_linear_wave(0, 1e-6, 0.0)
+            o.initialize()
+            o.run(0.5)
+            errs[(xo, n)] = o.linear_wave_errors(0, 1e-6, 0.0)[8]
+    assert errs[(3, 32)] < errs[(2, 32)] and errs[(3, 64)] < errs[(2, 64)]
+    assert errs[(3, 64)] / errs[(3, 32)] <= 0.4
+
+
+def test_roe_linear_wave_converges():
+    """The reference's linear-wave test runs HLLD and Roe (mhd_linwave.py:17)."""
+    e = []
+    for n in (32, 64):
+        o, _ = run_linwave((n, n // 2, n // 2), (n // 2, n // 2, n // 2), rs=2)
+        e.append(o.linear_wave_errors(0, 1e-6, 0.0))
+    assert e[1][8] / e[0][8] <= 0.4
+    assert e[1][17] <= 8.0
