@@ -45,10 +45,20 @@ struct BlockCoef {
   double dx[3];            // dx?f (== dx?v) of the block, coordinates.cpp:127-145
 };
 
-// Registered device arrays (AGX_ARR_* numbering).
+// Internal registers after the public AGX_ARR_* set (not reachable through the C ABI):
+// the second primitive/bcc buffer and the third face-field register used by the fused
+// stage kernel, which reads one buffer while it writes the other.
+constexpr int ARR_W2 = AGX_ARR_COUNT, ARR_BCC2 = AGX_ARR_COUNT + 1;
+constexpr int ARR_Q1F = AGX_ARR_COUNT + 2, ARR_Q2F = AGX_ARR_COUNT + 3, ARR_Q3F = AGX_ARR_COUNT + 4;
+constexpr int ARR_TOTAL = AGX_ARR_COUNT + 5;
+
+// Registered device arrays (AGX_ARR_* numbering, then the internal registers).  In the
+// mixed-precision form the primitive, bcc, flux, face-EMF and CT-weight arrays hold FP32
+// values (esz = 4) behind the same pointer; everything else is FP64.
 struct Arrays {
-  double *a[AGX_ARR_COUNT];
-  int nvar[AGX_ARR_COUNT];
+  double *a[ARR_TOTAL];
+  int nvar[ARR_TOTAL];
+  int esz[ARR_TOTAL];
 };
 
 // Multigrid level data: one periodic grid per level covering the whole mesh.
@@ -73,6 +83,7 @@ struct agx_mesh {
   agx_mesh_desc d;
   agx::Geo g;
   agx::Arrays arr;
+  bool mixed = false;                // AGX_PREC_MIXED
   hipStream_t stream = nullptr;
   double *pool = nullptr;            // all MHD arrays
   int64_t pool_bytes = 0;
@@ -83,6 +94,8 @@ struct agx_mesh {
   double *scal_host = nullptr;       // pinned host mirror
   double *red = nullptr;             // reduction partials
   int64_t red_n = 0;
+  double *dtpart = nullptr;          // per-workgroup time-step partials of the fused stage
+  int64_t dtpart_n = 0;
   std::vector<agx::BlockCoef> coef_h;
   std::vector<double> plm_h;
   double time = 0.0, dt = 0.0;

@@ -72,8 +72,14 @@ int timer_flush(agx_mesh *m) {
 }  // namespace agx
 
 // ================================================================ device helpers
-__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
-__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
+template <typename T>
+__device__ __forceinline__ T dmin(T a, T b) { return (b < a) ? b : a; }  // std::min
+template <typename T>
+__device__ __forceinline__ T dmax(T a, T b) { return (a < b) ? b : a; }  // std::max
+__device__ __forceinline__ double xsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float xsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double xabs(double x) { return fabs(x); }
+__device__ __forceinline__ float xabs(float x) { return fabsf(x); }
 #define SQR(x) ((x) * (x))
 
 struct PlaneIdx {
@@ -97,41 +103,56 @@ static dim3 plane_grid(int il, int iu, int jl, int ju, int kl, int ku, int nb) {
   return dim3(grid1d(n, 256), (unsigned)(ku - kl + 1), (unsigned)nb);
 }
 
+// Stage-kernel tiles: STAGE_TI x STAGE_TJ (i,j) columns per workgroup; grid.x = tiles of
+// one MeshBlock, grid.y = MeshBlock.
+constexpr int STAGE_TI = 16, STAGE_TJ = 16;
+static dim3 stage_grid(const Geo &g) {
+  int nti = (g.bx1 + STAGE_TI - 1) / STAGE_TI, ntj = (g.bx2 + STAGE_TJ - 1) / STAGE_TJ;
+  return dim3((unsigned)(nti * ntj), (unsigned)g.nb);
+}
+
+template <typename X>
+static inline X *as(double *p) { return reinterpret_cast<X *>(p); }
+
 // EquationOfState::FastMagnetosonicSpeed, eos/adiabatic_mhd.cpp:149-156
-__device__ __forceinline__ double fast_speed(double gamma, const double *prim, double bx) {
-  double asq = gamma * prim[IPR];
-  double vaxsq = bx * bx;
-  double ct2 = (prim[IBY] * prim[IBY] + prim[IBZ] * prim[IBZ]);
-  double qsq = vaxsq + ct2 + asq;
-  double tmp = vaxsq + ct2 - asq;
-  return sqrt(0.5 * (qsq + sqrt(tmp * tmp + 4.0 * asq * ct2)) / prim[IDN]);
+template <typename T>
+__device__ __forceinline__ T fast_speed(T gamma, const T *prim, T bx) {
+  T asq = gamma * prim[IPR];
+  T vaxsq = bx * bx;
+  T ct2 = (prim[IBY] * prim[IBY] + prim[IBZ] * prim[IBZ]);
+  T qsq = vaxsq + ct2 + asq;
+  T tmp = vaxsq + ct2 - asq;
+  return xsqrt(T(0.5) * (qsq + xsqrt(tmp * tmp + T(4.0) * asq * ct2)) / prim[IDN]);
 }
 
 // Hydro::GetWeightForCT, hydro/hydro.cpp:154-158
-__device__ __forceinline__ double ct_weight(double dflx, double rhol, double rhor, double dx,
-                                            double dt) {
-  double v_over_c = (1024.0) * dt * dflx / (dx * (rhol + rhor));
-  double tmp_min = dmin(0.5, v_over_c);
-  return 0.5 + dmax(-0.5, tmp_min);
+template <typename T>
+__device__ __forceinline__ T ct_weight(T dflx, T rhol, T rhor, T dx,
+                                            T dt) {
+  T v_over_c = (T(1024.0)) * dt * dflx / (dx * (rhol + rhor));
+  T tmp_min = dmin(T(0.5), v_over_c);
+  return T(0.5) + dmax(-T(0.5), tmp_min);
 }
 
+template <typename T>
 struct Cons1D {
-  double d, mx, my, mz, e, by, bz;
+  T d, mx, my, mz, e, by, bz;
 };
 
 // HLLD (Miyoshi & Kusano 2005) for adiabatic MHD at one interface,
 // hydro/rsolvers/mhd/hlld.cpp:56-369.  SMALL_NUMBER = 1e-8 (hlld.cpp:32).
-__device__ __forceinline__ void hlld(double gamma, const double *wli, const double *wri,
-                                     double bxi, double *flxi) {
-  const double SMALL = 1.0e-8;
-  double igm1 = 1.0 / (gamma - 1.0);
-  Cons1D ul, ur, ulst, uldst, urdst, urst, fl, fr;
-  double spd[5];
-  double bxsq = bxi * bxi;
-  double pbl = 0.5 * (bxsq + (SQR(wli[IBY]) + SQR(wli[IBZ])));
-  double pbr = 0.5 * (bxsq + (SQR(wri[IBY]) + SQR(wri[IBZ])));
-  double kel = 0.5 * wli[IDN] * (SQR(wli[IVX]) + (SQR(wli[IVY]) + SQR(wli[IVZ])));
-  double ker = 0.5 * wri[IDN] * (SQR(wri[IVX]) + (SQR(wri[IVY]) + SQR(wri[IVZ])));
+template <typename T>
+__device__ __forceinline__ void hlld(T gamma, const T *wli, const T *wri,
+                                     T bxi, T *flxi) {
+  const T SMALL = T(1.0e-8);
+  T igm1 = T(1.0) / (gamma - T(1.0));
+  Cons1D<T> ul, ur, ulst, uldst, urdst, urst, fl, fr;
+  T spd[5];
+  T bxsq = bxi * bxi;
+  T pbl = T(0.5) * (bxsq + (SQR(wli[IBY]) + SQR(wli[IBZ])));
+  T pbr = T(0.5) * (bxsq + (SQR(wri[IBY]) + SQR(wri[IBZ])));
+  T kel = T(0.5) * wli[IDN] * (SQR(wli[IVX]) + (SQR(wli[IVY]) + SQR(wli[IVZ])));
+  T ker = T(0.5) * wri[IDN] * (SQR(wri[IVX]) + (SQR(wri[IVY]) + SQR(wri[IVZ])));
   ul.d = wli[IDN];
   ul.mx = wli[IVX] * ul.d;
   ul.my = wli[IVY] * ul.d;
@@ -147,13 +168,13 @@ __device__ __forceinline__ void hlld(double gamma, const double *wli, const doub
   ur.by = wri[IBY];
   ur.bz = wri[IBZ];
 
-  double cfl = fast_speed(gamma, wli, bxi);
-  double cfr = fast_speed(gamma, wri, bxi);
+  T cfl = fast_speed(gamma, wli, bxi);
+  T cfr = fast_speed(gamma, wri, bxi);
   spd[0] = dmin(wli[IVX] - cfl, wri[IVX] - cfr);
   spd[4] = dmax(wli[IVX] + cfl, wri[IVX] + cfr);
 
-  double ptl = wli[IPR] + pbl;
-  double ptr = wri[IPR] + pbr;
+  T ptl = wli[IPR] + pbl;
+  T ptr = wri[IPR] + pbr;
   fl.d = ul.mx;
   fl.mx = ul.mx * wli[IVX] + ptl - bxsq;
   fl.my = ul.my * wli[IVX] - bxi * ul.by;
@@ -169,73 +190,73 @@ __device__ __forceinline__ void hlld(double gamma, const double *wli, const doub
   fr.by = ur.by * wri[IVX] - bxi * wri[IVY];
   fr.bz = ur.bz * wri[IVX] - bxi * wri[IVZ];
 
-  double sdl = spd[0] - wli[IVX];
-  double sdr = spd[4] - wri[IVX];
+  T sdl = spd[0] - wli[IVX];
+  T sdr = spd[4] - wri[IVX];
   spd[2] = (sdr * ur.mx - sdl * ul.mx + (ptl - ptr)) / (sdr * ur.d - sdl * ul.d);
-  double sdml = spd[0] - spd[2];
-  double sdmr = spd[4] - spd[2];
-  double sdml_inv = 1.0 / sdml;
-  double sdmr_inv = 1.0 / sdmr;
+  T sdml = spd[0] - spd[2];
+  T sdmr = spd[4] - spd[2];
+  T sdml_inv = T(1.0) / sdml;
+  T sdmr_inv = T(1.0) / sdmr;
   ulst.d = ul.d * sdl * sdml_inv;
   urst.d = ur.d * sdr * sdmr_inv;
-  double ulst_d_inv = 1.0 / ulst.d;
-  double urst_d_inv = 1.0 / urst.d;
-  double sqrtdl = sqrt(ulst.d);
-  double sqrtdr = sqrt(urst.d);
-  spd[1] = spd[2] - fabs(bxi) / sqrtdl;
-  spd[3] = spd[2] + fabs(bxi) / sqrtdr;
+  T ulst_d_inv = T(1.0) / ulst.d;
+  T urst_d_inv = T(1.0) / urst.d;
+  T sqrtdl = xsqrt(ulst.d);
+  T sqrtdr = xsqrt(urst.d);
+  spd[1] = spd[2] - xabs(bxi) / sqrtdl;
+  spd[3] = spd[2] + xabs(bxi) / sqrtdr;
 
-  double ptstl = ptl + ul.d * sdl * (spd[2] - wli[IVX]);
-  double ptstr = ptr + ur.d * sdr * (spd[2] - wri[IVX]);
-  double ptst = 0.5 * (ptstr + ptstl);
+  T ptstl = ptl + ul.d * sdl * (spd[2] - wli[IVX]);
+  T ptstr = ptr + ur.d * sdr * (spd[2] - wri[IVX]);
+  T ptst = T(0.5) * (ptstr + ptstl);
 
   ulst.mx = ulst.d * spd[2];
-  if (fabs(ul.d * sdl * sdml - bxsq) < SMALL * ptst) {
+  if (xabs(ul.d * sdl * sdml - bxsq) < SMALL * ptst) {
     ulst.my = ulst.d * wli[IVY];
     ulst.mz = ulst.d * wli[IVZ];
     ulst.by = ul.by;
     ulst.bz = ul.bz;
   } else {
-    double tmp = bxi * (sdl - sdml) / (ul.d * sdl * sdml - bxsq);
+    T tmp = bxi * (sdl - sdml) / (ul.d * sdl * sdml - bxsq);
     ulst.my = ulst.d * (wli[IVY] - ul.by * tmp);
     ulst.mz = ulst.d * (wli[IVZ] - ul.bz * tmp);
     tmp = (ul.d * SQR(sdl) - bxsq) / (ul.d * sdl * sdml - bxsq);
     ulst.by = ul.by * tmp;
     ulst.bz = ul.bz * tmp;
   }
-  double vbstl = (ulst.mx * bxi + (ulst.my * ulst.by + ulst.mz * ulst.bz)) * ulst_d_inv;
+  T vbstl = (ulst.mx * bxi + (ulst.my * ulst.by + ulst.mz * ulst.bz)) * ulst_d_inv;
   ulst.e = (sdl * ul.e - ptl * wli[IVX] + ptst * spd[2] +
             bxi * (wli[IVX] * bxi + (wli[IVY] * ul.by + wli[IVZ] * ul.bz) - vbstl)) * sdml_inv;
 
   urst.mx = urst.d * spd[2];
-  if (fabs(ur.d * sdr * sdmr - bxsq) < SMALL * ptst) {
+  if (xabs(ur.d * sdr * sdmr - bxsq) < SMALL * ptst) {
     urst.my = urst.d * wri[IVY];
     urst.mz = urst.d * wri[IVZ];
     urst.by = ur.by;
     urst.bz = ur.bz;
   } else {
-    double tmp = bxi * (sdr - sdmr) / (ur.d * sdr * sdmr - bxsq);
+    T tmp = bxi * (sdr - sdmr) / (ur.d * sdr * sdmr - bxsq);
     urst.my = urst.d * (wri[IVY] - ur.by * tmp);
     urst.mz = urst.d * (wri[IVZ] - ur.bz * tmp);
     tmp = (ur.d * SQR(sdr) - bxsq) / (ur.d * sdr * sdmr - bxsq);
     urst.by = ur.by * tmp;
     urst.bz = ur.bz * tmp;
   }
-  double vbstr = (urst.mx * bxi + (urst.my * urst.by + urst.mz * urst.bz)) * urst_d_inv;
+  T vbstr = (urst.mx * bxi + (urst.my * urst.by + urst.mz * urst.bz)) * urst_d_inv;
   urst.e = (sdr * ur.e - ptr * wri[IVX] + ptst * spd[2] +
             bxi * (wri[IVX] * bxi + (wri[IVY] * ur.by + wri[IVZ] * ur.bz) - vbstr)) * sdmr_inv;
 
-  if (0.5 * bxsq < SMALL * ptst) {
+  if (T(0.5) * bxsq < SMALL * ptst) {
     uldst = ulst;
     urdst = urst;
   } else {
-    double invsumd = 1.0 / (sqrtdl + sqrtdr);
-    double bxsig = (bxi > 0.0 ? 1.0 : -1.0);
+    T invsumd = T(1.0) / (sqrtdl + sqrtdr);
+    T bxsig = (bxi > T(0.0) ? T(1.0) : -T(1.0));
     uldst.d = ulst.d;
     urdst.d = urst.d;
     uldst.mx = ulst.mx;
     urdst.mx = urst.mx;
-    double tmp = invsumd * (sqrtdl * (ulst.my * ulst_d_inv) + sqrtdr * (urst.my * urst_d_inv) +
+    T tmp = invsumd * (sqrtdl * (ulst.my * ulst_d_inv) + sqrtdr * (urst.my * urst_d_inv) +
                             bxsig * (urst.by - ulst.by));
     uldst.my = uldst.d * tmp;
     urdst.my = urdst.d * tmp;
@@ -256,13 +277,13 @@ __device__ __forceinline__ void hlld(double gamma, const double *wli, const doub
 
   // flux selection (hlld.cpp:315-369); the differences are formed only for the branch
   // taken, which rounds identically to forming all of them first.
-  if (spd[0] >= 0.0) {
+  if (spd[0] >= T(0.0)) {
     flxi[IDN] = fl.d; flxi[IVX] = fl.mx; flxi[IVY] = fl.my; flxi[IVZ] = fl.mz;
     flxi[IEN] = fl.e; flxi[IBY] = fl.by; flxi[IBZ] = fl.bz;
-  } else if (spd[4] <= 0.0) {
+  } else if (spd[4] <= T(0.0)) {
     flxi[IDN] = fr.d; flxi[IVX] = fr.mx; flxi[IVY] = fr.my; flxi[IVZ] = fr.mz;
     flxi[IEN] = fr.e; flxi[IBY] = fr.by; flxi[IBZ] = fr.bz;
-  } else if (spd[1] >= 0.0) {
+  } else if (spd[1] >= T(0.0)) {
     flxi[IDN] = fl.d + spd[0] * (ulst.d - ul.d);
     flxi[IVX] = fl.mx + spd[0] * (ulst.mx - ul.mx);
     flxi[IVY] = fl.my + spd[0] * (ulst.my - ul.my);
@@ -270,7 +291,7 @@ __device__ __forceinline__ void hlld(double gamma, const double *wli, const doub
     flxi[IEN] = fl.e + spd[0] * (ulst.e - ul.e);
     flxi[IBY] = fl.by + spd[0] * (ulst.by - ul.by);
     flxi[IBZ] = fl.bz + spd[0] * (ulst.bz - ul.bz);
-  } else if (spd[2] >= 0.0) {
+  } else if (spd[2] >= T(0.0)) {
     flxi[IDN] = fl.d + spd[0] * (ulst.d - ul.d) + spd[1] * (uldst.d - ulst.d);
     flxi[IVX] = fl.mx + spd[0] * (ulst.mx - ul.mx) + spd[1] * (uldst.mx - ulst.mx);
     flxi[IVY] = fl.my + spd[0] * (ulst.my - ul.my) + spd[1] * (uldst.my - ulst.my);
@@ -278,7 +299,7 @@ __device__ __forceinline__ void hlld(double gamma, const double *wli, const doub
     flxi[IEN] = fl.e + spd[0] * (ulst.e - ul.e) + spd[1] * (uldst.e - ulst.e);
     flxi[IBY] = fl.by + spd[0] * (ulst.by - ul.by) + spd[1] * (uldst.by - ulst.by);
     flxi[IBZ] = fl.bz + spd[0] * (ulst.bz - ul.bz) + spd[1] * (uldst.bz - ulst.bz);
-  } else if (spd[3] > 0.0) {
+  } else if (spd[3] > T(0.0)) {
     flxi[IDN] = fr.d + spd[4] * (urst.d - ur.d) + spd[3] * (urdst.d - urst.d);
     flxi[IVX] = fr.mx + spd[4] * (urst.mx - ur.mx) + spd[3] * (urdst.mx - urst.mx);
     flxi[IVY] = fr.my + spd[4] * (urst.my - ur.my) + spd[3] * (urdst.my - urst.my);
@@ -298,46 +319,47 @@ __device__ __forceinline__ void hlld(double gamma, const double *wli, const doub
 }
 
 // HLLE for adiabatic MHD at one interface, hydro/rsolvers/mhd/hlle_mhd.cpp:56-171.
-__device__ __forceinline__ void hlle(double gamma, const double *wli, const double *wri,
-                                     double bxi, double *flxi) {
-  double gm1 = gamma - 1.0;
-  double wroe[NWAVE], fl[NWAVE], fr[NWAVE];
-  double sqrtdl = sqrt(wli[IDN]);
-  double sqrtdr = sqrt(wri[IDN]);
-  double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+template <typename T>
+__device__ __forceinline__ void hlle(T gamma, const T *wli, const T *wri,
+                                     T bxi, T *flxi) {
+  T gm1 = gamma - T(1.0);
+  T wroe[NWAVE], fl[NWAVE], fr[NWAVE];
+  T sqrtdl = xsqrt(wli[IDN]);
+  T sqrtdr = xsqrt(wri[IDN]);
+  T isdlpdr = T(1.0) / (sqrtdl + sqrtdr);
   wroe[IDN] = sqrtdl * sqrtdr;
   wroe[IVX] = (sqrtdl * wli[IVX] + sqrtdr * wri[IVX]) * isdlpdr;
   wroe[IVY] = (sqrtdl * wli[IVY] + sqrtdr * wri[IVY]) * isdlpdr;
   wroe[IVZ] = (sqrtdl * wli[IVZ] + sqrtdr * wri[IVZ]) * isdlpdr;
   wroe[IBY] = (sqrtdr * wli[IBY] + sqrtdl * wri[IBY]) * isdlpdr;
   wroe[IBZ] = (sqrtdr * wli[IBZ] + sqrtdl * wri[IBZ]) * isdlpdr;
-  double x = 0.5 * (SQR(wli[IBY] - wri[IBY]) + SQR(wli[IBZ] - wri[IBZ])) / (SQR(sqrtdl + sqrtdr));
-  double y = 0.5 * (wli[IDN] + wri[IDN]) / wroe[IDN];
-  double pbl = 0.5 * (bxi * bxi + SQR(wli[IBY]) + SQR(wli[IBZ]));
-  double pbr = 0.5 * (bxi * bxi + SQR(wri[IBY]) + SQR(wri[IBZ]));
-  double el = wli[IPR] / gm1 + 0.5 * wli[IDN] * (SQR(wli[IVX]) + SQR(wli[IVY]) + SQR(wli[IVZ])) + pbl;
-  double er = wri[IPR] / gm1 + 0.5 * wri[IDN] * (SQR(wri[IVX]) + SQR(wri[IVY]) + SQR(wri[IVZ])) + pbr;
-  double hroe = ((el + wli[IPR] + pbl) / sqrtdl + (er + wri[IPR] + pbr) / sqrtdr) * isdlpdr;
-  double cl = fast_speed(gamma, wli, bxi);
-  double cr = fast_speed(gamma, wri, bxi);
-  double btsq = SQR(wroe[IBY]) + SQR(wroe[IBZ]);
-  double vaxsq = bxi * bxi / wroe[IDN];
-  double bt_starsq = (gm1 - (gm1 - 1.0) * y) * btsq;
-  double hp = hroe - (vaxsq + btsq / wroe[IDN]);
-  double vsq = SQR(wroe[IVX]) + SQR(wroe[IVY]) + SQR(wroe[IVZ]);
-  double twid_asq = dmax((gm1 * (hp - 0.5 * vsq) - (gm1 - 1.0) * x), 0.0);
-  double ct2 = bt_starsq / wroe[IDN];
-  double tsum = vaxsq + ct2 + twid_asq;
-  double tdif = vaxsq + ct2 - twid_asq;
-  double cf2_cs2 = sqrt(tdif * tdif + 4.0 * twid_asq * ct2);
-  double cfsq = 0.5 * (tsum + cf2_cs2);
-  double a = sqrt(cfsq);
-  double al = dmin((wroe[IVX] - a), (wli[IVX] - cl));
-  double ar = dmax((wroe[IVX] + a), (wri[IVX] + cr));
-  double bp = ar > 0.0 ? ar : 0.0;
-  double bm = al < 0.0 ? al : 0.0;
-  double vxl = wli[IVX] - bm;
-  double vxr = wri[IVX] - bp;
+  T x = T(0.5) * (SQR(wli[IBY] - wri[IBY]) + SQR(wli[IBZ] - wri[IBZ])) / (SQR(sqrtdl + sqrtdr));
+  T y = T(0.5) * (wli[IDN] + wri[IDN]) / wroe[IDN];
+  T pbl = T(0.5) * (bxi * bxi + SQR(wli[IBY]) + SQR(wli[IBZ]));
+  T pbr = T(0.5) * (bxi * bxi + SQR(wri[IBY]) + SQR(wri[IBZ]));
+  T el = wli[IPR] / gm1 + T(0.5) * wli[IDN] * (SQR(wli[IVX]) + SQR(wli[IVY]) + SQR(wli[IVZ])) + pbl;
+  T er = wri[IPR] / gm1 + T(0.5) * wri[IDN] * (SQR(wri[IVX]) + SQR(wri[IVY]) + SQR(wri[IVZ])) + pbr;
+  T hroe = ((el + wli[IPR] + pbl) / sqrtdl + (er + wri[IPR] + pbr) / sqrtdr) * isdlpdr;
+  T cl = fast_speed(gamma, wli, bxi);
+  T cr = fast_speed(gamma, wri, bxi);
+  T btsq = SQR(wroe[IBY]) + SQR(wroe[IBZ]);
+  T vaxsq = bxi * bxi / wroe[IDN];
+  T bt_starsq = (gm1 - (gm1 - T(1.0)) * y) * btsq;
+  T hp = hroe - (vaxsq + btsq / wroe[IDN]);
+  T vsq = SQR(wroe[IVX]) + SQR(wroe[IVY]) + SQR(wroe[IVZ]);
+  T twid_asq = dmax((gm1 * (hp - T(0.5) * vsq) - (gm1 - T(1.0)) * x), T(0.0));
+  T ct2 = bt_starsq / wroe[IDN];
+  T tsum = vaxsq + ct2 + twid_asq;
+  T tdif = vaxsq + ct2 - twid_asq;
+  T cf2_cs2 = xsqrt(tdif * tdif + T(4.0) * twid_asq * ct2);
+  T cfsq = T(0.5) * (tsum + cf2_cs2);
+  T a = xsqrt(cfsq);
+  T al = dmin((wroe[IVX] - a), (wli[IVX] - cl));
+  T ar = dmax((wroe[IVX] + a), (wri[IVX] + cr));
+  T bp = ar > T(0.0) ? ar : T(0.0);
+  T bm = al < T(0.0) ? al : T(0.0);
+  T vxl = wli[IVX] - bm;
+  T vxr = wri[IVX] - bp;
   fl[IDN] = wli[IDN] * vxl;
   fr[IDN] = wri[IDN] * vxr;
   fl[IVX] = wli[IDN] * wli[IVX] * vxl + pbl - SQR(bxi);
@@ -356,78 +378,80 @@ __device__ __forceinline__ void hlle(double gamma, const double *wli, const doub
   fr[IBY] = wri[IBY] * vxr - bxi * wri[IVY];
   fl[IBZ] = wli[IBZ] * vxl - bxi * wli[IVZ];
   fr[IBZ] = wri[IBZ] * vxr - bxi * wri[IVZ];
-  double tmp = 0.0;
-  if (bp != bm) tmp = 0.5 * (bp + bm) / (bp - bm);
+  T tmp = T(0.0);
+  if (bp != bm) tmp = T(0.5) * (bp + bm) / (bp - bm);
 #pragma unroll
-  for (int n = 0; n < NWAVE; ++n) flxi[n] = 0.5 * (fl[n] + fr[n]) + (fl[n] - fr[n]) * tmp;
+  for (int n = 0; n < NWAVE; ++n) flxi[n] = T(0.5) * (fl[n] + fr[n]) + (fl[n] - fr[n]) * tmp;
 }
 
 // SIGN, defs.hpp.in:117
-__device__ __forceinline__ double sgn(double x) { return (x < 0.0) ? -1.0 : 1.0; }
+template <typename T>
+__device__ __forceinline__ T sgn(T x) { return (x < T(0.0)) ? -T(1.0) : T(1.0); }
 
 // PiecewiseParabolicX1..3 for one variable of one cell on a uniform Cartesian mesh
 // (reconstruct/ppm.cpp:58-338): CW84 fourth-order interface values with c1..c4 = 1/2,
 // c5 = 1/6, c6 = -1/6 (reconstruction.cpp:429-438), the CS08 extremum-preserving
 // limiter with C2 = 1.25 and the 1e-12 round-off guards.  q[0..4] = values at i-2..i+2.
-__device__ __forceinline__ void ppm_cell(const double *q, double &qminus, double &qplus) {
-  const double C2 = 1.25;
-  const double c1 = 0.5, c2 = 0.5, c3 = 0.5, c4 = 0.5, c5 = 1.0 / 6.0, c6 = -1.0 / 6.0;
-  const double q_im2 = q[0], q_im1 = q[1], qc = q[2], q_ip1 = q[3], q_ip2 = q[4];
-  double qa = (qc - q_im1);
-  double qb = (q_ip1 - qc);
-  double dd_im1 = c1 * qa + c2 * (q_im1 - q_im2);
-  double dd = c1 * qb + c2 * qa;
-  double dd_ip1 = c1 * (q_ip2 - q_ip1) + c2 * qb;
-  double dph = (c3 * q_im1 + c4 * qc) + (c5 * dd_im1 + c6 * dd);
-  double dph_ip1 = (c3 * qc + c4 * q_ip1) + (c5 * dd + c6 * dd_ip1);
-  double d2qc_im1 = q_im2 + qc - 2.0 * q_im1;
-  double d2qc = q_im1 + q_ip1 - 2.0 * qc;
-  double d2qc_ip1 = qc + q_ip2 - 2.0 * q_ip1;
+template <typename T>
+__device__ __forceinline__ void ppm_cell(const T *q, T &qminus, T &qplus) {
+  const T C2 = T(1.25);
+  const T c1 = T(0.5), c2 = T(0.5), c3 = T(0.5), c4 = T(0.5), c5 = T(1.0) / T(6.0), c6 = -T(1.0) / T(6.0);
+  const T q_im2 = q[0], q_im1 = q[1], qc = q[2], q_ip1 = q[3], q_ip2 = q[4];
+  T qa = (qc - q_im1);
+  T qb = (q_ip1 - qc);
+  T dd_im1 = c1 * qa + c2 * (q_im1 - q_im2);
+  T dd = c1 * qb + c2 * qa;
+  T dd_ip1 = c1 * (q_ip2 - q_ip1) + c2 * qb;
+  T dph = (c3 * q_im1 + c4 * qc) + (c5 * dd_im1 + c6 * dd);
+  T dph_ip1 = (c3 * qc + c4 * q_ip1) + (c5 * dd + c6 * dd_ip1);
+  T d2qc_im1 = q_im2 + qc - T(2.0) * q_im1;
+  T d2qc = q_im1 + q_ip1 - T(2.0) * qc;
+  T d2qc_ip1 = qc + q_ip2 - T(2.0) * q_ip1;
   {
-    double qa_tmp = dph - q_im1;
-    double qb_tmp = qc - dph;
-    double a = 3.0 * (q_im1 + qc - 2.0 * dph);
-    double qd = 0.0;
+    T qa_tmp = dph - q_im1;
+    T qb_tmp = qc - dph;
+    T a = T(3.0) * (q_im1 + qc - T(2.0) * dph);
+    T qd = T(0.0);
     if (sgn(a) == sgn(d2qc_im1) && sgn(a) == sgn(d2qc))
-      qd = sgn(a) * dmin(C2 * fabs(d2qc_im1), dmin(C2 * fabs(d2qc), fabs(a)));
-    double dph_tmp = 0.5 * (q_im1 + qc) - qd / 6.0;
-    if (qa_tmp * qb_tmp < 0.0) dph = dph_tmp;
+      qd = sgn(a) * dmin(C2 * xabs(d2qc_im1), dmin(C2 * xabs(d2qc), xabs(a)));
+    T dph_tmp = T(0.5) * (q_im1 + qc) - qd / T(6.0);
+    if (qa_tmp * qb_tmp < T(0.0)) dph = dph_tmp;
   }
   {
-    double qa_tmp = dph_ip1 - qc;
-    double qb_tmp = q_ip1 - dph_ip1;
-    double a = 3.0 * (qc + q_ip1 - 2.0 * dph_ip1);
-    double qd = 0.0;
+    T qa_tmp = dph_ip1 - qc;
+    T qb_tmp = q_ip1 - dph_ip1;
+    T a = T(3.0) * (qc + q_ip1 - T(2.0) * dph_ip1);
+    T qd = T(0.0);
     if (sgn(a) == sgn(d2qc) && sgn(a) == sgn(d2qc_ip1))
-      qd = sgn(a) * dmin(C2 * fabs(d2qc), dmin(C2 * fabs(d2qc_ip1), fabs(a)));
-    double dphip1_tmp = 0.5 * (qc + q_ip1) - qd / 6.0;
-    if (qa_tmp * qb_tmp < 0.0) dph_ip1 = dphip1_tmp;
+      qd = sgn(a) * dmin(C2 * xabs(d2qc), dmin(C2 * xabs(d2qc_ip1), xabs(a)));
+    T dphip1_tmp = T(0.5) * (qc + q_ip1) - qd / T(6.0);
+    if (qa_tmp * qb_tmp < T(0.0)) dph_ip1 = dphip1_tmp;
   }
-  double d2qf = 6.0 * (dph + dph_ip1 - 2.0 * qc);
-  double qm = dph, qp = dph_ip1;
-  double dqf_minus = qc - qm;
-  double dqf_plus = qp - qc;
-  double qa_tmp = dqf_minus * dqf_plus;
-  double qb_tmp = (q_ip1 - qc) * (qc - q_im1);
-  double qe = 0.0;
+  T d2qf = T(6.0) * (dph + dph_ip1 - T(2.0) * qc);
+  T qm = dph, qp = dph_ip1;
+  T dqf_minus = qc - qm;
+  T dqf_plus = qp - qc;
+  T qa_tmp = dqf_minus * dqf_plus;
+  T qb_tmp = (q_ip1 - qc) * (qc - q_im1);
+  T qe = T(0.0);
   if (sgn(d2qc_im1) == sgn(d2qc) && sgn(d2qc_im1) == sgn(d2qc_ip1) && sgn(d2qc_im1) == sgn(d2qf))
-    qe = sgn(d2qf) * dmin(dmin(C2 * fabs(d2qc_im1), C2 * fabs(d2qc)), dmin(C2 * fabs(d2qc_ip1), fabs(d2qf)));
-  double ma = dmax(fabs(q_im1), fabs(q_im2));
-  double mb = dmax(dmax(fabs(qc), fabs(q_ip1)), fabs(q_ip2));
-  double rho = 0.0;
-  if (fabs(d2qf) > (1.0e-12) * dmax(ma, mb)) rho = qe / d2qf;
-  double tmp_m = qc - rho * dqf_minus;
-  double tmp_p = qc + rho * dqf_plus;
-  double tmp2_m = qc - 2.0 * dqf_plus;
-  double tmp2_p = qc + 2.0 * dqf_minus;
-  if ((qa_tmp <= 0.0 || qb_tmp <= 0.0)) {
-    if (rho <= (1.0 - (1.0e-12))) {
+    qe = sgn(d2qf) * dmin(dmin(C2 * xabs(d2qc_im1), C2 * xabs(d2qc)), dmin(C2 * xabs(d2qc_ip1), xabs(d2qf)));
+  T ma = dmax(xabs(q_im1), xabs(q_im2));
+  T mb = dmax(dmax(xabs(qc), xabs(q_ip1)), xabs(q_ip2));
+  T rho = T(0.0);
+  if (xabs(d2qf) > (T(1.0e-12)) * dmax(ma, mb)) rho = qe / d2qf;
+  T tmp_m = qc - rho * dqf_minus;
+  T tmp_p = qc + rho * dqf_plus;
+  T tmp2_m = qc - T(2.0) * dqf_plus;
+  T tmp2_p = qc + T(2.0) * dqf_minus;
+  if ((qa_tmp <= T(0.0) || qb_tmp <= T(0.0))) {
+    if (rho <= (T(1.0) - (T(1.0e-12)))) {
       qm = tmp_m;
       qp = tmp_p;
     }
   } else {
-    if (fabs(dqf_minus) >= 2.0 * fabs(dqf_plus)) qm = tmp2_m;
-    if (fabs(dqf_plus) >= 2.0 * fabs(dqf_minus)) qp = tmp2_p;
+    if (xabs(dqf_minus) >= T(2.0) * xabs(dqf_plus)) qm = tmp2_m;
+    if (xabs(dqf_plus) >= T(2.0) * xabs(dqf_minus)) qp = tmp2_p;
   }
   qminus = qm;
   qplus = qp;
@@ -436,28 +460,29 @@ __device__ __forceinline__ void ppm_cell(const double *q, double &qminus, double
 // Roe solver for adiabatic MHD at one interface, hydro/rsolvers/mhd/roe_mhd.cpp:56-212
 // with RoeFlux (:245-498): Roe-averaged eigensystem, wave-strength dissipation, LLF
 // fallback for the hydro variables when an intermediate density is negative.
-__device__ __forceinline__ void roe(double gamma, const double *wli, const double *wri,
-                                    double bxi, double *flxi) {
-  double gm1 = gamma - 1.0;
-  double wroe[NWAVE], fl[NWAVE], fr[NWAVE], du[NWAVE], ev[NWAVE];
-  double sqrtdl = sqrt(wli[IDN]);
-  double sqrtdr = sqrt(wri[IDN]);
-  double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+template <typename T>
+__device__ __forceinline__ void roe(T gamma, const T *wli, const T *wri,
+                                    T bxi, T *flxi) {
+  T gm1 = gamma - T(1.0);
+  T wroe[NWAVE], fl[NWAVE], fr[NWAVE], du[NWAVE], ev[NWAVE];
+  T sqrtdl = xsqrt(wli[IDN]);
+  T sqrtdr = xsqrt(wri[IDN]);
+  T isdlpdr = T(1.0) / (sqrtdl + sqrtdr);
   wroe[IDN] = sqrtdl * sqrtdr;
   wroe[IVX] = (sqrtdl * wli[IVX] + sqrtdr * wri[IVX]) * isdlpdr;
   wroe[IVY] = (sqrtdl * wli[IVY] + sqrtdr * wri[IVY]) * isdlpdr;
   wroe[IVZ] = (sqrtdl * wli[IVZ] + sqrtdr * wri[IVZ]) * isdlpdr;
   wroe[IBY] = (sqrtdr * wli[IBY] + sqrtdl * wri[IBY]) * isdlpdr;
   wroe[IBZ] = (sqrtdr * wli[IBZ] + sqrtdl * wri[IBZ]) * isdlpdr;
-  double x = 0.5 * (SQR(wli[IBY] - wri[IBY]) + SQR(wli[IBZ] - wri[IBZ])) / (SQR(sqrtdl + sqrtdr));
-  double y = 0.5 * (wli[IDN] + wri[IDN]) / wroe[IDN];
-  double pbl = 0.5 * (bxi * bxi + SQR(wli[IBY]) + SQR(wli[IBZ]));
-  double pbr = 0.5 * (bxi * bxi + SQR(wri[IBY]) + SQR(wri[IBZ]));
-  double el = wli[IPR] / gm1 + 0.5 * wli[IDN] * (SQR(wli[IVX]) + SQR(wli[IVY]) + SQR(wli[IVZ])) + pbl;
-  double er = wri[IPR] / gm1 + 0.5 * wri[IDN] * (SQR(wri[IVX]) + SQR(wri[IVY]) + SQR(wri[IVZ])) + pbr;
+  T x = T(0.5) * (SQR(wli[IBY] - wri[IBY]) + SQR(wli[IBZ] - wri[IBZ])) / (SQR(sqrtdl + sqrtdr));
+  T y = T(0.5) * (wli[IDN] + wri[IDN]) / wroe[IDN];
+  T pbl = T(0.5) * (bxi * bxi + SQR(wli[IBY]) + SQR(wli[IBZ]));
+  T pbr = T(0.5) * (bxi * bxi + SQR(wri[IBY]) + SQR(wri[IBZ]));
+  T el = wli[IPR] / gm1 + T(0.5) * wli[IDN] * (SQR(wli[IVX]) + SQR(wli[IVY]) + SQR(wli[IVZ])) + pbl;
+  T er = wri[IPR] / gm1 + T(0.5) * wri[IDN] * (SQR(wri[IVX]) + SQR(wri[IVY]) + SQR(wri[IVZ])) + pbr;
   wroe[IPR] = ((el + wli[IPR] + pbl) / sqrtdl + (er + wri[IPR] + pbr) / sqrtdr) * isdlpdr;
-  double mxl = wli[IDN] * wli[IVX];
-  double mxr = wri[IDN] * wri[IVX];
+  T mxl = wli[IDN] * wli[IVX];
+  T mxr = wri[IDN] * wri[IVX];
   fl[IDN] = mxl;
   fr[IDN] = mxr;
   fl[IVX] = mxl * wli[IVX] + pbl - SQR(bxi);
@@ -484,73 +509,73 @@ __device__ __forceinline__ void roe(double gamma, const double *wli, const doubl
   du[IBY] = wri[IBY] - wli[IBY];
   du[IBZ] = wri[IBZ] - wli[IBZ];
 #pragma unroll
-  for (int n = 0; n < NWAVE; ++n) flxi[n] = 0.5 * (fl[n] + fr[n]);
+  for (int n = 0; n < NWAVE; ++n) flxi[n] = T(0.5) * (fl[n] + fr[n]);
   int llf_flag = 0;
   {
     /* RoeFlux, adiabatic branch (roe_mhd.cpp:245-498) */
-    const double b1 = bxi;
-    double d = wroe[IDN], v1 = wroe[IVX], v2 = wroe[IVY], v3 = wroe[IVZ];
-    double b2 = wroe[IBY], b3 = wroe[IBZ];
-    double di = 1.0 / d;
-    double btsq = b2 * b2 + b3 * b3;
-    double vaxsq = b1 * b1 * di;
-    double vsq = v1 * v1 + v2 * v2 + v3 * v3;
-    double hp = wroe[IPR] - (vaxsq + btsq * di);
-    double bt_starsq = (gm1 - (gm1 - 1.0) * y) * btsq;
-    double twid_csq = dmax((gm1 * (hp - 0.5 * vsq) - (gm1 - 1.0) * x), 1.0e-20);
-    double ct2 = bt_starsq * di;
-    double tsum = vaxsq + ct2 + twid_csq;
-    double tdif = vaxsq + ct2 - twid_csq;
-    double cf2_cs2 = sqrt(tdif * tdif + 4.0 * twid_csq * ct2);
-    double cfsq = 0.5 * (tsum + cf2_cs2);
-    double cf = sqrt(cfsq);
-    double cssq = twid_csq * vaxsq / cfsq;
-    double cs = sqrt(cssq);
-    double bt = sqrt(btsq);
-    double bt_star = sqrt(bt_starsq);
-    double bet2 = 0.0, bet3 = 0.0;
-    if (bt != 0.0) { bet2 = b2 / bt; bet3 = b3 / bt; }
-    double bet2_star = bet2 / sqrt(gm1 - (gm1 - 1.0) * y);
-    double bet3_star = bet3 / sqrt(gm1 - (gm1 - 1.0) * y);
-    double bet_starsq = bet2_star * bet2_star + bet3_star * bet3_star;
-    double vbet = v2 * bet2_star + v3 * bet3_star;
-    double q2_star = 0.0, q3_star = 0.0;
-    if (bet_starsq != 0.0) { q2_star = bet2_star / bet_starsq; q3_star = bet3_star / bet_starsq; }
-    double alpha_f, alpha_s;
-    if ((cfsq - cssq) <= 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
-    else if ((twid_csq - cssq) <= 0.0) { alpha_f = 0.0; alpha_s = 1.0; }
-    else if ((cfsq - twid_csq) <= 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
+    const T b1 = bxi;
+    T d = wroe[IDN], v1 = wroe[IVX], v2 = wroe[IVY], v3 = wroe[IVZ];
+    T b2 = wroe[IBY], b3 = wroe[IBZ];
+    T di = T(1.0) / d;
+    T btsq = b2 * b2 + b3 * b3;
+    T vaxsq = b1 * b1 * di;
+    T vsq = v1 * v1 + v2 * v2 + v3 * v3;
+    T hp = wroe[IPR] - (vaxsq + btsq * di);
+    T bt_starsq = (gm1 - (gm1 - T(1.0)) * y) * btsq;
+    T twid_csq = dmax((gm1 * (hp - T(0.5) * vsq) - (gm1 - T(1.0)) * x), T(1.0e-20));
+    T ct2 = bt_starsq * di;
+    T tsum = vaxsq + ct2 + twid_csq;
+    T tdif = vaxsq + ct2 - twid_csq;
+    T cf2_cs2 = xsqrt(tdif * tdif + T(4.0) * twid_csq * ct2);
+    T cfsq = T(0.5) * (tsum + cf2_cs2);
+    T cf = xsqrt(cfsq);
+    T cssq = twid_csq * vaxsq / cfsq;
+    T cs = xsqrt(cssq);
+    T bt = xsqrt(btsq);
+    T bt_star = xsqrt(bt_starsq);
+    T bet2 = T(0.0), bet3 = T(0.0);
+    if (bt != T(0.0)) { bet2 = b2 / bt; bet3 = b3 / bt; }
+    T bet2_star = bet2 / xsqrt(gm1 - (gm1 - T(1.0)) * y);
+    T bet3_star = bet3 / xsqrt(gm1 - (gm1 - T(1.0)) * y);
+    T bet_starsq = bet2_star * bet2_star + bet3_star * bet3_star;
+    T vbet = v2 * bet2_star + v3 * bet3_star;
+    T q2_star = T(0.0), q3_star = T(0.0);
+    if (bet_starsq != T(0.0)) { q2_star = bet2_star / bet_starsq; q3_star = bet3_star / bet_starsq; }
+    T alpha_f, alpha_s;
+    if ((cfsq - cssq) <= T(0.0)) { alpha_f = T(1.0); alpha_s = T(0.0); }
+    else if ((twid_csq - cssq) <= T(0.0)) { alpha_f = T(0.0); alpha_s = T(1.0); }
+    else if ((cfsq - twid_csq) <= T(0.0)) { alpha_f = T(1.0); alpha_s = T(0.0); }
     else {
-      alpha_f = sqrt((twid_csq - cssq) / (cfsq - cssq));
-      alpha_s = sqrt((cfsq - twid_csq) / (cfsq - cssq));
+      alpha_f = xsqrt((twid_csq - cssq) / (cfsq - cssq));
+      alpha_s = xsqrt((cfsq - twid_csq) / (cfsq - cssq));
     }
-    double sqrtd = sqrt(d);
-    double isqrtd = 1.0 / sqrtd;
-    double s = (b1 < 0.0) ? -1.0 : 1.0;
-    double twid_c = sqrt(twid_csq);
-    double qf = cf * alpha_f * s;
-    double qs = cs * alpha_s * s;
-    double af_prime = twid_c * alpha_f * isqrtd;
-    double as_prime = twid_c * alpha_s * isqrtd;
-    double afpbb = af_prime * bt_star * bet_starsq;
-    double aspbb = as_prime * bt_star * bet_starsq;
-    double vqstr = (v2 * q2_star + v3 * q3_star);
-    double vax = sqrt(vaxsq);
-    double norm = 0.5 / twid_csq;
-    double cff = norm * alpha_f * cf;
-    double css = norm * alpha_s * cs;
-    double qf_hat = qf * norm;
-    double qs_hat = qs * norm;
-    double af = norm * af_prime * d;
-    double as = norm * as_prime * d;
-    double afpb = norm * af_prime * bt_star;
-    double aspb = norm * as_prime * bt_star;
+    T sqrtd = xsqrt(d);
+    T isqrtd = T(1.0) / sqrtd;
+    T s = (b1 < T(0.0)) ? -T(1.0) : T(1.0);
+    T twid_c = xsqrt(twid_csq);
+    T qf = cf * alpha_f * s;
+    T qs = cs * alpha_s * s;
+    T af_prime = twid_c * alpha_f * isqrtd;
+    T as_prime = twid_c * alpha_s * isqrtd;
+    T afpbb = af_prime * bt_star * bet_starsq;
+    T aspbb = as_prime * bt_star * bet_starsq;
+    T vqstr = (v2 * q2_star + v3 * q3_star);
+    T vax = xsqrt(vaxsq);
+    T norm = T(0.5) / twid_csq;
+    T cff = norm * alpha_f * cf;
+    T css = norm * alpha_s * cs;
+    T qf_hat = qf * norm;
+    T qs_hat = qs * norm;
+    T af = norm * af_prime * d;
+    T as = norm * as_prime * d;
+    T afpb = norm * af_prime * bt_star;
+    T aspb = norm * as_prime * bt_star;
     ev[0] = v1 - cf; ev[1] = v1 - vax; ev[2] = v1 - cs; ev[3] = v1;
     ev[4] = v1 + cs; ev[5] = v1 + vax; ev[6] = v1 + cf;
-    double a[NWAVE];
-    double alpha_f_bar = alpha_f * gm1 * norm;
-    double alpha_s_bar = alpha_s * gm1 * norm;
-    double gm1a = gm1 / twid_csq;
+    T a[NWAVE];
+    T alpha_f_bar = alpha_f * gm1 * norm;
+    T alpha_s_bar = alpha_s * gm1 * norm;
+    T gm1a = gm1 / twid_csq;
     a[0] = du[0] * (alpha_f_bar * (vsq - hp) + cff * (cf + v1) - qs_hat * vqstr - aspb);
     a[0] -= du[1] * (alpha_f_bar * v1 + cff);
     a[0] -= du[2] * (alpha_f_bar * v2 - qs_hat * q2_star);
@@ -563,7 +588,7 @@ __device__ __forceinline__ void roe(double gamma, const double *wli, const doubl
     a[1] += du[3] * bet2;
     a[1] -= du[5] * sqrtd * bet3 * s;
     a[1] += du[6] * sqrtd * bet2 * s;
-    a[1] *= 0.5;
+    a[1] *= T(0.5);
     a[2] = du[0] * (alpha_s_bar * (vsq - hp) + css * (cs + v1) + qf_hat * vqstr + afpb);
     a[2] -= du[1] * (alpha_s_bar * v1 + css);
     a[2] -= du[2] * (alpha_s_bar * v2 + qf_hat * q2_star);
@@ -571,7 +596,7 @@ __device__ __forceinline__ void roe(double gamma, const double *wli, const doubl
     a[2] += du[4] * alpha_s_bar;
     a[2] -= du[5] * (af * q2_star + alpha_s_bar * b2);
     a[2] -= du[6] * (af * q3_star + alpha_s_bar * b3);
-    a[3] = du[0] * (1.0 - gm1a * (0.5 * vsq - (gm1 - 1.0) * x / gm1));
+    a[3] = du[0] * (T(1.0) - gm1a * (T(0.5) * vsq - (gm1 - T(1.0)) * x / gm1));
     a[3] += du[1] * gm1a * v1;
     a[3] += du[2] * gm1a * v2;
     a[3] += du[3] * gm1a * v3;
@@ -590,7 +615,7 @@ __device__ __forceinline__ void roe(double gamma, const double *wli, const doubl
     a[5] -= du[3] * bet2;
     a[5] -= du[5] * sqrtd * bet3 * s;
     a[5] += du[6] * sqrtd * bet2 * s;
-    a[5] *= 0.5;
+    a[5] *= T(0.5);
     a[6] = du[0] * (alpha_f_bar * (vsq - hp) + cff * (cf - v1) + qs_hat * vqstr - aspb);
     a[6] -= du[1] * (alpha_f_bar * v1 - cff);
     a[6] -= du[2] * (alpha_f_bar * v2 + qs_hat * q2_star);
@@ -598,16 +623,16 @@ __device__ __forceinline__ void roe(double gamma, const double *wli, const doubl
     a[6] += du[4] * alpha_f_bar;
     a[6] += du[5] * (as * q2_star - alpha_f_bar * b2);
     a[6] += du[6] * (as * q3_star - alpha_f_bar * b3);
-    double coeff[NWAVE];
-    for (int n = 0; n < NWAVE; ++n) coeff[n] = -0.5 * fabs(ev[n]) * a[n];
-    double dens = wli[IDN] + a[0] * alpha_f;
-    if (dens < 0.0) llf_flag = 1;
+    T coeff[NWAVE];
+    for (int n = 0; n < NWAVE; ++n) coeff[n] = -T(0.5) * xabs(ev[n]) * a[n];
+    T dens = wli[IDN] + a[0] * alpha_f;
+    if (dens < T(0.0)) llf_flag = 1;
     dens += a[2] * alpha_s;
-    if (dens < 0.0) llf_flag = 1;
+    if (dens < T(0.0)) llf_flag = 1;
     dens += a[3];
-    if (dens < 0.0) llf_flag = 1;
+    if (dens < T(0.0)) llf_flag = 1;
     dens += a[4] * alpha_s;
-    if (dens < 0.0) llf_flag = 1;
+    if (dens < T(0.0)) llf_flag = 1;
     flxi[0] += coeff[0] * alpha_f;
     flxi[0] += coeff[2] * alpha_s;
     flxi[0] += coeff[3];
@@ -635,7 +660,7 @@ __device__ __forceinline__ void roe(double gamma, const double *wli, const doubl
     flxi[4] += coeff[0] * (alpha_f * (hp - v1 * cf) + qs * vbet + aspbb);
     flxi[4] -= coeff[1] * (v2 * bet3 - v3 * bet2);
     flxi[4] += coeff[2] * (alpha_s * (hp - v1 * cs) - qf * vbet - afpbb);
-    flxi[4] += coeff[3] * (0.5 * vsq + (gm1 - 1.0) * x / gm1);
+    flxi[4] += coeff[3] * (T(0.5) * vsq + (gm1 - T(1.0)) * x / gm1);
     flxi[4] += coeff[4] * (alpha_s * (hp + v1 * cs) + qf * vbet - afpbb);
     flxi[4] += coeff[5] * (v1 * bet3 - v3 * bet2);
     flxi[4] += coeff[6] * (alpha_f * (hp + v1 * cf) - qs * vbet + aspbb);
@@ -652,21 +677,21 @@ __device__ __forceinline__ void roe(double gamma, const double *wli, const doubl
     flxi[6] += coeff[5] * bet2 * s * isqrtd;
     flxi[6] += coeff[6] * as_prime * bet3_star;
   }
-  if (ev[0] >= 0.0) {
+  if (ev[0] >= T(0.0)) {
     for (int n = 0; n < NWAVE; ++n) flxi[n] = fl[n];
   }
-  if (ev[NWAVE - 1] <= 0.0) {
+  if (ev[NWAVE - 1] <= T(0.0)) {
     for (int n = 0; n < NWAVE; ++n) flxi[n] = fr[n];
   }
   if (llf_flag != 0) {
-    double cfl = fast_speed(gamma, wli, bxi);
-    double cfr = fast_speed(gamma, wri, bxi);
-    double a = 0.5 * dmax((fabs(wli[IVX]) + cfl), (fabs(wri[IVX]) + cfr));
-    flxi[IDN] = 0.5 * (fl[IDN] + fr[IDN]) - a * du[IDN];
-    flxi[IVX] = 0.5 * (fl[IVX] + fr[IVX]) - a * du[IVX];
-    flxi[IVY] = 0.5 * (fl[IVY] + fr[IVY]) - a * du[IVY];
-    flxi[IVZ] = 0.5 * (fl[IVZ] + fr[IVZ]) - a * du[IVZ];
-    flxi[IEN] = 0.5 * (fl[IEN] + fr[IEN]) - a * du[IEN];
+    T cfl = fast_speed(gamma, wli, bxi);
+    T cfr = fast_speed(gamma, wri, bxi);
+    T a = T(0.5) * dmax((xabs(wli[IVX]) + cfl), (xabs(wri[IVX]) + cfr));
+    flxi[IDN] = T(0.5) * (fl[IDN] + fr[IDN]) - a * du[IDN];
+    flxi[IVX] = T(0.5) * (fl[IVX] + fr[IVX]) - a * du[IVX];
+    flxi[IVY] = T(0.5) * (fl[IVY] + fr[IVY]) - a * du[IVY];
+    flxi[IVZ] = T(0.5) * (fl[IVZ] + fr[IVZ]) - a * du[IVZ];
+    flxi[IEN] = T(0.5) * (fl[IEN] + fr[IEN]) - a * du[IEN];
   }
 }
 
@@ -675,59 +700,60 @@ __device__ __forceinline__ void roe(double gamma, const double *wli, const doubl
 // reconstruction (ORDER 1: reconstruct/dc.cpp:24-104; ORDER 2: the uniform-mesh van
 // Leer branch of reconstruct/plm.cpp:27-351), Riemann solver, face EMFs and the CT
 // upwind weight.  One thread per face.
-template <int DIR, int ORDER, int RS>
+template <int DIR, int ORDER, int RS, typename T>
 __global__ void __launch_bounds__(256)
-k_flux(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
-       const double *__restrict__ bn, double *__restrict__ flx, double *__restrict__ ey,
-       double *__restrict__ ez, double *__restrict__ wct, const BlockCoef *__restrict__ coef,
-       const double *__restrict__ plm, double gamma, double dt, double dfl, double pfl) {
+k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
+       const double *__restrict__ bn, T *__restrict__ flx, T *__restrict__ ey,
+       T *__restrict__ ez, T *__restrict__ wct, const BlockCoef *__restrict__ coef,
+       const double *__restrict__ plm, double gamma_, double dt, double dfl_, double pfl_) {
   int il = g.is - (DIR != 0), iu = g.ie + 1;
   int jl = g.js - (DIR != 1), ju = g.je + 1;
   int kl = g.ks - (DIR != 2);
   PlaneIdx p;
   if (!plane_index(il, iu, jl, ju, kl, p)) return;
+  const T gamma = T(gamma_), dfl = T(dfl_), pfl = T(pfl_);
   const int64_t st = (DIR == 0) ? 1 : (DIR == 1 ? (int64_t)g.P : (int64_t)g.J * g.P);
   const int64_t id = IDX(g, p.k, p.j, p.i);
-  const double *wb = w + (int64_t)p.b * NHYDRO * g.S;
-  const double *bb = bcc + (int64_t)p.b * 3 * g.S;
+  const T *wb = w + (int64_t)p.b * NHYDRO * g.S;
+  const T *bb = bcc + (int64_t)p.b * 3 * g.S;
   // transverse bcc components in the (IBY, IBZ) slots, plm.cpp:46-56,160-170,273-283
   constexpr int TB1 = (DIR == 0) ? IB2 : (DIR == 1 ? IB3 : IB1);
   constexpr int TB2 = (DIR == 0) ? IB3 : (DIR == 1 ? IB1 : IB2);
-  double wl[NWAVE], wr[NWAVE];
+  T wl[NWAVE], wr[NWAVE];
   int cidx = (DIR == 0) ? p.i : (DIR == 1 ? p.j : p.k);   // index of the cell above the face
   const double *pc = plm + ((int64_t)p.b * 3 + DIR) * 2 * g.PL;  // [dxp][dxm] rows
 #pragma unroll
   for (int n = 0; n < NWAVE; ++n) {
-    const double *q;
+    const T *q;
     if (n < NHYDRO) q = wb + (int64_t)n * g.S + id;
     else q = bb + (int64_t)((n == IBY) ? TB1 : TB2) * g.S + id;
     if (ORDER == 1) {
       wl[n] = q[-st];
       wr[n] = q[0];
     } else if (ORDER == 3) {
-      double q6[6];
+      T q6[6];
 #pragma unroll
       for (int t = 0; t < 6; ++t) q6[t] = q[(t - 3) * st];
-      double m0, p0, m1, p1;
+      T m0, p0, m1, p1;
       ppm_cell(q6, m0, p0);       // cell c-1: values at c-3..c+1
       ppm_cell(q6 + 1, m1, p1);   // cell c:   values at c-2..c+2
       wl[n] = p0;
       wr[n] = m1;
     } else {
-      double qm2 = q[-2 * st], qm1 = q[-st], q0 = q[0], qp1 = q[st];
+      T qm2 = q[-2 * st], qm1 = q[-st], q0 = q[0], qp1 = q[st];
       // left state: upper face of cell c-1
-      double dwl = (qm1 - qm2), dwr = (q0 - qm1);
-      double dw2 = dwl * dwr;
-      double dwm = 2.0 * dw2 / (dwl + dwr);
-      if (dw2 <= 0.0) dwm = 0.0;
-      wl[n] = qm1 + pc[cidx - 1] * dwm;
+      T dwl = (qm1 - qm2), dwr = (q0 - qm1);
+      T dw2 = dwl * dwr;
+      T dwm = T(2.0) * dw2 / (dwl + dwr);
+      if (dw2 <= T(0.0)) dwm = T(0.0);
+      wl[n] = qm1 + T(pc[cidx - 1]) * dwm;
       // right state: lower face of cell c
       dwl = (q0 - qm1);
       dwr = (qp1 - q0);
       dw2 = dwl * dwr;
-      dwm = 2.0 * dw2 / (dwl + dwr);
-      if (dw2 <= 0.0) dwm = 0.0;
-      wr[n] = q0 - pc[g.PL + cidx] * dwm;
+      dwm = T(2.0) * dw2 / (dwl + dwr);
+      if (dw2 <= T(0.0)) dwm = T(0.0);
+      wr[n] = q0 - T(pc[g.PL + cidx]) * dwm;
     }
   }
   if (ORDER == 3) {
@@ -738,16 +764,16 @@ k_flux(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
     wr[IPR] = (wr[IPR] > pfl) ? wr[IPR] : pfl;
   }
   constexpr int ivx = IVX + DIR, ivy = IVX + (DIR + 1) % 3, ivz = IVX + (DIR + 2) % 3;
-  double wli[NWAVE], wri[NWAVE], flxi[NWAVE];
+  T wli[NWAVE], wri[NWAVE], flxi[NWAVE];
   wli[IDN] = wl[IDN]; wli[IVX] = wl[ivx]; wli[IVY] = wl[ivy]; wli[IVZ] = wl[ivz];
   wli[IPR] = wl[IPR]; wli[IBY] = wl[IBY]; wli[IBZ] = wl[IBZ];
   wri[IDN] = wr[IDN]; wri[IVX] = wr[ivx]; wri[IVY] = wr[ivy]; wri[IVZ] = wr[ivz];
   wri[IPR] = wr[IPR]; wri[IBY] = wr[IBY]; wri[IBZ] = wr[IBZ];
-  double bxi = bn[(int64_t)p.b * g.S + id];
+  T bxi = T(bn[(int64_t)p.b * g.S + id]);
   if (RS == AGX_RSOLVER_HLLE) hlle(gamma, wli, wri, bxi, flxi);
   else if (RS == AGX_RSOLVER_ROE) roe(gamma, wli, wri, bxi, flxi);
   else hlld(gamma, wli, wri, bxi, flxi);
-  double *fb = flx + (int64_t)p.b * NHYDRO * g.S + id;
+  T *fb = flx + (int64_t)p.b * NHYDRO * g.S + id;
   fb[IDN * g.S] = flxi[IDN];
   fb[ivx * g.S] = flxi[IVX];
   fb[ivy * g.S] = flxi[IVY];
@@ -756,7 +782,7 @@ k_flux(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
   int64_t fo = (int64_t)p.b * g.S + id;
   ey[fo] = -flxi[IBY];
   ez[fo] = flxi[IBZ];
-  wct[fo] = ct_weight(flxi[IDN], wli[IDN], wri[IDN], coef[p.b].dx[DIR], dt);
+  wct[fo] = ct_weight(flxi[IDN], wli[IDN], wri[IDN], T(coef[p.b].dx[DIR]), T(dt));
 }
 
 // Field::ComputeCornerE, 3D Newtonian branch (field/calculate_corner_e.cpp:130-229).
@@ -766,6 +792,40 @@ __device__ __forceinline__ double cce_comp(const double *wb, const double *bb, i
   if (comp == 0) return wb[IVZ * S + id] * bb[IB2 * S + id] - wb[IVY * S + id] * bb[IB3 * S + id];
   if (comp == 1) return wb[IVX * S + id] * bb[IB3 * S + id] - wb[IVZ * S + id] * bb[IB1 * S + id];
   return wb[IVY * S + id] * bb[IB1 * S + id] - wb[IVX * S + id] * bb[IB2 * S + id];
+}
+
+// GS05 upwinded corner EMFs (calculate_corner_e.cpp:187-229), one edge each, in the
+// reference's operand order.  Arguments are named by position relative to the edge
+// (k,j,i): c_* are cell-centred E of the four cells around the edge, e?x?_* the face
+// EMFs and w?_* the CT weights of the four faces meeting at the edge.
+__device__ __forceinline__ double edge_e1(double c_k, double c_kjm, double c_km, double c_kmjm,
+                                          double e1x3, double e1x3_jm, double e1x2, double e1x2_km,
+                                          double w2, double w2_km, double w3, double w3_jm) {
+  double de1_l3 = (1.0 - w2_km) * (e1x3 - c_km) + (w2_km) * (e1x3_jm - c_kmjm);
+  double de1_r3 = (1.0 - w2) * (e1x3 - c_k) + (w2) * (e1x3_jm - c_kjm);
+  double de1_l2 = (1.0 - w3_jm) * (e1x2 - c_kjm) + (w3_jm) * (e1x2_km - c_kmjm);
+  double de1_r2 = (1.0 - w3) * (e1x2 - c_k) + (w3) * (e1x2_km - c_km);
+  return 0.25 * (de1_l3 + de1_r3 + de1_l2 + de1_r2 + e1x2_km + e1x2 + e1x3_jm + e1x3);
+}
+
+__device__ __forceinline__ double edge_e2(double c_k, double c_kim, double c_km, double c_kmim,
+                                          double e2x3, double e2x3_im, double e2x1, double e2x1_km,
+                                          double w1, double w1_km, double w3, double w3_im) {
+  double de2_l3 = (1.0 - w1_km) * (e2x3 - c_km) + (w1_km) * (e2x3_im - c_kmim);
+  double de2_r3 = (1.0 - w1) * (e2x3 - c_k) + (w1) * (e2x3_im - c_kim);
+  double de2_l1 = (1.0 - w3_im) * (e2x1 - c_kim) + (w3_im) * (e2x1_km - c_kmim);
+  double de2_r1 = (1.0 - w3) * (e2x1 - c_k) + (w3) * (e2x1_km - c_km);
+  return 0.25 * (de2_l3 + de2_r3 + de2_l1 + de2_r1 + e2x3_im + e2x3 + e2x1_km + e2x1);
+}
+
+__device__ __forceinline__ double edge_e3(double c_k, double c_im, double c_jm, double c_jmim,
+                                          double e3x2, double e3x2_im, double e3x1, double e3x1_jm,
+                                          double w1, double w1_jm, double w2, double w2_im) {
+  double de3_l2 = (1.0 - w1_jm) * (e3x2 - c_jm) + (w1_jm) * (e3x2_im - c_jmim);
+  double de3_r2 = (1.0 - w1) * (e3x2 - c_k) + (w1) * (e3x2_im - c_im);
+  double de3_l1 = (1.0 - w2_im) * (e3x1 - c_im) + (w2_im) * (e3x1_jm - c_jmim);
+  double de3_r1 = (1.0 - w2) * (e3x1 - c_k) + (w2) * (e3x1_jm - c_jm);
+  return 0.25 * (de3_l1 + de3_r1 + de3_l2 + de3_r2 + e3x2_im + e3x2 + e3x1_jm + e3x1);
 }
 
 __global__ void __launch_bounds__(256)
@@ -785,38 +845,20 @@ k_corner_e(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
   const double *wb = w + (int64_t)p.b * NHYDRO * S;
   const double *bb = bcc + (int64_t)p.b * 3 * S;
   // e1 at (k,j,i): cells (k-1|k, j-1|j, i)
-  {
-    double c_kjm = cce_comp(wb, bb, S, id - sj, 0), c_k = cce_comp(wb, bb, S, id, 0);
-    double c_km = cce_comp(wb, bb, S, id - sk, 0), c_kmjm = cce_comp(wb, bb, S, id - sk - sj, 0);
-    double de1_l3 = (1.0 - wx2[fo - sk]) * (e1x3[fo] - c_km) + (wx2[fo - sk]) * (e1x3[fo - sj] - c_kmjm);
-    double de1_r3 = (1.0 - wx2[fo]) * (e1x3[fo] - c_k) + (wx2[fo]) * (e1x3[fo - sj] - c_kjm);
-    double de1_l2 = (1.0 - wx3[fo - sj]) * (e1x2[fo] - c_kjm) + (wx3[fo - sj]) * (e1x2[fo - sk] - c_kmjm);
-    double de1_r2 = (1.0 - wx3[fo]) * (e1x2[fo] - c_k) + (wx3[fo]) * (e1x2[fo - sk] - c_km);
-    e1[fo] = 0.25 * (de1_l3 + de1_r3 + de1_l2 + de1_r2 + e1x2[fo - sk] + e1x2[fo] + e1x3[fo - sj] +
-                     e1x3[fo]);
-  }
+  e1[fo] = edge_e1(cce_comp(wb, bb, S, id, 0), cce_comp(wb, bb, S, id - sj, 0),
+                   cce_comp(wb, bb, S, id - sk, 0), cce_comp(wb, bb, S, id - sk - sj, 0),
+                   e1x3[fo], e1x3[fo - sj], e1x2[fo], e1x2[fo - sk], wx2[fo], wx2[fo - sk],
+                   wx3[fo], wx3[fo - sj]);
   // e2 at (k,j,i): cells (k-1|k, j, i-1|i)
-  {
-    double c_k = cce_comp(wb, bb, S, id, 1), c_kim = cce_comp(wb, bb, S, id - 1, 1);
-    double c_km = cce_comp(wb, bb, S, id - sk, 1), c_kmim = cce_comp(wb, bb, S, id - sk - 1, 1);
-    double de2_l3 = (1.0 - wx1[fo - sk]) * (e2x3[fo] - c_km) + (wx1[fo - sk]) * (e2x3[fo - 1] - c_kmim);
-    double de2_r3 = (1.0 - wx1[fo]) * (e2x3[fo] - c_k) + (wx1[fo]) * (e2x3[fo - 1] - c_kim);
-    double de2_l1 = (1.0 - wx3[fo - 1]) * (e2x1[fo] - c_kim) + (wx3[fo - 1]) * (e2x1[fo - sk] - c_kmim);
-    double de2_r1 = (1.0 - wx3[fo]) * (e2x1[fo] - c_k) + (wx3[fo]) * (e2x1[fo - sk] - c_km);
-    e2[fo] = 0.25 * (de2_l3 + de2_r3 + de2_l1 + de2_r1 + e2x3[fo - 1] + e2x3[fo] + e2x1[fo - sk] +
-                     e2x1[fo]);
-  }
+  e2[fo] = edge_e2(cce_comp(wb, bb, S, id, 1), cce_comp(wb, bb, S, id - 1, 1),
+                   cce_comp(wb, bb, S, id - sk, 1), cce_comp(wb, bb, S, id - sk - 1, 1),
+                   e2x3[fo], e2x3[fo - 1], e2x1[fo], e2x1[fo - sk], wx1[fo], wx1[fo - sk],
+                   wx3[fo], wx3[fo - 1]);
   // e3 at (k,j,i): cells (k, j-1|j, i-1|i)
-  {
-    double c_k = cce_comp(wb, bb, S, id, 2), c_im = cce_comp(wb, bb, S, id - 1, 2);
-    double c_jm = cce_comp(wb, bb, S, id - sj, 2), c_jmim = cce_comp(wb, bb, S, id - sj - 1, 2);
-    double de3_l2 = (1.0 - wx1[fo - sj]) * (e3x2[fo] - c_jm) + (wx1[fo - sj]) * (e3x2[fo - 1] - c_jmim);
-    double de3_r2 = (1.0 - wx1[fo]) * (e3x2[fo] - c_k) + (wx1[fo]) * (e3x2[fo - 1] - c_im);
-    double de3_l1 = (1.0 - wx2[fo - 1]) * (e3x1[fo] - c_im) + (wx2[fo - 1]) * (e3x1[fo - sj] - c_jmim);
-    double de3_r1 = (1.0 - wx2[fo]) * (e3x1[fo] - c_k) + (wx2[fo]) * (e3x1[fo - sj] - c_jm);
-    e3[fo] = 0.25 * (de3_l1 + de3_r1 + de3_l2 + de3_r2 + e3x2[fo - 1] + e3x2[fo] + e3x1[fo - sj] +
-                     e3x1[fo]);
-  }
+  e3[fo] = edge_e3(cce_comp(wb, bb, S, id, 2), cce_comp(wb, bb, S, id - 1, 2),
+                   cce_comp(wb, bb, S, id - sj, 2), cce_comp(wb, bb, S, id - sj - 1, 2),
+                   e3x2[fo], e3x2[fo - 1], e3x1[fo], e3x1[fo - sj], wx1[fo], wx1[fo - sj],
+                   wx2[fo], wx2[fo - 1]);
 }
 
 // Conserved-variable part of one VL2 stage for the interior cells
@@ -950,53 +992,6 @@ k_bvals_fc(Geo g, double *a) {
       a[(int64_t)s * g.S + IDX(g, p.k - o3 * g.bx3, p.j - o2 * g.bx2, p.i - o1 * g.bx1)];
 }
 
-// Field::CalculateCellCenteredField (uniform branch, field/field.cpp:112-180) and
-// EquationOfState::ConservedToPrimitive (eos/adiabatic_mhd.cpp:41-90) over interior +
-// ghosts; the density floor and the energy fix are written back to u.
-__global__ void __launch_bounds__(256)
-k_cons2prim(Geo g, double *__restrict__ u, double *__restrict__ w, double *__restrict__ bcc,
-            const double *__restrict__ b1, const double *__restrict__ b2,
-            const double *__restrict__ b3, double gm1, double dfl, double pfl) {
-  PlaneIdx p;
-  if (!plane_index(0, g.n1 - 1, 0, g.n2 - 1, 0, p)) return;
-  const int64_t S = g.S;
-  const int64_t id = IDX(g, p.k, p.j, p.i);
-  const int64_t fo = (int64_t)p.b * S + id;
-  const double bcc1 = 0.5 * b1[fo] + 0.5 * b1[fo + 1];
-  const double bcc2 = 0.5 * b2[fo] + 0.5 * b2[fo + g.P];
-  const double bcc3 = 0.5 * b3[fo] + 0.5 * b3[fo + (int64_t)g.J * g.P];
-  const int64_t bo = (int64_t)p.b * 3 * S + id;
-  bcc[bo] = bcc1;
-  bcc[bo + S] = bcc2;
-  bcc[bo + 2 * S] = bcc3;
-  const int64_t c = (int64_t)p.b * NHYDRO * S + id;
-  double u_d = u[c], u_m1 = u[c + S], u_m2 = u[c + 2 * S], u_m3 = u[c + 3 * S], u_e = u[c + 4 * S];
-  u_d = (u_d > dfl) ? u_d : dfl;
-  double di = 1.0 / u_d;
-  double pb = 0.5 * (SQR(bcc1) + SQR(bcc2) + SQR(bcc3));
-  double e_k = 0.5 * di * (SQR(u_m1) + SQR(u_m2) + SQR(u_m3));
-  double w_p = gm1 * (u_e - e_k - pb);
-  u_e = (w_p > pfl) ? u_e : ((pfl / gm1) + e_k + pb);
-  w_p = (w_p > pfl) ? w_p : pfl;
-  u[c] = u_d;
-  u[c + 4 * S] = u_e;
-  w[c] = u_d;
-  w[c + S] = u_m1 * di;
-  w[c + 2 * S] = u_m2 * di;
-  w[c + 3 * S] = u_m3 * di;
-  w[c + 4 * S] = w_p;
-}
-
-// ================================================================ fused stage kernels
-// The kernels below implement the same VL2 stage as k_integrate_u + k_ct + the ghost
-// exchange + k_cons2prim with fewer HBM passes, keeping the reference's data flow
-// exactly: interior cells are updated and converted to primitives in one pass; ghost
-// cells receive the neighbour's un-floored conserved state and are converted from
-// their own face field (own shared face + exchanged ghost faces), as
-// bvals_cc.cpp:300-336 followed by adiabatic_mhd.cpp:41-90 over the ghost zones do.
-// The density/pressure floors of interior cells are written back to u only after the
-// ghost copies are taken (k_floor_fix), so ghosts see the same un-floored values.
-
 // C2P of one cell in registers (eos/adiabatic_mhd.cpp:52-86).  Returns true when a
 // floor changed the conserved state.
 __device__ __forceinline__ bool c2p_cell(double &u_d, double u_m1, double u_m2, double u_m3,
@@ -1019,113 +1014,294 @@ __device__ __forceinline__ bool c2p_cell(double &u_d, double u_m1, double u_m2, 
   return fl;
 }
 
-// Interior cells of one VL2 stage: flux divergence + self-gravity (as k_integrate_u),
-// then bcc from the CT-updated faces and C2P.  Writes the un-floored u, w and bcc;
-// raises *floor_flag when any floor fired.
+// Field::CalculateCellCenteredField (uniform branch, field/field.cpp:112-180) and
+// EquationOfState::ConservedToPrimitive (eos/adiabatic_mhd.cpp:41-90) over interior +
+// ghosts; the density floor and the energy fix are written back to u.  TW is the
+// storage type of w and bcc (double, or float in the mixed-precision form).
+template <typename TW>
 __global__ void __launch_bounds__(256)
-k_integrate_c2p(Geo g, const double *src, double *dst, const double *__restrict__ f1,
-                const double *__restrict__ f2, const double *__restrict__ f3, double *w,
-                double *__restrict__ bcc, const double *__restrict__ phi,
-                const double *__restrict__ b1, const double *__restrict__ b2,
-                const double *__restrict__ b3, const BlockCoef *__restrict__ coef, double wght,
-                int grav, double gm1, double dfl, double pfl, int *floor_flag) {
+k_cons2prim(Geo g, double *__restrict__ u, TW *__restrict__ w, TW *__restrict__ bcc,
+            const double *__restrict__ b1, const double *__restrict__ b2,
+            const double *__restrict__ b3, double gm1, double dfl, double pfl) {
   PlaneIdx p;
-  if (!plane_index(g.is, g.ie, g.js, g.je, g.ks, p)) return;
+  if (!plane_index(0, g.n1 - 1, 0, g.n2 - 1, 0, p)) return;
   const int64_t S = g.S;
-  const int64_t sj = g.P, sk = (int64_t)g.J * g.P;
   const int64_t id = IDX(g, p.k, p.j, p.i);
-  const double dx1 = coef[p.b].dx[0], dx2 = coef[p.b].dx[1], dx3 = coef[p.b].dx[2];
-  const double a1 = dx2 * dx3, a2 = dx1 * dx3, a3 = dx1 * dx2;
-  const double vol = dx1 * dx2 * dx3;
-  const int64_t cb = (int64_t)p.b * NHYDRO * S + id;
-  double un[NHYDRO];
-#pragma unroll
-  for (int n = 0; n < NHYDRO; ++n) {
-    const int64_t o = cb + n * S;
-    double dflx = (a1 * f1[o + 1] - a1 * f1[o]);
-    dflx += (a2 * f2[o + sj] - a2 * f2[o]);
-    dflx += (a3 * f3[o + sk] - a3 * f3[o]);
-    un[n] = src[o] - wght * dflx / vol;
-  }
-  if (grav) {
-    const int64_t pb = (int64_t)p.b * S + id;
-    const double rho = w[cb];
-    const double phic = phi[pb];
-    {
-      double dtodx1 = wght / dx1;
-      double phil = 0.5 * (phi[pb - 1] + phic);
-      double phir = 0.5 * (phic + phi[pb + 1]);
-      un[IVX] -= dtodx1 * rho * (phir - phil);
-      un[IEN] -= dtodx1 * (f1[cb] * (phic - phil) + f1[cb + 1] * (phir - phic));
-    }
-    {
-      double dtodx2 = wght / dx2;
-      double phil = 0.5 * (phi[pb - sj] + phic);
-      double phir = 0.5 * (phic + phi[pb + sj]);
-      un[IVY] -= dtodx2 * rho * (phir - phil);
-      un[IEN] -= dtodx2 * (f2[cb] * (phic - phil) + f2[cb + sj] * (phir - phic));
-    }
-    {
-      double dtodx3 = wght / dx3;
-      double phil = 0.5 * (phi[pb - sk] + phic);
-      double phir = 0.5 * (phic + phi[pb + sk]);
-      un[IVZ] -= dtodx3 * rho * (phir - phil);
-      un[IEN] -= dtodx3 * (f3[cb] * (phic - phil) + f3[cb + sk] * (phir - phic));
-    }
-  }
-#pragma unroll
-  for (int n = 0; n < NHYDRO; ++n) dst[cb + n * S] = un[n];
   const int64_t fo = (int64_t)p.b * S + id;
   const double bcc1 = 0.5 * b1[fo] + 0.5 * b1[fo + 1];
-  const double bcc2 = 0.5 * b2[fo] + 0.5 * b2[fo + sj];
-  const double bcc3 = 0.5 * b3[fo] + 0.5 * b3[fo + sk];
+  const double bcc2 = 0.5 * b2[fo] + 0.5 * b2[fo + g.P];
+  const double bcc3 = 0.5 * b3[fo] + 0.5 * b3[fo + (int64_t)g.J * g.P];
   const int64_t bo = (int64_t)p.b * 3 * S + id;
-  bcc[bo] = bcc1;
-  bcc[bo + S] = bcc2;
-  bcc[bo + 2 * S] = bcc3;
-  double u_d = un[IDN], u_e = un[IEN], wv[5];
-  bool fl = c2p_cell(u_d, un[IVX], un[IVY], un[IVZ], u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
+  bcc[bo] = TW(bcc1);
+  bcc[bo + S] = TW(bcc2);
+  bcc[bo + 2 * S] = TW(bcc3);
+  const int64_t c = (int64_t)p.b * NHYDRO * S + id;
+  double u_d = u[c], u_e = u[c + 4 * S], wv[5];
+  c2p_cell(u_d, u[c + S], u[c + 2 * S], u[c + 3 * S], u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
+  u[c] = u_d;
+  u[c + 4 * S] = u_e;
 #pragma unroll
-  for (int n = 0; n < NHYDRO; ++n) w[cb + n * S] = wv[n];
-  if (fl) atomicOr(floor_flag, 1);
+  for (int n = 0; n < NHYDRO; ++n) w[c + n * S] = TW(wv[n]);
 }
 
-// Field::CT for all three face components in one launch (grid.y = component).
-__global__ void __launch_bounds__(256)
-k_ct3(Geo g, const double *s1, const double *s2, const double *s3, double *d1, double *d2,
-      double *d3, const double *__restrict__ e1, const double *__restrict__ e2,
-      const double *__restrict__ e3, const BlockCoef *__restrict__ coef, double wght) {
-  const int dir = blockIdx.y;
-  const int ni = g.ie - g.is + 1 + (dir == 0), nj = g.je - g.js + 1 + (dir == 1);
-  const int nk = g.ke - g.ks + 1 + (dir == 2);
-  int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= (int64_t)ni * nj * nk) return;
-  int i = g.is + (int)(q % ni);
-  int64_t r = q / ni;
-  int j = g.js + (int)(r % nj);
-  int k = g.ks + (int)(r / nj);
-  int b = blockIdx.z;
-  const int64_t sj = g.P, sk = (int64_t)g.J * g.P;
-  const int64_t fo = (int64_t)b * g.S + IDX(g, k, j, i);
-  const double dx1 = coef[b].dx[0], dx2 = coef[b].dx[1], dx3 = coef[b].dx[2];
-  if (dir == 0) {
-    double v = s1[fo];
-    double area = dx2 * dx3;
-    v -= (wght / area) * (dx3 * e3[fo + sj] - dx3 * e3[fo]);
-    v += (wght / area) * (dx2 * e2[fo + sk] - dx2 * e2[fo]);
-    d1[fo] = v;
-  } else if (dir == 1) {
-    double v = s2[fo];
-    double area = dx1 * dx3;
-    v += (wght / area) * (dx3 * e3[fo + 1] - dx3 * e3[fo]);
-    v -= (wght / area) * (dx1 * e1[fo + sk] - dx1 * e1[fo]);
-    d2[fo] = v;
-  } else {
-    double v = s3[fo];
-    double area = dx1 * dx2;
-    v -= (wght / area) * (dx2 * e2[fo + 1] - dx2 * e2[fo]);
-    v += (wght / area) * (dx1 * e1[fo + sj] - dx1 * e1[fo]);
-    d3[fo] = v;
+// Hydro::NewBlockTimeStep per cell (hydro/new_blockdt.cpp:78-97): min over the three
+// directions of dx/(|v|+c_f) with bx = bcc + |b_face - bcc| (lower face).
+__device__ __forceinline__ double cell_dt(double gamma, const double *wv, double c1, double c2,
+                                          double c3, double f1, double f2, double f3,
+                                          const double *dx) {
+  double wi[NWAVE];
+  wi[IDN] = wv[0]; wi[IVX] = wv[1]; wi[IVY] = wv[2]; wi[IVZ] = wv[3]; wi[IPR] = wv[4];
+  double bx = c1 + fabs(f1 - c1);
+  wi[IBY] = c2; wi[IBZ] = c3;
+  double cf = fast_speed(gamma, wi, bx);
+  double dt1 = dx[0] / (fabs(wi[IVX]) + cf);
+  wi[IBY] = c3; wi[IBZ] = c1;
+  bx = c2 + fabs(f2 - c2);
+  cf = fast_speed(gamma, wi, bx);
+  double dt2 = dx[1] / (fabs(wi[IVY]) + cf);
+  wi[IBY] = c1; wi[IBZ] = c2;
+  bx = c3 + fabs(f3 - c3);
+  cf = fast_speed(gamma, wi, bx);
+  double dt3 = dx[2] / (fabs(wi[IVZ]) + cf);
+  double m = DBL_MAX;
+  m = dmin(m, dt1);
+  m = dmin(m, dt2);
+  m = dmin(m, dt3);
+  return m;
+}
+
+// Minimum over a 256-thread workgroup (exact, so the order is free): wavefront shuffle,
+// then LDS across the 4 wavefronts; thread 0 returns the result.
+__device__ __forceinline__ double wg_min256(double m, double *sh) {
+  for (int off = 32; off > 0; off >>= 1) m = dmin(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  return dmin(dmin(sh[0], sh[1]), dmin(sh[2], sh[3]));
+}
+
+// ================================================================ fused stage kernel
+// One VL2 stage after the fluxes: corner EMFs (field/calculate_corner_e.cpp:130-229),
+// CT (field/ct.cpp:39-114), flux divergence and self-gravity source
+// (hydro/add_flux_divergence.cpp:50-94, hydro/srcterms/self_gravity.cpp:35-125), bcc
+// (field/field.cpp:112-180) and C2P (eos/adiabatic_mhd.cpp:41-90) of the interior cells,
+// and in the last stage the per-cell time-step bound (hydro/new_blockdt.cpp:57-160),
+// in one pass over HBM.  Every value is formed with the same operations as the task-level
+// kernels, so results are identical.
+//
+// A workgroup owns a TI x TJ tile of (i,j) columns of one MeshBlock (thread = column) and
+// walks the cell planes c = ks-1 .. ke+1 upwards.  Iteration c computes the cell-centred E
+// of plane c, the x1/x2 edges of edge plane c and the x3 edges of plane c, the new x3
+// faces of face plane c and the new x1/x2 faces of plane c-1, then updates cell plane
+// c-1.  LDS holds two planes of cell-centred E and edges and one plane of new x1/x2 faces;
+// fluxes, face EMFs and weights stream from HBM once.  Outputs go to registers other than
+// the inputs (u may be in place: each cell reads only its own src), so tiles that share
+// halo values never see each other's writes.
+template <typename TW>
+struct StageArgs {
+  const double *src;                 // conserved state the stage starts from
+  double *dst;                       // updated conserved state (interior)
+  const double *rho;                 // conserved state whose density feeds self-gravity
+  const double *phi;
+  const TW *f1, *f2, *f3;            // fluxes
+  const TW *e3x1, *e2x1, *e1x2, *e3x2, *e1x3, *e2x3;   // face EMFs
+  const TW *wx1, *wx2, *wx3;         // CT upwind weights
+  const TW *w, *bcc;                 // primitives / bcc the fluxes were computed from
+  TW *wn, *bccn;                     // new primitives / bcc (interior)
+  const double *s1, *s2, *s3;        // CT source faces
+  double *d1, *d2, *d3;              // CT destination faces (interior)
+  const BlockCoef *coef;
+  double wght, gm1, dfl, pfl, gamma;
+  int grav;
+  int *floor_flag;                   // raised when any floor fired
+  double *dtpart;                    // per-workgroup min of the dt bound, or null
+};
+
+template <int TI, int TJ, typename TW>
+__global__ void __launch_bounds__(TI * TJ)
+k_stage(Geo g, StageArgs<TW> a) {
+  static_assert(TI * TJ == 256, "k_stage runs 256-thread workgroups");
+  constexpr int NT = TI * TJ, CI = TI + 2, CJ = TJ + 2;
+  __shared__ double s_cce[2][3][CJ][CI];
+  __shared__ double s_e1[2][TJ + 1][TI];
+  __shared__ double s_e2[2][TJ][TI + 1];
+  __shared__ double s_e3[2][TJ + 1][TI + 1];
+  __shared__ double s_b1[TJ][TI + 1];
+  __shared__ double s_b2[TJ + 1][TI];
+  __shared__ double s_red[4];
+  const int tid = threadIdx.x;
+  const int tx = tid % TI, ty = tid / TI;
+  const int nti = (g.bx1 + TI - 1) / TI;
+  const int b = blockIdx.y;
+  const int i0 = g.is + (int)(blockIdx.x % nti) * TI;
+  const int j0 = g.js + (int)(blockIdx.x / nti) * TJ;
+  const int ni = min(TI, g.ie + 1 - i0), nj = min(TJ, g.je + 1 - j0);
+  const bool own = (tx < ni) && (ty < nj);
+  const int i = i0 + tx, j = j0 + ty;
+  const int64_t S = g.S, sj = g.P, sk = (int64_t)g.J * g.P;
+  const int64_t fb = (int64_t)b * S;
+  const int64_t cbase = (int64_t)b * NHYDRO * S, bbase = (int64_t)b * 3 * S;
+  const double dx[3] = {a.coef[b].dx[0], a.coef[b].dx[1], a.coef[b].dx[2]};
+  const double dx1 = dx[0], dx2 = dx[1], dx3 = dx[2];
+  const double wght = a.wght;
+  double b3_lo = 0.0;       // new b3 at the lower face of cell plane c-1
+  double dtmin = DBL_MAX;
+  bool floored = false;
+  for (int c = g.ks - 1; c <= g.ke + 1; ++c) {
+    const int cb = c & 1, pb = cb ^ 1;
+    // (1) cell-centred E = -v x B at plane c, cells (j0-1..j0+nj) x (i0-1..i0+ni)
+    {
+      const int wi = ni + 2, n = wi * (nj + 2);
+      for (int q = tid; q < n; q += NT) {
+        const int jj = q / wi, ii = q - jj * wi;
+        const int64_t id = IDX(g, c, j0 - 1 + jj, i0 - 1 + ii);
+        const TW *wb = a.w + cbase + id;
+        const TW *bb = a.bcc + bbase + id;
+        const double vx = wb[IVX * S], vy = wb[IVY * S], vz = wb[IVZ * S];
+        const double bx = bb[0], by = bb[S], bz = bb[2 * S];
+        s_cce[cb][0][jj][ii] = vz * by - vy * bz;
+        s_cce[cb][1][jj][ii] = vx * bz - vz * bx;
+        s_cce[cb][2][jj][ii] = vy * bx - vx * by;
+      }
+    }
+    __syncthreads();
+    // (2) e1, e2 at edge plane c (lower edges of cell plane c) and e3 at plane c
+    if (c >= g.ks) {
+      const int n1e = (nj + 1) * ni, n2e = nj * (ni + 1);
+      const int n3e = (c <= g.ke) ? (nj + 1) * (ni + 1) : 0;
+      for (int q = tid; q < n1e + n2e + n3e; q += NT) {
+        if (q < n1e) {
+          const int jj = q / ni, ii = q - jj * ni;
+          const int64_t fo = fb + IDX(g, c, j0 + jj, i0 + ii);
+          s_e1[cb][jj][ii] = edge_e1(
+              s_cce[cb][0][jj + 1][ii + 1], s_cce[cb][0][jj][ii + 1], s_cce[pb][0][jj + 1][ii + 1],
+              s_cce[pb][0][jj][ii + 1], (double)a.e1x3[fo], (double)a.e1x3[fo - sj],
+              (double)a.e1x2[fo], (double)a.e1x2[fo - sk], (double)a.wx2[fo],
+              (double)a.wx2[fo - sk], (double)a.wx3[fo], (double)a.wx3[fo - sj]);
+        } else if (q < n1e + n2e) {
+          const int r = q - n1e;
+          const int jj = r / (ni + 1), ii = r - jj * (ni + 1);
+          const int64_t fo = fb + IDX(g, c, j0 + jj, i0 + ii);
+          s_e2[cb][jj][ii] = edge_e2(
+              s_cce[cb][1][jj + 1][ii + 1], s_cce[cb][1][jj + 1][ii], s_cce[pb][1][jj + 1][ii + 1],
+              s_cce[pb][1][jj + 1][ii], (double)a.e2x3[fo], (double)a.e2x3[fo - 1],
+              (double)a.e2x1[fo], (double)a.e2x1[fo - sk], (double)a.wx1[fo],
+              (double)a.wx1[fo - sk], (double)a.wx3[fo], (double)a.wx3[fo - 1]);
+        } else {
+          const int r = q - n1e - n2e;
+          const int jj = r / (ni + 1), ii = r - jj * (ni + 1);
+          const int64_t fo = fb + IDX(g, c, j0 + jj, i0 + ii);
+          s_e3[cb][jj][ii] = edge_e3(
+              s_cce[cb][2][jj + 1][ii + 1], s_cce[cb][2][jj + 1][ii], s_cce[cb][2][jj][ii + 1],
+              s_cce[cb][2][jj][ii], (double)a.e3x2[fo], (double)a.e3x2[fo - 1],
+              (double)a.e3x1[fo], (double)a.e3x1[fo - sj], (double)a.wx1[fo],
+              (double)a.wx1[fo - sj], (double)a.wx2[fo], (double)a.wx2[fo - 1]);
+        }
+      }
+    }
+    __syncthreads();
+    // (3) CT: x3 faces of face plane c (own column), x1/x2 faces of plane c-1
+    double b3_hi = 0.0;
+    if (c >= g.ks) {
+      if (own) {
+        const int64_t fo = fb + IDX(g, c, j, i);
+        double v = a.s3[fo];
+        const double area = dx1 * dx2;
+        v -= (wght / area) * (dx2 * s_e2[cb][ty][tx + 1] - dx2 * s_e2[cb][ty][tx]);
+        v += (wght / area) * (dx1 * s_e1[cb][ty + 1][tx] - dx1 * s_e1[cb][ty][tx]);
+        a.d3[fo] = v;
+        b3_hi = v;
+      }
+      if (c - 1 >= g.ks) {
+        const int n1f = nj * (ni + 1), n2f = (nj + 1) * ni;
+        for (int q = tid; q < n1f + n2f; q += NT) {
+          if (q < n1f) {
+            const int jj = q / (ni + 1), ii = q - jj * (ni + 1);
+            const int64_t fo = fb + IDX(g, c - 1, j0 + jj, i0 + ii);
+            double v = a.s1[fo];
+            const double area = dx2 * dx3;
+            v -= (wght / area) * (dx3 * s_e3[pb][jj + 1][ii] - dx3 * s_e3[pb][jj][ii]);
+            v += (wght / area) * (dx2 * s_e2[cb][jj][ii] - dx2 * s_e2[pb][jj][ii]);
+            a.d1[fo] = v;
+            s_b1[jj][ii] = v;
+          } else {
+            const int r = q - n1f;
+            const int jj = r / ni, ii = r - jj * ni;
+            const int64_t fo = fb + IDX(g, c - 1, j0 + jj, i0 + ii);
+            double v = a.s2[fo];
+            const double area = dx1 * dx3;
+            v += (wght / area) * (dx3 * s_e3[pb][jj][ii + 1] - dx3 * s_e3[pb][jj][ii]);
+            v -= (wght / area) * (dx1 * s_e1[cb][jj][ii] - dx1 * s_e1[pb][jj][ii]);
+            a.d2[fo] = v;
+            s_b2[jj][ii] = v;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // (4) cell plane c-1: flux divergence, self-gravity, bcc, C2P, dt bound
+    if (c - 1 >= g.ks && own) {
+      const int64_t id = IDX(g, c - 1, j, i);
+      const double a1 = dx2 * dx3, a2 = dx1 * dx3, a3 = dx1 * dx2;
+      const double vol = dx1 * dx2 * dx3;
+      const int64_t cb0 = cbase + id;
+      double un[NHYDRO];
+#pragma unroll
+      for (int n = 0; n < NHYDRO; ++n) {
+        const int64_t o = cb0 + n * S;
+        double dflx = (a1 * (double)a.f1[o + 1] - a1 * (double)a.f1[o]);
+        dflx += (a2 * (double)a.f2[o + sj] - a2 * (double)a.f2[o]);
+        dflx += (a3 * (double)a.f3[o + sk] - a3 * (double)a.f3[o]);
+        un[n] = a.src[o] - wght * dflx / vol;
+      }
+      if (a.grav) {
+        const int64_t pbk = fb + id;
+        const double rho = a.rho[cb0];
+        const double phic = a.phi[pbk];
+        {
+          double dtodx1 = wght / dx1;
+          double phil = 0.5 * (a.phi[pbk - 1] + phic);
+          double phir = 0.5 * (phic + a.phi[pbk + 1]);
+          un[IVX] -= dtodx1 * rho * (phir - phil);
+          un[IEN] -= dtodx1 * ((double)a.f1[cb0] * (phic - phil) + (double)a.f1[cb0 + 1] * (phir - phic));
+        }
+        {
+          double dtodx2 = wght / dx2;
+          double phil = 0.5 * (a.phi[pbk - sj] + phic);
+          double phir = 0.5 * (phic + a.phi[pbk + sj]);
+          un[IVY] -= dtodx2 * rho * (phir - phil);
+          un[IEN] -= dtodx2 * ((double)a.f2[cb0] * (phic - phil) + (double)a.f2[cb0 + sj] * (phir - phic));
+        }
+        {
+          double dtodx3 = wght / dx3;
+          double phil = 0.5 * (a.phi[pbk - sk] + phic);
+          double phir = 0.5 * (phic + a.phi[pbk + sk]);
+          un[IVZ] -= dtodx3 * rho * (phir - phil);
+          un[IEN] -= dtodx3 * ((double)a.f3[cb0] * (phic - phil) + (double)a.f3[cb0 + sk] * (phir - phic));
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < NHYDRO; ++n) a.dst[cb0 + n * S] = un[n];
+      const double bcc1 = 0.5 * s_b1[ty][tx] + 0.5 * s_b1[ty][tx + 1];
+      const double bcc2 = 0.5 * s_b2[ty][tx] + 0.5 * s_b2[ty + 1][tx];
+      const double bcc3 = 0.5 * b3_lo + 0.5 * b3_hi;
+      const int64_t bo = bbase + id;
+      a.bccn[bo] = TW(bcc1);
+      a.bccn[bo + S] = TW(bcc2);
+      a.bccn[bo + 2 * S] = TW(bcc3);
+      double u_d = un[IDN], u_e = un[IEN], wv[5];
+      floored |= c2p_cell(u_d, un[IVX], un[IVY], un[IVZ], u_e, bcc1, bcc2, bcc3, a.gm1, a.dfl, a.pfl, wv);
+#pragma unroll
+      for (int n = 0; n < NHYDRO; ++n) a.wn[cb0 + n * S] = TW(wv[n]);
+      if (a.dtpart)
+        dtmin = dmin(dtmin, cell_dt(a.gamma, wv, bcc1, bcc2, bcc3, s_b1[ty][tx], s_b2[ty][tx],
+                                    b3_lo, dx));
+    }
+    b3_lo = b3_hi;
+  }
+  if (floored) atomicOr(a.floor_flag, 1);
+  if (a.dtpart) {
+    double m = wg_min256(dtmin, s_red);
+    if (tid == 0) a.dtpart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = m;
   }
 }
 
@@ -1203,11 +1379,13 @@ k_ghost_faces(Geo g, Shell s1, Shell s2, Shell s3, double *b1, double *b2, doubl
       a[(int64_t)src * g.S + IDX(g, k - o3 * g.bx3, j - o2 * g.bx2, i - o1 * g.bx1)];
 }
 
-// Ghost cells: copy the neighbour's (un-floored) conserved state, bcc from this block's
-// faces (ghost faces already exchanged), C2P with floors written to the ghost copy.
+// Ghost cells (bvals_cc.cpp:300-336 followed by adiabatic_mhd.cpp:41-90 over the ghost
+// zones): copy the neighbour's un-floored conserved state, bcc from this block's faces
+// (ghost faces already exchanged), C2P with floors written to the ghost copy.
+template <typename TW>
 __global__ void __launch_bounds__(256)
-k_ghost_cells(Geo g, Shell s, double *__restrict__ u, double *__restrict__ w,
-              double *__restrict__ bcc, const double *__restrict__ b1,
+k_ghost_cells(Geo g, Shell s, double *__restrict__ u, TW *__restrict__ w,
+              TW *__restrict__ bcc, const double *__restrict__ b1,
               const double *__restrict__ b2, const double *__restrict__ b3, double gm1,
               double dfl, double pfl) {
   int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1229,9 +1407,9 @@ k_ghost_cells(Geo g, Shell s, double *__restrict__ u, double *__restrict__ w,
   const double bcc2 = 0.5 * b2[fo] + 0.5 * b2[fo + g.P];
   const double bcc3 = 0.5 * b3[fo] + 0.5 * b3[fo + (int64_t)g.J * g.P];
   const int64_t bo = (int64_t)b * 3 * S + id;
-  bcc[bo] = bcc1;
-  bcc[bo + S] = bcc2;
-  bcc[bo + 2 * S] = bcc3;
+  bcc[bo] = TW(bcc1);
+  bcc[bo + S] = TW(bcc2);
+  bcc[bo + 2 * S] = TW(bcc3);
   double wv[5];
   c2p_cell(u_d, u_m1, u_m2, u_m3, u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
   u[c] = u_d;
@@ -1240,13 +1418,15 @@ k_ghost_cells(Geo g, Shell s, double *__restrict__ u, double *__restrict__ w,
   u[c + 3 * S] = u_m3;
   u[c + 4 * S] = u_e;
 #pragma unroll
-  for (int n = 0; n < NHYDRO; ++n) w[c + n * S] = wv[n];
+  for (int n = 0; n < NHYDRO; ++n) w[c + n * S] = TW(wv[n]);
 }
 
 // Write the interior floors back to u (ConservedToPrimitive's in-place update) once
-// the ghost copies are taken; a no-op unless k_integrate_c2p saw a floor.
+// the ghost copies are taken; a no-op unless the stage kernel saw a floor.  bcc is
+// recomputed from the faces with the formula of field.cpp:112-180.
 __global__ void __launch_bounds__(256)
-k_floor_fix(Geo g, double *__restrict__ u, const double *__restrict__ bcc, double gm1,
+k_floor_fix(Geo g, double *__restrict__ u, const double *__restrict__ b1,
+            const double *__restrict__ b2, const double *__restrict__ b3, double gm1,
             double dfl, double pfl, const int *__restrict__ floor_flag) {
   if (*floor_flag == 0) return;
   PlaneIdx p;
@@ -1254,22 +1434,24 @@ k_floor_fix(Geo g, double *__restrict__ u, const double *__restrict__ bcc, doubl
   const int64_t S = g.S;
   const int64_t id = IDX(g, p.k, p.j, p.i);
   const int64_t c = (int64_t)p.b * NHYDRO * S + id;
-  const int64_t bo = (int64_t)p.b * 3 * S + id;
+  const int64_t fo = (int64_t)p.b * S + id;
+  const double bcc1 = 0.5 * b1[fo] + 0.5 * b1[fo + 1];
+  const double bcc2 = 0.5 * b2[fo] + 0.5 * b2[fo + g.P];
+  const double bcc3 = 0.5 * b3[fo] + 0.5 * b3[fo + (int64_t)g.J * g.P];
   double u_d = u[c], u_e = u[c + 4 * S], wv[5];
-  c2p_cell(u_d, u[c + S], u[c + 2 * S], u[c + 3 * S], u_e, bcc[bo], bcc[bo + S], bcc[bo + 2 * S],
-           gm1, dfl, pfl, wv);
+  c2p_cell(u_d, u[c + S], u[c + 2 * S], u[c + 3 * S], u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
   u[c] = u_d;
   u[c + 4 * S] = u_e;
 }
 
 __global__ void k_reset_int(int *p) { *p = 0; }
 
-// Hydro::NewBlockTimeStep (hydro/new_blockdt.cpp:57-160): per-cell dx/(|v|+c_f) per
-// direction, min over all interior cells of all blocks.  min is exact, so the reduction
-// order does not matter: wavefront shuffle, then LDS across the workgroup's 4
-// wavefronts, one partial per workgroup; k_min_final reduces the partials.
+// Hydro::NewBlockTimeStep (hydro/new_blockdt.cpp:57-160): per-cell bound, min over all
+// interior cells of all blocks, one partial per workgroup; k_min_final reduces the
+// partials (min is exact, so the reduction order does not matter).
+template <typename TW>
 __global__ void __launch_bounds__(256)
-k_newdt(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
+k_newdt(Geo g, const TW *__restrict__ w, const TW *__restrict__ bcc,
         const double *__restrict__ b1, const double *__restrict__ b2,
         const double *__restrict__ b3, const BlockCoef *__restrict__ coef, double gamma,
         double *__restrict__ part) {
@@ -1282,33 +1464,14 @@ k_newdt(Geo g, const double *__restrict__ w, const double *__restrict__ bcc,
     const int64_t c = (int64_t)p.b * NHYDRO * S + id;
     const int64_t bo = (int64_t)p.b * 3 * S + id;
     const int64_t fo = (int64_t)p.b * S + id;
-    double wi[NWAVE];
-    wi[IDN] = w[c]; wi[IVX] = w[c + S]; wi[IVY] = w[c + 2 * S]; wi[IVZ] = w[c + 3 * S];
-    wi[IPR] = w[c + 4 * S];
-    double c1 = bcc[bo], c2 = bcc[bo + S], c3 = bcc[bo + 2 * S];
-    double bx = c1 + fabs(b1[fo] - c1);
-    wi[IBY] = c2; wi[IBZ] = c3;
-    double cf = fast_speed(gamma, wi, bx);
-    double dt1 = coef[p.b].dx[0] / (fabs(wi[IVX]) + cf);
-    wi[IBY] = c3; wi[IBZ] = c1;
-    bx = c2 + fabs(b2[fo] - c2);
-    cf = fast_speed(gamma, wi, bx);
-    double dt2 = coef[p.b].dx[1] / (fabs(wi[IVY]) + cf);
-    wi[IBY] = c1; wi[IBZ] = c2;
-    bx = c3 + fabs(b3[fo] - c3);
-    cf = fast_speed(gamma, wi, bx);
-    double dt3 = coef[p.b].dx[2] / (fabs(wi[IVZ]) + cf);
-    m = dmin(m, dt1);
-    m = dmin(m, dt2);
-    m = dmin(m, dt3);
+    double wv[5];
+#pragma unroll
+    for (int n = 0; n < NHYDRO; ++n) wv[n] = w[c + n * S];
+    m = cell_dt(gamma, wv, bcc[bo], bcc[bo + S], bcc[bo + 2 * S], b1[fo], b2[fo], b3[fo],
+                coef[p.b].dx);
   }
-  for (int off = 32; off > 0; off >>= 1) m = dmin(m, __shfl_xor(m, off, 64));
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    m = dmin(dmin(sh[0], sh[1]), dmin(sh[2], sh[3]));
-    part[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
-  }
+  m = wg_min256(m, sh);
+  if (threadIdx.x == 0) part[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
 }
 
 __global__ void __launch_bounds__(256) k_min_final(const double *__restrict__ part, int64_t n,
@@ -1383,8 +1546,8 @@ int check_desc(const agx_mesh_desc *d) {
     set_error("unknown Riemann solver %d", d->rsolver); return -8;
   }
   if (d->eos != 0) { set_error("only the adiabatic EOS is supported"); return -9; }
-  if (d->precision != AGX_PREC_FP64) {
-    set_error("precision %d not supported", d->precision); return -10;
+  if (d->precision != AGX_PREC_FP64 && d->precision != AGX_PREC_MIXED) {
+    set_error("precision %d not supported (0: FP64, 1: mixed)", d->precision); return -10;
   }
   if (!(d->gamma > 1.0)) { set_error("gamma must be > 1"); return -11; }
   return 0;
@@ -1422,24 +1585,31 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   g.PL = g.PL > g.n3 ? g.PL : g.n3;
   g.J = g.n2 + 1;
   g.S = (int64_t)(g.n3 + 1) * g.J * g.P;
-  static const int nv[AGX_ARR_COUNT] = {5, 5, 5, 3, 1, 1, 1, 1, 1, 1, 5, 5, 5,
-                                        1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+  // variables per array: the AGX_ARR_* set, then W2, BCC2, Q1F, Q2F, Q3F
+  static const int nv[ARR_TOTAL] = {5, 5, 5, 3, 1, 1, 1, 1, 1, 1, 5, 5, 5,
+                                    1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                                    5, 3, 1, 1, 1};
+  m->mixed = (desc->precision == AGX_PREC_MIXED);
   int64_t total = 0;
-  for (int a = 0; a < AGX_ARR_COUNT; ++a) {
+  for (int a = 0; a < ARR_TOTAL; ++a) {
+    bool f32 = m->mixed && (a == AGX_ARR_W || a == AGX_ARR_BCC || a == ARR_W2 || a == ARR_BCC2 ||
+                            (a >= AGX_ARR_FX1 && a <= AGX_ARR_WX3));
     m->arr.nvar[a] = nv[a];
-    total += nv[a];
+    m->arr.esz[a] = f32 ? 4 : 8;
+    // every array starts on a 256-B boundary (S is a multiple of 16 elements)
+    total += (int64_t)nv[a] * g.nb * g.S * m->arr.esz[a];
   }
-  m->pool_bytes = total * g.nb * g.S * (int64_t)sizeof(double);
+  m->pool_bytes = total;
   hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { int r = hip_fail(e, "hipStreamCreate"); delete m; return r; }
   e = hipMalloc(&m->pool, m->pool_bytes);
   if (e != hipSuccess) { int r = hip_fail(e, "hipMalloc(pool)"); agx_mesh_destroy(m); return r; }
   e = hipMemsetAsync(m->pool, 0, m->pool_bytes, m->stream);
   if (e != hipSuccess) { int r = hip_fail(e, "hipMemset(pool)"); agx_mesh_destroy(m); return r; }
-  double *ptr = m->pool;
-  for (int a = 0; a < AGX_ARR_COUNT; ++a) {
-    m->arr.a[a] = ptr;
-    ptr += (int64_t)nv[a] * g.nb * g.S;
+  char *ptr = reinterpret_cast<char *>(m->pool);
+  for (int a = 0; a < ARR_TOTAL; ++a) {
+    m->arr.a[a] = reinterpret_cast<double *>(ptr);
+    ptr += (int64_t)nv[a] * g.nb * g.S * m->arr.esz[a];
   }
   // per-block geometry
   m->coef_h.resize(g.nb);
@@ -1470,9 +1640,12 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   if (e == hipSuccess) e = hipMalloc(&m->scal, sizeof(double) * 64);
   {
     dim3 gr = plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb);
+    dim3 gs = stage_grid(g);
     m->red_n = (int64_t)gr.x * gr.y * gr.z;
+    m->dtpart_n = (int64_t)gs.x * gs.y;
   }
   if (e == hipSuccess) e = hipMalloc(&m->red, sizeof(double) * m->red_n);
+  if (e == hipSuccess) e = hipMalloc(&m->dtpart, sizeof(double) * m->dtpart_n);
   if (e == hipSuccess) e = hipHostMalloc(&m->scal_host, sizeof(double) * 64, hipHostMallocDefault);
   if (e == hipSuccess)
     e = hipMemcpyAsync(m->coef, m->coef_h.data(), sizeof(BlockCoef) * g.nb, hipMemcpyHostToDevice, m->stream);
@@ -1501,6 +1674,7 @@ int agx_mesh_destroy(agx_mesh *m) {
   if (m->scal) hipFree(m->scal);
   if (m->scal_host) hipHostFree(m->scal_host);
   if (m->red) hipFree(m->red);
+  if (m->dtpart) hipFree(m->dtpart);
   if (m->stream) hipStreamDestroy(m->stream);
   delete m;
   return 0;
@@ -1538,21 +1712,30 @@ int agx_array_dims(agx_mesh *m, int arr, int64_t *nblocks, int64_t dims[4]) {
 
 int64_t agx_mesh_bytes(agx_mesh *m) { return m ? m->pool_bytes + m->mg_bytes : 0; }
 
-// Strided 3D copies between the tight host layout and the padded device layout.
+// Strided 3D copies between the tight host layout and the padded device layout.  FP32
+// arrays of the mixed-precision form are staged through a host buffer and converted.
 static int copy_array(agx_mesh *m, int arr, double *host, bool to_device) {
   if (!m || arr < 0 || arr >= AGX_ARR_COUNT || !host) { set_error("bad argument"); return -1; }
   const Geo &g = m->g;
   int64_t dims[4];
   host_dims(m, arr, dims);
-  int64_t nslab = (int64_t)g.nb * dims[0];
+  const int esz = m->arr.esz[arr];
+  const int64_t nslab = (int64_t)g.nb * dims[0];
+  const int64_t nslab_el = dims[1] * dims[2] * dims[3];
+  std::vector<float> stage;
+  char *hbase = reinterpret_cast<char *>(host);
+  if (esz == 4) {
+    stage.resize((size_t)(nslab * nslab_el));
+    if (to_device)
+      for (size_t q = 0; q < stage.size(); ++q) stage[q] = (float)host[q];
+    hbase = reinterpret_cast<char *>(stage.data());
+  }
+  char *dbase = reinterpret_cast<char *>(m->arr.a[arr]);
   hipMemcpy3DParms p = {};
-  hipPitchedPtr hp = make_hipPitchedPtr(host, dims[3] * sizeof(double), dims[3], dims[2]);
-  hipPitchedPtr dp = make_hipPitchedPtr(m->arr.a[arr], g.P * sizeof(double), g.P, g.J);
   for (int64_t s = 0; s < nslab; ++s) {
-    hipPitchedPtr hs = hp, ds = dp;
-    hs.ptr = host + s * dims[1] * dims[2] * dims[3];
-    ds.ptr = m->arr.a[arr] + s * g.S;
-    p.extent = make_hipExtent(dims[3] * sizeof(double), dims[2], dims[1]);
+    hipPitchedPtr hs = make_hipPitchedPtr(hbase + s * nslab_el * esz, dims[3] * esz, dims[3], dims[2]);
+    hipPitchedPtr ds = make_hipPitchedPtr(dbase + s * g.S * esz, g.P * esz, g.P, g.J);
+    p.extent = make_hipExtent(dims[3] * esz, dims[2], dims[1]);
     if (to_device) { p.srcPtr = hs; p.dstPtr = ds; p.kind = hipMemcpyHostToDevice; }
     else { p.srcPtr = ds; p.dstPtr = hs; p.kind = hipMemcpyDeviceToHost; }
     p.srcPos = make_hipPos(0, 0, 0);
@@ -1560,6 +1743,8 @@ static int copy_array(agx_mesh *m, int arr, double *host, bool to_device) {
     AGX_HIP(hipMemcpy3DAsync(&p, m->stream));
   }
   AGX_HIP(hipStreamSynchronize(m->stream));
+  if (esz == 4 && !to_device)
+    for (size_t q = 0; q < stage.size(); ++q) host[q] = (double)stage[q];
   return 0;
 }
 
@@ -1600,12 +1785,16 @@ int agx_calculate_fluxes(agx_mesh *m, int order) {
   double **A = m->arr.a;
   const double gamma = m->d.gamma, dt = m->dt;
   const int rs = m->d.rsolver;
-#define FLUX_LAUNCH(DIR, ORD, RS, B, F, EY, EZ, WX)                                            \
-  hipLaunchKernelGGL((k_flux<DIR, ORD, RS>),                                                \
+#define FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, TW)                                      \
+  hipLaunchKernelGGL((k_flux<DIR, ORD, RS, TW>),                                            \
                      plane_grid(g.is - (DIR != 0), g.ie + 1, g.js - (DIR != 1), g.je + 1,    \
                                 g.ks - (DIR != 2), g.ke + 1, g.nb),                          \
-                     dim3(256), 0, m->stream, g, A[AGX_ARR_W], A[AGX_ARR_BCC], A[B], A[F],   \
-                     A[EY], A[EZ], A[WX], m->coef, m->plm, gamma, dt, m->d.dfloor, m->d.pfloor)
+                     dim3(256), 0, m->stream, g, as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]), \
+                     A[B], as<TW>(A[F]), as<TW>(A[EY]), as<TW>(A[EZ]), as<TW>(A[WX]), m->coef, \
+                     m->plm, gamma, dt, m->d.dfloor, m->d.pfloor)
+#define FLUX_LAUNCH(DIR, ORD, RS, B, F, EY, EZ, WX)                                            \
+  if (m->mixed) FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, float);                         \
+  else FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, double)
 #define FLUX_ORD(DIR, ORD, B, F, EY, EZ, WX)                                                    \
   if (rs == AGX_RSOLVER_HLLE) FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_HLLE, B, F, EY, EZ, WX);        \
   else if (rs == AGX_RSOLVER_ROE) FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_ROE, B, F, EY, EZ, WX);     \
@@ -1629,11 +1818,13 @@ int agx_calculate_fluxes(agx_mesh *m, int order) {
 #undef FLUX_DIR
 #undef FLUX_ORD
 #undef FLUX_LAUNCH
+#undef FLUX_LAUNCH_T
   return 0;
 }
 
 int agx_compute_corner_e(agx_mesh *m) {
   if (!m) { set_error("null mesh"); return -1; }
+  if (m->mixed) { set_error("the task-level corner-EMF entry point runs in FP64 only"); return -2; }
   const Geo &g = m->g;
   double **A = m->arr.a;
   timer_begin(m, AGX_K_CORNER_E);
@@ -1650,6 +1841,7 @@ int agx_compute_corner_e(agx_mesh *m) {
 int agx_integrate(agx_mesh *m, int stage) {
   if (!m) { set_error("null mesh"); return -1; }
   if (stage != 1 && stage != 2) { set_error("VL2 has stages 1 and 2, got %d", stage); return -2; }
+  if (m->mixed) { set_error("the task-level integrate entry point runs in FP64 only"); return -3; }
   const Geo &g = m->g;
   double **A = m->arr.a;
   // VL2 stage weights, time_integrator.cpp:199-234: beta = 0.5, 1.0
@@ -1708,48 +1900,78 @@ int agx_bvals_exchange(agx_mesh *m, int which) {
   return 0;
 }
 
-int agx_cons2prim(agx_mesh *m) {
-  if (!m) { set_error("null mesh"); return -1; }
+extern "C++" {
+template <typename TW>
+static void launch_cons2prim(agx_mesh *m) {
   const Geo &g = m->g;
   double **A = m->arr.a;
+  hipLaunchKernelGGL(k_cons2prim<TW>, plane_grid(0, g.n1 - 1, 0, g.n2 - 1, 0, g.n3 - 1, g.nb), dim3(256),
+                     0, m->stream, g, A[AGX_ARR_U], as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]),
+                     A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F], m->d.gamma - 1.0, m->d.dfloor,
+                     m->d.pfloor);
+}
+}  // extern "C++"
+
+int agx_cons2prim(agx_mesh *m) {
+  if (!m) { set_error("null mesh"); return -1; }
   timer_begin(m, AGX_K_C2P);
-  hipLaunchKernelGGL(k_cons2prim, plane_grid(0, g.n1 - 1, 0, g.n2 - 1, 0, g.n3 - 1, g.nb), dim3(256), 0,
-                     m->stream, g, A[AGX_ARR_U], A[AGX_ARR_W], A[AGX_ARR_BCC], A[AGX_ARR_B1F],
-                     A[AGX_ARR_B2F], A[AGX_ARR_B3F], m->d.gamma - 1.0, m->d.dfloor, m->d.pfloor);
+  if (m->mixed) launch_cons2prim<float>(m);
+  else launch_cons2prim<double>(m);
   timer_end(m);
   LAUNCH_CHECK("k_cons2prim");
   return 0;
 }
 
-int agx_new_block_dt(agx_mesh *m, double *min_dt) {
-  if (!m) { set_error("null mesh"); return -1; }
-  const Geo &g = m->g;
-  double **A = m->arr.a;
-  dim3 grid = plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb);
-  int64_t nwg = (int64_t)grid.x * grid.y * grid.z;
-  timer_begin(m, AGX_K_NEWDT);
-  hipLaunchKernelGGL(k_newdt, grid, dim3(256), 0, m->stream, g, A[AGX_ARR_W], A[AGX_ARR_BCC],
-                     A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F], m->coef, m->d.gamma, m->red);
-  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(256), 0, m->stream, m->red, nwg, m->scal);
-  timer_end(m);
-  LAUNCH_CHECK("k_newdt");
+// min over the nwg partials in part -> host; returns min_dt_hyperbolic * cfl
+// (new_blockdt.cpp:158).
+static int finish_block_dt(agx_mesh *m, const double *part, int64_t nwg, double *min_dt) {
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(256), 0, m->stream, part, nwg, m->scal);
+  LAUNCH_CHECK("k_min_final");
   AGX_HIP(hipMemcpyAsync(m->scal_host, m->scal, sizeof(double), hipMemcpyDeviceToHost, m->stream));
   AGX_HIP(hipStreamSynchronize(m->stream));
-  // min_dt_hyperbolic *= cfl_number, new_blockdt.cpp:158
   double v = m->scal_host[0] * m->d.cfl;
   if (min_dt) *min_dt = v;
   return 0;
+}
+
+extern "C++" {
+template <typename TW>
+static void launch_newdt(agx_mesh *m, dim3 grid) {
+  const Geo &g = m->g;
+  double **A = m->arr.a;
+  hipLaunchKernelGGL(k_newdt<TW>, grid, dim3(256), 0, m->stream, g, as<TW>(A[AGX_ARR_W]),
+                     as<TW>(A[AGX_ARR_BCC]), A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F], m->coef,
+                     m->d.gamma, m->red);
+}
+}  // extern "C++"
+
+int agx_new_block_dt(agx_mesh *m, double *min_dt) {
+  if (!m) { set_error("null mesh"); return -1; }
+  const Geo &g = m->g;
+  dim3 grid = plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb);
+  int64_t nwg = (int64_t)grid.x * grid.y * grid.z;
+  timer_begin(m, AGX_K_NEWDT);
+  if (m->mixed) launch_newdt<float>(m, grid);
+  else launch_newdt<double>(m, grid);
+  LAUNCH_CHECK("k_newdt");
+  int rc = finish_block_dt(m, m->red, nwg, min_dt);
+  timer_end(m);
+  return rc;
+}
+
+// Mesh::NewTimeStep, mesh/mesh.cpp:1083-1107
+static void set_new_dt(agx_mesh *m, double bdt, double tlim) {
+  double dt = 2.0 * m->dt;
+  dt = (bdt < dt) ? bdt : dt;
+  if (m->time < tlim && (tlim - m->time) < dt) dt = tlim - m->time;
+  m->dt = dt;
 }
 
 int agx_new_time_step(agx_mesh *m, double tlim) {
   double bdt = 0.0;
   int rc = agx_new_block_dt(m, &bdt);
   if (rc) return rc;
-  // Mesh::NewTimeStep, mesh/mesh.cpp:1083-1107
-  double dt = 2.0 * m->dt;
-  dt = (bdt < dt) ? bdt : dt;
-  if (m->time < tlim && (tlim - m->time) < dt) dt = tlim - m->time;
-  m->dt = dt;
+  set_new_dt(m, bdt, tlim);
   return 0;
 }
 
@@ -1764,39 +1986,89 @@ int agx_initialize(agx_mesh *m, double tlim) {
   return agx_new_time_step(m, tlim);                    // mesh.cpp:1646-1649
 }
 
-// One VL2 stage with the fused kernels: fluxes, corner EMFs, CT into the face register,
-// interior update + C2P into the conserved register, register swap, ghost faces,
-// ghost cells, interior floors.  Same results as agx_integrate + agx_bvals_exchange +
-// agx_cons2prim (time_integrator.cpp:1460-2034).
-static int stage_fused(agx_mesh *m, int stage) {
+// One VL2 stage with the fused kernels (time_integrator.cpp:1460-2034): fluxes, then
+// k_stage (corner EMFs, CT, update, C2P of the interior and, when dtpart is set, the
+// time-step partials), register rotation, ghost faces, ghost cells, interior floors.
+// Same results as agx_calculate_fluxes + agx_compute_corner_e + agx_integrate +
+// agx_bvals_exchange + agx_cons2prim.
+//
+// Registers: stage 1 reads u and b and writes u1 and b1; the swap makes (u, b) the
+// half-step state and (u1, b1) the start state.  Stage 2 updates u1 in place from the
+// fluxes of the half-step state and writes the new faces to the third register; the
+// rotation leaves b = new faces and b1 = half-step faces, as the reference's swap does.
+extern "C++" {
+template <typename TW>
+static void launch_stage(agx_mesh *m, int stage, double wght, int *flag, double *dtpart) {
+  const Geo &g = m->g;
+  double **A = m->arr.a;
+  StageArgs<TW> a;
+  a.src = (stage == 1) ? A[AGX_ARR_U] : A[AGX_ARR_U1];
+  a.dst = A[AGX_ARR_U1];
+  a.rho = A[AGX_ARR_U];
+  a.phi = A[AGX_ARR_PHI];
+  a.f1 = as<TW>(A[AGX_ARR_FX1]); a.f2 = as<TW>(A[AGX_ARR_FX2]); a.f3 = as<TW>(A[AGX_ARR_FX3]);
+  a.e3x1 = as<TW>(A[AGX_ARR_E3X1]); a.e2x1 = as<TW>(A[AGX_ARR_E2X1]);
+  a.e1x2 = as<TW>(A[AGX_ARR_E1X2]); a.e3x2 = as<TW>(A[AGX_ARR_E3X2]);
+  a.e1x3 = as<TW>(A[AGX_ARR_E1X3]); a.e2x3 = as<TW>(A[AGX_ARR_E2X3]);
+  a.wx1 = as<TW>(A[AGX_ARR_WX1]); a.wx2 = as<TW>(A[AGX_ARR_WX2]); a.wx3 = as<TW>(A[AGX_ARR_WX3]);
+  a.w = as<TW>(A[AGX_ARR_W]); a.bcc = as<TW>(A[AGX_ARR_BCC]);
+  a.wn = as<TW>(A[ARR_W2]); a.bccn = as<TW>(A[ARR_BCC2]);
+  const int src = (stage == 1) ? AGX_ARR_B1F : AGX_ARR_R1F;
+  const int dst = (stage == 1) ? AGX_ARR_R1F : ARR_Q1F;
+  a.s1 = A[src]; a.s2 = A[src + 1]; a.s3 = A[src + 2];
+  a.d1 = A[dst]; a.d2 = A[dst + 1]; a.d3 = A[dst + 2];
+  a.coef = m->coef;
+  a.wght = wght;
+  a.gm1 = m->d.gamma - 1.0;
+  a.dfl = m->d.dfloor;
+  a.pfl = m->d.pfloor;
+  a.gamma = m->d.gamma;
+  a.grav = m->d.self_gravity ? 1 : 0;
+  a.floor_flag = flag;
+  a.dtpart = dtpart;
+  hipLaunchKernelGGL((k_stage<STAGE_TI, STAGE_TJ, TW>), stage_grid(g), dim3(STAGE_TI * STAGE_TJ), 0,
+                     m->stream, g, a);
+}
+}  // extern "C++"
+
+extern "C++" {
+template <typename TW>
+static void launch_ghost_cells(agx_mesh *m, const Shell &cs) {
+  const Geo &g = m->g;
+  double **A = m->arr.a;
+  hipLaunchKernelGGL(k_ghost_cells<TW>, dim3(grid1d(cs.nk + cs.nj + cs.ni, 256), 1, g.nb), dim3(256), 0,
+                     m->stream, g, cs, A[AGX_ARR_U], as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]),
+                     A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F], m->d.gamma - 1.0, m->d.dfloor,
+                     m->d.pfloor);
+}
+}  // extern "C++"
+
+static int stage_fused(agx_mesh *m, int stage, bool want_dt) {
   const Geo &g = m->g;
   double **A = m->arr.a;
   int rc = agx_calculate_fluxes(m, stage == 1 ? 1 : m->d.xorder);
-  if (!rc) rc = agx_compute_corner_e(m);
   if (rc) return rc;
   const double wght = (stage == 1 ? 0.5 : 1.0) * m->dt;
-  const int uin = (stage == 1) ? AGX_ARR_U : AGX_ARR_U1;
-  const int bsrc = (stage == 1) ? AGX_ARR_B1F : AGX_ARR_R1F;
-  const double gm1 = m->d.gamma - 1.0, dfl = m->d.dfloor, pfl = m->d.pfloor;
   int *flag = reinterpret_cast<int *>(m->scal + 8);
   timer_begin(m, AGX_K_INTEGRATE);
-  {
-    int64_t n = (int64_t)(g.bx1 + 1) * (g.bx2 + 1) * (g.bx3 + 1);
-    hipLaunchKernelGGL(k_ct3, dim3(grid1d(n, 256), 3, g.nb), dim3(256), 0, m->stream, g, A[bsrc],
-                       A[bsrc + 1], A[bsrc + 2], A[AGX_ARR_R1F], A[AGX_ARR_R2F], A[AGX_ARR_R3F],
-                       A[AGX_ARR_E1], A[AGX_ARR_E2], A[AGX_ARR_E3], m->coef, wght);
-  }
   hipLaunchKernelGGL(k_reset_int, dim3(1), dim3(1), 0, m->stream, flag);
-  hipLaunchKernelGGL(k_integrate_c2p, plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb), dim3(256), 0,
-                     m->stream, g, A[uin], A[AGX_ARR_U1], A[AGX_ARR_FX1], A[AGX_ARR_FX2], A[AGX_ARR_FX3],
-                     A[AGX_ARR_W], A[AGX_ARR_BCC], A[AGX_ARR_PHI], A[AGX_ARR_R1F], A[AGX_ARR_R2F],
-                     A[AGX_ARR_R3F], m->coef, wght, m->d.self_gravity ? 1 : 0, gm1, dfl, pfl, flag);
+  if (m->mixed) launch_stage<float>(m, stage, wght, flag, want_dt ? m->dtpart : nullptr);
+  else launch_stage<double>(m, stage, wght, flag, want_dt ? m->dtpart : nullptr);
   timer_end(m);
-  LAUNCH_CHECK("k_integrate_c2p");
+  LAUNCH_CHECK("k_stage");
   std::swap(A[AGX_ARR_U], A[AGX_ARR_U1]);
-  std::swap(A[AGX_ARR_B1F], A[AGX_ARR_R1F]);
-  std::swap(A[AGX_ARR_B2F], A[AGX_ARR_R2F]);
-  std::swap(A[AGX_ARR_B3F], A[AGX_ARR_R3F]);
+  std::swap(A[AGX_ARR_W], A[ARR_W2]);
+  std::swap(A[AGX_ARR_BCC], A[ARR_BCC2]);
+  for (int d = 0; d < 3; ++d) {
+    if (stage == 1) {
+      std::swap(A[AGX_ARR_B1F + d], A[AGX_ARR_R1F + d]);
+    } else {
+      double *half = A[AGX_ARR_B1F + d], *start = A[AGX_ARR_R1F + d];
+      A[AGX_ARR_B1F + d] = A[ARR_Q1F + d];
+      A[AGX_ARR_R1F + d] = half;
+      A[ARR_Q1F + d] = start;
+    }
+  }
   Shell f1 = make_shell(g.n1 + 1, g.n2, g.n3, g.is, g.ie + 1, g.js, g.je, g.ks, g.ke);
   Shell f2 = make_shell(g.n1, g.n2 + 1, g.n3, g.is, g.ie, g.js, g.je + 1, g.ks, g.ke);
   Shell f3 = make_shell(g.n1, g.n2, g.n3 + 1, g.is, g.ie, g.js, g.je, g.ks, g.ke + 1);
@@ -1805,14 +2077,14 @@ static int stage_fused(agx_mesh *m, int stage) {
   timer_begin(m, AGX_K_BVALS);
   hipLaunchKernelGGL(k_ghost_faces, dim3(grid1d(nf, 256), 3, g.nb), dim3(256), 0, m->stream, g, f1, f2, f3,
                      A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F]);
-  hipLaunchKernelGGL(k_ghost_cells, dim3(grid1d(cs.nk + cs.nj + cs.ni, 256), 1, g.nb), dim3(256), 0,
-                     m->stream, g, cs, A[AGX_ARR_U], A[AGX_ARR_W], A[AGX_ARR_BCC], A[AGX_ARR_B1F],
-                     A[AGX_ARR_B2F], A[AGX_ARR_B3F], gm1, dfl, pfl);
+  if (m->mixed) launch_ghost_cells<float>(m, cs);
+  else launch_ghost_cells<double>(m, cs);
   timer_end(m);
   LAUNCH_CHECK("k_ghost");
   timer_begin(m, AGX_K_C2P);
   hipLaunchKernelGGL(k_floor_fix, plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb), dim3(256), 0,
-                     m->stream, g, A[AGX_ARR_U], A[AGX_ARR_BCC], gm1, dfl, pfl, flag);
+                     m->stream, g, A[AGX_ARR_U], A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F],
+                     m->d.gamma - 1.0, m->d.dfloor, m->d.pfloor, flag);
   timer_end(m);
   LAUNCH_CHECK("k_floor_fix");
   return 0;
@@ -1822,17 +2094,23 @@ int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
   if (!m) { set_error("null mesh"); return -1; }
   int32_t n = 0;
   int rc = 0;
+  const dim3 sg = stage_grid(m->g);
   while (n < ncycles && m->time < tlim) {
     for (int stage = 1; stage <= 2 && !rc; ++stage) {
-      // VL2 stage 1 is donor cell (time_integrator.cpp:1466-1467), inside stage_fused
-      rc = stage_fused(m, stage);
+      // VL2 stage 1 is donor cell (time_integrator.cpp:1466-1467), inside stage_fused;
+      // the last stage also leaves the NewBlockTimeStep partials in m->dtpart
+      rc = stage_fused(m, stage, stage == 2);
       if (!rc && m->d.self_gravity) rc = mg_solve(m, stage);   // main.cpp:464-465
     }
     if (rc) break;
     m->ncycle++;
     m->time += m->dt;
-    rc = agx_new_time_step(m, tlim);
+    double bdt = 0.0;
+    timer_begin(m, AGX_K_NEWDT);
+    rc = finish_block_dt(m, m->dtpart, (int64_t)sg.x * sg.y, &bdt);
+    timer_end(m);
     if (rc) break;
+    set_new_dt(m, bdt, tlim);
     ++n;
   }
   if (done) *done = n;

@@ -3,6 +3,8 @@
 Workload (value): 256^3 decaying MHD turbulence, 64 MeshBlocks of 64^3, VL2 + PLM +
 HLLD + CT, plain FP64, synthetic seeded initial data.  One step = one full VL2 cycle
 (both stages, ghost exchange, cons->prim, time-step reduction), inputs resident in HBM.
+The same workload in the mixed-precision form (FP32 reconstruction and Riemann fluxes,
+FP64 update, C2P and CT) is reported under "mixed".
 Also reported: V(1,1) cycles/s of the FP64 multigrid on the 256^3 periodic
 32-blob self-gravity problem, the dominant kernel's roofline, and the CPU oracle
 timed on a bounded sample of the same workload.
@@ -38,42 +40,46 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# Algorithmic HBM bytes per unit for each kernel class (FP64 words x 8 B); the unit
-# counts per launch are computed from the mesh geometry below.
-#   flux_x?   per face: read 5 prim + 2 transverse bcc + 1 normal face field,
-#             write 5 fluxes + 2 face EMFs + 1 CT weight            -> 16 words
-#   corner_e  per edge point (k,j,i): read 3 v + 3 bcc + 6 face EMFs + 3 weights,
-#             write 3 edge EMFs                                      -> 18 words
-#   c2p       per cell (interior + ghosts): read 5 cons + 3 faces, write 5 prim +
-#             3 bcc + 2 cons (floors)                                -> 18 words
-#   newdt     per interior cell: read 5 prim + 3 bcc + 3 faces       -> 11 words
-#   mg_smooth per cell of one colour sweep: read u, src, write u (whole level) -> 3 words
-BYTES_PER_UNIT = {"flux_x1": 128, "flux_x2": 128, "flux_x3": 128, "corner_e": 144,
-                  "c2p": 144, "newdt": 88}
+# Algorithmic HBM bytes per unit for each kernel class: compulsory words x word size,
+# for the plain FP64 form and the mixed form (w, bcc, fluxes, face EMFs and CT weights
+# stored in FP32; conserved state and face field FP64).  Unit counts per launch come from
+# the mesh geometry below.
+#   flux_x?    per face: read 5 prim + 2 transverse bcc (TW) + 1 normal face field (FP64),
+#              write 5 fluxes + 2 face EMFs + 1 CT weight (TW)
+#   integrate  per interior cell (k_stage): read 15 fluxes + 6 face EMFs + 3 weights +
+#              3 v + 3 bcc (TW) + 5 conserved + 3 CT-source faces (FP64); write 5 conserved
+#              + 3 faces (FP64) + 5 prim + 3 bcc (TW)
+BYTES_PER_UNIT = {
+    "fp64": {"flux_x1": 128, "flux_x2": 128, "flux_x3": 128, "integrate": 432},
+    "mixed": {"flux_x1": 68, "flux_x2": 68, "flux_x3": 68, "integrate": 280},
+}
+# whole VL2 cycle of a fully fused form, per zone (85 words of compulsory traffic):
+# 680 B in FP64, 552 B with w and bcc in FP32
+CYCLE_BYTES = {"fp64": 680.0, "mixed": 552.0}
 
 
 def units_per_launch(desc, cls):
     nb = (desc.nx1 // desc.bnx1) * (desc.nx2 // desc.bnx2) * (desc.nx3 // desc.bnx3)
-    b1, b2, b3, ng = desc.bnx1, desc.bnx2, desc.bnx3, desc.nghost
+    b1, b2, b3 = desc.bnx1, desc.bnx2, desc.bnx3
     if cls == "flux_x1":
         return nb * (b1 + 1) * (b2 + 2) * (b3 + 2)
     if cls == "flux_x2":
         return nb * (b1 + 2) * (b2 + 1) * (b3 + 2)
     if cls == "flux_x3":
         return nb * (b1 + 2) * (b2 + 2) * (b3 + 1)
-    if cls == "corner_e":
-        return nb * (b1 + 1) * (b2 + 1) * (b3 + 1)
-    if cls == "c2p":
-        return nb * (b1 + 2 * ng) * (b2 + 2 * ng) * (b3 + 2 * ng)
-    if cls == "newdt":
+    if cls == "integrate":
         return nb * b1 * b2 * b3
     raise KeyError(cls)
 
 
-def mhd_bench(agx, pgen, args):
+KCLASSES = ("flux_x1", "flux_x2", "flux_x3", "corner_e", "integrate", "bvals", "c2p", "newdt")
+
+
+def mhd_bench(agx, pgen, args, precision):
     N, B = args.n, args.block
     desc = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4,
-                         gamma=5.0 / 3.0, xorder=2, rsolver=0)
+                         gamma=5.0 / 3.0, xorder=2, rsolver=0,
+                         precision=1 if precision == "mixed" else 0)
     t0 = time.time()
     U, b1, b2, b3 = pgen.turbulence(desc, seed=2, mach=2.0, bx=0.1)
     m = agx.Mesh(desc)
@@ -85,7 +91,7 @@ def mhd_bench(agx, pgen, args):
     m.set("b3f", f3)
     del f1, f2, f3, b1, b2, b3
     m.initialize()
-    log(f"[bench] MHD {N}^3 in {desc.nx1 // B}^3 blocks of {B}^3 set up in {time.time() - t0:.1f}s, "
+    log(f"[bench] {precision} MHD {N}^3 in {desc.nx1 // B}^3 blocks of {B}^3 set up in {time.time() - t0:.1f}s, "
         f"device {m.device_bytes() / 1e9:.2f} GB")
     m.cycle(1e30, args.warmup)
     m.sync()
@@ -96,20 +102,37 @@ def mhd_bench(agx, pgen, args):
     assert done == args.steps
     sec = t1 - t0
     zc = float(N) ** 3 * args.steps / sec
-    log(f"[bench] {args.steps} cycles in {sec * 1e3:.1f} ms -> {zc:.4e} zone-cycles/s")
+    log(f"[bench] {precision}: {args.steps} cycles in {sec * 1e3:.1f} ms -> {zc:.4e} zone-cycles/s")
     # instrumented pass: HIP events on the mesh stream around every launch
     m.timing(True)
     m.timing_reset()
     m.cycle(1e30, 2)
     m.sync()
     per = {}
-    for cls in ("flux_x1", "flux_x2", "flux_x3", "corner_e", "integrate", "bvals", "c2p", "newdt"):
+    for cls in KCLASSES:
         ms, n = m.timing_get(cls)
         per[cls] = (ms, n)
     m.timing(False)
-    t_state = m.time()
+    state = None
+    if args.compare:
+        ng = desc.nghost
+        sl = (slice(None), slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
+        state = (m.time(), m.get("w")[sl].astype(np.float64))
     m.close()
-    return desc, zc, sec * 1e3 / args.steps, per, t_state
+    return desc, zc, sec * 1e3 / args.steps, per, state
+
+
+def roofline(desc, per, precision):
+    """Dominant kernel (among those with an algorithmic byte count) and its HBM rate."""
+    table = BYTES_PER_UNIT[precision]
+    dom = max(table, key=lambda c: per[c][0])
+    ms_tot, n_l = per[dom]
+    avg_s = ms_tot / max(n_l, 1) / 1e3
+    alg = table[dom] * units_per_launch(desc, dom)
+    achieved = alg / avg_s
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+            "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": alg}
 
 
 def mg_bench(agx, pgen, args):
@@ -152,8 +175,8 @@ def mg_bench(agx, pgen, args):
 
 def cpu_baseline(args):
     """The CPU oracle (FP64 restatement, OpenMP over MeshBlocks) on a bounded sample of
-    the same workload: a 64^3 periodic sub-box of the turbulence problem in 8 MeshBlocks
-    of 32^3, timed over a fixed number of VL2 cycles."""
+    the same workload: a 64^3 periodic sub-box of the turbulence problem in 16
+    MeshBlocks of 32x32x16, timed over a fixed number of VL2 cycles."""
     sys.path.insert(0, ROOT)
     from oracle.orc import Oracle, make_desc
     agx, pgen = load_pkg()
@@ -194,22 +217,19 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mg", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--compare", action="store_true",
+                    help="download w after both runs and report the mixed-vs-FP64 difference")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per run); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     agx, pgen = load_pkg()
-    desc, zc, ms_step, per, tstate = mhd_bench(agx, pgen, args)
-    # dominant kernel among those with a defined algorithmic byte count
-    dom = max(BYTES_PER_UNIT, key=lambda c: per[c][0])
-    ms_tot, n_l = per[dom]
-    avg_s = ms_tot / max(n_l, 1) / 1e3
-    alg = BYTES_PER_UNIT[dom] * units_per_launch(desc, dom)
-    achieved = alg / avg_s
-    total_ms = sum(v[0] for v in per.values())
-    log("[bench] per-cycle kernel time: " + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in per.items())
-        + f" (sum {total_ms / 2:.2f} ms/cycle)")
+    desc, zc, ms_step, per, st64 = mhd_bench(agx, pgen, args, "fp64")
+    rf = roofline(desc, per, "fp64")
+    log("[bench] fp64 per-cycle kernel time: " + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in per.items())
+        + f" (sum {sum(v[0] for v in per.values()) / 2:.2f} ms/cycle)")
     out = {
         "metric": "MHD zone-cycles/s + MG V-cycles/s at 256^3 on 1 MI355X; % of HBM roofline",
         "value": zc,
@@ -227,12 +247,29 @@ def main():
                                f"{(args.n // args.block) ** 3} MeshBlocks of {args.block}^3, "
                                f"VL2+PLM+HLLD+CT, plain FP64, one step = one VL2 cycle",
                    "mesh": [args.n] * 3, "meshblock": [args.block] * 3, "nghost": 2},
-        "cycle_roofline_frac": zc * 680.0 / HBM_PEAK,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": alg},
+        "cycle_roofline_frac": zc * CYCLE_BYTES["fp64"] / HBM_PEAK,
+        "roofline": rf,
         "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in per.items()},
     }
+    if not args.no_mixed:
+        mdesc, mzc, mms, mper, stmx = mhd_bench(agx, pgen, args, "mixed")
+        log("[bench] mixed per-cycle kernel time: " + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in mper.items())
+            + f" (sum {sum(v[0] for v in mper.values()) / 2:.2f} ms/cycle)")
+        mixed = {"value": mzc, "unit": "zone-cycles/s", "ms_per_step": mms, "vs_fp64": mzc / zc,
+                 "dtype": "f32 reconstruction+Riemann, f64 update/C2P/CT",
+                 "cycle_roofline_frac": mzc * CYCLE_BYTES["mixed"] / HBM_PEAK,
+                 "roofline": roofline(mdesc, mper, "mixed"),
+                 "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in mper.items()}}
+        if st64 is not None and stmx is not None:
+            (t64, w64), (tmx, wmx) = st64, stmx
+            mixed["same_cycle_count"] = bool(t64[2] == tmx[2])
+            mixed["rel_err_rho"] = float(np.abs(wmx[:, 0] - w64[:, 0]).max() / np.abs(w64[:, 0]).max())
+            mixed["rel_err_p"] = float(np.abs(wmx[:, 4] - w64[:, 4]).max() / np.abs(w64[:, 4]).max())
+            mixed["compare_note"] = (f"max-norm relative difference of rho and P vs the FP64 GPU path "
+                                     f"(bitwise equal to the oracle) after {int(t64[2])} cycles")
+            log(f"[bench] mixed vs fp64 after {int(t64[2])} cycles: rho {mixed['rel_err_rho']:.3e}, "
+                f"P {mixed['rel_err_p']:.3e}")
+        out["mixed"] = mixed
     if not args.no_mg:
         vps, iters, norm, t_solve, mper = mg_bench(agx, pgen, args)
         out["mg_vcycles_per_s"] = vps
