@@ -14,8 +14,12 @@ LIB = os.path.join(LIBDIR, "libathena_gamma_amd.so")
 SOURCES = ["agx_mhd.hip", "agx_mg.hip"]
 
 # -ffp-contract=off: the FP64 path rounds every expression as written (bitwise parity
-# with the CPU restatement).  Packed FP32 instructions are disabled for all HIP code.
+# with the CPU restatement).  -fno-hip-fp32-correctly-rounded-divide-sqrt: FP32 division
+# and square root (used only by the mixed-precision reconstruction/Riemann kernels) take
+# the hardware reciprocal/sqrt (about 1-2.5 ulp); FP64 code is unaffected.  Packed FP32
+# instructions are disabled for all HIP code.
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+            "-fno-hip-fp32-correctly-rounded-divide-sqrt",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-command-line-argument"]
 

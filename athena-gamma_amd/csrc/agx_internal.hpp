@@ -61,12 +61,16 @@ struct Arrays {
   int esz[ARR_TOTAL];
 };
 
-// Multigrid level data: one periodic grid per level covering the whole mesh.
-struct MGLevel {
+// Multigrid level data: one periodic grid per level covering the whole mesh.  The
+// mixed-precision form keeps a second, FP32 hierarchy for the correction V-cycles.
+template <typename T>
+struct MGLevelT {
   int n1, n2, n3;
   double dx;               // cell size used by the smoother/defect at this level
-  double *u, *src, *def;
+  T *u, *src, *def;
 };
+using MGLevel = MGLevelT<double>;
+using MGLevel32 = MGLevelT<float>;
 
 struct Timer {
   bool enabled = false;
@@ -84,6 +88,7 @@ struct agx_mesh {
   agx::Geo g;
   agx::Arrays arr;
   bool mixed = false;                // AGX_PREC_MIXED
+  int tile = 0;                      // stage-kernel tile shape (agx_autotune)
   hipStream_t stream = nullptr;
   double *pool = nullptr;            // all MHD arrays
   int64_t pool_bytes = 0;
@@ -107,7 +112,9 @@ struct agx_mesh {
   double mg_eps = -1.0, last_norm = 0.0, last_ave = 0.0;
   int32_t last_iters = 0;
   std::vector<agx::MGLevel> lev;
+  std::vector<agx::MGLevel32> lev32;   // mixed precision only
   double *mgpool = nullptr;
+  float *mgpool32 = nullptr;
   int64_t mg_bytes = 0;
   double rdx_root = 0.0, rdy_root = 0.0, rdz_root = 0.0;
   double rdx_blk = 0.0, rdy_blk = 0.0, rdz_blk = 0.0;

@@ -36,34 +36,36 @@ __device__ __forceinline__ int64_t LIX(int n1, int n2, int k, int j, int i) {
 // The reference sweeps i = il + ((color + k + j) & 1) in steps of 2 in block-local
 // indices with a one-cell ghost; with an even number of cells per block that is the set
 // of global cells with (I + J + K + color) even.  One thread per updated cell.
+template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_smooth(double *__restrict__ u, const double *__restrict__ src, int n1, int n2, int n3,
-            double dx2, double isix, int color) {
+k_mg_smooth(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3,
+            T dx2, T isix, int color) {
   int nh = (n1 + 1) / 2;
-  int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
   int64_t tot = (int64_t)nh * n2 * n3;
   if (t >= tot) return;
-  int h = (int)(t % nh);
-  int64_t r = t / nh;
-  int j = (int)(r % n2);
-  int k = (int)(r / n2);
+  int h = (int)(t % (unsigned)nh);
+  unsigned r = t / (unsigned)nh;
+  int j = (int)(r % (unsigned)n2);
+  int k = (int)(r / (unsigned)n2);
   int i = 2 * h + ((color + k + j) & 1);
   if (i >= n1) return;
   int64_t id = LIX(n1, n2, k, j, i);
-  double uc = u[id];
-  double ukp = u[LIX(n1, n2, wrapp(k, n3), j, i)], ujp = u[LIX(n1, n2, k, wrapp(j, n2), i)];
-  double uip = u[LIX(n1, n2, k, j, wrapp(i, n1))];
-  double ukm = u[LIX(n1, n2, wrapm(k, n3), j, i)], ujm = u[LIX(n1, n2, k, wrapm(j, n2), i)];
-  double uim = u[LIX(n1, n2, k, j, wrapm(i, n1))];
-  u[id] = uc - ((6.0 * uc - ukp - ujp - uip - ukm - ujm - uim) + src[id] * dx2) * isix;
+  T uc = u[id];
+  T ukp = u[LIX(n1, n2, wrapp(k, n3), j, i)], ujp = u[LIX(n1, n2, k, wrapp(j, n2), i)];
+  T uip = u[LIX(n1, n2, k, j, wrapp(i, n1))];
+  T ukm = u[LIX(n1, n2, wrapm(k, n3), j, i)], ujm = u[LIX(n1, n2, k, wrapm(j, n2), i)];
+  T uim = u[LIX(n1, n2, k, j, wrapm(i, n1))];
+  u[id] = uc - ((T(6.0) * uc - ukp - ujp - uip - ukm - ujm - uim) + src[id] * dx2) * isix;
 }
 
-__device__ __forceinline__ double defect_at(const double *u, const double *src, int n1, int n2,
-                                            int n3, int k, int j, int i, double idx2) {
+template <typename T>
+__device__ __forceinline__ T defect_at(const T *u, const T *src, int n1, int n2,
+                                       int n3, int k, int j, int i, T idx2) {
   // MGGravity::CalculateDefect, gravity/mg_gravity.cpp:283-314
   int64_t id = LIX(n1, n2, k, j, i);
-  double uc = u[id];
-  return (6.0 * uc - u[LIX(n1, n2, wrapp(k, n3), j, i)] - u[LIX(n1, n2, k, wrapp(j, n2), i)] -
+  T uc = u[id];
+  return (T(6.0) * uc - u[LIX(n1, n2, wrapp(k, n3), j, i)] - u[LIX(n1, n2, k, wrapp(j, n2), i)] -
           u[LIX(n1, n2, k, j, wrapp(i, n1))] - u[LIX(n1, n2, wrapm(k, n3), j, i)] -
           u[LIX(n1, n2, k, wrapm(j, n2), i)] - u[LIX(n1, n2, k, j, wrapm(i, n1))]) * idx2 + src[id];
 }
@@ -71,45 +73,47 @@ __device__ __forceinline__ double defect_at(const double *u, const double *src, 
 // RestrictBlock (multigrid/multigrid.cpp:331-355): defect on the fine level, 8-cell
 // average into the coarse source (Restrict, multigrid.cpp:684-724), coarse u zeroed
 // (ZeroClearData, non-FAS).  One thread per coarse cell.
+template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_defect_restrict(const double *__restrict__ u, const double *__restrict__ src,
-                     double *__restrict__ csrc, double *__restrict__ cu, int n1, int n2, int n3,
-                     double idx2) {
+k_mg_defect_restrict(const T *__restrict__ u, const T *__restrict__ src,
+                     T *__restrict__ csrc, T *__restrict__ cu, int n1, int n2, int n3,
+                     T idx2) {
   int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
-  int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)c1 * c2 * c3) return;
-  int i = (int)(t % c1);
-  int64_t r = t / c1;
-  int j = (int)(r % c2);
-  int k = (int)(r / c2);
+  int i = (int)(t % (unsigned)c1);
+  unsigned r = t / (unsigned)c1;
+  int j = (int)(r % (unsigned)c2);
+  int k = (int)(r / (unsigned)c2);
   int fi = 2 * i, fj = 2 * j, fk = 2 * k;
-  double d000 = defect_at(u, src, n1, n2, n3, fk, fj, fi, idx2);
-  double d001 = defect_at(u, src, n1, n2, n3, fk, fj, fi + 1, idx2);
-  double d010 = defect_at(u, src, n1, n2, n3, fk, fj + 1, fi, idx2);
-  double d011 = defect_at(u, src, n1, n2, n3, fk, fj + 1, fi + 1, idx2);
-  double d100 = defect_at(u, src, n1, n2, n3, fk + 1, fj, fi, idx2);
-  double d101 = defect_at(u, src, n1, n2, n3, fk + 1, fj, fi + 1, idx2);
-  double d110 = defect_at(u, src, n1, n2, n3, fk + 1, fj + 1, fi, idx2);
-  double d111 = defect_at(u, src, n1, n2, n3, fk + 1, fj + 1, fi + 1, idx2);
-  csrc[t] = 0.125 * (d000 + d001 + d010 + d011 + d100 + d101 + d110 + d111);
-  cu[t] = 0.0;
+  T d000 = defect_at(u, src, n1, n2, n3, fk, fj, fi, idx2);
+  T d001 = defect_at(u, src, n1, n2, n3, fk, fj, fi + 1, idx2);
+  T d010 = defect_at(u, src, n1, n2, n3, fk, fj + 1, fi, idx2);
+  T d011 = defect_at(u, src, n1, n2, n3, fk, fj + 1, fi + 1, idx2);
+  T d100 = defect_at(u, src, n1, n2, n3, fk + 1, fj, fi, idx2);
+  T d101 = defect_at(u, src, n1, n2, n3, fk + 1, fj, fi + 1, idx2);
+  T d110 = defect_at(u, src, n1, n2, n3, fk + 1, fj + 1, fi, idx2);
+  T d111 = defect_at(u, src, n1, n2, n3, fk + 1, fj + 1, fi + 1, idx2);
+  csrc[t] = T(0.125) * (d000 + d001 + d010 + d011 + d100 + d101 + d110 + d111);
+  cu[t] = T(0.0);
 }
 
 // Restrict (multigrid.cpp:684-724) of the source only, for RestrictFMGSource
 // (multigrid.cpp:253-262).
+template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_restrict_src(const double *__restrict__ src, double *__restrict__ csrc, int n1, int n2,
+k_mg_restrict_src(const T *__restrict__ src, T *__restrict__ csrc, int n1, int n2,
                   int n3) {
   int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
-  int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)c1 * c2 * c3) return;
-  int i = (int)(t % c1);
-  int64_t r = t / c1;
-  int j = (int)(r % c2);
-  int k = (int)(r / c2);
+  int i = (int)(t % (unsigned)c1);
+  unsigned r = t / (unsigned)c1;
+  int j = (int)(r % (unsigned)c2);
+  int k = (int)(r / (unsigned)c2);
   int fi = 2 * i, fj = 2 * j, fk = 2 * k;
 #define S(kk, jj, ii) src[LIX(n1, n2, kk, jj, ii)]
-  csrc[t] = 0.125 * (S(fk, fj, fi) + S(fk, fj, fi + 1) + S(fk, fj + 1, fi) + S(fk, fj + 1, fi + 1) +
+  csrc[t] = T(0.125) * (S(fk, fj, fi) + S(fk, fj, fi + 1) + S(fk, fj + 1, fi) + S(fk, fj + 1, fi + 1) +
                      S(fk + 1, fj, fi) + S(fk + 1, fj, fi + 1) + S(fk + 1, fj + 1, fi) +
                      S(fk + 1, fj + 1, fi + 1));
 #undef S
@@ -118,43 +122,45 @@ k_mg_restrict_src(const double *__restrict__ src, double *__restrict__ csrc, int
 // ProlongateAndCorrect (multigrid/multigrid.cpp:733-829): trilinear (27,9,3,1)/64
 // correction.  One thread per fine cell; (dk,dj,di) = -1 for the low child, +1 for the
 // high child along each axis.
+template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_prolong(const double *__restrict__ cu, double *__restrict__ u, int n1, int n2, int n3) {
-  int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+k_mg_prolong(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3) {
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)n1 * n2 * n3) return;
-  int fi = (int)(t % n1);
-  int64_t r = t / n1;
-  int fj = (int)(r % n2);
-  int fk = (int)(r / n2);
+  int fi = (int)(t % (unsigned)n1);
+  unsigned r = t / (unsigned)n1;
+  int fj = (int)(r % (unsigned)n2);
+  int fk = (int)(r / (unsigned)n2);
   int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
   int i = fi >> 1, j = fj >> 1, k = fk >> 1;
   int ip = wrap(i + ((fi & 1) ? 1 : -1), c1);
   int jp = wrap(j + ((fj & 1) ? 1 : -1), c2);
   int kp = wrap(k + ((fk & 1) ? 1 : -1), c3);
 #define C(kk, jj, ii) cu[LIX(c1, c2, kk, jj, ii)]
-  u[t] += 0.015625 * (27.0 * C(k, j, i) + C(kp, jp, ip) + 9.0 * (C(k, j, ip) + C(k, jp, i) + C(kp, j, i)) +
-                      3.0 * (C(kp, jp, i) + C(kp, j, ip) + C(k, jp, ip)));
+  u[t] += T(0.015625) * (T(27.0) * C(k, j, i) + C(kp, jp, ip) + T(9.0) * (C(k, j, ip) + C(k, jp, i) + C(kp, j, i)) +
+                         T(3.0) * (C(kp, jp, i) + C(kp, j, ip) + C(k, jp, ip)));
 #undef C
 }
 
 // FMGProlongate (multigrid/multigrid.cpp:838-1048): 27-point interpolation whose
 // weights are products of the 1-D weights (5,30,-3) (low child) or (-3,30,5) (high
 // child) over (c-1, c, c+1); summed in (k,j,i) row-major order, then / 32768.
+template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_fmg_prolong(const double *__restrict__ cu, double *__restrict__ u, int n1, int n2, int n3) {
-  int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+k_mg_fmg_prolong(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3) {
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)n1 * n2 * n3) return;
-  int fi = (int)(t % n1);
-  int64_t r = t / n1;
-  int fj = (int)(r % n2);
-  int fk = (int)(r / n2);
+  int fi = (int)(t % (unsigned)n1);
+  unsigned r = t / (unsigned)n1;
+  int fj = (int)(r % (unsigned)n2);
+  int fk = (int)(r / (unsigned)n2);
   int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
   int i = fi >> 1, j = fj >> 1, k = fk >> 1;
-  const double wlo[3] = {5.0, 30.0, -3.0}, whi[3] = {-3.0, 30.0, 5.0};
-  const double *wk = (fk & 1) ? whi : wlo;
-  const double *wj = (fj & 1) ? whi : wlo;
-  const double *wi = (fi & 1) ? whi : wlo;
-  double acc = 0.0;
+  const T wlo[3] = {T(5.0), T(30.0), T(-3.0)}, whi[3] = {T(-3.0), T(30.0), T(5.0)};
+  const T *wk = (fk & 1) ? whi : wlo;
+  const T *wj = (fj & 1) ? whi : wlo;
+  const T *wi = (fi & 1) ? whi : wlo;
+  T acc = T(0.0);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     int kk = wrap(k - 1 + a, c3);
@@ -164,30 +170,31 @@ k_mg_fmg_prolong(const double *__restrict__ cu, double *__restrict__ u, int n1, 
 #pragma unroll
       for (int e = 0; e < 3; ++e) {
         int ii = wrap(i - 1 + e, c1);
-        double term = (wk[a] * wj[b] * wi[e]) * cu[LIX(c1, c2, kk, jj, ii)];
+        T term = (wk[a] * wj[b] * wi[e]) * cu[LIX(c1, c2, kk, jj, ii)];
         acc = (a == 0 && b == 0 && e == 0) ? term : acc + term;
       }
     }
   }
-  u[t] = acc / 32768.0;
+  u[t] = acc / T(32768.0);
 }
 
-// block-partial sums (fixed order -> deterministic): mode 0 sum(a), mode 1 sum(def^2)
-template <int MODE>
+// block-partial sums in FP64 (fixed order -> deterministic): mode 0 sum(a), mode 1
+// sum(def^2)
+template <int MODE, typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_partial(const double *__restrict__ a, const double *__restrict__ src, int n1, int n2, int n3,
-             double idx2, double *__restrict__ part) {
+k_mg_partial(const T *__restrict__ a, const T *__restrict__ src, int n1, int n2, int n3,
+             T idx2, double *__restrict__ part) {
   __shared__ double sh[BS];
-  int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
   double v = 0.0;
   if (t < (int64_t)n1 * n2 * n3) {
     if (MODE == 0) {
       v = a[t];
     } else {
-      int i = (int)(t % n1);
-      int64_t r = t / n1;
-      int j = (int)(r % n2);
-      int k = (int)(r / n2);
+      int i = (int)(t % (unsigned)n1);
+      unsigned r = t / (unsigned)n1;
+      int j = (int)(r % (unsigned)n2);
+      int k = (int)(r / (unsigned)n2);
       double d = defect_at(a, src, n1, n2, n3, k, j, i, idx2);
       v = d * d;
     }
@@ -215,15 +222,55 @@ __global__ void __launch_bounds__(BS) k_mg_final(const double *__restrict__ part
   if (threadIdx.x == 0) *out = sh[0];
 }
 
-__global__ void __launch_bounds__(BS) k_mg_add(double *__restrict__ a, int64_t n,
+template <typename T>
+__global__ void __launch_bounds__(BS) k_mg_add(T *__restrict__ a, int64_t n,
                                                const double *__restrict__ val, double scale) {
-  int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (t < n) a[t] -= (*val) * scale;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  if (t < n) a[t] -= T((*val) * scale);
 }
 
-__global__ void __launch_bounds__(BS) k_mg_zero(double *__restrict__ a, int64_t n) {
-  int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (t < n) a[t] = 0.0;
+template <typename T>
+__global__ void __launch_bounds__(BS) k_mg_zero(T *__restrict__ a, int64_t n) {
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  if (t < n) a[t] = T(0.0);
+}
+
+// Mixed-precision defect correction, FP64 side.  The FP64 defect d = src - L(phi)
+// (the form of MGGravity::CalculateDefect) becomes the FP32 source of the correction
+// problem L(e) = d, and its squares are summed in FP64 for the defect norm.
+__global__ void __launch_bounds__(BS)
+k_mg_defect_to32(const double *__restrict__ u, const double *__restrict__ src, float *__restrict__ d32,
+                 int n1, int n2, int n3, double idx2, double *__restrict__ part) {
+  __shared__ double sh[BS];
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  double v = 0.0;
+  if (t < (int64_t)n1 * n2 * n3) {
+    int i = (int)(t % (unsigned)n1);
+    unsigned r = t / (unsigned)n1;
+    int j = (int)(r % (unsigned)n2);
+    int k = (int)(r / (unsigned)n2);
+    double d = defect_at(u, src, n1, n2, n3, k, j, i, idx2);
+    d32[t] = (float)d;
+    v = d * d;
+  }
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// phi += e (mode 0) or phi = e (mode 1), FP32 correction into the FP64 iterate; mode 2
+// converts an FP64 array to FP32 (src32 = src).
+__global__ void __launch_bounds__(BS)
+k_mg_mix(double *__restrict__ u, float *__restrict__ e, int64_t n, int mode) {
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  if (t >= n) return;
+  if (mode == 0) u[t] += (double)e[t];
+  else if (mode == 1) u[t] = (double)e[t];
+  else e[t] = (float)u[t];
 }
 
 // LoadSource (multigrid.cpp:194-231): src = fac * u(IDN) over every block interior,
@@ -232,12 +279,12 @@ __global__ void __launch_bounds__(BS)
 k_mg_load(Geo g, const double *__restrict__ u, const double *__restrict__ phi,
           double *__restrict__ src, double *__restrict__ mu, double fac, int load_phi) {
   int N1 = g.nb1 * g.bx1, N2 = g.nb2 * g.bx2, N3 = g.nb3 * g.bx3;
-  int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)N1 * N2 * N3) return;
-  int I = (int)(t % N1);
-  int64_t r = t / N1;
-  int J = (int)(r % N2);
-  int K = (int)(r / N2);
+  int I = (int)(t % (unsigned)N1);
+  unsigned r = t / (unsigned)N1;
+  int J = (int)(r % (unsigned)N2);
+  int K = (int)(r / (unsigned)N2);
   int b = ((K / g.bx3) * g.nb2 + (J / g.bx2)) * g.nb1 + (I / g.bx1);
   int64_t id = IDX(g, K % g.bx3 + g.ng, J % g.bx2 + g.ng, I % g.bx1 + g.ng);
   double v = u[(int64_t)b * NHYDRO * g.S + id];
@@ -249,14 +296,14 @@ k_mg_load(Geo g, const double *__restrict__ u, const double *__restrict__ phi,
 __global__ void __launch_bounds__(BS)
 k_mg_retrieve(Geo g, const double *__restrict__ mu, double *__restrict__ phi) {
   int m1 = g.bx1 + 2, m2 = g.bx2 + 2, m3 = g.bx3 + 2;
-  int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)m1 * m2 * m3 * g.nb) return;
-  int li = (int)(t % m1);
-  int64_t r = t / m1;
-  int lj = (int)(r % m2);
-  r /= m2;
-  int lk = (int)(r % m3);
-  int b = (int)(r / m3);
+  int li = (int)(t % (unsigned)m1);
+  unsigned r = t / (unsigned)m1;
+  int lj = (int)(r % (unsigned)m2);
+  r /= (unsigned)m2;
+  int lk = (int)(r % (unsigned)m3);
+  int b = (int)(r / (unsigned)m3);
   int N1 = g.nb1 * g.bx1, N2 = g.nb2 * g.bx2, N3 = g.nb3 * g.bx3;
   int lx1 = b % g.nb1, lx2 = (b / g.nb1) % g.nb2, lx3 = b / (g.nb1 * g.nb2);
   int I = wrap(lx1 * g.bx1 + li - 1, N1), J = wrap(lx2 * g.bx2 + lj - 1, N2),
@@ -268,7 +315,8 @@ k_mg_retrieve(Geo g, const double *__restrict__ mu, double *__restrict__ phi) {
 
 namespace agx {
 
-static int64_t lcells(const MGLevel &L) { return (int64_t)L.n1 * L.n2 * L.n3; }
+template <typename T>
+static int64_t lcells(const MGLevelT<T> &L) { return (int64_t)L.n1 * L.n2 * L.n3; }
 
 int mg_create(agx_mesh *m) {
   const agx_mesh_desc &d = m->d;
@@ -329,6 +377,24 @@ int mg_create(agx_mesh *m) {
     Lv.src = p; p += n;
     Lv.def = p; p += n;
   }
+  if (m->mixed) {
+    // FP32 hierarchy of the correction V-cycles: u (correction) and src per level
+    m->lev32.resize(L);
+    int64_t t32 = 0;
+    for (int l = 0; l < L; ++l) t32 += 2 * lcells(m->lev[l]);
+    AGX_HIP(hipMalloc(&m->mgpool32, t32 * (int64_t)sizeof(float)));
+    AGX_HIP(hipMemsetAsync(m->mgpool32, 0, t32 * sizeof(float), m->stream));
+    m->mg_bytes += t32 * (int64_t)sizeof(float);
+    float *q = m->mgpool32;
+    for (int l = 0; l < L; ++l) {
+      MGLevel32 &Lv = m->lev32[l];
+      Lv.n1 = m->lev[l].n1; Lv.n2 = m->lev[l].n2; Lv.n3 = m->lev[l].n3; Lv.dx = m->lev[l].dx;
+      int64_t n = lcells(Lv);
+      Lv.u = q; q += n;
+      Lv.src = q; q += n;
+      Lv.def = nullptr;
+    }
+  }
   int64_t need = (lcells(m->lev[L - 1]) + BS - 1) / BS + 1;
   if (need > m->red_n) {
     if (m->red) AGX_HIP(hipFree(m->red));
@@ -342,160 +408,183 @@ int mg_create(agx_mesh *m) {
 
 void mg_destroy(agx_mesh *m) {
   if (m->mgpool) hipFree(m->mgpool);
+  if (m->mgpool32) hipFree(m->mgpool32);
   m->mgpool = nullptr;
+  m->mgpool32 = nullptr;
   m->lev.clear();
+  m->lev32.clear();
   m->mg_on = 0;
 }
 
 // ---------------------------------------------------------------- level operations
-static int smooth(agx_mesh *m, int l, int color) {
-  MGLevel &L = m->lev[l];
+// Each works on one hierarchy H: the FP64 levels, or the FP32 correction levels of the
+// mixed-precision form.
+template <typename T>
+static void smooth(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l, int color) {
+  MGLevelT<T> &L = H[l];
   double dx2 = L.dx * L.dx;
   double isix = 1.15 / 6.0;
   int64_t n = (int64_t)((L.n1 + 1) / 2) * L.n2 * L.n3;
   timer_begin(m, AGX_K_MG_SMOOTH);
-  hipLaunchKernelGGL(k_mg_smooth, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, L.u, L.src, L.n1, L.n2,
-                     L.n3, dx2, isix, color ^ m->coffset);
+  hipLaunchKernelGGL(k_mg_smooth<T>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, L.u, L.src, L.n1, L.n2,
+                     L.n3, T(dx2), T(isix), color ^ m->coffset);
   timer_end(m);
-  return 0;
 }
 
-static int defect_restrict(agx_mesh *m, int l) {
-  MGLevel &F = m->lev[l], &C = m->lev[l - 1];
+template <typename T>
+static void defect_restrict(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {
+  MGLevelT<T> &F = H[l], &C = H[l - 1];
   double idx2 = 1.0 / (F.dx * F.dx);
   timer_begin(m, AGX_K_MG_RESTRICT);
-  hipLaunchKernelGGL(k_mg_defect_restrict, dim3(grid1d(lcells(C), BS)), dim3(BS), 0, m->stream, F.u, F.src,
-                     C.src, C.u, F.n1, F.n2, F.n3, idx2);
+  hipLaunchKernelGGL(k_mg_defect_restrict<T>, dim3(grid1d(lcells(C), BS)), dim3(BS), 0, m->stream, F.u,
+                     F.src, C.src, C.u, F.n1, F.n2, F.n3, T(idx2));
   timer_end(m);
-  return 0;
 }
 
-static int prolong(agx_mesh *m, int l) {   // l-1 -> l
-  MGLevel &F = m->lev[l], &C = m->lev[l - 1];
+template <typename T>
+static void prolong(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {   // l-1 -> l
+  MGLevelT<T> &F = H[l], &C = H[l - 1];
   timer_begin(m, AGX_K_MG_PROLONG);
-  hipLaunchKernelGGL(k_mg_prolong, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, C.u, F.u, F.n1,
+  hipLaunchKernelGGL(k_mg_prolong<T>, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, C.u, F.u, F.n1,
                      F.n2, F.n3);
   timer_end(m);
-  return 0;
 }
 
-static int fmg_prolong(agx_mesh *m, int l) {   // l-1 -> l
-  MGLevel &F = m->lev[l], &C = m->lev[l - 1];
+template <typename T>
+static void fmg_prolong(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {   // l-1 -> l
+  MGLevelT<T> &F = H[l], &C = H[l - 1];
   timer_begin(m, AGX_K_MG_PROLONG);
-  hipLaunchKernelGGL(k_mg_fmg_prolong, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, C.u, F.u,
+  hipLaunchKernelGGL(k_mg_fmg_prolong<T>, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, C.u, F.u,
                      F.n1, F.n2, F.n3);
   timer_end(m);
-  return 0;
 }
 
 // device sum of a level array (mode 0) or of its squared defect (mode 1) into scal[slot]
-static int level_sum(agx_mesh *m, int l, int mode, int slot) {
-  MGLevel &L = m->lev[l];
+template <typename T>
+static void level_sum(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l, int mode, int slot) {
+  MGLevelT<T> &L = H[l];
   int64_t nb = grid1d(lcells(L), BS);
   double idx2 = 1.0 / (L.dx * L.dx);
   timer_begin(m, AGX_K_MG_OTHER);
   if (mode == 0)
-    hipLaunchKernelGGL(k_mg_partial<0>, dim3((unsigned)nb), dim3(BS), 0, m->stream, L.u, L.src, L.n1, L.n2,
-                       L.n3, idx2, m->red);
+    hipLaunchKernelGGL((k_mg_partial<0, T>), dim3((unsigned)nb), dim3(BS), 0, m->stream, L.u, L.src, L.n1,
+                       L.n2, L.n3, T(idx2), m->red);
   else
-    hipLaunchKernelGGL(k_mg_partial<1>, dim3((unsigned)nb), dim3(BS), 0, m->stream, L.u, L.src, L.n1, L.n2,
-                       L.n3, idx2, m->red);
+    hipLaunchKernelGGL((k_mg_partial<1, T>), dim3((unsigned)nb), dim3(BS), 0, m->stream, L.u, L.src, L.n1,
+                       L.n2, L.n3, T(idx2), m->red);
   hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->red, nb, m->scal + slot);
   timer_end(m);
-  return 0;
 }
 
-static int level_sum_src(agx_mesh *m, int l, int slot) {
+static void level_sum_src(agx_mesh *m, int l, int slot) {
   MGLevel &L = m->lev[l];
   int64_t nb = grid1d(lcells(L), BS);
-  hipLaunchKernelGGL(k_mg_partial<0>, dim3((unsigned)nb), dim3(BS), 0, m->stream, L.src, L.src, L.n1,
-                     L.n2, L.n3, 0.0, m->red);
+  hipLaunchKernelGGL((k_mg_partial<0, double>), dim3((unsigned)nb), dim3(BS), 0, m->stream, L.src, L.src,
+                     L.n1, L.n2, L.n3, 0.0, m->red);
   hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->red, nb, m->scal + slot);
-  return 0;
 }
 
 static double mesh_volume(const agx_mesh_desc &d) {
   return (d.x1max - d.x1min) * (d.x2max - d.x2min) * (d.x3max - d.x3min);
 }
 
-// MultigridDriver::SubtractAverage (multigrid_driver.cpp:305-330) on the finest level:
-// ave = sum * dx*dy*dz / volume, subtracted on the device without a host round trip.
-static int subtract_average(agx_mesh *m, int type) {
+// MultigridDriver::SubtractAverage (multigrid_driver.cpp:305-330) on the finest FP64
+// level: ave = sum * dx*dy*dz / volume, subtracted on the device without a host round
+// trip.
+static void subtract_average(agx_mesh *m, int type) {
   int l = m->ntotal - 1;
   MGLevel &L = m->lev[l];
   const agx_mesh_desc &d = m->d;
   double dv = m->rdx_blk * m->rdy_blk * m->rdz_blk;
   if (type == 0) level_sum_src(m, l, 1);
-  else level_sum(m, l, 0, 1);
+  else level_sum(m, m->lev, l, 0, 1);
   double scale = dv / mesh_volume(d);
-  hipLaunchKernelGGL(k_mg_add, dim3(grid1d(lcells(L), BS)), dim3(BS), 0, m->stream, type ? L.u : L.src,
-                     lcells(L), m->scal + 1, scale);
-  return 0;
+  hipLaunchKernelGGL(k_mg_add<double>, dim3(grid1d(lcells(L), BS)), dim3(BS), 0, m->stream,
+                     type ? L.u : L.src, lcells(L), m->scal + 1, scale);
 }
 
-static void zero_level(agx_mesh *m, int l) {
-  MGLevel &L = m->lev[l];
-  hipLaunchKernelGGL(k_mg_zero, dim3(grid1d(lcells(L), BS)), dim3(BS), 0, m->stream, L.u, lcells(L));
+template <typename T>
+static void zero_level(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {
+  MGLevelT<T> &L = H[l];
+  hipLaunchKernelGGL(k_mg_zero<T>, dim3(grid1d(lcells(L), BS)), dim3(BS), 0, m->stream, L.u, lcells(L));
 }
 
 // SolveCoarsestGrid, multigrid_driver.cpp:778-819 (periodic: fsubtract_average_)
-static void solve_coarsest(agx_mesh *m) {
+template <typename T>
+static void solve_coarsest(agx_mesh *m, std::vector<MGLevelT<T>> &H) {
   const Geo &g = m->g;
   int ni = g.nb1 > g.nb2 ? g.nb1 : g.nb2;
   ni = ni > g.nb3 ? ni : g.nb3;
   ni >>= (m->nrootlevel - 1);
   if (ni == 1) {
-    zero_level(m, 0);
+    zero_level(m, H, 0);
     return;
   }
   // the root grid's average of level 0 is subtracted from the root grid's finest level
   // (Multigrid::CalculateTotal at the current level, SubtractAverage on u_[nlevel_-1],
   // multigrid.cpp:602-641)
-  MGLevel &L0 = m->lev[0];
-  MGLevel &LR = m->lev[m->nrootlevel - 1];
+  MGLevelT<T> &LR = H[m->nrootlevel - 1];
   int ll = m->nrootlevel - 1;
   double dxl = m->rdx_root * (double)(1 << ll), dyl = m->rdy_root * (double)(1 << ll),
          dzl = m->rdz_root * (double)(1 << ll);
   double scale = dxl * dyl * dzl / mesh_volume(m->d);
   for (int pass = 0; pass < 2; ++pass) {
     if (pass == 1) {
-      for (int it = 0; it < ni; ++it) { smooth(m, 0, 0); smooth(m, 0, 1); }
+      for (int it = 0; it < ni; ++it) { smooth(m, H, 0, 0); smooth(m, H, 0, 1); }
     }
-    level_sum(m, 0, 0, 2);
-    hipLaunchKernelGGL(k_mg_add, dim3(grid1d(lcells(LR), BS)), dim3(BS), 0, m->stream, LR.u, lcells(LR),
+    level_sum(m, H, 0, 0, 2);
+    hipLaunchKernelGGL(k_mg_add<T>, dim3(grid1d(lcells(LR), BS)), dim3(BS), 0, m->stream, LR.u, lcells(LR),
                        m->scal + 2, scale);
   }
-  (void)L0;
 }
 
 // SolveVCycle(1,1), multigrid_driver.cpp:684-693 with OneStepToCoarser (:629-677) and
 // OneStepToFiner (:583-622).  On the uniform periodic hierarchy every level step is:
 // coarser = red sweep, black sweep, defect+restrict; finer = prolongate+correct, red,
 // black.
-static void vcycle(agx_mesh *m) {
+template <typename T>
+static void vcycle(agx_mesh *m, std::vector<MGLevelT<T>> &H) {
   int start = m->current;
   m->coffset ^= 1;
   while (m->current > 0) {
     int l = m->current;
-    smooth(m, l, 0);
-    smooth(m, l, 1);
-    defect_restrict(m, l);
+    smooth(m, H, l, 0);
+    smooth(m, H, l, 1);
+    defect_restrict(m, H, l);
     m->current--;
   }
-  solve_coarsest(m);
+  solve_coarsest(m, H);
   while (m->current < start) {
     int l = m->current + 1;
-    prolong(m, l);
-    smooth(m, l, 0);
-    smooth(m, l, 1);
+    prolong(m, H, l);
+    smooth(m, H, l, 0);
+    smooth(m, H, l, 1);
     m->current++;
   }
 }
 
-static double defect_norm(agx_mesh *m) {
-  // MultigridDriver::CalculateDefectNorm (L2), multigrid_driver.cpp:826-858
-  int l = m->ntotal - 1;
-  level_sum(m, l, 1, 0);
+// SolveFMGCycle, multigrid_driver.cpp:700-714, on the hierarchy H whose finest source
+// is set; sources are restricted first (RestrictFMGSource, multigrid_driver.cpp:447-456).
+template <typename T>
+static void fmg(agx_mesh *m, std::vector<MGLevelT<T>> &H) {
+  int L = m->ntotal;
+  for (int l = L - 1; l > 0; --l) {
+    MGLevelT<T> &Fl = H[l], &Cl = H[l - 1];
+    hipLaunchKernelGGL(k_mg_restrict_src<T>, dim3(grid1d(lcells(Cl), BS)), dim3(BS), 0, m->stream, Fl.src,
+                       Cl.src, Fl.n1, Fl.n2, Fl.n3);
+  }
+  m->current = 0;
+  for (m->fmglevel = 0; m->fmglevel < L; m->fmglevel++) {
+    vcycle(m, H);
+    if (m->fmglevel != L - 1) {
+      fmg_prolong(m, H, m->current + 1);
+      m->current++;
+    }
+  }
+  m->fmglevel = L - 1;
+}
+
+static double finish_norm(agx_mesh *m) {
   hipMemcpyAsync(m->scal_host, m->scal, sizeof(double), hipMemcpyDeviceToHost, m->stream);
   hipStreamSynchronize(m->stream);
   const agx_mesh_desc &d = m->d;
@@ -504,45 +593,80 @@ static double defect_norm(agx_mesh *m) {
   return std::sqrt(norm);
 }
 
+static double defect_norm(agx_mesh *m) {
+  // MultigridDriver::CalculateDefectNorm (L2), multigrid_driver.cpp:826-858
+  level_sum(m, m->lev, m->ntotal - 1, 1, 0);
+  return finish_norm(m);
+}
+
+// Mixed precision: FP64 defect of the finest iterate into the FP32 finest source, its
+// squared sum into scal[0]; returns the launch count of partials.
+static void defect_to32(agx_mesh *m) {
+  int l = m->ntotal - 1;
+  MGLevel &F = m->lev[l];
+  MGLevel32 &F32 = m->lev32[l];
+  int64_t nb = grid1d(lcells(F), BS);
+  timer_begin(m, AGX_K_MG_OTHER);
+  hipLaunchKernelGGL(k_mg_defect_to32, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.u, F.src, F32.src, F.n1,
+                     F.n2, F.n3, 1.0 / (F.dx * F.dx), m->red);
+  hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->red, nb, m->scal);
+  timer_end(m);
+}
+
+// One defect-correction step: FP32 V(1,1) on L(e) = d from e = 0, then phi += e.  The
+// FP32 source must hold the current defect (defect_to32).
+static void correction_step(agx_mesh *m) {
+  int l = m->ntotal - 1;
+  MGLevel &F = m->lev[l];
+  MGLevel32 &F32 = m->lev32[l];
+  zero_level(m, m->lev32, l);
+  m->current = l;
+  vcycle(m, m->lev32);
+  timer_begin(m, AGX_K_MG_OTHER);
+  hipLaunchKernelGGL(k_mg_mix, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, F.u, F32.u, lcells(F), 0);
+  timer_end(m);
+}
+
 int mg_solve(agx_mesh *m, int stage) {
   (void)stage;
   const Geo &g = m->g;
   int L = m->ntotal;
   MGLevel &F = m->lev[L - 1];
   double **A = m->arr.a;
+  const bool mixed = m->mixed;
   hipLaunchKernelGGL(k_mg_load, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, g, A[AGX_ARR_U],
                      A[AGX_ARR_PHI], F.src, F.u, m->d.four_pi_G, m->mg_mode == 1 ? 1 : 0);
   m->current = L - 1;
   m->fmglevel = L - 1;
   subtract_average(m, 0);   // SetupMultigrid, multigrid_driver.cpp:438-439
   if (m->mg_mode == 0) {
-    // RestrictFMGSource on blocks and root (multigrid_driver.cpp:447-456)
-    for (int l = L - 1; l > 0; --l) {
-      MGLevel &Fl = m->lev[l], &Cl = m->lev[l - 1];
-      hipLaunchKernelGGL(k_mg_restrict_src, dim3(grid1d(lcells(Cl), BS)), dim3(BS), 0, m->stream, Fl.src,
-                         Cl.src, Fl.n1, Fl.n2, Fl.n3);
+    if (!mixed) {
+      fmg(m, m->lev);
+    } else {
+      // FMG in FP32 from the FP64 source; the result seeds the FP64 iterate
+      MGLevel32 &F32 = m->lev32[L - 1];
+      hipLaunchKernelGGL(k_mg_mix, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, F.src, F32.src,
+                         lcells(F), 2);
+      fmg(m, m->lev32);
+      hipLaunchKernelGGL(k_mg_mix, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, F.u, F32.u,
+                         lcells(F), 1);
     }
-    m->current = 0;
-    // SolveFMGCycle, multigrid_driver.cpp:700-714
-    for (m->fmglevel = 0; m->fmglevel < L; m->fmglevel++) {
-      vcycle(m);
-      if (m->fmglevel != L - 1) {
-        fmg_prolong(m, m->current + 1);
-        m->current++;
-      }
-    }
-    m->fmglevel = L - 1;
+    m->current = L - 1;
     subtract_average(m, 1);
   }
   int iters = 0;
   if (m->mg_eps >= 0.0) {   // SolveIterative, multigrid_driver.cpp:721-755
-    double def = defect_norm(m);
+    double def;
+    if (mixed) { defect_to32(m); def = finish_norm(m); }
+    else def = defect_norm(m);
     int n = 0;
     while (def > m->mg_eps) {
-      vcycle(m);
+      if (mixed) correction_step(m);
+      else vcycle(m, m->lev);
       iters++;
       double olddef = def;
-      def = defect_norm(m);
+      if (mixed) { defect_to32(m); def = finish_norm(m); }
+      else def = defect_norm(m);
       if (def / olddef > 0.8) {
         if (m->mg_eps == 0.0) break;
       }
@@ -552,7 +676,11 @@ int mg_solve(agx_mesh *m, int stage) {
     m->last_norm = def;
     subtract_average(m, 1);
   } else {   // SolveIterativeFixedTimes, multigrid_driver.cpp:762-771
-    for (int n = 0; n < m->mg_niter; ++n) { vcycle(m); iters++; }
+    for (int n = 0; n < m->mg_niter; ++n) {
+      if (mixed) { defect_to32(m); correction_step(m); }
+      else vcycle(m, m->lev);
+      iters++;
+    }
     subtract_average(m, 1);
     m->last_norm = defect_norm(m);
   }
@@ -564,9 +692,14 @@ int mg_solve(agx_mesh *m, int stage) {
   return 0;
 }
 
+// n V(1,1) cycles on the current finest state; in the mixed form one cycle is one
+// defect-correction step (FP64 defect, FP32 V-cycle, FP64 correction).
 int mg_vcycles(agx_mesh *m, int n) {
   m->current = m->ntotal - 1;
-  for (int q = 0; q < n; ++q) vcycle(m);
+  for (int q = 0; q < n; ++q) {
+    if (m->mixed) { defect_to32(m); correction_step(m); }
+    else vcycle(m, m->lev);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "mg_vcycles launches");
   return 0;

@@ -88,11 +88,12 @@ struct PlaneIdx {
 
 // (b,k,j,i) of this thread for the box [kl..ku]x[jl..ju]x[il..iu]; false if outside.
 __device__ __forceinline__ bool plane_index(int il, int iu, int jl, int ju, int kl, PlaneIdx &p) {
-  int ni = iu - il + 1, nj = ju - jl + 1;
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned ni = iu - il + 1, nj = ju - jl + 1;
+  const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= ni * nj) return false;
-  p.j = jl + q / ni;
-  p.i = il + q % ni;
+  const unsigned qj = q / ni;
+  p.j = jl + (int)qj;
+  p.i = il + (int)(q - qj * ni);
   p.k = kl + (int)blockIdx.y;
   p.b = blockIdx.z;
   return true;
@@ -103,11 +104,12 @@ static dim3 plane_grid(int il, int iu, int jl, int ju, int kl, int ku, int nb) {
   return dim3(grid1d(n, 256), (unsigned)(ku - kl + 1), (unsigned)nb);
 }
 
-// Stage-kernel tiles: STAGE_TI x STAGE_TJ (i,j) columns per workgroup; grid.x = tiles of
-// one MeshBlock, grid.y = MeshBlock.
-constexpr int STAGE_TI = 16, STAGE_TJ = 16;
-static dim3 stage_grid(const Geo &g) {
-  int nti = (g.bx1 + STAGE_TI - 1) / STAGE_TI, ntj = (g.bx2 + STAGE_TJ - 1) / STAGE_TJ;
+// Stage-kernel tiles: TI x TJ (i,j) columns per 256-thread workgroup; grid.x = tiles of
+// one MeshBlock, grid.y = MeshBlock.  The tile shape is picked per mesh by agx_autotune.
+constexpr int NTILE = 3;
+constexpr int TILE_I[NTILE] = {16, 32, 64}, TILE_J[NTILE] = {16, 8, 4};
+static dim3 stage_grid(const Geo &g, int tile) {
+  int nti = (g.bx1 + TILE_I[tile] - 1) / TILE_I[tile], ntj = (g.bx2 + TILE_J[tile] - 1) / TILE_J[tile];
   return dim3((unsigned)(nti * ntj), (unsigned)g.nb);
 }
 
@@ -1089,13 +1091,40 @@ __device__ __forceinline__ double wg_min256(double m, double *sh) {
 // kernels, so results are identical.
 //
 // A workgroup owns a TI x TJ tile of (i,j) columns of one MeshBlock (thread = column) and
-// walks the cell planes c = ks-1 .. ke+1 upwards.  Iteration c computes the cell-centred E
-// of plane c, the x1/x2 edges of edge plane c and the x3 edges of plane c, the new x3
-// faces of face plane c and the new x1/x2 faces of plane c-1, then updates cell plane
-// c-1.  LDS holds two planes of cell-centred E and edges and one plane of new x1/x2 faces;
-// fluxes, face EMFs and weights stream from HBM once.  Outputs go to registers other than
-// the inputs (u may be in place: each cell reads only its own src), so tiles that share
-// halo values never see each other's writes.
+// walks the cell planes upwards.  Iteration c computes the cell-centred E of plane c+1
+// and the corner EMFs of edge plane c and plane c into LDS (three planes of cell-centred
+// E, two of edges), then, after one barrier, each thread does the CT of the faces of its
+// cell in plane c-1 and of its x3 face in face plane c, and updates that cell.  Faces
+// shared by two cells are computed by both from identical inputs; fluxes, face EMFs and
+// weights stream from HBM once.  Outputs go to registers other than the inputs (u may
+// be in place: each cell reads only its own src), so tiles that share halo values never
+// see each other's writes.
+// Ghost copies written by their source (same-level periodic exchange, bvals_cc.cpp:201-336
+// and bvals_fc.cpp:344-402,583-700, run "push" instead of "pull"): index x of an array
+// whose own range along an axis is [lo, hi] (cells: [is, ie]; the normal axis of a face
+// array: [is, ie+1], the shared face being kept by both blocks) is a ghost of the +1
+// neighbour at x - bx when that lands in [lo - ng, lo - 1], and of the -1 neighbour at
+// x + bx when that lands in [hi + 1, hi + ng].  Returns the offset range [omin, omax].
+__device__ __forceinline__ void push_range(int x, int lo, int hi, int bx, int ng, int &omin,
+                                           int &omax) {
+  omax = (x - bx >= lo - ng && x - bx <= lo - 1) ? 1 : 0;
+  omin = (x + bx >= hi + 1 && x + bx <= hi + ng) ? -1 : 0;
+}
+
+// Calls f(dest_block, dest_index) for every neighbour ghost that copies element (k,j,i)
+// of block b; r[0..5] = (o1min, o1max, o2min, o2max, o3min, o3max).
+template <typename F>
+__device__ __forceinline__ void for_each_ghost(const Geo &g, int b, int k, int j, int i,
+                                               const int *r, F f) {
+  if (!(r[0] | r[1] | r[2] | r[3] | r[4] | r[5])) return;
+  for (int o3 = r[4]; o3 <= r[5]; ++o3)
+    for (int o2 = r[2]; o2 <= r[3]; ++o2)
+      for (int o1 = r[0]; o1 <= r[1]; ++o1) {
+        if (!(o1 | o2 | o3)) continue;
+        f(nbr_block(g, b, o1, o2, o3), IDX(g, k - o3 * g.bx3, j - o2 * g.bx2, i - o1 * g.bx1));
+      }
+}
+
 template <typename TW>
 struct StageArgs {
   const double *src;                 // conserved state the stage starts from
@@ -1112,21 +1141,18 @@ struct StageArgs {
   const BlockCoef *coef;
   double wght, gm1, dfl, pfl, gamma;
   int grav;
-  int *floor_flag;                   // raised when any floor fired
   double *dtpart;                    // per-workgroup min of the dt bound, or null
 };
 
 template <int TI, int TJ, typename TW>
-__global__ void __launch_bounds__(TI * TJ)
+__global__ void __launch_bounds__(TI * TJ, 4)   // 4 workgroups per CU: <= 128 VGPRs, 4 x 37 KB LDS
 k_stage(Geo g, StageArgs<TW> a) {
   static_assert(TI * TJ == 256, "k_stage runs 256-thread workgroups");
   constexpr int NT = TI * TJ, CI = TI + 2, CJ = TJ + 2;
-  __shared__ double s_cce[2][3][CJ][CI];
-  __shared__ double s_e1[2][TJ + 1][TI];
+  __shared__ double s_cce[3][3][CJ][CI];      // planes c-1, c, c+1 (slot = plane mod 3)
+  __shared__ double s_e1[2][TJ + 1][TI];      // edge planes c-1, c (slot = plane & 1)
   __shared__ double s_e2[2][TJ][TI + 1];
   __shared__ double s_e3[2][TJ + 1][TI + 1];
-  __shared__ double s_b1[TJ][TI + 1];
-  __shared__ double s_b2[TJ + 1][TI];
   __shared__ double s_red[4];
   const int tid = threadIdx.x;
   const int tx = tid % TI, ty = tid / TI;
@@ -1143,66 +1169,72 @@ k_stage(Geo g, StageArgs<TW> a) {
   const double dx[3] = {a.coef[b].dx[0], a.coef[b].dx[1], a.coef[b].dx[2]};
   const double dx1 = dx[0], dx2 = dx[1], dx3 = dx[2];
   const double wght = a.wght;
+  const int wi = ni + 2, ncce = wi * (nj + 2);
+  const int np = (nj + 1) * (ni + 1);
+  // cell-centred E = -v x B at plane c, cells (j0-1..j0+nj) x (i0-1..i0+ni), into slot
+  auto cce_plane = [&](int c) {
+    const int sl = (c + 3) % 3;
+    for (int q = tid; q < ncce; q += NT) {
+      const int jj = q / wi, ii = q - jj * wi;
+      const int64_t id = IDX(g, c, j0 - 1 + jj, i0 - 1 + ii);
+      const TW *wb = a.w + cbase + id;
+      const TW *bb = a.bcc + bbase + id;
+      const double vx = wb[IVX * S], vy = wb[IVY * S], vz = wb[IVZ * S];
+      const double bx = bb[0], by = bb[S], bz = bb[2 * S];
+      s_cce[sl][0][jj][ii] = vz * by - vy * bz;
+      s_cce[sl][1][jj][ii] = vx * bz - vz * bx;
+      s_cce[sl][2][jj][ii] = vy * bx - vx * by;
+    }
+  };
+  cce_plane(g.ks - 1);
+  cce_plane(g.ks);
+  __syncthreads();
   double b3_lo = 0.0;       // new b3 at the lower face of cell plane c-1
   double dtmin = DBL_MAX;
-  bool floored = false;
-  for (int c = g.ks - 1; c <= g.ke + 1; ++c) {
+  for (int c = g.ks; c <= g.ke + 1; ++c) {
     const int cb = c & 1, pb = cb ^ 1;
-    // (1) cell-centred E = -v x B at plane c, cells (j0-1..j0+nj) x (i0-1..i0+ni)
-    {
-      const int wi = ni + 2, n = wi * (nj + 2);
-      for (int q = tid; q < n; q += NT) {
-        const int jj = q / wi, ii = q - jj * wi;
-        const int64_t id = IDX(g, c, j0 - 1 + jj, i0 - 1 + ii);
-        const TW *wb = a.w + cbase + id;
-        const TW *bb = a.bcc + bbase + id;
-        const double vx = wb[IVX * S], vy = wb[IVY * S], vz = wb[IVZ * S];
-        const double bx = bb[0], by = bb[S], bz = bb[2 * S];
-        s_cce[cb][0][jj][ii] = vz * by - vy * bz;
-        s_cce[cb][1][jj][ii] = vx * bz - vz * bx;
-        s_cce[cb][2][jj][ii] = vy * bx - vx * by;
+    const int cc = c % 3, pc = (c + 2) % 3;
+    // (A) cell-centred E of plane c+1 (used by the next iteration), then the edges at the
+    // points (c, j0+jj, i0+ii), jj in [0,nj], ii in [0,ni]: e1 and e2 of edge plane c
+    // (lower edges of cell plane c; e1 for ii < ni, e2 for jj < nj) and e3 of plane c.
+    // One thread per point, so weights shared by the three edges load once.
+    if (c + 1 <= g.ke + 1) cce_plane(c + 1);
+    const bool do3 = (c <= g.ke);
+    for (int q = tid; q < np; q += NT) {
+      const int jj = q / (ni + 1), ii = q - jj * (ni + 1);
+      const bool do1 = ii < ni, do2 = jj < nj;
+      const int64_t fo = fb + IDX(g, c, j0 + jj, i0 + ii);
+      const double w3 = (do1 || do2) ? (double)a.wx3[fo] : 0.0;
+      if (do1) {
+        s_e1[cb][jj][ii] = edge_e1(
+            s_cce[cc][0][jj + 1][ii + 1], s_cce[cc][0][jj][ii + 1], s_cce[pc][0][jj + 1][ii + 1],
+            s_cce[pc][0][jj][ii + 1], (double)a.e1x3[fo], (double)a.e1x3[fo - sj],
+            (double)a.e1x2[fo], (double)a.e1x2[fo - sk], (double)a.wx2[fo],
+            (double)a.wx2[fo - sk], w3, (double)a.wx3[fo - sj]);
+      }
+      if (do2) {
+        s_e2[cb][jj][ii] = edge_e2(
+            s_cce[cc][1][jj + 1][ii + 1], s_cce[cc][1][jj + 1][ii], s_cce[pc][1][jj + 1][ii + 1],
+            s_cce[pc][1][jj + 1][ii], (double)a.e2x3[fo], (double)a.e2x3[fo - 1],
+            (double)a.e2x1[fo], (double)a.e2x1[fo - sk], (double)a.wx1[fo],
+            (double)a.wx1[fo - sk], w3, (double)a.wx3[fo - 1]);
+      }
+      if (do3) {
+        s_e3[cb][jj][ii] = edge_e3(
+            s_cce[cc][2][jj + 1][ii + 1], s_cce[cc][2][jj + 1][ii], s_cce[cc][2][jj][ii + 1],
+            s_cce[cc][2][jj][ii], (double)a.e3x2[fo], (double)a.e3x2[fo - 1],
+            (double)a.e3x1[fo], (double)a.e3x1[fo - sj], (double)a.wx1[fo],
+            (double)a.wx1[fo - sj], (double)a.wx2[fo], (double)a.wx2[fo - 1]);
       }
     }
     __syncthreads();
-    // (2) e1, e2 at edge plane c (lower edges of cell plane c) and e3 at plane c
-    if (c >= g.ks) {
-      const int n1e = (nj + 1) * ni, n2e = nj * (ni + 1);
-      const int n3e = (c <= g.ke) ? (nj + 1) * (ni + 1) : 0;
-      for (int q = tid; q < n1e + n2e + n3e; q += NT) {
-        if (q < n1e) {
-          const int jj = q / ni, ii = q - jj * ni;
-          const int64_t fo = fb + IDX(g, c, j0 + jj, i0 + ii);
-          s_e1[cb][jj][ii] = edge_e1(
-              s_cce[cb][0][jj + 1][ii + 1], s_cce[cb][0][jj][ii + 1], s_cce[pb][0][jj + 1][ii + 1],
-              s_cce[pb][0][jj][ii + 1], (double)a.e1x3[fo], (double)a.e1x3[fo - sj],
-              (double)a.e1x2[fo], (double)a.e1x2[fo - sk], (double)a.wx2[fo],
-              (double)a.wx2[fo - sk], (double)a.wx3[fo], (double)a.wx3[fo - sj]);
-        } else if (q < n1e + n2e) {
-          const int r = q - n1e;
-          const int jj = r / (ni + 1), ii = r - jj * (ni + 1);
-          const int64_t fo = fb + IDX(g, c, j0 + jj, i0 + ii);
-          s_e2[cb][jj][ii] = edge_e2(
-              s_cce[cb][1][jj + 1][ii + 1], s_cce[cb][1][jj + 1][ii], s_cce[pb][1][jj + 1][ii + 1],
-              s_cce[pb][1][jj + 1][ii], (double)a.e2x3[fo], (double)a.e2x3[fo - 1],
-              (double)a.e2x1[fo], (double)a.e2x1[fo - sk], (double)a.wx1[fo],
-              (double)a.wx1[fo - sk], (double)a.wx3[fo], (double)a.wx3[fo - 1]);
-        } else {
-          const int r = q - n1e - n2e;
-          const int jj = r / (ni + 1), ii = r - jj * (ni + 1);
-          const int64_t fo = fb + IDX(g, c, j0 + jj, i0 + ii);
-          s_e3[cb][jj][ii] = edge_e3(
-              s_cce[cb][2][jj + 1][ii + 1], s_cce[cb][2][jj + 1][ii], s_cce[cb][2][jj][ii + 1],
-              s_cce[cb][2][jj][ii], (double)a.e3x2[fo], (double)a.e3x2[fo - 1],
-              (double)a.e3x1[fo], (double)a.e3x1[fo - sj], (double)a.wx1[fo],
-              (double)a.wx1[fo - sj], (double)a.wx2[fo], (double)a.wx2[fo - 1]);
-        }
-      }
-    }
-    __syncthreads();
-    // (3) CT: x3 faces of face plane c (own column), x1/x2 faces of plane c-1
+    // (B) own column: CT of the x3 face of face plane c, then for cell (c-1, j, i) the
+    // CT of its four x1/x2 faces (the upper ones are the neighbours' lower faces,
+    // recomputed here from the same edges), flux divergence, self-gravity, bcc, C2P and
+    // the dt bound.
     double b3_hi = 0.0;
-    if (c >= g.ks) {
-      if (own) {
+    if (own) {
+      {
         const int64_t fo = fb + IDX(g, c, j, i);
         double v = a.s3[fo];
         const double area = dx1 * dx2;
@@ -1210,241 +1242,135 @@ k_stage(Geo g, StageArgs<TW> a) {
         v += (wght / area) * (dx1 * s_e1[cb][ty + 1][tx] - dx1 * s_e1[cb][ty][tx]);
         a.d3[fo] = v;
         b3_hi = v;
+        int r[6];
+        push_range(i, g.is, g.ie, g.bx1, g.ng, r[0], r[1]);
+        push_range(j, g.js, g.je, g.bx2, g.ng, r[2], r[3]);
+        push_range(c, g.ks, g.ke + 1, g.bx3, g.ng, r[4], r[5]);
+        for_each_ghost(g, b, c, j, i, r, [&](int nb, int64_t id2) { a.d3[(int64_t)nb * S + id2] = v; });
       }
       if (c - 1 >= g.ks) {
-        const int n1f = nj * (ni + 1), n2f = (nj + 1) * ni;
-        for (int q = tid; q < n1f + n2f; q += NT) {
-          if (q < n1f) {
-            const int jj = q / (ni + 1), ii = q - jj * (ni + 1);
-            const int64_t fo = fb + IDX(g, c - 1, j0 + jj, i0 + ii);
-            double v = a.s1[fo];
-            const double area = dx2 * dx3;
-            v -= (wght / area) * (dx3 * s_e3[pb][jj + 1][ii] - dx3 * s_e3[pb][jj][ii]);
-            v += (wght / area) * (dx2 * s_e2[cb][jj][ii] - dx2 * s_e2[pb][jj][ii]);
-            a.d1[fo] = v;
-            s_b1[jj][ii] = v;
-          } else {
-            const int r = q - n1f;
-            const int jj = r / ni, ii = r - jj * ni;
-            const int64_t fo = fb + IDX(g, c - 1, j0 + jj, i0 + ii);
-            double v = a.s2[fo];
-            const double area = dx1 * dx3;
-            v += (wght / area) * (dx3 * s_e3[pb][jj][ii + 1] - dx3 * s_e3[pb][jj][ii]);
-            v -= (wght / area) * (dx1 * s_e1[cb][jj][ii] - dx1 * s_e1[pb][jj][ii]);
-            a.d2[fo] = v;
-            s_b2[jj][ii] = v;
+        const int64_t id = IDX(g, c - 1, j, i);
+        const int64_t fo = fb + id;
+        const double area1 = dx2 * dx3, area2 = dx1 * dx3;
+        double b1l = a.s1[fo], b1h = a.s1[fo + 1], b2l = a.s2[fo], b2h = a.s2[fo + sj];
+        b1l -= (wght / area1) * (dx3 * s_e3[pb][ty + 1][tx] - dx3 * s_e3[pb][ty][tx]);
+        b1l += (wght / area1) * (dx2 * s_e2[cb][ty][tx] - dx2 * s_e2[pb][ty][tx]);
+        b1h -= (wght / area1) * (dx3 * s_e3[pb][ty + 1][tx + 1] - dx3 * s_e3[pb][ty][tx + 1]);
+        b1h += (wght / area1) * (dx2 * s_e2[cb][ty][tx + 1] - dx2 * s_e2[pb][ty][tx + 1]);
+        b2l += (wght / area2) * (dx3 * s_e3[pb][ty][tx + 1] - dx3 * s_e3[pb][ty][tx]);
+        b2l -= (wght / area2) * (dx1 * s_e1[cb][ty][tx] - dx1 * s_e1[pb][ty][tx]);
+        b2h += (wght / area2) * (dx3 * s_e3[pb][ty + 1][tx + 1] - dx3 * s_e3[pb][ty + 1][tx]);
+        b2h -= (wght / area2) * (dx1 * s_e1[cb][ty + 1][tx] - dx1 * s_e1[pb][ty + 1][tx]);
+        a.d1[fo] = b1l;
+        if (tx == ni - 1) a.d1[fo + 1] = b1h;
+        a.d2[fo] = b2l;
+        if (ty == nj - 1) a.d2[fo + sj] = b2h;
+        {
+          int rc[6], r[6];
+          push_range(i, g.is, g.ie, g.bx1, g.ng, rc[0], rc[1]);
+          push_range(j, g.js, g.je, g.bx2, g.ng, rc[2], rc[3]);
+          push_range(c - 1, g.ks, g.ke, g.bx3, g.ng, rc[4], rc[5]);
+          // x1 faces i (and i+1 at the tile's upper edge)
+          r[2] = rc[2]; r[3] = rc[3]; r[4] = rc[4]; r[5] = rc[5];
+          push_range(i, g.is, g.ie + 1, g.bx1, g.ng, r[0], r[1]);
+          for_each_ghost(g, b, c - 1, j, i, r, [&](int nb, int64_t id2) { a.d1[(int64_t)nb * S + id2] = b1l; });
+          if (tx == ni - 1) {
+            push_range(i + 1, g.is, g.ie + 1, g.bx1, g.ng, r[0], r[1]);
+            for_each_ghost(g, b, c - 1, j, i + 1, r,
+                           [&](int nb, int64_t id2) { a.d1[(int64_t)nb * S + id2] = b1h; });
+          }
+          // x2 faces j (and j+1 at the tile's upper edge)
+          r[0] = rc[0]; r[1] = rc[1];
+          push_range(j, g.js, g.je + 1, g.bx2, g.ng, r[2], r[3]);
+          for_each_ghost(g, b, c - 1, j, i, r, [&](int nb, int64_t id2) { a.d2[(int64_t)nb * S + id2] = b2l; });
+          if (ty == nj - 1) {
+            push_range(j + 1, g.js, g.je + 1, g.bx2, g.ng, r[2], r[3]);
+            for_each_ghost(g, b, c - 1, j + 1, i, r,
+                           [&](int nb, int64_t id2) { a.d2[(int64_t)nb * S + id2] = b2h; });
+          }
+          // cell copies are written after the update below, with the same ranges rc
+          r[0] = rc[0]; r[1] = rc[1]; r[2] = rc[2]; r[3] = rc[3];
+        }
+        const double a1 = dx2 * dx3, a2 = dx1 * dx3, a3 = dx1 * dx2;
+        const double vol = dx1 * dx2 * dx3;
+        const int64_t cb0 = cbase + id;
+        double un[NHYDRO];
+#pragma unroll
+        for (int n = 0; n < NHYDRO; ++n) {
+          const int64_t o = cb0 + n * S;
+          double dflx = (a1 * (double)a.f1[o + 1] - a1 * (double)a.f1[o]);
+          dflx += (a2 * (double)a.f2[o + sj] - a2 * (double)a.f2[o]);
+          dflx += (a3 * (double)a.f3[o + sk] - a3 * (double)a.f3[o]);
+          un[n] = a.src[o] - wght * dflx / vol;
+        }
+        if (a.grav) {
+          const double rho = a.rho[cb0];
+          const double phic = a.phi[fo];
+          {
+            double dtodx1 = wght / dx1;
+            double phil = 0.5 * (a.phi[fo - 1] + phic);
+            double phir = 0.5 * (phic + a.phi[fo + 1]);
+            un[IVX] -= dtodx1 * rho * (phir - phil);
+            un[IEN] -= dtodx1 * ((double)a.f1[cb0] * (phic - phil) + (double)a.f1[cb0 + 1] * (phir - phic));
+          }
+          {
+            double dtodx2 = wght / dx2;
+            double phil = 0.5 * (a.phi[fo - sj] + phic);
+            double phir = 0.5 * (phic + a.phi[fo + sj]);
+            un[IVY] -= dtodx2 * rho * (phir - phil);
+            un[IEN] -= dtodx2 * ((double)a.f2[cb0] * (phic - phil) + (double)a.f2[cb0 + sj] * (phir - phic));
+          }
+          {
+            double dtodx3 = wght / dx3;
+            double phil = 0.5 * (a.phi[fo - sk] + phic);
+            double phir = 0.5 * (phic + a.phi[fo + sk]);
+            un[IVZ] -= dtodx3 * rho * (phir - phil);
+            un[IEN] -= dtodx3 * ((double)a.f3[cb0] * (phic - phil) + (double)a.f3[cb0 + sk] * (phir - phic));
           }
         }
-      }
-    }
-    __syncthreads();
-    // (4) cell plane c-1: flux divergence, self-gravity, bcc, C2P, dt bound
-    if (c - 1 >= g.ks && own) {
-      const int64_t id = IDX(g, c - 1, j, i);
-      const double a1 = dx2 * dx3, a2 = dx1 * dx3, a3 = dx1 * dx2;
-      const double vol = dx1 * dx2 * dx3;
-      const int64_t cb0 = cbase + id;
-      double un[NHYDRO];
+        const double bcc1 = 0.5 * b1l + 0.5 * b1h;
+        const double bcc2 = 0.5 * b2l + 0.5 * b2h;
+        const double bcc3 = 0.5 * b3_lo + 0.5 * b3_hi;
+        const int64_t bo = bbase + id;
+        a.bccn[bo] = TW(bcc1);
+        a.bccn[bo + S] = TW(bcc2);
+        a.bccn[bo + 2 * S] = TW(bcc3);
+        // C2P writes its floors back into u (adiabatic_mhd.cpp:66-84)
+        double wv[5];
+        c2p_cell(un[IDN], un[IVX], un[IVY], un[IVZ], un[IEN], bcc1, bcc2, bcc3, a.gm1, a.dfl, a.pfl, wv);
 #pragma unroll
-      for (int n = 0; n < NHYDRO; ++n) {
-        const int64_t o = cb0 + n * S;
-        double dflx = (a1 * (double)a.f1[o + 1] - a1 * (double)a.f1[o]);
-        dflx += (a2 * (double)a.f2[o + sj] - a2 * (double)a.f2[o]);
-        dflx += (a3 * (double)a.f3[o + sk] - a3 * (double)a.f3[o]);
-        un[n] = a.src[o] - wght * dflx / vol;
-      }
-      if (a.grav) {
-        const int64_t pbk = fb + id;
-        const double rho = a.rho[cb0];
-        const double phic = a.phi[pbk];
-        {
-          double dtodx1 = wght / dx1;
-          double phil = 0.5 * (a.phi[pbk - 1] + phic);
-          double phir = 0.5 * (phic + a.phi[pbk + 1]);
-          un[IVX] -= dtodx1 * rho * (phir - phil);
-          un[IEN] -= dtodx1 * ((double)a.f1[cb0] * (phic - phil) + (double)a.f1[cb0 + 1] * (phir - phic));
-        }
-        {
-          double dtodx2 = wght / dx2;
-          double phil = 0.5 * (a.phi[pbk - sj] + phic);
-          double phir = 0.5 * (phic + a.phi[pbk + sj]);
-          un[IVY] -= dtodx2 * rho * (phir - phil);
-          un[IEN] -= dtodx2 * ((double)a.f2[cb0] * (phic - phil) + (double)a.f2[cb0 + sj] * (phir - phic));
-        }
-        {
-          double dtodx3 = wght / dx3;
-          double phil = 0.5 * (a.phi[pbk - sk] + phic);
-          double phir = 0.5 * (phic + a.phi[pbk + sk]);
-          un[IVZ] -= dtodx3 * rho * (phir - phil);
-          un[IEN] -= dtodx3 * ((double)a.f3[cb0] * (phic - phil) + (double)a.f3[cb0 + sk] * (phir - phic));
-        }
-      }
+        for (int n = 0; n < NHYDRO; ++n) a.dst[cb0 + n * S] = un[n];
 #pragma unroll
-      for (int n = 0; n < NHYDRO; ++n) a.dst[cb0 + n * S] = un[n];
-      const double bcc1 = 0.5 * s_b1[ty][tx] + 0.5 * s_b1[ty][tx + 1];
-      const double bcc2 = 0.5 * s_b2[ty][tx] + 0.5 * s_b2[ty + 1][tx];
-      const double bcc3 = 0.5 * b3_lo + 0.5 * b3_hi;
-      const int64_t bo = bbase + id;
-      a.bccn[bo] = TW(bcc1);
-      a.bccn[bo + S] = TW(bcc2);
-      a.bccn[bo + 2 * S] = TW(bcc3);
-      double u_d = un[IDN], u_e = un[IEN], wv[5];
-      floored |= c2p_cell(u_d, un[IVX], un[IVY], un[IVZ], u_e, bcc1, bcc2, bcc3, a.gm1, a.dfl, a.pfl, wv);
+        for (int n = 0; n < NHYDRO; ++n) a.wn[cb0 + n * S] = TW(wv[n]);
+        // ghost copies: a ghost cell's u, w and bcc equal those of its source cell (same
+        // conserved state, and bcc from the same face values), so they are copied
+        {
+          int rc[6];
+          push_range(i, g.is, g.ie, g.bx1, g.ng, rc[0], rc[1]);
+          push_range(j, g.js, g.je, g.bx2, g.ng, rc[2], rc[3]);
+          push_range(c - 1, g.ks, g.ke, g.bx3, g.ng, rc[4], rc[5]);
+          for_each_ghost(g, b, c - 1, j, i, rc, [&](int nb, int64_t id2) {
+            const int64_t c2 = (int64_t)nb * NHYDRO * S + id2, b2 = (int64_t)nb * 3 * S + id2;
 #pragma unroll
-      for (int n = 0; n < NHYDRO; ++n) a.wn[cb0 + n * S] = TW(wv[n]);
-      if (a.dtpart)
-        dtmin = dmin(dtmin, cell_dt(a.gamma, wv, bcc1, bcc2, bcc3, s_b1[ty][tx], s_b2[ty][tx],
-                                    b3_lo, dx));
+            for (int n = 0; n < NHYDRO; ++n) a.dst[c2 + n * S] = un[n];
+#pragma unroll
+            for (int n = 0; n < NHYDRO; ++n) a.wn[c2 + n * S] = TW(wv[n]);
+            a.bccn[b2] = TW(bcc1);
+            a.bccn[b2 + S] = TW(bcc2);
+            a.bccn[b2 + 2 * S] = TW(bcc3);
+          });
+        }
+        if (a.dtpart) dtmin = dmin(dtmin, cell_dt(a.gamma, wv, bcc1, bcc2, bcc3, b1l, b2l, b3_lo, dx));
+      }
     }
     b3_lo = b3_hi;
+    __syncthreads();
   }
-  if (floored) atomicOr(a.floor_flag, 1);
   if (a.dtpart) {
     double m = wg_min256(dtmin, s_red);
     if (tid == 0) a.dtpart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = m;
   }
 }
-
-// The q-th element of the shell of the box [0,N1)x[0,N2)x[0,N3) that lies outside the
-// inner box [a1,b1]x[a2,b2]x[a3,b3]: whole planes first, then whole rows of the inner
-// planes, then the cells of the inner rows.
-struct Shell {
-  int N1, N2, N3, a1, b1, a2, b2, a3, b3;
-  int64_t nk, nj, ni;
-};
-
-static Shell make_shell(int N1, int N2, int N3, int a1, int b1, int a2, int b2, int a3, int b3) {
-  Shell s{N1, N2, N3, a1, b1, a2, b2, a3, b3, 0, 0, 0};
-  s.nk = (int64_t)(a3 + (N3 - 1 - b3)) * N2 * N1;
-  s.nj = (int64_t)(b3 - a3 + 1) * (a2 + (N2 - 1 - b2)) * N1;
-  s.ni = (int64_t)(b3 - a3 + 1) * (b2 - a2 + 1) * (a1 + (N1 - 1 - b1));
-  return s;
-}
-
-__device__ __forceinline__ bool shell_elem(const Shell &s, int64_t q, int &k, int &j, int &i) {
-  int64_t plane = (int64_t)s.N1 * s.N2;
-  if (q < s.nk) {
-    int pidx = (int)(q / plane);
-    int64_t r = q - (int64_t)pidx * plane;
-    k = (pidx < s.a3) ? pidx : s.b3 + 1 + (pidx - s.a3);
-    j = (int)(r / s.N1);
-    i = (int)(r % s.N1);
-    return true;
-  }
-  q -= s.nk;
-  if (q < s.nj) {
-    int rows_out = s.a2 + (s.N2 - 1 - s.b2);
-    int64_t per = (int64_t)rows_out * s.N1;
-    int kk = (int)(q / per);
-    int64_t r = q - (int64_t)kk * per;
-    int rr = (int)(r / s.N1);
-    i = (int)(r % s.N1);
-    k = s.a3 + kk;
-    j = (rr < s.a2) ? rr : s.b2 + 1 + (rr - s.a2);
-    return true;
-  }
-  q -= s.nj;
-  if (q < s.ni) {
-    int cols_out = s.a1 + (s.N1 - 1 - s.b1);
-    int nrows = s.b2 - s.a2 + 1;
-    int64_t per = (int64_t)nrows * cols_out;
-    int kk = (int)(q / per);
-    int64_t r = q - (int64_t)kk * per;
-    int jj = (int)(r / cols_out);
-    int cc = (int)(r % cols_out);
-    k = s.a3 + kk;
-    j = s.a2 + jj;
-    i = (cc < s.a1) ? cc : s.b1 + 1 + (cc - s.a1);
-    return true;
-  }
-  return false;
-}
-
-// Ghost faces of all three components (bvals_fc.cpp:344-402,583-700); grid.y = comp.
-__global__ void __launch_bounds__(256)
-k_ghost_faces(Geo g, Shell s1, Shell s2, Shell s3, double *b1, double *b2, double *b3) {
-  const int dir = blockIdx.y;
-  const Shell &s = (dir == 0) ? s1 : (dir == 1 ? s2 : s3);
-  int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= s.nk + s.nj + s.ni) return;
-  int k, j, i;
-  shell_elem(s, q, k, j, i);
-  int o1 = (i < s.a1) ? -1 : (i > s.b1 ? 1 : 0);
-  int o2 = (j < s.a2) ? -1 : (j > s.b2 ? 1 : 0);
-  int o3 = (k < s.a3) ? -1 : (k > s.b3 ? 1 : 0);
-  int b = blockIdx.z;
-  int src = nbr_block(g, b, o1, o2, o3);
-  double *a = (dir == 0) ? b1 : (dir == 1 ? b2 : b3);
-  a[(int64_t)b * g.S + IDX(g, k, j, i)] =
-      a[(int64_t)src * g.S + IDX(g, k - o3 * g.bx3, j - o2 * g.bx2, i - o1 * g.bx1)];
-}
-
-// Ghost cells (bvals_cc.cpp:300-336 followed by adiabatic_mhd.cpp:41-90 over the ghost
-// zones): copy the neighbour's un-floored conserved state, bcc from this block's faces
-// (ghost faces already exchanged), C2P with floors written to the ghost copy.
-template <typename TW>
-__global__ void __launch_bounds__(256)
-k_ghost_cells(Geo g, Shell s, double *__restrict__ u, TW *__restrict__ w,
-              TW *__restrict__ bcc, const double *__restrict__ b1,
-              const double *__restrict__ b2, const double *__restrict__ b3, double gm1,
-              double dfl, double pfl) {
-  int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= s.nk + s.nj + s.ni) return;
-  int k, j, i;
-  shell_elem(s, q, k, j, i);
-  int o1 = (i < g.is) ? -1 : (i > g.ie ? 1 : 0);
-  int o2 = (j < g.js) ? -1 : (j > g.je ? 1 : 0);
-  int o3 = (k < g.ks) ? -1 : (k > g.ke ? 1 : 0);
-  int b = blockIdx.z;
-  int src = nbr_block(g, b, o1, o2, o3);
-  const int64_t S = g.S;
-  const int64_t id = IDX(g, k, j, i);
-  const int64_t sid = IDX(g, k - o3 * g.bx3, j - o2 * g.bx2, i - o1 * g.bx1);
-  const int64_t c = (int64_t)b * NHYDRO * S + id, cs = (int64_t)src * NHYDRO * S + sid;
-  double u_d = u[cs], u_m1 = u[cs + S], u_m2 = u[cs + 2 * S], u_m3 = u[cs + 3 * S], u_e = u[cs + 4 * S];
-  const int64_t fo = (int64_t)b * S + id;
-  const double bcc1 = 0.5 * b1[fo] + 0.5 * b1[fo + 1];
-  const double bcc2 = 0.5 * b2[fo] + 0.5 * b2[fo + g.P];
-  const double bcc3 = 0.5 * b3[fo] + 0.5 * b3[fo + (int64_t)g.J * g.P];
-  const int64_t bo = (int64_t)b * 3 * S + id;
-  bcc[bo] = TW(bcc1);
-  bcc[bo + S] = TW(bcc2);
-  bcc[bo + 2 * S] = TW(bcc3);
-  double wv[5];
-  c2p_cell(u_d, u_m1, u_m2, u_m3, u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
-  u[c] = u_d;
-  u[c + S] = u_m1;
-  u[c + 2 * S] = u_m2;
-  u[c + 3 * S] = u_m3;
-  u[c + 4 * S] = u_e;
-#pragma unroll
-  for (int n = 0; n < NHYDRO; ++n) w[c + n * S] = TW(wv[n]);
-}
-
-// Write the interior floors back to u (ConservedToPrimitive's in-place update) once
-// the ghost copies are taken; a no-op unless the stage kernel saw a floor.  bcc is
-// recomputed from the faces with the formula of field.cpp:112-180.
-__global__ void __launch_bounds__(256)
-k_floor_fix(Geo g, double *__restrict__ u, const double *__restrict__ b1,
-            const double *__restrict__ b2, const double *__restrict__ b3, double gm1,
-            double dfl, double pfl, const int *__restrict__ floor_flag) {
-  if (*floor_flag == 0) return;
-  PlaneIdx p;
-  if (!plane_index(g.is, g.ie, g.js, g.je, g.ks, p)) return;
-  const int64_t S = g.S;
-  const int64_t id = IDX(g, p.k, p.j, p.i);
-  const int64_t c = (int64_t)p.b * NHYDRO * S + id;
-  const int64_t fo = (int64_t)p.b * S + id;
-  const double bcc1 = 0.5 * b1[fo] + 0.5 * b1[fo + 1];
-  const double bcc2 = 0.5 * b2[fo] + 0.5 * b2[fo + g.P];
-  const double bcc3 = 0.5 * b3[fo] + 0.5 * b3[fo + (int64_t)g.J * g.P];
-  double u_d = u[c], u_e = u[c + 4 * S], wv[5];
-  c2p_cell(u_d, u[c + S], u[c + 2 * S], u[c + 3 * S], u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
-  u[c] = u_d;
-  u[c + 4 * S] = u_e;
-}
-
-__global__ void k_reset_int(int *p) { *p = 0; }
 
 // Hydro::NewBlockTimeStep (hydro/new_blockdt.cpp:57-160): per-cell bound, min over all
 // interior cells of all blocks, one partial per workgroup; k_min_final reduces the
@@ -1640,9 +1566,12 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   if (e == hipSuccess) e = hipMalloc(&m->scal, sizeof(double) * 64);
   {
     dim3 gr = plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb);
-    dim3 gs = stage_grid(g);
     m->red_n = (int64_t)gr.x * gr.y * gr.z;
-    m->dtpart_n = (int64_t)gs.x * gs.y;
+    m->dtpart_n = 0;
+    for (int t = 0; t < NTILE; ++t) {
+      dim3 gs = stage_grid(g, t);
+      m->dtpart_n = std::max(m->dtpart_n, (int64_t)gs.x * gs.y);
+    }
   }
   if (e == hipSuccess) e = hipMalloc(&m->red, sizeof(double) * m->red_n);
   if (e == hipSuccess) e = hipMalloc(&m->dtpart, sizeof(double) * m->dtpart_n);
@@ -1987,10 +1916,10 @@ int agx_initialize(agx_mesh *m, double tlim) {
 }
 
 // One VL2 stage with the fused kernels (time_integrator.cpp:1460-2034): fluxes, then
-// k_stage (corner EMFs, CT, update, C2P of the interior and, when dtpart is set, the
-// time-step partials), register rotation, ghost faces, ghost cells, interior floors.
-// Same results as agx_calculate_fluxes + agx_compute_corner_e + agx_integrate +
-// agx_bvals_exchange + agx_cons2prim.
+// k_stage (corner EMFs, CT, update, C2P with floors, the neighbours' ghost copies of the
+// new cells and faces and, when dtpart is set, the time-step partials), then the
+// register rotation.  Same results as agx_calculate_fluxes + agx_compute_corner_e +
+// agx_integrate + agx_bvals_exchange + agx_cons2prim.
 //
 // Registers: stage 1 reads u and b and writes u1 and b1; the swap makes (u, b) the
 // half-step state and (u1, b1) the start state.  Stage 2 updates u1 in place from the
@@ -1998,7 +1927,7 @@ int agx_initialize(agx_mesh *m, double tlim) {
 // rotation leaves b = new faces and b1 = half-step faces, as the reference's swap does.
 extern "C++" {
 template <typename TW>
-static void launch_stage(agx_mesh *m, int stage, double wght, int *flag, double *dtpart) {
+static void launch_stage(agx_mesh *m, int stage, double wght, double *dtpart) {
   const Geo &g = m->g;
   double **A = m->arr.a;
   StageArgs<TW> a;
@@ -2024,36 +1953,23 @@ static void launch_stage(agx_mesh *m, int stage, double wght, int *flag, double 
   a.pfl = m->d.pfloor;
   a.gamma = m->d.gamma;
   a.grav = m->d.self_gravity ? 1 : 0;
-  a.floor_flag = flag;
   a.dtpart = dtpart;
-  hipLaunchKernelGGL((k_stage<STAGE_TI, STAGE_TJ, TW>), stage_grid(g), dim3(STAGE_TI * STAGE_TJ), 0,
-                     m->stream, g, a);
+  const dim3 grid = stage_grid(g, m->tile);
+  if (m->tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW>), grid, dim3(256), 0, m->stream, g, a);
+  else if (m->tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW>), grid, dim3(256), 0, m->stream, g, a);
+  else hipLaunchKernelGGL((k_stage<16, 16, TW>), grid, dim3(256), 0, m->stream, g, a);
 }
 }  // extern "C++"
 
-extern "C++" {
-template <typename TW>
-static void launch_ghost_cells(agx_mesh *m, const Shell &cs) {
-  const Geo &g = m->g;
-  double **A = m->arr.a;
-  hipLaunchKernelGGL(k_ghost_cells<TW>, dim3(grid1d(cs.nk + cs.nj + cs.ni, 256), 1, g.nb), dim3(256), 0,
-                     m->stream, g, cs, A[AGX_ARR_U], as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]),
-                     A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F], m->d.gamma - 1.0, m->d.dfloor,
-                     m->d.pfloor);
-}
-}  // extern "C++"
 
 static int stage_fused(agx_mesh *m, int stage, bool want_dt) {
-  const Geo &g = m->g;
   double **A = m->arr.a;
   int rc = agx_calculate_fluxes(m, stage == 1 ? 1 : m->d.xorder);
   if (rc) return rc;
   const double wght = (stage == 1 ? 0.5 : 1.0) * m->dt;
-  int *flag = reinterpret_cast<int *>(m->scal + 8);
   timer_begin(m, AGX_K_INTEGRATE);
-  hipLaunchKernelGGL(k_reset_int, dim3(1), dim3(1), 0, m->stream, flag);
-  if (m->mixed) launch_stage<float>(m, stage, wght, flag, want_dt ? m->dtpart : nullptr);
-  else launch_stage<double>(m, stage, wght, flag, want_dt ? m->dtpart : nullptr);
+  if (m->mixed) launch_stage<float>(m, stage, wght, want_dt ? m->dtpart : nullptr);
+  else launch_stage<double>(m, stage, wght, want_dt ? m->dtpart : nullptr);
   timer_end(m);
   LAUNCH_CHECK("k_stage");
   std::swap(A[AGX_ARR_U], A[AGX_ARR_U1]);
@@ -2069,24 +1985,6 @@ static int stage_fused(agx_mesh *m, int stage, bool want_dt) {
       A[ARR_Q1F + d] = start;
     }
   }
-  Shell f1 = make_shell(g.n1 + 1, g.n2, g.n3, g.is, g.ie + 1, g.js, g.je, g.ks, g.ke);
-  Shell f2 = make_shell(g.n1, g.n2 + 1, g.n3, g.is, g.ie, g.js, g.je + 1, g.ks, g.ke);
-  Shell f3 = make_shell(g.n1, g.n2, g.n3 + 1, g.is, g.ie, g.js, g.je, g.ks, g.ke + 1);
-  Shell cs = make_shell(g.n1, g.n2, g.n3, g.is, g.ie, g.js, g.je, g.ks, g.ke);
-  int64_t nf = std::max(f1.nk + f1.nj + f1.ni, std::max(f2.nk + f2.nj + f2.ni, f3.nk + f3.nj + f3.ni));
-  timer_begin(m, AGX_K_BVALS);
-  hipLaunchKernelGGL(k_ghost_faces, dim3(grid1d(nf, 256), 3, g.nb), dim3(256), 0, m->stream, g, f1, f2, f3,
-                     A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F]);
-  if (m->mixed) launch_ghost_cells<float>(m, cs);
-  else launch_ghost_cells<double>(m, cs);
-  timer_end(m);
-  LAUNCH_CHECK("k_ghost");
-  timer_begin(m, AGX_K_C2P);
-  hipLaunchKernelGGL(k_floor_fix, plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb), dim3(256), 0,
-                     m->stream, g, A[AGX_ARR_U], A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F],
-                     m->d.gamma - 1.0, m->d.dfloor, m->d.pfloor, flag);
-  timer_end(m);
-  LAUNCH_CHECK("k_floor_fix");
   return 0;
 }
 
@@ -2094,7 +1992,7 @@ int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
   if (!m) { set_error("null mesh"); return -1; }
   int32_t n = 0;
   int rc = 0;
-  const dim3 sg = stage_grid(m->g);
+  const dim3 sg = stage_grid(m->g, m->tile);
   while (n < ncycles && m->time < tlim) {
     for (int stage = 1; stage <= 2 && !rc; ++stage) {
       // VL2 stage 1 is donor cell (time_integrator.cpp:1466-1467), inside stage_fused;
@@ -2144,6 +2042,51 @@ int agx_mg_vcycles(agx_mesh *m, int32_t n) {
   if (!m) { set_error("null mesh"); return -1; }
   if (!m->mg_on) { set_error("mesh created without self-gravity"); return -2; }
   return mg_vcycles(m, n);
+}
+
+// Tile-shape autotuner of the stage kernel (no counterpart in the reference).  Times
+// every tile shape on the current state with stage-1 launches, which read u, w, bcc, b
+// and the fluxes and write only the scratch registers (u1, b1, the second w/bcc), so
+// the state is unchanged; keeps the fastest.  Call between cycles.
+int agx_autotune(agx_mesh *m, int32_t *choice) {
+  if (!m) { set_error("null mesh"); return -1; }
+  int rc = agx_calculate_fluxes(m, 1);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  AGX_HIP(hipEventCreate(&e0));
+  AGX_HIP(hipEventCreate(&e1));
+  const double wght = 0.5 * m->dt;
+  float best = 0.f;
+  int best_t = 0;
+  const bool timing = m->timer.enabled;
+  m->timer.enabled = false;
+  for (int t = 0; t < NTILE; ++t) {
+    m->tile = t;
+    float ms = 0.f;
+    for (int rep = 0; rep < 4; ++rep) {
+      if (rep == 1) AGX_HIP(hipEventRecord(e0, m->stream));
+      if (m->mixed) launch_stage<float>(m, 1, wght, nullptr);
+      else launch_stage<double>(m, 1, wght, nullptr);
+    }
+    AGX_HIP(hipEventRecord(e1, m->stream));
+    AGX_HIP(hipEventSynchronize(e1));
+    AGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (t == 0 || ms < best) { best = ms; best_t = t; }
+  }
+  m->timer.enabled = timing;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  LAUNCH_CHECK("k_stage (autotune)");
+  m->tile = best_t;
+  if (choice) *choice = best_t;
+  return 0;
+}
+
+int agx_set_tile(agx_mesh *m, int32_t tile) {
+  if (!m) { set_error("null mesh"); return -1; }
+  if (tile < 0 || tile >= NTILE) { set_error("tile shape %d not in [0, %d)", tile, NTILE); return -2; }
+  m->tile = tile;
+  return 0;
 }
 
 int agx_timing_enable(agx_mesh *m, int enable) {

@@ -91,8 +91,9 @@ def mhd_bench(agx, pgen, args, precision):
     m.set("b3f", f3)
     del f1, f2, f3, b1, b2, b3
     m.initialize()
+    tile = m.autotune()
     log(f"[bench] {precision} MHD {N}^3 in {desc.nx1 // B}^3 blocks of {B}^3 set up in {time.time() - t0:.1f}s, "
-        f"device {m.device_bytes() / 1e9:.2f} GB")
+        f"device {m.device_bytes() / 1e9:.2f} GB, stage tile {('16x16', '32x8', '64x4')[tile]}")
     m.cycle(1e30, args.warmup)
     m.sync()
     t0 = time.perf_counter()
@@ -135,10 +136,11 @@ def roofline(desc, per, precision):
             "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": alg}
 
 
-def mg_bench(agx, pgen, args):
+def mg_bench(agx, pgen, args, precision="fp64"):
     N, B = args.n, args.block
     desc = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, self_gravity=1,
-                         four_pi_G=1.0, mg_mode=0, mg_threshold=0.0)
+                         four_pi_G=1.0, mg_mode=0, mg_threshold=0.0,
+                         precision=1 if precision == "mixed" else 0)
     rho = pgen.blobs(desc, seed=3)
     U = np.zeros((5,) + rho.shape)
     U[0] = rho
@@ -153,7 +155,8 @@ def mg_bench(agx, pgen, args):
     m.sync()
     t_solve = time.perf_counter() - t0
     norm, iters = m.mg_last()
-    log(f"[bench] MG FMG+{iters} V-cycles to {norm:.3e} (target {1e-10 * src_norm:.3e}) in {t_solve * 1e3:.1f} ms")
+    log(f"[bench] {precision} MG FMG+{iters} V-cycles to {norm:.3e} (target {1e-10 * src_norm:.3e}) "
+        f"in {t_solve * 1e3:.1f} ms")
     m.mg_vcycles(2)
     m.sync()
     nv = args.mg_cycles
@@ -169,7 +172,7 @@ def mg_bench(agx, pgen, args):
     per = {c: m.timing_get(c) for c in ("mg_smooth", "mg_restrict", "mg_prolong", "mg_other")}
     m.timing(False)
     m.close()
-    log(f"[bench] MG {nv} V-cycles in {sec * 1e3:.1f} ms -> {vps:.1f} V-cycles/s")
+    log(f"[bench] {precision} MG {nv} V-cycles in {sec * 1e3:.1f} ms -> {vps:.1f} V-cycles/s")
     return vps, iters, norm, t_solve, per
 
 
@@ -277,6 +280,14 @@ def main():
         out["mg_solve_ms_to_1e-10"] = t_solve * 1e3
         out["mg_solve_vcycles_after_fmg"] = iters
         out["mg_kernel_ms_per_vcycle"] = {k: v[0] / 3 for k, v in mper.items()}
+        if not args.no_mixed:
+            vps32, it32, norm32, ts32, mper32 = mg_bench(agx, pgen, args, "mixed")
+            out["mg_mixed"] = {
+                "vcycles_per_s": vps32, "vs_fp64": vps32 / vps,
+                "note": "one cycle = FP64 defect + FP32 V(1,1) + FP64 correction",
+                "roofline_frac": vps32 * float(args.n) ** 3 * 114.3 / HBM_PEAK,
+                "solve_ms_to_1e-10": ts32 * 1e3, "cycles_after_fmg": it32, "final_defect": norm32,
+                "kernel_ms_per_cycle": {k: v[0] / 3 for k, v in mper32.items()}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

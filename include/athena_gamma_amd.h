@@ -48,7 +48,7 @@ typedef struct agx_mesh_desc {
   int32_t self_gravity;         /* configure --grav: 0 none, 1 multigrid */
   int32_t mg_mode;              /* <gravity>/mgmode: 0 FMG, 1 MGI */
   int32_t mg_niter;             /* <gravity>/niteration (used when threshold < 0) */
-  int32_t precision;            /* AGX_PREC_FP64 or AGX_PREC_MIXED */
+  int32_t precision;            /* AGX_PREC_FP64 or AGX_PREC_MIXED (see below) */
   double x1min, x1max, x2min, x2max, x3min, x3max;   /* <mesh>/x?min, x?max */
   double gamma;                 /* <hydro>/gamma */
   double iso_cs;                /* <hydro>/iso_sound_speed (isothermal EOS) */
@@ -59,6 +59,14 @@ typedef struct agx_mesh_desc {
 } agx_mesh_desc;
 
 enum { AGX_RSOLVER_HLLD = 0, AGX_RSOLVER_HLLE = 1, AGX_RSOLVER_ROE = 2 };
+/* Precision forms.  FP64: every array and every operation in FP64, results identical to
+ * the reference's FP64 arithmetic.  MIXED: reconstruction and the Riemann solver run in
+ * FP32 from FP32 copies of w and bcc; fluxes, face EMFs and CT weights are stored in FP32;
+ * the conserved-variable update, self-gravity source, C2P with floors, corner EMFs, CT
+ * and the face field stay FP64 (div B at FP64 round-off).  The FP32 arrays (AGX_ARR_W,
+ * AGX_ARR_BCC, AGX_ARR_FX?, AGX_ARR_E?X?, AGX_ARR_WX?) are converted by agx_upload /
+ * agx_download, so the host side stays FP64.  In MIXED, agx_compute_corner_e and
+ * agx_integrate (task-level FP64 steps) return an error; agx_cycle runs the fused form. */
 enum { AGX_PREC_FP64 = 0, AGX_PREC_MIXED = 1 };
 
 /* Registered arrays (MeshBlock members in the reference). */
@@ -136,7 +144,10 @@ int agx_new_time_step(agx_mesh *m, double tlim);
 int agx_initialize(agx_mesh *m, double tlim);
 /* ncycles iterations of the main loop (main.cpp:430-489): both VL2 stages with a
  * gravity solve after each (main.cpp:459-466), time += dt, Mesh::NewTimeStep.
- * Stops early once time >= tlim.  Returns the number of cycles run in *done. */
+ * Stops early once time >= tlim.  Returns the number of cycles run in *done.  Each stage
+ * runs as fluxes + one fused kernel (corner EMFs, CT, update, C2P, and in the second
+ * stage the time-step bound) + ghost exchange: the same values as the task-level
+ * entry points above, in fewer passes over HBM. */
 int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done);
 
 /* ---- multigrid self-gravity */
@@ -152,6 +163,15 @@ int agx_mg_last(agx_mesh *m, double *norm, int32_t *iters);
  * (MultigridDriver::SolveVCycle, multigrid_driver.cpp:684-693): the unit of the
  * V-cycles/s benchmark. */
 int agx_mg_vcycles(agx_mesh *m, int32_t n);
+
+/* ---- tuning (no counterpart in the reference: launch shapes of this library).
+ * agx_autotune times each tile shape of the fused stage kernel on the current state
+ * (16x16, 32x8, 64x4 (i,j) columns per workgroup) and keeps the fastest; *choice gets
+ * its index.  It writes only scratch registers (u1, b1, fluxes), so call it between
+ * cycles, e.g. after agx_initialize.  agx_set_tile forces a shape.  Results do not
+ * depend on the shape. */
+int agx_autotune(agx_mesh *m, int32_t *choice);
+int agx_set_tile(agx_mesh *m, int32_t tile);
 
 /* ---- built-in timers (HIP events on the mesh stream around each launch) */
 int agx_timing_enable(agx_mesh *m, int enable);

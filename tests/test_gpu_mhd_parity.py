@@ -195,3 +195,23 @@ def test_ppm_fluxes_and_cycles(agx, rsolver):
     m.cycle(1e9, 3)
     for n in ("u", "w", "b1f", "b2f", "b3f"):
         close(m.get(n), o.get(n), n)
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("case", ["turb_8blk", "turb_ragged"])
+def test_cycles_every_tile_shape(agx, case, tile):
+    """Every tile shape of the fused stage kernel gives the oracle's cycles bit for bit,
+    and the autotuner leaves the state untouched."""
+    o, m = build_case(agx, case)
+    o.initialize()
+    m.initialize()
+    u0 = m.get("u")
+    choice = m.autotune()
+    assert 0 <= choice <= 2
+    assert np.array_equal(m.get("u"), u0)
+    m.set_tile(tile)
+    for _ in range(3):
+        o.cycle(1e9)
+    assert m.cycle(1e9, 3) == 3
+    for n in ("u", "w", "bcc", "b1f", "b2f", "b3f"):
+        close(m.get(n), o.get(n), f"{n} tile {tile}", rtol=0.0 if n != "bcc" else RTOL)

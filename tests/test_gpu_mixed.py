@@ -46,7 +46,10 @@ def test_mixed_fluxes(agx, rsolver):
     for n in ("fx1", "fx2", "fx3", "e3x1", "e2x1", "e1x2", "e3x2", "e1x3", "e2x3", "wx1", "wx2", "wx3"):
         e = rel_err(m.get(n), o.get(n))
         print(f"{n}: max rel err {e:.3e}", flush=True)
-        assert e <= 1e-5, n
+        # the CT weight is 0.5 + clamp(1024 dt F_rho / (dx (rho_l + rho_r))) (hydro.cpp:154-158):
+        # its argument amplifies the FP32 flux rounding by 1024, so its bar is 1e-4 absolute
+        # on a quantity in [0, 1]
+        assert e <= (1e-4 if n.startswith("wx") else 1e-5), n
 
 
 @pytest.mark.parametrize("rsolver", [0, 1])
@@ -80,3 +83,77 @@ def test_mixed_turbulence_run(agx, rsolver, shape):
     for n in ("b1f", "b2f", "b3f"):
         o2.set(n, m.get(n))
     assert o2.max_divb() < 1e-12
+
+
+@pytest.mark.parametrize("N,B", [(32, 8), (64, 16)])
+def test_mixed_mg_reaches_fp64_accuracy(agx, N, B):
+    """FP32 V-cycles inside the FP64 defect correction converge to the FP64 solution:
+    with threshold 0 (iterate until the defect stalls, multigrid_driver.cpp:738-745) the
+    potential matches the discrete FFT solution to 1e-10 of its maximum, as the FP64
+    path does to 1e-11."""
+    from tests.test_oracle_mg import blobs, fft_solution
+    rho = blobs(N, 1)
+    d = make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0, 1),) * 3, self_gravity=1, mg_mode=0,
+                  mg_threshold=0.0, four_pi_G=0.7, precision=1)
+    U = np.zeros((5, N, N, N))
+    U[0] = rho
+    m = agx.Mesh(d)
+    m.set("u", agx.scatter_cells(d, U))
+    m.mg_solve(0)
+    phi = agx.gather_cells(d, m.get("phi"))[0]
+    norm, iters = m.mg_last()
+    ref = fft_solution(rho, 0.7)
+    err = np.abs(phi - ref).max() / np.abs(ref).max()
+    print(f"N={N} B={B}: {iters} correction cycles, defect {norm:.3e}, max rel err {err:.3e}", flush=True)
+    assert err <= 1e-10
+    assert iters <= 30
+
+
+def test_mixed_mg_threshold_matches_oracle(agx):
+    """Absolute threshold 1e-10 x ||source||: the mixed solve stops below the threshold and
+    its potential agrees with the FP64 oracle's within the error the threshold allows."""
+    from tests.test_oracle_mg import blobs
+    N, B = 32, 8
+    rho = blobs(N, 3)
+    src_norm = float(np.sqrt(np.mean((rho - rho.mean()) ** 2)))
+    kw = dict(nx=(N, N, N), bnx=(B, B, B), box=((0, 1),) * 3, self_gravity=1, mg_mode=0,
+              mg_threshold=1e-10 * src_norm, four_pi_G=1.0)
+    U = np.zeros((5, N, N, N))
+    U[0] = rho
+    o = Oracle(make_desc(**kw))
+    ub = agx.scatter_cells(o.desc, U)
+    o.set("u", ub)
+    o.mg_solve(0)
+    m = agx.Mesh(make_desc(precision=1, **kw))
+    m.set("u", ub)
+    m.mg_solve(0)
+    norm, iters = m.mg_last()
+    assert norm <= 1e-10 * src_norm
+    a, b = m.get("phi"), o.get("phi")
+    err = np.abs(a - b).max() / np.abs(b).max()
+    print(f"mixed iters {iters} (oracle {o.mg_last()[1]}), norm {norm:.3e}, rel diff {err:.3e}", flush=True)
+    assert err <= 1e-8
+
+
+def test_mixed_selfgravity_cycles(agx):
+    """Coupled MHD + multigrid gravity in the mixed form against the FP64 oracle."""
+    kw = dict(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, self_gravity=1,
+              four_pi_G=4.0, mg_mode=0, mg_threshold=0.0)
+    d = make_desc(**kw)
+    st = turbulent_state(d, seed=21, mach=1.0, drho=0.4)
+    o = Oracle(d)
+    m = agx.Mesh(make_desc(precision=1, **kw))
+    load_state(o, st)
+    load_state(m, st)
+    tlim = 0.02
+    o.initialize()
+    m.initialize(tlim)
+    o.run(tlim)
+    m.run(tlim)
+    ng = d.nghost
+    wm, wo = interior(m.get("w"), ng), interior(o.get("w"), ng)
+    erho, ep = rel_err(wm[:, 0], wo[:, 0]), rel_err(wm[:, 4], wo[:, 4])
+    g1 = slice(ng, -ng)
+    ephi = rel_err(m.get("phi")[:, :, g1, g1, g1], o.get("phi")[:, :, g1, g1, g1])
+    print(f"cycles {m.time()[2]}/{o.time()[2]}: rho {erho:.3e} P {ep:.3e} phi {ephi:.3e}", flush=True)
+    assert erho <= RHO_P_TOL and ep <= RHO_P_TOL and ephi <= RHO_P_TOL
