@@ -49,15 +49,15 @@ class Desc(ctypes.Structure):
 def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)),
               nghost=2, xorder=2, rsolver=0, gamma=1.666666666666667, cfl=0.3,
               self_gravity=0, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=10,
-              precision=0):
+              precision=0, eos=0, iso_cs=1.0):
     bnx = bnx or nx
     d = Desc()
     d.nx1, d.nx2, d.nx3 = nx
     d.bnx1, d.bnx2, d.bnx3 = bnx
-    d.nghost, d.xorder, d.rsolver, d.eos = nghost, xorder, rsolver, 0
+    d.nghost, d.xorder, d.rsolver, d.eos = nghost, xorder, rsolver, eos
     d.self_gravity, d.mg_mode, d.mg_niter, d.precision = self_gravity, mg_mode, mg_niter, precision
     (d.x1min, d.x1max), (d.x2min, d.x2max), (d.x3min, d.x3max) = box
-    d.gamma, d.iso_cs, d.cfl = gamma, 1.0, cfl
+    d.gamma, d.iso_cs, d.cfl = gamma, iso_cs, cfl
     # EquationOfState defaults sqrt(1024*FLT_MIN), eos/adiabatic_mhd.cpp:26-31
     d.dfloor = d.pfloor = float(np.sqrt(1024 * np.float64(np.finfo(np.float32).tiny)))
     d.four_pi_G, d.mg_threshold = four_pi_G, mg_threshold

@@ -88,6 +88,7 @@ struct agx_mesh {
   agx::Geo g;
   agx::Arrays arr;
   bool mixed = false;                // AGX_PREC_MIXED
+  bool iso = false;                  // isothermal EOS (NHYDRO = 4)
   int tile = 0;                      // stage-kernel tile shape (agx_autotune)
   hipStream_t stream = nullptr;
   double *pool = nullptr;            // all MHD arrays

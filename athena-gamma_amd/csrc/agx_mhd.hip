@@ -127,6 +127,22 @@ __device__ __forceinline__ T fast_speed(T gamma, const T *prim, T bx) {
   return xsqrt(T(0.5) * (qsq + xsqrt(tmp * tmp + T(4.0) * asq * ct2)) / prim[IDN]);
 }
 
+// EquationOfState::FastMagnetosonicSpeed, eos/isothermal_mhd.cpp:122-129
+template <typename T>
+__device__ __forceinline__ T fast_speed_iso(T cs, const T *prim, T bx) {
+  T asq = (cs * cs) * prim[IDN];
+  T vaxsq = bx * bx;
+  T ct2 = prim[IBY] * prim[IBY] + prim[IBZ] * prim[IBZ];
+  T qsq = vaxsq + ct2 + asq;
+  T tmp = vaxsq + ct2 - asq;
+  return xsqrt(T(0.5) * (qsq + xsqrt(tmp * tmp + T(4.0) * asq * ct2)) / prim[IDN]);
+}
+
+template <bool ISO, typename T>
+__device__ __forceinline__ T fast_speed_eos(T gamma, T cs, const T *prim, T bx) {
+  return ISO ? fast_speed_iso(cs, prim, bx) : fast_speed(gamma, prim, bx);
+}
+
 // Hydro::GetWeightForCT, hydro/hydro.cpp:154-158
 template <typename T>
 __device__ __forceinline__ T ct_weight(T dflx, T rhol, T rhor, T dx,
@@ -320,9 +336,10 @@ __device__ __forceinline__ void hlld(T gamma, const T *wli, const T *wri,
   }
 }
 
-// HLLE for adiabatic MHD at one interface, hydro/rsolvers/mhd/hlle_mhd.cpp:56-171.
-template <typename T>
-__device__ __forceinline__ void hlle(T gamma, const T *wli, const T *wri,
+// HLLE at one interface, hydro/rsolvers/mhd/hlle_mhd.cpp:56-171: adiabatic, or the
+// isothermal branches (ISO; :83-87, :98-106, :140-150, energy flux 0).
+template <bool ISO, typename T>
+__device__ __forceinline__ void hlle(T gamma, T iso_cs, const T *wli, const T *wri,
                                      T bxi, T *flxi) {
   T gm1 = gamma - T(1.0);
   T wroe[NWAVE], fl[NWAVE], fr[NWAVE];
@@ -339,17 +356,26 @@ __device__ __forceinline__ void hlle(T gamma, const T *wli, const T *wri,
   T y = T(0.5) * (wli[IDN] + wri[IDN]) / wroe[IDN];
   T pbl = T(0.5) * (bxi * bxi + SQR(wli[IBY]) + SQR(wli[IBZ]));
   T pbr = T(0.5) * (bxi * bxi + SQR(wri[IBY]) + SQR(wri[IBZ]));
-  T el = wli[IPR] / gm1 + T(0.5) * wli[IDN] * (SQR(wli[IVX]) + SQR(wli[IVY]) + SQR(wli[IVZ])) + pbl;
-  T er = wri[IPR] / gm1 + T(0.5) * wri[IDN] * (SQR(wri[IVX]) + SQR(wri[IVY]) + SQR(wri[IVZ])) + pbr;
-  T hroe = ((el + wli[IPR] + pbl) / sqrtdl + (er + wri[IPR] + pbr) / sqrtdr) * isdlpdr;
-  T cl = fast_speed(gamma, wli, bxi);
-  T cr = fast_speed(gamma, wri, bxi);
+  T el = T(0.0), er = T(0.0), hroe = T(0.0);
+  if (!ISO) {
+    el = wli[IPR] / gm1 + T(0.5) * wli[IDN] * (SQR(wli[IVX]) + SQR(wli[IVY]) + SQR(wli[IVZ])) + pbl;
+    er = wri[IPR] / gm1 + T(0.5) * wri[IDN] * (SQR(wri[IVX]) + SQR(wri[IVY]) + SQR(wri[IVZ])) + pbr;
+    hroe = ((el + wli[IPR] + pbl) / sqrtdl + (er + wri[IPR] + pbr) / sqrtdr) * isdlpdr;
+  }
+  T cl = fast_speed_eos<ISO>(gamma, iso_cs, wli, bxi);
+  T cr = fast_speed_eos<ISO>(gamma, iso_cs, wri, bxi);
   T btsq = SQR(wroe[IBY]) + SQR(wroe[IBZ]);
   T vaxsq = bxi * bxi / wroe[IDN];
-  T bt_starsq = (gm1 - (gm1 - T(1.0)) * y) * btsq;
-  T hp = hroe - (vaxsq + btsq / wroe[IDN]);
-  T vsq = SQR(wroe[IVX]) + SQR(wroe[IVY]) + SQR(wroe[IVZ]);
-  T twid_asq = dmax((gm1 * (hp - T(0.5) * vsq) - (gm1 - T(1.0)) * x), T(0.0));
+  T bt_starsq, twid_asq;
+  if (!ISO) {
+    bt_starsq = (gm1 - (gm1 - T(1.0)) * y) * btsq;
+    T hp = hroe - (vaxsq + btsq / wroe[IDN]);
+    T vsq = SQR(wroe[IVX]) + SQR(wroe[IVY]) + SQR(wroe[IVZ]);
+    twid_asq = dmax((gm1 * (hp - T(0.5) * vsq) - (gm1 - T(1.0)) * x), T(0.0));
+  } else {
+    bt_starsq = btsq * y;
+    twid_asq = iso_cs * iso_cs + x;
+  }
   T ct2 = bt_starsq / wroe[IDN];
   T tsum = vaxsq + ct2 + twid_asq;
   T tdif = vaxsq + ct2 - twid_asq;
@@ -370,12 +396,19 @@ __device__ __forceinline__ void hlle(T gamma, const T *wli, const T *wri,
   fr[IVY] = wri[IDN] * wri[IVY] * vxr - bxi * wri[IBY];
   fl[IVZ] = wli[IDN] * wli[IVZ] * vxl - bxi * wli[IBZ];
   fr[IVZ] = wri[IDN] * wri[IVZ] * vxr - bxi * wri[IBZ];
-  fl[IVX] += wli[IPR];
-  fr[IVX] += wri[IPR];
-  fl[IEN] = el * vxl + wli[IVX] * (wli[IPR] + pbl - bxi * bxi);
-  fr[IEN] = er * vxr + wri[IVX] * (wri[IPR] + pbr - bxi * bxi);
-  fl[IEN] -= bxi * (wli[IBY] * wli[IVY] + wli[IBZ] * wli[IVZ]);
-  fr[IEN] -= bxi * (wri[IBY] * wri[IVY] + wri[IBZ] * wri[IVZ]);
+  if (!ISO) {
+    fl[IVX] += wli[IPR];
+    fr[IVX] += wri[IPR];
+    fl[IEN] = el * vxl + wli[IVX] * (wli[IPR] + pbl - bxi * bxi);
+    fr[IEN] = er * vxr + wri[IVX] * (wri[IPR] + pbr - bxi * bxi);
+    fl[IEN] -= bxi * (wli[IBY] * wli[IVY] + wli[IBZ] * wli[IVZ]);
+    fr[IEN] -= bxi * (wri[IBY] * wri[IVY] + wri[IBZ] * wri[IVZ]);
+  } else {
+    fl[IVX] += (iso_cs * iso_cs) * wli[IDN];
+    fr[IVX] += (iso_cs * iso_cs) * wri[IDN];
+    fl[IEN] = T(0.0);
+    fr[IEN] = T(0.0);
+  }
   fl[IBY] = wli[IBY] * vxl - bxi * wli[IVY];
   fr[IBY] = wri[IBY] * vxr - bxi * wri[IVY];
   fl[IBZ] = wli[IBZ] * vxl - bxi * wli[IVZ];
@@ -384,6 +417,128 @@ __device__ __forceinline__ void hlle(T gamma, const T *wli, const T *wri,
   if (bp != bm) tmp = T(0.5) * (bp + bm) / (bp - bm);
 #pragma unroll
   for (int n = 0; n < NWAVE; ++n) flxi[n] = T(0.5) * (fl[n] + fr[n]) + (fl[n] - fr[n]) * tmp;
+}
+
+// HLLD for isothermal MHD (Mignone 2007) at one interface,
+// hydro/rsolvers/mhd/hlld_iso.cpp:37-254; SMALL_NUMBER = 1e-8 (hlld_iso.cpp:31).
+template <typename T>
+struct ConsIso {
+  T d, mx, my, mz, by, bz;
+};
+
+template <typename T>
+__device__ __forceinline__ void hlld_iso(T cs, T dfloor, const T *wli, const T *wri, T bxi,
+                                         T *flxi) {
+  const T SMALL = T(1.0e-8);
+  ConsIso<T> ul, ur, ulst, urst, ucst, fl, fr;
+  T spd[5];
+  ul.d = wli[IDN];
+  ul.mx = wli[IVX] * ul.d;
+  ul.my = wli[IVY] * ul.d;
+  ul.mz = wli[IVZ] * ul.d;
+  ul.by = wli[IBY];
+  ul.bz = wli[IBZ];
+  ur.d = wri[IDN];
+  ur.mx = wri[IVX] * ur.d;
+  ur.my = wri[IVY] * ur.d;
+  ur.mz = wri[IVZ] * ur.d;
+  ur.by = wri[IBY];
+  ur.bz = wri[IBZ];
+  T cfl = fast_speed_iso(cs, wli, bxi);
+  T cfr = fast_speed_iso(cs, wri, bxi);
+  spd[0] = dmin(wli[IVX] - cfl, wri[IVX] - cfr);
+  spd[4] = dmax(wli[IVX] + cfl, wri[IVX] + cfr);
+  T bxsq = bxi * bxi;
+  T ptl = SQR(cs) * wli[IDN] + T(0.5) * (bxsq + SQR(wli[IBY]) + SQR(wli[IBZ]));
+  T ptr = SQR(cs) * wri[IDN] + T(0.5) * (bxsq + SQR(wri[IBY]) + SQR(wri[IBZ]));
+  fl.d = ul.mx;
+  fl.mx = ul.mx * wli[IVX] + ptl - bxsq;
+  fl.my = ul.my * wli[IVX] - bxi * ul.by;
+  fl.mz = ul.mz * wli[IVX] - bxi * ul.bz;
+  fl.by = ul.by * wli[IVX] - bxi * wli[IVY];
+  fl.bz = ul.bz * wli[IVX] - bxi * wli[IVZ];
+  fr.d = ur.mx;
+  fr.mx = ur.mx * wri[IVX] + ptr - bxsq;
+  fr.my = ur.my * wri[IVX] - bxi * ur.by;
+  fr.mz = ur.mz * wri[IVX] - bxi * ur.bz;
+  fr.by = ur.by * wri[IVX] - bxi * wri[IVY];
+  fr.bz = ur.bz * wri[IVX] - bxi * wri[IVZ];
+  // HLL averages, Mignone eqns. (15), (17), (29)
+  T idspd = T(1.0) / (spd[4] - spd[0]);
+  T dhll = (spd[4] * ur.d - spd[0] * ul.d - fr.d + fl.d) * idspd;
+  dhll = dmax(dhll, dfloor);
+  T sqrtdhll = xsqrt(dhll);
+  T fdhll = (spd[4] * fl.d - spd[0] * fr.d + spd[4] * spd[0] * (ur.d - ul.d)) * idspd;
+  T fmxhll = (spd[4] * fl.mx - spd[0] * fr.mx + spd[4] * spd[0] * (ur.mx - ul.mx)) * idspd;
+  T ustar = fdhll / dhll;
+  T mxhll = (spd[4] * ur.mx - spd[0] * ul.mx - fr.mx + fl.mx) * idspd;
+  spd[1] = ustar - xabs(bxi) / sqrtdhll;
+  spd[3] = ustar + xabs(bxi) / sqrtdhll;
+  // Ul*, Ur*: eqns. (20), (24), (30)-(33)
+  ulst.d = dhll;
+  ulst.mx = mxhll;
+  T tmp = (spd[0] - spd[1]) * (spd[0] - spd[3]);
+  if (xabs(spd[0] - spd[1]) < (SMALL) * cs) {
+    ulst.my = ul.my; ulst.mz = ul.mz; ulst.by = ul.by; ulst.bz = ul.bz;
+  } else {
+    T mfact = bxi * (ustar - wli[IVX]) / tmp;
+    T bfact = (ul.d * SQR(spd[0] - wli[IVX]) - bxsq) / (dhll * tmp);
+    ulst.my = dhll * wli[IVY] - ul.by * mfact;
+    ulst.mz = dhll * wli[IVZ] - ul.bz * mfact;
+    ulst.by = ul.by * bfact;
+    ulst.bz = ul.bz * bfact;
+  }
+  urst.d = dhll;
+  urst.mx = mxhll;
+  tmp = (spd[4] - spd[1]) * (spd[4] - spd[3]);
+  if (xabs(spd[4] - spd[3]) < (SMALL) * cs) {
+    urst.my = ur.my; urst.mz = ur.mz; urst.by = ur.by; urst.bz = ur.bz;
+  } else {
+    T mfact = bxi * (ustar - wri[IVX]) / tmp;
+    T bfact = (ur.d * SQR(spd[4] - wri[IVX]) - bxsq) / (dhll * tmp);
+    urst.my = dhll * wri[IVY] - ur.by * mfact;
+    urst.mz = dhll * wri[IVZ] - ur.bz * mfact;
+    urst.by = ur.by * bfact;
+    urst.bz = ur.bz * bfact;
+  }
+  // Uc*, eqns. (34)-(37)
+  T x = sqrtdhll * (bxi > T(0.0) ? T(1.0) : T(-1.0));
+  ucst.d = dhll;
+  ucst.mx = mxhll;
+  ucst.my = T(0.5) * (ulst.my + urst.my + (urst.by - ulst.by) * x);
+  ucst.mz = T(0.5) * (ulst.mz + urst.mz + (urst.bz - ulst.bz) * x);
+  ucst.by = T(0.5) * (ulst.by + urst.by + (urst.my - ulst.my) / x);
+  ucst.bz = T(0.5) * (ulst.bz + urst.bz + (urst.mz - ulst.mz) / x);
+  // flux selection, eqns. (38a)-(38e)
+  if (spd[0] >= T(0.0)) {
+    flxi[IDN] = fl.d; flxi[IVX] = fl.mx; flxi[IVY] = fl.my; flxi[IVZ] = fl.mz;
+    flxi[IBY] = fl.by; flxi[IBZ] = fl.bz;
+  } else if (spd[4] <= T(0.0)) {
+    flxi[IDN] = fr.d; flxi[IVX] = fr.mx; flxi[IVY] = fr.my; flxi[IVZ] = fr.mz;
+    flxi[IBY] = fr.by; flxi[IBZ] = fr.bz;
+  } else if (spd[1] >= T(0.0)) {
+    flxi[IDN] = fl.d + spd[0] * (ulst.d - ul.d);
+    flxi[IVX] = fl.mx + spd[0] * (ulst.mx - ul.mx);
+    flxi[IVY] = fl.my + spd[0] * (ulst.my - ul.my);
+    flxi[IVZ] = fl.mz + spd[0] * (ulst.mz - ul.mz);
+    flxi[IBY] = fl.by + spd[0] * (ulst.by - ul.by);
+    flxi[IBZ] = fl.bz + spd[0] * (ulst.bz - ul.bz);
+  } else if (spd[3] <= T(0.0)) {
+    flxi[IDN] = fr.d + spd[4] * (urst.d - ur.d);
+    flxi[IVX] = fr.mx + spd[4] * (urst.mx - ur.mx);
+    flxi[IVY] = fr.my + spd[4] * (urst.my - ur.my);
+    flxi[IVZ] = fr.mz + spd[4] * (urst.mz - ur.mz);
+    flxi[IBY] = fr.by + spd[4] * (urst.by - ur.by);
+    flxi[IBZ] = fr.bz + spd[4] * (urst.bz - ur.bz);
+  } else {
+    flxi[IDN] = dhll * ustar;
+    flxi[IVX] = fmxhll;
+    flxi[IVY] = ucst.my * ustar - bxi * ucst.by;
+    flxi[IVZ] = ucst.mz * ustar - bxi * ucst.bz;
+    flxi[IBY] = ucst.by * ustar - bxi * ucst.my / ucst.d;
+    flxi[IBZ] = ucst.bz * ustar - bxi * ucst.mz / ucst.d;
+  }
+  flxi[IEN] = T(0.0);
 }
 
 // SIGN, defs.hpp.in:117
@@ -702,18 +857,19 @@ __device__ __forceinline__ void roe(T gamma, const T *wli, const T *wri,
 // reconstruction (ORDER 1: reconstruct/dc.cpp:24-104; ORDER 2: the uniform-mesh van
 // Leer branch of reconstruct/plm.cpp:27-351), Riemann solver, face EMFs and the CT
 // upwind weight.  One thread per face.
-template <int DIR, int ORDER, int RS, typename T>
+template <int DIR, int ORDER, int RS, typename T, bool ISO>
 __global__ void __launch_bounds__(256)
 k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
        const double *__restrict__ bn, T *__restrict__ flx, T *__restrict__ ey,
        T *__restrict__ ez, T *__restrict__ wct, const BlockCoef *__restrict__ coef,
-       const double *__restrict__ plm, double gamma_, double dt, double dfl_, double pfl_) {
+       const double *__restrict__ plm, double gamma_, double cs_, double dt, double dfl_,
+       double pfl_) {
   int il = g.is - (DIR != 0), iu = g.ie + 1;
   int jl = g.js - (DIR != 1), ju = g.je + 1;
   int kl = g.ks - (DIR != 2);
   PlaneIdx p;
   if (!plane_index(il, iu, jl, ju, kl, p)) return;
-  const T gamma = T(gamma_), dfl = T(dfl_), pfl = T(pfl_);
+  const T gamma = T(gamma_), cs = T(cs_), dfl = T(dfl_), pfl = T(pfl_);
   const int64_t st = (DIR == 0) ? 1 : (DIR == 1 ? (int64_t)g.P : (int64_t)g.J * g.P);
   const int64_t id = IDX(g, p.k, p.j, p.i);
   const T *wb = w + (int64_t)p.b * NHYDRO * g.S;
@@ -726,6 +882,10 @@ k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
   const double *pc = plm + ((int64_t)p.b * 3 + DIR) * 2 * g.PL;  // [dxp][dxm] rows
 #pragma unroll
   for (int n = 0; n < NWAVE; ++n) {
+    if (ISO && n == IPR) {   // no pressure in the isothermal state vector
+      wl[n] = wr[n] = T(0.0);
+      continue;
+    }
     const T *q;
     if (n < NHYDRO) q = wb + (int64_t)n * g.S + id;
     else q = bb + (int64_t)((n == IBY) ? TB1 : TB2) * g.S + id;
@@ -759,11 +919,14 @@ k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
     }
   }
   if (ORDER == 3) {
-    // ApplyPrimitiveFloors on both reconstructed states (ppm.cpp:333-337)
+    // ApplyPrimitiveFloors on both reconstructed states (ppm.cpp:333-337); the
+    // isothermal EOS floors the density only (eos/isothermal_mhd.cpp:135-140)
     wl[IDN] = (wl[IDN] > dfl) ? wl[IDN] : dfl;
-    wl[IPR] = (wl[IPR] > pfl) ? wl[IPR] : pfl;
     wr[IDN] = (wr[IDN] > dfl) ? wr[IDN] : dfl;
-    wr[IPR] = (wr[IPR] > pfl) ? wr[IPR] : pfl;
+    if (!ISO) {
+      wl[IPR] = (wl[IPR] > pfl) ? wl[IPR] : pfl;
+      wr[IPR] = (wr[IPR] > pfl) ? wr[IPR] : pfl;
+    }
   }
   constexpr int ivx = IVX + DIR, ivy = IVX + (DIR + 1) % 3, ivz = IVX + (DIR + 2) % 3;
   T wli[NWAVE], wri[NWAVE], flxi[NWAVE];
@@ -772,7 +935,8 @@ k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
   wri[IDN] = wr[IDN]; wri[IVX] = wr[ivx]; wri[IVY] = wr[ivy]; wri[IVZ] = wr[ivz];
   wri[IPR] = wr[IPR]; wri[IBY] = wr[IBY]; wri[IBZ] = wr[IBZ];
   T bxi = T(bn[(int64_t)p.b * g.S + id]);
-  if (RS == AGX_RSOLVER_HLLE) hlle(gamma, wli, wri, bxi, flxi);
+  if (RS == AGX_RSOLVER_HLLE) hlle<ISO>(gamma, cs, wli, wri, bxi, flxi);
+  else if (ISO) hlld_iso(cs, dfl, wli, wri, bxi, flxi);
   else if (RS == AGX_RSOLVER_ROE) roe(gamma, wli, wri, bxi, flxi);
   else hlld(gamma, wli, wri, bxi, flxi);
   T *fb = flx + (int64_t)p.b * NHYDRO * g.S + id;
@@ -780,7 +944,7 @@ k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
   fb[ivx * g.S] = flxi[IVX];
   fb[ivy * g.S] = flxi[IVY];
   fb[ivz * g.S] = flxi[IVZ];
-  fb[IEN * g.S] = flxi[IEN];
+  if (!ISO) fb[IEN * g.S] = flxi[IEN];
   int64_t fo = (int64_t)p.b * g.S + id;
   ey[fo] = -flxi[IBY];
   ez[fo] = flxi[IBZ];
@@ -873,7 +1037,7 @@ __global__ void __launch_bounds__(256)
 k_integrate_u(Geo g, const double *src, double *dst, const double *__restrict__ f1,
               const double *__restrict__ f2, const double *__restrict__ f3,
               const double *__restrict__ w, const double *__restrict__ phi,
-              const BlockCoef *__restrict__ coef, double wght, int grav) {
+              const BlockCoef *__restrict__ coef, double wght, int grav, int iso) {
   PlaneIdx p;
   if (!plane_index(g.is, g.ie, g.js, g.je, g.ks, p)) return;
   const int64_t S = g.S;
@@ -901,21 +1065,21 @@ k_integrate_u(Geo g, const double *src, double *dst, const double *__restrict__ 
       double phil = 0.5 * (phi[pb - 1] + phic);
       double phir = 0.5 * (phic + phi[pb + 1]);
       un[IVX] -= dtodx1 * rho * (phir - phil);
-      un[IEN] -= dtodx1 * (f1[cb] * (phic - phil) + f1[cb + 1] * (phir - phic));
+      if (!iso) un[IEN] -= dtodx1 * (f1[cb] * (phic - phil) + f1[cb + 1] * (phir - phic));
     }
     {
       double dtodx2 = wght / dx2;
       double phil = 0.5 * (phi[pb - sj] + phic);
       double phir = 0.5 * (phic + phi[pb + sj]);
       un[IVY] -= dtodx2 * rho * (phir - phil);
-      un[IEN] -= dtodx2 * (f2[cb] * (phic - phil) + f2[cb + sj] * (phir - phic));
+      if (!iso) un[IEN] -= dtodx2 * (f2[cb] * (phic - phil) + f2[cb + sj] * (phir - phic));
     }
     {
       double dtodx3 = wght / dx3;
       double phil = 0.5 * (phi[pb - sk] + phic);
       double phir = 0.5 * (phic + phi[pb + sk]);
       un[IVZ] -= dtodx3 * rho * (phir - phil);
-      un[IEN] -= dtodx3 * (f3[cb] * (phic - phil) + f3[cb + sk] * (phir - phic));
+      if (!iso) un[IEN] -= dtodx3 * (f3[cb] * (phic - phil) + f3[cb + sk] * (phir - phic));
     }
   }
 #pragma unroll
@@ -1016,11 +1180,32 @@ __device__ __forceinline__ bool c2p_cell(double &u_d, double u_m1, double u_m2, 
   return fl;
 }
 
+// Isothermal C2P of one cell (eos/isothermal_mhd.cpp:40-73): density floor, velocities.
+__device__ __forceinline__ void c2p_cell_iso(double &u_d, double u_m1, double u_m2, double u_m3,
+                                             double dfl, double *wv) {
+  u_d = (u_d > dfl) ? u_d : dfl;
+  double di = 1.0 / u_d;
+  wv[0] = u_d;
+  wv[1] = u_m1 * di;
+  wv[2] = u_m2 * di;
+  wv[3] = u_m3 * di;
+  wv[4] = 0.0;
+}
+
+// C2P of one cell for either EOS; u_e is untouched in the isothermal case.
+template <bool ISO>
+__device__ __forceinline__ void c2p_eos(double &u_d, double u_m1, double u_m2, double u_m3,
+                                        double &u_e, double bcc1, double bcc2, double bcc3,
+                                        double gm1, double dfl, double pfl, double *wv) {
+  if (ISO) c2p_cell_iso(u_d, u_m1, u_m2, u_m3, dfl, wv);
+  else c2p_cell(u_d, u_m1, u_m2, u_m3, u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
+}
+
 // Field::CalculateCellCenteredField (uniform branch, field/field.cpp:112-180) and
-// EquationOfState::ConservedToPrimitive (eos/adiabatic_mhd.cpp:41-90) over interior +
-// ghosts; the density floor and the energy fix are written back to u.  TW is the
-// storage type of w and bcc (double, or float in the mixed-precision form).
-template <typename TW>
+// EquationOfState::ConservedToPrimitive (eos/adiabatic_mhd.cpp:41-90, isothermal_mhd.cpp:
+// 40-73) over interior + ghosts; the floors are written back to u.  TW is the storage
+// type of w and bcc (double, or float in the mixed-precision form).
+template <typename TW, bool ISO>
 __global__ void __launch_bounds__(256)
 k_cons2prim(Geo g, double *__restrict__ u, TW *__restrict__ w, TW *__restrict__ bcc,
             const double *__restrict__ b1, const double *__restrict__ b2,
@@ -1038,32 +1223,35 @@ k_cons2prim(Geo g, double *__restrict__ u, TW *__restrict__ w, TW *__restrict__ 
   bcc[bo + S] = TW(bcc2);
   bcc[bo + 2 * S] = TW(bcc3);
   const int64_t c = (int64_t)p.b * NHYDRO * S + id;
-  double u_d = u[c], u_e = u[c + 4 * S], wv[5];
-  c2p_cell(u_d, u[c + S], u[c + 2 * S], u[c + 3 * S], u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
+  double u_d = u[c], u_e = ISO ? 0.0 : u[c + 4 * S], wv[5];
+  c2p_eos<ISO>(u_d, u[c + S], u[c + 2 * S], u[c + 3 * S], u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
   u[c] = u_d;
-  u[c + 4 * S] = u_e;
+  if (!ISO) u[c + 4 * S] = u_e;
+  constexpr int NH = ISO ? 4 : NHYDRO;
 #pragma unroll
-  for (int n = 0; n < NHYDRO; ++n) w[c + n * S] = TW(wv[n]);
+  for (int n = 0; n < NH; ++n) w[c + n * S] = TW(wv[n]);
 }
 
 // Hydro::NewBlockTimeStep per cell (hydro/new_blockdt.cpp:78-97): min over the three
 // directions of dx/(|v|+c_f) with bx = bcc + |b_face - bcc| (lower face).
-__device__ __forceinline__ double cell_dt(double gamma, const double *wv, double c1, double c2,
-                                          double c3, double f1, double f2, double f3,
+template <bool ISO>
+__device__ __forceinline__ double cell_dt(double gamma, double cs, const double *wv, double c1,
+                                          double c2, double c3, double f1, double f2, double f3,
                                           const double *dx) {
   double wi[NWAVE];
-  wi[IDN] = wv[0]; wi[IVX] = wv[1]; wi[IVY] = wv[2]; wi[IVZ] = wv[3]; wi[IPR] = wv[4];
+  wi[IDN] = wv[0]; wi[IVX] = wv[1]; wi[IVY] = wv[2]; wi[IVZ] = wv[3];
+  wi[IPR] = ISO ? 0.0 : wv[4];
   double bx = c1 + fabs(f1 - c1);
   wi[IBY] = c2; wi[IBZ] = c3;
-  double cf = fast_speed(gamma, wi, bx);
+  double cf = fast_speed_eos<ISO>(gamma, cs, wi, bx);
   double dt1 = dx[0] / (fabs(wi[IVX]) + cf);
   wi[IBY] = c3; wi[IBZ] = c1;
   bx = c2 + fabs(f2 - c2);
-  cf = fast_speed(gamma, wi, bx);
+  cf = fast_speed_eos<ISO>(gamma, cs, wi, bx);
   double dt2 = dx[1] / (fabs(wi[IVY]) + cf);
   wi[IBY] = c1; wi[IBZ] = c2;
   bx = c3 + fabs(f3 - c3);
-  cf = fast_speed(gamma, wi, bx);
+  cf = fast_speed_eos<ISO>(gamma, cs, wi, bx);
   double dt3 = dx[2] / (fabs(wi[IVZ]) + cf);
   double m = DBL_MAX;
   m = dmin(m, dt1);
@@ -1139,15 +1327,16 @@ struct StageArgs {
   const double *s1, *s2, *s3;        // CT source faces
   double *d1, *d2, *d3;              // CT destination faces (interior)
   const BlockCoef *coef;
-  double wght, gm1, dfl, pfl, gamma;
+  double wght, gm1, dfl, pfl, gamma, cs;
   int grav;
   double *dtpart;                    // per-workgroup min of the dt bound, or null
 };
 
-template <int TI, int TJ, typename TW>
+template <int TI, int TJ, typename TW, bool ISO>
 __global__ void __launch_bounds__(TI * TJ, 4)   // 4 workgroups per CU: <= 128 VGPRs, 4 x 37 KB LDS
 k_stage(Geo g, StageArgs<TW> a) {
   static_assert(TI * TJ == 256, "k_stage runs 256-thread workgroups");
+  constexpr int NH = ISO ? 4 : NHYDRO;   // conserved variables (no energy if isothermal)
   constexpr int NT = TI * TJ, CI = TI + 2, CJ = TJ + 2;
   __shared__ double s_cce[3][3][CJ][CI];      // planes c-1, c, c+1 (slot = plane mod 3)
   __shared__ double s_e1[2][TJ + 1][TI];      // edge planes c-1, c (slot = plane & 1)
@@ -1295,8 +1484,9 @@ k_stage(Geo g, StageArgs<TW> a) {
         const double vol = dx1 * dx2 * dx3;
         const int64_t cb0 = cbase + id;
         double un[NHYDRO];
+        un[IEN] = 0.0;
 #pragma unroll
-        for (int n = 0; n < NHYDRO; ++n) {
+        for (int n = 0; n < NH; ++n) {
           const int64_t o = cb0 + n * S;
           double dflx = (a1 * (double)a.f1[o + 1] - a1 * (double)a.f1[o]);
           dflx += (a2 * (double)a.f2[o + sj] - a2 * (double)a.f2[o]);
@@ -1311,21 +1501,24 @@ k_stage(Geo g, StageArgs<TW> a) {
             double phil = 0.5 * (a.phi[fo - 1] + phic);
             double phir = 0.5 * (phic + a.phi[fo + 1]);
             un[IVX] -= dtodx1 * rho * (phir - phil);
-            un[IEN] -= dtodx1 * ((double)a.f1[cb0] * (phic - phil) + (double)a.f1[cb0 + 1] * (phir - phic));
+            if (!ISO)
+              un[IEN] -= dtodx1 * ((double)a.f1[cb0] * (phic - phil) + (double)a.f1[cb0 + 1] * (phir - phic));
           }
           {
             double dtodx2 = wght / dx2;
             double phil = 0.5 * (a.phi[fo - sj] + phic);
             double phir = 0.5 * (phic + a.phi[fo + sj]);
             un[IVY] -= dtodx2 * rho * (phir - phil);
-            un[IEN] -= dtodx2 * ((double)a.f2[cb0] * (phic - phil) + (double)a.f2[cb0 + sj] * (phir - phic));
+            if (!ISO)
+              un[IEN] -= dtodx2 * ((double)a.f2[cb0] * (phic - phil) + (double)a.f2[cb0 + sj] * (phir - phic));
           }
           {
             double dtodx3 = wght / dx3;
             double phil = 0.5 * (a.phi[fo - sk] + phic);
             double phir = 0.5 * (phic + a.phi[fo + sk]);
             un[IVZ] -= dtodx3 * rho * (phir - phil);
-            un[IEN] -= dtodx3 * ((double)a.f3[cb0] * (phic - phil) + (double)a.f3[cb0 + sk] * (phir - phic));
+            if (!ISO)
+              un[IEN] -= dtodx3 * ((double)a.f3[cb0] * (phic - phil) + (double)a.f3[cb0 + sk] * (phir - phic));
           }
         }
         const double bcc1 = 0.5 * b1l + 0.5 * b1h;
@@ -1337,11 +1530,12 @@ k_stage(Geo g, StageArgs<TW> a) {
         a.bccn[bo + 2 * S] = TW(bcc3);
         // C2P writes its floors back into u (adiabatic_mhd.cpp:66-84)
         double wv[5];
-        c2p_cell(un[IDN], un[IVX], un[IVY], un[IVZ], un[IEN], bcc1, bcc2, bcc3, a.gm1, a.dfl, a.pfl, wv);
+        c2p_eos<ISO>(un[IDN], un[IVX], un[IVY], un[IVZ], un[IEN], bcc1, bcc2, bcc3, a.gm1, a.dfl, a.pfl,
+                     wv);
 #pragma unroll
-        for (int n = 0; n < NHYDRO; ++n) a.dst[cb0 + n * S] = un[n];
+        for (int n = 0; n < NH; ++n) a.dst[cb0 + n * S] = un[n];
 #pragma unroll
-        for (int n = 0; n < NHYDRO; ++n) a.wn[cb0 + n * S] = TW(wv[n]);
+        for (int n = 0; n < NH; ++n) a.wn[cb0 + n * S] = TW(wv[n]);
         // ghost copies: a ghost cell's u, w and bcc equal those of its source cell (same
         // conserved state, and bcc from the same face values), so they are copied
         {
@@ -1352,15 +1546,16 @@ k_stage(Geo g, StageArgs<TW> a) {
           for_each_ghost(g, b, c - 1, j, i, rc, [&](int nb, int64_t id2) {
             const int64_t c2 = (int64_t)nb * NHYDRO * S + id2, b2 = (int64_t)nb * 3 * S + id2;
 #pragma unroll
-            for (int n = 0; n < NHYDRO; ++n) a.dst[c2 + n * S] = un[n];
+            for (int n = 0; n < NH; ++n) a.dst[c2 + n * S] = un[n];
 #pragma unroll
-            for (int n = 0; n < NHYDRO; ++n) a.wn[c2 + n * S] = TW(wv[n]);
+            for (int n = 0; n < NH; ++n) a.wn[c2 + n * S] = TW(wv[n]);
             a.bccn[b2] = TW(bcc1);
             a.bccn[b2 + S] = TW(bcc2);
             a.bccn[b2 + 2 * S] = TW(bcc3);
           });
         }
-        if (a.dtpart) dtmin = dmin(dtmin, cell_dt(a.gamma, wv, bcc1, bcc2, bcc3, b1l, b2l, b3_lo, dx));
+        if (a.dtpart)
+          dtmin = dmin(dtmin, cell_dt<ISO>(a.gamma, a.cs, wv, bcc1, bcc2, bcc3, b1l, b2l, b3_lo, dx));
       }
     }
     b3_lo = b3_hi;
@@ -1375,12 +1570,12 @@ k_stage(Geo g, StageArgs<TW> a) {
 // Hydro::NewBlockTimeStep (hydro/new_blockdt.cpp:57-160): per-cell bound, min over all
 // interior cells of all blocks, one partial per workgroup; k_min_final reduces the
 // partials (min is exact, so the reduction order does not matter).
-template <typename TW>
+template <typename TW, bool ISO>
 __global__ void __launch_bounds__(256)
 k_newdt(Geo g, const TW *__restrict__ w, const TW *__restrict__ bcc,
         const double *__restrict__ b1, const double *__restrict__ b2,
         const double *__restrict__ b3, const BlockCoef *__restrict__ coef, double gamma,
-        double *__restrict__ part) {
+        double cs, double *__restrict__ part) {
   __shared__ double sh[4];
   PlaneIdx p;
   double m = DBL_MAX;
@@ -1390,11 +1585,12 @@ k_newdt(Geo g, const TW *__restrict__ w, const TW *__restrict__ bcc,
     const int64_t c = (int64_t)p.b * NHYDRO * S + id;
     const int64_t bo = (int64_t)p.b * 3 * S + id;
     const int64_t fo = (int64_t)p.b * S + id;
-    double wv[5];
+    double wv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    constexpr int NH = ISO ? 4 : NHYDRO;
 #pragma unroll
-    for (int n = 0; n < NHYDRO; ++n) wv[n] = w[c + n * S];
-    m = cell_dt(gamma, wv, bcc[bo], bcc[bo + S], bcc[bo + 2 * S], b1[fo], b2[fo], b3[fo],
-                coef[p.b].dx);
+    for (int n = 0; n < NH; ++n) wv[n] = w[c + n * S];
+    m = cell_dt<ISO>(gamma, cs, wv, bcc[bo], bcc[bo + S], bcc[bo + 2 * S], b1[fo], b2[fo], b3[fo],
+                     coef[p.b].dx);
   }
   m = wg_min256(m, sh);
   if (threadIdx.x == 0) part[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
@@ -1471,7 +1667,11 @@ int check_desc(const agx_mesh_desc *d) {
       d->rsolver != AGX_RSOLVER_ROE) {
     set_error("unknown Riemann solver %d", d->rsolver); return -8;
   }
-  if (d->eos != 0) { set_error("only the adiabatic EOS is supported"); return -9; }
+  if (d->eos != 0 && d->eos != 1) { set_error("eos must be 0 (adiabatic) or 1 (isothermal)"); return -9; }
+  if (d->eos == 1 && d->rsolver == AGX_RSOLVER_ROE) {
+    set_error("the isothermal EOS supports HLLD (hlld_iso.cpp) and HLLE only"); return -9;
+  }
+  if (d->eos == 1 && !(d->iso_cs > 0.0)) { set_error("iso_cs must be > 0"); return -9; }
   if (d->precision != AGX_PREC_FP64 && d->precision != AGX_PREC_MIXED) {
     set_error("precision %d not supported (0: FP64, 1: mixed)", d->precision); return -10;
   }
@@ -1516,6 +1716,7 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
                                     1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
                                     5, 3, 1, 1, 1};
   m->mixed = (desc->precision == AGX_PREC_MIXED);
+  m->iso = (desc->eos == 1);
   int64_t total = 0;
   for (int a = 0; a < ARR_TOTAL; ++a) {
     bool f32 = m->mixed && (a == AGX_ARR_W || a == AGX_ARR_BCC || a == ARR_W2 || a == ARR_BCC2 ||
@@ -1627,6 +1828,10 @@ static void host_dims(const agx_mesh *m, int arr, int64_t dims[4]) {
     default: break;
   }
   dims[0] = m->arr.nvar[arr];
+  // isothermal: NHYDRO = 4 on the host side (the device keeps a fifth, unused slot)
+  if (m->iso && (arr == AGX_ARR_U || arr == AGX_ARR_U1 || arr == AGX_ARR_W || arr == AGX_ARR_FX1 ||
+                 arr == AGX_ARR_FX2 || arr == AGX_ARR_FX3))
+    dims[0] = 4;
   dims[1] = d3;
   dims[2] = d2;
   dims[3] = d1;
@@ -1661,9 +1866,11 @@ static int copy_array(agx_mesh *m, int arr, double *host, bool to_device) {
   }
   char *dbase = reinterpret_cast<char *>(m->arr.a[arr]);
   hipMemcpy3DParms p = {};
+  const int nv_dev = m->arr.nvar[arr];   // device slabs per block (may exceed the host's)
   for (int64_t s = 0; s < nslab; ++s) {
+    const int64_t sd = (s / dims[0]) * nv_dev + (s % dims[0]);
     hipPitchedPtr hs = make_hipPitchedPtr(hbase + s * nslab_el * esz, dims[3] * esz, dims[3], dims[2]);
-    hipPitchedPtr ds = make_hipPitchedPtr(dbase + s * g.S * esz, g.P * esz, g.P, g.J);
+    hipPitchedPtr ds = make_hipPitchedPtr(dbase + sd * g.S * esz, g.P * esz, g.P, g.J);
     p.extent = make_hipExtent(dims[3] * esz, dims[2], dims[1]);
     if (to_device) { p.srcPtr = hs; p.dstPtr = ds; p.kind = hipMemcpyHostToDevice; }
     else { p.srcPtr = ds; p.dstPtr = hs; p.kind = hipMemcpyDeviceToHost; }
@@ -1714,20 +1921,24 @@ int agx_calculate_fluxes(agx_mesh *m, int order) {
   double **A = m->arr.a;
   const double gamma = m->d.gamma, dt = m->dt;
   const int rs = m->d.rsolver;
-#define FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, TW)                                      \
-  hipLaunchKernelGGL((k_flux<DIR, ORD, RS, TW>),                                            \
+#define FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, TW, ISO)                                 \
+  hipLaunchKernelGGL((k_flux<DIR, ORD, RS, TW, ISO>),                                       \
                      plane_grid(g.is - (DIR != 0), g.ie + 1, g.js - (DIR != 1), g.je + 1,    \
                                 g.ks - (DIR != 2), g.ke + 1, g.nb),                          \
                      dim3(256), 0, m->stream, g, as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]), \
                      A[B], as<TW>(A[F]), as<TW>(A[EY]), as<TW>(A[EZ]), as<TW>(A[WX]), m->coef, \
-                     m->plm, gamma, dt, m->d.dfloor, m->d.pfloor)
+                     m->plm, gamma, m->d.iso_cs, dt, m->d.dfloor, m->d.pfloor)
+#define FLUX_LAUNCH_E(DIR, ORD, RS, B, F, EY, EZ, WX, ISO)                                     \
+  if (m->mixed) FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, float, ISO);                    \
+  else FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, double, ISO)
 #define FLUX_LAUNCH(DIR, ORD, RS, B, F, EY, EZ, WX)                                            \
-  if (m->mixed) FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, float);                         \
-  else FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, double)
+  if (m->iso) { FLUX_LAUNCH_E(DIR, ORD, (RS == AGX_RSOLVER_HLLE ? AGX_RSOLVER_HLLE : AGX_RSOLVER_HLLD), \
+                              B, F, EY, EZ, WX, true); }                                       \
+  else { FLUX_LAUNCH_E(DIR, ORD, RS, B, F, EY, EZ, WX, false); }
 #define FLUX_ORD(DIR, ORD, B, F, EY, EZ, WX)                                                    \
-  if (rs == AGX_RSOLVER_HLLE) FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_HLLE, B, F, EY, EZ, WX);        \
-  else if (rs == AGX_RSOLVER_ROE) FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_ROE, B, F, EY, EZ, WX);     \
-  else FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_HLLD, B, F, EY, EZ, WX);
+  if (rs == AGX_RSOLVER_HLLE) { FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_HLLE, B, F, EY, EZ, WX); }    \
+  else if (rs == AGX_RSOLVER_ROE) { FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_ROE, B, F, EY, EZ, WX); } \
+  else { FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_HLLD, B, F, EY, EZ, WX); }
 #define FLUX_DIR(DIR, B, F, EY, EZ, WX)                                                         \
   timer_begin(m, AGX_K_FLUX_X1 + DIR);                                                        \
   if (order == 1) {                                                                           \
@@ -1747,6 +1958,7 @@ int agx_calculate_fluxes(agx_mesh *m, int order) {
 #undef FLUX_DIR
 #undef FLUX_ORD
 #undef FLUX_LAUNCH
+#undef FLUX_LAUNCH_E
 #undef FLUX_LAUNCH_T
   return 0;
 }
@@ -1779,7 +1991,7 @@ int agx_integrate(agx_mesh *m, int stage) {
   timer_begin(m, AGX_K_INTEGRATE);
   hipLaunchKernelGGL(k_integrate_u, plane_grid(g.is, g.ie, g.js, g.je, g.ks, g.ke, g.nb), dim3(256), 0,
                      m->stream, g, A[uin], A[AGX_ARR_U1], A[AGX_ARR_FX1], A[AGX_ARR_FX2], A[AGX_ARR_FX3],
-                     A[AGX_ARR_W], A[AGX_ARR_PHI], m->coef, wght, m->d.self_gravity ? 1 : 0);
+                     A[AGX_ARR_W], A[AGX_ARR_PHI], m->coef, wght, m->d.self_gravity ? 1 : 0, m->iso ? 1 : 0);
   timer_end(m);
   LAUNCH_CHECK("k_integrate_u");
   const int b1 = (stage == 1) ? AGX_ARR_B1F : AGX_ARR_R1F;
@@ -1834,10 +2046,15 @@ template <typename TW>
 static void launch_cons2prim(agx_mesh *m) {
   const Geo &g = m->g;
   double **A = m->arr.a;
-  hipLaunchKernelGGL(k_cons2prim<TW>, plane_grid(0, g.n1 - 1, 0, g.n2 - 1, 0, g.n3 - 1, g.nb), dim3(256),
-                     0, m->stream, g, A[AGX_ARR_U], as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]),
-                     A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F], m->d.gamma - 1.0, m->d.dfloor,
-                     m->d.pfloor);
+  const dim3 grid = plane_grid(0, g.n1 - 1, 0, g.n2 - 1, 0, g.n3 - 1, g.nb);
+  if (m->iso)
+    hipLaunchKernelGGL((k_cons2prim<TW, true>), grid, dim3(256), 0, m->stream, g, A[AGX_ARR_U],
+                       as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]), A[AGX_ARR_B1F], A[AGX_ARR_B2F],
+                       A[AGX_ARR_B3F], m->d.gamma - 1.0, m->d.dfloor, m->d.pfloor);
+  else
+    hipLaunchKernelGGL((k_cons2prim<TW, false>), grid, dim3(256), 0, m->stream, g, A[AGX_ARR_U],
+                       as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]), A[AGX_ARR_B1F], A[AGX_ARR_B2F],
+                       A[AGX_ARR_B3F], m->d.gamma - 1.0, m->d.dfloor, m->d.pfloor);
 }
 }  // extern "C++"
 
@@ -1868,9 +2085,14 @@ template <typename TW>
 static void launch_newdt(agx_mesh *m, dim3 grid) {
   const Geo &g = m->g;
   double **A = m->arr.a;
-  hipLaunchKernelGGL(k_newdt<TW>, grid, dim3(256), 0, m->stream, g, as<TW>(A[AGX_ARR_W]),
-                     as<TW>(A[AGX_ARR_BCC]), A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F], m->coef,
-                     m->d.gamma, m->red);
+  if (m->iso)
+    hipLaunchKernelGGL((k_newdt<TW, true>), grid, dim3(256), 0, m->stream, g, as<TW>(A[AGX_ARR_W]),
+                       as<TW>(A[AGX_ARR_BCC]), A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F], m->coef,
+                       m->d.gamma, m->d.iso_cs, m->red);
+  else
+    hipLaunchKernelGGL((k_newdt<TW, false>), grid, dim3(256), 0, m->stream, g, as<TW>(A[AGX_ARR_W]),
+                       as<TW>(A[AGX_ARR_BCC]), A[AGX_ARR_B1F], A[AGX_ARR_B2F], A[AGX_ARR_B3F], m->coef,
+                       m->d.gamma, m->d.iso_cs, m->red);
 }
 }  // extern "C++"
 
@@ -1952,12 +2174,19 @@ static void launch_stage(agx_mesh *m, int stage, double wght, double *dtpart) {
   a.dfl = m->d.dfloor;
   a.pfl = m->d.pfloor;
   a.gamma = m->d.gamma;
+  a.cs = m->d.iso_cs;
   a.grav = m->d.self_gravity ? 1 : 0;
   a.dtpart = dtpart;
   const dim3 grid = stage_grid(g, m->tile);
-  if (m->tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW>), grid, dim3(256), 0, m->stream, g, a);
-  else if (m->tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW>), grid, dim3(256), 0, m->stream, g, a);
-  else hipLaunchKernelGGL((k_stage<16, 16, TW>), grid, dim3(256), 0, m->stream, g, a);
+  if (m->iso) {
+    if (m->tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, true>), grid, dim3(256), 0, m->stream, g, a);
+    else if (m->tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, true>), grid, dim3(256), 0, m->stream, g, a);
+    else hipLaunchKernelGGL((k_stage<16, 16, TW, true>), grid, dim3(256), 0, m->stream, g, a);
+  } else {
+    if (m->tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, false>), grid, dim3(256), 0, m->stream, g, a);
+    else if (m->tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, false>), grid, dim3(256), 0, m->stream, g, a);
+    else hipLaunchKernelGGL((k_stage<16, 16, TW, false>), grid, dim3(256), 0, m->stream, g, a);
+  }
 }
 }  // extern "C++"
 

@@ -44,7 +44,9 @@ typedef struct agx_mesh_desc {
   int32_t nghost;               /* NGHOST (configure --nghost): >= 2, >= 3 for PPM */
   int32_t xorder;               /* <time>/xorder: 1 donor cell, 2 PLM, 3 PPM */
   int32_t rsolver;              /* configure --flux: AGX_RSOLVER_* */
-  int32_t eos;                  /* configure --eos: 0 adiabatic */
+  int32_t eos;                  /* configure --eos: 0 adiabatic, 1 isothermal (NHYDRO = 4:
+                                   u, u1, w and the fluxes carry 4 variables on the host
+                                   side; HLLD means hlld_iso.cpp; Roe is rejected) */
   int32_t self_gravity;         /* configure --grav: 0 none, 1 multigrid */
   int32_t mg_mode;              /* <gravity>/mgmode: 0 FMG, 1 MGI */
   int32_t mg_niter;             /* <gravity>/niteration (used when threshold < 0) */
@@ -116,7 +118,8 @@ int agx_synchronize(agx_mesh *m);
 /* Hydro::CalculateFluxes(w, b, bcc, order), hydro/calculate_fluxes.cpp:36-378, with
  * Reconstruction::{DonorCell,PiecewiseLinear,PiecewiseParabolic}X1..3
  * (reconstruct/dc.cpp:24-104, plm.cpp:27-351, ppm.cpp:44-923), Hydro::RiemannSolver
- * (hydro/rsolvers/mhd/hlld.cpp:38-382, hlle_mhd.cpp:25-182, roe_mhd.cpp:43-498) and
+ * (hydro/rsolvers/mhd/hlld.cpp:38-382, hlld_iso.cpp:37-254, hlle_mhd.cpp:25-182,
+ * roe_mhd.cpp:43-498) and
  * Hydro::GetWeightForCT
  * (hydro/hydro.cpp:154-158). */
 int agx_calculate_fluxes(agx_mesh *m, int order);
@@ -132,7 +135,8 @@ int agx_integrate(agx_mesh *m, int stage);
  * neighbours (bvals/bvals_var.cpp:212-296, bvals/cc/bvals_cc.cpp:201-336,
  * bvals/fc/bvals_fc.cpp:344-402,583-700, utils/buffer_utils.cpp:24-89). */
 int agx_bvals_exchange(agx_mesh *m, int which);   /* bit 1: u, bit 2: b, bit 4: phi */
-/* EquationOfState::ConservedToPrimitive (eos/adiabatic_mhd.cpp:41-90) with
+/* EquationOfState::ConservedToPrimitive (eos/adiabatic_mhd.cpp:41-90,
+ * eos/isothermal_mhd.cpp:40-73) with
  * Field::CalculateCellCenteredField (field/field.cpp:112-180), interior + ghosts. */
 int agx_cons2prim(agx_mesh *m);
 /* Hydro::NewBlockTimeStep (hydro/new_blockdt.cpp:42-190), min over MeshBlocks. */

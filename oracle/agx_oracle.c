@@ -86,6 +86,8 @@ struct orc_ctx {
   int last_iters;
   int mg_mode, mg_niter;
   double mg_eps;
+  int iso;           /* isothermal EOS (eos = 1): NHYDRO = 4, the energy slot stays 0 */
+  int nhydro;
 };
 
 /* ---------------------------------------------------------------- helpers */
@@ -204,6 +206,8 @@ orc_ctx *orc_create(const orc_desc *d) {
                  c->x3f + g * (c->n3 + 1), c->x3v + g * (c->n3 + 1), c->dx3f + g * (c->n3 + 1));
   }
   c->dt = DBL_MAX;
+  c->iso = (d->eos == 1);
+  c->nhydro = c->iso ? 4 : NHYDRO;
   /* multigrid hierarchy: multigrid.cpp:32-144, multigrid_driver.cpp:343-349 */
   if (d->self_gravity) {
     int nl = 0;
@@ -391,6 +395,20 @@ static inline double fast_speed(double gamma, const double *prim, double bx) {
   return sqrt(0.5 * (qsq + sqrt(tmp * tmp + 4.0 * asq * ct2)) / prim[IDN]);
 }
 
+/* EquationOfState::FastMagnetosonicSpeed, eos/isothermal_mhd.cpp:122-129 */
+static inline double fast_speed_iso(double cs, const double *prim, double bx) {
+  double asq = (cs * cs) * prim[IDN];
+  double vaxsq = bx * bx;
+  double ct2 = prim[IBY] * prim[IBY] + prim[IBZ] * prim[IBZ];
+  double qsq = vaxsq + ct2 + asq;
+  double tmp = vaxsq + ct2 - asq;
+  return sqrt(0.5 * (qsq + sqrt(tmp * tmp + 4.0 * asq * ct2)) / prim[IDN]);
+}
+
+static inline double fast_speed_eos(const orc_ctx *c, const double *prim, double bx) {
+  return c->iso ? fast_speed_iso(c->d.iso_cs, prim, bx) : fast_speed(c->d.gamma, prim, bx);
+}
+
 /* Field::CalculateCellCenteredField (uniform branch, field/field.cpp:112-180) then
  * EquationOfState::ConservedToPrimitive (eos/adiabatic_mhd.cpp:41-90), over the
  * interior plus all ghost zones (time_integrator.cpp:1971-1977: every face has a
@@ -429,6 +447,7 @@ void orc_cons2prim(orc_ctx *c) {
           w[IVX * nc + id] = u_m1 * di;
           w[IVY * nc + id] = u_m2 * di;
           w[IVZ * nc + id] = u_m3 * di;
+          if (c->iso) continue;   /* eos/isothermal_mhd.cpp:40-73: no energy */
           double bcc1 = bcc[IB1 * nc + id], bcc2 = bcc[IB2 * nc + id], bcc3 = bcc[IB3 * nc + id];
           double pb = 0.5 * (SQR(bcc1) + SQR(bcc2) + SQR(bcc3));
           double e_k = 0.5 * di * (SQR(u_m1) + SQR(u_m2) + SQR(u_m3));
@@ -658,8 +677,8 @@ static void hlld(double gamma, const double *wli, const double *wri, double bxi,
 }
 
 /* HLLE for adiabatic MHD, hydro/rsolvers/mhd/hlle_mhd.cpp:25-182 (one interface). */
-static void hlle(double gamma, const double *wli, const double *wri, double bxi,
-                 double *flxi) {
+static void hlle(double gamma, int iso, double iso_cs, const double *wli, const double *wri,
+                 double bxi, double *flxi) {
   double gm1 = gamma - 1.0;
   double wroe[NWAVE], fl[NWAVE], fr[NWAVE];
   double sqrtdl = sqrt(wli[IDN]);
@@ -675,17 +694,26 @@ static void hlle(double gamma, const double *wli, const double *wri, double bxi,
   double y = 0.5 * (wli[IDN] + wri[IDN]) / wroe[IDN];
   double pbl = 0.5 * (bxi * bxi + SQR(wli[IBY]) + SQR(wli[IBZ]));
   double pbr = 0.5 * (bxi * bxi + SQR(wri[IBY]) + SQR(wri[IBZ]));
-  double el = wli[IPR] / gm1 + 0.5 * wli[IDN] * (SQR(wli[IVX]) + SQR(wli[IVY]) + SQR(wli[IVZ])) + pbl;
-  double er = wri[IPR] / gm1 + 0.5 * wri[IDN] * (SQR(wri[IVX]) + SQR(wri[IVY]) + SQR(wri[IVZ])) + pbr;
-  double hroe = ((el + wli[IPR] + pbl) / sqrtdl + (er + wri[IPR] + pbr) / sqrtdr) * isdlpdr;
-  double cl = fast_speed(gamma, wli, bxi);
-  double cr = fast_speed(gamma, wri, bxi);
+  double el = 0.0, er = 0.0, hroe = 0.0;
+  if (!iso) {   /* hlle_mhd.cpp:83-87 */
+    el = wli[IPR] / gm1 + 0.5 * wli[IDN] * (SQR(wli[IVX]) + SQR(wli[IVY]) + SQR(wli[IVZ])) + pbl;
+    er = wri[IPR] / gm1 + 0.5 * wri[IDN] * (SQR(wri[IVX]) + SQR(wri[IVY]) + SQR(wri[IVZ])) + pbr;
+    hroe = ((el + wli[IPR] + pbl) / sqrtdl + (er + wri[IPR] + pbr) / sqrtdr) * isdlpdr;
+  }
+  double cl = iso ? fast_speed_iso(iso_cs, wli, bxi) : fast_speed(gamma, wli, bxi);
+  double cr = iso ? fast_speed_iso(iso_cs, wri, bxi) : fast_speed(gamma, wri, bxi);
   double btsq = SQR(wroe[IBY]) + SQR(wroe[IBZ]);
   double vaxsq = bxi * bxi / wroe[IDN];
-  double bt_starsq = (gm1 - (gm1 - 1.0) * y) * btsq;
-  double hp = hroe - (vaxsq + btsq / wroe[IDN]);
-  double vsq = SQR(wroe[IVX]) + SQR(wroe[IVY]) + SQR(wroe[IVZ]);
-  double twid_asq = dmax((gm1 * (hp - 0.5 * vsq) - (gm1 - 1.0) * x), 0.0);
+  double bt_starsq, twid_asq;
+  if (!iso) {   /* hlle_mhd.cpp:98-106: eq. B18 (adiabatic) or B39 (isothermal) */
+    bt_starsq = (gm1 - (gm1 - 1.0) * y) * btsq;
+    double hp = hroe - (vaxsq + btsq / wroe[IDN]);
+    double vsq = SQR(wroe[IVX]) + SQR(wroe[IVY]) + SQR(wroe[IVZ]);
+    twid_asq = dmax((gm1 * (hp - 0.5 * vsq) - (gm1 - 1.0) * x), 0.0);
+  } else {
+    bt_starsq = btsq * y;
+    twid_asq = iso_cs * iso_cs + x;
+  }
   double ct2 = bt_starsq / wroe[IDN];
   double tsum = vaxsq + ct2 + twid_asq;
   double tdif = vaxsq + ct2 - twid_asq;
@@ -706,12 +734,19 @@ static void hlle(double gamma, const double *wli, const double *wri, double bxi,
   fr[IVY] = wri[IDN] * wri[IVY] * vxr - bxi * wri[IBY];
   fl[IVZ] = wli[IDN] * wli[IVZ] * vxl - bxi * wli[IBZ];
   fr[IVZ] = wri[IDN] * wri[IVZ] * vxr - bxi * wri[IBZ];
-  fl[IVX] += wli[IPR];
-  fr[IVX] += wri[IPR];
-  fl[IEN] = el * vxl + wli[IVX] * (wli[IPR] + pbl - bxi * bxi);
-  fr[IEN] = er * vxr + wri[IVX] * (wri[IPR] + pbr - bxi * bxi);
-  fl[IEN] -= bxi * (wli[IBY] * wli[IVY] + wli[IBZ] * wli[IVZ]);
-  fr[IEN] -= bxi * (wri[IBY] * wri[IVY] + wri[IBZ] * wri[IVZ]);
+  if (!iso) {   /* hlle_mhd.cpp:140-150 */
+    fl[IVX] += wli[IPR];
+    fr[IVX] += wri[IPR];
+    fl[IEN] = el * vxl + wli[IVX] * (wli[IPR] + pbl - bxi * bxi);
+    fr[IEN] = er * vxr + wri[IVX] * (wri[IPR] + pbr - bxi * bxi);
+    fl[IEN] -= bxi * (wli[IBY] * wli[IVY] + wli[IBZ] * wli[IVZ]);
+    fr[IEN] -= bxi * (wri[IBY] * wri[IVY] + wri[IBZ] * wri[IVZ]);
+  } else {
+    fl[IVX] += (iso_cs * iso_cs) * wli[IDN];
+    fr[IVX] += (iso_cs * iso_cs) * wri[IDN];
+    fl[IEN] = 0.0;
+    fr[IEN] = 0.0;
+  }
   fl[IBY] = wli[IBY] * vxl - bxi * wli[IVY];
   fr[IBY] = wri[IBY] * vxr - bxi * wri[IVY];
   fl[IBZ] = wli[IBZ] * vxl - bxi * wli[IVZ];
@@ -720,6 +755,128 @@ static void hlle(double gamma, const double *wli, const double *wri, double bxi,
   if (bp != bm) tmp = 0.5 * (bp + bm) / (bp - bm);
   for (int n = 0; n < NWAVE; ++n)
     flxi[n] = 0.5 * (fl[n] + fr[n]) + (fl[n] - fr[n]) * tmp;
+}
+
+/* HLLD for isothermal MHD (Mignone 2007) at one interface,
+ * hydro/rsolvers/mhd/hlld_iso.cpp:37-254; SMALL_NUMBER = 1e-8 (hlld_iso.cpp:31).  The
+ * energy slot of flxi is set to 0 (NHYDRO = 4 in the reference). */
+typedef struct { double d, mx, my, mz, by, bz; } cons_iso;
+
+static void hlld_iso(double cs, double dfloor, const double *wli, const double *wri,
+                     double bxi, double *flxi) {
+  const double SMALL = 1.0e-8;
+  cons_iso ul, ur, ulst, urst, ucst, fl, fr;
+  double spd[5];
+  ul.d = wli[IDN];
+  ul.mx = wli[IVX] * ul.d;
+  ul.my = wli[IVY] * ul.d;
+  ul.mz = wli[IVZ] * ul.d;
+  ul.by = wli[IBY];
+  ul.bz = wli[IBZ];
+  ur.d = wri[IDN];
+  ur.mx = wri[IVX] * ur.d;
+  ur.my = wri[IVY] * ur.d;
+  ur.mz = wri[IVZ] * ur.d;
+  ur.by = wri[IBY];
+  ur.bz = wri[IBZ];
+  /* speeds, Miyoshi & Kusano eqn. (67) (hlld_iso.cpp:97-101) */
+  double cfl = fast_speed_iso(cs, wli, bxi);
+  double cfr = fast_speed_iso(cs, wri, bxi);
+  spd[0] = dmin(wli[IVX] - cfl, wri[IVX] - cfr);
+  spd[4] = dmax(wli[IVX] + cfl, wri[IVX] + cfr);
+  /* L/R fluxes (hlld_iso.cpp:103-122) */
+  double bxsq = bxi * bxi;
+  double ptl = SQR(cs) * wli[IDN] + 0.5 * (bxsq + SQR(wli[IBY]) + SQR(wli[IBZ]));
+  double ptr = SQR(cs) * wri[IDN] + 0.5 * (bxsq + SQR(wri[IBY]) + SQR(wri[IBZ]));
+  fl.d = ul.mx;
+  fl.mx = ul.mx * wli[IVX] + ptl - bxsq;
+  fl.my = ul.my * wli[IVX] - bxi * ul.by;
+  fl.mz = ul.mz * wli[IVX] - bxi * ul.bz;
+  fl.by = ul.by * wli[IVX] - bxi * wli[IVY];
+  fl.bz = ul.bz * wli[IVX] - bxi * wli[IVZ];
+  fr.d = ur.mx;
+  fr.mx = ur.mx * wri[IVX] + ptr - bxsq;
+  fr.my = ur.my * wri[IVX] - bxi * ur.by;
+  fr.mz = ur.mz * wri[IVX] - bxi * ur.bz;
+  fr.by = ur.by * wri[IVX] - bxi * wri[IVY];
+  fr.bz = ur.bz * wri[IVX] - bxi * wri[IVZ];
+  /* HLL averages and Alfven speeds, Mignone eqns. (15), (17), (29) (hlld_iso.cpp:124-150) */
+  double idspd = 1.0 / (spd[4] - spd[0]);
+  double dhll = (spd[4] * ur.d - spd[0] * ul.d - fr.d + fl.d) * idspd;
+  dhll = dmax(dhll, dfloor);
+  double sqrtdhll = sqrt(dhll);
+  double fdhll = (spd[4] * fl.d - spd[0] * fr.d + spd[4] * spd[0] * (ur.d - ul.d)) * idspd;
+  double fmxhll = (spd[4] * fl.mx - spd[0] * fr.mx + spd[4] * spd[0] * (ur.mx - ul.mx)) * idspd;
+  double ustar = fdhll / dhll;
+  double mxhll = (spd[4] * ur.mx - spd[0] * ul.mx - fr.mx + fl.mx) * idspd;
+  spd[1] = ustar - fabs(bxi) / sqrtdhll;
+  spd[3] = ustar + fabs(bxi) / sqrtdhll;
+  /* Ul*, eqns. (20), (24), (30)-(33) (hlld_iso.cpp:152-173) */
+  ulst.d = dhll;
+  ulst.mx = mxhll;
+  double tmp = (spd[0] - spd[1]) * (spd[0] - spd[3]);
+  if (fabs(spd[0] - spd[1]) < (SMALL) * cs) {
+    ulst.my = ul.my; ulst.mz = ul.mz; ulst.by = ul.by; ulst.bz = ul.bz;
+  } else {
+    double mfact = bxi * (ustar - wli[IVX]) / tmp;
+    double bfact = (ul.d * SQR(spd[0] - wli[IVX]) - bxsq) / (dhll * tmp);
+    ulst.my = dhll * wli[IVY] - ul.by * mfact;
+    ulst.mz = dhll * wli[IVZ] - ul.bz * mfact;
+    ulst.by = ul.by * bfact;
+    ulst.bz = ul.bz * bfact;
+  }
+  /* Ur* (hlld_iso.cpp:175-195) */
+  urst.d = dhll;
+  urst.mx = mxhll;
+  tmp = (spd[4] - spd[1]) * (spd[4] - spd[3]);
+  if (fabs(spd[4] - spd[3]) < (SMALL) * cs) {
+    urst.my = ur.my; urst.mz = ur.mz; urst.by = ur.by; urst.bz = ur.bz;
+  } else {
+    double mfact = bxi * (ustar - wri[IVX]) / tmp;
+    double bfact = (ur.d * SQR(spd[4] - wri[IVX]) - bxsq) / (dhll * tmp);
+    urst.my = dhll * wri[IVY] - ur.by * mfact;
+    urst.mz = dhll * wri[IVZ] - ur.bz * mfact;
+    urst.by = ur.by * bfact;
+    urst.bz = ur.bz * bfact;
+  }
+  /* Uc*, eqns. (34)-(37) (hlld_iso.cpp:197-204) */
+  double x = sqrtdhll * (bxi > 0.0 ? 1.0 : -1.0);
+  ucst.d = dhll;
+  ucst.mx = mxhll;
+  ucst.my = 0.5 * (ulst.my + urst.my + (urst.by - ulst.by) * x);
+  ucst.mz = 0.5 * (ulst.mz + urst.mz + (urst.bz - ulst.bz) * x);
+  ucst.by = 0.5 * (ulst.by + urst.by + (urst.my - ulst.my) / x);
+  ucst.bz = 0.5 * (ulst.bz + urst.bz + (urst.mz - ulst.mz) / x);
+  /* flux selection, eqns. (38a)-(38e) (hlld_iso.cpp:206-245) */
+  if (spd[0] >= 0.0) {
+    flxi[IDN] = fl.d; flxi[IVX] = fl.mx; flxi[IVY] = fl.my; flxi[IVZ] = fl.mz;
+    flxi[IBY] = fl.by; flxi[IBZ] = fl.bz;
+  } else if (spd[4] <= 0.0) {
+    flxi[IDN] = fr.d; flxi[IVX] = fr.mx; flxi[IVY] = fr.my; flxi[IVZ] = fr.mz;
+    flxi[IBY] = fr.by; flxi[IBZ] = fr.bz;
+  } else if (spd[1] >= 0.0) {
+    flxi[IDN] = fl.d + spd[0] * (ulst.d - ul.d);
+    flxi[IVX] = fl.mx + spd[0] * (ulst.mx - ul.mx);
+    flxi[IVY] = fl.my + spd[0] * (ulst.my - ul.my);
+    flxi[IVZ] = fl.mz + spd[0] * (ulst.mz - ul.mz);
+    flxi[IBY] = fl.by + spd[0] * (ulst.by - ul.by);
+    flxi[IBZ] = fl.bz + spd[0] * (ulst.bz - ul.bz);
+  } else if (spd[3] <= 0.0) {
+    flxi[IDN] = fr.d + spd[4] * (urst.d - ur.d);
+    flxi[IVX] = fr.mx + spd[4] * (urst.mx - ur.mx);
+    flxi[IVY] = fr.my + spd[4] * (urst.my - ur.my);
+    flxi[IVZ] = fr.mz + spd[4] * (urst.mz - ur.mz);
+    flxi[IBY] = fr.by + spd[4] * (urst.by - ur.by);
+    flxi[IBZ] = fr.bz + spd[4] * (urst.bz - ur.bz);
+  } else {
+    flxi[IDN] = dhll * ustar;
+    flxi[IVX] = fmxhll;
+    flxi[IVY] = ucst.my * ustar - bxi * ucst.by;
+    flxi[IVZ] = ucst.mz * ustar - bxi * ucst.bz;
+    flxi[IBY] = ucst.by * ustar - bxi * ucst.my / ucst.d;
+    flxi[IBZ] = ucst.bz * ustar - bxi * ucst.mz / ucst.d;
+  }
+  flxi[IEN] = 0.0;
 }
 
 /* Roe solver for adiabatic MHD, hydro/rsolvers/mhd/roe_mhd.cpp:43-214 with RoeFlux
@@ -1061,9 +1218,11 @@ static inline void recon_cell(const orc_ctx *c, const double *w, const double *b
      * eos/adiabatic_mhd.cpp:162-172) */
     double dfl = c->d.dfloor, pfl = c->d.pfloor;
     wm[IDN] = (wm[IDN] > dfl) ? wm[IDN] : dfl;
-    wm[IPR] = (wm[IPR] > pfl) ? wm[IPR] : pfl;
     wp[IDN] = (wp[IDN] > dfl) ? wp[IDN] : dfl;
-    wp[IPR] = (wp[IPR] > pfl) ? wp[IPR] : pfl;
+    if (!c->iso) {   /* eos/isothermal_mhd.cpp:135-140 floors the density only */
+      wm[IPR] = (wm[IPR] > pfl) ? wm[IPR] : pfl;
+      wp[IPR] = (wp[IPR] > pfl) ? wp[IPR] : pfl;
+    }
     return;
   }
   double q[NWAVE][3];
@@ -1151,7 +1310,8 @@ void orc_calculate_fluxes(orc_ctx *c, int order) {
             wri[IPR] = rm[IPR]; wri[IBY] = rm[IBY]; wri[IBZ] = rm[IBZ];
             int64_t fid = (dir == 0) ? F1IDX(c, k, j, i) : (dir == 1 ? F2IDX(c, k, j, i) : F3IDX(c, k, j, i));
             double bxi = bn[fid];
-            if (rs == 1) hlle(gamma, wli, wri, bxi, flxi);
+            if (rs == 1) hlle(gamma, c->iso, c->d.iso_cs, wli, wri, bxi, flxi);
+            else if (c->iso) hlld_iso(c->d.iso_cs, c->d.dfloor, wli, wri, bxi, flxi);
             else if (rs == 2) roe(gamma, wli, wri, bxi, flxi);
             else hlld(gamma, wli, wri, bxi, flxi);
             flx[IDN * nf + fid] = flxi[IDN];
@@ -1312,7 +1472,7 @@ void orc_integrate(orc_ctx *c, int stage) {
           double a2 = dx1[i] * dx3[k], a2p = dx1[i] * dx3[k];
           double a3 = dx1[i] * dx2[j], a3p = dx1[i] * dx2[j];
           double vol = dx1[i] * dx2[j] * dx3[k];
-          for (int n = 0; n < NHYDRO; ++n) {
+          for (int n = 0; n < c->nhydro; ++n) {
             double dflx = (a1p * f1[n * c->nf1 + F1IDX(c, k, j, i + 1)] - a1 * f1[n * c->nf1 + F1IDX(c, k, j, i)]);
             dflx += (a2p * f2[n * c->nf2 + F2IDX(c, k, j + 1, i)] - a2 * f2[n * c->nf2 + F2IDX(c, k, j, i)]);
             dflx += (a3p * f3[n * c->nf3 + F3IDX(c, k + 1, j, i)] - a3 * f3[n * c->nf3 + F3IDX(c, k, j, i)]);
@@ -1334,7 +1494,7 @@ void orc_integrate(orc_ctx *c, int stage) {
             double phir = 0.5 * (phi[id] + phi[id + 1]);
             u[IVX * nc + id] -= dtodx1 * w[IDN * nc + id] * (phir - phil);
             double fl = f1[IDN * c->nf1 + F1IDX(c, k, j, i)], fr = f1[IDN * c->nf1 + F1IDX(c, k, j, i + 1)];
-            u[IEN * nc + id] -= dtodx1 * (fl * (phic - phil) + fr * (phir - phic));
+            if (!c->iso) u[IEN * nc + id] -= dtodx1 * (fl * (phic - phil) + fr * (phir - phic));
           }
       for (int k = c->ks; k <= c->ke; ++k)
         for (int j = c->js; j <= c->je; ++j)
@@ -1345,7 +1505,7 @@ void orc_integrate(orc_ctx *c, int stage) {
             double phil = 0.5 * (phi[id - c->n1] + phi[id]);
             double phir = 0.5 * (phi[id] + phi[id + c->n1]);
             u[IVY * nc + id] -= dtodx2 * w[IDN * nc + id] * (phir - phil);
-            u[IEN * nc + id] -= dtodx2 * (f2[IDN * c->nf2 + F2IDX(c, k, j, i)] * (phic - phil) +
+            if (!c->iso) u[IEN * nc + id] -= dtodx2 * (f2[IDN * c->nf2 + F2IDX(c, k, j, i)] * (phic - phil) +
                                           f2[IDN * c->nf2 + F2IDX(c, k, j + 1, i)] * (phir - phic));
           }
       int64_t sk = (int64_t)c->n1 * c->n2;
@@ -1358,7 +1518,7 @@ void orc_integrate(orc_ctx *c, int stage) {
             double phil = 0.5 * (phi[id - sk] + phi[id]);
             double phir = 0.5 * (phi[id] + phi[id + sk]);
             u[IVZ * nc + id] -= dtodx3 * w[IDN * nc + id] * (phir - phil);
-            u[IEN * nc + id] -= dtodx3 * (f3[IDN * c->nf3 + F3IDX(c, k, j, i)] * (phic - phil) +
+            if (!c->iso) u[IEN * nc + id] -= dtodx3 * (f3[IDN * c->nf3 + F3IDX(c, k, j, i)] * (phic - phil) +
                                           f3[IDN * c->nf3 + F3IDX(c, k + 1, j, i)] * (phir - phic));
           }
     }
@@ -1400,7 +1560,6 @@ void orc_integrate(orc_ctx *c, int stage) {
 /* ---------------------------------------------------------------- time step */
 /* Hydro::NewBlockTimeStep (hydro/new_blockdt.cpp:42-190), min over all blocks. */
 double orc_new_block_dt(orc_ctx *c) {
-  double gamma = c->d.gamma;
   double mind = DBL_MAX;
 #pragma omp parallel for schedule(dynamic, 1) reduction(min : mind)
   for (int g = 0; g < c->nblocks; ++g) {
@@ -1421,15 +1580,15 @@ double orc_new_block_dt(orc_ctx *c) {
           wi[IVZ] = w[IVZ * nc + id]; wi[IPR] = w[IPR * nc + id];
           double bx = bcc[IB1 * nc + id] + fabs(b1[F1IDX(c, k, j, i)] - bcc[IB1 * nc + id]);
           wi[IBY] = bcc[IB2 * nc + id]; wi[IBZ] = bcc[IB3 * nc + id];
-          double cf = fast_speed(gamma, wi, bx);
+          double cf = fast_speed_eos(c, wi, bx);
           double dt1 = dx1[i] / (fabs(wi[IVX]) + cf);
           wi[IBY] = bcc[IB3 * nc + id]; wi[IBZ] = bcc[IB1 * nc + id];
           bx = bcc[IB2 * nc + id] + fabs(b2[F2IDX(c, k, j, i)] - bcc[IB2 * nc + id]);
-          cf = fast_speed(gamma, wi, bx);
+          cf = fast_speed_eos(c, wi, bx);
           double dt2 = dx2[j] / (fabs(wi[IVY]) + cf);
           wi[IBY] = bcc[IB1 * nc + id]; wi[IBZ] = bcc[IB2 * nc + id];
           bx = bcc[IB3 * nc + id] + fabs(b3[F3IDX(c, k, j, i)] - bcc[IB3 * nc + id]);
-          cf = fast_speed(gamma, wi, bx);
+          cf = fast_speed_eos(c, wi, bx);
           double dt3 = dx3[k] / (fabs(wi[IVZ]) + cf);
           m = dmin(m, dt1); m = dmin(m, dt2); m = dmin(m, dt3);
         }
@@ -2264,9 +2423,16 @@ void orc_conserved_totals(orc_ctx *c, double *out8) {
  * (normal), IVY, IVZ, IPR, IBY, IBZ): rs 0 = HLLD, 1 = HLLE.  flx[7] as flxi. */
 void orc_riemann(int rs, double gamma, const double *wl, const double *wr, double bx,
                  double *flx) {
-  if (rs == 1) hlle(gamma, wl, wr, bx, flx);
+  if (rs == 1) hlle(gamma, 0, 0.0, wl, wr, bx, flx);
   else if (rs == 2) roe(gamma, wl, wr, bx, flx);
   else hlld(gamma, wl, wr, bx, flx);
+}
+
+/* isothermal solvers at one interface: rs 0 HLLD-iso, 1 HLLE (isothermal branch) */
+void orc_riemann_iso(int rs, double cs, double dfloor, const double *wl, const double *wr,
+                     double bx, double *flx) {
+  if (rs == 1) hlle(0.0, 1, cs, wl, wr, bx, flx);
+  else hlld_iso(cs, dfloor, wl, wr, bx, flx);
 }
 
 /* PLM on a 1-D pencil q[0..n-1] with face-offset factors dxp/dxm (plm.cpp:69-77,

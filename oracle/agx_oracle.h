@@ -87,6 +87,8 @@ double orc_max_divb(orc_ctx *c);
 /* unit-level probes */
 void orc_riemann(int rs, double gamma, const double *wl, const double *wr, double bx,
                  double *flx);
+void orc_riemann_iso(int rs, double cs, double dfloor, const double *wl, const double *wr,
+                     double bx, double *flx);
 void orc_plm_pencil(int n, const double *q, const double *dxp, const double *dxm,
                     double *wl, double *wr);
 void orc_ppm_pencil(int n, const double *q, double *wl, double *wr);

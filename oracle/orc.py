@@ -33,15 +33,15 @@ FLT_MIN = np.finfo(np.float32).tiny
 def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)),
               nghost=2, xorder=2, rsolver=0, gamma=1.666666666666667, cfl=0.3,
               self_gravity=0, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=10,
-              precision=0):
+              precision=0, eos=0, iso_cs=1.0):
     bnx = bnx or nx
     d = Desc()
     d.nx1, d.nx2, d.nx3 = nx
     d.bnx1, d.bnx2, d.bnx3 = bnx
-    d.nghost, d.xorder, d.rsolver, d.eos = nghost, xorder, rsolver, 0
+    d.nghost, d.xorder, d.rsolver, d.eos = nghost, xorder, rsolver, eos
     d.self_gravity, d.mg_mode, d.mg_niter, d.precision = self_gravity, mg_mode, mg_niter, precision
     (d.x1min, d.x1max), (d.x2min, d.x2max), (d.x3min, d.x3max) = box
-    d.gamma, d.iso_cs, d.cfl = gamma, 1.0, cfl
+    d.gamma, d.iso_cs, d.cfl = gamma, iso_cs, cfl
     # EquationOfState defaults, eos/adiabatic_mhd.cpp:26-31: sqrt(1024*FLT_MIN)
     d.dfloor = d.pfloor = float(np.sqrt(1024 * np.float64(FLT_MIN)))
     d.four_pi_G, d.mg_threshold = four_pi_G, mg_threshold
@@ -82,6 +82,8 @@ def _lib():
     lib.orc_max_divb.argtypes = [P]
     lib.orc_conserved_totals.argtypes = [P, D]
     lib.orc_riemann.argtypes = [ctypes.c_int, ctypes.c_double, D, D, ctypes.c_double, D]
+    lib.orc_riemann_iso.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, D, D,
+                                    ctypes.c_double, D]
     lib.orc_plm_pencil.argtypes = [ctypes.c_int, D, D, D, D, D]
     lib.orc_ppm_pencil.argtypes = [ctypes.c_int, D, D, D]
     return lib
@@ -219,6 +221,17 @@ def riemann(rs, gamma, wl, wr, bx):
     wr = np.ascontiguousarray(wr, dtype=np.float64)
     out = np.zeros(7)
     lib().orc_riemann(rs, gamma, _dp(wl), _dp(wr), bx, _dp(out))
+    return out
+
+
+def riemann_iso(rs, cs, wl, wr, bx, dfloor=None):
+    """One interface flux [7] (energy slot 0) from the oracle's isothermal HLLD (rs=0) or
+    HLLE (rs=1)."""
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    out = np.zeros(7)
+    dfl = float(np.sqrt(1024 * np.float64(FLT_MIN))) if dfloor is None else dfloor
+    lib().orc_riemann_iso(rs, cs, dfl, _dp(wl), _dp(wr), bx, _dp(out))
     return out
 
 

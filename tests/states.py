@@ -36,7 +36,7 @@ def scatter_faces(desc, b1, b2, b3):
 
 
 def turbulent_state(desc, seed=0, mach=1.0, b0=(0.1, 0.0, 0.0), db=0.05, drho=0.2, gamma=5 / 3,
-                    p0=None, vamp=None):
+                    p0=None, vamp=None, iso=False):
     """Smooth random MHD state: a few random Fourier modes in v, rho, and a vector
     potential; returns block arrays (u, b1f, b2f, b3f)."""
     from tests.conftest import load_agx
@@ -75,6 +75,8 @@ def turbulent_state(desc, seed=0, mach=1.0, b0=(0.1, 0.0, 0.0), db=0.05, drho=0.
     U[0] = rho
     U[1:4] = [rho * vv for vv in v]
     U[4] = p / (gamma - 1) + 0.5 * rho * sum(vv * vv for vv in v) + 0.5 * (bc1 ** 2 + bc2 ** 2 + bc3 ** 2)
+    if iso:   # isothermal EOS: no energy variable (the oracle keeps the slot at 0)
+        U[4] = 0.0
     u = agx.scatter_cells(desc, U)
     f1, f2, f3 = scatter_faces(desc, b1, b2, b3)
     return u, f1, f2, f3

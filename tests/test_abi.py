@@ -41,6 +41,10 @@ def test_desc_layout(agx):
     (dict(xorder=4), -6),
     (dict(rsolver=9), -8),
     (dict(gamma=1.0), -11),
+    (dict(eos=2), -9),                                  # unknown EOS
+    (dict(eos=1, rsolver=2), -9),                       # Roe has no isothermal form here
+    (dict(eos=1, iso_cs=0.0), -9),
+    (dict(precision=3), -10),
 ])
 def test_invalid_desc_rejected(agx, bad, code):
     kw = dict(nx=(32, 16, 16), bnx=(16, 16, 16))
