@@ -2,8 +2,9 @@
 //
 // Device layout: every registered array of every MeshBlock lives in one HBM buffer,
 // block-major.  A per-variable slab is a padded box of (n3+1) planes x (n2+1) rows x
-// P doubles (P = pitch, a multiple of 16 doubles = 128 B, P >= n1+1), so cell, face and
-// edge arrays share one index function IDX(k,j,i) = (k*J + j)*P + i with J = n2+1.
+// P elements (P = pitch, a multiple of 32 elements, P >= n1+1), so cell, face and edge
+// arrays share one index function IDX(k,j,i) = (k*J + j)*P + i with J = n2+1.  Each
+// array's base is offset so that element i = is of every row opens a 128-B line.
 #pragma once
 
 #include <hip/hip_runtime.h>

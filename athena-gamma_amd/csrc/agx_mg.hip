@@ -59,6 +59,47 @@ k_mg_smooth(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3
   u[id] = uc - ((T(6.0) * uc - ukp - ujp - uip - ukm - ujm - uim) + src[id] * dx2) * isix;
 }
 
+// The same red/black sweep with one thread per (i, i+1) pair of a row (n1 even): the
+// pair is read and written as one 16-B (8-B in FP32) vector, so rows are read and
+// written as whole cache lines; the cell of the pair that is not of this colour is
+// written back unchanged (no other thread writes it, and its readers see the same
+// value either way).  Arithmetic as k_mg_smooth.
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_mg_smooth_pairs(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3,
+                  T dx2, T isix, int color) {
+  using V = typename Vec2<T>::type;
+  const unsigned nh = (unsigned)n1 >> 1;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 pairs per level
+  if (t >= nh * (unsigned)n2 * (unsigned)n3) return;
+  const unsigned h = t % nh;
+  const unsigned r = t / nh;
+  const int j = (int)(r % (unsigned)n2);
+  const int k = (int)(r / (unsigned)n2);
+  const int odd = (color + k + j) & 1;   // 0: update i = 2h, 1: update i = 2h + 1
+  const int i = 2 * (int)h + odd;
+  V *u2 = reinterpret_cast<V *>(u);
+  const int64_t row = ((int64_t)k * n2 + j) * nh;
+  V c = u2[row + h];
+  V kp = u2[((int64_t)wrapp(k, n3) * n2 + j) * nh + h];
+  V jp = u2[((int64_t)k * n2 + wrapp(j, n2)) * nh + h];
+  V km = u2[((int64_t)wrapm(k, n3) * n2 + j) * nh + h];
+  V jm = u2[((int64_t)k * n2 + wrapm(j, n2)) * nh + h];
+  T uc = odd ? c.y : c.x;
+  T ukp = odd ? kp.y : kp.x, ujp = odd ? jp.y : jp.x;
+  T ukm = odd ? km.y : km.x, ujm = odd ? jm.y : jm.x;
+  T uip = odd ? u[LIX(n1, n2, k, j, wrapp(i, n1))] : c.y;
+  T uim = odd ? c.x : u[LIX(n1, n2, k, j, wrapm(i, n1))];
+  T v = uc - ((T(6.0) * uc - ukp - ujp - uip - ukm - ujm - uim) + src[LIX(n1, n2, k, j, i)] * dx2) * isix;
+  if (odd) c.y = v;
+  else c.x = v;
+  u2[row + h] = c;
+}
+
 template <typename T>
 __device__ __forceinline__ T defect_at(const T *u, const T *src, int n1, int n2,
                                        int n3, int k, int j, int i, T idx2) {
@@ -317,6 +358,7 @@ namespace agx {
 
 template <typename T>
 static int64_t lcells(const MGLevelT<T> &L) { return (int64_t)L.n1 * L.n2 * L.n3; }
+static int64_t pad32(int64_t n) { return (n + 63) / 64 * 64; }   // 64 elements: >= 256 B
 
 int mg_create(agx_mesh *m) {
   const agx_mesh_desc &d = m->d;
@@ -364,7 +406,7 @@ int mg_create(agx_mesh *m) {
     Lv.n3 = d.nx3 >> sh;
     if (l >= nrl) Lv.dx = m->rdx_blk * (double)(1 << sh);
     else Lv.dx = m->rdx_root * (double)(1 << (nrl - 1 - l));
-    tot += 3 * lcells(Lv);
+    tot += 3 * pad32(lcells(Lv));
   }
   m->mg_bytes = tot * (int64_t)sizeof(double);
   AGX_HIP(hipMalloc(&m->mgpool, m->mg_bytes));
@@ -372,7 +414,7 @@ int mg_create(agx_mesh *m) {
   double *p = m->mgpool;
   for (int l = 0; l < L; ++l) {
     MGLevel &Lv = m->lev[l];
-    int64_t n = lcells(Lv);
+    int64_t n = pad32(lcells(Lv));   // every array 256-B aligned (vector loads of pairs)
     Lv.u = p; p += n;
     Lv.src = p; p += n;
     Lv.def = p; p += n;
@@ -381,7 +423,7 @@ int mg_create(agx_mesh *m) {
     // FP32 hierarchy of the correction V-cycles: u (correction) and src per level
     m->lev32.resize(L);
     int64_t t32 = 0;
-    for (int l = 0; l < L; ++l) t32 += 2 * lcells(m->lev[l]);
+    for (int l = 0; l < L; ++l) t32 += 2 * pad32(lcells(m->lev[l]));
     AGX_HIP(hipMalloc(&m->mgpool32, t32 * (int64_t)sizeof(float)));
     AGX_HIP(hipMemsetAsync(m->mgpool32, 0, t32 * sizeof(float), m->stream));
     m->mg_bytes += t32 * (int64_t)sizeof(float);
@@ -389,7 +431,7 @@ int mg_create(agx_mesh *m) {
     for (int l = 0; l < L; ++l) {
       MGLevel32 &Lv = m->lev32[l];
       Lv.n1 = m->lev[l].n1; Lv.n2 = m->lev[l].n2; Lv.n3 = m->lev[l].n3; Lv.dx = m->lev[l].dx;
-      int64_t n = lcells(Lv);
+      int64_t n = pad32(lcells(Lv));
       Lv.u = q; q += n;
       Lv.src = q; q += n;
       Lv.def = nullptr;
@@ -426,8 +468,12 @@ static void smooth(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l, int color) {
   double isix = 1.15 / 6.0;
   int64_t n = (int64_t)((L.n1 + 1) / 2) * L.n2 * L.n3;
   timer_begin(m, AGX_K_MG_SMOOTH);
-  hipLaunchKernelGGL(k_mg_smooth<T>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, L.u, L.src, L.n1, L.n2,
-                     L.n3, T(dx2), T(isix), color ^ m->coffset);
+  if ((L.n1 & 1) == 0)
+    hipLaunchKernelGGL(k_mg_smooth_pairs<T>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, L.u, L.src, L.n1,
+                       L.n2, L.n3, T(dx2), T(isix), color ^ m->coffset);
+  else
+    hipLaunchKernelGGL(k_mg_smooth<T>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, L.u, L.src, L.n1, L.n2,
+                       L.n3, T(dx2), T(isix), color ^ m->coffset);
   timer_end(m);
 }
 

@@ -1528,6 +1528,19 @@ k_stage(Geo g, StageArgs<TW> a) {
         a.bccn[bo] = TW(bcc1);
         a.bccn[bo + S] = TW(bcc2);
         a.bccn[bo + 2 * S] = TW(bcc3);
+        // ghost copies of the un-floored conserved state (bvals_cc.cpp:201-336 sends u
+        // before ConservedToPrimitive; the receiver converts its ghosts itself)
+        {
+          int rc[6];
+          push_range(i, g.is, g.ie, g.bx1, g.ng, rc[0], rc[1]);
+          push_range(j, g.js, g.je, g.bx2, g.ng, rc[2], rc[3]);
+          push_range(c - 1, g.ks, g.ke, g.bx3, g.ng, rc[4], rc[5]);
+          for_each_ghost(g, b, c - 1, j, i, rc, [&](int nb, int64_t id2) {
+            const int64_t c2 = (int64_t)nb * NHYDRO * S + id2;
+#pragma unroll
+            for (int n = 0; n < NH; ++n) a.dst[c2 + n * S] = un[n];
+          });
+        }
         // C2P writes its floors back into u (adiabatic_mhd.cpp:66-84)
         double wv[5];
         c2p_eos<ISO>(un[IDN], un[IVX], un[IVY], un[IVZ], un[IEN], bcc1, bcc2, bcc3, a.gm1, a.dfl, a.pfl,
@@ -1536,24 +1549,6 @@ k_stage(Geo g, StageArgs<TW> a) {
         for (int n = 0; n < NH; ++n) a.dst[cb0 + n * S] = un[n];
 #pragma unroll
         for (int n = 0; n < NH; ++n) a.wn[cb0 + n * S] = TW(wv[n]);
-        // ghost copies: a ghost cell's u, w and bcc equal those of its source cell (same
-        // conserved state, and bcc from the same face values), so they are copied
-        {
-          int rc[6];
-          push_range(i, g.is, g.ie, g.bx1, g.ng, rc[0], rc[1]);
-          push_range(j, g.js, g.je, g.bx2, g.ng, rc[2], rc[3]);
-          push_range(c - 1, g.ks, g.ke, g.bx3, g.ng, rc[4], rc[5]);
-          for_each_ghost(g, b, c - 1, j, i, rc, [&](int nb, int64_t id2) {
-            const int64_t c2 = (int64_t)nb * NHYDRO * S + id2, b2 = (int64_t)nb * 3 * S + id2;
-#pragma unroll
-            for (int n = 0; n < NH; ++n) a.dst[c2 + n * S] = un[n];
-#pragma unroll
-            for (int n = 0; n < NH; ++n) a.wn[c2 + n * S] = TW(wv[n]);
-            a.bccn[b2] = TW(bcc1);
-            a.bccn[b2 + S] = TW(bcc2);
-            a.bccn[b2 + 2 * S] = TW(bcc3);
-          });
-        }
         if (a.dtpart)
           dtmin = dmin(dtmin, cell_dt<ISO>(a.gamma, a.cs, wv, bcc1, bcc2, bcc3, b1l, b2l, b3_lo, dx));
       }
@@ -1565,6 +1560,99 @@ k_stage(Geo g, StageArgs<TW> a) {
     double m = wg_min256(dtmin, s_red);
     if (tid == 0) a.dtpart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = m;
   }
+}
+
+// The q-th element of the shell of the box [0,N1)x[0,N2)x[0,N3) that lies outside the
+// inner box [a1,b1]x[a2,b2]x[a3,b3]: whole planes first, then whole rows of the inner
+// planes, then the cells of the inner rows.
+struct Shell {
+  int N1, N2, N3, a1, b1, a2, b2, a3, b3;
+  int64_t nk, nj, ni;
+};
+
+static Shell make_shell(int N1, int N2, int N3, int a1, int b1, int a2, int b2, int a3, int b3) {
+  Shell s{N1, N2, N3, a1, b1, a2, b2, a3, b3, 0, 0, 0};
+  s.nk = (int64_t)(a3 + (N3 - 1 - b3)) * N2 * N1;
+  s.nj = (int64_t)(b3 - a3 + 1) * (a2 + (N2 - 1 - b2)) * N1;
+  s.ni = (int64_t)(b3 - a3 + 1) * (b2 - a2 + 1) * (a1 + (N1 - 1 - b1));
+  return s;
+}
+
+__device__ __forceinline__ bool shell_elem(const Shell &s, unsigned q, int &k, int &j, int &i) {
+  // 32-bit index math: a shell has fewer than 2^31 elements per MeshBlock
+  const unsigned N1 = s.N1, plane = (unsigned)s.N1 * (unsigned)s.N2;
+  const unsigned nk = (unsigned)s.nk, nj = (unsigned)s.nj, ni = (unsigned)s.ni;
+  if (q < nk) {
+    int pidx = (int)(q / plane);
+    unsigned r = q - (unsigned)pidx * plane;
+    k = (pidx < s.a3) ? pidx : s.b3 + 1 + (pidx - s.a3);
+    j = (int)(r / N1);
+    i = (int)(r - (unsigned)j * N1);
+    return true;
+  }
+  q -= nk;
+  if (q < nj) {
+    const unsigned rows_out = s.a2 + (s.N2 - 1 - s.b2);
+    const unsigned per = rows_out * N1;
+    int kk = (int)(q / per);
+    unsigned r = q - (unsigned)kk * per;
+    int rr = (int)(r / N1);
+    i = (int)(r - (unsigned)rr * N1);
+    k = s.a3 + kk;
+    j = (rr < s.a2) ? rr : s.b2 + 1 + (rr - s.a2);
+    return true;
+  }
+  q -= nj;
+  if (q < ni) {
+    const unsigned cols_out = s.a1 + (s.N1 - 1 - s.b1);
+    const unsigned nrows = s.b2 - s.a2 + 1;
+    const unsigned per = nrows * cols_out;
+    int kk = (int)(q / per);
+    unsigned r = q - (unsigned)kk * per;
+    int jj = (int)(r / cols_out);
+    int cc = (int)(r - (unsigned)jj * cols_out);
+    k = s.a3 + kk;
+    j = s.a2 + jj;
+    i = (cc < s.a1) ? cc : s.b1 + 1 + (cc - s.a1);
+    return true;
+  }
+  return false;
+}
+
+// ConservedToPrimitive over the ghost zones (time_integrator.cpp:1971-1977,
+// eos/adiabatic_mhd.cpp:41-90 or isothermal_mhd.cpp:40-73) after the exchange: bcc
+// from this block's own faces (its boundary faces and the received ghost faces), C2P of
+// the received conserved state, floors written back.  One thread per ghost cell.  The
+// two copies of a face shared by neighbouring blocks are computed separately and may
+// differ by rounding, so a ghost's primitives are not copied from its source cell.
+template <typename TW, bool ISO>
+__global__ void __launch_bounds__(256)
+k_ghost_c2p(Geo g, Shell s, double *__restrict__ u, TW *__restrict__ w, TW *__restrict__ bcc,
+            const double *__restrict__ b1, const double *__restrict__ b2,
+            const double *__restrict__ b3, double gm1, double dfl, double pfl) {
+  const unsigned q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= s.nk + s.nj + s.ni) return;
+  int k, j, i;
+  shell_elem(s, q, k, j, i);
+  const int b = blockIdx.z;
+  const int64_t S = g.S;
+  const int64_t id = IDX(g, k, j, i);
+  const int64_t c = (int64_t)b * NHYDRO * S + id;
+  const int64_t fo = (int64_t)b * S + id;
+  const double bcc1 = 0.5 * b1[fo] + 0.5 * b1[fo + 1];
+  const double bcc2 = 0.5 * b2[fo] + 0.5 * b2[fo + g.P];
+  const double bcc3 = 0.5 * b3[fo] + 0.5 * b3[fo + (int64_t)g.J * g.P];
+  const int64_t bo = (int64_t)b * 3 * S + id;
+  bcc[bo] = TW(bcc1);
+  bcc[bo + S] = TW(bcc2);
+  bcc[bo + 2 * S] = TW(bcc3);
+  double u_d = u[c], u_e = ISO ? 0.0 : u[c + 4 * S], wv[5];
+  c2p_eos<ISO>(u_d, u[c + S], u[c + 2 * S], u[c + 3 * S], u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
+  u[c] = u_d;
+  if (!ISO) u[c + 4 * S] = u_e;
+  constexpr int NH = ISO ? 4 : NHYDRO;
+#pragma unroll
+  for (int n = 0; n < NH; ++n) w[c + n * S] = TW(wv[n]);
 }
 
 // Hydro::NewBlockTimeStep (hydro/new_blockdt.cpp:57-160): per-cell bound, min over all
@@ -1685,6 +1773,13 @@ extern "C" {
 
 const char *agx_last_error(void) { return agx::g_err.c_str(); }
 
+// bytes of one registered array: nv slabs per block, plus room for the alignment offset,
+// rounded to 256 B
+static int64_t arr_bytes(int nv, const Geo &g, int esz) {
+  int64_t b = ((int64_t)nv * g.nb * g.S + 32) * esz;
+  return (b + 255) / 256 * 256;
+}
+
 int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   if (!out) { set_error("null output pointer"); return -1; }
   *out = nullptr;
@@ -1706,7 +1801,8 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   g.is = g.ng; g.ie = g.ng + g.bx1 - 1;
   g.js = g.ng; g.je = g.ng + g.bx2 - 1;
   g.ks = g.ng; g.ke = g.ng + g.bx3 - 1;
-  g.P = ((g.n1 + 1) + 15) / 16 * 16;
+  // row pitch: a multiple of 32 elements, so FP64 and FP32 rows both start on a 128-B line
+  g.P = ((g.n1 + 1) + 31) / 32 * 32;
   g.PL = g.n1 > g.n2 ? g.n1 : g.n2;
   g.PL = g.PL > g.n3 ? g.PL : g.n3;
   g.J = g.n2 + 1;
@@ -1724,7 +1820,7 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
     m->arr.nvar[a] = nv[a];
     m->arr.esz[a] = f32 ? 4 : 8;
     // every array starts on a 256-B boundary (S is a multiple of 16 elements)
-    total += (int64_t)nv[a] * g.nb * g.S * m->arr.esz[a];
+    total += arr_bytes(nv[a], g, m->arr.esz[a]);
   }
   m->pool_bytes = total;
   hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
@@ -1733,10 +1829,14 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   if (e != hipSuccess) { int r = hip_fail(e, "hipMalloc(pool)"); agx_mesh_destroy(m); return r; }
   e = hipMemsetAsync(m->pool, 0, m->pool_bytes, m->stream);
   if (e != hipSuccess) { int r = hip_fail(e, "hipMemset(pool)"); agx_mesh_destroy(m); return r; }
+  // every array starts OFF elements past a 256-B boundary so that the first interior
+  // element of each row (i = is) opens a 128-B line: wavefront-wide row segments of the
+  // interior then map onto whole lines (no partially read or written lines)
+  const int OFF = (32 - g.ng % 32) % 32;
   char *ptr = reinterpret_cast<char *>(m->pool);
   for (int a = 0; a < ARR_TOTAL; ++a) {
-    m->arr.a[a] = reinterpret_cast<double *>(ptr);
-    ptr += (int64_t)nv[a] * g.nb * g.S * m->arr.esz[a];
+    m->arr.a[a] = reinterpret_cast<double *>(ptr + (int64_t)OFF * m->arr.esz[a]);
+    ptr += arr_bytes(nv[a], g, m->arr.esz[a]);
   }
   // per-block geometry
   m->coef_h.resize(g.nb);
@@ -2139,9 +2239,10 @@ int agx_initialize(agx_mesh *m, double tlim) {
 
 // One VL2 stage with the fused kernels (time_integrator.cpp:1460-2034): fluxes, then
 // k_stage (corner EMFs, CT, update, C2P with floors, the neighbours' ghost copies of the
-// new cells and faces and, when dtpart is set, the time-step partials), then the
-// register rotation.  Same results as agx_calculate_fluxes + agx_compute_corner_e +
-// agx_integrate + agx_bvals_exchange + agx_cons2prim.
+// new faces and of the un-floored conserved state and, when dtpart is set, the time-step
+// partials), the register rotation, then C2P over the ghost zones.  Same results as
+// agx_calculate_fluxes + agx_compute_corner_e + agx_integrate + agx_bvals_exchange +
+// agx_cons2prim.
 //
 // Registers: stage 1 reads u and b and writes u1 and b1; the swap makes (u, b) the
 // half-step state and (u1, b1) the start state.  Stage 2 updates u1 in place from the
@@ -2191,6 +2292,24 @@ static void launch_stage(agx_mesh *m, int stage, double wght, double *dtpart) {
 }  // extern "C++"
 
 
+extern "C++" {
+template <typename TW>
+static void launch_ghost_c2p(agx_mesh *m) {
+  const Geo &g = m->g;
+  double **A = m->arr.a;
+  const Shell cs = make_shell(g.n1, g.n2, g.n3, g.is, g.ie, g.js, g.je, g.ks, g.ke);
+  const dim3 grid(grid1d(cs.nk + cs.nj + cs.ni, 256), 1, g.nb);
+  if (m->iso)
+    hipLaunchKernelGGL((k_ghost_c2p<TW, true>), grid, dim3(256), 0, m->stream, g, cs, A[AGX_ARR_U],
+                       as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]), A[AGX_ARR_B1F], A[AGX_ARR_B2F],
+                       A[AGX_ARR_B3F], m->d.gamma - 1.0, m->d.dfloor, m->d.pfloor);
+  else
+    hipLaunchKernelGGL((k_ghost_c2p<TW, false>), grid, dim3(256), 0, m->stream, g, cs, A[AGX_ARR_U],
+                       as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]), A[AGX_ARR_B1F], A[AGX_ARR_B2F],
+                       A[AGX_ARR_B3F], m->d.gamma - 1.0, m->d.dfloor, m->d.pfloor);
+}
+}  // extern "C++"
+
 static int stage_fused(agx_mesh *m, int stage, bool want_dt) {
   double **A = m->arr.a;
   int rc = agx_calculate_fluxes(m, stage == 1 ? 1 : m->d.xorder);
@@ -2214,6 +2333,11 @@ static int stage_fused(agx_mesh *m, int stage, bool want_dt) {
       A[ARR_Q1F + d] = start;
     }
   }
+  timer_begin(m, AGX_K_BVALS);
+  if (m->mixed) launch_ghost_c2p<float>(m);
+  else launch_ghost_c2p<double>(m);
+  timer_end(m);
+  LAUNCH_CHECK("k_ghost_c2p");
   return 0;
 }
 
