@@ -41,6 +41,27 @@ __host__ __device__ inline int64_t IDX(const Geo &g, int k, int j, int i) {
   return ((int64_t)k * g.J + j) * g.P + i;
 }
 
+// XCD-aware workgroup order.  The dispatcher hands workgroup n of a launch to XCD
+// n mod 8, and each XCD has its own L2, so with the plain order a workgroup's stencil
+// neighbours (the next rows, planes or tiles) run on other XCDs and their shared lines
+// are fetched once per XCD.  xcd_slot renumbers the launch so that each XCD runs one
+// contiguous range of the index space, in order; it is a bijection on [0, total).
+__host__ __device__ inline unsigned xcd_slot(unsigned lin, unsigned total) {
+  constexpr unsigned NXCD = 8;
+  const unsigned q = total / NXCD, r = total % NXCD;
+  const unsigned x = lin % NXCD, s = lin / NXCD;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + s;
+}
+
+// (x, y, z) workgroup index of this workgroup in the XCD-aware order
+__device__ __forceinline__ uint3 xcd_block() {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned n = xcd_slot(lin, gx * gy * gridDim.z);
+  const unsigned yz = n / gx;
+  return make_uint3(n - yz * gx, yz % gy, yz / gy);
+}
+
 // Per-block constant geometry on the device (uniform Cartesian mesh).
 struct BlockCoef {
   double dx[3];            // dx?f (== dx?v) of the block, coordinates.cpp:127-145

@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(BS)
 k_mg_smooth(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3,
             T dx2, T isix, int color) {
   int nh = (n1 + 1) / 2;
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   int64_t tot = (int64_t)nh * n2 * n3;
   if (t >= tot) return;
   int h = (int)(t % (unsigned)nh);
@@ -74,7 +74,7 @@ k_mg_smooth_pairs(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, 
                   T dx2, T isix, int color) {
   using V = typename Vec2<T>::type;
   const unsigned nh = (unsigned)n1 >> 1;
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 pairs per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 pairs per level
   if (t >= nh * (unsigned)n2 * (unsigned)n3) return;
   const unsigned h = t % nh;
   const unsigned r = t / nh;
@@ -120,7 +120,7 @@ k_mg_defect_restrict(const T *__restrict__ u, const T *__restrict__ src,
                      T *__restrict__ csrc, T *__restrict__ cu, int n1, int n2, int n3,
                      T idx2) {
   int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)c1 * c2 * c3) return;
   int i = (int)(t % (unsigned)c1);
   unsigned r = t / (unsigned)c1;
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(BS)
 k_mg_restrict_src(const T *__restrict__ src, T *__restrict__ csrc, int n1, int n2,
                   int n3) {
   int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)c1 * c2 * c3) return;
   int i = (int)(t % (unsigned)c1);
   unsigned r = t / (unsigned)c1;
@@ -166,7 +166,7 @@ k_mg_restrict_src(const T *__restrict__ src, T *__restrict__ csrc, int n1, int n
 template <typename T>
 __global__ void __launch_bounds__(BS)
 k_mg_prolong(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3) {
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)n1 * n2 * n3) return;
   int fi = (int)(t % (unsigned)n1);
   unsigned r = t / (unsigned)n1;
@@ -189,7 +189,7 @@ k_mg_prolong(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3
 template <typename T>
 __global__ void __launch_bounds__(BS)
 k_mg_fmg_prolong(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3) {
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)n1 * n2 * n3) return;
   int fi = (int)(t % (unsigned)n1);
   unsigned r = t / (unsigned)n1;
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(BS)
 k_mg_partial(const T *__restrict__ a, const T *__restrict__ src, int n1, int n2, int n3,
              T idx2, double *__restrict__ part) {
   __shared__ double sh[BS];
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   double v = 0.0;
   if (t < (int64_t)n1 * n2 * n3) {
     if (MODE == 0) {
@@ -266,13 +266,13 @@ __global__ void __launch_bounds__(BS) k_mg_final(const double *__restrict__ part
 template <typename T>
 __global__ void __launch_bounds__(BS) k_mg_add(T *__restrict__ a, int64_t n,
                                                const double *__restrict__ val, double scale) {
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t < n) a[t] -= T((*val) * scale);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(BS) k_mg_zero(T *__restrict__ a, int64_t n) {
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t < n) a[t] = T(0.0);
 }
 
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(BS)
 k_mg_defect_to32(const double *__restrict__ u, const double *__restrict__ src, float *__restrict__ d32,
                  int n1, int n2, int n3, double idx2, double *__restrict__ part) {
   __shared__ double sh[BS];
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   double v = 0.0;
   if (t < (int64_t)n1 * n2 * n3) {
     int i = (int)(t % (unsigned)n1);
@@ -307,7 +307,7 @@ k_mg_defect_to32(const double *__restrict__ u, const double *__restrict__ src, f
 // converts an FP64 array to FP32 (src32 = src).
 __global__ void __launch_bounds__(BS)
 k_mg_mix(double *__restrict__ u, float *__restrict__ e, int64_t n, int mode) {
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= n) return;
   if (mode == 0) u[t] += (double)e[t];
   else if (mode == 1) u[t] = (double)e[t];
@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(BS)
 k_mg_load(Geo g, const double *__restrict__ u, const double *__restrict__ phi,
           double *__restrict__ src, double *__restrict__ mu, double fac, int load_phi) {
   int N1 = g.nb1 * g.bx1, N2 = g.nb2 * g.bx2, N3 = g.nb3 * g.bx3;
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)N1 * N2 * N3) return;
   int I = (int)(t % (unsigned)N1);
   unsigned r = t / (unsigned)N1;
@@ -337,7 +337,7 @@ k_mg_load(Geo g, const double *__restrict__ u, const double *__restrict__ phi,
 __global__ void __launch_bounds__(BS)
 k_mg_retrieve(Geo g, const double *__restrict__ mu, double *__restrict__ phi) {
   int m1 = g.bx1 + 2, m2 = g.bx2 + 2, m3 = g.bx3 + 2;
-  const unsigned t = blockIdx.x * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)m1 * m2 * m3 * g.nb) return;
   int li = (int)(t % (unsigned)m1);
   unsigned r = t / (unsigned)m1;

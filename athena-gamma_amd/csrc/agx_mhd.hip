@@ -87,15 +87,19 @@ struct PlaneIdx {
 };
 
 // (b,k,j,i) of this thread for the box [kl..ku]x[jl..ju]x[il..iu]; false if outside.
-__device__ __forceinline__ bool plane_index(int il, int iu, int jl, int ju, int kl, PlaneIdx &p) {
+// With xcd, workgroups are taken in the XCD-aware order (xcd_block): consecutive planes
+// of a block then run on one XCD, which pays off for kernels with a k stencil.
+__device__ __forceinline__ bool plane_index(int il, int iu, int jl, int ju, int kl, PlaneIdx &p,
+                                            bool xcd = false) {
   const unsigned ni = iu - il + 1, nj = ju - jl + 1;
-  const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint3 wg = xcd ? xcd_block() : make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
+  const unsigned q = wg.x * blockDim.x + threadIdx.x;
   if (q >= ni * nj) return false;
   const unsigned qj = q / ni;
   p.j = jl + (int)qj;
   p.i = il + (int)(q - qj * ni);
-  p.k = kl + (int)blockIdx.y;
-  p.b = blockIdx.z;
+  p.k = kl + (int)wg.y;
+  p.b = wg.z;
   return true;
 }
 
@@ -104,10 +108,11 @@ static dim3 plane_grid(int il, int iu, int jl, int ju, int kl, int ku, int nb) {
   return dim3(grid1d(n, 256), (unsigned)(ku - kl + 1), (unsigned)nb);
 }
 
-// Stage-kernel tiles: TI x TJ (i,j) columns per 256-thread workgroup; grid.x = tiles of
-// one MeshBlock, grid.y = MeshBlock.  The tile shape is picked per mesh by agx_autotune.
-constexpr int NTILE = 3;
-constexpr int TILE_I[NTILE] = {16, 32, 64}, TILE_J[NTILE] = {16, 8, 4};
+// Stage-kernel tiles: TI x TJ (i,j) columns per workgroup of TI*TJ threads (256, or 512
+// for the 64 x 8 tile); grid.x = tiles of one MeshBlock, grid.y = MeshBlock.  The tile
+// shape is picked per mesh by agx_autotune.
+constexpr int NTILE = 4;
+constexpr int TILE_I[NTILE] = {16, 32, 64, 64}, TILE_J[NTILE] = {16, 8, 4, 8};
 static dim3 stage_grid(const Geo &g, int tile) {
   int nti = (g.bx1 + TILE_I[tile] - 1) / TILE_I[tile], ntj = (g.bx2 + TILE_J[tile] - 1) / TILE_J[tile];
   return dim3((unsigned)(nti * ntj), (unsigned)g.nb);
@@ -868,7 +873,7 @@ k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
   int jl = g.js - (DIR != 1), ju = g.je + 1;
   int kl = g.ks - (DIR != 2);
   PlaneIdx p;
-  if (!plane_index(il, iu, jl, ju, kl, p)) return;
+  if (!plane_index(il, iu, jl, ju, kl, p, DIR == 2)) return;   // x3: 4-plane stencil
   const T gamma = T(gamma_), cs = T(cs_), dfl = T(dfl_), pfl = T(pfl_);
   const int64_t st = (DIR == 0) ? 1 : (DIR == 1 ? (int64_t)g.P : (int64_t)g.J * g.P);
   const int64_t id = IDX(g, p.k, p.j, p.i);
@@ -1260,14 +1265,19 @@ __device__ __forceinline__ double cell_dt(double gamma, double cs, const double 
   return m;
 }
 
-// Minimum over a 256-thread workgroup (exact, so the order is free): wavefront shuffle,
-// then LDS across the 4 wavefronts; thread 0 returns the result.
-__device__ __forceinline__ double wg_min256(double m, double *sh) {
+// Minimum over a workgroup of NW wavefronts (exact, so the order is free): wavefront
+// shuffle, then LDS across the wavefronts; every thread returns the result.
+template <int NW>
+__device__ __forceinline__ double wg_min(double m, double *sh) {
   for (int off = 32; off > 0; off >>= 1) m = dmin(m, __shfl_xor(m, off, 64));
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
-  return dmin(dmin(sh[0], sh[1]), dmin(sh[2], sh[3]));
+  double r = sh[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) r = dmin(r, sh[w]);
+  return r;
 }
+__device__ __forceinline__ double wg_min256(double m, double *sh) { return wg_min<4>(m, sh); }
 
 // ================================================================ fused stage kernel
 // One VL2 stage after the fluxes: corner EMFs (field/calculate_corner_e.cpp:130-229),
@@ -1333,22 +1343,23 @@ struct StageArgs {
 };
 
 template <int TI, int TJ, typename TW, bool ISO>
-__global__ void __launch_bounds__(TI * TJ, 4)   // 4 workgroups per CU: <= 128 VGPRs, 4 x 37 KB LDS
+__global__ void __launch_bounds__(TI * TJ, 4)   // >= 4 waves per SIMD: <= 128 VGPRs
 k_stage(Geo g, StageArgs<TW> a) {
-  static_assert(TI * TJ == 256, "k_stage runs 256-thread workgroups");
+  static_assert(TI * TJ == 256 || TI * TJ == 512, "k_stage runs 256- or 512-thread workgroups");
   constexpr int NH = ISO ? 4 : NHYDRO;   // conserved variables (no energy if isothermal)
   constexpr int NT = TI * TJ, CI = TI + 2, CJ = TJ + 2;
   __shared__ double s_cce[3][3][CJ][CI];      // planes c-1, c, c+1 (slot = plane mod 3)
   __shared__ double s_e1[2][TJ + 1][TI];      // edge planes c-1, c (slot = plane & 1)
   __shared__ double s_e2[2][TJ][TI + 1];
   __shared__ double s_e3[2][TJ + 1][TI + 1];
-  __shared__ double s_red[4];
+  __shared__ double s_red[TI * TJ / 64];
   const int tid = threadIdx.x;
   const int tx = tid % TI, ty = tid / TI;
   const int nti = (g.bx1 + TI - 1) / TI;
-  const int b = blockIdx.y;
-  const int i0 = g.is + (int)(blockIdx.x % nti) * TI;
-  const int j0 = g.js + (int)(blockIdx.x / nti) * TJ;
+  const uint3 wg = xcd_block();   // whole MeshBlocks per XCD: tiles share halo lines in L2
+  const int b = wg.y;
+  const int i0 = g.is + (int)(wg.x % nti) * TI;
+  const int j0 = g.js + (int)(wg.x / nti) * TJ;
   const int ni = min(TI, g.ie + 1 - i0), nj = min(TJ, g.je + 1 - j0);
   const bool own = (tx < ni) && (ty < nj);
   const int i = i0 + tx, j = j0 + ty;
@@ -1557,7 +1568,7 @@ k_stage(Geo g, StageArgs<TW> a) {
     __syncthreads();
   }
   if (a.dtpart) {
-    double m = wg_min256(dtmin, s_red);
+    double m = wg_min<TI * TJ / 64>(dtmin, s_red);
     if (tid == 0) a.dtpart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = m;
   }
 }
@@ -1630,11 +1641,12 @@ __global__ void __launch_bounds__(256)
 k_ghost_c2p(Geo g, Shell s, double *__restrict__ u, TW *__restrict__ w, TW *__restrict__ bcc,
             const double *__restrict__ b1, const double *__restrict__ b2,
             const double *__restrict__ b3, double gm1, double dfl, double pfl) {
-  const unsigned q = blockIdx.x * 256 + threadIdx.x;
+  const uint3 wg = xcd_block();
+  const unsigned q = wg.x * 256 + threadIdx.x;
   if (q >= s.nk + s.nj + s.ni) return;
   int k, j, i;
   shell_elem(s, q, k, j, i);
-  const int b = blockIdx.z;
+  const int b = wg.z;
   const int64_t S = g.S;
   const int64_t id = IDX(g, k, j, i);
   const int64_t c = (int64_t)b * NHYDRO * S + id;
@@ -1948,6 +1960,11 @@ int64_t agx_mesh_bytes(agx_mesh *m) { return m ? m->pool_bytes + m->mg_bytes : 0
 
 // Strided 3D copies between the tight host layout and the padded device layout.  FP32
 // arrays of the mixed-precision form are staged through a host buffer and converted.
+// Host <-> device copy of one registered array.  Each device slab (one variable of one
+// block, padded layout) is packed or unpacked on the host in a pinned staging buffer and
+// moved whole with one contiguous copy; two staging halves alternate so that packing
+// overlaps the transfer.  Padding elements are written as zeros (they hold zeros from
+// agx_mesh_create on and no kernel writes them).
 static int copy_array(agx_mesh *m, int arr, double *host, bool to_device) {
   if (!m || arr < 0 || arr >= AGX_ARR_COUNT || !host) { set_error("bad argument"); return -1; }
   const Geo &g = m->g;
@@ -1956,31 +1973,72 @@ static int copy_array(agx_mesh *m, int arr, double *host, bool to_device) {
   const int esz = m->arr.esz[arr];
   const int64_t nslab = (int64_t)g.nb * dims[0];
   const int64_t nslab_el = dims[1] * dims[2] * dims[3];
-  std::vector<float> stage;
-  char *hbase = reinterpret_cast<char *>(host);
-  if (esz == 4) {
-    stage.resize((size_t)(nslab * nslab_el));
-    if (to_device)
-      for (size_t q = 0; q < stage.size(); ++q) stage[q] = (float)host[q];
-    hbase = reinterpret_cast<char *>(stage.data());
-  }
-  char *dbase = reinterpret_cast<char *>(m->arr.a[arr]);
-  hipMemcpy3DParms p = {};
+  const int64_t slab_bytes = g.S * esz;
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nslab, (int64_t(64) << 20) / slab_bytes));
   const int nv_dev = m->arr.nvar[arr];   // device slabs per block (may exceed the host's)
-  for (int64_t s = 0; s < nslab; ++s) {
-    const int64_t sd = (s / dims[0]) * nv_dev + (s % dims[0]);
-    hipPitchedPtr hs = make_hipPitchedPtr(hbase + s * nslab_el * esz, dims[3] * esz, dims[3], dims[2]);
-    hipPitchedPtr ds = make_hipPitchedPtr(dbase + sd * g.S * esz, g.P * esz, g.P, g.J);
-    p.extent = make_hipExtent(dims[3] * esz, dims[2], dims[1]);
-    if (to_device) { p.srcPtr = hs; p.dstPtr = ds; p.kind = hipMemcpyHostToDevice; }
-    else { p.srcPtr = ds; p.dstPtr = hs; p.kind = hipMemcpyDeviceToHost; }
-    p.srcPos = make_hipPos(0, 0, 0);
-    p.dstPos = make_hipPos(0, 0, 0);
-    AGX_HIP(hipMemcpy3DAsync(&p, m->stream));
+  char *dbase = reinterpret_cast<char *>(m->arr.a[arr]);
+  char *stage = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool used[2] = {false, false};
+  hipError_t e = hipHostMalloc((void **)&stage, (size_t)(2 * chunk * slab_bytes), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+  auto pack = [&](int64_t s, char *buf) {   // host slab s -> padded slab in buf
+    std::memset(buf, 0, (size_t)slab_bytes);
+    const double *h = host + s * nslab_el;
+    for (int64_t k = 0; k < dims[1]; ++k)
+      for (int64_t j = 0; j < dims[2]; ++j) {
+        const double *row = h + (k * dims[2] + j) * dims[3];
+        const int64_t o = IDX(g, (int)k, (int)j, 0);
+        if (esz == 4) {
+          float *d = reinterpret_cast<float *>(buf) + o;
+          for (int64_t i = 0; i < dims[3]; ++i) d[i] = (float)row[i];
+        } else {
+          std::memcpy(reinterpret_cast<double *>(buf) + o, row, (size_t)dims[3] * sizeof(double));
+        }
+      }
+  };
+  auto unpack = [&](int64_t s, const char *buf) {
+    double *h = host + s * nslab_el;
+    for (int64_t k = 0; k < dims[1]; ++k)
+      for (int64_t j = 0; j < dims[2]; ++j) {
+        double *row = h + (k * dims[2] + j) * dims[3];
+        const int64_t o = IDX(g, (int)k, (int)j, 0);
+        if (esz == 4) {
+          const float *d = reinterpret_cast<const float *>(buf) + o;
+          for (int64_t i = 0; i < dims[3]; ++i) row[i] = (double)d[i];
+        } else {
+          std::memcpy(row, reinterpret_cast<const double *>(buf) + o, (size_t)dims[3] * sizeof(double));
+        }
+      }
+  };
+  int h = 0;
+  for (int64_t c0 = 0; e == hipSuccess && c0 < nslab; c0 += chunk, h ^= 1) {
+    const int64_t n = std::min(chunk, nslab - c0);
+    char *buf = stage + h * chunk * slab_bytes;
+    if (used[h]) e = hipEventSynchronize(ev[h]);
+    for (int64_t s = c0; e == hipSuccess && s < c0 + n; ++s) {
+      const int64_t sd = (s / dims[0]) * nv_dev + (s % dims[0]);
+      char *sb = buf + (s - c0) * slab_bytes;
+      if (to_device) {
+        pack(s, sb);
+        e = hipMemcpyAsync(dbase + sd * slab_bytes, sb, (size_t)slab_bytes, hipMemcpyHostToDevice, m->stream);
+      } else {
+        e = hipMemcpyAsync(sb, dbase + sd * slab_bytes, (size_t)slab_bytes, hipMemcpyDeviceToHost, m->stream);
+      }
+    }
+    if (e == hipSuccess && !to_device) {
+      e = hipStreamSynchronize(m->stream);
+      for (int64_t s = c0; e == hipSuccess && s < c0 + n; ++s) unpack(s, buf + (s - c0) * slab_bytes);
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev[h], m->stream);
+    used[h] = true;
   }
-  AGX_HIP(hipStreamSynchronize(m->stream));
-  if (esz == 4 && !to_device)
-    for (size_t q = 0; q < stage.size(); ++q) host[q] = (double)stage[q];
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (ev[0]) hipEventDestroy(ev[0]);
+  if (ev[1]) hipEventDestroy(ev[1]);
+  if (stage) hipHostFree(stage);
+  if (e != hipSuccess) return hip_fail(e, to_device ? "agx_upload" : "agx_download");
   return 0;
 }
 
@@ -2282,10 +2340,12 @@ static void launch_stage(agx_mesh *m, int stage, double wght, double *dtpart) {
   if (m->iso) {
     if (m->tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, true>), grid, dim3(256), 0, m->stream, g, a);
     else if (m->tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, true>), grid, dim3(256), 0, m->stream, g, a);
+    else if (m->tile == 3) hipLaunchKernelGGL((k_stage<64, 8, TW, true>), grid, dim3(512), 0, m->stream, g, a);
     else hipLaunchKernelGGL((k_stage<16, 16, TW, true>), grid, dim3(256), 0, m->stream, g, a);
   } else {
     if (m->tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, false>), grid, dim3(256), 0, m->stream, g, a);
     else if (m->tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, false>), grid, dim3(256), 0, m->stream, g, a);
+    else if (m->tile == 3) hipLaunchKernelGGL((k_stage<64, 8, TW, false>), grid, dim3(512), 0, m->stream, g, a);
     else hipLaunchKernelGGL((k_stage<16, 16, TW, false>), grid, dim3(256), 0, m->stream, g, a);
   }
 }

@@ -93,7 +93,7 @@ def mhd_bench(agx, pgen, args, precision):
     m.initialize()
     tile = m.autotune()
     log(f"[bench] {precision} MHD {N}^3 in {desc.nx1 // B}^3 blocks of {B}^3 set up in {time.time() - t0:.1f}s, "
-        f"device {m.device_bytes() / 1e9:.2f} GB, stage tile {('16x16', '32x8', '64x4')[tile]}")
+        f"device {m.device_bytes() / 1e9:.2f} GB, stage tile {('16x16', '32x8', '64x4', '64x8')[tile]}")
     m.cycle(1e30, args.warmup)
     m.sync()
     t0 = time.perf_counter()

@@ -170,8 +170,8 @@ int agx_mg_vcycles(agx_mesh *m, int32_t n);
 
 /* ---- tuning (no counterpart in the reference: launch shapes of this library).
  * agx_autotune times each tile shape of the fused stage kernel on the current state
- * (16x16, 32x8, 64x4 (i,j) columns per workgroup) and keeps the fastest; *choice gets
- * its index.  It writes only scratch registers (u1, b1, fluxes), so call it between
+ * (index 0..3: 16x16, 32x8, 64x4 (i,j) columns per 256-thread workgroup, 64x8 per
+ * 512-thread workgroup) and keeps the fastest; *choice gets its index.  It writes only scratch registers (u1, b1, fluxes), so call it between
  * cycles, e.g. after agx_initialize.  agx_set_tile forces a shape.  Results do not
  * depend on the shape. */
 int agx_autotune(agx_mesh *m, int32_t *choice);
