@@ -13,6 +13,7 @@
 // follows MultigridDriver step for step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -30,6 +31,14 @@ __device__ __forceinline__ int wrap(int i, int n) { return (i < 0) ? i + n : (i 
 
 __device__ __forceinline__ int64_t LIX(int n1, int n2, int k, int j, int i) {
   return ((int64_t)k * n2 + j) * n1 + i;
+}
+
+// One red-black Gauss-Seidel update (MGGravity::Smooth, gravity/mg_gravity.cpp:259-267),
+// the operand order of the reference.
+template <typename T>
+__device__ __forceinline__ T gs_value(T uc, T ukp, T ujp, T uip, T ukm, T ujm, T uim, T s, T dx2,
+                                      T isix) {
+  return uc - ((T(6.0) * uc - ukp - ujp - uip - ukm - ujm - uim) + s * dx2) * isix;
 }
 
 // MGGravity::Smooth (gravity/mg_gravity.cpp:220-273), omega = 1.15 (mg_gravity.hpp:43).
@@ -56,7 +65,7 @@ k_mg_smooth(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3
   T uip = u[LIX(n1, n2, k, j, wrapp(i, n1))];
   T ukm = u[LIX(n1, n2, wrapm(k, n3), j, i)], ujm = u[LIX(n1, n2, k, wrapm(j, n2), i)];
   T uim = u[LIX(n1, n2, k, j, wrapm(i, n1))];
-  u[id] = uc - ((T(6.0) * uc - ukp - ujp - uip - ukm - ujm - uim) + src[id] * dx2) * isix;
+  u[id] = gs_value(uc, ukp, ujp, uip, ukm, ujm, uim, src[id], dx2, isix);
 }
 
 // The same red/black sweep with one thread per (i, i+1) pair of a row (n1 even): the
@@ -94,7 +103,7 @@ k_mg_smooth_pairs(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, 
   T ukm = odd ? km.y : km.x, ujm = odd ? jm.y : jm.x;
   T uip = odd ? u[LIX(n1, n2, k, j, wrapp(i, n1))] : c.y;
   T uim = odd ? c.x : u[LIX(n1, n2, k, j, wrapm(i, n1))];
-  T v = uc - ((T(6.0) * uc - ukp - ujp - uip - ukm - ujm - uim) + src[LIX(n1, n2, k, j, i)] * dx2) * isix;
+  T v = gs_value(uc, ukp, ujp, uip, ukm, ujm, uim, src[LIX(n1, n2, k, j, i)], dx2, isix);
   if (odd) c.y = v;
   else c.x = v;
   u2[row + h] = c;
@@ -109,6 +118,23 @@ __device__ __forceinline__ T defect_at(const T *u, const T *src, int n1, int n2,
   return (T(6.0) * uc - u[LIX(n1, n2, wrapp(k, n3), j, i)] - u[LIX(n1, n2, k, wrapp(j, n2), i)] -
           u[LIX(n1, n2, k, j, wrapp(i, n1))] - u[LIX(n1, n2, wrapm(k, n3), j, i)] -
           u[LIX(n1, n2, k, wrapm(j, n2), i)] - u[LIX(n1, n2, k, j, wrapm(i, n1))]) * idx2 + src[id];
+}
+
+// 8-cell average of the fine defect over the children of coarse cell (k, j, i)
+// (Restrict, multigrid.cpp:684-724).
+template <typename T>
+__device__ __forceinline__ T restricted_defect(const T *u, const T *src, int n1, int n2, int n3,
+                                               int k, int j, int i, T idx2) {
+  int fi = 2 * i, fj = 2 * j, fk = 2 * k;
+  T d000 = defect_at(u, src, n1, n2, n3, fk, fj, fi, idx2);
+  T d001 = defect_at(u, src, n1, n2, n3, fk, fj, fi + 1, idx2);
+  T d010 = defect_at(u, src, n1, n2, n3, fk, fj + 1, fi, idx2);
+  T d011 = defect_at(u, src, n1, n2, n3, fk, fj + 1, fi + 1, idx2);
+  T d100 = defect_at(u, src, n1, n2, n3, fk + 1, fj, fi, idx2);
+  T d101 = defect_at(u, src, n1, n2, n3, fk + 1, fj, fi + 1, idx2);
+  T d110 = defect_at(u, src, n1, n2, n3, fk + 1, fj + 1, fi, idx2);
+  T d111 = defect_at(u, src, n1, n2, n3, fk + 1, fj + 1, fi + 1, idx2);
+  return T(0.125) * (d000 + d001 + d010 + d011 + d100 + d101 + d110 + d111);
 }
 
 // RestrictBlock (multigrid/multigrid.cpp:331-355): defect on the fine level, 8-cell
@@ -126,16 +152,7 @@ k_mg_defect_restrict(const T *__restrict__ u, const T *__restrict__ src,
   unsigned r = t / (unsigned)c1;
   int j = (int)(r % (unsigned)c2);
   int k = (int)(r / (unsigned)c2);
-  int fi = 2 * i, fj = 2 * j, fk = 2 * k;
-  T d000 = defect_at(u, src, n1, n2, n3, fk, fj, fi, idx2);
-  T d001 = defect_at(u, src, n1, n2, n3, fk, fj, fi + 1, idx2);
-  T d010 = defect_at(u, src, n1, n2, n3, fk, fj + 1, fi, idx2);
-  T d011 = defect_at(u, src, n1, n2, n3, fk, fj + 1, fi + 1, idx2);
-  T d100 = defect_at(u, src, n1, n2, n3, fk + 1, fj, fi, idx2);
-  T d101 = defect_at(u, src, n1, n2, n3, fk + 1, fj, fi + 1, idx2);
-  T d110 = defect_at(u, src, n1, n2, n3, fk + 1, fj + 1, fi, idx2);
-  T d111 = defect_at(u, src, n1, n2, n3, fk + 1, fj + 1, fi + 1, idx2);
-  csrc[t] = T(0.125) * (d000 + d001 + d010 + d011 + d100 + d101 + d110 + d111);
+  csrc[t] = restricted_defect(u, src, n1, n2, n3, k, j, i, idx2);
   cu[t] = T(0.0);
 }
 
@@ -160,6 +177,20 @@ k_mg_restrict_src(const T *__restrict__ src, T *__restrict__ csrc, int n1, int n
 #undef S
 }
 
+// Trilinear correction of fine cell (fk, fj, fi) from the coarse level (c1, c2, c3)
+// (ProlongateAndCorrect, multigrid.cpp:733-829).
+template <typename T>
+__device__ __forceinline__ T prolong_corr(const T *cu, int c1, int c2, int c3, int fk, int fj, int fi) {
+  int i = fi >> 1, j = fj >> 1, k = fk >> 1;
+  int ip = wrap(i + ((fi & 1) ? 1 : -1), c1);
+  int jp = wrap(j + ((fj & 1) ? 1 : -1), c2);
+  int kp = wrap(k + ((fk & 1) ? 1 : -1), c3);
+#define C(kk, jj, ii) cu[LIX(c1, c2, kk, jj, ii)]
+  return T(0.015625) * (T(27.0) * C(k, j, i) + C(kp, jp, ip) + T(9.0) * (C(k, j, ip) + C(k, jp, i) + C(kp, j, i)) +
+                        T(3.0) * (C(kp, jp, i) + C(kp, j, ip) + C(k, jp, ip)));
+#undef C
+}
+
 // ProlongateAndCorrect (multigrid/multigrid.cpp:733-829): trilinear (27,9,3,1)/64
 // correction.  One thread per fine cell; (dk,dj,di) = -1 for the low child, +1 for the
 // high child along each axis.
@@ -172,15 +203,7 @@ k_mg_prolong(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3
   unsigned r = t / (unsigned)n1;
   int fj = (int)(r % (unsigned)n2);
   int fk = (int)(r / (unsigned)n2);
-  int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
-  int i = fi >> 1, j = fj >> 1, k = fk >> 1;
-  int ip = wrap(i + ((fi & 1) ? 1 : -1), c1);
-  int jp = wrap(j + ((fj & 1) ? 1 : -1), c2);
-  int kp = wrap(k + ((fk & 1) ? 1 : -1), c3);
-#define C(kk, jj, ii) cu[LIX(c1, c2, kk, jj, ii)]
-  u[t] += T(0.015625) * (T(27.0) * C(k, j, i) + C(kp, jp, ip) + T(9.0) * (C(k, j, ip) + C(k, jp, i) + C(kp, j, i)) +
-                         T(3.0) * (C(kp, jp, i) + C(kp, j, ip) + C(k, jp, ip)));
-#undef C
+  u[t] += prolong_corr(cu, n1 >> 1, n2 >> 1, n3 >> 1, fk, fj, fi);
 }
 
 // FMGProlongate (multigrid/multigrid.cpp:838-1048): 27-point interpolation whose
@@ -219,25 +242,106 @@ k_mg_fmg_prolong(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, in
   u[t] = acc / T(32768.0);
 }
 
-// block-partial sums in FP64 (fixed order -> deterministic): mode 0 sum(a), mode 1
-// sum(def^2)
+// The coarse end of a V(1,1) cycle in one workgroup: from level `top` (at most
+// COARSE_CELLS cells) down to the 1-cell level 0 (red sweep, black sweep, defect +
+// restriction per level), level 0 zeroed (SolveCoarsestGrid of a 1-cell periodic root,
+// multigrid_driver.cpp:781-787), and back up (prolongation + correction, red, black).
+// Each step is the per-cell arithmetic of k_mg_smooth, k_mg_defect_restrict and
+// k_mg_prolong, separated by workgroup barriers (which order global-memory accesses
+// within the workgroup), so the result equals the per-level launches; it replaces about
+// six launches per level.
+constexpr int COARSE_CELLS = 4096, COARSE_BS = 1024, MAXLEV = 24;
+template <typename T>
+struct CoarseLevels {
+  T *u[MAXLEV];
+  T *src[MAXLEV];
+  int n1[MAXLEV], n2[MAXLEV], n3[MAXLEV];
+  T dx2[MAXLEV], idx2[MAXLEV];
+};
+
+template <typename T>
+__device__ __forceinline__ void coarse_smooth(const CoarseLevels<T> &L, int l, int color, T isix) {
+  const int n1 = L.n1[l], n2 = L.n2[l], n3 = L.n3[l];
+  const int nh = (n1 + 1) / 2;
+  T *u = L.u[l];
+  const T *src = L.src[l];
+  for (int t = threadIdx.x; t < nh * n2 * n3; t += COARSE_BS) {
+    int h = t % nh, r = t / nh;
+    int j = r % n2, k = r / n2;
+    int i = 2 * h + ((color + k + j) & 1);
+    if (i >= n1) continue;
+    int64_t id = LIX(n1, n2, k, j, i);
+    T uc = u[id];
+    T ukp = u[LIX(n1, n2, wrapp(k, n3), j, i)], ujp = u[LIX(n1, n2, k, wrapp(j, n2), i)];
+    T uip = u[LIX(n1, n2, k, j, wrapp(i, n1))];
+    T ukm = u[LIX(n1, n2, wrapm(k, n3), j, i)], ujm = u[LIX(n1, n2, k, wrapm(j, n2), i)];
+    T uim = u[LIX(n1, n2, k, j, wrapm(i, n1))];
+    u[id] = gs_value(uc, ukp, ujp, uip, ukm, ujm, uim, src[id], L.dx2[l], isix);
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(COARSE_BS) k_mg_coarse(CoarseLevels<T> L, int top, int coffset, T isix) {
+  for (int l = top; l > 0; --l) {
+    coarse_smooth(L, l, 0 ^ coffset, isix);
+    coarse_smooth(L, l, 1 ^ coffset, isix);
+    const int c1 = L.n1[l - 1], c2 = L.n2[l - 1], c3 = L.n3[l - 1];
+    for (int t = threadIdx.x; t < c1 * c2 * c3; t += COARSE_BS) {
+      int i = t % c1, r = t / c1;
+      int j = r % c2, k = r / c2;
+      L.src[l - 1][t] = restricted_defect<T>(L.u[l], L.src[l], L.n1[l], L.n2[l], L.n3[l], k, j, i, L.idx2[l]);
+      L.u[l - 1][t] = T(0.0);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) L.u[0][0] = T(0.0);
+  __syncthreads();
+  for (int l = 1; l <= top; ++l) {
+    const int n1 = L.n1[l], n2 = L.n2[l], n3 = L.n3[l];
+    for (int t = threadIdx.x; t < n1 * n2 * n3; t += COARSE_BS) {
+      int fi = t % n1, r = t / n1;
+      int fj = r % n2, fk = r / n2;
+      L.u[l][t] += prolong_corr<T>(L.u[l - 1], L.n1[l - 1], L.n2[l - 1], L.n3[l - 1], fk, fj, fi);
+    }
+    __syncthreads();
+    coarse_smooth(L, l, 0 ^ coffset, isix);
+    coarse_smooth(L, l, 1 ^ coffset, isix);
+  }
+}
+
+// Reductions: a launch of at most NPART workgroups, workgroup w summing the contiguous
+// range [t0, t1) of the level (XCD-aware order); the partials are summed by one
+// workgroup (k_mg_final).  The order is fixed, so results are deterministic.
+constexpr int NPART = 2048;
+__device__ __forceinline__ void wg_range(unsigned n, unsigned &t0, unsigned &t1) {
+  const unsigned w = xcd_slot(blockIdx.x, gridDim.x);
+  const unsigned per = ((n + gridDim.x - 1) / gridDim.x + BS - 1) / BS * BS;
+  t0 = w * per;
+  t1 = (t0 + per < n) ? t0 + per : n;
+  if (t0 > n) t0 = n;
+}
+
+// block-partial sums in FP64: mode 0 sum(a), mode 1 sum(def^2)
 template <int MODE, typename T>
 __global__ void __launch_bounds__(BS)
 k_mg_partial(const T *__restrict__ a, const T *__restrict__ src, int n1, int n2, int n3,
              T idx2, double *__restrict__ part) {
   __shared__ double sh[BS];
-  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned ncell = (unsigned)n1 * n2 * n3;   // < 2^32 cells per level
+  unsigned t0, t1;
+  wg_range(ncell, t0, t1);
   double v = 0.0;
-  if (t < (int64_t)n1 * n2 * n3) {
+  for (unsigned t = t0 + threadIdx.x; t < t1; t += BS) {
     if (MODE == 0) {
-      v = a[t];
+      v += a[t];
     } else {
       int i = (int)(t % (unsigned)n1);
       unsigned r = t / (unsigned)n1;
       int j = (int)(r % (unsigned)n2);
       int k = (int)(r / (unsigned)n2);
       double d = defect_at(a, src, n1, n2, n3, k, j, i, idx2);
-      v = d * d;
+      v += d * d;
     }
   }
   sh[threadIdx.x] = v;
@@ -253,6 +357,7 @@ __global__ void __launch_bounds__(BS) k_mg_final(const double *__restrict__ part
                                                  double *__restrict__ out) {
   __shared__ double sh[BS];
   double v = 0.0;
+#pragma unroll 8
   for (int64_t q = threadIdx.x; q < n; q += BS) v += part[q];
   sh[threadIdx.x] = v;
   __syncthreads();
@@ -283,16 +388,18 @@ __global__ void __launch_bounds__(BS)
 k_mg_defect_to32(const double *__restrict__ u, const double *__restrict__ src, float *__restrict__ d32,
                  int n1, int n2, int n3, double idx2, double *__restrict__ part) {
   __shared__ double sh[BS];
-  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
+  const unsigned ncell = (unsigned)n1 * n2 * n3;   // < 2^32 cells per level
+  unsigned t0, t1;
+  wg_range(ncell, t0, t1);
   double v = 0.0;
-  if (t < (int64_t)n1 * n2 * n3) {
+  for (unsigned t = t0 + threadIdx.x; t < t1; t += BS) {
     int i = (int)(t % (unsigned)n1);
     unsigned r = t / (unsigned)n1;
     int j = (int)(r % (unsigned)n2);
     int k = (int)(r / (unsigned)n2);
     double d = defect_at(u, src, n1, n2, n3, k, j, i, idx2);
     d32[t] = (float)d;
-    v = d * d;
+    v += d * d;
   }
   sh[threadIdx.x] = v;
   __syncthreads();
@@ -359,6 +466,7 @@ namespace agx {
 template <typename T>
 static int64_t lcells(const MGLevelT<T> &L) { return (int64_t)L.n1 * L.n2 * L.n3; }
 static int64_t pad32(int64_t n) { return (n + 63) / 64 * 64; }   // 64 elements: >= 256 B
+static int64_t npart(int64_t n) { return std::min<int64_t>(grid1d(n, BS), NPART); }
 
 int mg_create(agx_mesh *m) {
   const agx_mesh_desc &d = m->d;
@@ -509,7 +617,7 @@ static void fmg_prolong(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {   // 
 template <typename T>
 static void level_sum(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l, int mode, int slot) {
   MGLevelT<T> &L = H[l];
-  int64_t nb = grid1d(lcells(L), BS);
+  int64_t nb = npart(lcells(L));
   double idx2 = 1.0 / (L.dx * L.dx);
   timer_begin(m, AGX_K_MG_OTHER);
   if (mode == 0)
@@ -524,7 +632,7 @@ static void level_sum(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l, int mode,
 
 static void level_sum_src(agx_mesh *m, int l, int slot) {
   MGLevel &L = m->lev[l];
-  int64_t nb = grid1d(lcells(L), BS);
+  int64_t nb = npart(lcells(L));
   hipLaunchKernelGGL((k_mg_partial<0, double>), dim3((unsigned)nb), dim3(BS), 0, m->stream, L.src, L.src,
                      L.n1, L.n2, L.n3, 0.0, m->red);
   hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->red, nb, m->scal + slot);
@@ -588,18 +696,46 @@ static void solve_coarsest(agx_mesh *m, std::vector<MGLevelT<T>> &H) {
 // OneStepToFiner (:583-622).  On the uniform periodic hierarchy every level step is:
 // coarser = red sweep, black sweep, defect+restrict; finer = prolongate+correct, red,
 // black.
+// Highest level handled by k_mg_coarse, or -1 if the root grid is not a single cell.
+template <typename T>
+static int coarse_top(agx_mesh *m, const std::vector<MGLevelT<T>> &H) {
+  const MGLevelT<T> &L0 = H[0];
+  if (L0.n1 != 1 || L0.n2 != 1 || L0.n3 != 1 || (int)H.size() > MAXLEV) return -1;
+  int top = 0;
+  while (top + 1 < (int)H.size() && lcells(H[top + 1]) <= COARSE_CELLS) ++top;
+  return top;
+}
+
+template <typename T>
+static void coarse_cycle(agx_mesh *m, std::vector<MGLevelT<T>> &H, int top) {
+  CoarseLevels<T> c;
+  for (int l = 0; l <= top; ++l) {
+    c.u[l] = H[l].u;
+    c.src[l] = H[l].src;
+    c.n1[l] = H[l].n1; c.n2[l] = H[l].n2; c.n3[l] = H[l].n3;
+    c.dx2[l] = T(H[l].dx * H[l].dx);
+    c.idx2[l] = T(1.0 / (H[l].dx * H[l].dx));
+  }
+  timer_begin(m, AGX_K_MG_SMOOTH);
+  hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(COARSE_BS), 0, m->stream, c, top, m->coffset,
+                     T(1.15 / 6.0));
+  timer_end(m);
+}
+
 template <typename T>
 static void vcycle(agx_mesh *m, std::vector<MGLevelT<T>> &H) {
   int start = m->current;
   m->coffset ^= 1;
-  while (m->current > 0) {
+  const int ct = coarse_top(m, H);
+  while (m->current > 0 && m->current > ct) {
     int l = m->current;
     smooth(m, H, l, 0);
     smooth(m, H, l, 1);
     defect_restrict(m, H, l);
     m->current--;
   }
-  solve_coarsest(m, H);
+  if (ct >= 0) coarse_cycle(m, H, m->current);   // down to level 0 and back to current
+  else solve_coarsest(m, H);
   while (m->current < start) {
     int l = m->current + 1;
     prolong(m, H, l);
@@ -651,7 +787,7 @@ static void defect_to32(agx_mesh *m) {
   int l = m->ntotal - 1;
   MGLevel &F = m->lev[l];
   MGLevel32 &F32 = m->lev32[l];
-  int64_t nb = grid1d(lcells(F), BS);
+  int64_t nb = npart(lcells(F));
   timer_begin(m, AGX_K_MG_OTHER);
   hipLaunchKernelGGL(k_mg_defect_to32, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.u, F.src, F32.src, F.n1,
                      F.n2, F.n3, 1.0 / (F.dx * F.dx), m->red);

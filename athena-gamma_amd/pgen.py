@@ -52,6 +52,36 @@ def turbulence(desc, seed=2, mach=2.0, bx=0.1, p0=None):
     return U, b1, b2, b3
 
 
+def turbulence_lognormal(desc, seed=5, mach=5.0, sigma_lnrho=1.0, beta=1.0):
+    """Large-run start: lognormal density rho = exp(sigma g) / mean (g a unit-rms k^-4
+    field on 2 <= |k| <= 8, so <rho> = 1 and the rms of ln rho is sigma), isothermal-like
+    P = 0.6 rho / gamma (uniform sound speed sqrt(0.6)), velocity as in turbulence() at
+    rms Mach `mach`, uniform Bx with plasma beta 2 <P> / Bx^2 = `beta`."""
+    gamma = desc.gamma
+    shape = (desc.nx3, desc.nx2, desc.nx1)
+    rho = np.exp(sigma_lnrho * gaussian_field(shape, seed * 1000 + 7))
+    rho /= rho.mean()
+    pbar = 0.6 / gamma
+    cs = np.sqrt(0.6)
+    bx = np.sqrt(2.0 * pbar / beta)
+    U = np.empty((5,) + shape)
+    U[0] = rho
+    U[4] = pbar * rho / (gamma - 1.0) + 0.5 * bx * bx
+    v2 = np.zeros(shape)
+    v = [gaussian_field(shape, seed * 1000 + c) for c in range(3)]
+    vrms = np.sqrt(np.mean(v[0] ** 2 + v[1] ** 2 + v[2] ** 2))
+    for c in range(3):
+        vc = v[c] * (mach * cs / vrms)
+        U[1 + c] = rho * vc
+        v2 += vc * vc
+    del v
+    U[4] += 0.5 * rho * v2
+    b1 = np.full(shape, bx)
+    b2 = np.zeros(shape)
+    b3 = np.zeros(shape)
+    return U, b1, b2, b3
+
+
 def blobs(desc, seed=3, nblobs=32):
     """Self-gravity benchmark density: 1 + sum of Gaussian blobs (widths 0.02-0.08,
     log-uniform amplitudes 1-100, periodic minimum-image distance) on the unit box."""

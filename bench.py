@@ -176,6 +176,90 @@ def mg_bench(agx, pgen, args, precision="fp64"):
     return vps, iters, norm, t_solve, per
 
 
+def large_bench(agx, pgen, args):
+    """The large single-GPU run: 512^3 periodic MHD + multigrid self-gravity in 512
+    MeshBlocks of 64^3, VL2+PPM+HLLD+CT, lognormal density (sigma_lnrho = 1), rms Mach 5,
+    beta = 1, 4 pi G = 1.  The mixed form (FP32 reconstruction/Riemann and FP32 V-cycles in
+    FP64 defect correction) is timed over `steps` cycles and compared with the plain FP64
+    path run from the same state for the same cycles."""
+    N, B = args.large_n, 64
+    base = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4, gamma=5.0 / 3.0,
+                         xorder=3, nghost=3, rsolver=0)
+    t0 = time.time()
+    U, b1, b2, b3 = pgen.turbulence_lognormal(base, seed=5, mach=5.0, sigma_lnrho=1.0, beta=1.0)
+    src_norm = float(np.sqrt(np.mean((U[0] - U[0].mean()) ** 2)))
+    ub = agx.scatter_cells(base, U)
+    del U
+    faces = pgen.scatter_faces(base, b1, b2, b3)
+    del b1, b2, b3
+    log(f"[large] {N}^3 lognormal turbulence built in {time.time() - t0:.1f}s")
+    res = {}
+    for prec in ("fp64", "mixed"):
+        desc = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4,
+                             gamma=5.0 / 3.0, xorder=3, nghost=3, rsolver=0, self_gravity=1,
+                             four_pi_G=1.0, mg_mode=0, mg_threshold=1e-10 * src_norm,
+                             precision=1 if prec == "mixed" else 0)
+        m = agx.Mesh(desc)
+        m.set("u", ub)
+        for n, f in zip(("b1f", "b2f", "b3f"), faces):
+            m.set(n, f)
+        m.initialize()
+        tile = m.autotune()
+        m.calculate_fluxes(3)
+        m.sync()
+        t0 = time.perf_counter()
+        done = m.cycle(1e30, args.steps)
+        m.sync()
+        sec = time.perf_counter() - t0
+        assert done == args.steps
+        ng = desc.nghost
+        sl = (slice(None), slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
+        w = m.get("w")[sl][:, [0, 4]].astype(np.float64)
+        bcc = m.get("bcc")[sl].astype(np.float64)
+        m.timing(True)
+        m.timing_reset()
+        m.cycle(1e30, 2)
+        m.sync()
+        per = {c: m.timing_get(c) for c in KCLASSES + ("mg_smooth", "mg_restrict", "mg_prolong", "mg_other")}
+        m.timing(False)
+        res[prec] = dict(zc=float(N) ** 3 * args.steps / sec, ms=sec * 1e3 / args.steps, w=w, bcc=bcc,
+                         per=per, tile=tile, bytes=m.device_bytes(), rf=roofline(desc, per, prec))
+        log(f"[large] {prec}: {args.steps} cycles in {sec:.2f}s -> {res[prec]['zc']:.4e} zone-cycles/s, "
+            f"device {m.device_bytes() / 1e9:.1f} GB, stage tile {tile}; per cycle: "
+            + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in per.items() if v[0] > 0))
+        m.close()
+    a, b = res["mixed"], res["fp64"]
+    err = lambda x, y: float(np.abs(x - y).max() / np.abs(y).max())
+    l1 = lambda x, y: float(np.abs(x - y).mean() / np.abs(y).mean())
+    out = {
+        "metric": "MHD+MG zone-cycles/s at 512^3 on 1 MI355X, mixed precision; % of HBM roofline",
+        "value": a["zc"], "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": a["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 reconstruction+Riemann and MG smoothing, f64 update/C2P/CT/defect",
+        "data": "synthetic",
+        "config": {"workload": f"{N}^3 self-gravitating MHD turbulence (lognormal rho, sigma_lnrho=1, rms Mach 5, "
+                               f"beta=1, 4piG=1), {(N // B) ** 3} MeshBlocks of {B}^3, VL2+PPM+HLLD+CT, "
+                               f"FMG + V-cycles to 1e-10 x rms(rho - <rho>) each stage",
+                   "mesh": [N] * 3, "meshblock": [B] * 3, "nghost": 3},
+        "roofline": a["rf"],
+        "fp64": {"value": b["zc"], "ms_per_step": b["ms"], "roofline": b["rf"]},
+        "vs_fp64": a["zc"] / b["zc"],
+        "rel_err_rho": err(a["w"][:, 0], b["w"][:, 0]),
+        "rel_err_p": err(a["w"][:, 1], b["w"][:, 1]),
+        "rel_err_bcc": err(a["bcc"], b["bcc"]),
+        "rel_l1_rho": l1(a["w"][:, 0], b["w"][:, 0]),
+        "rel_l1_p": l1(a["w"][:, 1], b["w"][:, 1]),
+        "rel_l1_bcc": l1(a["bcc"], b["bcc"]),
+        "error_note": "max-norm (rel_err_*) and mean (rel_l1_*) differences relative to the FP64 path's "
+                      "max and mean magnitude after the same cycles",
+        "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in a["per"].items()},
+    }
+    log(f"[large] mixed vs fp64 after {args.steps} cycles: max rho {out['rel_err_rho']:.3e}, "
+        f"P {out['rel_err_p']:.3e}, bcc {out['rel_err_bcc']:.3e}; mean rho {out['rel_l1_rho']:.3e}, "
+        f"P {out['rel_l1_p']:.3e}, bcc {out['rel_l1_bcc']:.3e}")
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(args):
     """The CPU oracle (FP64 restatement, OpenMP over MeshBlocks) on a bounded sample of
     the same workload: a 64^3 periodic sub-box of the turbulence problem in 16
@@ -223,12 +307,18 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--compare", action="store_true",
                     help="download w after both runs and report the mixed-vs-FP64 difference")
+    ap.add_argument("--large", action="store_true",
+                    help="run the 512^3 PPM + self-gravity mixed-precision case instead")
+    ap.add_argument("--large-n", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per run); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     agx, pgen = load_pkg()
+    if args.large:
+        large_bench(agx, pgen, args)
+        return
     desc, zc, ms_step, per, st64 = mhd_bench(agx, pgen, args, "fp64")
     rf = roofline(desc, per, "fp64")
     log("[bench] fp64 per-cycle kernel time: " + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in per.items())

@@ -34,15 +34,16 @@ def interior(a, ng):
     return a[:, :, ng:-ng, ng:-ng, ng:-ng]
 
 
-@pytest.mark.parametrize("rsolver", [0, 1])
-def test_mixed_fluxes(agx, rsolver):
-    desc = make_desc(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, rsolver=rsolver)
+@pytest.mark.parametrize("rsolver,order", [(0, 2), (1, 2), (0, 3)])
+def test_mixed_fluxes(agx, rsolver, order):
+    desc = make_desc(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, rsolver=rsolver,
+                     xorder=order, nghost=3 if order == 3 else 2)
     o, m = mixed_pair(agx, desc, turbulent_state(desc, seed=11, mach=1.5))
     o.initialize()
     m.initialize()
     assert abs(m.time()[1] - o.time()[1]) <= 1e-5 * o.time()[1]
-    o.L.orc_calculate_fluxes(o.h, 2)
-    m.calculate_fluxes(2)
+    o.L.orc_calculate_fluxes(o.h, order)
+    m.calculate_fluxes(order)
     for n in ("fx1", "fx2", "fx3", "e3x1", "e2x1", "e1x2", "e3x2", "e1x3", "e2x3", "wx1", "wx2", "wx3"):
         e = rel_err(m.get(n), o.get(n))
         print(f"{n}: max rel err {e:.3e}", flush=True)
@@ -52,11 +53,14 @@ def test_mixed_fluxes(agx, rsolver):
         assert e <= (1e-4 if n.startswith("wx") else 1e-5), n
 
 
-@pytest.mark.parametrize("rsolver", [0, 1])
-@pytest.mark.parametrize("shape", ["cube", "ragged"])
+@pytest.mark.parametrize("rsolver,shape", [(0, "cube"), (1, "cube"), (0, "ragged"), (1, "ragged"),
+                                            (0, "cube_ppm")])
 def test_mixed_turbulence_run(agx, rsolver, shape):
     if shape == "cube":
         desc = make_desc(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, rsolver=rsolver)
+    elif shape == "cube_ppm":
+        desc = make_desc(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, rsolver=rsolver,
+                         xorder=3, nghost=3)
     else:
         desc = make_desc(nx=(24, 16, 8), bnx=(8, 4, 8), box=((0, 1.5), (0, 1), (0, 0.5)), rsolver=rsolver)
     o, m = mixed_pair(agx, desc, turbulent_state(desc, seed=11, mach=1.5))
@@ -73,12 +77,26 @@ def test_mixed_turbulence_run(agx, rsolver, shape):
     ep = rel_err(wm[:, 4], wo[:, 4])
     um, uo = interior(m.get("u"), ng), interior(o.get("u"), ng)
     print(f"rho {erho:.3e}  P {ep:.3e}  E {rel_err(um[:, 4], uo[:, 4]):.3e}", flush=True)
-    assert erho <= RHO_P_TOL and ep <= RHO_P_TOL
+    if shape == "cube_ppm":
+        # PPM's extremum limiter (ppm.cpp:190-250) branches on signs of second differences
+        # and on rho <= 1 - 1e-12; at a few cells those tests flip on the FP32 rounding of
+        # w, and the max-norm error there is set by the branch jump, not by rounding.
+        # Bars: mean error at FP32 rounding (1e-6), max error 1e-4.
+        l1 = lambda a, b: np.abs(a - b).mean() / np.abs(b).mean()
+        print(f"L1: rho {l1(wm[:, 0], wo[:, 0]):.3e}  P {l1(wm[:, 4], wo[:, 4]):.3e}", flush=True)
+        assert l1(wm[:, 0], wo[:, 0]) <= 1e-6 and l1(wm[:, 4], wo[:, 4]) <= 1e-6
+        assert erho <= 1e-4 and ep <= 1e-4
+    else:
+        assert erho <= RHO_P_TOL and ep <= RHO_P_TOL
     for n in ("b1f", "b2f", "b3f"):
         sl = (slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
-        e = rel_err(m.get(n)[sl], o.get(n)[sl])
+        a, b = m.get(n)[sl], o.get(n)[sl]
+        e = rel_err(a, b)
         print(f"{n}: {e:.3e}", flush=True)
-        assert e <= RHO_P_TOL
+        if shape == "cube_ppm":   # the same limiter branch flips as above, B ~ 0.1
+            assert np.abs(a - b).mean() / np.abs(b).mean() <= 1e-6 and e <= 1e-3
+        else:
+            assert e <= RHO_P_TOL
     o2 = Oracle(desc)
     for n in ("b1f", "b2f", "b3f"):
         o2.set(n, m.get(n))
