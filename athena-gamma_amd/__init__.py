@@ -31,7 +31,7 @@ EXPORTS = [
     "agx_synchronize", "agx_calculate_fluxes", "agx_compute_corner_e", "agx_integrate",
     "agx_bvals_exchange", "agx_cons2prim", "agx_new_block_dt", "agx_new_time_step",
     "agx_initialize", "agx_cycle", "agx_mg_solve", "agx_mg_set_mode", "agx_mg_last",
-    "agx_mg_vcycles", "agx_autotune", "agx_set_tile", "agx_timing_enable", "agx_timing_reset",
+    "agx_mg_vcycles", "agx_autotune", "agx_set_tile", "agx_set_graph", "agx_timing_enable", "agx_timing_reset",
     "agx_timing_get",
 ]
 
@@ -104,6 +104,7 @@ def lib():
     L.agx_mg_vcycles.argtypes = [P, ctypes.c_int32]
     L.agx_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
     L.agx_set_tile.argtypes = [P, ctypes.c_int32]
+    L.agx_set_graph.argtypes = [P, ctypes.c_int32]
     L.agx_timing_enable.argtypes = [P, I]
     L.agx_timing_reset.argtypes = [P]
     L.agx_timing_get.argtypes = [P, I, D, ctypes.POINTER(ctypes.c_int64)]
@@ -236,6 +237,10 @@ class Mesh:
         c = ctypes.c_int32()
         _check(self.L.agx_autotune(self.h, ctypes.byref(c)), "agx_autotune")
         return c.value
+
+    def set_graph(self, enable=True):
+        """Replay agx_cycle as one hipGraph launch per cycle (no self-gravity)."""
+        _check(self.L.agx_set_graph(self.h, 1 if enable else 0), "agx_set_graph")
 
     def set_tile(self, tile):
         _check(self.L.agx_set_tile(self.h, tile), "agx_set_tile")

@@ -62,6 +62,11 @@ __device__ __forceinline__ uint3 xcd_block() {
   return make_uint3(n - yz * gx, yz % gy, yz / gy);
 }
 
+// Device scalar slots (agx_mesh::scal, 64 doubles): 0-3 reduction results; the time
+// step, time and tlim of the fused cycle, which the flux and stage kernels read from
+// device memory so that a captured cycle replays with the current step.
+constexpr int SC_DT = 40, SC_TIME = 41, SC_TLIM = 42;
+
 // Per-block constant geometry on the device (uniform Cartesian mesh).
 struct BlockCoef {
   double dx[3];            // dx?f (== dx?v) of the block, coordinates.cpp:127-145
@@ -142,6 +147,15 @@ struct agx_mesh {
   double rdx_root = 0.0, rdy_root = 0.0, rdz_root = 0.0;
   double rdx_blk = 0.0, rdy_blk = 0.0, rdz_blk = 0.0;
   agx::Timer timer;
+  // hipGraph replay of agx_cycle: one executable graph per register assignment (the face
+  // registers rotate with period two cycles), keyed by the pointer table it starts from
+  bool graph_on = true;
+  struct CycleGraph {
+    hipGraphExec_t exec = nullptr;
+    double *before[agx::ARR_TOTAL];
+    double *after[agx::ARR_TOTAL];
+  };
+  std::vector<CycleGraph> graphs;
 };
 
 namespace agx {

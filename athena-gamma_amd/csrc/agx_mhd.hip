@@ -867,7 +867,7 @@ __global__ void __launch_bounds__(256)
 k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
        const double *__restrict__ bn, T *__restrict__ flx, T *__restrict__ ey,
        T *__restrict__ ez, T *__restrict__ wct, const BlockCoef *__restrict__ coef,
-       const double *__restrict__ plm, double gamma_, double cs_, double dt, double dfl_,
+       const double *__restrict__ plm, double gamma_, double cs_, const double *__restrict__ dtp, double dfl_,
        double pfl_) {
   int il = g.is - (DIR != 0), iu = g.ie + 1;
   int jl = g.js - (DIR != 1), ju = g.je + 1;
@@ -953,7 +953,7 @@ k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
   int64_t fo = (int64_t)p.b * g.S + id;
   ey[fo] = -flxi[IBY];
   ez[fo] = flxi[IBZ];
-  wct[fo] = ct_weight(flxi[IDN], wli[IDN], wri[IDN], T(coef[p.b].dx[DIR]), T(dt));
+  wct[fo] = ct_weight(flxi[IDN], wli[IDN], wri[IDN], T(coef[p.b].dx[DIR]), T(*dtp));
 }
 
 // Field::ComputeCornerE, 3D Newtonian branch (field/calculate_corner_e.cpp:130-229).
@@ -1337,7 +1337,9 @@ struct StageArgs {
   const double *s1, *s2, *s3;        // CT source faces
   double *d1, *d2, *d3;              // CT destination faces (interior)
   const BlockCoef *coef;
-  double wght, gm1, dfl, pfl, gamma, cs;
+  const double *dtp;                 // device time step (scal[SC_DT])
+  double wfac;                       // stage weight: wght = wfac * dt
+  double gm1, dfl, pfl, gamma, cs;
   int grav;
   double *dtpart;                    // per-workgroup min of the dt bound, or null
 };
@@ -1368,7 +1370,7 @@ k_stage(Geo g, StageArgs<TW> a) {
   const int64_t cbase = (int64_t)b * NHYDRO * S, bbase = (int64_t)b * 3 * S;
   const double dx[3] = {a.coef[b].dx[0], a.coef[b].dx[1], a.coef[b].dx[2]};
   const double dx1 = dx[0], dx2 = dx[1], dx3 = dx[2];
-  const double wght = a.wght;
+  const double wght = a.wfac * *a.dtp;
   const int wi = ni + 2, ncce = wi * (nj + 2);
   const int np = (nj + 1) * (ni + 1);
   // cell-centred E = -v x B at plane c, cells (j0-1..j0+nj) x (i0-1..i0+ni), into slot
@@ -1905,9 +1907,12 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   return 0;
 }
 
+static void graph_reset(agx_mesh *m);
+
 int agx_mesh_destroy(agx_mesh *m) {
   if (!m) return 0;
   if (m->stream) hipStreamSynchronize(m->stream);
+  graph_reset(m);
   for (auto &ev : m->timer.ev) hipEventDestroy(ev);
   mg_destroy(m);
   if (m->pool) hipFree(m->pool);
@@ -2071,13 +2076,52 @@ int agx_synchronize(agx_mesh *m) {
     if (e_ != hipSuccess) return agx::hip_fail(e_, what);      \
   } while (0)
 
+// Writes the host's dt, time and tlim into the device scalars read by the flux and stage
+// kernels and by k_advance.
+__global__ void k_set_scalars(double *__restrict__ s, double dt, double time, double tlim) {
+  if (threadIdx.x == 0) {
+    s[SC_DT] = dt;
+    s[SC_TIME] = time;
+    s[SC_TLIM] = tlim;
+  }
+}
+
+// End of a cycle on the device: time += dt, then Mesh::NewTimeStep (mesh.cpp:1078-1119)
+// from the block minimum in s[0] (k_min_final): dt = min(cfl * min, 2 dt), clipped to
+// tlim - time.  The same operations as agx_cycle's host path.
+__global__ void k_advance(double *__restrict__ s, double cfl) {
+  if (threadIdx.x == 0) {
+    const double dt = s[SC_DT], tlim = s[SC_TLIM];
+    const double time = s[SC_TIME] + dt;
+    const double bdt = s[0] * cfl;
+    double ndt = 2.0 * dt;
+    ndt = (bdt < ndt) ? bdt : ndt;
+    if (time < tlim && (tlim - time) < ndt) ndt = tlim - time;
+    s[SC_TIME] = time;
+    s[SC_DT] = ndt;
+  }
+}
+
+static int push_scalars(agx_mesh *m, double tlim) {
+  hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, m->stream, m->scal, m->dt, m->time, tlim);
+  LAUNCH_CHECK("k_set_scalars");
+  return 0;
+}
+
+static int calculate_fluxes(agx_mesh *m, int order);
+
 int agx_calculate_fluxes(agx_mesh *m, int order) {
   if (!m) { set_error("null mesh"); return -1; }
+  int rc = push_scalars(m, DBL_MAX);
+  return rc ? rc : calculate_fluxes(m, order);
+}
+
+static int calculate_fluxes(agx_mesh *m, int order) {
   if (order < 1 || order > 3) { set_error("order %d not supported", order); return -2; }
   if (order == 3 && m->g.ng < 3) { set_error("PPM needs nghost >= 3 (reconstruction.cpp:90-100)"); return -3; }
   const Geo &g = m->g;
   double **A = m->arr.a;
-  const double gamma = m->d.gamma, dt = m->dt;
+  const double gamma = m->d.gamma;
   const int rs = m->d.rsolver;
 #define FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, TW, ISO)                                 \
   hipLaunchKernelGGL((k_flux<DIR, ORD, RS, TW, ISO>),                                       \
@@ -2085,7 +2129,7 @@ int agx_calculate_fluxes(agx_mesh *m, int order) {
                                 g.ks - (DIR != 2), g.ke + 1, g.nb),                          \
                      dim3(256), 0, m->stream, g, as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]), \
                      A[B], as<TW>(A[F]), as<TW>(A[EY]), as<TW>(A[EZ]), as<TW>(A[WX]), m->coef, \
-                     m->plm, gamma, m->d.iso_cs, dt, m->d.dfloor, m->d.pfloor)
+                     m->plm, gamma, m->d.iso_cs, m->scal + SC_DT, m->d.dfloor, m->d.pfloor)
 #define FLUX_LAUNCH_E(DIR, ORD, RS, B, F, EY, EZ, WX, ISO)                                     \
   if (m->mixed) FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, float, ISO);                    \
   else FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, double, ISO)
@@ -2308,7 +2352,7 @@ int agx_initialize(agx_mesh *m, double tlim) {
 // rotation leaves b = new faces and b1 = half-step faces, as the reference's swap does.
 extern "C++" {
 template <typename TW>
-static void launch_stage(agx_mesh *m, int stage, double wght, double *dtpart) {
+static void launch_stage(agx_mesh *m, int stage, double wfac, double *dtpart) {
   const Geo &g = m->g;
   double **A = m->arr.a;
   StageArgs<TW> a;
@@ -2328,7 +2372,8 @@ static void launch_stage(agx_mesh *m, int stage, double wght, double *dtpart) {
   a.s1 = A[src]; a.s2 = A[src + 1]; a.s3 = A[src + 2];
   a.d1 = A[dst]; a.d2 = A[dst + 1]; a.d3 = A[dst + 2];
   a.coef = m->coef;
-  a.wght = wght;
+  a.dtp = m->scal + SC_DT;
+  a.wfac = wfac;
   a.gm1 = m->d.gamma - 1.0;
   a.dfl = m->d.dfloor;
   a.pfl = m->d.pfloor;
@@ -2372,12 +2417,12 @@ static void launch_ghost_c2p(agx_mesh *m) {
 
 static int stage_fused(agx_mesh *m, int stage, bool want_dt) {
   double **A = m->arr.a;
-  int rc = agx_calculate_fluxes(m, stage == 1 ? 1 : m->d.xorder);
+  int rc = calculate_fluxes(m, stage == 1 ? 1 : m->d.xorder);
   if (rc) return rc;
-  const double wght = (stage == 1 ? 0.5 : 1.0) * m->dt;
+  const double wfac = (stage == 1 ? 0.5 : 1.0);   // wght = wfac * dt on the device
   timer_begin(m, AGX_K_INTEGRATE);
-  if (m->mixed) launch_stage<float>(m, stage, wght, want_dt ? m->dtpart : nullptr);
-  else launch_stage<double>(m, stage, wght, want_dt ? m->dtpart : nullptr);
+  if (m->mixed) launch_stage<float>(m, stage, wfac, want_dt ? m->dtpart : nullptr);
+  else launch_stage<double>(m, stage, wfac, want_dt ? m->dtpart : nullptr);
   timer_end(m);
   LAUNCH_CHECK("k_stage");
   std::swap(A[AGX_ARR_U], A[AGX_ARR_U1]);
@@ -2401,12 +2446,98 @@ static int stage_fused(agx_mesh *m, int stage, bool want_dt) {
   return 0;
 }
 
+static void graph_reset(agx_mesh *m) {
+  for (auto &cg : m->graphs)
+    if (cg.exec) hipGraphExecDestroy(cg.exec);
+  m->graphs.clear();
+}
+
+// One fused cycle with everything on the device (flux and stage kernels read dt from
+// m->scal, k_advance forms the next dt and time), for stream capture.
+static int cycle_device(agx_mesh *m) {
+  const dim3 sg = stage_grid(m->g, m->tile);
+  int rc = stage_fused(m, 1, false);
+  if (!rc) rc = stage_fused(m, 2, true);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(256), 0, m->stream, m->dtpart, (int64_t)sg.x * sg.y, m->scal);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, m->stream, m->scal, m->d.cfl);
+  LAUNCH_CHECK("k_advance");
+  return 0;
+}
+
+// The executable graph of the cycle that starts from the current register assignment,
+// captured on first use; the host's register table is left as the cycle leaves it.
+static int graph_for_cycle(agx_mesh *m, hipGraphExec_t *exec) {
+  double **A = m->arr.a;
+  for (auto &cg : m->graphs)
+    if (std::memcmp(cg.before, A, sizeof(cg.before)) == 0) {
+      std::memcpy(A, cg.after, sizeof(cg.after));
+      *exec = cg.exec;
+      return 0;
+    }
+  if (m->graphs.size() >= 4) graph_reset(m);
+  agx_mesh::CycleGraph cg;
+  std::memcpy(cg.before, A, sizeof(cg.before));
+  hipGraph_t graph = nullptr;
+  AGX_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+  int rc = cycle_device(m);
+  hipError_t e = hipStreamEndCapture(m->stream, &graph);
+  if (rc || e != hipSuccess) {
+    std::memcpy(A, cg.before, sizeof(cg.before));
+    if (graph) hipGraphDestroy(graph);
+    return rc ? rc : hip_fail(e, "hipStreamEndCapture");
+  }
+  e = hipGraphInstantiate(&cg.exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    std::memcpy(A, cg.before, sizeof(cg.before));
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  std::memcpy(cg.after, A, sizeof(cg.after));
+  m->graphs.push_back(cg);
+  *exec = cg.exec;
+  return 0;
+}
+
+// agx_cycle with each cycle replayed as one hipGraph launch; the host reads back dt and
+// time after each cycle (the tlim test), as the eager path does.
+static int cycle_graphs(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
+  int32_t n = 0;
+  int rc = push_scalars(m, tlim);
+  while (!rc && n < ncycles && m->time < tlim) {
+    hipGraphExec_t exec = nullptr;
+    rc = graph_for_cycle(m, &exec);
+    if (rc) break;
+    hipError_t e = hipGraphLaunch(exec, m->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(m->scal_host + SC_DT, m->scal + SC_DT, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                         m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "cycle graph"); break; }
+    m->dt = m->scal_host[SC_DT];
+    m->time = m->scal_host[SC_TIME];
+    m->ncycle++;
+    ++n;
+  }
+  if (done) *done = n;
+  return rc;
+}
+
+int agx_set_graph(agx_mesh *m, int32_t enable) {
+  if (!m) { set_error("null mesh"); return -1; }
+  m->graph_on = enable != 0;
+  if (!m->graph_on) graph_reset(m);
+  return 0;
+}
+
 int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
   if (!m) { set_error("null mesh"); return -1; }
+  if (m->graph_on && !m->timer.enabled && !m->d.self_gravity) return cycle_graphs(m, tlim, ncycles, done);
   int32_t n = 0;
   int rc = 0;
   const dim3 sg = stage_grid(m->g, m->tile);
   while (n < ncycles && m->time < tlim) {
+    rc = push_scalars(m, tlim);
     for (int stage = 1; stage <= 2 && !rc; ++stage) {
       // VL2 stage 1 is donor cell (time_integrator.cpp:1466-1467), inside stage_fused;
       // the last stage also leaves the NewBlockTimeStep partials in m->dtpart
@@ -2468,7 +2599,8 @@ int agx_autotune(agx_mesh *m, int32_t *choice) {
   hipEvent_t e0, e1;
   AGX_HIP(hipEventCreate(&e0));
   AGX_HIP(hipEventCreate(&e1));
-  const double wght = 0.5 * m->dt;
+  const double wfac = 0.5;
+  graph_reset(m);
   float best = 0.f;
   int best_t = 0;
   const bool timing = m->timer.enabled;
@@ -2478,8 +2610,8 @@ int agx_autotune(agx_mesh *m, int32_t *choice) {
     float ms = 0.f;
     for (int rep = 0; rep < 4; ++rep) {
       if (rep == 1) AGX_HIP(hipEventRecord(e0, m->stream));
-      if (m->mixed) launch_stage<float>(m, 1, wght, nullptr);
-      else launch_stage<double>(m, 1, wght, nullptr);
+      if (m->mixed) launch_stage<float>(m, 1, wfac, nullptr);
+      else launch_stage<double>(m, 1, wfac, nullptr);
     }
     AGX_HIP(hipEventRecord(e1, m->stream));
     AGX_HIP(hipEventSynchronize(e1));
@@ -2499,6 +2631,7 @@ int agx_set_tile(agx_mesh *m, int32_t tile) {
   if (!m) { set_error("null mesh"); return -1; }
   if (tile < 0 || tile >= NTILE) { set_error("tile shape %d not in [0, %d)", tile, NTILE); return -2; }
   m->tile = tile;
+  graph_reset(m);
   return 0;
 }
 

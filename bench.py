@@ -104,6 +104,17 @@ def mhd_bench(agx, pgen, args, precision):
     sec = t1 - t0
     zc = float(N) ** 3 * args.steps / sec
     log(f"[bench] {precision}: {args.steps} cycles in {sec * 1e3:.1f} ms -> {zc:.4e} zone-cycles/s")
+    # the same cycles replayed as one hipGraph launch per cycle
+    m.set_graph(True)
+    m.cycle(1e30, 2)
+    m.sync()
+    t0 = time.perf_counter()
+    m.cycle(1e30, args.steps)
+    m.sync()
+    gsec = time.perf_counter() - t0
+    m.set_graph(False)
+    log(f"[bench] {precision} hipGraph replay: {args.steps} cycles in {gsec * 1e3:.1f} ms -> "
+        f"{float(N) ** 3 * args.steps / gsec:.4e} zone-cycles/s")
     # instrumented pass: HIP events on the mesh stream around every launch
     m.timing(True)
     m.timing_reset()
@@ -310,7 +321,19 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="run the 512^3 PPM + self-gravity mixed-precision case instead")
     ap.add_argument("--large-n", type=int, default=512)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device three times (scale for rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE: bytes per dispatch of a known copy)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("[bench] copy probe: 3 x 1 GiB device-to-device copies")
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per run); "
               "run with --gpus 1", file=sys.stderr)

@@ -177,6 +177,16 @@ int agx_mg_vcycles(agx_mesh *m, int32_t n);
 int agx_autotune(agx_mesh *m, int32_t *choice);
 int agx_set_tile(agx_mesh *m, int32_t tile);
 
+/* ---- hipGraph replay of agx_cycle (no counterpart in the reference: launch mechanism).
+ * With enable != 0, agx_cycle captures one whole VL2 cycle (fluxes, stage kernels,
+ * ghost C2P, the dt reduction and the device-side time advance) into a hipGraph on first
+ * use and replays it, one graph launch per cycle; dt and time stay on the device and are
+ * read back once per cycle for the tlim test.  Results are identical to the eager path.
+ * Used only without self-gravity (the multigrid solve decides its V-cycle count on the
+ * host, multigrid_driver.cpp:721-755) and while the built-in timers are off; otherwise
+ * agx_cycle runs eagerly.  On by default; enable = 0 selects the eager path. */
+int agx_set_graph(agx_mesh *m, int32_t enable);
+
 /* ---- built-in timers (HIP events on the mesh stream around each launch) */
 int agx_timing_enable(agx_mesh *m, int enable);
 int agx_timing_reset(agx_mesh *m);

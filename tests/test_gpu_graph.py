@@ -1,0 +1,112 @@
+"""hipGraph replay of the fused cycle (agx_set_graph) against the oracle and the eager path.
+
+A captured cycle reads dt and time from device memory and advances them on the device,
+and the face registers rotate with period two cycles, so one graph per register
+assignment is replayed.  Each test replays the graphs several times with host work
+(downloads, eager cycles) between replays and checks every replay: bit for bit against
+the oracle in FP64, bit for bit against the eager GPU path in the mixed form.
+"""
+import numpy as np
+import pytest
+
+from oracle.orc import Oracle, make_desc
+from tests.states import load_state, turbulent_state
+
+pytestmark = pytest.mark.gpu
+FIELDS = ("u", "w", "b1f", "b2f", "b3f")
+
+
+def turb_desc(**kw):
+    return make_desc(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, **kw)
+
+
+def same(a, b, name):
+    d = np.abs(a - b).max()
+    print(f"{name}: max|diff| {d:.3e}", flush=True)
+    assert np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_graph_cycles_match_oracle(agx, order):
+    desc = turb_desc(xorder=order, nghost=3 if order == 3 else 2)
+    st = turbulent_state(desc, seed=5, mach=1.5)
+    o = Oracle(desc)
+    m = agx.Mesh(desc)
+    load_state(o, st)
+    load_state(m, st)
+    o.initialize()
+    m.initialize()
+    m.set_graph(True)
+    total = 0
+    # graph, graph again (both register parities), eager, graph: every replay checked
+    for n, graph in ((3, True), (4, True), (1, False), (2, True)):
+        m.set_graph(graph)
+        assert m.cycle(1e9, n) == n
+        for _ in range(n):
+            o.cycle(1e9)
+        total += n
+        assert m.time() == o.time(), (m.time(), o.time())
+        for f in FIELDS:
+            same(m.get(f), o.get(f), f"{f} after {total} cycles (graph={graph})")
+
+
+def test_graph_stops_at_tlim(agx):
+    """The last step is clipped to tlim on the device, as Mesh::NewTimeStep does."""
+    desc = turb_desc()
+    st = turbulent_state(desc, seed=6, mach=1.0)
+    o = Oracle(desc)
+    m = agx.Mesh(desc)
+    load_state(o, st)
+    load_state(m, st)
+    tlim = 0.013
+    o.initialize()
+    m.initialize(tlim)
+    m.set_graph(True)
+    nm = m.cycle(tlim, 1000)
+    no = o.run(tlim)
+    print(f"cycles: graph {nm}, oracle {no}; time {m.time()}", flush=True)
+    assert m.time() == o.time()
+    assert m.time()[0] == tlim
+    for f in FIELDS:
+        same(m.get(f), o.get(f), f)
+    assert m.cycle(tlim, 5) == 0
+
+
+def test_graph_mixed_equals_eager(agx):
+    desc = turb_desc(precision=1)
+    st = turbulent_state(desc, seed=7, mach=1.5)
+    runs = []
+    for graph in (False, True):
+        m = agx.Mesh(desc)
+        load_state(m, st)
+        m.initialize()
+        m.set_graph(graph)
+        out = []
+        for n in (2, 3, 2):
+            assert m.cycle(1e9, n) == n
+            out.append((m.time(), [m.get(f) for f in FIELDS]))
+        runs.append(out)
+        m.close()
+    for (te, ae), (tg, ag) in zip(*runs):
+        assert te == tg
+        for f, a, b in zip(FIELDS, ae, ag):
+            same(b, a, f"mixed {f}")
+
+
+def test_graph_rebuilt_after_tile_change(agx):
+    desc = turb_desc()
+    st = turbulent_state(desc, seed=8, mach=1.5)
+    o = Oracle(desc)
+    m = agx.Mesh(desc)
+    load_state(o, st)
+    load_state(m, st)
+    o.initialize()
+    m.initialize()
+    m.set_graph(True)
+    for tile in (1, 3, 0):
+        m.set_tile(tile)
+        assert m.cycle(1e9, 2) == 2
+        for _ in range(2):
+            o.cycle(1e9)
+        for f in FIELDS:
+            same(m.get(f), o.get(f), f"{f} tile {tile}")
