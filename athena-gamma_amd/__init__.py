@@ -43,13 +43,16 @@ class Desc(ctypes.Structure):
         "eos", "self_gravity", "mg_mode", "mg_niter", "precision")] + \
         [(n, ctypes.c_double) for n in (
             "x1min", "x1max", "x2min", "x2max", "x3min", "x3max", "gamma", "iso_cs",
-            "cfl", "dfloor", "pfloor", "four_pi_G", "mg_threshold")]
+            "cfl", "dfloor", "pfloor", "four_pi_G", "mg_threshold")] + \
+        [(n, ctypes.c_int32) for n in ("mg_bc", "mporder", "mp_auto_origin")] + \
+        [("mp_origin", ctypes.c_double * 3)]
 
 
 def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)),
               nghost=2, xorder=2, rsolver=0, gamma=1.666666666666667, cfl=0.3,
               self_gravity=0, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=10,
-              precision=0, eos=0, iso_cs=1.0):
+              precision=0, eos=0, iso_cs=1.0, mg_bc=0, mporder=4, mp_auto_origin=1,
+              mp_origin=(0.0, 0.0, 0.0)):
     bnx = bnx or nx
     d = Desc()
     d.nx1, d.nx2, d.nx3 = nx
@@ -61,6 +64,8 @@ def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)
     # EquationOfState defaults sqrt(1024*FLT_MIN), eos/adiabatic_mhd.cpp:26-31
     d.dfloor = d.pfloor = float(np.sqrt(1024 * np.float64(np.finfo(np.float32).tiny)))
     d.four_pi_G, d.mg_threshold = four_pi_G, mg_threshold
+    d.mg_bc, d.mporder, d.mp_auto_origin = mg_bc, mporder, mp_auto_origin
+    d.mp_origin[:] = mp_origin
     return d
 
 

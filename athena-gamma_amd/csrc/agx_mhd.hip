@@ -1778,6 +1778,12 @@ int check_desc(const agx_mesh_desc *d) {
     set_error("precision %d not supported (0: FP64, 1: mixed)", d->precision); return -10;
   }
   if (!(d->gamma > 1.0)) { set_error("gamma must be > 1"); return -11; }
+  if (d->mg_bc != 0 && d->mg_bc != 1) {
+    set_error("mg_bc %d not supported (0: periodic, 1: multipole)", d->mg_bc); return -14;
+  }
+  if (d->mg_bc == 1 && d->mporder != 2 && d->mporder != 4) {
+    set_error("mporder must be 2 or 4 (mg_gravity.cpp:97-105)"); return -14;
+  }
   return 0;
 }
 

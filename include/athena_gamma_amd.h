@@ -58,6 +58,12 @@ typedef struct agx_mesh_desc {
   double dfloor, pfloor;        /* <hydro>/dfloor, pfloor */
   double four_pi_G;             /* Mesh::SetFourPiG */
   double mg_threshold;          /* <gravity>/threshold (absolute L2 defect norm) */
+  int32_t mg_bc;                /* <gravity>/ix1_bc..ox3_bc: 0 periodic (all faces), 1 multipole
+                                   (all faces; mgbval_multipole.cpp, with FAS as
+                                   multigrid_driver.cpp:292-295 requires) */
+  int32_t mporder;              /* <gravity>/mporder: 2 or 4 (multipole only) */
+  int32_t mp_auto_origin;       /* <gravity>/auto_mporigin: 1 = expand about the centre of mass */
+  double mp_origin[3];          /* <gravity>/mporigin_x1..x3 (auto_mporigin = 0) */
 } agx_mesh_desc;
 
 enum { AGX_RSOLVER_HLLD = 0, AGX_RSOLVER_HLLE = 1, AGX_RSOLVER_ROE = 2 };

@@ -63,7 +63,10 @@ typedef struct {
   int nlev;          /* levels, 0 = coarsest */
   int n1[24], n2[24], n3[24];   /* active cells per level */
   double *u[24], *src[24], *def[24];
+  double *uold[24];  /* FAS: u at StoreOldData (multigrid.cpp:645-653) */
   double rdx;        /* cell size of the finest level */
+  double xmin[3], xmax[3];   /* RegionSize of the block (or of the root grid) */
+  int bnd[6];        /* physical multipole boundary on face ix1, ox1, ix2, ox2, ix3, ox3 */
 } mglevels;
 
 struct orc_ctx {
@@ -88,6 +91,9 @@ struct orc_ctx {
   double mg_eps;
   int iso;           /* isothermal EOS (eos = 1): NHYDRO = 4, the energy slot stays 0 */
   int nhydro;
+  int fas;           /* ffas_: multipole boundaries (multigrid_driver.cpp:292-295) */
+  double mpo[3];     /* expansion origin (mpo_) */
+  double mpc[25];    /* scaled multipole coefficients (mpcoeff_) */
 };
 
 /* ---------------------------------------------------------------- helpers */
@@ -151,6 +157,7 @@ static void mg_alloc(mglevels *m, int nx1, int nx2, int nx3, int nlev, double rd
     m->u[l] = (double *)xcalloc(sz * sizeof(double));
     m->src[l] = (double *)xcalloc(sz * sizeof(double));
     m->def[l] = (double *)xcalloc(sz * sizeof(double));
+    m->uold[l] = (double *)xcalloc(sz * sizeof(double));
   }
 }
 
@@ -222,12 +229,27 @@ orc_ctx *orc_create(const orc_desc *d) {
                                          : uniform_gen(meshgen_x(lx1 + 1, c->nb1), d->x1min, d->x1max);
       (void)lx2; (void)lx3;
       mg_alloc(&c->mgb[g], d->bnx1, d->bnx2, d->bnx3, nl, (bmax - bmin) / (double)d->bnx1);
+      int lx[3] = {lx1, lx2, lx3}, nbd[3] = {c->nb1, c->nb2, c->nb3};
+      double mn[3] = {d->x1min, d->x2min, d->x3min}, mx[3] = {d->x1max, d->x2max, d->x3max};
+      for (int dd = 0; dd < 3; ++dd) {
+        mglevels *m = &c->mgb[g];
+        m->xmin[dd] = (lx[dd] == 0) ? mn[dd] : uniform_gen(meshgen_x(lx[dd], nbd[dd]), mn[dd], mx[dd]);
+        m->xmax[dd] = (lx[dd] == nbd[dd] - 1) ? mx[dd]
+                                              : uniform_gen(meshgen_x(lx[dd] + 1, nbd[dd]), mn[dd], mx[dd]);
+        m->bnd[2 * dd] = (d->mg_bc == 1) && lx[dd] == 0;
+        m->bnd[2 * dd + 1] = (d->mg_bc == 1) && lx[dd] == nbd[dd] - 1;
+      }
     }
     int nrl = 0;
     for (int l = 0; l < 20; ++l)
       if (c->nb1 % (1 << l) == 0 && c->nb2 % (1 << l) == 0 && c->nb3 % (1 << l) == 0) nrl = l + 1;
     c->nrootlevel = nrl;
     mg_alloc(&c->mgr, c->nb1, c->nb2, c->nb3, nrl, (d->x1max - d->x1min) / (double)c->nb1);
+    c->mgr.xmin[0] = d->x1min; c->mgr.xmin[1] = d->x2min; c->mgr.xmin[2] = d->x3min;
+    c->mgr.xmax[0] = d->x1max; c->mgr.xmax[1] = d->x2max; c->mgr.xmax[2] = d->x3max;
+    for (int f = 0; f < 6; ++f) c->mgr.bnd[f] = (d->mg_bc == 1);
+    c->fas = (d->mg_bc == 1);
+    c->mpo[0] = d->mp_origin[0]; c->mpo[1] = d->mp_origin[1]; c->mpo[2] = d->mp_origin[2];
     c->ntotal = nrl + nl - 1;
     c->mg_mode = d->mg_mode;
     c->mg_niter = d->mg_niter;
@@ -238,7 +260,7 @@ orc_ctx *orc_create(const orc_desc *d) {
 
 static void mg_free(mglevels *m) {
   for (int l = 0; l < m->nlev; ++l) {
-    free(m->u[l]); free(m->src[l]); free(m->def[l]);
+    free(m->u[l]); free(m->src[l]); free(m->def[l]); free(m->uold[l]);
   }
 }
 
@@ -1699,8 +1721,12 @@ static void mg_restrict(mglevels *m, double *dst, const double *src, int l) {
 }
 
 /* Multigrid::ProlongateAndCorrect, multigrid/multigrid.cpp:733-829 (l-1 -> l) */
-static void mg_prolongate(mglevels *m, int l) {
+static void mg_prolongate(mglevels *m, int l, int fas) {
   int lc = l - 1;
+  if (fas) {   /* ProlongateAndCorrectBlock, FAS: u -= uold on the coarse level first */
+    size_t sz = (size_t)(m->n1[lc] + 2) * (m->n2[lc] + 2) * (m->n3[lc] + 2);
+    for (size_t q = 0; q < sz; ++q) m->u[lc][q] -= m->uold[lc][q];
+  }
   const double *s = m->u[lc];
   double *d = m->u[l];
   for (int k = 1; k <= m->n3[lc]; ++k) {
@@ -1764,6 +1790,8 @@ static void mg_fmg_prolongate(mglevels *m, int l) {
 
 /* Same-level ghost fill of every block's level l (bvals_mg.cpp:447-520,634-747,
  * 866-915): each ghost cell copies the neighbour block's interior cell. */
+static void mg_phys_bc(const orc_ctx *c, mglevels *m, int l);
+
 static void mg_fill_blocks(orc_ctx *c, int l, int which) {
   int nb = c->nblocks;
 #pragma omp parallel for schedule(dynamic, 1)
@@ -1771,6 +1799,7 @@ static void mg_fill_blocks(orc_ctx *c, int l, int which) {
     mglevels *m = &c->mgb[g];
     int n = m->n1[l];
     double *a = (which == 0) ? m->u[l] : m->src[l];
+    const int lxo[3] = {g % c->nb1, (g / c->nb1) % c->nb2, g / (c->nb1 * c->nb2)};
     for (int k = 0; k <= n + 1; ++k) {
       int o3 = (k == 0) ? -1 : (k == n + 1 ? 1 : 0);
       for (int j = 0; j <= n + 1; ++j) {
@@ -1778,6 +1807,9 @@ static void mg_fill_blocks(orc_ctx *c, int l, int which) {
         for (int i = 0; i <= n + 1; ++i) {
           int o1 = (i == 0) ? -1 : (i == n + 1 ? 1 : 0);
           if (!o1 && !o2 && !o3) continue;
+          if (c->d.mg_bc == 1 && (lxo[0] + o1 < 0 || lxo[0] + o1 >= c->nb1 || lxo[1] + o2 < 0 ||
+                                  lxo[1] + o2 >= c->nb2 || lxo[2] + o3 < 0 || lxo[2] + o3 >= c->nb3))
+            continue;   /* no neighbour beyond a physical boundary */
           mglevels *s = &c->mgb[nbr_gid(c, g, o1, o2, o3)];
           double *sa = (which == 0) ? s->u[l] : s->src[l];
           a[MIDX(m, l, k, j, i)] = sa[MIDX(m, l, k - o3 * n, j - o2 * n, i - o1 * n)];
@@ -1785,12 +1817,18 @@ static void mg_fill_blocks(orc_ctx *c, int l, int which) {
       }
     }
   }
+  if (c->d.mg_bc == 1 && which == 0)
+    for (int g = 0; g < nb; ++g) mg_phys_bc(c, &c->mgb[g], l);
 }
 
 /* Root grid periodic physical boundaries, MGBoundaryValues::ApplyPhysicalBoundaries
  * (bvals_mg.cpp:321-370) with MGPeriodic* (multigrid/mgbval_periodic.cpp:26-144):
  * x1 faces over interior (j,k), then x2 over all i, then x3 over all (i,j). */
-static void mg_root_bc(mglevels *m, int l) {
+static void mg_root_bc(orc_ctx *c, mglevels *m, int l) {
+  if (c->d.mg_bc == 1) {
+    mg_phys_bc(c, m, l);
+    return;
+  }
   int n1 = m->n1[l], n2 = m->n2[l], n3 = m->n3[l];
   double *u = m->u[l];
   for (int k = 1; k <= n3; ++k)
@@ -1815,10 +1853,226 @@ static void mg_zero(mglevels *m, int l) {
   memset(m->u[l], 0, sz * sizeof(double));
 }
 
+/* MGCoordinates::CalculateMGCoordinates (multigrid.cpp:1321-1350), ngh = 1: face
+ * coordinate x?f(i) and centre x?v(i) of level l, i = 0..n+1 (0 and n+1 ghosts). */
+static double mgc_f(const mglevels *m, int l, int dir, int i) {
+  int nc = dir == 0 ? m->n1[l] : (dir == 1 ? m->n2[l] : m->n3[l]);
+  if (i == 1) return m->xmin[dir];
+  if (i == nc + 1) return m->xmax[dir];
+  double dx = (m->xmax[dir] - m->xmin[dir]) / nc;
+  return m->xmin[dir] + (i - 1) * dx;
+}
+static double mgc_v(const mglevels *m, int l, int dir, int i) {
+  return 0.5 * (mgc_f(m, l, dir, i) + mgc_f(m, l, dir, i + 1));
+}
+
+/* Multipole potential at (x, y, z) relative to the origin, MGMultipoleInnerX1..
+ * OuterX3 (multigrid/mgbval_multipole.cpp:28-426), mporder 2 or 4. */
+static double mp_phis(const orc_ctx *c, double x, double y, double z) {
+  const double *mp = c->mpc;
+  double x2 = x * x, y2 = y * y, z2 = z * z;
+  double xy = x * y, yz = y * z, zx = z * x;
+  double r2 = x2 + y2 + z2;
+  double ir2 = 1.0 / r2, ir1 = sqrt(ir2);
+  double ir3 = ir2 * ir1, ir5 = ir3 * ir2;
+  if (c->d.mporder == 2)
+    return ir1 * mp[0] + ir3 * (mp[1] * y + mp[2] * z + mp[3] * x) +
+           ir5 * (mp[4] * xy + mp[5] * yz + (3.0 * z2 - r2) * mp[6] + mp[7] * zx + mp[8] * 0.5 * (x2 - y2));
+  double ir7 = ir5 * ir2, ir9 = ir7 * ir2;
+  double hx2my2 = 0.5 * (x2 - y2);
+  double x2mty2 = x2 - 3.0 * y2;
+  double tx2my2 = 3.0 * x2 - y2;
+  return ir1 * mp[0] + ir3 * (mp[1] * y + mp[2] * z + mp[3] * x) +
+         ir5 * (mp[4] * xy + mp[5] * yz + (3.0 * z2 - r2) * mp[6] + mp[7] * zx + mp[8] * 0.5 * (x2 - y2)) +
+         ir7 * (y * tx2my2 * mp[9] + x * x2mty2 * mp[15] + xy * z * mp[10] + z * hx2my2 * mp[14] +
+                (5.0 * z2 - r2) * (y * mp[11] + x * mp[13]) + z * (z2 - 3.0 * r2) * mp[12]) +
+         ir9 * (xy * hx2my2 * mp[16] + 0.125 * (x2 * x2mty2 - y2 * tx2my2) * mp[24] + yz * tx2my2 * mp[17] +
+                zx * x2mty2 * mp[23] + (7.0 * z2 - r2) * (xy * mp[18] + hx2my2 * mp[22]) +
+                (7.0 * z2 - 3.0 * r2) * (yz * mp[19] + zx * mp[21]) +
+                (35.0 * z2 * z2 - 30.0 * z2 * r2 + 3.0 * r2 * r2) * mp[20]);
+}
+
+/* MGBoundaryValues::ApplyPhysicalBoundaries (bvals_mg.cpp:321-368) with the multipole
+ * functions: ghost = 2 phi_s(face point) - adjacent cell.  x1 faces over the interior
+ * (j, k) (plus the j/k ghosts where that face is not physical), then x2 faces over all
+ * i, then x3 faces over all (i, j). */
+static void mg_phys_bc(const orc_ctx *c, mglevels *m, int l) {
+  int n1 = m->n1[l], n2 = m->n2[l], n3 = m->n3[l];
+  double *u = m->u[l];
+  const double xo = c->mpo[0], yo = c->mpo[1], zo = c->mpo[2];
+  int bjs = m->bnd[2] ? 1 : 0, bje = m->bnd[3] ? n2 : n2 + 1;
+  int bks = m->bnd[4] ? 1 : 0, bke = m->bnd[5] ? n3 : n3 + 1;
+  for (int side = 0; side < 2; ++side) {
+    if (!m->bnd[side]) continue;
+    int ia = side ? n1 : 1, ig = side ? n1 + 1 : 0;
+    double x = mgc_f(m, l, 0, side ? n1 + 1 : 1) - xo;
+    for (int k = bks; k <= bke; ++k) {
+      double z = mgc_v(m, l, 2, k) - zo;
+      for (int j = bjs; j <= bje; ++j) {
+        double y = mgc_v(m, l, 1, j) - yo;
+        u[MIDX(m, l, k, j, ig)] = 2.0 * mp_phis(c, x, y, z) - u[MIDX(m, l, k, j, ia)];
+      }
+    }
+  }
+  for (int side = 0; side < 2; ++side) {
+    if (!m->bnd[2 + side]) continue;
+    int ja = side ? n2 : 1, jg = side ? n2 + 1 : 0;
+    double y = mgc_f(m, l, 1, side ? n2 + 1 : 1) - yo;
+    for (int k = bks; k <= bke; ++k) {
+      double z = mgc_v(m, l, 2, k) - zo;
+      for (int i = 0; i <= n1 + 1; ++i) {
+        double x = mgc_v(m, l, 0, i) - xo;
+        u[MIDX(m, l, k, jg, i)] = 2.0 * mp_phis(c, x, y, z) - u[MIDX(m, l, k, ja, i)];
+      }
+    }
+  }
+  for (int side = 0; side < 2; ++side) {
+    if (!m->bnd[4 + side]) continue;
+    int ka = side ? n3 : 1, kg = side ? n3 + 1 : 0;
+    double z = mgc_f(m, l, 2, side ? n3 + 1 : 1) - zo;
+    for (int j = 0; j <= n2 + 1; ++j) {
+      double y = mgc_v(m, l, 1, j) - yo;
+      for (int i = 0; i <= n1 + 1; ++i) {
+        double x = mgc_v(m, l, 0, i) - xo;
+        u[MIDX(m, l, kg, j, i)] = 2.0 * mp_phis(c, x, y, z) - u[MIDX(m, l, ka, j, i)];
+      }
+    }
+  }
+}
+
+/* FAS: StoreOldData (multigrid.cpp:645-653) and MGGravity::CalculateFASRHS
+ * (gravity/mg_gravity.cpp:324-351) on level l. */
+static void mg_fas_rhs(mglevels *m, int l) {
+  int ll = m->nlev - 1 - l;
+  double dx = m->rdx * (double)(1 << ll);
+  double idx2 = 1.0 / SQR(dx);
+  size_t sz = (size_t)(m->n1[l] + 2) * (m->n2[l] + 2) * (m->n3[l] + 2);
+  memcpy(m->uold[l], m->u[l], sz * sizeof(double));
+  int nx = m->n1[l] + 2, ny = m->n2[l] + 2;
+  int64_t sj = nx, sk = (int64_t)nx * ny;
+  const double *u = m->u[l];
+  double *src = m->src[l];
+  for (int k = 1; k <= m->n3[l]; ++k)
+    for (int j = 1; j <= m->n2[l]; ++j)
+      for (int i = 1; i <= m->n1[l]; ++i) {
+        int64_t id = MIDX(m, l, k, j, i);
+        src[id] -= (6.0 * u[id] - u[id + sk] - u[id + sj] - u[id + 1] - u[id - sk] - u[id - sj] - u[id - 1]) * idx2;
+      }
+}
+
+/* MultigridDriver::CalculateCenterOfMass (multigrid_driver.cpp:1984-2015) with
+ * Multigrid::CalculateCenterOfMass (multigrid.cpp:1270-1298), blocks in gid order. */
+static void mp_center_of_mass(orc_ctx *c) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int g = 0; g < c->nblocks; ++g) {
+    mglevels *m = &c->mgb[g];
+    int l = m->nlev - 1;
+    double vol = (mgc_f(m, l, 0, 2) - mgc_f(m, l, 0, 1)) * (mgc_f(m, l, 1, 2) - mgc_f(m, l, 1, 1)) *
+                 (mgc_f(m, l, 2, 2) - mgc_f(m, l, 2, 1));
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    for (int k = 1; k <= m->n3[l]; ++k) {
+      double z = mgc_v(m, l, 2, k);
+      for (int j = 1; j <= m->n2[l]; ++j) {
+        double y = mgc_v(m, l, 1, j);
+        for (int i = 1; i <= m->n1[l]; ++i) {
+          double x = mgc_v(m, l, 0, i);
+          double s = m->src[l][MIDX(m, l, k, j, i)] * vol;
+          m0 += s;
+          m1 += s * y;
+          m2 += s * z;
+          m3 += s * x;
+        }
+      }
+    }
+    acc[0] += m0; acc[1] += m1; acc[2] += m2; acc[3] += m3;
+  }
+  double im = 1.0 / acc[0];
+  c->mpo[0] = im * acc[3];
+  c->mpo[1] = im * acc[1];
+  c->mpo[2] = im * acc[2];
+}
+
+/* MultigridDriver::CalculateMultipoleCoefficients + ScaleMultipoleCoefficients
+ * (multigrid_driver.cpp:1903-1981) with Multigrid::CalculateMultipoleCoefficients
+ * (multigrid.cpp:1054-1264), no "nodipole". */
+static void mp_coefficients(orc_ctx *c) {
+  const int np = (c->d.mporder == 4) ? 25 : 9;
+  double acc[25] = {0.0};
+  const double xo = c->mpo[0], yo = c->mpo[1], zo = c->mpo[2];
+  for (int g = 0; g < c->nblocks; ++g) {
+    mglevels *m = &c->mgb[g];
+    int l = m->nlev - 1;
+    double vol = (mgc_f(m, l, 0, 2) - mgc_f(m, l, 0, 1)) * (mgc_f(m, l, 1, 2) - mgc_f(m, l, 1, 1)) *
+                 (mgc_f(m, l, 2, 2) - mgc_f(m, l, 2, 1));
+    double mm[25] = {0.0};
+    for (int k = 1; k <= m->n3[l]; ++k) {
+      double z = mgc_v(m, l, 2, k) - zo;
+      double z2 = z * z;
+      for (int j = 1; j <= m->n2[l]; ++j) {
+        double y = mgc_v(m, l, 1, j) - yo;
+        double y2 = y * y, yz = y * z;
+        for (int i = 1; i <= m->n1[l]; ++i) {
+          double x = mgc_v(m, l, 0, i) - xo;
+          double x2 = x * x, xy = x * y, zx = z * x;
+          double r2 = x2 + y2 + z2;
+          double s = m->src[l][MIDX(m, l, k, j, i)] * vol;
+          mm[0] += s;
+          mm[1] += s * y;
+          mm[2] += s * z;
+          mm[3] += s * x;
+          mm[4] += s * xy;
+          mm[5] += s * yz;
+          mm[6] += s * (3.0 * z2 - r2);
+          mm[7] += s * zx;
+          if (np == 9) {
+            mm[8] += s * 0.5 * (x2 - y2);
+            continue;
+          }
+          double hx2my2 = 0.5 * (x2 - y2);
+          double x2mty2 = x2 - 3.0 * y2;
+          double tx2my2 = 3.0 * x2 - y2;
+          double fz2mr2 = 5.0 * z2 - r2;
+          double sz2mr2 = 7.0 * z2 - r2;
+          double sz2mtr2 = 7.0 * z2 - 3.0 * r2;
+          mm[8] += s * hx2my2;
+          mm[9] += s * y * tx2my2;
+          mm[10] += s * xy * z;
+          mm[11] += s * y * fz2mr2;
+          mm[12] += s * z * (z2 - 3.0 * r2);
+          mm[13] += s * x * fz2mr2;
+          mm[14] += s * z * hx2my2;
+          mm[15] += s * x * x2mty2;
+          mm[16] += s * xy * hx2my2;
+          mm[17] += s * yz * tx2my2;
+          mm[18] += s * xy * sz2mr2;
+          mm[19] += s * yz * sz2mtr2;
+          mm[20] += s * (35.0 * z2 * z2 - 30.0 * z2 * r2 + 3.0 * r2 * r2);
+          mm[21] += s * zx * sz2mtr2;
+          mm[22] += s * hx2my2 * sz2mr2;
+          mm[23] += s * zx * x2mty2;
+          mm[24] += s * 0.125 * (x2 * x2mty2 - y2 * tx2my2);
+        }
+      }
+    }
+    for (int q = 0; q < np; ++q) acc[q] += mm[q];
+  }
+  const double PI_ = 3.14159265358979323846;
+  const double c0 = -0.25 / PI_, c1 = -0.25 / PI_, c2 = -0.0625 / PI_, c2a = -0.75 / PI_;
+  const double c30 = -0.0625 / PI_, c31 = -0.0625 * 1.5 / PI_, c32 = -0.25 * 15.0 / PI_,
+               c33 = -0.0625 * 2.5 / PI_;
+  const double c40 = -0.0625 * 0.0625 / PI_, c41 = -0.0625 * 2.5 / PI_, c42 = -0.0625 * 5.0 / PI_,
+               c43 = -0.0625 * 17.5 / PI_, c44 = -0.25 * 35.0 / PI_;
+  const double sc[25] = {c0, c1, c1, c1, c2a, c2a, c2, c2a, c2a, c33, c32, c31, c30,
+                         c31, c32, c33, c44, c43, c42, c41, c40, c41, c42, c43, c44};
+  for (int q = 0; q < 25; ++q) c->mpc[q] = (q < np) ? acc[q] * sc[q] : 0.0;
+}
+
 /* blocks: smooth red/black with ghost fills, then RestrictBlock
  * (mg_task_list.cpp:382-... nsmooth=1; multigrid.cpp:331-355) */
 static void mg_blocks_to_coarser(orc_ctx *c, int bl) {
   mg_fill_blocks(c, bl, 0);
+  if (c->fas && c->current < c->fmglevel)   /* MultigridTaskList::CalculateFASRHS */
+    for (int g = 0; g < c->nblocks; ++g) mg_fas_rhs(&c->mgb[g], bl);
 #pragma omp parallel for schedule(dynamic, 1)
   for (int g = 0; g < c->nblocks; ++g) mg_smooth(c, &c->mgb[g], bl, 0);
   mg_fill_blocks(c, bl, 0);
@@ -1830,20 +2084,23 @@ static void mg_blocks_to_coarser(orc_ctx *c, int bl) {
     mglevels *m = &c->mgb[g];
     mg_defect(m, bl);
     mg_restrict(m, m->src[bl - 1], m->def[bl], bl);
-    mg_zero(m, bl - 1);
+    if (c->fas) mg_restrict(m, m->u[bl - 1], m->u[bl], bl);   /* FAS restricts u too */
+    else mg_zero(m, bl - 1);
   }
 }
 
 static void mg_root_to_coarser(orc_ctx *c, int rl) {
   mglevels *m = &c->mgr;
-  mg_root_bc(m, rl);
+  mg_root_bc(c, m, rl);
+  if (c->fas && rl < c->fmglevel) mg_fas_rhs(m, rl);
   mg_smooth(c, m, rl, 0);
-  mg_root_bc(m, rl);
+  mg_root_bc(c, m, rl);
   mg_smooth(c, m, rl, 1);
-  mg_root_bc(m, rl);
+  mg_root_bc(c, m, rl);
   mg_defect(m, rl);
   mg_restrict(m, m->src[rl - 1], m->def[rl], rl);
-  mg_zero(m, rl - 1);
+  if (c->fas) mg_restrict(m, m->u[rl - 1], m->u[rl], rl);
+  else mg_zero(m, rl - 1);
 }
 
 /* TransferFromBlocksToRoot (multigrid_driver.cpp:467-527), non-FAS */
@@ -1853,12 +2110,13 @@ static void mg_blocks_to_root(orc_ctx *c, int initflag) {
   for (int g = 0; g < c->nblocks; ++g) {
     int lx1 = g % c->nb1, lx2 = (g / c->nb1) % c->nb2, lx3 = g / (c->nb1 * c->nb2);
     r->src[rl][MIDX(r, rl, lx3 + 1, lx2 + 1, lx1 + 1)] = c->mgb[g].src[0][MIDX(&c->mgb[g], 0, 1, 1, 1)];
+    if (c->fas && !initflag)
+      r->u[rl][MIDX(r, rl, lx3 + 1, lx2 + 1, lx1 + 1)] = c->mgb[g].u[0][MIDX(&c->mgb[g], 0, 1, 1, 1)];
   }
-  (void)initflag;
 }
 
 /* TransferFromRootToBlocks / SetFromRootGrid (multigrid.cpp:480-550) */
-static void mg_root_to_blocks(orc_ctx *c) {
+static void mg_root_to_blocks(orc_ctx *c, int folddata) {
   mglevels *r = &c->mgr;
   int rl = c->nrootlevel - 1;
   for (int g = 0; g < c->nblocks; ++g) {
@@ -1867,7 +2125,10 @@ static void mg_root_to_blocks(orc_ctx *c) {
     for (int k = 0; k <= 2; ++k)
       for (int j = 0; j <= 2; ++j)
         for (int i = 0; i <= 2; ++i)
+        {
           m->u[0][MIDX(m, 0, k, j, i)] = r->u[rl][MIDX(r, rl, lx3 + k, lx2 + j, lx1 + i)];
+          if (folddata) m->uold[0][MIDX(m, 0, k, j, i)] = r->uold[rl][MIDX(r, rl, lx3 + k, lx2 + j, lx1 + i)];
+        }
   }
 }
 
@@ -1878,7 +2139,7 @@ static void mg_one_step_to_coarser(orc_ctx *c) {
     mg_blocks_to_coarser(c, bl);
     if (cur == c->nrootlevel) {
       mg_blocks_to_root(c, 0);
-      mg_zero(&c->mgr, c->nrootlevel - 1);
+      if (!c->fas) mg_zero(&c->mgr, c->nrootlevel - 1);
     }
   } else {
     mg_root_to_coarser(c, cur);
@@ -1912,26 +2173,28 @@ static void mg_solve_coarsest(orc_ctx *c) {
   int ni = c->nb1 > c->nb2 ? c->nb1 : c->nb2;
   ni = ni > c->nb3 ? ni : c->nb3;
   ni >>= (c->nrootlevel - 1);
-  if (ni == 1) {
+  /* periodic: fsubtract_average_; multipole: FAS without average subtraction */
+  if (!c->fas && ni == 1) {
     mg_zero(&c->mgr, 0);
   } else {
-    mg_root_total_sub(c, 0);
-    mg_root_bc(&c->mgr, 0);
+    if (!c->fas) mg_root_total_sub(c, 0);
+    mg_root_bc(c, &c->mgr, 0);
+    if (c->fas) mg_fas_rhs(&c->mgr, 0);
     for (int it = 0; it < ni; ++it) {
       mg_smooth(c, &c->mgr, 0, 0);
-      mg_root_bc(&c->mgr, 0);
+      mg_root_bc(c, &c->mgr, 0);
       mg_smooth(c, &c->mgr, 0, 1);
-      mg_root_bc(&c->mgr, 0);
+      mg_root_bc(c, &c->mgr, 0);
     }
-    mg_root_total_sub(c, 0);
+    if (!c->fas) mg_root_total_sub(c, 0);
   }
 }
 
 static void mg_one_step_to_finer(orc_ctx *c) {
   int cur = c->current;
   if (cur == c->nrootlevel - 1) {          /* multigrid_driver.cpp:586-590 */
-    mg_root_bc(&c->mgr, cur);
-    mg_root_to_blocks(c);
+    mg_root_bc(c, &c->mgr, cur);
+    mg_root_to_blocks(c, c->fas);
   }
   if (cur >= c->nrootlevel - 1) {          /* block levels, SetMGTaskListToFiner */
     int bl = cur - c->nrootlevel + 1;      /* coarse block level */
@@ -1939,7 +2202,7 @@ static void mg_one_step_to_finer(orc_ctx *c) {
      * otherwise they are exchanged first (mg_task_list.cpp:287-300) */
     if (cur != c->nrootlevel - 1) mg_fill_blocks(c, bl, 0);
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int g = 0; g < c->nblocks; ++g) mg_prolongate(&c->mgb[g], bl + 1);
+    for (int g = 0; g < c->nblocks; ++g) mg_prolongate(&c->mgb[g], bl + 1, c->fas);
     mg_fill_blocks(c, bl + 1, 0);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int g = 0; g < c->nblocks; ++g) mg_smooth(c, &c->mgb[g], bl + 1, 0);
@@ -1949,11 +2212,11 @@ static void mg_one_step_to_finer(orc_ctx *c) {
     if (cur == c->ntotal - 2) mg_fill_blocks(c, bl + 1, 0);   /* flag 2: last */
   } else {                                  /* root grid */
     mglevels *m = &c->mgr;
-    mg_root_bc(m, cur);
-    mg_prolongate(m, cur + 1);
-    mg_root_bc(m, cur + 1);
+    mg_root_bc(c, m, cur);
+    mg_prolongate(m, cur + 1, c->fas);
+    mg_root_bc(c, m, cur + 1);
     mg_smooth(c, m, cur + 1, 0);
-    mg_root_bc(m, cur + 1);
+    mg_root_bc(c, m, cur + 1);
     mg_smooth(c, m, cur + 1, 1);
   }
   c->current++;
@@ -1970,8 +2233,8 @@ static void mg_vcycle(orc_ctx *c) {   /* SolveVCycle(1,1), multigrid_driver.cpp:
 static void mg_fmg_prolongate_step(orc_ctx *c) {   /* multigrid_driver.cpp:553-575 */
   int cur = c->current;
   if (cur == c->nrootlevel - 1) {
-    mg_root_bc(&c->mgr, cur);
-    mg_root_to_blocks(c);
+    mg_root_bc(c, &c->mgr, cur);
+    mg_root_to_blocks(c, 0);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int g = 0; g < c->nblocks; ++g) mg_fmg_prolongate(&c->mgb[g], 1);
   } else if (cur >= c->nrootlevel - 1) {
@@ -1980,7 +2243,7 @@ static void mg_fmg_prolongate_step(orc_ctx *c) {   /* multigrid_driver.cpp:553-5
 #pragma omp parallel for schedule(dynamic, 1)
     for (int g = 0; g < c->nblocks; ++g) mg_fmg_prolongate(&c->mgb[g], bl + 1);
   } else {
-    mg_root_bc(&c->mgr, cur);
+    mg_root_bc(c, &c->mgr, cur);
     mg_fmg_prolongate(&c->mgr, cur + 1);
   }
   c->current++;
@@ -2056,6 +2319,11 @@ void orc_mg_set_mode(orc_ctx *c, int mode, double threshold, int niter) {
 }
 double orc_mg_last_norm(orc_ctx *c) { return c->last_norm; }
 int orc_mg_last_iters(orc_ctx *c) { return c->last_iters; }
+void orc_mg_multipole(orc_ctx *c, double *out28) {
+  for (int q = 0; q < 3; ++q) out28[q] = c->mpo[q];
+  for (int q = 0; q < 25; ++q) out28[3 + q] = c->mpc[q];
+}
+double orc_mp_phis(orc_ctx *c, double x, double y, double z) { return mp_phis(c, x, y, z); }
 
 /* MGGravityDriver::Solve, gravity/mg_gravity.cpp:166-211 (periodic boundaries:
  * fsubtract_average_ = true, no FAS, no source mask). */
@@ -2079,8 +2347,15 @@ void orc_mg_solve(orc_ctx *c, int stage) {
   }
   c->current = c->ntotal - 1;
   c->fmglevel = c->ntotal - 1;
-  /* SetupMultigrid, multigrid_driver.cpp:343-460 */
-  c->last_ave = mg_subtract_average(c, 0);
+  /* SetupMultigrid, multigrid_driver.cpp:343-460: periodic boundaries subtract the mean
+   * source; multipole boundaries compute the expansion (about the centre of mass with
+   * auto_mporigin) instead */
+  if (!c->fas) {
+    c->last_ave = mg_subtract_average(c, 0);
+  } else {
+    if (c->d.mp_auto_origin) mp_center_of_mass(c);
+    mp_coefficients(c);
+  }
   if (c->mg_mode == 0) {
     for (int g = 0; g < c->nblocks; ++g) {   /* RestrictFMGSource, multigrid.cpp:253-262 */
       mglevels *m = &c->mgb[g];
@@ -2095,7 +2370,7 @@ void orc_mg_solve(orc_ctx *c, int stage) {
       if (c->fmglevel != c->ntotal - 1) mg_fmg_prolongate_step(c);
     }
     c->fmglevel = c->ntotal - 1;
-    mg_subtract_average(c, 1);
+    if (!c->fas) mg_subtract_average(c, 1);
   }
   int iters = 0;
   if (c->mg_eps >= 0.0) {       /* SolveIterative, multigrid_driver.cpp:721-755 */
@@ -2113,10 +2388,10 @@ void orc_mg_solve(orc_ctx *c, int stage) {
       n++;
     }
     c->last_norm = def;
-    mg_subtract_average(c, 1);
+    if (!c->fas) mg_subtract_average(c, 1);
   } else {                       /* SolveIterativeFixedTimes, :762-771 */
     for (int n = 0; n < c->mg_niter; ++n) { mg_vcycle(c); iters++; }
-    mg_subtract_average(c, 1);
+    if (!c->fas) mg_subtract_average(c, 1);
     c->last_norm = mg_defect_norm(c);
   }
   c->last_iters = iters;

@@ -42,6 +42,10 @@ typedef struct orc_desc {
   double x1min, x1max, x2min, x2max, x3min, x3max;
   double gamma, iso_cs, cfl, dfloor, pfloor;
   double four_pi_G, mg_threshold;
+  int32_t mg_bc;                /* 0 periodic, 1 multipole on all six faces */
+  int32_t mporder;              /* 2 or 4 (multipole) */
+  int32_t mp_auto_origin;       /* 1: expansion about the centre of mass */
+  double mp_origin[3];          /* expansion origin when mp_auto_origin = 0 */
 } orc_desc;
 
 typedef struct orc_ctx orc_ctx;
@@ -76,6 +80,10 @@ void orc_cycle(orc_ctx *c, double tlim);                 /* main.cpp:430-489 */
 void orc_mg_solve(orc_ctx *c, int stage);                /* mg_gravity.cpp:166-211 */
 double orc_mg_last_norm(orc_ctx *c);
 int orc_mg_last_iters(orc_ctx *c);
+/* multipole boundaries: out[0..2] = expansion origin, out[3..27] = scaled coefficients of
+ * the last solve; the boundary potential phi_s at (x, y, z) relative to that origin */
+void orc_mg_multipole(orc_ctx *c, double *out28);
+double orc_mp_phis(orc_ctx *c, double x, double y, double z);
 void orc_mg_set_mode(orc_ctx *c, int mode, double threshold, int niter);
 
 /* problem generators / diagnostics (pgen restatements) */

@@ -24,7 +24,9 @@ class Desc(ctypes.Structure):
         "eos", "self_gravity", "mg_mode", "mg_niter", "precision")] + \
         [(n, ctypes.c_double) for n in (
             "x1min", "x1max", "x2min", "x2max", "x3min", "x3max", "gamma", "iso_cs",
-            "cfl", "dfloor", "pfloor", "four_pi_G", "mg_threshold")]
+            "cfl", "dfloor", "pfloor", "four_pi_G", "mg_threshold")] + \
+        [(n, ctypes.c_int32) for n in ("mg_bc", "mporder", "mp_auto_origin")] + \
+        [("mp_origin", ctypes.c_double * 3)]
 
 
 FLT_MIN = np.finfo(np.float32).tiny
@@ -33,7 +35,8 @@ FLT_MIN = np.finfo(np.float32).tiny
 def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)),
               nghost=2, xorder=2, rsolver=0, gamma=1.666666666666667, cfl=0.3,
               self_gravity=0, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=10,
-              precision=0, eos=0, iso_cs=1.0):
+              precision=0, eos=0, iso_cs=1.0, mg_bc=0, mporder=4, mp_auto_origin=1,
+              mp_origin=(0.0, 0.0, 0.0)):
     bnx = bnx or nx
     d = Desc()
     d.nx1, d.nx2, d.nx3 = nx
@@ -45,6 +48,8 @@ def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)
     # EquationOfState defaults, eos/adiabatic_mhd.cpp:26-31: sqrt(1024*FLT_MIN)
     d.dfloor = d.pfloor = float(np.sqrt(1024 * np.float64(FLT_MIN)))
     d.four_pi_G, d.mg_threshold = four_pi_G, mg_threshold
+    d.mg_bc, d.mporder, d.mp_auto_origin = mg_bc, mporder, mp_auto_origin
+    d.mp_origin[:] = mp_origin
     return d
 
 
@@ -75,6 +80,9 @@ def _lib():
     lib.orc_mg_last_norm.argtypes = [P]
     lib.orc_mg_last_iters.restype = ctypes.c_int
     lib.orc_mg_last_iters.argtypes = [P]
+    lib.orc_mg_multipole.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+    lib.orc_mp_phis.restype = ctypes.c_double
+    lib.orc_mp_phis.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     lib.orc_mg_set_mode.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_int]
     lib.orc_pgen_linear_wave.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     lib.orc_linear_wave_errors.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double, D]
@@ -209,6 +217,15 @@ class Oracle:
 
     def mg_last(self):
         return self.L.orc_mg_last_norm(self.h), self.L.orc_mg_last_iters(self.h)
+
+    def mg_multipole(self):
+        """(origin[3], coefficients[25]) of the last multipole solve."""
+        out = np.zeros(28)
+        self.L.orc_mg_multipole(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out[:3], out[3:]
+
+    def mp_phis(self, x, y, z):
+        return self.L.orc_mp_phis(self.h, x, y, z)
 
 
 def _dp(a):

@@ -31,8 +31,9 @@ def test_desc_layout(agx):
     from oracle.orc import Desc as ODesc
     assert ctypes.sizeof(agx.Desc) == ctypes.sizeof(ODesc)
     assert [f[0] for f in agx.Desc._fields_] == [f[0] for f in ODesc._fields_]
-    # 14 int32 (56 B) then 13 doubles, naturally aligned
-    assert ctypes.sizeof(agx.Desc) == 56 + 13 * 8
+    # 14 int32 (56 B), 13 doubles, 3 int32 (+4 B padding), 3 doubles, naturally aligned
+    assert agx.Desc.mp_origin.offset == 56 + 13 * 8 + 12 + 4
+    assert ctypes.sizeof(agx.Desc) == 56 + 13 * 8 + 16 + 3 * 8
 
 
 @pytest.mark.parametrize("bad,code", [
