@@ -99,6 +99,16 @@ struct MGLevelT {
 using MGLevel = MGLevelT<double>;
 using MGLevel32 = MGLevelT<float>;
 
+// Multigrid level with a one-cell ghost frame, for non-periodic (multipole) boundaries:
+// (n1+2) x (n2+2) x (n3+2) cells, index ((k*(n2+2) + j)*(n1+2) + i), k,j,i in [0, n+1].
+// uold is the FAS copy of u (Multigrid::StoreOldData).
+struct MGLevelB {
+  int n1, n2, n3;
+  double dx;               // cell size of the smoother/defect (as MGLevel::dx)
+  double h[3];             // cell size per direction for coordinates: (xmax - xmin) / n
+  double *u, *src, *uold;
+};
+
 struct Timer {
   bool enabled = false;
   std::vector<hipEvent_t> ev;        // pairs
@@ -144,6 +154,11 @@ struct agx_mesh {
   double *mgpool = nullptr;
   float *mgpool32 = nullptr;
   int64_t mg_bytes = 0;
+  // multipole boundaries (mg_bc = 1): framed levels, coefficients and origin on the device
+  bool mp_on = false;
+  std::vector<agx::MGLevelB> levb;
+  double *mgpoolb = nullptr;
+  double *mpdev = nullptr;           // [0..2] origin, [3..27] coefficients, [28..] partials
   double rdx_root = 0.0, rdy_root = 0.0, rdz_root = 0.0;
   double rdx_blk = 0.0, rdy_blk = 0.0, rdz_blk = 0.0;
   agx::Timer timer;

@@ -459,6 +459,394 @@ k_mg_retrieve(Geo g, const double *__restrict__ mu, double *__restrict__ phi) {
   phi[(int64_t)b * g.S + IDX(g, lk - 1 + g.ng, lj - 1 + g.ng, li - 1 + g.ng)] = mu[LIX(N1, N2, K, J, I)];
 }
 
+
+// ================================================================ multipole boundaries
+// Non-periodic multigrid with MGMultipole* boundaries (multigrid/mgbval_multipole.cpp)
+// and the full approximation scheme the reference uses with them
+// (multigrid_driver.cpp:292-295).  Each level is one grid over the whole mesh with a
+// one-cell ghost frame (MGLevelB).  A ghost is 2 phi_s(p) - (the cell inside it), phi_s
+// the multipole potential at the boundary point p; edge and corner ghosts take "the cell
+// inside" along the last face that writes them (x3 after x2 after x1,
+// MGBoundaryValues::ApplyPhysicalBoundaries, bvals_mg.cpp:321-368), resolved here as a
+// chain so that one launch fills every ghost.  mp[0..2] is the expansion origin,
+// mp[3..27] the scaled coefficients.
+__device__ __forceinline__ unsigned BX(int n1, int n2, int k, int j, int i) {
+  return ((unsigned)k * (unsigned)(n2 + 2) + (unsigned)j) * (unsigned)(n1 + 2) + (unsigned)i;
+}
+
+struct MPGeom {
+  int n1, n2, n3;
+  double xmin[3], xmax[3], h[3];
+};
+
+// MGCoordinates::CalculateMGCoordinates (multigrid.cpp:1321-1350) on the whole mesh:
+// face i (1 .. n+1, end faces exact) and centre i (0 .. n+1) along direction d
+__device__ __forceinline__ double mp_face(const MPGeom &G, int d, int i) {
+  const int n = d == 0 ? G.n1 : (d == 1 ? G.n2 : G.n3);
+  if (i == 1) return G.xmin[d];
+  if (i == n + 1) return G.xmax[d];
+  return G.xmin[d] + (i - 1) * G.h[d];
+}
+__device__ __forceinline__ double mp_centre(const MPGeom &G, int d, int i) {
+  return 0.5 * (mp_face(G, d, i) + mp_face(G, d, i + 1));
+}
+
+// phi_s at (x, y, z) relative to the origin, mgbval_multipole.cpp:36-88 (mporder 2, 4)
+__device__ double mp_phis(const double *__restrict__ mpc, int order, double x, double y, double z) {
+  double x2 = x * x, y2 = y * y, z2 = z * z;
+  double xy = x * y, yz = y * z, zx = z * x;
+  double r2 = x2 + y2 + z2;
+  double ir2 = 1.0 / r2, ir1 = sqrt(ir2);
+  double ir3 = ir2 * ir1, ir5 = ir3 * ir2;
+  if (order == 2)
+    return ir1 * mpc[0] + ir3 * (mpc[1] * y + mpc[2] * z + mpc[3] * x) +
+           ir5 * (mpc[4] * xy + mpc[5] * yz + (3.0 * z2 - r2) * mpc[6] + mpc[7] * zx + mpc[8] * 0.5 * (x2 - y2));
+  double ir7 = ir5 * ir2, ir9 = ir7 * ir2;
+  double hx2my2 = 0.5 * (x2 - y2);
+  double x2mty2 = x2 - 3.0 * y2;
+  double tx2my2 = 3.0 * x2 - y2;
+  return ir1 * mpc[0] + ir3 * (mpc[1] * y + mpc[2] * z + mpc[3] * x) +
+         ir5 * (mpc[4] * xy + mpc[5] * yz + (3.0 * z2 - r2) * mpc[6] + mpc[7] * zx + mpc[8] * 0.5 * (x2 - y2)) +
+         ir7 * (y * tx2my2 * mpc[9] + x * x2mty2 * mpc[15] + xy * z * mpc[10] + z * hx2my2 * mpc[14] +
+                (5.0 * z2 - r2) * (y * mpc[11] + x * mpc[13]) + z * (z2 - 3.0 * r2) * mpc[12]) +
+         ir9 * (xy * hx2my2 * mpc[16] + 0.125 * (x2 * x2mty2 - y2 * tx2my2) * mpc[24] + yz * tx2my2 * mpc[17] +
+                zx * x2mty2 * mpc[23] + (7.0 * z2 - r2) * (xy * mpc[18] + hx2my2 * mpc[22]) +
+                (7.0 * z2 - 3.0 * r2) * (yz * mpc[19] + zx * mpc[21]) +
+                (35.0 * z2 * z2 - 30.0 * z2 * r2 + 3.0 * r2 * r2) * mpc[20]);
+}
+
+// Fills every ghost of a framed level.  Thread q enumerates the shell: the two k planes,
+// then the two j rows of each inner plane, then the two i ends of each inner row.
+__global__ void __launch_bounds__(BS)
+k_mb_bc(double *__restrict__ u, MPGeom G, const double *__restrict__ mp, int order) {
+  const int n1 = G.n1, n2 = G.n2, n3 = G.n3;
+  const unsigned w1 = n1 + 2, w2 = n2 + 2;
+  const unsigned np = 2 * w1 * w2, nr = 2 * (unsigned)n3 * w1, nc = 2 * (unsigned)n3 * n2;
+  unsigned q = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  int k, j, i;
+  if (q < np) {
+    const unsigned side = q / (w1 * w2), r = q - side * w1 * w2;
+    k = side ? n3 + 1 : 0; j = r / w1; i = r - j * w1;
+  } else if ((q -= np) < nr) {
+    const unsigned row = q / w1;
+    i = q - row * w1; k = 1 + row / 2; j = (row & 1) ? n2 + 1 : 0;
+  } else if ((q -= nr) < nc) {
+    const unsigned row = q / 2;
+    i = (q & 1) ? n1 + 1 : 0; k = 1 + row / n2; j = 1 + row % n2;
+  } else {
+    return;
+  }
+  // chain of boundary points from this ghost inwards (at most 3), then the interior cell
+  double px[3], py[3], pz[3];
+  int depth = 0, kk = k, jj = j, ii = i;
+  while (depth < 3) {
+    if (kk == 0 || kk == n3 + 1) {
+      px[depth] = mp_centre(G, 0, ii); py[depth] = mp_centre(G, 1, jj);
+      pz[depth] = mp_face(G, 2, kk == 0 ? 1 : n3 + 1);
+      kk = (kk == 0) ? 1 : n3;
+    } else if (jj == 0 || jj == n2 + 1) {
+      px[depth] = mp_centre(G, 0, ii); pz[depth] = mp_centre(G, 2, kk);
+      py[depth] = mp_face(G, 1, jj == 0 ? 1 : n2 + 1);
+      jj = (jj == 0) ? 1 : n2;
+    } else if (ii == 0 || ii == n1 + 1) {
+      py[depth] = mp_centre(G, 1, jj); pz[depth] = mp_centre(G, 2, kk);
+      px[depth] = mp_face(G, 0, ii == 0 ? 1 : n1 + 1);
+      ii = (ii == 0) ? 1 : n1;
+    } else {
+      break;
+    }
+    ++depth;
+  }
+  double v = u[BX(n1, n2, kk, jj, ii)];
+  for (int d = depth - 1; d >= 0; --d)
+    v = 2.0 * mp_phis(mp + 3, order, px[d] - mp[0], py[d] - mp[1], pz[d] - mp[2]) - v;
+  u[BX(n1, n2, k, j, i)] = v;
+}
+
+// red-black update of the framed level (MGGravity::Smooth, mg_gravity.cpp:220-273)
+__global__ void __launch_bounds__(BS)
+k_mb_smooth(double *__restrict__ u, const double *__restrict__ src, int n1, int n2, int n3, double dx2,
+            double isix, int color) {
+  const int nh = (n1 + 1) / 2;
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  if (t >= (unsigned)nh * n2 * n3) return;
+  const int h = t % nh, r = t / nh;
+  const int j = 1 + r % n2, k = 1 + r / n2;
+  const int i = 1 + 2 * h + ((color + k + j) & 1);   // (color + k + j) parity in framed indices
+  if (i > n1) return;
+  const unsigned id = BX(n1, n2, k, j, i), sj = n1 + 2, sk = (unsigned)(n1 + 2) * (n2 + 2);
+  u[id] = gs_value(u[id], u[id + sk], u[id + sj], u[id + 1], u[id - sk], u[id - sj], u[id - 1], src[id], dx2,
+                   isix);
+}
+
+__device__ __forceinline__ double mb_defect(const double *u, const double *src, unsigned id, unsigned sj,
+                                            unsigned sk, double idx2) {
+  return (6.0 * u[id] - u[id + sk] - u[id + sj] - u[id + 1] - u[id - sk] - u[id - sj] - u[id - 1]) * idx2 +
+         src[id];
+}
+
+// FAS: uold = u over the framed level (StoreOldData, multigrid.cpp:645-653), then
+// src -= (6u - sum of neighbours) idx2 on the interior (CalculateFASRHS,
+// mg_gravity.cpp:324-351)
+__global__ void __launch_bounds__(BS)
+k_mb_fas(const double *__restrict__ u, double *__restrict__ src, double *__restrict__ uold, int n1, int n2,
+         int n3, double idx2) {
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  const unsigned w1 = n1 + 2, w2 = n2 + 2;
+  if (t >= w1 * w2 * (unsigned)(n3 + 2)) return;
+  uold[t] = u[t];
+  const int i = t % w1, r = t / w1, j = r % w2, k = r / w2;
+  if (i < 1 || i > n1 || j < 1 || j > n2 || k < 1 || k > n3) return;
+  const unsigned sj = w1, sk = w1 * w2;
+  src[t] -= (6.0 * u[t] - u[t + sk] - u[t + sj] - u[t + 1] - u[t - sk] - u[t - sj] - u[t - 1]) * idx2;
+}
+
+// RestrictBlock (multigrid.cpp:331-355): defect average into the coarse source and, in
+// FAS, the average of u into the coarse u.  One thread per coarse interior cell.
+__global__ void __launch_bounds__(BS)
+k_mb_restrict(const double *__restrict__ u, const double *__restrict__ src, double *__restrict__ csrc,
+              double *__restrict__ cu, int n1, int n2, int n3, double idx2) {
+  const int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  if (t >= (unsigned)c1 * c2 * c3) return;
+  const int i = 1 + t % c1, r = t / c1, j = 1 + r % c2, k = 1 + r / c2;
+  const int fi = 2 * i - 1, fj = 2 * j - 1, fk = 2 * k - 1;
+  const unsigned sj = n1 + 2, sk = (unsigned)(n1 + 2) * (n2 + 2);
+#define F(kk, jj, ii) BX(n1, n2, kk, jj, ii)
+  const unsigned f[8] = {F(fk, fj, fi), F(fk, fj, fi + 1), F(fk, fj + 1, fi), F(fk, fj + 1, fi + 1),
+                         F(fk + 1, fj, fi), F(fk + 1, fj, fi + 1), F(fk + 1, fj + 1, fi), F(fk + 1, fj + 1, fi + 1)};
+#undef F
+  double d[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) d[q] = mb_defect(u, src, f[q], sj, sk, idx2);
+  const unsigned c = BX(c1, c2, k, j, i);
+  csrc[c] = 0.125 * (d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7]);
+  cu[c] = 0.125 * (u[f[0]] + u[f[1]] + u[f[2]] + u[f[3]] + u[f[4]] + u[f[5]] + u[f[6]] + u[f[7]]);
+}
+
+// ProlongateAndCorrect with FAS (multigrid.cpp:361-379, 733-829): the fine interior
+// gains the trilinear interpolation of (u - uold) of the framed coarse level.
+__global__ void __launch_bounds__(BS)
+k_mb_prolong(const double *__restrict__ cu, const double *__restrict__ cuold, double *__restrict__ fu,
+             int n1, int n2, int n3) {
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  if (t >= (unsigned)n1 * n2 * n3) return;
+  const int fi = 1 + t % n1, r = t / n1, fj = 1 + r % n2, fk = 1 + r / n2;
+  const int c1 = n1 >> 1, c2 = n2 >> 1;
+  const int i = (fi - 1) / 2 + 1, j = (fj - 1) / 2 + 1, k = (fk - 1) / 2 + 1;
+  const int di = ((fi - 1) & 1) ? 1 : -1, dj = ((fj - 1) & 1) ? 1 : -1, dk = ((fk - 1) & 1) ? 1 : -1;
+#define C(kk, jj, ii) (cu[BX(c1, c2, kk, jj, ii)] - cuold[BX(c1, c2, kk, jj, ii)])
+  fu[BX(n1, n2, fk, fj, fi)] +=
+      0.015625 * (27.0 * C(k, j, i) + C(k + dk, j + dj, i + di) + 9.0 * (C(k, j, i + di) + C(k, j + dj, i) + C(k + dk, j, i)) +
+                  3.0 * (C(k + dk, j + dj, i) + C(k + dk, j, i + di) + C(k, j + dj, i + di)));
+#undef C
+}
+
+// FMGProlongate (multigrid.cpp:838-1048) from the framed coarse level
+__global__ void __launch_bounds__(BS)
+k_mb_fmg_prolong(const double *__restrict__ cu, double *__restrict__ fu, int n1, int n2, int n3) {
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  if (t >= (unsigned)n1 * n2 * n3) return;
+  const int fi = 1 + t % n1, r = t / n1, fj = 1 + r % n2, fk = 1 + r / n2;
+  const int c1 = n1 >> 1, c2 = n2 >> 1;
+  const int i = (fi - 1) / 2 + 1, j = (fj - 1) / 2 + 1, k = (fk - 1) / 2 + 1;
+  const double wlo[3] = {5.0, 30.0, -3.0}, whi[3] = {-3.0, 30.0, 5.0};
+  const double *wk = ((fk - 1) & 1) ? whi : wlo;
+  const double *wj = ((fj - 1) & 1) ? whi : wlo;
+  const double *wi = ((fi - 1) & 1) ? whi : wlo;
+  double acc = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        const double term = (wk[a] * wj[b] * wi[e]) * cu[BX(c1, c2, k - 1 + a, j - 1 + b, i - 1 + e)];
+        acc = (a == 0 && b == 0 && e == 0) ? term : acc + term;
+      }
+  fu[BX(n1, n2, fk, fj, fi)] = acc / 32768.0;
+}
+
+// RestrictFMGSource for a framed level (source only)
+__global__ void __launch_bounds__(BS)
+k_mb_restrict_src(const double *__restrict__ src, double *__restrict__ csrc, int n1, int n2, int n3) {
+  const int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  if (t >= (unsigned)c1 * c2 * c3) return;
+  const int i = 1 + t % c1, r = t / c1, j = 1 + r % c2, k = 1 + r / c2;
+  const int fi = 2 * i - 1, fj = 2 * j - 1, fk = 2 * k - 1;
+#define S(kk, jj, ii) src[BX(n1, n2, kk, jj, ii)]
+  csrc[BX(c1, c2, k, j, i)] = 0.125 * (S(fk, fj, fi) + S(fk, fj, fi + 1) + S(fk, fj + 1, fi) + S(fk, fj + 1, fi + 1) +
+                                       S(fk + 1, fj, fi) + S(fk + 1, fj, fi + 1) + S(fk + 1, fj + 1, fi) +
+                                       S(fk + 1, fj + 1, fi + 1));
+#undef S
+}
+
+// squared defect of the framed finest level, partial sums per workgroup
+__global__ void __launch_bounds__(BS)
+k_mb_defect_partial(const double *__restrict__ u, const double *__restrict__ src, int n1, int n2, int n3,
+                    double idx2, double *__restrict__ part) {
+  __shared__ double sh[BS];
+  unsigned t0, t1;
+  wg_range((unsigned)n1 * n2 * n3, t0, t1);
+  const unsigned sj = n1 + 2, sk = (unsigned)(n1 + 2) * (n2 + 2);
+  double v = 0.0;
+  for (unsigned t = t0 + threadIdx.x; t < t1; t += BS) {
+    const int i = 1 + t % n1, r = t / n1, j = 1 + r % n2, k = 1 + r / n2;
+    const double d = mb_defect(u, src, BX(n1, n2, k, j, i), sj, sk, idx2);
+    v += d * d;
+  }
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Multipole moments of the finest source (Multigrid::CalculateMultipoleCoefficients,
+// multigrid.cpp:1054-1264) or, with com, the centre-of-mass sums
+// (Multigrid::CalculateCenterOfMass, multigrid.cpp:1270-1298); per-workgroup partials
+// part[q * gridDim.x + w] for moment q.
+constexpr int NMOM = 25;
+__global__ void __launch_bounds__(BS)
+k_mb_moments(const double *__restrict__ src, MPGeom G, const double *__restrict__ mp, int order, int com,
+             double *__restrict__ part) {
+  __shared__ double sh[BS];
+  const int n1 = G.n1, n2 = G.n2, n3 = G.n3;
+  unsigned t0, t1;
+  wg_range((unsigned)n1 * n2 * n3, t0, t1);
+  const double vol = (mp_face(G, 0, 2) - mp_face(G, 0, 1)) * (mp_face(G, 1, 2) - mp_face(G, 1, 1)) *
+                     (mp_face(G, 2, 2) - mp_face(G, 2, 1));
+  const double xo = com ? 0.0 : mp[0], yo = com ? 0.0 : mp[1], zo = com ? 0.0 : mp[2];
+  const int nm = com ? 4 : (order == 4 ? 25 : 9);
+  double m[NMOM];
+#pragma unroll
+  for (int q = 0; q < NMOM; ++q) m[q] = 0.0;
+  for (unsigned t = t0 + threadIdx.x; t < t1; t += BS) {
+    const int i = 1 + t % n1, r = t / n1, j = 1 + r % n2, k = 1 + r / n2;
+    const double z = mp_centre(G, 2, k) - zo, y = mp_centre(G, 1, j) - yo, x = mp_centre(G, 0, i) - xo;
+    const double s = src[BX(n1, n2, k, j, i)] * vol;
+    m[0] += s;
+    m[1] += s * y;
+    m[2] += s * z;
+    m[3] += s * x;
+    if (com) continue;
+    const double z2 = z * z, y2 = y * y, yz = y * z, x2 = x * x, xy = x * y, zx = z * x;
+    const double r2 = x2 + y2 + z2;
+    m[4] += s * xy;
+    m[5] += s * yz;
+    m[6] += s * (3.0 * z2 - r2);
+    m[7] += s * zx;
+    if (order == 2) {
+      m[8] += s * 0.5 * (x2 - y2);
+      continue;
+    }
+    const double hx2my2 = 0.5 * (x2 - y2), x2mty2 = x2 - 3.0 * y2, tx2my2 = 3.0 * x2 - y2;
+    const double fz2mr2 = 5.0 * z2 - r2, sz2mr2 = 7.0 * z2 - r2, sz2mtr2 = 7.0 * z2 - 3.0 * r2;
+    m[8] += s * hx2my2;
+    m[9] += s * y * tx2my2;
+    m[10] += s * xy * z;
+    m[11] += s * y * fz2mr2;
+    m[12] += s * z * (z2 - 3.0 * r2);
+    m[13] += s * x * fz2mr2;
+    m[14] += s * z * hx2my2;
+    m[15] += s * x * x2mty2;
+    m[16] += s * xy * hx2my2;
+    m[17] += s * yz * tx2my2;
+    m[18] += s * xy * sz2mr2;
+    m[19] += s * yz * sz2mtr2;
+    m[20] += s * (35.0 * z2 * z2 - 30.0 * z2 * r2 + 3.0 * r2 * r2);
+    m[21] += s * zx * sz2mtr2;
+    m[22] += s * hx2my2 * sz2mr2;
+    m[23] += s * zx * x2mty2;
+    m[24] += s * 0.125 * (x2 * x2mty2 - y2 * tx2my2);
+  }
+  for (int q = 0; q < nm; ++q) {
+    sh[threadIdx.x] = m[q];
+    __syncthreads();
+    for (int st = BS / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(unsigned)q * gridDim.x + blockIdx.x] = sh[0];
+    __syncthreads();
+  }
+}
+
+// Sums the partials in a fixed order; com: origin = (m3, m1, m2) / m0
+// (MultigridDriver::CalculateCenterOfMass, multigrid_driver.cpp:1984-2015); otherwise
+// the coefficients scaled as ScaleMultipoleCoefficients (multigrid_driver.cpp:1930-1981).
+__global__ void __launch_bounds__(BS)
+k_mb_moments_final(const double *__restrict__ part, int nwg, int order, int com, double *__restrict__ mp) {
+  __shared__ double sh[BS];
+  const int nm = com ? 4 : (order == 4 ? 25 : 9);
+  double acc[NMOM];
+  for (int q = 0; q < nm; ++q) {
+    double v = 0.0;
+    for (int w = threadIdx.x; w < nwg; w += BS) v += part[q * nwg + w];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = BS / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+      __syncthreads();
+    }
+    acc[q] = sh[0];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (com) {
+    const double im = 1.0 / acc[0];
+    mp[0] = im * acc[3];
+    mp[1] = im * acc[1];
+    mp[2] = im * acc[2];
+    return;
+  }
+  const double PI_ = 3.14159265358979323846;
+  const double c0 = -0.25 / PI_, c1 = -0.25 / PI_, c2 = -0.0625 / PI_, c2a = -0.75 / PI_;
+  const double c30 = -0.0625 / PI_, c31 = -0.0625 * 1.5 / PI_, c32 = -0.25 * 15.0 / PI_, c33 = -0.0625 * 2.5 / PI_;
+  const double c40 = -0.0625 * 0.0625 / PI_, c41 = -0.0625 * 2.5 / PI_, c42 = -0.0625 * 5.0 / PI_,
+               c43 = -0.0625 * 17.5 / PI_, c44 = -0.25 * 35.0 / PI_;
+  const double sc[NMOM] = {c0, c1, c1, c1, c2a, c2a, c2, c2a, c2a, c33, c32, c31, c30,
+                           c31, c32, c33, c44, c43, c42, c41, c40, c41, c42, c43, c44};
+  for (int q = 0; q < NMOM; ++q) mp[3 + q] = (q < nm) ? acc[q] * sc[q] : 0.0;
+}
+
+// LoadSource (multigrid.cpp:194-231) into the framed finest level; MGI mode also loads
+// u = phi (LoadFinestData, multigrid.cpp:160-180)
+__global__ void __launch_bounds__(BS)
+k_mb_load(Geo g, const double *__restrict__ u, const double *__restrict__ phi, double *__restrict__ src,
+          double *__restrict__ mu, double fac, int load_phi) {
+  const int N1 = g.nb1 * g.bx1, N2 = g.nb2 * g.bx2, N3 = g.nb3 * g.bx3;
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  if (t >= (unsigned)N1 * N2 * N3) return;
+  const int I = t % N1, r = t / N1, J = r % N2, K = r / N2;
+  const int b = ((K / g.bx3) * g.nb2 + (J / g.bx2)) * g.nb1 + (I / g.bx1);
+  const int64_t id = IDX(g, K % g.bx3 + g.ng, J % g.bx2 + g.ng, I % g.bx1 + g.ng);
+  const double v = u[(int64_t)b * NHYDRO * g.S + id];
+  const unsigned f = BX(N1, N2, K + 1, J + 1, I + 1);
+  src[f] = (fac == 1.0) ? v : v * fac;
+  if (load_phi) mu[f] = phi[(int64_t)b * g.S + id];
+}
+
+// RetrieveResult with one ghost layer (multigrid.cpp:269-288) from the framed finest level
+__global__ void __launch_bounds__(BS)
+k_mb_retrieve(Geo g, const double *__restrict__ mu, double *__restrict__ phi) {
+  const int m1 = g.bx1 + 2, m2 = g.bx2 + 2, m3 = g.bx3 + 2;
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  if (t >= (unsigned)m1 * m2 * m3 * g.nb) return;
+  const int li = t % m1;
+  unsigned r = t / m1;
+  const int lj = r % m2;
+  r /= m2;
+  const int lk = r % m3, b = r / m3;
+  const int N1 = g.nb1 * g.bx1, N2 = g.nb2 * g.bx2;
+  const int lx1 = b % g.nb1, lx2 = (b / g.nb1) % g.nb2, lx3 = b / (g.nb1 * g.nb2);
+  const int I = lx1 * g.bx1 + li, J = lx2 * g.bx2 + lj, K = lx3 * g.bx3 + lk;   // framed: global + 1
+  phi[(int64_t)b * g.S + IDX(g, lk - 1 + g.ng, lj - 1 + g.ng, li - 1 + g.ng)] = mu[BX(N1, N2, K, J, I)];
+}
+
 }  // namespace
 
 namespace agx {
@@ -467,6 +855,224 @@ template <typename T>
 static int64_t lcells(const MGLevelT<T> &L) { return (int64_t)L.n1 * L.n2 * L.n3; }
 static int64_t pad32(int64_t n) { return (n + 63) / 64 * 64; }   // 64 elements: >= 256 B
 static int64_t npart(int64_t n) { return std::min<int64_t>(grid1d(n, BS), NPART); }
+
+static double finish_norm(agx_mesh *m);
+
+// ---------------------------------------------------------------- multipole hierarchy
+static int64_t bcells(const MGLevelB &L) { return (int64_t)(L.n1 + 2) * (L.n2 + 2) * (L.n3 + 2); }
+static int64_t bghosts(const MGLevelB &L) { return bcells(L) - (int64_t)L.n1 * L.n2 * L.n3; }
+
+// Framed levels with the same dimensions and smoother cell sizes as the periodic ones.
+static int mp_create(agx_mesh *m) {
+  const int L = m->ntotal;
+  m->levb.resize(L);
+  int64_t tot = 0;
+  for (int l = 0; l < L; ++l) {
+    MGLevelB &B = m->levb[l];
+    B.n1 = m->lev[l].n1; B.n2 = m->lev[l].n2; B.n3 = m->lev[l].n3;
+    B.dx = m->lev[l].dx;
+    B.h[0] = (m->d.x1max - m->d.x1min) / B.n1;
+    B.h[1] = (m->d.x2max - m->d.x2min) / B.n2;
+    B.h[2] = (m->d.x3max - m->d.x3min) / B.n3;
+    tot += 3 * pad32(bcells(B));
+  }
+  AGX_HIP(hipMalloc(&m->mgpoolb, tot * (int64_t)sizeof(double)));
+  AGX_HIP(hipMemsetAsync(m->mgpoolb, 0, tot * sizeof(double), m->stream));
+  m->mg_bytes += tot * (int64_t)sizeof(double);
+  double *p = m->mgpoolb;
+  for (int l = 0; l < L; ++l) {
+    MGLevelB &B = m->levb[l];
+    const int64_t n = pad32(bcells(B));
+    B.u = p; p += n;
+    B.src = p; p += n;
+    B.uold = p; p += n;
+  }
+  AGX_HIP(hipMalloc(&m->mpdev, sizeof(double) * (28 + NMOM * NPART)));
+  AGX_HIP(hipMemsetAsync(m->mpdev, 0, sizeof(double) * (28 + NMOM * NPART), m->stream));
+  m->mp_on = true;
+  return 0;
+}
+
+static MPGeom mp_geom(const agx_mesh *m, const MGLevelB &B) {
+  MPGeom G;
+  G.n1 = B.n1; G.n2 = B.n2; G.n3 = B.n3;
+  G.xmin[0] = m->d.x1min; G.xmin[1] = m->d.x2min; G.xmin[2] = m->d.x3min;
+  G.xmax[0] = m->d.x1max; G.xmax[1] = m->d.x2max; G.xmax[2] = m->d.x3max;
+  for (int d = 0; d < 3; ++d) G.h[d] = B.h[d];
+  return G;
+}
+
+static void bc_b(agx_mesh *m, int l) {
+  MGLevelB &B = m->levb[l];
+  timer_begin(m, AGX_K_MG_OTHER);
+  hipLaunchKernelGGL(k_mb_bc, dim3(grid1d(bghosts(B), BS)), dim3(BS), 0, m->stream, B.u, mp_geom(m, B), m->mpdev,
+                     m->d.mporder);
+  timer_end(m);
+}
+static void smooth_b(agx_mesh *m, int l, int color) {
+  MGLevelB &B = m->levb[l];
+  const int64_t n = (int64_t)((B.n1 + 1) / 2) * B.n2 * B.n3;
+  timer_begin(m, AGX_K_MG_SMOOTH);
+  hipLaunchKernelGGL(k_mb_smooth, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, B.u, B.src, B.n1, B.n2, B.n3,
+                     B.dx * B.dx, 1.15 / 6.0, color ^ m->coffset);
+  timer_end(m);
+}
+static void fas_b(agx_mesh *m, int l) {
+  MGLevelB &B = m->levb[l];
+  timer_begin(m, AGX_K_MG_OTHER);
+  hipLaunchKernelGGL(k_mb_fas, dim3(grid1d(bcells(B), BS)), dim3(BS), 0, m->stream, B.u, B.src, B.uold, B.n1,
+                     B.n2, B.n3, 1.0 / (B.dx * B.dx));
+  timer_end(m);
+}
+
+// SolveVCycle(1,1) (multigrid_driver.cpp:684-693) with the FAS steps of
+// OneStepToCoarser / SolveCoarsestGrid / OneStepToFiner (:583-677, :778-819) and the
+// MeshBlock task lists (mg_task_list.cpp:258-262, 287-510).  Boundaries are filled before
+// every stencil step, as the reference's exchanges and ApplyPhysicalBoundaries are.
+static void vcycle_b(agx_mesh *m) {
+  int start = m->current;
+  m->coffset ^= 1;
+  while (m->current > 0) {
+    const int l = m->current;
+    MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
+    bc_b(m, l);
+    if (l < m->fmglevel) fas_b(m, l);
+    smooth_b(m, l, 0);
+    bc_b(m, l);
+    smooth_b(m, l, 1);
+    bc_b(m, l);
+    timer_begin(m, AGX_K_MG_RESTRICT);
+    hipLaunchKernelGGL(k_mb_restrict, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream,
+                       F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
+    timer_end(m);
+    m->current--;
+  }
+  // SolveCoarsestGrid with FAS (no average subtraction): ni sweeps on the root level
+  const Geo &g = m->g;
+  int ni = std::max(g.nb1, std::max(g.nb2, g.nb3)) >> (m->nrootlevel - 1);
+  bc_b(m, 0);
+  fas_b(m, 0);
+  for (int it = 0; it < ni; ++it) {
+    smooth_b(m, 0, 0);
+    bc_b(m, 0);
+    smooth_b(m, 0, 1);
+    bc_b(m, 0);
+  }
+  while (m->current < start) {
+    const int l = m->current + 1;
+    MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
+    bc_b(m, l - 1);
+    timer_begin(m, AGX_K_MG_PROLONG);
+    hipLaunchKernelGGL(k_mb_prolong, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, C.u,
+                       C.uold, F.u, F.n1, F.n2, F.n3);
+    timer_end(m);
+    bc_b(m, l);
+    smooth_b(m, l, 0);
+    bc_b(m, l);
+    smooth_b(m, l, 1);
+    bc_b(m, l);
+    m->current++;
+  }
+}
+
+// SolveFMGCycle (multigrid_driver.cpp:700-714) after RestrictFMGSource
+static void fmg_b(agx_mesh *m) {
+  const int L = m->ntotal;
+  for (int l = L - 1; l > 0; --l) {
+    MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
+    hipLaunchKernelGGL(k_mb_restrict_src, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream,
+                       F.src, C.src, F.n1, F.n2, F.n3);
+  }
+  m->current = 0;
+  for (m->fmglevel = 0; m->fmglevel < L; m->fmglevel++) {
+    vcycle_b(m);
+    if (m->fmglevel != L - 1) {   // FMGProlongate, multigrid_driver.cpp:553-575
+      const int l = m->current + 1;
+      MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
+      bc_b(m, l - 1);
+      timer_begin(m, AGX_K_MG_PROLONG);
+      hipLaunchKernelGGL(k_mb_fmg_prolong, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream,
+                         C.u, F.u, F.n1, F.n2, F.n3);
+      timer_end(m);
+      m->current++;
+    }
+  }
+  m->fmglevel = L - 1;
+}
+
+// defect norm of the framed finest level (reads its ghost frame as it stands)
+static double defect_norm_b(agx_mesh *m) {
+  MGLevelB &F = m->levb[m->ntotal - 1];
+  const int64_t nb = npart((int64_t)F.n1 * F.n2 * F.n3);
+  hipLaunchKernelGGL(k_mb_defect_partial, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.u, F.src, F.n1, F.n2,
+                     F.n3, 1.0 / (F.dx * F.dx), m->red);
+  hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->red, nb, m->scal);
+  return finish_norm(m);
+}
+
+// MultipoleCoefficients of the finest source (SetupMultigrid, multigrid_driver.cpp:441-445)
+static void mp_coefficients(agx_mesh *m) {
+  MGLevelB &F = m->levb[m->ntotal - 1];
+  const int64_t nb = npart((int64_t)F.n1 * F.n2 * F.n3);
+  const MPGeom G = mp_geom(m, F);
+  double *part = m->mpdev + 28;
+  if (m->d.mp_auto_origin) {
+    hipLaunchKernelGGL(k_mb_moments, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.src, G, m->mpdev,
+                       m->d.mporder, 1, part);
+    hipLaunchKernelGGL(k_mb_moments_final, dim3(1), dim3(BS), 0, m->stream, part, (int)nb, m->d.mporder, 1,
+                       m->mpdev);
+  } else {
+    (void)hipMemcpyAsync(m->mpdev, m->d.mp_origin, 3 * sizeof(double), hipMemcpyHostToDevice, m->stream);
+  }
+  hipLaunchKernelGGL(k_mb_moments, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.src, G, m->mpdev, m->d.mporder,
+                     0, part);
+  hipLaunchKernelGGL(k_mb_moments_final, dim3(1), dim3(BS), 0, m->stream, part, (int)nb, m->d.mporder, 0,
+                     m->mpdev);
+}
+
+// MGGravityDriver::Solve (mg_gravity.cpp:166-211) with multipole boundaries
+static int mg_solve_b(agx_mesh *m) {
+  const Geo &g = m->g;
+  const int L = m->ntotal;
+  MGLevelB &F = m->levb[L - 1];
+  double **A = m->arr.a;
+  hipLaunchKernelGGL(k_mb_load, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, g,
+                     A[AGX_ARR_U], A[AGX_ARR_PHI], F.src, F.u, m->d.four_pi_G, m->mg_mode == 1 ? 1 : 0);
+  m->current = L - 1;
+  m->fmglevel = L - 1;
+  mp_coefficients(m);
+  if (m->mg_mode == 0) fmg_b(m);
+  m->current = L - 1;
+  int iters = 0;
+  if (m->mg_eps >= 0.0) {   // SolveIterative, multigrid_driver.cpp:721-755
+    double def = defect_norm_b(m);
+    int n = 0;
+    while (def > m->mg_eps) {
+      vcycle_b(m);
+      iters++;
+      double olddef = def;
+      def = defect_norm_b(m);
+      if (def / olddef > 0.8) {
+        if (m->mg_eps == 0.0) break;
+      }
+      if (n > 100) break;
+      n++;
+    }
+    m->last_norm = def;
+  } else {   // SolveIterativeFixedTimes, multigrid_driver.cpp:762-771
+    for (int n = 0; n < m->mg_niter; ++n) {
+      vcycle_b(m);
+      iters++;
+    }
+    m->last_norm = defect_norm_b(m);
+  }
+  m->last_iters = iters;
+  const int64_t nret = (int64_t)(g.bx1 + 2) * (g.bx2 + 2) * (g.bx3 + 2) * g.nb;
+  hipLaunchKernelGGL(k_mb_retrieve, dim3(grid1d(nret, BS)), dim3(BS), 0, m->stream, g, F.u, A[AGX_ARR_PHI]);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "multipole mg_solve launches");
+  return 0;
+}
 
 int mg_create(agx_mesh *m) {
   const agx_mesh_desc &d = m->d;
@@ -551,12 +1157,26 @@ int mg_create(agx_mesh *m) {
     m->red_n = need;
     AGX_HIP(hipMalloc(&m->red, sizeof(double) * m->red_n));
   }
+  if (d.mg_bc == 1) {
+    if (m->mixed) {
+      set_error("multipole multigrid boundaries run in FP64 only (precision = 0)");
+      return -22;
+    }
+    int rc = mp_create(m);
+    if (rc) return rc;
+  }
   AGX_HIP(hipStreamSynchronize(m->stream));
   m->mg_on = 1;
   return 0;
 }
 
 void mg_destroy(agx_mesh *m) {
+  if (m->mgpoolb) hipFree(m->mgpoolb);
+  if (m->mpdev) hipFree(m->mpdev);
+  m->mgpoolb = nullptr;
+  m->mpdev = nullptr;
+  m->levb.clear();
+  m->mp_on = false;
   if (m->mgpool) hipFree(m->mgpool);
   if (m->mgpool32) hipFree(m->mgpool32);
   m->mgpool = nullptr;
@@ -811,6 +1431,7 @@ static void correction_step(agx_mesh *m) {
 
 int mg_solve(agx_mesh *m, int stage) {
   (void)stage;
+  if (m->mp_on) return mg_solve_b(m);
   const Geo &g = m->g;
   int L = m->ntotal;
   MGLevel &F = m->lev[L - 1];
@@ -878,6 +1499,11 @@ int mg_solve(agx_mesh *m, int stage) {
 // defect-correction step (FP64 defect, FP32 V-cycle, FP64 correction).
 int mg_vcycles(agx_mesh *m, int n) {
   m->current = m->ntotal - 1;
+  if (m->mp_on) {
+    for (int q = 0; q < n; ++q) vcycle_b(m);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, "mg_vcycles launches");
+  }
   for (int q = 0; q < n; ++q) {
     if (m->mixed) { defect_to32(m); correction_step(m); }
     else vcycle(m, m->lev);
