@@ -109,6 +109,107 @@ k_mg_smooth_pairs(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, 
   u2[row + h] = c;
 }
 
+// Both sweeps of one red-black pass (red = colour c0 first, then black) in one launch:
+// reads uin, writes uout, so every red update sees the old black values and every black
+// update the new red ones, as two in-place sweeps do.  A workgroup owns a TI x TJ tile
+// of (i,j) columns and a chunk of KC planes, walking the planes upwards: it keeps the
+// old values of three planes of (tile + 2-cell ring) and the new red values of three
+// planes of (tile + 1-cell ring) in LDS, and writes plane q-1 of its tile (red copied,
+// black updated) after computing the red values of plane q.  Red cells of the ring and
+// of the planes just outside the chunk are recomputed by every workgroup that needs
+// them, from the same inputs.  Periodic wrap in all directions; every extent even.
+// Arithmetic: gs_value, the operands of k_mg_smooth.
+constexpr int RB_TI = 64, RB_TJ = 8, RB_KC = 16;
+template <typename T>
+__global__ void __launch_bounds__(RB_TI * RB_TJ)
+k_mg_smooth_rb(const T *__restrict__ uin, T *__restrict__ uout, const T *__restrict__ src, int n1, int n2,
+               int n3, T dx2, T isix, int c0) {
+  constexpr int TI = RB_TI, TJ = RB_TJ, NT = TI * TJ, OI = TI + 4, OJ = TJ + 4, RI = TI + 2, RJ = TJ + 2;
+  __shared__ T s_old[3][OJ][OI];
+  __shared__ T s_red[3][RJ][RI];
+  const int tid = threadIdx.x;
+  const int nti = n1 / TI, ntj = n2 / TJ;
+  const unsigned wg = xcd_slot(blockIdx.x, gridDim.x);
+  const int tile = wg % (unsigned)(nti * ntj), chunk = wg / (unsigned)(nti * ntj);
+  const int i0 = (tile % nti) * TI, j0 = (tile / nti) * TJ;
+  const int k0 = chunk * RB_KC, k1 = min(k0 + RB_KC, n3);   // black/written planes [k0, k1)
+  auto slot = [](int q) { return ((q % 3) + 3) % 3; };
+  auto wr = [](int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); };
+  auto load_old = [&](int q) {
+    const int kq = wr(q, n3), sl = slot(q);
+    for (int t = tid; t < OI * OJ; t += NT) {
+      const int jj = t / OI, ii = t - jj * OI;
+      s_old[sl][jj][ii] = uin[LIX(n1, n2, kq, wr(j0 - 2 + jj, n2), wr(i0 - 2 + ii, n1))];
+    }
+  };
+  auto red_plane = [&](int q) {
+    const int kq = wr(q, n3), so = slot(q), sm = slot(q - 1), sp = slot(q + 1);
+    for (int t = tid; t < RI * RJ; t += NT) {
+      const int jj = t / RI, ii = t - jj * RI;
+      const int gj = wr(j0 - 1 + jj, n2), gi = wr(i0 - 1 + ii, n1);
+      if ((gi + gj + kq + c0) & 1) continue;   // black cell
+      s_red[so][jj][ii] = gs_value(s_old[so][jj + 1][ii + 1], s_old[sp][jj + 1][ii + 1], s_old[so][jj + 2][ii + 1],
+                                   s_old[so][jj + 1][ii + 2], s_old[sm][jj + 1][ii + 1], s_old[so][jj][ii + 1],
+                                   s_old[so][jj + 1][ii], src[LIX(n1, n2, kq, gj, gi)], dx2, isix);
+    }
+  };
+  auto black_plane = [&](int q) {   // q in [k0, k1): writes the tile of plane q
+    const int so = slot(q), sm = slot(q - 1), sp = slot(q + 1);
+    for (int t = tid; t < NT; t += NT) {
+      const int jj = t / TI, ii = t - jj * TI;
+      const int gj = j0 + jj, gi = i0 + ii;
+      const int64_t id = LIX(n1, n2, q, gj, gi);
+      T v;
+      if (((gi + gj + q + c0) & 1) == 0) {
+        v = s_red[so][jj + 1][ii + 1];
+      } else {
+        v = gs_value(s_old[so][jj + 2][ii + 2], s_red[sp][jj + 1][ii + 1], s_red[so][jj + 2][ii + 1],
+                     s_red[so][jj + 1][ii + 2], s_red[sm][jj + 1][ii + 1], s_red[so][jj][ii + 1],
+                     s_red[so][jj + 1][ii], src[id], dx2, isix);
+      }
+      uout[id] = v;
+    }
+  };
+  // the old values of plane q+1 are fetched into registers one iteration ahead, so their
+  // latency overlaps the red and black updates of the previous iteration
+  constexpr int NPF = (OI * OJ + NT - 1) / NT;
+  T pf[NPF];
+  auto fetch = [&](int q) {
+    const int kq = wr(q, n3);
+#pragma unroll
+    for (int r = 0; r < NPF; ++r) {
+      const int t = tid + r * NT;
+      if (t < OI * OJ) {
+        const int jj = t / OI, ii = t - jj * OI;
+        pf[r] = uin[LIX(n1, n2, kq, wr(j0 - 2 + jj, n2), wr(i0 - 2 + ii, n1))];
+      }
+    }
+  };
+  auto stash = [&](int q) {
+    const int sl = slot(q);
+#pragma unroll
+    for (int r = 0; r < NPF; ++r) {
+      const int t = tid + r * NT;
+      if (t < OI * OJ) {
+        const int jj = t / OI, ii = t - jj * OI;
+        s_old[sl][jj][ii] = pf[r];
+      }
+    }
+  };
+  load_old(k0 - 2);
+  load_old(k0 - 1);
+  fetch(k0);
+  for (int q = k0 - 1; q <= k1; ++q) {
+    stash(q + 1);
+    if (q + 2 <= k1 + 1) fetch(q + 2);
+    __syncthreads();
+    red_plane(q);
+    __syncthreads();
+    if (q - 1 >= k0) black_plane(q - 1);
+    __syncthreads();
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T defect_at(const T *u, const T *src, int n1, int n2,
                                        int n3, int k, int j, int i, T idx2) {
@@ -1137,7 +1238,7 @@ int mg_create(agx_mesh *m) {
     // FP32 hierarchy of the correction V-cycles: u (correction) and src per level
     m->lev32.resize(L);
     int64_t t32 = 0;
-    for (int l = 0; l < L; ++l) t32 += 2 * pad32(lcells(m->lev[l]));
+    for (int l = 0; l < L; ++l) t32 += 3 * pad32(lcells(m->lev[l]));
     AGX_HIP(hipMalloc(&m->mgpool32, t32 * (int64_t)sizeof(float)));
     AGX_HIP(hipMemsetAsync(m->mgpool32, 0, t32 * sizeof(float), m->stream));
     m->mg_bytes += t32 * (int64_t)sizeof(float);
@@ -1148,7 +1249,7 @@ int mg_create(agx_mesh *m) {
       int64_t n = pad32(lcells(Lv));
       Lv.u = q; q += n;
       Lv.src = q; q += n;
-      Lv.def = nullptr;
+      Lv.def = q; q += n;   // spare array of the fused red-black pass
     }
   }
   int64_t need = (lcells(m->lev[L - 1]) + BS - 1) / BS + 1;
@@ -1203,6 +1304,30 @@ static void smooth(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l, int color) {
     hipLaunchKernelGGL(k_mg_smooth<T>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, L.u, L.src, L.n1, L.n2,
                        L.n3, T(dx2), T(isix), color ^ m->coffset);
   timer_end(m);
+}
+
+// One red-black pass (colour 0 then colour 1) on level l: fused into one launch into the
+// level's spare array (then swapped with u) on FP64 levels of at least 2^26 cells, two
+// sweeps otherwise.  Measured on MI355X (V(1,1) cycles/s, two sweeps -> fused): 512^3
+// FP64 186 -> 231; at 256^3 (134 MB per level array, which the 256 MB last-level cache
+// holds between the two sweeps) two sweeps are faster (1746 vs 1600), as they are for
+// FP32 levels at 512^3 (231 vs 213).
+template <typename T>
+static void smooth_pass(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {
+  MGLevelT<T> &L = H[l];
+  const bool fused = sizeof(T) == 8 && L.def && L.n1 % RB_TI == 0 && L.n2 % RB_TJ == 0 && (L.n3 & 1) == 0 &&
+                     L.n3 >= 4 && lcells(L) >= ((int64_t)1 << 26);
+  if (!fused) {
+    smooth(m, H, l, 0);
+    smooth(m, H, l, 1);
+    return;
+  }
+  const int64_t nwg = (int64_t)(L.n1 / RB_TI) * (L.n2 / RB_TJ) * ((L.n3 + RB_KC - 1) / RB_KC);
+  timer_begin(m, AGX_K_MG_SMOOTH);
+  hipLaunchKernelGGL(k_mg_smooth_rb<T>, dim3((unsigned)nwg), dim3(RB_TI * RB_TJ), 0, m->stream, L.u, L.def,
+                     L.src, L.n1, L.n2, L.n3, T(L.dx * L.dx), T(1.15 / 6.0), m->coffset);
+  timer_end(m);
+  std::swap(L.u, L.def);
 }
 
 template <typename T>
@@ -1349,8 +1474,7 @@ static void vcycle(agx_mesh *m, std::vector<MGLevelT<T>> &H) {
   const int ct = coarse_top(m, H);
   while (m->current > 0 && m->current > ct) {
     int l = m->current;
-    smooth(m, H, l, 0);
-    smooth(m, H, l, 1);
+    smooth_pass(m, H, l);
     defect_restrict(m, H, l);
     m->current--;
   }
@@ -1359,8 +1483,7 @@ static void vcycle(agx_mesh *m, std::vector<MGLevelT<T>> &H) {
   while (m->current < start) {
     int l = m->current + 1;
     prolong(m, H, l);
-    smooth(m, H, l, 0);
-    smooth(m, H, l, 1);
+    smooth_pass(m, H, l);
     m->current++;
   }
 }

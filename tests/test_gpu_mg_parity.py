@@ -62,3 +62,44 @@ def test_absolute_threshold_iterations():
     assert ig == io
     assert ng <= 1e-9 and no <= 1e-9
     assert np.abs(pg - po).max() <= 1e-12 * np.abs(po).max()
+
+
+
+def test_fused_red_black_levels():
+    """FP64 levels of >= 2^26 cells run both sweeps of each red-black pass in one launch
+    (k_mg_smooth_rb); 512 x 512 x 256 in MeshBlocks of 256^3 is the smallest such mesh
+    with a cubic-MeshBlock hierarchy.  Fixed V-cycle counts: the oracle's iterate
+    sequence, same bar."""
+    agx = load_agx()
+    N1, N3, B = 512, 256, 256
+    d = make_desc(nx=(N1, N1, N3), bnx=(B, B, B), box=((0, 1), (0, 1), (0, 0.5)), self_gravity=1,
+                  mg_mode=0, mg_threshold=-1.0, mg_niter=1, four_pi_G=1.0)
+    x = (np.arange(N1) + 0.5) / N1
+    z = (np.arange(N3) + 0.5) / N1
+    Z, Y, X = np.meshgrid(z, x, x, indexing="ij")
+    rho = 1.0 + 20.0 * np.exp(-((X - 0.3) ** 2 + (Y - 0.6) ** 2 + (Z - 0.2) ** 2) / 0.05 ** 2) \
+        + 5.0 * np.exp(-((X - 0.7) ** 2 + (Y - 0.25) ** 2 + (Z - 0.35) ** 2) / 0.08 ** 2)
+    U = np.zeros((5, N3, N1, N1))
+    U[0] = rho
+    ub = agx.scatter_cells(d, U)
+    del U
+    o = Oracle(d)
+    o.set("u", ub)
+    o.mg_solve(0)
+    m = agx.Mesh(d)
+    m.set("u", ub)
+    m.mg_solve(0)
+    a, b = m.get("phi"), o.get("phi")
+    # the source mean and the final mean of u are sums over 2^26 cells in another order
+    # (a constant shift of phi); the rest of the difference is held to the bar
+    diff = a - b
+    offset = diff.mean()
+    scale = np.abs(b).max()
+    err = np.abs(diff - offset).max() / scale
+    (ng, ig), (no, io) = m.mg_last(), o.mg_last()
+    print(f"512x512x256: defect {ng:.6e} vs {no:.6e}; constant offset {offset / scale:.3e}, "
+          f"max|dphi - offset|/max|phi| {err:.3e}", flush=True)
+    assert abs(offset) <= 1e-11 * scale
+    assert err <= 1e-12
+    assert ig == io == 1
+    assert abs(ng - no) <= 1e-9 * no
