@@ -383,9 +383,10 @@ __device__ __forceinline__ void coarse_smooth(const CoarseLevels<T> &L, int l, i
 }
 
 template <typename T>
-__global__ void __launch_bounds__(COARSE_BS) k_mg_coarse(CoarseLevels<T> L, int top, int coffset, T isix) {
+__global__ void __launch_bounds__(COARSE_BS)
+k_mg_coarse(CoarseLevels<T> L, int top, int coffset, T isix, int first_sweep_done) {
   for (int l = top; l > 0; --l) {
-    coarse_smooth(L, l, 0 ^ coffset, isix);
+    if (!(first_sweep_done && l == top)) coarse_smooth(L, l, 0 ^ coffset, isix);
     coarse_smooth(L, l, 1 ^ coffset, isix);
     const int c1 = L.n1[l - 1], c2 = L.n2[l - 1], c3 = L.n3[l - 1];
     for (int t = threadIdx.x; t < c1 * c2 * c3; t += COARSE_BS) {
@@ -484,22 +485,37 @@ __global__ void __launch_bounds__(BS) k_mg_zero(T *__restrict__ a, int64_t n) {
 
 // Mixed-precision defect correction, FP64 side.  The FP64 defect d = src - L(phi)
 // (the form of MGGravity::CalculateDefect) becomes the FP32 source of the correction
-// problem L(e) = d, and its squares are summed in FP64 for the defect norm.
+// problem L(e) = d, and its squares are summed in FP64 for the defect norm.  With ein,
+// phi is first corrected by the FP32 correction of the previous cycle (phi + e, into
+// uout, read back by no other thread of this launch), so the correction and the next
+// defect take one pass.  e0 (not ein) receives the start of the next correction V-cycle: its first
+// sweep (colour cred) from e = 0 gives gs_value(0, ..., 0, d) at those cells and 0
+// elsewhere, so that cycle starts with its second sweep.
 __global__ void __launch_bounds__(BS)
-k_mg_defect_to32(const double *__restrict__ u, const double *__restrict__ src, float *__restrict__ d32,
-                 int n1, int n2, int n3, double idx2, double *__restrict__ part) {
+k_mg_defect_to32(const double *__restrict__ u, const float *__restrict__ ein, double *__restrict__ uout,
+                 const double *__restrict__ src, float *__restrict__ d32, float *__restrict__ e0, int n1, int n2,
+                 int n3, double idx2, float dx2f, float isixf, int cred, double *__restrict__ part) {
   __shared__ double sh[BS];
   const unsigned ncell = (unsigned)n1 * n2 * n3;   // < 2^32 cells per level
   unsigned t0, t1;
   wg_range(ncell, t0, t1);
   double v = 0.0;
+  auto phi = [&](int64_t q) { return ein ? u[q] + (double)ein[q] : u[q]; };
   for (unsigned t = t0 + threadIdx.x; t < t1; t += BS) {
-    int i = (int)(t % (unsigned)n1);
-    unsigned r = t / (unsigned)n1;
-    int j = (int)(r % (unsigned)n2);
-    int k = (int)(r / (unsigned)n2);
-    double d = defect_at(u, src, n1, n2, n3, k, j, i, idx2);
-    d32[t] = (float)d;
+    const int i = (int)(t % (unsigned)n1);
+    const unsigned r = t / (unsigned)n1;
+    const int j = (int)(r % (unsigned)n2);
+    const int k = (int)(r / (unsigned)n2);
+    const double uc = phi(t);
+    const double d = (6.0 * uc - phi(LIX(n1, n2, wrapp(k, n3), j, i)) - phi(LIX(n1, n2, k, wrapp(j, n2), i)) -
+                      phi(LIX(n1, n2, k, j, wrapp(i, n1))) - phi(LIX(n1, n2, wrapm(k, n3), j, i)) -
+                      phi(LIX(n1, n2, k, wrapm(j, n2), i)) - phi(LIX(n1, n2, k, j, wrapm(i, n1)))) * idx2 + src[t];
+    if (uout) uout[t] = uc;
+    const float df = (float)d;
+    d32[t] = df;
+    e0[t] = (((i + j + k + cred) & 1) == 0)
+                ? gs_value(0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, df, dx2f, isixf)
+                : 0.0f;
     v += d * d;
   }
   sh[threadIdx.x] = v;
@@ -1452,7 +1468,7 @@ static int coarse_top(agx_mesh *m, const std::vector<MGLevelT<T>> &H) {
 }
 
 template <typename T>
-static void coarse_cycle(agx_mesh *m, std::vector<MGLevelT<T>> &H, int top) {
+static void coarse_cycle(agx_mesh *m, std::vector<MGLevelT<T>> &H, int top, bool first_sweep_done) {
   CoarseLevels<T> c;
   for (int l = 0; l <= top; ++l) {
     c.u[l] = H[l].u;
@@ -1463,22 +1479,23 @@ static void coarse_cycle(agx_mesh *m, std::vector<MGLevelT<T>> &H, int top) {
   }
   timer_begin(m, AGX_K_MG_SMOOTH);
   hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(COARSE_BS), 0, m->stream, c, top, m->coffset,
-                     T(1.15 / 6.0));
+                     T(1.15 / 6.0), first_sweep_done ? 1 : 0);
   timer_end(m);
 }
 
 template <typename T>
-static void vcycle(agx_mesh *m, std::vector<MGLevelT<T>> &H) {
+static void vcycle(agx_mesh *m, std::vector<MGLevelT<T>> &H, bool first_sweep_done = false) {
   int start = m->current;
   m->coffset ^= 1;
   const int ct = coarse_top(m, H);
   while (m->current > 0 && m->current > ct) {
     int l = m->current;
-    smooth_pass(m, H, l);
+    if (first_sweep_done && l == start) smooth(m, H, l, 1);
+    else smooth_pass(m, H, l);
     defect_restrict(m, H, l);
     m->current--;
   }
-  if (ct >= 0) coarse_cycle(m, H, m->current);   // down to level 0 and back to current
+  if (ct >= 0) coarse_cycle(m, H, m->current, first_sweep_done && m->current == start);
   else solve_coarsest(m, H);
   while (m->current < start) {
     int l = m->current + 1;
@@ -1525,31 +1542,35 @@ static double defect_norm(agx_mesh *m) {
 }
 
 // Mixed precision: FP64 defect of the finest iterate into the FP32 finest source, its
-// squared sum into scal[0]; returns the launch count of partials.
-static void defect_to32(agx_mesh *m) {
+// squared sum into scal[0], and the first sweep of the next correction V-cycle into the
+// FP32 correction (k_mg_defect_to32).  With correct, phi += e of the cycle just run is
+// applied in the same pass (into the spare array, then swapped).
+static void defect_to32(agx_mesh *m, bool correct) {
   int l = m->ntotal - 1;
   MGLevel &F = m->lev[l];
   MGLevel32 &F32 = m->lev32[l];
   int64_t nb = npart(lcells(F));
+  const int cred = 0 ^ (m->coffset ^ 1);   // colour of the next V-cycle's first sweep
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mg_defect_to32, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.u, F.src, F32.src, F.n1,
-                     F.n2, F.n3, 1.0 / (F.dx * F.dx), m->red);
+  // e of the last cycle is read at neighbour cells, so the next cycle's start goes to the
+  // spare FP32 array (then swapped), as the corrected phi goes to the FP64 one
+  hipLaunchKernelGGL(k_mg_defect_to32, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.u,
+                     correct ? F32.u : (const float *)nullptr, correct ? F.def : (double *)nullptr, F.src, F32.src,
+                     F32.def, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx), float(F32.dx * F32.dx), float(1.15 / 6.0), cred,
+                     m->red);
   hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->red, nb, m->scal);
   timer_end(m);
+  if (correct) std::swap(F.u, F.def);
+  std::swap(F32.u, F32.def);
 }
 
-// One defect-correction step: FP32 V(1,1) on L(e) = d from e = 0, then phi += e.  The
-// FP32 source must hold the current defect (defect_to32).
+// One defect-correction step: FP32 V(1,1) on L(e) = d from e = 0 (its first sweep
+// already applied by defect_to32), then phi += e together with the next defect.  The
+// FP32 source and correction must come from defect_to32.
 static void correction_step(agx_mesh *m) {
-  int l = m->ntotal - 1;
-  MGLevel &F = m->lev[l];
-  MGLevel32 &F32 = m->lev32[l];
-  zero_level(m, m->lev32, l);
-  m->current = l;
-  vcycle(m, m->lev32);
-  timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mg_mix, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, F.u, F32.u, lcells(F), 0);
-  timer_end(m);
+  m->current = m->ntotal - 1;
+  vcycle(m, m->lev32, true);
+  defect_to32(m, true);
 }
 
 int mg_solve(agx_mesh *m, int stage) {
@@ -1583,7 +1604,7 @@ int mg_solve(agx_mesh *m, int stage) {
   int iters = 0;
   if (m->mg_eps >= 0.0) {   // SolveIterative, multigrid_driver.cpp:721-755
     double def;
-    if (mixed) { defect_to32(m); def = finish_norm(m); }
+    if (mixed) { defect_to32(m, false); def = finish_norm(m); }
     else def = defect_norm(m);
     int n = 0;
     while (def > m->mg_eps) {
@@ -1591,7 +1612,7 @@ int mg_solve(agx_mesh *m, int stage) {
       else vcycle(m, m->lev);
       iters++;
       double olddef = def;
-      if (mixed) { defect_to32(m); def = finish_norm(m); }
+      if (mixed) def = finish_norm(m);   // the defect came with the correction
       else def = defect_norm(m);
       if (def / olddef > 0.8) {
         if (m->mg_eps == 0.0) break;
@@ -1602,8 +1623,9 @@ int mg_solve(agx_mesh *m, int stage) {
     m->last_norm = def;
     subtract_average(m, 1);
   } else {   // SolveIterativeFixedTimes, multigrid_driver.cpp:762-771
+    if (mixed && m->mg_niter > 0) defect_to32(m, false);
     for (int n = 0; n < m->mg_niter; ++n) {
-      if (mixed) { defect_to32(m); correction_step(m); }
+      if (mixed) correction_step(m);
       else vcycle(m, m->lev);
       iters++;
     }
@@ -1627,8 +1649,9 @@ int mg_vcycles(agx_mesh *m, int n) {
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "mg_vcycles launches");
   }
+  if (m->mixed && n > 0) defect_to32(m, false);
   for (int q = 0; q < n; ++q) {
-    if (m->mixed) { defect_to32(m); correction_step(m); }
+    if (m->mixed) correction_step(m);
     else vcycle(m, m->lev);
   }
   hipError_t e = hipGetLastError();

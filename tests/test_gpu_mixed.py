@@ -103,7 +103,7 @@ def test_mixed_turbulence_run(agx, rsolver, shape):
     assert o2.max_divb() < 1e-12
 
 
-@pytest.mark.parametrize("N,B", [(32, 8), (64, 16)])
+@pytest.mark.parametrize("N,B", [(32, 8), (64, 16), (256, 64)])
 def test_mixed_mg_reaches_fp64_accuracy(agx, N, B):
     """FP32 V-cycles inside the FP64 defect correction converge to the FP64 solution:
     with threshold 0 (iterate until the defect stalls, multigrid_driver.cpp:738-745) the
