@@ -527,6 +527,90 @@ k_mg_defect_to32(const double *__restrict__ u, const float *__restrict__ ein, do
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// k_mg_defect_to32 for levels with n1 % DT_TI == 0 and n2 % DT_TJ == 0, tiled like
+// k_mg_smooth_rb: a workgroup owns a DT_TI x DT_TJ tile of (i,j) columns and a chunk of
+// DT_KC planes, and keeps phi (= u + e) of three planes of (tile + 1-cell ring) in LDS,
+// the next plane being fetched into registers while the current one is processed, so
+// u and e are read about once instead of seven times.  Per cell the arithmetic is that
+// of k_mg_defect_to32 (bit for bit); the partial sums of d^2 are per workgroup.
+constexpr int DT_TI = 64, DT_TJ = 8, DT_KC = 16;
+template <bool CORR>
+__global__ void __launch_bounds__(DT_TI * DT_TJ)
+k_mg_defect_to32_t(const double *__restrict__ u, const float *__restrict__ ein, double *__restrict__ uout,
+                   const double *__restrict__ src, float *__restrict__ d32, float *__restrict__ e0, int n1,
+                   int n2, int n3, double idx2, float dx2f, float isixf, int cred, double *__restrict__ part) {
+  constexpr int TI = DT_TI, TJ = DT_TJ, NT = TI * TJ, RI = TI + 2, RJ = TJ + 2;
+  constexpr int NPF = (RI * RJ + NT - 1) / NT;
+  __shared__ double s_phi[3][RJ][RI];
+  __shared__ double sh[NT];
+  const int tid = threadIdx.x;
+  const int nti = n1 / TI, ntj = n2 / TJ;
+  const unsigned wg = xcd_slot(blockIdx.x, gridDim.x);
+  const int tile = wg % (unsigned)(nti * ntj), chunk = wg / (unsigned)(nti * ntj);
+  const int i0 = (tile % nti) * TI, j0 = (tile / nti) * TJ;
+  const int k0 = chunk * DT_KC, k1 = min(k0 + DT_KC, n3);   // planes [k0, k1) of this workgroup
+  auto slot = [](int q) { return ((q % 3) + 3) % 3; };
+  auto wr = [](int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); };
+  double pf[NPF];
+  auto fetch = [&](int q) {
+    const int kq = wr(q, n3);
+#pragma unroll
+    for (int r = 0; r < NPF; ++r) {
+      const int t = tid + r * NT;
+      if (t < RI * RJ) {
+        const int jj = t / RI, ii = t - jj * RI;
+        const int64_t id = LIX(n1, n2, kq, wr(j0 - 1 + jj, n2), wr(i0 - 1 + ii, n1));
+        pf[r] = CORR ? u[id] + (double)ein[id] : u[id];
+      }
+    }
+  };
+  auto stash = [&](int q) {
+    const int sl = slot(q);
+#pragma unroll
+    for (int r = 0; r < NPF; ++r) {
+      const int t = tid + r * NT;
+      if (t < RI * RJ) {
+        const int jj = t / RI, ii = t - jj * RI;
+        s_phi[sl][jj][ii] = pf[r];
+      }
+    }
+  };
+  const int jj = tid / TI, ii = tid - (tid / TI) * TI;
+  const int gj = j0 + jj, gi = i0 + ii;
+  double v = 0.0;
+  fetch(k0 - 1);
+  stash(k0 - 1);
+  fetch(k0);
+  stash(k0);
+  fetch(k0 + 1);
+  for (int q = k0; q < k1; ++q) {
+    stash(q + 1);   // the slot of plane q - 2, last read before the previous barrier
+    if (q + 2 <= k1) fetch(q + 2);
+    __syncthreads();
+    const int so = slot(q), sm = slot(q - 1), sp = slot(q + 1);
+    const double uc = s_phi[so][jj + 1][ii + 1];
+    const double d = (6.0 * uc - s_phi[sp][jj + 1][ii + 1] - s_phi[so][jj + 2][ii + 1] - s_phi[so][jj + 1][ii + 2] -
+                      s_phi[sm][jj + 1][ii + 1] - s_phi[so][jj][ii + 1] - s_phi[so][jj + 1][ii]) * idx2;
+    const int64_t id = LIX(n1, n2, q, gj, gi);
+    const double dd = d + src[id];
+    if (CORR) uout[id] = uc;
+    const float df = (float)dd;
+    d32[id] = df;
+    e0[id] = (((gi + gj + q + cred) & 1) == 0)
+                 ? gs_value(0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, df, dx2f, isixf)
+                 : 0.0f;
+    v += dd * dd;
+    __syncthreads();
+  }
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] += sh[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) part[blockIdx.x] = sh[0];
+}
+
 // phi += e (mode 0) or phi = e (mode 1), FP32 correction into the FP64 iterate; mode 2
 // converts an FP64 array to FP32 (src32 = src).
 __global__ void __launch_bounds__(BS)
@@ -1549,15 +1633,26 @@ static void defect_to32(agx_mesh *m, bool correct) {
   int l = m->ntotal - 1;
   MGLevel &F = m->lev[l];
   MGLevel32 &F32 = m->lev32[l];
-  int64_t nb = npart(lcells(F));
   const int cred = 0 ^ (m->coffset ^ 1);   // colour of the next V-cycle's first sweep
+  const bool tiled = F.n1 % DT_TI == 0 && F.n2 % DT_TJ == 0;
+  const int64_t nb = tiled ? (int64_t)(F.n1 / DT_TI) * (F.n2 / DT_TJ) * ((F.n3 + DT_KC - 1) / DT_KC)
+                           : npart(lcells(F));   // <= red_n (cells / BS + 1) either way
+  const double idx2 = 1.0 / (F.dx * F.dx);
+  const float dx2f = float(F32.dx * F32.dx), isixf = float(1.15 / 6.0);
   timer_begin(m, AGX_K_MG_OTHER);
   // e of the last cycle is read at neighbour cells, so the next cycle's start goes to the
   // spare FP32 array (then swapped), as the corrected phi goes to the FP64 one
-  hipLaunchKernelGGL(k_mg_defect_to32, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.u,
-                     correct ? F32.u : (const float *)nullptr, correct ? F.def : (double *)nullptr, F.src, F32.src,
-                     F32.def, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx), float(F32.dx * F32.dx), float(1.15 / 6.0), cred,
-                     m->red);
+  if (tiled && correct)
+    hipLaunchKernelGGL(k_mg_defect_to32_t<true>, dim3((unsigned)nb), dim3(DT_TI * DT_TJ), 0, m->stream, F.u,
+                       F32.u, F.def, F.src, F32.src, F32.def, F.n1, F.n2, F.n3, idx2, dx2f, isixf, cred, m->red);
+  else if (tiled)
+    hipLaunchKernelGGL(k_mg_defect_to32_t<false>, dim3((unsigned)nb), dim3(DT_TI * DT_TJ), 0, m->stream, F.u,
+                       (const float *)nullptr, (double *)nullptr, F.src, F32.src, F32.def, F.n1, F.n2, F.n3, idx2,
+                       dx2f, isixf, cred, m->red);
+  else
+    hipLaunchKernelGGL(k_mg_defect_to32, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.u,
+                       correct ? F32.u : (const float *)nullptr, correct ? F.def : (double *)nullptr, F.src,
+                       F32.src, F32.def, F.n1, F.n2, F.n3, idx2, dx2f, isixf, cred, m->red);
   hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->red, nb, m->scal);
   timer_end(m);
   if (correct) std::swap(F.u, F.def);

@@ -21,7 +21,7 @@ def gaussian_field(shape, seed, kmin=2, kmax=8, power=-4.0, box=(1.0, 1.0, 1.0))
     sel = (K >= kmin) & (K <= kmax)
     amp[sel] = K[sel] ** (power / 2.0)
     fk = amp * (rng.normal(size=K.shape) + 1j * rng.normal(size=K.shape))
-    f = np.fft.irfftn(fk, s=shape)
+    f = np.fft.irfftn(fk, s=shape, axes=(0, 1, 2))
     f -= f.mean()
     f /= np.sqrt(np.mean(f * f))
     return f
@@ -52,18 +52,20 @@ def turbulence(desc, seed=2, mach=2.0, bx=0.1, p0=None):
     return U, b1, b2, b3
 
 
-def turbulence_lognormal(desc, seed=5, mach=5.0, sigma_lnrho=1.0, beta=1.0):
+def turbulence_lognormal(desc, seed=5, mach=5.0, sigma_lnrho=1.0, beta=1.0, bx=None):
     """Large-run start: lognormal density rho = exp(sigma g) / mean (g a unit-rms k^-4
     field on 2 <= |k| <= 8, so <rho> = 1 and the rms of ln rho is sigma), isothermal-like
     P = 0.6 rho / gamma (uniform sound speed sqrt(0.6)), velocity as in turbulence() at
-    rms Mach `mach`, uniform Bx with plasma beta 2 <P> / Bx^2 = `beta`."""
+    rms Mach `mach`, uniform Bx with plasma beta 2 <P> / Bx^2 = `beta` (or Bx = `bx`
+    when given)."""
     gamma = desc.gamma
     shape = (desc.nx3, desc.nx2, desc.nx1)
     rho = np.exp(sigma_lnrho * gaussian_field(shape, seed * 1000 + 7))
     rho /= rho.mean()
     pbar = 0.6 / gamma
     cs = np.sqrt(0.6)
-    bx = np.sqrt(2.0 * pbar / beta)
+    if bx is None:
+        bx = np.sqrt(2.0 * pbar / beta)
     U = np.empty((5,) + shape)
     U[0] = rho
     U[4] = pbar * rho / (gamma - 1.0) + 0.5 * bx * bx

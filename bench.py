@@ -5,9 +5,11 @@ HLLD + CT, plain FP64, synthetic seeded initial data.  One step = one full VL2 c
 (both stages, ghost exchange, cons->prim, time-step reduction), inputs resident in HBM.
 The same workload in the mixed-precision form (FP32 reconstruction and Riemann fluxes,
 FP64 update, C2P and CT) is reported under "mixed".
-Also reported: V(1,1) cycles/s of the FP64 multigrid on the 256^3 periodic
-32-blob self-gravity problem, the dominant kernel's roofline, and the CPU oracle
-timed on a bounded sample of the same workload.
+Also reported: V(1,1) cycles/s of the multigrid (FP64, mixed, multipole boundaries) on
+the 256^3 32-blob self-gravity problem; the coupled 256^3 MHD + multigrid self-gravity
+cycle (VL2+PLM+HLLD+CT, lognormal density) in FP64 and in the mixed form, with the
+mixed-vs-FP64 ratio and differences ("self_gravity"); the dominant kernel's roofline;
+and the CPU oracle timed on a bounded sample of the MHD workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -193,27 +195,28 @@ def mg_bench(agx, pgen, args, precision="fp64"):
     return vps, iters, norm, t_solve, per
 
 
-def large_bench(agx, pgen, args):
-    """The large single-GPU run: 512^3 periodic MHD + multigrid self-gravity in 512
-    MeshBlocks of 64^3, VL2+PPM+HLLD+CT, lognormal density (sigma_lnrho = 1), rms Mach 5,
-    beta = 1, 4 pi G = 1.  The mixed form (FP32 reconstruction/Riemann and FP32 V-cycles in
-    FP64 defect correction) is timed over `steps` cycles and compared with the plain FP64
-    path run from the same state for the same cycles."""
-    N, B = args.large_n, 64
+def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None):
+    """MHD + multigrid self-gravity at N^3 in MeshBlocks of B^3: lognormal density
+    (sigma_lnrho = 1), rms Mach `mach`, uniform Bx (plasma beta 1, or Bx = `bx`), 4 pi G = 1, VL2 + PLM (xorder 2) or PPM
+    (xorder 3) + HLLD + CT, FMG + V-cycles to 1e-10 x rms(rho - <rho>) at every stage.  The
+    plain FP64 path and the mixed form (FP32 reconstruction/Riemann and FP32 V-cycles in
+    FP64 defect correction) run `steps` cycles each from the same state; returns both
+    rates and the mixed-vs-FP64 differences after those cycles."""
+    ng = 3 if xorder == 3 else 2
     base = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4, gamma=5.0 / 3.0,
-                         xorder=3, nghost=3, rsolver=0)
+                         xorder=xorder, nghost=ng, rsolver=0)
     t0 = time.time()
-    U, b1, b2, b3 = pgen.turbulence_lognormal(base, seed=5, mach=5.0, sigma_lnrho=1.0, beta=1.0)
+    U, b1, b2, b3 = pgen.turbulence_lognormal(base, seed=5, mach=mach, sigma_lnrho=1.0, beta=1.0, bx=bx)
     src_norm = float(np.sqrt(np.mean((U[0] - U[0].mean()) ** 2)))
     ub = agx.scatter_cells(base, U)
     del U
     faces = pgen.scatter_faces(base, b1, b2, b3)
     del b1, b2, b3
-    log(f"[large] {N}^3 lognormal turbulence built in {time.time() - t0:.1f}s")
+    log(f"[{tag}] {N}^3 lognormal turbulence built in {time.time() - t0:.1f}s")
     res = {}
     for prec in ("fp64", "mixed"):
         desc = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4,
-                             gamma=5.0 / 3.0, xorder=3, nghost=3, rsolver=0, self_gravity=1,
+                             gamma=5.0 / 3.0, xorder=xorder, nghost=ng, rsolver=0, self_gravity=1,
                              four_pi_G=1.0, mg_mode=0, mg_threshold=1e-10 * src_norm,
                              precision=1 if prec == "mixed" else 0)
         m = agx.Mesh(desc)
@@ -222,26 +225,26 @@ def large_bench(agx, pgen, args):
             m.set(n, f)
         m.initialize()
         tile = m.autotune()
-        m.calculate_fluxes(3)
+        m.calculate_fluxes(xorder)
         m.sync()
         t0 = time.perf_counter()
-        done = m.cycle(1e30, args.steps)
+        done = m.cycle(1e30, steps)
         m.sync()
         sec = time.perf_counter() - t0
-        assert done == args.steps
-        ng = desc.nghost
+        assert done == steps
         sl = (slice(None), slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
         w = m.get("w")[sl][:, [0, 4]].astype(np.float64)
         bcc = m.get("bcc")[sl].astype(np.float64)
+        phi = m.get("phi")[sl].astype(np.float64)
         m.timing(True)
         m.timing_reset()
         m.cycle(1e30, 2)
         m.sync()
         per = {c: m.timing_get(c) for c in KCLASSES + ("mg_smooth", "mg_restrict", "mg_prolong", "mg_other")}
         m.timing(False)
-        res[prec] = dict(zc=float(N) ** 3 * args.steps / sec, ms=sec * 1e3 / args.steps, w=w, bcc=bcc,
+        res[prec] = dict(zc=float(N) ** 3 * steps / sec, ms=sec * 1e3 / steps, w=w, bcc=bcc, phi=phi,
                          per=per, tile=tile, bytes=m.device_bytes(), rf=roofline(desc, per, prec))
-        log(f"[large] {prec}: {args.steps} cycles in {sec:.2f}s -> {res[prec]['zc']:.4e} zone-cycles/s, "
+        log(f"[{tag}] {prec}: {steps} cycles in {sec:.2f}s -> {res[prec]['zc']:.4e} zone-cycles/s, "
             f"device {m.device_bytes() / 1e9:.1f} GB, stage tile {tile}; per cycle: "
             + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in per.items() if v[0] > 0))
         m.close()
@@ -249,31 +252,50 @@ def large_bench(agx, pgen, args):
     err = lambda x, y: float(np.abs(x - y).max() / np.abs(y).max())
     l1 = lambda x, y: float(np.abs(x - y).mean() / np.abs(y).mean())
     out = {
-        "metric": "MHD+MG zone-cycles/s at 512^3 on 1 MI355X, mixed precision; % of HBM roofline",
-        "value": a["zc"], "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
-        "ms_per_step": a["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 reconstruction+Riemann and MG smoothing, f64 update/C2P/CT/defect",
-        "data": "synthetic",
-        "config": {"workload": f"{N}^3 self-gravitating MHD turbulence (lognormal rho, sigma_lnrho=1, rms Mach 5, "
-                               f"beta=1, 4piG=1), {(N // B) ** 3} MeshBlocks of {B}^3, VL2+PPM+HLLD+CT, "
-                               f"FMG + V-cycles to 1e-10 x rms(rho - <rho>) each stage",
-                   "mesh": [N] * 3, "meshblock": [B] * 3, "nghost": 3},
-        "roofline": a["rf"],
-        "fp64": {"value": b["zc"], "ms_per_step": b["ms"], "roofline": b["rf"]},
+        "workload": f"{N}^3 self-gravitating MHD turbulence (lognormal rho, sigma_lnrho=1, rms Mach {mach:g}, "
+                    f"{'beta=1' if bx is None else f'Bx={bx:g}'}, 4piG=1), {(N // B) ** 3} MeshBlocks of {B}^3, "
+                    f"VL2+{'PPM' if xorder == 3 else 'PLM'}+HLLD+CT, FMG + V-cycles to 1e-10 x rms(rho - <rho>) "
+                    f"each stage",
+        "mixed": {"value": a["zc"], "ms_per_step": a["ms"], "roofline": a["rf"],
+                  "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in a["per"].items()}},
+        "fp64": {"value": b["zc"], "ms_per_step": b["ms"], "roofline": b["rf"],
+                 "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in b["per"].items()}},
         "vs_fp64": a["zc"] / b["zc"],
         "rel_err_rho": err(a["w"][:, 0], b["w"][:, 0]),
         "rel_err_p": err(a["w"][:, 1], b["w"][:, 1]),
         "rel_err_bcc": err(a["bcc"], b["bcc"]),
+        "rel_err_phi": err(a["phi"], b["phi"]),
         "rel_l1_rho": l1(a["w"][:, 0], b["w"][:, 0]),
         "rel_l1_p": l1(a["w"][:, 1], b["w"][:, 1]),
         "rel_l1_bcc": l1(a["bcc"], b["bcc"]),
-        "error_note": "max-norm (rel_err_*) and mean (rel_l1_*) differences relative to the FP64 path's "
-                      "max and mean magnitude after the same cycles",
-        "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in a["per"].items()},
+        "error_note": f"max-norm (rel_err_*) and mean (rel_l1_*) differences relative to the FP64 path's "
+                      f"max and mean magnitude after the same {steps} cycles",
     }
-    log(f"[large] mixed vs fp64 after {args.steps} cycles: max rho {out['rel_err_rho']:.3e}, "
-        f"P {out['rel_err_p']:.3e}, bcc {out['rel_err_bcc']:.3e}; mean rho {out['rel_l1_rho']:.3e}, "
-        f"P {out['rel_l1_p']:.3e}, bcc {out['rel_l1_bcc']:.3e}")
+    log(f"[{tag}] mixed vs fp64: {out['vs_fp64']:.3f}x; after {steps} cycles max rho {out['rel_err_rho']:.3e}, "
+        f"P {out['rel_err_p']:.3e}, bcc {out['rel_err_bcc']:.3e}, phi {out['rel_err_phi']:.3e}; "
+        f"mean rho {out['rel_l1_rho']:.3e}, P {out['rel_l1_p']:.3e}, bcc {out['rel_l1_bcc']:.3e}")
+    return out
+
+
+def large_bench(agx, pgen, args):
+    """The large single-GPU run: 512^3 periodic MHD + multigrid self-gravity in 512
+    MeshBlocks of 64^3, VL2+PPM+HLLD+CT, lognormal density (sigma_lnrho = 1), rms Mach 5,
+    beta = 1, 4 pi G = 1; the mixed form is the headline value."""
+    N, B = args.large_n, 64
+    r = sg_run(agx, pgen, N, B, 3, 5.0, args.steps, "large")
+    a = r["mixed"]
+    out = {
+        "metric": "MHD+MG zone-cycles/s at 512^3 on 1 MI355X, mixed precision; % of HBM roofline",
+        "value": a["value"], "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": a["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 reconstruction+Riemann and MG smoothing, f64 update/C2P/CT/defect",
+        "data": "synthetic",
+        "config": {"workload": r["workload"], "mesh": [N] * 3, "meshblock": [B] * 3, "nghost": 3},
+        "roofline": a["roofline"],
+        "fp64": {k: r["fp64"][k] for k in ("value", "ms_per_step", "roofline")},
+        "kernel_ms_per_cycle": a["kernel_ms_per_cycle"],
+    }
+    out.update({k: v for k, v in r.items() if k.startswith("rel_") or k in ("vs_fp64", "error_note")})
     print(json.dumps(out), flush=True)
 
 
@@ -322,6 +344,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-sg", action="store_true",
+                    help="skip the 256^3 MHD + self-gravity leg (FP64 and mixed)")
+    ap.add_argument("--sg-steps", type=int, default=10)
     ap.add_argument("--compare", action="store_true",
                     help="download w after both runs and report the mixed-vs-FP64 difference")
     ap.add_argument("--large", action="store_true",
@@ -413,6 +438,8 @@ def main():
             "final_defect": nmp,
             "note": "isolated blobs, multipole order-4 boundaries about the centre of mass, FAS V(1,1), FP64",
             "kernel_ms_per_vcycle": {k: v[0] / 3 for k, v in mpper.items()}}
+    if not args.no_sg:
+        out["self_gravity"] = sg_run(agx, pgen, args.n, args.block, 2, 2.0, args.sg_steps, "sg", bx=0.1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

@@ -26,6 +26,18 @@ def load_agx():
     return mod
 
 
+def load_pgen():
+    """Seeded synthetic initial data of the package (athena-gamma_amd/pgen.py, numpy)."""
+    name = "agx_pgen"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "athena-gamma_amd", "pgen.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
 @pytest.fixture(scope="session")
 def agx():
     return load_agx()
