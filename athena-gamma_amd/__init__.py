@@ -33,6 +33,9 @@ EXPORTS = [
     "agx_initialize", "agx_cycle", "agx_mg_solve", "agx_mg_set_mode", "agx_mg_last",
     "agx_mg_vcycles", "agx_autotune", "agx_set_tile", "agx_set_graph", "agx_timing_enable", "agx_timing_reset",
     "agx_timing_get",
+    "agx_ref_list_create", "agx_ref_list_destroy", "agx_ref_emf_count", "agx_ref_restrict_cc",
+    "agx_ref_restrict_fc", "agx_ref_prolong_cc", "agx_ref_prolong_fc_shared", "agx_ref_prolong_fc_internal",
+    "agx_ref_flux_from_finer", "agx_ref_emf_to_coarser", "agx_ref_sync",
 ]
 
 
@@ -68,6 +71,26 @@ def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)
     d.mp_origin[:] = mp_origin
     return d
 
+
+class RefItem(ctypes.Structure):
+    """agx_ref_item: one work item of a refinement operator."""
+    _fields_ = [(n, ctypes.c_int32) for n in ("fb", "cb", "si", "ei", "sj", "ej", "sk", "ek",
+                                              "fid", "fi1", "fi2", "pad")]
+
+
+class RefGeom(ctypes.Structure):
+    """agx_ref_geom: block interior size and the ghost widths of blocks and coarse buffers."""
+    _fields_ = [(n, ctypes.c_int32) for n in ("nx1", "nx2", "nx3", "ng", "cng", "pad")]
+
+
+class RefCoords(ctypes.Structure):
+    """agx_ref_coords: device pointers to per-block x?v and dx?f tables."""
+    _fields_ = [(n, ctypes.c_void_p) for n in ("x1v", "x2v", "x3v", "dx1f", "dx2f", "dx3f")]
+
+
+REF_KINDS = dict(restrict_cc=0, restrict_fc1=1, restrict_fc2=2, restrict_fc3=3, prolong_cc=4,
+                 prolong_fc1=5, prolong_fc2=6, prolong_fc3=7, prolong_fc_int=8, flux_from_finer=9,
+                 emf_to_coarser=10)
 
 _LIB = None
 
@@ -113,6 +136,20 @@ def lib():
     L.agx_timing_enable.argtypes = [P, I]
     L.agx_timing_reset.argtypes = [P]
     L.agx_timing_get.argtypes = [P, I, D, ctypes.POINTER(ctypes.c_int64)]
+    G, C = ctypes.POINTER(RefGeom), RefCoords
+    L.agx_ref_list_create.argtypes = [G, ctypes.c_int32, ctypes.POINTER(RefItem), ctypes.c_int32,
+                                      ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(P)]
+    L.agx_ref_list_destroy.argtypes = [P]
+    L.agx_ref_emf_count.restype = ctypes.c_int64
+    L.agx_ref_emf_count.argtypes = [G, ctypes.c_int32]
+    L.agx_ref_restrict_cc.argtypes = [P, ctypes.c_int32, P, C, P, P]
+    L.agx_ref_restrict_fc.argtypes = [P, P, C, P, P]
+    L.agx_ref_prolong_cc.argtypes = [P, ctypes.c_int32, P, C, P, C, P]
+    L.agx_ref_prolong_fc_shared.argtypes = [P, P, C, P, C, P]
+    L.agx_ref_prolong_fc_internal.argtypes = [P, P, P, P, C, P]
+    L.agx_ref_flux_from_finer.argtypes = [P, ctypes.c_int32, P, P, P, C, P, P, P, P]
+    L.agx_ref_emf_to_coarser.argtypes = [P, P, P, P, C, P, ctypes.c_int64, P]
+    L.agx_ref_sync.argtypes = [P]
     _LIB = L
     return L
 
@@ -297,3 +334,58 @@ def gather_cells(desc, a):
           l1 * desc.bnx1:(l1 + 1) * desc.bnx1] = a[g, :, ng:ng + desc.bnx3, ng:ng + desc.bnx2,
                                                    ng:ng + desc.bnx1]
     return G
+
+
+class RefList:
+    """A device work list of one refinement operator (agx_ref_list_create).  The
+    operator calls take device pointers (integers) to arrays in the layouts of
+    include/athena_gamma_amd.h and run on the default stream; sync() waits."""
+
+    def __init__(self, geom, kind, items, nb_fine, nb_coarse):
+        self.geom = geom
+        self.kind = REF_KINDS[kind] if isinstance(kind, str) else int(kind)
+        arr = (RefItem * max(len(items), 1))(*items)
+        self.h = ctypes.c_void_p()
+        _check(lib().agx_ref_list_create(ctypes.byref(geom), self.kind, arr, len(items), nb_fine, nb_coarse,
+                                         ctypes.byref(self.h)), "agx_ref_list_create")
+
+    def close(self):
+        if self.h:
+            lib().agx_ref_list_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def restrict_cc(self, nvar, fine, fc, coarse):
+        _check(lib().agx_ref_restrict_cc(self.h, nvar, fine, fc, coarse, None), "agx_ref_restrict_cc")
+
+    def restrict_fc(self, fine, fc, coarse):
+        _check(lib().agx_ref_restrict_fc(self.h, fine, fc, coarse, None), "agx_ref_restrict_fc")
+
+    def prolong_cc(self, nvar, coarse, cc, fine, fc):
+        _check(lib().agx_ref_prolong_cc(self.h, nvar, coarse, cc, fine, fc, None), "agx_ref_prolong_cc")
+
+    def prolong_fc_shared(self, coarse, cc, fine, fc):
+        _check(lib().agx_ref_prolong_fc_shared(self.h, coarse, cc, fine, fc, None), "agx_ref_prolong_fc_shared")
+
+    def prolong_fc_internal(self, b1, b2, b3, fc):
+        _check(lib().agx_ref_prolong_fc_internal(self.h, b1, b2, b3, fc, None), "agx_ref_prolong_fc_internal")
+
+    def flux_from_finer(self, nvar, f1, f2, f3, fc, c1, c2, c3):
+        _check(lib().agx_ref_flux_from_finer(self.h, nvar, f1, f2, f3, fc, c1, c2, c3, None),
+               "agx_ref_flux_from_finer")
+
+    def emf_to_coarser(self, e1, e2, e3, fc, buf, stride):
+        _check(lib().agx_ref_emf_to_coarser(self.h, e1, e2, e3, fc, buf, stride, None), "agx_ref_emf_to_coarser")
+
+    @staticmethod
+    def sync():
+        _check(lib().agx_ref_sync(None), "agx_ref_sync")
+
+
+def ref_emf_count(geom, fid):
+    return int(lib().agx_ref_emf_count(ctypes.byref(geom), fid))

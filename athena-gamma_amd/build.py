@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libathena_gamma_amd.so")
-SOURCES = ["agx_mhd.hip", "agx_mg.hip"]
+SOURCES = ["agx_mhd.hip", "agx_mg.hip", "agx_refine.hip"]
 
 # -ffp-contract=off: the FP64 path rounds every expression as written (bitwise parity
 # with the CPU restatement).  -fno-hip-fp32-correctly-rounded-divide-sqrt: FP32 division

@@ -1,0 +1,676 @@
+// agx_refine.hip -- static-mesh-refinement operators for gfx950 (3D Cartesian blocks).
+//
+// Restriction, prolongation and flux/EMF correction of the reference's MeshRefinement
+// and flux-correction boundary variables, batched over device-resident work lists: one
+// launch runs every item of a list, a (G x nitems) grid in which workgroup (x, m) walks
+// item m's index range with stride G * 256.  A list is checked and uploaded once
+// (agx_ref_list_create), so the kernels index without further bounds tests.  Arrays are
+// the reference's dense AthenaArray layouts, block-major; every expression keeps the
+// reference's operand order, so results equal the reference's FP64 arithmetic
+// (-ffp-contract=off).  On a 3D Cartesian block the areas, volumes and edge lengths are
+// products of dx?f (coordinates/coordinates.cpp:328-537) and the face-area positions
+// x?s? are x?v (coordinates/cartesian.cpp:106-119).
+//
+// All of these operators are bound by HBM traffic and index arithmetic.  Each reads
+// O(1) values per output and does no reuse worth an LDS stage, and the items are small
+// (a block face or a block of coarse cells).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+#include "agx_internal.hpp"
+
+struct agx_ref_list {
+  agx_ref_geom g;
+  int32_t kind = 0;
+  int32_t nitems = 0;
+  int32_t G = 1;              // workgroups per item
+  int64_t emf_max = 0;        // largest per-item output of an EMF_TO_COARSER list
+  agx_ref_item *items = nullptr;   // device
+};
+
+namespace agx {
+namespace {
+
+constexpr int RBS = 256;
+
+struct RG {            // derived geometry (see agx_ref_geom)
+  int n1, n2, n3;      // block cells incl. ghosts
+  int c1, c2, c3;      // coarse buffer cells incl. ghosts
+  int is, js, ks, ie, je, ke;
+  int cis, cjs, cks;
+  int h1, h2, h3;      // nx / 2
+};
+
+RG make_rg(const agx_ref_geom &g) {
+  RG r;
+  r.n1 = g.nx1 + 2 * g.ng; r.n2 = g.nx2 + 2 * g.ng; r.n3 = g.nx3 + 2 * g.ng;
+  r.c1 = g.nx1 / 2 + 2 * g.cng; r.c2 = g.nx2 / 2 + 2 * g.cng; r.c3 = g.nx3 / 2 + 2 * g.cng;
+  r.is = r.js = r.ks = g.ng;
+  r.ie = g.ng + g.nx1 - 1; r.je = g.ng + g.nx2 - 1; r.ke = g.ng + g.nx3 - 1;
+  r.cis = r.cjs = r.cks = g.cng;
+  r.h1 = g.nx1 / 2; r.h2 = g.nx2 / 2; r.h3 = g.nx3 / 2;
+  return r;
+}
+
+// element (b, n, k, j, i) of a block-major array of nvar variables of d3 x d2 x d1
+__device__ __forceinline__ int64_t ix(int d3, int d2, int d1, int nvar, int b, int n, int k, int j, int i) {
+  return ((((int64_t)b * nvar + n) * d3 + k) * d2 + j) * (int64_t)d1 + i;
+}
+
+__device__ __forceinline__ double ref_sign(double x) { return (x < 0.0) ? -1.0 : 1.0; }   // SIGN
+// 0.5*(SIGN(gm) + SIGN(gp))*min(|gm|, |gp|), mesh_refinement.cpp:427-438
+__device__ __forceinline__ double minmod_slope(double gm, double gp) {
+  const double am = fabs(gm), ap = fabs(gp);
+  return 0.5 * (ref_sign(gm) + ref_sign(gp)) * ((ap < am) ? ap : am);
+}
+
+struct Range {
+  int si, sj, sk, w1, w2, w3;
+};
+__device__ __forceinline__ Range item_range(const agx_ref_item &it) {
+  return Range{it.si, it.sj, it.sk, it.ei - it.si + 1, it.ej - it.sj + 1, it.ek - it.sk + 1};
+}
+__device__ __forceinline__ int64_t first_t() { return (int64_t)blockIdx.x * RBS + threadIdx.x; }
+__device__ __forceinline__ int64_t stride_t() { return (int64_t)gridDim.x * RBS; }
+
+// ---------------------------------------------------------------- restriction
+// RestrictCellCenteredValues, mesh_refinement.cpp:106-136
+__global__ void __launch_bounds__(RBS)
+k_ref_restrict_cc(RG r, int nvar, const double *__restrict__ fine, agx_ref_coords fc,
+                  double *__restrict__ coarse, const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const Range q = item_range(it);
+  const int64_t per = (int64_t)q.w1 * q.w2 * q.w3, cnt = per * nvar;
+  const double *d1 = fc.dx1f + (int64_t)it.fb * r.n1, *d2 = fc.dx2f + (int64_t)it.fb * r.n2,
+               *d3 = fc.dx3f + (int64_t)it.fb * r.n3;
+  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
+    const int n = (int)(t / per);
+    int64_t s = t - n * per;
+    const int ci = q.si + (int)(s % q.w1);
+    s /= q.w1;
+    const int cj = q.sj + (int)(s % q.w2), ck = q.sk + (int)(s / q.w2);
+    const int i = (ci - r.cis) * 2 + r.is, j = (cj - r.cjs) * 2 + r.js, k = (ck - r.cks) * 2 + r.ks;
+    double v[2][2][2];   // CellVolume at (k+a, j+b, i+c) = dx1f*dx2f*dx3f
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) v[a][b][c] = d1[i + c] * d2[j + b] * d3[k + a];
+    const double tvol = ((v[0][0][0] + v[0][1][0]) + (v[0][0][1] + v[0][1][1])) +
+                        ((v[1][0][0] + v[1][1][0]) + (v[1][0][1] + v[1][1][1]));
+    auto F = [&](int a, int b, int c) { return fine[ix(r.n3, r.n2, r.n1, nvar, it.fb, n, k + a, j + b, i + c)]; };
+    const double sum = (((F(0, 0, 0) * v[0][0][0] + F(0, 1, 0) * v[0][1][0]) +
+                         (F(0, 0, 1) * v[0][0][1] + F(0, 1, 1) * v[0][1][1])) +
+                        ((F(1, 0, 0) * v[1][0][0] + F(1, 1, 0) * v[1][1][0]) +
+                         (F(1, 0, 1) * v[1][0][1] + F(1, 1, 1) * v[1][1][1])));
+    coarse[ix(r.c3, r.c2, r.c1, nvar, it.cb, n, ck, cj, ci)] = sum / tvol;
+  }
+}
+
+// RestrictFieldX1/X2/X3, mesh_refinement.cpp:185-214, 244-279, 321-344
+template <int DIR>
+__global__ void __launch_bounds__(RBS)
+k_ref_restrict_fc(RG r, const double *__restrict__ fine, agx_ref_coords fc, double *__restrict__ coarse,
+                  const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const Range q = item_range(it);
+  const int64_t cnt = (int64_t)q.w1 * q.w2 * q.w3;
+  const int f3 = r.n3 + (DIR == 2), f2 = r.n2 + (DIR == 1), f1 = r.n1 + (DIR == 0);
+  const int q3 = r.c3 + (DIR == 2), q2 = r.c2 + (DIR == 1), q1 = r.c1 + (DIR == 0);
+  const double *d1 = fc.dx1f + (int64_t)it.fb * r.n1, *d2 = fc.dx2f + (int64_t)it.fb * r.n2,
+               *d3 = fc.dx3f + (int64_t)it.fb * r.n3;
+  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
+    const int ci = q.si + (int)(t % q.w1);
+    const int64_t s = t / q.w1;
+    const int cj = q.sj + (int)(s % q.w2), ck = q.sk + (int)(s / q.w2);
+    const int i = (ci - r.cis) * 2 + r.is, j = (cj - r.cjs) * 2 + r.js, k = (ck - r.cks) * 2 + r.ks;
+    auto F = [&](int a, int b, int c) { return fine[ix(f3, f2, f1, 1, it.fb, 0, k + a, j + b, i + c)]; };
+    double val;
+    if (DIR == 0) {   // Face1Area = dx2f(j)*dx3f(k)
+      const double a00 = d2[j] * d3[k], a01 = d2[j + 1] * d3[k];
+      const double a10 = d2[j] * d3[k + 1], a11 = d2[j + 1] * d3[k + 1];
+      const double tarea = a00 + a01 + a10 + a11;
+      val = (F(0, 0, 0) * a00 + F(0, 1, 0) * a01 + F(1, 0, 0) * a10 + F(1, 1, 0) * a11) / tarea;
+    } else if (DIR == 1) {   // Face2Area = dx1f(i)*dx3f(k)
+      const double s00 = d1[i] * d3[k], s01 = d1[i + 1] * d3[k];
+      const double s10 = d1[i] * d3[k + 1], s11 = d1[i + 1] * d3[k + 1];
+      const double tarea = s00 + s01 + s10 + s11;
+      val = (F(0, 0, 0) * s00 + F(0, 0, 1) * s01 + F(1, 0, 0) * s10 + F(1, 0, 1) * s11) / tarea;
+    } else {   // Face3Area = dx1f(i)*dx2f(j)
+      const double s00 = d1[i] * d2[j], s01 = d1[i + 1] * d2[j];
+      const double s10 = d1[i] * d2[j + 1], s11 = d1[i + 1] * d2[j + 1];
+      const double tarea = s00 + s01 + s10 + s11;
+      val = (F(0, 0, 0) * s00 + F(0, 0, 1) * s01 + F(0, 1, 0) * s10 + F(0, 1, 1) * s11) / tarea;
+    }
+    coarse[ix(q3, q2, q1, 1, it.cb, 0, ck, cj, ci)] = val;
+  }
+}
+
+// ---------------------------------------------------------------- prolongation
+// ProlongateCellCenteredValues (3D), mesh_refinement.cpp:386-458
+__global__ void __launch_bounds__(RBS)
+k_ref_prolong_cc(RG r, int nvar, const double *__restrict__ coarse, agx_ref_coords cc, double *__restrict__ fine,
+                 agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const Range q = item_range(it);
+  const int64_t per = (int64_t)q.w1 * q.w2 * q.w3, cnt = per * nvar;
+  const double *X1 = cc.x1v + (int64_t)it.cb * r.c1, *X2 = cc.x2v + (int64_t)it.cb * r.c2,
+               *X3 = cc.x3v + (int64_t)it.cb * r.c3;
+  const double *x1 = fc.x1v + (int64_t)it.fb * r.n1, *x2 = fc.x2v + (int64_t)it.fb * r.n2,
+               *x3 = fc.x3v + (int64_t)it.fb * r.n3;
+  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
+    const int n = (int)(t / per);
+    int64_t s = t - n * per;
+    const int i = q.si + (int)(s % q.w1);
+    s /= q.w1;
+    const int j = q.sj + (int)(s % q.w2), k = q.sk + (int)(s / q.w2);
+    const int fi = (i - r.cis) * 2 + r.is, fj = (j - r.cjs) * 2 + r.js, fk = (k - r.cks) * 2 + r.ks;
+    const double dx3m = X3[k] - X3[k - 1], dx3p = X3[k + 1] - X3[k];
+    const double dx3fm = X3[k] - x3[fk], dx3fp = x3[fk + 1] - X3[k];
+    const double dx2m = X2[j] - X2[j - 1], dx2p = X2[j + 1] - X2[j];
+    const double dx2fm = X2[j] - x2[fj], dx2fp = x2[fj + 1] - X2[j];
+    const double dx1m = X1[i] - X1[i - 1], dx1p = X1[i + 1] - X1[i];
+    const double dx1fm = X1[i] - x1[fi], dx1fp = x1[fi + 1] - X1[i];
+    auto C = [&](int a, int b, int c) { return coarse[ix(r.c3, r.c2, r.c1, nvar, it.cb, n, k + a, j + b, i + c)]; };
+    const double cv = C(0, 0, 0);
+    const double g1 = minmod_slope((cv - C(0, 0, -1)) / dx1m, (C(0, 0, 1) - cv) / dx1p);
+    const double g2 = minmod_slope((cv - C(0, -1, 0)) / dx2m, (C(0, 1, 0) - cv) / dx2p);
+    const double g3 = minmod_slope((cv - C(-1, 0, 0)) / dx3m, (C(1, 0, 0) - cv) / dx3p);
+    auto FO = [&](int a, int b, int c) -> double & {
+      return fine[ix(r.n3, r.n2, r.n1, nvar, it.fb, n, fk + a, fj + b, fi + c)];
+    };
+    FO(0, 0, 0) = cv - (g1 * dx1fm + g2 * dx2fm + g3 * dx3fm);
+    FO(0, 0, 1) = cv + (g1 * dx1fp - g2 * dx2fm - g3 * dx3fm);
+    FO(0, 1, 0) = cv - (g1 * dx1fm - g2 * dx2fp + g3 * dx3fm);
+    FO(0, 1, 1) = cv + (g1 * dx1fp + g2 * dx2fp - g3 * dx3fm);
+    FO(1, 0, 0) = cv - (g1 * dx1fm + g2 * dx2fm - g3 * dx3fp);
+    FO(1, 0, 1) = cv + (g1 * dx1fp - g2 * dx2fm + g3 * dx3fp);
+    FO(1, 1, 0) = cv - (g1 * dx1fm - g2 * dx2fp - g3 * dx3fp);
+    FO(1, 1, 1) = cv + (g1 * dx1fp + g2 * dx2fp + g3 * dx3fp);
+  }
+}
+
+// ProlongateSharedFieldX1/X2/X3 (3D), mesh_refinement.cpp:541-581, 626-674, 718-762
+template <int DIR>
+__global__ void __launch_bounds__(RBS)
+k_ref_prolong_fc_shared(RG r, const double *__restrict__ coarse, agx_ref_coords cc, double *__restrict__ fine,
+                        agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const Range q = item_range(it);
+  const int64_t cnt = (int64_t)q.w1 * q.w2 * q.w3;
+  const int f3 = r.n3 + (DIR == 2), f2 = r.n2 + (DIR == 1), f1 = r.n1 + (DIR == 0);
+  const int q3 = r.c3 + (DIR == 2), q2 = r.c2 + (DIR == 1), q1 = r.c1 + (DIR == 0);
+  const double *X1 = cc.x1v + (int64_t)it.cb * r.c1, *X2 = cc.x2v + (int64_t)it.cb * r.c2,
+               *X3 = cc.x3v + (int64_t)it.cb * r.c3;
+  const double *x1 = fc.x1v + (int64_t)it.fb * r.n1, *x2 = fc.x2v + (int64_t)it.fb * r.n2,
+               *x3 = fc.x3v + (int64_t)it.fb * r.n3;
+  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
+    const int i = q.si + (int)(t % q.w1);
+    const int64_t s = t / q.w1;
+    const int j = q.sj + (int)(s % q.w2), k = q.sk + (int)(s / q.w2);
+    const int fi = (i - r.cis) * 2 + r.is, fj = (j - r.cjs) * 2 + r.js, fk = (k - r.cks) * 2 + r.ks;
+    auto C = [&](int a, int b, int c) { return coarse[ix(q3, q2, q1, 1, it.cb, 0, k + a, j + b, i + c)]; };
+    auto FO = [&](int a, int b, int c) -> double & { return fine[ix(f3, f2, f1, 1, it.fb, 0, fk + a, fj + b, fi + c)]; };
+    const double cv = C(0, 0, 0);
+    if (DIR == 0) {
+      const double g2 = minmod_slope((cv - C(0, -1, 0)) / (X2[j] - X2[j - 1]), (C(0, 1, 0) - cv) / (X2[j + 1] - X2[j]));
+      const double g3 = minmod_slope((cv - C(-1, 0, 0)) / (X3[k] - X3[k - 1]), (C(1, 0, 0) - cv) / (X3[k + 1] - X3[k]));
+      FO(0, 0, 0) = cv - g2 * (X2[j] - x2[fj]) - g3 * (X3[k] - x3[fk]);
+      FO(0, 1, 0) = cv + g2 * (x2[fj + 1] - X2[j]) - g3 * (X3[k] - x3[fk]);
+      FO(1, 0, 0) = cv - g2 * (X2[j] - x2[fj]) + g3 * (x3[fk + 1] - X3[k]);
+      FO(1, 1, 0) = cv + g2 * (x2[fj + 1] - X2[j]) + g3 * (x3[fk + 1] - X3[k]);
+    } else if (DIR == 1) {
+      const double g1 = minmod_slope((cv - C(0, 0, -1)) / (X1[i] - X1[i - 1]), (C(0, 0, 1) - cv) / (X1[i + 1] - X1[i]));
+      const double g3 = minmod_slope((cv - C(-1, 0, 0)) / (X3[k] - X3[k - 1]), (C(1, 0, 0) - cv) / (X3[k + 1] - X3[k]));
+      FO(0, 0, 0) = cv - g1 * (X1[i] - x1[fi]) - g3 * (X3[k] - x3[fk]);
+      FO(0, 0, 1) = cv + g1 * (x1[fi + 1] - X1[i]) - g3 * (X3[k] - x3[fk]);
+      FO(1, 0, 0) = cv - g1 * (X1[i] - x1[fi]) + g3 * (x3[fk + 1] - X3[k]);
+      FO(1, 0, 1) = cv + g1 * (x1[fi + 1] - X1[i]) + g3 * (x3[fk + 1] - X3[k]);
+    } else {
+      const double g1 = minmod_slope((cv - C(0, 0, -1)) / (X1[i] - X1[i - 1]), (C(0, 0, 1) - cv) / (X1[i + 1] - X1[i]));
+      const double g2 = minmod_slope((cv - C(0, -1, 0)) / (X2[j] - X2[j - 1]), (C(0, 1, 0) - cv) / (X2[j + 1] - X2[j]));
+      FO(0, 0, 0) = cv - g1 * (X1[i] - x1[fi]) - g2 * (X2[j] - x2[fj]);
+      FO(0, 0, 1) = cv + g1 * (x1[fi + 1] - X1[i]) - g2 * (X2[j] - x2[fj]);
+      FO(0, 1, 0) = cv - g1 * (X1[i] - x1[fi]) + g2 * (x2[fj + 1] - X2[j]);
+      FO(0, 1, 1) = cv + g1 * (x1[fi + 1] - X1[i]) + g2 * (x2[fj + 1] - X2[j]);
+    }
+  }
+}
+
+// ProlongateInternalField (3D), mesh_refinement.cpp:831-946: the 12 interior faces of
+// each 2x2x2 group of fine cells from its 24 outer faces; groups share no written face.
+__global__ void __launch_bounds__(RBS)
+k_ref_prolong_fc_internal(RG r, double *__restrict__ b1, double *__restrict__ b2, double *__restrict__ b3,
+                          agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const Range q = item_range(it);
+  const int64_t cnt = (int64_t)q.w1 * q.w2 * q.w3;
+  const int fb = it.fb;
+  const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
+               *d3 = fc.dx3f + (int64_t)fb * r.n3;
+  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
+    const int i = q.si + (int)(t % q.w1);
+    const int64_t s = t / q.w1;
+    const int j = q.sj + (int)(s % q.w2), k = q.sk + (int)(s / q.w2);
+    const int fi = (i - r.cis) * 2 + r.is, fj = (j - r.cjs) * 2 + r.js, fk = (k - r.cks) * 2 + r.ks;
+    auto B1 = [&](int kk, int jj, int ii) -> double & { return b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, kk, jj, ii)]; };
+    auto B2 = [&](int kk, int jj, int ii) -> double & { return b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, kk, jj, ii)]; };
+    auto B3 = [&](int kk, int jj, int ii) -> double & { return b3[ix(r.n3 + 1, r.n2, r.n1, 1, fb, 0, kk, jj, ii)]; };
+    // Face?Area(fk+a, fj+b, ii)
+    auto A1 = [&](int a, int b) { return d2[fj + b] * d3[fk + a]; };
+    auto A2 = [&](int a, int ii) { return d1[ii] * d3[fk + a]; };
+    auto A3 = [&](int b, int ii) { return d1[ii] * d2[fj + b]; };
+    double Uxx = 0.0, Vyy = 0.0, Wzz = 0.0, Uxyz = 0.0, Vxyz = 0.0, Wxyz = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      const int js = 2 * jj - 1, fjj = fj + jj, fjp = fj + 2 * jj;
+#pragma unroll
+      for (int ii = 0; ii < 2; ++ii) {
+        const int is = 2 * ii - 1, fii = fi + ii, fip = fi + 2 * ii;
+        const double x2lo = B2(fk, fjp, fii) * A2(0, fii), x2hi = B2(fk + 1, fjp, fii) * A2(1, fii);
+        const double x3lo = B3(fk, fjj, fii) * A3(jj, fii), x3hi = B3(fk + 2, fjj, fii) * A3(jj, fii);
+        const double x1lo = B1(fk, fjj, fip) * A1(0, jj), x1hi = B1(fk + 1, fjj, fip) * A1(1, jj);
+        Uxx += is * (js * (x2lo + x2hi) + (x3hi - x3lo));
+        Vyy += js * ((x3hi - x3lo) + is * (x1lo + x1hi));
+        Wzz += is * (x1hi - x1lo) + js * (x2hi - x2lo);
+        Uxyz += is * js * (x1hi - x1lo);
+        Vxyz += is * js * (x2hi - x2lo);
+        Wxyz += is * js * (x3hi - x3lo);
+      }
+    }
+    const double s1 = d1[fi] + d1[fi + 1], s2 = d2[fj] + d2[fj + 1], s3 = d3[fk] + d3[fk + 1];
+    const double Sdx1 = s1 * s1, Sdx2 = s2 * s2, Sdx3 = s3 * s3;
+    Uxx *= 0.125; Vyy *= 0.125; Wzz *= 0.125;
+    Uxyz *= 0.125 / (Sdx2 + Sdx3);
+    Vxyz *= 0.125 / (Sdx1 + Sdx3);
+    Wxyz *= 0.125 / (Sdx1 + Sdx2);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        double v = 0.5 * (B1(fk + a, fj + b, fi) * A1(a, b) + B1(fk + a, fj + b, fi + 2) * A1(a, b)) + Uxx;
+        v = a ? v + Sdx3 * Vxyz : v - Sdx3 * Vxyz;
+        v = b ? v + Sdx2 * Wxyz : v - Sdx2 * Wxyz;
+        B1(fk + a, fj + b, fi + 1) = v / A1(a, b);
+      }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        double v = 0.5 * (B2(fk + a, fj, fi + c) * A2(a, fi + c) + B2(fk + a, fj + 2, fi + c) * A2(a, fi + c)) + Vyy;
+        v = a ? v + Sdx3 * Uxyz : v - Sdx3 * Uxyz;
+        v = c ? v + Sdx1 * Wxyz : v - Sdx1 * Wxyz;
+        B2(fk + a, fj + 1, fi + c) = v / A2(a, fi + c);
+      }
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        double v = 0.5 * (B3(fk + 2, fj + b, fi + c) * A3(b, fi + c) + B3(fk, fj + b, fi + c) * A3(b, fi + c)) + Wzz;
+        v = b ? v + Sdx2 * Uxyz : v - Sdx2 * Uxyz;
+        v = c ? v + Sdx1 * Vxyz : v - Sdx1 * Vxyz;
+        B3(fk + 1, fj + b, fi + c) = v / A3(b, fi + c);
+      }
+  }
+}
+
+// ---------------------------------------------------------------- flux correction
+// LoadFluxBoundaryBufferToCoarser (flux_correction_cc.cpp:77-169) on the finer block and
+// SetFluxBoundaryFromFiner (:240-283) on the coarser one, as one direct write: value
+// (n, a, b) of the reference's buffer is the finer block's average over fine faces
+// (2a, 2b) + {0,1}^2 of its face opposite `fid`, stored at coarse face (lo_a + a, lo_b + b).
+__global__ void __launch_bounds__(RBS)
+k_ref_flux_from_finer(RG r, int nvar, const double *__restrict__ ff1, const double *__restrict__ ff2,
+                      const double *__restrict__ ff3, agx_ref_coords fc, double *__restrict__ cf1,
+                      double *__restrict__ cf2, double *__restrict__ cf3, const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const int dir = it.fid >> 1, side = it.fid & 1;
+  const int na = dir == 2 ? r.h2 : r.h3;   // slow index of the face: k (x1, x2 faces) or j (x3)
+  const int nbb = dir == 0 ? r.h2 : r.h1;  // fast index: j (x1 face) or i
+  const int64_t per = (int64_t)na * nbb, cnt = per * nvar;
+  const double *d1 = fc.dx1f + (int64_t)it.fb * r.n1, *d2 = fc.dx2f + (int64_t)it.fb * r.n2,
+               *d3 = fc.dx3f + (int64_t)it.fb * r.n3;
+  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
+    const int n = (int)(t / per);
+    const int64_t s = t - n * per;
+    const int a = (int)(s / nbb), b = (int)(s - (int64_t)a * nbb);
+    if (dir == 0) {
+      const int ic = side ? r.ie + 1 : r.is, i = side ? r.is : r.ie + 1;
+      const int k = r.ks + 2 * a, j = r.js + 2 * b;
+      const double amm = d2[j] * d3[k], amp = d2[j + 1] * d3[k], apm = d2[j] * d3[k + 1], app = d2[j + 1] * d3[k + 1];
+      const double tarea = amm + amp + apm + app;
+      auto FX = [&](int p, int q) { return ff1[ix(r.n3, r.n2, r.n1 + 1, nvar, it.fb, n, k + p, j + q, i)]; };
+      const double v = (FX(0, 0) * amm + FX(0, 1) * amp + FX(1, 0) * apm + FX(1, 1) * app) / tarea;
+      cf1[ix(r.n3, r.n2, r.n1 + 1, nvar, it.cb, n, r.ks + (it.fi2 ? r.h3 : 0) + a, r.js + (it.fi1 ? r.h2 : 0) + b, ic)] = v;
+    } else if (dir == 1) {
+      const int jc = side ? r.je + 1 : r.js, j = side ? r.js : r.je + 1;
+      const int k = r.ks + 2 * a, i = r.is + 2 * b;
+      const double s00 = d1[i] * d3[k], s01 = d1[i + 1] * d3[k], s10 = d1[i] * d3[k + 1], s11 = d1[i + 1] * d3[k + 1];
+      const double tarea = s00 + s01 + s10 + s11;
+      auto FX = [&](int p, int q) { return ff2[ix(r.n3, r.n2 + 1, r.n1, nvar, it.fb, n, k + p, j, i + q)]; };
+      const double v = (FX(0, 0) * s00 + FX(0, 1) * s01 + FX(1, 0) * s10 + FX(1, 1) * s11) / tarea;
+      cf2[ix(r.n3, r.n2 + 1, r.n1, nvar, it.cb, n, r.ks + (it.fi2 ? r.h3 : 0) + a, jc, r.is + (it.fi1 ? r.h1 : 0) + b)] = v;
+    } else {
+      const int kc = side ? r.ke + 1 : r.ks, k = side ? r.ks : r.ke + 1;
+      const int j = r.js + 2 * a, i = r.is + 2 * b;
+      const double s00 = d1[i] * d2[j], s01 = d1[i + 1] * d2[j], s10 = d1[i] * d2[j + 1], s11 = d1[i + 1] * d2[j + 1];
+      const double tarea = s00 + s01 + s10 + s11;
+      auto FX = [&](int p, int q) { return ff3[ix(r.n3 + 1, r.n2, r.n1, nvar, it.fb, n, k, j + p, i + q)]; };
+      const double v = (FX(0, 0) * s00 + FX(0, 1) * s01 + FX(1, 0) * s10 + FX(1, 1) * s11) / tarea;
+      cf3[ix(r.n3 + 1, r.n2, r.n1, nvar, it.cb, n, kc, r.js + (it.fi2 ? r.h2 : 0) + a, r.is + (it.fi1 ? r.h1 : 0) + b)] = v;
+    }
+  }
+}
+
+// number of values of the reference's EMF buffer to a coarser face neighbour (3D):
+// two edge components, each restricted over pairs of fine edges
+__host__ __device__ inline int64_t emf_part(const RG &r, int dir, int part) {
+  if (dir == 0) return part == 0 ? (int64_t)(r.h3 + 1) * r.h2 : (int64_t)r.h3 * (r.h2 + 1);
+  if (dir == 1) return part == 0 ? (int64_t)(r.h3 + 1) * r.h1 : (int64_t)r.h3 * (r.h1 + 1);
+  return part == 0 ? (int64_t)(r.h2 + 1) * r.h1 : (int64_t)r.h2 * (r.h1 + 1);
+}
+
+// LoadFluxBoundaryBufferToCoarser for face neighbours of a 3D block,
+// flux_correction_fc.cpp:315-418: the buffer order of the reference
+__global__ void __launch_bounds__(RBS)
+k_ref_emf_to_coarser(RG r, const double *__restrict__ e1, const double *__restrict__ e2, const double *__restrict__ e3,
+                     agx_ref_coords fc, double *__restrict__ buf, int64_t stride,
+                     const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const int dir = it.fid >> 1, side = it.fid & 1, fb = it.fb;
+  const int64_t pa = emf_part(r, dir, 0), cnt = pa + emf_part(r, dir, 1);
+  const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
+               *d3 = fc.dx3f + (int64_t)fb * r.n3;
+  auto E1 = [&](int k, int j, int i) { return e1[ix(r.n3 + 1, r.n2 + 1, r.n1, 1, fb, 0, k, j, i)]; };
+  auto E2 = [&](int k, int j, int i) { return e2[ix(r.n3 + 1, r.n2, r.n1 + 1, 1, fb, 0, k, j, i)]; };
+  auto E3 = [&](int k, int j, int i) { return e3[ix(r.n3, r.n2 + 1, r.n1 + 1, 1, fb, 0, k, j, i)]; };
+  double *out = buf + (int64_t)blockIdx.y * stride;
+  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
+    const bool first = t < pa;
+    const int64_t s = first ? t : t - pa;
+    double v;
+    if (dir == 0) {
+      const int i = side ? r.ie + 1 : r.is;
+      if (first) {   // e2, k = ks..ke+1 step 2, j = js..je step 2
+        const int k = r.ks + 2 * (int)(s / r.h2), j = r.js + 2 * (int)(s % r.h2);
+        v = (E2(k, j, i) * d2[j] + E2(k, j + 1, i) * d2[j + 1]) / (d2[j] + d2[j + 1]);
+      } else {       // e3, k = ks..ke step 2, j = js..je+1 step 2
+        const int k = r.ks + 2 * (int)(s / (r.h2 + 1)), j = r.js + 2 * (int)(s % (r.h2 + 1));
+        v = (E3(k, j, i) * d3[k] + E3(k + 1, j, i) * d3[k + 1]) / (d3[k] + d3[k + 1]);
+      }
+    } else if (dir == 1) {
+      const int j = side ? r.je + 1 : r.js;
+      if (first) {   // e1, k = ks..ke+1 step 2, i = is..ie step 2
+        const int k = r.ks + 2 * (int)(s / r.h1), i = r.is + 2 * (int)(s % r.h1);
+        v = (E1(k, j, i) * d1[i] + E1(k, j, i + 1) * d1[i + 1]) / (d1[i] + d1[i + 1]);
+      } else {       // e3, k = ks..ke step 2, i = is..ie+1 step 2
+        const int k = r.ks + 2 * (int)(s / (r.h1 + 1)), i = r.is + 2 * (int)(s % (r.h1 + 1));
+        v = (E3(k, j, i) * d3[k] + E3(k + 1, j, i) * d3[k + 1]) / (d3[k] + d3[k + 1]);
+      }
+    } else {
+      const int k = side ? r.ke + 1 : r.ks;
+      if (first) {   // e1, j = js..je+1 step 2, i = is..ie step 2
+        const int j = r.js + 2 * (int)(s / r.h1), i = r.is + 2 * (int)(s % r.h1);
+        v = (E1(k, j, i) * d1[i] + E1(k, j, i + 1) * d1[i + 1]) / (d1[i] + d1[i + 1]);
+      } else {       // e2, j = js..je step 2, i = is..ie+1 step 2
+        const int j = r.js + 2 * (int)(s / (r.h1 + 1)), i = r.is + 2 * (int)(s % (r.h1 + 1));
+        v = (E2(k, j, i) * d2[j] + E2(k, j + 1, i) * d2[j + 1]) / (d2[j] + d2[j + 1]);
+      }
+    }
+    out[t] = v;
+  }
+}
+
+// ---------------------------------------------------------------- host side
+bool in(int lo, int hi, int mn, int mx) { return lo <= hi && lo >= mn && hi <= mx; }
+
+// fine index range of coarse range [lo, hi] (f = (c - cs) * 2 + fs), plus `extra` cells
+bool fine_in(int lo, int hi, int cs, int fs, int extra, int mx) {
+  return in((lo - cs) * 2 + fs, (hi - cs) * 2 + fs + extra, 0, mx);
+}
+
+int check_item(const RG &r, int kind, const agx_ref_item &it, int nbf, int nbc, int m) {
+  auto bad = [&](const char *what) {
+    set_error("refinement work list, item %d: %s", m, what);
+    return -40;
+  };
+  const bool uses_cb = kind != AGX_REF_PROLONG_FC_INT && kind != AGX_REF_EMF_TO_COARSER;
+  if (it.fb < 0 || it.fb >= nbf) return bad("fine-side block index out of range");
+  if (uses_cb && (it.cb < 0 || it.cb >= nbc)) return bad("coarse-side block index out of range");
+  const int cs[3] = {r.cis, r.cjs, r.cks}, fs[3] = {r.is, r.js, r.ks};
+  const int cn[3] = {r.c1, r.c2, r.c3}, fn[3] = {r.n1, r.n2, r.n3};
+  const int lo[3] = {it.si, it.sj, it.sk}, hi[3] = {it.ei, it.ej, it.ek};
+  switch (kind) {
+    case AGX_REF_RESTRICT_CC:
+      for (int d = 0; d < 3; ++d) {
+        if (!in(lo[d], hi[d], 0, cn[d] - 1)) return bad("coarse range outside the coarse buffer");
+        if (!fine_in(lo[d], hi[d], cs[d], fs[d], 1, fn[d] - 1)) return bad("fine cells outside the block");
+      }
+      return 0;
+    case AGX_REF_RESTRICT_FC1: case AGX_REF_RESTRICT_FC2: case AGX_REF_RESTRICT_FC3: {
+      const int dir = kind - AGX_REF_RESTRICT_FC1;
+      for (int d = 0; d < 3; ++d) {
+        const int e = d == dir;   // one more face than cells along the face normal
+        if (!in(lo[d], hi[d], 0, cn[d] - 1 + e)) return bad("coarse range outside the coarse faces");
+        if (!fine_in(lo[d], hi[d], cs[d], fs[d], e ? 0 : 1, fn[d] - 1 + e)) return bad("fine faces outside the block");
+      }
+      return 0;
+    }
+    case AGX_REF_PROLONG_CC:
+      for (int d = 0; d < 3; ++d) {
+        if (!in(lo[d] - 1, hi[d] + 1, 0, cn[d] - 1)) return bad("coarse stencil outside the coarse buffer");
+        if (!fine_in(lo[d], hi[d], cs[d], fs[d], 1, fn[d] - 1)) return bad("fine cells outside the block");
+      }
+      return 0;
+    case AGX_REF_PROLONG_FC1: case AGX_REF_PROLONG_FC2: case AGX_REF_PROLONG_FC3: {
+      const int dir = kind - AGX_REF_PROLONG_FC1;
+      for (int d = 0; d < 3; ++d) {
+        if (d == dir) {
+          if (!in(lo[d], hi[d], 0, cn[d])) return bad("coarse range outside the coarse faces");
+          if (!fine_in(lo[d], hi[d], cs[d], fs[d], 0, fn[d])) return bad("fine faces outside the block");
+        } else {
+          if (!in(lo[d] - 1, hi[d] + 1, 0, cn[d] - 1)) return bad("coarse stencil outside the coarse faces");
+          if (!fine_in(lo[d], hi[d], cs[d], fs[d], 1, fn[d] - 1)) return bad("fine faces outside the block");
+        }
+      }
+      return 0;
+    }
+    case AGX_REF_PROLONG_FC_INT:
+      for (int d = 0; d < 3; ++d) {
+        // faces fi..fi+2 along d (face dims n+1), cells fi..fi+1 across
+        if (!in(lo[d], hi[d], 0, cn[d] - 1)) return bad("coarse range outside the coarse buffer");
+        if (!fine_in(lo[d], hi[d], cs[d], fs[d], 2, fn[d])) return bad("fine faces outside the block");
+      }
+      return 0;
+    case AGX_REF_FLUX_FROM_FINER:
+      if (it.fid < 0 || it.fid > 5) return bad("fid must be 0..5");
+      if ((it.fi1 & ~1) || (it.fi2 & ~1)) return bad("fi1 and fi2 must be 0 or 1");
+      return 0;
+    case AGX_REF_EMF_TO_COARSER:
+      if (it.fid < 0 || it.fid > 5) return bad("fid must be 0..5");
+      return 0;
+    default:
+      return bad("unknown operator kind");
+  }
+}
+
+int64_t item_count(const RG &r, int kind, const agx_ref_item &it) {
+  if (kind == AGX_REF_FLUX_FROM_FINER) {
+    const int dir = it.fid >> 1;
+    return (int64_t)(dir == 2 ? r.h2 : r.h3) * (dir == 0 ? r.h2 : r.h1);
+  }
+  if (kind == AGX_REF_EMF_TO_COARSER) return emf_part(r, it.fid >> 1, 0) + emf_part(r, it.fid >> 1, 1);
+  return (int64_t)(it.ei - it.si + 1) * (it.ej - it.sj + 1) * (it.ek - it.sk + 1);
+}
+
+int check_call(const agx_ref_list *l, int kind_lo, int kind_hi, const char *what) {
+  if (!l) { set_error("%s: null work list", what); return -41; }
+  if (l->kind < kind_lo || l->kind > kind_hi) {
+    set_error("%s: work list was built for operator kind %d", what, l->kind);
+    return -42;
+  }
+  return 0;
+}
+
+int launch_status(const char *what) {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, what);
+}
+
+}  // namespace
+}  // namespace agx
+
+using namespace agx;
+
+extern "C" {
+
+int agx_ref_list_create(const agx_ref_geom *g, int32_t kind, const agx_ref_item *items, int32_t nitems,
+                        int32_t nb_fine, int32_t nb_coarse, agx_ref_list **out) {
+  if (!g || !out || (nitems > 0 && !items) || nitems < 0) { set_error("agx_ref_list_create: null argument"); return -1; }
+  *out = nullptr;
+  if (kind < 0 || kind >= AGX_REF_KINDS) { set_error("agx_ref_list_create: unknown kind %d", kind); return -43; }
+  if (g->nx1 < 2 || g->nx2 < 2 || g->nx3 < 2 || (g->nx1 | g->nx2 | g->nx3) & 1) {
+    set_error("refinement operators need 3D blocks with even nx1, nx2, nx3 >= 2"); return -44;
+  }
+  if (g->ng < 1 || g->cng < 1) { set_error("ng and cng must be >= 1"); return -45; }
+  if (nitems > 65535) { set_error("at most 65535 items per work list"); return -46; }
+  const RG r = make_rg(*g);
+  int64_t maxc = 0, emf_max = 0;
+  for (int m = 0; m < nitems; ++m) {
+    int rc = check_item(r, kind, items[m], nb_fine, nb_coarse, m);
+    if (rc) return rc;
+    const int64_t c = item_count(r, kind, items[m]);
+    maxc = std::max(maxc, c);
+    if (kind == AGX_REF_EMF_TO_COARSER) emf_max = std::max(emf_max, c);
+  }
+  agx_ref_list *l = new agx_ref_list;
+  l->g = *g;
+  l->kind = kind;
+  l->nitems = nitems;
+  l->emf_max = emf_max;
+  const int nvar_hint = (kind == AGX_REF_RESTRICT_CC || kind == AGX_REF_PROLONG_CC ||
+                         kind == AGX_REF_FLUX_FROM_FINER) ? 5 : 1;
+  l->G = (int)std::min<int64_t>(64, std::max<int64_t>(1, (maxc * nvar_hint + RBS - 1) / RBS));
+  if (nitems > 0) {
+    hipError_t e = hipMalloc(&l->items, sizeof(agx_ref_item) * nitems);
+    if (e == hipSuccess) e = hipMemcpy(l->items, items, sizeof(agx_ref_item) * nitems, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (l->items) hipFree(l->items);
+      delete l;
+      return hip_fail(e, "agx_ref_list_create");
+    }
+  }
+  *out = l;
+  return 0;
+}
+
+int agx_ref_list_destroy(agx_ref_list *l) {
+  if (!l) return 0;
+  if (l->items) hipFree(l->items);
+  delete l;
+  return 0;
+}
+
+int64_t agx_ref_emf_count(const agx_ref_geom *g, int32_t fid) {
+  if (!g || fid < 0 || fid > 5) return -1;
+  const RG r = make_rg(*g);
+  return emf_part(r, fid >> 1, 0) + emf_part(r, fid >> 1, 1);
+}
+
+int agx_ref_restrict_cc(const agx_ref_list *l, int32_t nvar, const double *fine, agx_ref_coords fc,
+                        double *coarse, void *stream) {
+  if (int rc = check_call(l, AGX_REF_RESTRICT_CC, AGX_REF_RESTRICT_CC, "agx_ref_restrict_cc")) return rc;
+  if (nvar < 1) { set_error("agx_ref_restrict_cc: nvar must be >= 1"); return -47; }
+  if (l->nitems == 0) return 0;
+  hipLaunchKernelGGL(k_ref_restrict_cc, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                     make_rg(l->g), nvar, fine, fc, coarse, l->items);
+  return launch_status("agx_ref_restrict_cc");
+}
+
+int agx_ref_restrict_fc(const agx_ref_list *l, const double *fine, agx_ref_coords fc, double *coarse, void *stream) {
+  if (int rc = check_call(l, AGX_REF_RESTRICT_FC1, AGX_REF_RESTRICT_FC3, "agx_ref_restrict_fc")) return rc;
+  if (l->nitems == 0) return 0;
+  const dim3 grid(l->G, l->nitems);
+  const RG r = make_rg(l->g);
+  const hipStream_t s = (hipStream_t)stream;
+  if (l->kind == AGX_REF_RESTRICT_FC1)
+    hipLaunchKernelGGL(k_ref_restrict_fc<0>, grid, dim3(RBS), 0, s, r, fine, fc, coarse, l->items);
+  else if (l->kind == AGX_REF_RESTRICT_FC2)
+    hipLaunchKernelGGL(k_ref_restrict_fc<1>, grid, dim3(RBS), 0, s, r, fine, fc, coarse, l->items);
+  else
+    hipLaunchKernelGGL(k_ref_restrict_fc<2>, grid, dim3(RBS), 0, s, r, fine, fc, coarse, l->items);
+  return launch_status("agx_ref_restrict_fc");
+}
+
+int agx_ref_prolong_cc(const agx_ref_list *l, int32_t nvar, const double *coarse, agx_ref_coords cc,
+                       double *fine, agx_ref_coords fc, void *stream) {
+  if (int rc = check_call(l, AGX_REF_PROLONG_CC, AGX_REF_PROLONG_CC, "agx_ref_prolong_cc")) return rc;
+  if (nvar < 1) { set_error("agx_ref_prolong_cc: nvar must be >= 1"); return -47; }
+  if (l->nitems == 0) return 0;
+  hipLaunchKernelGGL(k_ref_prolong_cc, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                     make_rg(l->g), nvar, coarse, cc, fine, fc, l->items);
+  return launch_status("agx_ref_prolong_cc");
+}
+
+int agx_ref_prolong_fc_shared(const agx_ref_list *l, const double *coarse, agx_ref_coords cc, double *fine,
+                              agx_ref_coords fc, void *stream) {
+  if (int rc = check_call(l, AGX_REF_PROLONG_FC1, AGX_REF_PROLONG_FC3, "agx_ref_prolong_fc_shared")) return rc;
+  if (l->nitems == 0) return 0;
+  const dim3 grid(l->G, l->nitems);
+  const RG r = make_rg(l->g);
+  const hipStream_t s = (hipStream_t)stream;
+  if (l->kind == AGX_REF_PROLONG_FC1)
+    hipLaunchKernelGGL(k_ref_prolong_fc_shared<0>, grid, dim3(RBS), 0, s, r, coarse, cc, fine, fc, l->items);
+  else if (l->kind == AGX_REF_PROLONG_FC2)
+    hipLaunchKernelGGL(k_ref_prolong_fc_shared<1>, grid, dim3(RBS), 0, s, r, coarse, cc, fine, fc, l->items);
+  else
+    hipLaunchKernelGGL(k_ref_prolong_fc_shared<2>, grid, dim3(RBS), 0, s, r, coarse, cc, fine, fc, l->items);
+  return launch_status("agx_ref_prolong_fc_shared");
+}
+
+int agx_ref_prolong_fc_internal(const agx_ref_list *l, double *b1f, double *b2f, double *b3f, agx_ref_coords fc,
+                                void *stream) {
+  if (int rc = check_call(l, AGX_REF_PROLONG_FC_INT, AGX_REF_PROLONG_FC_INT, "agx_ref_prolong_fc_internal"))
+    return rc;
+  if (l->nitems == 0) return 0;
+  hipLaunchKernelGGL(k_ref_prolong_fc_internal, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                     make_rg(l->g), b1f, b2f, b3f, fc, l->items);
+  return launch_status("agx_ref_prolong_fc_internal");
+}
+
+int agx_ref_flux_from_finer(const agx_ref_list *l, int32_t nvar, const double *ff1, const double *ff2,
+                            const double *ff3, agx_ref_coords fc, double *cf1, double *cf2, double *cf3,
+                            void *stream) {
+  if (int rc = check_call(l, AGX_REF_FLUX_FROM_FINER, AGX_REF_FLUX_FROM_FINER, "agx_ref_flux_from_finer"))
+    return rc;
+  if (nvar < 1) { set_error("agx_ref_flux_from_finer: nvar must be >= 1"); return -47; }
+  if (l->nitems == 0) return 0;
+  hipLaunchKernelGGL(k_ref_flux_from_finer, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                     make_rg(l->g), nvar, ff1, ff2, ff3, fc, cf1, cf2, cf3, l->items);
+  return launch_status("agx_ref_flux_from_finer");
+}
+
+int agx_ref_emf_to_coarser(const agx_ref_list *l, const double *e1, const double *e2, const double *e3,
+                           agx_ref_coords fc, double *buf, int64_t stride, void *stream) {
+  if (int rc = check_call(l, AGX_REF_EMF_TO_COARSER, AGX_REF_EMF_TO_COARSER, "agx_ref_emf_to_coarser")) return rc;
+  if (stride < l->emf_max) {
+    set_error("agx_ref_emf_to_coarser: stride %lld below the largest item (%lld values)", (long long)stride,
+              (long long)l->emf_max);
+    return -48;
+  }
+  if (l->nitems == 0) return 0;
+  hipLaunchKernelGGL(k_ref_emf_to_coarser, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                     make_rg(l->g), e1, e2, e3, fc, buf, stride, l->items);
+  return launch_status("agx_ref_emf_to_coarser");
+}
+
+int agx_ref_sync(void *stream) {
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "agx_ref_sync");
+}
+
+}  // extern "C"

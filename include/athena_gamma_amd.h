@@ -6,8 +6,8 @@
  * fields, fluxes, EMFs, potential, multigrid hierarchy) in HBM.  Each entry point runs
  * one task of the reference's task lists for ALL MeshBlocks in one batch of kernel
  * launches on the mesh's HIP stream, and replaces the reference method cited beside
- * it (paths relative to the reference's src/).  No entry point allocates after
- * agx_mesh_create, except agx_mesh_create itself.
+ * it (paths relative to the reference's src/).  Device memory is allocated only by
+ * agx_mesh_create and agx_ref_list_create.
  *
  * Conventions
  *   - Return value: 0 on success; > 0 a HIP error code; < 0 a contract violation
@@ -192,6 +192,93 @@ int agx_set_tile(agx_mesh *m, int32_t tile);
  * host, multigrid_driver.cpp:721-755) and while the built-in timers are off; otherwise
  * agx_cycle runs eagerly.  On by default; enable = 0 selects the eager path. */
 int agx_set_graph(agx_mesh *m, int32_t enable);
+
+/* ---- static-mesh-refinement operators.
+ * Batched forms of the per-MeshBlock refinement routines for 3D Cartesian blocks.  They
+ * work on device arrays in the reference's dense AthenaArray order, block-major:
+ *   cells      (nb, nvar, n3, n2, n1)
+ *   faces      x1f (nb, n3, n2, n1+1), x2f (nb, n3, n2+1, n1), x3f (nb, n3+1, n2, n1)
+ *   fluxes     as faces, with nvar
+ *   edges      x1e (nb, n3+1, n2+1, n1), x2e (nb, n3+1, n2, n1+1), x3e (nb, n3, n2+1, n1+1)
+ * Blocks have n = nx + 2 ng cells per direction.  Their coarse buffers (coarse_cons_,
+ * coarse_b_) have nc = nx/2 + 2 cng, coarse interior from index cng; the reference uses
+ * cng = NGHOST (mesh/meshblock.cpp:59-85: ncc1 = nx1/2 + 2*NGHOST, cis = NGHOST).  Coordinates are
+ * per-block device tables x?v and dx?f of length n (blocks) or nc (coarse buffers).  On
+ * Cartesian blocks the face positions x?s? are x?v (coordinates/cartesian.cpp:106-119),
+ * areas and volumes are products of dx?f, and edge lengths are dx?f
+ * (coordinates/coordinates.cpp:328-537).
+ * A work list (agx_ref_list) holds the items of one operator in device memory.  It is
+ * built and checked once, since refinement is static; every item of a list runs in one
+ * launch.  Items: fb = block on the fine side, cb = block or coarse buffer on the coarse
+ * side, [si..ei] x [sj..ej] x [sk..ek] = coarse index range (the reference's
+ * csi..cek / si..ek arguments), fid / fi1 / fi2 = face and quadrant (NeighborBlock::fid,
+ * NeighborIndexes::fi1/fi2).  All calls are asynchronous on `stream` (a hipStream_t, or
+ * NULL for the default stream).  Items of one list must not write the same element. */
+typedef struct {
+  int32_t fb, cb;
+  int32_t si, ei, sj, ej, sk, ek;
+  int32_t fid, fi1, fi2, pad;
+} agx_ref_item;
+typedef struct {
+  int32_t nx1, nx2, nx3;   /* interior cells of a block (even) */
+  int32_t ng, cng;         /* ghost width of the blocks and of the coarse buffers (= ng in
+                              the reference) */
+  int32_t pad;
+} agx_ref_geom;
+typedef struct {
+  const double *x1v, *x2v, *x3v, *dx1f, *dx2f, *dx3f;
+} agx_ref_coords;
+enum {
+  AGX_REF_RESTRICT_CC = 0,     /* MeshRefinement::RestrictCellCenteredValues, mesh_refinement.cpp:106-136 */
+  AGX_REF_RESTRICT_FC1 = 1,    /* RestrictFieldX1, mesh_refinement.cpp:185-214 */
+  AGX_REF_RESTRICT_FC2 = 2,    /* RestrictFieldX2, mesh_refinement.cpp:244-279 */
+  AGX_REF_RESTRICT_FC3 = 3,    /* RestrictFieldX3, mesh_refinement.cpp:321-344 */
+  AGX_REF_PROLONG_CC = 4,      /* ProlongateCellCenteredValues, mesh_refinement.cpp:386-458 */
+  AGX_REF_PROLONG_FC1 = 5,     /* ProlongateSharedFieldX1, mesh_refinement.cpp:541-581 */
+  AGX_REF_PROLONG_FC2 = 6,     /* ProlongateSharedFieldX2, mesh_refinement.cpp:626-674 */
+  AGX_REF_PROLONG_FC3 = 7,     /* ProlongateSharedFieldX3, mesh_refinement.cpp:718-762 */
+  AGX_REF_PROLONG_FC_INT = 8,  /* ProlongateInternalField, mesh_refinement.cpp:831-946 */
+  AGX_REF_FLUX_FROM_FINER = 9, /* flux_correction_cc.cpp:77-169 (LoadFluxBoundaryBufferToCoarser)
+                                  + 240-283 (SetFluxBoundaryFromFiner): fb = finer block,
+                                  cb = coarser block, fid = face of cb, fi1/fi2 = quadrant */
+  AGX_REF_EMF_TO_COARSER = 10, /* flux_correction_fc.cpp:315-418 (LoadFluxBoundaryBufferToCoarser,
+                                  3D face neighbours): fb = finer block, fid = its face */
+  AGX_REF_KINDS = 11
+};
+typedef struct agx_ref_list agx_ref_list;
+/* Checks every item against the geometry and the pool sizes (nb_fine blocks on the fine
+ * side, nb_coarse on the coarse side) for operator `kind`, and copies the list to the
+ * device. */
+int agx_ref_list_create(const agx_ref_geom *g, int32_t kind, const agx_ref_item *items, int32_t nitems,
+                        int32_t nb_fine, int32_t nb_coarse, agx_ref_list **out);
+int agx_ref_list_destroy(agx_ref_list *l);
+/* number of values one item of an AGX_REF_EMF_TO_COARSER list writes (the buffer size
+ * of the reference) for face fid */
+int64_t agx_ref_emf_count(const agx_ref_geom *g, int32_t fid);
+/* fine cells -> coarse buffer (kind RESTRICT_CC) */
+int agx_ref_restrict_cc(const agx_ref_list *l, int32_t nvar, const double *fine, agx_ref_coords fine_coords,
+                        double *coarse, void *stream);
+/* fine faces -> coarse faces of the direction of the list (RESTRICT_FC1..3) */
+int agx_ref_restrict_fc(const agx_ref_list *l, const double *fine, agx_ref_coords fine_coords, double *coarse,
+                        void *stream);
+/* coarse buffer -> fine cells, minmod-limited (PROLONG_CC) */
+int agx_ref_prolong_cc(const agx_ref_list *l, int32_t nvar, const double *coarse, agx_ref_coords coarse_coords,
+                       double *fine, agx_ref_coords fine_coords, void *stream);
+/* coarse faces -> fine faces shared with the coarse grid (PROLONG_FC1..3) */
+int agx_ref_prolong_fc_shared(const agx_ref_list *l, const double *coarse, agx_ref_coords coarse_coords,
+                              double *fine, agx_ref_coords fine_coords, void *stream);
+/* interior fine faces from the shared ones, divergence-preserving (PROLONG_FC_INT) */
+int agx_ref_prolong_fc_internal(const agx_ref_list *l, double *b1f, double *b2f, double *b3f,
+                                agx_ref_coords fine_coords, void *stream);
+/* coarser-block face fluxes of each quadrant <- area-weighted finer fluxes (FLUX_FROM_FINER) */
+int agx_ref_flux_from_finer(const agx_ref_list *l, int32_t nvar, const double *fine_x1flux,
+                            const double *fine_x2flux, const double *fine_x3flux, agx_ref_coords fine_coords,
+                            double *coarse_x1flux, double *coarse_x2flux, double *coarse_x3flux, void *stream);
+/* restricted edge EMFs of each item's face into buf + item * stride (EMF_TO_COARSER) */
+int agx_ref_emf_to_coarser(const agx_ref_list *l, const double *e1, const double *e2, const double *e3,
+                           agx_ref_coords fine_coords, double *buf, int64_t stride, void *stream);
+/* waits for `stream` */
+int agx_ref_sync(void *stream);
 
 /* ---- built-in timers (HIP events on the mesh stream around each launch) */
 int agx_timing_enable(agx_mesh *m, int enable);

@@ -102,6 +102,39 @@ void orc_plm_pencil(int n, const double *q, const double *dxp, const double *dxm
 void orc_ppm_pencil(int n, const double *q, double *wl, double *wr);
 void orc_conserved_totals(orc_ctx *c, double *out8);
 
+/* static-mesh-refinement operators (agx_oracle_refine.c), 3D Cartesian blocks in the
+ * dense layouts described there; items are processed in list order */
+typedef struct {
+  int32_t fb, cb;                   /* fine-side and coarse-side block */
+  int32_t si, ei, sj, ej, sk, ek;   /* coarse index range (inclusive) */
+  int32_t fid, fi1, fi2, pad;       /* face and quadrant (flux/EMF correction) */
+} orc_ref_item;
+typedef struct {
+  int32_t nx1, nx2, nx3;            /* interior cells of a block */
+  int32_t ng, cng;                  /* ghost width of the blocks and of the coarse buffers */
+  int32_t pad;
+} orc_ref_geom;
+typedef struct {
+  const double *x1v, *x2v, *x3v, *dx1f, *dx2f, *dx3f;   /* per block, length n */
+} orc_ref_coords;
+void orc_ref_restrict_cc(const orc_ref_geom *g, int nvar, const double *fine, orc_ref_coords fc,
+                         double *coarse, const orc_ref_item *it, int nit);
+void orc_ref_restrict_fc(const orc_ref_geom *g, int dir, const double *fine, orc_ref_coords fc,
+                         double *coarse, const orc_ref_item *it, int nit);
+void orc_ref_prolong_cc(const orc_ref_geom *g, int nvar, const double *coarse, orc_ref_coords cc,
+                        double *fine, orc_ref_coords fc, const orc_ref_item *it, int nit);
+void orc_ref_prolong_fc_shared(const orc_ref_geom *g, int dir, const double *coarse,
+                               orc_ref_coords cc, double *fine, orc_ref_coords fc,
+                               const orc_ref_item *it, int nit);
+void orc_ref_prolong_fc_internal(const orc_ref_geom *g, double *b1, double *b2, double *b3,
+                                 orc_ref_coords fc, const orc_ref_item *it, int nit);
+void orc_ref_flux_from_finer(const orc_ref_geom *g, int nvar, const double *ff1, const double *ff2,
+                             const double *ff3, orc_ref_coords fc, double *cf1, double *cf2,
+                             double *cf3, const orc_ref_item *it, int nit);
+void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const double *e2,
+                            const double *e3, orc_ref_coords fc, double *buf, int64_t stride,
+                            const orc_ref_item *it, int nit);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,449 @@
+/* agx_oracle_refine.c -- CPU restatement of the static-mesh-refinement operators
+ * (test infrastructure: the checker of the product's refinement kernels, never shipped
+ * or called by the product).
+ *
+ * Each function restates one reference routine for 3D Cartesian blocks, in plain
+ * FP64 C with the reference's operand order, over a list of work items:
+ *   orc_ref_restrict_cc        MeshRefinement::RestrictCellCenteredValues
+ *                              (mesh/mesh_refinement.cpp:106-136)
+ *   orc_ref_restrict_fc        RestrictFieldX1/X2/X3 (mesh_refinement.cpp:185-214,
+ *                              244-279, 321-344)
+ *   orc_ref_prolong_cc         ProlongateCellCenteredValues (mesh_refinement.cpp:386-458)
+ *   orc_ref_prolong_fc_shared  ProlongateSharedFieldX1/X2/X3 (mesh_refinement.cpp:541-581,
+ *                              626-674, 718-762)
+ *   orc_ref_prolong_fc_internal ProlongateInternalField (mesh_refinement.cpp:831-946)
+ *   orc_ref_flux_from_finer    CellCenteredBoundaryVariable::LoadFluxBoundaryBufferToCoarser
+ *                              + SetFluxBoundaryFromFiner (bvals/cc/flux_correction_cc.cpp:
+ *                              77-169, 230-283), the buffer replaced by a direct write
+ *   orc_ref_emf_to_coarser     FaceCenteredBoundaryVariable::LoadFluxBoundaryBufferToCoarser,
+ *                              face neighbours of a 3D block (bvals/fc/flux_correction_fc.cpp:
+ *                              315-418), packed in the reference's buffer order
+ *
+ * Layouts (dense, AthenaArray order, block-major): cells (nb, nvar, n3, n2, n1); faces
+ * x1f (nb, n3, n2, n1+1), x2f (nb, n3, n2+1, n1), x3f (nb, n3+1, n2, n1); hydro fluxes
+ * as faces with nvar; edges x1e (nb, n3+1, n2+1, n1), x2e (nb, n3+1, n2, n1+1),
+ * x3e (nb, n3, n2+1, n1+1).  Fine blocks have n = nx + 2 ng cells per direction, the
+ * coarse buffers nc = nx/2 + 2 cng with their interior from cng (the reference: cng =
+ * NGHOST, mesh/meshblock.cpp:59-85).  Coordinates are per block:
+ * x?v and dx?f of length n (fine) or nc (coarse); on Cartesian blocks the face-area
+ * positions x?s? equal x?v (coordinates/cartesian.cpp:106-119), areas and volumes are
+ * products of dx?f (coordinates/coordinates.cpp:436-537) and edge lengths are dx?f
+ * (coordinates.cpp:367-373).
+ */
+#include <math.h>
+#include <stdint.h>
+
+#include "agx_oracle.h"
+
+#define REF_SIGN(x) (((x) < 0.0) ? -1.0 : 1.0)   /* SIGN, defs.hpp.in:117 */
+
+static inline double minmod_slope(double gm, double gp) {
+  return 0.5 * (REF_SIGN(gm) + REF_SIGN(gp)) * fmin(fabs(gm), fabs(gp));
+}
+
+/* element (b, n, k, j, i) of a block-major array of nvar variables of d3 x d2 x d1 */
+static inline int64_t ix(int64_t d3, int64_t d2, int64_t d1, int64_t nvar, int64_t b, int64_t n,
+                         int64_t k, int64_t j, int64_t i) {
+  return ((((b * nvar + n) * d3 + k) * d2 + j) * d1) + i;
+}
+
+typedef struct {
+  int n1, n2, n3;     /* fine block cells incl. ghosts */
+  int c1, c2, c3;     /* coarse buffer cells incl. ghosts */
+  int is, js, ks;     /* fine interior start (= ng) */
+  int ie, je, ke;
+  int cis, cjs, cks;  /* coarse interior start (= cng) */
+} rgeo;
+
+static rgeo mkgeo(const orc_ref_geom *g) {
+  rgeo r;
+  r.n1 = g->nx1 + 2 * g->ng; r.n2 = g->nx2 + 2 * g->ng; r.n3 = g->nx3 + 2 * g->ng;
+  r.c1 = g->nx1 / 2 + 2 * g->cng; r.c2 = g->nx2 / 2 + 2 * g->cng; r.c3 = g->nx3 / 2 + 2 * g->cng;
+  r.is = r.js = r.ks = g->ng;
+  r.ie = g->ng + g->nx1 - 1; r.je = g->ng + g->nx2 - 1; r.ke = g->ng + g->nx3 - 1;
+  r.cis = r.cjs = r.cks = g->cng;
+  return r;
+}
+
+/* ---------------------------------------------------------------- restriction */
+void orc_ref_restrict_cc(const orc_ref_geom *g, int nvar, const double *fine, orc_ref_coords fc,
+                         double *coarse, const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  for (int m = 0; m < nit; ++m) {
+    const int fb = it[m].fb, cb = it[m].cb;
+    const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
+                 *d3 = fc.dx3f + (int64_t)fb * r.n3;
+    for (int n = 0; n < nvar; ++n)
+      for (int ck = it[m].sk; ck <= it[m].ek; ++ck) {
+        int k = (ck - r.cks) * 2 + r.ks;
+        for (int cj = it[m].sj; cj <= it[m].ej; ++cj) {
+          int j = (cj - r.cjs) * 2 + r.js;
+          for (int ci = it[m].si; ci <= it[m].ei; ++ci) {
+            int i = (ci - r.cis) * 2 + r.is;
+            /* CellVolume: dx1f(i)*dx2f(j)*dx3f(k); v[a][b][c] at (k+a, j+b, i+c) */
+            double v[2][2][2];
+            for (int a = 0; a < 2; ++a)
+              for (int bb = 0; bb < 2; ++bb)
+                for (int c = 0; c < 2; ++c) v[a][bb][c] = d1[i + c] * d2[j + bb] * d3[k + a];
+            double tvol = ((v[0][0][0] + v[0][1][0]) + (v[0][0][1] + v[0][1][1])) +
+                          ((v[1][0][0] + v[1][1][0]) + (v[1][0][1] + v[1][1][1]));
+#define F(a, bb, c) fine[ix(r.n3, r.n2, r.n1, nvar, fb, n, k + (a), j + (bb), i + (c))]
+            double s = (((F(0, 0, 0) * v[0][0][0] + F(0, 1, 0) * v[0][1][0]) +
+                         (F(0, 0, 1) * v[0][0][1] + F(0, 1, 1) * v[0][1][1])) +
+                        ((F(1, 0, 0) * v[1][0][0] + F(1, 1, 0) * v[1][1][0]) +
+                         (F(1, 0, 1) * v[1][0][1] + F(1, 1, 1) * v[1][1][1])));
+#undef F
+            coarse[ix(r.c3, r.c2, r.c1, nvar, cb, n, ck, cj, ci)] = s / tvol;
+          }
+        }
+      }
+  }
+}
+
+/* dir 0: x1f (fine (n3, n2, n1+1) -> coarse (c3, c2, c1+1)); 1: x2f; 2: x3f */
+void orc_ref_restrict_fc(const orc_ref_geom *g, int dir, const double *fine, orc_ref_coords fc,
+                         double *coarse, const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  const int f3 = r.n3 + (dir == 2), f2 = r.n2 + (dir == 1), f1 = r.n1 + (dir == 0);
+  const int q3 = r.c3 + (dir == 2), q2 = r.c2 + (dir == 1), q1 = r.c1 + (dir == 0);
+  for (int m = 0; m < nit; ++m) {
+    const int fb = it[m].fb, cb = it[m].cb;
+    const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
+                 *d3 = fc.dx3f + (int64_t)fb * r.n3;
+    for (int ck = it[m].sk; ck <= it[m].ek; ++ck) {
+      int k = (ck - r.cks) * 2 + r.ks;
+      for (int cj = it[m].sj; cj <= it[m].ej; ++cj) {
+        int j = (cj - r.cjs) * 2 + r.js;
+        for (int ci = it[m].si; ci <= it[m].ei; ++ci) {
+          int i = (ci - r.cis) * 2 + r.is;
+#define F(a, bb, c) fine[ix(f3, f2, f1, 1, fb, 0, k + (a), j + (bb), i + (c))]
+          double val;
+          if (dir == 0) {   /* Face1Area(k, j, i) = dx2f(j)*dx3f(k) */
+            double a00 = d2[j] * d3[k], a01 = d2[j + 1] * d3[k];
+            double a10 = d2[j] * d3[k + 1], a11 = d2[j + 1] * d3[k + 1];
+            double tarea = a00 + a01 + a10 + a11;
+            val = (F(0, 0, 0) * a00 + F(0, 1, 0) * a01 + F(1, 0, 0) * a10 + F(1, 1, 0) * a11) / tarea;
+          } else if (dir == 1) {   /* Face2Area(k, j, i) = dx1f(i)*dx3f(k) */
+            double s00 = d1[i] * d3[k], s01 = d1[i + 1] * d3[k];
+            double s10 = d1[i] * d3[k + 1], s11 = d1[i + 1] * d3[k + 1];
+            double tarea = s00 + s01 + s10 + s11;
+            val = (F(0, 0, 0) * s00 + F(0, 0, 1) * s01 + F(1, 0, 0) * s10 + F(1, 0, 1) * s11) / tarea;
+          } else {   /* Face3Area(k, j, i) = dx1f(i)*dx2f(j) */
+            double s00 = d1[i] * d2[j], s01 = d1[i + 1] * d2[j];
+            double s10 = d1[i] * d2[j + 1], s11 = d1[i + 1] * d2[j + 1];
+            double tarea = s00 + s01 + s10 + s11;
+            val = (F(0, 0, 0) * s00 + F(0, 0, 1) * s01 + F(0, 1, 0) * s10 + F(0, 1, 1) * s11) / tarea;
+          }
+#undef F
+          coarse[ix(q3, q2, q1, 1, cb, 0, ck, cj, ci)] = val;
+        }
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- prolongation */
+void orc_ref_prolong_cc(const orc_ref_geom *g, int nvar, const double *coarse, orc_ref_coords cc,
+                        double *fine, orc_ref_coords fc, const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  for (int m = 0; m < nit; ++m) {
+    const int fb = it[m].fb, cb = it[m].cb;
+    const double *X1 = cc.x1v + (int64_t)cb * r.c1, *X2 = cc.x2v + (int64_t)cb * r.c2,
+                 *X3 = cc.x3v + (int64_t)cb * r.c3;
+    const double *x1 = fc.x1v + (int64_t)fb * r.n1, *x2 = fc.x2v + (int64_t)fb * r.n2,
+                 *x3 = fc.x3v + (int64_t)fb * r.n3;
+    for (int n = 0; n < nvar; ++n)
+      for (int k = it[m].sk; k <= it[m].ek; ++k) {
+        int fk = (k - r.cks) * 2 + r.ks;
+        double dx3m = X3[k] - X3[k - 1], dx3p = X3[k + 1] - X3[k];
+        double dx3fm = X3[k] - x3[fk], dx3fp = x3[fk + 1] - X3[k];
+        for (int j = it[m].sj; j <= it[m].ej; ++j) {
+          int fj = (j - r.cjs) * 2 + r.js;
+          double dx2m = X2[j] - X2[j - 1], dx2p = X2[j + 1] - X2[j];
+          double dx2fm = X2[j] - x2[fj], dx2fp = x2[fj + 1] - X2[j];
+          for (int i = it[m].si; i <= it[m].ei; ++i) {
+            int fi = (i - r.cis) * 2 + r.is;
+            double dx1m = X1[i] - X1[i - 1], dx1p = X1[i + 1] - X1[i];
+            double dx1fm = X1[i] - x1[fi], dx1fp = x1[fi + 1] - X1[i];
+#define C(a, bb, c) coarse[ix(r.c3, r.c2, r.c1, nvar, cb, n, k + (a), j + (bb), i + (c))]
+            double cv = C(0, 0, 0);
+            double g1 = minmod_slope((cv - C(0, 0, -1)) / dx1m, (C(0, 0, 1) - cv) / dx1p);
+            double g2 = minmod_slope((cv - C(0, -1, 0)) / dx2m, (C(0, 1, 0) - cv) / dx2p);
+            double g3 = minmod_slope((cv - C(-1, 0, 0)) / dx3m, (C(1, 0, 0) - cv) / dx3p);
+#undef C
+#define FO(a, bb, c) fine[ix(r.n3, r.n2, r.n1, nvar, fb, n, fk + (a), fj + (bb), fi + (c))]
+            FO(0, 0, 0) = cv - (g1 * dx1fm + g2 * dx2fm + g3 * dx3fm);
+            FO(0, 0, 1) = cv + (g1 * dx1fp - g2 * dx2fm - g3 * dx3fm);
+            FO(0, 1, 0) = cv - (g1 * dx1fm - g2 * dx2fp + g3 * dx3fm);
+            FO(0, 1, 1) = cv + (g1 * dx1fp + g2 * dx2fp - g3 * dx3fm);
+            FO(1, 0, 0) = cv - (g1 * dx1fm + g2 * dx2fm - g3 * dx3fp);
+            FO(1, 0, 1) = cv + (g1 * dx1fp - g2 * dx2fm + g3 * dx3fp);
+            FO(1, 1, 0) = cv - (g1 * dx1fm - g2 * dx2fp - g3 * dx3fp);
+            FO(1, 1, 1) = cv + (g1 * dx1fp + g2 * dx2fp + g3 * dx3fp);
+#undef FO
+          }
+        }
+      }
+  }
+}
+
+/* Shared faces: dir 0 prolongates x1f from the coarse x1f buffer, slopes along x2, x3;
+ * dir 1 x2f (slopes x1, x3); dir 2 x3f (slopes x1, x2).  The item range is in coarse
+ * face indices. */
+void orc_ref_prolong_fc_shared(const orc_ref_geom *g, int dir, const double *coarse,
+                               orc_ref_coords cc, double *fine, orc_ref_coords fc,
+                               const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  const int f3 = r.n3 + (dir == 2), f2 = r.n2 + (dir == 1), f1 = r.n1 + (dir == 0);
+  const int q3 = r.c3 + (dir == 2), q2 = r.c2 + (dir == 1), q1 = r.c1 + (dir == 0);
+  for (int m = 0; m < nit; ++m) {
+    const int fb = it[m].fb, cb = it[m].cb;
+    const double *X1 = cc.x1v + (int64_t)cb * r.c1, *X2 = cc.x2v + (int64_t)cb * r.c2,
+                 *X3 = cc.x3v + (int64_t)cb * r.c3;
+    const double *x1 = fc.x1v + (int64_t)fb * r.n1, *x2 = fc.x2v + (int64_t)fb * r.n2,
+                 *x3 = fc.x3v + (int64_t)fb * r.n3;
+    for (int k = it[m].sk; k <= it[m].ek; ++k) {
+      int fk = (k - r.cks) * 2 + r.ks;
+      for (int j = it[m].sj; j <= it[m].ej; ++j) {
+        int fj = (j - r.cjs) * 2 + r.js;
+        for (int i = it[m].si; i <= it[m].ei; ++i) {
+          int fi = (i - r.cis) * 2 + r.is;
+#define C(a, bb, c) coarse[ix(q3, q2, q1, 1, cb, 0, k + (a), j + (bb), i + (c))]
+#define FO(a, bb, c) fine[ix(f3, f2, f1, 1, fb, 0, fk + (a), fj + (bb), fi + (c))]
+          double cv = C(0, 0, 0);
+          if (dir == 0) {
+            double g2 = minmod_slope((cv - C(0, -1, 0)) / (X2[j] - X2[j - 1]),
+                                     (C(0, 1, 0) - cv) / (X2[j + 1] - X2[j]));
+            double g3 = minmod_slope((cv - C(-1, 0, 0)) / (X3[k] - X3[k - 1]),
+                                     (C(1, 0, 0) - cv) / (X3[k + 1] - X3[k]));
+            FO(0, 0, 0) = cv - g2 * (X2[j] - x2[fj]) - g3 * (X3[k] - x3[fk]);
+            FO(0, 1, 0) = cv + g2 * (x2[fj + 1] - X2[j]) - g3 * (X3[k] - x3[fk]);
+            FO(1, 0, 0) = cv - g2 * (X2[j] - x2[fj]) + g3 * (x3[fk + 1] - X3[k]);
+            FO(1, 1, 0) = cv + g2 * (x2[fj + 1] - X2[j]) + g3 * (x3[fk + 1] - X3[k]);
+          } else if (dir == 1) {
+            double g1 = minmod_slope((cv - C(0, 0, -1)) / (X1[i] - X1[i - 1]),
+                                     (C(0, 0, 1) - cv) / (X1[i + 1] - X1[i]));
+            double g3 = minmod_slope((cv - C(-1, 0, 0)) / (X3[k] - X3[k - 1]),
+                                     (C(1, 0, 0) - cv) / (X3[k + 1] - X3[k]));
+            FO(0, 0, 0) = cv - g1 * (X1[i] - x1[fi]) - g3 * (X3[k] - x3[fk]);
+            FO(0, 0, 1) = cv + g1 * (x1[fi + 1] - X1[i]) - g3 * (X3[k] - x3[fk]);
+            FO(1, 0, 0) = cv - g1 * (X1[i] - x1[fi]) + g3 * (x3[fk + 1] - X3[k]);
+            FO(1, 0, 1) = cv + g1 * (x1[fi + 1] - X1[i]) + g3 * (x3[fk + 1] - X3[k]);
+          } else {
+            double g1 = minmod_slope((cv - C(0, 0, -1)) / (X1[i] - X1[i - 1]),
+                                     (C(0, 0, 1) - cv) / (X1[i + 1] - X1[i]));
+            double g2 = minmod_slope((cv - C(0, -1, 0)) / (X2[j] - X2[j - 1]),
+                                     (C(0, 1, 0) - cv) / (X2[j + 1] - X2[j]));
+            FO(0, 0, 0) = cv - g1 * (X1[i] - x1[fi]) - g2 * (X2[j] - x2[fj]);
+            FO(0, 0, 1) = cv + g1 * (x1[fi + 1] - X1[i]) - g2 * (X2[j] - x2[fj]);
+            FO(0, 1, 0) = cv - g1 * (X1[i] - x1[fi]) + g2 * (x2[fj + 1] - X2[j]);
+            FO(0, 1, 1) = cv + g1 * (x1[fi + 1] - X1[i]) + g2 * (x2[fj + 1] - X2[j]);
+          }
+#undef C
+#undef FO
+        }
+      }
+    }
+  }
+}
+
+/* Interior faces of each 2x2x2 group of fine cells from its 24 outer faces; the item
+ * range is in coarse cell indices. */
+void orc_ref_prolong_fc_internal(const orc_ref_geom *g, double *b1, double *b2, double *b3,
+                                 orc_ref_coords fc, const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  for (int m = 0; m < nit; ++m) {
+    const int fb = it[m].fb;
+    const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
+                 *d3 = fc.dx3f + (int64_t)fb * r.n3;
+    for (int k = it[m].sk; k <= it[m].ek; ++k) {
+      int fk = (k - r.cks) * 2 + r.ks;
+      for (int j = it[m].sj; j <= it[m].ej; ++j) {
+        int fj = (j - r.cjs) * 2 + r.js;
+        for (int i = it[m].si; i <= it[m].ei; ++i) {
+          int fi = (i - r.cis) * 2 + r.is;
+#define B1(a, bb, c) b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, (a), (bb), (c))]
+#define B2(a, bb, c) b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, (a), (bb), (c))]
+#define B3(a, bb, c) b3[ix(r.n3 + 1, r.n2, r.n1, 1, fb, 0, (a), (bb), (c))]
+/* areas: A1(a,b,i) = Face1Area(fk+a, fj+b, i) etc. */
+#define A1(a, bb, c) (d2[fj + (bb)] * d3[fk + (a)])
+#define A2(a, bb, c) (d1[(c)] * d3[fk + (a)])
+#define A3(a, bb, c) (d1[(c)] * d2[fj + (bb)])
+          double Uxx = 0.0, Vyy = 0.0, Wzz = 0.0, Uxyz = 0.0, Vxyz = 0.0, Wxyz = 0.0;
+          for (int jj = 0; jj < 2; ++jj) {
+            int js = 2 * jj - 1, fjj = fj + jj, fjp = fj + 2 * jj;
+            for (int ii = 0; ii < 2; ++ii) {
+              int is = 2 * ii - 1, fii = fi + ii, fip = fi + 2 * ii;
+              double x2lo = B2(fk, fjp, fii) * A2(0, 2 * jj, fii);
+              double x2hi = B2(fk + 1, fjp, fii) * A2(1, 2 * jj, fii);
+              double x3lo = B3(fk, fjj, fii) * A3(0, jj, fii);
+              double x3hi = B3(fk + 2, fjj, fii) * A3(2, jj, fii);
+              double x1lo = B1(fk, fjj, fip) * A1(0, jj, fip);
+              double x1hi = B1(fk + 1, fjj, fip) * A1(1, jj, fip);
+              Uxx += is * (js * (x2lo + x2hi) + (x3hi - x3lo));
+              Vyy += js * ((x3hi - x3lo) + is * (x1lo + x1hi));
+              Wzz += is * (x1hi - x1lo) + js * (x2hi - x2lo);
+              Uxyz += is * js * (x1hi - x1lo);
+              Vxyz += is * js * (x2hi - x2lo);
+              Wxyz += is * js * (x3hi - x3lo);
+            }
+          }
+          double Sdx1 = (d1[fi] + d1[fi + 1]) * (d1[fi] + d1[fi + 1]);
+          double Sdx2 = (d2[fj] + d2[fj + 1]) * (d2[fj] + d2[fj + 1]);
+          double Sdx3 = (d3[fk] + d3[fk + 1]) * (d3[fk] + d3[fk + 1]);
+          Uxx *= 0.125; Vyy *= 0.125; Wzz *= 0.125;
+          Uxyz *= 0.125 / (Sdx2 + Sdx3);
+          Vxyz *= 0.125 / (Sdx1 + Sdx3);
+          Wxyz *= 0.125 / (Sdx1 + Sdx2);
+          for (int a = 0; a < 2; ++a)
+            for (int bb = 0; bb < 2; ++bb) {
+              double s3 = a ? 1.0 : -1.0, s2 = bb ? 1.0 : -1.0;
+              /* x1f(fk+a, fj+bb, fi+1): + Uxx, sign(a) Sdx3*Vxyz, sign(bb) Sdx2*Wxyz */
+              double t = 0.5 * (B1(fk + a, fj + bb, fi) * A1(a, bb, fi) +
+                                B1(fk + a, fj + bb, fi + 2) * A1(a, bb, fi + 2)) + Uxx;
+              t = (s3 > 0) ? t + Sdx3 * Vxyz : t - Sdx3 * Vxyz;
+              t = (s2 > 0) ? t + Sdx2 * Wxyz : t - Sdx2 * Wxyz;
+              B1(fk + a, fj + bb, fi + 1) = t / A1(a, bb, fi + 1);
+            }
+          for (int a = 0; a < 2; ++a)
+            for (int c = 0; c < 2; ++c) {
+              /* x2f(fk+a, fj+1, fi+c): + Vyy, sign(a) Sdx3*Uxyz, sign(c) Sdx1*Wxyz */
+              double t = 0.5 * (B2(fk + a, fj, fi + c) * A2(a, 0, fi + c) +
+                                B2(fk + a, fj + 2, fi + c) * A2(a, 2, fi + c)) + Vyy;
+              t = a ? t + Sdx3 * Uxyz : t - Sdx3 * Uxyz;
+              t = c ? t + Sdx1 * Wxyz : t - Sdx1 * Wxyz;
+              B2(fk + a, fj + 1, fi + c) = t / A2(a, 1, fi + c);
+            }
+          for (int bb = 0; bb < 2; ++bb)
+            for (int c = 0; c < 2; ++c) {
+              /* x3f(fk+1, fj+bb, fi+c): + Wzz, sign(bb) Sdx2*Uxyz, sign(c) Sdx1*Vxyz */
+              double t = 0.5 * (B3(fk + 2, fj + bb, fi + c) * A3(2, bb, fi + c) +
+                                B3(fk, fj + bb, fi + c) * A3(0, bb, fi + c)) + Wzz;
+              t = bb ? t + Sdx2 * Uxyz : t - Sdx2 * Uxyz;
+              t = c ? t + Sdx1 * Vxyz : t - Sdx1 * Vxyz;
+              B3(fk + 1, fj + bb, fi + c) = t / A3(1, bb, fi + c);
+            }
+#undef B1
+#undef B2
+#undef B3
+#undef A1
+#undef A2
+#undef A3
+        }
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- flux correction */
+/* Item: fb = the finer block, cb = the coarser block, fid = the face of the coarser block
+ * that the finer block touches (0 inner_x1 .. 5 outer_x3), fi1/fi2 = its quadrant on that
+ * face (NeighborIndexes::fi1/fi2).  The coarser block's face fluxes of that quadrant are
+ * replaced by the area-weighted averages of the finer block's fluxes on its opposite
+ * face. */
+void orc_ref_flux_from_finer(const orc_ref_geom *g, int nvar, const double *ff1, const double *ff2,
+                             const double *ff3, orc_ref_coords fc, double *cf1, double *cf2,
+                             double *cf3, const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  const int h1 = g->nx1 / 2, h2 = g->nx2 / 2, h3 = g->nx3 / 2;
+  for (int m = 0; m < nit; ++m) {
+    const int fb = it[m].fb, cb = it[m].cb, fid = it[m].fid, dir = fid >> 1, side = fid & 1;
+    const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
+                 *d3 = fc.dx3f + (int64_t)fb * r.n3;
+    for (int n = 0; n < nvar; ++n) {
+      if (dir == 0) {
+        int ic = side ? r.ie + 1 : r.is, i = side ? r.is : r.ie + 1;
+        int jl = r.js + (it[m].fi1 ? h2 : 0), kl = r.ks + (it[m].fi2 ? h3 : 0);
+        for (int kc = 0; kc < h3; ++kc)
+          for (int jc = 0; jc < h2; ++jc) {
+            int k = r.ks + 2 * kc, j = r.js + 2 * jc;
+            double amm = d2[j] * d3[k], amp = d2[j + 1] * d3[k];
+            double apm = d2[j] * d3[k + 1], app = d2[j + 1] * d3[k + 1];
+            double tarea = amm + amp + apm + app;
+#define FX(a, bb) ff1[ix(r.n3, r.n2, r.n1 + 1, nvar, fb, n, k + (a), j + (bb), i)]
+            double v = (FX(0, 0) * amm + FX(0, 1) * amp + FX(1, 0) * apm + FX(1, 1) * app) / tarea;
+#undef FX
+            cf1[ix(r.n3, r.n2, r.n1 + 1, nvar, cb, n, kl + kc, jl + jc, ic)] = v;
+          }
+      } else if (dir == 1) {
+        int jc0 = side ? r.je + 1 : r.js, j = side ? r.js : r.je + 1;
+        int il = r.is + (it[m].fi1 ? h1 : 0), kl = r.ks + (it[m].fi2 ? h3 : 0);
+        for (int kc = 0; kc < h3; ++kc)
+          for (int ic = 0; ic < h1; ++ic) {
+            int k = r.ks + 2 * kc, i = r.is + 2 * ic;
+            double s00 = d1[i] * d3[k], s01 = d1[i + 1] * d3[k];
+            double s10 = d1[i] * d3[k + 1], s11 = d1[i + 1] * d3[k + 1];
+            double tarea = s00 + s01 + s10 + s11;
+#define FX(a, c) ff2[ix(r.n3, r.n2 + 1, r.n1, nvar, fb, n, k + (a), j, i + (c))]
+            double v = (FX(0, 0) * s00 + FX(0, 1) * s01 + FX(1, 0) * s10 + FX(1, 1) * s11) / tarea;
+#undef FX
+            cf2[ix(r.n3, r.n2 + 1, r.n1, nvar, cb, n, kl + kc, jc0, il + ic)] = v;
+          }
+      } else {
+        int kc0 = side ? r.ke + 1 : r.ks, k = side ? r.ks : r.ke + 1;
+        int il = r.is + (it[m].fi1 ? h1 : 0), jl = r.js + (it[m].fi2 ? h2 : 0);
+        for (int jc = 0; jc < h2; ++jc)
+          for (int ic = 0; ic < h1; ++ic) {
+            int j = r.js + 2 * jc, i = r.is + 2 * ic;
+            double s00 = d1[i] * d2[j], s01 = d1[i + 1] * d2[j];
+            double s10 = d1[i] * d2[j + 1], s11 = d1[i + 1] * d2[j + 1];
+            double tarea = s00 + s01 + s10 + s11;
+#define FX(bb, c) ff3[ix(r.n3 + 1, r.n2, r.n1, nvar, fb, n, k, j + (bb), i + (c))]
+            double v = (FX(0, 0) * s00 + FX(0, 1) * s01 + FX(1, 0) * s10 + FX(1, 1) * s11) / tarea;
+#undef FX
+            cf3[ix(r.n3 + 1, r.n2, r.n1, nvar, cb, n, kc0, jl + jc, il + ic)] = v;
+          }
+      }
+    }
+  }
+}
+
+/* Item: fb = the finer block, fid = its face towards the coarser neighbour.  buf + m *
+ * stride receives the restricted edge EMFs of that face in the reference's order. */
+void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const double *e2,
+                            const double *e3, orc_ref_coords fc, double *buf, int64_t stride,
+                            const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  for (int m = 0; m < nit; ++m) {
+    const int fb = it[m].fb, fid = it[m].fid, dir = fid >> 1, side = fid & 1;
+    const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
+                 *d3 = fc.dx3f + (int64_t)fb * r.n3;
+    double *out = buf + (int64_t)m * stride;
+    int64_t p = 0;
+#define E1(k, j, i) e1[ix(r.n3 + 1, r.n2 + 1, r.n1, 1, fb, 0, (k), (j), (i))]
+#define E2(k, j, i) e2[ix(r.n3 + 1, r.n2, r.n1 + 1, 1, fb, 0, (k), (j), (i))]
+#define E3(k, j, i) e3[ix(r.n3, r.n2 + 1, r.n1 + 1, 1, fb, 0, (k), (j), (i))]
+    if (dir == 0) {
+      int i = side ? r.ie + 1 : r.is;
+      for (int k = r.ks; k <= r.ke + 1; k += 2)
+        for (int j = r.js; j <= r.je; j += 2) {
+          double l1 = d2[j], l2 = d2[j + 1];   /* GetEdge2Length = dx2f(j) */
+          out[p++] = (E2(k, j, i) * l1 + E2(k, j + 1, i) * l2) / (l1 + l2);
+        }
+      for (int k = r.ks; k <= r.ke; k += 2)
+        for (int j = r.js; j <= r.je + 1; j += 2) {
+          double l1 = d3[k], l2 = d3[k + 1];   /* GetEdge3Length = dx3f(k) */
+          out[p++] = (E3(k, j, i) * l1 + E3(k + 1, j, i) * l2) / (l1 + l2);
+        }
+    } else if (dir == 1) {
+      int j = side ? r.je + 1 : r.js;
+      for (int k = r.ks; k <= r.ke + 1; k += 2)
+        for (int i = r.is; i <= r.ie; i += 2)   /* Edge1Length = dx1f(i) */
+          out[p++] = (E1(k, j, i) * d1[i] + E1(k, j, i + 1) * d1[i + 1]) / (d1[i] + d1[i + 1]);
+      for (int k = r.ks; k <= r.ke; k += 2)
+        for (int i = r.is; i <= r.ie + 1; i += 2)
+          out[p++] = (E3(k, j, i) * d3[k] + E3(k + 1, j, i) * d3[k + 1]) / (d3[k] + d3[k + 1]);
+    } else {
+      int k = side ? r.ke + 1 : r.ks;
+      for (int j = r.js; j <= r.je + 1; j += 2)
+        for (int i = r.is; i <= r.ie; i += 2)
+          out[p++] = (E1(k, j, i) * d1[i] + E1(k, j, i + 1) * d1[i + 1]) / (d1[i] + d1[i + 1]);
+      for (int j = r.js; j <= r.je; j += 2)
+        for (int i = r.is; i <= r.ie + 1; i += 2)
+          out[p++] = (E2(k, j, i) * d2[j] + E2(k, j + 1, i) * d2[j + 1]) / (d2[j] + d2[j + 1]);
+    }
+#undef E1
+#undef E2
+#undef E3
+  }
+}

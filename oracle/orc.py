@@ -270,3 +270,86 @@ def ppm_pencil(q):
     wr = np.full(n, np.nan)
     lib().orc_ppm_pencil(n, _dp(q), _dp(wl), _dp(wr))
     return wl, wr
+
+
+# ---------------------------------------------------------------- refinement operators
+class RefItem(ctypes.Structure):
+    """orc_ref_item (same layout as agx_ref_item)."""
+    _fields_ = [(n, ctypes.c_int32) for n in ("fb", "cb", "si", "ei", "sj", "ej", "sk", "ek",
+                                              "fid", "fi1", "fi2", "pad")]
+
+
+class RefGeom(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("nx1", "nx2", "nx3", "ng", "cng", "pad")]
+
+
+class RefCoords(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("x1v", "x2v", "x3v", "dx1f", "dx2f", "dx3f")]
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def ref_coords(tabs):
+    """tabs: dict of contiguous float64 arrays x1v, x2v, x3v, dx1f, dx2f, dx3f (nb, n)."""
+    return RefCoords(*[_ptr(tabs[k]) for k in ("x1v", "x2v", "x3v", "dx1f", "dx2f", "dx3f")])
+
+
+def _items(items):
+    return (RefItem * max(len(items), 1))(*items), len(items)
+
+
+def _ref_lib():
+    L = lib()
+    if not getattr(L, "_ref_bound", False):
+        G, C, IT, V = ctypes.POINTER(RefGeom), RefCoords, ctypes.POINTER(RefItem), ctypes.c_void_p
+        L.orc_ref_restrict_cc.argtypes = [G, ctypes.c_int, V, C, V, IT, ctypes.c_int]
+        L.orc_ref_restrict_fc.argtypes = [G, ctypes.c_int, V, C, V, IT, ctypes.c_int]
+        L.orc_ref_prolong_cc.argtypes = [G, ctypes.c_int, V, C, V, C, IT, ctypes.c_int]
+        L.orc_ref_prolong_fc_shared.argtypes = [G, ctypes.c_int, V, C, V, C, IT, ctypes.c_int]
+        L.orc_ref_prolong_fc_internal.argtypes = [G, V, V, V, C, IT, ctypes.c_int]
+        L.orc_ref_flux_from_finer.argtypes = [G, ctypes.c_int, V, V, V, C, V, V, V, IT, ctypes.c_int]
+        L.orc_ref_emf_to_coarser.argtypes = [G, V, V, V, C, V, ctypes.c_int64, IT, ctypes.c_int]
+        L._ref_bound = True
+    return L
+
+
+def ref_restrict_cc(g, nvar, fine, ftab, coarse, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_restrict_cc(ctypes.byref(g), nvar, _ptr(fine), ref_coords(ftab), _ptr(coarse), a, n)
+
+
+def ref_restrict_fc(g, d, fine, ftab, coarse, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_restrict_fc(ctypes.byref(g), d, _ptr(fine), ref_coords(ftab), _ptr(coarse), a, n)
+
+
+def ref_prolong_cc(g, nvar, coarse, ctab, fine, ftab, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_prolong_cc(ctypes.byref(g), nvar, _ptr(coarse), ref_coords(ctab), _ptr(fine),
+                                  ref_coords(ftab), a, n)
+
+
+def ref_prolong_fc_shared(g, d, coarse, ctab, fine, ftab, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_prolong_fc_shared(ctypes.byref(g), d, _ptr(coarse), ref_coords(ctab), _ptr(fine),
+                                         ref_coords(ftab), a, n)
+
+
+def ref_prolong_fc_internal(g, b1, b2, b3, ftab, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_prolong_fc_internal(ctypes.byref(g), _ptr(b1), _ptr(b2), _ptr(b3), ref_coords(ftab), a, n)
+
+
+def ref_flux_from_finer(g, nvar, f1, f2, f3, ftab, c1, c2, c3, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_flux_from_finer(ctypes.byref(g), nvar, _ptr(f1), _ptr(f2), _ptr(f3), ref_coords(ftab),
+                                       _ptr(c1), _ptr(c2), _ptr(c3), a, n)
+
+
+def ref_emf_to_coarser(g, e1, e2, e3, ftab, buf, stride, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_emf_to_coarser(ctypes.byref(g), _ptr(e1), _ptr(e2), _ptr(e3), ref_coords(ftab), _ptr(buf),
+                                      stride, a, n)

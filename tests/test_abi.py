@@ -70,3 +70,32 @@ def test_scatter_gather_roundtrip(agx):
     d = agx.make_desc(nx=(8, 12, 4), bnx=(4, 4, 2))
     G = np.random.default_rng(0).normal(size=(3, 4, 12, 8))
     assert np.array_equal(agx.gather_cells(d, agx.scatter_cells(d, G)), G)
+
+
+@pytest.mark.parametrize("kind,it,nbf,nbc,code", [
+    ("restrict_cc", dict(fb=0, cb=0, si=0, ei=5, sj=2, ej=5, sk=2, ek=5), 1, 1, -40),    # fine i = -2
+    ("restrict_cc", dict(fb=1, cb=0, si=2, ei=5, sj=2, ej=5, sk=2, ek=5), 1, 1, -40),    # block out of range
+    ("prolong_cc", dict(fb=0, cb=0, si=0, ei=3, sj=2, ej=5, sk=2, ek=5), 1, 1, -40),     # stencil at -1
+    ("prolong_fc1", dict(fb=0, cb=0, si=2, ei=9, sj=2, ej=5, sk=2, ek=5), 1, 1, -40),    # x1 face 9 > nc
+    ("flux_from_finer", dict(fb=0, cb=0, fid=6), 1, 1, -40),
+    ("flux_from_finer", dict(fb=0, cb=0, fid=1, fi1=2), 1, 1, -40),
+])
+def test_refinement_lists_reject_bad_items(agx, kind, it, nbf, nbc, code):
+    """agx_ref_list_create checks every item on the host before any device work."""
+    g = agx.RefGeom(8, 8, 8, 2, 2, 0)
+    full = dict(fb=0, cb=0, si=0, ei=0, sj=0, ej=0, sk=0, ek=0, fid=0, fi1=0, fi2=0, pad=0)
+    full.update(it)
+    items = (agx.RefItem * 1)(agx.RefItem(**full))
+    h = ctypes.c_void_p()
+    rc = agx.lib().agx_ref_list_create(ctypes.byref(g), agx.REF_KINDS[kind], items, 1, nbf, nbc, ctypes.byref(h))
+    assert rc == code, agx.lib().agx_last_error()
+    assert not h.value
+    assert b"item 0" in agx.lib().agx_last_error()
+
+
+def test_refinement_emf_count(agx):
+    g = agx.RefGeom(8, 6, 4, 2, 2, 0)
+    # x1 face: e2 (nx3/2 + 1) x nx2/2 + e3 nx3/2 x (nx2/2 + 1), flux_correction_fc.cpp:334-356
+    assert agx.ref_emf_count(g, 0) == 3 * 3 + 2 * 4
+    assert agx.ref_emf_count(g, 3) == 3 * 4 + 2 * 5
+    assert agx.ref_emf_count(g, 5) == 4 * 4 + 3 * 5
