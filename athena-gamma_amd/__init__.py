@@ -36,6 +36,7 @@ EXPORTS = [
     "agx_ref_list_create", "agx_ref_list_destroy", "agx_ref_emf_count", "agx_ref_restrict_cc",
     "agx_ref_restrict_fc", "agx_ref_prolong_cc", "agx_ref_prolong_fc_shared", "agx_ref_prolong_fc_internal",
     "agx_ref_flux_from_finer", "agx_ref_emf_to_coarser", "agx_ref_sync",
+    "agx_device_alloc", "agx_device_free", "agx_device_copy",
 ]
 
 
@@ -150,6 +151,9 @@ def lib():
     L.agx_ref_flux_from_finer.argtypes = [P, ctypes.c_int32, P, P, P, C, P, P, P, P]
     L.agx_ref_emf_to_coarser.argtypes = [P, P, P, P, C, P, ctypes.c_int64, P]
     L.agx_ref_sync.argtypes = [P]
+    L.agx_device_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(P)]
+    L.agx_device_free.argtypes = [P]
+    L.agx_device_copy.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
     _LIB = L
     return L
 
@@ -389,3 +393,44 @@ class RefList:
 
 def ref_emf_count(geom, fid):
     return int(lib().agx_ref_emf_count(ctypes.byref(geom), fid))
+
+
+class DeviceArray:
+    """A device copy of a host array (agx_device_alloc / agx_device_copy).  `ptr` is
+    the device address the refinement operators take; get() copies back."""
+
+    def __init__(self, host):
+        host = np.ascontiguousarray(host)
+        self.shape, self.dtype, self.nbytes = host.shape, host.dtype, host.nbytes
+        self.p = ctypes.c_void_p()
+        _check(lib().agx_device_alloc(self.nbytes, ctypes.byref(self.p)), "agx_device_alloc")
+        _check(lib().agx_device_copy(self.p, host.ctypes.data_as(ctypes.c_void_p), self.nbytes, 0),
+               "agx_device_copy")
+
+    @property
+    def ptr(self):
+        return self.p.value
+
+    def get(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().agx_device_copy(out.ctypes.data_as(ctypes.c_void_p), self.p, self.nbytes, 1),
+               "agx_device_copy")
+        return out
+
+    def close(self):
+        if self.p:
+            lib().agx_device_free(self.p)
+            self.p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_coords(tab):
+    """RefCoords over device copies of per-block tables x1v, x2v, x3v, dx1f, dx2f, dx3f;
+    returns (coords, arrays) - keep the arrays alive while the coords are in use."""
+    arrs = [DeviceArray(tab[k]) for k in ("x1v", "x2v", "x3v", "dx1f", "dx2f", "dx3f")]
+    return RefCoords(*[a.ptr for a in arrs]), arrs

@@ -28,6 +28,7 @@ struct agx_ref_list {
   int32_t nitems = 0;
   int32_t G = 1;              // workgroups per item
   int64_t emf_max = 0;        // largest per-item output of an EMF_TO_COARSER list
+  int64_t max_count = 0;      // largest per-item index count (times nvar must stay < 2^31)
   agx_ref_item *items = nullptr;   // device
 };
 
@@ -73,8 +74,10 @@ struct Range {
 __device__ __forceinline__ Range item_range(const agx_ref_item &it) {
   return Range{it.si, it.sj, it.sk, it.ei - it.si + 1, it.ej - it.sj + 1, it.ek - it.sk + 1};
 }
-__device__ __forceinline__ int64_t first_t() { return (int64_t)blockIdx.x * RBS + threadIdx.x; }
-__device__ __forceinline__ int64_t stride_t() { return (int64_t)gridDim.x * RBS; }
+// Per-item counts are below 2^32 (checked on the host), so the walk and the index
+// decomposition use 32-bit unsigned arithmetic (64-bit divisions cost several times more).
+__device__ __forceinline__ unsigned first_t() { return blockIdx.x * RBS + threadIdx.x; }
+__device__ __forceinline__ unsigned stride_t() { return gridDim.x * RBS; }
 
 // ---------------------------------------------------------------- restriction
 // RestrictCellCenteredValues, mesh_refinement.cpp:106-136
@@ -83,15 +86,15 @@ k_ref_restrict_cc(RG r, int nvar, const double *__restrict__ fine, agx_ref_coord
                   double *__restrict__ coarse, const agx_ref_item *__restrict__ items) {
   const agx_ref_item it = items[blockIdx.y];
   const Range q = item_range(it);
-  const int64_t per = (int64_t)q.w1 * q.w2 * q.w3, cnt = per * nvar;
+  const unsigned w1 = q.w1, w2 = q.w2, per = w1 * w2 * q.w3, cnt = per * nvar;
   const double *d1 = fc.dx1f + (int64_t)it.fb * r.n1, *d2 = fc.dx2f + (int64_t)it.fb * r.n2,
                *d3 = fc.dx3f + (int64_t)it.fb * r.n3;
-  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
     const int n = (int)(t / per);
-    int64_t s = t - n * per;
-    const int ci = q.si + (int)(s % q.w1);
-    s /= q.w1;
-    const int cj = q.sj + (int)(s % q.w2), ck = q.sk + (int)(s / q.w2);
+    unsigned s = t - n * per;
+    const int ci = q.si + (int)(s % w1);
+    s /= w1;
+    const int cj = q.sj + (int)(s % w2), ck = q.sk + (int)(s / w2);
     const int i = (ci - r.cis) * 2 + r.is, j = (cj - r.cjs) * 2 + r.js, k = (ck - r.cks) * 2 + r.ks;
     double v[2][2][2];   // CellVolume at (k+a, j+b, i+c) = dx1f*dx2f*dx3f
 #pragma unroll
@@ -118,15 +121,15 @@ k_ref_restrict_fc(RG r, const double *__restrict__ fine, agx_ref_coords fc, doub
                   const agx_ref_item *__restrict__ items) {
   const agx_ref_item it = items[blockIdx.y];
   const Range q = item_range(it);
-  const int64_t cnt = (int64_t)q.w1 * q.w2 * q.w3;
+  const unsigned cnt = (unsigned)q.w1 * q.w2 * q.w3;
   const int f3 = r.n3 + (DIR == 2), f2 = r.n2 + (DIR == 1), f1 = r.n1 + (DIR == 0);
   const int q3 = r.c3 + (DIR == 2), q2 = r.c2 + (DIR == 1), q1 = r.c1 + (DIR == 0);
   const double *d1 = fc.dx1f + (int64_t)it.fb * r.n1, *d2 = fc.dx2f + (int64_t)it.fb * r.n2,
                *d3 = fc.dx3f + (int64_t)it.fb * r.n3;
-  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
-    const int ci = q.si + (int)(t % q.w1);
-    const int64_t s = t / q.w1;
-    const int cj = q.sj + (int)(s % q.w2), ck = q.sk + (int)(s / q.w2);
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
+    const int ci = q.si + (int)(t % (unsigned)q.w1);
+    const unsigned s = t / (unsigned)q.w1;
+    const int cj = q.sj + (int)(s % (unsigned)q.w2), ck = q.sk + (int)(s / (unsigned)q.w2);
     const int i = (ci - r.cis) * 2 + r.is, j = (cj - r.cjs) * 2 + r.js, k = (ck - r.cks) * 2 + r.ks;
     auto F = [&](int a, int b, int c) { return fine[ix(f3, f2, f1, 1, it.fb, 0, k + a, j + b, i + c)]; };
     double val;
@@ -157,17 +160,17 @@ k_ref_prolong_cc(RG r, int nvar, const double *__restrict__ coarse, agx_ref_coor
                  agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
   const agx_ref_item it = items[blockIdx.y];
   const Range q = item_range(it);
-  const int64_t per = (int64_t)q.w1 * q.w2 * q.w3, cnt = per * nvar;
+  const unsigned w1 = q.w1, w2 = q.w2, per = w1 * w2 * q.w3, cnt = per * nvar;
   const double *X1 = cc.x1v + (int64_t)it.cb * r.c1, *X2 = cc.x2v + (int64_t)it.cb * r.c2,
                *X3 = cc.x3v + (int64_t)it.cb * r.c3;
   const double *x1 = fc.x1v + (int64_t)it.fb * r.n1, *x2 = fc.x2v + (int64_t)it.fb * r.n2,
                *x3 = fc.x3v + (int64_t)it.fb * r.n3;
-  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
     const int n = (int)(t / per);
-    int64_t s = t - n * per;
-    const int i = q.si + (int)(s % q.w1);
-    s /= q.w1;
-    const int j = q.sj + (int)(s % q.w2), k = q.sk + (int)(s / q.w2);
+    unsigned s = t - n * per;
+    const int i = q.si + (int)(s % w1);
+    s /= w1;
+    const int j = q.sj + (int)(s % w2), k = q.sk + (int)(s / w2);
     const int fi = (i - r.cis) * 2 + r.is, fj = (j - r.cjs) * 2 + r.js, fk = (k - r.cks) * 2 + r.ks;
     const double dx3m = X3[k] - X3[k - 1], dx3p = X3[k + 1] - X3[k];
     const double dx3fm = X3[k] - x3[fk], dx3fp = x3[fk + 1] - X3[k];
@@ -201,17 +204,17 @@ k_ref_prolong_fc_shared(RG r, const double *__restrict__ coarse, agx_ref_coords 
                         agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
   const agx_ref_item it = items[blockIdx.y];
   const Range q = item_range(it);
-  const int64_t cnt = (int64_t)q.w1 * q.w2 * q.w3;
+  const unsigned cnt = (unsigned)q.w1 * q.w2 * q.w3;
   const int f3 = r.n3 + (DIR == 2), f2 = r.n2 + (DIR == 1), f1 = r.n1 + (DIR == 0);
   const int q3 = r.c3 + (DIR == 2), q2 = r.c2 + (DIR == 1), q1 = r.c1 + (DIR == 0);
   const double *X1 = cc.x1v + (int64_t)it.cb * r.c1, *X2 = cc.x2v + (int64_t)it.cb * r.c2,
                *X3 = cc.x3v + (int64_t)it.cb * r.c3;
   const double *x1 = fc.x1v + (int64_t)it.fb * r.n1, *x2 = fc.x2v + (int64_t)it.fb * r.n2,
                *x3 = fc.x3v + (int64_t)it.fb * r.n3;
-  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
-    const int i = q.si + (int)(t % q.w1);
-    const int64_t s = t / q.w1;
-    const int j = q.sj + (int)(s % q.w2), k = q.sk + (int)(s / q.w2);
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
+    const int i = q.si + (int)(t % (unsigned)q.w1);
+    const unsigned s = t / (unsigned)q.w1;
+    const int j = q.sj + (int)(s % (unsigned)q.w2), k = q.sk + (int)(s / (unsigned)q.w2);
     const int fi = (i - r.cis) * 2 + r.is, fj = (j - r.cjs) * 2 + r.js, fk = (k - r.cks) * 2 + r.ks;
     auto C = [&](int a, int b, int c) { return coarse[ix(q3, q2, q1, 1, it.cb, 0, k + a, j + b, i + c)]; };
     auto FO = [&](int a, int b, int c) -> double & { return fine[ix(f3, f2, f1, 1, it.fb, 0, fk + a, fj + b, fi + c)]; };
@@ -248,14 +251,14 @@ k_ref_prolong_fc_internal(RG r, double *__restrict__ b1, double *__restrict__ b2
                           agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
   const agx_ref_item it = items[blockIdx.y];
   const Range q = item_range(it);
-  const int64_t cnt = (int64_t)q.w1 * q.w2 * q.w3;
+  const unsigned cnt = (unsigned)q.w1 * q.w2 * q.w3;
   const int fb = it.fb;
   const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
                *d3 = fc.dx3f + (int64_t)fb * r.n3;
-  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
-    const int i = q.si + (int)(t % q.w1);
-    const int64_t s = t / q.w1;
-    const int j = q.sj + (int)(s % q.w2), k = q.sk + (int)(s / q.w2);
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
+    const int i = q.si + (int)(t % (unsigned)q.w1);
+    const unsigned s = t / (unsigned)q.w1;
+    const int j = q.sj + (int)(s % (unsigned)q.w2), k = q.sk + (int)(s / (unsigned)q.w2);
     const int fi = (i - r.cis) * 2 + r.is, fj = (j - r.cjs) * 2 + r.js, fk = (k - r.cks) * 2 + r.ks;
     auto B1 = [&](int kk, int jj, int ii) -> double & { return b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, kk, jj, ii)]; };
     auto B2 = [&](int kk, int jj, int ii) -> double & { return b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, kk, jj, ii)]; };
@@ -331,13 +334,13 @@ k_ref_flux_from_finer(RG r, int nvar, const double *__restrict__ ff1, const doub
   const int dir = it.fid >> 1, side = it.fid & 1;
   const int na = dir == 2 ? r.h2 : r.h3;   // slow index of the face: k (x1, x2 faces) or j (x3)
   const int nbb = dir == 0 ? r.h2 : r.h1;  // fast index: j (x1 face) or i
-  const int64_t per = (int64_t)na * nbb, cnt = per * nvar;
+  const unsigned per = (unsigned)na * nbb, cnt = per * nvar;
   const double *d1 = fc.dx1f + (int64_t)it.fb * r.n1, *d2 = fc.dx2f + (int64_t)it.fb * r.n2,
                *d3 = fc.dx3f + (int64_t)it.fb * r.n3;
-  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
     const int n = (int)(t / per);
-    const int64_t s = t - n * per;
-    const int a = (int)(s / nbb), b = (int)(s - (int64_t)a * nbb);
+    const unsigned s = t - n * per;
+    const int a = (int)(s / (unsigned)nbb), b = (int)(s - (unsigned)a * nbb);
     if (dir == 0) {
       const int ic = side ? r.ie + 1 : r.is, i = side ? r.is : r.ie + 1;
       const int k = r.ks + 2 * a, j = r.js + 2 * b;
@@ -382,42 +385,42 @@ k_ref_emf_to_coarser(RG r, const double *__restrict__ e1, const double *__restri
                      const agx_ref_item *__restrict__ items) {
   const agx_ref_item it = items[blockIdx.y];
   const int dir = it.fid >> 1, side = it.fid & 1, fb = it.fb;
-  const int64_t pa = emf_part(r, dir, 0), cnt = pa + emf_part(r, dir, 1);
+  const unsigned pa = (unsigned)emf_part(r, dir, 0), cnt = pa + (unsigned)emf_part(r, dir, 1);
   const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
                *d3 = fc.dx3f + (int64_t)fb * r.n3;
   auto E1 = [&](int k, int j, int i) { return e1[ix(r.n3 + 1, r.n2 + 1, r.n1, 1, fb, 0, k, j, i)]; };
   auto E2 = [&](int k, int j, int i) { return e2[ix(r.n3 + 1, r.n2, r.n1 + 1, 1, fb, 0, k, j, i)]; };
   auto E3 = [&](int k, int j, int i) { return e3[ix(r.n3, r.n2 + 1, r.n1 + 1, 1, fb, 0, k, j, i)]; };
   double *out = buf + (int64_t)blockIdx.y * stride;
-  for (int64_t t = first_t(); t < cnt; t += stride_t()) {
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
     const bool first = t < pa;
-    const int64_t s = first ? t : t - pa;
+    const unsigned s = first ? t : t - pa;
     double v;
     if (dir == 0) {
       const int i = side ? r.ie + 1 : r.is;
       if (first) {   // e2, k = ks..ke+1 step 2, j = js..je step 2
-        const int k = r.ks + 2 * (int)(s / r.h2), j = r.js + 2 * (int)(s % r.h2);
+        const int k = r.ks + 2 * (int)(s / (unsigned)r.h2), j = r.js + 2 * (int)(s % (unsigned)r.h2);
         v = (E2(k, j, i) * d2[j] + E2(k, j + 1, i) * d2[j + 1]) / (d2[j] + d2[j + 1]);
       } else {       // e3, k = ks..ke step 2, j = js..je+1 step 2
-        const int k = r.ks + 2 * (int)(s / (r.h2 + 1)), j = r.js + 2 * (int)(s % (r.h2 + 1));
+        const int k = r.ks + 2 * (int)(s / (unsigned)(r.h2 + 1)), j = r.js + 2 * (int)(s % (unsigned)(r.h2 + 1));
         v = (E3(k, j, i) * d3[k] + E3(k + 1, j, i) * d3[k + 1]) / (d3[k] + d3[k + 1]);
       }
     } else if (dir == 1) {
       const int j = side ? r.je + 1 : r.js;
       if (first) {   // e1, k = ks..ke+1 step 2, i = is..ie step 2
-        const int k = r.ks + 2 * (int)(s / r.h1), i = r.is + 2 * (int)(s % r.h1);
+        const int k = r.ks + 2 * (int)(s / (unsigned)r.h1), i = r.is + 2 * (int)(s % (unsigned)r.h1);
         v = (E1(k, j, i) * d1[i] + E1(k, j, i + 1) * d1[i + 1]) / (d1[i] + d1[i + 1]);
       } else {       // e3, k = ks..ke step 2, i = is..ie+1 step 2
-        const int k = r.ks + 2 * (int)(s / (r.h1 + 1)), i = r.is + 2 * (int)(s % (r.h1 + 1));
+        const int k = r.ks + 2 * (int)(s / (unsigned)(r.h1 + 1)), i = r.is + 2 * (int)(s % (unsigned)(r.h1 + 1));
         v = (E3(k, j, i) * d3[k] + E3(k + 1, j, i) * d3[k + 1]) / (d3[k] + d3[k + 1]);
       }
     } else {
       const int k = side ? r.ke + 1 : r.ks;
       if (first) {   // e1, j = js..je+1 step 2, i = is..ie step 2
-        const int j = r.js + 2 * (int)(s / r.h1), i = r.is + 2 * (int)(s % r.h1);
+        const int j = r.js + 2 * (int)(s / (unsigned)r.h1), i = r.is + 2 * (int)(s % (unsigned)r.h1);
         v = (E1(k, j, i) * d1[i] + E1(k, j, i + 1) * d1[i + 1]) / (d1[i] + d1[i + 1]);
       } else {       // e2, j = js..je step 2, i = is..ie+1 step 2
-        const int j = r.js + 2 * (int)(s / (r.h1 + 1)), i = r.is + 2 * (int)(s % (r.h1 + 1));
+        const int j = r.js + 2 * (int)(s / (unsigned)(r.h1 + 1)), i = r.is + 2 * (int)(s % (unsigned)(r.h1 + 1));
         v = (E2(k, j, i) * d2[j] + E2(k, j + 1, i) * d2[j + 1]) / (d2[j] + d2[j + 1]);
       }
     }
@@ -507,11 +510,15 @@ int64_t item_count(const RG &r, int kind, const agx_ref_item &it) {
   return (int64_t)(it.ei - it.si + 1) * (it.ej - it.sj + 1) * (it.ek - it.sk + 1);
 }
 
-int check_call(const agx_ref_list *l, int kind_lo, int kind_hi, const char *what) {
+int check_call(const agx_ref_list *l, int kind_lo, int kind_hi, const char *what, int nvar = 1) {
   if (!l) { set_error("%s: null work list", what); return -41; }
   if (l->kind < kind_lo || l->kind > kind_hi) {
     set_error("%s: work list was built for operator kind %d", what, l->kind);
     return -42;
+  }
+  if (nvar < 1 || nvar > 64 || l->max_count * nvar >= ((int64_t)1 << 31)) {
+    set_error("%s: nvar must be 1..64 and items x nvar below 2^31 elements", what);
+    return -47;
   }
   return 0;
 }
@@ -538,6 +545,9 @@ int agx_ref_list_create(const agx_ref_geom *g, int32_t kind, const agx_ref_item 
   }
   if (g->ng < 1 || g->cng < 1) { set_error("ng and cng must be >= 1"); return -45; }
   if (nitems > 65535) { set_error("at most 65535 items per work list"); return -46; }
+  if ((int64_t)(g->nx1 + 2 * g->ng) * (g->nx2 + 2 * g->ng) * (g->nx3 + 2 * g->ng) > (1 << 24)) {
+    set_error("refinement operators take blocks of at most 2^24 cells"); return -49;
+  }
   const RG r = make_rg(*g);
   int64_t maxc = 0, emf_max = 0;
   for (int m = 0; m < nitems; ++m) {
@@ -552,6 +562,7 @@ int agx_ref_list_create(const agx_ref_geom *g, int32_t kind, const agx_ref_item 
   l->kind = kind;
   l->nitems = nitems;
   l->emf_max = emf_max;
+  l->max_count = maxc;
   const int nvar_hint = (kind == AGX_REF_RESTRICT_CC || kind == AGX_REF_PROLONG_CC ||
                          kind == AGX_REF_FLUX_FROM_FINER) ? 5 : 1;
   l->G = (int)std::min<int64_t>(64, std::max<int64_t>(1, (maxc * nvar_hint + RBS - 1) / RBS));
@@ -583,8 +594,7 @@ int64_t agx_ref_emf_count(const agx_ref_geom *g, int32_t fid) {
 
 int agx_ref_restrict_cc(const agx_ref_list *l, int32_t nvar, const double *fine, agx_ref_coords fc,
                         double *coarse, void *stream) {
-  if (int rc = check_call(l, AGX_REF_RESTRICT_CC, AGX_REF_RESTRICT_CC, "agx_ref_restrict_cc")) return rc;
-  if (nvar < 1) { set_error("agx_ref_restrict_cc: nvar must be >= 1"); return -47; }
+  if (int rc = check_call(l, AGX_REF_RESTRICT_CC, AGX_REF_RESTRICT_CC, "agx_ref_restrict_cc", nvar)) return rc;
   if (l->nitems == 0) return 0;
   hipLaunchKernelGGL(k_ref_restrict_cc, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
                      make_rg(l->g), nvar, fine, fc, coarse, l->items);
@@ -608,8 +618,7 @@ int agx_ref_restrict_fc(const agx_ref_list *l, const double *fine, agx_ref_coord
 
 int agx_ref_prolong_cc(const agx_ref_list *l, int32_t nvar, const double *coarse, agx_ref_coords cc,
                        double *fine, agx_ref_coords fc, void *stream) {
-  if (int rc = check_call(l, AGX_REF_PROLONG_CC, AGX_REF_PROLONG_CC, "agx_ref_prolong_cc")) return rc;
-  if (nvar < 1) { set_error("agx_ref_prolong_cc: nvar must be >= 1"); return -47; }
+  if (int rc = check_call(l, AGX_REF_PROLONG_CC, AGX_REF_PROLONG_CC, "agx_ref_prolong_cc", nvar)) return rc;
   if (l->nitems == 0) return 0;
   hipLaunchKernelGGL(k_ref_prolong_cc, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
                      make_rg(l->g), nvar, coarse, cc, fine, fc, l->items);
@@ -645,9 +654,8 @@ int agx_ref_prolong_fc_internal(const agx_ref_list *l, double *b1f, double *b2f,
 int agx_ref_flux_from_finer(const agx_ref_list *l, int32_t nvar, const double *ff1, const double *ff2,
                             const double *ff3, agx_ref_coords fc, double *cf1, double *cf2, double *cf3,
                             void *stream) {
-  if (int rc = check_call(l, AGX_REF_FLUX_FROM_FINER, AGX_REF_FLUX_FROM_FINER, "agx_ref_flux_from_finer"))
+  if (int rc = check_call(l, AGX_REF_FLUX_FROM_FINER, AGX_REF_FLUX_FROM_FINER, "agx_ref_flux_from_finer", nvar))
     return rc;
-  if (nvar < 1) { set_error("agx_ref_flux_from_finer: nvar must be >= 1"); return -47; }
   if (l->nitems == 0) return 0;
   hipLaunchKernelGGL(k_ref_flux_from_finer, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
                      make_rg(l->g), nvar, ff1, ff2, ff3, fc, cf1, cf2, cf3, l->items);
@@ -671,6 +679,31 @@ int agx_ref_emf_to_coarser(const agx_ref_list *l, const double *e1, const double
 int agx_ref_sync(void *stream) {
   hipError_t e = hipStreamSynchronize((hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "agx_ref_sync");
+}
+
+int agx_device_alloc(int64_t bytes, void **out) {
+  if (!out || bytes < 0) { set_error("agx_device_alloc: bad argument"); return -1; }
+  *out = nullptr;
+  if (bytes == 0) return 0;
+  hipError_t e = hipMalloc(out, (size_t)bytes);
+  if (e != hipSuccess) { *out = nullptr; return hip_fail(e, "agx_device_alloc"); }
+  return 0;
+}
+
+int agx_device_free(void *p) {
+  if (!p) return 0;
+  hipError_t e = hipFree(p);
+  return e == hipSuccess ? 0 : hip_fail(e, "agx_device_free");
+}
+
+int agx_device_copy(void *dst, const void *src, int64_t bytes, int32_t kind) {
+  if (bytes < 0 || kind < 0 || kind > 2 || (bytes > 0 && (!dst || !src))) {
+    set_error("agx_device_copy: bad argument"); return -1;
+  }
+  if (bytes == 0) return 0;
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  hipError_t e = hipMemcpy(dst, src, (size_t)bytes, k);
+  return e == hipSuccess ? 0 : hip_fail(e, "agx_device_copy");
 }
 
 }  // extern "C"

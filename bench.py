@@ -9,7 +9,8 @@ Also reported: V(1,1) cycles/s of the multigrid (FP64, mixed, multipole boundari
 the 256^3 32-blob self-gravity problem; the coupled 256^3 MHD + multigrid self-gravity
 cycle (VL2+PLM+HLLD+CT, lognormal density) in FP64 and in the mixed form, with the
 mixed-vs-FP64 ratio and differences ("self_gravity"); the dominant kernel's roofline;
-and the CPU oracle timed on a bounded sample of the MHD workload.
+the refinement operators' bandwidth on 512 blocks of 32^3 ("refinement"); and the CPU
+oracle timed on a bounded sample of the MHD workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -299,6 +300,56 @@ def large_bench(agx, pgen, args):
     print(json.dumps(out), flush=True)
 
 
+def ref_bench(agx, args):
+    """Refinement operators (static mesh refinement) on 512 blocks of 32^3 (ng 2, 5
+    variables, coarse buffers 16^3 + 2 ng): whole-block cell restriction and prolongation,
+    and the interior face prolongation; algorithmic bytes per launch over the mean launch
+    time of `reps` launches (device-resident inputs)."""
+    nb, nx, ng, nvar, reps = 512, 32, 2, 5, 20
+    n, nc = nx + 2 * ng, nx // 2 + 2 * ng
+    g = agx.RefGeom(nx, nx, nx, ng, ng, 0)
+    # uniform coordinates per block (the operators read per-block tables)
+    h = 1.0 / 256
+    tab_f = {k: np.tile((np.arange(n) - ng + 0.5) * h, (nb, 1)) for k in ("x1v", "x2v", "x3v")}
+    tab_f.update({k: np.full((nb, n), h) for k in ("dx1f", "dx2f", "dx3f")})
+    tab_c = {k: np.tile((np.arange(nc) - ng + 0.5) * 2 * h, (nb, 1)) for k in ("x1v", "x2v", "x3v")}
+    tab_c.update({k: np.full((nb, nc), 2 * h) for k in ("dx1f", "dx2f", "dx3f")})
+    fc, keep_f = agx.device_coords(tab_f)
+    cc, keep_c = agx.device_coords(tab_c)
+    rs = np.random.default_rng(1)
+    fine = agx.DeviceArray(rs.random((nb, nvar, n, n, n)))
+    coarse = agx.DeviceArray(rs.random((nb, nvar, nc, nc, nc)))
+    b = [agx.DeviceArray(rs.random((nb, n + (d == 2), n + (d == 1), n + (d == 0)))) for d in range(3)]
+    lo, hi = ng, ng + nx // 2 - 1
+    items = [agx.RefItem(q, q, lo, hi, lo, hi, lo, hi, 0, 0, 0, 0) for q in range(nb)]
+    ncoarse = nb * (nx // 2) ** 3
+    cases = [("restrict_cc", lambda L: L.restrict_cc(nvar, fine.ptr, fc, coarse.ptr),
+              ncoarse * nvar * (8 * 8 + 8)),
+             ("prolong_cc", lambda L: L.prolong_cc(nvar, coarse.ptr, cc, fine.ptr, fc),
+              ncoarse * nvar * (8 + 8 * 8)),
+             # 24 outer faces read (each shared by two groups: 12 per group), 12 written
+             ("prolong_fc_int", lambda L: L.prolong_fc_internal(b[0].ptr, b[1].ptr, b[2].ptr, fc),
+              ncoarse * (12 + 12) * 8)]
+    out = {"workload": f"{nb} blocks of {nx}^3 (nghost {ng}), {nvar} variables, whole-block ranges"}
+    for name, call, alg in cases:
+        L = agx.RefList(g, name, items, nb, nb)
+        call(L)
+        agx.RefList.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call(L)
+        agx.RefList.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        L.close()
+        gbs = alg / (ms * 1e-3) / 1e9
+        out[name] = {"ms_per_launch": ms, "alg_bytes": alg, "achieved_GBs": gbs, "frac": gbs * 1e9 / HBM_PEAK}
+        log(f"[ref] {name}: {ms:.3f} ms per launch, {alg / 1e6:.0f} MB algorithmic -> {gbs:.0f} GB/s "
+            f"({100 * gbs * 1e9 / HBM_PEAK:.0f}% of 8 TB/s)")
+    for a in [fine, coarse] + b + keep_f + keep_c:
+        a.close()
+    return out
+
+
 def cpu_baseline(args):
     """The CPU oracle (FP64 restatement, OpenMP over MeshBlocks) on a bounded sample of
     the same workload: a 64^3 periodic sub-box of the turbulence problem in 16
@@ -340,13 +391,14 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--mg-cycles", type=int, default=20)
-    ap.add_argument("--cpu-cycles", type=int, default=150)
+    ap.add_argument("--cpu-cycles", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-sg", action="store_true",
                     help="skip the 256^3 MHD + self-gravity leg (FP64 and mixed)")
     ap.add_argument("--sg-steps", type=int, default=10)
+    ap.add_argument("--no-ref", action="store_true", help="skip the refinement-operator leg")
     ap.add_argument("--compare", action="store_true",
                     help="download w after both runs and report the mixed-vs-FP64 difference")
     ap.add_argument("--large", action="store_true",
@@ -438,6 +490,8 @@ def main():
             "final_defect": nmp,
             "note": "isolated blobs, multipole order-4 boundaries about the centre of mass, FAS V(1,1), FP64",
             "kernel_ms_per_vcycle": {k: v[0] / 3 for k, v in mpper.items()}}
+    if not args.no_ref:
+        out["refinement"] = ref_bench(agx, args)
     if not args.no_sg:
         out["self_gravity"] = sg_run(agx, pgen, args.n, args.block, 2, 2.0, args.sg_steps, "sg", bx=0.1)
     if not args.no_cpu:

@@ -7,7 +7,7 @@
  * one task of the reference's task lists for ALL MeshBlocks in one batch of kernel
  * launches on the mesh's HIP stream, and replaces the reference method cited beside
  * it (paths relative to the reference's src/).  Device memory is allocated only by
- * agx_mesh_create and agx_ref_list_create.
+ * agx_mesh_create, agx_ref_list_create and agx_device_alloc.
  *
  * Conventions
  *   - Return value: 0 on success; > 0 a HIP error code; < 0 a contract violation
@@ -279,6 +279,14 @@ int agx_ref_emf_to_coarser(const agx_ref_list *l, const double *e1, const double
                            agx_ref_coords fine_coords, double *buf, int64_t stride, void *stream);
 /* waits for `stream` */
 int agx_ref_sync(void *stream);
+
+/* ---- device memory for callers without an allocator of their own (no counterpart in the
+ * reference; the refinement operators take device pointers).  Plain device allocations
+ * on the current device; copies are synchronous.  kind: 0 host to device, 1 device to
+ * host, 2 device to device. */
+int agx_device_alloc(int64_t bytes, void **out);
+int agx_device_free(void *p);
+int agx_device_copy(void *dst, const void *src, int64_t bytes, int32_t kind);
 
 /* ---- built-in timers (HIP events on the mesh stream around each launch) */
 int agx_timing_enable(agx_mesh *m, int enable);

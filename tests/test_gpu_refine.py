@@ -13,26 +13,29 @@ from oracle import orc
 from tests.refine_cases import Geo, geom_struct, item
 
 pytestmark = pytest.mark.gpu
-torch = pytest.importorskip("torch")
 
 
 class Dev:
-    """Device copies of host arrays (torch is only the allocator here)."""
+    """Device copies of host arrays through the library's agx_device_* entry points."""
 
     def __init__(self):
         self.keep = []
+        from tests.conftest import load_agx
+        self.agx = load_agx()
 
     def put(self, a):
-        t = torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+        t = self.agx.DeviceArray(a)
         self.keep.append(t)
         return t
 
     def coords(self, agx, tab):
-        return agx.RefCoords(*[self.put(tab[k]).data_ptr() for k in ("x1v", "x2v", "x3v", "dx1f", "dx2f", "dx3f")])
+        c, arrs = agx.device_coords(tab)
+        self.keep += arrs
+        return c
 
 
 def same(gpu_t, ref, name):
-    a = gpu_t.cpu().numpy()
+    a = gpu_t.get()
     d = np.abs(a - ref).max() if a.size else 0.0
     print(f"{name}: max|gpu - oracle| = {d:.3e}, bitwise {np.array_equal(a, ref)}", flush=True)
     assert np.array_equal(a, ref), name
@@ -66,14 +69,14 @@ def test_restrict_and_prolong_cc(agx, ng):
     fc, cc = d.coords(agx, g.fine), d.coords(agx, g.coarse)
     fine_t, coarse_t = d.put(fine), d.put(coarse.copy())
     lr = lists(agx, g, "restrict_cc", ritems, g.nb, g.nb)
-    lr.restrict_cc(nvar, fine_t.data_ptr(), fc, coarse_t.data_ptr())
+    lr.restrict_cc(nvar, fine_t.ptr, fc, coarse_t.ptr)
     agx.RefList.sync()
     ref = coarse.copy()
     orc.ref_restrict_cc(geom_struct(orc.RefGeom, g), nvar, fine, g.fine, ref, oitems(ritems))
     same(coarse_t, ref, f"restrict_cc ng={ng}")
     fine2_t = d.put(fine.copy())
     lp = lists(agx, g, "prolong_cc", pitems, g.nb, g.nb)
-    lp.prolong_cc(nvar, coarse_t.data_ptr(), cc, fine2_t.data_ptr(), fc)
+    lp.prolong_cc(nvar, coarse_t.ptr, cc, fine2_t.ptr, fc)
     agx.RefList.sync()
     ref2 = fine.copy()
     orc.ref_prolong_cc(geom_struct(orc.RefGeom, g), nvar, ref, g.coarse, ref2, g.fine, oitems(pitems))
@@ -96,14 +99,14 @@ def test_face_restrict_and_shared_prolong(agx, d):
     fc, cc = dv.coords(agx, g.fine), dv.coords(agx, g.coarse)
     fine_t, coarse_t = dv.put(fine), dv.put(coarse.copy())
     G = geom_struct(orc.RefGeom, g)
-    lists(agx, g, f"restrict_fc{d + 1}", ritems, g.nb, g.nb).restrict_fc(fine_t.data_ptr(), fc, coarse_t.data_ptr())
+    lists(agx, g, f"restrict_fc{d + 1}", ritems, g.nb, g.nb).restrict_fc(fine_t.ptr, fc, coarse_t.ptr)
     agx.RefList.sync()
     ref = coarse.copy()
     orc.ref_restrict_fc(G, d, fine, g.fine, ref, oitems(ritems))
     same(coarse_t, ref, f"restrict_fc dir {d}")
     fine2_t = dv.put(fine.copy())
-    lists(agx, g, f"prolong_fc{d + 1}", pitems, g.nb, g.nb).prolong_fc_shared(coarse_t.data_ptr(), cc,
-                                                                             fine2_t.data_ptr(), fc)
+    lists(agx, g, f"prolong_fc{d + 1}", pitems, g.nb, g.nb).prolong_fc_shared(coarse_t.ptr, cc,
+                                                                             fine2_t.ptr, fc)
     agx.RefList.sync()
     ref2 = fine.copy()
     orc.ref_prolong_fc_shared(G, d, ref, g.coarse, ref2, g.fine, oitems(pitems))
@@ -119,8 +122,8 @@ def test_internal_field(agx):
     dv = Dev()
     fc = dv.coords(agx, g.fine)
     bt = [dv.put(x.copy()) for x in b]
-    lists(agx, g, "prolong_fc_int", items, g.nb, g.nb).prolong_fc_internal(bt[0].data_ptr(), bt[1].data_ptr(),
-                                                                           bt[2].data_ptr(), fc)
+    lists(agx, g, "prolong_fc_int", items, g.nb, g.nb).prolong_fc_internal(bt[0].ptr, bt[1].ptr,
+                                                                           bt[2].ptr, fc)
     agx.RefList.sync()
     ref = [x.copy() for x in b]
     orc.ref_prolong_fc_internal(geom_struct(orc.RefGeom, g), ref[0], ref[1], ref[2], g.fine, oitems(items))
@@ -143,7 +146,7 @@ def test_flux_from_finer_and_emf_to_coarser(agx):
     fft = [dv.put(x) for x in ff]
     cft = [dv.put(x.copy()) for x in cf]
     lists(agx, g, "flux_from_finer", items, g.nb, g.nb).flux_from_finer(
-        nvar, *[t.data_ptr() for t in fft], fc, *[t.data_ptr() for t in cft])
+        nvar, *[t.ptr for t in fft], fc, *[t.ptr for t in cft])
     agx.RefList.sync()
     ref = [x.copy() for x in cf]
     orc.ref_flux_from_finer(geom_struct(orc.RefGeom, g), nvar, ff[0], ff[1], ff[2], g.fine, ref[0], ref[1], ref[2],
@@ -157,8 +160,8 @@ def test_flux_from_finer_and_emf_to_coarser(agx):
     buf = np.full(stride * len(eitems), -7.0)
     et = [dv.put(x) for x in e]
     bt = dv.put(buf.copy())
-    lists(agx, g, "emf_to_coarser", eitems, g.nb, g.nb).emf_to_coarser(*[t.data_ptr() for t in et], fc,
-                                                                        bt.data_ptr(), stride)
+    lists(agx, g, "emf_to_coarser", eitems, g.nb, g.nb).emf_to_coarser(*[t.ptr for t in et], fc,
+                                                                        bt.ptr, stride)
     agx.RefList.sync()
     orc.ref_emf_to_coarser(geom_struct(orc.RefGeom, g), e[0], e[1], e[2], g.fine, buf, stride, oitems(eitems))
     same(bt, buf, "emf_to_coarser")
