@@ -75,6 +75,20 @@ def units_per_launch(desc, cls):
     raise KeyError(cls)
 
 
+def max_divb(desc, m):
+    """max |div B| over the interior cells (B per unit length) from the downloaded face
+    field, host-side numpy: the round-off measure of constrained transport."""
+    ng, b = desc.nghost, desc.bnx1
+    dx = [(desc.x1max - desc.x1min) / desc.nx1, (desc.x2max - desc.x2min) / desc.nx2,
+          (desc.x3max - desc.x3min) / desc.nx3]
+    b1, b2, b3 = m.get("b1f"), m.get("b2f"), m.get("b3f")
+    c = slice(ng, ng + b)
+    d = (b1[:, c, c, ng + 1:ng + b + 1] - b1[:, c, c, c]) / dx[0]
+    d += (b2[:, c, ng + 1:ng + b + 1, c] - b2[:, c, c, c]) / dx[1]
+    d += (b3[:, ng + 1:ng + b + 1, c, c] - b3[:, c, c, c]) / dx[2]
+    return float(np.abs(d).max())
+
+
 KCLASSES = ("flux_x1", "flux_x2", "flux_x3", "corner_e", "integrate", "bvals", "c2p", "newdt")
 
 
@@ -132,7 +146,8 @@ def mhd_bench(agx, pgen, args, precision):
     if args.compare:
         ng = desc.nghost
         sl = (slice(None), slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
-        state = (m.time(), m.get("w")[sl].astype(np.float64))
+        state = (m.time(), m.get("w")[sl].astype(np.float64), max_divb(desc, m))
+        log(f"[bench] {precision} max |div B| after {int(state[0][2])} cycles: {state[2]:.3e}")
     m.close()
     return desc, zc, sec * 1e3 / args.steps, per, state
 
@@ -237,13 +252,14 @@ def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None):
         w = m.get("w")[sl][:, [0, 4]].astype(np.float64)
         bcc = m.get("bcc")[sl].astype(np.float64)
         phi = m.get("phi")[sl].astype(np.float64)
+        divb = max_divb(desc, m)
         m.timing(True)
         m.timing_reset()
         m.cycle(1e30, 2)
         m.sync()
         per = {c: m.timing_get(c) for c in KCLASSES + ("mg_smooth", "mg_restrict", "mg_prolong", "mg_other")}
         m.timing(False)
-        res[prec] = dict(zc=float(N) ** 3 * steps / sec, ms=sec * 1e3 / steps, w=w, bcc=bcc, phi=phi,
+        res[prec] = dict(zc=float(N) ** 3 * steps / sec, ms=sec * 1e3 / steps, w=w, bcc=bcc, phi=phi, divb=divb,
                          per=per, tile=tile, bytes=m.device_bytes(), rf=roofline(desc, per, prec))
         log(f"[{tag}] {prec}: {steps} cycles in {sec:.2f}s -> {res[prec]['zc']:.4e} zone-cycles/s, "
             f"device {m.device_bytes() / 1e9:.1f} GB, stage tile {tile}; per cycle: "
@@ -257,9 +273,9 @@ def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None):
                     f"{'beta=1' if bx is None else f'Bx={bx:g}'}, 4piG=1), {(N // B) ** 3} MeshBlocks of {B}^3, "
                     f"VL2+{'PPM' if xorder == 3 else 'PLM'}+HLLD+CT, FMG + V-cycles to 1e-10 x rms(rho - <rho>) "
                     f"each stage",
-        "mixed": {"value": a["zc"], "ms_per_step": a["ms"], "roofline": a["rf"],
+        "mixed": {"value": a["zc"], "ms_per_step": a["ms"], "roofline": a["rf"], "max_divb": a["divb"],
                   "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in a["per"].items()}},
-        "fp64": {"value": b["zc"], "ms_per_step": b["ms"], "roofline": b["rf"],
+        "fp64": {"value": b["zc"], "ms_per_step": b["ms"], "roofline": b["rf"], "max_divb": b["divb"],
                  "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in b["per"].items()}},
         "vs_fp64": a["zc"] / b["zc"],
         "rel_err_rho": err(a["w"][:, 0], b["w"][:, 0]),
@@ -293,7 +309,8 @@ def large_bench(agx, pgen, args):
         "data": "synthetic",
         "config": {"workload": r["workload"], "mesh": [N] * 3, "meshblock": [B] * 3, "nghost": 3},
         "roofline": a["roofline"],
-        "fp64": {k: r["fp64"][k] for k in ("value", "ms_per_step", "roofline")},
+        "fp64": {k: r["fp64"][k] for k in ("value", "ms_per_step", "roofline", "max_divb")},
+        "max_divb": a["max_divb"],
         "kernel_ms_per_cycle": a["kernel_ms_per_cycle"],
     }
     out.update({k: v for k, v in r.items() if k.startswith("rel_") or k in ("vs_fp64", "error_note")})
@@ -399,8 +416,8 @@ def main():
                     help="skip the 256^3 MHD + self-gravity leg (FP64 and mixed)")
     ap.add_argument("--sg-steps", type=int, default=10)
     ap.add_argument("--no-ref", action="store_true", help="skip the refinement-operator leg")
-    ap.add_argument("--compare", action="store_true",
-                    help="download w after both runs and report the mixed-vs-FP64 difference")
+    ap.add_argument("--no-compare", dest="compare", action="store_false",
+                    help="skip downloading w and B after both runs (mixed-vs-FP64 difference, div B)")
     ap.add_argument("--large", action="store_true",
                     help="run the 512^3 PPM + self-gravity mixed-precision case instead")
     ap.add_argument("--large-n", type=int, default=512)
@@ -460,7 +477,9 @@ def main():
                  "roofline": roofline(mdesc, mper, "mixed"),
                  "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in mper.items()}}
         if st64 is not None and stmx is not None:
-            (t64, w64), (tmx, wmx) = st64, stmx
+            (t64, w64, db64), (tmx, wmx, dbmx) = st64, stmx
+            out["max_divb"] = db64
+            mixed["max_divb"] = dbmx
             mixed["same_cycle_count"] = bool(t64[2] == tmx[2])
             mixed["rel_err_rho"] = float(np.abs(wmx[:, 0] - w64[:, 0]).max() / np.abs(w64[:, 0]).max())
             mixed["rel_err_p"] = float(np.abs(wmx[:, 4] - w64[:, 4]).max() / np.abs(w64[:, 4]).max())

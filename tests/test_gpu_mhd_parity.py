@@ -156,7 +156,7 @@ def test_divb_preserved(agx):
     o = Oracle(desc)
     for n in ("b1f", "b2f", "b3f"):
         o.set(n, m.get(n))
-    assert o.max_divb() < 1e-12
+    assert o.max_divb() < 1e-13   # absolute, B per unit length
 
 
 def test_floors_path(agx):

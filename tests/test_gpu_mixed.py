@@ -100,7 +100,7 @@ def test_mixed_turbulence_run(agx, rsolver, shape):
     o2 = Oracle(desc)
     for n in ("b1f", "b2f", "b3f"):
         o2.set(n, m.get(n))
-    assert o2.max_divb() < 1e-12
+    assert o2.max_divb() < 1e-13   # absolute, B per unit length
 
 
 @pytest.mark.parametrize("N,B", [(32, 8), (64, 16), (256, 64)])
