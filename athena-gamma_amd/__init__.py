@@ -36,7 +36,7 @@ EXPORTS = [
     "agx_ref_list_create", "agx_ref_list_destroy", "agx_ref_emf_count", "agx_ref_restrict_cc",
     "agx_ref_restrict_fc", "agx_ref_prolong_cc", "agx_ref_prolong_fc_shared", "agx_ref_prolong_fc_internal",
     "agx_ref_flux_from_finer", "agx_ref_emf_to_coarser", "agx_ref_sync",
-    "agx_device_alloc", "agx_device_free", "agx_device_copy",
+    "agx_device_alloc", "agx_device_free", "agx_device_copy", "agx_ref_prim_to_cons", "agx_ref_prolong_ranges",
 ]
 
 
@@ -91,7 +91,7 @@ class RefCoords(ctypes.Structure):
 
 REF_KINDS = dict(restrict_cc=0, restrict_fc1=1, restrict_fc2=2, restrict_fc3=3, prolong_cc=4,
                  prolong_fc1=5, prolong_fc2=6, prolong_fc3=7, prolong_fc_int=8, flux_from_finer=9,
-                 emf_to_coarser=10)
+                 emf_to_coarser=10, prim_to_cons=11)
 
 _LIB = None
 
@@ -151,6 +151,9 @@ def lib():
     L.agx_ref_flux_from_finer.argtypes = [P, ctypes.c_int32, P, P, P, C, P, P, P, P]
     L.agx_ref_emf_to_coarser.argtypes = [P, P, P, P, C, P, ctypes.c_int64, P]
     L.agx_ref_sync.argtypes = [P]
+    L.agx_ref_prim_to_cons.argtypes = [P, P, P, P, P, P, P, ctypes.c_double, ctypes.c_int32, P]
+    IP = ctypes.POINTER(ctypes.c_int32)
+    L.agx_ref_prolong_ranges.argtypes = [G, IP, IP, IP, IP]
     L.agx_device_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(P)]
     L.agx_device_free.argtypes = [P]
     L.agx_device_copy.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
@@ -383,6 +386,9 @@ class RefList:
         _check(lib().agx_ref_flux_from_finer(self.h, nvar, f1, f2, f3, fc, c1, c2, c3, None),
                "agx_ref_flux_from_finer")
 
+    def prim_to_cons(self, w, b1, b2, b3, bcc, u, gamma, iso=False):
+        _check(lib().agx_ref_prim_to_cons(self.h, w, b1, b2, b3, bcc, u, gamma, int(iso), None), "agx_ref_prim_to_cons")
+
     def emf_to_coarser(self, e1, e2, e3, fc, buf, stride):
         _check(lib().agx_ref_emf_to_coarser(self.h, e1, e2, e3, fc, buf, stride, None), "agx_ref_emf_to_coarser")
 
@@ -393,6 +399,15 @@ class RefList:
 
 def ref_emf_count(geom, fid):
     return int(lib().agx_ref_emf_count(ctypes.byref(geom), fid))
+
+
+def ref_prolong_ranges(geom, ox, lx_odd, nblevel):
+    """agx_ref_prolong_ranges: [si, ei, sj, ej, sk, ek, il, iu, jl, ju, kl, ku]."""
+    I3, I27, I12 = ctypes.c_int32 * 3, ctypes.c_int32 * 27, ctypes.c_int32 * 12
+    out = I12()
+    _check(lib().agx_ref_prolong_ranges(ctypes.byref(geom), I3(*ox), I3(*lx_odd), I27(*nblevel), out),
+           "agx_ref_prolong_ranges")
+    return list(out)
 
 
 class DeviceArray:

@@ -132,7 +132,6 @@ struct agx_mesh {
   int64_t pool_bytes = 0;
   agx::BlockCoef *coef = nullptr;    // [nb]
   double *plm = nullptr;             // PLM coefficients [nb][3 dirs][2][P]
-  double *cce = nullptr;             // unused placeholder (cell-centred E computed on the fly)
   double *scal = nullptr;            // device scalars (dt reduction etc.), 64 doubles
   double *scal_host = nullptr;       // pinned host mirror
   double *red = nullptr;             // reduction partials

@@ -321,6 +321,49 @@ k_ref_prolong_fc_internal(RG r, double *__restrict__ b1, double *__restrict__ b2
   }
 }
 
+// Field::CalculateCellCenteredField, uniform branch (field.cpp:112-176: lw = rw = 0.5), then
+// EquationOfState::PrimitiveToConserved (adiabatic_mhd.cpp:99-136; isothermal_mhd.cpp:
+// 82-108 without energy) on the fine cells of each coarse range.
+template <bool ISO>
+__global__ void __launch_bounds__(RBS)
+k_ref_prim_to_cons(RG r, const double *__restrict__ w, const double *__restrict__ b1,
+                   const double *__restrict__ b2, const double *__restrict__ b3, double *__restrict__ bcc,
+                   double *__restrict__ u, double igm1, const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const Range q = item_range(it);
+  const unsigned w1 = 2 * q.w1, w2 = 2 * q.w2, cnt = w1 * w2 * 2 * q.w3;   // fine cells
+  constexpr int NH = ISO ? 4 : 5;
+  const int fi0 = (q.si - r.cis) * 2 + r.is, fj0 = (q.sj - r.cjs) * 2 + r.js, fk0 = (q.sk - r.cks) * 2 + r.ks;
+  const int fb = it.fb;
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
+    const int i = fi0 + (int)(t % w1);
+    const unsigned s = t / w1;
+    const int j = fj0 + (int)(s % w2), k = fk0 + (int)(s / w2);
+    const double bc1 = 0.5 * b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, k, j, i)] +
+                       0.5 * b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, k, j, i + 1)];
+    const double bc2 = 0.5 * b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, k, j, i)] +
+                       0.5 * b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, k, j + 1, i)];
+    const double bc3 = 0.5 * b3[ix(r.n3 + 1, r.n2, r.n1, 1, fb, 0, k, j, i)] +
+                       0.5 * b3[ix(r.n3 + 1, r.n2, r.n1, 1, fb, 0, k + 1, j, i)];
+    bcc[ix(r.n3, r.n2, r.n1, 3, fb, 0, k, j, i)] = bc1;
+    bcc[ix(r.n3, r.n2, r.n1, 3, fb, 1, k, j, i)] = bc2;
+    bcc[ix(r.n3, r.n2, r.n1, 3, fb, 2, k, j, i)] = bc3;
+    const double w_d = w[ix(r.n3, r.n2, r.n1, NH, fb, 0, k, j, i)];
+    const double w_vx = w[ix(r.n3, r.n2, r.n1, NH, fb, 1, k, j, i)];
+    const double w_vy = w[ix(r.n3, r.n2, r.n1, NH, fb, 2, k, j, i)];
+    const double w_vz = w[ix(r.n3, r.n2, r.n1, NH, fb, 3, k, j, i)];
+    u[ix(r.n3, r.n2, r.n1, NH, fb, 0, k, j, i)] = w_d;
+    u[ix(r.n3, r.n2, r.n1, NH, fb, 1, k, j, i)] = w_vx * w_d;
+    u[ix(r.n3, r.n2, r.n1, NH, fb, 2, k, j, i)] = w_vy * w_d;
+    u[ix(r.n3, r.n2, r.n1, NH, fb, 3, k, j, i)] = w_vz * w_d;
+    if (!ISO) {
+      const double w_p = w[ix(r.n3, r.n2, r.n1, NH, fb, 4, k, j, i)];
+      u[ix(r.n3, r.n2, r.n1, NH, fb, 4, k, j, i)] =
+          w_p * igm1 + 0.5 * (w_d * (w_vx * w_vx + w_vy * w_vy + w_vz * w_vz) + (bc1 * bc1 + bc2 * bc2 + bc3 * bc3));
+    }
+  }
+}
+
 // ---------------------------------------------------------------- flux correction
 // LoadFluxBoundaryBufferToCoarser (flux_correction_cc.cpp:77-169) on the finer block and
 // SetFluxBoundaryFromFiner (:240-283) on the coarser one, as one direct write: value
@@ -441,7 +484,8 @@ int check_item(const RG &r, int kind, const agx_ref_item &it, int nbf, int nbc, 
     set_error("refinement work list, item %d: %s", m, what);
     return -40;
   };
-  const bool uses_cb = kind != AGX_REF_PROLONG_FC_INT && kind != AGX_REF_EMF_TO_COARSER;
+  const bool uses_cb = kind != AGX_REF_PROLONG_FC_INT && kind != AGX_REF_EMF_TO_COARSER &&
+                       kind != AGX_REF_PRIM_TO_CONS;
   if (it.fb < 0 || it.fb >= nbf) return bad("fine-side block index out of range");
   if (uses_cb && (it.cb < 0 || it.cb >= nbc)) return bad("coarse-side block index out of range");
   const int cs[3] = {r.cis, r.cjs, r.cks}, fs[3] = {r.is, r.js, r.ks};
@@ -489,6 +533,13 @@ int check_item(const RG &r, int kind, const agx_ref_item &it, int nbf, int nbc, 
         if (!fine_in(lo[d], hi[d], cs[d], fs[d], 2, fn[d])) return bad("fine faces outside the block");
       }
       return 0;
+    case AGX_REF_PRIM_TO_CONS:
+      for (int d = 0; d < 3; ++d) {
+        // fine cells (c - cs)*2 + fs + {0,1}; their upper faces at +2 (face dims n+1)
+        if (!in(lo[d], hi[d], 0, cn[d] - 1)) return bad("coarse range outside the coarse buffer");
+        if (!fine_in(lo[d], hi[d], cs[d], fs[d], 1, fn[d] - 1)) return bad("fine cells outside the block");
+      }
+      return 0;
     case AGX_REF_FLUX_FROM_FINER:
       if (it.fid < 0 || it.fid > 5) return bad("fid must be 0..5");
       if ((it.fi1 & ~1) || (it.fi2 & ~1)) return bad("fi1 and fi2 must be 0 or 1");
@@ -507,6 +558,7 @@ int64_t item_count(const RG &r, int kind, const agx_ref_item &it) {
     return (int64_t)(dir == 2 ? r.h2 : r.h3) * (dir == 0 ? r.h2 : r.h1);
   }
   if (kind == AGX_REF_EMF_TO_COARSER) return emf_part(r, it.fid >> 1, 0) + emf_part(r, it.fid >> 1, 1);
+  if (kind == AGX_REF_PRIM_TO_CONS) return 8 * (int64_t)(it.ei - it.si + 1) * (it.ej - it.sj + 1) * (it.ek - it.sk + 1);
   return (int64_t)(it.ei - it.si + 1) * (it.ej - it.sj + 1) * (it.ek - it.sk + 1);
 }
 
@@ -674,6 +726,67 @@ int agx_ref_emf_to_coarser(const agx_ref_list *l, const double *e1, const double
   hipLaunchKernelGGL(k_ref_emf_to_coarser, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
                      make_rg(l->g), e1, e2, e3, fc, buf, stride, l->items);
   return launch_status("agx_ref_emf_to_coarser");
+}
+
+int agx_ref_prim_to_cons(const agx_ref_list *l, const double *w, const double *b1f, const double *b2f,
+                         const double *b3f, double *bcc, double *u, double gamma, int32_t iso, void *stream) {
+  if (int rc = check_call(l, AGX_REF_PRIM_TO_CONS, AGX_REF_PRIM_TO_CONS, "agx_ref_prim_to_cons", iso ? 4 : 5))
+    return rc;
+  if (!iso && !(gamma > 1.0)) { set_error("agx_ref_prim_to_cons: gamma must be > 1"); return -50; }
+  if (l->nitems == 0) return 0;
+  const double igm1 = iso ? 0.0 : 1.0 / (gamma - 1.0);
+  if (iso)
+    hipLaunchKernelGGL(k_ref_prim_to_cons<true>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                       make_rg(l->g), w, b1f, b2f, b3f, bcc, u, igm1, l->items);
+  else
+    hipLaunchKernelGGL(k_ref_prim_to_cons<false>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                       make_rg(l->g), w, b1f, b2f, b3f, bcc, u, igm1, l->items);
+  return launch_status("agx_ref_prim_to_cons");
+}
+
+// BoundaryValues::ProlongateBoundaries loop limits (bvals_refine.cpp:172-199) and the face
+// ranges of ProlongateGhostCells: a face shared with a neighbour at the same or a finer
+// level is not prolongated (that neighbour supplies it).
+int agx_ref_prolong_ranges(const agx_ref_geom *g, const int32_t ox[3], const int32_t lx_odd[3],
+                           const int32_t nblevel[27], int32_t out[12]) {
+  if (!g || !ox || !lx_odd || !nblevel || !out) { set_error("agx_ref_prolong_ranges: null argument"); return -1; }
+  for (int d = 0; d < 3; ++d)
+    if (ox[d] < -1 || ox[d] > 1 || (lx_odd[d] & ~1)) {
+      set_error("agx_ref_prolong_ranges: ox must be -1..1 and lx_odd 0 or 1"); return -51;
+    }
+  if (nblevel[((ox[2] + 1) * 3 + ox[1] + 1) * 3 + ox[0] + 1] >= 0) {
+    set_error("agx_ref_prolong_ranges: the neighbour at ox is not coarser"); return -52;
+  }
+  const RG r = make_rg(*g);
+  const int cn = (g->ng + 1) / 2 + 1 - 1;   // cnghost - 1, meshblock.cpp:83
+  const int cs[3] = {r.cis, r.cjs, r.cks};
+  const int ce[3] = {r.cis + r.h1 - 1, r.cjs + r.h2 - 1, r.cks + r.h3 - 1};
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    if (ox[d] == 0) {
+      lo[d] = cs[d]; hi[d] = ce[d];
+      if (lx_odd[d] == 0) hi[d] += cn;
+      else lo[d] -= cn;
+    } else if (ox[d] > 0) {
+      lo[d] = ce[d] + 1; hi[d] = ce[d] + cn;
+    } else {
+      lo[d] = cs[d] - cn; hi[d] = cs[d] - 1;
+    }
+  }
+  auto lev = [&](int a, int b, int c) { return nblevel[(c * 3 + b) * 3 + a]; };   // (i, j, k) in 0..2
+  int fl[3], fh[3];
+  for (int d = 0; d < 3; ++d) {
+    fl[d] = lo[d];
+    fh[d] = hi[d] + 1;
+    int a[3] = {ox[0] + 1, ox[1] + 1, ox[2] + 1}, b[3] = {ox[0] + 1, ox[1] + 1, ox[2] + 1};
+    a[d] = ox[d];       // the neighbour across the lower face of the range along d
+    b[d] = ox[d] + 2;   // and across the upper face
+    if (ox[d] >= 0 && lev(a[0], a[1], a[2]) >= 0) fl[d]++;
+    if (ox[d] <= 0 && lev(b[0], b[1], b[2]) >= 0) fh[d]--;
+  }
+  out[0] = lo[0]; out[1] = hi[0]; out[2] = lo[1]; out[3] = hi[1]; out[4] = lo[2]; out[5] = hi[2];
+  out[6] = fl[0]; out[7] = fh[0]; out[8] = fl[1]; out[9] = fh[1]; out[10] = fl[2]; out[11] = fh[2];
+  return 0;
 }
 
 int agx_ref_sync(void *stream) {

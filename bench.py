@@ -76,8 +76,9 @@ def units_per_launch(desc, cls):
 
 
 def max_divb(desc, m):
-    """max |div B| over the interior cells (B per unit length) from the downloaded face
-    field, host-side numpy: the round-off measure of constrained transport."""
+    """max |div B| over the interior cells from the downloaded face field (host-side
+    numpy), as (absolute in B per unit length, dimensionless max|div B| * dx / max|B|):
+    the round-off measure of constrained transport."""
     ng, b = desc.nghost, desc.bnx1
     dx = [(desc.x1max - desc.x1min) / desc.nx1, (desc.x2max - desc.x2min) / desc.nx2,
           (desc.x3max - desc.x3min) / desc.nx3]
@@ -86,7 +87,9 @@ def max_divb(desc, m):
     d = (b1[:, c, c, ng + 1:ng + b + 1] - b1[:, c, c, c]) / dx[0]
     d += (b2[:, c, ng + 1:ng + b + 1, c] - b2[:, c, c, c]) / dx[1]
     d += (b3[:, ng + 1:ng + b + 1, c, c] - b3[:, c, c, c]) / dx[2]
-    return float(np.abs(d).max())
+    bmax = max(np.abs(b1[:, c, c, c]).max(), np.abs(b2[:, c, c, c]).max(), np.abs(b3[:, c, c, c]).max())
+    dmax = float(np.abs(d).max())
+    return dmax, dmax * min(dx) / float(bmax)
 
 
 KCLASSES = ("flux_x1", "flux_x2", "flux_x3", "corner_e", "integrate", "bvals", "c2p", "newdt")
@@ -147,7 +150,8 @@ def mhd_bench(agx, pgen, args, precision):
         ng = desc.nghost
         sl = (slice(None), slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
         state = (m.time(), m.get("w")[sl].astype(np.float64), max_divb(desc, m))
-        log(f"[bench] {precision} max |div B| after {int(state[0][2])} cycles: {state[2]:.3e}")
+        log(f"[bench] {precision} max |div B| after {int(state[0][2])} cycles: {state[2][0]:.3e} "
+            f"(x dx / max|B|: {state[2][1]:.3e})")
     m.close()
     return desc, zc, sec * 1e3 / args.steps, per, state
 
@@ -273,9 +277,11 @@ def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None):
                     f"{'beta=1' if bx is None else f'Bx={bx:g}'}, 4piG=1), {(N // B) ** 3} MeshBlocks of {B}^3, "
                     f"VL2+{'PPM' if xorder == 3 else 'PLM'}+HLLD+CT, FMG + V-cycles to 1e-10 x rms(rho - <rho>) "
                     f"each stage",
-        "mixed": {"value": a["zc"], "ms_per_step": a["ms"], "roofline": a["rf"], "max_divb": a["divb"],
+        "mixed": {"value": a["zc"], "ms_per_step": a["ms"], "roofline": a["rf"], "max_divb": a["divb"][0],
+                  "max_divb_dx_over_B": a["divb"][1],
                   "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in a["per"].items()}},
-        "fp64": {"value": b["zc"], "ms_per_step": b["ms"], "roofline": b["rf"], "max_divb": b["divb"],
+        "fp64": {"value": b["zc"], "ms_per_step": b["ms"], "roofline": b["rf"], "max_divb": b["divb"][0],
+                 "max_divb_dx_over_B": b["divb"][1],
                  "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in b["per"].items()}},
         "vs_fp64": a["zc"] / b["zc"],
         "rel_err_rho": err(a["w"][:, 0], b["w"][:, 0]),
@@ -309,8 +315,8 @@ def large_bench(agx, pgen, args):
         "data": "synthetic",
         "config": {"workload": r["workload"], "mesh": [N] * 3, "meshblock": [B] * 3, "nghost": 3},
         "roofline": a["roofline"],
-        "fp64": {k: r["fp64"][k] for k in ("value", "ms_per_step", "roofline", "max_divb")},
-        "max_divb": a["max_divb"],
+        "fp64": {k: r["fp64"][k] for k in ("value", "ms_per_step", "roofline", "max_divb", "max_divb_dx_over_B")},
+        "max_divb": a["max_divb"], "max_divb_dx_over_B": a["max_divb_dx_over_B"],
         "kernel_ms_per_cycle": a["kernel_ms_per_cycle"],
     }
     out.update({k: v for k, v in r.items() if k.startswith("rel_") or k in ("vs_fp64", "error_note")})
@@ -478,8 +484,8 @@ def main():
                  "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in mper.items()}}
         if st64 is not None and stmx is not None:
             (t64, w64, db64), (tmx, wmx, dbmx) = st64, stmx
-            out["max_divb"] = db64
-            mixed["max_divb"] = dbmx
+            out["max_divb"], out["max_divb_dx_over_B"] = db64
+            mixed["max_divb"], mixed["max_divb_dx_over_B"] = dbmx
             mixed["same_cycle_count"] = bool(t64[2] == tmx[2])
             mixed["rel_err_rho"] = float(np.abs(wmx[:, 0] - w64[:, 0]).max() / np.abs(w64[:, 0]).max())
             mixed["rel_err_p"] = float(np.abs(wmx[:, 4] - w64[:, 4]).max() / np.abs(w64[:, 4]).max())

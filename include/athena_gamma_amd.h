@@ -243,7 +243,11 @@ enum {
                                   cb = coarser block, fid = face of cb, fi1/fi2 = quadrant */
   AGX_REF_EMF_TO_COARSER = 10, /* flux_correction_fc.cpp:315-418 (LoadFluxBoundaryBufferToCoarser,
                                   3D face neighbours): fb = finer block, fid = its face */
-  AGX_REF_KINDS = 11
+  AGX_REF_PRIM_TO_CONS = 11,   /* Field::CalculateCellCenteredField (uniform branch, field.cpp:112-176)
+                                  + EquationOfState::PrimitiveToConserved (adiabatic_mhd.cpp:99-136,
+                                  isothermal_mhd.cpp:82-108) on the fine cells of each coarse range,
+                                  the last step of ProlongateGhostCells (bvals_refine.cpp) */
+  AGX_REF_KINDS = 12
 };
 typedef struct agx_ref_list agx_ref_list;
 /* Checks every item against the geometry and the pool sizes (nb_fine blocks on the fine
@@ -277,6 +281,21 @@ int agx_ref_flux_from_finer(const agx_ref_list *l, int32_t nvar, const double *f
 /* restricted edge EMFs of each item's face into buf + item * stride (EMF_TO_COARSER) */
 int agx_ref_emf_to_coarser(const agx_ref_list *l, const double *e1, const double *e2, const double *e3,
                            agx_ref_coords fine_coords, double *buf, int64_t stride, void *stream);
+/* fine cells of the item ranges: bcc from the faces, then the conserved variables from
+ * w and bcc (nvar 5 with the adiabatic EOS, gamma; 4 with iso != 0) (PRIM_TO_CONS) */
+int agx_ref_prim_to_cons(const agx_ref_list *l, const double *w, const double *b1f, const double *b2f,
+                         const double *b3f, double *bcc, double *u, double gamma, int32_t iso, void *stream);
+/* Loop limits of BoundaryValues::ProlongateBoundaries for one coarser neighbour
+ * (bvals_refine.cpp:172-199) and of the face prolongation in ProlongateGhostCells.  ox[3]
+ * is the neighbour's offset (NeighborIndexes::ox1..3); lx_odd[3] = loc.lx1..3 & 1;
+ * nblevel[27] holds the levels of the 3x3x3 neighbourhood minus this block's level, at
+ * index ((ox3+1)*3 + ox2+1)*3 + ox1+1 (-1 where there is no neighbour).  out[0..5] = si, ei, sj, ej, sk, ek (coarse cells:
+ * PROLONG_CC, PROLONG_FC_INT and PRIM_TO_CONS items), out[6..7] = il, iu (PROLONG_FC1
+ * faces, with sj..ek), out[8..9] = jl, ju (PROLONG_FC2), out[10..11] = kl, ku
+ * (PROLONG_FC3).  The reference's cnghost = (NGHOST + 1)/2 + 1 with NGHOST = g->ng.  Host
+ * only; no device work. */
+int agx_ref_prolong_ranges(const agx_ref_geom *g, const int32_t ox[3], const int32_t lx_odd[3],
+                           const int32_t nblevel[27], int32_t out[12]);
 /* waits for `stream` */
 int agx_ref_sync(void *stream);
 

@@ -134,6 +134,11 @@ void orc_ref_flux_from_finer(const orc_ref_geom *g, int nvar, const double *ff1,
 void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const double *e2,
                             const double *e3, orc_ref_coords fc, double *buf, int64_t stride,
                             const orc_ref_item *it, int nit);
+void orc_ref_prim_to_cons(const orc_ref_geom *g, const double *w, const double *b1, const double *b2,
+                          const double *b3, double *bcc, double *u, double gamma, int iso,
+                          const orc_ref_item *it, int nit);
+int orc_ref_prolong_ranges(const orc_ref_geom *g, const int *ox, const int *lx_odd, const int *nblevel,
+                           int *out);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,12 @@
  *   orc_ref_emf_to_coarser     FaceCenteredBoundaryVariable::LoadFluxBoundaryBufferToCoarser,
  *                              face neighbours of a 3D block (bvals/fc/flux_correction_fc.cpp:
  *                              315-418), packed in the reference's buffer order
+ *   orc_ref_prim_to_cons       Field::CalculateCellCenteredField (uniform branch, field/field.cpp:
+ *                              112-176) + EquationOfState::PrimitiveToConserved
+ *                              (eos/adiabatic_mhd.cpp:99-136, isothermal_mhd.cpp:82-108)
+ *   orc_ref_prolong_ranges     the loop limits of BoundaryValues::ProlongateBoundaries and
+ *                              ProlongateGhostCells (bvals/bvals_refine.cpp:172-199, face
+ *                              ranges of ProlongateGhostCells)
  *
  * Layouts (dense, AthenaArray order, block-major): cells (nb, nvar, n3, n2, n1); faces
  * x1f (nb, n3, n2, n1+1), x2f (nb, n3, n2+1, n1), x3f (nb, n3+1, n2, n1); hydro fluxes
@@ -446,4 +452,77 @@ void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const doubl
 #undef E2
 #undef E3
   }
+}
+
+/* ---------------------------------------------------------------- ghost prolongation */
+void orc_ref_prim_to_cons(const orc_ref_geom *g, const double *w, const double *b1, const double *b2,
+                          const double *b3, double *bcc, double *u, double gamma, int iso,
+                          const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  const int nh = iso ? 4 : 5;
+  const double igm1 = iso ? 0.0 : 1.0 / (gamma - 1.0);
+  for (int m = 0; m < nit; ++m) {
+    const int fb = it[m].fb;
+    const int fil = (it[m].si - r.cis) * 2 + r.is, fiu = (it[m].ei - r.cis) * 2 + r.is + 1;
+    const int fjl = (it[m].sj - r.cjs) * 2 + r.js, fju = (it[m].ej - r.cjs) * 2 + r.js + 1;
+    const int fkl = (it[m].sk - r.cks) * 2 + r.ks, fku = (it[m].ek - r.cks) * 2 + r.ks + 1;
+    for (int k = fkl; k <= fku; ++k)
+      for (int j = fjl; j <= fju; ++j)
+        for (int i = fil; i <= fiu; ++i) {
+          /* uniform branch: lw = rw = 0.5 */
+          double c1 = 0.5 * b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, k, j, i)] +
+                      0.5 * b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, k, j, i + 1)];
+          double c2 = 0.5 * b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, k, j, i)] +
+                      0.5 * b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, k, j + 1, i)];
+          double c3 = 0.5 * b3[ix(r.n3 + 1, r.n2, r.n1, 1, fb, 0, k, j, i)] +
+                      0.5 * b3[ix(r.n3 + 1, r.n2, r.n1, 1, fb, 0, k + 1, j, i)];
+          bcc[ix(r.n3, r.n2, r.n1, 3, fb, 0, k, j, i)] = c1;
+          bcc[ix(r.n3, r.n2, r.n1, 3, fb, 1, k, j, i)] = c2;
+          bcc[ix(r.n3, r.n2, r.n1, 3, fb, 2, k, j, i)] = c3;
+#define W(n) w[ix(r.n3, r.n2, r.n1, nh, fb, (n), k, j, i)]
+#define U(n) u[ix(r.n3, r.n2, r.n1, nh, fb, (n), k, j, i)]
+          double w_d = W(0), w_vx = W(1), w_vy = W(2), w_vz = W(3);
+          U(0) = w_d;
+          U(1) = w_vx * w_d;
+          U(2) = w_vy * w_d;
+          U(3) = w_vz * w_d;
+          if (!iso)
+            U(4) = W(4) * igm1 + 0.5 * (w_d * (w_vx * w_vx + w_vy * w_vy + w_vz * w_vz) + (c1 * c1 + c2 * c2 + c3 * c3));
+#undef W
+#undef U
+        }
+  }
+}
+
+int orc_ref_prolong_ranges(const orc_ref_geom *g, const int *ox, const int *lx_odd, const int *nblevel,
+                           int *out) {
+  rgeo r = mkgeo(g);
+  int cnghost = (g->ng + 1) / 2 + 1, cn = cnghost - 1;   /* meshblock.cpp:83 */
+  if (nblevel[((ox[2] + 1) * 3 + ox[1] + 1) * 3 + ox[0] + 1] >= 0) return -1;
+  int cs[3] = {r.cis, r.cjs, r.cks};
+  int ce[3] = {r.cis + g->nx1 / 2 - 1, r.cjs + g->nx2 / 2 - 1, r.cks + g->nx3 / 2 - 1};
+  int s[3], e[3];
+  for (int d = 0; d < 3; ++d) {   /* bvals_refine.cpp:172-199 */
+    if (ox[d] == 0) {
+      s[d] = cs[d]; e[d] = ce[d];
+      if (lx_odd[d] == 0) e[d] += cn; else s[d] -= cn;
+    } else if (ox[d] > 0) {
+      s[d] = ce[d] + 1; e[d] = ce[d] + cn;
+    } else {
+      s[d] = cs[d] - cn; e[d] = cs[d] - 1;
+    }
+  }
+  /* face ranges of ProlongateGhostCells: nblevel[ox3+1][ox2+1][ox1] etc. */
+#define LEV(i, j, k) nblevel[((k) * 3 + (j)) * 3 + (i)]
+  int il = s[0], iu = e[0] + 1, jl = s[1], ju = e[1] + 1, kl = s[2], ku = e[2] + 1;
+  if (ox[0] >= 0 && LEV(ox[0], ox[1] + 1, ox[2] + 1) >= 0) il++;
+  if (ox[0] <= 0 && LEV(ox[0] + 2, ox[1] + 1, ox[2] + 1) >= 0) iu--;
+  if (ox[1] >= 0 && LEV(ox[0] + 1, ox[1], ox[2] + 1) >= 0) jl++;
+  if (ox[1] <= 0 && LEV(ox[0] + 1, ox[1] + 2, ox[2] + 1) >= 0) ju--;
+  if (ox[2] >= 0 && LEV(ox[0] + 1, ox[1] + 1, ox[2]) >= 0) kl++;
+  if (ox[2] <= 0 && LEV(ox[0] + 1, ox[1] + 1, ox[2] + 2) >= 0) ku--;
+#undef LEV
+  out[0] = s[0]; out[1] = e[0]; out[2] = s[1]; out[3] = e[1]; out[4] = s[2]; out[5] = e[2];
+  out[6] = il; out[7] = iu; out[8] = jl; out[9] = ju; out[10] = kl; out[11] = ku;
+  return 0;
 }

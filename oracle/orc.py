@@ -312,6 +312,9 @@ def _ref_lib():
         L.orc_ref_prolong_fc_internal.argtypes = [G, V, V, V, C, IT, ctypes.c_int]
         L.orc_ref_flux_from_finer.argtypes = [G, ctypes.c_int, V, V, V, C, V, V, V, IT, ctypes.c_int]
         L.orc_ref_emf_to_coarser.argtypes = [G, V, V, V, C, V, ctypes.c_int64, IT, ctypes.c_int]
+        L.orc_ref_prim_to_cons.argtypes = [G, V, V, V, V, V, V, ctypes.c_double, ctypes.c_int, IT, ctypes.c_int]
+        IP = ctypes.POINTER(ctypes.c_int)
+        L.orc_ref_prolong_ranges.argtypes = [G, IP, IP, IP, IP]
         L._ref_bound = True
     return L
 
@@ -353,3 +356,18 @@ def ref_emf_to_coarser(g, e1, e2, e3, ftab, buf, stride, items):
     a, n = _items(items)
     _ref_lib().orc_ref_emf_to_coarser(ctypes.byref(g), _ptr(e1), _ptr(e2), _ptr(e3), ref_coords(ftab), _ptr(buf),
                                       stride, a, n)
+
+
+def ref_prim_to_cons(g, w, b1, b2, b3, bcc, u, gamma, iso, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_prim_to_cons(ctypes.byref(g), _ptr(w), _ptr(b1), _ptr(b2), _ptr(b3), _ptr(bcc), _ptr(u),
+                                    gamma, int(iso), a, n)
+
+
+def ref_prolong_ranges(g, ox, lx_odd, nblevel):
+    """nblevel: 27 levels relative to the block, index ((ox3+1)*3 + ox2+1)*3 + ox1+1."""
+    I3, I27, I12 = ctypes.c_int * 3, ctypes.c_int * 27, ctypes.c_int * 12
+    out = I12()
+    rc = _ref_lib().orc_ref_prolong_ranges(ctypes.byref(g), I3(*ox), I3(*lx_odd), I27(*nblevel), out)
+    assert rc == 0
+    return list(out)

@@ -165,3 +165,57 @@ def test_flux_from_finer_and_emf_to_coarser(agx):
     agx.RefList.sync()
     orc.ref_emf_to_coarser(geom_struct(orc.RefGeom, g), e[0], e[1], e[2], g.fine, buf, stride, oitems(eitems))
     same(bt, buf, "emf_to_coarser")
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_ghost_prolongation_pipeline(agx, case):
+    """ProlongateGhostCells for one coarser neighbour, with ranges from
+    agx_ref_prolong_ranges: cell prolongation of w, shared faces over the face ranges,
+    interior faces, then bcc and u on the prolonged cells; bitwise against the oracle."""
+    import itertools
+    rng = np.random.default_rng(30 + case)
+    oxs = [ox for ox in itertools.product((-1, 0, 1), repeat=3) if ox != (0, 0, 0)]
+    ox = oxs[rng.integers(len(oxs))]
+    par = tuple(int(v) for v in rng.integers(0, 2, 3))
+    lv = [-1] * 27 if case == 0 else list(int(v) for v in rng.integers(-1, 2, 27))
+    lv[13] = 0
+    lv[((ox[2] + 1) * 3 + ox[1] + 1) * 3 + ox[0] + 1] = -1
+    g = Geo(nx=(8, 8, 8), ng=2, nb=1, seed=31 + case, stretch=0.2)
+    cw = rng.normal(size=g.ccells(5))
+    cw[:, 0] = np.abs(cw[:, 0]) + 0.5
+    cb = [rng.normal(size=g.faces(d, coarse=True)) for d in range(3)]
+    fw = rng.normal(size=g.cells(5))
+    fb = [rng.normal(size=g.faces(d)) for d in range(3)]
+    bcc0, u0 = rng.normal(size=g.cells(3)), rng.normal(size=g.cells(5))
+    gamma = 5.0 / 3.0
+    ga = geom_struct(agx.RefGeom, g)
+    r = agx.ref_prolong_ranges(ga, ox, par, lv)
+    print(f"ox {ox} parity {par}: ranges {r}", flush=True)
+    cc = (0, 0, r[0], r[1], r[2], r[3], r[4], r[5])
+    fr = [(0, 0, r[6], r[7], r[2], r[3], r[4], r[5]), (0, 0, r[0], r[1], r[8], r[9], r[4], r[5]),
+          (0, 0, r[0], r[1], r[2], r[3], r[10], r[11])]
+    dv = Dev()
+    fcd, ccd = dv.coords(agx, g.fine), dv.coords(agx, g.coarse)
+    cw_t, fw_t = dv.put(cw), dv.put(fw)
+    cb_t, fb_t = [dv.put(x) for x in cb], [dv.put(x) for x in fb]
+    bcc_t, u_t = dv.put(bcc0), dv.put(u0)
+    lists(agx, g, "prolong_cc", [cc], 1, 1).prolong_cc(5, cw_t.ptr, ccd, fw_t.ptr, fcd)
+    for d in range(3):
+        lists(agx, g, f"prolong_fc{d + 1}", [fr[d]], 1, 1).prolong_fc_shared(cb_t[d].ptr, ccd, fb_t[d].ptr, fcd)
+    lists(agx, g, "prolong_fc_int", [cc], 1, 1).prolong_fc_internal(fb_t[0].ptr, fb_t[1].ptr, fb_t[2].ptr, fcd)
+    lists(agx, g, "prim_to_cons", [cc], 1, 1).prim_to_cons(fw_t.ptr, fb_t[0].ptr, fb_t[1].ptr, fb_t[2].ptr,
+                                                           bcc_t.ptr, u_t.ptr, gamma)
+    agx.RefList.sync()
+    G = geom_struct(orc.RefGeom, g)
+    assert orc.ref_prolong_ranges(G, ox, par, lv) == r
+    rw, rb, rbcc, ru = fw.copy(), [x.copy() for x in fb], bcc0.copy(), u0.copy()
+    orc.ref_prolong_cc(G, 5, cw, g.coarse, rw, g.fine, oitems([cc]))
+    for d in range(3):
+        orc.ref_prolong_fc_shared(G, d, cb[d], g.coarse, rb[d], g.fine, oitems([fr[d]]))
+    orc.ref_prolong_fc_internal(G, rb[0], rb[1], rb[2], g.fine, oitems([cc]))
+    orc.ref_prim_to_cons(G, rw, rb[0], rb[1], rb[2], rbcc, ru, gamma, 0, oitems([cc]))
+    same(fw_t, rw, "w")
+    for d in range(3):
+        same(fb_t[d], rb[d], f"x{d + 1}f")
+    same(bcc_t, rbcc, "bcc")
+    same(u_t, ru, "u")

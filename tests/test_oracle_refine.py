@@ -264,3 +264,138 @@ def test_emf_to_coarser_buffer_order(fid):
     assert len(exp) == agx.ref_emf_count(agx.RefGeom(g.nx[0], g.nx[1], g.nx[2], g.ng, g.cng, 0), fid)
     np.testing.assert_allclose(buf[:len(exp)], exp, rtol=0, atol=1e-13)
     assert np.isnan(buf[len(exp):]).all()
+
+
+# ---------------------------------------------------------------- ghost prolongation
+def test_prim_to_cons_inverts_cons2prim():
+    """orc_ref_prim_to_cons (PrimitiveToConserved + uniform CalculateCellCenteredField)
+    followed by the oracle's ConservedToPrimitive returns the primitives."""
+    from oracle.orc import Oracle, make_desc
+    g = Geo(nx=(8, 8, 8), ng=2, nb=1, seed=21)
+    desc = make_desc(nx=(8, 8, 8), bnx=(8, 8, 8), box=((0.0, 1.0),) * 3, gamma=5.0 / 3.0)
+    rs = np.random.default_rng(22)
+    w = np.zeros(g.cells(5))
+    w[:, 0] = rs.uniform(0.5, 2.0, w[:, 0].shape)
+    w[:, 1:4] = rs.normal(size=w[:, 1:4].shape)
+    w[:, 4] = rs.uniform(0.5, 2.0, w[:, 4].shape)
+    A = (lambda x, y, z: np.sin(6.0 * y) * np.cos(5.0 * z), lambda x, y, z: np.cos(7.0 * x) * np.sin(4.0 * z),
+         lambda x, y, z: np.sin(5.0 * x) * np.cos(6.0 * y))
+    faces = [f[None].copy() for f in curl_faces(g, g.fine, 0, A)]
+    bcc = np.zeros(g.cells(3))
+    u = np.zeros(g.cells(5))
+    c0, c1 = g.cis(0), g.cie(0)
+    orc.ref_prim_to_cons(G(g), w, faces[0], faces[1], faces[2], bcc, u, 5.0 / 3.0, 0, [It(0, 0, c0, c1, c0, c1, c0, c1)])
+    o = Oracle(desc)
+    o.set("u", u)
+    for n, f in zip(("b1f", "b2f", "b3f"), faces):
+        o.set(n, f)
+    o.L.orc_cons2prim(o.h)
+    s = np.s_[:, :, g.ng:g.ng + 8, g.ng:g.ng + 8, g.ng:g.ng + 8]
+    np.testing.assert_allclose(o.get("w")[s], w[s], rtol=1e-13, atol=1e-14)
+    np.testing.assert_array_equal(o.get("bcc")[s], bcc[s])
+
+
+def nblevel_cases(ox, rng):
+    """Neighbourhoods with the neighbour at ox coarser: all coarser, and random mixes of
+    coarser / same / finer elsewhere."""
+    idx = ((ox[2] + 1) * 3 + ox[1] + 1) * 3 + ox[0] + 1
+    cases = [[-1] * 27]
+    for _ in range(3):
+        lv = list(rng.integers(-1, 2, 27))
+        lv[13] = 0
+        lv[idx] = -1
+        cases.append(lv)
+    return cases
+
+
+@pytest.mark.parametrize("ng", [2, 4])
+def test_prolong_ranges_cover_the_ghost_slab(ng):
+    """For every coarser-neighbour offset and block parity, the fine cells of the ranges
+    stay inside the block array and cover the ghost slab facing that neighbour: along
+    ox = -1 exactly the ng ghosts below, along +1 the ng above, along 0 the interior plus
+    ng ghosts on the side the coarser neighbour extends to (the block's parity).  A face
+    range drops its lower (upper) face exactly when the neighbour across it is at the
+    same or a finer level.  The product's host function gives the same ranges."""
+    import itertools
+    from tests.conftest import load_agx
+    agx = load_agx()
+    g = Geo(nx=(8, 8, 8), ng=ng, nb=1)
+    go, ga = G(g), agx.RefGeom(8, 8, 8, ng, ng, 0)
+    rng = np.random.default_rng(23)
+    is_, ie = g.ng, g.ng + 7
+    for ox in itertools.product((-1, 0, 1), repeat=3):
+        if ox == (0, 0, 0):
+            continue
+        for par in itertools.product((0, 1), repeat=3):
+            for lv in nblevel_cases(ox, rng):
+                r = orc.ref_prolong_ranges(go, ox, par, lv)
+                assert r == agx.ref_prolong_ranges(ga, ox, par, lv)
+                for d in range(3):
+                    s, e = r[2 * d], r[2 * d + 1]
+                    fs, fe = (s - g.cng) * 2 + g.ng, (e - g.cng) * 2 + g.ng + 1
+                    assert 0 <= fs <= fe <= g.n[d] - 1
+                    if ox[d] < 0:
+                        assert (fs, fe) == (is_ - ng, is_ - 1)
+                    elif ox[d] > 0:
+                        assert (fs, fe) == (ie + 1, ie + ng)
+                    else:
+                        assert (fs, fe) == ((is_, ie + ng) if par[d] == 0 else (is_ - ng, ie))
+                    lo_nb = [o + 1 for o in ox]
+                    hi_nb = [o + 1 for o in ox]
+                    lo_nb[d] = ox[d]
+                    hi_nb[d] = ox[d] + 2
+                    lev = lambda a: lv[(a[2] * 3 + a[1]) * 3 + a[0]]
+                    assert r[6 + 2 * d] == s + (1 if ox[d] >= 0 and lev(lo_nb) >= 0 else 0)
+                    assert r[7 + 2 * d] == e + 1 - (1 if ox[d] <= 0 and lev(hi_nb) >= 0 else 0)
+
+
+def ghost_pipeline(g, ox, par, lv, coarse_w, coarse_b, fine_w, fine_b, gamma=5.0 / 3.0):
+    """ProlongateGhostCells for one coarser neighbour, in the oracle: cell prolongation of
+    w, shared faces over the face ranges, interior faces, then bcc and u."""
+    r = orc.ref_prolong_ranges(G(g), ox, par, lv)
+    cc = It(0, 0, r[0], r[1], r[2], r[3], r[4], r[5])
+    orc.ref_prolong_cc(G(g), 5, coarse_w, g.coarse, fine_w, g.fine, [cc])
+    fr = [It(0, 0, r[6], r[7], r[2], r[3], r[4], r[5]), It(0, 0, r[0], r[1], r[8], r[9], r[4], r[5]),
+          It(0, 0, r[0], r[1], r[2], r[3], r[10], r[11])]
+    for d in range(3):
+        orc.ref_prolong_fc_shared(G(g), d, coarse_b[d], g.coarse, fine_b[d], g.fine, [fr[d]])
+    orc.ref_prolong_fc_internal(G(g), fine_b[0], fine_b[1], fine_b[2], g.fine, [cc])
+    bcc = np.zeros(g.cells(3))
+    u = np.zeros(g.cells(5))
+    orc.ref_prim_to_cons(G(g), fine_w, fine_b[0], fine_b[1], fine_b[2], bcc, u, gamma, 0, [cc])
+    return r, bcc, u
+
+
+@pytest.mark.parametrize("ox", [(-1, 0, 0), (0, 1, 0), (1, -1, 0), (-1, 1, 1)])
+def test_ghost_prolongation_keeps_divb_zero(ox):
+    """With every neighbour coarser, the prolonged ghost cells of a divergence-free coarse
+    field are divergence-free, and a linear primitive field is reproduced there."""
+    g = Geo(nx=(8, 8, 8), ng=2, nb=1, seed=24)
+    A = (lambda x, y, z: np.sin(2.0 * y + 0.3) * np.cos(1.5 * z), lambda x, y, z: np.cos(1.7 * x) * np.sin(2.2 * z),
+         lambda x, y, z: np.sin(1.9 * x + 0.5) * np.cos(2.4 * y))
+    cb = [f[None].copy() for f in curl_faces(g, g.coarse, 0, A)]
+    coef = [(1.0, 0.1, 0.2, -0.1), (0.3, 0.2, 0.0, 0.1), (0.0, -0.2, 0.1, 0.3), (0.1, 0.0, 0.0, 0.2),
+            (1.5, 0.1, -0.1, 0.2)]
+    X, Y, Z = centres(g.coarse, 0)
+    cw = np.zeros(g.ccells(5))
+    for n in range(5):
+        cw[0, n] = lin(coef[n], X[None, None, :], Y[None, :, None], Z[:, None, None])
+    fw = np.zeros(g.cells(5))
+    fb = [np.zeros(g.faces(d)) for d in range(3)]
+    r, bcc, u = ghost_pipeline(g, ox, (0, 1, 0), [-1] * 27, cw, cb, fw, fb)
+    sl = [slice((r[2 * d] - g.cng) * 2 + g.ng, (r[2 * d + 1] - g.cng) * 2 + g.ng + 2) for d in range(3)]
+    dx, dy, dz = g.fine["dx1f"][0], g.fine["dx2f"][0], g.fine["dx3f"][0]
+    f1 = fb[0][0] * (dy[None, :, None] * dz[:, None, None])
+    f2 = fb[1][0] * (dx[None, None, :] * dz[:, None, None])
+    f3 = fb[2][0] * (dx[None, None, :] * dy[None, :, None])
+    k, j, i = sl[2], sl[1], sl[0]
+    ip = slice(i.start + 1, i.stop + 1)
+    jp = slice(j.start + 1, j.stop + 1)
+    kp = slice(k.start + 1, k.stop + 1)
+    div = (f1[k, j, ip] - f1[k, j, i]) + (f2[k, jp, i] - f2[k, j, i]) + (f3[kp, j, i] - f3[k, j, i])
+    assert np.abs(div).max() <= 1e-14 * np.abs(f1[k, j, i]).max()
+    x, y, z = centres(g.fine, 0)
+    for n in range(5):
+        ref = lin(coef[n], x[None, None, :], y[None, :, None], z[:, None, None])[k, j, i]
+        np.testing.assert_allclose(fw[0, n][k, j, i], ref, rtol=0, atol=1e-13)
+    assert (u[0, 0][k, j, i] == fw[0, 0][k, j, i]).all()
