@@ -289,8 +289,9 @@ int agx_ref_prim_to_cons(const agx_ref_list *l, const double *w, const double *b
  * (bvals_refine.cpp:172-199) and of the face prolongation in ProlongateGhostCells.  ox[3]
  * is the neighbour's offset (NeighborIndexes::ox1..3); lx_odd[3] = loc.lx1..3 & 1;
  * nblevel[27] holds the levels of the 3x3x3 neighbourhood minus this block's level, at
- * index ((ox3+1)*3 + ox2+1)*3 + ox1+1 (-1 where there is no neighbour).  out[0..5] = si, ei, sj, ej, sk, ek (coarse cells:
- * PROLONG_CC, PROLONG_FC_INT and PRIM_TO_CONS items), out[6..7] = il, iu (PROLONG_FC1
+ * index ((ox3+1)*3 + ox2+1)*3 + ox1+1 (-1 where there is no neighbour).  out[0..5] =
+ * si, ei, sj, ej, sk, ek (coarse cells: PROLONG_CC, PROLONG_FC_INT and PRIM_TO_CONS
+ * items), out[6..7] = il, iu (PROLONG_FC1
  * faces, with sj..ek), out[8..9] = jl, ju (PROLONG_FC2), out[10..11] = kl, ku
  * (PROLONG_FC3).  The reference's cnghost = (NGHOST + 1)/2 + 1 with NGHOST = g->ng.  Host
  * only; no device work. */
