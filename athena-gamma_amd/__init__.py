@@ -37,6 +37,7 @@ EXPORTS = [
     "agx_ref_restrict_fc", "agx_ref_prolong_cc", "agx_ref_prolong_fc_shared", "agx_ref_prolong_fc_internal",
     "agx_ref_flux_from_finer", "agx_ref_emf_to_coarser", "agx_ref_sync",
     "agx_device_alloc", "agx_device_free", "agx_device_copy", "agx_ref_prim_to_cons", "agx_ref_prolong_ranges",
+    "agx_upload_blocks", "agx_download_blocks",
 ]
 
 
@@ -116,6 +117,9 @@ def lib():
     L.agx_upload.argtypes = [P, I, D]
     L.agx_download.argtypes = [P, I, D]
     L.agx_set_time.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32]
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    L.agx_upload_blocks.argtypes = [P, I, ctypes.c_int32, I32P, I32P, ctypes.POINTER(D)]
+    L.agx_download_blocks.argtypes = [P, I, ctypes.c_int32, I32P, I32P, ctypes.POINTER(D)]
     L.agx_get_time.argtypes = [P, D, D, ctypes.POINTER(ctypes.c_int32)]
     L.agx_synchronize.argtypes = [P]
     L.agx_calculate_fluxes.argtypes = [P, I]
@@ -220,6 +224,27 @@ class Mesh:
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape(name))
         _check(self.L.agx_upload(self.h, ARR[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
                "agx_upload")
+
+    def _blocks(self, name, blocks, arrays, up):
+        n = len(blocks)
+        org = (ctypes.c_int32 * (3 * n))(*[v for o, _ in blocks for v in o])
+        siz = (ctypes.c_int32 * (3 * n))(*[v for _, z in blocks for v in z])
+        ptrs = (ctypes.POINTER(ctypes.c_double) * n)(*[a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+                                                        for a in arrays])
+        f = self.L.agx_upload_blocks if up else self.L.agx_download_blocks
+        _check(f(self.h, ARR[name], n, org, siz, ptrs), "agx_upload_blocks" if up else "agx_download_blocks")
+
+    def set_blocks(self, name, blocks, arrays):
+        """Upload MeshBlocks of uneven sizes: blocks = [((i0, j0, k0), (nx1, nx2, nx3)), ...],
+        arrays in the reference layout with ghosts (agx_upload_blocks)."""
+        arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+        self._blocks(name, blocks, arrays, True)
+
+    def get_blocks(self, name, blocks, shapes):
+        """Download MeshBlocks of uneven sizes, ghosts included (agx_download_blocks)."""
+        arrays = [np.zeros(s) for s in shapes]
+        self._blocks(name, blocks, arrays, False)
+        return arrays
 
     def time(self):
         t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()

@@ -2058,6 +2058,98 @@ int agx_upload(agx_mesh *m, int arr, const double *host) {
 }
 int agx_download(agx_mesh *m, int arr, double *host) { return copy_array(m, arr, host, false); }
 
+// MeshBlocks of uneven sizes on one level, mapped onto the device lattice of bnx blocks by
+// global index.  Cell arrays (u, u1, w, bcc, phi) and face fields (b, b1).  One download
+// of the lattice array, the per-block copies on the host, and (upload) one upload back.
+static int copy_blocks(agx_mesh *m, int arr, int32_t nblocks, const int32_t *origin, const int32_t *size,
+                       double *const *host, bool to_device) {
+  if (!m || arr < 0 || arr >= AGX_ARR_COUNT || nblocks < 0 || (nblocks > 0 && (!origin || !size || !host))) {
+    set_error("agx_%s_blocks: bad argument", to_device ? "upload" : "download");
+    return -1;
+  }
+  int fdir = -1;   // face normal of a face array
+  switch (arr) {
+    case AGX_ARR_U: case AGX_ARR_U1: case AGX_ARR_W: case AGX_ARR_BCC: case AGX_ARR_PHI: break;
+    case AGX_ARR_B1F: case AGX_ARR_R1F: fdir = 0; break;
+    case AGX_ARR_B2F: case AGX_ARR_R2F: fdir = 1; break;
+    case AGX_ARR_B3F: case AGX_ARR_R3F: fdir = 2; break;
+    default: set_error("agx_%s_blocks: array %d is not a cell array or face field", to_device ? "upload" : "download", arr);
+      return -53;
+  }
+  const Geo &g = m->g;
+  const int bn[3] = {g.bx1, g.bx2, g.bx3}, nl[3] = {g.nb1, g.nb2, g.nb3}, ng = g.ng;
+  const int N[3] = {g.bx1 * g.nb1, g.bx2 * g.nb2, g.bx3 * g.nb3};
+  for (int b = 0; b < nblocks; ++b)
+    for (int d = 0; d < 3; ++d) {
+      const int o = origin[3 * b + d], z = size[3 * b + d];
+      if (o < 0 || z < bn[d] || o % bn[d] || z % bn[d] || o + z > N[d]) {
+        set_error("block %d: origin and size must be multiples of the MeshBlock size %d inside the mesh (dir %d)",
+                  b, bn[d], d + 1);
+        return -54;
+      }
+    }
+  int64_t dims[4];
+  host_dims(m, arr, dims);
+  const int64_t nv = dims[0], L3 = dims[1], L2 = dims[2], L1 = dims[3];   // lattice block dims
+  std::vector<double> lat((size_t)(g.nb * nv * L3 * L2 * L1));
+  int rc = copy_array(m, arr, lat.data(), false);
+  if (rc) return rc;
+  // lattice element of global (cell or face) index I along d; second copy of a face shared
+  // by two lattice blocks (the lower block's upper face), or -1
+  auto lat_idx = [&](int d, int I, int &lb, int &loc, int &lb2, int &loc2) {
+    I = ((I % N[d]) + N[d]) % N[d];
+    lb = I / bn[d];
+    loc = I % bn[d] + ng;
+    lb2 = loc2 = -1;
+    if (d == fdir && I % bn[d] == 0) {
+      lb2 = (lb - 1 + nl[d]) % nl[d];
+      loc2 = bn[d] + ng;
+    }
+  };
+  for (int b = 0; b < nblocks; ++b) {
+    int n[3];
+    for (int d = 0; d < 3; ++d) n[d] = size[3 * b + d] + 2 * ng + (d == fdir ? 1 : 0);
+    double *h = host[b];
+    for (int v = 0; v < nv; ++v)
+      for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+          for (int i = 0; i < n[0]; ++i) {
+            const int loc3[3] = {i, j, k};
+            bool interior = true;
+            for (int d = 0; d < 3; ++d) {
+              const int hi = ng + size[3 * b + d] - 1 + (d == fdir ? 1 : 0);
+              if (loc3[d] < ng || loc3[d] > hi) interior = false;
+            }
+            if (to_device && !interior) continue;
+            int lb[3], lc[3], lb2[3], lc2[3];
+            for (int d = 0; d < 3; ++d) lat_idx(d, origin[3 * b + d] + loc3[d] - ng, lb[d], lc[d], lb2[d], lc2[d]);
+            const int64_t blk = ((int64_t)lb[2] * nl[1] + lb[1]) * nl[0] + lb[0];
+            const int64_t li = (((blk * nv + v) * L3 + lc[2]) * L2 + lc[1]) * L1 + lc[0];
+            double &hv = h[(((int64_t)v * n[2] + k) * n[1] + j) * n[0] + i];
+            if (!to_device) { hv = lat[li]; continue; }
+            lat[li] = hv;
+            if (fdir >= 0 && lb2[fdir] >= 0) {   // the same face in the lower lattice block
+              int b2[3] = {lb[0], lb[1], lb[2]}, c2[3] = {lc[0], lc[1], lc[2]};
+              b2[fdir] = lb2[fdir];
+              c2[fdir] = lc2[fdir];
+              const int64_t blk2 = ((int64_t)b2[2] * nl[1] + b2[1]) * nl[0] + b2[0];
+              lat[(((blk2 * nv + v) * L3 + c2[2]) * L2 + c2[1]) * L1 + c2[0]] = hv;
+            }
+          }
+  }
+  return to_device ? copy_array(m, arr, lat.data(), true) : 0;
+}
+
+int agx_upload_blocks(agx_mesh *m, int arr, int32_t nblocks, const int32_t *origin, const int32_t *size,
+                      const double *const *host) {
+  return copy_blocks(m, arr, nblocks, origin, size, const_cast<double *const *>(host), true);
+}
+
+int agx_download_blocks(agx_mesh *m, int arr, int32_t nblocks, const int32_t *origin, const int32_t *size,
+                        double *const *host) {
+  return copy_blocks(m, arr, nblocks, origin, size, host, false);
+}
+
 int agx_set_time(agx_mesh *m, double time, double dt, int32_t ncycle) {
   if (!m) { set_error("null mesh"); return -1; }
   m->time = time; m->dt = dt; m->ncycle = ncycle;

@@ -116,6 +116,20 @@ int64_t agx_mesh_bytes(agx_mesh *m);
 /* ---- host <-> device (AthenaArray copies at init/output only) */
 int agx_upload(agx_mesh *m, int arr, const double *host);
 int agx_download(agx_mesh *m, int arr, double *host);
+/* MeshBlocks of uneven sizes on one level (no counterpart in the reference's interface:
+ * there each MeshBlock, whatever its size, owns its AthenaArrays).  The device holds the
+ * mesh as the uniform lattice of desc.bnx blocks, so one launch updates every block and
+ * the results equal those of the uniform decomposition.  Block b covers mesh cells
+ * origin[3b+d] .. origin[3b+d] + size[3b+d] - 1 along direction d; both are multiples of
+ * bnx_d.  host[b] is its array in the layout above with n_d = size_d + 2*nghost (+1 along
+ * a face array's normal).  Upload copies the block's interior cells (faces: both boundary
+ * faces included).  Download fills the whole array, ghosts included, from the lattice
+ * (periodic mesh).  Cell arrays (U, U1, W, BCC, PHI) and face fields (B?F, R?F) only;
+ * for initialisation and output (one lattice round trip per call). */
+int agx_upload_blocks(agx_mesh *m, int arr, int32_t nblocks, const int32_t *origin, const int32_t *size,
+                      const double *const *host);
+int agx_download_blocks(agx_mesh *m, int arr, int32_t nblocks, const int32_t *origin, const int32_t *size,
+                        double *const *host);
 int agx_set_time(agx_mesh *m, double time, double dt, int32_t ncycle);
 int agx_get_time(agx_mesh *m, double *time, double *dt, int32_t *ncycle);
 int agx_synchronize(agx_mesh *m);
