@@ -37,7 +37,7 @@ EXPORTS = [
     "agx_ref_restrict_fc", "agx_ref_prolong_cc", "agx_ref_prolong_fc_shared", "agx_ref_prolong_fc_internal",
     "agx_ref_flux_from_finer", "agx_ref_emf_to_coarser", "agx_ref_sync",
     "agx_device_alloc", "agx_device_free", "agx_device_copy", "agx_ref_prim_to_cons", "agx_ref_prolong_ranges",
-    "agx_upload_blocks", "agx_download_blocks",
+    "agx_upload_blocks", "agx_download_blocks", "agx_set_mg_fuse",
 ]
 
 
@@ -137,6 +137,7 @@ def lib():
     L.agx_mg_vcycles.argtypes = [P, ctypes.c_int32]
     L.agx_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
     L.agx_set_tile.argtypes = [P, ctypes.c_int32]
+    L.agx_set_mg_fuse.argtypes = [P, ctypes.c_int32]
     L.agx_set_graph.argtypes = [P, ctypes.c_int32]
     L.agx_timing_enable.argtypes = [P, I]
     L.agx_timing_reset.argtypes = [P]
@@ -315,6 +316,9 @@ class Mesh:
     def set_graph(self, enable=True):
         """Replay agx_cycle as one hipGraph launch per cycle (no self-gravity)."""
         _check(self.L.agx_set_graph(self.h, 1 if enable else 0), "agx_set_graph")
+
+    def set_mg_fuse(self, mode):
+        _check(self.L.agx_set_mg_fuse(self.h, mode), "agx_set_mg_fuse")
 
     def set_tile(self, tile):
         _check(self.L.agx_set_tile(self.h, tile), "agx_set_tile")

@@ -146,6 +146,7 @@ struct agx_mesh {
   int mg_on = 0;
   int nrootlevel = 0, nmblevel = 0, ntotal = 0, current = 0, fmglevel = 0, coffset = 0;
   int mg_mode = 0, mg_niter = 0;
+  int mg_fuse = -1;                  // red-black pass form: -1 by level size, 0 two sweeps, 1 fused
   double mg_eps = -1.0, last_norm = 0.0, last_ave = 0.0;
   int32_t last_iters = 0;
   std::vector<agx::MGLevel> lev;
@@ -195,5 +196,6 @@ int mg_create(agx_mesh *m);
 void mg_destroy(agx_mesh *m);
 int mg_solve(agx_mesh *m, int stage);
 int mg_vcycles(agx_mesh *m, int n);
+int mg_autotune(agx_mesh *m);
 
 }  // namespace agx

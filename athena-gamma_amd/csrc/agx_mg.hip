@@ -1407,16 +1407,19 @@ static void smooth(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l, int color) {
 }
 
 // One red-black pass (colour 0 then colour 1) on level l: fused into one launch into the
-// level's spare array (then swapped with u) on FP64 levels of at least 2^26 cells, two
-// sweeps otherwise.  Measured on MI355X (V(1,1) cycles/s, two sweeps -> fused): 512^3
-// FP64 186 -> 231; at 256^3 (134 MB per level array, which the 256 MB last-level cache
-// holds between the two sweeps) two sweeps are faster (1746 vs 1600), as they are for
-// FP32 levels at 512^3 (231 vs 213).
+// level's spare array (then swapped with u), or two sweeps.  By default the fused form
+// runs on FP64 levels of at least 2^26 cells.  Measured on MI355X (V(1,1) cycles/s, two
+// sweeps -> fused): 512^3 FP64 186 -> 231; at 256^3 (134 MB per level array, which the
+// 256 MB last-level cache holds between the two sweeps) two sweeps are faster (1746 vs
+// 1600), as they are for FP32 levels at 512^3 (231 vs 213).  agx_autotune times both
+// (mg_fuse 0 or 1: fused on every level of at least 2^21 cells) and keeps the faster;
+// the results are identical either way.
 template <typename T>
 static void smooth_pass(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {
   MGLevelT<T> &L = H[l];
-  const bool fused = sizeof(T) == 8 && L.def && L.n1 % RB_TI == 0 && L.n2 % RB_TJ == 0 && (L.n3 & 1) == 0 &&
-                     L.n3 >= 4 && lcells(L) >= ((int64_t)1 << 26);
+  const bool big = m->mg_fuse < 0 ? (sizeof(T) == 8 && lcells(L) >= ((int64_t)1 << 26))
+                                  : (m->mg_fuse == 1 && lcells(L) >= ((int64_t)1 << 21));
+  const bool fused = big && L.def && L.n1 % RB_TI == 0 && L.n2 % RB_TJ == 0 && (L.n3 & 1) == 0 && L.n3 >= 4;
   if (!fused) {
     smooth(m, H, l, 0);
     smooth(m, H, l, 1);
@@ -1733,6 +1736,38 @@ int mg_solve(agx_mesh *m, int stage) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "mg_solve launches");
   return 0;
+}
+
+// Times V-cycles (on whatever the finest level holds; the next solve reloads it) with
+// the two red-black pass forms and keeps the faster (smooth_pass).
+int mg_autotune(agx_mesh *m) {
+  if (!m->mg_on || m->mp_on) return 0;
+  hipEvent_t e0, e1;
+  AGX_HIP(hipEventCreate(&e0));
+  AGX_HIP(hipEventCreate(&e1));
+  const bool timing = m->timer.enabled;
+  m->timer.enabled = false;
+  float best = 0.f;
+  int best_f = 0;
+  int rc = 0;
+  for (int f = 0; f <= 1 && rc == 0; ++f) {
+    m->mg_fuse = f;
+    rc = mg_vcycles(m, 1);   // warm-up
+    if (rc) break;
+    AGX_HIP(hipEventRecord(e0, m->stream));
+    rc = mg_vcycles(m, 3);
+    if (rc) break;
+    AGX_HIP(hipEventRecord(e1, m->stream));
+    AGX_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    AGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (f == 0 || ms < best) { best = ms; best_f = f; }
+  }
+  m->timer.enabled = timing;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  m->mg_fuse = best_f;
+  return rc;
 }
 
 // n V(1,1) cycles on the current finest state; in the mixed form one cycle is one

@@ -2722,6 +2722,13 @@ int agx_autotune(agx_mesh *m, int32_t *choice) {
   LAUNCH_CHECK("k_stage (autotune)");
   m->tile = best_t;
   if (choice) *choice = best_t;
+  return mg_autotune(m);
+}
+
+int agx_set_mg_fuse(agx_mesh *m, int32_t mode) {
+  if (!m) { set_error("null mesh"); return -1; }
+  if (mode < -1 || mode > 1) { set_error("red-black pass form %d not in {-1, 0, 1}", mode); return -2; }
+  m->mg_fuse = mode;
   return 0;
 }
 

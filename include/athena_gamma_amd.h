@@ -191,11 +191,17 @@ int agx_mg_vcycles(agx_mesh *m, int32_t n);
 /* ---- tuning (no counterpart in the reference: launch shapes of this library).
  * agx_autotune times each tile shape of the fused stage kernel on the current state
  * (index 0..3: 16x16, 32x8, 64x4 (i,j) columns per 256-thread workgroup, 64x8 per
- * 512-thread workgroup) and keeps the fastest; *choice gets its index.  It writes only scratch registers (u1, b1, fluxes), so call it between
- * cycles, e.g. after agx_initialize.  agx_set_tile forces a shape.  Results do not
- * depend on the shape. */
+ * 512-thread workgroup) and keeps the fastest; *choice gets its index.  With multigrid
+ * gravity (periodic boundaries) it also times V-cycles with the red-black pass fused into
+ * one launch and as two sweeps, and keeps the faster.  It writes only scratch registers
+ * (u1, b1, fluxes, the multigrid levels, which every solve reloads), so call it between
+ * cycles, e.g. after agx_initialize.  agx_set_tile forces a shape.  Results do not depend
+ * on either choice. */
 int agx_autotune(agx_mesh *m, int32_t *choice);
 int agx_set_tile(agx_mesh *m, int32_t tile);
+/* multigrid red-black pass form: -1 by level size (fused on FP64 levels of >= 2^26
+ * cells, the default), 0 two sweeps, 1 fused on every level of >= 2^21 cells */
+int agx_set_mg_fuse(agx_mesh *m, int32_t mode);
 
 /* ---- hipGraph replay of agx_cycle (no counterpart in the reference: launch mechanism).
  * With enable != 0, agx_cycle captures one whole VL2 cycle (fluxes, stage kernels,
