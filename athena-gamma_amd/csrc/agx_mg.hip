@@ -238,6 +238,48 @@ __device__ __forceinline__ T restricted_defect(const T *u, const T *src, int n1,
   return T(0.125) * (d000 + d001 + d010 + d011 + d100 + d101 + d110 + d111);
 }
 
+template <typename T>
+__device__ __forceinline__ typename Vec2<T>::type load2(const T *a, int64_t id) {   // id even
+  return *reinterpret_cast<const typename Vec2<T>::type *>(a + id);
+}
+
+// restricted_defect with the fine values read as (i, i+1) pairs: a cross of 12 row pairs
+// (k-1..k+2 x j-1..j+2 without the corners), 8 i-neighbours and 4 source pairs, 24 loads
+// instead of 64.  Every defect and the average use the operands and order of
+// restricted_defect / defect_at, so the result is the same bit for bit.
+template <typename T>
+__device__ __forceinline__ T restricted_defect_pairs(const T *u, const T *src, int n1, int n2, int n3,
+                                                     int k, int j, int i, T idx2) {
+  using V = typename Vec2<T>::type;
+  const int fi = 2 * i, fj = 2 * j, fk = 2 * k;
+  const int kk[4] = {wrapm(fk, n3), fk, fk + 1, wrapp(fk + 1, n3)};
+  const int jj[4] = {wrapm(fj, n2), fj, fj + 1, wrapp(fj + 1, n2)};
+  const int im = wrapm(fi, n1), ip = wrapp(fi + 1, n1);
+  V P[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if ((a == 1 || a == 2) || (b == 1 || b == 2)) P[a][b] = load2(u, LIX(n1, n2, kk[a], jj[b], fi));
+  T d[2][2][2];
+#pragma unroll
+  for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj) {
+      const int a = 1 + dk, b = 1 + dj;
+      const V sv = load2(src, LIX(n1, n2, kk[a], jj[b], fi));
+      const T um = u[LIX(n1, n2, kk[a], jj[b], im)], up = u[LIX(n1, n2, kk[a], jj[b], ip)];
+      // di = 0: cell fi (i+1 neighbour is the pair's second value)
+      d[dk][dj][0] = (T(6.0) * P[a][b].x - P[a + 1][b].x - P[a][b + 1].x - P[a][b].y - P[a - 1][b].x -
+                      P[a][b - 1].x - um) * idx2 + sv.x;
+      // di = 1: cell fi + 1 (i-1 neighbour is the pair's first value)
+      d[dk][dj][1] = (T(6.0) * P[a][b].y - P[a + 1][b].y - P[a][b + 1].y - up - P[a - 1][b].y -
+                      P[a][b - 1].y - P[a][b].x) * idx2 + sv.y;
+    }
+  return T(0.125) * (d[0][0][0] + d[0][0][1] + d[0][1][0] + d[0][1][1] + d[1][0][0] + d[1][0][1] +
+                     d[1][1][0] + d[1][1][1]);
+}
+
 // RestrictBlock (multigrid/multigrid.cpp:331-355): defect on the fine level, 8-cell
 // average into the coarse source (Restrict, multigrid.cpp:684-724), coarse u zeroed
 // (ZeroClearData, non-FAS).  One thread per coarse cell.
@@ -253,7 +295,7 @@ k_mg_defect_restrict(const T *__restrict__ u, const T *__restrict__ src,
   unsigned r = t / (unsigned)c1;
   int j = (int)(r % (unsigned)c2);
   int k = (int)(r / (unsigned)c2);
-  csrc[t] = restricted_defect(u, src, n1, n2, n3, k, j, i, idx2);
+  csrc[t] = restricted_defect_pairs(u, src, n1, n2, n3, k, j, i, idx2);
   cu[t] = T(0.0);
 }
 
@@ -305,6 +347,41 @@ k_mg_prolong(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3
   int fj = (int)(r % (unsigned)n2);
   int fk = (int)(r / (unsigned)n2);
   u[t] += prolong_corr(cu, n1 >> 1, n2 >> 1, n3 >> 1, fk, fj, fi);
+}
+
+// k_mg_prolong with one thread per (i, i+1) pair of fine cells, the two children of coarse
+// cell i along x1: they share the coarse values at i-1, i, i+1 of four coarse rows, and
+// the fine pair is read and written as one vector.  Per cell the arithmetic of
+// prolong_corr.
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_mg_prolong_pairs(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3) {
+  using V = typename Vec2<T>::type;
+  const unsigned nh = (unsigned)n1 >> 1;
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 pairs per level
+  if (t >= nh * (unsigned)n2 * (unsigned)n3) return;
+  const int i = (int)(t % nh);
+  const unsigned r = t / nh;
+  const int fj = (int)(r % (unsigned)n2), fk = (int)(r / (unsigned)n2);
+  const int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
+  const int j = fj >> 1, k = fk >> 1;
+  const int jp = wrap(j + ((fj & 1) ? 1 : -1), c2), kp = wrap(k + ((fk & 1) ? 1 : -1), c3);
+  const int im = wrap(i - 1, c1), ipp = wrap(i + 1, c1);
+#define C(kk, jj, ii) cu[LIX(c1, c2, kk, jj, ii)]
+  const T a0 = C(k, j, i), am = C(k, j, im), ap = C(k, j, ipp);
+  const T b0 = C(k, jp, i), bm = C(k, jp, im), bp = C(k, jp, ipp);
+  const T c0 = C(kp, j, i), cm = C(kp, j, im), cp = C(kp, j, ipp);
+  const T e0 = C(kp, jp, i), em = C(kp, jp, im), ep = C(kp, jp, ipp);
+#undef C
+  // prolong_corr with ip = i - 1 (low child) and i + 1 (high child)
+  const T lo = T(0.015625) * (T(27.0) * a0 + em + T(9.0) * (am + b0 + c0) + T(3.0) * (e0 + cm + bm));
+  const T hi = T(0.015625) * (T(27.0) * a0 + ep + T(9.0) * (ap + b0 + c0) + T(3.0) * (e0 + cp + bp));
+  V *u2 = reinterpret_cast<V *>(u);
+  const int64_t id = ((int64_t)fk * n2 + fj) * nh + i;
+  V v = u2[id];
+  v.x += lo;
+  v.y += hi;
+  u2[id] = v;
 }
 
 // FMGProlongate (multigrid/multigrid.cpp:838-1048): 27-point interpolation whose
@@ -1447,8 +1524,12 @@ template <typename T>
 static void prolong(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {   // l-1 -> l
   MGLevelT<T> &F = H[l], &C = H[l - 1];
   timer_begin(m, AGX_K_MG_PROLONG);
-  hipLaunchKernelGGL(k_mg_prolong<T>, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, C.u, F.u, F.n1,
-                     F.n2, F.n3);
+  if ((F.n1 & 1) == 0)   // every level above the root: n1 = 2 * (coarse n1)
+    hipLaunchKernelGGL(k_mg_prolong_pairs<T>, dim3(grid1d(lcells(F) / 2, BS)), dim3(BS), 0, m->stream, C.u, F.u,
+                       F.n1, F.n2, F.n3);
+  else
+    hipLaunchKernelGGL(k_mg_prolong<T>, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, C.u, F.u, F.n1,
+                       F.n2, F.n3);
   timer_end(m);
 }
 
