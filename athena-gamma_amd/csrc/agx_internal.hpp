@@ -107,6 +107,7 @@ struct MGLevelB {
   double dx;               // cell size of the smoother/defect (as MGLevel::dx)
   double h[3];             // cell size per direction for coordinates: (xmax - xmin) / n
   double *u, *src, *uold;
+  double *bcv;             // multipole boundary values of each ghost's chain (3 per ghost)
 };
 
 struct Timer {

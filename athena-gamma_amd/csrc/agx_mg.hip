@@ -793,52 +793,79 @@ __device__ double mp_phis(const double *__restrict__ mpc, int order, double x, d
                 (35.0 * z2 * z2 - 30.0 * z2 * r2 + 3.0 * r2 * r2) * mpc[20]);
 }
 
-// Fills every ghost of a framed level.  Thread q enumerates the shell: the two k planes,
-// then the two j rows of each inner plane, then the two i ends of each inner row.
-__global__ void __launch_bounds__(BS)
-k_mb_bc(double *__restrict__ u, MPGeom G, const double *__restrict__ mp, int order) {
+// Ghost q of a level's frame and its chain of boundary points inwards.  q enumerates the
+// shell: the two k planes, then the two j rows of each inner plane, then the two i ends
+// of each inner row.  The reference fills x1 faces, then x2 faces over all i, then x3
+// faces over all (i, j), so an edge or corner ghost is 2 phi_s(p) minus the value inside
+// it along the last face that writes it (at most three points); the chain ends at the
+// interior cell (kk, jj, ii).  False past the last ghost.
+struct GhostChain {
+  int k, j, i, kk, jj, ii, depth;
+  double px[3], py[3], pz[3];
+};
+__device__ __forceinline__ bool ghost_chain(const MPGeom &G, unsigned q, GhostChain &c) {
   const int n1 = G.n1, n2 = G.n2, n3 = G.n3;
   const unsigned w1 = n1 + 2, w2 = n2 + 2;
   const unsigned np = 2 * w1 * w2, nr = 2 * (unsigned)n3 * w1, nc = 2 * (unsigned)n3 * n2;
-  unsigned q = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-  int k, j, i;
   if (q < np) {
     const unsigned side = q / (w1 * w2), r = q - side * w1 * w2;
-    k = side ? n3 + 1 : 0; j = r / w1; i = r - j * w1;
+    c.k = side ? n3 + 1 : 0; c.j = r / w1; c.i = r - c.j * w1;
   } else if ((q -= np) < nr) {
     const unsigned row = q / w1;
-    i = q - row * w1; k = 1 + row / 2; j = (row & 1) ? n2 + 1 : 0;
+    c.i = q - row * w1; c.k = 1 + row / 2; c.j = (row & 1) ? n2 + 1 : 0;
   } else if ((q -= nr) < nc) {
     const unsigned row = q / 2;
-    i = (q & 1) ? n1 + 1 : 0; k = 1 + row / n2; j = 1 + row % n2;
+    c.i = (q & 1) ? n1 + 1 : 0; c.k = 1 + row / n2; c.j = 1 + row % n2;
   } else {
-    return;
+    return false;
   }
-  // chain of boundary points from this ghost inwards (at most 3), then the interior cell
-  double px[3], py[3], pz[3];
-  int depth = 0, kk = k, jj = j, ii = i;
-  while (depth < 3) {
-    if (kk == 0 || kk == n3 + 1) {
-      px[depth] = mp_centre(G, 0, ii); py[depth] = mp_centre(G, 1, jj);
-      pz[depth] = mp_face(G, 2, kk == 0 ? 1 : n3 + 1);
-      kk = (kk == 0) ? 1 : n3;
-    } else if (jj == 0 || jj == n2 + 1) {
-      px[depth] = mp_centre(G, 0, ii); pz[depth] = mp_centre(G, 2, kk);
-      py[depth] = mp_face(G, 1, jj == 0 ? 1 : n2 + 1);
-      jj = (jj == 0) ? 1 : n2;
-    } else if (ii == 0 || ii == n1 + 1) {
-      py[depth] = mp_centre(G, 1, jj); pz[depth] = mp_centre(G, 2, kk);
-      px[depth] = mp_face(G, 0, ii == 0 ? 1 : n1 + 1);
-      ii = (ii == 0) ? 1 : n1;
+  c.depth = 0;
+  c.kk = c.k; c.jj = c.j; c.ii = c.i;
+  while (c.depth < 3) {
+    const int d = c.depth;
+    if (c.kk == 0 || c.kk == n3 + 1) {
+      c.px[d] = mp_centre(G, 0, c.ii); c.py[d] = mp_centre(G, 1, c.jj);
+      c.pz[d] = mp_face(G, 2, c.kk == 0 ? 1 : n3 + 1);
+      c.kk = (c.kk == 0) ? 1 : n3;
+    } else if (c.jj == 0 || c.jj == n2 + 1) {
+      c.px[d] = mp_centre(G, 0, c.ii); c.pz[d] = mp_centre(G, 2, c.kk);
+      c.py[d] = mp_face(G, 1, c.jj == 0 ? 1 : n2 + 1);
+      c.jj = (c.jj == 0) ? 1 : n2;
+    } else if (c.ii == 0 || c.ii == n1 + 1) {
+      c.py[d] = mp_centre(G, 1, c.jj); c.pz[d] = mp_centre(G, 2, c.kk);
+      c.px[d] = mp_face(G, 0, c.ii == 0 ? 1 : n1 + 1);
+      c.ii = (c.ii == 0) ? 1 : n1;
     } else {
       break;
     }
-    ++depth;
+    ++c.depth;
   }
-  double v = u[BX(n1, n2, kk, jj, ii)];
-  for (int d = depth - 1; d >= 0; --d)
-    v = 2.0 * mp_phis(mp + 3, order, px[d] - mp[0], py[d] - mp[1], pz[d] - mp[2]) - v;
-  u[BX(n1, n2, k, j, i)] = v;
+  return true;
+}
+
+// The multipole expansion at every boundary point of every ghost's chain, once per solve
+// and level (the coefficients and origin are fixed during a solve): bcv[3q + d].
+__global__ void __launch_bounds__(BS)
+k_mb_bc_fill(double *__restrict__ bcv, MPGeom G, const double *__restrict__ mp, int order) {
+  const unsigned q = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  GhostChain c;
+  if (!ghost_chain(G, q, c)) return;
+  for (int d = 0; d < c.depth; ++d)
+    bcv[3 * (int64_t)q + d] = mp_phis(mp + 3, order, c.px[d] - mp[0], c.py[d] - mp[1], c.pz[d] - mp[2]);
+}
+
+// MGMultipoleInnerX1..OuterX3 (mgbval_multipole.cpp:28-426) on every ghost of a level in
+// one launch: ghost = 2 phi_s(p) - (the value inside it), along the chain, with phi_s from
+// bcv (the same doubles k_mb_bc_fill computed).
+__global__ void __launch_bounds__(BS)
+k_mb_bc(double *__restrict__ u, MPGeom G, const double *__restrict__ bcv) {
+  const unsigned q = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  GhostChain c;
+  if (!ghost_chain(G, q, c)) return;
+  const int n1 = G.n1, n2 = G.n2;
+  double v = u[BX(n1, n2, c.kk, c.jj, c.ii)];
+  for (int d = c.depth - 1; d >= 0; --d) v = 2.0 * bcv[3 * (int64_t)q + d] - v;
+  u[BX(n1, n2, c.k, c.j, c.i)] = v;
 }
 
 // red-black update of the framed level (MGGravity::Smooth, mg_gravity.cpp:220-273)
@@ -856,6 +883,7 @@ k_mb_smooth(double *__restrict__ u, const double *__restrict__ src, int n1, int 
   u[id] = gs_value(u[id], u[id + sk], u[id + sj], u[id + 1], u[id - sk], u[id - sj], u[id - 1], src[id], dx2,
                    isix);
 }
+
 
 __device__ __forceinline__ double mb_defect(const double *u, const double *src, unsigned id, unsigned sj,
                                             unsigned sk, double idx2) {
@@ -1152,7 +1180,7 @@ static int mp_create(agx_mesh *m) {
     B.h[0] = (m->d.x1max - m->d.x1min) / B.n1;
     B.h[1] = (m->d.x2max - m->d.x2min) / B.n2;
     B.h[2] = (m->d.x3max - m->d.x3min) / B.n3;
-    tot += 3 * pad32(bcells(B));
+    tot += 3 * pad32(bcells(B)) + pad32(3 * bghosts(B));
   }
   AGX_HIP(hipMalloc(&m->mgpoolb, tot * (int64_t)sizeof(double)));
   AGX_HIP(hipMemsetAsync(m->mgpoolb, 0, tot * sizeof(double), m->stream));
@@ -1164,6 +1192,7 @@ static int mp_create(agx_mesh *m) {
     B.u = p; p += n;
     B.src = p; p += n;
     B.uold = p; p += n;
+    B.bcv = p; p += pad32(3 * bghosts(B));
   }
   AGX_HIP(hipMalloc(&m->mpdev, sizeof(double) * (28 + NMOM * NPART)));
   AGX_HIP(hipMemsetAsync(m->mpdev, 0, sizeof(double) * (28 + NMOM * NPART), m->stream));
@@ -1183,8 +1212,7 @@ static MPGeom mp_geom(const agx_mesh *m, const MGLevelB &B) {
 static void bc_b(agx_mesh *m, int l) {
   MGLevelB &B = m->levb[l];
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mb_bc, dim3(grid1d(bghosts(B), BS)), dim3(BS), 0, m->stream, B.u, mp_geom(m, B), m->mpdev,
-                     m->d.mporder);
+  hipLaunchKernelGGL(k_mb_bc, dim3(grid1d(bghosts(B), BS)), dim3(BS), 0, m->stream, B.u, mp_geom(m, B), B.bcv);
   timer_end(m);
 }
 static void smooth_b(agx_mesh *m, int l, int color) {
@@ -1306,6 +1334,12 @@ static void mp_coefficients(agx_mesh *m) {
                      0, part);
   hipLaunchKernelGGL(k_mb_moments_final, dim3(1), dim3(BS), 0, m->stream, part, (int)nb, m->d.mporder, 0,
                      m->mpdev);
+  // the boundary values of every level for this solve (the ghost fills then only gather)
+  for (int l = 0; l < m->ntotal; ++l) {
+    MGLevelB &B = m->levb[l];
+    hipLaunchKernelGGL(k_mb_bc_fill, dim3(grid1d(bghosts(B), BS)), dim3(BS), 0, m->stream, B.bcv, mp_geom(m, B),
+                       m->mpdev, m->d.mporder);
+  }
 }
 
 // MGGravityDriver::Solve (mg_gravity.cpp:166-211) with multipole boundaries
