@@ -7,8 +7,8 @@ Against the reference's own regression thresholds for 3D linear waves
   - L- and R-going fast waves give the same RMS error at the precision the
     reference prints and compares (%e, linear_wave.cpp:405; :159-162).
 The reference applies these to runs with static refinement regions; here the mesh is
-uniform (no refinement), which is the configuration of the L/R test (:88-95) and of
-BASELINE config 1.
+uniform (no refinement), which is the configuration of the L/R test (:88-95) and of the
+single 64x32x32 MeshBlock linear fast-wave case (periodic 3x1.5x1.5 box).
 Plus invariants: conservation to round-off, div B preserved, uniform states stay
 uniform, bit-identical results for any MeshBlock decomposition, and HLLD (5 waves)
 less diffusive than HLLE (2 waves).
