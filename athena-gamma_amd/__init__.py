@@ -38,6 +38,8 @@ EXPORTS = [
     "agx_ref_flux_from_finer", "agx_ref_emf_to_coarser", "agx_ref_sync",
     "agx_device_alloc", "agx_device_free", "agx_device_copy", "agx_ref_prim_to_cons", "agx_ref_prolong_ranges",
     "agx_upload_blocks", "agx_download_blocks", "agx_set_mg_fuse",
+    "agx_ref_emf_count_same_level", "agx_ref_emf_same_level", "agx_ref_emf_set", "agx_ref_emf_clear",
+    "agx_ref_emf_average", "agx_ref_count_fine_edges",
 ]
 
 
@@ -92,7 +94,8 @@ class RefCoords(ctypes.Structure):
 
 REF_KINDS = dict(restrict_cc=0, restrict_fc1=1, restrict_fc2=2, restrict_fc3=3, prolong_cc=4,
                  prolong_fc1=5, prolong_fc2=6, prolong_fc3=7, prolong_fc_int=8, flux_from_finer=9,
-                 emf_to_coarser=10, prim_to_cons=11)
+                 emf_to_coarser=10, prim_to_cons=11, emf_same_level=12, emf_set_same=13, emf_set_finer=14,
+                 emf_clear=15, emf_average=16)
 
 _LIB = None
 
@@ -159,6 +162,13 @@ def lib():
     L.agx_ref_prim_to_cons.argtypes = [P, P, P, P, P, P, P, ctypes.c_double, ctypes.c_int32, P]
     IP = ctypes.POINTER(ctypes.c_int32)
     L.agx_ref_prolong_ranges.argtypes = [G, IP, IP, IP, IP]
+    L.agx_ref_emf_count_same_level.restype = ctypes.c_int64
+    L.agx_ref_emf_count_same_level.argtypes = [G, ctypes.c_int32]
+    L.agx_ref_emf_same_level.argtypes = [P, P, P, P, P, ctypes.c_int64, P]
+    L.agx_ref_emf_set.argtypes = [P, P, P, P, P, ctypes.c_int64, P]
+    L.agx_ref_emf_clear.argtypes = [P, P, P, P, P]
+    L.agx_ref_emf_average.argtypes = [P, P, P, P, P]
+    L.agx_ref_count_fine_edges.argtypes = [IP, IP, IP]
     L.agx_device_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(P)]
     L.agx_device_free.argtypes = [P]
     L.agx_device_copy.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
@@ -421,6 +431,18 @@ class RefList:
     def emf_to_coarser(self, e1, e2, e3, fc, buf, stride):
         _check(lib().agx_ref_emf_to_coarser(self.h, e1, e2, e3, fc, buf, stride, None), "agx_ref_emf_to_coarser")
 
+    def emf_same_level(self, e1, e2, e3, buf, stride):
+        _check(lib().agx_ref_emf_same_level(self.h, e1, e2, e3, buf, stride, None), "agx_ref_emf_same_level")
+
+    def emf_set(self, e1, e2, e3, buf, stride):
+        _check(lib().agx_ref_emf_set(self.h, e1, e2, e3, buf, stride, None), "agx_ref_emf_set")
+
+    def emf_clear(self, e1, e2, e3):
+        _check(lib().agx_ref_emf_clear(self.h, e1, e2, e3, None), "agx_ref_emf_clear")
+
+    def emf_average(self, e1, e2, e3):
+        _check(lib().agx_ref_emf_average(self.h, e1, e2, e3, None), "agx_ref_emf_average")
+
     @staticmethod
     def sync():
         _check(lib().agx_ref_sync(None), "agx_ref_sync")
@@ -428,6 +450,18 @@ class RefList:
 
 def ref_emf_count(geom, fid):
     return int(lib().agx_ref_emf_count(ctypes.byref(geom), fid))
+
+
+def ref_emf_count_same_level(geom, fid):
+    return int(lib().agx_ref_emf_count_same_level(ctypes.byref(geom), fid))
+
+
+def ref_count_fine_edges(nblevel):
+    """agx_ref_count_fine_edges: (edge_flag[12] as bools, nedge_fine[12])."""
+    I27, I12 = ctypes.c_int32 * 27, ctypes.c_int32 * 12
+    fl, nf = I12(), I12()
+    _check(lib().agx_ref_count_fine_edges(I27(*nblevel), fl, nf), "agx_ref_count_fine_edges")
+    return [bool(x) for x in fl], list(nf)
 
 
 def ref_prolong_ranges(geom, ox, lx_odd, nblevel):

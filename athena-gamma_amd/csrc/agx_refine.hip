@@ -27,8 +27,10 @@ struct agx_ref_list {
   int32_t kind = 0;
   int32_t nitems = 0;
   int32_t G = 1;              // workgroups per item
-  int64_t emf_max = 0;        // largest per-item output of an EMF_TO_COARSER list
+  int64_t emf_max = 0;        // largest per-item buffer length of an EMF list
   int64_t max_count = 0;      // largest per-item index count (times nvar must stay < 2^31)
+  int32_t nphase = 1;         // EMF_SET lists: launches in order, items sorted by phase
+  int32_t phase_off[17] = {0};
   agx_ref_item *items = nullptr;   // device
 };
 
@@ -419,6 +421,12 @@ __host__ __device__ inline int64_t emf_part(const RG &r, int dir, int part) {
   if (dir == 1) return part == 0 ? (int64_t)(r.h3 + 1) * r.h1 : (int64_t)r.h3 * (r.h1 + 1);
   return part == 0 ? (int64_t)(r.h2 + 1) * r.h1 : (int64_t)r.h2 * (r.h1 + 1);
 }
+// values of a to-coarser buffer for face fid 0..5 or edge 6 + eid (one restricted line)
+__host__ __device__ inline int64_t emf_coarser_count(const RG &r, int fid) {
+  if (fid < 6) return emf_part(r, fid >> 1, 0) + emf_part(r, fid >> 1, 1);
+  const int eid = fid - 6;
+  return eid < 4 ? r.h3 : (eid < 8 ? r.h2 : r.h1);
+}
 
 // LoadFluxBoundaryBufferToCoarser for face neighbours of a 3D block,
 // flux_correction_fc.cpp:315-418: the buffer order of the reference
@@ -428,13 +436,33 @@ k_ref_emf_to_coarser(RG r, const double *__restrict__ e1, const double *__restri
                      const agx_ref_item *__restrict__ items) {
   const agx_ref_item it = items[blockIdx.y];
   const int dir = it.fid >> 1, side = it.fid & 1, fb = it.fb;
-  const unsigned pa = (unsigned)emf_part(r, dir, 0), cnt = pa + (unsigned)emf_part(r, dir, 1);
   const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
                *d3 = fc.dx3f + (int64_t)fb * r.n3;
   auto E1 = [&](int k, int j, int i) { return e1[ix(r.n3 + 1, r.n2 + 1, r.n1, 1, fb, 0, k, j, i)]; };
   auto E2 = [&](int k, int j, int i) { return e2[ix(r.n3 + 1, r.n2, r.n1 + 1, 1, fb, 0, k, j, i)]; };
   auto E3 = [&](int k, int j, int i) { return e3[ix(r.n3, r.n2 + 1, r.n1 + 1, 1, fb, 0, k, j, i)]; };
   double *out = buf + (int64_t)blockIdx.y * stride;
+  if (it.fid >= 6) {   // edge neighbours, flux_correction_fc.cpp:481-546: one line, pairs along it
+    const int eid = it.fid - 6, lo = eid & 1, hi = (eid >> 1) & 1;
+    const unsigned cnt = (unsigned)emf_coarser_count(r, it.fid);
+    for (unsigned t = first_t(); t < cnt; t += stride_t()) {
+      const int s2 = 2 * (int)t;
+      double v;
+      if (eid < 4) {
+        const int i = lo ? r.ie + 1 : r.is, j = hi ? r.je + 1 : r.js, k = r.ks + s2;
+        v = (E3(k, j, i) * d3[k] + E3(k + 1, j, i) * d3[k + 1]) / (d3[k] + d3[k + 1]);
+      } else if (eid < 8) {
+        const int i = lo ? r.ie + 1 : r.is, k = hi ? r.ke + 1 : r.ks, j = r.js + s2;
+        v = (E2(k, j, i) * d2[j] + E2(k, j + 1, i) * d2[j + 1]) / (d2[j] + d2[j + 1]);
+      } else {
+        const int j = lo ? r.je + 1 : r.js, k = hi ? r.ke + 1 : r.ks, i = r.is + s2;
+        v = (E1(k, j, i) * d1[i] + E1(k, j, i + 1) * d1[i + 1]) / (d1[i] + d1[i + 1]);
+      }
+      out[t] = v;
+    }
+    return;
+  }
+  const unsigned pa = (unsigned)emf_part(r, dir, 0), cnt = pa + (unsigned)emf_part(r, dir, 1);
   for (unsigned t = first_t(); t < cnt; t += stride_t()) {
     const bool first = t < pa;
     const unsigned s = first ? t : t - pa;
@@ -471,6 +499,147 @@ k_ref_emf_to_coarser(RG r, const double *__restrict__ e1, const double *__restri
   }
 }
 
+// ---------------------------------------------------------------- EMF correction steps
+// The elements an item touches form one or two boxes of one edge array each (e1, e2,
+// e3), walked k outer, j, i inner as the reference's loops, so the packed buffers keep
+// the reference's order.  Faces are fid 0..5, edges 6 + eid (0..3 x1x2, 4..7 x1x3,
+// 8..11 x2x3; bit 0 = upper side along the first direction, bit 1 along the second).
+struct EBox {
+  int arr;              // 0, 1, 2 = e1, e2, e3
+  int k0, j0, i0, nk, nj, ni;
+};
+struct EBoxes {
+  EBox b[2];
+  unsigned cnt0, cnt;   // elements of box 0, of both
+  double scale;         // AVERAGE: the factor
+};
+
+__host__ __device__ inline EBox ebox(int arr, int k0, int k1, int j0, int j1, int i0, int i1) {
+  return EBox{arr, k0, j0, i0, k1 - k0 + 1, j1 - j0 + 1, i1 - i0 + 1};
+}
+__host__ __device__ inline int face_at(const RG &r, int dir, int upper) {
+  return dir == 0 ? (upper ? r.ie + 1 : r.is) : (dir == 1 ? (upper ? r.je + 1 : r.js) : (upper ? r.ke + 1 : r.ks));
+}
+
+// Boxes of item `it` for an EMF kind:
+//  SAME_LEVEL, SET_SAME: LoadFluxBoundaryBufferSameLevel / SetFluxBoundarySameLevel
+//    (flux_correction_fc.cpp:51-313, 689-940): the whole face or edge;
+//  SET_FINER: SetFluxBoundaryFromFiner (:944-1185): the quadrant fi1/fi2 of a face,
+//    the half fi1 of an edge;
+//  CLEAR: ClearCoarseFluxBoundary (:1214-1348): inside the face (not on its own edges),
+//    or the whole edge;
+//  AVERAGE: AverageFluxBoundary (:1355-1543): inside the face (fi1 = 0, a neighbour on
+//    the same level), the two lines where the finer blocks meet (fi1 = 1), or the whole
+//    edge (fi1 = number of finest-level blocks around it).
+__host__ __device__ inline EBoxes emf_boxes(const RG &r, int kind, const agx_ref_item &it) {
+  EBoxes o;
+  o.scale = 0.5;
+  o.b[1] = EBox{0, 0, 0, 0, 0, 0, 0};
+  const int fid = it.fid;
+  int il = r.is, iu = r.ie, jl = r.js, ju = r.je, kl = r.ks, ku = r.ke;
+  const bool finer = kind == AGX_REF_EMF_SET_FINER;
+  if (fid < 6) {
+    const int dir = fid >> 1, x = face_at(r, dir, fid & 1);
+    if (kind == AGX_REF_EMF_CLEAR || (kind == AGX_REF_EMF_AVERAGE && it.fi1 == 0)) {
+      if (dir == 0) {
+        o.b[0] = ebox(1, r.ks + 1, r.ke, r.js, r.je, x, x);
+        o.b[1] = ebox(2, r.ks, r.ke, r.js + 1, r.je, x, x);
+      } else if (dir == 1) {
+        o.b[0] = ebox(0, r.ks + 1, r.ke, x, x, r.is, r.ie);
+        o.b[1] = ebox(2, r.ks, r.ke, x, x, r.is + 1, r.ie);
+      } else {
+        o.b[0] = ebox(0, x, x, r.js + 1, r.je, r.is, r.ie);
+        o.b[1] = ebox(1, x, x, r.js, r.je, r.is + 1, r.ie);
+      }
+    } else if (kind == AGX_REF_EMF_AVERAGE) {
+      if (dir == 0) {
+        o.b[0] = ebox(1, r.ks + r.h3, r.ks + r.h3, r.js, r.je, x, x);
+        o.b[1] = ebox(2, r.ks, r.ke, r.js + r.h2, r.js + r.h2, x, x);
+      } else if (dir == 1) {
+        o.b[0] = ebox(0, r.ks + r.h3, r.ks + r.h3, x, x, r.is, r.ie);
+        o.b[1] = ebox(2, r.ks, r.ke, x, x, r.is + r.h1, r.is + r.h1);
+      } else {
+        o.b[0] = ebox(0, x, x, r.js + r.h2, r.js + r.h2, r.is, r.ie);
+        o.b[1] = ebox(1, x, x, r.js, r.je, r.is + r.h1, r.is + r.h1);
+      }
+    } else {
+      if (dir == 0) {
+        if (finer) {
+          if (it.fi1 == 0) ju = r.js + r.h2 - 1; else jl = r.js + r.h2;
+          if (it.fi2 == 0) ku = r.ks + r.h3 - 1; else kl = r.ks + r.h3;
+        }
+        o.b[0] = ebox(1, kl, ku + 1, jl, ju, x, x);
+        o.b[1] = ebox(2, kl, ku, jl, ju + 1, x, x);
+      } else if (dir == 1) {
+        if (finer) {
+          if (it.fi1 == 0) iu = r.is + r.h1 - 1; else il = r.is + r.h1;
+          if (it.fi2 == 0) ku = r.ks + r.h3 - 1; else kl = r.ks + r.h3;
+        }
+        o.b[0] = ebox(0, kl, ku + 1, x, x, il, iu);
+        o.b[1] = ebox(2, kl, ku, x, x, il, iu + 1);
+      } else {
+        if (finer) {
+          if (it.fi1 == 0) iu = r.is + r.h1 - 1; else il = r.is + r.h1;
+          if (it.fi2 == 0) ju = r.js + r.h2 - 1; else jl = r.js + r.h2;
+        }
+        o.b[0] = ebox(0, x, x, jl, ju + 1, il, iu);
+        o.b[1] = ebox(1, x, x, jl, ju, il, iu + 1);
+      }
+    }
+  } else {
+    const int eid = fid - 6, lo = eid & 1, hi = (eid >> 1) & 1;
+    if (kind == AGX_REF_EMF_AVERAGE) o.scale = 1.0 / (double)it.fi1;
+    if (eid < 4) {
+      const int i = face_at(r, 0, lo), j = face_at(r, 1, hi);
+      if (finer) { if (it.fi1 == 0) ku = r.ks + r.h3 - 1; else kl = r.ks + r.h3; }
+      o.b[0] = ebox(2, kl, ku, j, j, i, i);
+    } else if (eid < 8) {
+      const int i = face_at(r, 0, lo), k = face_at(r, 2, hi);
+      if (finer) { if (it.fi1 == 0) ju = r.js + r.h2 - 1; else jl = r.js + r.h2; }
+      o.b[0] = ebox(1, k, k, jl, ju, i, i);
+    } else {
+      const int j = face_at(r, 1, lo), k = face_at(r, 2, hi);
+      if (finer) { if (it.fi1 == 0) iu = r.is + r.h1 - 1; else il = r.is + r.h1; }
+      o.b[0] = ebox(0, k, k, j, j, il, iu);
+    }
+  }
+  o.cnt0 = (unsigned)(o.b[0].nk * o.b[0].nj * o.b[0].ni);
+  o.cnt = o.cnt0 + (unsigned)(o.b[1].nk * o.b[1].nj * o.b[1].ni);
+  return o;
+}
+
+enum { EOP_PACK = 0, EOP_ADD = 1, EOP_ZERO = 2, EOP_SCALE = 3 };
+
+// One kernel for the four element operations: PACK buf[item] <- e (block fb), ADD
+// e (block cb) += buf[fb], ZERO e (block cb) = 0, SCALE e (block cb) *= factor.
+template <int OP>
+__global__ void __launch_bounds__(RBS)
+k_ref_emf_op(RG r, int kind, double *__restrict__ e1, double *__restrict__ e2, double *__restrict__ e3,
+             double *__restrict__ buf, int64_t stride, const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const EBoxes q = emf_boxes(r, kind, it);
+  const int b = OP == EOP_PACK ? it.fb : it.cb;
+  double *bb = OP == EOP_PACK ? buf + (int64_t)blockIdx.y * stride
+                              : (OP == EOP_ADD ? buf + (int64_t)it.fb * stride : nullptr);
+  for (unsigned t = first_t(); t < q.cnt; t += stride_t()) {
+    const bool second = t >= q.cnt0;
+    const EBox &x = second ? q.b[1] : q.b[0];
+    unsigned s = second ? t - q.cnt0 : t;
+    const int i = x.i0 + (int)(s % (unsigned)x.ni);
+    s /= (unsigned)x.ni;
+    const int j = x.j0 + (int)(s % (unsigned)x.nj), k = x.k0 + (int)(s / (unsigned)x.nj);
+    double *e;
+    int64_t off;
+    if (x.arr == 0) { e = e1; off = ix(r.n3 + 1, r.n2 + 1, r.n1, 1, b, 0, k, j, i); }
+    else if (x.arr == 1) { e = e2; off = ix(r.n3 + 1, r.n2, r.n1 + 1, 1, b, 0, k, j, i); }
+    else { e = e3; off = ix(r.n3, r.n2 + 1, r.n1 + 1, 1, b, 0, k, j, i); }
+    if (OP == EOP_PACK) bb[t] = e[off];
+    else if (OP == EOP_ADD) e[off] += bb[t];
+    else if (OP == EOP_ZERO) e[off] = 0.0;
+    else e[off] *= q.scale;
+  }
+}
+
 // ---------------------------------------------------------------- host side
 bool in(int lo, int hi, int mn, int mx) { return lo <= hi && lo >= mn && hi <= mx; }
 
@@ -485,8 +654,9 @@ int check_item(const RG &r, int kind, const agx_ref_item &it, int nbf, int nbc, 
     return -40;
   };
   const bool uses_cb = kind != AGX_REF_PROLONG_FC_INT && kind != AGX_REF_EMF_TO_COARSER &&
-                       kind != AGX_REF_PRIM_TO_CONS;
-  if (it.fb < 0 || it.fb >= nbf) return bad("fine-side block index out of range");
+                       kind != AGX_REF_PRIM_TO_CONS && kind != AGX_REF_EMF_SAME_LEVEL;
+  const bool uses_fb = kind != AGX_REF_EMF_CLEAR && kind != AGX_REF_EMF_AVERAGE;
+  if (uses_fb && (it.fb < 0 || it.fb >= nbf)) return bad("fine-side block (or buffer slot) index out of range");
   if (uses_cb && (it.cb < 0 || it.cb >= nbc)) return bad("coarse-side block index out of range");
   const int cs[3] = {r.cis, r.cjs, r.cks}, fs[3] = {r.is, r.js, r.ks};
   const int cn[3] = {r.c1, r.c2, r.c3}, fn[3] = {r.n1, r.n2, r.n3};
@@ -544,12 +714,39 @@ int check_item(const RG &r, int kind, const agx_ref_item &it, int nbf, int nbc, 
       if (it.fid < 0 || it.fid > 5) return bad("fid must be 0..5");
       if ((it.fi1 & ~1) || (it.fi2 & ~1)) return bad("fi1 and fi2 must be 0 or 1");
       return 0;
-    case AGX_REF_EMF_TO_COARSER:
-      if (it.fid < 0 || it.fid > 5) return bad("fid must be 0..5");
+    case AGX_REF_EMF_TO_COARSER: case AGX_REF_EMF_SAME_LEVEL: case AGX_REF_EMF_SET_SAME:
+    case AGX_REF_EMF_CLEAR:
+      if (it.fid < 0 || it.fid > 17) return bad("fid must be a face 0..5 or 6 + an edge 0..11");
+      return 0;
+    case AGX_REF_EMF_SET_FINER:
+      if (it.fid < 0 || it.fid > 17) return bad("fid must be a face 0..5 or 6 + an edge 0..11");
+      if ((it.fi1 & ~1) || (it.fi2 & ~1) || (it.fid >= 6 && it.fi2 != 0))
+        return bad("fi1 and fi2 must be 0 or 1 (fi2 0 on an edge)");
+      return 0;
+    case AGX_REF_EMF_AVERAGE:
+      if (it.fid < 0 || it.fid > 17) return bad("fid must be a face 0..5 or 6 + an edge 0..11");
+      if (it.fid < 6 ? (it.fi1 & ~1) != 0 : (it.fi1 < 2 || it.fi1 > 4))
+        return bad("fi1 must be 0 or 1 on a face and the block count 2..4 on an edge");
       return 0;
     default:
       return bad("unknown operator kind");
   }
+}
+
+bool is_emf_box_kind(int kind) {
+  return kind == AGX_REF_EMF_SAME_LEVEL || kind == AGX_REF_EMF_SET_SAME || kind == AGX_REF_EMF_SET_FINER ||
+         kind == AGX_REF_EMF_CLEAR || kind == AGX_REF_EMF_AVERAGE;
+}
+
+// Launch order of an EMF_SET item: the reference adds neighbours in the order x1 faces,
+// x2 faces, x3 faces, edges (bvals_base.cpp:299-700), and an element on a face's edge
+// takes one value from each of these classes, so the classes run one after another.
+// Finer quadrants of one face meet on two lines, which two items of the class write;
+// each quadrant (or edge half) gets its own launch, so no launch writes an element twice.
+int emf_phase(int kind, const agx_ref_item &it) {
+  const int cls = it.fid < 6 ? it.fid >> 1 : 3;
+  if (kind != AGX_REF_EMF_SET_FINER) return cls * 4;
+  return cls * 4 + (it.fid < 6 ? it.fi1 + 2 * it.fi2 : it.fi1);
 }
 
 int64_t item_count(const RG &r, int kind, const agx_ref_item &it) {
@@ -557,7 +754,8 @@ int64_t item_count(const RG &r, int kind, const agx_ref_item &it) {
     const int dir = it.fid >> 1;
     return (int64_t)(dir == 2 ? r.h2 : r.h3) * (dir == 0 ? r.h2 : r.h1);
   }
-  if (kind == AGX_REF_EMF_TO_COARSER) return emf_part(r, it.fid >> 1, 0) + emf_part(r, it.fid >> 1, 1);
+  if (kind == AGX_REF_EMF_TO_COARSER) return emf_coarser_count(r, it.fid);
+  if (is_emf_box_kind(kind)) return emf_boxes(r, kind, it).cnt;
   if (kind == AGX_REF_PRIM_TO_CONS) return 8 * (int64_t)(it.ei - it.si + 1) * (it.ej - it.sj + 1) * (it.ek - it.sk + 1);
   return (int64_t)(it.ei - it.si + 1) * (it.ej - it.sj + 1) * (it.ek - it.sk + 1);
 }
@@ -607,20 +805,55 @@ int agx_ref_list_create(const agx_ref_geom *g, int32_t kind, const agx_ref_item 
     if (rc) return rc;
     const int64_t c = item_count(r, kind, items[m]);
     maxc = std::max(maxc, c);
-    if (kind == AGX_REF_EMF_TO_COARSER) emf_max = std::max(emf_max, c);
+    if (kind == AGX_REF_EMF_TO_COARSER || is_emf_box_kind(kind)) emf_max = std::max(emf_max, c);
   }
+  // EMF steps that write into blocks: one item per (block, face or edge[, quadrant]), so
+  // that the items of one launch write disjoint elements
+  std::vector<agx_ref_item> sorted(items, items + nitems);
+  int32_t phase_off[17] = {0};
+  int nphase = 1;
+  if (kind == AGX_REF_EMF_SET_SAME || kind == AGX_REF_EMF_SET_FINER || kind == AGX_REF_EMF_CLEAR ||
+      kind == AGX_REF_EMF_AVERAGE) {
+    std::vector<int64_t> keys(nitems);
+    for (int m = 0; m < nitems; ++m) {
+      const agx_ref_item &it = items[m];
+      keys[m] = (((int64_t)it.cb * 18 + it.fid) * 2 + (kind == AGX_REF_EMF_SET_FINER ? it.fi1 : 0)) * 2 +
+                (kind == AGX_REF_EMF_SET_FINER ? it.fi2 : 0);
+    }
+    std::vector<int64_t> k2 = keys;
+    std::sort(k2.begin(), k2.end());
+    if (std::adjacent_find(k2.begin(), k2.end()) != k2.end()) {
+      set_error("refinement work list: two items for the same block, face or edge (and quadrant)");
+      return -40;
+    }
+    if (kind == AGX_REF_EMF_SET_SAME || kind == AGX_REF_EMF_SET_FINER) {
+      nphase = 16;
+      std::stable_sort(sorted.begin(), sorted.end(), [&](const agx_ref_item &a, const agx_ref_item &b) {
+        return emf_phase(kind, a) < emf_phase(kind, b);
+      });
+      int m = 0;
+      for (int p = 0; p < nphase; ++p) {
+        phase_off[p] = m;
+        while (m < nitems && emf_phase(kind, sorted[m]) == p) ++m;
+      }
+      phase_off[nphase] = nitems;
+    }
+  }
+  if (nphase == 1) phase_off[1] = nitems;
   agx_ref_list *l = new agx_ref_list;
   l->g = *g;
   l->kind = kind;
   l->nitems = nitems;
   l->emf_max = emf_max;
   l->max_count = maxc;
+  l->nphase = nphase;
+  std::copy(phase_off, phase_off + 17, l->phase_off);
   const int nvar_hint = (kind == AGX_REF_RESTRICT_CC || kind == AGX_REF_PROLONG_CC ||
                          kind == AGX_REF_FLUX_FROM_FINER) ? 5 : 1;
   l->G = (int)std::min<int64_t>(64, std::max<int64_t>(1, (maxc * nvar_hint + RBS - 1) / RBS));
   if (nitems > 0) {
     hipError_t e = hipMalloc(&l->items, sizeof(agx_ref_item) * nitems);
-    if (e == hipSuccess) e = hipMemcpy(l->items, items, sizeof(agx_ref_item) * nitems, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(l->items, sorted.data(), sizeof(agx_ref_item) * nitems, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       if (l->items) hipFree(l->items);
       delete l;
@@ -639,9 +872,39 @@ int agx_ref_list_destroy(agx_ref_list *l) {
 }
 
 int64_t agx_ref_emf_count(const agx_ref_geom *g, int32_t fid) {
-  if (!g || fid < 0 || fid > 5) return -1;
-  const RG r = make_rg(*g);
-  return emf_part(r, fid >> 1, 0) + emf_part(r, fid >> 1, 1);
+  if (!g || fid < 0 || fid > 17) return -1;
+  return emf_coarser_count(make_rg(*g), fid);
+}
+
+int64_t agx_ref_emf_count_same_level(const agx_ref_geom *g, int32_t fid) {
+  if (!g || fid < 0 || fid > 17) return -1;
+  agx_ref_item it{};
+  it.fid = fid;
+  return emf_boxes(make_rg(*g), AGX_REF_EMF_SAME_LEVEL, it).cnt;
+}
+
+int agx_ref_count_fine_edges(const int32_t nblevel[27], int32_t edge_flag[12], int32_t nedge_fine[12]) {
+  if (!nblevel || !edge_flag || !nedge_fine) { set_error("agx_ref_count_fine_edges: null argument"); return -1; }
+  if (nblevel[13] != 0) { set_error("agx_ref_count_fine_edges: nblevel[13] (the block itself) must be 0"); return -51; }
+  auto lv = [&](int o3, int o2, int o1) { return nblevel[((o3 + 1) * 3 + o2 + 1) * 3 + o1 + 1]; };
+  int eid = 0;
+  // x1x2 edges (ox2 outer, ox1 inner), x1x3 (ox3, ox1), x2x3 (ox3, ox2): the four
+  // positions around each edge; a finer one restarts the count at its level
+  for (int plane = 0; plane < 3; ++plane)
+    for (int a = -1; a <= 1; a += 2)
+      for (int c = -1; c <= 1; c += 2) {
+        int nf = 0, fl = 0;
+        for (int u = std::max(a - 1, -1); u <= std::min(a + 1, 1); ++u)
+          for (int v = std::max(c - 1, -1); v <= std::min(c + 1, 1); ++v) {
+            const int l = plane == 0 ? lv(0, u, v) : (plane == 1 ? lv(u, 0, v) : lv(u, v, 0));
+            if (l > 1) { set_error("agx_ref_count_fine_edges: levels differ by more than one"); return -51; }
+            if (l > fl) { ++fl; nf = 0; }
+            if (l == fl) ++nf;
+          }
+        edge_flag[eid] = fl == 0;
+        nedge_fine[eid++] = nf;
+      }
+  return 0;
 }
 
 int agx_ref_restrict_cc(const agx_ref_list *l, int32_t nvar, const double *fine, agx_ref_coords fc,
@@ -787,6 +1050,56 @@ int agx_ref_prolong_ranges(const agx_ref_geom *g, const int32_t ox[3], const int
   out[0] = lo[0]; out[1] = hi[0]; out[2] = lo[1]; out[3] = hi[1]; out[4] = lo[2]; out[5] = hi[2];
   out[6] = fl[0]; out[7] = fh[0]; out[8] = fl[1]; out[9] = fh[1]; out[10] = fl[2]; out[11] = fh[2];
   return 0;
+}
+
+int agx_ref_emf_same_level(const agx_ref_list *l, const double *e1, const double *e2, const double *e3, double *buf,
+                           int64_t stride, void *stream) {
+  if (int rc = check_call(l, AGX_REF_EMF_SAME_LEVEL, AGX_REF_EMF_SAME_LEVEL, "agx_ref_emf_same_level")) return rc;
+  if (stride < l->emf_max) {
+    set_error("agx_ref_emf_same_level: stride %lld below the largest item (%lld values)", (long long)stride,
+              (long long)l->emf_max);
+    return -48;
+  }
+  if (l->nitems == 0) return 0;
+  hipLaunchKernelGGL(k_ref_emf_op<EOP_PACK>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                     make_rg(l->g), l->kind, const_cast<double *>(e1), const_cast<double *>(e2),
+                     const_cast<double *>(e3), buf, stride, l->items);
+  return launch_status("agx_ref_emf_same_level");
+}
+
+int agx_ref_emf_set(const agx_ref_list *l, double *e1, double *e2, double *e3, const double *buf, int64_t stride,
+                    void *stream) {
+  if (int rc = check_call(l, AGX_REF_EMF_SET_SAME, AGX_REF_EMF_SET_FINER, "agx_ref_emf_set")) return rc;
+  if (stride < l->emf_max) {
+    set_error("agx_ref_emf_set: stride %lld below the largest item (%lld values)", (long long)stride,
+              (long long)l->emf_max);
+    return -48;
+  }
+  const RG r = make_rg(l->g);
+  for (int p = 0; p < l->nphase; ++p) {
+    const int a = l->phase_off[p], n = l->phase_off[p + 1] - a;
+    if (n == 0) continue;
+    hipLaunchKernelGGL(k_ref_emf_op<EOP_ADD>, dim3(l->G, n), dim3(RBS), 0, (hipStream_t)stream, r, l->kind, e1, e2,
+                       e3, const_cast<double *>(buf), stride, l->items + a);
+    if (int rc = launch_status("agx_ref_emf_set")) return rc;
+  }
+  return 0;
+}
+
+int agx_ref_emf_clear(const agx_ref_list *l, double *e1, double *e2, double *e3, void *stream) {
+  if (int rc = check_call(l, AGX_REF_EMF_CLEAR, AGX_REF_EMF_CLEAR, "agx_ref_emf_clear")) return rc;
+  if (l->nitems == 0) return 0;
+  hipLaunchKernelGGL(k_ref_emf_op<EOP_ZERO>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                     make_rg(l->g), l->kind, e1, e2, e3, nullptr, (int64_t)0, l->items);
+  return launch_status("agx_ref_emf_clear");
+}
+
+int agx_ref_emf_average(const agx_ref_list *l, double *e1, double *e2, double *e3, void *stream) {
+  if (int rc = check_call(l, AGX_REF_EMF_AVERAGE, AGX_REF_EMF_AVERAGE, "agx_ref_emf_average")) return rc;
+  if (l->nitems == 0) return 0;
+  hipLaunchKernelGGL(k_ref_emf_op<EOP_SCALE>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                     make_rg(l->g), l->kind, e1, e2, e3, nullptr, (int64_t)0, l->items);
+  return launch_status("agx_ref_emf_average");
 }
 
 int agx_ref_sync(void *stream) {

@@ -1,0 +1,281 @@
+"""Static-refinement block tree, neighbour search and EMF-correction work lists (host).
+
+A mesh of 3D MeshBlocks on several refinement levels is given as its leaves, each a
+LogicalLocation (level, lx1, lx2, lx3) with level 0 = the root grid of nrb blocks.  This
+module finds every block's face and edge neighbours the way the reference does:
+  - MeshBlockTree::FindNeighbor (mesh/meshblock_tree.cpp:362-450): the same-level
+    location at the offset, wrapped on periodic boundaries, is a leaf (same level),
+    lies inside a coarser leaf, or is refined (its children are the neighbours);
+  - BoundaryBase::SearchAndSetNeighbors (bvals/bvals_base.cpp:299-700): the neighbour
+    order (x1, x2, x3 faces, then x1x2, x1x3, x2x3 edges), the finer children of a
+    face (f2 outer, f1 inner) or an edge, the face id fid and edge id eid
+    (NeighborBlock::SetNeighbor, bvals_base.cpp:43-66), the rule that a coarser edge
+    neighbour is kept only by the block at that corner of its parent, and nblevel;
+  - FaceCenteredBoundaryVariable::CountFineEdges (bvals/fc/bvals_fc.cpp:1128-1191).
+It then builds the work lists of the EMF correction of one cycle in the order of
+SendFluxCorrection / ReceiveFluxCorrection (bvals/fc/flux_correction_fc.cpp:623-688,
+1610-1660): pack every same-level and to-coarser buffer from the blocks' own EMFs, then
+per block add the same-level buffers, clear the parts bordering finer blocks, add the
+finer buffers and average.
+
+Plain host logic: no device work.  Corners, polar and shearing boundaries are not
+handled (the EMF correction uses faces and edges only).
+"""
+from dataclasses import dataclass
+
+# AGX_REF_* kinds of the EMF correction (include/athena_gamma_amd.h)
+K_EMF_TO_COARSER, K_EMF_SAME_LEVEL, K_EMF_SET_SAME, K_EMF_SET_FINER, K_EMF_CLEAR, K_EMF_AVERAGE = \
+    10, 12, 13, 14, 15, 16
+
+
+@dataclass
+class Neighbor:
+    gid: int
+    rel: int          # neighbour level minus this block's level: -1, 0, +1
+    kind: str         # "face" or "edge"
+    ox: tuple         # (ox1, ox2, ox3)
+    fid: int = -1     # BoundaryFace for faces
+    eid: int = -1     # edge id for edges
+    fi1: int = 0
+    fi2: int = 0
+
+    @property
+    def code(self):
+        """fid of a work item: the face 0..5 or 6 + the edge id."""
+        return self.fid if self.kind == "face" else 6 + self.eid
+
+
+def face_id(ox):
+    for d in range(3):
+        if ox[d]:
+            return 2 * d + (ox[d] > 0)
+    raise ValueError(ox)
+
+
+def edge_id(ox):
+    o1, o2, o3 = ox
+    if o3 == 0:
+        return ((o1 + 1) >> 1) | ((o2 + 1) & 2)
+    if o2 == 0:
+        return 4 + (((o1 + 1) >> 1) | ((o3 + 1) & 2))
+    return 8 + (((o2 + 1) >> 1) | ((o3 + 1) & 2))
+
+
+class SMRTree:
+    def __init__(self, nrb, leaves, periodic=(True, True, True)):
+        self.nrb = tuple(int(x) for x in nrb)
+        self.leaves = [tuple(int(v) for v in loc) for loc in leaves]
+        self.periodic = tuple(bool(p) for p in periodic)
+        self.gid = {loc: g for g, loc in enumerate(self.leaves)}
+        if len(self.gid) != len(self.leaves):
+            raise ValueError("duplicate leaves")
+        self.maxlev = max(loc[0] for loc in self.leaves)
+        vol = sum(8 ** (self.maxlev - loc[0]) for loc in self.leaves)
+        if vol != self.nrb[0] * self.nrb[1] * self.nrb[2] * 8 ** self.maxlev:
+            raise ValueError("leaves do not tile the root grid")
+        for loc in self.leaves:
+            if self._covering(loc) != loc:
+                raise ValueError(f"leaf {loc} overlaps another leaf")
+        self._nb = [self._search(g) for g in range(len(self.leaves))]
+        for g in range(len(self.leaves)):
+            for lev in self.nblevel(g):
+                if lev > 1:
+                    raise ValueError("neighbouring blocks differ by more than one level")
+
+    # ---- tree queries
+    def _covering(self, loc):
+        """the leaf containing the level-`loc[0]` location, at its level or coarser"""
+        lev, x, y, z = loc
+        while lev >= 0:
+            if (lev, x, y, z) in self.gid:
+                return (lev, x, y, z)
+            lev, x, y, z = lev - 1, x >> 1, y >> 1, z >> 1
+        return None
+
+    def _wrap(self, loc, ox):
+        lev = loc[0]
+        pos = []
+        for d in range(3):
+            n = self.nrb[d] << lev
+            p = loc[1 + d] + ox[d]
+            if p < 0 or p >= n:
+                if not self.periodic[d]:
+                    return None
+                p %= n
+            pos.append(p)
+        return (lev, *pos)
+
+    def find(self, g, ox):
+        """FindNeighbor: ('same'|'coarser', gid) or ('finer', location at this level) or None"""
+        loc = self.leaves[g]
+        t = self._wrap(loc, ox)
+        if t is None:
+            return None
+        if t in self.gid:
+            return ("same", self.gid[t])
+        c = self._covering(t)
+        if c is not None:
+            return ("coarser", self.gid[c])
+        return ("finer", t)
+
+    def _child(self, t, c1, c2, c3):
+        lev, x, y, z = t
+        return self.gid[(lev + 1, 2 * x + c1, 2 * y + c2, 2 * z + c3)]
+
+    # ---- SearchAndSetNeighbors (faces and edges)
+    def _search(self, g):
+        loc = self.leaves[g]
+        myox = [((loc[1 + d] & 1) * 2 - 1) for d in range(3)]
+        out = []
+        for d in range(3):                       # faces
+            for n in (-1, 1):
+                ox = [0, 0, 0]
+                ox[d] = n
+                r = self.find(g, ox)
+                if r is None:
+                    continue
+                fid = face_id(ox)
+                if r[0] == "finer":
+                    ff = 1 - (n + 1) // 2
+                    for f2 in range(2):
+                        for f1 in range(2):   # GetLeaf: f1, f2 along the face's first and second tangent
+                            if d == 0:
+                                idx = (ff, f1, f2)
+                            elif d == 1:
+                                idx = (f1, ff, f2)
+                            else:
+                                idx = (f1, f2, ff)
+                            out.append(Neighbor(self._child(r[1], *idx), 1, "face", tuple(ox), fid=fid, fi1=f1, fi2=f2))
+                else:
+                    out.append(Neighbor(r[1], 0 if r[0] == "same" else -1, "face", tuple(ox), fid=fid))
+        for da, db in ((0, 1), (0, 2), (1, 2)):   # edges: x1x2, x1x3, x2x3
+            for m in (-1, 1):
+                for n in (-1, 1):
+                    ox = [0, 0, 0]
+                    ox[da], ox[db] = n, m
+                    r = self.find(g, ox)
+                    if r is None:
+                        continue
+                    eid = edge_id(ox)
+                    if r[0] == "finer":
+                        ffa, ffb = 1 - (n + 1) // 2, 1 - (m + 1) // 2
+                        free = 3 - da - db
+                        for f1 in range(2):
+                            idx = [0, 0, 0]
+                            idx[da], idx[db], idx[free] = ffa, ffb, f1
+                            out.append(Neighbor(self._child(r[1], *idx), 1, "edge", tuple(ox), eid=eid, fi1=f1))
+                    else:
+                        rel = 0 if r[0] == "same" else -1
+                        if rel == 0 or (myox[da] == n and myox[db] == m):
+                            out.append(Neighbor(r[1], rel, "edge", tuple(ox), eid=eid))
+        return out
+
+    def neighbors(self, g):
+        return self._nb[g]
+
+    def nblevel(self, g):
+        """27 levels relative to block g at ((ox3+1)*3 + ox2+1)*3 + ox1+1; -1 where the
+        neighbour is coarser or missing (as the reference's -1 for a missing one)."""
+        lv = [-1] * 27
+        lv[13] = 0
+        for o3 in (-1, 0, 1):
+            for o2 in (-1, 0, 1):
+                for o1 in (-1, 0, 1):
+                    if (o1, o2, o3) == (0, 0, 0):
+                        continue
+                    r = self.find(g, (o1, o2, o3))
+                    if r is None:
+                        continue
+                    lv[((o3 + 1) * 3 + o2 + 1) * 3 + o1 + 1] = {"same": 0, "coarser": -1, "finer": 1}[r[0]]
+        return lv
+
+    def boundary_is_block(self, g, fid):
+        """the face has a block or periodic boundary (AverageFluxBoundary's test)"""
+        ox = [0, 0, 0]
+        ox[fid >> 1] = 1 if fid & 1 else -1
+        return self._wrap(self.leaves[g], ox) is not None
+
+
+def count_fine_edges(nblevel):
+    """CountFineEdges on 27 relative levels: (edge_flag[12], nedge_fine[12])."""
+    def lv(o3, o2, o1):
+        return nblevel[((o3 + 1) * 3 + o2 + 1) * 3 + o1 + 1]
+    flags, counts = [], []
+    for plane in range(3):
+        for os_ in (-1, 1):
+            for of in (-1, 1):
+                nf, fl = 0, 0
+                for a in range(max(os_ - 1, -1), min(os_ + 1, 1) + 1):
+                    for c in range(max(of - 1, -1), min(of + 1, 1) + 1):
+                        lev = lv(0, a, c) if plane == 0 else (lv(a, 0, c) if plane == 1 else lv(a, c, 0))
+                        if lev > fl:
+                            fl, nf = fl + 1, 0
+                        if lev == fl:
+                            nf += 1
+                flags.append(fl == 0)
+                counts.append(nf)
+    return flags, counts
+
+
+def _item(fb=0, cb=0, fid=0, fi1=0, fi2=0):
+    return dict(fb=fb, cb=cb, si=0, ei=0, sj=0, ej=0, sk=0, ek=0, fid=fid, fi1=fi1, fi2=fi2, pad=0)
+
+
+@dataclass
+class EMFPlan:
+    """Work lists (dicts of agx_ref_item fields) of one EMF correction, in the reference's
+    order.  to_coarser and same_level pack slots 0.. of their own buffers; set_same reads
+    slots of the same-level buffer, set_finer slots of the to-coarser buffer."""
+    to_coarser: list
+    same_level: list
+    set_same: list
+    clear: list
+    set_finer: list
+    average: list
+
+
+def emf_plan(tree):
+    nblk = len(tree.leaves)
+    flags = [count_fine_edges(tree.nblevel(g)) for g in range(nblk)]
+    to_coarser, same_level = [], []
+    slot_same, slot_fine = {}, {}
+    for g in range(nblk):                        # SendFluxCorrection, every block
+        for nb in tree.neighbors(g):
+            if nb.rel == 0 and (nb.kind == "face" or flags[g][0][nb.eid]):
+                slot_same[(nb.gid, tuple(-o for o in nb.ox), g)] = len(same_level)
+                same_level.append(_item(fb=g, fid=nb.code))
+            elif nb.rel == -1:
+                slot_fine[(nb.gid, tuple(-o for o in nb.ox), g)] = len(to_coarser)
+                to_coarser.append(_item(fb=g, fid=nb.code))
+    set_same, clear, set_finer, average = [], [], [], []
+    for g in range(nblk):                        # ReceiveFluxCorrection, every block
+        eflag, nfine = flags[g]
+        lv = tree.nblevel(g)
+        for nb in tree.neighbors(g):
+            if nb.rel == 0 and (nb.kind == "face" or eflag[nb.eid]):
+                set_same.append(_item(fb=slot_same[(g, nb.ox, nb.gid)], cb=g, fid=nb.code))
+        for fid in range(6):                     # ClearCoarseFluxBoundary
+            ox = [0, 0, 0]
+            ox[fid >> 1] = 1 if fid & 1 else -1
+            if lv[((ox[2] + 1) * 3 + ox[1] + 1) * 3 + ox[0] + 1] > 0:
+                clear.append(_item(cb=g, fid=fid))
+        for e in range(12):
+            if not eflag[e]:
+                clear.append(_item(cb=g, fid=6 + e))
+        for nb in tree.neighbors(g):
+            if nb.rel == 1:
+                set_finer.append(_item(fb=slot_fine[(g, nb.ox, nb.gid)], cb=g, fid=nb.code, fi1=nb.fi1, fi2=nb.fi2))
+        for fid in range(6):                     # AverageFluxBoundary
+            if not tree.boundary_is_block(g, fid):
+                continue
+            ox = [0, 0, 0]
+            ox[fid >> 1] = 1 if fid & 1 else -1
+            rel = lv[((ox[2] + 1) * 3 + ox[1] + 1) * 3 + ox[0] + 1]
+            if rel == 0:
+                average.append(_item(cb=g, fid=fid, fi1=0))
+            elif rel > 0:
+                average.append(_item(cb=g, fid=fid, fi1=1))
+        for e in range(12):
+            if nfine[e] != 1:
+                average.append(_item(cb=g, fid=6 + e, fi1=nfine[e]))
+    return EMFPlan(to_coarser, same_level, set_same, clear, set_finer, average)

@@ -261,13 +261,27 @@ enum {
   AGX_REF_FLUX_FROM_FINER = 9, /* flux_correction_cc.cpp:77-169 (LoadFluxBoundaryBufferToCoarser)
                                   + 240-283 (SetFluxBoundaryFromFiner): fb = finer block,
                                   cb = coarser block, fid = face of cb, fi1/fi2 = quadrant */
-  AGX_REF_EMF_TO_COARSER = 10, /* flux_correction_fc.cpp:315-418 (LoadFluxBoundaryBufferToCoarser,
-                                  3D face neighbours): fb = finer block, fid = its face */
+  AGX_REF_EMF_TO_COARSER = 10, /* flux_correction_fc.cpp:315-546 (LoadFluxBoundaryBufferToCoarser,
+                                  3D face and edge neighbours): fb = finer block, fid = its face
+                                  0..5 or 6 + its edge (NeighborBlock::eid) */
   AGX_REF_PRIM_TO_CONS = 11,   /* Field::CalculateCellCenteredField (uniform branch, field.cpp:112-176)
                                   + EquationOfState::PrimitiveToConserved (adiabatic_mhd.cpp:99-136,
                                   isothermal_mhd.cpp:82-108) on the fine cells of each coarse range,
                                   the last step of ProlongateGhostCells (bvals_refine.cpp) */
-  AGX_REF_KINDS = 12
+  AGX_REF_EMF_SAME_LEVEL = 12, /* LoadFluxBoundaryBufferSameLevel, flux_correction_fc.cpp:51-313:
+                                  fb = block, fid = its face or 6 + edge */
+  AGX_REF_EMF_SET_SAME = 13,   /* SetFluxBoundarySameLevel, flux_correction_fc.cpp:689-940 */
+  AGX_REF_EMF_SET_FINER = 14,  /* SetFluxBoundaryFromFiner, flux_correction_fc.cpp:944-1185:
+                                  cb = receiving block, fb = buffer slot, fid = cb's face or
+                                  6 + edge, fi1/fi2 = the finer block's quadrant (fi1 = half
+                                  on an edge) */
+  AGX_REF_EMF_CLEAR = 15,      /* ClearCoarseFluxBoundary, flux_correction_fc.cpp:1214-1348: cb,
+                                  fid = a face with finer neighbours or an edge with a finer
+                                  block around it */
+  AGX_REF_EMF_AVERAGE = 16,    /* AverageFluxBoundary, flux_correction_fc.cpp:1355-1543: cb, fid =
+                                  a face with a block or periodic boundary and fi1 = 0 (same
+                                  level) or 1 (finer), or an edge and fi1 = nedge_fine (2..4) */
+  AGX_REF_KINDS = 17
 };
 typedef struct agx_ref_list agx_ref_list;
 /* Checks every item against the geometry and the pool sizes (nb_fine blocks on the fine
@@ -277,8 +291,10 @@ int agx_ref_list_create(const agx_ref_geom *g, int32_t kind, const agx_ref_item 
                         int32_t nb_fine, int32_t nb_coarse, agx_ref_list **out);
 int agx_ref_list_destroy(agx_ref_list *l);
 /* number of values one item of an AGX_REF_EMF_TO_COARSER list writes (the buffer size
- * of the reference) for face fid */
+ * of the reference) for face fid 0..5 or edge 6 + eid */
 int64_t agx_ref_emf_count(const agx_ref_geom *g, int32_t fid);
+/* the same for an AGX_REF_EMF_SAME_LEVEL item (BoundaryBase's same-level EMF buffer) */
+int64_t agx_ref_emf_count_same_level(const agx_ref_geom *g, int32_t fid);
 /* fine cells -> coarse buffer (kind RESTRICT_CC) */
 int agx_ref_restrict_cc(const agx_ref_list *l, int32_t nvar, const double *fine, agx_ref_coords fine_coords,
                         double *coarse, void *stream);
@@ -298,9 +314,35 @@ int agx_ref_prolong_fc_internal(const agx_ref_list *l, double *b1f, double *b2f,
 int agx_ref_flux_from_finer(const agx_ref_list *l, int32_t nvar, const double *fine_x1flux,
                             const double *fine_x2flux, const double *fine_x3flux, agx_ref_coords fine_coords,
                             double *coarse_x1flux, double *coarse_x2flux, double *coarse_x3flux, void *stream);
-/* restricted edge EMFs of each item's face into buf + item * stride (EMF_TO_COARSER) */
+/* restricted edge EMFs of each item's face or edge into buf + item * stride (EMF_TO_COARSER) */
 int agx_ref_emf_to_coarser(const agx_ref_list *l, const double *e1, const double *e2, const double *e3,
                            agx_ref_coords fine_coords, double *buf, int64_t stride, void *stream);
+/* EMF correction across blocks, one cycle (FaceCenteredBoundaryVariable::SendFluxCorrection
+ * and ReceiveFluxCorrection, flux_correction_fc.cpp:623-688, 1610-1660).  Edge arrays are
+ * x1e (nb, n3+1, n2+1, n1), x2e (nb, n3+1, n2, n1+1), x3e (nb, n3, n2+1, n1+1).  Order of
+ * one correction: pack every buffer (agx_ref_emf_same_level, agx_ref_emf_to_coarser) from
+ * the blocks' own EMFs; then agx_ref_emf_set with the SET_SAME list, agx_ref_emf_clear,
+ * agx_ref_emf_set with the SET_FINER list, agx_ref_emf_average.  A block's items follow
+ * its neighbour order; the library launches the x1-face, x2-face, x3-face and edge items
+ * one class after another (and finer quadrants one after another), which gives every
+ * element the reference's summation order.  One item per (cb, fid[, fi1, fi2]) in a
+ * list (checked). */
+/* each item's own EMFs on its face or edge into buf + item * stride (EMF_SAME_LEVEL) */
+int agx_ref_emf_same_level(const agx_ref_list *l, const double *e1, const double *e2, const double *e3,
+                           double *buf, int64_t stride, void *stream);
+/* e += buf + item.fb * stride on each item's face, quadrant or edge (EMF_SET_SAME, EMF_SET_FINER) */
+int agx_ref_emf_set(const agx_ref_list *l, double *e1, double *e2, double *e3, const double *buf,
+                    int64_t stride, void *stream);
+/* zero the EMFs that finer neighbours will supply (EMF_CLEAR) */
+int agx_ref_emf_clear(const agx_ref_list *l, double *e1, double *e2, double *e3, void *stream);
+/* divide the summed EMFs by the number of contributions (EMF_AVERAGE) */
+int agx_ref_emf_average(const agx_ref_list *l, double *e1, double *e2, double *e3, void *stream);
+/* FaceCenteredBoundaryVariable::CountFineEdges (bvals/fc/bvals_fc.cpp:1128-1191): from the
+ * 27 neighbour levels relative to the block (index as for agx_ref_prolong_ranges,
+ * nblevel[13] = 0, negative = coarser or missing), edge_flag[e] = 1 when no finer block
+ * touches edge e and nedge_fine[e] = the number of finest-level blocks around it.  Host
+ * only. */
+int agx_ref_count_fine_edges(const int32_t nblevel[27], int32_t edge_flag[12], int32_t nedge_fine[12]);
 /* fine cells of the item ranges: bcc from the faces, then the conserved variables from
  * w and bcc (nvar 5 with the adiabatic EOS, gamma; 4 with iso != 0) (PRIM_TO_CONS) */
 int agx_ref_prim_to_cons(const agx_ref_list *l, const double *w, const double *b1f, const double *b2f,

@@ -134,6 +134,16 @@ void orc_ref_flux_from_finer(const orc_ref_geom *g, int nvar, const double *ff1,
 void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const double *e2,
                             const double *e3, orc_ref_coords fc, double *buf, int64_t stride,
                             const orc_ref_item *it, int nit);
+void orc_ref_emf_same_level(const orc_ref_geom *g, const double *e1, const double *e2,
+                            const double *e3, double *buf, int64_t stride, const orc_ref_item *it,
+                            int nit);
+void orc_ref_emf_set(const orc_ref_geom *g, double *e1, double *e2, double *e3, const double *buf,
+                     int64_t stride, int finer, const orc_ref_item *it, int nit);
+void orc_ref_emf_clear(const orc_ref_geom *g, double *e1, double *e2, double *e3,
+                       const orc_ref_item *it, int nit);
+void orc_ref_emf_average(const orc_ref_geom *g, double *e1, double *e2, double *e3,
+                         const orc_ref_item *it, int nit);
+void orc_ref_count_fine_edges(const int *nblevel, int *edge_flag, int *nedge_fine);
 void orc_ref_prim_to_cons(const orc_ref_geom *g, const double *w, const double *b1, const double *b2,
                           const double *b3, double *bcc, double *u, double gamma, int iso,
                           const orc_ref_item *it, int nit);

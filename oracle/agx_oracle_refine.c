@@ -16,8 +16,15 @@
  *                              + SetFluxBoundaryFromFiner (bvals/cc/flux_correction_cc.cpp:
  *                              77-169, 230-283), the buffer replaced by a direct write
  *   orc_ref_emf_to_coarser     FaceCenteredBoundaryVariable::LoadFluxBoundaryBufferToCoarser,
- *                              face neighbours of a 3D block (bvals/fc/flux_correction_fc.cpp:
- *                              315-418), packed in the reference's buffer order
+ *                              face and edge neighbours of a 3D block
+ *                              (bvals/fc/flux_correction_fc.cpp:315-546), packed in the
+ *                              reference's buffer order
+ *   orc_ref_emf_same_level     LoadFluxBoundaryBufferSameLevel (flux_correction_fc.cpp:51-313)
+ *   orc_ref_emf_set            SetFluxBoundarySameLevel / SetFluxBoundaryFromFiner
+ *                              (flux_correction_fc.cpp:689-1185)
+ *   orc_ref_emf_clear          ClearCoarseFluxBoundary (flux_correction_fc.cpp:1214-1348)
+ *   orc_ref_emf_average        AverageFluxBoundary (flux_correction_fc.cpp:1355-1543)
+ *   orc_ref_count_fine_edges   CountFineEdges (bvals/fc/bvals_fc.cpp:1128-1191)
  *   orc_ref_prim_to_cons       Field::CalculateCellCenteredField (uniform branch, field/field.cpp:
  *                              112-176) + EquationOfState::PrimitiveToConserved
  *                              (eos/adiabatic_mhd.cpp:99-136, isothermal_mhd.cpp:82-108)
@@ -404,8 +411,9 @@ void orc_ref_flux_from_finer(const orc_ref_geom *g, int nvar, const double *ff1,
   }
 }
 
-/* Item: fb = the finer block, fid = its face towards the coarser neighbour.  buf + m *
- * stride receives the restricted edge EMFs of that face in the reference's order. */
+/* Item: fb = the finer block, fid = its face (0..5) or 6 + its edge (NeighborBlock::eid)
+ * towards the coarser neighbour.  buf + m * stride receives the restricted edge EMFs of
+ * that face or edge in the reference's order. */
 void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const double *e2,
                             const double *e3, orc_ref_coords fc, double *buf, int64_t stride,
                             const orc_ref_item *it, int nit) {
@@ -439,7 +447,7 @@ void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const doubl
       for (int k = r.ks; k <= r.ke; k += 2)
         for (int i = r.is; i <= r.ie + 1; i += 2)
           out[p++] = (E3(k, j, i) * d3[k] + E3(k + 1, j, i) * d3[k + 1]) / (d3[k] + d3[k + 1]);
-    } else {
+    } else if (dir == 2) {
       int k = side ? r.ke + 1 : r.ks;
       for (int j = r.js; j <= r.je + 1; j += 2)
         for (int i = r.is; i <= r.ie; i += 2)
@@ -447,11 +455,285 @@ void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const doubl
       for (int j = r.js; j <= r.je; j += 2)
         for (int i = r.is; i <= r.ie + 1; i += 2)
           out[p++] = (E2(k, j, i) * d2[j] + E2(k, j + 1, i) * d2[j + 1]) / (d2[j] + d2[j + 1]);
+    } else {   /* edge neighbours, flux_correction_fc.cpp:481-546 */
+      const int eid = fid - 6, lo = eid & 1, hi = (eid >> 1) & 1;
+      if (eid < 4) {          /* x1x2 edge: e3 along k */
+        int i = lo ? r.ie + 1 : r.is, j = hi ? r.je + 1 : r.js;
+        for (int k = r.ks; k <= r.ke; k += 2)
+          out[p++] = (E3(k, j, i) * d3[k] + E3(k + 1, j, i) * d3[k + 1]) / (d3[k] + d3[k + 1]);
+      } else if (eid < 8) {   /* x1x3 edge: e2 along j */
+        int i = lo ? r.ie + 1 : r.is, k = hi ? r.ke + 1 : r.ks;
+        for (int j = r.js; j <= r.je; j += 2)
+          out[p++] = (E2(k, j, i) * d2[j] + E2(k, j + 1, i) * d2[j + 1]) / (d2[j] + d2[j + 1]);
+      } else {                /* x2x3 edge: e1 along i */
+        int j = lo ? r.je + 1 : r.js, k = hi ? r.ke + 1 : r.ks;
+        for (int i = r.is; i <= r.ie; i += 2)
+          out[p++] = (E1(k, j, i) * d1[i] + E1(k, j, i + 1) * d1[i + 1]) / (d1[i] + d1[i + 1]);
+      }
     }
 #undef E1
 #undef E2
 #undef E3
   }
+}
+
+/* ---------------------------------------------------------------- EMF correction
+ * The steps of FaceCenteredBoundaryVariable::SendFluxCorrection / ReceiveFluxCorrection
+ * (bvals/fc/flux_correction_fc.cpp) on 3D Cartesian blocks without shearing or polar
+ * boundaries.  A face is fid 0..5 (BoundaryFace), an edge 6 + eid (NeighborBlock::eid:
+ * 0..3 x1x2, 4..7 x1x3, 8..11 x2x3; bit 0 picks the upper side along the first
+ * direction, bit 1 along the second). */
+#define EE1(b, k, j, i) e1[ix(r.n3 + 1, r.n2 + 1, r.n1, 1, (b), 0, (k), (j), (i))]
+#define EE2(b, k, j, i) e2[ix(r.n3 + 1, r.n2, r.n1 + 1, 1, (b), 0, (k), (j), (i))]
+#define EE3(b, k, j, i) e3[ix(r.n3, r.n2 + 1, r.n1 + 1, 1, (b), 0, (k), (j), (i))]
+
+/* position of the face / edge of a block: fixed index along each normal direction */
+static inline int face_at(const rgeo *r, int dir, int upper) {
+  if (dir == 0) return upper ? r->ie + 1 : r->is;
+  if (dir == 1) return upper ? r->je + 1 : r->js;
+  return upper ? r->ke + 1 : r->ks;
+}
+
+/* LoadFluxBoundaryBufferSameLevel (flux_correction_fc.cpp:51-313), 3D, no shearing box.
+ * Item: fb = the block, fid = its face or edge towards the neighbour; buf + m * stride
+ * receives the block's own edge EMFs there in the reference's order. */
+void orc_ref_emf_same_level(const orc_ref_geom *g, const double *e1, const double *e2,
+                            const double *e3, double *buf, int64_t stride, const orc_ref_item *it,
+                            int nit) {
+  rgeo r = mkgeo(g);
+  for (int m = 0; m < nit; ++m) {
+    const int b = it[m].fb, fid = it[m].fid;
+    double *out = buf + (int64_t)m * stride;
+    int64_t p = 0;
+    if (fid < 6) {
+      const int dir = fid >> 1, x = face_at(&r, dir, fid & 1);
+      if (dir == 0) {
+        for (int k = r.ks; k <= r.ke + 1; k++)
+          for (int j = r.js; j <= r.je; j++) out[p++] = EE2(b, k, j, x);
+        for (int k = r.ks; k <= r.ke; k++)
+          for (int j = r.js; j <= r.je + 1; j++) out[p++] = EE3(b, k, j, x);
+      } else if (dir == 1) {
+        for (int k = r.ks; k <= r.ke + 1; k++)
+          for (int i = r.is; i <= r.ie; i++) out[p++] = EE1(b, k, x, i);
+        for (int k = r.ks; k <= r.ke; k++)
+          for (int i = r.is; i <= r.ie + 1; i++) out[p++] = EE3(b, k, x, i);
+      } else {
+        for (int j = r.js; j <= r.je + 1; j++)
+          for (int i = r.is; i <= r.ie; i++) out[p++] = EE1(b, x, j, i);
+        for (int j = r.js; j <= r.je; j++)
+          for (int i = r.is; i <= r.ie + 1; i++) out[p++] = EE2(b, x, j, i);
+      }
+    } else {
+      const int eid = fid - 6, lo = eid & 1, hi = (eid >> 1) & 1;
+      if (eid < 4) {
+        const int i = face_at(&r, 0, lo), j = face_at(&r, 1, hi);
+        for (int k = r.ks; k <= r.ke; k++) out[p++] = EE3(b, k, j, i);
+      } else if (eid < 8) {
+        const int i = face_at(&r, 0, lo), k = face_at(&r, 2, hi);
+        for (int j = r.js; j <= r.je; j++) out[p++] = EE2(b, k, j, i);
+      } else {
+        const int j = face_at(&r, 1, lo), k = face_at(&r, 2, hi);
+        for (int i = r.is; i <= r.ie; i++) out[p++] = EE1(b, k, j, i);
+      }
+    }
+  }
+}
+
+/* SetFluxBoundarySameLevel (flux_correction_fc.cpp:689-940, finer = 0) and
+ * SetFluxBoundaryFromFiner (:944-1185, finer = 1), 3D, no shearing or polar boundary.
+ * Item: cb = the receiving block, fb = the buffer slot (buf + fb * stride), fid = the
+ * receiving block's face or edge, fi1 / fi2 = the finer neighbour's half along the
+ * first / second tangential direction (faces) or along the edge (fi1).  Items run in
+ * list order, which is the reference's neighbour order. */
+void orc_ref_emf_set(const orc_ref_geom *g, double *e1, double *e2, double *e3, const double *buf,
+                     int64_t stride, int finer, const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  const int nx1 = g->nx1, nx2 = g->nx2, nx3 = g->nx3;
+  for (int m = 0; m < nit; ++m) {
+    const int b = it[m].cb, fid = it[m].fid;
+    const double *in = buf + (int64_t)it[m].fb * stride;
+    int64_t p = 0;
+    int il = r.is, iu = r.ie, jl = r.js, ju = r.je, kl = r.ks, ku = r.ke;
+    if (fid < 6) {
+      const int dir = fid >> 1, x = face_at(&r, dir, fid & 1);
+      if (dir == 0) {
+        if (finer) {
+          if (it[m].fi1 == 0) ju = r.js + nx2 / 2 - 1; else jl = r.js + nx2 / 2;
+          if (it[m].fi2 == 0) ku = r.ks + nx3 / 2 - 1; else kl = r.ks + nx3 / 2;
+        }
+        for (int k = kl; k <= ku + 1; k++)
+          for (int j = jl; j <= ju; j++) EE2(b, k, j, x) += in[p++];
+        for (int k = kl; k <= ku; k++)
+          for (int j = jl; j <= ju + 1; j++) EE3(b, k, j, x) += in[p++];
+      } else if (dir == 1) {
+        if (finer) {
+          if (it[m].fi1 == 0) iu = r.is + nx1 / 2 - 1; else il = r.is + nx1 / 2;
+          if (it[m].fi2 == 0) ku = r.ks + nx3 / 2 - 1; else kl = r.ks + nx3 / 2;
+        }
+        for (int k = kl; k <= ku + 1; k++)
+          for (int i = il; i <= iu; i++) EE1(b, k, x, i) += in[p++];
+        for (int k = kl; k <= ku; k++)
+          for (int i = il; i <= iu + 1; i++) EE3(b, k, x, i) += in[p++];
+      } else {
+        if (finer) {
+          if (it[m].fi1 == 0) iu = r.is + nx1 / 2 - 1; else il = r.is + nx1 / 2;
+          if (it[m].fi2 == 0) ju = r.js + nx2 / 2 - 1; else jl = r.js + nx2 / 2;
+        }
+        for (int j = jl; j <= ju + 1; j++)
+          for (int i = il; i <= iu; i++) EE1(b, x, j, i) += in[p++];
+        for (int j = jl; j <= ju; j++)
+          for (int i = il; i <= iu + 1; i++) EE2(b, x, j, i) += in[p++];
+      }
+    } else {
+      const int eid = fid - 6, lo = eid & 1, hi = (eid >> 1) & 1;
+      if (eid < 4) {
+        const int i = face_at(&r, 0, lo), j = face_at(&r, 1, hi);
+        if (finer) { if (it[m].fi1 == 0) ku = r.ks + nx3 / 2 - 1; else kl = r.ks + nx3 / 2; }
+        for (int k = kl; k <= ku; k++) EE3(b, k, j, i) += in[p++];
+      } else if (eid < 8) {
+        const int i = face_at(&r, 0, lo), k = face_at(&r, 2, hi);
+        if (finer) { if (it[m].fi1 == 0) ju = r.js + nx2 / 2 - 1; else jl = r.js + nx2 / 2; }
+        for (int j = jl; j <= ju; j++) EE2(b, k, j, i) += in[p++];
+      } else {
+        const int j = face_at(&r, 1, lo), k = face_at(&r, 2, hi);
+        if (finer) { if (it[m].fi1 == 0) iu = r.is + nx1 / 2 - 1; else il = r.is + nx1 / 2; }
+        for (int i = il; i <= iu; i++) EE1(b, k, j, i) += in[p++];
+      }
+    }
+  }
+}
+
+/* ClearCoarseFluxBoundary (flux_correction_fc.cpp:1214-1348), 3D.  Item: cb = the block,
+ * fid = a face whose neighbour is finer (its EMFs inside the face, not on the face's
+ * own edges, are zeroed) or an edge with a finer block around it (edge_flag_ false). */
+void orc_ref_emf_clear(const orc_ref_geom *g, double *e1, double *e2, double *e3,
+                       const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  for (int m = 0; m < nit; ++m) {
+    const int b = it[m].cb, fid = it[m].fid;
+    if (fid < 6) {
+      const int dir = fid >> 1, x = face_at(&r, dir, fid & 1);
+      if (dir == 0) {
+        for (int k = r.ks + 1; k <= r.ke; k++)
+          for (int j = r.js; j <= r.je; j++) EE2(b, k, j, x) = 0.0;
+        for (int k = r.ks; k <= r.ke; k++)
+          for (int j = r.js + 1; j <= r.je; j++) EE3(b, k, j, x) = 0.0;
+      } else if (dir == 1) {
+        for (int k = r.ks + 1; k <= r.ke; k++)
+          for (int i = r.is; i <= r.ie; i++) EE1(b, k, x, i) = 0.0;
+        for (int k = r.ks; k <= r.ke; k++)
+          for (int i = r.is + 1; i <= r.ie; i++) EE3(b, k, x, i) = 0.0;
+      } else {
+        for (int j = r.js + 1; j <= r.je; j++)
+          for (int i = r.is; i <= r.ie; i++) EE1(b, x, j, i) = 0.0;
+        for (int j = r.js; j <= r.je; j++)
+          for (int i = r.is + 1; i <= r.ie; i++) EE2(b, x, j, i) = 0.0;
+      }
+    } else {
+      const int eid = fid - 6, lo = eid & 1, hi = (eid >> 1) & 1;
+      if (eid < 4) {
+        const int i = face_at(&r, 0, lo), j = face_at(&r, 1, hi);
+        for (int k = r.ks; k <= r.ke; k++) EE3(b, k, j, i) = 0.0;
+      } else if (eid < 8) {
+        const int i = face_at(&r, 0, lo), k = face_at(&r, 2, hi);
+        for (int j = r.js; j <= r.je; j++) EE2(b, k, j, i) = 0.0;
+      } else {
+        const int j = face_at(&r, 1, lo), k = face_at(&r, 2, hi);
+        for (int i = r.is; i <= r.ie; i++) EE1(b, k, j, i) = 0.0;
+      }
+    }
+  }
+}
+
+/* AverageFluxBoundary (flux_correction_fc.cpp:1355-1543), 3D, no shearing box.  Item: cb =
+ * the block; fid = a face with a block or periodic boundary and fi1 = 0 for a neighbour
+ * on the same level (the EMFs inside the face are halved) or 1 for finer neighbours (the
+ * two lines where the finer blocks meet are halved); or fid = an edge and fi1 = the
+ * number of finest-level blocks around it (nedge_fine_ >= 2; the edge is scaled by
+ * 1/fi1). */
+void orc_ref_emf_average(const orc_ref_geom *g, double *e1, double *e2, double *e3,
+                         const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  const int nx1 = g->nx1, nx2 = g->nx2, nx3 = g->nx3;
+  for (int m = 0; m < nit; ++m) {
+    const int b = it[m].cb, fid = it[m].fid;
+    if (fid < 6) {
+      const int dir = fid >> 1, x = face_at(&r, dir, fid & 1);
+      if (it[m].fi1 == 0) {
+        if (dir == 0) {
+          for (int k = r.ks + 1; k <= r.ke; k++)
+            for (int j = r.js; j <= r.je; j++) EE2(b, k, j, x) *= 0.5;
+          for (int k = r.ks; k <= r.ke; k++)
+            for (int j = r.js + 1; j <= r.je; j++) EE3(b, k, j, x) *= 0.5;
+        } else if (dir == 1) {
+          for (int k = r.ks + 1; k <= r.ke; k++)
+            for (int i = r.is; i <= r.ie; i++) EE1(b, k, x, i) *= 0.5;
+          for (int k = r.ks; k <= r.ke; k++)
+            for (int i = r.is + 1; i <= r.ie; i++) EE3(b, k, x, i) *= 0.5;
+        } else {
+          for (int j = r.js + 1; j <= r.je; j++)
+            for (int i = r.is; i <= r.ie; i++) EE1(b, x, j, i) *= 0.5;
+          for (int j = r.js; j <= r.je; j++)
+            for (int i = r.is + 1; i <= r.ie; i++) EE2(b, x, j, i) *= 0.5;
+        }
+      } else {
+        if (dir == 0) {
+          const int k = r.ks + nx3 / 2, j = r.js + nx2 / 2;
+          for (int jj = r.js; jj <= r.je; jj++) EE2(b, k, jj, x) *= 0.5;
+          for (int kk = r.ks; kk <= r.ke; kk++) EE3(b, kk, j, x) *= 0.5;
+        } else if (dir == 1) {
+          const int k = r.ks + nx3 / 2, i = r.is + nx1 / 2;
+          for (int ii = r.is; ii <= r.ie; ii++) EE1(b, k, x, ii) *= 0.5;
+          for (int kk = r.ks; kk <= r.ke; kk++) EE3(b, kk, x, i) *= 0.5;
+        } else {
+          const int j = r.js + nx2 / 2, i = r.is + nx1 / 2;
+          for (int ii = r.is; ii <= r.ie; ii++) EE1(b, x, j, ii) *= 0.5;
+          for (int jj = r.js; jj <= r.je; jj++) EE2(b, x, jj, i) *= 0.5;
+        }
+      }
+    } else {
+      const int eid = fid - 6, lo = eid & 1, hi = (eid >> 1) & 1;
+      const double div = 1.0 / (double)it[m].fi1;
+      if (eid < 4) {
+        const int i = face_at(&r, 0, lo), j = face_at(&r, 1, hi);
+        for (int k = r.ks; k <= r.ke; k++) EE3(b, k, j, i) *= div;
+      } else if (eid < 8) {
+        const int i = face_at(&r, 0, lo), k = face_at(&r, 2, hi);
+        for (int j = r.js; j <= r.je; j++) EE2(b, k, j, i) *= div;
+      } else {
+        const int j = face_at(&r, 1, lo), k = face_at(&r, 2, hi);
+        for (int i = r.is; i <= r.ie; i++) EE1(b, k, j, i) *= div;
+      }
+    }
+  }
+}
+#undef EE1
+#undef EE2
+#undef EE3
+
+/* CountFineEdges (bvals/fc/bvals_fc.cpp:1128-1191) for a 3D block.  nblevel[27] holds the
+ * neighbourhood's levels relative to the block at ((ox3+1)*3 + ox2+1)*3 + ox1+1 (any
+ * negative value for a coarser or missing neighbour); edge_flag[e] = no finer block
+ * touches edge e, nedge_fine[e] = the number of blocks at the finest level around it. */
+void orc_ref_count_fine_edges(const int *nblevel, int *edge_flag, int *nedge_fine) {
+#define LV(o3, o2, o1) nblevel[((o3) + 1) * 9 + ((o2) + 1) * 3 + (o1) + 1]
+  int eid = 0;
+  for (int pass = 0; pass < 3; ++pass)   /* x1x2 (ox2, ox1), x1x3 (ox3, ox1), x2x3 (ox3, ox2) */
+    for (int os = -1; os <= 1; os += 2)
+      for (int of = -1; of <= 1; of += 2) {
+        int nf = 0, fl = 0;
+        const int as = os - 1 < -1 ? -1 : os - 1, ae = os + 1 > 1 ? 1 : os + 1;
+        const int bs = of - 1 < -1 ? -1 : of - 1, be = of + 1 > 1 ? 1 : of + 1;
+        for (int a = as; a <= ae; a++)
+          for (int c = bs; c <= be; c++) {
+            const int l = pass == 0 ? LV(0, a, c) : (pass == 1 ? LV(a, 0, c) : LV(a, c, 0));
+            if (l > fl) { fl++; nf = 0; }
+            if (l == fl) nf++;
+          }
+        edge_flag[eid] = (fl == 0);
+        nedge_fine[eid++] = nf;
+      }
+#undef LV
 }
 
 /* ---------------------------------------------------------------- ghost prolongation */

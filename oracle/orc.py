@@ -315,6 +315,11 @@ def _ref_lib():
         L.orc_ref_prim_to_cons.argtypes = [G, V, V, V, V, V, V, ctypes.c_double, ctypes.c_int, IT, ctypes.c_int]
         IP = ctypes.POINTER(ctypes.c_int)
         L.orc_ref_prolong_ranges.argtypes = [G, IP, IP, IP, IP]
+        L.orc_ref_emf_same_level.argtypes = [G, V, V, V, V, ctypes.c_int64, IT, ctypes.c_int]
+        L.orc_ref_emf_set.argtypes = [G, V, V, V, V, ctypes.c_int64, ctypes.c_int, IT, ctypes.c_int]
+        L.orc_ref_emf_clear.argtypes = [G, V, V, V, IT, ctypes.c_int]
+        L.orc_ref_emf_average.argtypes = [G, V, V, V, IT, ctypes.c_int]
+        L.orc_ref_count_fine_edges.argtypes = [IP, IP, IP]
         L._ref_bound = True
     return L
 
@@ -371,3 +376,31 @@ def ref_prolong_ranges(g, ox, lx_odd, nblevel):
     rc = _ref_lib().orc_ref_prolong_ranges(ctypes.byref(g), I3(*ox), I3(*lx_odd), I27(*nblevel), out)
     assert rc == 0
     return list(out)
+
+
+def ref_emf_same_level(g, e1, e2, e3, buf, stride, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_emf_same_level(ctypes.byref(g), _ptr(e1), _ptr(e2), _ptr(e3), _ptr(buf), stride, a, n)
+
+
+def ref_emf_set(g, e1, e2, e3, buf, stride, finer, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_emf_set(ctypes.byref(g), _ptr(e1), _ptr(e2), _ptr(e3), _ptr(buf), stride, int(finer), a, n)
+
+
+def ref_emf_clear(g, e1, e2, e3, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_emf_clear(ctypes.byref(g), _ptr(e1), _ptr(e2), _ptr(e3), a, n)
+
+
+def ref_emf_average(g, e1, e2, e3, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_emf_average(ctypes.byref(g), _ptr(e1), _ptr(e2), _ptr(e3), a, n)
+
+
+def ref_count_fine_edges(nblevel):
+    """CountFineEdges: (edge_flag[12], nedge_fine[12]) from 27 relative levels (self = 0)."""
+    I27, I12 = ctypes.c_int * 27, ctypes.c_int * 12
+    fl, nf = I12(), I12()
+    _ref_lib().orc_ref_count_fine_edges(I27(*nblevel), fl, nf)
+    return [bool(x) for x in fl], list(nf)

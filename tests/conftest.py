@@ -38,6 +38,18 @@ def load_pgen():
     return mod
 
 
+def load_smr():
+    """Host block tree and EMF-correction work lists of the package (athena-gamma_amd/smr.py)."""
+    name = "agx_smr"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "athena-gamma_amd", "smr.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
 @pytest.fixture(scope="session")
 def agx():
     return load_agx()

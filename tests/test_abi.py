@@ -79,6 +79,11 @@ def test_scatter_gather_roundtrip(agx):
     ("prolong_fc1", dict(fb=0, cb=0, si=2, ei=9, sj=2, ej=5, sk=2, ek=5), 1, 1, -40),    # x1 face 9 > nc
     ("flux_from_finer", dict(fb=0, cb=0, fid=6), 1, 1, -40),
     ("flux_from_finer", dict(fb=0, cb=0, fid=1, fi1=2), 1, 1, -40),
+    ("emf_to_coarser", dict(fb=0, fid=18), 1, 1, -40),
+    ("emf_set_finer", dict(fb=0, cb=0, fid=7, fi2=1), 1, 1, -40),       # an edge has one half index
+    ("emf_set_same", dict(fb=3, cb=0, fid=0), 2, 1, -40),               # buffer slot out of range
+    ("emf_average", dict(cb=0, fid=8, fi1=1), 1, 1, -40),               # an edge needs 2..4 blocks
+    ("emf_clear", dict(cb=1, fid=0), 1, 1, -40),
 ])
 def test_refinement_lists_reject_bad_items(agx, kind, it, nbf, nbc, code):
     """agx_ref_list_create checks every item on the host before any device work."""
@@ -99,3 +104,24 @@ def test_refinement_emf_count(agx):
     assert agx.ref_emf_count(g, 0) == 3 * 3 + 2 * 4
     assert agx.ref_emf_count(g, 3) == 3 * 4 + 2 * 5
     assert agx.ref_emf_count(g, 5) == 4 * 4 + 3 * 5
+
+
+def test_refinement_emf_lists_reject_duplicates(agx):
+    """two items writing the same face of one block cannot share a launch"""
+    g = agx.RefGeom(8, 8, 8, 2, 2, 0)
+    it = dict(fb=0, cb=1, si=0, ei=0, sj=0, ej=0, sk=0, ek=0, fid=2, fi1=0, fi2=0, pad=0)
+    items = (agx.RefItem * 2)(agx.RefItem(**it), agx.RefItem(**it))
+    h = ctypes.c_void_p()
+    for kind in ("emf_set_same", "emf_clear", "emf_average"):
+        rc = agx.lib().agx_ref_list_create(ctypes.byref(g), agx.REF_KINDS[kind], items, 2, 4, 4, ctypes.byref(h))
+        assert rc == -40 and not h.value, kind
+        assert b"same block" in agx.lib().agx_last_error()
+
+
+def test_refinement_emf_counts_edges(agx):
+    g = agx.RefGeom(8, 6, 4, 2, 2, 0)
+    # edges to a coarser block: one restricted line, flux_correction_fc.cpp:481-546
+    assert [agx.ref_emf_count(g, 6 + e) for e in (0, 4, 8)] == [2, 3, 4]
+    # same level: x1 face e2 (nx3+1) x nx2 + e3 nx3 x (nx2+1), edges one line
+    assert agx.ref_emf_count_same_level(g, 0) == 5 * 6 + 4 * 7
+    assert [agx.ref_emf_count_same_level(g, 6 + e) for e in (0, 4, 8)] == [4, 6, 8]
