@@ -39,7 +39,7 @@ EXPORTS = [
     "agx_device_alloc", "agx_device_free", "agx_device_copy", "agx_ref_prim_to_cons", "agx_ref_prolong_ranges",
     "agx_upload_blocks", "agx_download_blocks", "agx_set_mg_fuse",
     "agx_ref_emf_count_same_level", "agx_ref_emf_same_level", "agx_ref_emf_set", "agx_ref_emf_clear",
-    "agx_ref_emf_average", "agx_ref_count_fine_edges",
+    "agx_ref_emf_average", "agx_ref_count_fine_edges", "agx_ref_copy_cc",
 ]
 
 
@@ -95,7 +95,7 @@ class RefCoords(ctypes.Structure):
 REF_KINDS = dict(restrict_cc=0, restrict_fc1=1, restrict_fc2=2, restrict_fc3=3, prolong_cc=4,
                  prolong_fc1=5, prolong_fc2=6, prolong_fc3=7, prolong_fc_int=8, flux_from_finer=9,
                  emf_to_coarser=10, prim_to_cons=11, emf_same_level=12, emf_set_same=13, emf_set_finer=14,
-                 emf_clear=15, emf_average=16)
+                 emf_clear=15, emf_average=16, copy_cc_bb=17, copy_cc_cb=18, copy_cc_bc=19)
 
 _LIB = None
 
@@ -169,6 +169,7 @@ def lib():
     L.agx_ref_emf_clear.argtypes = [P, P, P, P, P]
     L.agx_ref_emf_average.argtypes = [P, P, P, P, P]
     L.agx_ref_count_fine_edges.argtypes = [IP, IP, IP]
+    L.agx_ref_copy_cc.argtypes = [P, ctypes.c_int32, P, P, P]
     L.agx_device_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(P)]
     L.agx_device_free.argtypes = [P]
     L.agx_device_copy.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
@@ -430,6 +431,9 @@ class RefList:
 
     def emf_to_coarser(self, e1, e2, e3, fc, buf, stride):
         _check(lib().agx_ref_emf_to_coarser(self.h, e1, e2, e3, fc, buf, stride, None), "agx_ref_emf_to_coarser")
+
+    def copy_cc(self, nvar, src, dst):
+        _check(lib().agx_ref_copy_cc(self.h, nvar, src, dst, None), "agx_ref_copy_cc")
 
     def emf_same_level(self, e1, e2, e3, buf, stride):
         _check(lib().agx_ref_emf_same_level(self.h, e1, e2, e3, buf, stride, None), "agx_ref_emf_same_level")

@@ -499,6 +499,28 @@ k_ref_emf_to_coarser(RG r, const double *__restrict__ e1, const double *__restri
   }
 }
 
+// ---------------------------------------------------------------- ghost exchange (cells)
+// The buffer round trips of CellCenteredBoundaryVariable (bvals/cc/bvals_cc.cpp:201-480)
+// as direct box copies: item m copies nvar x [si..ei] x [sj..ej] x [sk..ek] of block (or
+// coarse buffer) fb to the box of the same shape starting at (fid, fi1, fi2) = (i, j, k)
+// of block (or coarse buffer) cb.  S = source dims, D = destination dims.
+__global__ void __launch_bounds__(RBS)
+k_ref_copy_cc(int s1, int s2, int s3, int d1, int d2, int d3, int nvar, const double *__restrict__ src,
+              double *__restrict__ dst, const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const Range q = item_range(it);
+  const unsigned w1 = q.w1, w2 = q.w2, per = w1 * w2 * q.w3, cnt = per * nvar;
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
+    const int n = (int)(t / per);
+    unsigned s = t - n * per;
+    const int a = (int)(s % w1);
+    s /= w1;
+    const int b = (int)(s % w2), c = (int)(s / w2);
+    dst[ix(d3, d2, d1, nvar, it.cb, n, it.fi2 + c, it.fi1 + b, it.fid + a)] =
+        src[ix(s3, s2, s1, nvar, it.fb, n, q.sk + c, q.sj + b, q.si + a)];
+  }
+}
+
 // ---------------------------------------------------------------- EMF correction steps
 // The elements an item touches form one or two boxes of one edge array each (e1, e2,
 // e3), walked k outer, j, i inner as the reference's loops, so the packed buffers keep
@@ -714,6 +736,16 @@ int check_item(const RG &r, int kind, const agx_ref_item &it, int nbf, int nbc, 
       if (it.fid < 0 || it.fid > 5) return bad("fid must be 0..5");
       if ((it.fi1 & ~1) || (it.fi2 & ~1)) return bad("fi1 and fi2 must be 0 or 1");
       return 0;
+    case AGX_REF_COPY_CC_BB: case AGX_REF_COPY_CC_CB: case AGX_REF_COPY_CC_BC: {
+      const bool src_coarse = kind == AGX_REF_COPY_CC_CB, dst_coarse = kind == AGX_REF_COPY_CC_BC;
+      const int dst0[3] = {it.fid, it.fi1, it.fi2};
+      for (int d = 0; d < 3; ++d) {
+        const int sn = src_coarse ? cn[d] : fn[d], dn = dst_coarse ? cn[d] : fn[d];
+        if (!in(lo[d], hi[d], 0, sn - 1)) return bad("source range outside the source array");
+        if (!in(dst0[d], dst0[d] + hi[d] - lo[d], 0, dn - 1)) return bad("destination box outside the destination array");
+      }
+      return 0;
+    }
     case AGX_REF_EMF_TO_COARSER: case AGX_REF_EMF_SAME_LEVEL: case AGX_REF_EMF_SET_SAME:
     case AGX_REF_EMF_CLEAR:
       if (it.fid < 0 || it.fid > 17) return bad("fid must be a face 0..5 or 6 + an edge 0..11");
@@ -849,7 +881,8 @@ int agx_ref_list_create(const agx_ref_geom *g, int32_t kind, const agx_ref_item 
   l->nphase = nphase;
   std::copy(phase_off, phase_off + 17, l->phase_off);
   const int nvar_hint = (kind == AGX_REF_RESTRICT_CC || kind == AGX_REF_PROLONG_CC ||
-                         kind == AGX_REF_FLUX_FROM_FINER) ? 5 : 1;
+                         kind == AGX_REF_FLUX_FROM_FINER || kind == AGX_REF_COPY_CC_BB ||
+                         kind == AGX_REF_COPY_CC_CB || kind == AGX_REF_COPY_CC_BC) ? 5 : 1;
   l->G = (int)std::min<int64_t>(64, std::max<int64_t>(1, (maxc * nvar_hint + RBS - 1) / RBS));
   if (nitems > 0) {
     hipError_t e = hipMalloc(&l->items, sizeof(agx_ref_item) * nitems);
@@ -1050,6 +1083,17 @@ int agx_ref_prolong_ranges(const agx_ref_geom *g, const int32_t ox[3], const int
   out[0] = lo[0]; out[1] = hi[0]; out[2] = lo[1]; out[3] = hi[1]; out[4] = lo[2]; out[5] = hi[2];
   out[6] = fl[0]; out[7] = fh[0]; out[8] = fl[1]; out[9] = fh[1]; out[10] = fl[2]; out[11] = fh[2];
   return 0;
+}
+
+int agx_ref_copy_cc(const agx_ref_list *l, int32_t nvar, const double *src, double *dst, void *stream) {
+  if (int rc = check_call(l, AGX_REF_COPY_CC_BB, AGX_REF_COPY_CC_BC, "agx_ref_copy_cc", nvar)) return rc;
+  if (l->nitems == 0) return 0;
+  const RG r = make_rg(l->g);
+  const bool sc = l->kind == AGX_REF_COPY_CC_CB, dc = l->kind == AGX_REF_COPY_CC_BC;
+  hipLaunchKernelGGL(k_ref_copy_cc, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream, sc ? r.c1 : r.n1,
+                     sc ? r.c2 : r.n2, sc ? r.c3 : r.n3, dc ? r.c1 : r.n1, dc ? r.c2 : r.n2, dc ? r.c3 : r.n3, nvar,
+                     src, dst, l->items);
+  return launch_status("agx_ref_copy_cc");
 }
 
 int agx_ref_emf_same_level(const agx_ref_list *l, const double *e1, const double *e2, const double *e3, double *buf,

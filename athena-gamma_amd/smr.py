@@ -2,7 +2,7 @@
 
 A mesh of 3D MeshBlocks on several refinement levels is given as its leaves, each a
 LogicalLocation (level, lx1, lx2, lx3) with level 0 = the root grid of nrb blocks.  This
-module finds every block's face and edge neighbours the way the reference does:
+module finds every block's face, edge and corner neighbours the way the reference does:
   - MeshBlockTree::FindNeighbor (mesh/meshblock_tree.cpp:362-450): the same-level
     location at the offset, wrapped on periodic boundaries, is a leaf (same level),
     lies inside a coarser leaf, or is refined (its children are the neighbours);
@@ -12,14 +12,19 @@ module finds every block's face and edge neighbours the way the reference does:
     (NeighborBlock::SetNeighbor, bvals_base.cpp:43-66), the rule that a coarser edge
     neighbour is kept only by the block at that corner of its parent, and nblevel;
   - FaceCenteredBoundaryVariable::CountFineEdges (bvals/fc/bvals_fc.cpp:1128-1191).
-It then builds the work lists of the EMF correction of one cycle in the order of
-SendFluxCorrection / ReceiveFluxCorrection (bvals/fc/flux_correction_fc.cpp:623-688,
-1610-1660): pack every same-level and to-coarser buffer from the blocks' own EMFs, then
-per block add the same-level buffers, clear the parts bordering finer blocks, add the
-finer buffers and average.
+It then builds the work lists of
+  - the EMF correction of one cycle in the order of SendFluxCorrection /
+    ReceiveFluxCorrection (bvals/fc/flux_correction_fc.cpp:623-688, 1610-1660): pack
+    every same-level and to-coarser buffer from the blocks' own EMFs, then per block add
+    the same-level buffers, clear the parts bordering finer blocks, add the finer
+    buffers and average (faces and edges);
+  - the ghost exchange of cell-centred variables across levels
+    (CellCenteredBoundaryVariable, bvals/cc/bvals_cc.cpp:201-479, and the cell part of
+    BoundaryValues::ProlongateBoundaries, bvals/bvals_refine.cpp:93-276): same-level
+    copies, restriction and copy to coarser blocks, copy of coarser data into the coarse
+    buffer, restriction of same-level ghosts and prolongation (faces, edges, corners).
 
-Plain host logic: no device work.  Corners, polar and shearing boundaries are not
-handled (the EMF correction uses faces and edges only).
+Plain host logic: no device work.  Polar and shearing boundaries are not handled.
 """
 from dataclasses import dataclass
 
@@ -32,7 +37,7 @@ K_EMF_TO_COARSER, K_EMF_SAME_LEVEL, K_EMF_SET_SAME, K_EMF_SET_FINER, K_EMF_CLEAR
 class Neighbor:
     gid: int
     rel: int          # neighbour level minus this block's level: -1, 0, +1
-    kind: str         # "face" or "edge"
+    kind: str         # "face", "edge" or "corner"
     ox: tuple         # (ox1, ox2, ox3)
     fid: int = -1     # BoundaryFace for faces
     eid: int = -1     # edge id for edges
@@ -168,10 +173,22 @@ class SMRTree:
                         rel = 0 if r[0] == "same" else -1
                         if rel == 0 or (myox[da] == n and myox[db] == m):
                             out.append(Neighbor(r[1], rel, "edge", tuple(ox), eid=eid))
+        for l in (-1, 1):                         # corners
+            for m in (-1, 1):
+                for n in (-1, 1):
+                    ox = (n, m, l)
+                    r = self.find(g, ox)
+                    if r is None:
+                        continue
+                    if r[0] == "finer":
+                        out.append(Neighbor(self._child(r[1], 1 - (n + 1) // 2, 1 - (m + 1) // 2, 1 - (l + 1) // 2),
+                                            1, "corner", ox))
+                    elif r[0] == "same" or myox == [n, m, l]:
+                        out.append(Neighbor(r[1], 0 if r[0] == "same" else -1, "corner", ox))
         return out
 
-    def neighbors(self, g):
-        return self._nb[g]
+    def neighbors(self, g, corners=True):
+        return self._nb[g] if corners else [n for n in self._nb[g] if n.kind != "corner"]
 
     def nblevel(self, g):
         """27 levels relative to block g at ((ox3+1)*3 + ox2+1)*3 + ox1+1; -1 where the
@@ -240,7 +257,7 @@ def emf_plan(tree):
     to_coarser, same_level = [], []
     slot_same, slot_fine = {}, {}
     for g in range(nblk):                        # SendFluxCorrection, every block
-        for nb in tree.neighbors(g):
+        for nb in tree.neighbors(g, corners=False):
             if nb.rel == 0 and (nb.kind == "face" or flags[g][0][nb.eid]):
                 slot_same[(nb.gid, tuple(-o for o in nb.ox), g)] = len(same_level)
                 same_level.append(_item(fb=g, fid=nb.code))
@@ -251,7 +268,7 @@ def emf_plan(tree):
     for g in range(nblk):                        # ReceiveFluxCorrection, every block
         eflag, nfine = flags[g]
         lv = tree.nblevel(g)
-        for nb in tree.neighbors(g):
+        for nb in tree.neighbors(g, corners=False):
             if nb.rel == 0 and (nb.kind == "face" or eflag[nb.eid]):
                 set_same.append(_item(fb=slot_same[(g, nb.ox, nb.gid)], cb=g, fid=nb.code))
         for fid in range(6):                     # ClearCoarseFluxBoundary
@@ -262,7 +279,7 @@ def emf_plan(tree):
         for e in range(12):
             if not eflag[e]:
                 clear.append(_item(cb=g, fid=6 + e))
-        for nb in tree.neighbors(g):
+        for nb in tree.neighbors(g, corners=False):
             if nb.rel == 1:
                 set_finer.append(_item(fb=slot_fine[(g, nb.ox, nb.gid)], cb=g, fid=nb.code, fi1=nb.fi1, fi2=nb.fi2))
         for fid in range(6):                     # AverageFluxBoundary
@@ -279,3 +296,153 @@ def emf_plan(tree):
             if nfine[e] != 1:
                 average.append(_item(cb=g, fid=6 + e, fi1=nfine[e]))
     return EMFPlan(to_coarser, same_level, set_same, clear, set_finer, average)
+
+
+# ---------------------------------------------------------------- cell-centred ghosts
+@dataclass
+class CCPlan:
+    """Work lists (dicts of agx_ref_item fields) of one cell-centred ghost exchange with
+    prolongation, to run in this order: copy_same (AGX_REF_COPY_CC_BB, block -> block),
+    restrict_send (RESTRICT_CC, block -> its own coarse buffer), copy_from_finer
+    (COPY_CC_CB, coarse buffer -> coarser block's ghosts), copy_to_finer (COPY_CC_BC,
+    block -> finer block's coarse buffer), restrict_same (RESTRICT_CC) and prolong
+    (PROLONG_CC).  per_neighbor lists, for each block in neighbour order, its coarser
+    neighbours' (restrict_same items, prolong item): the reference's own order."""
+    copy_same: list
+    restrict_send: list
+    copy_from_finer: list
+    copy_to_finer: list
+    restrict_same: list
+    prolong: list
+    per_neighbor: list
+
+
+def _box(fb, cb, si, ei, sj, ej, sk, ek, dst=(0, 0, 0)):
+    return dict(fb=fb, cb=cb, si=si, ei=ei, sj=sj, ej=ej, sk=sk, ek=ek, fid=dst[0], fi1=dst[1], fi2=dst[2], pad=0)
+
+
+def _fi_axes(ox):
+    """which of fi1 / fi2 splits each direction with ox == 0 (bvals_cc.cpp:262-283)"""
+    ax = [None, None, None]
+    free = [d for d in range(3) if ox[d] == 0]
+    if len(free) == 2:
+        ax[free[0]], ax[free[1]] = "fi1", "fi2"
+    elif len(free) == 1:
+        ax[free[0]] = "fi1"
+    return ax
+
+
+def cc_plan(tree, nx, ng):
+    """Ghost exchange of cubic blocks of nx^3 cells and ng ghosts; coarse buffers of
+    nx/2 + 2 ng cells, interior from ng (mesh/meshblock.cpp:59-85)."""
+    nblk = len(tree.leaves)
+    lo, hi = ng, ng + nx - 1                    # is, ie (same in j, k)
+    clo, chi = ng, ng + nx // 2 - 1             # cis, cie
+    cng = (ng + 1) // 2 + 1                     # cnghost
+    h = nx // 2
+    copy_same, restrict_send, copy_from_finer, copy_to_finer = [], [], [], []
+    restrict_same, prolong, per_neighbor = [], [], []
+
+    def back(src, dst, ox):
+        """the neighbour entry of block src that points at block dst from the other side"""
+        o = tuple(-x for x in ox)
+        for nb in tree.neighbors(src):
+            if nb.gid == dst and nb.ox == o:
+                return nb
+        raise KeyError((src, dst, ox))
+
+    for g in range(nblk):
+        lx = tree.leaves[g][1:]
+        for nb in tree.neighbors(g):
+            ox, s = nb.ox, nb.gid
+            if nb.rel == 0:
+                # LoadBoundaryBufferSameLevel of s towards g, SetBoundarySameLevel of g
+                src, dst = [], []
+                for d in range(3):
+                    o = -ox[d]
+                    src.append((hi - ng + 1 if o > 0 else lo, lo + ng - 1 if o < 0 else hi))
+                    dst.append(lo if ox[d] == 0 else (hi + 1 if ox[d] > 0 else lo - ng))
+                copy_same.append(_box(s, g, *src[0], *src[1], *src[2], dst=dst))
+            elif nb.rel == 1:
+                # s restricts its slab into its coarse buffer and packs it
+                # (LoadBoundaryBufferToCoarser); g unpacks (SetBoundaryFromFiner)
+                cn = ng - 1
+                src, dst = [], []
+                ax = _fi_axes(ox)
+                for d in range(3):
+                    o = -ox[d]
+                    src.append((chi - cn if o > 0 else clo, clo + cn if o < 0 else chi))
+                    if ox[d] == 0:
+                        dst.append(lo + (h if getattr(nb, ax[d]) == 1 else 0))
+                    else:
+                        dst.append(hi + 1 if ox[d] > 0 else lo - ng)
+                restrict_send.append(_box(s, s, *src[0], *src[1], *src[2]))
+                copy_from_finer.append(_box(s, g, *src[0], *src[1], *src[2], dst=dst))
+            else:
+                # s packs for its finer neighbour g (LoadBoundaryBufferToFiner), g unpacks
+                # into its coarse buffer (SetBoundaryFromCoarser)
+                sb = back(s, g, ox)
+                cn = cng - 1
+                ax = _fi_axes(sb.ox)
+                src, dst = [], []
+                for d in range(3):
+                    o = sb.ox[d]
+                    a, b = (hi - cn if o > 0 else lo), (lo + cn if o < 0 else hi)
+                    if o == 0:
+                        if getattr(sb, ax[d]) == 1:
+                            a += h - cng
+                        else:
+                            b -= h - cng
+                    src.append((a, b))
+                    if ox[d] == 0:
+                        dst.append(clo if lx[d] & 1 == 0 else clo - cng)
+                    else:
+                        dst.append(chi + 1 if ox[d] > 0 else clo - cng)
+                copy_to_finer.append(_box(s, g, *src[0], *src[1], *src[2], dst=dst))
+        # ProlongateBoundaries: per coarser neighbour, restrict the same-level ghost
+        # positions around it, then prolongate its ghost range
+        lv = tree.nblevel(g)
+        for nb in tree.neighbors(g):
+            if nb.rel >= 0:
+                continue
+            ox = nb.ox
+            rs = []
+            rng = [range(max(ox[d] - 1, -1), min(ox[d] + 1, 1) + 1) for d in range(3)]
+            for nk in rng[2]:
+                for nj in rng[1]:
+                    for ni in rng[0]:
+                        if (ni, nj, nk) == (0, 0, 0) or lv[((nk + 1) * 3 + nj + 1) * 3 + ni + 1] != 0:
+                            continue
+                        r = []
+                        for d, nn in enumerate((ni, nj, nk)):
+                            if nn == 0:
+                                a, b = clo, chi
+                                if ox[d] == 1:
+                                    a = chi
+                                elif ox[d] == -1:
+                                    b = clo
+                            elif nn == 1:
+                                a = b = chi + 1
+                            else:
+                                a = b = clo - 1
+                            r.append((a, b))
+                        rs.append(_box(g, g, *r[0], *r[1], *r[2]))
+            pr = []
+            cn = cng - 1
+            for d in range(3):
+                if ox[d] == 0:
+                    a, b = clo, chi
+                    if lx[d] & 1 == 0:
+                        b += cn
+                    else:
+                        a -= cn
+                elif ox[d] > 0:
+                    a, b = chi + 1, chi + cn
+                else:
+                    a, b = clo - cn, clo - 1
+                pr.append((a, b))
+            p = _box(g, g, *pr[0], *pr[1], *pr[2])
+            restrict_same += rs
+            prolong.append(p)
+            per_neighbor.append((rs, p))
+    return CCPlan(copy_same, restrict_send, copy_from_finer, copy_to_finer, restrict_same, prolong, per_neighbor)

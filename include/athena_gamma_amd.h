@@ -281,7 +281,18 @@ enum {
   AGX_REF_EMF_AVERAGE = 16,    /* AverageFluxBoundary, flux_correction_fc.cpp:1355-1543: cb, fid =
                                   a face with a block or periodic boundary and fi1 = 0 (same
                                   level) or 1 (finer), or an edge and fi1 = nedge_fine (2..4) */
-  AGX_REF_KINDS = 17
+  AGX_REF_COPY_CC_BB = 17,     /* cell-centred ghost exchange between blocks of one level:
+                                  LoadBoundaryBufferSameLevel + SetBoundarySameLevel
+                                  (bvals/cc/bvals_cc.cpp:201-221, 300-341) */
+  AGX_REF_COPY_CC_CB = 18,     /* coarse buffer of a finer block -> ghosts of a coarser one:
+                                  the pack of LoadBoundaryBufferToCoarser (bvals_cc.cpp:223-248,
+                                  after AGX_REF_RESTRICT_CC) + SetBoundaryFromFiner (:407-479) */
+  AGX_REF_COPY_CC_BC = 19,     /* coarser block -> coarse buffer of a finer one:
+                                  LoadBoundaryBufferToFiner + SetBoundaryFromCoarser
+                                  (bvals_cc.cpp:250-298, 343-405).  Copy items: fb = source,
+                                  [si..ek] = source range, cb = destination, (fid, fi1, fi2) =
+                                  the destination's first (i, j, k) */
+  AGX_REF_KINDS = 20
 };
 typedef struct agx_ref_list agx_ref_list;
 /* Checks every item against the geometry and the pool sizes (nb_fine blocks on the fine
@@ -327,6 +338,10 @@ int agx_ref_emf_to_coarser(const agx_ref_list *l, const double *e1, const double
  * one class after another (and finer quadrants one after another), which gives every
  * element the reference's summation order.  One item per (cb, fid[, fi1, fi2]) in a
  * list (checked). */
+/* box copies of nvar cell-centred variables (COPY_CC_BB: blocks to blocks, COPY_CC_CB:
+ * coarse buffers to blocks, COPY_CC_BC: blocks to coarse buffers); items of one list
+ * must not write the same element */
+int agx_ref_copy_cc(const agx_ref_list *l, int32_t nvar, const double *src, double *dst, void *stream);
 /* each item's own EMFs on its face or edge into buf + item * stride (EMF_SAME_LEVEL) */
 int agx_ref_emf_same_level(const agx_ref_list *l, const double *e1, const double *e2, const double *e3,
                            double *buf, int64_t stride, void *stream);

@@ -134,6 +134,8 @@ void orc_ref_flux_from_finer(const orc_ref_geom *g, int nvar, const double *ff1,
 void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const double *e2,
                             const double *e3, orc_ref_coords fc, double *buf, int64_t stride,
                             const orc_ref_item *it, int nit);
+void orc_ref_copy_cc(const orc_ref_geom *g, int nvar, const double *src, int src_coarse, double *dst,
+                     int dst_coarse, const orc_ref_item *it, int nit);
 void orc_ref_emf_same_level(const orc_ref_geom *g, const double *e1, const double *e2,
                             const double *e3, double *buf, int64_t stride, const orc_ref_item *it,
                             int nit);

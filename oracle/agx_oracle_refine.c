@@ -19,6 +19,8 @@
  *                              face and edge neighbours of a 3D block
  *                              (bvals/fc/flux_correction_fc.cpp:315-546), packed in the
  *                              reference's buffer order
+ *   orc_ref_copy_cc            the pack + unpack of the cell-centred ghost exchanges
+ *                              (bvals/cc/bvals_cc.cpp:201-479)
  *   orc_ref_emf_same_level     LoadFluxBoundaryBufferSameLevel (flux_correction_fc.cpp:51-313)
  *   orc_ref_emf_set            SetFluxBoundarySameLevel / SetFluxBoundaryFromFiner
  *                              (flux_correction_fc.cpp:689-1185)
@@ -474,6 +476,27 @@ void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const doubl
 #undef E1
 #undef E2
 #undef E3
+  }
+}
+
+/* ---------------------------------------------------------------- ghost exchange (cells)
+ * The pack + unpack of CellCenteredBoundaryVariable's same-level, to-coarser and
+ * to-finer exchanges (bvals/cc/bvals_cc.cpp:201-479) as one copy per item: nvar x the
+ * source range of block or coarse buffer fb (src_coarse) to the box from (fid, fi1, fi2)
+ * = (i, j, k) of block or coarse buffer cb (dst_coarse), in PackData / UnpackData
+ * order (n, k, j, i). */
+void orc_ref_copy_cc(const orc_ref_geom *g, int nvar, const double *src, int src_coarse, double *dst,
+                     int dst_coarse, const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  const int s1 = src_coarse ? r.c1 : r.n1, s2 = src_coarse ? r.c2 : r.n2, s3 = src_coarse ? r.c3 : r.n3;
+  const int d1 = dst_coarse ? r.c1 : r.n1, d2 = dst_coarse ? r.c2 : r.n2, d3 = dst_coarse ? r.c3 : r.n3;
+  for (int m = 0; m < nit; ++m) {
+    for (int n = 0; n < nvar; ++n)
+      for (int k = it[m].sk; k <= it[m].ek; ++k)
+        for (int j = it[m].sj; j <= it[m].ej; ++j)
+          for (int i = it[m].si; i <= it[m].ei; ++i)
+            dst[ix(d3, d2, d1, nvar, it[m].cb, n, it[m].fi2 + k - it[m].sk, it[m].fi1 + j - it[m].sj,
+                   it[m].fid + i - it[m].si)] = src[ix(s3, s2, s1, nvar, it[m].fb, n, k, j, i)];
   }
 }
 

@@ -315,6 +315,7 @@ def _ref_lib():
         L.orc_ref_prim_to_cons.argtypes = [G, V, V, V, V, V, V, ctypes.c_double, ctypes.c_int, IT, ctypes.c_int]
         IP = ctypes.POINTER(ctypes.c_int)
         L.orc_ref_prolong_ranges.argtypes = [G, IP, IP, IP, IP]
+        L.orc_ref_copy_cc.argtypes = [G, ctypes.c_int, V, ctypes.c_int, V, ctypes.c_int, IT, ctypes.c_int]
         L.orc_ref_emf_same_level.argtypes = [G, V, V, V, V, ctypes.c_int64, IT, ctypes.c_int]
         L.orc_ref_emf_set.argtypes = [G, V, V, V, V, ctypes.c_int64, ctypes.c_int, IT, ctypes.c_int]
         L.orc_ref_emf_clear.argtypes = [G, V, V, V, IT, ctypes.c_int]
@@ -404,3 +405,8 @@ def ref_count_fine_edges(nblevel):
     fl, nf = I12(), I12()
     _ref_lib().orc_ref_count_fine_edges(I27(*nblevel), fl, nf)
     return [bool(x) for x in fl], list(nf)
+
+
+def ref_copy_cc(g, nvar, src, src_coarse, dst, dst_coarse, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_copy_cc(ctypes.byref(g), nvar, _ptr(src), int(src_coarse), _ptr(dst), int(dst_coarse), a, n)
