@@ -28,6 +28,8 @@ Plain host logic: no device work.  Polar and shearing boundaries are not handled
 """
 from dataclasses import dataclass
 
+import numpy as np
+
 # AGX_REF_* kinds of the EMF correction (include/athena_gamma_amd.h)
 K_EMF_TO_COARSER, K_EMF_SAME_LEVEL, K_EMF_SET_SAME, K_EMF_SET_FINER, K_EMF_CLEAR, K_EMF_AVERAGE = \
     10, 12, 13, 14, 15, 16
@@ -446,3 +448,22 @@ def cc_plan(tree, nx, ng):
             prolong.append(p)
             per_neighbor.append((rs, p))
     return CCPlan(copy_same, restrict_send, copy_from_finer, copy_to_finer, restrict_same, prolong, per_neighbor)
+
+
+def block_coords(tree, nx, ng, coarse=False, box=(1.0, 1.0, 1.0)):
+    """Per-block x?v / dx?f tables (dict of (nb, n) float64 arrays) of cubic blocks of
+    nx^3 cells on a uniform Cartesian root grid covering [0, box): the blocks' own cells
+    (n = nx + 2 ng) or their coarse buffers (n = nx/2 + 2 ng, coarse cell ng starting
+    at the block's first face)."""
+    n = (nx // 2 + 2 * ng) if coarse else nx + 2 * ng
+    tab = {k: [] for k in ("x1v", "x2v", "x3v", "dx1f", "dx2f", "dx3f")}
+    for (lev, *lx) in tree.leaves:
+        for d in range(3):
+            h = box[d] / ((tree.nrb[d] * nx) << lev)
+            if coarse:
+                f = (lx[d] * nx + 2 * (np.arange(n + 1) - ng)) * h
+            else:
+                f = (lx[d] * nx - ng + np.arange(n + 1)) * h
+            tab[f"x{d + 1}v"].append(0.5 * (f[1:] + f[:-1]))
+            tab[f"dx{d + 1}f"].append(f[1:] - f[:-1])
+    return {k: np.ascontiguousarray(np.array(v)) for k, v in tab.items()}

@@ -370,6 +370,118 @@ def ref_bench(agx, args):
             f"({100 * gbs * 1e9 / HBM_PEAK:.0f}% of 8 TB/s)")
     for a in [fine, coarse] + b + keep_f + keep_c:
         a.close()
+    out["smr"] = smr_bench(agx, reps)
+    return out
+
+
+def _box_cells(d):
+    return (d["ei"] - d["si"] + 1) * (d["ej"] - d["sj"] + 1) * (d["ek"] - d["sk"] + 1)
+
+
+def smr_bench(agx, reps):
+    """Whole-mesh SMR steps on a two-level mesh: 4^3 root blocks of 32^3 cells (ng 2) on
+    the unit cube with periodic boundaries, the central 2^3 root blocks refined (56 + 64
+    = 120 blocks).  Times one cell-centred ghost exchange with prolongation (5 variables;
+    same-level copies, restriction and copy to coarser blocks, copy to finer blocks'
+    coarse buffers, same-level ghost restriction, prolongation: 6 launches) and one EMF
+    correction (pack, set, clear, set from finer, average: 6 launches).  Algorithmic
+    bytes: every value moved is read and written once (8 B each), restriction reads 8
+    fine values per coarse one, prolongation writes 8 fine values per coarse one."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("agx_smr", os.path.join(ROOT, "athena-gamma_amd", "smr.py"))
+    smr = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(smr)
+    nx, ng, nvar = 32, 2, 5
+    refine = {(x, y, z) for x in (1, 2) for y in (1, 2) for z in (1, 2)}
+    leaves = [(0, x, y, z) for z in range(4) for y in range(4) for x in range(4) if (x, y, z) not in refine]
+    for (x, y, z) in sorted(refine):
+        leaves += [(1, 2 * x + a, 2 * y + b, 2 * z + c) for c in range(2) for b in range(2) for a in range(2)]
+    t = smr.SMRTree((4, 4, 4), leaves)
+    nb = len(leaves)
+    n, nc = nx + 2 * ng, nx // 2 + 2 * ng
+    g = agx.RefGeom(nx, nx, nx, ng, ng, 0)
+    fc, k1 = agx.device_coords(smr.block_coords(t, nx, ng))
+    cc, k2 = agx.device_coords(smr.block_coords(t, nx, ng, coarse=True))
+    rs = np.random.default_rng(5)
+    u = agx.DeviceArray(rs.random((nb, nvar, n, n, n)))
+    cu = agx.DeviceArray(rs.random((nb, nvar, nc, nc, nc)))
+    e = [agx.DeviceArray(rs.standard_normal(s)) for s in
+         ((nb, n + 1, n + 1, n), (nb, n + 1, n, n + 1), (nb, n, n + 1, n + 1))]
+    keep = k1 + k2 + [u, cu] + e
+    out = {"workload": f"{nb} blocks of {nx}^3 (ng {ng}) on two levels, 4^3 root blocks with the central 2^3 refined"}
+
+    def timed(run):
+        run()
+        agx.RefList.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        agx.RefList.sync()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    # cell-centred ghost exchange
+    p = smr.cc_plan(t, nx, ng)
+    L = lambda kind, items: agx.RefList(g, kind, [agx.RefItem(**d) for d in items], nb, nb)   # noqa: E731
+    ls = [L("copy_cc_bb", p.copy_same), L("restrict_cc", p.restrict_send), L("copy_cc_cb", p.copy_from_finer),
+          L("copy_cc_bc", p.copy_to_finer), L("restrict_cc", p.restrict_same), L("prolong_cc", p.prolong)]
+
+    def cc_run():
+        ls[0].copy_cc(nvar, u.ptr, u.ptr)
+        ls[1].restrict_cc(nvar, u.ptr, fc, cu.ptr)
+        ls[2].copy_cc(nvar, cu.ptr, u.ptr)
+        ls[3].copy_cc(nvar, u.ptr, cu.ptr)
+        ls[4].restrict_cc(nvar, u.ptr, fc, cu.ptr)
+        ls[5].prolong_cc(nvar, cu.ptr, cc, u.ptr, fc)
+    ms = timed(cc_run)
+    copies = sum(_box_cells(d) for d in p.copy_same + p.copy_from_finer + p.copy_to_finer)
+    restr = sum(_box_cells(d) for d in p.restrict_send + p.restrict_same)
+    prol = sum(_box_cells(d) for d in p.prolong)
+    alg = nvar * 8 * (2 * copies + 9 * restr + 9 * prol)
+    ghosts = nvar * (copies - sum(_box_cells(d) for d in p.copy_to_finer) + 8 * prol)
+    out["cc_exchange"] = {"ms": ms, "launches": 6, "alg_bytes": alg, "achieved_GBs": alg / (ms * 1e-3) / 1e9,
+                          "frac": alg / (ms * 1e-3) / HBM_PEAK, "ghost_values": ghosts}
+    log(f"[smr] cell ghost exchange + prolongation, {nb} blocks x {nvar} vars: {ms:.3f} ms "
+        f"({alg / 1e6:.0f} MB algorithmic, {alg / (ms * 1e-3) / 1e9:.0f} GB/s)")
+    for x in ls:
+        x.close()
+    # EMF correction
+    q = smr.emf_plan(t)
+    ss = max(agx.ref_emf_count_same_level(g, d["fid"]) for d in q.same_level)
+    sc = max(agx.ref_emf_count(g, d["fid"]) for d in q.to_coarser)
+    bs = agx.DeviceArray(np.zeros((len(q.same_level), ss)))
+    bc = agx.DeviceArray(np.zeros((len(q.to_coarser), sc)))
+    keep += [bs, bc]
+    le = [L("emf_same_level", q.same_level), L("emf_to_coarser", q.to_coarser),
+          agx.RefList(g, "emf_set_same", [agx.RefItem(**d) for d in q.set_same], len(q.same_level), nb),
+          L("emf_clear", q.clear),
+          agx.RefList(g, "emf_set_finer", [agx.RefItem(**d) for d in q.set_finer], len(q.to_coarser), nb),
+          L("emf_average", q.average)]
+    ep = [x.ptr for x in e]
+
+    def emf_run():
+        le[0].emf_same_level(*ep, bs.ptr, ss)
+        le[1].emf_to_coarser(*ep, fc, bc.ptr, sc)
+        le[2].emf_set(*ep, bs.ptr, ss)
+        le[3].emf_clear(*ep)
+        le[4].emf_set(*ep, bc.ptr, sc)
+        le[5].emf_average(*ep)
+    ms = timed(emf_run)
+    nsame = sum(agx.ref_emf_count_same_level(g, d["fid"]) for d in q.same_level)
+    ncoarse = sum(agx.ref_emf_count(g, d["fid"]) for d in q.to_coarser)
+    face_in = 2 * (nx - 1) * nx          # EMFs inside one face, not on its own edges
+    nclear = sum(face_in if d["fid"] < 6 else nx for d in q.clear)
+    navg = sum((face_in if d["fi1"] == 0 else 2 * nx) if d["fid"] < 6 else nx for d in q.average)
+    # pack same level: read + write; pack to coarser: 2 reads + write; set: buffer read,
+    # EMF read + write; clear: write; average: read + write (8 B each)
+    alg = 8 * (2 * nsame + 3 * ncoarse + 3 * (nsame + ncoarse) + nclear + 2 * navg)
+    out["emf_correction"] = {"ms": ms, "launches": "6 (the set lists run as one launch per class and quadrant)",
+                             "values_exchanged": nsame + ncoarse, "alg_bytes": alg,
+                             "achieved_GBs": alg / (ms * 1e-3) / 1e9, "frac": alg / (ms * 1e-3) / HBM_PEAK}
+    log(f"[smr] EMF correction, {nb} blocks: {ms:.3f} ms ({nsame + ncoarse} values exchanged)")
+    for x in le:
+        x.close()
+    for a in keep:
+        a.close()
     return out
 
 
