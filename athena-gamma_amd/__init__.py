@@ -39,7 +39,7 @@ EXPORTS = [
     "agx_device_alloc", "agx_device_free", "agx_device_copy", "agx_ref_prim_to_cons", "agx_ref_prolong_ranges",
     "agx_upload_blocks", "agx_download_blocks", "agx_set_mg_fuse",
     "agx_ref_emf_count_same_level", "agx_ref_emf_same_level", "agx_ref_emf_set", "agx_ref_emf_clear",
-    "agx_ref_emf_average", "agx_ref_count_fine_edges", "agx_ref_copy_cc",
+    "agx_ref_emf_average", "agx_ref_count_fine_edges", "agx_ref_copy_cc", "agx_ref_pack_fc", "agx_ref_unpack_fc",
 ]
 
 
@@ -79,7 +79,7 @@ def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)
 class RefItem(ctypes.Structure):
     """agx_ref_item: one work item of a refinement operator."""
     _fields_ = [(n, ctypes.c_int32) for n in ("fb", "cb", "si", "ei", "sj", "ej", "sk", "ek",
-                                              "fid", "fi1", "fi2", "pad")]
+                                              "fid", "fi1", "fi2", "aux")]
 
 
 class RefGeom(ctypes.Structure):
@@ -95,7 +95,8 @@ class RefCoords(ctypes.Structure):
 REF_KINDS = dict(restrict_cc=0, restrict_fc1=1, restrict_fc2=2, restrict_fc3=3, prolong_cc=4,
                  prolong_fc1=5, prolong_fc2=6, prolong_fc3=7, prolong_fc_int=8, flux_from_finer=9,
                  emf_to_coarser=10, prim_to_cons=11, emf_same_level=12, emf_set_same=13, emf_set_finer=14,
-                 emf_clear=15, emf_average=16, copy_cc_bb=17, copy_cc_cb=18, copy_cc_bc=19)
+                 emf_clear=15, emf_average=16, copy_cc_bb=17, copy_cc_cb=18, copy_cc_bc=19,
+                 pack_fc_b=20, pack_fc_c=21, unpack_fc_b=22, unpack_fc_c=23)
 
 _LIB = None
 
@@ -170,6 +171,8 @@ def lib():
     L.agx_ref_emf_average.argtypes = [P, P, P, P, P]
     L.agx_ref_count_fine_edges.argtypes = [IP, IP, IP]
     L.agx_ref_copy_cc.argtypes = [P, ctypes.c_int32, P, P, P]
+    L.agx_ref_pack_fc.argtypes = [P, P, P, P, P, ctypes.c_int64, P]
+    L.agx_ref_unpack_fc.argtypes = [P, P, P, P, P, ctypes.c_int64, P]
     L.agx_device_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(P)]
     L.agx_device_free.argtypes = [P]
     L.agx_device_copy.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
@@ -434,6 +437,12 @@ class RefList:
 
     def copy_cc(self, nvar, src, dst):
         _check(lib().agx_ref_copy_cc(self.h, nvar, src, dst, None), "agx_ref_copy_cc")
+
+    def pack_fc(self, f1, f2, f3, buf, stride):
+        _check(lib().agx_ref_pack_fc(self.h, f1, f2, f3, buf, stride, None), "agx_ref_pack_fc")
+
+    def unpack_fc(self, f1, f2, f3, buf, stride):
+        _check(lib().agx_ref_unpack_fc(self.h, f1, f2, f3, buf, stride, None), "agx_ref_unpack_fc")
 
     def emf_same_level(self, e1, e2, e3, buf, stride):
         _check(lib().agx_ref_emf_same_level(self.h, e1, e2, e3, buf, stride, None), "agx_ref_emf_same_level")

@@ -521,6 +521,36 @@ k_ref_copy_cc(int s1, int s2, int s3, int d1, int d2, int d3, int nvar, const do
   }
 }
 
+// ---------------------------------------------------------------- ghost exchange (faces)
+// FaceCenteredBoundaryVariable's buffer round trips (bvals/fc/bvals_fc.cpp:344-977): a
+// pack item copies component aux & 3 (1..3 = x1f, x2f, x3f) over [si..ei] x [sj..ej] x
+// [sk..ek] of block (or coarse buffer) fb into buf + item * stride, in PackData order;
+// an unpack item writes buf + fb * stride into the same-shaped range of block (or coarse
+// buffer) cb.  Shared faces are read by one neighbour's pack and written by another's
+// unpack, so every pack runs before any unpack, as with the reference's buffers.
+// UNPACK = false: pack.  COARSE: the face arrays are coarse buffers.
+template <bool UNPACK>
+__global__ void __launch_bounds__(RBS)
+k_ref_fc_move(int n1, int n2, int n3, double *__restrict__ f1, double *__restrict__ f2, double *__restrict__ f3,
+              double *__restrict__ buf, int64_t stride, const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const Range q = item_range(it);
+  const int c = (it.aux & 3) - 1;
+  double *f = c == 0 ? f1 : (c == 1 ? f2 : f3);
+  const int d1 = n1 + (c == 0), d2 = n2 + (c == 1), d3 = n3 + (c == 2);
+  const int b = UNPACK ? it.cb : it.fb;
+  double *row = buf + (UNPACK ? (int64_t)it.fb : (int64_t)blockIdx.y) * stride;
+  const unsigned w1 = q.w1, w2 = q.w2, cnt = w1 * w2 * q.w3;
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
+    const int i = q.si + (int)(t % w1);
+    const unsigned s = t / w1;
+    const int j = q.sj + (int)(s % w2), k = q.sk + (int)(s / w2);
+    const int64_t o = ix(d3, d2, d1, 1, b, 0, k, j, i);
+    if (UNPACK) f[o] = row[t];
+    else row[t] = f[o];
+  }
+}
+
 // ---------------------------------------------------------------- EMF correction steps
 // The elements an item touches form one or two boxes of one edge array each (e1, e2,
 // e3), walked k outer, j, i inner as the reference's loops, so the packed buffers keep
@@ -676,7 +706,8 @@ int check_item(const RG &r, int kind, const agx_ref_item &it, int nbf, int nbc, 
     return -40;
   };
   const bool uses_cb = kind != AGX_REF_PROLONG_FC_INT && kind != AGX_REF_EMF_TO_COARSER &&
-                       kind != AGX_REF_PRIM_TO_CONS && kind != AGX_REF_EMF_SAME_LEVEL;
+                       kind != AGX_REF_PRIM_TO_CONS && kind != AGX_REF_EMF_SAME_LEVEL &&
+                       kind != AGX_REF_PACK_FC_B && kind != AGX_REF_PACK_FC_C;
   const bool uses_fb = kind != AGX_REF_EMF_CLEAR && kind != AGX_REF_EMF_AVERAGE;
   if (uses_fb && (it.fb < 0 || it.fb >= nbf)) return bad("fine-side block (or buffer slot) index out of range");
   if (uses_cb && (it.cb < 0 || it.cb >= nbc)) return bad("coarse-side block index out of range");
@@ -743,6 +774,16 @@ int check_item(const RG &r, int kind, const agx_ref_item &it, int nbf, int nbc, 
         const int sn = src_coarse ? cn[d] : fn[d], dn = dst_coarse ? cn[d] : fn[d];
         if (!in(lo[d], hi[d], 0, sn - 1)) return bad("source range outside the source array");
         if (!in(dst0[d], dst0[d] + hi[d] - lo[d], 0, dn - 1)) return bad("destination box outside the destination array");
+      }
+      return 0;
+    }
+    case AGX_REF_PACK_FC_B: case AGX_REF_PACK_FC_C: case AGX_REF_UNPACK_FC_B: case AGX_REF_UNPACK_FC_C: {
+      const int c = (it.aux & 3) - 1;
+      if (c < 0 || (it.aux >> 2) > 15) return bad("aux must be a component 1..3 plus 4 x a launch group 0..15");
+      const bool coarse = kind == AGX_REF_PACK_FC_C || kind == AGX_REF_UNPACK_FC_C;
+      for (int d = 0; d < 3; ++d) {
+        const int nn = (coarse ? cn[d] : fn[d]) + (d == c);
+        if (!in(lo[d], hi[d], 0, nn - 1)) return bad("range outside the face array");
       }
       return 0;
     }
@@ -837,7 +878,8 @@ int agx_ref_list_create(const agx_ref_geom *g, int32_t kind, const agx_ref_item 
     if (rc) return rc;
     const int64_t c = item_count(r, kind, items[m]);
     maxc = std::max(maxc, c);
-    if (kind == AGX_REF_EMF_TO_COARSER || is_emf_box_kind(kind)) emf_max = std::max(emf_max, c);
+    if (kind == AGX_REF_EMF_TO_COARSER || is_emf_box_kind(kind) || (kind >= AGX_REF_PACK_FC_B && kind <= AGX_REF_UNPACK_FC_C))
+      emf_max = std::max(emf_max, c);
   }
   // EMF steps that write into blocks: one item per (block, face or edge[, quadrant]), so
   // that the items of one launch write disjoint elements
@@ -870,6 +912,19 @@ int agx_ref_list_create(const agx_ref_geom *g, int32_t kind, const agx_ref_item 
       }
       phase_off[nphase] = nitems;
     }
+  }
+  if (kind == AGX_REF_UNPACK_FC_B || kind == AGX_REF_UNPACK_FC_C) {
+    // launch groups in increasing aux >> 2: the caller's neighbour order (faces, edges,
+    // corners; finer quadrants in turn), for elements two neighbours both write
+    nphase = 16;
+    std::stable_sort(sorted.begin(), sorted.end(),
+                     [](const agx_ref_item &a, const agx_ref_item &b) { return (a.aux >> 2) < (b.aux >> 2); });
+    int m = 0;
+    for (int p = 0; p < nphase; ++p) {
+      phase_off[p] = m;
+      while (m < nitems && (sorted[m].aux >> 2) == p) ++m;
+    }
+    phase_off[nphase] = nitems;
   }
   if (nphase == 1) phase_off[1] = nitems;
   agx_ref_list *l = new agx_ref_list;
@@ -1094,6 +1149,44 @@ int agx_ref_copy_cc(const agx_ref_list *l, int32_t nvar, const double *src, doub
                      sc ? r.c2 : r.n2, sc ? r.c3 : r.n3, dc ? r.c1 : r.n1, dc ? r.c2 : r.n2, dc ? r.c3 : r.n3, nvar,
                      src, dst, l->items);
   return launch_status("agx_ref_copy_cc");
+}
+
+int agx_ref_pack_fc(const agx_ref_list *l, const double *x1f, const double *x2f, const double *x3f, double *buf,
+                    int64_t stride, void *stream) {
+  if (int rc = check_call(l, AGX_REF_PACK_FC_B, AGX_REF_PACK_FC_C, "agx_ref_pack_fc")) return rc;
+  if (stride < l->emf_max) {
+    set_error("agx_ref_pack_fc: stride %lld below the largest item (%lld values)", (long long)stride,
+              (long long)l->emf_max);
+    return -48;
+  }
+  if (l->nitems == 0) return 0;
+  const RG r = make_rg(l->g);
+  const bool c = l->kind == AGX_REF_PACK_FC_C;
+  hipLaunchKernelGGL(k_ref_fc_move<false>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                     c ? r.c1 : r.n1, c ? r.c2 : r.n2, c ? r.c3 : r.n3, const_cast<double *>(x1f),
+                     const_cast<double *>(x2f), const_cast<double *>(x3f), buf, stride, l->items);
+  return launch_status("agx_ref_pack_fc");
+}
+
+int agx_ref_unpack_fc(const agx_ref_list *l, double *x1f, double *x2f, double *x3f, const double *buf, int64_t stride,
+                      void *stream) {
+  if (int rc = check_call(l, AGX_REF_UNPACK_FC_B, AGX_REF_UNPACK_FC_C, "agx_ref_unpack_fc")) return rc;
+  if (stride < l->emf_max) {
+    set_error("agx_ref_unpack_fc: stride %lld below the largest item (%lld values)", (long long)stride,
+              (long long)l->emf_max);
+    return -48;
+  }
+  const RG r = make_rg(l->g);
+  const bool c = l->kind == AGX_REF_UNPACK_FC_C;
+  for (int p = 0; p < l->nphase; ++p) {
+    const int a = l->phase_off[p], n = l->phase_off[p + 1] - a;
+    if (n == 0) continue;
+    hipLaunchKernelGGL(k_ref_fc_move<true>, dim3(l->G, n), dim3(RBS), 0, (hipStream_t)stream, c ? r.c1 : r.n1,
+                       c ? r.c2 : r.n2, c ? r.c3 : r.n3, x1f, x2f, x3f, const_cast<double *>(buf), stride,
+                       l->items + a);
+    if (int rc = launch_status("agx_ref_unpack_fc")) return rc;
+  }
+  return 0;
 }
 
 int agx_ref_emf_same_level(const agx_ref_list *l, const double *e1, const double *e2, const double *e3, double *buf,

@@ -237,7 +237,7 @@ def count_fine_edges(nblevel):
 
 
 def _item(fb=0, cb=0, fid=0, fi1=0, fi2=0):
-    return dict(fb=fb, cb=cb, si=0, ei=0, sj=0, ej=0, sk=0, ek=0, fid=fid, fi1=fi1, fi2=fi2, pad=0)
+    return dict(fb=fb, cb=cb, si=0, ei=0, sj=0, ej=0, sk=0, ek=0, fid=fid, fi1=fi1, fi2=fi2, aux=0)
 
 
 @dataclass
@@ -320,7 +320,7 @@ class CCPlan:
 
 
 def _box(fb, cb, si, ei, sj, ej, sk, ek, dst=(0, 0, 0)):
-    return dict(fb=fb, cb=cb, si=si, ei=ei, sj=sj, ej=ej, sk=sk, ek=ek, fid=dst[0], fi1=dst[1], fi2=dst[2], pad=0)
+    return dict(fb=fb, cb=cb, si=si, ei=ei, sj=sj, ej=ej, sk=sk, ek=ek, fid=dst[0], fi1=dst[1], fi2=dst[2], aux=0)
 
 
 def _fi_axes(ox):
@@ -467,3 +467,270 @@ def block_coords(tree, nx, ng, coarse=False, box=(1.0, 1.0, 1.0)):
             tab[f"x{d + 1}v"].append(0.5 * (f[1:] + f[:-1]))
             tab[f"dx{d + 1}f"].append(f[1:] - f[:-1])
     return {k: np.ascontiguousarray(np.array(v)) for k, v in tab.items()}
+
+
+# ---------------------------------------------------------------- face-field ghosts
+@dataclass
+class FCPlan:
+    """Work lists of one face-field (x1f, x2f, x3f) ghost exchange with prolongation, to
+    run in this order: restrict_send[c] (AGX_REF_RESTRICT_FC1..3, block faces -> own
+    coarse faces), pack_b (PACK_FC_B: same-level and to-finer packs, buffer slots 0..),
+    pack_c (PACK_FC_C: to-coarser packs, slots from len(pack_b)), unpack_b (UNPACK_FC_B:
+    same-level and from-finer, into block faces), unpack_c (UNPACK_FC_C: from coarser,
+    into coarse faces), restrict_same[c] (RESTRICT_FC), prolong_shared[c] (PROLONG_FC1..3)
+    and prolong_int (PROLONG_FC_INT).  aux = component (1..3) + 4 x launch group (the
+    neighbour class and quadrant, in the reference's order).  per_neighbor: per block and
+    coarser neighbour, (restrict[c], shared[c], internal item) in the reference's order."""
+    restrict_send: list
+    pack_b: list
+    pack_c: list
+    unpack_b: list
+    unpack_c: list
+    restrict_same: list
+    prolong_shared: list
+    prolong_int: list
+    per_neighbor: list
+
+
+def _group(nb):
+    cls = {"face": 0, "edge": 1, "corner": 2}[nb.kind]
+    return cls * 4 + ((nb.fi1 + 2 * nb.fi2) if nb.rel == 1 else 0)
+
+
+def _rng_item(fb, cb, r, aux=0):
+    (si, ei), (sj, ej), (sk, ek) = r
+    return dict(fb=fb, cb=cb, si=si, ei=ei, sj=sj, ej=ej, sk=sk, ek=ek, fid=0, fi1=0, fi2=0, aux=aux)
+
+
+def fc_plan(tree, nx, ng):
+    """Face-field ghost exchange of cubic blocks (bvals/fc/bvals_fc.cpp:344-977, the face
+    part of RestrictGhostCellsOnSameLevel and ProlongateGhostCells, bvals_refine.cpp:
+    289-330, 483-521)."""
+    nblk = len(tree.leaves)
+    lo, hi = ng, ng + nx - 1
+    clo, chi = ng, ng + nx // 2 - 1
+    cng = (ng + 1) // 2 + 1
+    h = nx // 2
+    ml = tree.maxlev > 0                       # Mesh::multilevel
+    fi_ax = _fi_axes
+
+    def fi_of(nb, ax, d):
+        return getattr(nb, ax[d]) if ax[d] else 0
+
+    def sl_pack(o, nonface, c):
+        r = []
+        for d in range(3):
+            if d == c:
+                a, b = (lo, hi + 1) if o[d] == 0 else ((hi - ng + 1, hi) if o[d] > 0 else (lo + 1, lo + ng))
+                if ml and nonface:
+                    if o[d] > 0:
+                        b += 1
+                    elif o[d] < 0:
+                        a -= 1
+            else:
+                a, b = (lo, hi) if o[d] == 0 else ((hi - ng + 1, hi) if o[d] > 0 else (lo, lo + ng - 1))
+            r.append((a, b))
+        return r
+
+    def sl_unpack(ox, nonface, c):
+        r = []
+        for d in range(3):
+            if d == c:
+                a, b = (lo, hi + 1) if ox[d] == 0 else ((hi + 2, hi + ng + 1) if ox[d] > 0 else (lo - ng, lo - 1))
+                if ml and nonface:
+                    if ox[d] > 0:
+                        a -= 1
+                    elif ox[d] < 0:
+                        b += 1
+            else:
+                a, b = (lo, hi) if ox[d] == 0 else ((hi + 1, hi + ng) if ox[d] > 0 else (lo - ng, lo - 1))
+            r.append((a, b))
+        return r
+
+    def tc_pack(o, nonface, c):                # LoadBoundaryBufferToCoarser, coarse faces
+        r = []
+        for d in range(3):
+            if d == c:
+                a, b = (clo, chi + 1) if o[d] == 0 else ((chi - ng + 1, chi) if o[d] > 0 else (clo + 1, clo + ng))
+                if nonface:
+                    if o[d] > 0:
+                        b += 1
+                    elif o[d] < 0:
+                        a -= 1
+            else:
+                a, b = (clo, chi) if o[d] == 0 else ((chi - ng + 1, chi) if o[d] > 0 else (clo, clo + ng - 1))
+            r.append((a, b))
+        return r
+
+    def ff_unpack(nb, c):                      # SetBoundaryFromFiner
+        ox, nonface, ax = nb.ox, nb.kind != "face", fi_ax(nb.ox)
+        r = []
+        for d in range(3):
+            e = d == c
+            if ox[d] == 0:
+                a, b = lo, hi + e
+                if fi_of(nb, ax, d) == 1:
+                    a += h
+                else:
+                    b -= h
+            elif ox[d] > 0:
+                a, b = hi + 1 + e, hi + ng + e
+            else:
+                a, b = lo - ng, lo - 1
+            if e and nonface:
+                if ox[d] > 0:
+                    a -= 1
+                elif ox[d] < 0:
+                    b += 1
+            r.append((a, b))
+        return r
+
+    def tf_pack(sb, c):                        # LoadBoundaryBufferToFiner, sender's view sb
+        o, ax = sb.ox, fi_ax(sb.ox)
+        r = []
+        for d in range(3):
+            e = d == c
+            if o[d] == 0:
+                a, b = lo, hi + e
+                if fi_of(sb, ax, d) == 1:
+                    a += h - cng
+                else:
+                    b -= h - cng
+            elif o[d] > 0:
+                a, b = (hi + 1 - cng, hi + 1) if e else (hi - (cng - 1), hi)
+            else:
+                a, b = (lo, lo + cng) if e else (lo, lo + cng - 1)
+            r.append((a, b))
+        return r
+
+    def fc_unpack(ox, lx, c):                  # SetBoundaryFromCoarser, coarse faces
+        r = []
+        for d in range(3):
+            e = d == c
+            if ox[d] == 0:
+                a, b = clo, chi + e
+                if lx[d] & 1 == 0:
+                    b += cng
+                else:
+                    a -= cng
+            elif ox[d] > 0:
+                a, b = chi + 1, chi + cng + e
+            else:
+                a, b = clo - cng, clo - 1 + e
+            r.append((a, b))
+        return r
+
+    def back(src, dst, ox):
+        o = tuple(-x for x in ox)
+        for nb in tree.neighbors(src):
+            if nb.gid == dst and nb.ox == o:
+                return nb
+        raise KeyError((src, dst, ox))
+
+    restrict_send = [[], [], []]
+    pack_b, pack_c, unpack_b, unpack_c = [], [], [], []
+    pend_c = []                                # unpacks reading pack_c slots (offset later)
+    for g in range(nblk):
+        lx = tree.leaves[g][1:]
+        for nb in tree.neighbors(g):
+            s, ox, grp = nb.gid, nb.ox, _group(nb)
+            o = tuple(-x for x in ox)
+            for c in range(3):
+                aux = (c + 1) + 4 * grp
+                if nb.rel == 0:
+                    slot = len(pack_b)
+                    pack_b.append(_rng_item(s, s, sl_pack(o, nb.kind != "face", c), c + 1))
+                    unpack_b.append(_rng_item(slot, g, sl_unpack(ox, nb.kind != "face", c), aux))
+                elif nb.rel == 1:
+                    r = tc_pack(o, nb.kind != "face", c)
+                    restrict_send[c].append(_rng_item(s, s, r))
+                    pend_c.append(len(unpack_b))
+                    unpack_b.append(_rng_item(len(pack_c), g, ff_unpack(nb, c), aux))
+                    pack_c.append(_rng_item(s, s, r, c + 1))
+                else:
+                    sb = back(s, g, ox)
+                    slot = len(pack_b)
+                    pack_b.append(_rng_item(s, s, tf_pack(sb, c), c + 1))
+                    unpack_c.append(_rng_item(slot, g, fc_unpack(ox, lx, c), aux))
+    for m in pend_c:
+        unpack_b[m]["fb"] += len(pack_b)
+
+    restrict_same, prolong_shared, prolong_int, per_neighbor = [[], [], []], [[], [], []], [], []
+    cn = cng - 1
+    for g in range(nblk):
+        lx = tree.leaves[g][1:]
+        lv = tree.nblevel(g)
+        L = lambda o: lv[((o[2] + 1) * 3 + o[1] + 1) * 3 + o[0] + 1]   # noqa: E731
+        for nb in tree.neighbors(g):
+            if nb.rel >= 0:
+                continue
+            ox = nb.ox
+            rs = [[], [], []]
+            rng = [range(max(ox[d] - 1, -1), min(ox[d] + 1, 1) + 1) for d in range(3)]
+            for nk in rng[2]:
+                for nj in rng[1]:
+                    for ni in rng[0]:
+                        pos = (ni, nj, nk)
+                        if pos == (0, 0, 0) or L(pos) != 0:
+                            continue
+                        r = []
+                        for d, nn in enumerate(pos):
+                            if nn == 0:
+                                a, b = clo, chi
+                                if ox[d] == 1:
+                                    a = chi
+                                elif ox[d] == -1:
+                                    b = clo
+                            elif nn == 1:
+                                a = b = chi + 1
+                            else:
+                                a = b = clo - 1
+                            r.append((a, b))
+                        for c in range(3):
+                            a, b = r[c][0], r[c][1] + 1
+                            pm = list(pos)
+                            pm[c] -= 1
+                            pp = list(pos)
+                            pp[c] += 1
+                            if a == clo and L(pm) < 0:
+                                a += 1
+                            if b == chi + 1 and L(pp) < 0:
+                                b -= 1
+                            rr = list(r)
+                            rr[c] = (a, b)
+                            rs[c].append(_rng_item(g, g, rr))
+            cell = []
+            for d in range(3):
+                if ox[d] == 0:
+                    a, b = clo, chi
+                    if lx[d] & 1 == 0:
+                        b += cn
+                    else:
+                        a -= cn
+                elif ox[d] > 0:
+                    a, b = chi + 1, chi + cn
+                else:
+                    a, b = clo - cn, clo - 1
+                cell.append((a, b))
+            sh = []
+            for c in range(3):
+                a, b = cell[c][0], cell[c][1] + 1
+                om = list(ox)
+                om[c] -= 1
+                op = list(ox)
+                op[c] += 1
+                if ox[c] >= 0 and L(om) >= 0:
+                    a += 1
+                if ox[c] <= 0 and L(op) >= 0:
+                    b -= 1
+                rr = list(cell)
+                rr[c] = (a, b)
+                sh.append(_rng_item(g, g, rr))
+            it = _rng_item(g, g, cell)
+            for c in range(3):
+                restrict_same[c] += rs[c]
+                prolong_shared[c].append(sh[c])
+            prolong_int.append(it)
+            per_neighbor.append((rs, sh, it))
+    return FCPlan(restrict_send, pack_b, pack_c, unpack_b, unpack_c, restrict_same, prolong_shared, prolong_int,
+                  per_neighbor)

@@ -237,7 +237,8 @@ int agx_set_graph(agx_mesh *m, int32_t enable);
 typedef struct {
   int32_t fb, cb;
   int32_t si, ei, sj, ej, sk, ek;
-  int32_t fid, fi1, fi2, pad;
+  int32_t fid, fi1, fi2, aux;   /* aux: face component and launch group of the face
+                                   pack/unpack kinds, 0 otherwise */
 } agx_ref_item;
 typedef struct {
   int32_t nx1, nx2, nx3;   /* interior cells of a block (even) */
@@ -292,7 +293,15 @@ enum {
                                   (bvals_cc.cpp:250-298, 343-405).  Copy items: fb = source,
                                   [si..ek] = source range, cb = destination, (fid, fi1, fi2) =
                                   the destination's first (i, j, k) */
-  AGX_REF_KINDS = 20
+  AGX_REF_PACK_FC_B = 20,      /* face-field ghost exchange (bvals/fc/bvals_fc.cpp:344-977), pack
+                                  from block faces: fb = block, [si..ek] = range of component
+                                  aux & 3 (1..3 = x1f, x2f, x3f), into buf + item * stride */
+  AGX_REF_PACK_FC_C = 21,      /* the same from coarse-buffer faces (after AGX_REF_RESTRICT_FC*) */
+  AGX_REF_UNPACK_FC_B = 22,    /* unpack into block faces: cb = block, [si..ek] = range of
+                                  component aux & 3, fb = buffer slot; items run in launch groups
+                                  of increasing aux >> 2 (0..15) */
+  AGX_REF_UNPACK_FC_C = 23,    /* the same into coarse-buffer faces */
+  AGX_REF_KINDS = 24
 };
 typedef struct agx_ref_list agx_ref_list;
 /* Checks every item against the geometry and the pool sizes (nb_fine blocks on the fine
@@ -342,6 +351,14 @@ int agx_ref_emf_to_coarser(const agx_ref_list *l, const double *e1, const double
  * coarse buffers to blocks, COPY_CC_BC: blocks to coarse buffers); items of one list
  * must not write the same element */
 int agx_ref_copy_cc(const agx_ref_list *l, int32_t nvar, const double *src, double *dst, void *stream);
+/* face-field ghost exchange: packs (PACK_FC_B / _C) read x?f of blocks or coarse
+ * buffers into buf + item * stride; unpacks (UNPACK_FC_B / _C) write buf + fb * stride
+ * into x?f, one launch per group (aux >> 2) in increasing order.  Run every pack of an
+ * exchange before its unpacks. */
+int agx_ref_pack_fc(const agx_ref_list *l, const double *x1f, const double *x2f, const double *x3f, double *buf,
+                    int64_t stride, void *stream);
+int agx_ref_unpack_fc(const agx_ref_list *l, double *x1f, double *x2f, double *x3f, const double *buf,
+                      int64_t stride, void *stream);
 /* each item's own EMFs on its face or edge into buf + item * stride (EMF_SAME_LEVEL) */
 int agx_ref_emf_same_level(const agx_ref_list *l, const double *e1, const double *e2, const double *e3,
                            double *buf, int64_t stride, void *stream);

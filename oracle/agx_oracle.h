@@ -107,7 +107,7 @@ void orc_conserved_totals(orc_ctx *c, double *out8);
 typedef struct {
   int32_t fb, cb;                   /* fine-side and coarse-side block */
   int32_t si, ei, sj, ej, sk, ek;   /* coarse index range (inclusive) */
-  int32_t fid, fi1, fi2, pad;       /* face and quadrant (flux/EMF correction) */
+  int32_t fid, fi1, fi2, aux;       /* face and quadrant (flux/EMF correction) */
 } orc_ref_item;
 typedef struct {
   int32_t nx1, nx2, nx3;            /* interior cells of a block */
@@ -136,6 +136,10 @@ void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const doubl
                             const orc_ref_item *it, int nit);
 void orc_ref_copy_cc(const orc_ref_geom *g, int nvar, const double *src, int src_coarse, double *dst,
                      int dst_coarse, const orc_ref_item *it, int nit);
+void orc_ref_pack_fc(const orc_ref_geom *g, double *f1, double *f2, double *f3, int coarse, double *buf,
+                     int64_t stride, const orc_ref_item *it, int nit);
+void orc_ref_unpack_fc(const orc_ref_geom *g, double *f1, double *f2, double *f3, int coarse, double *buf,
+                       int64_t stride, const orc_ref_item *it, int nit);
 void orc_ref_emf_same_level(const orc_ref_geom *g, const double *e1, const double *e2,
                             const double *e3, double *buf, int64_t stride, const orc_ref_item *it,
                             int nit);

@@ -21,6 +21,8 @@
  *                              reference's buffer order
  *   orc_ref_copy_cc            the pack + unpack of the cell-centred ghost exchanges
  *                              (bvals/cc/bvals_cc.cpp:201-479)
+ *   orc_ref_pack_fc, _unpack_fc the pack and unpack of the face-field ghost exchanges
+ *                              (bvals/fc/bvals_fc.cpp:344-977)
  *   orc_ref_emf_same_level     LoadFluxBoundaryBufferSameLevel (flux_correction_fc.cpp:51-313)
  *   orc_ref_emf_set            SetFluxBoundarySameLevel / SetFluxBoundaryFromFiner
  *                              (flux_correction_fc.cpp:689-1185)
@@ -498,6 +500,42 @@ void orc_ref_copy_cc(const orc_ref_geom *g, int nvar, const double *src, int src
             dst[ix(d3, d2, d1, nvar, it[m].cb, n, it[m].fi2 + k - it[m].sk, it[m].fi1 + j - it[m].sj,
                    it[m].fid + i - it[m].si)] = src[ix(s3, s2, s1, nvar, it[m].fb, n, k, j, i)];
   }
+}
+
+/* ---------------------------------------------------------------- ghost exchange (faces)
+ * PackData / UnpackData of FaceCenteredBoundaryVariable (bvals/fc/bvals_fc.cpp:344-977)
+ * with the ranges given per item: component aux & 3 (1..3 = x1f, x2f, x3f) of block or
+ * coarse buffer (coarse) fb into buf + m * stride (pack), or buf + fb * stride into
+ * block or coarse buffer cb (unpack), items in list order. */
+static void fc_move(const orc_ref_geom *g, double *f1, double *f2, double *f3, int coarse, double *buf,
+                    int64_t stride, int unpack, const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  const int n1 = coarse ? r.c1 : r.n1, n2 = coarse ? r.c2 : r.n2, n3 = coarse ? r.c3 : r.n3;
+  for (int m = 0; m < nit; ++m) {
+    const int c = (it[m].aux & 3) - 1;
+    double *f = c == 0 ? f1 : (c == 1 ? f2 : f3);
+    const int d1 = n1 + (c == 0), d2 = n2 + (c == 1), d3 = n3 + (c == 2);
+    const int b = unpack ? it[m].cb : it[m].fb;
+    double *row = buf + (unpack ? (int64_t)it[m].fb : (int64_t)m) * stride;
+    int64_t p = 0;
+    for (int k = it[m].sk; k <= it[m].ek; ++k)
+      for (int j = it[m].sj; j <= it[m].ej; ++j)
+        for (int i = it[m].si; i <= it[m].ei; ++i) {
+          double *e = &f[ix(d3, d2, d1, 1, b, 0, k, j, i)];
+          if (unpack) *e = row[p++];
+          else row[p++] = *e;
+        }
+  }
+}
+
+void orc_ref_pack_fc(const orc_ref_geom *g, double *f1, double *f2, double *f3, int coarse, double *buf,
+                     int64_t stride, const orc_ref_item *it, int nit) {
+  fc_move(g, f1, f2, f3, coarse, buf, stride, 0, it, nit);
+}
+
+void orc_ref_unpack_fc(const orc_ref_geom *g, double *f1, double *f2, double *f3, int coarse, double *buf,
+                       int64_t stride, const orc_ref_item *it, int nit) {
+  fc_move(g, f1, f2, f3, coarse, buf, stride, 1, it, nit);
 }
 
 /* ---------------------------------------------------------------- EMF correction

@@ -276,7 +276,7 @@ def ppm_pencil(q):
 class RefItem(ctypes.Structure):
     """orc_ref_item (same layout as agx_ref_item)."""
     _fields_ = [(n, ctypes.c_int32) for n in ("fb", "cb", "si", "ei", "sj", "ej", "sk", "ek",
-                                              "fid", "fi1", "fi2", "pad")]
+                                              "fid", "fi1", "fi2", "aux")]
 
 
 class RefGeom(ctypes.Structure):
@@ -315,6 +315,8 @@ def _ref_lib():
         L.orc_ref_prim_to_cons.argtypes = [G, V, V, V, V, V, V, ctypes.c_double, ctypes.c_int, IT, ctypes.c_int]
         IP = ctypes.POINTER(ctypes.c_int)
         L.orc_ref_prolong_ranges.argtypes = [G, IP, IP, IP, IP]
+        L.orc_ref_pack_fc.argtypes = [G, V, V, V, ctypes.c_int, V, ctypes.c_int64, IT, ctypes.c_int]
+        L.orc_ref_unpack_fc.argtypes = [G, V, V, V, ctypes.c_int, V, ctypes.c_int64, IT, ctypes.c_int]
         L.orc_ref_copy_cc.argtypes = [G, ctypes.c_int, V, ctypes.c_int, V, ctypes.c_int, IT, ctypes.c_int]
         L.orc_ref_emf_same_level.argtypes = [G, V, V, V, V, ctypes.c_int64, IT, ctypes.c_int]
         L.orc_ref_emf_set.argtypes = [G, V, V, V, V, ctypes.c_int64, ctypes.c_int, IT, ctypes.c_int]
@@ -410,3 +412,14 @@ def ref_count_fine_edges(nblevel):
 def ref_copy_cc(g, nvar, src, src_coarse, dst, dst_coarse, items):
     a, n = _items(items)
     _ref_lib().orc_ref_copy_cc(ctypes.byref(g), nvar, _ptr(src), int(src_coarse), _ptr(dst), int(dst_coarse), a, n)
+
+
+def ref_pack_fc(g, f, coarse, buf, stride, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_pack_fc(ctypes.byref(g), _ptr(f[0]), _ptr(f[1]), _ptr(f[2]), int(coarse), _ptr(buf), stride, a, n)
+
+
+def ref_unpack_fc(g, f, coarse, buf, stride, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_unpack_fc(ctypes.byref(g), _ptr(f[0]), _ptr(f[1]), _ptr(f[2]), int(coarse), _ptr(buf), stride,
+                                 a, n)

@@ -107,3 +107,116 @@ def mark_uncovered(cu, p, ng, mask):
                 if any(np.isnan(cu[(b, slice(None)) + s]).any() for s in st):
                     fk, fj, fi = [(c - ng) * 2 + ng for c in (ck, cj, ci)]
                     mask[b, :, fk:fk + 2, fj:fj + 2, fi:fi + 2] = True
+
+
+# ---------------------------------------------------------------- face fields
+BCOEF = [(1.0, 4.0, 2.0, -3.0), (-1.0, 1.0, -2.0, 5.0), (2.0, 3.0, -1.0, -2.0)]   # div B = 4 - 2 - 2 = 0
+
+
+def face_coords(t, nx, ng, coarse=False):
+    """x?f face positions per block (block faces or coarse-buffer faces)"""
+    n = (nx // 2 + 2 * ng) if coarse else nx + 2 * ng
+    out = []
+    for d in range(3):
+        rows = []
+        for (lev, *lx) in t.leaves:
+            hh = 1.0 / ((NRB[d] * nx) << lev)
+            rows.append(((lx[d] * nx + 2 * (np.arange(n + 1) - ng)) if coarse else (lx[d] * nx - ng + np.arange(n + 1))) * hh)
+        out.append(np.array(rows))
+    return out
+
+
+def linear_faces(t, nx, ng, tab, coarse=False):
+    """b_c on its faces: a linear, divergence-free field (dyadic, so restriction,
+    prolongation and the interior-face formula are exact)"""
+    xf = face_coords(t, nx, ng, coarse)
+    nb, n = tab["x1v"].shape
+    out = []
+    for c in range(3):
+        pos = [xf[d] if d == c else tab[f"x{d + 1}v"] for d in range(3)]
+        c0, c1, c2, c3 = BCOEF[c]
+        out.append(np.ascontiguousarray(c0 + c1 * pos[0][:, None, None, :] + c2 * pos[1][:, None, :, None]
+                                        + c3 * pos[2][:, :, None, None]))
+    return out
+
+
+def poison_face_ghosts(t, f, nx, ng):
+    lo, hi = ng, ng + nx - 1
+    for g in range(len(t.leaves)):
+        for ox in itertools.product((-1, 0, 1), repeat=3):
+            if ox == (0, 0, 0) or t.find(g, ox) is None:
+                continue
+            for c in range(3):
+                sl = []
+                for d in (2, 1, 0):
+                    e = d == c
+                    o = ox[d]
+                    sl.append(slice(lo, hi + 1 + e) if o == 0 else
+                              (slice(hi + 1 + e, hi + ng + 1 + e) if o > 0 else slice(0, lo)))
+                f[c][(g, *sl)] = np.nan
+
+
+def run_oracle_fc(orc, t, f, cf, nx, ng, ftab, ctab, plan=None, order="reference", coverage=None):
+    """One face-field exchange with prolongation on copies of the block faces f[3] and
+    coarse faces cf[3].  order 'reference': packs, unpacks in list (neighbour) order,
+    then per block and coarser neighbour restriction, shared and interior prolongation;
+    'batched': unpacks by launch group and the prolongation step list by list, as the
+    library runs them.  coverage: a list that receives, per shared prolongation, whether
+    a coarse face its minmod stencil reads was still NaN."""
+    plan = plan or load_smr().fc_plan(t, nx, ng)
+    g = geom(orc, nx, ng)
+    f = [np.ascontiguousarray(a.copy()) for a in f]
+    cf = [np.ascontiguousarray(a.copy()) for a in cf]
+    I = lambda ds: [orc.RefItem(**d) for d in ds]   # noqa: E731
+    for c in range(3):
+        orc.ref_restrict_fc(g, c, f[c], ftab, cf[c], I(plan.restrict_send[c]))
+    stride = max([1] + [(d["ei"] - d["si"] + 1) * (d["ej"] - d["sj"] + 1) * (d["ek"] - d["sk"] + 1)
+                        for d in plan.pack_b + plan.pack_c])
+    buf = np.zeros((max(1, len(plan.pack_b) + len(plan.pack_c)), stride))
+    orc.ref_pack_fc(g, f, 0, buf, stride, I(plan.pack_b))
+    bc = np.ascontiguousarray(buf[len(plan.pack_b):]) if plan.pack_c else np.zeros((1, stride))
+    orc.ref_pack_fc(g, cf, 1, bc, stride, I(plan.pack_c))
+    if plan.pack_c:
+        buf[len(plan.pack_b):] = bc
+    ub, uc = plan.unpack_b, plan.unpack_c
+    if order == "batched":
+        ub = sorted(ub, key=lambda d: d["aux"] >> 2)
+        uc = sorted(uc, key=lambda d: d["aux"] >> 2)
+    orc.ref_unpack_fc(g, f, 0, buf, stride, I(ub))
+    orc.ref_unpack_fc(g, cf, 1, buf, stride, I(uc))
+    if order == "reference":
+        for rs, sh, it in plan.per_neighbor:
+            for c in range(3):
+                orc.ref_restrict_fc(g, c, f[c], ftab, cf[c], I(rs[c]))
+            for c in range(3):
+                if coverage is not None:
+                    coverage.append(_fc_stencil_has_nan(cf[c], sh[c], c))
+                orc.ref_prolong_fc_shared(g, c, cf[c], ctab, f[c], ftab, I([sh[c]]))
+            orc.ref_prolong_fc_internal(g, f[0], f[1], f[2], ftab, I([it]))
+    else:
+        for c in range(3):
+            orc.ref_restrict_fc(g, c, f[c], ftab, cf[c], I(plan.restrict_same[c]))
+        for c in range(3):
+            orc.ref_prolong_fc_shared(g, c, cf[c], ctab, f[c], ftab, I(plan.prolong_shared[c]))
+        orc.ref_prolong_fc_internal(g, f[0], f[1], f[2], ftab, I(plan.prolong_int))
+    return f, cf
+
+
+def _fc_stencil_has_nan(cfc, p, c):
+    """ProlongateSharedField reads each coarse face and its four neighbours along the two
+    tangential directions (mesh_refinement.cpp:541-762), no diagonal ones"""
+    a = cfc[p["cb"]]
+    lo = [p["si"], p["sj"], p["sk"]]
+    hi = [p["ei"], p["ej"], p["ek"]]
+    boxes = [(0, 0, 0)]
+    for d in range(3):
+        if d != c:
+            for s in (-1, 1):
+                o = [0, 0, 0]
+                o[d] = s
+                boxes.append(tuple(o))
+    for o in boxes:
+        sub = a[lo[2] + o[2]:hi[2] + 1 + o[2], lo[1] + o[1]:hi[1] + 1 + o[1], lo[0] + o[0]:hi[0] + 1 + o[0]]
+        if np.isnan(sub).any():
+            return True
+    return False

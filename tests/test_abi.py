@@ -88,7 +88,7 @@ def test_scatter_gather_roundtrip(agx):
 def test_refinement_lists_reject_bad_items(agx, kind, it, nbf, nbc, code):
     """agx_ref_list_create checks every item on the host before any device work."""
     g = agx.RefGeom(8, 8, 8, 2, 2, 0)
-    full = dict(fb=0, cb=0, si=0, ei=0, sj=0, ej=0, sk=0, ek=0, fid=0, fi1=0, fi2=0, pad=0)
+    full = dict(fb=0, cb=0, si=0, ei=0, sj=0, ej=0, sk=0, ek=0, fid=0, fi1=0, fi2=0, aux=0)
     full.update(it)
     items = (agx.RefItem * 1)(agx.RefItem(**full))
     h = ctypes.c_void_p()
@@ -109,7 +109,7 @@ def test_refinement_emf_count(agx):
 def test_refinement_emf_lists_reject_duplicates(agx):
     """two items writing the same face of one block cannot share a launch"""
     g = agx.RefGeom(8, 8, 8, 2, 2, 0)
-    it = dict(fb=0, cb=1, si=0, ei=0, sj=0, ej=0, sk=0, ek=0, fid=2, fi1=0, fi2=0, pad=0)
+    it = dict(fb=0, cb=1, si=0, ei=0, sj=0, ej=0, sk=0, ek=0, fid=2, fi1=0, fi2=0, aux=0)
     items = (agx.RefItem * 2)(agx.RefItem(**it), agx.RefItem(**it))
     h = ctypes.c_void_p()
     for kind in ("emf_set_same", "emf_clear", "emf_average"):
