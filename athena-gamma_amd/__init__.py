@@ -40,6 +40,7 @@ EXPORTS = [
     "agx_upload_blocks", "agx_download_blocks", "agx_set_mg_fuse",
     "agx_ref_emf_count_same_level", "agx_ref_emf_same_level", "agx_ref_emf_set", "agx_ref_emf_clear",
     "agx_ref_emf_average", "agx_ref_count_fine_edges", "agx_ref_copy_cc", "agx_ref_pack_fc", "agx_ref_unpack_fc",
+    "agx_ref_cons_to_prim",
 ]
 
 
@@ -96,7 +97,7 @@ REF_KINDS = dict(restrict_cc=0, restrict_fc1=1, restrict_fc2=2, restrict_fc3=3, 
                  prolong_fc1=5, prolong_fc2=6, prolong_fc3=7, prolong_fc_int=8, flux_from_finer=9,
                  emf_to_coarser=10, prim_to_cons=11, emf_same_level=12, emf_set_same=13, emf_set_finer=14,
                  emf_clear=15, emf_average=16, copy_cc_bb=17, copy_cc_cb=18, copy_cc_bc=19,
-                 pack_fc_b=20, pack_fc_c=21, unpack_fc_b=22, unpack_fc_c=23)
+                 pack_fc_b=20, pack_fc_c=21, unpack_fc_b=22, unpack_fc_c=23, cons_to_prim=24)
 
 _LIB = None
 
@@ -172,6 +173,8 @@ def lib():
     L.agx_ref_count_fine_edges.argtypes = [IP, IP, IP]
     L.agx_ref_copy_cc.argtypes = [P, ctypes.c_int32, P, P, P]
     L.agx_ref_pack_fc.argtypes = [P, P, P, P, P, ctypes.c_int64, P]
+    L.agx_ref_cons_to_prim.argtypes = [P, P, P, P, P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int32, P]
     L.agx_ref_unpack_fc.argtypes = [P, P, P, P, P, ctypes.c_int64, P]
     L.agx_device_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(P)]
     L.agx_device_free.argtypes = [P]
@@ -437,6 +440,10 @@ class RefList:
 
     def copy_cc(self, nvar, src, dst):
         _check(lib().agx_ref_copy_cc(self.h, nvar, src, dst, None), "agx_ref_copy_cc")
+
+    def cons_to_prim(self, u, b1, b2, b3, w, bcc, gamma, dfloor, pfloor, iso=False):
+        _check(lib().agx_ref_cons_to_prim(self.h, u, b1, b2, b3, w, bcc, gamma, dfloor, pfloor, int(iso), None),
+               "agx_ref_cons_to_prim")
 
     def pack_fc(self, f1, f2, f3, buf, stride):
         _check(lib().agx_ref_pack_fc(self.h, f1, f2, f3, buf, stride, None), "agx_ref_pack_fc")

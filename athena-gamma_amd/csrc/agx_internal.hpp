@@ -199,4 +199,47 @@ int mg_solve(agx_mesh *m, int stage);
 int mg_vcycles(agx_mesh *m, int n);
 int mg_autotune(agx_mesh *m);
 
+// C2P of one cell in registers (eos/adiabatic_mhd.cpp:52-86).  Returns true when a
+// floor changed the conserved state.
+__device__ __forceinline__ bool c2p_cell(double &u_d, double u_m1, double u_m2, double u_m3,
+                                         double &u_e, double bcc1, double bcc2, double bcc3,
+                                         double gm1, double dfl, double pfl, double *wv) {
+  bool fl = !(u_d > dfl);
+  u_d = (u_d > dfl) ? u_d : dfl;
+  double di = 1.0 / u_d;
+  double pb = 0.5 * (bcc1 * bcc1 + bcc2 * bcc2 + bcc3 * bcc3);
+  double e_k = 0.5 * di * (u_m1 * u_m1 + u_m2 * u_m2 + u_m3 * u_m3);
+  double w_p = gm1 * (u_e - e_k - pb);
+  fl = fl || !(w_p > pfl);
+  u_e = (w_p > pfl) ? u_e : ((pfl / gm1) + e_k + pb);
+  w_p = (w_p > pfl) ? w_p : pfl;
+  wv[0] = u_d;
+  wv[1] = u_m1 * di;
+  wv[2] = u_m2 * di;
+  wv[3] = u_m3 * di;
+  wv[4] = w_p;
+  return fl;
+}
+
+// Isothermal C2P of one cell (eos/isothermal_mhd.cpp:40-73): density floor, velocities.
+__device__ __forceinline__ void c2p_cell_iso(double &u_d, double u_m1, double u_m2, double u_m3,
+                                             double dfl, double *wv) {
+  u_d = (u_d > dfl) ? u_d : dfl;
+  double di = 1.0 / u_d;
+  wv[0] = u_d;
+  wv[1] = u_m1 * di;
+  wv[2] = u_m2 * di;
+  wv[3] = u_m3 * di;
+  wv[4] = 0.0;
+}
+
+// C2P of one cell for either EOS; u_e is untouched in the isothermal case.
+template <bool ISO>
+__device__ __forceinline__ void c2p_eos(double &u_d, double u_m1, double u_m2, double u_m3,
+                                        double &u_e, double bcc1, double bcc2, double bcc3,
+                                        double gm1, double dfl, double pfl, double *wv) {
+  if (ISO) c2p_cell_iso(u_d, u_m1, u_m2, u_m3, dfl, wv);
+  else c2p_cell(u_d, u_m1, u_m2, u_m3, u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
+}
+
 }  // namespace agx

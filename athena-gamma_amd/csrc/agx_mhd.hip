@@ -1163,48 +1163,7 @@ k_bvals_fc(Geo g, double *a) {
       a[(int64_t)s * g.S + IDX(g, p.k - o3 * g.bx3, p.j - o2 * g.bx2, p.i - o1 * g.bx1)];
 }
 
-// C2P of one cell in registers (eos/adiabatic_mhd.cpp:52-86).  Returns true when a
-// floor changed the conserved state.
-__device__ __forceinline__ bool c2p_cell(double &u_d, double u_m1, double u_m2, double u_m3,
-                                         double &u_e, double bcc1, double bcc2, double bcc3,
-                                         double gm1, double dfl, double pfl, double *wv) {
-  bool fl = !(u_d > dfl);
-  u_d = (u_d > dfl) ? u_d : dfl;
-  double di = 1.0 / u_d;
-  double pb = 0.5 * (SQR(bcc1) + SQR(bcc2) + SQR(bcc3));
-  double e_k = 0.5 * di * (SQR(u_m1) + SQR(u_m2) + SQR(u_m3));
-  double w_p = gm1 * (u_e - e_k - pb);
-  fl = fl || !(w_p > pfl);
-  u_e = (w_p > pfl) ? u_e : ((pfl / gm1) + e_k + pb);
-  w_p = (w_p > pfl) ? w_p : pfl;
-  wv[0] = u_d;
-  wv[1] = u_m1 * di;
-  wv[2] = u_m2 * di;
-  wv[3] = u_m3 * di;
-  wv[4] = w_p;
-  return fl;
-}
-
-// Isothermal C2P of one cell (eos/isothermal_mhd.cpp:40-73): density floor, velocities.
-__device__ __forceinline__ void c2p_cell_iso(double &u_d, double u_m1, double u_m2, double u_m3,
-                                             double dfl, double *wv) {
-  u_d = (u_d > dfl) ? u_d : dfl;
-  double di = 1.0 / u_d;
-  wv[0] = u_d;
-  wv[1] = u_m1 * di;
-  wv[2] = u_m2 * di;
-  wv[3] = u_m3 * di;
-  wv[4] = 0.0;
-}
-
-// C2P of one cell for either EOS; u_e is untouched in the isothermal case.
-template <bool ISO>
-__device__ __forceinline__ void c2p_eos(double &u_d, double u_m1, double u_m2, double u_m3,
-                                        double &u_e, double bcc1, double bcc2, double bcc3,
-                                        double gm1, double dfl, double pfl, double *wv) {
-  if (ISO) c2p_cell_iso(u_d, u_m1, u_m2, u_m3, dfl, wv);
-  else c2p_cell(u_d, u_m1, u_m2, u_m3, u_e, bcc1, bcc2, bcc3, gm1, dfl, pfl, wv);
-}
+// c2p_cell, c2p_cell_iso, c2p_eos: agx_internal.hpp (shared with the refinement operators).
 
 // Field::CalculateCellCenteredField (uniform branch, field/field.cpp:112-180) and
 // EquationOfState::ConservedToPrimitive (eos/adiabatic_mhd.cpp:41-90, isothermal_mhd.cpp:

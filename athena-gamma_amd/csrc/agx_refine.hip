@@ -366,6 +366,45 @@ k_ref_prim_to_cons(RG r, const double *__restrict__ w, const double *__restrict_
   }
 }
 
+// Field::CalculateCellCenteredField (uniform branch) and EquationOfState::
+// ConservedToPrimitive (eos/adiabatic_mhd.cpp:41-90, isothermal_mhd.cpp:40-73) on the
+// coarse-buffer cells of each item's range, as ApplyPhysicalBoundariesOnCoarseLevel
+// calls them (bvals_refine.cpp:336-384) before the cell prolongation: coarse faces ->
+// coarse bcc, coarse conserved -> coarse primitive, floors written back.
+template <bool ISO>
+__global__ void __launch_bounds__(RBS)
+k_ref_cons_to_prim(RG r, double *__restrict__ u, const double *__restrict__ b1, const double *__restrict__ b2,
+                   const double *__restrict__ b3, double *__restrict__ w, double *__restrict__ bcc, double gm1,
+                   double dfl, double pfl, const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const Range q = item_range(it);
+  const unsigned w1 = q.w1, w2 = q.w2, cnt = w1 * w2 * q.w3;
+  constexpr int NH = ISO ? 4 : 5;
+  const int cb = it.cb;
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
+    const int i = q.si + (int)(t % w1);
+    const unsigned s = t / w1;
+    const int j = q.sj + (int)(s % w2), k = q.sk + (int)(s / w2);
+    const double bc1 = 0.5 * b1[ix(r.c3, r.c2, r.c1 + 1, 1, cb, 0, k, j, i)] +
+                       0.5 * b1[ix(r.c3, r.c2, r.c1 + 1, 1, cb, 0, k, j, i + 1)];
+    const double bc2 = 0.5 * b2[ix(r.c3, r.c2 + 1, r.c1, 1, cb, 0, k, j, i)] +
+                       0.5 * b2[ix(r.c3, r.c2 + 1, r.c1, 1, cb, 0, k, j + 1, i)];
+    const double bc3 = 0.5 * b3[ix(r.c3 + 1, r.c2, r.c1, 1, cb, 0, k, j, i)] +
+                       0.5 * b3[ix(r.c3 + 1, r.c2, r.c1, 1, cb, 0, k + 1, j, i)];
+    bcc[ix(r.c3, r.c2, r.c1, 3, cb, 0, k, j, i)] = bc1;
+    bcc[ix(r.c3, r.c2, r.c1, 3, cb, 1, k, j, i)] = bc2;
+    bcc[ix(r.c3, r.c2, r.c1, 3, cb, 2, k, j, i)] = bc3;
+    const int64_t o0 = ix(r.c3, r.c2, r.c1, NH, cb, 0, k, j, i);
+    const int64_t S = (int64_t)r.c3 * r.c2 * r.c1;
+    double u_d = u[o0], u_e = ISO ? 0.0 : u[o0 + 4 * S], wv[5];
+    c2p_eos<ISO>(u_d, u[o0 + S], u[o0 + 2 * S], u[o0 + 3 * S], u_e, bc1, bc2, bc3, gm1, dfl, pfl, wv);
+    u[o0] = u_d;
+    if (!ISO) u[o0 + 4 * S] = u_e;
+#pragma unroll
+    for (int n = 0; n < NH; ++n) w[o0 + n * S] = wv[n];
+  }
+}
+
 // ---------------------------------------------------------------- flux correction
 // LoadFluxBoundaryBufferToCoarser (flux_correction_cc.cpp:77-169) on the finer block and
 // SetFluxBoundaryFromFiner (:240-283) on the coarser one, as one direct write: value
@@ -708,7 +747,7 @@ int check_item(const RG &r, int kind, const agx_ref_item &it, int nbf, int nbc, 
   const bool uses_cb = kind != AGX_REF_PROLONG_FC_INT && kind != AGX_REF_EMF_TO_COARSER &&
                        kind != AGX_REF_PRIM_TO_CONS && kind != AGX_REF_EMF_SAME_LEVEL &&
                        kind != AGX_REF_PACK_FC_B && kind != AGX_REF_PACK_FC_C;
-  const bool uses_fb = kind != AGX_REF_EMF_CLEAR && kind != AGX_REF_EMF_AVERAGE;
+  const bool uses_fb = kind != AGX_REF_EMF_CLEAR && kind != AGX_REF_EMF_AVERAGE && kind != AGX_REF_CONS_TO_PRIM;
   if (uses_fb && (it.fb < 0 || it.fb >= nbf)) return bad("fine-side block (or buffer slot) index out of range");
   if (uses_cb && (it.cb < 0 || it.cb >= nbc)) return bad("coarse-side block index out of range");
   const int cs[3] = {r.cis, r.cjs, r.cks}, fs[3] = {r.is, r.js, r.ks};
@@ -777,6 +816,10 @@ int check_item(const RG &r, int kind, const agx_ref_item &it, int nbf, int nbc, 
       }
       return 0;
     }
+    case AGX_REF_CONS_TO_PRIM:
+      for (int d = 0; d < 3; ++d)   // cells and their upper faces (face dims nc + 1)
+        if (!in(lo[d], hi[d], 0, cn[d] - 1)) return bad("range outside the coarse buffer");
+      return 0;
     case AGX_REF_PACK_FC_B: case AGX_REF_PACK_FC_C: case AGX_REF_UNPACK_FC_B: case AGX_REF_UNPACK_FC_C: {
       const int c = (it.aux & 3) - 1;
       if (c < 0 || (it.aux >> 2) > 15) return bad("aux must be a component 1..3 plus 4 x a launch group 0..15");
@@ -1093,6 +1136,23 @@ int agx_ref_prim_to_cons(const agx_ref_list *l, const double *w, const double *b
     hipLaunchKernelGGL(k_ref_prim_to_cons<false>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
                        make_rg(l->g), w, b1f, b2f, b3f, bcc, u, igm1, l->items);
   return launch_status("agx_ref_prim_to_cons");
+}
+
+int agx_ref_cons_to_prim(const agx_ref_list *l, double *u, const double *b1f, const double *b2f, const double *b3f,
+                         double *w, double *bcc, double gamma, double dfloor, double pfloor, int32_t iso,
+                         void *stream) {
+  if (int rc = check_call(l, AGX_REF_CONS_TO_PRIM, AGX_REF_CONS_TO_PRIM, "agx_ref_cons_to_prim", iso ? 4 : 5))
+    return rc;
+  if (!iso && !(gamma > 1.0)) { set_error("agx_ref_cons_to_prim: gamma must be > 1"); return -50; }
+  if (l->nitems == 0) return 0;
+  const RG r = make_rg(l->g);
+  if (iso)
+    hipLaunchKernelGGL(k_ref_cons_to_prim<true>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream, r, u, b1f,
+                       b2f, b3f, w, bcc, 0.0, dfloor, pfloor, l->items);
+  else
+    hipLaunchKernelGGL(k_ref_cons_to_prim<false>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream, r, u,
+                       b1f, b2f, b3f, w, bcc, gamma - 1.0, dfloor, pfloor, l->items);
+  return launch_status("agx_ref_cons_to_prim");
 }
 
 // BoundaryValues::ProlongateBoundaries loop limits (bvals_refine.cpp:172-199) and the face

@@ -734,3 +734,55 @@ def fc_plan(tree, nx, ng):
             per_neighbor.append((rs, sh, it))
     return FCPlan(restrict_send, pack_b, pack_c, unpack_b, unpack_c, restrict_same, prolong_shared, prolong_int,
                   per_neighbor)
+
+
+# ---------------------------------------------------------------- MHD ghost prolongation
+def mhd_prolong_plan(tree, nx, ng, cc=None, fc=None):
+    """ProlongateBoundaries for MHD on every block (bvals/bvals_refine.cpp:93-229): per
+    coarser neighbour, restriction of the same-level ghosts (u, faces), ConservedToPrimitive
+    on the coarse buffer over the prolongation range widened by one cell towards every
+    existing neighbour (ApplyPhysicalBoundariesOnCoarseLevel, :336-384), prolongation of
+    the primitives and faces, then bcc and u on the fine cells (ProlongateGhostCells).
+
+    Returns (slots, order).  order: one dict of single-step lists per block and coarser
+    neighbour, in the reference's order.  slots[k]: the same lists for the k-th coarser
+    neighbour of every block, batched; running the slots in turn gives the reference's
+    per-block sequence (a floor in ConservedToPrimitive makes the sequence matter)."""
+    cc = cc or cc_plan(tree, nx, ng)
+    fc = fc or fc_plan(tree, nx, ng)
+    order, slots = [], []
+    m = 0
+    for g in range(len(tree.leaves)):
+        k = 0
+        for nb in tree.neighbors(g):
+            if nb.rel >= 0:
+                continue
+            rs_c, p = cc.per_neighbor[m]
+            rs_f, sh, it = fc.per_neighbor[m]
+            m += 1
+            r = [(p["si"], p["ei"]), (p["sj"], p["ej"]), (p["sk"], p["ek"])]
+            for d in range(3):
+                if nb.ox[d] == 0:
+                    e = [0, 0, 0]
+                    e[d] = -1
+                    fm = 1 if tree.find(g, tuple(e)) is not None else 0
+                    e[d] = 1
+                    fp = 1 if tree.find(g, tuple(e)) is not None else 0
+                else:
+                    fm = fp = 1
+                r[d] = (r[d][0] - fm, r[d][1] + fp)
+            step = dict(restrict_cc=list(rs_c), restrict_fc=[list(x) for x in rs_f],
+                        c2p=[_rng_item(g, g, r)], prolong_cc=[p], prolong_fc=[[x] for x in sh],
+                        prolong_int=[it], p2c=[dict(it)])
+            order.append(step)
+            if len(slots) <= k:
+                slots.append(dict(restrict_cc=[], restrict_fc=[[], [], []], c2p=[], prolong_cc=[],
+                                  prolong_fc=[[], [], []], prolong_int=[], p2c=[]))
+            sl = slots[k]
+            for key in ("restrict_cc", "c2p", "prolong_cc", "prolong_int", "p2c"):
+                sl[key] += step[key]
+            for c in range(3):
+                sl["restrict_fc"][c] += step["restrict_fc"][c]
+                sl["prolong_fc"][c] += step["prolong_fc"][c]
+            k += 1
+    return slots, order

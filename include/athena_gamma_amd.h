@@ -301,7 +301,10 @@ enum {
                                   component aux & 3, fb = buffer slot; items run in launch groups
                                   of increasing aux >> 2 (0..15) */
   AGX_REF_UNPACK_FC_C = 23,    /* the same into coarse-buffer faces */
-  AGX_REF_KINDS = 24
+  AGX_REF_CONS_TO_PRIM = 24,   /* CalculateCellCenteredField + ConservedToPrimitive on coarse-buffer
+                                  cells, as ApplyPhysicalBoundariesOnCoarseLevel calls them
+                                  (bvals_refine.cpp:336-384): cb = block, [si..ek] = coarse cells */
+  AGX_REF_KINDS = 25
 };
 typedef struct agx_ref_list agx_ref_list;
 /* Checks every item against the geometry and the pool sizes (nb_fine blocks on the fine
@@ -379,6 +382,11 @@ int agx_ref_count_fine_edges(const int32_t nblevel[27], int32_t edge_flag[12], i
  * w and bcc (nvar 5 with the adiabatic EOS, gamma; 4 with iso != 0) (PRIM_TO_CONS) */
 int agx_ref_prim_to_cons(const agx_ref_list *l, const double *w, const double *b1f, const double *b2f,
                          const double *b3f, double *bcc, double *u, double gamma, int32_t iso, void *stream);
+/* coarse conserved variables and faces -> coarse bcc and primitives on the item ranges,
+ * floors written back to u (CONS_TO_PRIM; nvar 5 with the adiabatic EOS, 4 with iso) */
+int agx_ref_cons_to_prim(const agx_ref_list *l, double *u, const double *b1f, const double *b2f, const double *b3f,
+                         double *w, double *bcc, double gamma, double dfloor, double pfloor, int32_t iso,
+                         void *stream);
 /* Loop limits of BoundaryValues::ProlongateBoundaries for one coarser neighbour
  * (bvals_refine.cpp:172-199) and of the face prolongation in ProlongateGhostCells.  ox[3]
  * is the neighbour's offset (NeighborIndexes::ox1..3); lx_odd[3] = loc.lx1..3 & 1;

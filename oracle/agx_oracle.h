@@ -136,6 +136,9 @@ void orc_ref_emf_to_coarser(const orc_ref_geom *g, const double *e1, const doubl
                             const orc_ref_item *it, int nit);
 void orc_ref_copy_cc(const orc_ref_geom *g, int nvar, const double *src, int src_coarse, double *dst,
                      int dst_coarse, const orc_ref_item *it, int nit);
+void orc_ref_cons_to_prim(const orc_ref_geom *g, double *u, const double *b1, const double *b2,
+                          const double *b3, double *w, double *bcc, double gamma, double dfl,
+                          double pfl, int iso, const orc_ref_item *it, int nit);
 void orc_ref_pack_fc(const orc_ref_geom *g, double *f1, double *f2, double *f3, int coarse, double *buf,
                      int64_t stride, const orc_ref_item *it, int nit);
 void orc_ref_unpack_fc(const orc_ref_geom *g, double *f1, double *f2, double *f3, int coarse, double *buf,

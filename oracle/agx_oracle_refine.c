@@ -21,6 +21,8 @@
  *                              reference's buffer order
  *   orc_ref_copy_cc            the pack + unpack of the cell-centred ghost exchanges
  *                              (bvals/cc/bvals_cc.cpp:201-479)
+ *   orc_ref_cons_to_prim       the coarse-buffer ConservedToPrimitive of
+ *                              ApplyPhysicalBoundariesOnCoarseLevel (bvals/bvals_refine.cpp:336-384)
  *   orc_ref_pack_fc, _unpack_fc the pack and unpack of the face-field ghost exchanges
  *                              (bvals/fc/bvals_fc.cpp:344-977)
  *   orc_ref_emf_same_level     LoadFluxBoundaryBufferSameLevel (flux_correction_fc.cpp:51-313)
@@ -499,6 +501,59 @@ void orc_ref_copy_cc(const orc_ref_geom *g, int nvar, const double *src, int src
           for (int i = it[m].si; i <= it[m].ei; ++i)
             dst[ix(d3, d2, d1, nvar, it[m].cb, n, it[m].fi2 + k - it[m].sk, it[m].fi1 + j - it[m].sj,
                    it[m].fid + i - it[m].si)] = src[ix(s3, s2, s1, nvar, it[m].fb, n, k, j, i)];
+  }
+}
+
+/* Field::CalculateCellCenteredField (uniform branch, field/field.cpp:112-176) over the
+ * item's coarse cells, then EquationOfState::ConservedToPrimitive (eos/adiabatic_mhd.cpp:
+ * 41-90; isothermal_mhd.cpp:40-73 without energy) on them, floors written back to u:
+ * the call in BoundaryValues::ApplyPhysicalBoundariesOnCoarseLevel (bvals/
+ * bvals_refine.cpp:336-384).  Arrays are coarse buffers; items in list order. */
+void orc_ref_cons_to_prim(const orc_ref_geom *g, double *u, const double *b1, const double *b2,
+                          const double *b3, double *w, double *bcc, double gamma, double dfl,
+                          double pfl, int iso, const orc_ref_item *it, int nit) {
+  rgeo r = mkgeo(g);
+  const int nh = iso ? 4 : 5;
+  const double gm1 = gamma - 1.0;
+  const int64_t S = (int64_t)r.c3 * r.c2 * r.c1;
+  for (int m = 0; m < nit; ++m) {
+    const int cb = it[m].cb;
+    for (int k = it[m].sk; k <= it[m].ek; ++k)
+      for (int j = it[m].sj; j <= it[m].ej; ++j)
+        for (int i = it[m].si; i <= it[m].ei; ++i) {
+          bcc[ix(r.c3, r.c2, r.c1, 3, cb, 0, k, j, i)] = 0.5 * b1[ix(r.c3, r.c2, r.c1 + 1, 1, cb, 0, k, j, i)] +
+                                                         0.5 * b1[ix(r.c3, r.c2, r.c1 + 1, 1, cb, 0, k, j, i + 1)];
+          bcc[ix(r.c3, r.c2, r.c1, 3, cb, 1, k, j, i)] = 0.5 * b2[ix(r.c3, r.c2 + 1, r.c1, 1, cb, 0, k, j, i)] +
+                                                         0.5 * b2[ix(r.c3, r.c2 + 1, r.c1, 1, cb, 0, k, j + 1, i)];
+          bcc[ix(r.c3, r.c2, r.c1, 3, cb, 2, k, j, i)] = 0.5 * b3[ix(r.c3 + 1, r.c2, r.c1, 1, cb, 0, k, j, i)] +
+                                                         0.5 * b3[ix(r.c3 + 1, r.c2, r.c1, 1, cb, 0, k + 1, j, i)];
+        }
+    for (int k = it[m].sk; k <= it[m].ek; ++k)
+      for (int j = it[m].sj; j <= it[m].ej; ++j)
+        for (int i = it[m].si; i <= it[m].ei; ++i) {
+          const int64_t o = ix(r.c3, r.c2, r.c1, nh, cb, 0, k, j, i);
+          double u_d = u[o];
+          const double u_m1 = u[o + S], u_m2 = u[o + 2 * S], u_m3 = u[o + 3 * S];
+          u_d = (u_d > dfl) ? u_d : dfl;
+          u[o] = u_d;
+          w[o] = u_d;
+          const double di = 1.0 / u_d;
+          w[o + S] = u_m1 * di;
+          w[o + 2 * S] = u_m2 * di;
+          w[o + 3 * S] = u_m3 * di;
+          if (iso) continue;
+          const double c1 = bcc[ix(r.c3, r.c2, r.c1, 3, cb, 0, k, j, i)];
+          const double c2 = bcc[ix(r.c3, r.c2, r.c1, 3, cb, 1, k, j, i)];
+          const double c3 = bcc[ix(r.c3, r.c2, r.c1, 3, cb, 2, k, j, i)];
+          const double pb = 0.5 * (c1 * c1 + c2 * c2 + c3 * c3);
+          const double e_k = 0.5 * di * (u_m1 * u_m1 + u_m2 * u_m2 + u_m3 * u_m3);
+          double u_e = u[o + 4 * S];
+          double w_p = gm1 * (u_e - e_k - pb);
+          u_e = (w_p > pfl) ? u_e : ((pfl / gm1) + e_k + pb);
+          w_p = (w_p > pfl) ? w_p : pfl;
+          u[o + 4 * S] = u_e;
+          w[o + 4 * S] = w_p;
+        }
   }
 }
 

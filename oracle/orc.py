@@ -315,6 +315,8 @@ def _ref_lib():
         L.orc_ref_prim_to_cons.argtypes = [G, V, V, V, V, V, V, ctypes.c_double, ctypes.c_int, IT, ctypes.c_int]
         IP = ctypes.POINTER(ctypes.c_int)
         L.orc_ref_prolong_ranges.argtypes = [G, IP, IP, IP, IP]
+        L.orc_ref_cons_to_prim.argtypes = [G, V, V, V, V, V, V, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, IT, ctypes.c_int]
         L.orc_ref_pack_fc.argtypes = [G, V, V, V, ctypes.c_int, V, ctypes.c_int64, IT, ctypes.c_int]
         L.orc_ref_unpack_fc.argtypes = [G, V, V, V, ctypes.c_int, V, ctypes.c_int64, IT, ctypes.c_int]
         L.orc_ref_copy_cc.argtypes = [G, ctypes.c_int, V, ctypes.c_int, V, ctypes.c_int, IT, ctypes.c_int]
@@ -423,3 +425,9 @@ def ref_unpack_fc(g, f, coarse, buf, stride, items):
     a, n = _items(items)
     _ref_lib().orc_ref_unpack_fc(ctypes.byref(g), _ptr(f[0]), _ptr(f[1]), _ptr(f[2]), int(coarse), _ptr(buf), stride,
                                  a, n)
+
+
+def ref_cons_to_prim(g, u, b, w, bcc, gamma, dfloor, pfloor, iso, items):
+    a, n = _items(items)
+    _ref_lib().orc_ref_cons_to_prim(ctypes.byref(g), _ptr(u), _ptr(b[0]), _ptr(b[1]), _ptr(b[2]), _ptr(w), _ptr(bcc),
+                                    gamma, dfloor, pfloor, int(iso), a, n)

@@ -220,3 +220,71 @@ def _fc_stencil_has_nan(cfc, p, c):
         if np.isnan(sub).any():
             return True
     return False
+
+
+# ---------------------------------------------------------------- MHD pipeline
+GAMMA, DFLOOR, PFLOOR = 5.0 / 3.0, 1e-4, 1e-3
+
+
+def mhd_state(t, nx, ng, seed, floors=True):
+    """random primitive state and faces; u from them; a few cells below the floors"""
+    rng = np.random.default_rng(seed)
+    nb = len(t.leaves)
+    n, nc = nx + 2 * ng, nx // 2 + 2 * ng
+    b = [rng.normal(0.0, 0.5, s) for s in ((nb, n, n, n + 1), (nb, n, n + 1, n), (nb, n + 1, n, n))]
+    bcc = np.stack([0.5 * (b[0][..., :-1] + b[0][..., 1:]), 0.5 * (b[1][:, :, :-1] + b[1][:, :, 1:]),
+                    0.5 * (b[2][:, :-1] + b[2][:, 1:])], axis=1)
+    rho = rng.uniform(0.5, 1.5, (nb, n, n, n))
+    v = rng.normal(0.0, 0.3, (nb, 3, n, n, n))
+    p = rng.uniform(0.3, 1.0, (nb, n, n, n))
+    u = np.empty((nb, 5, n, n, n))
+    u[:, 0] = rho
+    u[:, 1:4] = rho[:, None] * v
+    u[:, 4] = p / (GAMMA - 1.0) + 0.5 * rho * (v ** 2).sum(1) + 0.5 * (bcc ** 2).sum(1)
+    if floors:
+        sel = rng.random((nb, n, n, n)) < 0.02
+        u[:, 0][sel] = 1e-6
+        sel = rng.random((nb, n, n, n)) < 0.02
+        u[:, 4][sel] -= p[sel] / (GAMMA - 1.0) * 1.5
+    arrs = dict(u=u, b=b, w=rng.standard_normal((nb, 5, n, n, n)), bcc=rng.standard_normal((nb, 3, n, n, n)),
+                cu=rng.standard_normal((nb, 5, nc, nc, nc)), cw=rng.standard_normal((nb, 5, nc, nc, nc)),
+                cbcc=rng.standard_normal((nb, 3, nc, nc, nc)),
+                cb=[rng.standard_normal(s) for s in ((nb, nc, nc, nc + 1), (nb, nc, nc + 1, nc), (nb, nc + 1, nc, nc))])
+    return {k: (np.ascontiguousarray(v) if not isinstance(v, list) else [np.ascontiguousarray(x) for x in v])
+            for k, v in arrs.items()}
+
+
+def run_oracle_mhd(orc, t, A, nx, ng, ftab, ctab, order="reference"):
+    """ghost exchange of u (cells) and b (faces) across levels, then ProlongateBoundaries
+    for MHD, on copies of the arrays in A"""
+    smr = load_smr()
+    cc, fc = smr.cc_plan(t, nx, ng), smr.fc_plan(t, nx, ng)
+    slots, seq = smr.mhd_prolong_plan(t, nx, ng, cc, fc)
+    A = {k: (v.copy() if not isinstance(v, list) else [x.copy() for x in v]) for k, v in A.items()}
+    g = geom(orc, nx, ng)
+    I = lambda ds: [orc.RefItem(**d) for d in ds]   # noqa: E731
+    u, cu = A["u"], A["cu"]
+    orc.ref_copy_cc(g, 5, u, 0, u, 0, I(cc.copy_same))
+    orc.ref_restrict_cc(g, 5, u, ftab, cu, I(cc.restrict_send))
+    orc.ref_copy_cc(g, 5, cu, 1, u, 0, I(cc.copy_from_finer))
+    orc.ref_copy_cc(g, 5, u, 0, cu, 1, I(cc.copy_to_finer))
+    A["b"], A["cb"] = run_oracle_fc(orc, t, A["b"], A["cb"], nx, ng, ftab, ctab, plan=_exchange_only(fc),
+                                    order=order)
+    for st in (seq if order == "reference" else slots):
+        orc.ref_restrict_cc(g, 5, u, ftab, cu, I(st["restrict_cc"]))
+        for c in range(3):
+            orc.ref_restrict_fc(g, c, A["b"][c], ftab, A["cb"][c], I(st["restrict_fc"][c]))
+        orc.ref_cons_to_prim(g, cu, A["cb"], A["cw"], A["cbcc"], GAMMA, DFLOOR, PFLOOR, 0, I(st["c2p"]))
+        orc.ref_prolong_cc(g, 5, A["cw"], ctab, A["w"], ftab, I(st["prolong_cc"]))
+        for c in range(3):
+            orc.ref_prolong_fc_shared(g, c, A["cb"][c], ctab, A["b"][c], ftab, I(st["prolong_fc"][c]))
+        orc.ref_prolong_fc_internal(g, A["b"][0], A["b"][1], A["b"][2], ftab, I(st["prolong_int"]))
+        orc.ref_prim_to_cons(g, A["w"], A["b"][0], A["b"][1], A["b"][2], A["bcc"], u, GAMMA, 0, I(st["p2c"]))
+    return A
+
+
+def _exchange_only(fc):
+    """the face plan without its prolongation step (which the MHD pipeline runs itself)"""
+    import dataclasses
+    return dataclasses.replace(fc, restrict_same=[[], [], []], prolong_shared=[[], [], []], prolong_int=[],
+                               per_neighbor=[])

@@ -1,0 +1,90 @@
+"""Pins the MHD ghost prolongation across refinement levels (athena-gamma_amd/smr.py:
+mhd_prolong_plan; bvals/bvals_refine.cpp:93-229, 336-384): per coarser neighbour,
+restriction of same-level ghosts, ConservedToPrimitive on the coarse buffer
+(oracle/agx_oracle_refine.c: orc_ref_cons_to_prim), prolongation of primitives and
+faces, then bcc and u.
+
+  - the per-slot batched order equals the reference's per-block neighbour order bit for
+    bit, on random states with 2% of cells below the density or pressure floor (a floor
+    makes the order of the conversions matter);
+  - a uniform MHD state comes back in every ghost cell: faces and primitives exactly,
+    conserved variables to round-off (the u -> w -> u round trip);
+  - orc_ref_cons_to_prim: u -> w inverts the primitive-to-conserved map to round-off,
+    floors raise density and pressure to the floor values exactly.
+"""
+import numpy as np
+import pytest
+
+from oracle import orc
+from tests import ghost_cases as gc
+
+NX, NG = 8, 2
+
+
+@pytest.mark.parametrize("refine", [((1, 1, 1),), ((1, 1, 1), (2, 1, 1))])
+def test_slot_order_equals_reference_order(refine):
+    t = gc.tree(refine)
+    ftab, ctab = gc.coords(t, NX, NG), gc.coords(t, NX, NG, coarse=True)
+    A = gc.mhd_state(t, NX, NG, seed=4)
+    a = gc.run_oracle_mhd(orc, t, A, NX, NG, ftab, ctab, order="reference")
+    b = gc.run_oracle_mhd(orc, t, A, NX, NG, ftab, ctab, order="batched")
+    for k in a:
+        x, y = (a[k], b[k]) if isinstance(a[k], list) else ([a[k]], [b[k]])
+        for xx, yy in zip(x, y):
+            assert np.array_equal(xx, yy), k
+
+
+def test_uniform_state_in_every_ghost():
+    t = gc.tree(((1, 1, 1),))
+    ftab, ctab = gc.coords(t, NX, NG), gc.coords(t, NX, NG, coarse=True)
+    A = gc.mhd_state(t, NX, NG, seed=1, floors=False)
+    w0 = np.array([1.25, 0.5, -0.25, 0.125, 0.75])
+    bf = (0.5, -0.75, 0.25)
+    for c in range(3):
+        A["b"][c][:] = bf[c]
+    u0 = np.empty(5)
+    u0[0] = w0[0]
+    u0[1:4] = w0[0] * w0[1:4]
+    u0[4] = w0[4] / (gc.GAMMA - 1.0) + 0.5 * w0[0] * (w0[1:4] ** 2).sum() + 0.5 * sum(x * x for x in bf)
+    A["u"][:] = u0[None, :, None, None, None]
+    gc.poison_exchanged_ghosts(t, A["u"], NX, NG)
+    gc.poison_face_ghosts(t, A["b"], NX, NG)
+    out = gc.run_oracle_mhd(orc, t, A, NX, NG, ftab, ctab)
+    for c in range(3):
+        assert np.all(out["b"][c] == bf[c])
+    assert np.allclose(out["u"], u0[None, :, None, None, None], rtol=1e-14, atol=0)
+    # the fine ghost cells the prolongation wrote hold the state's primitives
+    changed = np.any(out["w"] != A["w"], axis=1)
+    assert changed.sum() > 1000
+    assert np.allclose(out["w"].transpose(1, 0, 2, 3, 4)[:, changed], w0[:, None], rtol=1e-14, atol=0)
+
+
+def test_cons_to_prim_inverts_and_floors():
+    t = gc.tree(((1, 1, 1),))
+    g = gc.geom(orc, NX, NG)
+    A = gc.mhd_state(t, NX, NG, seed=2, floors=False)
+    nb, nc = len(t.leaves), NX // 2 + 2 * NG
+    rng = np.random.default_rng(3)
+    w = np.empty((nb, 5, nc, nc, nc))
+    w[:, 0] = rng.uniform(0.5, 1.5, w[:, 0].shape)
+    w[:, 1:4] = rng.normal(0, 0.3, w[:, 1:4].shape)
+    w[:, 4] = rng.uniform(0.3, 1.0, w[:, 4].shape)
+    cb = A["cb"]
+    bcc = np.stack([0.5 * (cb[0][..., :-1] + cb[0][..., 1:]), 0.5 * (cb[1][:, :, :-1] + cb[1][:, :, 1:]),
+                    0.5 * (cb[2][:, :-1] + cb[2][:, 1:])], axis=1)
+    u = np.empty_like(w)
+    u[:, 0] = w[:, 0]
+    u[:, 1:4] = w[:, 0:1] * w[:, 1:4]
+    u[:, 4] = w[:, 4] / (gc.GAMMA - 1.0) + 0.5 * w[:, 0] * (w[:, 1:4] ** 2).sum(1) + 0.5 * (bcc ** 2).sum(1)
+    u[3, 0, 3, 3, 3] = 1e-7                       # below the density floor
+    u[4, 4, 2, 3, 4] = 0.0                         # negative pressure
+    item = [orc.RefItem(fb=0, cb=b, si=0, ei=nc - 1, sj=0, ej=nc - 1, sk=0, ek=nc - 1) for b in range(nb)]
+    cw, cbcc = np.zeros_like(w), np.zeros((nb, 3, nc, nc, nc))
+    uu = u.copy()
+    orc.ref_cons_to_prim(g, uu, cb, cw, cbcc, gc.GAMMA, gc.DFLOOR, gc.PFLOOR, 0, item)
+    assert np.array_equal(cbcc, bcc)
+    mask = np.ones(w[:, 0].shape, bool)
+    mask[3, 3, 3, 3] = mask[4, 2, 3, 4] = False
+    assert np.allclose(cw.transpose(1, 0, 2, 3, 4)[:, mask], w.transpose(1, 0, 2, 3, 4)[:, mask], rtol=1e-12, atol=1e-13)
+    assert cw[3, 0, 3, 3, 3] == gc.DFLOOR and uu[3, 0, 3, 3, 3] == gc.DFLOOR
+    assert cw[4, 4, 2, 3, 4] == gc.PFLOOR and uu[4, 4, 2, 3, 4] > 0.0
