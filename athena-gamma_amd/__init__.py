@@ -12,6 +12,7 @@ The directory name is not a Python identifier; load it with importlib, e.g.
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -532,3 +533,138 @@ def device_coords(tab):
     returns (coords, arrays) - keep the arrays alive while the coords are in use."""
     arrs = [DeviceArray(tab[k]) for k in ("x1v", "x2v", "x3v", "dx1f", "dx2f", "dx3f")]
     return RefCoords(*[a.ptr for a in arrs]), arrs
+
+
+def _load_smr():
+    import importlib.util
+    name = "agx_smr"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                                     "smr.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+class SMRBoundary:
+    """The boundary step of a static-refinement MHD mesh on the device, with every work
+    list built once from the block tree (smr.py):
+      exchange_cells(u, cu)           CellCenteredBoundaryVariable across levels
+      exchange_faces(b, cb)           FaceCenteredBoundaryVariable across levels
+      prolongate(u, b, w, bcc, cu, cb, cw, cbcc)
+                                      BoundaryValues::ProlongateBoundaries (MHD), per
+                                      slot of coarser neighbours
+      correct_emf(e1, e2, e3)         SendFluxCorrection / ReceiveFluxCorrection
+    Arguments are device pointers (integers) to the reference's dense layouts, block-
+    major: u, w (nb, nhydro, n, n, n), bcc (nb, 3, n, n, n), b = (x1f, x2f, x3f)
+    (nb, n, n, n+1) ..., coarse buffers cu, cw, cbcc, cb with nc = nx/2 + 2 ng, edges
+    e1 (nb, n+1, n+1, n) ...  Cubic blocks of nx^3 cells, ng ghosts, the adiabatic EOS.
+    Launches are asynchronous on the default stream; sync() waits."""
+
+    def __init__(self, tree, nx, ng, gamma, dfloor, pfloor, box=(1.0, 1.0, 1.0)):
+        smr = _load_smr()
+        self.tree, self.nx, self.ng = tree, nx, ng
+        self.gamma, self.dfloor, self.pfloor = gamma, dfloor, pfloor
+        nb = self.nb = len(tree.leaves)
+        self.geom = RefGeom(nx, nx, nx, ng, ng, 0)
+        self._keep = []
+        self.fc, a1 = device_coords(smr.block_coords(tree, nx, ng, box=box))
+        self.cc, a2 = device_coords(smr.block_coords(tree, nx, ng, coarse=True, box=box))
+        self._keep += a1 + a2
+        self._lists = []
+        L = self._list
+        ccp, fcp = smr.cc_plan(tree, nx, ng), smr.fc_plan(tree, nx, ng)
+        slots, _ = smr.mhd_prolong_plan(tree, nx, ng, ccp, fcp)
+        self.c_same, self.c_send = L("copy_cc_bb", ccp.copy_same), L("restrict_cc", ccp.restrict_send)
+        self.c_from_finer, self.c_to_finer = L("copy_cc_cb", ccp.copy_from_finer), L("copy_cc_bc", ccp.copy_to_finer)
+        self.f_send = [L(f"restrict_fc{c + 1}", fcp.restrict_send[c]) for c in range(3)]
+        self.nslot_b = len(fcp.pack_b)
+        nbuf = max(1, len(fcp.pack_b) + len(fcp.pack_c))
+        self.f_stride = max([1] + [_box_size(d) for d in fcp.pack_b + fcp.pack_c])
+        self.f_pack_b, self.f_pack_c = L("pack_fc_b", fcp.pack_b), L("pack_fc_c", fcp.pack_c)
+        self.f_unpack_b = L("unpack_fc_b", fcp.unpack_b, nbuf)
+        self.f_unpack_c = L("unpack_fc_c", fcp.unpack_c, nbuf)
+        self.f_buf = DeviceArray(np.zeros((nbuf, self.f_stride)))
+        self.slots = []
+        for st in slots:
+            self.slots.append(dict(
+                rcc=L("restrict_cc", st["restrict_cc"]),
+                rfc=[L(f"restrict_fc{c + 1}", st["restrict_fc"][c]) for c in range(3)],
+                c2p=L("cons_to_prim", st["c2p"]), pcc=L("prolong_cc", st["prolong_cc"]),
+                pfc=[L(f"prolong_fc{c + 1}", st["prolong_fc"][c]) for c in range(3)],
+                pint=L("prolong_fc_int", st["prolong_int"]), p2c=L("prim_to_cons", st["p2c"])))
+        q = smr.emf_plan(tree)
+        g = self.geom
+        self.e_ss = max([1] + [ref_emf_count_same_level(g, d["fid"]) for d in q.same_level])
+        self.e_sc = max([1] + [ref_emf_count(g, d["fid"]) for d in q.to_coarser])
+        self.e_buf_s = DeviceArray(np.zeros((max(1, len(q.same_level)), self.e_ss)))
+        self.e_buf_c = DeviceArray(np.zeros((max(1, len(q.to_coarser)), self.e_sc)))
+        self.e_same, self.e_coarse = L("emf_same_level", q.same_level), L("emf_to_coarser", q.to_coarser)
+        self.e_set_same = L("emf_set_same", q.set_same, max(1, len(q.same_level)))
+        self.e_clear = L("emf_clear", q.clear)
+        self.e_set_finer = L("emf_set_finer", q.set_finer, max(1, len(q.to_coarser)))
+        self.e_avg = L("emf_average", q.average)
+        self._keep += [self.f_buf, self.e_buf_s, self.e_buf_c]
+
+    def _list(self, kind, items, nbf=None):
+        lst = RefList(self.geom, kind, [RefItem(**d) for d in items], self.nb if nbf is None else nbf, self.nb)
+        self._lists.append(lst)
+        return lst
+
+    def exchange_cells(self, u, cu, nvar=5):
+        self.c_same.copy_cc(nvar, u, u)
+        self.c_send.restrict_cc(nvar, u, self.fc, cu)
+        self.c_from_finer.copy_cc(nvar, cu, u)
+        self.c_to_finer.copy_cc(nvar, u, cu)
+
+    def exchange_faces(self, b, cb):
+        for c in range(3):
+            self.f_send[c].restrict_fc(b[c], self.fc, cb[c])
+        p = self.f_buf.ptr
+        self.f_pack_b.pack_fc(*b, p, self.f_stride)
+        self.f_pack_c.pack_fc(*cb, p + self.nslot_b * self.f_stride * 8, self.f_stride)
+        self.f_unpack_b.unpack_fc(*b, p, self.f_stride)
+        self.f_unpack_c.unpack_fc(*cb, p, self.f_stride)
+
+    def prolongate(self, u, b, w, bcc, cu, cb, cw, cbcc):
+        for s in self.slots:
+            s["rcc"].restrict_cc(5, u, self.fc, cu)
+            for c in range(3):
+                s["rfc"][c].restrict_fc(b[c], self.fc, cb[c])
+            s["c2p"].cons_to_prim(cu, *cb, cw, cbcc, self.gamma, self.dfloor, self.pfloor)
+            s["pcc"].prolong_cc(5, cw, self.cc, w, self.fc)
+            for c in range(3):
+                s["pfc"][c].prolong_fc_shared(cb[c], self.cc, b[c], self.fc)
+            s["pint"].prolong_fc_internal(*b, self.fc)
+            s["p2c"].prim_to_cons(w, *b, bcc, u, self.gamma)
+
+    def correct_emf(self, e1, e2, e3):
+        self.e_same.emf_same_level(e1, e2, e3, self.e_buf_s.ptr, self.e_ss)
+        self.e_coarse.emf_to_coarser(e1, e2, e3, self.fc, self.e_buf_c.ptr, self.e_sc)
+        self.e_set_same.emf_set(e1, e2, e3, self.e_buf_s.ptr, self.e_ss)
+        self.e_clear.emf_clear(e1, e2, e3)
+        self.e_set_finer.emf_set(e1, e2, e3, self.e_buf_c.ptr, self.e_sc)
+        self.e_avg.emf_average(e1, e2, e3)
+
+    def boundary_step(self, u, b, w, bcc, cu, cb, cw, cbcc):
+        """ghost zones of u and b everywhere: both exchanges, then the prolongation"""
+        self.exchange_cells(u, cu)
+        self.exchange_faces(b, cb)
+        self.prolongate(u, b, w, bcc, cu, cb, cw, cbcc)
+
+    @staticmethod
+    def sync():
+        RefList.sync()
+
+    def close(self):
+        for x in self._lists:
+            x.close()
+        for a in self._keep:
+            a.close()
+        self._lists, self._keep = [], []
+
+
+def _box_size(d):
+    return (d["ei"] - d["si"] + 1) * (d["ej"] - d["sj"] + 1) * (d["ek"] - d["sk"] + 1)

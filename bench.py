@@ -482,6 +482,24 @@ def smr_bench(agx, reps):
         x.close()
     for a in keep:
         a.close()
+    # the whole MHD boundary step through the package's driver: exchanges of u and b
+    # across levels, then ProlongateBoundaries per slot of coarser neighbours
+    sb = agx.SMRBoundary(t, nx, ng, 5.0 / 3.0, 1e-8, 1e-8)
+    A = dict(u=rs.random((nb, 5, n, n, n)) + 1.0, w=np.zeros((nb, 5, n, n, n)), bcc=np.zeros((nb, 3, n, n, n)),
+             cu=np.ones((nb, 5, nc, nc, nc)), cw=np.zeros((nb, 5, nc, nc, nc)), cbcc=np.zeros((nb, 3, nc, nc, nc)))
+    D = {k: agx.DeviceArray(v) for k, v in A.items()}
+    b = [agx.DeviceArray(rs.standard_normal(s_)) for s_ in ((nb, n, n, n + 1), (nb, n, n + 1, n), (nb, n + 1, n, n))]
+    cb = [agx.DeviceArray(np.zeros(s_)) for s_ in ((nb, nc, nc, nc + 1), (nb, nc, nc + 1, nc), (nb, nc + 1, nc, nc))]
+    P = {k: v.ptr for k, v in D.items()}
+    ms = timed(lambda: sb.boundary_step(P["u"], [x.ptr for x in b], P["w"], P["bcc"], P["cu"], [x.ptr for x in cb],
+                                        P["cw"], P["cbcc"]))
+    out["mhd_boundary_step"] = {"ms": ms, "slots": len(sb.slots),
+                                "what": "u (5 variables) and b exchanged across levels, then ProlongateBoundaries "
+                                        "(restriction, cons->prim on the coarse buffer, prolongation, prim->cons)"}
+    log(f"[smr] MHD boundary step (exchanges + prolongation, {len(sb.slots)} slots), {nb} blocks: {ms:.3f} ms")
+    sb.close()
+    for a in list(D.values()) + b + cb:
+        a.close()
     return out
 
 

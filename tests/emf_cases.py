@@ -72,20 +72,22 @@ def linear_emfs(t, nx, ng, tab):
 
 def edge_table(t, E, nx, ng):
     """{(level, arr, gx, gy, gz): [values]} over every edge of every block's active
-    region: start points in the level's global index space, wrapped periodically."""
+    region: start points in the level's global index space, wrapped in periodic
+    directions."""
     tab = {}
     lo, hi = ng, ng + nx
     for b, (lev, *lx) in enumerate(t.leaves):
-        N = [(NRB[d] * nx) << lev for d in range(3)]
+        N = [(t.nrb[d] * nx) << lev for d in range(3)]
         for a in range(3):
             rng = [range(lo, hi + 1 if d != a else hi) for d in range(3)]   # i, j, k ranges
+            wrap = [N[d] if t.periodic[d] else 1 << 30 for d in range(3)]   # positions, periodic wrap only
             for k in rng[2]:
-                gz = (lx[2] * nx + k - ng) % N[2]
+                gz = (lx[2] * nx + k - ng) % wrap[2]
                 for j in rng[1]:
-                    gy = (lx[1] * nx + j - ng) % N[1]
+                    gy = (lx[1] * nx + j - ng) % wrap[1]
                     row = E[a][b, k, j]
                     for i in rng[0]:
-                        gx = (lx[0] * nx + i - ng) % N[0]
+                        gx = (lx[0] * nx + i - ng) % wrap[0]
                         tab.setdefault((lev, a, gx, gy, gz), []).append(float(row[i]))
     return tab
 

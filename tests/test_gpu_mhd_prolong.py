@@ -81,3 +81,30 @@ def test_mhd_ghosts_match_oracle(agx, refine):
         for xx, yy in zip(x, y):
             print(f"  {k}: max|gpu - oracle| {np.abs(xx - yy).max():.3e}", flush=True)
             assert np.array_equal(xx, yy), k
+
+
+def test_smr_boundary_driver(agx):
+    """the package's SMRBoundary (every list built once from the tree) against the
+    oracle: boundary_step on an MHD state with floors, correct_emf on random EMFs"""
+    from tests import emf_cases as ec
+    smr = load_smr()
+    t = gc.tree(((1, 1, 1), (2, 1, 1)))
+    ftab, ctab = gc.coords(t, NX, NG), gc.coords(t, NX, NG, coarse=True)
+    A = gc.mhd_state(t, NX, NG, seed=12)
+    ref = gc.run_oracle_mhd(orc, t, A, NX, NG, ftab, ctab, order="reference")
+    E = ec.random_emfs(len(t.leaves), NX, NG, seed=13)
+    eref, _ = ec.run_oracle(orc, t, E, NX, NG, smr.block_coords(t, NX, NG))
+    sb = agx.SMRBoundary(t, NX, NG, gc.GAMMA, gc.DFLOOR, gc.PFLOOR)
+    D = {k: ([agx.DeviceArray(x) for x in v] if isinstance(v, list) else agx.DeviceArray(v)) for k, v in A.items()}
+    de = [agx.DeviceArray(x) for x in E]
+    P = {k: ([x.ptr for x in v] if isinstance(v, list) else v.ptr) for k, v in D.items()}
+    sb.boundary_step(P["u"], P["b"], P["w"], P["bcc"], P["cu"], P["cb"], P["cw"], P["cbcc"])
+    sb.correct_emf(*[x.ptr for x in de])
+    sb.sync()
+    for k in ref:
+        x, y = (ref[k], D[k]) if isinstance(ref[k], list) else ([ref[k]], [D[k]])
+        for xx, yy in zip(x, y):
+            assert np.array_equal(xx, yy.get()), k
+    for a in range(3):
+        assert np.array_equal(eref[a], de[a].get()), f"e{a + 1}"
+    sb.close()

@@ -137,3 +137,17 @@ def test_neighbor_lists_reciprocal(mesh):
     # coarse block 0 (root (1,0,0)) has its x1- and x1+ faces on the refined root block
     quads = sorted((n.fid, n.fi1, n.fi2) for n in t.neighbors(0) if n.rel == 1 and n.kind == "face")
     assert quads == sorted((f, a, b) for f in (0, 1) for a in (0, 1) for b in (0, 1))
+
+
+def test_random_emfs_consistent_with_physical_boundaries():
+    """4^3 root blocks, outflow boundaries, two refined root blocks side by side (fine
+    blocks of two refined regions meet): the same invariants; faces on the domain
+    boundary are not averaged (AverageFluxBoundary skips them)"""
+    from tests import ghost_cases as gc
+    t = gc.tree(((1, 1, 1), (2, 1, 1)))
+    tab = load_smr().block_coords(t, NX, NG)
+    E = ec.random_emfs(len(t.leaves), NX, NG, seed=21)
+    out, _ = ec.run_oracle(orc, t, E, NX, NG, tab)
+    spread, err, ncmp = ec.check_consistency(t, out, NX, NG)
+    print(f"spread {spread:.2e}, coarse vs fine {err:.2e} over {ncmp} edges", flush=True)
+    assert ncmp > 1000 and spread < 4e-15 and err < 4e-15
