@@ -9,8 +9,10 @@ Also reported: V(1,1) cycles/s of the multigrid (FP64, mixed, multipole boundari
 the 256^3 32-blob self-gravity problem; the coupled 256^3 MHD + multigrid self-gravity
 cycle (VL2+PLM+HLLD+CT, lognormal density) in FP64 and in the mixed form, with the
 mixed-vs-FP64 ratio and differences ("self_gravity"); the dominant kernel's roofline;
-the refinement operators' bandwidth on 512 blocks of 32^3 ("refinement"); and the CPU
-oracle timed on a bounded sample of the MHD workload.
+the refinement operators' bandwidth on 512 blocks of 32^3 ("refinement"), and whole-mesh
+SMR steps on a 120-block two-level mesh (cell ghost exchange, EMF correction, the MHD
+boundary step; "refinement.smr"); and the CPU oracle timed on a bounded sample of the
+MHD workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
