@@ -41,7 +41,8 @@ EXPORTS = [
     "agx_upload_blocks", "agx_download_blocks", "agx_set_mg_fuse",
     "agx_ref_emf_count_same_level", "agx_ref_emf_same_level", "agx_ref_emf_set", "agx_ref_emf_clear",
     "agx_ref_emf_average", "agx_ref_count_fine_edges", "agx_ref_copy_cc", "agx_ref_pack_fc", "agx_ref_unpack_fc",
-    "agx_ref_cons_to_prim",
+    "agx_ref_cons_to_prim", "agx_stream_create", "agx_stream_destroy", "agx_graph_begin", "agx_graph_end",
+    "agx_graph_launch", "agx_graph_destroy",
 ]
 
 
@@ -173,6 +174,12 @@ def lib():
     L.agx_ref_emf_average.argtypes = [P, P, P, P, P]
     L.agx_ref_count_fine_edges.argtypes = [IP, IP, IP]
     L.agx_ref_copy_cc.argtypes = [P, ctypes.c_int32, P, P, P]
+    L.agx_stream_create.argtypes = [ctypes.POINTER(P)]
+    L.agx_stream_destroy.argtypes = [P]
+    L.agx_graph_begin.argtypes = [P]
+    L.agx_graph_end.argtypes = [P, ctypes.POINTER(P)]
+    L.agx_graph_launch.argtypes = [P, P]
+    L.agx_graph_destroy.argtypes = [P]
     L.agx_ref_pack_fc.argtypes = [P, P, P, P, P, ctypes.c_int64, P]
     L.agx_ref_cons_to_prim.argtypes = [P, P, P, P, P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int32, P]
@@ -395,8 +402,9 @@ class RefList:
     operator calls take device pointers (integers) to arrays in the layouts of
     include/athena_gamma_amd.h and run on the default stream; sync() waits."""
 
-    def __init__(self, geom, kind, items, nb_fine, nb_coarse):
+    def __init__(self, geom, kind, items, nb_fine, nb_coarse, stream=None):
         self.geom = geom
+        self.stream = stream   # hipStream_t from agx_stream_create, or None (default stream)
         self.kind = REF_KINDS[kind] if isinstance(kind, str) else int(kind)
         arr = (RefItem * max(len(items), 1))(*items)
         self.h = ctypes.c_void_p()
@@ -415,58 +423,58 @@ class RefList:
             pass
 
     def restrict_cc(self, nvar, fine, fc, coarse):
-        _check(lib().agx_ref_restrict_cc(self.h, nvar, fine, fc, coarse, None), "agx_ref_restrict_cc")
+        _check(lib().agx_ref_restrict_cc(self.h, nvar, fine, fc, coarse, self.stream), "agx_ref_restrict_cc")
 
     def restrict_fc(self, fine, fc, coarse):
-        _check(lib().agx_ref_restrict_fc(self.h, fine, fc, coarse, None), "agx_ref_restrict_fc")
+        _check(lib().agx_ref_restrict_fc(self.h, fine, fc, coarse, self.stream), "agx_ref_restrict_fc")
 
     def prolong_cc(self, nvar, coarse, cc, fine, fc):
-        _check(lib().agx_ref_prolong_cc(self.h, nvar, coarse, cc, fine, fc, None), "agx_ref_prolong_cc")
+        _check(lib().agx_ref_prolong_cc(self.h, nvar, coarse, cc, fine, fc, self.stream), "agx_ref_prolong_cc")
 
     def prolong_fc_shared(self, coarse, cc, fine, fc):
-        _check(lib().agx_ref_prolong_fc_shared(self.h, coarse, cc, fine, fc, None), "agx_ref_prolong_fc_shared")
+        _check(lib().agx_ref_prolong_fc_shared(self.h, coarse, cc, fine, fc, self.stream), "agx_ref_prolong_fc_shared")
 
     def prolong_fc_internal(self, b1, b2, b3, fc):
-        _check(lib().agx_ref_prolong_fc_internal(self.h, b1, b2, b3, fc, None), "agx_ref_prolong_fc_internal")
+        _check(lib().agx_ref_prolong_fc_internal(self.h, b1, b2, b3, fc, self.stream), "agx_ref_prolong_fc_internal")
 
     def flux_from_finer(self, nvar, f1, f2, f3, fc, c1, c2, c3):
-        _check(lib().agx_ref_flux_from_finer(self.h, nvar, f1, f2, f3, fc, c1, c2, c3, None),
+        _check(lib().agx_ref_flux_from_finer(self.h, nvar, f1, f2, f3, fc, c1, c2, c3, self.stream),
                "agx_ref_flux_from_finer")
 
     def prim_to_cons(self, w, b1, b2, b3, bcc, u, gamma, iso=False):
-        _check(lib().agx_ref_prim_to_cons(self.h, w, b1, b2, b3, bcc, u, gamma, int(iso), None), "agx_ref_prim_to_cons")
+        _check(lib().agx_ref_prim_to_cons(self.h, w, b1, b2, b3, bcc, u, gamma, int(iso), self.stream), "agx_ref_prim_to_cons")
 
     def emf_to_coarser(self, e1, e2, e3, fc, buf, stride):
-        _check(lib().agx_ref_emf_to_coarser(self.h, e1, e2, e3, fc, buf, stride, None), "agx_ref_emf_to_coarser")
+        _check(lib().agx_ref_emf_to_coarser(self.h, e1, e2, e3, fc, buf, stride, self.stream), "agx_ref_emf_to_coarser")
 
     def copy_cc(self, nvar, src, dst):
-        _check(lib().agx_ref_copy_cc(self.h, nvar, src, dst, None), "agx_ref_copy_cc")
+        _check(lib().agx_ref_copy_cc(self.h, nvar, src, dst, self.stream), "agx_ref_copy_cc")
 
     def cons_to_prim(self, u, b1, b2, b3, w, bcc, gamma, dfloor, pfloor, iso=False):
-        _check(lib().agx_ref_cons_to_prim(self.h, u, b1, b2, b3, w, bcc, gamma, dfloor, pfloor, int(iso), None),
+        _check(lib().agx_ref_cons_to_prim(self.h, u, b1, b2, b3, w, bcc, gamma, dfloor, pfloor, int(iso), self.stream),
                "agx_ref_cons_to_prim")
 
     def pack_fc(self, f1, f2, f3, buf, stride):
-        _check(lib().agx_ref_pack_fc(self.h, f1, f2, f3, buf, stride, None), "agx_ref_pack_fc")
+        _check(lib().agx_ref_pack_fc(self.h, f1, f2, f3, buf, stride, self.stream), "agx_ref_pack_fc")
 
     def unpack_fc(self, f1, f2, f3, buf, stride):
-        _check(lib().agx_ref_unpack_fc(self.h, f1, f2, f3, buf, stride, None), "agx_ref_unpack_fc")
+        _check(lib().agx_ref_unpack_fc(self.h, f1, f2, f3, buf, stride, self.stream), "agx_ref_unpack_fc")
 
     def emf_same_level(self, e1, e2, e3, buf, stride):
-        _check(lib().agx_ref_emf_same_level(self.h, e1, e2, e3, buf, stride, None), "agx_ref_emf_same_level")
+        _check(lib().agx_ref_emf_same_level(self.h, e1, e2, e3, buf, stride, self.stream), "agx_ref_emf_same_level")
 
     def emf_set(self, e1, e2, e3, buf, stride):
-        _check(lib().agx_ref_emf_set(self.h, e1, e2, e3, buf, stride, None), "agx_ref_emf_set")
+        _check(lib().agx_ref_emf_set(self.h, e1, e2, e3, buf, stride, self.stream), "agx_ref_emf_set")
 
     def emf_clear(self, e1, e2, e3):
-        _check(lib().agx_ref_emf_clear(self.h, e1, e2, e3, None), "agx_ref_emf_clear")
+        _check(lib().agx_ref_emf_clear(self.h, e1, e2, e3, self.stream), "agx_ref_emf_clear")
 
     def emf_average(self, e1, e2, e3):
-        _check(lib().agx_ref_emf_average(self.h, e1, e2, e3, None), "agx_ref_emf_average")
+        _check(lib().agx_ref_emf_average(self.h, e1, e2, e3, self.stream), "agx_ref_emf_average")
 
     @staticmethod
-    def sync():
-        _check(lib().agx_ref_sync(None), "agx_ref_sync")
+    def sync(stream=None):
+        _check(lib().agx_ref_sync(stream), "agx_ref_sync")
 
 
 def ref_emf_count(geom, fid):
@@ -509,6 +517,14 @@ class DeviceArray:
     @property
     def ptr(self):
         return self.p.value
+
+    def set(self, host):
+        """copy a host array of the same shape and dtype in (synchronous)"""
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        if host.shape != self.shape:
+            raise AgxError(f"DeviceArray.set: shape {host.shape} != {self.shape}")
+        _check(lib().agx_device_copy(self.p, host.ctypes.data_as(ctypes.c_void_p), self.nbytes, 0),
+               "agx_device_copy")
 
     def get(self):
         out = np.empty(self.shape, self.dtype)
@@ -561,10 +577,18 @@ class SMRBoundary:
     major: u, w (nb, nhydro, n, n, n), bcc (nb, 3, n, n, n), b = (x1f, x2f, x3f)
     (nb, n, n, n+1) ..., coarse buffers cu, cw, cbcc, cb with nc = nx/2 + 2 ng, edges
     e1 (nb, n+1, n+1, n) ...  Cubic blocks of nx^3 cells, ng ghosts, the adiabatic EOS.
-    Launches are asynchronous on the default stream; sync() waits."""
+    Launches are asynchronous on the driver's own stream (agx_stream_create) or, with
+    own_stream=False, on the default stream; sync() waits.  capture(...) records one
+    boundary step (and optionally the EMF correction) with fixed arrays into a hipGraph;
+    replay() launches it (about 100 kernels in one launch)."""
 
-    def __init__(self, tree, nx, ng, gamma, dfloor, pfloor, box=(1.0, 1.0, 1.0)):
+    def __init__(self, tree, nx, ng, gamma, dfloor, pfloor, box=(1.0, 1.0, 1.0), own_stream=True):
         smr = _load_smr()
+        self.stream, self.graph = None, None
+        if own_stream:
+            st = ctypes.c_void_p()
+            _check(lib().agx_stream_create(ctypes.byref(st)), "agx_stream_create")
+            self.stream = st.value
         self.tree, self.nx, self.ng = tree, nx, ng
         self.gamma, self.dfloor, self.pfloor = gamma, dfloor, pfloor
         nb = self.nb = len(tree.leaves)
@@ -609,7 +633,8 @@ class SMRBoundary:
         self._keep += [self.f_buf, self.e_buf_s, self.e_buf_c]
 
     def _list(self, kind, items, nbf=None):
-        lst = RefList(self.geom, kind, [RefItem(**d) for d in items], self.nb if nbf is None else nbf, self.nb)
+        lst = RefList(self.geom, kind, [RefItem(**d) for d in items], self.nb if nbf is None else nbf, self.nb,
+                      stream=self.stream)
         self._lists.append(lst)
         return lst
 
@@ -654,16 +679,42 @@ class SMRBoundary:
         self.exchange_faces(b, cb)
         self.prolongate(u, b, w, bcc, cu, cb, cw, cbcc)
 
-    @staticmethod
-    def sync():
-        RefList.sync()
+    def capture(self, u, b, w, bcc, cu, cb, cw, cbcc, emf=None):
+        """record boundary_step (then correct_emf(*emf) if given) into a graph"""
+        if not self.stream:
+            raise AgxError("capture needs the driver's own stream")
+        self.sync()
+        _check(lib().agx_graph_begin(self.stream), "agx_graph_begin")
+        try:
+            self.boundary_step(u, b, w, bcc, cu, cb, cw, cbcc)
+            if emf is not None:
+                self.correct_emf(*emf)
+        finally:
+            ex = ctypes.c_void_p()
+            rc = lib().agx_graph_end(self.stream, ctypes.byref(ex))
+        _check(rc, "agx_graph_end")
+        if self.graph:
+            lib().agx_graph_destroy(self.graph)
+        self.graph = ex.value
+
+    def replay(self):
+        _check(lib().agx_graph_launch(self.graph, self.stream), "agx_graph_launch")
+
+    def sync(self):
+        RefList.sync(self.stream)
 
     def close(self):
+        if self.graph:
+            lib().agx_graph_destroy(self.graph)
+            self.graph = None
         for x in self._lists:
             x.close()
         for a in self._keep:
             a.close()
         self._lists, self._keep = [], []
+        if self.stream:
+            lib().agx_stream_destroy(self.stream)
+            self.stream = None
 
 
 def _box_size(d):

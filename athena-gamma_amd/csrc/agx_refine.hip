@@ -1299,6 +1299,56 @@ int agx_ref_emf_average(const agx_ref_list *l, double *e1, double *e2, double *e
   return launch_status("agx_ref_emf_average");
 }
 
+int agx_stream_create(void **out) {
+  if (!out) { set_error("agx_stream_create: null argument"); return -1; }
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);   // default priority
+  if (e != hipSuccess) return hip_fail(e, "agx_stream_create");
+  *out = st;
+  return 0;
+}
+
+int agx_stream_destroy(void *stream) {
+  if (!stream) return 0;
+  hipError_t e = hipStreamDestroy((hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "agx_stream_destroy");
+}
+
+int agx_graph_begin(void *stream) {
+  if (!stream) { set_error("agx_graph_begin: capture needs a stream from agx_stream_create"); return -1; }
+  hipError_t e = hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal);
+  return e == hipSuccess ? 0 : hip_fail(e, "agx_graph_begin");
+}
+
+int agx_graph_end(void *stream, void **exec_out) {
+  if (!stream || !exec_out) { set_error("agx_graph_end: null argument"); return -1; }
+  *exec_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &graph);
+  if (e != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    return hip_fail(e, "agx_graph_end (capture)");
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return hip_fail(e, "agx_graph_end (instantiate)");
+  *exec_out = exec;
+  return 0;
+}
+
+int agx_graph_launch(void *exec, void *stream) {
+  if (!exec) { set_error("agx_graph_launch: null graph"); return -1; }
+  hipError_t e = hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "agx_graph_launch");
+}
+
+int agx_graph_destroy(void *exec) {
+  if (!exec) return 0;
+  hipError_t e = hipGraphExecDestroy((hipGraphExec_t)exec);
+  return e == hipSuccess ? 0 : hip_fail(e, "agx_graph_destroy");
+}
+
 int agx_ref_sync(void *stream) {
   hipError_t e = hipStreamSynchronize((hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "agx_ref_sync");

@@ -412,13 +412,13 @@ def smr_bench(agx, reps):
     keep = k1 + k2 + [u, cu] + e
     out = {"workload": f"{nb} blocks of {nx}^3 (ng {ng}) on two levels, 4^3 root blocks with the central 2^3 refined"}
 
-    def timed(run):
+    def timed(run, sync=agx.RefList.sync):
         run()
-        agx.RefList.sync()
+        sync()
         t0 = time.perf_counter()
         for _ in range(reps):
             run()
-        agx.RefList.sync()
+        sync()
         return (time.perf_counter() - t0) * 1e3 / reps
 
     # cell-centred ghost exchange
@@ -493,12 +493,15 @@ def smr_bench(agx, reps):
     b = [agx.DeviceArray(rs.standard_normal(s_)) for s_ in ((nb, n, n, n + 1), (nb, n, n + 1, n), (nb, n + 1, n, n))]
     cb = [agx.DeviceArray(np.zeros(s_)) for s_ in ((nb, nc, nc, nc + 1), (nb, nc, nc + 1, nc), (nb, nc + 1, nc, nc))]
     P = {k: v.ptr for k, v in D.items()}
-    ms = timed(lambda: sb.boundary_step(P["u"], [x.ptr for x in b], P["w"], P["bcc"], P["cu"], [x.ptr for x in cb],
-                                        P["cw"], P["cbcc"]))
-    out["mhd_boundary_step"] = {"ms": ms, "slots": len(sb.slots),
+    args_ = (P["u"], [x.ptr for x in b], P["w"], P["bcc"], P["cu"], [x.ptr for x in cb], P["cw"], P["cbcc"])
+    ms = timed(lambda: sb.boundary_step(*args_), sb.sync)
+    sb.capture(*args_)
+    msg = timed(sb.replay, sb.sync)
+    out["mhd_boundary_step"] = {"ms": ms, "ms_graph_replay": msg, "slots": len(sb.slots),
                                 "what": "u (5 variables) and b exchanged across levels, then ProlongateBoundaries "
                                         "(restriction, cons->prim on the coarse buffer, prolongation, prim->cons)"}
-    log(f"[smr] MHD boundary step (exchanges + prolongation, {len(sb.slots)} slots), {nb} blocks: {ms:.3f} ms")
+    log(f"[smr] MHD boundary step (exchanges + prolongation, {len(sb.slots)} slots), {nb} blocks: {ms:.3f} ms "
+        f"eager, {msg:.3f} ms as one hipGraph replay")
     sb.close()
     for a in list(D.values()) + b + cb:
         a.close()

@@ -401,6 +401,18 @@ int agx_ref_prolong_ranges(const agx_ref_geom *g, const int32_t ox[3], const int
                            const int32_t nblevel[27], int32_t out[12]);
 /* waits for `stream` */
 int agx_ref_sync(void *stream);
+/* Streams and graphs for callers without a runtime of their own (no counterpart in the
+ * reference).  agx_stream_create makes a non-blocking stream of default priority.
+ * agx_graph_begin / agx_graph_end capture the launches issued on that stream in
+ * between (the refinement entry points with that stream) into an executable graph;
+ * agx_graph_launch replays it on a stream.  A graph keeps the pointers and work lists
+ * it was captured with: replay only while they are alive and unchanged. */
+int agx_stream_create(void **out);
+int agx_stream_destroy(void *stream);
+int agx_graph_begin(void *stream);
+int agx_graph_end(void *stream, void **exec_out);
+int agx_graph_launch(void *exec, void *stream);
+int agx_graph_destroy(void *exec);
 
 /* ---- device memory for callers without an allocator of their own (no counterpart in the
  * reference; the refinement operators take device pointers).  Plain device allocations

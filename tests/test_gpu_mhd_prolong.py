@@ -108,3 +108,53 @@ def test_smr_boundary_driver(agx):
     for a in range(3):
         assert np.array_equal(eref[a], de[a].get()), f"e{a + 1}"
     sb.close()
+
+
+def test_smr_boundary_graph_replay(agx):
+    """the boundary step and EMF correction captured in a hipGraph: three replays, each
+    from the same initial arrays, with eager work and synchronisations in between, each
+    identical bit for bit to the eager run"""
+    from tests import emf_cases as ec
+    t = gc.tree(((1, 1, 1), (2, 1, 1)))
+    A = gc.mhd_state(t, NX, NG, seed=14)
+    E = ec.random_emfs(len(t.leaves), NX, NG, seed=15)
+    sb = agx.SMRBoundary(t, NX, NG, gc.GAMMA, gc.DFLOOR, gc.PFLOOR)
+    D = {k: ([agx.DeviceArray(x) for x in v] if isinstance(v, list) else agx.DeviceArray(v)) for k, v in A.items()}
+    de = [agx.DeviceArray(x) for x in E]
+    P = {k: ([x.ptr for x in v] if isinstance(v, list) else v.ptr) for k, v in D.items()}
+    args = (P["u"], P["b"], P["w"], P["bcc"], P["cu"], P["cb"], P["cw"], P["cbcc"])
+    emf = [x.ptr for x in de]
+
+    def reset():
+        for k, v in A.items():
+            if isinstance(v, list):
+                for d, h in zip(D[k], v):
+                    d.set(h)
+            else:
+                D[k].set(v)
+        for d, h in zip(de, E):
+            d.set(h)
+
+    def snapshot():
+        out = {k: ([x.get() for x in v] if isinstance(v, list) else [v.get()]) for k, v in D.items()}
+        out["emf"] = [x.get() for x in de]
+        return out
+    sb.boundary_step(*args)
+    sb.correct_emf(*emf)
+    sb.sync()
+    eager = snapshot()
+    reset()
+    sb.capture(*args, emf=emf)
+    other = [agx.DeviceArray(x) for x in ec.random_emfs(len(t.leaves), NX, NG, seed=16)]
+    for rep in range(3):
+        reset()
+        sb.replay()
+        sb.sync()
+        got = snapshot()
+        for k in eager:
+            for xx, yy in zip(eager[k], got[k]):
+                assert np.array_equal(xx, yy), (rep, k)
+        print(f"replay {rep}: identical to the eager run", flush=True)
+        sb.correct_emf(*[x.ptr for x in other])     # eager work between replays
+        sb.sync()
+    sb.close()
