@@ -55,14 +55,14 @@ class Desc(ctypes.Structure):
             "x1min", "x1max", "x2min", "x2max", "x3min", "x3max", "gamma", "iso_cs",
             "cfl", "dfloor", "pfloor", "four_pi_G", "mg_threshold")] + \
         [(n, ctypes.c_int32) for n in ("mg_bc", "mporder", "mp_auto_origin")] + \
-        [("mp_origin", ctypes.c_double * 3)]
+        [("mp_origin", ctypes.c_double * 3), ("mp_nodipole", ctypes.c_int32)]
 
 
 def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)),
               nghost=2, xorder=2, rsolver=0, gamma=1.666666666666667, cfl=0.3,
               self_gravity=0, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=10,
               precision=0, eos=0, iso_cs=1.0, mg_bc=0, mporder=4, mp_auto_origin=1,
-              mp_origin=(0.0, 0.0, 0.0)):
+              mp_origin=(0.0, 0.0, 0.0), mp_nodipole=0):
     bnx = bnx or nx
     d = Desc()
     d.nx1, d.nx2, d.nx3 = nx
@@ -76,6 +76,7 @@ def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)
     d.four_pi_G, d.mg_threshold = four_pi_G, mg_threshold
     d.mg_bc, d.mporder, d.mp_auto_origin = mg_bc, mporder, mp_auto_origin
     d.mp_origin[:] = mp_origin
+    d.mp_nodipole = mp_nodipole
     return d
 
 

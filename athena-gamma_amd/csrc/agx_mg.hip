@@ -1085,7 +1085,8 @@ k_mb_moments(const double *__restrict__ src, MPGeom G, const double *__restrict_
 // (MultigridDriver::CalculateCenterOfMass, multigrid_driver.cpp:1984-2015); otherwise
 // the coefficients scaled as ScaleMultipoleCoefficients (multigrid_driver.cpp:1930-1981).
 __global__ void __launch_bounds__(BS)
-k_mb_moments_final(const double *__restrict__ part, int nwg, int order, int com, double *__restrict__ mp) {
+k_mb_moments_final(const double *__restrict__ part, int nwg, int order, int com, int nodipole,
+                   double *__restrict__ mp) {
   __shared__ double sh[BS];
   const int nm = com ? 4 : (order == 4 ? 25 : 9);
   double acc[NMOM];
@@ -1116,6 +1117,7 @@ k_mb_moments_final(const double *__restrict__ part, int nwg, int order, int com,
                c43 = -0.0625 * 17.5 / PI_, c44 = -0.25 * 35.0 / PI_;
   const double sc[NMOM] = {c0, c1, c1, c1, c2a, c2a, c2, c2a, c2a, c33, c32, c31, c30,
                            c31, c32, c33, c44, c43, c42, c41, c40, c41, c42, c43, c44};
+  if (nodipole) acc[1] = acc[2] = acc[3] = 0.0;   // nodipole: those sums are never formed
   for (int q = 0; q < NMOM; ++q) mp[3 + q] = (q < nm) ? acc[q] * sc[q] : 0.0;
 }
 
@@ -1325,7 +1327,7 @@ static void mp_coefficients(agx_mesh *m) {
   if (m->d.mp_auto_origin) {
     hipLaunchKernelGGL(k_mb_moments, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.src, G, m->mpdev,
                        m->d.mporder, 1, part);
-    hipLaunchKernelGGL(k_mb_moments_final, dim3(1), dim3(BS), 0, m->stream, part, (int)nb, m->d.mporder, 1,
+    hipLaunchKernelGGL(k_mb_moments_final, dim3(1), dim3(BS), 0, m->stream, part, (int)nb, m->d.mporder, 1, 0,
                        m->mpdev);
   } else {
     (void)hipMemcpyAsync(m->mpdev, m->d.mp_origin, 3 * sizeof(double), hipMemcpyHostToDevice, m->stream);
@@ -1333,7 +1335,7 @@ static void mp_coefficients(agx_mesh *m) {
   hipLaunchKernelGGL(k_mb_moments, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.src, G, m->mpdev, m->d.mporder,
                      0, part);
   hipLaunchKernelGGL(k_mb_moments_final, dim3(1), dim3(BS), 0, m->stream, part, (int)nb, m->d.mporder, 0,
-                     m->mpdev);
+                     m->d.mp_nodipole ? 1 : 0, m->mpdev);
   // the boundary values of every level for this solve (the ghost fills then only gather)
   for (int l = 0; l < m->ntotal; ++l) {
     MGLevelB &B = m->levb[l];

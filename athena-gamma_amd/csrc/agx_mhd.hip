@@ -1740,6 +1740,9 @@ int check_desc(const agx_mesh_desc *d) {
   if (d->mg_bc != 0 && d->mg_bc != 1) {
     set_error("mg_bc %d not supported (0: periodic, 1: multipole)", d->mg_bc); return -14;
   }
+  if (d->mg_bc == 1 && d->mp_auto_origin && d->mp_nodipole) {
+    set_error("auto_mporigin and nodipole cannot be used together (mg_gravity.cpp:106-113)"); return -14;
+  }
   if (d->mg_bc == 1 && d->mporder != 2 && d->mporder != 4) {
     set_error("mporder must be 2 or 4 (mg_gravity.cpp:97-105)"); return -14;
   }

@@ -64,6 +64,9 @@ typedef struct agx_mesh_desc {
   int32_t mporder;              /* <gravity>/mporder: 2 or 4 (multipole only) */
   int32_t mp_auto_origin;       /* <gravity>/auto_mporigin: 1 = expand about the centre of mass */
   double mp_origin[3];          /* <gravity>/mporigin_x1..x3 (auto_mporigin = 0) */
+  int32_t mp_nodipole;          /* <gravity>/nodipole: 1 = leave out the dipole moments
+                                   (multigrid.cpp:1072, 1200); needs mp_auto_origin = 0
+                                   (mg_gravity.cpp:106-113) */
 } agx_mesh_desc;
 
 enum { AGX_RSOLVER_HLLD = 0, AGX_RSOLVER_HLLE = 1, AGX_RSOLVER_ROE = 2 };

@@ -1994,7 +1994,8 @@ static void mp_center_of_mass(orc_ctx *c) {
 
 /* MultigridDriver::CalculateMultipoleCoefficients + ScaleMultipoleCoefficients
  * (multigrid_driver.cpp:1903-1981) with Multigrid::CalculateMultipoleCoefficients
- * (multigrid.cpp:1054-1264), no "nodipole". */
+ * (multigrid.cpp:1054-1264); with "nodipole" the dipole sums are not formed
+ * (multigrid.cpp:1072-1130, 1200-1226) and stay zero. */
 static void mp_coefficients(orc_ctx *c) {
   const int np = (c->d.mporder == 4) ? 25 : 9;
   double acc[25] = {0.0};
@@ -2017,9 +2018,11 @@ static void mp_coefficients(orc_ctx *c) {
           double r2 = x2 + y2 + z2;
           double s = m->src[l][MIDX(m, l, k, j, i)] * vol;
           mm[0] += s;
-          mm[1] += s * y;
-          mm[2] += s * z;
-          mm[3] += s * x;
+          if (!c->d.mp_nodipole) {
+            mm[1] += s * y;
+            mm[2] += s * z;
+            mm[3] += s * x;
+          }
           mm[4] += s * xy;
           mm[5] += s * yz;
           mm[6] += s * (3.0 * z2 - r2);

@@ -46,6 +46,7 @@ typedef struct orc_desc {
   int32_t mporder;              /* 2 or 4 (multipole) */
   int32_t mp_auto_origin;       /* 1: expansion about the centre of mass */
   double mp_origin[3];          /* expansion origin when mp_auto_origin = 0 */
+  int32_t mp_nodipole;          /* nodipole: dipole moments left out */
 } orc_desc;
 
 typedef struct orc_ctx orc_ctx;

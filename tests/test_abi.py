@@ -31,9 +31,11 @@ def test_desc_layout(agx):
     from oracle.orc import Desc as ODesc
     assert ctypes.sizeof(agx.Desc) == ctypes.sizeof(ODesc)
     assert [f[0] for f in agx.Desc._fields_] == [f[0] for f in ODesc._fields_]
-    # 14 int32 (56 B), 13 doubles, 3 int32 (+4 B padding), 3 doubles, naturally aligned
+    # 14 int32 (56 B), 13 doubles, 3 int32 (+4 B padding), 3 doubles, 1 int32 (+4 B
+    # padding), naturally aligned
     assert agx.Desc.mp_origin.offset == 56 + 13 * 8 + 12 + 4
-    assert ctypes.sizeof(agx.Desc) == 56 + 13 * 8 + 16 + 3 * 8
+    assert agx.Desc.mp_nodipole.offset == 56 + 13 * 8 + 16 + 3 * 8
+    assert ctypes.sizeof(agx.Desc) == 56 + 13 * 8 + 16 + 3 * 8 + 8
 
 
 @pytest.mark.parametrize("bad,code", [
@@ -46,6 +48,7 @@ def test_desc_layout(agx):
     (dict(eos=1, rsolver=2), -9),                       # Roe has no isothermal form here
     (dict(eos=1, iso_cs=0.0), -9),
     (dict(precision=3), -10),
+    (dict(self_gravity=1, mg_bc=1, mp_auto_origin=1, mp_nodipole=1), -14),   # mg_gravity.cpp:106-113
 ])
 def test_invalid_desc_rejected(agx, bad, code):
     kw = dict(nx=(32, 16, 16), bnx=(16, 16, 16))

@@ -33,16 +33,16 @@ def run_pair(agx, n, bnx, rho, nx=None, **kw):
     return o, m
 
 
-@pytest.mark.parametrize("order,auto", [(4, 1), (2, 1), (4, 0)])
-def test_fixed_cycles_match(agx, order, auto):
+@pytest.mark.parametrize("order,auto,nodip", [(4, 1, 0), (2, 1, 0), (4, 0, 0), (4, 0, 1), (2, 0, 1)])
+def test_fixed_cycles_match(agx, order, auto, nodip):
     n = 32
     rho = gaussian(n, 0.05, c=(0.08, -0.05, 0.04))
-    o, m = run_pair(agx, n, (16, 16, 16), rho, mporder=order, mp_auto_origin=auto,
+    o, m = run_pair(agx, n, (16, 16, 16), rho, mporder=order, mp_auto_origin=auto, mp_nodipole=nodip,
                     mp_origin=(0.01, 0.02, -0.03), mg_threshold=-1.0, mg_niter=3)
     a, b = m.get("phi"), o.get("phi")
     err = np.abs(a - b).max() / np.abs(b).max()
     (nm, im), (no, io) = m.mg_last(), o.mg_last()
-    print(f"order {order} auto {auto}: phi rel diff {err:.3e}; defect {nm:.6e} vs {no:.6e}", flush=True)
+    print(f"order {order} auto {auto} nodipole {nodip}: phi rel diff {err:.3e}; defect {nm:.6e} vs {no:.6e}", flush=True)
     assert err <= 1e-9
     assert im == io == 3
     assert abs(nm - no) <= 1e-6 * no

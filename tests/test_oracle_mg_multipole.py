@@ -156,3 +156,23 @@ def test_fas_fixed_cycles_reduce_defect():
         norms.append(o.mg_last()[0])
     print("defect norms", norms)
     assert all(b < 0.5 * a for a, b in zip(norms, norms[1:]))
+
+
+def test_nodipole_drops_only_the_dipole(offset_blob):
+    """<gravity>/nodipole (multigrid.cpp:1072, 1200): the dipole coefficients are zero,
+    every other coefficient is the same bits as without the option, and the boundary
+    potential loses exactly the dipole term"""
+    n, rho = offset_blob
+    kw = dict(mporder=4, mp_auto_origin=0, mp_origin=(0.0, 0.0, 0.0), mg_threshold=-1.0, mg_niter=1)
+    o1, _ = solved(n, 16, rho, **kw)
+    o2, _ = solved(n, 16, rho, mp_nodipole=1, **kw)
+    _, c1 = o1.mg_multipole()
+    _, c2 = o2.mg_multipole()
+    assert np.all(np.asarray(c2[1:4]) == 0.0) and np.abs(np.asarray(c1[1:4])).max() > 1e-6
+    assert np.array_equal(np.asarray(c1[:1]), np.asarray(c2[:1])) and np.array_equal(np.asarray(c1[4:]),
+                                                                                    np.asarray(c2[4:]))
+    p = np.array([0.9, -0.7, 1.1])
+    r2 = float(p @ p)
+    # dipole term of the expansion: (coef1 y + coef2 z + coef3 x) / r^3
+    dip = (c1[1] * p[1] + c1[2] * p[2] + c1[3] * p[0]) * r2 ** -1.5
+    assert o1.mp_phis(*p) - o2.mp_phis(*p) == pytest.approx(dip, rel=1e-12)
