@@ -42,7 +42,8 @@ EXPORTS = [
     "agx_ref_emf_count_same_level", "agx_ref_emf_same_level", "agx_ref_emf_set", "agx_ref_emf_clear",
     "agx_ref_emf_average", "agx_ref_count_fine_edges", "agx_ref_copy_cc", "agx_ref_pack_fc", "agx_ref_unpack_fc",
     "agx_ref_cons_to_prim", "agx_stream_create", "agx_stream_destroy", "agx_graph_begin", "agx_graph_end",
-    "agx_graph_launch", "agx_graph_destroy",
+    "agx_graph_launch", "agx_graph_destroy", "agx_ref_restrict_all", "agx_ref_prolong_all",
+    "agx_ref_internal_to_cons",
 ]
 
 
@@ -175,6 +176,9 @@ def lib():
     L.agx_ref_emf_average.argtypes = [P, P, P, P, P]
     L.agx_ref_count_fine_edges.argtypes = [IP, IP, IP]
     L.agx_ref_copy_cc.argtypes = [P, ctypes.c_int32, P, P, P]
+    L.agx_ref_restrict_all.argtypes = [P, P, P, P, ctypes.c_int32, P, P, P, P, C, P, P, P, P, P]
+    L.agx_ref_prolong_all.argtypes = [P, P, P, P, ctypes.c_int32, P, P, P, P, C, P, P, P, P, C, P]
+    L.agx_ref_internal_to_cons.argtypes = [P, P, P, P, C, P, P, P, ctypes.c_double, ctypes.c_int32, P]
     L.agx_stream_create.argtypes = [ctypes.POINTER(P)]
     L.agx_stream_destroy.argtypes = [P]
     L.agx_graph_begin.argtypes = [P]
@@ -654,7 +658,18 @@ class SMRBoundary:
         self.f_unpack_b.unpack_fc(*b, p, self.f_stride)
         self.f_unpack_c.unpack_fc(*cb, p, self.f_stride)
 
-    def prolongate(self, u, b, w, bcc, cu, cb, cw, cbcc):
+    def prolongate(self, u, b, w, bcc, cu, cb, cw, cbcc, fused=True):
+        if fused:   # four launches per slot
+            L = lib()
+            for s in self.slots:
+                _check(L.agx_ref_restrict_all(s["rcc"].h, *[x.h for x in s["rfc"]], 5, u, *b, self.fc, cu, *cb,
+                                              self.stream), "agx_ref_restrict_all")
+                s["c2p"].cons_to_prim(cu, *cb, cw, cbcc, self.gamma, self.dfloor, self.pfloor)
+                _check(L.agx_ref_prolong_all(s["pcc"].h, *[x.h for x in s["pfc"]], 5, cw, *cb, self.cc, w, *b,
+                                             self.fc, self.stream), "agx_ref_prolong_all")
+                _check(L.agx_ref_internal_to_cons(s["pint"].h, *b, self.fc, w, bcc, u, self.gamma, 0, self.stream),
+                       "agx_ref_internal_to_cons")
+            return
         for s in self.slots:
             s["rcc"].restrict_cc(5, u, self.fc, cu)
             for c in range(3):
@@ -674,11 +689,11 @@ class SMRBoundary:
         self.e_set_finer.emf_set(e1, e2, e3, self.e_buf_c.ptr, self.e_sc)
         self.e_avg.emf_average(e1, e2, e3)
 
-    def boundary_step(self, u, b, w, bcc, cu, cb, cw, cbcc):
+    def boundary_step(self, u, b, w, bcc, cu, cb, cw, cbcc, fused=True):
         """ghost zones of u and b everywhere: both exchanges, then the prolongation"""
         self.exchange_cells(u, cu)
         self.exchange_faces(b, cb)
-        self.prolongate(u, b, w, bcc, cu, cb, cw, cbcc)
+        self.prolongate(u, b, w, bcc, cu, cb, cw, cbcc, fused)
 
     def capture(self, u, b, w, bcc, cu, cb, cw, cbcc, emf=None):
         """record boundary_step (then correct_emf(*emf) if given) into a graph"""

@@ -83,10 +83,9 @@ __device__ __forceinline__ unsigned stride_t() { return gridDim.x * RBS; }
 
 // ---------------------------------------------------------------- restriction
 // RestrictCellCenteredValues, mesh_refinement.cpp:106-136
-__global__ void __launch_bounds__(RBS)
-k_ref_restrict_cc(RG r, int nvar, const double *__restrict__ fine, agx_ref_coords fc,
-                  double *__restrict__ coarse, const agx_ref_item *__restrict__ items) {
-  const agx_ref_item it = items[blockIdx.y];
+__device__ __forceinline__ void
+b_ref_restrict_cc(RG r, int nvar, const double *__restrict__ fine, agx_ref_coords fc,
+                  double *__restrict__ coarse, const agx_ref_item &it) {
   const Range q = item_range(it);
   const unsigned w1 = q.w1, w2 = q.w2, per = w1 * w2 * q.w3, cnt = per * nvar;
   const double *d1 = fc.dx1f + (int64_t)it.fb * r.n1, *d2 = fc.dx2f + (int64_t)it.fb * r.n2,
@@ -116,12 +115,17 @@ k_ref_restrict_cc(RG r, int nvar, const double *__restrict__ fine, agx_ref_coord
   }
 }
 
+__global__ void __launch_bounds__(RBS)
+k_ref_restrict_cc(RG r, int nvar, const double *__restrict__ fine, agx_ref_coords fc,
+                  double *__restrict__ coarse, const agx_ref_item *__restrict__ items) {
+  b_ref_restrict_cc(r, nvar, fine, fc, coarse, items[blockIdx.y]);
+}
+
 // RestrictFieldX1/X2/X3, mesh_refinement.cpp:185-214, 244-279, 321-344
 template <int DIR>
-__global__ void __launch_bounds__(RBS)
-k_ref_restrict_fc(RG r, const double *__restrict__ fine, agx_ref_coords fc, double *__restrict__ coarse,
-                  const agx_ref_item *__restrict__ items) {
-  const agx_ref_item it = items[blockIdx.y];
+__device__ __forceinline__ void
+b_ref_restrict_fc(RG r, const double *__restrict__ fine, agx_ref_coords fc, double *__restrict__ coarse,
+                  const agx_ref_item &it) {
   const Range q = item_range(it);
   const unsigned cnt = (unsigned)q.w1 * q.w2 * q.w3;
   const int f3 = r.n3 + (DIR == 2), f2 = r.n2 + (DIR == 1), f1 = r.n1 + (DIR == 0);
@@ -155,12 +159,18 @@ k_ref_restrict_fc(RG r, const double *__restrict__ fine, agx_ref_coords fc, doub
   }
 }
 
+template <int DIR>
+__global__ void __launch_bounds__(RBS)
+k_ref_restrict_fc(RG r, const double *__restrict__ fine, agx_ref_coords fc, double *__restrict__ coarse,
+                  const agx_ref_item *__restrict__ items) {
+  b_ref_restrict_fc<DIR>(r, fine, fc, coarse, items[blockIdx.y]);
+}
+
 // ---------------------------------------------------------------- prolongation
 // ProlongateCellCenteredValues (3D), mesh_refinement.cpp:386-458
-__global__ void __launch_bounds__(RBS)
-k_ref_prolong_cc(RG r, int nvar, const double *__restrict__ coarse, agx_ref_coords cc, double *__restrict__ fine,
-                 agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
-  const agx_ref_item it = items[blockIdx.y];
+__device__ __forceinline__ void
+b_ref_prolong_cc(RG r, int nvar, const double *__restrict__ coarse, agx_ref_coords cc, double *__restrict__ fine,
+                 agx_ref_coords fc, const agx_ref_item &it) {
   const Range q = item_range(it);
   const unsigned w1 = q.w1, w2 = q.w2, per = w1 * w2 * q.w3, cnt = per * nvar;
   const double *X1 = cc.x1v + (int64_t)it.cb * r.c1, *X2 = cc.x2v + (int64_t)it.cb * r.c2,
@@ -199,12 +209,17 @@ k_ref_prolong_cc(RG r, int nvar, const double *__restrict__ coarse, agx_ref_coor
   }
 }
 
+__global__ void __launch_bounds__(RBS)
+k_ref_prolong_cc(RG r, int nvar, const double *__restrict__ coarse, agx_ref_coords cc, double *__restrict__ fine,
+                 agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
+  b_ref_prolong_cc(r, nvar, coarse, cc, fine, fc, items[blockIdx.y]);
+}
+
 // ProlongateSharedFieldX1/X2/X3 (3D), mesh_refinement.cpp:541-581, 626-674, 718-762
 template <int DIR>
-__global__ void __launch_bounds__(RBS)
-k_ref_prolong_fc_shared(RG r, const double *__restrict__ coarse, agx_ref_coords cc, double *__restrict__ fine,
-                        agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
-  const agx_ref_item it = items[blockIdx.y];
+__device__ __forceinline__ void
+b_ref_prolong_fc_shared(RG r, const double *__restrict__ coarse, agx_ref_coords cc, double *__restrict__ fine,
+                        agx_ref_coords fc, const agx_ref_item &it) {
   const Range q = item_range(it);
   const unsigned cnt = (unsigned)q.w1 * q.w2 * q.w3;
   const int f3 = r.n3 + (DIR == 2), f2 = r.n2 + (DIR == 1), f1 = r.n1 + (DIR == 0);
@@ -246,21 +261,21 @@ k_ref_prolong_fc_shared(RG r, const double *__restrict__ coarse, agx_ref_coords 
   }
 }
 
+template <int DIR>
+__global__ void __launch_bounds__(RBS)
+k_ref_prolong_fc_shared(RG r, const double *__restrict__ coarse, agx_ref_coords cc, double *__restrict__ fine,
+                        agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
+  b_ref_prolong_fc_shared<DIR>(r, coarse, cc, fine, fc, items[blockIdx.y]);
+}
+
 // ProlongateInternalField (3D), mesh_refinement.cpp:831-946: the 12 interior faces of
 // each 2x2x2 group of fine cells from its 24 outer faces; groups share no written face.
-__global__ void __launch_bounds__(RBS)
-k_ref_prolong_fc_internal(RG r, double *__restrict__ b1, double *__restrict__ b2, double *__restrict__ b3,
-                          agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
-  const agx_ref_item it = items[blockIdx.y];
-  const Range q = item_range(it);
-  const unsigned cnt = (unsigned)q.w1 * q.w2 * q.w3;
-  const int fb = it.fb;
-  const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
-               *d3 = fc.dx3f + (int64_t)fb * r.n3;
-  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
-    const int i = q.si + (int)(t % (unsigned)q.w1);
-    const unsigned s = t / (unsigned)q.w1;
-    const int j = q.sj + (int)(s % (unsigned)q.w2), k = q.sk + (int)(s / (unsigned)q.w2);
+// ProlongateInternalField for the 2x2x2 fine group of coarse cell (k, j, i) of block fb:
+// the 12 faces inside the group from its 24 outer faces (mesh_refinement.cpp:831-946).
+__device__ __forceinline__ void
+internal_group(const RG &r, double *__restrict__ b1, double *__restrict__ b2, double *__restrict__ b3,
+               const double *__restrict__ d1, const double *__restrict__ d2, const double *__restrict__ d3, int fb,
+               int i, int j, int k) {
     const int fi = (i - r.cis) * 2 + r.is, fj = (j - r.cjs) * 2 + r.js, fk = (k - r.cks) * 2 + r.ks;
     auto B1 = [&](int kk, int jj, int ii) -> double & { return b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, kk, jj, ii)]; };
     auto B2 = [&](int kk, int jj, int ii) -> double & { return b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, kk, jj, ii)]; };
@@ -320,27 +335,41 @@ k_ref_prolong_fc_internal(RG r, double *__restrict__ b1, double *__restrict__ b2
         v = c ? v + Sdx1 * Vxyz : v - Sdx1 * Vxyz;
         B3(fk + 1, fj + b, fi + c) = v / A3(b, fi + c);
       }
+}
+
+__device__ __forceinline__ void
+b_ref_prolong_fc_internal(RG r, double *__restrict__ b1, double *__restrict__ b2, double *__restrict__ b3,
+                          agx_ref_coords fc, const agx_ref_item &it) {
+  const Range q = item_range(it);
+  const unsigned cnt = (unsigned)q.w1 * q.w2 * q.w3;
+  const int fb = it.fb;
+  const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
+               *d3 = fc.dx3f + (int64_t)fb * r.n3;
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
+    const int i = q.si + (int)(t % (unsigned)q.w1);
+    const unsigned s = t / (unsigned)q.w1;
+    const int j = q.sj + (int)(s % (unsigned)q.w2), k = q.sk + (int)(s / (unsigned)q.w2);
+    internal_group(r, b1, b2, b3, d1, d2, d3, fb, i, j, k);
   }
+}
+
+__global__ void __launch_bounds__(RBS)
+k_ref_prolong_fc_internal(RG r, double *__restrict__ b1, double *__restrict__ b2, double *__restrict__ b3,
+                          agx_ref_coords fc, const agx_ref_item *__restrict__ items) {
+  b_ref_prolong_fc_internal(r, b1, b2, b3, fc, items[blockIdx.y]);
 }
 
 // Field::CalculateCellCenteredField, uniform branch (field.cpp:112-176: lw = rw = 0.5), then
 // EquationOfState::PrimitiveToConserved (adiabatic_mhd.cpp:99-136; isothermal_mhd.cpp:
 // 82-108 without energy) on the fine cells of each coarse range.
+// bcc and u of fine cell (k, j, i) of block fb from its faces and w (the loop body of
+// CalculateCellCenteredField + PrimitiveToConserved)
 template <bool ISO>
-__global__ void __launch_bounds__(RBS)
-k_ref_prim_to_cons(RG r, const double *__restrict__ w, const double *__restrict__ b1,
-                   const double *__restrict__ b2, const double *__restrict__ b3, double *__restrict__ bcc,
-                   double *__restrict__ u, double igm1, const agx_ref_item *__restrict__ items) {
-  const agx_ref_item it = items[blockIdx.y];
-  const Range q = item_range(it);
-  const unsigned w1 = 2 * q.w1, w2 = 2 * q.w2, cnt = w1 * w2 * 2 * q.w3;   // fine cells
+__device__ __forceinline__ void
+p2c_cell(const RG &r, const double *__restrict__ w, const double *__restrict__ b1, const double *__restrict__ b2,
+         const double *__restrict__ b3, double *__restrict__ bcc, double *__restrict__ u, double igm1, int fb, int i,
+         int j, int k) {
   constexpr int NH = ISO ? 4 : 5;
-  const int fi0 = (q.si - r.cis) * 2 + r.is, fj0 = (q.sj - r.cjs) * 2 + r.js, fk0 = (q.sk - r.cks) * 2 + r.ks;
-  const int fb = it.fb;
-  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
-    const int i = fi0 + (int)(t % w1);
-    const unsigned s = t / w1;
-    const int j = fj0 + (int)(s % w2), k = fk0 + (int)(s / w2);
     const double bc1 = 0.5 * b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, k, j, i)] +
                        0.5 * b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, k, j, i + 1)];
     const double bc2 = 0.5 * b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, k, j, i)] +
@@ -363,7 +392,31 @@ k_ref_prim_to_cons(RG r, const double *__restrict__ w, const double *__restrict_
       u[ix(r.n3, r.n2, r.n1, NH, fb, 4, k, j, i)] =
           w_p * igm1 + 0.5 * (w_d * (w_vx * w_vx + w_vy * w_vy + w_vz * w_vz) + (bc1 * bc1 + bc2 * bc2 + bc3 * bc3));
     }
+}
+
+template <bool ISO>
+__device__ __forceinline__ void
+b_ref_prim_to_cons(RG r, const double *__restrict__ w, const double *__restrict__ b1,
+                   const double *__restrict__ b2, const double *__restrict__ b3, double *__restrict__ bcc,
+                   double *__restrict__ u, double igm1, const agx_ref_item &it) {
+  const Range q = item_range(it);
+  const unsigned w1 = 2 * q.w1, w2 = 2 * q.w2, cnt = w1 * w2 * 2 * q.w3;   // fine cells
+  const int fi0 = (q.si - r.cis) * 2 + r.is, fj0 = (q.sj - r.cjs) * 2 + r.js, fk0 = (q.sk - r.cks) * 2 + r.ks;
+  const int fb = it.fb;
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
+    const int i = fi0 + (int)(t % w1);
+    const unsigned s = t / w1;
+    const int j = fj0 + (int)(s % w2), k = fk0 + (int)(s / w2);
+    p2c_cell<ISO>(r, w, b1, b2, b3, bcc, u, igm1, fb, i, j, k);
   }
+}
+
+template <bool ISO>
+__global__ void __launch_bounds__(RBS)
+k_ref_prim_to_cons(RG r, const double *__restrict__ w, const double *__restrict__ b1,
+                   const double *__restrict__ b2, const double *__restrict__ b3, double *__restrict__ bcc,
+                   double *__restrict__ u, double igm1, const agx_ref_item *__restrict__ items) {
+  b_ref_prim_to_cons<ISO>(r, w, b1, b2, b3, bcc, u, igm1, items[blockIdx.y]);
 }
 
 // Field::CalculateCellCenteredField (uniform branch) and EquationOfState::
@@ -402,6 +455,77 @@ k_ref_cons_to_prim(RG r, double *__restrict__ u, const double *__restrict__ b1, 
     if (!ISO) u[o0 + 4 * S] = u_e;
 #pragma unroll
     for (int n = 0; n < NH; ++n) w[o0 + n * S] = wv[n];
+  }
+}
+
+// ---------------------------------------------------------------- fused launches
+// One launch for several lists of a step whose items touch disjoint data: workgroup row
+// y runs item y of the concatenated lists.  The arithmetic is the single-list kernels'.
+struct ItemSpan {
+  const agx_ref_item *p[4];
+  int n[4];
+};
+__device__ __forceinline__ int span_pick(const ItemSpan &sp, int &y) {
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    if (y < sp.n[l]) return l;
+    y -= sp.n[l];
+  }
+  return -1;
+}
+
+// RestrictCellCenteredValues + RestrictFieldX1..3 (the same-level ghost restriction of
+// one slot): lists RESTRICT_CC, FC1, FC2, FC3
+__global__ void __launch_bounds__(RBS)
+k_ref_restrict_all(RG r, int nvar, const double *__restrict__ u, const double *__restrict__ b1,
+                   const double *__restrict__ b2, const double *__restrict__ b3, agx_ref_coords fc,
+                   double *__restrict__ cu, double *__restrict__ cb1, double *__restrict__ cb2,
+                   double *__restrict__ cb3, ItemSpan sp) {
+  int y = blockIdx.y;
+  const int l = span_pick(sp, y);
+  if (l == 0) b_ref_restrict_cc(r, nvar, u, fc, cu, sp.p[0][y]);
+  else if (l == 1) b_ref_restrict_fc<0>(r, b1, fc, cb1, sp.p[1][y]);
+  else if (l == 2) b_ref_restrict_fc<1>(r, b2, fc, cb2, sp.p[2][y]);
+  else if (l == 3) b_ref_restrict_fc<2>(r, b3, fc, cb3, sp.p[3][y]);
+}
+
+// ProlongateCellCenteredValues + ProlongateSharedFieldX1..3: lists PROLONG_CC, FC1..3
+__global__ void __launch_bounds__(RBS)
+k_ref_prolong_all(RG r, int nvar, const double *__restrict__ cw, const double *__restrict__ cb1,
+                  const double *__restrict__ cb2, const double *__restrict__ cb3, agx_ref_coords cc,
+                  double *__restrict__ w, double *__restrict__ b1, double *__restrict__ b2, double *__restrict__ b3,
+                  agx_ref_coords fc, ItemSpan sp) {
+  int y = blockIdx.y;
+  const int l = span_pick(sp, y);
+  if (l == 0) b_ref_prolong_cc(r, nvar, cw, cc, w, fc, sp.p[0][y]);
+  else if (l == 1) b_ref_prolong_fc_shared<0>(r, cb1, cc, b1, fc, sp.p[1][y]);
+  else if (l == 2) b_ref_prolong_fc_shared<1>(r, cb2, cc, b2, fc, sp.p[2][y]);
+  else if (l == 3) b_ref_prolong_fc_shared<2>(r, cb3, cc, b3, fc, sp.p[3][y]);
+}
+
+// ProlongateInternalField, then CalculateCellCenteredField + PrimitiveToConserved, per
+// 2x2x2 group: the thread that forms a group's 12 inner faces then converts its 8 cells,
+// whose other faces are the group's outer (shared) faces, final before this launch.
+template <bool ISO>
+__global__ void __launch_bounds__(RBS)
+k_ref_internal_p2c(RG r, double *__restrict__ b1, double *__restrict__ b2, double *__restrict__ b3,
+                   agx_ref_coords fc, const double *__restrict__ w, double *__restrict__ bcc,
+                   double *__restrict__ u, double igm1, const agx_ref_item *__restrict__ items) {
+  const agx_ref_item it = items[blockIdx.y];
+  const Range q = item_range(it);
+  const unsigned cnt = (unsigned)q.w1 * q.w2 * q.w3;
+  const int fb = it.fb;
+  const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
+               *d3 = fc.dx3f + (int64_t)fb * r.n3;
+  for (unsigned t = first_t(); t < cnt; t += stride_t()) {
+    const int i = q.si + (int)(t % (unsigned)q.w1);
+    const unsigned s = t / (unsigned)q.w1;
+    const int j = q.sj + (int)(s % (unsigned)q.w2), k = q.sk + (int)(s / (unsigned)q.w2);
+    internal_group(r, b1, b2, b3, d1, d2, d3, fb, i, j, k);
+    const int fi = (i - r.cis) * 2 + r.is, fj = (j - r.cjs) * 2 + r.js, fk = (k - r.cks) * 2 + r.ks;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      p2c_cell<ISO>(r, w, b1, b2, b3, bcc, u, igm1, fb, fi + (c & 1), fj + ((c >> 1) & 1), fk + (c >> 2));
   }
 }
 
@@ -1136,6 +1260,80 @@ int agx_ref_prim_to_cons(const agx_ref_list *l, const double *w, const double *b
     hipLaunchKernelGGL(k_ref_prim_to_cons<false>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
                        make_rg(l->g), w, b1f, b2f, b3f, bcc, u, igm1, l->items);
   return launch_status("agx_ref_prim_to_cons");
+}
+
+// the lists of a fused launch: kinds in order, one geometry, at most 65535 rows
+static int fuse_lists(const agx_ref_list *const *ls, const int *kinds, int nvar, const char *what, ItemSpan &sp,
+                      int &G, const agx_ref_list *&first) {
+  int rows = 0;
+  G = 1;
+  first = nullptr;
+  for (int l = 0; l < 4; ++l) {
+    sp.p[l] = nullptr;
+    sp.n[l] = 0;
+    if (!ls[l]) continue;
+    if (int rc = check_call(ls[l], kinds[l], kinds[l], what, nvar)) return rc;
+    if (first && (first->g.nx1 != ls[l]->g.nx1 || first->g.nx2 != ls[l]->g.nx2 || first->g.nx3 != ls[l]->g.nx3 ||
+                  first->g.ng != ls[l]->g.ng || first->g.cng != ls[l]->g.cng)) {
+      set_error("%s: the lists have different geometries", what);
+      return -42;
+    }
+    if (!first) first = ls[l];
+    sp.p[l] = ls[l]->items;
+    sp.n[l] = ls[l]->nitems;
+    rows += ls[l]->nitems;
+    G = std::max(G, ls[l]->G);
+  }
+  if (rows > 65535) { set_error("%s: more than 65535 items in one launch", what); return -46; }
+  return rows;
+}
+
+int agx_ref_restrict_all(const agx_ref_list *lcc, const agx_ref_list *lf1, const agx_ref_list *lf2,
+                         const agx_ref_list *lf3, int32_t nvar, const double *u, const double *b1f, const double *b2f,
+                         const double *b3f, agx_ref_coords fc, double *cu, double *cb1f, double *cb2f, double *cb3f,
+                         void *stream) {
+  const agx_ref_list *ls[4] = {lcc, lf1, lf2, lf3};
+  const int kinds[4] = {AGX_REF_RESTRICT_CC, AGX_REF_RESTRICT_FC1, AGX_REF_RESTRICT_FC2, AGX_REF_RESTRICT_FC3};
+  ItemSpan sp;
+  int G;
+  const agx_ref_list *f;
+  const int rows = fuse_lists(ls, kinds, nvar, "agx_ref_restrict_all", sp, G, f);
+  if (rows <= 0) return rows;
+  hipLaunchKernelGGL(k_ref_restrict_all, dim3(G, rows), dim3(RBS), 0, (hipStream_t)stream, make_rg(f->g), nvar, u,
+                     b1f, b2f, b3f, fc, cu, cb1f, cb2f, cb3f, sp);
+  return launch_status("agx_ref_restrict_all");
+}
+
+int agx_ref_prolong_all(const agx_ref_list *lcc, const agx_ref_list *lf1, const agx_ref_list *lf2,
+                        const agx_ref_list *lf3, int32_t nvar, const double *cw, const double *cb1f, const double *cb2f,
+                        const double *cb3f, agx_ref_coords cc, double *w, double *b1f, double *b2f, double *b3f,
+                        agx_ref_coords fc, void *stream) {
+  const agx_ref_list *ls[4] = {lcc, lf1, lf2, lf3};
+  const int kinds[4] = {AGX_REF_PROLONG_CC, AGX_REF_PROLONG_FC1, AGX_REF_PROLONG_FC2, AGX_REF_PROLONG_FC3};
+  ItemSpan sp;
+  int G;
+  const agx_ref_list *f;
+  const int rows = fuse_lists(ls, kinds, nvar, "agx_ref_prolong_all", sp, G, f);
+  if (rows <= 0) return rows;
+  hipLaunchKernelGGL(k_ref_prolong_all, dim3(G, rows), dim3(RBS), 0, (hipStream_t)stream, make_rg(f->g), nvar, cw,
+                     cb1f, cb2f, cb3f, cc, w, b1f, b2f, b3f, fc, sp);
+  return launch_status("agx_ref_prolong_all");
+}
+
+int agx_ref_internal_to_cons(const agx_ref_list *l, double *b1f, double *b2f, double *b3f, agx_ref_coords fc,
+                             const double *w, double *bcc, double *u, double gamma, int32_t iso, void *stream) {
+  if (int rc = check_call(l, AGX_REF_PROLONG_FC_INT, AGX_REF_PROLONG_FC_INT, "agx_ref_internal_to_cons", iso ? 4 : 5))
+    return rc;
+  if (!iso && !(gamma > 1.0)) { set_error("agx_ref_internal_to_cons: gamma must be > 1"); return -50; }
+  if (l->nitems == 0) return 0;
+  const double igm1 = iso ? 0.0 : 1.0 / (gamma - 1.0);
+  if (iso)
+    hipLaunchKernelGGL(k_ref_internal_p2c<true>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                       make_rg(l->g), b1f, b2f, b3f, fc, w, bcc, u, igm1, l->items);
+  else
+    hipLaunchKernelGGL(k_ref_internal_p2c<false>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
+                       make_rg(l->g), b1f, b2f, b3f, fc, w, bcc, u, igm1, l->items);
+  return launch_status("agx_ref_internal_to_cons");
 }
 
 int agx_ref_cons_to_prim(const agx_ref_list *l, double *u, const double *b1f, const double *b2f, const double *b3f,

@@ -495,13 +495,15 @@ def smr_bench(agx, reps):
     P = {k: v.ptr for k, v in D.items()}
     args_ = (P["u"], [x.ptr for x in b], P["w"], P["bcc"], P["cu"], [x.ptr for x in cb], P["cw"], P["cbcc"])
     ms = timed(lambda: sb.boundary_step(*args_), sb.sync)
+    ms_unfused = timed(lambda: sb.boundary_step(*args_, fused=False), sb.sync)
     sb.capture(*args_)
     msg = timed(sb.replay, sb.sync)
-    out["mhd_boundary_step"] = {"ms": ms, "ms_graph_replay": msg, "slots": len(sb.slots),
+    out["mhd_boundary_step"] = {"ms": ms, "ms_one_launch_per_list": ms_unfused, "ms_graph_replay": msg,
+                                "slots": len(sb.slots),
                                 "what": "u (5 variables) and b exchanged across levels, then ProlongateBoundaries "
                                         "(restriction, cons->prim on the coarse buffer, prolongation, prim->cons)"}
     log(f"[smr] MHD boundary step (exchanges + prolongation, {len(sb.slots)} slots), {nb} blocks: {ms:.3f} ms "
-        f"eager, {msg:.3f} ms as one hipGraph replay")
+        f"(fused slot launches; {ms_unfused:.3f} ms with one launch per list), {msg:.3f} ms as one hipGraph replay")
     sb.close()
     for a in list(D.values()) + b + cb:
         a.close()

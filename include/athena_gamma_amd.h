@@ -385,6 +385,23 @@ int agx_ref_count_fine_edges(const int32_t nblevel[27], int32_t edge_flag[12], i
  * w and bcc (nvar 5 with the adiabatic EOS, gamma; 4 with iso != 0) (PRIM_TO_CONS) */
 int agx_ref_prim_to_cons(const agx_ref_list *l, const double *w, const double *b1f, const double *b2f,
                          const double *b3f, double *bcc, double *u, double gamma, int32_t iso, void *stream);
+/* Fused launches for one slot of the ghost prolongation (no counterpart in the
+ * reference; the lists and results are those of the single-list calls).  Any list may
+ * be NULL.  restrict_all: RESTRICT_CC (u -> cu, nvar) and RESTRICT_FC1..3 (b -> cb) in
+ * one launch; prolong_all: PROLONG_CC (cw -> w) and PROLONG_FC1..3 (shared faces);
+ * internal_to_cons: a PROLONG_FC_INT list, each 2x2x2 group's inner faces then bcc and u
+ * of its 8 cells (agx_ref_prolong_fc_internal followed by agx_ref_prim_to_cons on the
+ * same ranges). */
+int agx_ref_restrict_all(const agx_ref_list *lcc, const agx_ref_list *lf1, const agx_ref_list *lf2,
+                         const agx_ref_list *lf3, int32_t nvar, const double *u, const double *b1f, const double *b2f,
+                         const double *b3f, agx_ref_coords fine_coords, double *cu, double *cb1f, double *cb2f,
+                         double *cb3f, void *stream);
+int agx_ref_prolong_all(const agx_ref_list *lcc, const agx_ref_list *lf1, const agx_ref_list *lf2,
+                        const agx_ref_list *lf3, int32_t nvar, const double *cw, const double *cb1f, const double *cb2f,
+                        const double *cb3f, agx_ref_coords coarse_coords, double *w, double *b1f, double *b2f,
+                        double *b3f, agx_ref_coords fine_coords, void *stream);
+int agx_ref_internal_to_cons(const agx_ref_list *l, double *b1f, double *b2f, double *b3f, agx_ref_coords fine_coords,
+                             const double *w, double *bcc, double *u, double gamma, int32_t iso, void *stream);
 /* coarse conserved variables and faces -> coarse bcc and primitives on the item ranges,
  * floors written back to u (CONS_TO_PRIM; nvar 5 with the adiabatic EOS, 4 with iso) */
 int agx_ref_cons_to_prim(const agx_ref_list *l, double *u, const double *b1f, const double *b2f, const double *b3f,

@@ -83,9 +83,11 @@ def test_mhd_ghosts_match_oracle(agx, refine):
             assert np.array_equal(xx, yy), k
 
 
-def test_smr_boundary_driver(agx):
+@pytest.mark.parametrize("fused", [True, False])
+def test_smr_boundary_driver(agx, fused):
     """the package's SMRBoundary (every list built once from the tree) against the
-    oracle: boundary_step on an MHD state with floors, correct_emf on random EMFs"""
+    oracle: boundary_step on an MHD state with floors (fused: four launches per slot of
+    the prolongation, or one per list), correct_emf on random EMFs"""
     from tests import emf_cases as ec
     smr = load_smr()
     t = gc.tree(((1, 1, 1), (2, 1, 1)))
@@ -98,7 +100,7 @@ def test_smr_boundary_driver(agx):
     D = {k: ([agx.DeviceArray(x) for x in v] if isinstance(v, list) else agx.DeviceArray(v)) for k, v in A.items()}
     de = [agx.DeviceArray(x) for x in E]
     P = {k: ([x.ptr for x in v] if isinstance(v, list) else v.ptr) for k, v in D.items()}
-    sb.boundary_step(P["u"], P["b"], P["w"], P["bcc"], P["cu"], P["cb"], P["cw"], P["cbcc"])
+    sb.boundary_step(P["u"], P["b"], P["w"], P["bcc"], P["cu"], P["cb"], P["cw"], P["cbcc"], fused=fused)
     sb.correct_emf(*[x.ptr for x in de])
     sb.sync()
     for k in ref:
