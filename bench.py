@@ -155,10 +155,30 @@ def mhd_bench(agx, pgen, args, precision):
         log(f"[bench] {precision} max |div B| after {int(state[0][2])} cycles: {state[2][0]:.3e} "
             f"(x dx / max|B|: {state[2][1]:.3e})")
     m.close()
+    mhd_bench.tile = ("16, 16", "32, 8", "64, 4", "64, 8")[tile]
     return desc, zc, sec * 1e3 / args.steps, per, state
 
 
-def roofline(desc, per, precision):
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_stage_turb256_fp64_hbm_traffic.txt")
+
+
+def measured_traffic(desc, precision, tile):
+    """HBM bytes per launch of the stage kernel with this tile and precision, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (already scaled by the copy-kernel
+    factors), when the workload is the one profiled (256^3 in 64^3 blocks); else None."""
+    if (desc.nx1, desc.bnx1) != (256, 64) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    name = f"k_stage<{tile}, {'double' if precision == 'fp64' else 'float'}, false>"
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith(name):
+            f = line[len(name):].split()
+            return {"bytes": (float(f[1]) + float(f[2])) * 1e9, "read_GB": float(f[1]), "write_GB": float(f[2]),
+                    "source": "profiles/k_stage_turb256_fp64_hbm_traffic.txt (rocprofv3 --pmc FETCH_SIZE, "
+                              "WRITE_SIZE; scaled by the 1 GiB copy factors)"}
+    return None
+
+
+def roofline(desc, per, precision, tile=None):
     """Dominant kernel (among those with an algorithmic byte count) and its HBM rate."""
     table = BYTES_PER_UNIT[precision]
     dom = max(table, key=lambda c: per[c][0])
@@ -166,9 +186,14 @@ def roofline(desc, per, precision):
     avg_s = ms_tot / max(n_l, 1) / 1e3
     alg = table[dom] * units_per_launch(desc, dom)
     achieved = alg / avg_s
-    return {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-            "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": alg}
+    tr = measured_traffic(desc, precision, tile) if (dom == "integrate" and tile) else None
+    out = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+           "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": tr["bytes"] if tr else None,
+           "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": alg}
+    if tr:
+        out["traffic_detail"] = {"read_GB": tr["read_GB"], "write_GB": tr["write_GB"], "source": tr["source"],
+                                 "traffic_rate_GBs": tr["bytes"] / avg_s / 1e9}
+    return out
 
 
 def mg_bench(agx, pgen, args, precision="fp64"):
@@ -586,7 +611,7 @@ def main():
         large_bench(agx, pgen, args)
         return
     desc, zc, ms_step, per, st64 = mhd_bench(agx, pgen, args, "fp64")
-    rf = roofline(desc, per, "fp64")
+    rf = roofline(desc, per, "fp64", mhd_bench.tile)
     log("[bench] fp64 per-cycle kernel time: " + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in per.items())
         + f" (sum {sum(v[0] for v in per.values()) / 2:.2f} ms/cycle)")
     out = {
@@ -612,12 +637,13 @@ def main():
     }
     if not args.no_mixed:
         mdesc, mzc, mms, mper, stmx = mhd_bench(agx, pgen, args, "mixed")
+        mtile = mhd_bench.tile
         log("[bench] mixed per-cycle kernel time: " + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in mper.items())
             + f" (sum {sum(v[0] for v in mper.values()) / 2:.2f} ms/cycle)")
         mixed = {"value": mzc, "unit": "zone-cycles/s", "ms_per_step": mms, "vs_fp64": mzc / zc,
                  "dtype": "f32 reconstruction+Riemann, f64 update/C2P/CT",
                  "cycle_roofline_frac": mzc * CYCLE_BYTES["mixed"] / HBM_PEAK,
-                 "roofline": roofline(mdesc, mper, "mixed"),
+                 "roofline": roofline(mdesc, mper, "mixed", mtile),
                  "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in mper.items()}}
         if st64 is not None and stmx is not None:
             (t64, w64, db64), (tmx, wmx, dbmx) = st64, stmx
