@@ -147,7 +147,11 @@ struct agx_mesh {
   int mg_on = 0;
   int nrootlevel = 0, nmblevel = 0, ntotal = 0, current = 0, fmglevel = 0, coffset = 0;
   int mg_mode = 0, mg_niter = 0;
-  int mg_fuse = -1;                  // red-black pass form: -1 by level size, 0 two sweeps, 1 fused
+  int mg_fuse = -1;
+  // source mask (agx_mg_set_source_mask_sphere): src = 0 where |x - c|^2 > r^2
+  int srcmask = 0;
+  double srcmask_c[3] = {0.0, 0.0, 0.0};
+  double srcmask_r2 = 0.0;                  // red-black pass form: -1 by level size, 0 two sweeps, 1 fused
   double mg_eps = -1.0, last_norm = 0.0, last_ave = 0.0;
   int32_t last_iters = 0;
   std::vector<agx::MGLevel> lev;

@@ -699,11 +699,33 @@ k_mg_mix(double *__restrict__ u, float *__restrict__ e, int64_t n, int mode) {
   else e[t] = (float)u[t];
 }
 
+// Source mask of MultigridDriver::SetupMultigrid (multigrid_driver.cpp:430-436), in the
+// form of the reference's collapse problem (pgen/collapse.cpp:66-82): the source is zero
+// at cell centres with |x - c|^2 > r^2.  Centres as MGCoordinates (whole-mesh faces,
+// end faces exact).
+struct SrcMask {
+  int on;
+  int n[3];
+  double xmin[3], xmax[3], c[3], r2;
+};
+__device__ __forceinline__ double mask_centre(const SrcMask &k, int d, int i) {   // cell i = 0 .. n-1
+  const double h = (k.xmax[d] - k.xmin[d]) / k.n[d];
+  const double a = (i == 0) ? k.xmin[d] : k.xmin[d] + i * h;
+  const double b = (i + 1 == k.n[d]) ? k.xmax[d] : k.xmin[d] + (i + 1) * h;
+  return 0.5 * (a + b);
+}
+__device__ __forceinline__ bool masked_out(const SrcMask &k, int I, int J, int K) {
+  if (!k.on) return false;
+  const double x = mask_centre(k, 0, I) - k.c[0], y = mask_centre(k, 1, J) - k.c[1],
+               z = mask_centre(k, 2, K) - k.c[2];
+  return x * x + y * y + z * z > k.r2;
+}
+
 // LoadSource (multigrid.cpp:194-231): src = fac * u(IDN) over every block interior,
 // and in MGI mode LoadFinestData (multigrid.cpp:160-180): u = phi.
 __global__ void __launch_bounds__(BS)
 k_mg_load(Geo g, const double *__restrict__ u, const double *__restrict__ phi,
-          double *__restrict__ src, double *__restrict__ mu, double fac, int load_phi) {
+          double *__restrict__ src, double *__restrict__ mu, double fac, int load_phi, SrcMask mk) {
   int N1 = g.nb1 * g.bx1, N2 = g.nb2 * g.bx2, N3 = g.nb3 * g.bx3;
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)N1 * N2 * N3) return;
@@ -714,7 +736,7 @@ k_mg_load(Geo g, const double *__restrict__ u, const double *__restrict__ phi,
   int b = ((K / g.bx3) * g.nb2 + (J / g.bx2)) * g.nb1 + (I / g.bx1);
   int64_t id = IDX(g, K % g.bx3 + g.ng, J % g.bx2 + g.ng, I % g.bx1 + g.ng);
   double v = u[(int64_t)b * NHYDRO * g.S + id];
-  src[t] = (fac == 1.0) ? v : v * fac;
+  src[t] = masked_out(mk, I, J, K) ? 0.0 : ((fac == 1.0) ? v : v * fac);
   if (load_phi) mu[t] = phi[(int64_t)b * g.S + id];
 }
 
@@ -1125,7 +1147,7 @@ k_mb_moments_final(const double *__restrict__ part, int nwg, int order, int com,
 // u = phi (LoadFinestData, multigrid.cpp:160-180)
 __global__ void __launch_bounds__(BS)
 k_mb_load(Geo g, const double *__restrict__ u, const double *__restrict__ phi, double *__restrict__ src,
-          double *__restrict__ mu, double fac, int load_phi) {
+          double *__restrict__ mu, double fac, int load_phi, SrcMask mk) {
   const int N1 = g.nb1 * g.bx1, N2 = g.nb2 * g.bx2, N3 = g.nb3 * g.bx3;
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (t >= (unsigned)N1 * N2 * N3) return;
@@ -1134,7 +1156,7 @@ k_mb_load(Geo g, const double *__restrict__ u, const double *__restrict__ phi, d
   const int64_t id = IDX(g, K % g.bx3 + g.ng, J % g.bx2 + g.ng, I % g.bx1 + g.ng);
   const double v = u[(int64_t)b * NHYDRO * g.S + id];
   const unsigned f = BX(N1, N2, K + 1, J + 1, I + 1);
-  src[f] = (fac == 1.0) ? v : v * fac;
+  src[f] = masked_out(mk, I, J, K) ? 0.0 : ((fac == 1.0) ? v : v * fac);
   if (load_phi) mu[f] = phi[(int64_t)b * g.S + id];
 }
 
@@ -1200,6 +1222,18 @@ static int mp_create(agx_mesh *m) {
   AGX_HIP(hipMemsetAsync(m->mpdev, 0, sizeof(double) * (28 + NMOM * NPART), m->stream));
   m->mp_on = true;
   return 0;
+}
+
+static SrcMask src_mask(const agx_mesh *m) {
+  SrcMask k;
+  k.on = m->srcmask;
+  const Geo &g = m->g;
+  k.n[0] = g.nb1 * g.bx1; k.n[1] = g.nb2 * g.bx2; k.n[2] = g.nb3 * g.bx3;
+  k.xmin[0] = m->d.x1min; k.xmin[1] = m->d.x2min; k.xmin[2] = m->d.x3min;
+  k.xmax[0] = m->d.x1max; k.xmax[1] = m->d.x2max; k.xmax[2] = m->d.x3max;
+  for (int d = 0; d < 3; ++d) k.c[d] = m->srcmask_c[d];
+  k.r2 = m->srcmask_r2;
+  return k;
 }
 
 static MPGeom mp_geom(const agx_mesh *m, const MGLevelB &B) {
@@ -1351,7 +1385,7 @@ static int mg_solve_b(agx_mesh *m) {
   MGLevelB &F = m->levb[L - 1];
   double **A = m->arr.a;
   hipLaunchKernelGGL(k_mb_load, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, g,
-                     A[AGX_ARR_U], A[AGX_ARR_PHI], F.src, F.u, m->d.four_pi_G, m->mg_mode == 1 ? 1 : 0);
+                     A[AGX_ARR_U], A[AGX_ARR_PHI], F.src, F.u, m->d.four_pi_G, m->mg_mode == 1 ? 1 : 0, src_mask(m));
   m->current = L - 1;
   m->fmglevel = L - 1;
   mp_coefficients(m);
@@ -1797,7 +1831,7 @@ int mg_solve(agx_mesh *m, int stage) {
   double **A = m->arr.a;
   const bool mixed = m->mixed;
   hipLaunchKernelGGL(k_mg_load, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, g, A[AGX_ARR_U],
-                     A[AGX_ARR_PHI], F.src, F.u, m->d.four_pi_G, m->mg_mode == 1 ? 1 : 0);
+                     A[AGX_ARR_PHI], F.src, F.u, m->d.four_pi_G, m->mg_mode == 1 ? 1 : 0, src_mask(m));
   m->current = L - 1;
   m->fmglevel = L - 1;
   subtract_average(m, 0);   // SetupMultigrid, multigrid_driver.cpp:438-439
@@ -1907,3 +1941,12 @@ int mg_vcycles(agx_mesh *m, int n) {
 }
 
 }  // namespace agx
+
+extern "C" int agx_mg_set_source_mask_sphere(agx_mesh *m, double xc, double yc, double zc, double radius) {
+  if (!m) { set_error("agx_mg_set_source_mask_sphere: null mesh"); return -1; }
+  if (!m->mg_on) { set_error("agx_mg_set_source_mask_sphere: the mesh has no multigrid gravity"); return -21; }
+  m->srcmask = radius > 0.0 ? 1 : 0;
+  m->srcmask_c[0] = xc; m->srcmask_c[1] = yc; m->srcmask_c[2] = zc;
+  m->srcmask_r2 = radius * radius;   // collapse.cpp:68: rc2 = rc*rc
+  return 0;
+}

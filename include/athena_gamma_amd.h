@@ -205,6 +205,13 @@ int agx_set_tile(agx_mesh *m, int32_t tile);
 /* multigrid red-black pass form: -1 by level size (fused on FP64 levels of >= 2^26
  * cells, the default), 0 two sweeps, 1 fused on every level of >= 2^21 cells */
 int agx_set_mg_fuse(agx_mesh *m, int32_t mode);
+/* Source mask of MultigridDriver::SetupMultigrid (multigrid_driver.cpp:430-436): after
+ * LoadSource, the source is zero at every finest-level cell centre with |x - c|^2 > r^2,
+ * the mask the reference's collapse problem enrolls (pgen/collapse.cpp:66-82, there with
+ * c = 0).  Applied before the mean subtraction (periodic) or the multipole expansion.
+ * radius <= 0 switches it off.  The reference takes an arbitrary user function here;
+ * this entry point offers its spherical form. */
+int agx_mg_set_source_mask_sphere(agx_mesh *m, double xc, double yc, double zc, double radius);
 
 /* ---- hipGraph replay of agx_cycle (no counterpart in the reference: launch mechanism).
  * With enable != 0, agx_cycle captures one whole VL2 cycle (fluxes, stage kernels,

@@ -89,6 +89,8 @@ struct orc_ctx {
   int last_iters;
   int mg_mode, mg_niter;
   double mg_eps;
+  int srcmask;       /* source mask: src = 0 where |x - c|^2 > r^2 (pgen/collapse.cpp:66-82) */
+  double srcmask_c[3], srcmask_r2;
   int iso;           /* isothermal EOS (eos = 1): NHYDRO = 4, the energy slot stays 0 */
   int nhydro;
   int fas;           /* ffas_: multipole boundaries (multigrid_driver.cpp:292-295) */
@@ -2320,6 +2322,13 @@ static double mg_defect_norm(orc_ctx *c) {
 void orc_mg_set_mode(orc_ctx *c, int mode, double threshold, int niter) {
   c->mg_mode = mode; c->mg_eps = threshold; c->mg_niter = niter;
 }
+/* MultigridDriver's srcmask_ (multigrid_driver.cpp:430-436) with the collapse problem's
+ * mask (pgen/collapse.cpp:66-82), centred at c; radius <= 0 switches it off. */
+void orc_mg_set_source_mask_sphere(orc_ctx *c, double xc, double yc, double zc, double radius) {
+  c->srcmask = radius > 0.0;
+  c->srcmask_c[0] = xc; c->srcmask_c[1] = yc; c->srcmask_c[2] = zc;
+  c->srcmask_r2 = radius * radius;
+}
 double orc_mg_last_norm(orc_ctx *c) { return c->last_norm; }
 int orc_mg_last_iters(orc_ctx *c) { return c->last_iters; }
 void orc_mg_multipole(orc_ctx *c, double *out28) {
@@ -2329,7 +2338,8 @@ void orc_mg_multipole(orc_ctx *c, double *out28) {
 double orc_mp_phis(orc_ctx *c, double x, double y, double z) { return mp_phis(c, x, y, z); }
 
 /* MGGravityDriver::Solve, gravity/mg_gravity.cpp:166-211 (periodic boundaries:
- * fsubtract_average_ = true, no FAS, no source mask). */
+ * fsubtract_average_ = true, no FAS; multipole boundaries with FAS; optional source
+ * mask). */
 void orc_mg_solve(orc_ctx *c, int stage) {
   (void)stage;
   int lf = c->nmblevel - 1;
@@ -2347,6 +2357,19 @@ void orc_mg_solve(orc_ctx *c, int stage) {
           m->src[lf][MIDX(m, lf, k, j, i)] = (fpg == 1.0) ? v : v * fpg;
           if (c->mg_mode == 1) m->u[lf][MIDX(m, lf, k, j, i)] = phi[hid];
         }
+    if (c->srcmask) {   /* Multigrid::ApplySourceMask (multigrid.cpp:239-246) */
+      for (int k = 1; k <= m->n3[lf]; ++k) {
+        double z = mgc_v(m, lf, 2, k) - c->srcmask_c[2];
+        for (int j = 1; j <= m->n2[lf]; ++j) {
+          double y = mgc_v(m, lf, 1, j) - c->srcmask_c[1];
+          for (int i = 1; i <= m->n1[lf]; ++i) {
+            double x = mgc_v(m, lf, 0, i) - c->srcmask_c[0];
+            double r2 = x * x + y * y + z * z;
+            if (r2 > c->srcmask_r2) m->src[lf][MIDX(m, lf, k, j, i)] = 0.0;
+          }
+        }
+      }
+    }
   }
   c->current = c->ntotal - 1;
   c->fmglevel = c->ntotal - 1;

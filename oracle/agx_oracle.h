@@ -86,6 +86,7 @@ int orc_mg_last_iters(orc_ctx *c);
 void orc_mg_multipole(orc_ctx *c, double *out28);
 double orc_mp_phis(orc_ctx *c, double x, double y, double z);
 void orc_mg_set_mode(orc_ctx *c, int mode, double threshold, int niter);
+void orc_mg_set_source_mask_sphere(orc_ctx *c, double xc, double yc, double zc, double radius);
 
 /* problem generators / diagnostics (pgen restatements) */
 void orc_pgen_linear_wave(orc_ctx *c, int wave_flag, double amp, double vflow);

@@ -85,6 +85,8 @@ def _lib():
     lib.orc_mp_phis.restype = ctypes.c_double
     lib.orc_mp_phis.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     lib.orc_mg_set_mode.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    lib.orc_mg_set_source_mask_sphere.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_double]
     lib.orc_pgen_linear_wave.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     lib.orc_linear_wave_errors.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double, D]
     lib.orc_max_divb.restype = ctypes.c_double
@@ -215,6 +217,9 @@ class Oracle:
 
     def mg_set_mode(self, mode, threshold, niter):
         self.L.orc_mg_set_mode(self.h, mode, threshold, niter)
+
+    def mg_set_source_mask_sphere(self, xc, yc, zc, radius):
+        self.L.orc_mg_set_source_mask_sphere(self.h, xc, yc, zc, radius)
 
     def mg_last(self):
         return self.L.orc_mg_last_norm(self.h), self.L.orc_mg_last_iters(self.h)

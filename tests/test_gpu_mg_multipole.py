@@ -72,3 +72,34 @@ def test_noncubic_root_grid(agx):
     err = np.abs(a - b).max() / np.abs(b).max()
     print(f"non-cubic root: phi rel diff {err:.3e}; defect {m.mg_last()} vs {o.mg_last()}", flush=True)
     assert err <= 1e-9
+
+
+@pytest.mark.parametrize("bc", [dict(mg_bc=0), dict(mg_bc=1, mporder=4, mp_auto_origin=0)])
+def test_source_mask_matches(agx, bc):
+    """agx_mg_set_source_mask_sphere against the oracle's mask, and against a density
+    that is already zero outside the sphere (bitwise on the GPU)"""
+    from tests.test_oracle_mg_srcmask import C, N, RC, inside
+    rho = 1.0 + gaussian(N, 0.1, c=(0.1, 0.0, -0.05), amp=5.0)
+    d = make_desc(nx=(N, N, N), bnx=(16, 16, 16), box=BOX, self_gravity=1, mg_mode=0, four_pi_G=1.0,
+                  mg_threshold=-1.0, mg_niter=4, **bc)
+    phis = []
+    for dens, mask in ((rho, True), (np.where(inside(N), rho, 0.0), False)):
+        U = np.zeros((5, N, N, N))
+        U[0] = dens
+        ub = agx.scatter_cells(d, U)
+        m = agx.Mesh(d)
+        m.set("u", ub)
+        if mask:
+            m.mg_set_source_mask_sphere(*C, RC)
+            o = Oracle(d)
+            o.set("u", ub)
+            o.mg_set_source_mask_sphere(*C, RC)
+            o.mg_solve(0)
+        m.mg_solve(0)
+        phis.append(m.get("phi"))
+        if mask:
+            ref = o.get("phi")
+            err = np.abs(phis[0] - ref).max() / np.abs(ref).max()
+            print(f"{bc}: masked phi rel diff vs oracle {err:.3e}", flush=True)
+            assert err <= 1e-9
+    assert np.array_equal(phis[0], phis[1])
