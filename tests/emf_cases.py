@@ -95,17 +95,16 @@ def edge_table(t, E, nx, ng):
 def check_consistency(t, E, nx, ng, tol=4e-15):
     """(same-level spread, coarse-vs-restricted-fine error, coarse edges compared):
     every edge shared by blocks of one level carries one value in all of them, and every
-    coarse edge that fine edges cover equals their length-weighted mean (equal lengths)."""
+    edge of one level that edges of the next finer level cover equals their length-
+    weighted mean (equal lengths)."""
     tab = edge_table(t, E, nx, ng)
     spread = max((max(v) - min(v) for v in tab.values() if len(v) > 1), default=0.0)
     err, ncmp = 0.0, 0
     for (lev, a, gx, gy, gz), v in tab.items():
-        if lev != 0:
-            continue
         p = [2 * gx, 2 * gy, 2 * gz]
         q = list(p)
         q[a] += 1
-        fa, fb = tab.get((1, a, *p)), tab.get((1, a, *q))
+        fa, fb = tab.get((lev + 1, a, *p)), tab.get((lev + 1, a, *q))
         if fa is None or fb is None:
             continue
         ncmp += 1

@@ -16,13 +16,21 @@ NRB = (4, 4, 4)
 COEF = [(1.0, 3.0, -2.0, 5.0), (-2.0, 1.0, 4.0, -3.0)]
 
 
-def tree(refine):
+def tree(refine, refine2=()):
+    """root blocks in `refine` refined once; level-1 locations in `refine2` once more"""
     smr = load_smr()
     leaves = [(0, x, y, z) for z, y, x in itertools.product(*(range(n) for n in NRB[::-1]))
               if (x, y, z) not in refine]
     for (x, y, z) in refine:
-        leaves += [(1, 2 * x + c1, 2 * y + c2, 2 * z + c3) for c3, c2, c1 in itertools.product(range(2), repeat=3)]
+        leaves += [(1, 2 * x + c1, 2 * y + c2, 2 * z + c3) for c3, c2, c1 in itertools.product(range(2), repeat=3)
+                   if (2 * x + c1, 2 * y + c2, 2 * z + c3) not in refine2]
+    for (x, y, z) in refine2:
+        leaves += [(2, 2 * x + c1, 2 * y + c2, 2 * z + c3) for c3, c2, c1 in itertools.product(range(2), repeat=3)]
     return smr.SMRTree(NRB, leaves, periodic=(False, False, False))
+
+
+# three levels: the central 2^3 root blocks refined, then one inner level-1 block again
+THREE = (tuple((x, y, z) for z in (1, 2) for y in (1, 2) for x in (1, 2)), ((3, 3, 3),))
 
 
 def coords(t, nx, ng, coarse=False):

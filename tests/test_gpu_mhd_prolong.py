@@ -83,14 +83,14 @@ def test_mhd_ghosts_match_oracle(agx, refine):
             assert np.array_equal(xx, yy), k
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_smr_boundary_driver(agx, fused):
+@pytest.mark.parametrize("fused,levels", [(True, 2), (False, 2), (True, 3)])
+def test_smr_boundary_driver(agx, fused, levels):
     """the package's SMRBoundary (every list built once from the tree) against the
     oracle: boundary_step on an MHD state with floors (fused: four launches per slot of
     the prolongation, or one per list), correct_emf on random EMFs"""
     from tests import emf_cases as ec
     smr = load_smr()
-    t = gc.tree(((1, 1, 1), (2, 1, 1)))
+    t = gc.tree(((1, 1, 1), (2, 1, 1))) if levels == 2 else gc.tree(*gc.THREE)
     ftab, ctab = gc.coords(t, NX, NG), gc.coords(t, NX, NG, coarse=True)
     A = gc.mhd_state(t, NX, NG, seed=12)
     ref = gc.run_oracle_mhd(orc, t, A, NX, NG, ftab, ctab, order="reference")
