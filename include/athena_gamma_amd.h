@@ -7,7 +7,8 @@
  * one task of the reference's task lists for ALL MeshBlocks in one batch of kernel
  * launches on the mesh's HIP stream, and replaces the reference method cited beside
  * it (paths relative to the reference's src/).  Device memory is allocated only by
- * agx_mesh_create, agx_ref_list_create and agx_device_alloc.
+ * agx_mesh_create, agx_ref_list_create and agx_device_alloc (and the HIP runtime's own
+ * objects by agx_stream_create and agx_graph_end).
  *
  * Conventions
  *   - Return value: 0 on success; > 0 a HIP error code; < 0 a contract violation
