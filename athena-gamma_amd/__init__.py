@@ -43,7 +43,7 @@ EXPORTS = [
     "agx_ref_emf_average", "agx_ref_count_fine_edges", "agx_ref_copy_cc", "agx_ref_pack_fc", "agx_ref_unpack_fc",
     "agx_ref_cons_to_prim", "agx_stream_create", "agx_stream_destroy", "agx_graph_begin", "agx_graph_end",
     "agx_graph_launch", "agx_graph_destroy", "agx_ref_restrict_all", "agx_ref_prolong_all",
-    "agx_ref_internal_to_cons", "agx_mg_set_source_mask_sphere",
+    "agx_ref_internal_to_cons", "agx_mg_set_source_mask_sphere", "agx_set_mg_coarse",
 ]
 
 
@@ -147,6 +147,7 @@ def lib():
     L.agx_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
     L.agx_set_tile.argtypes = [P, ctypes.c_int32]
     L.agx_set_mg_fuse.argtypes = [P, ctypes.c_int32]
+    L.agx_set_mg_coarse.argtypes = [P, ctypes.c_int32]
     L.agx_mg_set_source_mask_sphere.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_double]
     L.agx_set_graph.argtypes = [P, ctypes.c_int32]
@@ -348,6 +349,9 @@ class Mesh:
     def set_graph(self, enable=True):
         """Replay agx_cycle as one hipGraph launch per cycle (no self-gravity)."""
         _check(self.L.agx_set_graph(self.h, 1 if enable else 0), "agx_set_graph")
+
+    def set_mg_coarse(self, on):
+        _check(lib().agx_set_mg_coarse(self.h, int(on)), "agx_set_mg_coarse")
 
     def mg_set_source_mask_sphere(self, xc, yc, zc, radius):
         _check(lib().agx_mg_set_source_mask_sphere(self.h, xc, yc, zc, radius), "agx_mg_set_source_mask_sphere")

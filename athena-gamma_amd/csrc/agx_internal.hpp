@@ -148,6 +148,7 @@ struct agx_mesh {
   int nrootlevel = 0, nmblevel = 0, ntotal = 0, current = 0, fmglevel = 0, coffset = 0;
   int mg_mode = 0, mg_niter = 0;
   int mg_fuse = -1;
+  int mg_coarse = 1;   // multipole V-cycles: the coarse levels in one workgroup (k_mb_coarse)
   // source mask (agx_mg_set_source_mask_sphere): src = 0 where |x - c|^2 > r^2
   int srcmask = 0;
   double srcmask_c[3] = {0.0, 0.0, 0.0};
