@@ -532,6 +532,50 @@ def smr_bench(agx, reps):
     sb.close()
     for a in list(D.values()) + b + cb:
         a.close()
+    out["three_levels"] = smr_three_levels(agx, smr, reps, timed)
+    return out
+
+
+def smr_three_levels(agx, smr, reps, timed):
+    """The MHD boundary step and the EMF correction on a three-level mesh of 16^3 blocks:
+    8^3 root blocks, the central 2^3 refined, the central 2^3 of those refined again
+    (624 blocks; levels 0, 1, 2), through SMRBoundary."""
+    nx, ng = 16, 2
+    r1 = {(x, y, z) for x in (3, 4) for y in (3, 4) for z in (3, 4)}
+    r2 = {(x, y, z) for x in (7, 8) for y in (7, 8) for z in (7, 8)}
+    leaves = [(0, x, y, z) for z in range(8) for y in range(8) for x in range(8) if (x, y, z) not in r1]
+    for (x, y, z) in sorted(r1):
+        leaves += [(1, 2 * x + a, 2 * y + b, 2 * z + c) for c in range(2) for b in range(2) for a in range(2)
+                   if (2 * x + a, 2 * y + b, 2 * z + c) not in r2]
+    for (x, y, z) in sorted(r2):
+        leaves += [(2, 2 * x + a, 2 * y + b, 2 * z + c) for c in range(2) for b in range(2) for a in range(2)]
+    t0 = time.time()
+    t = smr.SMRTree((8, 8, 8), leaves)
+    nb = len(leaves)
+    sb = agx.SMRBoundary(t, nx, ng, 5.0 / 3.0, 1e-8, 1e-8)
+    setup = time.time() - t0
+    n, nc = nx + 2 * ng, nx // 2 + 2 * ng
+    rs = np.random.default_rng(7)
+    D = dict(u=agx.DeviceArray(rs.random((nb, 5, n, n, n)) + 1.0), w=agx.DeviceArray(np.zeros((nb, 5, n, n, n))),
+             bcc=agx.DeviceArray(np.zeros((nb, 3, n, n, n))), cu=agx.DeviceArray(np.ones((nb, 5, nc, nc, nc))),
+             cw=agx.DeviceArray(np.zeros((nb, 5, nc, nc, nc))), cbcc=agx.DeviceArray(np.zeros((nb, 3, nc, nc, nc))))
+    b = [agx.DeviceArray(rs.standard_normal(s_)) for s_ in ((nb, n, n, n + 1), (nb, n, n + 1, n), (nb, n + 1, n, n))]
+    cb = [agx.DeviceArray(np.zeros(s_)) for s_ in ((nb, nc, nc, nc + 1), (nb, nc, nc + 1, nc), (nb, nc + 1, nc, nc))]
+    e = [agx.DeviceArray(rs.standard_normal(s_)) for s_ in ((nb, n + 1, n + 1, n), (nb, n + 1, n, n + 1),
+                                                           (nb, n, n + 1, n + 1))]
+    P = {k: v.ptr for k, v in D.items()}
+    args_ = (P["u"], [x.ptr for x in b], P["w"], P["bcc"], P["cu"], [x.ptr for x in cb], P["cw"], P["cbcc"])
+    ms = timed(lambda: sb.boundary_step(*args_), sb.sync)
+    ms_emf = timed(lambda: sb.correct_emf(*[x.ptr for x in e]), sb.sync)
+    out = {"workload": f"{nb} blocks of {nx}^3 (ng {ng}) on three levels: 8^3 root blocks, the central 2^3 "
+                       f"refined, the central 2^3 of those refined again", "blocks": nb,
+           "mhd_boundary_step_ms": ms, "emf_correction_ms": ms_emf, "slots": len(sb.slots),
+           "host_setup_s": setup}
+    log(f"[smr] three levels, {nb} blocks of {nx}^3: MHD boundary step {ms:.3f} ms, EMF correction "
+        f"{ms_emf:.3f} ms (tree, plans and lists built in {setup:.1f} s)")
+    sb.close()
+    for a in list(D.values()) + b + cb + e:
+        a.close()
     return out
 
 
