@@ -975,7 +975,7 @@ k_mb_prolong(const double *__restrict__ cu, const double *__restrict__ cuold, do
 // arithmetic of k_mb_bc, k_mb_fas, k_mb_smooth, k_mb_restrict and k_mb_prolong, separated
 // by workgroup barriers, so the potential equals the per-level launches bit for bit.
 // Every barrier is reached by every thread: loops have workgroup-uniform bounds.
-constexpr int MB_COARSE_CELLS = 4096, MB_MAXC = 8;
+constexpr int MB_COARSE_CELLS = 512, MB_MAXC = 8;
 struct CoarseB {
   double *u[MB_MAXC], *src[MB_MAXC], *uold[MB_MAXC];
   const double *bcv[MB_MAXC];

@@ -206,7 +206,7 @@ int agx_set_tile(agx_mesh *m, int32_t tile);
 /* multigrid red-black pass form: -1 by level size (fused on FP64 levels of >= 2^26
  * cells, the default), 0 two sweeps, 1 fused on every level of >= 2^21 cells */
 int agx_set_mg_fuse(agx_mesh *m, int32_t mode);
-/* Multipole V-cycles: run the coarse levels (at most 4096 cells each) as one
+/* Multipole V-cycles: run the coarse levels (at most 512 cells each) as one
  * single-workgroup launch (on = 1, the default) or one launch per step (0).  The
  * potentials are identical bit for bit; no counterpart in the reference. */
 int agx_set_mg_coarse(agx_mesh *m, int32_t on);
