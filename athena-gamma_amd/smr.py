@@ -768,6 +768,13 @@ def mhd_prolong_plan(tree, nx, ng, cc=None, fc=None):
                     fm = 1 if tree.find(g, tuple(e)) is not None else 0
                     e[d] = 1
                     fp = 1 if tree.find(g, tuple(e)) is not None else 0
+                    if not (fm and fp):
+                        # ApplyPhysicalBoundariesOnCoarseLevel (:401-445) would run the
+                        # domain's boundary function on the coarse buffer here
+                        raise ValueError(f"block {g} has a coarser neighbour and a physical x{d + 1} "
+                                         "boundary: boundary functions on the coarse buffer are "
+                                         "not built (periodic meshes, or refinement away from "
+                                         "the domain boundary)")
                 else:
                     fm = fp = 1
                 r[d] = (r[d][0] - fm, r[d][1] + fp)

@@ -17,6 +17,7 @@ import pytest
 
 from oracle import orc
 from tests import ghost_cases as gc
+from tests.conftest import load_smr
 
 NX, NG = 8, 2
 
@@ -88,3 +89,17 @@ def test_cons_to_prim_inverts_and_floors():
     assert np.allclose(cw.transpose(1, 0, 2, 3, 4)[:, mask], w.transpose(1, 0, 2, 3, 4)[:, mask], rtol=1e-12, atol=1e-13)
     assert cw[3, 0, 3, 3, 3] == gc.DFLOOR and uu[3, 0, 3, 3, 3] == gc.DFLOOR
     assert cw[4, 4, 2, 3, 4] == gc.PFLOOR and uu[4, 4, 2, 3, 4] > 0.0
+
+
+def test_physical_boundary_on_coarse_buffer_rejected():
+    """a refined block on the domain boundary next to a coarser block would need the
+    domain's boundary functions on its coarse buffer (bvals_refine.cpp:401-441): the plan
+    refuses it; refined blocks away from the boundary, and periodic meshes, are accepted"""
+    smr = load_smr()
+    with pytest.raises(ValueError, match="physical x"):
+        smr.mhd_prolong_plan(gc.tree(((0, 1, 1),)), 8, 2)
+    slots, _ = smr.mhd_prolong_plan(gc.tree(((1, 1, 1),)), 8, 2)
+    assert slots
+    t = smr.SMRTree(gc.NRB, gc.tree(((0, 1, 1),)).leaves, periodic=(True, True, True))
+    slots, _ = smr.mhd_prolong_plan(t, 8, 2)
+    assert slots
