@@ -110,6 +110,14 @@ struct MGLevelB {
   double *bcv;             // multipole boundary values of each ghost's chain (3 per ghost)
 };
 
+// FP32 framed level of the mixed-precision multipole form: the correction, its source
+// and the FAS copy; the ghosts hold the homogeneous multipole condition of the correction
+struct MGLevelB32 {
+  int n1, n2, n3;
+  double dx;
+  float *u, *src, *uold;
+};
+
 struct Timer {
   bool enabled = false;
   std::vector<hipEvent_t> ev;        // pairs
@@ -164,6 +172,8 @@ struct agx_mesh {
   bool mp_on = false;
   std::vector<agx::MGLevelB> levb;
   double *mgpoolb = nullptr;
+  std::vector<agx::MGLevelB32> levb32;   // mixed precision with multipole boundaries
+  float *mgpoolb32 = nullptr;
   double *mpdev = nullptr;           // [0..2] origin, [3..27] coefficients, [28..] partials
   double rdx_root = 0.0, rdy_root = 0.0, rdz_root = 0.0;
   double rdx_blk = 0.0, rdy_blk = 0.0, rdz_blk = 0.0;

@@ -890,10 +890,23 @@ k_mb_bc(double *__restrict__ u, MPGeom G, const double *__restrict__ bcv) {
   u[BX(n1, n2, c.k, c.j, c.i)] = v;
 }
 
-// red-black update of the framed level (MGGravity::Smooth, mg_gravity.cpp:220-273)
+// The same chain for a correction of the potential: the boundary values are those of
+// the potential itself, so the correction's ghosts take 2 * 0 - (the value inside).
 __global__ void __launch_bounds__(BS)
-k_mb_smooth(double *__restrict__ u, const double *__restrict__ src, int n1, int n2, int n3, double dx2,
-            double isix, int color) {
+k_mb_bc_hom(float *__restrict__ u, MPGeom G) {
+  const unsigned q = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  GhostChain c;
+  if (!ghost_chain(G, q, c)) return;
+  const int n1 = G.n1, n2 = G.n2;
+  float v = u[BX(n1, n2, c.kk, c.jj, c.ii)];
+  for (int d = c.depth - 1; d >= 0; --d) v = -v;
+  u[BX(n1, n2, c.k, c.j, c.i)] = v;
+}
+
+// red-black update of the framed level (MGGravity::Smooth, mg_gravity.cpp:220-273)
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_mb_smooth(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3, T dx2, T isix, int color) {
   const int nh = (n1 + 1) / 2;
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (t >= (unsigned)nh * n2 * n3) return;
@@ -907,18 +920,18 @@ k_mb_smooth(double *__restrict__ u, const double *__restrict__ src, int n1, int 
 }
 
 
-__device__ __forceinline__ double mb_defect(const double *u, const double *src, unsigned id, unsigned sj,
-                                            unsigned sk, double idx2) {
-  return (6.0 * u[id] - u[id + sk] - u[id + sj] - u[id + 1] - u[id - sk] - u[id - sj] - u[id - 1]) * idx2 +
+template <typename T>
+__device__ __forceinline__ T mb_defect(const T *u, const T *src, unsigned id, unsigned sj, unsigned sk, T idx2) {
+  return (T(6.0) * u[id] - u[id + sk] - u[id + sj] - u[id + 1] - u[id - sk] - u[id - sj] - u[id - 1]) * idx2 +
          src[id];
 }
 
 // FAS: uold = u over the framed level (StoreOldData, multigrid.cpp:645-653), then
 // src -= (6u - sum of neighbours) idx2 on the interior (CalculateFASRHS,
 // mg_gravity.cpp:324-351)
+template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mb_fas(const double *__restrict__ u, double *__restrict__ src, double *__restrict__ uold, int n1, int n2,
-         int n3, double idx2) {
+k_mb_fas(const T *__restrict__ u, T *__restrict__ src, T *__restrict__ uold, int n1, int n2, int n3, T idx2) {
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   const unsigned w1 = n1 + 2, w2 = n2 + 2;
   if (t >= w1 * w2 * (unsigned)(n3 + 2)) return;
@@ -926,14 +939,15 @@ k_mb_fas(const double *__restrict__ u, double *__restrict__ src, double *__restr
   const int i = t % w1, r = t / w1, j = r % w2, k = r / w2;
   if (i < 1 || i > n1 || j < 1 || j > n2 || k < 1 || k > n3) return;
   const unsigned sj = w1, sk = w1 * w2;
-  src[t] -= (6.0 * u[t] - u[t + sk] - u[t + sj] - u[t + 1] - u[t - sk] - u[t - sj] - u[t - 1]) * idx2;
+  src[t] -= (T(6.0) * u[t] - u[t + sk] - u[t + sj] - u[t + 1] - u[t - sk] - u[t - sj] - u[t - 1]) * idx2;
 }
 
 // RestrictBlock (multigrid.cpp:331-355): defect average into the coarse source and, in
 // FAS, the average of u into the coarse u.  One thread per coarse interior cell.
+template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mb_restrict(const double *__restrict__ u, const double *__restrict__ src, double *__restrict__ csrc,
-              double *__restrict__ cu, int n1, int n2, int n3, double idx2) {
+k_mb_restrict(const T *__restrict__ u, const T *__restrict__ src, T *__restrict__ csrc, T *__restrict__ cu,
+              int n1, int n2, int n3, T idx2) {
   const int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (t >= (unsigned)c1 * c2 * c3) return;
@@ -944,19 +958,19 @@ k_mb_restrict(const double *__restrict__ u, const double *__restrict__ src, doub
   const unsigned f[8] = {F(fk, fj, fi), F(fk, fj, fi + 1), F(fk, fj + 1, fi), F(fk, fj + 1, fi + 1),
                          F(fk + 1, fj, fi), F(fk + 1, fj, fi + 1), F(fk + 1, fj + 1, fi), F(fk + 1, fj + 1, fi + 1)};
 #undef F
-  double d[8];
+  T d[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) d[q] = mb_defect(u, src, f[q], sj, sk, idx2);
   const unsigned c = BX(c1, c2, k, j, i);
-  csrc[c] = 0.125 * (d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7]);
-  cu[c] = 0.125 * (u[f[0]] + u[f[1]] + u[f[2]] + u[f[3]] + u[f[4]] + u[f[5]] + u[f[6]] + u[f[7]]);
+  csrc[c] = T(0.125) * (d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7]);
+  cu[c] = T(0.125) * (u[f[0]] + u[f[1]] + u[f[2]] + u[f[3]] + u[f[4]] + u[f[5]] + u[f[6]] + u[f[7]]);
 }
 
 // ProlongateAndCorrect with FAS (multigrid.cpp:361-379, 733-829): the fine interior
 // gains the trilinear interpolation of (u - uold) of the framed coarse level.
+template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mb_prolong(const double *__restrict__ cu, const double *__restrict__ cuold, double *__restrict__ fu,
-             int n1, int n2, int n3) {
+k_mb_prolong(const T *__restrict__ cu, const T *__restrict__ cuold, T *__restrict__ fu, int n1, int n2, int n3) {
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (t >= (unsigned)n1 * n2 * n3) return;
   const int fi = 1 + t % n1, r = t / n1, fj = 1 + r % n2, fk = 1 + r / n2;
@@ -965,9 +979,36 @@ k_mb_prolong(const double *__restrict__ cu, const double *__restrict__ cuold, do
   const int di = ((fi - 1) & 1) ? 1 : -1, dj = ((fj - 1) & 1) ? 1 : -1, dk = ((fk - 1) & 1) ? 1 : -1;
 #define C(kk, jj, ii) (cu[BX(c1, c2, kk, jj, ii)] - cuold[BX(c1, c2, kk, jj, ii)])
   fu[BX(n1, n2, fk, fj, fi)] +=
-      0.015625 * (27.0 * C(k, j, i) + C(k + dk, j + dj, i + di) + 9.0 * (C(k, j, i + di) + C(k, j + dj, i) + C(k + dk, j, i)) +
-                  3.0 * (C(k + dk, j + dj, i) + C(k + dk, j, i + di) + C(k, j + dj, i + di)));
+      T(0.015625) * (T(27.0) * C(k, j, i) + C(k + dk, j + dj, i + di) +
+                     T(9.0) * (C(k, j, i + di) + C(k, j + dj, i) + C(k + dk, j, i)) +
+                     T(3.0) * (C(k + dk, j + dj, i) + C(k + dk, j, i + di) + C(k, j + dj, i + di)));
 #undef C
+}
+
+// Mixed precision with multipole boundaries: the FP64 defect of the framed finest level
+// (ghosts filled) becomes the FP32 source of the correction cycle, whose correction
+// starts at zero over the framed level.
+__global__ void __launch_bounds__(BS)
+k_mb_defect32(const double *__restrict__ u, const double *__restrict__ src, float *__restrict__ src32,
+              float *__restrict__ e32, int n1, int n2, int n3, double idx2) {
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  const unsigned w1 = n1 + 2, w2 = n2 + 2;
+  if (t >= w1 * w2 * (unsigned)(n3 + 2)) return;
+  e32[t] = 0.f;
+  const int i = t % w1, r = t / w1, j = r % w2, k = r / w2;
+  if (i < 1 || i > n1 || j < 1 || j > n2 || k < 1 || k > n3) return;
+  src32[t] = (float)mb_defect(u, src, t, w1, w1 * w2, idx2);
+}
+
+// phi += e on the interior of the framed finest level
+__global__ void __launch_bounds__(BS)
+k_mb_add32(double *__restrict__ u, const float *__restrict__ e, int n1, int n2, int n3) {
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  const unsigned w1 = n1 + 2, w2 = n2 + 2;
+  if (t >= w1 * w2 * (unsigned)(n3 + 2)) return;
+  const int i = t % w1, r = t / w1, j = r % w2, k = r / w2;
+  if (i < 1 || i > n1 || j < 1 || j > n2 || k < 1 || k > n3) return;
+  u[t] += (double)e[t];
 }
 
 // The coarse end of a multipole V(1,1) cycle (vcycle_b) in one workgroup: levels `top`
@@ -1380,14 +1421,14 @@ static void smooth_b(agx_mesh *m, int l, int color) {
   MGLevelB &B = m->levb[l];
   const int64_t n = (int64_t)((B.n1 + 1) / 2) * B.n2 * B.n3;
   timer_begin(m, AGX_K_MG_SMOOTH);
-  hipLaunchKernelGGL(k_mb_smooth, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, B.u, B.src, B.n1, B.n2, B.n3,
+  hipLaunchKernelGGL(k_mb_smooth<double>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, B.u, B.src, B.n1, B.n2, B.n3,
                      B.dx * B.dx, 1.15 / 6.0, color ^ m->coffset);
   timer_end(m);
 }
 static void fas_b(agx_mesh *m, int l) {
   MGLevelB &B = m->levb[l];
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mb_fas, dim3(grid1d(bcells(B), BS)), dim3(BS), 0, m->stream, B.u, B.src, B.uold, B.n1,
+  hipLaunchKernelGGL(k_mb_fas<double>, dim3(grid1d(bcells(B), BS)), dim3(BS), 0, m->stream, B.u, B.src, B.uold, B.n1,
                      B.n2, B.n3, 1.0 / (B.dx * B.dx));
   timer_end(m);
 }
@@ -1439,7 +1480,7 @@ static void vcycle_b(agx_mesh *m) {
       smooth_b(m, l, 1);
       bc_b(m, l);
       timer_begin(m, AGX_K_MG_RESTRICT);
-      hipLaunchKernelGGL(k_mb_restrict, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream,
+      hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream,
                          F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
       timer_end(m);
       m->current--;
@@ -1451,7 +1492,7 @@ static void vcycle_b(agx_mesh *m) {
       MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
       bc_b(m, l - 1);
       timer_begin(m, AGX_K_MG_PROLONG);
-      hipLaunchKernelGGL(k_mb_prolong, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, C.u,
+      hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, C.u,
                          C.uold, F.u, F.n1, F.n2, F.n3);
       timer_end(m);
       bc_b(m, l);
@@ -1473,7 +1514,7 @@ static void vcycle_b(agx_mesh *m) {
     smooth_b(m, l, 1);
     bc_b(m, l);
     timer_begin(m, AGX_K_MG_RESTRICT);
-    hipLaunchKernelGGL(k_mb_restrict, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream,
+    hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream,
                        F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
     timer_end(m);
     m->current--;
@@ -1494,7 +1535,7 @@ static void vcycle_b(agx_mesh *m) {
     MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
     bc_b(m, l - 1);
     timer_begin(m, AGX_K_MG_PROLONG);
-    hipLaunchKernelGGL(k_mb_prolong, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, C.u,
+    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, C.u,
                        C.uold, F.u, F.n1, F.n2, F.n3);
     timer_end(m);
     bc_b(m, l);
@@ -1529,6 +1570,98 @@ static void fmg_b(agx_mesh *m) {
     }
   }
   m->fmglevel = L - 1;
+}
+
+// ---- mixed precision with multipole boundaries: FP32 V(1,1) cycles with FAS on the
+// correction (ghosts by the homogeneous chain), inside the FP64 defect correction
+static void bc_b32(agx_mesh *m, int l) {
+  timer_begin(m, AGX_K_MG_OTHER);
+  hipLaunchKernelGGL(k_mb_bc_hom, dim3(grid1d(bghosts(m->levb[l]), BS)), dim3(BS), 0, m->stream, m->levb32[l].u,
+                     mp_geom(m, m->levb[l]));
+  timer_end(m);
+}
+static void smooth_b32(agx_mesh *m, int l, int color) {
+  MGLevelB32 &B = m->levb32[l];
+  const int64_t n = (int64_t)((B.n1 + 1) / 2) * B.n2 * B.n3;
+  timer_begin(m, AGX_K_MG_SMOOTH);
+  hipLaunchKernelGGL(k_mb_smooth<float>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, B.u, B.src, B.n1, B.n2,
+                     B.n3, float(B.dx * B.dx), float(1.15 / 6.0), color ^ m->coffset);
+  timer_end(m);
+}
+static void fas_b32(agx_mesh *m, int l) {
+  MGLevelB32 &B = m->levb32[l];
+  timer_begin(m, AGX_K_MG_OTHER);
+  hipLaunchKernelGGL(k_mb_fas<float>, dim3(grid1d(bcells(m->levb[l]), BS)), dim3(BS), 0, m->stream, B.u, B.src,
+                     B.uold, B.n1, B.n2, B.n3, float(1.0 / (B.dx * B.dx)));
+  timer_end(m);
+}
+static void vcycle_b32(agx_mesh *m) {   // the per-level sequence of vcycle_b
+  const int start = m->current;
+  m->coffset ^= 1;
+  while (m->current > 0) {
+    const int l = m->current;
+    MGLevelB32 &F = m->levb32[l], &C = m->levb32[l - 1];
+    bc_b32(m, l);
+    if (l < m->fmglevel) fas_b32(m, l);
+    smooth_b32(m, l, 0);
+    bc_b32(m, l);
+    smooth_b32(m, l, 1);
+    bc_b32(m, l);
+    timer_begin(m, AGX_K_MG_RESTRICT);
+    hipLaunchKernelGGL(k_mb_restrict<float>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0,
+                       m->stream, F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, float(1.0 / (F.dx * F.dx)));
+    timer_end(m);
+    m->current--;
+  }
+  const Geo &g = m->g;
+  const int ni = std::max(g.nb1, std::max(g.nb2, g.nb3)) >> (m->nrootlevel - 1);
+  bc_b32(m, 0);
+  fas_b32(m, 0);
+  for (int it = 0; it < ni; ++it) {
+    smooth_b32(m, 0, 0);
+    bc_b32(m, 0);
+    smooth_b32(m, 0, 1);
+    bc_b32(m, 0);
+  }
+  while (m->current < start) {
+    const int l = m->current + 1;
+    MGLevelB32 &F = m->levb32[l], &C = m->levb32[l - 1];
+    bc_b32(m, l - 1);
+    timer_begin(m, AGX_K_MG_PROLONG);
+    hipLaunchKernelGGL(k_mb_prolong<float>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0,
+                       m->stream, C.u, C.uold, F.u, F.n1, F.n2, F.n3);
+    timer_end(m);
+    bc_b32(m, l);
+    smooth_b32(m, l, 0);
+    bc_b32(m, l);
+    smooth_b32(m, l, 1);
+    bc_b32(m, l);
+    m->current++;
+  }
+}
+
+// One defect-correction step: ghosts of phi from the multipole expansion, d = defect in
+// FP64, FP32 V(1,1) on the correction from zero, phi += e, ghosts again (the defect norm
+// and the next step read them).
+static void correction_step_b(agx_mesh *m) {
+  const int L = m->ntotal;
+  MGLevelB &F = m->levb[L - 1];
+  MGLevelB32 &E = m->levb32[L - 1];
+  bc_b(m, L - 1);
+  timer_begin(m, AGX_K_MG_OTHER);
+  hipLaunchKernelGGL(k_mb_defect32, dim3(grid1d(bcells(F), BS)), dim3(BS), 0, m->stream, F.u, F.src, E.src,
+                     E.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
+  timer_end(m);
+  const int keep = m->fmglevel;
+  m->current = L - 1;
+  m->fmglevel = L - 1;
+  vcycle_b32(m);
+  m->fmglevel = keep;
+  timer_begin(m, AGX_K_MG_OTHER);
+  hipLaunchKernelGGL(k_mb_add32, dim3(grid1d(bcells(F), BS)), dim3(BS), 0, m->stream, F.u, E.u, F.n1, F.n2,
+                     F.n3);
+  timer_end(m);
+  bc_b(m, L - 1);
 }
 
 // defect norm of the framed finest level (reads its ghost frame as it stands)
@@ -1585,7 +1718,8 @@ static int mg_solve_b(agx_mesh *m) {
     double def = defect_norm_b(m);
     int n = 0;
     while (def > m->mg_eps) {
-      vcycle_b(m);
+      if (m->mixed) correction_step_b(m);
+      else vcycle_b(m);
       iters++;
       double olddef = def;
       def = defect_norm_b(m);
@@ -1598,7 +1732,8 @@ static int mg_solve_b(agx_mesh *m) {
     m->last_norm = def;
   } else {   // SolveIterativeFixedTimes, multigrid_driver.cpp:762-771
     for (int n = 0; n < m->mg_niter; ++n) {
-      vcycle_b(m);
+      if (m->mixed) correction_step_b(m);
+      else vcycle_b(m);
       iters++;
     }
     m->last_norm = defect_norm_b(m);
@@ -1695,12 +1830,25 @@ int mg_create(agx_mesh *m) {
     AGX_HIP(hipMalloc(&m->red, sizeof(double) * m->red_n));
   }
   if (d.mg_bc == 1) {
-    if (m->mixed) {
-      set_error("multipole multigrid boundaries run in FP64 only (precision = 0)");
-      return -22;
-    }
     int rc = mp_create(m);
     if (rc) return rc;
+    if (m->mixed) {   // FP32 framed levels: correction, source, FAS copy
+      m->levb32.resize(L);
+      int64_t t32 = 0;
+      for (int l = 0; l < L; ++l) t32 += 3 * pad32(bcells(m->levb[l]));
+      AGX_HIP(hipMalloc(&m->mgpoolb32, t32 * (int64_t)sizeof(float)));
+      AGX_HIP(hipMemsetAsync(m->mgpoolb32, 0, t32 * sizeof(float), m->stream));
+      m->mg_bytes += t32 * (int64_t)sizeof(float);
+      float *q = m->mgpoolb32;
+      for (int l = 0; l < L; ++l) {
+        MGLevelB32 &B = m->levb32[l];
+        B.n1 = m->levb[l].n1; B.n2 = m->levb[l].n2; B.n3 = m->levb[l].n3; B.dx = m->levb[l].dx;
+        const int64_t n = pad32(bcells(m->levb[l]));
+        B.u = q; q += n;
+        B.src = q; q += n;
+        B.uold = q; q += n;
+      }
+    }
   }
   AGX_HIP(hipStreamSynchronize(m->stream));
   m->mg_on = 1;
@@ -1710,9 +1858,12 @@ int mg_create(agx_mesh *m) {
 void mg_destroy(agx_mesh *m) {
   if (m->mgpoolb) hipFree(m->mgpoolb);
   if (m->mpdev) hipFree(m->mpdev);
+  if (m->mgpoolb32) hipFree(m->mgpoolb32);
   m->mgpoolb = nullptr;
+  m->mgpoolb32 = nullptr;
   m->mpdev = nullptr;
   m->levb.clear();
+  m->levb32.clear();
   m->mp_on = false;
   if (m->mgpool) hipFree(m->mgpool);
   if (m->mgpool32) hipFree(m->mgpool32);
@@ -2115,7 +2266,10 @@ int mg_autotune(agx_mesh *m) {
 int mg_vcycles(agx_mesh *m, int n) {
   m->current = m->ntotal - 1;
   if (m->mp_on) {
-    for (int q = 0; q < n; ++q) vcycle_b(m);
+    for (int q = 0; q < n; ++q) {
+      if (m->mixed) correction_step_b(m);
+      else vcycle_b(m);
+    }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "mg_vcycles launches");
   }

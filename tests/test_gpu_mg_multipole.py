@@ -131,3 +131,41 @@ def test_coarse_kernel_identical(agx, mode, thr, nit):
         m.close()
     print(f"mode {mode}: cycles {out[0][1]} vs {out[1][1]}", flush=True)
     assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+
+
+@pytest.mark.parametrize("thr", ["stall", "absolute"])
+def test_mixed_precision_matches(agx, thr):
+    """Mixed precision with multipole boundaries: FP64 FMG, then FP32 V(1,1) correction
+    cycles (FAS, the correction's ghosts by the homogeneous multipole chain) inside the
+    FP64 defect correction.  The fixed point is the FP64 discrete solution with the
+    multipole boundary values, so the potential agrees with the FP64 oracle's to the
+    accuracy of the stopping rule: threshold 0 (iterate until the defect stalls) within
+    1e-9 of the potential's maximum; an absolute threshold of 1e-10 x ||source||, reached
+    by both, within 1e-8 (the bar of the periodic mixed form)."""
+    n = 32
+    rho = gaussian(n, 0.06, c=(-0.05, 0.03, 0.02))
+    src_norm = float(np.sqrt(np.mean(rho ** 2)))
+    eps = 0.0 if thr == "stall" else 1e-10 * src_norm
+    kw = dict(nx=(n, n, n), bnx=(8, 8, 8), box=BOX, self_gravity=1, mg_bc=1, mg_mode=0, four_pi_G=1.0,
+              mporder=4, mg_threshold=eps)
+    U = np.zeros((5, n, n, n))
+    U[0] = rho
+    d64 = make_desc(**kw)
+    ub = agx.scatter_cells(d64, U)
+    o = Oracle(d64)
+    o.set("u", ub)
+    o.mg_solve(0)
+    m = agx.Mesh(make_desc(precision=1, **kw))
+    m.set("u", ub)
+    m.mg_solve(0)
+    a, b = m.get("phi"), o.get("phi")
+    err = np.abs(a - b).max() / np.abs(b).max()
+    (nm, im), (no, io) = m.mg_last(), o.mg_last()
+    print(f"{thr}: phi rel diff {err:.3e}; mixed {im} cycles, defect {nm:.3e}; FP64 oracle {io} cycles, "
+          f"defect {no:.3e}", flush=True)
+    assert im <= io + 3   # FP32 corrections cost at most a few more cycles than FP64 V-cycles
+    if thr == "stall":
+        assert err <= 1e-9
+    else:
+        assert nm <= eps and no <= eps
+        assert err <= 1e-8
