@@ -44,6 +44,7 @@ EXPORTS = [
     "agx_ref_cons_to_prim", "agx_stream_create", "agx_stream_destroy", "agx_graph_begin", "agx_graph_end",
     "agx_graph_launch", "agx_graph_destroy", "agx_ref_restrict_all", "agx_ref_prolong_all",
     "agx_ref_internal_to_cons", "agx_mg_set_source_mask_sphere", "agx_set_mg_coarse",
+    "agx_history", "agx_history_sum",
 ]
 
 
@@ -192,6 +193,9 @@ def lib():
     L.agx_ref_cons_to_prim.argtypes = [P, P, P, P, P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int32, P]
     L.agx_ref_unpack_fc.argtypes = [P, P, P, P, P, ctypes.c_int64, P]
+    L.agx_history.argtypes = [P, D, ctypes.c_int32]
+    L.agx_history_sum.argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.c_int32, ctypes.c_int32, P, P, P]
     L.agx_device_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(P)]
     L.agx_device_free.argtypes = [P]
     L.agx_device_copy.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
@@ -376,6 +380,14 @@ class Mesh:
 
     def device_bytes(self):
         return self.L.agx_mesh_bytes(self.h)
+
+    def history(self):
+        """The built-in history sums (agx_history): mass, 3 momenta, 3 kinetic energies,
+        [total energy], [gravitational energy], 3 magnetic energies."""
+        out = np.zeros(16)
+        _check(self.L.agx_history(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 16), "agx_history")
+        n = (4 if self.desc.eos == 1 else 5) + 3 + (1 if self.desc.self_gravity else 0) + 3
+        return out[:n].copy()
 
 
 # ---------------------------------------------------------------- host utilities

@@ -118,6 +118,12 @@ struct MGLevelB32 {
   float *u, *src, *uold;
 };
 
+// History sums (agx_diag.hip): at most HIST_NW_MAX workgroups, each writing HIST_NV
+// partial sums (5 hydro + 3 kinetic + 1 gravitational + 3 magnetic at most).
+constexpr int HIST_NW_MAX = 2048;
+constexpr int HIST_NV = 12;
+constexpr int64_t HIST_PARTIAL_DOUBLES = (int64_t)HIST_NW_MAX * HIST_NV;
+
 struct Timer {
   bool enabled = false;
   std::vector<hipEvent_t> ev;        // pairs
@@ -135,7 +141,7 @@ struct agx_mesh {
   agx::Arrays arr;
   bool mixed = false;                // AGX_PREC_MIXED
   bool iso = false;                  // isothermal EOS (NHYDRO = 4)
-  int tile = 0;                      // stage-kernel tile shape (agx_autotune)
+  int tile = 5;                      // stage form and tile shape (agx_autotune; 5 = k_vl2, 32 x 8)
   hipStream_t stream = nullptr;
   double *pool = nullptr;            // all MHD arrays
   int64_t pool_bytes = 0;
@@ -145,6 +151,8 @@ struct agx_mesh {
   double *scal_host = nullptr;       // pinned host mirror
   double *red = nullptr;             // reduction partials
   int64_t red_n = 0;
+  double *hist = nullptr;            // history partial sums + result (agx_history)
+  double *cellvol = nullptr;         // [nb] cell volume dx1*dx2*dx3 of each MeshBlock
   double *dtpart = nullptr;          // per-workgroup time-step partials of the fused stage
   int64_t dtpart_n = 0;
   std::vector<agx::BlockCoef> coef_h;
@@ -170,6 +178,7 @@ struct agx_mesh {
   int64_t mg_bytes = 0;
   // multipole boundaries (mg_bc = 1): framed levels, coefficients and origin on the device
   bool mp_on = false;
+  bool mp_solved = false;            // a multipole solve has loaded the levels (mg_vcycles needs it)
   std::vector<agx::MGLevelB> levb;
   double *mgpoolb = nullptr;
   std::vector<agx::MGLevelB32> levb32;   // mixed precision with multipole boundaries
@@ -193,6 +202,14 @@ namespace agx {
 
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what);
+
+// history sums over dense AthenaArray inputs (agx_diag.hip)
+int history_check_dense(const double *u, const double *bcc, int32_t nblocks, int32_t nhydro, int32_t nx1,
+                        int32_t nx2, int32_t nx3, int32_t ng, const double *vol, const double *hst,
+                        const char *who);
+int history_dense(const double *u, const double *bcc, const double *phi, int32_t nblocks, int32_t nhydro,
+                  int32_t nx1, int32_t nx2, int32_t nx3, int32_t ng, const double *vol, double *partial,
+                  double *hst, hipStream_t st);
 
 #define AGX_HIP(call)                                   \
   do {                                                  \

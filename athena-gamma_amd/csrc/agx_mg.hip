@@ -1711,6 +1711,7 @@ static int mg_solve_b(agx_mesh *m) {
   m->current = L - 1;
   m->fmglevel = L - 1;
   mp_coefficients(m);
+  m->mp_solved = true;
   if (m->mg_mode == 0) fmg_b(m);
   m->current = L - 1;
   int iters = 0;
@@ -2266,6 +2267,12 @@ int mg_autotune(agx_mesh *m) {
 int mg_vcycles(agx_mesh *m, int n) {
   m->current = m->ntotal - 1;
   if (m->mp_on) {
+    // the FAS levels, the boundary values and the source come from a solve
+    if (!m->mp_solved) {
+      set_error("agx_mg_vcycles: multipole boundaries need an agx_mg_solve first (it loads the source, "
+                "the multipole boundary values and the FAS levels)");
+      return -22;
+    }
     for (int q = 0; q < n; ++q) {
       if (m->mixed) correction_step_b(m);
       else vcycle_b(m);

@@ -108,11 +108,13 @@ static dim3 plane_grid(int il, int iu, int jl, int ju, int kl, int ku, int nb) {
   return dim3(grid1d(n, 256), (unsigned)(ku - kl + 1), (unsigned)nb);
 }
 
-// Stage-kernel tiles: TI x TJ (i,j) columns per workgroup of TI*TJ threads (256, or 512
-// for the 64 x 8 tile); grid.x = tiles of one MeshBlock, grid.y = MeshBlock.  The tile
-// shape is picked per mesh by agx_autotune.
-constexpr int NTILE = 4;
-constexpr int TILE_I[NTILE] = {16, 32, 64, 64}, TILE_J[NTILE] = {16, 8, 4, 8};
+// Stage forms and tiles: TI x TJ (i,j) columns per workgroup of TI*TJ threads (256, or
+// 512 for the 64 x 8 tile); grid.x = tiles of one MeshBlock, grid.y = MeshBlock.  Forms
+// 0-3 run the fluxes as three k_flux launches and the rest of the stage as k_stage with
+// the tile shown; forms 4-5 run the whole stage as one k_vl2 launch.  The form is picked
+// per mesh by agx_autotune (default: 5).
+constexpr int NTILE = 6, TILE_VL2 = 4, TILE_DEFAULT = 5;
+constexpr int TILE_I[NTILE] = {16, 32, 64, 64, 16, 32}, TILE_J[NTILE] = {16, 8, 4, 8, 16, 8};
 static dim3 stage_grid(const Geo &g, int tile) {
   int nti = (g.bx1 + TILE_I[tile] - 1) / TILE_I[tile], ntj = (g.bx2 + TILE_J[tile] - 1) / TILE_J[tile];
   return dim3((unsigned)(nti * ntj), (unsigned)g.nb);
@@ -858,10 +860,95 @@ __device__ __forceinline__ void roe(T gamma, const T *wli, const T *wri,
 }
 
 // ================================================================ kernels
-// Hydro::CalculateFluxes for direction DIR (hydro/calculate_fluxes.cpp:56-321):
-// reconstruction (ORDER 1: reconstruct/dc.cpp:24-104; ORDER 2: the uniform-mesh van
-// Leer branch of reconstruct/plm.cpp:27-351), Riemann solver, face EMFs and the CT
-// upwind weight.  One thread per face.
+// One face of Hydro::CalculateFluxes along direction dir (hydro/calculate_fluxes.cpp:
+// 56-321): reconstruction from the cells on both sides (ORDER 1: reconstruct/dc.cpp:
+// 24-104; ORDER 2: the uniform-mesh van Leer branch of reconstruct/plm.cpp:27-351;
+// ORDER 3: reconstruct/ppm.cpp:58-338), the Riemann solver, the face EMFs and the CT
+// upwind weight.  The states are assembled directly in the solver's local frame (IVX the
+// velocity along dir, IBY/IBZ the transverse bcc components of plm.cpp:46-56,160-170,
+// 273-283); each variable is reconstructed on its own, so this equals reconstructing in
+// the mesh frame and permuting.  dir may be a compile-time constant (k_flux) or a
+// run-time value (the fused stage kernel, one code path for every direction).
+// wb, bb: w and bcc of the block; id: index of the cell above the face; st: stride
+// along dir; pc: the block's PLM factor rows for dir; cidx: the cell index along dir.
+// Outputs: f[0..4] = mass, momentum along dir, dir+1, dir+2, energy flux; ey = -F(By),
+// ez = F(Bz) (hlld.cpp:371-377); wct = GetWeightForCT.
+template <int ORDER, int RS, typename T, bool ISO>
+__device__ __forceinline__ void face_flux(int dir, const T *__restrict__ wb, const T *__restrict__ bb,
+                                          int64_t S, int64_t id, int64_t st, const double *__restrict__ pc,
+                                          int PL, int cidx, T bxi, T gamma, T cs, T dfl, T pfl, T dx, T dt, T *f,
+                                          T &ey, T &ez, T &wct) {
+  const int d1 = (dir == 2) ? 0 : dir + 1, d2 = (dir == 0) ? 2 : dir - 1;   // (dir+1)%3, (dir+2)%3
+  T wli[NWAVE], wri[NWAVE];
+#pragma unroll
+  for (int n = 0; n < NWAVE; ++n) {
+    if (ISO && n == IPR) {   // no pressure in the isothermal state vector
+      wli[n] = wri[n] = T(0.0);
+      continue;
+    }
+    const T *q;
+    if (n == IDN) q = wb + id;
+    else if (n == IVX) q = wb + (int64_t)(IVX + dir) * S + id;
+    else if (n == IVY) q = wb + (int64_t)(IVX + d1) * S + id;
+    else if (n == IVZ) q = wb + (int64_t)(IVX + d2) * S + id;
+    else if (n == IPR) q = wb + (int64_t)IPR * S + id;
+    else if (n == IBY) q = bb + (int64_t)d1 * S + id;
+    else q = bb + (int64_t)d2 * S + id;
+    if (ORDER == 1) {
+      wli[n] = q[-st];
+      wri[n] = q[0];
+    } else if (ORDER == 3) {
+      T q6[6];
+#pragma unroll
+      for (int t = 0; t < 6; ++t) q6[t] = q[(t - 3) * st];
+      T m0, p0, m1, p1;
+      ppm_cell(q6, m0, p0);       // cell c-1: values at c-3..c+1
+      ppm_cell(q6 + 1, m1, p1);   // cell c:   values at c-2..c+2
+      wli[n] = p0;
+      wri[n] = m1;
+    } else {
+      T qm2 = q[-2 * st], qm1 = q[-st], q0 = q[0], qp1 = q[st];
+      // left state: upper face of cell c-1
+      T dwl = (qm1 - qm2), dwr = (q0 - qm1);
+      T dw2 = dwl * dwr;
+      T dwm = T(2.0) * dw2 / (dwl + dwr);
+      if (dw2 <= T(0.0)) dwm = T(0.0);
+      wli[n] = qm1 + T(pc[cidx - 1]) * dwm;
+      // right state: lower face of cell c
+      dwl = (q0 - qm1);
+      dwr = (qp1 - q0);
+      dw2 = dwl * dwr;
+      dwm = T(2.0) * dw2 / (dwl + dwr);
+      if (dw2 <= T(0.0)) dwm = T(0.0);
+      wri[n] = q0 - T(pc[PL + cidx]) * dwm;
+    }
+  }
+  if (ORDER == 3) {
+    // ApplyPrimitiveFloors on both reconstructed states (ppm.cpp:333-337); the
+    // isothermal EOS floors the density only (eos/isothermal_mhd.cpp:135-140)
+    wli[IDN] = (wli[IDN] > dfl) ? wli[IDN] : dfl;
+    wri[IDN] = (wri[IDN] > dfl) ? wri[IDN] : dfl;
+    if (!ISO) {
+      wli[IPR] = (wli[IPR] > pfl) ? wli[IPR] : pfl;
+      wri[IPR] = (wri[IPR] > pfl) ? wri[IPR] : pfl;
+    }
+  }
+  T flxi[NWAVE];
+  if (RS == AGX_RSOLVER_HLLE) hlle<ISO>(gamma, cs, wli, wri, bxi, flxi);
+  else if (ISO) hlld_iso(cs, dfl, wli, wri, bxi, flxi);
+  else if (RS == AGX_RSOLVER_ROE) roe(gamma, wli, wri, bxi, flxi);
+  else hlld(gamma, wli, wri, bxi, flxi);
+  f[0] = flxi[IDN];
+  f[1] = flxi[IVX];
+  f[2] = flxi[IVY];
+  f[3] = flxi[IVZ];
+  f[4] = ISO ? T(0.0) : flxi[IEN];
+  ey = -flxi[IBY];
+  ez = flxi[IBZ];
+  wct = ct_weight(flxi[IDN], wli[IDN], wri[IDN], dx, dt);
+}
+
+// Hydro::CalculateFluxes for direction DIR, one thread per face (face_flux).
 template <int DIR, int ORDER, int RS, typename T, bool ISO>
 __global__ void __launch_bounds__(256)
 k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
@@ -874,86 +961,25 @@ k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
   int kl = g.ks - (DIR != 2);
   PlaneIdx p;
   if (!plane_index(il, iu, jl, ju, kl, p, DIR == 2)) return;   // x3: 4-plane stencil
-  const T gamma = T(gamma_), cs = T(cs_), dfl = T(dfl_), pfl = T(pfl_);
   const int64_t st = (DIR == 0) ? 1 : (DIR == 1 ? (int64_t)g.P : (int64_t)g.J * g.P);
   const int64_t id = IDX(g, p.k, p.j, p.i);
-  const T *wb = w + (int64_t)p.b * NHYDRO * g.S;
-  const T *bb = bcc + (int64_t)p.b * 3 * g.S;
-  // transverse bcc components in the (IBY, IBZ) slots, plm.cpp:46-56,160-170,273-283
-  constexpr int TB1 = (DIR == 0) ? IB2 : (DIR == 1 ? IB3 : IB1);
-  constexpr int TB2 = (DIR == 0) ? IB3 : (DIR == 1 ? IB1 : IB2);
-  T wl[NWAVE], wr[NWAVE];
-  int cidx = (DIR == 0) ? p.i : (DIR == 1 ? p.j : p.k);   // index of the cell above the face
+  const int cidx = (DIR == 0) ? p.i : (DIR == 1 ? p.j : p.k);   // index of the cell above the face
   const double *pc = plm + ((int64_t)p.b * 3 + DIR) * 2 * g.PL;  // [dxp][dxm] rows
-#pragma unroll
-  for (int n = 0; n < NWAVE; ++n) {
-    if (ISO && n == IPR) {   // no pressure in the isothermal state vector
-      wl[n] = wr[n] = T(0.0);
-      continue;
-    }
-    const T *q;
-    if (n < NHYDRO) q = wb + (int64_t)n * g.S + id;
-    else q = bb + (int64_t)((n == IBY) ? TB1 : TB2) * g.S + id;
-    if (ORDER == 1) {
-      wl[n] = q[-st];
-      wr[n] = q[0];
-    } else if (ORDER == 3) {
-      T q6[6];
-#pragma unroll
-      for (int t = 0; t < 6; ++t) q6[t] = q[(t - 3) * st];
-      T m0, p0, m1, p1;
-      ppm_cell(q6, m0, p0);       // cell c-1: values at c-3..c+1
-      ppm_cell(q6 + 1, m1, p1);   // cell c:   values at c-2..c+2
-      wl[n] = p0;
-      wr[n] = m1;
-    } else {
-      T qm2 = q[-2 * st], qm1 = q[-st], q0 = q[0], qp1 = q[st];
-      // left state: upper face of cell c-1
-      T dwl = (qm1 - qm2), dwr = (q0 - qm1);
-      T dw2 = dwl * dwr;
-      T dwm = T(2.0) * dw2 / (dwl + dwr);
-      if (dw2 <= T(0.0)) dwm = T(0.0);
-      wl[n] = qm1 + T(pc[cidx - 1]) * dwm;
-      // right state: lower face of cell c
-      dwl = (q0 - qm1);
-      dwr = (qp1 - q0);
-      dw2 = dwl * dwr;
-      dwm = T(2.0) * dw2 / (dwl + dwr);
-      if (dw2 <= T(0.0)) dwm = T(0.0);
-      wr[n] = q0 - T(pc[g.PL + cidx]) * dwm;
-    }
-  }
-  if (ORDER == 3) {
-    // ApplyPrimitiveFloors on both reconstructed states (ppm.cpp:333-337); the
-    // isothermal EOS floors the density only (eos/isothermal_mhd.cpp:135-140)
-    wl[IDN] = (wl[IDN] > dfl) ? wl[IDN] : dfl;
-    wr[IDN] = (wr[IDN] > dfl) ? wr[IDN] : dfl;
-    if (!ISO) {
-      wl[IPR] = (wl[IPR] > pfl) ? wl[IPR] : pfl;
-      wr[IPR] = (wr[IPR] > pfl) ? wr[IPR] : pfl;
-    }
-  }
+  T f[5], fey, fez, fw;
+  face_flux<ORDER, RS, T, ISO>(DIR, w + (int64_t)p.b * NHYDRO * g.S, bcc + (int64_t)p.b * 3 * g.S, g.S, id, st, pc,
+                               g.PL, cidx, T(bn[(int64_t)p.b * g.S + id]), T(gamma_), T(cs_), T(dfl_), T(pfl_),
+                               T(coef[p.b].dx[DIR]), T(*dtp), f, fey, fez, fw);
   constexpr int ivx = IVX + DIR, ivy = IVX + (DIR + 1) % 3, ivz = IVX + (DIR + 2) % 3;
-  T wli[NWAVE], wri[NWAVE], flxi[NWAVE];
-  wli[IDN] = wl[IDN]; wli[IVX] = wl[ivx]; wli[IVY] = wl[ivy]; wli[IVZ] = wl[ivz];
-  wli[IPR] = wl[IPR]; wli[IBY] = wl[IBY]; wli[IBZ] = wl[IBZ];
-  wri[IDN] = wr[IDN]; wri[IVX] = wr[ivx]; wri[IVY] = wr[ivy]; wri[IVZ] = wr[ivz];
-  wri[IPR] = wr[IPR]; wri[IBY] = wr[IBY]; wri[IBZ] = wr[IBZ];
-  T bxi = T(bn[(int64_t)p.b * g.S + id]);
-  if (RS == AGX_RSOLVER_HLLE) hlle<ISO>(gamma, cs, wli, wri, bxi, flxi);
-  else if (ISO) hlld_iso(cs, dfl, wli, wri, bxi, flxi);
-  else if (RS == AGX_RSOLVER_ROE) roe(gamma, wli, wri, bxi, flxi);
-  else hlld(gamma, wli, wri, bxi, flxi);
   T *fb = flx + (int64_t)p.b * NHYDRO * g.S + id;
-  fb[IDN * g.S] = flxi[IDN];
-  fb[ivx * g.S] = flxi[IVX];
-  fb[ivy * g.S] = flxi[IVY];
-  fb[ivz * g.S] = flxi[IVZ];
-  if (!ISO) fb[IEN * g.S] = flxi[IEN];
+  fb[IDN * g.S] = f[0];
+  fb[ivx * g.S] = f[1];
+  fb[ivy * g.S] = f[2];
+  fb[ivz * g.S] = f[3];
+  if (!ISO) fb[IEN * g.S] = f[4];
   int64_t fo = (int64_t)p.b * g.S + id;
-  ey[fo] = -flxi[IBY];
-  ez[fo] = flxi[IBZ];
-  wct[fo] = ct_weight(flxi[IDN], wli[IDN], wri[IDN], T(coef[p.b].dx[DIR]), T(*dtp));
+  ey[fo] = fey;
+  ez[fo] = fez;
+  wct[fo] = fw;
 }
 
 // Field::ComputeCornerE, 3D Newtonian branch (field/calculate_corner_e.cpp:130-229).
@@ -1534,6 +1560,404 @@ k_stage(Geo g, StageArgs<TW> a) {
   }
 }
 
+// ================================================================ whole VL2 stage in one kernel
+// k_vl2 does everything of one VL2 stage after the ghost exchange: the fluxes of all
+// three directions (face_flux: reconstruction, Riemann solver, face EMFs, CT weights),
+// the corner EMFs, CT, the flux divergence with the self-gravity source, bcc, C2P with
+// floors, the ghost copies of u and the faces, and in the last stage the time-step bound.
+// Fluxes, face EMFs and weights live only in LDS and registers: per interior cell it
+// reads w, bcc, the faces and u (and phi) and writes u, w, bcc and the faces, so the HBM
+// traffic of a stage is that of k_stage minus the 24 flux words per cell, and the flux
+// kernels' own traffic is gone.  Every value is formed by the same operations, in the
+// same order, as the split path (k_flux + k_stage), so results are identical bit for bit.
+//
+// A workgroup owns a TI x TJ tile of (i,j) columns of one MeshBlock and walks the cell
+// planes c = ks-1 .. ke+1.  Per plane:
+//   F  faces of plane c: x1 faces i0..i0+ni of rows j0-1..j0+nj, x2 faces j0..j0+nj of
+//      columns i0-1..i0+ni, and x3 faces of face plane c (below cell plane c) of the tile
+//      and its one-cell ring without corners; then the cell-centred E of plane c.  Thread
+//      t solves the x1, x2 and x3 faces of its own column, and the rest (tile edges and
+//      ring, 5 ni + 5 nj + 4 faces) go one per thread.  Face EMFs and weights go to LDS;
+//      the hydro fluxes of a thread's own x1 and x3 faces stay in its registers, those of
+//      the x2 faces and of the tile's right-edge x1 faces go to LDS.
+//   E  corner EMFs: e1, e2 of edge plane c (from faces of planes c-1 and c) and e3 of
+//      plane c, one point per thread, as k_stage.
+//   U  CT of the own x3 face of face plane c; the update of the own cell of plane c-1
+//      (its x1/x2 flux differences were summed in the previous iteration, its x3 fluxes
+//      are the lower and upper faces solved in the previous and this iteration); then the
+//      x1/x2 flux differences of the own cell of plane c, and the face EMFs, weights and
+//      cell-centred E that the next plane's edges need are kept as "previous plane".
+// Three barriers per plane.  Thread (tx, ty) reads the x1 flux of face i+1 from lane
+// +1 (the same row of the same wavefront, since TI divides 64), the last column from LDS.
+template <typename TW>
+struct VL2Args {
+  const TW *w, *bcc;                 // primitives / bcc of the stage's fluxes
+  TW *wn, *bccn;                     // new primitives / bcc (interior)
+  const double *bn1, *bn2, *bn3;     // face field whose normal components enter the Riemann problems
+  const double *src;                 // conserved state the stage starts from
+  double *dst;                       // updated conserved state (interior + pushed ghosts)
+  const double *rho;                 // conserved state whose density feeds self-gravity
+  const double *phi;
+  const double *s1, *s2, *s3;        // CT source faces
+  double *d1, *d2, *d3;              // CT destination faces
+  const BlockCoef *coef;
+  const double *plm;                 // PLM face-offset factors [nb][3][2][PL]
+  const double *dtp;                 // device time step
+  double wfac;                       // stage weight: wght = wfac * dt
+  double gm1, dfl, pfl, gamma, cs;
+  int grav;
+  double *dtpart;                    // per-workgroup min of the dt bound, or null
+};
+
+// fluxes in the mesh frame from the solver's frame of direction dir
+template <typename T>
+__device__ __forceinline__ void flux_to_mesh(int dir, const T *f, T *out) {
+  out[0] = f[0];
+  out[4] = f[4];
+  if (dir == 0) { out[1] = f[1]; out[2] = f[2]; out[3] = f[3]; }
+  else if (dir == 1) { out[1] = f[3]; out[2] = f[1]; out[3] = f[2]; }
+  else { out[1] = f[2]; out[2] = f[3]; out[3] = f[1]; }
+}
+
+template <int TI, int TJ, int ORDER, int RS, typename TW, bool ISO>
+__global__ void __launch_bounds__(TI * TJ, 2)   // >= 2 waves per SIMD
+k_vl2(Geo g, VL2Args<TW> a) {
+  static_assert(64 % TI == 0 && TI * TJ == 256, "k_vl2 tiles: 256 threads, rows within a wavefront");
+  constexpr int NH = ISO ? 4 : NHYDRO;
+  constexpr int NT = TI * TJ;
+  // current plane: x1 faces [ey=e3x1, ez=e2x1, w1][row jj+1][face ii]; x2 faces
+  // [ey=e1x2, ez=e3x2, w2][face jj][column ii+1]; x3 faces [ey=e2x3, ez=e1x3, w3][jj+1][ii+1]
+  __shared__ TW s_f1[3][TJ + 2][TI + 1];
+  __shared__ TW s_f2[3][TJ + 1][TI + 2];
+  __shared__ TW s_f3[3][TJ + 2][TI + 2];
+  // previous plane: x1 (e2x1, w1) of rows [0, nj); x2 (e1x2, w2) of columns [0, ni)
+  __shared__ TW s_f1p[2][TJ][TI + 1];
+  __shared__ TW s_f2p[2][TJ + 1][TI];
+  // cell-centred E of plane c (3 components) and of plane c-1 (component 0 on columns
+  // [0, ni), component 1 on rows [0, nj))
+  __shared__ double s_cce[3][TJ + 2][TI + 2];
+  __shared__ double s_ccp0[TJ + 2][TI];
+  __shared__ double s_ccp1[TJ][TI + 2];
+  __shared__ double s_e1[2][TJ + 1][TI];      // edge planes c-1, c (slot = plane & 1)
+  __shared__ double s_e2[2][TJ][TI + 1];
+  __shared__ double s_e3[2][TJ + 1][TI + 1];
+  __shared__ TW s_h2[NHYDRO][TJ + 1][TI];     // x2 hydro fluxes (mesh frame) of plane c
+  __shared__ TW s_h1e[NHYDRO][TJ];            // x1 hydro fluxes of the tile's right-edge faces
+  __shared__ double s_red[NT / 64];
+  const int tid = threadIdx.x;
+  const int tx = tid % TI, ty = tid / TI;
+  const int nti = (g.bx1 + TI - 1) / TI;
+  const uint3 wg = xcd_block();   // whole MeshBlocks per XCD: tiles share halo lines in L2
+  const int b = wg.y;
+  const int i0 = g.is + (int)(wg.x % nti) * TI;
+  const int j0 = g.js + (int)(wg.x / nti) * TJ;
+  const int ni = min(TI, g.ie + 1 - i0), nj = min(TJ, g.je + 1 - j0);
+  const bool own = (tx < ni) && (ty < nj);
+  const int i = i0 + tx, j = j0 + ty;
+  const int64_t S = g.S, sj = g.P, sk = (int64_t)g.J * g.P;
+  const int64_t fb = (int64_t)b * S;
+  const int64_t cbase = (int64_t)b * NHYDRO * S, bbase = (int64_t)b * 3 * S;
+  const double dx[3] = {a.coef[b].dx[0], a.coef[b].dx[1], a.coef[b].dx[2]};
+  const double dx1 = dx[0], dx2 = dx[1], dx3 = dx[2];
+  const double dtv = *a.dtp;
+  const double wght = a.wfac * dtv;
+  const TW gT = TW(a.gamma), csT = TW(a.cs), dflT = TW(a.dfl), pflT = TW(a.pfl), dtT = TW(dtv);
+  const TW dxT[3] = {TW(dx1), TW(dx2), TW(dx3)};
+  const TW *wb = a.w + cbase;
+  const TW *bb = a.bcc + bbase;
+  const double *pcb = a.plm + (int64_t)b * 3 * 2 * g.PL;
+  const int nx1e = nj + 2 * (ni + 1), nx2e = ni + 2 * (nj + 1), nx3e = 2 * ni + 2 * nj;
+  const int ntask = 3 * NT + nx1e + nx2e + nx3e;
+  const int wi = ni + 2, ncce = wi * (nj + 2);
+  const int np = (nj + 1) * (ni + 1);
+  const double a1 = dx2 * dx3, a2 = dx1 * dx3, a3 = dx1 * dx2;
+  const double vol = dx1 * dx2 * dx3;
+
+  TW f1own[5], f3hi[5];          // this plane's own x1 and x3 face fluxes (mesh frame)
+  double f3lo[5];                // x3 fluxes of the own cell's lower face
+  double dflx12[5];              // own cell's x1 + x2 flux differences (next cell plane)
+  double g1l = 0.0, g1h = 0.0, g2l = 0.0, g2h = 0.0;   // their mass fluxes (self-gravity)
+#pragma unroll
+  for (int n = 0; n < 5; ++n) { f1own[n] = f3hi[n] = TW(0.0); f3lo[n] = dflx12[n] = 0.0; }
+  double b3_lo = 0.0;            // new b3 at the lower face of cell plane c-1
+  double dtmin = DBL_MAX;
+
+  for (int c = g.ks - 1; c <= g.ke + 1; ++c) {
+    const int cb = c & 1, pb = cb ^ 1;
+    // ---- F: faces of plane c, cell-centred E of plane c
+    const bool x3 = (c >= g.ks);
+    for (int q = tid; q < ntask; q += NT) {
+      int dir, ii, jj;
+      if (q < 3 * NT) {
+        dir = q / NT;
+        ii = tx;
+        jj = ty;
+        if (!own) continue;
+      } else {
+        int e = q - 3 * NT;
+        if (e < nx1e) {                        // x1: right edge, then rows -1 and nj
+          dir = 0;
+          if (e < nj) { ii = ni; jj = e; }
+          else if (e - nj <= ni) { jj = -1; ii = e - nj; }
+          else { jj = nj; ii = e - nj - (ni + 1); }
+        } else if ((e -= nx1e) < nx2e) {       // x2: top edge, then columns -1 and ni
+          dir = 1;
+          if (e < ni) { jj = nj; ii = e; }
+          else if (e - ni <= nj) { ii = -1; jj = e - ni; }
+          else { ii = ni; jj = e - ni - (nj + 1); }
+        } else {                               // x3: the ring without its corners
+          e -= nx2e;
+          dir = 2;
+          if (e < ni) { jj = -1; ii = e; }
+          else if (e < 2 * ni) { jj = nj; ii = e - ni; }
+          else if (e < 2 * ni + nj) { ii = -1; jj = e - 2 * ni; }
+          else { ii = ni; jj = e - 2 * ni - nj; }
+        }
+      }
+      if (dir == 2 && !x3) continue;
+      const int fi = i0 + ii, fj = j0 + jj;
+      const int64_t id = IDX(g, c, fj, fi);
+      const int64_t st = (dir == 0) ? 1 : (dir == 1 ? sj : sk);
+      const int cidx = (dir == 0) ? fi : (dir == 1 ? fj : c);
+      const double *bn = (dir == 0) ? a.bn1 : (dir == 1 ? a.bn2 : a.bn3);
+      const TW dxd = (dir == 0) ? dxT[0] : (dir == 1 ? dxT[1] : dxT[2]);
+      TW f[5], ey, ez, wc;
+      face_flux<ORDER, RS, TW, ISO>(dir, wb, bb, S, id, st, pcb + (int64_t)dir * 2 * g.PL, g.PL, cidx,
+                                    TW(bn[fb + id]), gT, csT, dflT, pflT, dxd, dtT, f, ey, ez, wc);
+      if (dir == 0) {
+        s_f1[0][jj + 1][ii] = ey; s_f1[1][jj + 1][ii] = ez; s_f1[2][jj + 1][ii] = wc;
+      } else if (dir == 1) {
+        s_f2[0][jj][ii + 1] = ey; s_f2[1][jj][ii + 1] = ez; s_f2[2][jj][ii + 1] = wc;
+      } else {
+        s_f3[0][jj + 1][ii + 1] = ey; s_f3[1][jj + 1][ii + 1] = ez; s_f3[2][jj + 1][ii + 1] = wc;
+      }
+      TW fm[5];
+      flux_to_mesh(dir, f, fm);
+      if (q < NT) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) f1own[n] = fm[n];
+      } else if (q < 2 * NT) {
+#pragma unroll
+        for (int n = 0; n < NH; ++n) s_h2[n][jj][ii] = fm[n];
+      } else if (q < 3 * NT) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) f3hi[n] = fm[n];
+      } else if (dir == 0 && ii == ni && jj >= 0 && jj < nj) {
+#pragma unroll
+        for (int n = 0; n < NH; ++n) s_h1e[n][jj] = fm[n];
+      } else if (dir == 1 && jj == nj && ii >= 0 && ii < ni) {
+#pragma unroll
+        for (int n = 0; n < NH; ++n) s_h2[n][nj][ii] = fm[n];
+      }
+    }
+    for (int q = tid; q < ncce; q += NT) {
+      const int jj = q / wi, ii = q - jj * wi;
+      const int64_t id = IDX(g, c, j0 - 1 + jj, i0 - 1 + ii);
+      const TW *wq = wb + id;
+      const TW *bq = bb + id;
+      const double vx = wq[IVX * S], vy = wq[IVY * S], vz = wq[IVZ * S];
+      const double bx = bq[0], by = bq[S], bz = bq[2 * S];
+      s_cce[0][jj][ii] = vz * by - vy * bz;
+      s_cce[1][jj][ii] = vx * bz - vz * bx;
+      s_cce[2][jj][ii] = vy * bx - vx * by;
+    }
+    __syncthreads();
+    // ---- E: corner EMFs of edge plane c (e1, e2) and of plane c (e3)
+    if (c >= g.ks) {
+      const bool do3 = (c <= g.ke);
+      for (int q = tid; q < np; q += NT) {
+        const int jj = q / (ni + 1), ii = q - jj * (ni + 1);
+        const bool do1 = ii < ni, do2 = jj < nj;
+        if (do1) {
+          s_e1[cb][jj][ii] = edge_e1(
+              s_cce[0][jj + 1][ii + 1], s_cce[0][jj][ii + 1], s_ccp0[jj + 1][ii], s_ccp0[jj][ii],
+              (double)s_f3[1][jj + 1][ii + 1], (double)s_f3[1][jj][ii + 1], (double)s_f2[0][jj][ii + 1],
+              (double)s_f2p[0][jj][ii], (double)s_f2[2][jj][ii + 1], (double)s_f2p[1][jj][ii],
+              (double)s_f3[2][jj + 1][ii + 1], (double)s_f3[2][jj][ii + 1]);
+        }
+        if (do2) {
+          s_e2[cb][jj][ii] = edge_e2(
+              s_cce[1][jj + 1][ii + 1], s_cce[1][jj + 1][ii], s_ccp1[jj][ii + 1], s_ccp1[jj][ii],
+              (double)s_f3[0][jj + 1][ii + 1], (double)s_f3[0][jj + 1][ii], (double)s_f1[1][jj + 1][ii],
+              (double)s_f1p[0][jj][ii], (double)s_f1[2][jj + 1][ii], (double)s_f1p[1][jj][ii],
+              (double)s_f3[2][jj + 1][ii + 1], (double)s_f3[2][jj + 1][ii]);
+        }
+        if (do3) {
+          s_e3[cb][jj][ii] = edge_e3(
+              s_cce[2][jj + 1][ii + 1], s_cce[2][jj + 1][ii], s_cce[2][jj][ii + 1], s_cce[2][jj][ii],
+              (double)s_f2[1][jj][ii + 1], (double)s_f2[1][jj][ii], (double)s_f1[0][jj + 1][ii],
+              (double)s_f1[0][jj][ii], (double)s_f1[2][jj + 1][ii], (double)s_f1[2][jj][ii],
+              (double)s_f2[2][jj][ii + 1], (double)s_f2[2][jj][ii]);
+        }
+      }
+    }
+    __syncthreads();
+    // ---- U: CT of the own x3 face of face plane c, update of the own cell of plane c-1
+    double b3_hi = 0.0;
+    if (own && c >= g.ks) {
+      {
+        const int64_t fo = fb + IDX(g, c, j, i);
+        double v = a.s3[fo];
+        const double area = dx1 * dx2;
+        v -= (wght / area) * (dx2 * s_e2[cb][ty][tx + 1] - dx2 * s_e2[cb][ty][tx]);
+        v += (wght / area) * (dx1 * s_e1[cb][ty + 1][tx] - dx1 * s_e1[cb][ty][tx]);
+        a.d3[fo] = v;
+        b3_hi = v;
+        int r[6];
+        push_range(i, g.is, g.ie, g.bx1, g.ng, r[0], r[1]);
+        push_range(j, g.js, g.je, g.bx2, g.ng, r[2], r[3]);
+        push_range(c, g.ks, g.ke + 1, g.bx3, g.ng, r[4], r[5]);
+        for_each_ghost(g, b, c, j, i, r, [&](int nb, int64_t id2) { a.d3[(int64_t)nb * S + id2] = v; });
+      }
+      if (c - 1 >= g.ks) {
+        const int64_t id = IDX(g, c - 1, j, i);
+        const int64_t fo = fb + id;
+        const double area1 = dx2 * dx3, area2 = dx1 * dx3;
+        double b1l = a.s1[fo], b1h = a.s1[fo + 1], b2l = a.s2[fo], b2h = a.s2[fo + sj];
+        b1l -= (wght / area1) * (dx3 * s_e3[pb][ty + 1][tx] - dx3 * s_e3[pb][ty][tx]);
+        b1l += (wght / area1) * (dx2 * s_e2[cb][ty][tx] - dx2 * s_e2[pb][ty][tx]);
+        b1h -= (wght / area1) * (dx3 * s_e3[pb][ty + 1][tx + 1] - dx3 * s_e3[pb][ty][tx + 1]);
+        b1h += (wght / area1) * (dx2 * s_e2[cb][ty][tx + 1] - dx2 * s_e2[pb][ty][tx + 1]);
+        b2l += (wght / area2) * (dx3 * s_e3[pb][ty][tx + 1] - dx3 * s_e3[pb][ty][tx]);
+        b2l -= (wght / area2) * (dx1 * s_e1[cb][ty][tx] - dx1 * s_e1[pb][ty][tx]);
+        b2h += (wght / area2) * (dx3 * s_e3[pb][ty + 1][tx + 1] - dx3 * s_e3[pb][ty + 1][tx]);
+        b2h -= (wght / area2) * (dx1 * s_e1[cb][ty + 1][tx] - dx1 * s_e1[pb][ty + 1][tx]);
+        a.d1[fo] = b1l;
+        if (tx == ni - 1) a.d1[fo + 1] = b1h;
+        a.d2[fo] = b2l;
+        if (ty == nj - 1) a.d2[fo + sj] = b2h;
+        {
+          int rc[6], r[6];
+          push_range(i, g.is, g.ie, g.bx1, g.ng, rc[0], rc[1]);
+          push_range(j, g.js, g.je, g.bx2, g.ng, rc[2], rc[3]);
+          push_range(c - 1, g.ks, g.ke, g.bx3, g.ng, rc[4], rc[5]);
+          r[2] = rc[2]; r[3] = rc[3]; r[4] = rc[4]; r[5] = rc[5];
+          push_range(i, g.is, g.ie + 1, g.bx1, g.ng, r[0], r[1]);
+          for_each_ghost(g, b, c - 1, j, i, r, [&](int nb, int64_t id2) { a.d1[(int64_t)nb * S + id2] = b1l; });
+          if (tx == ni - 1) {
+            push_range(i + 1, g.is, g.ie + 1, g.bx1, g.ng, r[0], r[1]);
+            for_each_ghost(g, b, c - 1, j, i + 1, r,
+                           [&](int nb, int64_t id2) { a.d1[(int64_t)nb * S + id2] = b1h; });
+          }
+          r[0] = rc[0]; r[1] = rc[1];
+          push_range(j, g.js, g.je + 1, g.bx2, g.ng, r[2], r[3]);
+          for_each_ghost(g, b, c - 1, j, i, r, [&](int nb, int64_t id2) { a.d2[(int64_t)nb * S + id2] = b2l; });
+          if (ty == nj - 1) {
+            push_range(j + 1, g.js, g.je + 1, g.bx2, g.ng, r[2], r[3]);
+            for_each_ghost(g, b, c - 1, j + 1, i, r,
+                           [&](int nb, int64_t id2) { a.d2[(int64_t)nb * S + id2] = b2h; });
+          }
+        }
+        const int64_t cb0 = cbase + id;
+        double un[NHYDRO];
+        un[IEN] = 0.0;
+#pragma unroll
+        for (int n = 0; n < NH; ++n) {
+          const int64_t o = cb0 + n * S;
+          double dflx = dflx12[n];
+          dflx += (a3 * (double)f3hi[n] - a3 * f3lo[n]);
+          un[n] = a.src[o] - wght * dflx / vol;
+        }
+        if (a.grav) {
+          const double rho = a.rho[cb0];
+          const double phic = a.phi[fo];
+          {
+            double dtodx1 = wght / dx1;
+            double phil = 0.5 * (a.phi[fo - 1] + phic);
+            double phir = 0.5 * (phic + a.phi[fo + 1]);
+            un[IVX] -= dtodx1 * rho * (phir - phil);
+            if (!ISO) un[IEN] -= dtodx1 * (g1l * (phic - phil) + g1h * (phir - phic));
+          }
+          {
+            double dtodx2 = wght / dx2;
+            double phil = 0.5 * (a.phi[fo - sj] + phic);
+            double phir = 0.5 * (phic + a.phi[fo + sj]);
+            un[IVY] -= dtodx2 * rho * (phir - phil);
+            if (!ISO) un[IEN] -= dtodx2 * (g2l * (phic - phil) + g2h * (phir - phic));
+          }
+          {
+            double dtodx3 = wght / dx3;
+            double phil = 0.5 * (a.phi[fo - sk] + phic);
+            double phir = 0.5 * (phic + a.phi[fo + sk]);
+            un[IVZ] -= dtodx3 * rho * (phir - phil);
+            if (!ISO) un[IEN] -= dtodx3 * (f3lo[IDN] * (phic - phil) + (double)f3hi[IDN] * (phir - phic));
+          }
+        }
+        const double bcc1 = 0.5 * b1l + 0.5 * b1h;
+        const double bcc2 = 0.5 * b2l + 0.5 * b2h;
+        const double bcc3 = 0.5 * b3_lo + 0.5 * b3_hi;
+        const int64_t bo = bbase + id;
+        a.bccn[bo] = TW(bcc1);
+        a.bccn[bo + S] = TW(bcc2);
+        a.bccn[bo + 2 * S] = TW(bcc3);
+        {
+          int rc[6];
+          push_range(i, g.is, g.ie, g.bx1, g.ng, rc[0], rc[1]);
+          push_range(j, g.js, g.je, g.bx2, g.ng, rc[2], rc[3]);
+          push_range(c - 1, g.ks, g.ke, g.bx3, g.ng, rc[4], rc[5]);
+          for_each_ghost(g, b, c - 1, j, i, rc, [&](int nb, int64_t id2) {
+            const int64_t c2 = (int64_t)nb * NHYDRO * S + id2;
+#pragma unroll
+            for (int n = 0; n < NH; ++n) a.dst[c2 + n * S] = un[n];
+          });
+        }
+        double wv[5];
+        c2p_eos<ISO>(un[IDN], un[IVX], un[IVY], un[IVZ], un[IEN], bcc1, bcc2, bcc3, a.gm1, a.dfl, a.pfl, wv);
+#pragma unroll
+        for (int n = 0; n < NH; ++n) a.dst[cb0 + n * S] = un[n];
+#pragma unroll
+        for (int n = 0; n < NH; ++n) a.wn[cb0 + n * S] = TW(wv[n]);
+        if (a.dtpart)
+          dtmin = dmin(dtmin, cell_dt<ISO>(a.gamma, a.cs, wv, bcc1, bcc2, bcc3, b1l, b2l, b3_lo, dx));
+      }
+    }
+    b3_lo = b3_hi;
+    // x1 + x2 flux differences of the own cell of plane c (summed as k_stage does)
+    TW f1hi[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) f1hi[n] = __shfl_down(f1own[n], 1, 64);
+    if (own && c >= g.ks && c <= g.ke) {
+#pragma unroll
+      for (int n = 0; n < NH; ++n) {
+        const double fh = (tx == ni - 1) ? (double)s_h1e[n][ty] : (double)f1hi[n];
+        double d = (a1 * fh - a1 * (double)f1own[n]);
+        d += (a2 * (double)s_h2[n][ty + 1][tx] - a2 * (double)s_h2[n][ty][tx]);
+        dflx12[n] = d;
+      }
+      g1l = (double)f1own[IDN];
+      g1h = (tx == ni - 1) ? (double)s_h1e[IDN][ty] : (double)f1hi[IDN];
+      g2l = (double)s_h2[IDN][ty][tx];
+      g2h = (double)s_h2[IDN][ty + 1][tx];
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) f3lo[n] = (double)f3hi[n];
+    // the face data and cell-centred E of plane c become the previous plane
+    for (int q = tid; q < nj * (ni + 1); q += NT) {
+      const int jj = q / (ni + 1), ii = q - jj * (ni + 1);
+      s_f1p[0][jj][ii] = s_f1[1][jj + 1][ii];
+      s_f1p[1][jj][ii] = s_f1[2][jj + 1][ii];
+    }
+    for (int q = tid; q < (nj + 1) * ni; q += NT) {
+      const int jj = q / ni, ii = q - jj * ni;
+      s_f2p[0][jj][ii] = s_f2[0][jj][ii + 1];
+      s_f2p[1][jj][ii] = s_f2[2][jj][ii + 1];
+    }
+    for (int q = tid; q < (nj + 2) * ni; q += NT) {
+      const int jj = q / ni, ii = q - jj * ni;
+      s_ccp0[jj][ii] = s_cce[0][jj][ii + 1];
+    }
+    for (int q = tid; q < nj * (ni + 2); q += NT) {
+      const int jj = q / (ni + 2), ii = q - jj * (ni + 2);
+      s_ccp1[jj][ii] = s_cce[1][jj + 1][ii];
+    }
+    __syncthreads();
+  }
+  if (a.dtpart) {
+    double m = wg_min<NT / 64>(dtmin, s_red);
+    if (tid == 0) a.dtpart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = m;
+  }
+}
+
 // The q-th element of the shell of the box [0,N1)x[0,N2)x[0,N3) that lies outside the
 // inner box [a1,b1]x[a2,b2]x[a3,b3]: whole planes first, then whole rows of the inner
 // planes, then the cells of the inner rows.
@@ -1858,6 +2282,13 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   }
   if (e == hipSuccess) e = hipMalloc(&m->red, sizeof(double) * m->red_n);
   if (e == hipSuccess) e = hipMalloc(&m->dtpart, sizeof(double) * m->dtpart_n);
+  if (e == hipSuccess) e = hipMalloc(&m->hist, sizeof(double) * (HIST_PARTIAL_DOUBLES + HIST_NV));
+  if (e == hipSuccess) e = hipMalloc(&m->cellvol, sizeof(double) * g.nb);
+  // Coordinates::CellVolume, Cartesian: dx1f*dx2f*dx3f (coordinates.cpp:521-531)
+  std::vector<double> vol_h(g.nb);
+  for (int b = 0; b < g.nb; ++b) vol_h[b] = m->coef_h[b].dx[0] * m->coef_h[b].dx[1] * m->coef_h[b].dx[2];
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(m->cellvol, vol_h.data(), sizeof(double) * g.nb, hipMemcpyHostToDevice, m->stream);
   if (e == hipSuccess) e = hipHostMalloc(&m->scal_host, sizeof(double) * 64, hipHostMallocDefault);
   if (e == hipSuccess)
     e = hipMemcpyAsync(m->coef, m->coef_h.data(), sizeof(BlockCoef) * g.nb, hipMemcpyHostToDevice, m->stream);
@@ -1890,6 +2321,8 @@ int agx_mesh_destroy(agx_mesh *m) {
   if (m->scal_host) hipHostFree(m->scal_host);
   if (m->red) hipFree(m->red);
   if (m->dtpart) hipFree(m->dtpart);
+  if (m->hist) hipFree(m->hist);
+  if (m->cellvol) hipFree(m->cellvol);
   if (m->stream) hipStreamDestroy(m->stream);
   delete m;
   return 0;
@@ -2457,6 +2890,70 @@ static void launch_stage(agx_mesh *m, int stage, double wfac, double *dtpart) {
 }  // extern "C++"
 
 
+// The whole stage as one k_vl2 launch (fused form); tile 0 = 16 x 16, 1 = 32 x 8 columns.
+extern "C++" {
+template <int TI, int TJ, int ORDER, typename TW, bool ISO>
+static void launch_vl2_rs(agx_mesh *m, int rs, const VL2Args<TW> &a, dim3 grid) {
+  if (ISO) {
+    if (rs == AGX_RSOLVER_HLLE)
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLE, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+    else
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLD, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+  } else {
+    if (rs == AGX_RSOLVER_HLLE)
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLE, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+    else if (rs == AGX_RSOLVER_ROE)
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_ROE, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+    else
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLD, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+  }
+}
+
+template <int TI, int TJ, typename TW, bool ISO>
+static void launch_vl2_order(agx_mesh *m, int order, const VL2Args<TW> &a, dim3 grid) {
+  if (order == 1) launch_vl2_rs<TI, TJ, 1, TW, ISO>(m, m->d.rsolver, a, grid);
+  else if (order == 2) launch_vl2_rs<TI, TJ, 2, TW, ISO>(m, m->d.rsolver, a, grid);
+  else launch_vl2_rs<TI, TJ, 3, TW, ISO>(m, m->d.rsolver, a, grid);
+}
+
+template <typename TW>
+static void launch_vl2(agx_mesh *m, int stage, int order, double wfac, double *dtpart) {
+  const Geo &g = m->g;
+  double **A = m->arr.a;
+  VL2Args<TW> a;
+  a.w = as<TW>(A[AGX_ARR_W]); a.bcc = as<TW>(A[AGX_ARR_BCC]);
+  a.wn = as<TW>(A[ARR_W2]); a.bccn = as<TW>(A[ARR_BCC2]);
+  a.bn1 = A[AGX_ARR_B1F]; a.bn2 = A[AGX_ARR_B2F]; a.bn3 = A[AGX_ARR_B3F];
+  a.src = (stage == 1) ? A[AGX_ARR_U] : A[AGX_ARR_U1];
+  a.dst = A[AGX_ARR_U1];
+  a.rho = A[AGX_ARR_U];
+  a.phi = A[AGX_ARR_PHI];
+  const int src = (stage == 1) ? AGX_ARR_B1F : AGX_ARR_R1F;
+  const int dst = (stage == 1) ? AGX_ARR_R1F : ARR_Q1F;
+  a.s1 = A[src]; a.s2 = A[src + 1]; a.s3 = A[src + 2];
+  a.d1 = A[dst]; a.d2 = A[dst + 1]; a.d3 = A[dst + 2];
+  a.coef = m->coef;
+  a.plm = m->plm;
+  a.dtp = m->scal + SC_DT;
+  a.wfac = wfac;
+  a.gm1 = m->d.gamma - 1.0;
+  a.dfl = m->d.dfloor;
+  a.pfl = m->d.pfloor;
+  a.gamma = m->d.gamma;
+  a.cs = m->d.iso_cs;
+  a.grav = m->d.self_gravity ? 1 : 0;
+  a.dtpart = dtpart;
+  const dim3 grid = stage_grid(g, m->tile);
+  if (m->iso) {
+    if (m->tile == 5) launch_vl2_order<32, 8, TW, true>(m, order, a, grid);
+    else launch_vl2_order<16, 16, TW, true>(m, order, a, grid);
+  } else {
+    if (m->tile == 5) launch_vl2_order<32, 8, TW, false>(m, order, a, grid);
+    else launch_vl2_order<16, 16, TW, false>(m, order, a, grid);
+  }
+}
+}  // extern "C++"
+
 extern "C++" {
 template <typename TW>
 static void launch_ghost_c2p(agx_mesh *m) {
@@ -2477,14 +2974,24 @@ static void launch_ghost_c2p(agx_mesh *m) {
 
 static int stage_fused(agx_mesh *m, int stage, bool want_dt) {
   double **A = m->arr.a;
-  int rc = calculate_fluxes(m, stage == 1 ? 1 : m->d.xorder);
-  if (rc) return rc;
+  const int order = (stage == 1) ? 1 : m->d.xorder;
   const double wfac = (stage == 1 ? 0.5 : 1.0);   // wght = wfac * dt on the device
-  timer_begin(m, AGX_K_INTEGRATE);
-  if (m->mixed) launch_stage<float>(m, stage, wfac, want_dt ? m->dtpart : nullptr);
-  else launch_stage<double>(m, stage, wfac, want_dt ? m->dtpart : nullptr);
-  timer_end(m);
-  LAUNCH_CHECK("k_stage");
+  if (m->tile >= TILE_VL2) {
+    if (order == 3 && m->g.ng < 3) { set_error("PPM needs nghost >= 3 (reconstruction.cpp:90-100)"); return -3; }
+    timer_begin(m, AGX_K_INTEGRATE);
+    if (m->mixed) launch_vl2<float>(m, stage, order, wfac, want_dt ? m->dtpart : nullptr);
+    else launch_vl2<double>(m, stage, order, wfac, want_dt ? m->dtpart : nullptr);
+    timer_end(m);
+    LAUNCH_CHECK("k_vl2");
+  } else {
+    int rc = calculate_fluxes(m, order);
+    if (rc) return rc;
+    timer_begin(m, AGX_K_INTEGRATE);
+    if (m->mixed) launch_stage<float>(m, stage, wfac, want_dt ? m->dtpart : nullptr);
+    else launch_stage<double>(m, stage, wfac, want_dt ? m->dtpart : nullptr);
+    timer_end(m);
+    LAUNCH_CHECK("k_stage");
+  }
   std::swap(A[AGX_ARR_U], A[AGX_ARR_U1]);
   std::swap(A[AGX_ARR_W], A[ARR_W2]);
   std::swap(A[AGX_ARR_BCC], A[ARR_BCC2]);
@@ -2648,14 +3155,14 @@ int agx_mg_vcycles(agx_mesh *m, int32_t n) {
   return mg_vcycles(m, n);
 }
 
-// Tile-shape autotuner of the stage kernel (no counterpart in the reference).  Times
-// every tile shape on the current state with stage-1 launches, which read u, w, bcc, b
-// and the fluxes and write only the scratch registers (u1, b1, the second w/bcc), so
-// the state is unchanged; keeps the fastest.  Call between cycles.
+// Stage-form autotuner (no counterpart in the reference).  Times every form on the
+// current state with stage-1 launches at the mesh's reconstruction order (the split
+// forms with their k_flux launches), which read u, w, bcc and b and write only scratch
+// registers (fluxes, u1, b1, the second w/bcc), so the state is unchanged; keeps the
+// fastest.  Call between cycles.
 int agx_autotune(agx_mesh *m, int32_t *choice) {
   if (!m) { set_error("null mesh"); return -1; }
-  int rc = agx_calculate_fluxes(m, 1);
-  if (rc) return rc;
+  const int order = m->d.xorder;
   hipEvent_t e0, e1;
   AGX_HIP(hipEventCreate(&e0));
   AGX_HIP(hipEventCreate(&e1));
@@ -2670,8 +3177,15 @@ int agx_autotune(agx_mesh *m, int32_t *choice) {
     float ms = 0.f;
     for (int rep = 0; rep < 4; ++rep) {
       if (rep == 1) AGX_HIP(hipEventRecord(e0, m->stream));
-      if (m->mixed) launch_stage<float>(m, 1, wfac, nullptr);
-      else launch_stage<double>(m, 1, wfac, nullptr);
+      if (t >= TILE_VL2) {   // the whole stage in one launch
+        if (m->mixed) launch_vl2<float>(m, 1, order, wfac, nullptr);
+        else launch_vl2<double>(m, 1, order, wfac, nullptr);
+      } else {               // fluxes, then the stage kernel
+        int rc = calculate_fluxes(m, order);
+        if (rc) return rc;
+        if (m->mixed) launch_stage<float>(m, 1, wfac, nullptr);
+        else launch_stage<double>(m, 1, wfac, nullptr);
+      }
     }
     AGX_HIP(hipEventRecord(e1, m->stream));
     AGX_HIP(hipEventSynchronize(e1));
@@ -2681,7 +3195,7 @@ int agx_autotune(agx_mesh *m, int32_t *choice) {
   m->timer.enabled = timing;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  LAUNCH_CHECK("k_stage (autotune)");
+  LAUNCH_CHECK("stage forms (autotune)");
   m->tile = best_t;
   if (choice) *choice = best_t;
   return mg_autotune(m);

@@ -54,10 +54,18 @@ def log(*a):
 #   integrate  per interior cell (k_stage): read 15 fluxes + 6 face EMFs + 3 weights +
 #              3 v + 3 bcc (TW) + 5 conserved + 3 CT-source faces (FP64); write 5 conserved
 #              + 3 faces (FP64) + 5 prim + 3 bcc (TW)
+#   vl2        per interior cell (k_vl2, the whole stage in one launch; fluxes stay in
+#              LDS): read 5 prim + 3 bcc (TW), 5 conserved + 3 face-field words for the
+#              Riemann problems and CT (FP64) and, in the second stage, the 3 CT-source
+#              faces (another register); write 5 conserved + 3 faces (FP64) + 5 prim +
+#              3 bcc (TW).  Stage 1: 32 words, stage 2: 35 (FP64); a launch is counted
+#              as the mean of the two (268 B FP64, 204 B mixed)
 BYTES_PER_UNIT = {
-    "fp64": {"flux_x1": 128, "flux_x2": 128, "flux_x3": 128, "integrate": 432},
-    "mixed": {"flux_x1": 68, "flux_x2": 68, "flux_x3": 68, "integrate": 280},
+    "fp64": {"flux_x1": 128, "flux_x2": 128, "flux_x3": 128, "integrate": 432, "vl2": (256 + 280) / 2},
+    "mixed": {"flux_x1": 68, "flux_x2": 68, "flux_x3": 68, "integrate": 280, "vl2": (192 + 216) / 2},
 }
+STAGE_FORMS = ("16x16", "32x8", "64x4", "64x8", "one launch 16x16", "one launch 32x8")
+VL2_FORM = 4
 # whole VL2 cycle of a fully fused form, per zone (85 words of compulsory traffic):
 # 680 B in FP64, 552 B with w and bcc in FP32
 CYCLE_BYTES = {"fp64": 680.0, "mixed": 552.0}
@@ -72,7 +80,7 @@ def units_per_launch(desc, cls):
         return nb * (b1 + 2) * (b2 + 1) * (b3 + 2)
     if cls == "flux_x3":
         return nb * (b1 + 2) * (b2 + 2) * (b3 + 1)
-    if cls == "integrate":
+    if cls in ("integrate", "vl2"):
         return nb * b1 * b2 * b3
     raise KeyError(cls)
 
@@ -115,7 +123,7 @@ def mhd_bench(agx, pgen, args, precision):
     m.initialize()
     tile = m.autotune()
     log(f"[bench] {precision} MHD {N}^3 in {desc.nx1 // B}^3 blocks of {B}^3 set up in {time.time() - t0:.1f}s, "
-        f"device {m.device_bytes() / 1e9:.2f} GB, stage tile {('16x16', '32x8', '64x4', '64x8')[tile]}")
+        f"device {m.device_bytes() / 1e9:.2f} GB, stage form {STAGE_FORMS[tile]}")
     m.cycle(1e30, args.warmup)
     m.sync()
     t0 = time.perf_counter()
@@ -155,39 +163,56 @@ def mhd_bench(agx, pgen, args, precision):
         log(f"[bench] {precision} max |div B| after {int(state[0][2])} cycles: {state[2][0]:.3e} "
             f"(x dx / max|B|: {state[2][1]:.3e})")
     m.close()
-    mhd_bench.tile = ("16, 16", "32, 8", "64, 4", "64, 8")[tile]
+    mhd_bench.tile = tile
     return desc, zc, sec * 1e3 / args.steps, per, state
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_stage_turb256_fp64_hbm_traffic.txt")
 
 
+def kernel_name(precision, tile, order=2):
+    """rocprof name of the stage kernel of this form (adiabatic, HLLD)."""
+    t = "double" if precision == "fp64" else "float"
+    if tile >= VL2_FORM:
+        ti = "16, 16" if tile == 4 else "32, 8"
+        return f"k_vl2<{ti}, {order}, 0, {t}, false>"
+    return f"k_stage<{('16, 16', '32, 8', '64, 4', '64, 8')[tile]}, {t}, false>"
+
+
 def measured_traffic(desc, precision, tile):
-    """HBM bytes per launch of the stage kernel with this tile and precision, from the
+    """HBM bytes per launch of the stage kernel of this form and precision, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (already scaled by the copy-kernel
-    factors), when the workload is the one profiled (256^3 in 64^3 blocks); else None."""
+    factors), when the workload is the one profiled (256^3 in 64^3 blocks); else None.
+    For the one-launch form the profile lists the PLM stage (order 2)."""
     if (desc.nx1, desc.bnx1) != (256, 64) or not os.path.exists(TRAFFIC_PROFILE):
         return None
-    name = f"k_stage<{tile}, {'double' if precision == 'fp64' else 'float'}, false>"
+    name = kernel_name(precision, tile)
     for line in open(TRAFFIC_PROFILE):
         if line.startswith(name):
             f = line[len(name):].split()
             return {"bytes": (float(f[1]) + float(f[2])) * 1e9, "read_GB": float(f[1]), "write_GB": float(f[2]),
-                    "source": "profiles/k_stage_turb256_fp64_hbm_traffic.txt (rocprofv3 --pmc FETCH_SIZE, "
+                    "kernel": name,
+                    "source": f"{os.path.relpath(TRAFFIC_PROFILE, ROOT)} (rocprofv3 --pmc FETCH_SIZE, "
                               "WRITE_SIZE; scaled by the 1 GiB copy factors)"}
     return None
 
 
 def roofline(desc, per, precision, tile=None):
-    """Dominant kernel (among those with an algorithmic byte count) and its HBM rate."""
-    table = BYTES_PER_UNIT[precision]
-    dom = max(table, key=lambda c: per[c][0])
-    ms_tot, n_l = per[dom]
+    """Dominant kernel (among those with an algorithmic byte count) and its HBM rate.
+    tile: the stage form agx_autotune picked (index into STAGE_FORMS)."""
+    table = dict(BYTES_PER_UNIT[precision])
+    # the integrate timer class holds k_stage (split forms) or k_vl2 (one-launch forms)
+    vl2 = tile is not None and tile >= VL2_FORM
+    table.pop("integrate" if vl2 else "vl2")
+    dom = max(table, key=lambda c: per["integrate" if c == "vl2" else c][0])
+    ms_tot, n_l = per["integrate" if dom == "vl2" else dom]
     avg_s = ms_tot / max(n_l, 1) / 1e3
     alg = table[dom] * units_per_launch(desc, dom)
     achieved = alg / avg_s
-    tr = measured_traffic(desc, precision, tile) if (dom == "integrate" and tile) else None
-    out = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+    tr = measured_traffic(desc, precision, tile) if (dom in ("integrate", "vl2") and tile is not None) else None
+    out = {"bound": "hbm",
+           "kernel": kernel_name(precision, tile, desc.xorder) if dom in ("integrate", "vl2") else dom,
+           "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": tr["bytes"] if tr else None,
            "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": alg}
     if tr:
@@ -292,9 +317,9 @@ def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None):
         per = {c: m.timing_get(c) for c in KCLASSES + ("mg_smooth", "mg_restrict", "mg_prolong", "mg_other")}
         m.timing(False)
         res[prec] = dict(zc=float(N) ** 3 * steps / sec, ms=sec * 1e3 / steps, w=w, bcc=bcc, phi=phi, divb=divb,
-                         per=per, tile=tile, bytes=m.device_bytes(), rf=roofline(desc, per, prec))
+                         per=per, tile=tile, bytes=m.device_bytes(), rf=roofline(desc, per, prec, tile))
         log(f"[{tag}] {prec}: {steps} cycles in {sec:.2f}s -> {res[prec]['zc']:.4e} zone-cycles/s, "
-            f"device {m.device_bytes() / 1e9:.1f} GB, stage tile {tile}; per cycle: "
+            f"device {m.device_bytes() / 1e9:.1f} GB, stage form {STAGE_FORMS[tile]}; per cycle: "
             + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in per.items() if v[0] > 0))
         m.close()
     a, b = res["mixed"], res["fp64"]
@@ -580,6 +605,24 @@ def smr_three_levels(agx, smr, reps, timed):
     return out
 
 
+def host_cpu():
+    """The host's CPU as lscpu reports it: model, sockets x physical cores per socket x
+    threads per core, and the logical CPUs this process may use."""
+    info = {}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:   # lscpu missing: report the logical count only
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"), "threads_per_core": info.get("Thread(s) per core"),
+            "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
 def cpu_baseline(args):
     """The CPU oracle (FP64 restatement, OpenMP over MeshBlocks) on a bounded sample of
     the same workload: a 64^3 periodic sub-box of the turbulence problem in 16
@@ -606,8 +649,12 @@ def cpu_baseline(args):
         o.cycle(1e30)
     sec = time.perf_counter() - t0
     zc = 64.0 ** 3 * ncyc / sec
-    log(f"[bench] CPU oracle: {ncyc} cycles of 64^3 in {sec:.1f}s -> {zc:.3e} zone-cycles/s ({threads} threads)")
-    return {"value": zc, "unit": "zone-cycles/s", "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
+    used = int(os.environ.get("OMP_NUM_THREADS", threads))
+    host = host_cpu()
+    log(f"[bench] CPU oracle: {ncyc} cycles of 64^3 in {sec:.1f}s -> {zc:.3e} zone-cycles/s ({used} threads; "
+        f"host {host})")
+    return {"value": zc, "unit": "zone-cycles/s", "cores": used, "host": host,
+            "build": "gcc -O3 -fopenmp -ffp-contract=off (oracle/Makefile)",
             "kind": "port",
             "sample": f"{ncyc} VL2+PLM+HLLD+CT cycles of a 64^3 periodic sub-box in 16 MeshBlocks of 32x32x16 "
                       f"(same turbulence generator), oracle/agx_oracle.c FP64, OpenMP over MeshBlocks"}
@@ -677,6 +724,7 @@ def main():
                                f"VL2+PLM+HLLD+CT, plain FP64, one step = one VL2 cycle",
                    "mesh": [args.n] * 3, "meshblock": [args.block] * 3, "nghost": 2},
         "cycle_roofline_frac": zc * CYCLE_BYTES["fp64"] / HBM_PEAK,
+        "stage_form": STAGE_FORMS[mhd_bench.tile],
         "roofline": rf,
         "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in per.items()},
     }
@@ -688,6 +736,7 @@ def main():
         mixed = {"value": mzc, "unit": "zone-cycles/s", "ms_per_step": mms, "vs_fp64": mzc / zc,
                  "dtype": "f32 reconstruction+Riemann, f64 update/C2P/CT",
                  "cycle_roofline_frac": mzc * CYCLE_BYTES["mixed"] / HBM_PEAK,
+                 "stage_form": STAGE_FORMS[mtile],
                  "roofline": roofline(mdesc, mper, "mixed", mtile),
                  "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in mper.items()}}
         if st64 is not None and stmx is not None:

@@ -189,13 +189,36 @@ int agx_mg_set_mode(agx_mesh *m, int mode, double threshold, int32_t niter);
 int agx_mg_last(agx_mesh *m, double *norm, int32_t *iters);
 /* n V(1,1) cycles on the current finest-level state without setup or norms
  * (MultigridDriver::SolveVCycle, multigrid_driver.cpp:684-693): the unit of the
- * V-cycles/s benchmark. */
+ * V-cycles/s benchmark.  With multipole boundaries (mg_bc = 1) the cycles continue the
+ * last agx_mg_solve (its source, boundary values and FAS levels); before any solve the
+ * call returns -22. */
 int agx_mg_vcycles(agx_mesh *m, int32_t n);
 
+/* ---- history diagnostics: HistoryOutput::WriteOutputFile, the built-in volume sums
+ * (outputs/history.cpp:37-39,135-200) over the interior cells of every MeshBlock:
+ *   mass, 1-mom, 2-mom, 3-mom, 1-KE, 2-KE, 3-KE, [tot-E (adiabatic)], [grav-E =
+ *   sum 0.5 vol rho phi (self-gravity)], 1-ME, 2-ME, 3-ME (cell-centred 0.5 vol bcc^2),
+ * NHYDRO + 3 + (self-gravity) + 3 values, in that order.  The sums are reduced on the
+ * device in a fixed order (reproducible bit for bit; the reference's sequential sum
+ * differs only in rounding).  agx_history writes them to host memory hst (nmax >= the
+ * count) from the mesh's current u, bcc and phi. */
+int agx_history(agx_mesh *m, double *hst, int32_t nmax);
+/* The same sums over dense device arrays in the reference's AthenaArray order:
+ * u (nblocks, nhydro, n3, n2, n1), bcc (nblocks, 3, n3, n2, n1), phi (nblocks, n3, n2, n1)
+ * or null (no grav-E), n_d = nx_d + 2 ng; vol[b] = the cell volume dx1 dx2 dx3 of block b
+ * (device array); hst = device array of the result.  Runs on `stream` (null: the default
+ * stream) and returns after the result is written.  Partial sums go to device memory this
+ * call allocates and frees. */
+int agx_history_sum(const double *u, const double *bcc, const double *phi, int32_t nblocks, int32_t nhydro,
+                    int32_t nx1, int32_t nx2, int32_t nx3, int32_t ng, const double *vol, double *hst,
+                    void *stream);
+
 /* ---- tuning (no counterpart in the reference: launch shapes of this library).
- * agx_autotune times each tile shape of the fused stage kernel on the current state
- * (index 0..3: 16x16, 32x8, 64x4 (i,j) columns per 256-thread workgroup, 64x8 per
- * 512-thread workgroup) and keeps the fastest; *choice gets its index.  With multigrid
+ * agx_autotune times each form of a VL2 stage on the current state and keeps the
+ * fastest; *choice gets its index.  Forms 0..3: three flux launches, then one stage
+ * kernel over 16x16, 32x8, 64x4 (i,j) columns per 256-thread workgroup or 64x8 per
+ * 512-thread workgroup; forms 4..5: the whole stage (fluxes in LDS) in one launch over
+ * 16x16 or 32x8 columns (form 5 is the default).  With multigrid
  * gravity (periodic boundaries) it also times V-cycles with the red-black pass fused into
  * one launch and as two sweeps, and keeps the faster.  It writes only scratch registers
  * (u1, b1, fluxes, the multigrid levels, which every solve reloads), so call it between
@@ -208,7 +231,9 @@ int agx_set_tile(agx_mesh *m, int32_t tile);
 int agx_set_mg_fuse(agx_mesh *m, int32_t mode);
 /* Multipole V-cycles: run the coarse levels (at most 512 cells each) as one
  * single-workgroup launch (on = 1, the default) or one launch per step (0).  The
- * potentials are identical bit for bit; no counterpart in the reference. */
+ * potentials are identical bit for bit; no counterpart in the reference.  Applies to
+ * the FP64 V-cycles (FP64 form, and the FMG of the mixed form); the FP32 correction
+ * cycles of the mixed form launch their coarse levels per step. */
 int agx_set_mg_coarse(agx_mesh *m, int32_t on);
 /* Source mask of MultigridDriver::SetupMultigrid (multigrid_driver.cpp:430-436): after
  * LoadSource, the source is zero at every finest-level cell centre with |x - c|^2 > r^2,

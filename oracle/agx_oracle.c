@@ -2719,6 +2719,48 @@ void orc_conserved_totals(orc_ctx *c, double *out8) {
   }
 }
 
+/* HistoryOutput::WriteOutputFile, the built-in sums of the non-orbital branch
+ * (outputs/history.cpp:37-39,135-200): volume sums over the interior cells of every
+ * MeshBlock, block by block in k, j, i order, of mass, the three momenta, the three
+ * partitioned kinetic energies, the total energy (adiabatic only), the gravitational
+ * energy 0.5*rho*phi (with self-gravity) and the three cell-centred magnetic energies.
+ * out gets NHYDRO + 3 + (self-gravity) + 3 values in that order; returns the count. */
+int orc_history(orc_ctx *c, double *out) {
+  const int nh = c->nhydro, sg = c->d.self_gravity ? 1 : 0;
+  const int nout = nh + 3 + sg + 3;
+  for (int n = 0; n < nout; ++n) out[n] = 0.0;
+  for (int g = 0; g < c->nblocks; ++g) {
+    const double *u = c->arr[ARR_U] + g * c->arrsize[ARR_U];
+    const double *bcc = c->arr[ARR_BCC] + g * c->arrsize[ARR_BCC];
+    const double *phi = c->arr[ARR_PHI] + g * c->arrsize[ARR_PHI];
+    for (int k = c->ks; k <= c->ke; k++)
+      for (int j = c->js; j <= c->je; j++)
+        for (int i = c->is; i <= c->ie; i++) {
+          /* Coordinates::CellVolume (coordinates.cpp:521-531): dx1f*dx2f*dx3f */
+          const double vol = c->dx1f[g * (c->n1 + 1) + i] * c->dx2f[g * (c->n2 + 1) + j] *
+                             c->dx3f[g * (c->n3 + 1) + k];
+          const int64_t id = CIDX(c, k, j, i);
+          const double u_d = u[IDN * c->ncell + id], u_mx = u[IVX * c->ncell + id];
+          const double u_my = u[IVY * c->ncell + id], u_mz = u[IVZ * c->ncell + id];
+          out[0] += vol * u_d;
+          out[1] += vol * u_mx;
+          out[2] += vol * u_my;
+          out[3] += vol * u_mz;
+          out[4] += vol * 0.5 * (u_mx * u_mx) / u_d;
+          out[5] += vol * 0.5 * (u_my * u_my) / u_d;
+          out[6] += vol * 0.5 * (u_mz * u_mz) / u_d;
+          if (!c->iso) out[7] += vol * u[IEN * c->ncell + id];
+          if (sg) out[nh + 3] += vol * 0.5 * u_d * phi[id];
+          const double b1 = bcc[IB1 * c->ncell + id], b2 = bcc[IB2 * c->ncell + id];
+          const double b3 = bcc[IB3 * c->ncell + id];
+          out[nh + 3 + sg] += vol * 0.5 * b1 * b1;
+          out[nh + 3 + sg + 1] += vol * 0.5 * b2 * b2;
+          out[nh + 3 + sg + 2] += vol * 0.5 * b3 * b3;
+        }
+  }
+  return nout;
+}
+
 /* ================================================================ unit-level probes */
 /* One Riemann-solver call in the solver's local frame (wave arrays ordered IDN, IVX
  * (normal), IVY, IVZ, IPR, IBY, IBZ): rs 0 = HLLD, 1 = HLLE.  flx[7] as flxi. */

@@ -103,6 +103,8 @@ void orc_plm_pencil(int n, const double *q, const double *dxp, const double *dxm
                     double *wl, double *wr);
 void orc_ppm_pencil(int n, const double *q, double *wl, double *wr);
 void orc_conserved_totals(orc_ctx *c, double *out8);
+/* history.cpp:135-200: built-in volume sums; returns the count written to out */
+int orc_history(orc_ctx *c, double *out);
 
 /* static-mesh-refinement operators (agx_oracle_refine.c), 3D Cartesian blocks in the
  * dense layouts described there; items are processed in list order */

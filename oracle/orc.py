@@ -92,6 +92,8 @@ def _lib():
     lib.orc_max_divb.restype = ctypes.c_double
     lib.orc_max_divb.argtypes = [P]
     lib.orc_conserved_totals.argtypes = [P, D]
+    lib.orc_history.restype = ctypes.c_int
+    lib.orc_history.argtypes = [P, D]
     lib.orc_riemann.argtypes = [ctypes.c_int, ctypes.c_double, D, D, ctypes.c_double, D]
     lib.orc_riemann_iso.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, D, D,
                                     ctypes.c_double, D]
@@ -211,6 +213,12 @@ class Oracle:
         out = np.zeros(8)
         self.L.orc_conserved_totals(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return out
+
+    def history(self):
+        """history.cpp built-in sums (orc_history)."""
+        out = np.zeros(16)
+        n = self.L.orc_history(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out[:n].copy()
 
     def mg_solve(self, stage=0):
         self.L.orc_mg_solve(self.h, stage)

@@ -103,7 +103,7 @@ def test_graph_rebuilt_after_tile_change(agx):
     o.initialize()
     m.initialize()
     m.set_graph(True)
-    for tile in (1, 3, 0):
+    for tile in (1, 5, 3, 4, 0):
         m.set_tile(tile)
         assert m.cycle(1e9, 2) == 2
         for _ in range(2):

@@ -169,3 +169,21 @@ def test_mixed_precision_matches(agx, thr):
     else:
         assert nm <= eps and no <= eps
         assert err <= 1e-8
+
+
+def test_vcycles_need_a_solve_first(agx):
+    """Multipole V-cycles continue a solve (its source, boundary values and FAS levels):
+    before any agx_mg_solve the entry point refuses with -22 instead of cycling an unloaded
+    problem; after one it runs."""
+    d = make_desc(nx=(16, 16, 16), bnx=(8, 8, 8), box=BOX, self_gravity=1, mg_bc=1, mg_mode=0, four_pi_G=1.0,
+                  mporder=4)
+    m = agx.Mesh(d)
+    U = np.zeros((5, 16, 16, 16))
+    U[0] = gaussian(16, 0.08)
+    m.set("u", agx.scatter_cells(d, U))
+    with pytest.raises(agx.AgxError, match="-22"):
+        m.mg_vcycles(1)
+    m.mg_solve(0)
+    m.mg_vcycles(2)
+    m.sync()
+    m.close()

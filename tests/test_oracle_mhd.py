@@ -49,7 +49,7 @@ def test_linwave_convergence(lw32, lw64):
 def test_linwave_divb_roundoff(lw64):
     # dimensionless round-off measure max|div B| * dx / |B| (dx = 3/64, |B| = O(1) for the
     # fast wave of linear_wave.cpp); the absolute value is about 1.8e-13
-    print(f"max |div B| after t = 1: {lw64[2]:.3e}, times dx: {lw64[2] * 3.0 / 64:.3e}", flush=True)
+    print(f"max |div B| after t = 0.5: {lw64[2]:.3e}, times dx: {lw64[2] * 3.0 / 64:.3e}", flush=True)
     assert lw64[2] * 3.0 / 64 < 1e-13
 
 
