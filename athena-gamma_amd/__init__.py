@@ -44,7 +44,7 @@ EXPORTS = [
     "agx_ref_cons_to_prim", "agx_stream_create", "agx_stream_destroy", "agx_graph_begin", "agx_graph_end",
     "agx_graph_launch", "agx_graph_destroy", "agx_ref_restrict_all", "agx_ref_prolong_all",
     "agx_ref_internal_to_cons", "agx_mg_set_source_mask_sphere", "agx_set_mg_coarse",
-    "agx_history", "agx_history_sum",
+    "agx_history", "agx_history_sum", "agx_history_sum_noalloc",
 ]
 
 
@@ -194,8 +194,9 @@ def lib():
                                        ctypes.c_int32, P]
     L.agx_ref_unpack_fc.argtypes = [P, P, P, P, P, ctypes.c_int64, P]
     L.agx_history.argtypes = [P, D, ctypes.c_int32]
-    L.agx_history_sum.argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
-                                  ctypes.c_int32, ctypes.c_int32, P, P, P]
+    for fn in ("agx_history_sum", "agx_history_sum_noalloc"):
+        getattr(L, fn).argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.c_int32, ctypes.c_int32, P, P, P]
     L.agx_device_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(P)]
     L.agx_device_free.argtypes = [P]
     L.agx_device_copy.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]

@@ -212,6 +212,17 @@ int agx_history(agx_mesh *m, double *hst, int32_t nmax);
 int agx_history_sum(const double *u, const double *bcc, const double *phi, int32_t nblocks, int32_t nhydro,
                     int32_t nx1, int32_t nx2, int32_t nx3, int32_t ng, const double *vol, double *hst,
                     void *stream);
+/* agx_history_sum without any device allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under stream capture, tight loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results (bit for bit); the partial sums go to a
+ * static device array of 2048 x 12 doubles (192 KiB) that each GPU holds until the process
+ * ends, and the call returns once its launches are enqueued on `stream` (no
+ * synchronisation).  Every size fits (the work split caps the partials at 2048 rows);
+ * nblocks * nx3 * nx2 > 2^31 - 1 is rejected.  All calls on one GPU share that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race. */
+int agx_history_sum_noalloc(const double *u, const double *bcc, const double *phi, int32_t nblocks,
+                            int32_t nhydro, int32_t nx1, int32_t nx2, int32_t nx3, int32_t ng, const double *vol,
+                            double *hst, void *stream);
 
 /* ---- tuning (no counterpart in the reference: launch shapes of this library).
  * agx_autotune times each form of a VL2 stage on the current state and keeps the
