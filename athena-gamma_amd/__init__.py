@@ -45,6 +45,7 @@ EXPORTS = [
     "agx_graph_launch", "agx_graph_destroy", "agx_ref_restrict_all", "agx_ref_prolong_all",
     "agx_ref_internal_to_cons", "agx_mg_set_source_mask_sphere", "agx_set_mg_coarse",
     "agx_history", "agx_history_sum", "agx_history_sum_noalloc",
+    "agx_restart_write", "agx_restart_read", "agx_restart_write_host", "agx_restart_read_host",
 ]
 
 
@@ -197,6 +198,12 @@ def lib():
     for fn in ("agx_history_sum", "agx_history_sum_noalloc"):
         getattr(L, fn).argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_int32, P, P, P]
+    C = ctypes.c_char_p
+    L.agx_restart_write.argtypes = [P, C, C]
+    L.agx_restart_read.argtypes = [P, C]
+    L.agx_restart_write_host.argtypes = [C, ctypes.POINTER(Desc), C, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int32, D, D, D, D]
+    L.agx_restart_read_host.argtypes = [C, ctypes.POINTER(Desc), D, D, ctypes.POINTER(ctypes.c_int32), D, D, D, D]
     L.agx_device_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(P)]
     L.agx_device_free.argtypes = [P]
     L.agx_device_copy.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
@@ -382,6 +389,17 @@ class Mesh:
     def device_bytes(self):
         return self.L.agx_mesh_bytes(self.h)
 
+    def restart_write(self, path, params=None):
+        """Write a restart file in the reference's format (agx_restart_write)."""
+        _check(self.L.agx_restart_write(self.h, os.fsencode(path), None if params is None else params.encode()),
+               "agx_restart_write")
+
+    def restart_read(self, path):
+        """Load u, b, time, dt and ncycle from a restart file (agx_restart_read); call
+        initialize() next, as the reference's Mesh::Initialize follows its restart
+        constructor."""
+        _check(self.L.agx_restart_read(self.h, os.fsencode(path)), "agx_restart_read")
+
     def history(self):
         """The built-in history sums (agx_history): mass, 3 momenta, 3 kinetic energies,
         [total energy], [gravitational energy], 3 magnetic energies."""
@@ -392,6 +410,34 @@ class Mesh:
 
 
 # ---------------------------------------------------------------- host utilities
+def restart_write_host(path, desc, time, dt, ncycle, u, b1f, b2f, b3f, params=None):
+    """agx_restart_write_host: a restart file from host arrays (no device work)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, b1f, b2f, b3f)]
+    D = ctypes.POINTER(ctypes.c_double)
+    _check(lib().agx_restart_write_host(os.fsencode(path), ctypes.byref(_desc(desc)),
+                                        None if params is None else params.encode(), time, dt, ncycle,
+                                        *[a.ctypes.data_as(D) for a in arrs]), "agx_restart_write_host")
+
+
+def restart_read_host(path, desc, shapes):
+    """agx_restart_read_host into new host arrays of the given shapes (u, b1f, b2f, b3f);
+    returns (time, dt, ncycle, u, b1f, b2f, b3f)."""
+    arrs = [np.zeros(s) for s in shapes]
+    t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+    D = ctypes.POINTER(ctypes.c_double)
+    _check(lib().agx_restart_read_host(os.fsencode(path), ctypes.byref(_desc(desc)), ctypes.byref(t), ctypes.byref(dt),
+                                       ctypes.byref(n), *[a.ctypes.data_as(D) for a in arrs]), "agx_restart_read_host")
+    return (t.value, dt.value, n.value, *arrs)
+
+
+def _desc(desc):
+    if isinstance(desc, Desc):
+        return desc
+    d = Desc()
+    for name, _ in Desc._fields_:
+        setattr(d, name, getattr(desc, name))
+    return d
+
 def block_lattice(desc):
     return desc.nx1 // desc.bnx1, desc.nx2 // desc.bnx2, desc.nx3 // desc.bnx3
 

@@ -12,8 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libathena_gamma_amd.so")
-# every HIP source of csrc/, largest first so the longest compile starts first
-SOURCES = sorted((f for f in os.listdir(CSRC) if f.endswith(".hip")),
+# every source of csrc/ (HIP kernels in .hip, host-only code in .cpp), largest first so
+# the longest compile starts first
+SOURCES = sorted((f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp"))),
                  key=lambda f: -os.path.getsize(os.path.join(CSRC, f)))
 
 # -ffp-contract=off: the FP64 path rounds every expression as written (bitwise parity

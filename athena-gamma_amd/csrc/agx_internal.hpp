@@ -141,7 +141,8 @@ struct agx_mesh {
   agx::Arrays arr;
   bool mixed = false;                // AGX_PREC_MIXED
   bool iso = false;                  // isothermal EOS (NHYDRO = 4)
-  int tile = 5;                      // stage form and tile shape (agx_autotune; 5 = k_vl2, 32 x 8)
+  int tile = 5;                      // stage form of the last VL2 stage (agx_autotune; 5 = k_vl2, 32 x 8)
+  int tile1 = 5;                     // stage form of the first (donor-cell) stage
   hipStream_t stream = nullptr;
   double *pool = nullptr;            // all MHD arrays
   int64_t pool_bytes = 0;
@@ -159,6 +160,7 @@ struct agx_mesh {
   std::vector<double> plm_h;
   double time = 0.0, dt = 0.0;
   int32_t ncycle = 0;
+  bool restarted = false;            // dt came from a restart file: agx_initialize keeps it
   // multigrid
   int mg_on = 0;
   int nrootlevel = 0, nmblevel = 0, ntotal = 0, current = 0, fmglevel = 0, coffset = 0;

@@ -165,10 +165,16 @@ struct Cons1D {
 };
 
 // HLLD (Miyoshi & Kusano 2005) for adiabatic MHD at one interface,
-// hydro/rsolvers/mhd/hlld.cpp:56-369.  SMALL_NUMBER = 1e-8 (hlld.cpp:32).
-template <typename T>
-__device__ __forceinline__ void hlld(T gamma, const T *wli, const T *wri,
-                                     T bxi, T *flxi) {
+// hydro/rsolvers/mhd/hlld.cpp:56-369, SMALL_NUMBER = 1e-8 (hlld.cpp:32); with LOW, the
+// low-dissipation variant LHLLD (Minoshima, Kitamura & Miyoshi 2021),
+// hydro/rsolvers/mhd/lhlld.cpp:56-389: the shock detector theta = th1^4 (from the
+// normal and transverse velocity differences dvn, dvt of the face) scales the pressure
+// jump in S_M, and phi = chi (2 - chi) the velocity jump in the star total pressure
+// (lhlld.cpp:158-187); the rest is HLLD, with lhlld.cpp's grouping in the one place it
+// differs (the transverse-field factor, :212).
+template <typename T, bool LOW>
+__device__ __forceinline__ void hlld_core(T gamma, const T *wli, const T *wri, T bxi, T dvn, T dvt,
+                                          T *flxi) {
   const T SMALL = T(1.0e-8);
   T igm1 = T(1.0) / (gamma - T(1.0));
   Cons1D<T> ul, ur, ulst, uldst, urdst, urst, fl, fr;
@@ -217,7 +223,14 @@ __device__ __forceinline__ void hlld(T gamma, const T *wli, const T *wri,
 
   T sdl = spd[0] - wli[IVX];
   T sdr = spd[4] - wri[IVX];
-  spd[2] = (sdr * ur.mx - sdl * ul.mx + (ptl - ptr)) / (sdr * ur.d - sdl * ul.d);
+  const T cfmax = dmax(cfl, cfr);
+  if (LOW) {   // shock detector, lhlld.cpp:159-162
+    T th1 = dmin(T(1.0), (cfmax - dmin(dvn, T(0.0))) / (cfmax - dmin(dvt, T(0.0))));
+    T th = th1 * th1 * th1 * th1;
+    spd[2] = (sdr * ur.mx - sdl * ul.mx + th * (ptl - ptr)) / (sdr * ur.d - sdl * ul.d);
+  } else {
+    spd[2] = (sdr * ur.mx - sdl * ul.mx + (ptl - ptr)) / (sdr * ur.d - sdl * ul.d);
+  }
   T sdml = spd[0] - spd[2];
   T sdmr = spd[4] - spd[2];
   T sdml_inv = T(1.0) / sdml;
@@ -231,9 +244,19 @@ __device__ __forceinline__ void hlld(T gamma, const T *wli, const T *wri,
   spd[1] = spd[2] - xabs(bxi) / sqrtdl;
   spd[3] = spd[2] + xabs(bxi) / sqrtdr;
 
-  T ptstl = ptl + ul.d * sdl * (spd[2] - wli[IVX]);
-  T ptstr = ptr + ur.d * sdr * (spd[2] - wri[IVX]);
-  T ptst = T(0.5) * (ptstr + ptstl);
+  T ptst;
+  if (LOW) {   // low-Mach star pressure, lhlld.cpp:182-187
+    T sdld = sdl * ul.d, sdrd = sdr * ur.d;
+    T clsq = ((pbl + kel) + xsqrt(SQR(pbl + kel) - T(2.0) * kel * bxsq)) / ul.d;
+    T crsq = ((pbr + ker) + xsqrt(SQR(pbr + ker) - T(2.0) * ker * bxsq)) / ur.d;
+    T chi = dmin(T(1.0), xsqrt(dmax(clsq, crsq)) / cfmax);
+    T phi = chi * (T(2.0) - chi);
+    ptst = (sdrd * ptl - sdld * ptr + phi * sdrd * sdld * (wri[IVX] - wli[IVX])) / (sdrd - sdld);
+  } else {
+    T ptstl = ptl + ul.d * sdl * (spd[2] - wli[IVX]);
+    T ptstr = ptr + ur.d * sdr * (spd[2] - wri[IVX]);
+    ptst = T(0.5) * (ptstr + ptstl);
+  }
 
   ulst.mx = ulst.d * spd[2];
   if (xabs(ul.d * sdl * sdml - bxsq) < SMALL * ptst) {
@@ -245,7 +268,7 @@ __device__ __forceinline__ void hlld(T gamma, const T *wli, const T *wri,
     T tmp = bxi * (sdl - sdml) / (ul.d * sdl * sdml - bxsq);
     ulst.my = ulst.d * (wli[IVY] - ul.by * tmp);
     ulst.mz = ulst.d * (wli[IVZ] - ul.bz * tmp);
-    tmp = (ul.d * SQR(sdl) - bxsq) / (ul.d * sdl * sdml - bxsq);
+    tmp = (LOW ? (sdl * ul.d) * sdl - bxsq : ul.d * SQR(sdl) - bxsq) / (ul.d * sdl * sdml - bxsq);
     ulst.by = ul.by * tmp;
     ulst.bz = ul.bz * tmp;
   }
@@ -263,7 +286,7 @@ __device__ __forceinline__ void hlld(T gamma, const T *wli, const T *wri,
     T tmp = bxi * (sdr - sdmr) / (ur.d * sdr * sdmr - bxsq);
     urst.my = urst.d * (wri[IVY] - ur.by * tmp);
     urst.mz = urst.d * (wri[IVZ] - ur.bz * tmp);
-    tmp = (ur.d * SQR(sdr) - bxsq) / (ur.d * sdr * sdmr - bxsq);
+    tmp = (LOW ? (sdr * ur.d) * sdr - bxsq : ur.d * SQR(sdr) - bxsq) / (ur.d * sdr * sdmr - bxsq);
     urst.by = ur.by * tmp;
     urst.bz = ur.bz * tmp;
   }
@@ -341,6 +364,11 @@ __device__ __forceinline__ void hlld(T gamma, const T *wli, const T *wri,
     flxi[IBY] = fr.by + spd[4] * (urst.by - ur.by);
     flxi[IBZ] = fr.bz + spd[4] * (urst.bz - ur.bz);
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void hlld(T gamma, const T *wli, const T *wri, T bxi, T *flxi) {
+  hlld_core<T, false>(gamma, wli, wri, bxi, T(0.0), T(0.0), flxi);
 }
 
 // HLLE at one interface, hydro/rsolvers/mhd/hlle_mhd.cpp:56-171: adiabatic, or the
@@ -875,9 +903,9 @@ __device__ __forceinline__ void roe(T gamma, const T *wli, const T *wri,
 // ez = F(Bz) (hlld.cpp:371-377); wct = GetWeightForCT.
 template <int ORDER, int RS, typename T, bool ISO>
 __device__ __forceinline__ void face_flux(int dir, const T *__restrict__ wb, const T *__restrict__ bb,
-                                          int64_t S, int64_t id, int64_t st, const double *__restrict__ pc,
-                                          int PL, int cidx, T bxi, T gamma, T cs, T dfl, T pfl, T dx, T dt, T *f,
-                                          T &ey, T &ez, T &wct) {
+                                          int64_t S, int64_t id, int64_t st, int64_t st1, int64_t st2,
+                                          const double *__restrict__ pc, int PL, int cidx, T bxi, T gamma, T cs,
+                                          T dfl, T pfl, T dx, T dt, T *f, T &ey, T &ez, T &wct) {
   const int d1 = (dir == 2) ? 0 : dir + 1, d2 = (dir == 0) ? 2 : dir - 1;   // (dir+1)%3, (dir+2)%3
   T wli[NWAVE], wri[NWAVE];
 #pragma unroll
@@ -937,7 +965,21 @@ __device__ __forceinline__ void face_flux(int dir, const T *__restrict__ wb, con
   if (RS == AGX_RSOLVER_HLLE) hlle<ISO>(gamma, cs, wli, wri, bxi, flxi);
   else if (ISO) hlld_iso(cs, dfl, wli, wri, bxi, flxi);
   else if (RS == AGX_RSOLVER_ROE) roe(gamma, wli, wri, bxi, flxi);
-  else hlld(gamma, wli, wri, bxi, flxi);
+  else if (RS == AGX_RSOLVER_LHLLD) {
+    // Hydro::CalculateVelocityDifferences at this face (calculate_velocity_differences.cpp:
+    // 23-68): the normal velocity jump, and the smallest one-sided difference of each
+    // transverse velocity along its own direction in the two cells beside the face
+    const T *vn = wb + (int64_t)(IVX + dir) * S, *v1 = wb + (int64_t)(IVX + d1) * S;
+    const T *v2 = wb + (int64_t)(IVX + d2) * S;
+    const int64_t l = id - st;
+    T dvn = vn[id] - vn[l];
+    T dvl = dmin(v1[l + st1] - v1[l], v1[l] - v1[l - st1]);
+    T dvr = dmin(v1[id + st1] - v1[id], v1[id] - v1[id - st1]);
+    T dwl = dmin(v2[l + st2] - v2[l], v2[l] - v2[l - st2]);
+    T dwr = dmin(v2[id + st2] - v2[id], v2[id] - v2[id - st2]);
+    T dvt = dmin(dmin(dvl, dvr), dmin(dwl, dwr));
+    hlld_core<T, true>(gamma, wli, wri, bxi, dvn, dvt, flxi);
+  } else hlld(gamma, wli, wri, bxi, flxi);
   f[0] = flxi[IDN];
   f[1] = flxi[IVX];
   f[2] = flxi[IVY];
@@ -961,13 +1003,14 @@ k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
   int kl = g.ks - (DIR != 2);
   PlaneIdx p;
   if (!plane_index(il, iu, jl, ju, kl, p, DIR == 2)) return;   // x3: 4-plane stencil
-  const int64_t st = (DIR == 0) ? 1 : (DIR == 1 ? (int64_t)g.P : (int64_t)g.J * g.P);
+  const int64_t sd[3] = {1, (int64_t)g.P, (int64_t)g.J * g.P};   // strides along i, j, k
   const int64_t id = IDX(g, p.k, p.j, p.i);
   const int cidx = (DIR == 0) ? p.i : (DIR == 1 ? p.j : p.k);   // index of the cell above the face
   const double *pc = plm + ((int64_t)p.b * 3 + DIR) * 2 * g.PL;  // [dxp][dxm] rows
   T f[5], fey, fez, fw;
-  face_flux<ORDER, RS, T, ISO>(DIR, w + (int64_t)p.b * NHYDRO * g.S, bcc + (int64_t)p.b * 3 * g.S, g.S, id, st, pc,
-                               g.PL, cidx, T(bn[(int64_t)p.b * g.S + id]), T(gamma_), T(cs_), T(dfl_), T(pfl_),
+  face_flux<ORDER, RS, T, ISO>(DIR, w + (int64_t)p.b * NHYDRO * g.S, bcc + (int64_t)p.b * 3 * g.S, g.S, id,
+                               sd[DIR], sd[(DIR + 1) % 3], sd[(DIR + 2) % 3], pc, g.PL, cidx,
+                               T(bn[(int64_t)p.b * g.S + id]), T(gamma_), T(cs_), T(dfl_), T(pfl_),
                                T(coef[p.b].dx[DIR]), T(*dtp), f, fey, fez, fw);
   constexpr int ivx = IVX + DIR, ivy = IVX + (DIR + 1) % 3, ivz = IVX + (DIR + 2) % 3;
   T *fb = flx + (int64_t)p.b * NHYDRO * g.S + id;
@@ -1717,12 +1760,15 @@ k_vl2(Geo g, VL2Args<TW> a) {
       if (dir == 2 && !x3) continue;
       const int fi = i0 + ii, fj = j0 + jj;
       const int64_t id = IDX(g, c, fj, fi);
+      // strides along dir and the two transverse directions (dir+1, dir+2) mod 3
       const int64_t st = (dir == 0) ? 1 : (dir == 1 ? sj : sk);
+      const int64_t st1 = (dir == 0) ? sj : (dir == 1 ? sk : 1);
+      const int64_t st2 = (dir == 0) ? sk : (dir == 1 ? 1 : sj);
       const int cidx = (dir == 0) ? fi : (dir == 1 ? fj : c);
       const double *bn = (dir == 0) ? a.bn1 : (dir == 1 ? a.bn2 : a.bn3);
       const TW dxd = (dir == 0) ? dxT[0] : (dir == 1 ? dxT[1] : dxT[2]);
       TW f[5], ey, ez, wc;
-      face_flux<ORDER, RS, TW, ISO>(dir, wb, bb, S, id, st, pcb + (int64_t)dir * 2 * g.PL, g.PL, cidx,
+      face_flux<ORDER, RS, TW, ISO>(dir, wb, bb, S, id, st, st1, st2, pcb + (int64_t)dir * 2 * g.PL, g.PL, cidx,
                                     TW(bn[fb + id]), gT, csT, dflT, pflT, dxd, dtT, f, ey, ez, wc);
       if (dir == 0) {
         s_f1[0][jj + 1][ii] = ey; s_f1[1][jj + 1][ii] = ez; s_f1[2][jj + 1][ii] = wc;
@@ -2149,8 +2195,11 @@ int check_desc(const agx_mesh_desc *d) {
     set_error("nghost larger than a MeshBlock"); return -7;
   }
   if (d->rsolver != AGX_RSOLVER_HLLD && d->rsolver != AGX_RSOLVER_HLLE &&
-      d->rsolver != AGX_RSOLVER_ROE) {
+      d->rsolver != AGX_RSOLVER_ROE && d->rsolver != AGX_RSOLVER_LHLLD) {
     set_error("unknown Riemann solver %d", d->rsolver); return -8;
+  }
+  if (d->eos == 1 && d->rsolver == AGX_RSOLVER_LHLLD) {   // configure.py:321-322
+    set_error("LHLLD flux cannot be used with the isothermal EOS"); return -9;
   }
   if (d->eos != 0 && d->eos != 1) { set_error("eos must be 0 (adiabatic) or 1 (isothermal)"); return -9; }
   if (d->eos == 1 && d->rsolver == AGX_RSOLVER_ROE) {
@@ -2633,6 +2682,7 @@ static int calculate_fluxes(agx_mesh *m, int order) {
 #define FLUX_ORD(DIR, ORD, B, F, EY, EZ, WX)                                                    \
   if (rs == AGX_RSOLVER_HLLE) { FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_HLLE, B, F, EY, EZ, WX); }    \
   else if (rs == AGX_RSOLVER_ROE) { FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_ROE, B, F, EY, EZ, WX); } \
+  else if (rs == AGX_RSOLVER_LHLLD) { FLUX_LAUNCH_E(DIR, ORD, AGX_RSOLVER_LHLLD, B, F, EY, EZ, WX, false); } \
   else { FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_HLLD, B, F, EY, EZ, WX); }
 #define FLUX_DIR(DIR, B, F, EY, EZ, WX)                                                         \
   timer_begin(m, AGX_K_FLUX_X1 + DIR);                                                        \
@@ -2828,7 +2878,10 @@ int agx_initialize(agx_mesh *m, double tlim) {
   if (!rc) rc = agx_bvals_exchange(m, 3);               // mesh.cpp:1404-1453
   if (!rc) rc = agx_cons2prim(m);                       // mesh.cpp:1543-1545
   if (rc) return rc;
-  m->dt = DBL_MAX;                                      // mesh.cpp:91
+  // a fresh Mesh starts from dt = DBL_MAX (mesh.cpp:91); after agx_restart_read the dt of
+  // the file caps the first step, as the restart constructor's dt does (mesh.cpp:667)
+  if (!m->restarted) m->dt = DBL_MAX;
+  m->restarted = false;
   return agx_new_time_step(m, tlim);                    // mesh.cpp:1646-1649
 }
 
@@ -2845,7 +2898,7 @@ int agx_initialize(agx_mesh *m, double tlim) {
 // rotation leaves b = new faces and b1 = half-step faces, as the reference's swap does.
 extern "C++" {
 template <typename TW>
-static void launch_stage(agx_mesh *m, int stage, double wfac, double *dtpart) {
+static void launch_stage(agx_mesh *m, int tile, int stage, double wfac, double *dtpart) {
   const Geo &g = m->g;
   double **A = m->arr.a;
   StageArgs<TW> a;
@@ -2874,16 +2927,16 @@ static void launch_stage(agx_mesh *m, int stage, double wfac, double *dtpart) {
   a.cs = m->d.iso_cs;
   a.grav = m->d.self_gravity ? 1 : 0;
   a.dtpart = dtpart;
-  const dim3 grid = stage_grid(g, m->tile);
+  const dim3 grid = stage_grid(g, tile);
   if (m->iso) {
-    if (m->tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, true>), grid, dim3(256), 0, m->stream, g, a);
-    else if (m->tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, true>), grid, dim3(256), 0, m->stream, g, a);
-    else if (m->tile == 3) hipLaunchKernelGGL((k_stage<64, 8, TW, true>), grid, dim3(512), 0, m->stream, g, a);
+    if (tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, true>), grid, dim3(256), 0, m->stream, g, a);
+    else if (tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, true>), grid, dim3(256), 0, m->stream, g, a);
+    else if (tile == 3) hipLaunchKernelGGL((k_stage<64, 8, TW, true>), grid, dim3(512), 0, m->stream, g, a);
     else hipLaunchKernelGGL((k_stage<16, 16, TW, true>), grid, dim3(256), 0, m->stream, g, a);
   } else {
-    if (m->tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, false>), grid, dim3(256), 0, m->stream, g, a);
-    else if (m->tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, false>), grid, dim3(256), 0, m->stream, g, a);
-    else if (m->tile == 3) hipLaunchKernelGGL((k_stage<64, 8, TW, false>), grid, dim3(512), 0, m->stream, g, a);
+    if (tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, false>), grid, dim3(256), 0, m->stream, g, a);
+    else if (tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, false>), grid, dim3(256), 0, m->stream, g, a);
+    else if (tile == 3) hipLaunchKernelGGL((k_stage<64, 8, TW, false>), grid, dim3(512), 0, m->stream, g, a);
     else hipLaunchKernelGGL((k_stage<16, 16, TW, false>), grid, dim3(256), 0, m->stream, g, a);
   }
 }
@@ -2904,6 +2957,8 @@ static void launch_vl2_rs(agx_mesh *m, int rs, const VL2Args<TW> &a, dim3 grid) 
       hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLE, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
     else if (rs == AGX_RSOLVER_ROE)
       hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_ROE, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+    else if (rs == AGX_RSOLVER_LHLLD)
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_LHLLD, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
     else
       hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLD, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
   }
@@ -2917,7 +2972,7 @@ static void launch_vl2_order(agx_mesh *m, int order, const VL2Args<TW> &a, dim3 
 }
 
 template <typename TW>
-static void launch_vl2(agx_mesh *m, int stage, int order, double wfac, double *dtpart) {
+static void launch_vl2(agx_mesh *m, int tile, int stage, int order, double wfac, double *dtpart) {
   const Geo &g = m->g;
   double **A = m->arr.a;
   VL2Args<TW> a;
@@ -2943,12 +2998,12 @@ static void launch_vl2(agx_mesh *m, int stage, int order, double wfac, double *d
   a.cs = m->d.iso_cs;
   a.grav = m->d.self_gravity ? 1 : 0;
   a.dtpart = dtpart;
-  const dim3 grid = stage_grid(g, m->tile);
+  const dim3 grid = stage_grid(g, tile);
   if (m->iso) {
-    if (m->tile == 5) launch_vl2_order<32, 8, TW, true>(m, order, a, grid);
+    if (tile == 5) launch_vl2_order<32, 8, TW, true>(m, order, a, grid);
     else launch_vl2_order<16, 16, TW, true>(m, order, a, grid);
   } else {
-    if (m->tile == 5) launch_vl2_order<32, 8, TW, false>(m, order, a, grid);
+    if (tile == 5) launch_vl2_order<32, 8, TW, false>(m, order, a, grid);
     else launch_vl2_order<16, 16, TW, false>(m, order, a, grid);
   }
 }
@@ -2976,19 +3031,22 @@ static int stage_fused(agx_mesh *m, int stage, bool want_dt) {
   double **A = m->arr.a;
   const int order = (stage == 1) ? 1 : m->d.xorder;
   const double wfac = (stage == 1 ? 0.5 : 1.0);   // wght = wfac * dt on the device
-  if (m->tile >= TILE_VL2) {
+  // each stage runs the form tuned for its reconstruction order; the last stage's form
+  // sets the layout of the time-step partials (m->tile)
+  const int tile = (stage == 1) ? m->tile1 : m->tile;
+  if (tile >= TILE_VL2) {
     if (order == 3 && m->g.ng < 3) { set_error("PPM needs nghost >= 3 (reconstruction.cpp:90-100)"); return -3; }
     timer_begin(m, AGX_K_INTEGRATE);
-    if (m->mixed) launch_vl2<float>(m, stage, order, wfac, want_dt ? m->dtpart : nullptr);
-    else launch_vl2<double>(m, stage, order, wfac, want_dt ? m->dtpart : nullptr);
+    if (m->mixed) launch_vl2<float>(m, tile, stage, order, wfac, want_dt ? m->dtpart : nullptr);
+    else launch_vl2<double>(m, tile, stage, order, wfac, want_dt ? m->dtpart : nullptr);
     timer_end(m);
     LAUNCH_CHECK("k_vl2");
   } else {
     int rc = calculate_fluxes(m, order);
     if (rc) return rc;
     timer_begin(m, AGX_K_INTEGRATE);
-    if (m->mixed) launch_stage<float>(m, stage, wfac, want_dt ? m->dtpart : nullptr);
-    else launch_stage<double>(m, stage, wfac, want_dt ? m->dtpart : nullptr);
+    if (m->mixed) launch_stage<float>(m, tile, stage, wfac, want_dt ? m->dtpart : nullptr);
+    else launch_stage<double>(m, tile, stage, wfac, want_dt ? m->dtpart : nullptr);
     timer_end(m);
     LAUNCH_CHECK("k_stage");
   }
@@ -3162,42 +3220,45 @@ int agx_mg_vcycles(agx_mesh *m, int32_t n) {
 // fastest.  Call between cycles.
 int agx_autotune(agx_mesh *m, int32_t *choice) {
   if (!m) { set_error("null mesh"); return -1; }
-  const int order = m->d.xorder;
   hipEvent_t e0, e1;
   AGX_HIP(hipEventCreate(&e0));
   AGX_HIP(hipEventCreate(&e1));
   const double wfac = 0.5;
   graph_reset(m);
-  float best = 0.f;
-  int best_t = 0;
   const bool timing = m->timer.enabled;
   m->timer.enabled = false;
-  for (int t = 0; t < NTILE; ++t) {
-    m->tile = t;
-    float ms = 0.f;
-    for (int rep = 0; rep < 4; ++rep) {
-      if (rep == 1) AGX_HIP(hipEventRecord(e0, m->stream));
-      if (t >= TILE_VL2) {   // the whole stage in one launch
-        if (m->mixed) launch_vl2<float>(m, 1, order, wfac, nullptr);
-        else launch_vl2<double>(m, 1, order, wfac, nullptr);
-      } else {               // fluxes, then the stage kernel
-        int rc = calculate_fluxes(m, order);
-        if (rc) return rc;
-        if (m->mixed) launch_stage<float>(m, 1, wfac, nullptr);
-        else launch_stage<double>(m, 1, wfac, nullptr);
+  int best_of[4] = {0, 0, 0, 0};   // per reconstruction order
+  for (int order : {1, m->d.xorder}) {
+    float best = 0.f;
+    int best_t = 0;
+    for (int t = 0; t < NTILE; ++t) {
+      float ms = 0.f;
+      for (int rep = 0; rep < 4; ++rep) {
+        if (rep == 1) AGX_HIP(hipEventRecord(e0, m->stream));
+        if (t >= TILE_VL2) {   // the whole stage in one launch
+          if (m->mixed) launch_vl2<float>(m, t, 1, order, wfac, nullptr);
+          else launch_vl2<double>(m, t, 1, order, wfac, nullptr);
+        } else {               // fluxes, then the stage kernel
+          int rc = calculate_fluxes(m, order);
+          if (rc) return rc;
+          if (m->mixed) launch_stage<float>(m, t, 1, wfac, nullptr);
+          else launch_stage<double>(m, t, 1, wfac, nullptr);
+        }
       }
+      AGX_HIP(hipEventRecord(e1, m->stream));
+      AGX_HIP(hipEventSynchronize(e1));
+      AGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (t == 0 || ms < best) { best = ms; best_t = t; }
     }
-    AGX_HIP(hipEventRecord(e1, m->stream));
-    AGX_HIP(hipEventSynchronize(e1));
-    AGX_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (t == 0 || ms < best) { best = ms; best_t = t; }
+    best_of[order] = best_t;
   }
   m->timer.enabled = timing;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   LAUNCH_CHECK("stage forms (autotune)");
-  m->tile = best_t;
-  if (choice) *choice = best_t;
+  m->tile1 = best_of[1];
+  m->tile = best_of[m->d.xorder];
+  if (choice) *choice = m->tile;
   return mg_autotune(m);
 }
 
@@ -3210,8 +3271,8 @@ int agx_set_mg_fuse(agx_mesh *m, int32_t mode) {
 
 int agx_set_tile(agx_mesh *m, int32_t tile) {
   if (!m) { set_error("null mesh"); return -1; }
-  if (tile < 0 || tile >= NTILE) { set_error("tile shape %d not in [0, %d)", tile, NTILE); return -2; }
-  m->tile = tile;
+  if (tile < 0 || tile >= NTILE) { set_error("stage form %d not in [0, %d)", tile, NTILE); return -2; }
+  m->tile = m->tile1 = tile;
   graph_reset(m);
   return 0;
 }

@@ -70,7 +70,10 @@ typedef struct agx_mesh_desc {
                                    (mg_gravity.cpp:106-113) */
 } agx_mesh_desc;
 
-enum { AGX_RSOLVER_HLLD = 0, AGX_RSOLVER_HLLE = 1, AGX_RSOLVER_ROE = 2 };
+/* Riemann solvers (configure --flux): hlld.cpp, hlle_mhd.cpp, roe_mhd.cpp, and lhlld.cpp
+ * (low-dissipation HLLD with the shock detector of calculate_velocity_differences.cpp;
+ * adiabatic only, configure.py:321-322) */
+enum { AGX_RSOLVER_HLLD = 0, AGX_RSOLVER_HLLE = 1, AGX_RSOLVER_ROE = 2, AGX_RSOLVER_LHLLD = 3 };
 /* Precision forms.  FP64: every array and every operation in FP64, results identical to
  * the reference's FP64 arithmetic.  MIXED: reconstruction and the Riemann solver run in
  * FP32 from FP32 copies of w and bcc; fluxes, face EMFs and CT weights are stored in FP32;
@@ -224,9 +227,31 @@ int agx_history_sum_noalloc(const double *u, const double *bcc, const double *ph
                             int32_t nhydro, int32_t nx1, int32_t nx2, int32_t nx3, int32_t ng, const double *vol,
                             double *hst, void *stream);
 
+/* ---- restart files (RestartOutput::WriteOutputFile, outputs/restart.cpp:37-207, and
+ * the restart constructors Mesh::Mesh(pin, resfile), mesh/mesh.cpp:551-700).  The file
+ * layout is the reference's: the parameter text (ParameterInput::ParameterDump) ending in
+ * "<par_end>", the header (nbtotal, root_level, RegionSize mesh_size, time, dt, ncycle,
+ * datasize), the ID list (LogicalLocation + cost per MeshBlock, Z order of
+ * MeshBlockTree::GetMeshBlockList) and per MeshBlock u then b.x1f, b.x2f, b.x3f with
+ * ghosts.  Uniform root-level meshes.  params: the caller's parameter text without
+ * "<par_end>", or null for the inputs the descriptor carries.  The gravitational
+ * potential is not stored; agx_initialize after agx_restart_read solves for it, exchanges
+ * ghosts, converts to primitives and takes the first step with the file's dt as the cap
+ * (mesh.cpp:1646-1649 with the restored dt). */
+int agx_restart_write(agx_mesh *m, const char *path, const char *params);
+int agx_restart_read(agx_mesh *m, const char *path);
+/* The same on host arrays in the layout of agx_upload (block gid order, NHYDRO = 4 for
+ * the isothermal EOS); no device work. */
+int agx_restart_write_host(const char *path, const agx_mesh_desc *d, const char *params, double time, double dt,
+                           int32_t ncycle, const double *u, const double *b1f, const double *b2f,
+                           const double *b3f);
+int agx_restart_read_host(const char *path, const agx_mesh_desc *d, double *time, double *dt, int32_t *ncycle,
+                          double *u, double *b1f, double *b2f, double *b3f);
+
 /* ---- tuning (no counterpart in the reference: launch shapes of this library).
- * agx_autotune times each form of a VL2 stage on the current state and keeps the
- * fastest; *choice gets its index.  Forms 0..3: three flux launches, then one stage
+ * agx_autotune times each form of a VL2 stage on the current state, for the first
+ * (donor-cell) stage and for the second (the mesh's order) separately, and keeps the
+ * fastest for each; *choice gets the second stage's form.  Forms 0..3: three flux launches, then one stage
  * kernel over 16x16, 32x8, 64x4 (i,j) columns per 256-thread workgroup or 64x8 per
  * 512-thread workgroup; forms 4..5: the whole stage (fluxes in LDS) in one launch over
  * 16x16 or 32x8 columns (form 5 is the default).  With multigrid
@@ -236,7 +261,7 @@ int agx_history_sum_noalloc(const double *u, const double *bcc, const double *ph
  * cycles, e.g. after agx_initialize.  agx_set_tile forces a shape.  Results do not depend
  * on either choice. */
 int agx_autotune(agx_mesh *m, int32_t *choice);
-int agx_set_tile(agx_mesh *m, int32_t tile);
+int agx_set_tile(agx_mesh *m, int32_t tile);   /* the same form for both stages */
 /* multigrid red-black pass form: -1 by level size (fused on FP64 levels of >= 2^26
  * cells, the default), 0 two sweeps, 1 fused on every level of >= 2^21 cells */
 int agx_set_mg_fuse(agx_mesh *m, int32_t mode);

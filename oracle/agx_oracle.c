@@ -700,6 +700,223 @@ static void hlld(double gamma, const double *wli, const double *wri, double bxi,
   }
 }
 
+/* Low-dissipation HLLD (Minoshima et al. 2021) for adiabatic MHD,
+ * hydro/rsolvers/mhd/lhlld.cpp:38-389 (one interface).  dvn, dvt are the normal and
+ * transverse velocity differences of Hydro::CalculateVelocityDifferences at this face
+ * (the shock detector, lhlld.cpp:159-161).  Restated literally: every star and
+ * double-star difference is formed before the flux is selected (lhlld.cpp:287-378). */
+static void lhlld(double gamma, const double *wli, const double *wri, double bxi, double dvn,
+                  double dvt, double *flxi) {
+  double igm1 = 1.0 / (gamma - 1.0);
+  cons1d ul, ur, ulst, uldst, urdst, urst, fl, fr;
+  double spd[5];
+  double bxsq = bxi * bxi;
+  double pbl = 0.5 * (bxsq + (SQR(wli[IBY]) + SQR(wli[IBZ])));
+  double pbr = 0.5 * (bxsq + (SQR(wri[IBY]) + SQR(wri[IBZ])));
+  double kel = 0.5 * wli[IDN] * (SQR(wli[IVX]) + (SQR(wli[IVY]) + SQR(wli[IVZ])));
+  double ker = 0.5 * wri[IDN] * (SQR(wri[IVX]) + (SQR(wri[IVY]) + SQR(wri[IVZ])));
+  ul.d = wli[IDN];
+  ul.mx = wli[IVX] * ul.d;
+  ul.my = wli[IVY] * ul.d;
+  ul.mz = wli[IVZ] * ul.d;
+  ul.e = wli[IPR] * igm1 + kel + pbl;
+  ul.by = wli[IBY];
+  ul.bz = wli[IBZ];
+  ur.d = wri[IDN];
+  ur.mx = wri[IVX] * ur.d;
+  ur.my = wri[IVY] * ur.d;
+  ur.mz = wri[IVZ] * ur.d;
+  ur.e = wri[IPR] * igm1 + ker + pbr;
+  ur.by = wri[IBY];
+  ur.bz = wri[IBZ];
+  /* Step 2: speeds (lhlld.cpp:115-120) */
+  double cfl = fast_speed(gamma, wli, bxi);
+  double cfr = fast_speed(gamma, wri, bxi);
+  spd[0] = dmin(wli[IVX] - cfl, wri[IVX] - cfr);
+  spd[4] = dmax(wli[IVX] + cfl, wri[IVX] + cfr);
+  double cfmax = dmax(cfl, cfr);
+  /* Step 3: L/R fluxes (lhlld.cpp:130-148) */
+  double ptl = wli[IPR] + pbl;
+  double ptr = wri[IPR] + pbr;
+  fl.d = ul.mx;
+  fl.mx = ul.mx * wli[IVX] + ptl - bxsq;
+  fl.my = ul.my * wli[IVX] - bxi * ul.by;
+  fl.mz = ul.mz * wli[IVX] - bxi * ul.bz;
+  fl.e = wli[IVX] * (ul.e + ptl - bxsq) - bxi * (wli[IVY] * ul.by + wli[IVZ] * ul.bz);
+  fl.by = ul.by * wli[IVX] - bxi * wli[IVY];
+  fl.bz = ul.bz * wli[IVX] - bxi * wli[IVZ];
+  fr.d = ur.mx;
+  fr.mx = ur.mx * wri[IVX] + ptr - bxsq;
+  fr.my = ur.my * wri[IVX] - bxi * ur.by;
+  fr.mz = ur.mz * wri[IVX] - bxi * ur.bz;
+  fr.e = wri[IVX] * (ur.e + ptr - bxsq) - bxi * (wri[IVY] * ur.by + wri[IVZ] * ur.bz);
+  fr.by = ur.by * wri[IVX] - bxi * wri[IVY];
+  fr.bz = ur.bz * wri[IVX] - bxi * wri[IVZ];
+  /* Step 4: middle and Alfven speeds with the shock detector (lhlld.cpp:150-178) */
+  double sdl = spd[0] - wli[IVX];
+  double sdr = spd[4] - wri[IVX];
+  double sdld = sdl * ul.d;
+  double sdrd = sdr * ur.d;
+  double th1 = dmin(1.0, (cfmax - dmin(dvn, 0.0)) / (cfmax - dmin(dvt, 0.0)));
+  double th = th1 * th1 * th1 * th1;
+  spd[2] = (sdr * ur.mx - sdl * ul.mx + th * (ptl - ptr)) / (sdrd - sdld);
+  double sdml = spd[0] - spd[2];
+  double sdmr = spd[4] - spd[2];
+  double sdml_inv = 1.0 / sdml;
+  double sdmr_inv = 1.0 / sdmr;
+  ulst.d = sdld * sdml_inv;
+  urst.d = sdrd * sdmr_inv;
+  double ulst_d_inv = 1.0 / ulst.d;
+  double urst_d_inv = 1.0 / urst.d;
+  double sqrtdl = sqrt(ulst.d);
+  double sqrtdr = sqrt(urst.d);
+  spd[1] = spd[2] - fabs(bxi) / sqrtdl;
+  spd[3] = spd[2] + fabs(bxi) / sqrtdr;
+  /* Step 5: intermediate states; the total pressure with the low-Mach factor phi
+   * (lhlld.cpp:182-187) */
+  double clsq = ((pbl + kel) + sqrt(SQR(pbl + kel) - 2.0 * kel * bxsq)) / ul.d;
+  double crsq = ((pbr + ker) + sqrt(SQR(pbr + ker) - 2.0 * ker * bxsq)) / ur.d;
+  double chi = dmin(1.0, sqrt(dmax(clsq, crsq)) / cfmax);
+  double phi = chi * (2.0 - chi);
+  double ptst = (sdrd * ptl - sdld * ptr + phi * sdrd * sdld * (wri[IVX] - wli[IVX])) / (sdrd - sdld);
+  ulst.mx = ulst.d * spd[2];
+  if (fabs(sdld * sdml - bxsq) < SMALL_NUMBER * ptst) {
+    ulst.my = ulst.d * wli[IVY];
+    ulst.mz = ulst.d * wli[IVZ];
+    ulst.by = ul.by;
+    ulst.bz = ul.bz;
+  } else {
+    double tmp = bxi * (sdl - sdml) / (sdld * sdml - bxsq);
+    ulst.my = ulst.d * (wli[IVY] - ul.by * tmp);
+    ulst.mz = ulst.d * (wli[IVZ] - ul.bz * tmp);
+    tmp = (sdld * sdl - bxsq) / (sdld * sdml - bxsq);
+    ulst.by = ul.by * tmp;
+    ulst.bz = ul.bz * tmp;
+  }
+  double vbstl = (ulst.mx * bxi + (ulst.my * ulst.by + ulst.mz * ulst.bz)) * ulst_d_inv;
+  ulst.e = (sdl * ul.e - ptl * wli[IVX] + ptst * spd[2] +
+            bxi * (wli[IVX] * bxi + (wli[IVY] * ul.by + wli[IVZ] * ul.bz) - vbstl)) * sdml_inv;
+  urst.mx = urst.d * spd[2];
+  if (fabs(sdrd * sdmr - bxsq) < SMALL_NUMBER * ptst) {
+    urst.my = urst.d * wri[IVY];
+    urst.mz = urst.d * wri[IVZ];
+    urst.by = ur.by;
+    urst.bz = ur.bz;
+  } else {
+    double tmp = bxi * (sdr - sdmr) / (sdrd * sdmr - bxsq);
+    urst.my = urst.d * (wri[IVY] - ur.by * tmp);
+    urst.mz = urst.d * (wri[IVZ] - ur.bz * tmp);
+    tmp = (sdrd * sdr - bxsq) / (sdrd * sdmr - bxsq);
+    urst.by = ur.by * tmp;
+    urst.bz = ur.bz * tmp;
+  }
+  double vbstr = (urst.mx * bxi + (urst.my * urst.by + urst.mz * urst.bz)) * urst_d_inv;
+  urst.e = (sdr * ur.e - ptr * wri[IVX] + ptst * spd[2] +
+            bxi * (wri[IVX] * bxi + (wri[IVY] * ur.by + wri[IVZ] * ur.bz) - vbstr)) * sdmr_inv;
+  if (0.5 * bxsq < SMALL_NUMBER * ptst) {
+    uldst = ulst;
+    urdst = urst;
+  } else {
+    double invsumd = 1.0 / (sqrtdl + sqrtdr);
+    double bxsig = (bxi > 0.0 ? 1.0 : -1.0);
+    uldst.d = ulst.d;
+    urdst.d = urst.d;
+    uldst.mx = ulst.mx;
+    urdst.mx = urst.mx;
+    double tmp = invsumd * (sqrtdl * (ulst.my * ulst_d_inv) + sqrtdr * (urst.my * urst_d_inv) +
+                            bxsig * (urst.by - ulst.by));
+    uldst.my = uldst.d * tmp;
+    urdst.my = urdst.d * tmp;
+    tmp = invsumd * (sqrtdl * (ulst.mz * ulst_d_inv) + sqrtdr * (urst.mz * urst_d_inv) +
+                     bxsig * (urst.bz - ulst.bz));
+    uldst.mz = uldst.d * tmp;
+    urdst.mz = urdst.d * tmp;
+    tmp = invsumd * (sqrtdl * urst.by + sqrtdr * ulst.by +
+                     bxsig * sqrtdl * sqrtdr * ((urst.my * urst_d_inv) - (ulst.my * ulst_d_inv)));
+    uldst.by = urdst.by = tmp;
+    tmp = invsumd * (sqrtdl * urst.bz + sqrtdr * ulst.bz +
+                     bxsig * sqrtdl * sqrtdr * ((urst.mz * urst_d_inv) - (ulst.mz * ulst_d_inv)));
+    uldst.bz = urdst.bz = tmp;
+    tmp = spd[2] * bxi + (uldst.my * uldst.by + uldst.mz * uldst.bz) / uldst.d;
+    uldst.e = ulst.e - sqrtdl * bxsig * (vbstl - tmp);
+    urdst.e = urst.e + sqrtdr * bxsig * (vbstr - tmp);
+  }
+  /* Step 6: flux differences of every state, then the selection (lhlld.cpp:287-378) */
+  cons1d dld, dl, drd, dr;
+  dld.d = spd[1] * (uldst.d - ulst.d); dld.mx = spd[1] * (uldst.mx - ulst.mx);
+  dld.my = spd[1] * (uldst.my - ulst.my); dld.mz = spd[1] * (uldst.mz - ulst.mz);
+  dld.e = spd[1] * (uldst.e - ulst.e); dld.by = spd[1] * (uldst.by - ulst.by);
+  dld.bz = spd[1] * (uldst.bz - ulst.bz);
+  dl.d = spd[0] * (ulst.d - ul.d); dl.mx = spd[0] * (ulst.mx - ul.mx);
+  dl.my = spd[0] * (ulst.my - ul.my); dl.mz = spd[0] * (ulst.mz - ul.mz);
+  dl.e = spd[0] * (ulst.e - ul.e); dl.by = spd[0] * (ulst.by - ul.by);
+  dl.bz = spd[0] * (ulst.bz - ul.bz);
+  drd.d = spd[3] * (urdst.d - urst.d); drd.mx = spd[3] * (urdst.mx - urst.mx);
+  drd.my = spd[3] * (urdst.my - urst.my); drd.mz = spd[3] * (urdst.mz - urst.mz);
+  drd.e = spd[3] * (urdst.e - urst.e); drd.by = spd[3] * (urdst.by - urst.by);
+  drd.bz = spd[3] * (urdst.bz - urst.bz);
+  dr.d = spd[4] * (urst.d - ur.d); dr.mx = spd[4] * (urst.mx - ur.mx);
+  dr.my = spd[4] * (urst.my - ur.my); dr.mz = spd[4] * (urst.mz - ur.mz);
+  dr.e = spd[4] * (urst.e - ur.e); dr.by = spd[4] * (urst.by - ur.by);
+  dr.bz = spd[4] * (urst.bz - ur.bz);
+  const cons1d *f;
+  if (spd[0] >= 0.0 || spd[4] <= 0.0) {
+    f = (spd[0] >= 0.0) ? &fl : &fr;
+    flxi[IDN] = f->d; flxi[IVX] = f->mx; flxi[IVY] = f->my; flxi[IVZ] = f->mz;
+    flxi[IEN] = f->e; flxi[IBY] = f->by; flxi[IBZ] = f->bz;
+  } else if (spd[1] >= 0.0) {
+    flxi[IDN] = fl.d + dl.d; flxi[IVX] = fl.mx + dl.mx; flxi[IVY] = fl.my + dl.my;
+    flxi[IVZ] = fl.mz + dl.mz; flxi[IEN] = fl.e + dl.e; flxi[IBY] = fl.by + dl.by;
+    flxi[IBZ] = fl.bz + dl.bz;
+  } else if (spd[2] >= 0.0) {
+    flxi[IDN] = fl.d + dl.d + dld.d; flxi[IVX] = fl.mx + dl.mx + dld.mx;
+    flxi[IVY] = fl.my + dl.my + dld.my; flxi[IVZ] = fl.mz + dl.mz + dld.mz;
+    flxi[IEN] = fl.e + dl.e + dld.e; flxi[IBY] = fl.by + dl.by + dld.by;
+    flxi[IBZ] = fl.bz + dl.bz + dld.bz;
+  } else if (spd[3] > 0.0) {
+    flxi[IDN] = fr.d + dr.d + drd.d; flxi[IVX] = fr.mx + dr.mx + drd.mx;
+    flxi[IVY] = fr.my + dr.my + drd.my; flxi[IVZ] = fr.mz + dr.mz + drd.mz;
+    flxi[IEN] = fr.e + dr.e + drd.e; flxi[IBY] = fr.by + dr.by + drd.by;
+    flxi[IBZ] = fr.bz + dr.bz + drd.bz;
+  } else {
+    flxi[IDN] = fr.d + dr.d; flxi[IVX] = fr.mx + dr.mx; flxi[IVY] = fr.my + dr.my;
+    flxi[IVZ] = fr.mz + dr.mz; flxi[IEN] = fr.e + dr.e; flxi[IBY] = fr.by + dr.by;
+    flxi[IBZ] = fr.bz + dr.bz;
+  }
+}
+
+/* Hydro::CalculateVelocityDifferences, 3D branch (calculate_velocity_differences.cpp:
+ * 23-68), at the face below cell (k,j,i) along dir: the normal velocity jump dvn and the
+ * smallest one-sided transverse velocity difference dvt of the two cells beside the face. */
+static void velocity_differences(const orc_ctx *c, const double *w, int dir, int k, int j, int i,
+                                 double *dvn, double *dvt) {
+  const int64_t nc = c->ncell;
+#define WV(n, kk, jj, ii) w[(n) * nc + CIDX(c, kk, jj, ii)]
+  if (dir == 0) {
+    *dvn = WV(IVX, k, j, i) - WV(IVX, k, j, i - 1);
+    double dvl = dmin(WV(IVY, k, j + 1, i - 1) - WV(IVY, k, j, i - 1), WV(IVY, k, j, i - 1) - WV(IVY, k, j - 1, i - 1));
+    double dvr = dmin(WV(IVY, k, j + 1, i) - WV(IVY, k, j, i), WV(IVY, k, j, i) - WV(IVY, k, j - 1, i));
+    double dwl = dmin(WV(IVZ, k + 1, j, i - 1) - WV(IVZ, k, j, i - 1), WV(IVZ, k, j, i - 1) - WV(IVZ, k - 1, j, i - 1));
+    double dwr = dmin(WV(IVZ, k + 1, j, i) - WV(IVZ, k, j, i), WV(IVZ, k, j, i) - WV(IVZ, k - 1, j, i));
+    *dvt = dmin(dmin(dvl, dvr), dmin(dwl, dwr));
+  } else if (dir == 1) {
+    *dvn = WV(IVY, k, j, i) - WV(IVY, k, j - 1, i);
+    double dvl = dmin(WV(IVZ, k + 1, j - 1, i) - WV(IVZ, k, j - 1, i), WV(IVZ, k, j - 1, i) - WV(IVZ, k - 1, j - 1, i));
+    double dvr = dmin(WV(IVZ, k + 1, j, i) - WV(IVZ, k, j, i), WV(IVZ, k, j, i) - WV(IVZ, k - 1, j, i));
+    double dwl = dmin(WV(IVX, k, j - 1, i + 1) - WV(IVX, k, j - 1, i), WV(IVX, k, j - 1, i) - WV(IVX, k, j - 1, i - 1));
+    double dwr = dmin(WV(IVX, k, j, i + 1) - WV(IVX, k, j, i), WV(IVX, k, j, i) - WV(IVX, k, j, i - 1));
+    *dvt = dmin(dmin(dvl, dvr), dmin(dwl, dwr));
+  } else {
+    *dvn = WV(IVZ, k, j, i) - WV(IVZ, k - 1, j, i);
+    double dvl = dmin(WV(IVX, k - 1, j, i + 1) - WV(IVX, k - 1, j, i), WV(IVX, k - 1, j, i) - WV(IVX, k - 1, j, i - 1));
+    double dvr = dmin(WV(IVX, k, j, i + 1) - WV(IVX, k, j, i), WV(IVX, k, j, i) - WV(IVX, k, j, i - 1));
+    double dwl = dmin(WV(IVY, k - 1, j + 1, i) - WV(IVY, k - 1, j, i), WV(IVY, k - 1, j, i) - WV(IVY, k - 1, j - 1, i));
+    double dwr = dmin(WV(IVY, k, j + 1, i) - WV(IVY, k, j, i), WV(IVY, k, j, i) - WV(IVY, k, j - 1, i));
+    *dvt = dmin(dmin(dvl, dvr), dmin(dwl, dwr));
+  }
+#undef WV
+}
+
 /* HLLE for adiabatic MHD, hydro/rsolvers/mhd/hlle_mhd.cpp:25-182 (one interface). */
 static void hlle(double gamma, int iso, double iso_cs, const double *wli, const double *wri,
                  double bxi, double *flxi) {
@@ -1337,7 +1554,11 @@ void orc_calculate_fluxes(orc_ctx *c, int order) {
             if (rs == 1) hlle(gamma, c->iso, c->d.iso_cs, wli, wri, bxi, flxi);
             else if (c->iso) hlld_iso(c->d.iso_cs, c->d.dfloor, wli, wri, bxi, flxi);
             else if (rs == 2) roe(gamma, wli, wri, bxi, flxi);
-            else hlld(gamma, wli, wri, bxi, flxi);
+            else if (rs == 3) {
+              double dvn, dvt;
+              velocity_differences(c, w, dir, k, j, i, &dvn, &dvt);
+              lhlld(gamma, wli, wri, bxi, dvn, dvt, flxi);
+            } else hlld(gamma, wli, wri, bxi, flxi);
             flx[IDN * nf + fid] = flxi[IDN];
             flx[ivx * nf + fid] = flxi[IVX];
             flx[ivy * nf + fid] = flxi[IVY];
@@ -2768,7 +2989,14 @@ void orc_riemann(int rs, double gamma, const double *wl, const double *wr, doubl
                  double *flx) {
   if (rs == 1) hlle(gamma, 0, 0.0, wl, wr, bx, flx);
   else if (rs == 2) roe(gamma, wl, wr, bx, flx);
+  else if (rs == 3) lhlld(gamma, wl, wr, bx, 0.0, 0.0, flx);
   else hlld(gamma, wl, wr, bx, flx);
+}
+
+/* LHLLD at one interface with the shock detector's velocity differences given */
+void orc_riemann_lhlld(double gamma, const double *wl, const double *wr, double bx, double dvn,
+                       double dvt, double *flx) {
+  lhlld(gamma, wl, wr, bx, dvn, dvt, flx);
 }
 
 /* isothermal solvers at one interface: rs 0 HLLD-iso, 1 HLLE (isothermal branch) */

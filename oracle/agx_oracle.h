@@ -33,7 +33,7 @@ typedef struct orc_desc {
   int32_t bnx1, bnx2, bnx3;     /* MeshBlock cells */
   int32_t nghost;
   int32_t xorder;               /* 1 donor cell, 2 PLM, 3 PPM */
-  int32_t rsolver;              /* 0 HLLD, 1 HLLE */
+  int32_t rsolver;              /* 0 HLLD, 1 HLLE, 2 Roe, 3 LHLLD */
   int32_t eos;                  /* 0 adiabatic */
   int32_t self_gravity;         /* 0 off, 1 multigrid */
   int32_t mg_mode;              /* 0 FMG (+iterate), 1 MGI */
@@ -97,6 +97,8 @@ double orc_max_divb(orc_ctx *c);
 /* unit-level probes */
 void orc_riemann(int rs, double gamma, const double *wl, const double *wr, double bx,
                  double *flx);
+void orc_riemann_lhlld(double gamma, const double *wl, const double *wr, double bx, double dvn,
+                       double dvt, double *flx);      /* lhlld.cpp:38-389 */
 void orc_riemann_iso(int rs, double cs, double dfloor, const double *wl, const double *wr,
                      double bx, double *flx);
 void orc_plm_pencil(int n, const double *q, const double *dxp, const double *dxm,

@@ -97,6 +97,7 @@ def _lib():
     lib.orc_riemann.argtypes = [ctypes.c_int, ctypes.c_double, D, D, ctypes.c_double, D]
     lib.orc_riemann_iso.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, D, D,
                                     ctypes.c_double, D]
+    lib.orc_riemann_lhlld.argtypes = [ctypes.c_double, D, D, ctypes.c_double, ctypes.c_double, ctypes.c_double, D]
     lib.orc_plm_pencil.argtypes = [ctypes.c_int, D, D, D, D, D]
     lib.orc_ppm_pencil.argtypes = [ctypes.c_int, D, D, D]
     return lib
@@ -252,6 +253,16 @@ def riemann(rs, gamma, wl, wr, bx):
     wr = np.ascontiguousarray(wr, dtype=np.float64)
     out = np.zeros(7)
     lib().orc_riemann(rs, gamma, _dp(wl), _dp(wr), bx, _dp(out))
+    return out
+
+
+def riemann_lhlld(gamma, wl, wr, bx, dvn=0.0, dvt=0.0):
+    """One interface flux [7] from the oracle's LHLLD with the shock detector's normal and
+    transverse velocity differences dvn, dvt."""
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    out = np.zeros(7)
+    lib().orc_riemann_lhlld(gamma, _dp(wl), _dp(wr), bx, dvn, dvt, _dp(out))
     return out
 
 

@@ -63,7 +63,7 @@ def test_initialize(agx, case):
 
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("order", [1, 2])
-@pytest.mark.parametrize("rsolver", [0, 1, 2])
+@pytest.mark.parametrize("rsolver", [0, 1, 2, 3])
 def test_fluxes_and_emf(agx, case, order, rsolver):
     o, m = build_case(agx, case, rsolver)
     o.initialize()
@@ -109,7 +109,7 @@ def test_stage_updates(agx, case):
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("rsolver", [0, 1, 2])
+@pytest.mark.parametrize("rsolver", [0, 1, 2, 3])
 def test_cycles(agx, case, rsolver):
     o, m = build_case(agx, case, rsolver)
     o.initialize()

@@ -34,7 +34,7 @@ def interior(a, ng):
     return a[:, :, ng:-ng, ng:-ng, ng:-ng]
 
 
-@pytest.mark.parametrize("rsolver,order", [(0, 2), (1, 2), (0, 3)])
+@pytest.mark.parametrize("rsolver,order", [(0, 2), (1, 2), (0, 3), (3, 2)])
 def test_mixed_fluxes(agx, rsolver, order):
     desc = make_desc(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, rsolver=rsolver,
                      xorder=order, nghost=3 if order == 3 else 2)

@@ -43,6 +43,9 @@ CASES = [
     ("plm_hlld", dict(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4), dict(mach=1.5)),
     ("plm_hlle", dict(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, rsolver=1), dict(mach=1.5)),
     ("plm_roe", dict(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, rsolver=2), dict(mach=1.5)),
+    ("plm_lhlld", dict(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, rsolver=3), dict(mach=1.5)),
+    ("ppm_lhlld", dict(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, xorder=3, nghost=3,
+                       rsolver=3), dict(mach=1.5)),
     ("dc_hlld", dict(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, xorder=1), dict(mach=1.5)),
     ("ppm_hlld", dict(nx=(32, 32, 32), bnx=(16, 16, 16), box=((0, 1),) * 3, cfl=0.4, xorder=3, nghost=3),
      dict(mach=1.5)),
