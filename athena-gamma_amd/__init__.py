@@ -46,6 +46,9 @@ EXPORTS = [
     "agx_ref_internal_to_cons", "agx_mg_set_source_mask_sphere", "agx_set_mg_coarse",
     "agx_history", "agx_history_sum", "agx_history_sum_noalloc",
     "agx_restart_write", "agx_restart_read", "agx_restart_write_host", "agx_restart_read_host",
+    "agx_mg_setup", "agx_mg_smooth", "agx_mg_fill_ghost", "agx_mg_fas", "agx_mg_restrict", "agx_mg_prolong",
+    "agx_mg_fmg_prolong", "agx_mg_solve_coarsest", "agx_mg_defect_norm", "agx_mg_retrieve", "agx_mg_coffset",
+    "agx_mg_levels",
 ]
 
 
@@ -58,14 +61,14 @@ class Desc(ctypes.Structure):
             "x1min", "x1max", "x2min", "x2max", "x3min", "x3max", "gamma", "iso_cs",
             "cfl", "dfloor", "pfloor", "four_pi_G", "mg_threshold")] + \
         [(n, ctypes.c_int32) for n in ("mg_bc", "mporder", "mp_auto_origin")] + \
-        [("mp_origin", ctypes.c_double * 3), ("mp_nodipole", ctypes.c_int32)]
+        [("mp_origin", ctypes.c_double * 3), ("mp_nodipole", ctypes.c_int32), ("char_proj", ctypes.c_int32)]
 
 
 def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)),
               nghost=2, xorder=2, rsolver=0, gamma=1.666666666666667, cfl=0.3,
               self_gravity=0, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=10,
               precision=0, eos=0, iso_cs=1.0, mg_bc=0, mporder=4, mp_auto_origin=1,
-              mp_origin=(0.0, 0.0, 0.0), mp_nodipole=0):
+              mp_origin=(0.0, 0.0, 0.0), mp_nodipole=0, char_proj=0):
     bnx = bnx or nx
     d = Desc()
     d.nx1, d.nx2, d.nx3 = nx
@@ -80,6 +83,7 @@ def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)
     d.mg_bc, d.mporder, d.mp_auto_origin = mg_bc, mporder, mp_auto_origin
     d.mp_origin[:] = mp_origin
     d.mp_nodipole = mp_nodipole
+    d.char_proj = char_proj
     return d
 
 
@@ -198,6 +202,14 @@ def lib():
     for fn in ("agx_history_sum", "agx_history_sum_noalloc"):
         getattr(L, fn).argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_int32, P, P, P]
+    for fn in ("agx_mg_setup", "agx_mg_solve_coarsest", "agx_mg_retrieve"):
+        getattr(L, fn).argtypes = [P]
+    for fn in ("agx_mg_fill_ghost", "agx_mg_fas", "agx_mg_restrict", "agx_mg_prolong", "agx_mg_fmg_prolong"):
+        getattr(L, fn).argtypes = [P, ctypes.c_int32]
+    L.agx_mg_smooth.argtypes = [P, ctypes.c_int32, ctypes.c_int32]
+    L.agx_mg_defect_norm.argtypes = [P, D]
+    L.agx_mg_coffset.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.agx_mg_levels.argtypes = [P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
     C = ctypes.c_char_p
     L.agx_restart_write.argtypes = [P, C, C]
     L.agx_restart_read.argtypes = [P, C]
@@ -388,6 +400,26 @@ class Mesh:
 
     def device_bytes(self):
         return self.L.agx_mesh_bytes(self.h)
+
+    def mg_step(self, name, *args):
+        """One per-level multigrid step: agx_mg_<name>(mesh, *args)."""
+        _check(getattr(self.L, "agx_mg_" + name)(self.h, *args), "agx_mg_" + name)
+
+    def mg_levels(self):
+        n = ctypes.c_int32()
+        dims = (ctypes.c_int32 * 96)()
+        _check(self.L.agx_mg_levels(self.h, ctypes.byref(n), dims, 32), "agx_mg_levels")
+        return [tuple(dims[3 * l:3 * l + 3]) for l in range(n.value)]
+
+    def mg_defect_norm(self):
+        v = ctypes.c_double()
+        _check(self.L.agx_mg_defect_norm(self.h, ctypes.byref(v)), "agx_mg_defect_norm")
+        return v.value
+
+    def mg_coffset(self, flip=False):
+        v = ctypes.c_int32()
+        _check(self.L.agx_mg_coffset(self.h, 1 if flip else 0, ctypes.byref(v)), "agx_mg_coffset")
+        return v.value
 
     def restart_write(self, path, params=None):
         """Write a restart file in the reference's format (agx_restart_write)."""

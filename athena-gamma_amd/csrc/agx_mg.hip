@@ -2290,7 +2290,217 @@ int mg_vcycles(agx_mesh *m, int n) {
   return 0;
 }
 
+// ---------------------------------------------------------------- per-level steps
+// The multigrid at the granularity of the reference's MultigridTaskList
+// (multigrid/mg_task_list.cpp:183-280, 287-530) and MGGravity / Multigrid methods, on
+// the FP64 hierarchy (every level of the whole mesh, finest = nlevels - 1), for callers
+// that drive their own cycles.  They use the same kernels as agx_mg_solve, so a V(1,1)
+// cycle composed of them equals agx_mg_vcycles(1) bit for bit.
+static int mg_level_check(agx_mesh *m, int level, int lo, const char *who) {
+  if (!m) { set_error("%s: null mesh", who); return -1; }
+  if (!m->mg_on) { set_error("%s: the mesh has no multigrid gravity", who); return -21; }
+  if (level < lo || level >= m->ntotal) {
+    set_error("%s: level %d not in [%d, %d]", who, level, lo, m->ntotal - 1);
+    return -2;
+  }
+  return 0;
+}
+
+static int launches_ok(const char *who) {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, who);
+}
+
 }  // namespace agx
+
+using namespace agx;
+
+// MGGravityDriver::Solve's set-up: Multigrid::LoadSource (src = 4 pi G rho, multigrid.cpp:
+// 194-220), the source mask and MultigridDriver::SetupMultigrid (mean subtraction on
+// periodic meshes; multipole coefficients and boundary values, multigrid_driver.cpp:
+// 430-460), then RestrictFMGSource (:447-456) down to the root level.
+extern "C" int agx_mg_setup(agx_mesh *m) {
+  int rc = mg_level_check(m, 0, 0, "agx_mg_setup");
+  if (rc) return rc;
+  const Geo &g = m->g;
+  const int L = m->ntotal;
+  double **A = m->arr.a;
+  if (m->mp_on) {
+    MGLevelB &F = m->levb[L - 1];
+    hipLaunchKernelGGL(k_mb_load, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, g,
+                       A[AGX_ARR_U], A[AGX_ARR_PHI], F.src, F.u, m->d.four_pi_G, m->mg_mode == 1 ? 1 : 0,
+                       src_mask(m));
+    mp_coefficients(m);
+    m->mp_solved = true;
+    for (int l = L - 1; l > 0; --l) {
+      MGLevelB &Fl = m->levb[l], &Cl = m->levb[l - 1];
+      hipLaunchKernelGGL(k_mb_restrict_src, dim3(grid1d((int64_t)Cl.n1 * Cl.n2 * Cl.n3, BS)), dim3(BS), 0,
+                         m->stream, Fl.src, Cl.src, Fl.n1, Fl.n2, Fl.n3);
+    }
+  } else {
+    MGLevel &F = m->lev[L - 1];
+    hipLaunchKernelGGL(k_mg_load, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, g, A[AGX_ARR_U],
+                       A[AGX_ARR_PHI], F.src, F.u, m->d.four_pi_G, m->mg_mode == 1 ? 1 : 0, src_mask(m));
+    subtract_average(m, 0);
+    for (int l = L - 1; l > 0; --l) {
+      MGLevel &Fl = m->lev[l], &Cl = m->lev[l - 1];
+      hipLaunchKernelGGL(k_mg_restrict_src<double>, dim3(grid1d(lcells(Cl), BS)), dim3(BS), 0, m->stream, Fl.src,
+                         Cl.src, Fl.n1, Fl.n2, Fl.n3);
+    }
+  }
+  m->current = L - 1;
+  m->fmglevel = L - 1;
+  return launches_ok("agx_mg_setup");
+}
+
+// MGGravity::Smooth (mg_gravity.cpp:220-273): one colour of the red-black sweep on a level
+// (multipole meshes: interior cells; fill the ghosts first, agx_mg_fill_ghost).  The colour
+// is combined with the driver's colour offset (agx_mg_coffset) as mg_gravity.cpp:227,243 do.
+extern "C" int agx_mg_smooth(agx_mesh *m, int32_t level, int32_t color) {
+  int rc = mg_level_check(m, level, 0, "agx_mg_smooth");
+  if (rc) return rc;
+  if (color != 0 && color != 1) { set_error("agx_mg_smooth: colour must be 0 or 1"); return -3; }
+  if (m->mp_on) smooth_b(m, level, color);
+  else smooth(m, m->lev, level, color);
+  return launches_ok("agx_mg_smooth");
+}
+
+// The ghost fill of a level before a stencil step (MGBoundaryValues, bvals_mg.cpp:321-370
+// with MGMultipoleInnerX1..OuterX3, mgbval_multipole.cpp:28-426).  Periodic levels have no
+// stored ghosts (neighbours are read through the periodic index wrap): nothing to do.
+extern "C" int agx_mg_fill_ghost(agx_mesh *m, int32_t level) {
+  int rc = mg_level_check(m, level, 0, "agx_mg_fill_ghost");
+  if (rc) return rc;
+  if (m->mp_on) bc_b(m, level);
+  return launches_ok("agx_mg_fill_ghost");
+}
+
+// Multigrid::StoreOldData + MGGravity::CalculateFASRHS (multigrid.cpp:648-653,
+// mg_gravity.cpp:324-351) on a level; multipole (FAS) meshes only.
+extern "C" int agx_mg_fas(agx_mesh *m, int32_t level) {
+  int rc = mg_level_check(m, level, 0, "agx_mg_fas");
+  if (rc) return rc;
+  if (!m->mp_on) { set_error("agx_mg_fas: FAS runs with multipole boundaries only (multigrid_driver.cpp:292-295)"); return -22; }
+  fas_b(m, level);
+  return launches_ok("agx_mg_fas");
+}
+
+// MGGravity::CalculateDefect + Multigrid::Restrict (mg_gravity.cpp:283-314,
+// multigrid.cpp:684-724): the defect of `level` restricted into the source of level-1
+// (with FAS also u), in one pass.
+extern "C" int agx_mg_restrict(agx_mesh *m, int32_t level) {
+  int rc = mg_level_check(m, level, 1, "agx_mg_restrict");
+  if (rc) return rc;
+  if (m->mp_on) {
+    MGLevelB &F = m->levb[level], &C = m->levb[level - 1];
+    hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0,
+                       m->stream, F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
+  } else {
+    defect_restrict(m, m->lev, level);
+  }
+  return launches_ok("agx_mg_restrict");
+}
+
+// Multigrid::ProlongateAndCorrect (multigrid.cpp:733-829): level-1 -> level.
+extern "C" int agx_mg_prolong(agx_mesh *m, int32_t level) {
+  int rc = mg_level_check(m, level, 1, "agx_mg_prolong");
+  if (rc) return rc;
+  if (m->mp_on) {
+    MGLevelB &F = m->levb[level], &C = m->levb[level - 1];
+    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream,
+                       C.u, C.uold, F.u, F.n1, F.n2, F.n3);
+  } else {
+    prolong(m, m->lev, level);
+  }
+  return launches_ok("agx_mg_prolong");
+}
+
+// Multigrid::FMGProlongate (multigrid.cpp:838-1048): level-1 -> level (multipole meshes:
+// fill level-1's ghosts first, as SolveFMGCycle's exchange does).
+extern "C" int agx_mg_fmg_prolong(agx_mesh *m, int32_t level) {
+  int rc = mg_level_check(m, level, 1, "agx_mg_fmg_prolong");
+  if (rc) return rc;
+  if (m->mp_on) {
+    MGLevelB &F = m->levb[level], &C = m->levb[level - 1];
+    hipLaunchKernelGGL(k_mb_fmg_prolong, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, C.u,
+                       F.u, F.n1, F.n2, F.n3);
+  } else {
+    fmg_prolong(m, m->lev, level);
+  }
+  return launches_ok("agx_mg_fmg_prolong");
+}
+
+// MultigridDriver::SolveCoarsestGrid (multigrid_driver.cpp:778-819) on level 0.
+extern "C" int agx_mg_solve_coarsest(agx_mesh *m) {
+  int rc = mg_level_check(m, 0, 0, "agx_mg_solve_coarsest");
+  if (rc) return rc;
+  if (m->mp_on) {
+    const Geo &g = m->g;
+    const int ni = std::max(g.nb1, std::max(g.nb2, g.nb3)) >> (m->nrootlevel - 1);
+    bc_b(m, 0);
+    fas_b(m, 0);
+    for (int it = 0; it < ni; ++it) {
+      smooth_b(m, 0, 0);
+      bc_b(m, 0);
+      smooth_b(m, 0, 1);
+      bc_b(m, 0);
+    }
+  } else {
+    solve_coarsest(m, m->lev);
+  }
+  return launches_ok("agx_mg_solve_coarsest");
+}
+
+// MultigridDriver::CalculateDefectNorm (L2, multigrid_driver.cpp:826-858) of the finest level.
+extern "C" int agx_mg_defect_norm(agx_mesh *m, double *norm) {
+  int rc = mg_level_check(m, 0, 0, "agx_mg_defect_norm");
+  if (rc) return rc;
+  if (!norm) { set_error("agx_mg_defect_norm: null output"); return -1; }
+  *norm = m->mp_on ? defect_norm_b(m) : defect_norm(m);
+  return launches_ok("agx_mg_defect_norm");
+}
+
+// Multigrid::RetrieveResult (multigrid.cpp:266-288): the finest u, with one ghost cell,
+// into Gravity::phi; on periodic meshes after SubtractAverage of u, as
+// MGGravityDriver::Solve does (multigrid_driver.cpp:755-771).
+extern "C" int agx_mg_retrieve(agx_mesh *m) {
+  int rc = mg_level_check(m, 0, 0, "agx_mg_retrieve");
+  if (rc) return rc;
+  const Geo &g = m->g;
+  const int64_t nret = (int64_t)(g.bx1 + 2) * (g.bx2 + 2) * (g.bx3 + 2) * g.nb;
+  if (m->mp_on) {
+    hipLaunchKernelGGL(k_mb_retrieve, dim3(grid1d(nret, BS)), dim3(BS), 0, m->stream, g, m->levb[m->ntotal - 1].u,
+                       m->arr.a[AGX_ARR_PHI]);
+  } else {
+    subtract_average(m, 1);
+    hipLaunchKernelGGL(k_mg_retrieve, dim3(grid1d(nret, BS)), dim3(BS), 0, m->stream, g, m->lev[m->ntotal - 1].u,
+                       m->arr.a[AGX_ARR_PHI]);
+  }
+  return launches_ok("agx_mg_retrieve");
+}
+
+// The red-black colour offset of the driver (MultigridDriver::coffset_, flipped at the
+// start of every V-cycle, multigrid_driver.cpp:686); flip != 0 flips it first.
+extern "C" int agx_mg_coffset(agx_mesh *m, int32_t flip, int32_t *value) {
+  int rc = mg_level_check(m, 0, 0, "agx_mg_coffset");
+  if (rc) return rc;
+  if (flip) m->coffset ^= 1;
+  if (value) *value = m->coffset;
+  return 0;
+}
+
+// Levels of the hierarchy: n = nlevels; dims[3 l + d] = cells of level l along d.
+extern "C" int agx_mg_levels(agx_mesh *m, int32_t *n, int32_t *dims, int32_t cap) {
+  int rc = mg_level_check(m, 0, 0, "agx_mg_levels");
+  if (rc) return rc;
+  if (n) *n = m->ntotal;
+  for (int l = 0; dims && l < m->ntotal && l < cap; ++l) {
+    const int n1 = m->mp_on ? m->levb[l].n1 : m->lev[l].n1, n2 = m->mp_on ? m->levb[l].n2 : m->lev[l].n2,
+              n3 = m->mp_on ? m->levb[l].n3 : m->lev[l].n3;
+    dims[3 * l] = n1; dims[3 * l + 1] = n2; dims[3 * l + 2] = n3;
+  }
+  return 0;
+}
 
 extern "C" int agx_mg_set_source_mask_sphere(agx_mesh *m, double xc, double yc, double zc, double radius) {
   if (!m) { set_error("agx_mg_set_source_mask_sphere: null mesh"); return -1; }

@@ -649,6 +649,126 @@ __device__ __forceinline__ void ppm_cell(const T *q, T &qminus, T &qplus) {
   qplus = qp;
 }
 
+// Eigensystem of the primitive-variable MHD Jacobian at one cell state, shared by
+// Reconstruction::LeftEigenmatrixDotVector and RightEigenmatrixDotVector
+// (reconstruct/characteristic.cpp:36-262, 280-500; Stone et al. 2008, ApJS 178, 137,
+// eqs. A10, A16, A17).  w: the cell's state in the solver's local frame (IVX along the
+// sweep, IBY/IBZ the transverse field); b1: its cell-centred normal field.  asq is
+// gamma P / rho, or iso_cs^2 for the isothermal EOS.
+template <typename T>
+struct CharEigen {
+  T d, id, sqrtd, isqrtd, cf, cs, a, asq, bet2, bet3, alpha_f, alpha_s, s;
+};
+
+template <typename T, bool ISO>
+__device__ __forceinline__ void char_eigen(T gamma, T iso_cs, T b1, const T *w, CharEigen<T> &e) {
+  const T iso_cs2 = iso_cs * iso_cs;
+  e.d = w[IDN];
+  e.id = T(1.0) / w[IDN];
+  e.sqrtd = xsqrt(w[IDN]);
+  e.isqrtd = T(1.0) / e.sqrtd;
+  T btsq = SQR(w[IBY]) + SQR(w[IBZ]);
+  T bxsq = b1 * b1;
+  T gamp = ISO ? iso_cs2 * w[IDN] : gamma * w[IPR];
+  T tdif = bxsq + btsq - gamp;   // fast and slow speeds, eq. A10
+  T cf2_cs2 = xsqrt(tdif * tdif + T(4.0) * gamp * btsq);
+  T cfsq = T(0.5) * (bxsq + btsq + gamp + cf2_cs2);
+  T cssq = gamp * bxsq / cfsq;
+  cfsq *= e.id;
+  e.cf = xsqrt(cfsq);
+  cssq *= e.id;
+  e.cs = xsqrt(cssq);
+  if (ISO) {
+    e.asq = iso_cs2;
+    e.a = iso_cs;
+  } else {
+    e.asq = gamp * e.id;
+    e.a = xsqrt(e.asq);
+  }
+  T bt = xsqrt(btsq);   // transverse unit vector, eq. A17 (zero when B_t = 0)
+  e.bet2 = T(0.0);
+  e.bet3 = T(0.0);
+  if (bt != T(0.0)) {
+    e.bet2 = w[IBY] / bt;
+    e.bet3 = w[IBZ] / bt;
+  }
+  const T asq = e.asq;   // normalisations, eq. A16
+  if ((cfsq - cssq) <= T(0.0)) { e.alpha_f = T(1.0); e.alpha_s = T(0.0); }
+  else if ((asq - cssq) <= T(0.0)) { e.alpha_f = T(0.0); e.alpha_s = T(1.0); }
+  else if ((cfsq - asq) <= T(0.0)) { e.alpha_f = T(1.0); e.alpha_s = T(0.0); }
+  else {
+    e.alpha_f = xsqrt((asq - cssq) / (cfsq - cssq));
+    e.alpha_s = xsqrt((cfsq - asq) / (cfsq - cssq));
+  }
+  e.s = sgn(b1);
+}
+
+// LeftEigenmatrixDotVector in the local frame: primitive vector v to its characteristic
+// fields ch (eq. A18, characteristic.cpp:85-124: fast-, Alfven-, slow-, entropy, slow+,
+// Alfven+, fast+; isothermal eq. A22, :166-197: the same without the entropy field).
+template <typename T, bool ISO>
+__device__ __forceinline__ void char_left(const CharEigen<T> &e, const T *v, T *ch) {
+  T vt = e.bet2 * v[IVY] + e.bet3 * v[IVZ];
+  T bt = e.bet2 * v[IBY] + e.bet3 * v[IBZ];
+  T alf_m = T(0.5) * (e.bet2 * (v[IBZ] * e.s * e.isqrtd + v[IVZ]) - e.bet3 * (v[IBY] * e.s * e.isqrtd + v[IVY]));
+  T alf_p = T(0.5) * (e.bet2 * (v[IBZ] * e.s * e.isqrtd - v[IVZ]) - e.bet3 * (v[IBY] * e.s * e.isqrtd - v[IVY]));
+  T af_prime = T(0.5) * e.alpha_f / (e.a * e.sqrtd);
+  T as_prime = T(0.5) * e.alpha_s / (e.a * e.sqrtd);
+  if (!ISO) {
+    T nf = T(0.5) / e.asq;
+    T qf = nf * e.cf * e.alpha_f * e.s;
+    T qs = nf * e.cs * e.alpha_s * e.s;
+    T pr = v[IPR] * e.id;
+    ch[0] = nf * e.alpha_f * (pr - e.cf * v[IVX]) + qs * vt + as_prime * bt;
+    ch[1] = alf_m;
+    ch[2] = nf * e.alpha_s * (pr - e.cs * v[IVX]) - qf * vt - af_prime * bt;
+    ch[3] = v[IDN] - v[IPR] / e.asq;
+    ch[4] = nf * e.alpha_s * (pr + e.cs * v[IVX]) + qf * vt - af_prime * bt;
+    ch[5] = alf_p;
+    ch[6] = nf * e.alpha_f * (pr + e.cf * v[IVX]) - qs * vt + as_prime * bt;
+  } else {
+    const T c2 = e.asq;
+    T qf = T(0.5) * (e.cf * e.alpha_f * e.s) / c2;
+    T qs = T(0.5) * (e.cs * e.alpha_s * e.s) / c2;
+    T dn = v[IDN] * e.id;
+    ch[0] = T(0.5) * e.alpha_f * (dn - e.cf * v[IVX] / c2) + qs * vt + as_prime * bt;
+    ch[1] = alf_m;
+    ch[2] = T(0.5) * e.alpha_s * (dn - e.cs * v[IVX] / c2) - qf * vt - af_prime * bt;
+    ch[3] = T(0.5) * e.alpha_s * (dn + e.cs * v[IVX] / c2) + qf * vt - af_prime * bt;
+    ch[4] = alf_p;
+    ch[5] = T(0.5) * e.alpha_f * (dn + e.cf * v[IVX] / c2) - qs * vt + as_prime * bt;
+    ch[6] = T(0.0);
+  }
+}
+
+// RightEigenmatrixDotVector in the local frame: characteristic fields ch back to the
+// primitive vector v (eq. A12, characteristic.cpp:330-352; isothermal :400-419, whose
+// state has no pressure: that slot is set to 0).
+template <typename T, bool ISO>
+__device__ __forceinline__ void char_right(const CharEigen<T> &e, const T *ch, T *v) {
+  constexpr int fm = 0, am = 1, sm = 2;   // wave pairs: fast, Alfven, slow
+  constexpr int sp = ISO ? 3 : 4, ap = ISO ? 4 : 5, fp = ISO ? 5 : 6;
+  T qf = e.cf * e.alpha_f * e.s;
+  T qs = e.cs * e.alpha_s * e.s;
+  T af = e.a * e.alpha_f * e.sqrtd;
+  T as = e.a * e.alpha_s * e.sqrtd;
+  T compr = e.alpha_f * (ch[fm] + ch[fp]) + e.alpha_s * (ch[sm] + ch[sp]);
+  T tr = qs * (ch[fm] - ch[fp]) + qf * (ch[sp] - ch[sm]);
+  T bc = as * (ch[fm] + ch[fp]) - af * (ch[sm] + ch[sp]);
+  v[IDN] = e.d * compr;
+  if (!ISO) v[IDN] += ch[3];
+  v[IVX] = e.cf * e.alpha_f * (ch[fp] - ch[fm]) + e.cs * e.alpha_s * (ch[sp] - ch[sm]);
+  v[IVY] = e.bet2 * tr + e.bet3 * (ch[ap] - ch[am]);
+  v[IVZ] = e.bet3 * tr + e.bet2 * (ch[am] - ch[ap]);
+  v[IPR] = ISO ? T(0.0) : e.d * e.asq * compr;
+  v[IBY] = e.bet2 * bc - e.bet3 * e.s * e.sqrtd * (ch[ap] + ch[am]);
+  v[IBZ] = e.bet3 * bc + e.bet2 * e.s * e.sqrtd * (ch[ap] + ch[am]);
+}
+
+// Reconstruction orders of face_flux beyond the reference's xorder 1..3: PLM and PPM
+// limited in characteristic variables (xorder "2c" and "3c", reconstruction.cpp:53-66).
+constexpr int ORDER_PLM_CHAR = 4, ORDER_PPM_CHAR = 5;
+
 // Roe solver for adiabatic MHD at one interface, hydro/rsolvers/mhd/roe_mhd.cpp:56-212
 // with RoeFlux (:245-498): Roe-averaged eigensystem, wave-strength dissipation, LLF
 // fallback for the hydro variables when an intermediate density is negative.
@@ -908,6 +1028,69 @@ __device__ __forceinline__ void face_flux(int dir, const T *__restrict__ wb, con
                                           T dfl, T pfl, T dx, T dt, T *f, T &ey, T &ez, T &wct) {
   const int d1 = (dir == 2) ? 0 : dir + 1, d2 = (dir == 0) ? 2 : dir - 1;   // (dir+1)%3, (dir+2)%3
   T wli[NWAVE], wri[NWAVE];
+  if constexpr (ORDER >= ORDER_PLM_CHAR) {
+    // limiting in characteristic variables (plm.cpp:59-63,96-99,109-136; ppm.cpp:
+    // 99-107,313-316): each cell's stencil is projected with the left eigenvectors of that
+    // cell's own state, each field limited on its own, and the result projected back
+    // with its right eigenvectors.  NP cells c-NP/2 .. c+NP/2-1 around the face.
+    constexpr int NP = (ORDER == ORDER_PPM_CHAR) ? 6 : 4, H = NP / 2;
+    constexpr int NWC = ISO ? 6 : 7;   // characteristic fields
+    T q[NP][NWAVE];
+    const T *src[NWAVE] = {wb, wb + (int64_t)(IVX + dir) * S, wb + (int64_t)(IVX + d1) * S,
+                           wb + (int64_t)(IVX + d2) * S, wb + (int64_t)IPR * S, bb + (int64_t)d1 * S,
+                           bb + (int64_t)d2 * S};
+#pragma unroll
+    for (int t = 0; t < NP; ++t)
+#pragma unroll
+      for (int n = 0; n < NWAVE; ++n) q[t][n] = (ISO && n == IPR) ? T(0.0) : src[n][id + (t - H) * st];
+    const T *bn = bb + (int64_t)dir * S + id;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {   // 0: cell c-1 (left state), 1: cell c (right)
+      const int cc = H - 1 + side;             // the cell's index in q
+      CharEigen<T> e;
+      char_eigen<T, ISO>(gamma, cs, bn[(side - 1) * st], q[cc], e);
+      T res[NWAVE], lim[NWAVE];
+      if constexpr (ORDER == ORDER_PPM_CHAR) {
+        T ch[5][NWAVE];
+#pragma unroll
+        for (int t = 0; t < 5; ++t) char_left<T, ISO>(e, q[cc - 2 + t], ch[t]);
+#pragma unroll
+        for (int n = 0; n < NWC; ++n) {
+          T c5[5] = {ch[0][n], ch[1][n], ch[2][n], ch[3][n], ch[4][n]}, qm, qp;
+          ppm_cell(c5, qm, qp);
+          lim[n] = side ? qm : qp;
+        }
+        if (NWC < NWAVE) lim[NWAVE - 1] = T(0.0);
+        char_right<T, ISO>(e, lim, side ? wri : wli);
+      } else {
+        T dl[NWAVE], dr[NWAVE], cl[NWAVE], cr[NWAVE];
+#pragma unroll
+        for (int n = 0; n < NWAVE; ++n) {
+          dl[n] = q[cc][n] - q[cc - 1][n];
+          dr[n] = q[cc + 1][n] - q[cc][n];
+        }
+        char_left<T, ISO>(e, dl, cl);
+        char_left<T, ISO>(e, dr, cr);
+#pragma unroll
+        for (int n = 0; n < NWC; ++n) {
+          T dw2 = cl[n] * cr[n];
+          lim[n] = T(2.0) * dw2 / (cl[n] + cr[n]);
+          if (dw2 <= T(0.0)) lim[n] = T(0.0);
+        }
+        if (NWC < NWAVE) lim[NWAVE - 1] = T(0.0);
+        char_right<T, ISO>(e, lim, res);
+        if (side == 0) {
+          const T f = T(pc[cidx - 1]);
+#pragma unroll
+          for (int n = 0; n < NWAVE; ++n) wli[n] = q[cc][n] + f * res[n];
+        } else {
+          const T f = T(pc[PL + cidx]);
+#pragma unroll
+          for (int n = 0; n < NWAVE; ++n) wri[n] = q[cc][n] - f * res[n];
+        }
+      }
+    }
+  } else {
 #pragma unroll
   for (int n = 0; n < NWAVE; ++n) {
     if (ISO && n == IPR) {   // no pressure in the isothermal state vector
@@ -951,9 +1134,11 @@ __device__ __forceinline__ void face_flux(int dir, const T *__restrict__ wb, con
       wri[n] = q0 - T(pc[PL + cidx]) * dwm;
     }
   }
-  if (ORDER == 3) {
-    // ApplyPrimitiveFloors on both reconstructed states (ppm.cpp:333-337); the
-    // isothermal EOS floors the density only (eos/isothermal_mhd.cpp:135-140)
+  }
+  if (ORDER >= 3) {
+    // ApplyPrimitiveFloors on both reconstructed states (ppm.cpp:333-337, and after a
+    // characteristic PLM, plm.cpp:128-136); the isothermal EOS floors the density only
+    // (eos/isothermal_mhd.cpp:135-140)
     wli[IDN] = (wli[IDN] > dfl) ? wli[IDN] : dfl;
     wri[IDN] = (wri[IDN] > dfl) ? wri[IDN] : dfl;
     if (!ISO) {
@@ -2191,6 +2376,10 @@ int check_desc(const agx_mesh_desc *d) {
   if (d->xorder == 3 && d->nghost < 3) {
     set_error("xorder=3 (PPM) needs nghost >= 3 (reconstruction.cpp:90-100)"); return -13;
   }
+  if (d->char_proj != 0 && d->char_proj != 1) { set_error("char_proj must be 0 or 1"); return -6; }
+  if (d->char_proj && d->xorder == 1) {   // reconstruction.cpp:49-71 knows no "1c"
+    set_error("characteristic projection needs xorder 2 or 3"); return -6;
+  }
   if (d->nghost > d->bnx1 || d->nghost > d->bnx2 || d->nghost > d->bnx3) {
     set_error("nghost larger than a MeshBlock"); return -7;
   }
@@ -2245,6 +2434,8 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   m->d = *desc;
   // CFL clamp of the VL2 integrator in 3D, time_integrator.cpp:212-215,885-896
   if (m->d.cfl > 0.5) m->d.cfl = 0.5;
+  // characteristic projection is built into the split forms only: 32 x 8 for its stage
+  if (m->d.char_proj && m->d.xorder > 1) m->tile = 1;
   Geo &g = m->g;
   g.nb1 = desc->nx1 / desc->bnx1;
   g.nb2 = desc->nx2 / desc->bnx2;
@@ -2684,12 +2875,18 @@ static int calculate_fluxes(agx_mesh *m, int order) {
   else if (rs == AGX_RSOLVER_ROE) { FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_ROE, B, F, EY, EZ, WX); } \
   else if (rs == AGX_RSOLVER_LHLLD) { FLUX_LAUNCH_E(DIR, ORD, AGX_RSOLVER_LHLLD, B, F, EY, EZ, WX, false); } \
   else { FLUX_LAUNCH(DIR, ORD, AGX_RSOLVER_HLLD, B, F, EY, EZ, WX); }
+  // xorder "2c"/"3c": PLM/PPM limited in characteristic variables
+  const bool chp = m->d.char_proj && order > 1;
 #define FLUX_DIR(DIR, B, F, EY, EZ, WX)                                                         \
   timer_begin(m, AGX_K_FLUX_X1 + DIR);                                                        \
   if (order == 1) {                                                                           \
     FLUX_ORD(DIR, 1, B, F, EY, EZ, WX)                                                        \
+  } else if (order == 2 && chp) {                                                             \
+    FLUX_ORD(DIR, ORDER_PLM_CHAR, B, F, EY, EZ, WX)                                           \
   } else if (order == 2) {                                                                    \
     FLUX_ORD(DIR, 2, B, F, EY, EZ, WX)                                                        \
+  } else if (chp) {                                                                           \
+    FLUX_ORD(DIR, ORDER_PPM_CHAR, B, F, EY, EZ, WX)                                           \
   } else {                                                                                    \
     FLUX_ORD(DIR, 3, B, F, EY, EZ, WX)                                                        \
   }                                                                                           \
@@ -3036,6 +3233,9 @@ static int stage_fused(agx_mesh *m, int stage, bool want_dt) {
   const int tile = (stage == 1) ? m->tile1 : m->tile;
   if (tile >= TILE_VL2) {
     if (order == 3 && m->g.ng < 3) { set_error("PPM needs nghost >= 3 (reconstruction.cpp:90-100)"); return -3; }
+    if (order > 1 && m->d.char_proj) {
+      set_error("characteristic projection runs in the split stage forms only"); return -4;
+    }
     timer_begin(m, AGX_K_INTEGRATE);
     if (m->mixed) launch_vl2<float>(m, tile, stage, order, wfac, want_dt ? m->dtpart : nullptr);
     else launch_vl2<double>(m, tile, stage, order, wfac, want_dt ? m->dtpart : nullptr);
@@ -3231,7 +3431,9 @@ int agx_autotune(agx_mesh *m, int32_t *choice) {
   for (int order : {1, m->d.xorder}) {
     float best = 0.f;
     int best_t = 0;
-    for (int t = 0; t < NTILE; ++t) {
+    // characteristic projection has the split forms only
+    const int ntile = (order > 1 && m->d.char_proj) ? TILE_VL2 : NTILE;
+    for (int t = 0; t < ntile; ++t) {
       float ms = 0.f;
       for (int rep = 0; rep < 4; ++rep) {
         if (rep == 1) AGX_HIP(hipEventRecord(e0, m->stream));
@@ -3272,6 +3474,10 @@ int agx_set_mg_fuse(agx_mesh *m, int32_t mode) {
 int agx_set_tile(agx_mesh *m, int32_t tile) {
   if (!m) { set_error("null mesh"); return -1; }
   if (tile < 0 || tile >= NTILE) { set_error("stage form %d not in [0, %d)", tile, NTILE); return -2; }
+  if (tile >= TILE_VL2 && m->d.char_proj && m->d.xorder > 1) {
+    set_error("characteristic projection runs in the split stage forms (0..%d) only", TILE_VL2 - 1);
+    return -3;
+  }
   m->tile = m->tile1 = tile;
   graph_reset(m);
   return 0;

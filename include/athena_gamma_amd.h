@@ -68,6 +68,10 @@ typedef struct agx_mesh_desc {
   int32_t mp_nodipole;          /* <gravity>/nodipole: 1 = leave out the dipole moments
                                    (multigrid.cpp:1072, 1200); needs mp_auto_origin = 0
                                    (mg_gravity.cpp:106-113) */
+  int32_t char_proj;            /* <time>/xorder = "2c" or "3c": 1 = limit PLM slopes or PPM
+                                   values in the characteristic variables of the cell
+                                   (reconstruction.cpp:53-66; characteristic.cpp); needs
+                                   xorder 2 or 3 */
 } agx_mesh_desc;
 
 /* Riemann solvers (configure --flux): hlld.cpp, hlle_mhd.cpp, roe_mhd.cpp, and lhlld.cpp
@@ -247,6 +251,36 @@ int agx_restart_write_host(const char *path, const agx_mesh_desc *d, const char 
                            const double *b3f);
 int agx_restart_read_host(const char *path, const agx_mesh_desc *d, double *time, double *dt, int32_t *ncycle,
                           double *u, double *b1f, double *b2f, double *b3f);
+
+/* ---- multigrid at task granularity (MultigridTaskList, multigrid/mg_task_list.cpp:
+ * 183-280, 287-530; MGGravity::Smooth/CalculateDefect/CalculateFASRHS, mg_gravity.hpp:
+ * 34-40; Multigrid::Restrict/ProlongateAndCorrect/FMGProlongate, multigrid.hpp:120-127)
+ * for callers that drive their own cycles.  Each step runs on one level of the FP64
+ * hierarchy of the whole mesh (level 0 = root, agx_mg_levels gives the count and sizes)
+ * with the kernels of agx_mg_solve, so a V(1,1) composed of them equals agx_mg_vcycles(1)
+ * bit for bit.  Order of a V(1,1) from the finest level F, after agx_mg_coffset(m, 1, 0):
+ *   periodic:  l = F..1: smooth(l,0), smooth(l,1), restrict(l); solve_coarsest;
+ *              l = 1..F: prolong(l), smooth(l,0), smooth(l,1)
+ *   multipole: l = F..1: fill_ghost(l), [fas(l) if l < F], smooth(l,0), fill_ghost(l),
+ *              smooth(l,1), fill_ghost(l), restrict(l); solve_coarsest;
+ *              l = 1..F: fill_ghost(l-1), prolong(l), fill_ghost(l), smooth(l,0),
+ *              fill_ghost(l), smooth(l,1), fill_ghost(l)
+ * agx_mg_setup loads the source (LoadSource, the source mask, SetupMultigrid: mean
+ * subtraction or multipole coefficients) and restricts it to every level (RestrictFMGSource);
+ * agx_mg_retrieve writes phi (RetrieveResult).  Periodic levels keep no ghosts (periodic
+ * index wrap): agx_mg_fill_ghost does nothing there.  agx_mg_fas: multipole (FAS) only. */
+int agx_mg_setup(agx_mesh *m);
+int agx_mg_smooth(agx_mesh *m, int32_t level, int32_t color);
+int agx_mg_fill_ghost(agx_mesh *m, int32_t level);
+int agx_mg_fas(agx_mesh *m, int32_t level);
+int agx_mg_restrict(agx_mesh *m, int32_t level);          /* level -> level - 1 */
+int agx_mg_prolong(agx_mesh *m, int32_t level);           /* level - 1 -> level */
+int agx_mg_fmg_prolong(agx_mesh *m, int32_t level);       /* level - 1 -> level */
+int agx_mg_solve_coarsest(agx_mesh *m);
+int agx_mg_defect_norm(agx_mesh *m, double *norm);       /* finest level, L2 */
+int agx_mg_retrieve(agx_mesh *m);
+int agx_mg_coffset(agx_mesh *m, int32_t flip, int32_t *value);
+int agx_mg_levels(agx_mesh *m, int32_t *nlevels, int32_t *dims, int32_t cap);
 
 /* ---- tuning (no counterpart in the reference: launch shapes of this library).
  * agx_autotune times each form of a VL2 stage on the current state, for the first

@@ -1436,6 +1436,117 @@ static void ppm_cell(const double *q, double *qminus_out, double *qplus_out) {
   *qplus_out = qplus;
 }
 
+/* The eigensystem of the primitive-variable MHD Jacobian at one cell state, shared by
+ * Reconstruction::LeftEigenmatrixDotVector and RightEigenmatrixDotVector (characteristic.cpp:
+ * 36-262, 280-500; Stone et al. 2008, ApJS 178, 137, eqs. A10, A16, A17).  w is the cell's
+ * primitive state in the 7-slot order with (IBY, IBZ) its transverse field, b1 its
+ * cell-centred normal field; asq = gamma P / rho (adiabatic) or iso_cs^2 (isothermal). */
+typedef struct {
+  double d, id, sqrtd, isqrtd, cf, cs, a, asq, bet2, bet3, alpha_f, alpha_s, s;
+} orc_eigen;
+
+static void char_eigen(const orc_ctx *c, double b1, const double *w, orc_eigen *e) {
+  double iso_cs2 = SQR(c->d.iso_cs);
+  e->d = w[IDN];
+  e->id = 1.0 / w[IDN];
+  e->sqrtd = sqrt(w[IDN]);
+  e->isqrtd = 1.0 / e->sqrtd;
+  double btsq = SQR(w[IBY]) + SQR(w[IBZ]);
+  double bxsq = b1 * b1;
+  double gamp = c->iso ? iso_cs2 * w[IDN] : c->d.gamma * w[IPR];
+  double tdif = bxsq + btsq - gamp;                        /* eq. A10 */
+  double cf2_cs2 = sqrt(tdif * tdif + 4.0 * gamp * btsq);
+  double cfsq = 0.5 * (bxsq + btsq + gamp + cf2_cs2);
+  double cssq = gamp * bxsq / cfsq;
+  cfsq *= e->id;
+  e->cf = sqrt(cfsq);
+  cssq *= e->id;
+  e->cs = sqrt(cssq);
+  if (c->iso) { e->asq = iso_cs2; e->a = c->d.iso_cs; }
+  else { e->asq = gamp * e->id; e->a = sqrt(e->asq); }
+  double bt = sqrt(btsq);                                  /* eq. A17 */
+  e->bet2 = 0.0;
+  e->bet3 = 0.0;
+  if (bt != 0.0) { e->bet2 = w[IBY] / bt; e->bet3 = w[IBZ] / bt; }
+  double asq = e->asq;                                     /* eq. A16 */
+  if ((cfsq - cssq) <= 0.0) { e->alpha_f = 1.0; e->alpha_s = 0.0; }
+  else if ((asq - cssq) <= 0.0) { e->alpha_f = 0.0; e->alpha_s = 1.0; }
+  else if ((cfsq - asq) <= 0.0) { e->alpha_f = 1.0; e->alpha_s = 0.0; }
+  else {
+    e->alpha_f = sqrt((asq - cssq) / (cfsq - cssq));
+    e->alpha_s = sqrt((cfsq - asq) / (cfsq - cssq));
+  }
+  e->s = sgn(b1);
+}
+
+/* Number of characteristic fields: 7 adiabatic, 6 isothermal (no entropy wave). */
+static inline int char_nwave(const orc_ctx *c) { return c->iso ? 6 : 7; }
+
+/* LeftEigenmatrixDotVector: the primitive vector v (7 slots, mesh-frame velocities, ivx the
+ * slot of the sweep's velocity) to its characteristic fields ch: eq. A18 (adiabatic,
+ * characteristic.cpp:85-124: fast-, Alfven-, slow-, entropy, slow+, Alfven+, fast+) or
+ * eq. A22 (isothermal, :166-197: the same without the entropy wave). */
+static void char_left(const orc_ctx *c, const orc_eigen *e, int ivx, const double *v, double *ch) {
+  int ivy = IVX + ((ivx - IVX) + 1) % 3, ivz = IVX + ((ivx - IVX) + 2) % 3;
+  double vt = e->bet2 * v[ivy] + e->bet3 * v[ivz];
+  double bt = e->bet2 * v[IBY] + e->bet3 * v[IBZ];
+  double alf_m = 0.5 * (e->bet2 * (v[IBZ] * e->s * e->isqrtd + v[ivz]) -
+                        e->bet3 * (v[IBY] * e->s * e->isqrtd + v[ivy]));
+  double alf_p = 0.5 * (e->bet2 * (v[IBZ] * e->s * e->isqrtd - v[ivz]) -
+                        e->bet3 * (v[IBY] * e->s * e->isqrtd - v[ivy]));
+  double af_prime = 0.5 * e->alpha_f / (e->a * e->sqrtd);
+  double as_prime = 0.5 * e->alpha_s / (e->a * e->sqrtd);
+  if (!c->iso) {
+    double nf = 0.5 / e->asq;
+    double qf = nf * e->cf * e->alpha_f * e->s;
+    double qs = nf * e->cs * e->alpha_s * e->s;
+    double pr = v[IPR] * e->id;
+    ch[0] = nf * e->alpha_f * (pr - e->cf * v[ivx]) + qs * vt + as_prime * bt;
+    ch[1] = alf_m;
+    ch[2] = nf * e->alpha_s * (pr - e->cs * v[ivx]) - qf * vt - af_prime * bt;
+    ch[3] = v[IDN] - v[IPR] / e->asq;
+    ch[4] = nf * e->alpha_s * (pr + e->cs * v[ivx]) + qf * vt - af_prime * bt;
+    ch[5] = alf_p;
+    ch[6] = nf * e->alpha_f * (pr + e->cf * v[ivx]) - qs * vt + as_prime * bt;
+  } else {
+    double c2 = e->asq;
+    double qf = 0.5 * (e->cf * e->alpha_f * e->s) / c2;
+    double qs = 0.5 * (e->cs * e->alpha_s * e->s) / c2;
+    double dn = v[IDN] * e->id;
+    ch[0] = 0.5 * e->alpha_f * (dn - e->cf * v[ivx] / c2) + qs * vt + as_prime * bt;
+    ch[1] = alf_m;
+    ch[2] = 0.5 * e->alpha_s * (dn - e->cs * v[ivx] / c2) - qf * vt - af_prime * bt;
+    ch[3] = 0.5 * e->alpha_s * (dn + e->cs * v[ivx] / c2) + qf * vt - af_prime * bt;
+    ch[4] = alf_p;
+    ch[5] = 0.5 * e->alpha_f * (dn + e->cf * v[ivx] / c2) - qs * vt + as_prime * bt;
+  }
+}
+
+/* RightEigenmatrixDotVector: characteristic fields ch back to the primitive vector v
+ * (eq. A12; characteristic.cpp:330-352 adiabatic, :400-419 isothermal, whose state has no
+ * pressure: that slot is set to 0 here). */
+static void char_right(const orc_ctx *c, const orc_eigen *e, int ivx, const double *ch, double *v) {
+  int ivy = IVX + ((ivx - IVX) + 1) % 3, ivz = IVX + ((ivx - IVX) + 2) % 3;
+  /* slots of the wave pairs: fast (fm, fp), Alfven (am, ap), slow (sm, sp) */
+  const int fm = 0, am = 1, sm = 2;
+  const int sp = c->iso ? 3 : 4, ap = c->iso ? 4 : 5, fp = c->iso ? 5 : 6;
+  double qf = e->cf * e->alpha_f * e->s;
+  double qs = e->cs * e->alpha_s * e->s;
+  double af = e->a * e->alpha_f * e->sqrtd;
+  double as = e->a * e->alpha_s * e->sqrtd;
+  double compr = e->alpha_f * (ch[fm] + ch[fp]) + e->alpha_s * (ch[sm] + ch[sp]);
+  double tr = qs * (ch[fm] - ch[fp]) + qf * (ch[sp] - ch[sm]);
+  double bc = as * (ch[fm] + ch[fp]) - af * (ch[sm] + ch[sp]);
+  v[IDN] = e->d * compr;
+  if (!c->iso) v[IDN] += ch[3];
+  v[ivx] = e->cf * e->alpha_f * (ch[fp] - ch[fm]) + e->cs * e->alpha_s * (ch[sp] - ch[sm]);
+  v[ivy] = e->bet2 * tr + e->bet3 * (ch[ap] - ch[am]);
+  v[ivz] = e->bet3 * tr + e->bet2 * (ch[am] - ch[ap]);
+  v[IPR] = c->iso ? 0.0 : e->d * e->asq * compr;
+  v[IBY] = e->bet2 * bc - e->bet3 * e->s * e->sqrtd * (ch[ap] + ch[am]);
+  v[IBZ] = e->bet3 * bc + e->bet2 * e->s * e->sqrtd * (ch[ap] + ch[am]);
+}
+
 static inline void recon_cell(const orc_ctx *c, const double *w, const double *bcc,
                               int dir, int order, int g, int k, int j, int i,
                               double *wm, double *wp) {
@@ -1448,12 +1559,42 @@ static inline void recon_cell(const orc_ctx *c, const double *w, const double *b
    * (IBY,IBZ) = the two transverse bcc components (plm.cpp:46-56,160-170,273-283) */
   int tb1 = (dir == 0) ? IB2 : (dir == 1 ? IB3 : IB1);
   int tb2 = (dir == 0) ? IB3 : (dir == 1 ? IB1 : IB2);
+  const int ch = c->d.char_proj && order > 1;
+  orc_eigen eg;
+  if (ch) {
+    double wc[NWAVE];
+    for (int n = 0; n < NHYDRO; ++n) wc[n] = w[n * nc + id];
+    wc[IBY] = bcc[tb1 * nc + id];
+    wc[IBZ] = bcc[tb2 * nc + id];
+    char_eigen(c, bcc[dir * nc + id], wc, &eg);
+  }
   if (order == 3) {
-    for (int n = 0; n < NWAVE; ++n) {
-      const double *a = (n < NHYDRO) ? w + n * nc : bcc + (n == IBY ? tb1 : tb2) * nc;
-      double q5[5];
-      for (int t = 0; t < 5; ++t) q5[t] = a[id + (t - 2) * st];
-      ppm_cell(q5, &wm[n], &wp[n]);
+    if (ch) {
+      /* ppm.cpp:99-107,313-316: the five cells' primitive vectors projected with the
+       * centre cell's left eigenvectors, each characteristic field limited on its own,
+       * the two interface values projected back with its right eigenvectors */
+      double qc[5][NWAVE], v[NWAVE], cm[NWAVE], cp[NWAVE];
+      for (int t = 0; t < 5; ++t) {
+        int64_t o = id + (t - 2) * st;
+        for (int n = 0; n < NHYDRO; ++n) v[n] = w[n * nc + o];
+        v[IBY] = bcc[tb1 * nc + o];
+        v[IBZ] = bcc[tb2 * nc + o];
+        char_left(c, &eg, IVX + dir, v, qc[t]);
+      }
+      for (int n = 0; n < char_nwave(c); ++n) {
+        double q5[5];
+        for (int t = 0; t < 5; ++t) q5[t] = qc[t][n];
+        ppm_cell(q5, &cm[n], &cp[n]);
+      }
+      char_right(c, &eg, IVX + dir, cm, wm);
+      char_right(c, &eg, IVX + dir, cp, wp);
+    } else {
+      for (int n = 0; n < NWAVE; ++n) {
+        const double *a = (n < NHYDRO) ? w + n * nc : bcc + (n == IBY ? tb1 : tb2) * nc;
+        double q5[5];
+        for (int t = 0; t < 5; ++t) q5[t] = a[id + (t - 2) * st];
+        ppm_cell(q5, &wm[n], &wp[n]);
+      }
     }
     /* ApplyPrimitiveFloors on both reconstructed states (ppm.cpp:333-337,
      * eos/adiabatic_mhd.cpp:162-172) */
@@ -1492,6 +1633,33 @@ static inline void recon_cell(const orc_ctx *c, const double *w, const double *b
                  *dx = c->dx3f + g * (c->n3 + 1);
     dxp = (xf[k + 1] - xv[k]) / dx[k];
     dxm = (xv[k] - xf[k]) / dx[k];
+  }
+  if (ch) {
+    /* plm.cpp:59-63,96-99,109-136: the two one-sided differences projected with the
+     * cell's left eigenvectors, the van Leer slope of each characteristic field, the
+     * slope projected back, then the EOS floors on both states */
+    double dl[NWAVE], dr[NWAVE], cl[NWAVE], cr[NWAVE], cm[NWAVE], dm[NWAVE];
+    for (int n = 0; n < NWAVE; ++n) { dl[n] = q[n][1] - q[n][0]; dr[n] = q[n][2] - q[n][1]; }
+    char_left(c, &eg, IVX + dir, dl, cl);
+    char_left(c, &eg, IVX + dir, dr, cr);
+    for (int n = 0; n < char_nwave(c); ++n) {
+      double dw2 = cl[n] * cr[n];
+      cm[n] = 2.0 * dw2 / (cl[n] + cr[n]);
+      if (dw2 <= 0.0) cm[n] = 0.0;
+    }
+    char_right(c, &eg, IVX + dir, cm, dm);
+    for (int n = 0; n < NWAVE; ++n) {
+      wp[n] = q[n][1] + dxp * dm[n];
+      wm[n] = q[n][1] - dxm * dm[n];
+    }
+    double dfl = c->d.dfloor, pfl = c->d.pfloor;
+    wm[IDN] = (wm[IDN] > dfl) ? wm[IDN] : dfl;
+    wp[IDN] = (wp[IDN] > dfl) ? wp[IDN] : dfl;
+    if (!c->iso) {
+      wm[IPR] = (wm[IPR] > pfl) ? wm[IPR] : pfl;
+      wp[IPR] = (wp[IPR] > pfl) ? wp[IPR] : pfl;
+    }
+    return;
   }
   for (int n = 0; n < NWAVE; ++n) {
     double dwl = (q[n][1] - q[n][0]);
@@ -3019,6 +3187,43 @@ void orc_plm_pencil(int n, const double *q, const double *dxp, const double *dxm
     wl[i + 1] = q[i] + dxp[i] * dwm;
     wr[i] = q[i] - dxm[i] * dwm;
   }
+}
+
+/* The eigenvector projections at one state (w7: IDN, IVX, IVY, IVZ, IPR, IBY, IBZ in the
+ * mesh frame with (IBY, IBZ) the field transverse to dir; b1 the normal field): L[r][n] =
+ * characteristic field r of the unit primitive vector n, R[n][r] = primitive slot n of
+ * the unit characteristic vector r (7 x 7, row-major; isothermal fills 6 fields). */
+void orc_char_matrices(int iso, double gamma, double iso_cs, int dir, const double *w7, double b1,
+                       double *L, double *R) {
+  orc_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.iso = iso;
+  c.d.gamma = gamma;
+  c.d.iso_cs = iso_cs;
+  orc_eigen e;
+  char_eigen(&c, b1, w7, &e);
+  for (int n = 0; n < NWAVE * NWAVE; ++n) L[n] = R[n] = 0.0;
+  for (int n = 0; n < NWAVE; ++n) {
+    double v[NWAVE] = {0}, ch[NWAVE] = {0};
+    v[n] = 1.0;
+    char_left(&c, &e, IVX + dir, v, ch);
+    for (int r = 0; r < NWAVE; ++r) L[r * NWAVE + n] = ch[r];
+  }
+  for (int r = 0; r < char_nwave(&c); ++r) {
+    double ch[NWAVE] = {0}, v[NWAVE];
+    ch[r] = 1.0;
+    char_right(&c, &e, IVX + dir, ch, v);
+    for (int n = 0; n < NWAVE; ++n) R[n * NWAVE + r] = v[n];
+  }
+}
+
+/* The reconstructed states of one cell of block g along dir (the 7-slot mesh-frame order):
+ * wm at its lower face, wp at its upper face, as the flux calculation uses them. */
+void orc_recon_cell(const orc_ctx *c, int dir, int order, int g, int k, int j, int i, double *wm,
+                    double *wp) {
+  const double *w = c->arr[ARR_W] + g * c->arrsize[ARR_W];
+  const double *bcc = c->arr[ARR_BCC] + g * c->arrsize[ARR_BCC];
+  recon_cell(c, w, bcc, dir, order, g, k, j, i, wm, wp);
 }
 
 /* PPM on a 1-D pencil: wl[i+1] (upper face of cell i), wr[i], i = 2..n-3. */

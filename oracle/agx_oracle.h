@@ -47,6 +47,7 @@ typedef struct orc_desc {
   int32_t mp_auto_origin;       /* 1: expansion about the centre of mass */
   double mp_origin[3];          /* expansion origin when mp_auto_origin = 0 */
   int32_t mp_nodipole;          /* nodipole: dipole moments left out */
+  int32_t char_proj;            /* xorder "2c"/"3c": characteristic projection */
 } orc_desc;
 
 typedef struct orc_ctx orc_ctx;
@@ -104,6 +105,10 @@ void orc_riemann_iso(int rs, double cs, double dfloor, const double *wl, const d
 void orc_plm_pencil(int n, const double *q, const double *dxp, const double *dxm,
                     double *wl, double *wr);
 void orc_ppm_pencil(int n, const double *q, double *wl, double *wr);
+void orc_char_matrices(int iso, double gamma, double iso_cs, int dir, const double *w7, double b1,
+                       double *L, double *R);
+void orc_recon_cell(const orc_ctx *c, int dir, int order, int g, int k, int j, int i, double *wm,
+                    double *wp);
 void orc_conserved_totals(orc_ctx *c, double *out8);
 /* history.cpp:135-200: built-in volume sums; returns the count written to out */
 int orc_history(orc_ctx *c, double *out);

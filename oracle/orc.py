@@ -26,7 +26,7 @@ class Desc(ctypes.Structure):
             "x1min", "x1max", "x2min", "x2max", "x3min", "x3max", "gamma", "iso_cs",
             "cfl", "dfloor", "pfloor", "four_pi_G", "mg_threshold")] + \
         [(n, ctypes.c_int32) for n in ("mg_bc", "mporder", "mp_auto_origin")] + \
-        [("mp_origin", ctypes.c_double * 3), ("mp_nodipole", ctypes.c_int32)]
+        [("mp_origin", ctypes.c_double * 3), ("mp_nodipole", ctypes.c_int32), ("char_proj", ctypes.c_int32)]
 
 
 FLT_MIN = np.finfo(np.float32).tiny
@@ -36,7 +36,7 @@ def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)
               nghost=2, xorder=2, rsolver=0, gamma=1.666666666666667, cfl=0.3,
               self_gravity=0, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=10,
               precision=0, eos=0, iso_cs=1.0, mg_bc=0, mporder=4, mp_auto_origin=1,
-              mp_origin=(0.0, 0.0, 0.0), mp_nodipole=0):
+              mp_origin=(0.0, 0.0, 0.0), mp_nodipole=0, char_proj=0):
     bnx = bnx or nx
     d = Desc()
     d.nx1, d.nx2, d.nx3 = nx
@@ -51,6 +51,7 @@ def make_desc(nx=(64, 32, 32), bnx=None, box=((0.0, 3.0), (0.0, 1.5), (0.0, 1.5)
     d.mg_bc, d.mporder, d.mp_auto_origin = mg_bc, mporder, mp_auto_origin
     d.mp_origin[:] = mp_origin
     d.mp_nodipole = mp_nodipole
+    d.char_proj = char_proj
     return d
 
 
@@ -100,6 +101,10 @@ def _lib():
     lib.orc_riemann_lhlld.argtypes = [ctypes.c_double, D, D, ctypes.c_double, ctypes.c_double, ctypes.c_double, D]
     lib.orc_plm_pencil.argtypes = [ctypes.c_int, D, D, D, D, D]
     lib.orc_ppm_pencil.argtypes = [ctypes.c_int, D, D, D]
+    lib.orc_char_matrices.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, D,
+                                      ctypes.c_double, D, D]
+    I = ctypes.c_int
+    lib.orc_recon_cell.argtypes = [P, I, I, I, I, I, I, D, D]
     return lib
 
 
@@ -162,6 +167,13 @@ class Oracle:
 
     def get(self, name):
         return self.view(name).copy()
+
+    def recon_cell(self, dir, order, g, k, j, i):
+        """(wm, wp): the reconstructed states of one cell at its lower and upper faces along
+        dir, 7 slots in the mesh frame (IBY, IBZ = the field transverse to dir)."""
+        wm, wp = np.zeros(7), np.zeros(7)
+        self.L.orc_recon_cell(self.h, dir, order, g, k, j, i, _dp(wm), _dp(wp))
+        return wm, wp
 
     def set(self, name, arr):
         self.view(name)[...] = arr
@@ -286,6 +298,17 @@ def plm_pencil(q, dxp=None, dxm=None):
     wr = np.full(n, np.nan)
     lib().orc_plm_pencil(n, _dp(q), _dp(dxp), _dp(dxm), _dp(wl), _dp(wr))
     return wl, wr
+
+
+def char_matrices(w7, b1, dir=0, gamma=5.0 / 3.0, iso=False, iso_cs=1.0):
+    """(L, R) of the eigenvector projections at one state: L @ v gives the characteristic
+    fields of the primitive vector v, R @ ch the primitive vector (7 x 7; isothermal: the
+    first 6 fields)."""
+    w7 = np.ascontiguousarray(w7, dtype=np.float64)
+    L = np.zeros((7, 7))
+    R = np.zeros((7, 7))
+    lib().orc_char_matrices(int(iso), gamma, iso_cs, dir, _dp(w7), b1, _dp(L), _dp(R))
+    return L, R
 
 
 def ppm_pencil(q):
