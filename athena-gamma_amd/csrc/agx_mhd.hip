@@ -584,8 +584,13 @@ __device__ __forceinline__ T sgn(T x) { return (x < T(0.0)) ? -T(1.0) : T(1.0); 
 // (reconstruct/ppm.cpp:58-338): CW84 fourth-order interface values with c1..c4 = 1/2,
 // c5 = 1/6, c6 = -1/6 (reconstruction.cpp:429-438), the CS08 extremum-preserving
 // limiter with C2 = 1.25 and the 1e-12 round-off guards.  q[0..4] = values at i-2..i+2.
-template <typename T>
-__device__ __forceinline__ void ppm_cell(const T *q, T &qminus, T &qplus) {
+// GUARD (FP32 only): *near set when one of the limiter's discontinuous tests -- the sign
+// of (dph - q_im1)(qc - dph) at either interface, of dqf_minus dqf_plus, and the
+// 1e-12 guard on d2qf where it is used -- has an operand within NEAR_TIE x FP32
+// rounding of max|q|, so FP32 arithmetic cannot be trusted to decide it as FP64 would.
+constexpr float NEAR_TIE = 64.0f * 5.9604645e-8f;   // 64 x 2^-24
+template <typename T, bool GUARD = false>
+__device__ __forceinline__ void ppm_cell(const T *q, T &qminus, T &qplus, bool *near = nullptr) {
   const T C2 = T(1.25);
   const T c1 = T(0.5), c2 = T(0.5), c3 = T(0.5), c4 = T(0.5), c5 = T(1.0) / T(6.0), c6 = -T(1.0) / T(6.0);
   const T q_im2 = q[0], q_im1 = q[1], qc = q[2], q_ip1 = q[3], q_ip2 = q[4];
@@ -598,6 +603,14 @@ __device__ __forceinline__ void ppm_cell(const T *q, T &qminus, T &qplus) {
   T dph_ip1 = (c3 * qc + c4 * q_ip1) + (c5 * dd + c6 * dd_ip1);
   T d2qc_im1 = q_im2 + qc - T(2.0) * q_im1;
   T d2qc = q_im1 + q_ip1 - T(2.0) * qc;
+  T tie = T(0.0);   // GUARD: operands below this decide no discontinuous test
+  if (GUARD) {
+    const T mq = dmax(dmax(dmax(xabs(q_im2), xabs(q_im1)), dmax(xabs(qc), xabs(q_ip1))), xabs(q_ip2));
+    tie = T(NEAR_TIE) * mq;
+    // below 1e-12 the products of the sign tests can underflow in FP32: decide in FP64
+    *near = mq < T(1.0e-12) || xabs(dph - q_im1) <= tie || xabs(qc - dph) <= tie ||
+            xabs(dph_ip1 - qc) <= tie || xabs(q_ip1 - dph_ip1) <= tie;
+  }
   T d2qc_ip1 = qc + q_ip2 - T(2.0) * q_ip1;
   {
     T qa_tmp = dph - q_im1;
@@ -625,6 +638,11 @@ __device__ __forceinline__ void ppm_cell(const T *q, T &qminus, T &qplus) {
   T dqf_plus = qp - qc;
   T qa_tmp = dqf_minus * dqf_plus;
   T qb_tmp = (q_ip1 - qc) * (qc - q_im1);
+  if (GUARD) {   // (q_ip1 - qc), (qc - q_im1): exact signs from the inputs themselves
+    *near = *near || xabs(dqf_minus) <= tie || xabs(dqf_plus) <= tie ||
+            (qb_tmp == T(0.0) && q_ip1 != qc && qc != q_im1) ||
+            ((qa_tmp <= T(0.0) || qb_tmp <= T(0.0)) && xabs(d2qf) <= tie);
+  }
   T qe = T(0.0);
   if (sgn(d2qc_im1) == sgn(d2qc) && sgn(d2qc_im1) == sgn(d2qc_ip1) && sgn(d2qc_im1) == sgn(d2qf))
     qe = sgn(d2qf) * dmin(dmin(C2 * xabs(d2qc_im1), C2 * xabs(d2qc)), dmin(C2 * xabs(d2qc_ip1), xabs(d2qf)));
@@ -768,6 +786,25 @@ __device__ __forceinline__ void char_right(const CharEigen<T> &e, const T *ch, T
 // Reconstruction orders of face_flux beyond the reference's xorder 1..3: PLM and PPM
 // limited in characteristic variables (xorder "2c" and "3c", reconstruction.cpp:53-66).
 constexpr int ORDER_PLM_CHAR = 4, ORDER_PPM_CHAR = 5;
+
+// PPM of one cell as the flux calculation uses it.  FP64: ppm_cell.  FP32 (mixed form):
+// ppm_cell in FP32, re-run in FP64 on the same (FP32) inputs wherever a discontinuous
+// limiter test is a near-tie, so every branch is taken as FP64 arithmetic takes it on
+// those inputs and the FP32 rounding only moves the values continuously.
+__device__ __forceinline__ void ppm_limited(const double *q, double &qminus, double &qplus) {
+  ppm_cell(q, qminus, qplus);
+}
+__device__ __forceinline__ void ppm_limited(const float *q, float &qminus, float &qplus) {
+  bool near;
+  ppm_cell<float, true>(q, qminus, qplus, &near);
+  if (near) {
+    const double qd[5] = {q[0], q[1], q[2], q[3], q[4]};
+    double m, p;
+    ppm_cell(qd, m, p);
+    qminus = float(m);
+    qplus = float(p);
+  }
+}
 
 // Roe solver for adiabatic MHD at one interface, hydro/rsolvers/mhd/roe_mhd.cpp:56-212
 // with RoeFlux (:245-498): Roe-averaged eigensystem, wave-strength dissipation, LLF
@@ -1057,7 +1094,7 @@ __device__ __forceinline__ void face_flux(int dir, const T *__restrict__ wb, con
 #pragma unroll
         for (int n = 0; n < NWC; ++n) {
           T c5[5] = {ch[0][n], ch[1][n], ch[2][n], ch[3][n], ch[4][n]}, qm, qp;
-          ppm_cell(c5, qm, qp);
+          ppm_limited(c5, qm, qp);
           lim[n] = side ? qm : qp;
         }
         if (NWC < NWAVE) lim[NWAVE - 1] = T(0.0);
@@ -1113,8 +1150,8 @@ __device__ __forceinline__ void face_flux(int dir, const T *__restrict__ wb, con
 #pragma unroll
       for (int t = 0; t < 6; ++t) q6[t] = q[(t - 3) * st];
       T m0, p0, m1, p1;
-      ppm_cell(q6, m0, p0);       // cell c-1: values at c-3..c+1
-      ppm_cell(q6 + 1, m1, p1);   // cell c:   values at c-2..c+2
+      ppm_limited(q6, m0, p0);       // cell c-1: values at c-3..c+1
+      ppm_limited(q6 + 1, m1, p1);   // cell c:   values at c-2..c+2
       wli[n] = p0;
       wri[n] = m1;
     } else {

@@ -268,13 +268,16 @@ def mg_bench(agx, pgen, args, precision="fp64"):
     return vps, iters, norm, t_solve, per
 
 
-def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None):
+def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None, sensitivity=False):
     """MHD + multigrid self-gravity at N^3 in MeshBlocks of B^3: lognormal density
     (sigma_lnrho = 1), rms Mach `mach`, uniform Bx (plasma beta 1, or Bx = `bx`), 4 pi G = 1, VL2 + PLM (xorder 2) or PPM
     (xorder 3) + HLLD + CT, FMG + V-cycles to 1e-10 x rms(rho - <rho>) at every stage.  The
     plain FP64 path and the mixed form (FP32 reconstruction/Riemann and FP32 V-cycles in
     FP64 defect correction) run `steps` cycles each from the same state; returns both
-    rates and the mixed-vs-FP64 differences after those cycles."""
+    rates and the mixed-vs-FP64 differences after those cycles.  sensitivity: also run the
+    FP64 path from the state with its conserved variables multiplied by (1 + 3e-8 N(0,1))
+    and report how far that moves it (the discretization's own response to a perturbation
+    below FP32 rounding, the scale the mixed-vs-FP64 differences are read against)."""
     ng = 3 if xorder == 3 else 2
     base = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4, gamma=5.0 / 3.0,
                          xorder=xorder, nghost=ng, rsolver=0)
@@ -287,13 +290,16 @@ def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None):
     del b1, b2, b3
     log(f"[{tag}] {N}^3 lognormal turbulence built in {time.time() - t0:.1f}s")
     res = {}
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed") + (("fp64_perturbed",) if sensitivity else ()):
         desc = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4,
                              gamma=5.0 / 3.0, xorder=xorder, nghost=ng, rsolver=0, self_gravity=1,
                              four_pi_G=1.0, mg_mode=0, mg_threshold=1e-10 * src_norm,
                              precision=1 if prec == "mixed" else 0)
         m = agx.Mesh(desc)
-        m.set("u", ub)
+        if prec == "fp64_perturbed":
+            m.set("u", ub * (1.0 + 3e-8 * np.random.default_rng(1).standard_normal(ub.shape)))
+        else:
+            m.set("u", ub)
         for n, f in zip(("b1f", "b2f", "b3f"), faces):
             m.set(n, f)
         m.initialize()
@@ -317,7 +323,8 @@ def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None):
         per = {c: m.timing_get(c) for c in KCLASSES + ("mg_smooth", "mg_restrict", "mg_prolong", "mg_other")}
         m.timing(False)
         res[prec] = dict(zc=float(N) ** 3 * steps / sec, ms=sec * 1e3 / steps, w=w, bcc=bcc, phi=phi, divb=divb,
-                         per=per, tile=tile, bytes=m.device_bytes(), rf=roofline(desc, per, prec, tile))
+                         per=per, tile=tile, bytes=m.device_bytes(),
+                         rf=roofline(desc, per, "mixed" if prec == "mixed" else "fp64", tile))
         log(f"[{tag}] {prec}: {steps} cycles in {sec:.2f}s -> {res[prec]['zc']:.4e} zone-cycles/s, "
             f"device {m.device_bytes() / 1e9:.1f} GB, stage form {STAGE_FORMS[tile]}; per cycle: "
             + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in per.items() if v[0] > 0))
@@ -347,6 +354,15 @@ def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None):
         "error_note": f"max-norm (rel_err_*) and mean (rel_l1_*) differences relative to the FP64 path's "
                       f"max and mean magnitude after the same {steps} cycles",
     }
+    if sensitivity:
+        c = res["fp64_perturbed"]
+        out["fp64_self_sensitivity"] = {
+            "perturbation": "conserved variables x (1 + 3e-8 N(0,1)), seed 1",
+            "rel_err_rho": err(c["w"][:, 0], b["w"][:, 0]), "rel_err_p": err(c["w"][:, 1], b["w"][:, 1]),
+            "rel_err_bcc": err(c["bcc"], b["bcc"]), "rel_err_phi": err(c["phi"], b["phi"]),
+            "rel_l1_rho": l1(c["w"][:, 0], b["w"][:, 0]), "rel_l1_p": l1(c["w"][:, 1], b["w"][:, 1])}
+        log(f"[{tag}] FP64 vs FP64 from the perturbed state: " +
+            ", ".join(f"{k} {v:.3e}" for k, v in out["fp64_self_sensitivity"].items() if k != "perturbation"))
     log(f"[{tag}] mixed vs fp64: {out['vs_fp64']:.3f}x; after {steps} cycles max rho {out['rel_err_rho']:.3e}, "
         f"P {out['rel_err_p']:.3e}, bcc {out['rel_err_bcc']:.3e}, phi {out['rel_err_phi']:.3e}; "
         f"mean rho {out['rel_l1_rho']:.3e}, P {out['rel_l1_p']:.3e}, bcc {out['rel_l1_bcc']:.3e}")
@@ -358,7 +374,7 @@ def large_bench(agx, pgen, args):
     MeshBlocks of 64^3, VL2+PPM+HLLD+CT, lognormal density (sigma_lnrho = 1), rms Mach 5,
     beta = 1, 4 pi G = 1; the mixed form is the headline value."""
     N, B = args.large_n, 64
-    r = sg_run(agx, pgen, N, B, 3, 5.0, args.steps, "large")
+    r = sg_run(agx, pgen, N, B, 3, 5.0, args.steps, "large", sensitivity=True)
     a = r["mixed"]
     out = {
         "metric": "MHD+MG zone-cycles/s at 512^3 on 1 MI355X, mixed precision; % of HBM roofline",
@@ -372,7 +388,8 @@ def large_bench(agx, pgen, args):
         "max_divb": a["max_divb"], "max_divb_dx_over_B": a["max_divb_dx_over_B"],
         "kernel_ms_per_cycle": a["kernel_ms_per_cycle"],
     }
-    out.update({k: v for k, v in r.items() if k.startswith("rel_") or k in ("vs_fp64", "error_note")})
+    out.update({k: v for k, v in r.items()
+                if k.startswith("rel_") or k in ("vs_fp64", "error_note", "fp64_self_sensitivity")})
     print(json.dumps(out), flush=True)
 
 

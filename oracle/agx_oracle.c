@@ -92,6 +92,8 @@ struct orc_ctx {
   int srcmask;       /* source mask: src = 0 where |x - c|^2 > r^2 (pgen/collapse.cpp:66-82) */
   double srcmask_c[3], srcmask_r2;
   int iso;           /* isothermal EOS (eos = 1): NHYDRO = 4, the energy slot stays 0 */
+  double noise_eps;  /* test-only: primitive perturbation after each C2P (orc_set_noise) */
+  uint64_t noise_state;
   int nhydro;
   int fas;           /* ffas_: multipole boundaries (multigrid_driver.cpp:292-295) */
   double mpo[3];     /* expansion origin (mpo_) */
@@ -2034,6 +2036,33 @@ void orc_initialize(orc_ctx *c) {
 /* One cycle of the main loop (main.cpp:458-489) with the VL2 task list: stage 1 uses
  * donor-cell reconstruction (time_integrator.cpp:1466-1467), stage 2 xorder; MG solve
  * after each stage (main.cpp:459-466); then time += dt and NewTimeStep. */
+/* Test-only perturbation of the primitives the next reconstruction reads (not part of
+ * the reference's algorithm): eps > 0 multiplies every w and bcc value by (1 + eps u),
+ * u uniform in [-1, 1) from a xorshift64* stream seeded with seed; eps < 0 rounds them to
+ * FP32; eps = 0 turns it off.  It emulates FP32 storage of the primitives, to measure how
+ * far the FP64 discretization itself moves under perturbations of that size. */
+void orc_set_noise(orc_ctx *c, double eps, uint64_t seed) {
+  c->noise_eps = eps;
+  c->noise_state = seed * 0x9E3779B97F4A7C15ull + 1;
+}
+
+static void perturb_primitives(orc_ctx *c) {
+  double *arrs[2] = {c->arr[ARR_W], c->arr[ARR_BCC]};
+  int64_t n[2] = {c->arrsize[ARR_W] * c->nblocks, c->arrsize[ARR_BCC] * c->nblocks};
+  for (int a = 0; a < 2; ++a)
+    for (int64_t i = 0; i < n[a]; ++i) {
+      if (c->noise_eps < 0.0) {
+        arrs[a][i] = (double)(float)arrs[a][i];
+        continue;
+      }
+      uint64_t x = c->noise_state;
+      x ^= x >> 12; x ^= x << 25; x ^= x >> 27;
+      c->noise_state = x;
+      double u = (double)((x * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
+      arrs[a][i] *= 1.0 + c->noise_eps * (2.0 * u - 1.0);
+    }
+}
+
 void orc_cycle(orc_ctx *c, double tlim) {
   for (int stage = 1; stage <= 2; ++stage) {
     orc_calculate_fluxes(c, stage == 1 ? 1 : c->d.xorder);
@@ -2042,6 +2071,7 @@ void orc_cycle(orc_ctx *c, double tlim) {
     orc_bvals_cc(c, ARR_U);
     orc_bvals_fc(c, 0);
     orc_cons2prim(c);
+    if (c->noise_eps != 0.0) perturb_primitives(c);
     if (c->d.self_gravity) orc_mg_solve(c, stage);
   }
   c->ncycle++;

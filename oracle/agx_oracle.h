@@ -105,6 +105,7 @@ void orc_riemann_iso(int rs, double cs, double dfloor, const double *wl, const d
 void orc_plm_pencil(int n, const double *q, const double *dxp, const double *dxm,
                     double *wl, double *wr);
 void orc_ppm_pencil(int n, const double *q, double *wl, double *wr);
+void orc_set_noise(orc_ctx *c, double eps, uint64_t seed);
 void orc_char_matrices(int iso, double gamma, double iso_cs, int dir, const double *w7, double b1,
                        double *L, double *R);
 void orc_recon_cell(const orc_ctx *c, int dir, int order, int g, int k, int j, int i, double *wm,

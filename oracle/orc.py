@@ -101,6 +101,7 @@ def _lib():
     lib.orc_riemann_lhlld.argtypes = [ctypes.c_double, D, D, ctypes.c_double, ctypes.c_double, ctypes.c_double, D]
     lib.orc_plm_pencil.argtypes = [ctypes.c_int, D, D, D, D, D]
     lib.orc_ppm_pencil.argtypes = [ctypes.c_int, D, D, D]
+    lib.orc_set_noise.argtypes = [P, ctypes.c_double, ctypes.c_uint64]
     lib.orc_char_matrices.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, D,
                                       ctypes.c_double, D, D]
     I = ctypes.c_int
@@ -167,6 +168,11 @@ class Oracle:
 
     def get(self, name):
         return self.view(name).copy()
+
+    def set_noise(self, eps, seed=0):
+        """test-only: perturb w and bcc after every C2P of a cycle by a relative eps
+        (uniform, seeded), or round them to FP32 (eps < 0); 0 turns it off"""
+        self.L.orc_set_noise(self.h, eps, seed)
 
     def recon_cell(self, dir, order, g, k, j, i):
         """(wm, wp): the reconstructed states of one cell at its lower and upper faces along

@@ -88,3 +88,37 @@ def load_state(target, state):
     target.set("b1f", f1)
     target.set("b2f", f2)
     target.set("b3f", f3)
+
+
+def oracle_self_sensitivity(desc, state, advance, eps=6e-8, seeds=tuple(range(1, 9))):
+    """How far the FP64 oracle moves from itself when the primitives its reconstruction
+    reads are rounded to FP32 after every C2P, or multiplied there by (1 + eps u), u
+    uniform in [-1, 1) (seeded; eps = 2^-24, the FP32 rounding bound): the FP64
+    discretization's own response to perturbations of FP32-rounding size, which bounds
+    what any form that stores or rounds the primitives in FP32 can be held to (PPM's
+    limiter is discontinuous at its sign tests, so a perturbation that decides a near-tie
+    the other way moves that cell by O(dq)).  advance(o) runs the workload.  Returns the
+    max over the runs of the max-norm difference, relative to the unperturbed maximum, of
+    rho, P and bcc (interior) and of each face field."""
+    from oracle.orc import Oracle
+    ng = desc.nghost
+    sl = (slice(None), slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
+    fs = (slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
+
+    def run(e, seed):
+        o = Oracle(desc)
+        load_state(o, state)
+        o.initialize()
+        o.set_noise(e, seed)
+        advance(o)
+        return o.get("w")[sl], [o.get(n)[fs] for n in ("b1f", "b2f", "b3f")], o.get("bcc")[sl]
+    w0, b0, c0 = run(0.0, 0)
+    out = {"rho": 0.0, "P": 0.0, "bcc": 0.0, "b1f": 0.0, "b2f": 0.0, "b3f": 0.0}
+    for e, seed in [(-1.0, 0)] + [(eps, s) for s in seeds]:
+        w1, b1, c1 = run(e, seed)
+        out["bcc"] = max(out["bcc"], float(np.abs(c1 - c0).max() / np.abs(c0).max()))
+        for k, n in (("rho", 0), ("P", 4)):
+            out[k] = max(out[k], float(np.abs(w1[:, n] - w0[:, n]).max() / np.abs(w0[:, n]).max()))
+        for k, a, b in zip(("b1f", "b2f", "b3f"), b1, b0):
+            out[k] = max(out[k], float(np.abs(a - b).max() / np.abs(b).max()))
+    return out

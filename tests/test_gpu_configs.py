@@ -11,8 +11,10 @@
 - The large run's state at reduced size (pgen.turbulence_lognormal: sigma_lnrho = 1,
   rms Mach 5, beta = 1, 4 pi G = 1) with PPM + HLLD + CT and multigrid self-gravity
   (FMG + V-cycles to 1e-10 x rms(rho - <rho>) each stage) at 32^3 in 8 MeshBlocks of 16^3
-  (nghost 3): FP64 within 1e-12 of the oracle (potential 1e-10); the mixed form against
-  the FP64 oracle with the bars stated in the test.
+  (nghost 3): FP64 within 1e-12 of the oracle (potential 1e-10); the mixed form within
+  1e-5 of the FP64 oracle in rho and P (max norm; or twice the FP64 oracle's own response
+  to FP32 rounding of its primitives, if larger), 1e-6 in the potential and the means,
+  1e-4 in bcc (the CT weight's amplification of FP32 mass-flux rounding, stated in the test).
 
 div B bars: absolute max |div B| (B per unit length) is held to 1e-13 where dx is at most
 1/32; the dimensionless max |div B| dx / max|B| (the round-off measure that does not
@@ -23,6 +25,7 @@ import pytest
 
 from oracle.orc import Oracle, make_desc
 from tests.conftest import load_pgen
+from tests.states import oracle_self_sensitivity
 
 pytestmark = pytest.mark.gpu
 
@@ -49,6 +52,7 @@ def load_global(agx, targets, desc, U, b1, b2, b3):
         t.set("u", ub)
         for n, f in zip(("b1f", "b2f", "b3f"), faces):
             t.set(n, f)
+    return ub, faces
 
 
 def divb(desc, m):
@@ -132,7 +136,7 @@ def test_lognormal_ppm_selfgravity_fp64_and_mixed(agx):
     desc = make_desc(mg_threshold=thr, **base)
     dmix = make_desc(mg_threshold=thr, precision=1, **base)
     o, m, mx = Oracle(desc), agx.Mesh(desc), agx.Mesh(dmix)
-    load_global(agx, (o, m, mx), desc, U, b1, b2, b3)
+    ub, faces = load_global(agx, (o, m, mx), desc, U, b1, b2, b3)
     o.initialize()
     m.initialize()
     mx.initialize()
@@ -153,11 +157,21 @@ def test_lognormal_ppm_selfgravity_fp64_and_mixed(agx):
     print(f"  mixed vs FP64 oracle after {ncyc} cycles, max norm: "
           + ", ".join(f"{k} {v:.3e}" for k, v in err.items())
           + "; mean: " + ", ".join(f"{k} {v:.3e}" for k, v in mean.items()), flush=True)
-    # bars: the potential and the means to 1e-6; the max norm of rho, P, bcc to 1e-4
-    # (PPM's extremum limiter branches on signs of second differences, which FP32 w can
-    # flip at isolated cells of a Mach-5, 1000:1 density field)
+    # bars: the potential and the means to 1e-6; the max norm of rho, P, bcc to 1e-5, or to
+    # twice the FP64 oracle's own response to FP32 rounding of its primitives if that is
+    # larger (PPM's extremum limiter is discontinuous at its sign tests, so that
+    # response can reach 1e-5..1e-4 at a few cells of a Mach-5, 1000:1 density field)
+    sens = oracle_self_sensitivity(desc, (ub,) + tuple(faces), lambda x: [x.cycle(1e30) for _ in range(ncyc)])
+    print("  FP64 oracle's response to FP32-rounded primitives: "
+          + ", ".join(f"{k} {sens[k]:.3e}" for k in ("rho", "P", "bcc")), flush=True)
     assert err["phi"] <= 1e-6 and mean["rho"] <= 1e-6 and mean["P"] <= 1e-6
-    assert err["rho"] <= 1e-4 and err["P"] <= 1e-4 and err["bcc"] <= 1e-4
+    for k in ("rho", "P"):
+        assert err[k] <= max(1e-5, 2.0 * sens[k]), k
+    # bcc: 1e-4.  The CT upwind weight is 0.5 + clamp(1024 dt F_rho / (dx (rho_l + rho_r)),
+    # -0.5, 0.5) (hydro.cpp:154-158): it multiplies the FP32 rounding of the mass flux by
+    # ~1024 CFL, an absolute weight error of ~2e-5, which the corner EMFs carry into B
+    # where the face and cell-centred EMFs differ most (the Mach-5 shocks here)
+    assert err["bcc"] <= 1e-4
     for tag, mm in (("fp64", m), ("mixed", mx)):
         dabs, drel = divb(desc, mm)
         print(f"  {tag} max |div B| {dabs:.3e} (x dx / max|B| {drel:.3e})", flush=True)

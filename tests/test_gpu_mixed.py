@@ -4,13 +4,17 @@ The mixed form reconstructs and solves the Riemann problem in FP32 from FP32 cop
 w and bcc, and keeps the conserved-variable update, C2P with floors, the corner EMFs and
 CT in FP64.  Bars (stated per test): density and pressure within 1e-5 of the FP64 oracle
 relative to their maximum over the mesh; the face field's div B at FP64 round-off,
-because CT preserves it for any EMF values; the time step within 1e-5.
+because CT preserves it for any EMF values; the time step within 1e-5.  With PPM the
+max-norm bar is twice the FP64 oracle's own response to FP32 rounding of its primitives
+after every C2P (PPM's limiter is discontinuous at its sign tests, so that response is
+1e-5..4e-5 here, set by a few cells, while the mean stays at 1e-7); the mixed form runs
+the PPM arithmetic itself in FP64, so only the FP32 storage of w and bcc perturbs it.
 """
 import numpy as np
 import pytest
 
 from oracle.orc import Oracle, make_desc
-from tests.states import load_state, turbulent_state
+from tests.states import load_state, oracle_self_sensitivity, turbulent_state
 
 pytestmark = pytest.mark.gpu
 RHO_P_TOL = 1e-5
@@ -77,26 +81,27 @@ def test_mixed_turbulence_run(agx, rsolver, shape):
     ep = rel_err(wm[:, 4], wo[:, 4])
     um, uo = interior(m.get("u"), ng), interior(o.get("u"), ng)
     print(f"rho {erho:.3e}  P {ep:.3e}  E {rel_err(um[:, 4], uo[:, 4]):.3e}", flush=True)
+    bar = {"rho": RHO_P_TOL, "P": RHO_P_TOL, "b1f": RHO_P_TOL, "b2f": RHO_P_TOL, "b3f": RHO_P_TOL}
     if shape == "cube_ppm":
-        # PPM's extremum limiter (ppm.cpp:190-250) branches on signs of second differences
-        # and on rho <= 1 - 1e-12; at a few cells those tests flip on the FP32 rounding of
-        # w, and the max-norm error there is set by the branch jump, not by rounding.
-        # Bars: mean error at FP32 rounding (1e-6), max error 1e-4.
+        # PPM's extremum limiter (ppm.cpp:190-250) is discontinuous at its sign tests: the
+        # FP64 oracle itself moves by 1e-5..4e-5 in max norm at a few cells when its
+        # primitives are rounded to FP32, and no form that stores them in FP32 can be held
+        # below that.  Bars: twice the oracle's own response (at least 1e-5) in max norm,
+        # 1e-6 in the mean.
+        sens = oracle_self_sensitivity(desc, turbulent_state(desc, seed=11, mach=1.5), lambda x: x.run(tlim))
+        print("FP64 oracle's response to FP32-rounded primitives: " + ", ".join(f"{k} {v:.3e}" for k, v in sens.items()),
+              flush=True)
+        bar = {k: max(RHO_P_TOL, 2.0 * v) for k, v in sens.items()}
         l1 = lambda a, b: np.abs(a - b).mean() / np.abs(b).mean()
         print(f"L1: rho {l1(wm[:, 0], wo[:, 0]):.3e}  P {l1(wm[:, 4], wo[:, 4]):.3e}", flush=True)
         assert l1(wm[:, 0], wo[:, 0]) <= 1e-6 and l1(wm[:, 4], wo[:, 4]) <= 1e-6
-        assert erho <= 1e-4 and ep <= 1e-4
-    else:
-        assert erho <= RHO_P_TOL and ep <= RHO_P_TOL
+    assert erho <= bar["rho"] and ep <= bar["P"]
     for n in ("b1f", "b2f", "b3f"):
         sl = (slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
         a, b = m.get(n)[sl], o.get(n)[sl]
         e = rel_err(a, b)
-        print(f"{n}: {e:.3e}", flush=True)
-        if shape == "cube_ppm":   # the same limiter branch flips as above, B ~ 0.1
-            assert np.abs(a - b).mean() / np.abs(b).mean() <= 1e-6 and e <= 1e-3
-        else:
-            assert e <= RHO_P_TOL
+        print(f"{n}: {e:.3e} (bar {bar[n]:.3e})", flush=True)
+        assert e <= bar[n]
     o2 = Oracle(desc)
     for n in ("b1f", "b2f", "b3f"):
         o2.set(n, m.get(n))
