@@ -167,7 +167,7 @@ def mhd_bench(agx, pgen, args, precision):
     return desc, zc, sec * 1e3 / args.steps, per, state
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_stage_turb256_fp64_hbm_traffic.txt")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "mhd_turb256_plm_hlld_fp64_mixed_hbm_traffic.txt")
 
 
 def kernel_name(precision, tile, order=2):
