@@ -46,6 +46,7 @@ EXPORTS = [
     "agx_ref_internal_to_cons", "agx_mg_set_source_mask_sphere", "agx_set_mg_coarse",
     "agx_history", "agx_history_sum", "agx_history_sum_noalloc",
     "agx_restart_write", "agx_restart_read", "agx_restart_write_host", "agx_restart_read_host",
+    "agx_vtk_write", "agx_vtk_write_host", "agx_block_gids",
     "agx_mg_setup", "agx_mg_smooth", "agx_mg_fill_ghost", "agx_mg_fas", "agx_mg_restrict", "agx_mg_prolong",
     "agx_mg_fmg_prolong", "agx_mg_solve_coarsest", "agx_mg_defect_norm", "agx_mg_retrieve", "agx_mg_coffset",
     "agx_mg_levels",
@@ -216,6 +217,10 @@ def lib():
     L.agx_restart_write_host.argtypes = [C, ctypes.POINTER(Desc), C, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int32, D, D, D, D]
     L.agx_restart_read_host.argtypes = [C, ctypes.POINTER(Desc), D, D, ctypes.POINTER(ctypes.c_int32), D, D, D, D]
+    L.agx_vtk_write.argtypes = [P, C, C, ctypes.c_int32, C, ctypes.c_int32]
+    L.agx_vtk_write_host.argtypes = [C, C, ctypes.c_int32, C, ctypes.c_int32, ctypes.POINTER(Desc), ctypes.c_double,
+                                     ctypes.c_int32, D, D, D, D]
+    L.agx_block_gids.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(ctypes.c_int64)]
     L.agx_device_alloc.argtypes = [ctypes.c_int64, ctypes.POINTER(P)]
     L.agx_device_free.argtypes = [P]
     L.agx_device_copy.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
@@ -426,6 +431,11 @@ class Mesh:
         _check(self.L.agx_restart_write(self.h, os.fsencode(path), None if params is None else params.encode()),
                "agx_restart_write")
 
+    def vtk_write(self, basename, file_id="out1", file_number=0, variable="prim", ghost=False):
+        """VTK legacy files of the current state, one per MeshBlock (agx_vtk_write)."""
+        _check(self.L.agx_vtk_write(self.h, os.fsencode(basename), file_id.encode(), file_number, variable.encode(),
+                                    1 if ghost else 0), "agx_vtk_write")
+
     def restart_read(self, path):
         """Load u, b, time, dt and ncycle from a restart file (agx_restart_read); call
         initialize() next, as the reference's Mesh::Initialize follows its restart
@@ -460,6 +470,27 @@ def restart_read_host(path, desc, shapes):
     _check(lib().agx_restart_read_host(os.fsencode(path), ctypes.byref(_desc(desc)), ctypes.byref(t), ctypes.byref(dt),
                                        ctypes.byref(n), *[a.ctypes.data_as(D) for a in arrs]), "agx_restart_read_host")
     return (t.value, dt.value, n.value, *arrs)
+
+
+def vtk_write_host(basename, desc, time, ncycle, u, w, bcc, phi=None, file_id="out1", file_number=0,
+                   variable="prim", ghost=False):
+    """agx_vtk_write_host: VTK legacy files from host arrays (no device work)."""
+    D = ctypes.POINTER(ctypes.c_double)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, w, bcc)]
+    ph = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+    _check(lib().agx_vtk_write_host(os.fsencode(basename), file_id.encode(), file_number, variable.encode(),
+                                    1 if ghost else 0, ctypes.byref(_desc(desc)), time, ncycle,
+                                    *[a.ctypes.data_as(D) for a in arrs],
+                                    None if ph is None else ph.ctypes.data_as(D)), "agx_vtk_write_host")
+
+
+def block_gids(desc):
+    """agx_block_gids: the lattice index of each of the reference's MeshBlock gids."""
+    nb = (desc.nx1 // desc.bnx1) * (desc.nx2 // desc.bnx2) * (desc.nx3 // desc.bnx3)
+    out = np.zeros(nb, dtype=np.int64)
+    _check(lib().agx_block_gids(ctypes.byref(_desc(desc)), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
+           "agx_block_gids")
+    return out
 
 
 def _desc(desc):

@@ -202,6 +202,17 @@ struct agx_mesh {
 
 namespace agx {
 
+// Face coordinates xf[0..n] and centres xv[0..n-1] (n = nbx + 2 ng) of the block at
+// logical position lx along one direction of a uniform mesh, and its cell width
+// (coordinates/coordinates.cpp:127-145, cartesian.cpp:26-36, mesh/mesh.cpp:1668-1690).
+void block_coords(int nmesh, int nblk, int nbx, int ng, int lx, double xmin, double xmax,
+                  std::vector<double> &xf, std::vector<double> &xv, double &dx);
+
+// Lattice index (lx3*nb2 + lx2)*nb1 + lx1 of each MeshBlock in the reference's gid order:
+// the Z order in which MeshBlockTree::GetMeshBlockList lists the root grid
+// (mesh/meshblock_tree.cpp:87-110,336-352).
+std::vector<int64_t> gid_lattice_order(int nb1, int nb2, int nb3);
+
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what);
 

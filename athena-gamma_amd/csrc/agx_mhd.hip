@@ -2364,7 +2364,7 @@ __global__ void __launch_bounds__(256) k_min_final(const double *__restrict__ pa
 }
 
 // ================================================================ host side
-namespace {
+namespace agx {
 
 double meshgen_x(int64_t index, int64_t nrange) {   // mesh/mesh.hpp:389-405
   int64_t noffset = index - (nrange) / 2;
@@ -2394,6 +2394,12 @@ void block_coords(int nmesh, int nblk, int nbx, int ng, int lx, double xmin, dou
   xf[iu + 1] = bmax;
   for (int i = 0; i < n; ++i) xv[i] = 0.5 * (xf[i + 1] + xf[i]);
 }
+
+}  // namespace agx
+
+using agx::block_coords;
+
+namespace {
 
 int check_desc(const agx_mesh_desc *d) {
   if (!d) { set_error("null descriptor"); return -1; }

@@ -123,11 +123,33 @@ struct File {
 };
 
 }  // namespace
+
+std::vector<int64_t> gid_lattice_order(int nb1, int nb2, int nb3) {
+  std::vector<LocF> locs;
+  const int root = root_level_of(nb1, nb2, nb3);
+  zorder(0, 0, 0, 0, root, nb1, nb2, nb3, locs);
+  std::vector<int64_t> out;
+  out.reserve(locs.size());
+  for (const LocF &l : locs) out.push_back((l.lx3 * nb2 + l.lx2) * nb1 + l.lx1);
+  return out;
+}
+
 }  // namespace agx
 
 using namespace agx;
 
 extern "C" {
+
+int agx_block_gids(const agx_mesh_desc *d, int64_t *lattice_of_gid) {
+  Layout L;
+  int rc = layout_of(d, L, "agx_block_gids");
+  if (rc) return rc;
+  if (!lattice_of_gid) { set_error("agx_block_gids: null output"); return -1; }
+  const std::vector<int64_t> o = gid_lattice_order(L.nb1, L.nb2, L.nb3);
+  for (size_t q = 0; q < o.size(); ++q) lattice_of_gid[q] = o[q];
+  return 0;
+}
+
 
 int agx_restart_write_host(const char *path, const agx_mesh_desc *d, const char *params, double time, double dt,
                            int32_t ncycle, const double *u, const double *b1f, const double *b2f,

@@ -22,8 +22,10 @@
  *        x3f   [nblocks][n3+1][n2][n1]
  *        x1e   [nblocks][n3+1][n2+1][n1] x2e [nblocks][n3+1][n2][n1+1]
  *        x3e   [nblocks][n3][n2+1][n1+1]
- *     Block gid = (lx3*nb2 + lx2)*nb1 + lx1.  Device storage is padded (row pitch a
- *     multiple of 16 doubles); agx_upload/agx_download convert.
+ *     Block index = (lx3*nb2 + lx2)*nb1 + lx1 (lattice order; the reference numbers
+ *     MeshBlocks (gid) in the Z order of its block tree, and agx_block_gids gives the
+ *     lattice index of each gid).  Device storage is padded (row pitch a multiple of
+ *     16 doubles); agx_upload/agx_download convert.
  *   - Not re-entrant per mesh: call a given agx_mesh from one host thread.  Different
  *     meshes are independent.
  */
@@ -244,13 +246,33 @@ int agx_history_sum_noalloc(const double *u, const double *bcc, const double *ph
  * (mesh.cpp:1646-1649 with the restored dt). */
 int agx_restart_write(agx_mesh *m, const char *path, const char *params);
 int agx_restart_read(agx_mesh *m, const char *path);
-/* The same on host arrays in the layout of agx_upload (block gid order, NHYDRO = 4 for
- * the isothermal EOS); no device work. */
+/* The same on host arrays in the layout of agx_upload (lattice block order, NHYDRO = 4
+ * for the isothermal EOS); no device work. */
 int agx_restart_write_host(const char *path, const agx_mesh_desc *d, const char *params, double time, double dt,
                            int32_t ncycle, const double *u, const double *b1f, const double *b2f,
                            const double *b3f);
 int agx_restart_read_host(const char *path, const agx_mesh_desc *d, double *time, double *dt, int32_t *ncycle,
                           double *u, double *b1f, double *b2f, double *b3f);
+
+/* ---- VTK legacy output (VTKOutput::WriteOutputFile, outputs/vtk.cpp:54-220): one file
+ * per MeshBlock, "<basename>.block<gid>.<file_id>.<file_number as %05d>.vtk", binary
+ * big-endian float32 on a rectilinear grid of the block's face coordinates, the cell
+ * data of OutputType::LoadOutputData (outputs/outputs.cpp:355-660) for the comma-
+ * separated tokens of <output>/variable: prim (rho, press, vel, phi, Bcc), cons (dens,
+ * Etot, mom, phi, Bcc), D, d, E, p, m, m1-m3, v, v1-v3 (vx-vz), phi, bcc, bcc1-bcc3;
+ * other tokens are refused (-3).  ghost != 0: include_ghost_zones.  Slices and sums are
+ * not supported. */
+int agx_vtk_write(agx_mesh *m, const char *basename, const char *file_id, int32_t file_number,
+                  const char *variable, int32_t ghost);
+/* The same from host arrays in the layout of agx_download (phi may be null without
+ * self-gravity); no device work. */
+int agx_vtk_write_host(const char *basename, const char *file_id, int32_t file_number, const char *variable,
+                       int32_t ghost, const agx_mesh_desc *d, double time, int32_t ncycle, const double *u,
+                       const double *w, const double *bcc, const double *phi);
+/* The reference's MeshBlock numbering (MeshBlockTree::GetMeshBlockList, Z order,
+ * mesh/meshblock_tree.cpp:336-352): lattice_of_gid[gid] = (lx3*nb2 + lx2)*nb1 + lx1 for
+ * gid = 0 .. nblocks-1; no device work. */
+int agx_block_gids(const agx_mesh_desc *d, int64_t *lattice_of_gid);
 
 /* ---- multigrid at task granularity (MultigridTaskList, multigrid/mg_task_list.cpp:
  * 183-280, 287-530; MGGravity::Smooth/CalculateDefect/CalculateFASRHS, mg_gravity.hpp:

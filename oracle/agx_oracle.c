@@ -166,6 +166,9 @@ static void mg_alloc(mglevels *m, int nx1, int nx2, int nx3, int nlev, double rd
 }
 
 orc_ctx *orc_create(const orc_desc *d) {
+  /* the multigrid hierarchy is built from logically cubic MeshBlocks of 2^n cells
+   * (multigrid.cpp:35-42 refuses others); anything else is refused here too */
+  if (d->self_gravity && (d->bnx1 != d->bnx2 || d->bnx1 != d->bnx3 || (d->bnx1 & (d->bnx1 - 1)))) return NULL;
   orc_ctx *c = (orc_ctx *)xcalloc(sizeof(orc_ctx));
   c->d = *d;
   c->nb1 = d->nx1 / d->bnx1;
@@ -2025,11 +2028,18 @@ void orc_new_time_step(orc_ctx *c, double tlim) {
 /* The blocking part of Mesh::Initialize (mesh.cpp:1401-1649): first MG solve, ghost
  * exchange of u and b, ConservedToPrimitive on interior + ghosts, first dt. */
 void orc_initialize(orc_ctx *c) {
+  c->dt = DBL_MAX;
+  orc_initialize_restarted(c);
+}
+
+/* Mesh::Initialize after the restart constructor (mesh.cpp:1401-1651 with the time, dt
+ * and ncycle of the file, mesh.cpp:551-700): the first NewTimeStep is capped by twice
+ * the restored dt (set with orc_set_time first). */
+void orc_initialize_restarted(orc_ctx *c) {
   if (c->d.self_gravity) orc_mg_solve(c, 0);
   orc_bvals_cc(c, ARR_U);
   orc_bvals_fc(c, 0);
   orc_cons2prim(c);
-  c->dt = DBL_MAX;
   orc_new_time_step(c, c->tlim_hint > 0 ? c->tlim_hint : DBL_MAX);
 }
 

@@ -76,6 +76,7 @@ void orc_integrate(orc_ctx *c, int stage);               /* time_integrator.cpp:
 double orc_new_block_dt(orc_ctx *c);                     /* new_blockdt.cpp:42-190 */
 void orc_new_time_step(orc_ctx *c, double tlim);         /* mesh.cpp:1078-1119 */
 void orc_initialize(orc_ctx *c);                         /* mesh.cpp:1367-1660 (subset) */
+void orc_initialize_restarted(orc_ctx *c);               /* the same after a restart file */
 void orc_cycle(orc_ctx *c, double tlim);                 /* main.cpp:430-489 */
 
 /* multigrid gravity */

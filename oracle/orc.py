@@ -72,7 +72,7 @@ def _lib():
     for f in ("orc_bvals_cc", "orc_bvals_fc", "orc_calculate_fluxes", "orc_integrate",
               "orc_mg_solve"):
         getattr(lib, f).argtypes = [P, ctypes.c_int]
-    for f in ("orc_cons2prim", "orc_corner_e", "orc_initialize"):
+    for f in ("orc_cons2prim", "orc_corner_e", "orc_initialize", "orc_initialize_restarted"):
         getattr(lib, f).argtypes = [P]
     lib.orc_new_block_dt.restype = ctypes.c_double
     lib.orc_new_block_dt.argtypes = [P]
@@ -203,8 +203,10 @@ class Oracle:
         self.L.orc_set_time(self.h, t, dt, ncycle)
 
     # reference steps
-    def initialize(self):
-        self.L.orc_initialize(self.h)
+    def initialize(self, restarted=False):
+        """Mesh::Initialize; restarted: after a restart file's time, dt and ncycle were set
+        (set_time), so the first time step is capped by twice that dt"""
+        (self.L.orc_initialize_restarted if restarted else self.L.orc_initialize)(self.h)
 
     def cycle(self, tlim):
         self.L.orc_cycle(self.h, tlim)

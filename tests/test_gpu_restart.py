@@ -4,7 +4,9 @@ mesh and initialised (Mesh::Initialize after the restart constructor: gravity so
 ghost exchange, C2P, NewTimeStep capped by the file's dt) continues exactly as the
 uninterrupted run, bit for bit, FP64, with and without multigrid self-gravity.  The file
 also round-trips through the oracle: its arrays equal the oracle's state after the same
-cycles."""
+cycles.  agx_vtk_write writes the reference's VTK files from the device state."""
+import os
+
 import numpy as np
 import pytest
 
@@ -47,3 +49,35 @@ def test_restart_continues_bitwise(agx, sg, tmp_path):
         assert np.array_equal(b.get(f), a.get(f)), f
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("sg", [0, 1])
+def test_vtk_from_the_device_equals_the_host_writer(agx, sg, tmp_path):
+    """agx_vtk_write (the device state after 2 cycles) writes the same bytes as
+    agx_vtk_write_host on the downloaded arrays, whose format tests/test_vtk_format.py
+    checks against the oracle; and the arrays match the oracle's after the same cycles."""
+    d = make_desc(nx=(32, 16, 16), bnx=(16, 16, 16), box=((0, 2), (0, 1), (0, 1)), cfl=0.4, self_gravity=sg,
+                  four_pi_G=1.0, mg_mode=0, mg_threshold=1e-9 if sg else -1.0)
+    st = turbulent_state(d, seed=5, mach=1.2)
+    m = agx.Mesh(d)
+    load_state(m, st)
+    m.initialize()
+    assert m.cycle(1e30, 2) == 2
+    (tmp_path / "dev").mkdir()
+    (tmp_path / "host").mkdir()
+    m.vtk_write(str(tmp_path / "dev" / "run"), "out3", 4, "prim,cons")
+    t, _, n = m.time()
+    agx.vtk_write_host(str(tmp_path / "host" / "run"), d, t, n, m.get("u"), m.get("w"), m.get("bcc"),
+                       m.get("phi") if sg else None, file_id="out3", file_number=4, variable="prim,cons")
+    names = sorted(os.listdir(tmp_path / "dev"))
+    assert names == sorted(os.listdir(tmp_path / "host")) and len(names) == 2
+    for nm in names:
+        assert (tmp_path / "dev" / nm).read_bytes() == (tmp_path / "host" / nm).read_bytes(), nm
+    o = Oracle(d)
+    load_state(o, st)
+    o.initialize()
+    for _ in range(2):
+        o.cycle(1e30)
+    for f in ("u", "w"):
+        assert np.abs(m.get(f) - o.get(f)).max() <= 1e-12 * np.abs(o.get(f)).max(), f
+    m.close()
