@@ -1011,36 +1011,43 @@ k_mb_add32(double *__restrict__ u, const float *__restrict__ e, int n1, int n2, 
   u[t] += (double)e[t];
 }
 
-// The coarse end of a multipole V(1,1) cycle (vcycle_b) in one workgroup: levels `top`
-// (at most MB_COARSE_CELLS interior cells) down to 0 and back, each step the per-element
-// arithmetic of k_mb_bc, k_mb_fas, k_mb_smooth, k_mb_restrict and k_mb_prolong, separated
-// by workgroup barriers, so the potential equals the per-level launches bit for bit.
-// Every barrier is reached by every thread: loops have workgroup-uniform bounds.
+// The coarse end of a multipole V(1,1) cycle (vcycle_b, and vcycle_b32 in FP32) in one
+// workgroup: levels `top` (at most MB_COARSE_CELLS interior cells) down to 0 and back,
+// each step the per-element arithmetic of k_mb_bc (k_mb_bc_hom for the FP32 correction,
+// HOM), k_mb_fas, k_mb_smooth, k_mb_restrict and k_mb_prolong, separated by workgroup
+// barriers, so the potential equals the per-level launches bit for bit.  Every barrier
+// is reached by every thread: loops have workgroup-uniform bounds.
 constexpr int MB_COARSE_CELLS = 512, MB_MAXC = 8;
+template <typename T>
 struct CoarseB {
-  double *u[MB_MAXC], *src[MB_MAXC], *uold[MB_MAXC];
+  T *u[MB_MAXC], *src[MB_MAXC], *uold[MB_MAXC];
   const double *bcv[MB_MAXC];
   MPGeom G[MB_MAXC];
   double dx[MB_MAXC];
 };
-__device__ __forceinline__ void mbc_bc(const CoarseB &L, int l) {
+template <typename T, bool HOM>
+__device__ __forceinline__ void mbc_bc(const CoarseB<T> &L, int l) {
   const MPGeom &G = L.G[l];
   const unsigned ng = 2u * (G.n1 + 2) * (G.n2 + 2) + 2u * G.n3 * (G.n1 + 2) + 2u * G.n3 * G.n2;
-  double *u = L.u[l];
+  T *u = L.u[l];
   for (unsigned q = threadIdx.x; q < ng; q += COARSE_BS) {
     GhostChain c;
     if (!ghost_chain(G, q, c)) continue;
-    double v = u[BX(G.n1, G.n2, c.kk, c.jj, c.ii)];
-    for (int d = c.depth - 1; d >= 0; --d) v = 2.0 * L.bcv[l][3 * (int64_t)q + d] - v;
+    T v = u[BX(G.n1, G.n2, c.kk, c.jj, c.ii)];
+    for (int d = c.depth - 1; d >= 0; --d) {
+      if (HOM) v = -v;
+      else v = 2.0 * L.bcv[l][3 * (int64_t)q + d] - v;
+    }
     u[BX(G.n1, G.n2, c.k, c.j, c.i)] = v;
   }
   __syncthreads();
 }
-__device__ __forceinline__ void mbc_smooth(const CoarseB &L, int l, int color) {
+template <typename T>
+__device__ __forceinline__ void mbc_smooth(const CoarseB<T> &L, int l, int color) {
   const int n1 = L.G[l].n1, n2 = L.G[l].n2, n3 = L.G[l].n3, nh = (n1 + 1) / 2;
-  double *u = L.u[l];
-  const double *src = L.src[l];
-  const double dx2 = L.dx[l] * L.dx[l], isix = 1.15 / 6.0;
+  T *u = L.u[l];
+  const T *src = L.src[l];
+  const T dx2 = T(L.dx[l] * L.dx[l]), isix = T(1.15 / 6.0);
   for (unsigned t = threadIdx.x; t < (unsigned)nh * n2 * n3; t += COARSE_BS) {
     const int h = t % nh, r = t / nh;
     const int j = 1 + r % n2, k = 1 + r / n2;
@@ -1052,26 +1059,28 @@ __device__ __forceinline__ void mbc_smooth(const CoarseB &L, int l, int color) {
   }
   __syncthreads();
 }
-__device__ __forceinline__ void mbc_fas(const CoarseB &L, int l) {
+template <typename T>
+__device__ __forceinline__ void mbc_fas(const CoarseB<T> &L, int l) {
   const int n1 = L.G[l].n1, n2 = L.G[l].n2, n3 = L.G[l].n3;
   const unsigned w1 = n1 + 2, w2 = n2 + 2, sj = w1, sk = w1 * w2;
-  const double idx2 = 1.0 / (L.dx[l] * L.dx[l]);
-  double *src = L.src[l], *uold = L.uold[l];
-  const double *u = L.u[l];
+  const T idx2 = T(1.0 / (L.dx[l] * L.dx[l]));
+  T *src = L.src[l], *uold = L.uold[l];
+  const T *u = L.u[l];
   for (unsigned t = threadIdx.x; t < w1 * w2 * (unsigned)(n3 + 2); t += COARSE_BS) {
     uold[t] = u[t];
     const int i = t % w1, r = t / w1, j = r % w2, k = r / w2;
     if (i < 1 || i > n1 || j < 1 || j > n2 || k < 1 || k > n3) continue;
-    src[t] -= (6.0 * u[t] - u[t + sk] - u[t + sj] - u[t + 1] - u[t - sk] - u[t - sj] - u[t - 1]) * idx2;
+    src[t] -= (T(6.0) * u[t] - u[t + sk] - u[t + sj] - u[t + 1] - u[t - sk] - u[t - sj] - u[t - 1]) * idx2;
   }
   __syncthreads();
 }
-__device__ __forceinline__ void mbc_restrict(const CoarseB &L, int l) {   // level l -> l-1
+template <typename T>
+__device__ __forceinline__ void mbc_restrict(const CoarseB<T> &L, int l) {   // level l -> l-1
   const int n1 = L.G[l].n1, n2 = L.G[l].n2, n3 = L.G[l].n3;
   const int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
   const unsigned sj = n1 + 2, sk = (unsigned)(n1 + 2) * (n2 + 2);
-  const double idx2 = 1.0 / (L.dx[l] * L.dx[l]);
-  const double *u = L.u[l], *src = L.src[l];
+  const T idx2 = T(1.0 / (L.dx[l] * L.dx[l]));
+  const T *u = L.u[l], *src = L.src[l];
   for (unsigned t = threadIdx.x; t < (unsigned)c1 * c2 * c3; t += COARSE_BS) {
     const int i = 1 + t % c1, r = t / c1, j = 1 + r % c2, k = 1 + r / c2;
     const int fi = 2 * i - 1, fj = 2 * j - 1, fk = 2 * k - 1;
@@ -1079,60 +1088,63 @@ __device__ __forceinline__ void mbc_restrict(const CoarseB &L, int l) {   // lev
     const unsigned f[8] = {F(fk, fj, fi), F(fk, fj, fi + 1), F(fk, fj + 1, fi), F(fk, fj + 1, fi + 1),
                            F(fk + 1, fj, fi), F(fk + 1, fj, fi + 1), F(fk + 1, fj + 1, fi), F(fk + 1, fj + 1, fi + 1)};
 #undef F
-    double d[8];
+    T d[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) d[q] = mb_defect(u, src, f[q], sj, sk, idx2);
     const unsigned c = BX(c1, c2, k, j, i);
-    L.src[l - 1][c] = 0.125 * (d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7]);
-    L.u[l - 1][c] = 0.125 * (u[f[0]] + u[f[1]] + u[f[2]] + u[f[3]] + u[f[4]] + u[f[5]] + u[f[6]] + u[f[7]]);
+    L.src[l - 1][c] = T(0.125) * (d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7]);
+    L.u[l - 1][c] = T(0.125) * (u[f[0]] + u[f[1]] + u[f[2]] + u[f[3]] + u[f[4]] + u[f[5]] + u[f[6]] + u[f[7]]);
   }
   __syncthreads();
 }
-__device__ __forceinline__ void mbc_prolong(const CoarseB &L, int l) {   // level l-1 -> l
+template <typename T>
+__device__ __forceinline__ void mbc_prolong(const CoarseB<T> &L, int l) {   // level l-1 -> l
   const int n1 = L.G[l].n1, n2 = L.G[l].n2, n3 = L.G[l].n3;
   const int c1 = n1 >> 1, c2 = n2 >> 1;
-  const double *cu = L.u[l - 1], *cuold = L.uold[l - 1];
-  double *fu = L.u[l];
+  const T *cu = L.u[l - 1], *cuold = L.uold[l - 1];
+  T *fu = L.u[l];
   for (unsigned t = threadIdx.x; t < (unsigned)n1 * n2 * n3; t += COARSE_BS) {
     const int fi = 1 + t % n1, r = t / n1, fj = 1 + r % n2, fk = 1 + r / n2;
     const int i = (fi - 1) / 2 + 1, j = (fj - 1) / 2 + 1, k = (fk - 1) / 2 + 1;
     const int di = ((fi - 1) & 1) ? 1 : -1, dj = ((fj - 1) & 1) ? 1 : -1, dk = ((fk - 1) & 1) ? 1 : -1;
 #define C(kk, jj, ii) (cu[BX(c1, c2, kk, jj, ii)] - cuold[BX(c1, c2, kk, jj, ii)])
     fu[BX(n1, n2, fk, fj, fi)] +=
-        0.015625 * (27.0 * C(k, j, i) + C(k + dk, j + dj, i + di) + 9.0 * (C(k, j, i + di) + C(k, j + dj, i) + C(k + dk, j, i)) +
-                    3.0 * (C(k + dk, j + dj, i) + C(k + dk, j, i + di) + C(k, j + dj, i + di)));
+        T(0.015625) * (T(27.0) * C(k, j, i) + C(k + dk, j + dj, i + di) +
+                       T(9.0) * (C(k, j, i + di) + C(k, j + dj, i) + C(k + dk, j, i)) +
+                       T(3.0) * (C(k + dk, j + dj, i) + C(k + dk, j, i + di) + C(k, j + dj, i + di)));
 #undef C
   }
   __syncthreads();
 }
 
+template <typename T, bool HOM>
 __global__ void __launch_bounds__(COARSE_BS)
-k_mb_coarse(CoarseB L, int top, int fmglevel, int coffset, int ni) {
+k_mb_coarse(CoarseB<T> L, int top, int fmglevel, int coffset, int ni) {
   for (int l = top; l > 0; --l) {
-    mbc_bc(L, l);
+    mbc_bc<T, HOM>(L, l);
     if (l < fmglevel) mbc_fas(L, l);
     mbc_smooth(L, l, 0 ^ coffset);
-    mbc_bc(L, l);
+    mbc_bc<T, HOM>(L, l);
     mbc_smooth(L, l, 1 ^ coffset);
-    mbc_bc(L, l);
+    mbc_bc<T, HOM>(L, l);
     mbc_restrict(L, l);
   }
-  mbc_bc(L, 0);
+  mbc_bc<T, HOM>(L, 0);
   mbc_fas(L, 0);
   for (int it = 0; it < ni; ++it) {
     mbc_smooth(L, 0, 0 ^ coffset);
-    mbc_bc(L, 0);
+    mbc_bc<T, HOM>(L, 0);
     mbc_smooth(L, 0, 1 ^ coffset);
-    mbc_bc(L, 0);
+    mbc_bc<T, HOM>(L, 0);
   }
   for (int l = 1; l <= top; ++l) {
-    mbc_bc(L, l - 1);
+    mbc_bc<T, HOM>(L, l - 1);
     mbc_prolong(L, l);
-    mbc_bc(L, l);
+    mbc_bc<T, HOM>(L, l);
     mbc_smooth(L, l, 0 ^ coffset);
-    mbc_bc(L, l);
+    mbc_bc<T, HOM>(L, l);
     mbc_smooth(L, l, 1 ^ coffset);
-    mbc_bc(L, l);
+    mbc_bc<T, HOM>(L, l);
   }
 }
 
@@ -1450,7 +1462,7 @@ static int coarse_top_b(const agx_mesh *m) {
 }
 
 static void coarse_b(agx_mesh *m, int top, int ni) {
-  CoarseB c;
+  CoarseB<double> c;
   for (int l = 0; l <= top; ++l) {
     MGLevelB &B = m->levb[l];
     c.u[l] = B.u; c.src[l] = B.src; c.uold[l] = B.uold; c.bcv[l] = B.bcv;
@@ -1458,7 +1470,23 @@ static void coarse_b(agx_mesh *m, int top, int ni) {
     c.dx[l] = B.dx;
   }
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mb_coarse, dim3(1), dim3(COARSE_BS), 0, m->stream, c, top, m->fmglevel, m->coffset, ni);
+  hipLaunchKernelGGL((k_mb_coarse<double, false>), dim3(1), dim3(COARSE_BS), 0, m->stream, c, top, m->fmglevel,
+                     m->coffset, ni);
+  timer_end(m);
+}
+
+// the same for the FP32 correction cycle (homogeneous boundary chain)
+static void coarse_b32(agx_mesh *m, int top, int ni) {
+  CoarseB<float> c;
+  for (int l = 0; l <= top; ++l) {
+    MGLevelB32 &B = m->levb32[l];
+    c.u[l] = B.u; c.src[l] = B.src; c.uold[l] = B.uold; c.bcv[l] = nullptr;
+    c.G[l] = mp_geom(m, m->levb[l]);
+    c.dx[l] = B.dx;
+  }
+  timer_begin(m, AGX_K_MG_OTHER);
+  hipLaunchKernelGGL((k_mb_coarse<float, true>), dim3(1), dim3(COARSE_BS), 0, m->stream, c, top, m->fmglevel,
+                     m->coffset, ni);
   timer_end(m);
 }
 
@@ -1598,7 +1626,8 @@ static void fas_b32(agx_mesh *m, int l) {
 static void vcycle_b32(agx_mesh *m) {   // the per-level sequence of vcycle_b
   const int start = m->current;
   m->coffset ^= 1;
-  while (m->current > 0) {
+  const int ct = coarse_top_b(m);   // levels at or below it in one launch (coarse_b32)
+  while (m->current > 0 && m->current > ct) {
     const int l = m->current;
     MGLevelB32 &F = m->levb32[l], &C = m->levb32[l - 1];
     bc_b32(m, l);
@@ -1615,13 +1644,17 @@ static void vcycle_b32(agx_mesh *m) {   // the per-level sequence of vcycle_b
   }
   const Geo &g = m->g;
   const int ni = std::max(g.nb1, std::max(g.nb2, g.nb3)) >> (m->nrootlevel - 1);
-  bc_b32(m, 0);
-  fas_b32(m, 0);
-  for (int it = 0; it < ni; ++it) {
-    smooth_b32(m, 0, 0);
+  if (ct >= 0) {
+    coarse_b32(m, m->current, ni);
+  } else {
     bc_b32(m, 0);
-    smooth_b32(m, 0, 1);
-    bc_b32(m, 0);
+    fas_b32(m, 0);
+    for (int it = 0; it < ni; ++it) {
+      smooth_b32(m, 0, 0);
+      bc_b32(m, 0);
+      smooth_b32(m, 0, 1);
+      bc_b32(m, 0);
+    }
   }
   while (m->current < start) {
     const int l = m->current + 1;

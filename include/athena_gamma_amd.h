@@ -324,8 +324,7 @@ int agx_set_mg_fuse(agx_mesh *m, int32_t mode);
 /* Multipole V-cycles: run the coarse levels (at most 512 cells each) as one
  * single-workgroup launch (on = 1, the default) or one launch per step (0).  The
  * potentials are identical bit for bit; no counterpart in the reference.  Applies to
- * the FP64 V-cycles (FP64 form, and the FMG of the mixed form); the FP32 correction
- * cycles of the mixed form launch their coarse levels per step. */
+ * the FP64 V-cycles and to the FP32 correction cycles of the mixed form. */
 int agx_set_mg_coarse(agx_mesh *m, int32_t on);
 /* Source mask of MultigridDriver::SetupMultigrid (multigrid_driver.cpp:430-436): after
  * LoadSource, the source is zero at every finest-level cell centre with |x - c|^2 > r^2,

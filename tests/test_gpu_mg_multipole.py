@@ -106,16 +106,18 @@ def test_source_mask_matches(agx, bc):
 
 
 @pytest.mark.parametrize("mode,thr,nit", [(0, -1.0, 3), (0, 0.0, 0), (1, -1.0, 2)])
-def test_coarse_kernel_identical(agx, mode, thr, nit):
+@pytest.mark.parametrize("precision", [0, 1])
+def test_coarse_kernel_identical(agx, mode, thr, nit, precision):
     """the coarse end of the multipole V-cycle in one workgroup (k_mb_coarse) against one
     launch per step: the same potential bit for bit (FMG + fixed cycles, threshold 0,
-    MGI), on a 4x2x2 root grid of 16^3 blocks"""
+    MGI), on a 4x2x2 root grid of 16^3 blocks; FP64, and the mixed form, whose FP32
+    correction cycles use the kernel's homogeneous-boundary FP32 instance"""
     n = (64, 32, 32)
     x = [np.linspace(-0.5, 0.5, m, endpoint=False) + 0.5 / m for m in n]
     Z, Y, X = np.meshgrid(x[2], x[1], x[0], indexing="ij")
     rho = np.exp(-((X - 0.1) ** 2 + Y ** 2 + (Z + 0.05) ** 2) / 0.02)
     d = make_desc(nx=n, bnx=(16, 16, 16), box=((-0.5, 0.5),) * 3, self_gravity=1, mg_bc=1, mg_mode=mode,
-                  four_pi_G=1.0, mporder=4, mg_threshold=thr, mg_niter=nit)
+                  four_pi_G=1.0, mporder=4, mg_threshold=thr, mg_niter=nit, precision=precision)
     U = np.zeros((5, n[2], n[1], n[0]))
     U[0] = rho
     ub = agx.scatter_cells(d, U)
