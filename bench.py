@@ -197,6 +197,23 @@ def measured_traffic(desc, precision, tile):
     return None
 
 
+VALU_PROFILE = os.path.join(ROOT, "profiles", "mhd_turb256_plm_hlld_stage_kernels_valu.txt")
+
+
+def measured_valu_issue(desc, kernel):
+    """Wave-level VALU instructions of the kernel over what 1024 SIMDs issue in its time
+    (one 64-lane instruction per 4 cycles), from the committed rocprofv3 SQ-counter
+    summary of the 256^3 workload; None for other workloads or kernels."""
+    if (desc.nx1, desc.bnx1) != (256, 64) or not os.path.exists(VALU_PROFILE):
+        return None
+    for line in open(VALU_PROFILE):
+        if line.startswith(kernel):
+            f = line[len(kernel):].split()
+            return {"frac": float(f[2]), "insts_valu": float(f[1]), "cycles": float(f[0]),
+                    "source": os.path.relpath(VALU_PROFILE, ROOT) + " (rocprofv3 --pmc SQ_INSTS_VALU, GRBM_GUI_ACTIVE)"}
+    return None
+
+
 def roofline(desc, per, precision, tile=None):
     """Dominant kernel (among those with an algorithmic byte count) and its HBM rate.
     tile: the stage form agx_autotune picked (index into STAGE_FORMS)."""
@@ -215,6 +232,9 @@ def roofline(desc, per, precision, tile=None):
            "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": tr["bytes"] if tr else None,
            "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": alg}
+    vi = measured_valu_issue(desc, out["kernel"])
+    if vi is not None:
+        out["valu_issue"] = vi
     if tr:
         out["traffic_detail"] = {"read_GB": tr["read_GB"], "write_GB": tr["write_GB"], "source": tr["source"],
                                  "traffic_rate_GBs": tr["bytes"] / avg_s / 1e9}
@@ -316,15 +336,22 @@ def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None, sensitivity=False
         bcc = m.get("bcc")[sl].astype(np.float64)
         phi = m.get("phi")[sl].astype(np.float64)
         divb = max_divb(desc, m)
+        vc = m.mg_last()[1]   # V-cycles after FMG of the last solve
         m.timing(True)
         m.timing_reset()
         m.cycle(1e30, 2)
         m.sync()
         per = {c: m.timing_get(c) for c in KCLASSES + ("mg_smooth", "mg_restrict", "mg_prolong", "mg_other")}
         m.timing(False)
+        # whole-cycle byte model: the MHD cycle (680 B FP64 / 552 B mixed per zone) and two
+        # solves, each load + FMG (about 8/7 of a finest-level V-cycle) + vc V-cycles at
+        # 148.6 B (FP64) or 114.3 B (mixed correction cycle) per zone, + retrieve (16 B)
+        key = "mixed" if prec == "mixed" else "fp64"
+        vbytes = 114.3 if key == "mixed" else 148.6
+        bcyc = CYCLE_BYTES[key] + 2 * (24.0 + vbytes * (8.0 / 7.0 + vc) + 16.0)
         res[prec] = dict(zc=float(N) ** 3 * steps / sec, ms=sec * 1e3 / steps, w=w, bcc=bcc, phi=phi, divb=divb,
-                         per=per, tile=tile, bytes=m.device_bytes(),
-                         rf=roofline(desc, per, "mixed" if prec == "mixed" else "fp64", tile))
+                         per=per, tile=tile, bytes=m.device_bytes(), vc=vc, bcyc=bcyc,
+                         rf=roofline(desc, per, key, tile))
         log(f"[{tag}] {prec}: {steps} cycles in {sec:.2f}s -> {res[prec]['zc']:.4e} zone-cycles/s, "
             f"device {m.device_bytes() / 1e9:.1f} GB, stage form {STAGE_FORMS[tile]}; per cycle: "
             + ", ".join(f"{k} {v[0] / 2:.2f} ms" for k, v in per.items() if v[0] > 0))
@@ -338,10 +365,12 @@ def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None, sensitivity=False
                     f"VL2+{'PPM' if xorder == 3 else 'PLM'}+HLLD+CT, FMG + V-cycles to 1e-10 x rms(rho - <rho>) "
                     f"each stage",
         "mixed": {"value": a["zc"], "ms_per_step": a["ms"], "roofline": a["rf"], "max_divb": a["divb"][0],
-                  "max_divb_dx_over_B": a["divb"][1],
+                  "max_divb_dx_over_B": a["divb"][1], "vcycles_per_solve": a["vc"],
+                  "cycle_bytes_per_zone_model": a["bcyc"], "cycle_roofline_frac": a["zc"] * a["bcyc"] / HBM_PEAK,
                   "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in a["per"].items()}},
         "fp64": {"value": b["zc"], "ms_per_step": b["ms"], "roofline": b["rf"], "max_divb": b["divb"][0],
-                 "max_divb_dx_over_B": b["divb"][1],
+                 "max_divb_dx_over_B": b["divb"][1], "vcycles_per_solve": b["vc"],
+                 "cycle_bytes_per_zone_model": b["bcyc"], "cycle_roofline_frac": b["zc"] * b["bcyc"] / HBM_PEAK,
                  "kernel_ms_per_cycle": {k: v[0] / 2 for k, v in b["per"].items()}},
         "vs_fp64": a["zc"] / b["zc"],
         "rel_err_rho": err(a["w"][:, 0], b["w"][:, 0]),
@@ -387,6 +416,8 @@ def large_bench(agx, pgen, args):
         "fp64": {k: r["fp64"][k] for k in ("value", "ms_per_step", "roofline", "max_divb", "max_divb_dx_over_B")},
         "max_divb": a["max_divb"], "max_divb_dx_over_B": a["max_divb_dx_over_B"],
         "kernel_ms_per_cycle": a["kernel_ms_per_cycle"],
+        "cycle_roofline_frac": a["cycle_roofline_frac"],
+        "cycle_bytes_per_zone_model": a["cycle_bytes_per_zone_model"],
     }
     out.update({k: v for k, v in r.items()
                 if k.startswith("rel_") or k in ("vs_fp64", "error_note", "fp64_self_sensitivity")})
