@@ -154,6 +154,8 @@ struct agx_mesh {
   int64_t red_n = 0;
   double *hist = nullptr;            // history partial sums + result (agx_history)
   double *cellvol = nullptr;         // [nb] cell volume dx1*dx2*dx3 of each MeshBlock
+  uint8_t *nearf = nullptr;          // mixed form: per-face near-tie flags of the FP32 PPM fluxes
+  int64_t nearf_n = 0;               // bytes of nearf, nb*S rounded up to the fix-pass span
   double *dtpart = nullptr;          // per-workgroup time-step partials of the fused stage
   int64_t dtpart_n = 0;
   std::vector<agx::BlockCoef> coef_h;

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <type_traits>
 
 #include "agx_internal.hpp"
 
@@ -584,16 +585,34 @@ __device__ __forceinline__ T sgn(T x) { return (x < T(0.0)) ? -T(1.0) : T(1.0); 
 // (reconstruct/ppm.cpp:58-338): CW84 fourth-order interface values with c1..c4 = 1/2,
 // c5 = 1/6, c6 = -1/6 (reconstruction.cpp:429-438), the CS08 extremum-preserving
 // limiter with C2 = 1.25 and the 1e-12 round-off guards.  q[0..4] = values at i-2..i+2.
-// GUARD (FP32 only): *near set when one of the limiter's discontinuous tests -- the sign
-// of (dph - q_im1)(qc - dph) at either interface, of dqf_minus dqf_plus, and the
-// 1e-12 guard on d2qf where it is used -- has an operand within NEAR_TIE x FP32
-// rounding of max|q|, so FP32 arithmetic cannot be trusted to decide it as FP64 would.
+// GUARD (FP32 only): the limiter runs on the stencil minus its centre value (exact where
+// neighbours are within a factor 2, as in smooth or nearly uniform fields), so FP32
+// rounding scales with the stencil's variation rather than its magnitude; the centre
+// value is added back at the end.  The limiter differs from itself under translation only
+// in the 1e-12 guard on d2qf, which keeps the magnitudes of the values themselves.  *near
+// is set when one of the discontinuous tests -- the sign of (dph - q_im1)(qc - dph) at
+// either interface, of dqf_minus dqf_plus, and the d2qf tests (its sign and the 1e-12
+// guard) where they are used -- has an operand within NEAR_TIE x FP32 rounding of the
+// largest |offset value|, so FP32 arithmetic cannot be trusted to decide it as FP64 does
+// on the same inputs.  A constant stencil is exact (every operand is 0 in both).
 constexpr float NEAR_TIE = 64.0f * 5.9604645e-8f;   // 64 x 2^-24
 template <typename T, bool GUARD = false>
 __device__ __forceinline__ void ppm_cell(const T *q, T &qminus, T &qplus, bool *near = nullptr) {
   const T C2 = T(1.25);
   const T c1 = T(0.5), c2 = T(0.5), c3 = T(0.5), c4 = T(0.5), c5 = T(1.0) / T(6.0), c6 = -T(1.0) / T(6.0);
-  const T q_im2 = q[0], q_im1 = q[1], qc = q[2], q_ip1 = q[3], q_ip2 = q[4];
+  T q_im2 = q[0], q_im1 = q[1], qc = q[2], q_ip1 = q[3], q_ip2 = q[4];
+  const T ma = dmax(xabs(q_im1), xabs(q_im2));
+  const T mb = dmax(dmax(xabs(qc), xabs(q_ip1)), xabs(q_ip2));
+  const T t12 = T(1.0e-12) * dmax(ma, mb);   // round-off guard on d2qf
+  T ref = T(0.0);
+  if constexpr (GUARD) {
+    ref = qc;
+    q_im2 -= ref;
+    q_im1 -= ref;
+    q_ip1 -= ref;
+    q_ip2 -= ref;
+    qc = T(0.0);
+  }
   T qa = (qc - q_im1);
   T qb = (q_ip1 - qc);
   T dd_im1 = c1 * qa + c2 * (q_im1 - q_im2);
@@ -604,12 +623,14 @@ __device__ __forceinline__ void ppm_cell(const T *q, T &qminus, T &qplus, bool *
   T d2qc_im1 = q_im2 + qc - T(2.0) * q_im1;
   T d2qc = q_im1 + q_ip1 - T(2.0) * qc;
   T tie = T(0.0);   // GUARD: operands below this decide no discontinuous test
-  if (GUARD) {
-    const T mq = dmax(dmax(dmax(xabs(q_im2), xabs(q_im1)), dmax(xabs(qc), xabs(q_ip1))), xabs(q_ip2));
+  bool vary = false;   // GUARD: the stencil is not constant
+  if constexpr (GUARD) {
+    const T mq = dmax(dmax(xabs(q_im2), xabs(q_im1)), dmax(xabs(q_ip1), xabs(q_ip2)));
+    vary = mq > T(0.0);
     tie = T(NEAR_TIE) * mq;
     // below 1e-12 the products of the sign tests can underflow in FP32: decide in FP64
-    *near = mq < T(1.0e-12) || xabs(dph - q_im1) <= tie || xabs(qc - dph) <= tie ||
-            xabs(dph_ip1 - qc) <= tie || xabs(q_ip1 - dph_ip1) <= tie;
+    *near = vary && (mq < T(1.0e-12) || xabs(dph - q_im1) <= tie || xabs(qc - dph) <= tie ||
+                     xabs(dph_ip1 - qc) <= tie || xabs(q_ip1 - dph_ip1) <= tie);
   }
   T d2qc_ip1 = qc + q_ip2 - T(2.0) * q_ip1;
   {
@@ -638,18 +659,18 @@ __device__ __forceinline__ void ppm_cell(const T *q, T &qminus, T &qplus, bool *
   T dqf_plus = qp - qc;
   T qa_tmp = dqf_minus * dqf_plus;
   T qb_tmp = (q_ip1 - qc) * (qc - q_im1);
-  if (GUARD) {   // (q_ip1 - qc), (qc - q_im1): exact signs from the inputs themselves
-    *near = *near || xabs(dqf_minus) <= tie || xabs(dqf_plus) <= tie ||
-            (qb_tmp == T(0.0) && q_ip1 != qc && qc != q_im1) ||
-            ((qa_tmp <= T(0.0) || qb_tmp <= T(0.0)) && xabs(d2qf) <= tie);
+  if constexpr (GUARD) {   // (q_ip1 - qc), (qc - q_im1): exact signs from the inputs themselves
+    const bool ext = qa_tmp <= T(0.0) || qb_tmp <= T(0.0);
+    *near = *near || (vary && (xabs(dqf_minus) <= tie || xabs(dqf_plus) <= tie ||
+                               (qb_tmp == T(0.0) && q_ip1 != qc && qc != q_im1) ||
+                               (ext && (xabs(d2qf) <= tie ||
+                                        xabs(xabs(d2qf) - t12) <= tie + T(9.5367432e-7) * t12))));
   }
   T qe = T(0.0);
   if (sgn(d2qc_im1) == sgn(d2qc) && sgn(d2qc_im1) == sgn(d2qc_ip1) && sgn(d2qc_im1) == sgn(d2qf))
     qe = sgn(d2qf) * dmin(dmin(C2 * xabs(d2qc_im1), C2 * xabs(d2qc)), dmin(C2 * xabs(d2qc_ip1), xabs(d2qf)));
-  T ma = dmax(xabs(q_im1), xabs(q_im2));
-  T mb = dmax(dmax(xabs(qc), xabs(q_ip1)), xabs(q_ip2));
   T rho = T(0.0);
-  if (xabs(d2qf) > (T(1.0e-12)) * dmax(ma, mb)) rho = qe / d2qf;
+  if (xabs(d2qf) > t12) rho = qe / d2qf;
   T tmp_m = qc - rho * dqf_minus;
   T tmp_p = qc + rho * dqf_plus;
   T tmp2_m = qc - T(2.0) * dqf_plus;
@@ -662,6 +683,10 @@ __device__ __forceinline__ void ppm_cell(const T *q, T &qminus, T &qplus, bool *
   } else {
     if (xabs(dqf_minus) >= T(2.0) * xabs(dqf_plus)) qm = tmp2_m;
     if (xabs(dqf_plus) >= T(2.0) * xabs(dqf_minus)) qp = tmp2_p;
+  }
+  if constexpr (GUARD) {
+    qm += ref;
+    qp += ref;
   }
   qminus = qm;
   qplus = qp;
@@ -803,6 +828,21 @@ __device__ __forceinline__ void ppm_limited(const float *q, float &qminus, float
     ppm_cell(qd, m, p);
     qminus = float(m);
     qplus = float(p);
+  }
+}
+
+// The PPM limiter of face_flux.  FLAG (FP32 only): the guarded FP32 limiter alone, its
+// near-tie flag OR-ed into near, and no FP64 re-run in line -- the caller recomputes a
+// flagged face with ppm_limited in a second pass (k_flux_fix), which gives the same
+// values as ppm_limited in line while keeping the FP64 code out of the main flux kernel.
+template <bool FLAG, typename T>
+__device__ __forceinline__ void ppm_face(const T *q, T &qminus, T &qplus, bool &near) {
+  if constexpr (FLAG) {
+    bool n;
+    ppm_cell<T, true>(q, qminus, qplus, &n);
+    near = near || n;
+  } else {
+    ppm_limited(q, qminus, qplus);
   }
 }
 
@@ -1058,12 +1098,14 @@ __device__ __forceinline__ void roe(T gamma, const T *wli, const T *wri,
 // along dir; pc: the block's PLM factor rows for dir; cidx: the cell index along dir.
 // Outputs: f[0..4] = mass, momentum along dir, dir+1, dir+2, energy flux; ey = -F(By),
 // ez = F(Bz) (hlld.cpp:371-377); wct = GetWeightForCT.
-template <int ORDER, int RS, typename T, bool ISO>
+template <int ORDER, int RS, typename T, bool ISO, bool FLAG = false>
 __device__ __forceinline__ void face_flux(int dir, const T *__restrict__ wb, const T *__restrict__ bb,
                                           int64_t S, int64_t id, int64_t st, int64_t st1, int64_t st2,
                                           const double *__restrict__ pc, int PL, int cidx, T bxi, T gamma, T cs,
-                                          T dfl, T pfl, T dx, T dt, T *f, T &ey, T &ez, T &wct) {
+                                          T dfl, T pfl, T dx, T dt, T *f, T &ey, T &ez, T &wct,
+                                          bool *nearp = nullptr) {
   const int d1 = (dir == 2) ? 0 : dir + 1, d2 = (dir == 0) ? 2 : dir - 1;   // (dir+1)%3, (dir+2)%3
+  bool near = false;   // FLAG: a PPM limiter test of this face was a near-tie (ppm_face)
   T wli[NWAVE], wri[NWAVE];
   if constexpr (ORDER >= ORDER_PLM_CHAR) {
     // limiting in characteristic variables (plm.cpp:59-63,96-99,109-136; ppm.cpp:
@@ -1094,7 +1136,7 @@ __device__ __forceinline__ void face_flux(int dir, const T *__restrict__ wb, con
 #pragma unroll
         for (int n = 0; n < NWC; ++n) {
           T c5[5] = {ch[0][n], ch[1][n], ch[2][n], ch[3][n], ch[4][n]}, qm, qp;
-          ppm_limited(c5, qm, qp);
+          ppm_face<FLAG>(c5, qm, qp, near);
           lim[n] = side ? qm : qp;
         }
         if (NWC < NWAVE) lim[NWAVE - 1] = T(0.0);
@@ -1150,8 +1192,8 @@ __device__ __forceinline__ void face_flux(int dir, const T *__restrict__ wb, con
 #pragma unroll
       for (int t = 0; t < 6; ++t) q6[t] = q[(t - 3) * st];
       T m0, p0, m1, p1;
-      ppm_limited(q6, m0, p0);       // cell c-1: values at c-3..c+1
-      ppm_limited(q6 + 1, m1, p1);   // cell c:   values at c-2..c+2
+      ppm_face<FLAG>(q6, m0, p0, near);       // cell c-1: values at c-3..c+1
+      ppm_face<FLAG>(q6 + 1, m1, p1, near);   // cell c:   values at c-2..c+2
       wli[n] = p0;
       wri[n] = m1;
     } else {
@@ -1210,41 +1252,114 @@ __device__ __forceinline__ void face_flux(int dir, const T *__restrict__ wb, con
   ey = -flxi[IBY];
   ez = flxi[IBZ];
   wct = ct_weight(flxi[IDN], wli[IDN], wri[IDN], dx, dt);
+  if constexpr (FLAG) *nearp = near;
 }
 
-// Hydro::CalculateFluxes for direction DIR, one thread per face (face_flux).
+// FP32 PPM faces (the mixed form) are computed in two passes: k_flux with the guarded FP32
+// limiter only, flagging each face where a limiter test was a near-tie, then k_flux_fix
+// recomputing the flagged faces with the FP64 re-run (ppm_limited).
+template <int ORDER, typename T>
+constexpr bool flux_flagged() {
+  return std::is_same<T, float>::value && (ORDER == 3 || ORDER == ORDER_PPM_CHAR);
+}
+
+// One face of Hydro::CalculateFluxes in direction DIR: face_flux at (b, id), written to
+// the face arrays; FLAG also writes the face's near-tie flag (0 or 1) to nearf.
+template <int DIR, int ORDER, int RS, typename T, bool ISO, bool FLAG>
+__device__ __forceinline__ void flux_face(const Geo &g, int b, int64_t id, int cidx, const T *__restrict__ w,
+                                          const T *__restrict__ bcc, const double *__restrict__ bn,
+                                          T *__restrict__ flx, T *__restrict__ ey, T *__restrict__ ez,
+                                          T *__restrict__ wct, const BlockCoef *__restrict__ coef,
+                                          const double *__restrict__ plm, double gamma_, double cs_,
+                                          const double *__restrict__ dtp, double dfl_, double pfl_,
+                                          uint8_t *__restrict__ nearf) {
+  const int64_t sd[3] = {1, (int64_t)g.P, (int64_t)g.J * g.P};   // strides along i, j, k
+  const double *pc = plm + ((int64_t)b * 3 + DIR) * 2 * g.PL;    // [dxp][dxm] rows
+  T f[5], fey, fez, fw;
+  bool near;
+  face_flux<ORDER, RS, T, ISO, FLAG>(DIR, w + (int64_t)b * NHYDRO * g.S, bcc + (int64_t)b * 3 * g.S, g.S, id,
+                                     sd[DIR], sd[(DIR + 1) % 3], sd[(DIR + 2) % 3], pc, g.PL, cidx,
+                                     T(bn[(int64_t)b * g.S + id]), T(gamma_), T(cs_), T(dfl_), T(pfl_),
+                                     T(coef[b].dx[DIR]), T(*dtp), f, fey, fez, fw, &near);
+  constexpr int ivx = IVX + DIR, ivy = IVX + (DIR + 1) % 3, ivz = IVX + (DIR + 2) % 3;
+  T *fb = flx + (int64_t)b * NHYDRO * g.S + id;
+  fb[IDN * g.S] = f[0];
+  fb[ivx * g.S] = f[1];
+  fb[ivy * g.S] = f[2];
+  fb[ivz * g.S] = f[3];
+  if (!ISO) fb[IEN * g.S] = f[4];
+  int64_t fo = (int64_t)b * g.S + id;
+  ey[fo] = fey;
+  ez[fo] = fez;
+  wct[fo] = fw;
+  if constexpr (FLAG) nearf[fo] = near ? 1 : 0;
+}
+
+// Hydro::CalculateFluxes for direction DIR, one thread per face (face_flux).  For FP32 PPM
+// nearf (one byte per cell of the block arrays) receives each face's near-tie flag.
 template <int DIR, int ORDER, int RS, typename T, bool ISO>
 __global__ void __launch_bounds__(256)
 k_flux(Geo g, const T *__restrict__ w, const T *__restrict__ bcc,
        const double *__restrict__ bn, T *__restrict__ flx, T *__restrict__ ey,
        T *__restrict__ ez, T *__restrict__ wct, const BlockCoef *__restrict__ coef,
        const double *__restrict__ plm, double gamma_, double cs_, const double *__restrict__ dtp, double dfl_,
-       double pfl_) {
+       double pfl_, uint8_t *__restrict__ nearf) {
   int il = g.is - (DIR != 0), iu = g.ie + 1;
   int jl = g.js - (DIR != 1), ju = g.je + 1;
   int kl = g.ks - (DIR != 2);
   PlaneIdx p;
   if (!plane_index(il, iu, jl, ju, kl, p, DIR == 2)) return;   // x3: 4-plane stencil
-  const int64_t sd[3] = {1, (int64_t)g.P, (int64_t)g.J * g.P};   // strides along i, j, k
-  const int64_t id = IDX(g, p.k, p.j, p.i);
   const int cidx = (DIR == 0) ? p.i : (DIR == 1 ? p.j : p.k);   // index of the cell above the face
-  const double *pc = plm + ((int64_t)p.b * 3 + DIR) * 2 * g.PL;  // [dxp][dxm] rows
-  T f[5], fey, fez, fw;
-  face_flux<ORDER, RS, T, ISO>(DIR, w + (int64_t)p.b * NHYDRO * g.S, bcc + (int64_t)p.b * 3 * g.S, g.S, id,
-                               sd[DIR], sd[(DIR + 1) % 3], sd[(DIR + 2) % 3], pc, g.PL, cidx,
-                               T(bn[(int64_t)p.b * g.S + id]), T(gamma_), T(cs_), T(dfl_), T(pfl_),
-                               T(coef[p.b].dx[DIR]), T(*dtp), f, fey, fez, fw);
-  constexpr int ivx = IVX + DIR, ivy = IVX + (DIR + 1) % 3, ivz = IVX + (DIR + 2) % 3;
-  T *fb = flx + (int64_t)p.b * NHYDRO * g.S + id;
-  fb[IDN * g.S] = f[0];
-  fb[ivx * g.S] = f[1];
-  fb[ivy * g.S] = f[2];
-  fb[ivz * g.S] = f[3];
-  if (!ISO) fb[IEN * g.S] = f[4];
-  int64_t fo = (int64_t)p.b * g.S + id;
-  ey[fo] = fey;
-  ez[fo] = fez;
-  wct[fo] = fw;
+  flux_face<DIR, ORDER, RS, T, ISO, flux_flagged<ORDER, T>()>(g, p.b, IDX(g, p.k, p.j, p.i), cidx, w, bcc, bn,
+                                                                flx, ey, ez, wct, coef, plm, gamma_, cs_, dtp,
+                                                                dfl_, pfl_, nearf);
+}
+
+// Second pass of FP32 PPM fluxes: each workgroup scans FIX_SPAN bytes of nearf (16 per
+// thread, one 16-byte load), gathers the flagged faces into an LDS list, clears their
+// flags (so nearf is all zero between passes) and recomputes those faces with the FP64
+// re-run of the limiter, so the near-tie faces occupy whole waves instead of single
+// lanes of every wave of k_flux.  nflag (the length of nearf) is a multiple of FIX_SPAN.
+constexpr int FIX_PER = 16, FIX_SPAN = 256 * FIX_PER;
+template <int DIR, int ORDER, int RS, bool ISO>
+__global__ void __launch_bounds__(256)
+k_flux_fix(Geo g, const float *__restrict__ w, const float *__restrict__ bcc,
+           const double *__restrict__ bn, float *__restrict__ flx, float *__restrict__ ey,
+           float *__restrict__ ez, float *__restrict__ wct, const BlockCoef *__restrict__ coef,
+           const double *__restrict__ plm, double gamma_, double cs_, const double *__restrict__ dtp,
+           double dfl_, double pfl_, uint8_t *__restrict__ nearf, int64_t nflag) {
+  __shared__ int list[FIX_SPAN];
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * FIX_SPAN;
+  const int64_t off = base + (int64_t)threadIdx.x * FIX_PER;
+  if (off < nflag) {
+    uint4 v = *reinterpret_cast<const uint4 *>(nearf + off);
+    if (v.x | v.y | v.z | v.w) {
+      const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int t = 0; t < FIX_PER; ++t)
+        if ((wv[t >> 2] >> (8 * (t & 3))) & 0xffu) list[atomicAdd(&cnt, 1)] = threadIdx.x * FIX_PER + t;
+      *reinterpret_cast<uint4 *>(nearf + off) = make_uint4(0u, 0u, 0u, 0u);
+    }
+  }
+  __syncthreads();
+  const int n = cnt;
+  const int il = g.is - (DIR != 0), jl = g.js - (DIR != 1), kl = g.ks - (DIR != 2);
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const int64_t fo = base + list[e];
+    const int b = (int)(fo / g.S);
+    const int64_t id = fo - (int64_t)b * g.S;
+    const int k = (int)(id / ((int64_t)g.J * g.P));
+    const int r = (int)(id - (int64_t)k * g.J * g.P);
+    const int j = r / g.P, i = r - j * g.P;
+    // only faces of this direction are ever flagged; the test keeps the stencil in range
+    if (b >= g.nb || i < il || i > g.ie + 1 || j < jl || j > g.je + 1 || k < kl || k > g.ke + 1) continue;
+    const int cidx = (DIR == 0) ? i : (DIR == 1 ? j : k);
+    flux_face<DIR, ORDER, RS, float, ISO, false>(g, b, id, cidx, w, bcc, bn, flx, ey, ez, wct, coef, plm, gamma_,
+                                                 cs_, dtp, dfl_, pfl_, nullptr);
+  }
 }
 
 // Field::ComputeCornerE, 3D Newtonian branch (field/calculate_corner_e.cpp:130-229).
@@ -2567,6 +2682,11 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   if (e == hipSuccess) e = hipMalloc(&m->dtpart, sizeof(double) * m->dtpart_n);
   if (e == hipSuccess) e = hipMalloc(&m->hist, sizeof(double) * (HIST_PARTIAL_DOUBLES + HIST_NV));
   if (e == hipSuccess) e = hipMalloc(&m->cellvol, sizeof(double) * g.nb);
+  if (m->mixed && g.ng >= 3) {   // near-tie flags of the FP32 PPM flux pass, all zero between passes
+    m->nearf_n = ((int64_t)g.nb * g.S + FIX_SPAN - 1) / FIX_SPAN * FIX_SPAN;
+    if (e == hipSuccess) e = hipMalloc(&m->nearf, m->nearf_n);
+    if (e == hipSuccess) e = hipMemsetAsync(m->nearf, 0, m->nearf_n, m->stream);
+  }
   // Coordinates::CellVolume, Cartesian: dx1f*dx2f*dx3f (coordinates.cpp:521-531)
   std::vector<double> vol_h(g.nb);
   for (int b = 0; b < g.nb; ++b) vol_h[b] = m->coef_h[b].dx[0] * m->coef_h[b].dx[1] * m->coef_h[b].dx[2];
@@ -2606,6 +2726,7 @@ int agx_mesh_destroy(agx_mesh *m) {
   if (m->dtpart) hipFree(m->dtpart);
   if (m->hist) hipFree(m->hist);
   if (m->cellvol) hipFree(m->cellvol);
+  if (m->nearf) hipFree(m->nearf);
   if (m->stream) hipStreamDestroy(m->stream);
   delete m;
   return 0;
@@ -2892,20 +3013,34 @@ int agx_calculate_fluxes(agx_mesh *m, int order) {
   return rc ? rc : calculate_fluxes(m, order);
 }
 
+extern "C++" {
+// k_flux in direction DIR from the face-centred field B to the face arrays F, EY, EZ, WX;
+// FP32 PPM adds the k_flux_fix pass over the faces k_flux flagged.
+template <int DIR, int ORD, int RS, typename TW, bool ISO>
+static void launch_flux(agx_mesh *m, int B, int F, int EY, int EZ, int WX) {
+  const Geo &g = m->g;
+  double **A = m->arr.a;
+  hipLaunchKernelGGL((k_flux<DIR, ORD, RS, TW, ISO>),
+                     plane_grid(g.is - (DIR != 0), g.ie + 1, g.js - (DIR != 1), g.je + 1, g.ks - (DIR != 2),
+                                g.ke + 1, g.nb),
+                     dim3(256), 0, m->stream, g, as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]), A[B], as<TW>(A[F]),
+                     as<TW>(A[EY]), as<TW>(A[EZ]), as<TW>(A[WX]), m->coef, m->plm, m->d.gamma, m->d.iso_cs,
+                     m->scal + SC_DT, m->d.dfloor, m->d.pfloor, m->nearf);
+  if constexpr (flux_flagged<ORD, TW>()) {
+    hipLaunchKernelGGL((k_flux_fix<DIR, ORD, RS, ISO>), dim3((unsigned)(m->nearf_n / FIX_SPAN)), dim3(256), 0,
+                       m->stream, g, as<float>(A[AGX_ARR_W]), as<float>(A[AGX_ARR_BCC]), A[B], as<float>(A[F]),
+                       as<float>(A[EY]), as<float>(A[EZ]), as<float>(A[WX]), m->coef, m->plm, m->d.gamma,
+                       m->d.iso_cs, m->scal + SC_DT, m->d.dfloor, m->d.pfloor, m->nearf, m->nearf_n);
+  }
+}
+}  // extern "C++"
+
 static int calculate_fluxes(agx_mesh *m, int order) {
   if (order < 1 || order > 3) { set_error("order %d not supported", order); return -2; }
   if (order == 3 && m->g.ng < 3) { set_error("PPM needs nghost >= 3 (reconstruction.cpp:90-100)"); return -3; }
-  const Geo &g = m->g;
-  double **A = m->arr.a;
-  const double gamma = m->d.gamma;
   const int rs = m->d.rsolver;
 #define FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, TW, ISO)                                 \
-  hipLaunchKernelGGL((k_flux<DIR, ORD, RS, TW, ISO>),                                       \
-                     plane_grid(g.is - (DIR != 0), g.ie + 1, g.js - (DIR != 1), g.je + 1,    \
-                                g.ks - (DIR != 2), g.ke + 1, g.nb),                          \
-                     dim3(256), 0, m->stream, g, as<TW>(A[AGX_ARR_W]), as<TW>(A[AGX_ARR_BCC]), \
-                     A[B], as<TW>(A[F]), as<TW>(A[EY]), as<TW>(A[EZ]), as<TW>(A[WX]), m->coef, \
-                     m->plm, gamma, m->d.iso_cs, m->scal + SC_DT, m->d.dfloor, m->d.pfloor)
+  launch_flux<DIR, ORD, RS, TW, ISO>(m, B, F, EY, EZ, WX)
 #define FLUX_LAUNCH_E(DIR, ORD, RS, B, F, EY, EZ, WX, ISO)                                     \
   if (m->mixed) FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, float, ISO);                    \
   else FLUX_LAUNCH_T(DIR, ORD, RS, B, F, EY, EZ, WX, double, ISO)

@@ -21,7 +21,8 @@ VOL = 2.0 * 1.0 * 0.5
 
 
 def mesh(eos=0, sg=1):
-    d = make_desc(nx=(16, 8, 8), bnx=(8, 4, 4), box=BOX, self_gravity=sg, eos=eos)
+    # logically cubic MeshBlocks, as the multigrid hierarchy of the phi column needs
+    d = make_desc(nx=(16, 8, 8), bnx=(4, 4, 4), box=BOX, self_gravity=sg, eos=eos)
     return d, Oracle(d)
 
 
