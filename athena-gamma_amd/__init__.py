@@ -49,7 +49,7 @@ EXPORTS = [
     "agx_vtk_write", "agx_vtk_write_host", "agx_block_gids",
     "agx_mg_setup", "agx_mg_smooth", "agx_mg_fill_ghost", "agx_mg_fas", "agx_mg_restrict", "agx_mg_prolong",
     "agx_mg_fmg_prolong", "agx_mg_solve_coarsest", "agx_mg_defect_norm", "agx_mg_retrieve", "agx_mg_coffset",
-    "agx_mg_levels",
+    "agx_mg_levels", "agx_array_ptr", "agx_mesh_stream", "agx_mesh_set_levels",
 ]
 
 
@@ -96,7 +96,7 @@ class RefItem(ctypes.Structure):
 
 class RefGeom(ctypes.Structure):
     """agx_ref_geom: block interior size and the ghost widths of blocks and coarse buffers."""
-    _fields_ = [(n, ctypes.c_int32) for n in ("nx1", "nx2", "nx3", "ng", "cng", "pad")]
+    _fields_ = [(n, ctypes.c_int32) for n in ("nx1", "nx2", "nx3", "ng", "cng", "layout")]
 
 
 class RefCoords(ctypes.Structure):
@@ -130,6 +130,10 @@ def lib():
     L.agx_array_dims.argtypes = [P, I, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.agx_mesh_bytes.restype = ctypes.c_int64
     L.agx_mesh_bytes.argtypes = [P]
+    L.agx_array_ptr.argtypes = [P, I, ctypes.POINTER(ctypes.c_void_p)]
+    L.agx_mesh_stream.argtypes = [P, ctypes.POINTER(ctypes.c_void_p)]
+    L.agx_mesh_set_levels.argtypes = [P, ctypes.POINTER(ctypes.c_int32), D, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int64)]
     L.agx_upload.argtypes = [P, I, D]
     L.agx_download.argtypes = [P, I, D]
     L.agx_set_time.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32]
@@ -405,6 +409,27 @@ class Mesh:
 
     def device_bytes(self):
         return self.L.agx_mesh_bytes(self.h)
+
+    def ptr(self, name):
+        """device address of a registered array (agx_array_ptr)"""
+        v = ctypes.c_void_p()
+        _check(self.L.agx_array_ptr(self.h, ARR[name], ctypes.byref(v)), "agx_array_ptr")
+        return v.value
+
+    def stream(self):
+        v = ctypes.c_void_p()
+        _check(self.L.agx_mesh_stream(self.h, ctypes.byref(v)), "agx_mesh_stream")
+        return v.value
+
+    def set_levels(self, nrb, box, leaves):
+        """agx_mesh_set_levels: block b becomes the MeshBlock at LogicalLocation leaves[b] =
+        (level, lx1, lx2, lx3) of a root grid of nrb blocks over box = ((x1min, x1max), ...)"""
+        n = len(leaves)
+        lev = (ctypes.c_int32 * n)(*[int(x[0]) for x in leaves])
+        lx = (ctypes.c_int64 * (3 * n))(*[int(v) for x in leaves for v in x[1:]])
+        nr = (ctypes.c_int32 * 3)(*[int(x) for x in nrb])
+        bx = (ctypes.c_double * 6)(*[float(v) for b in box for v in b])
+        _check(self.L.agx_mesh_set_levels(self.h, nr, bx, lev, lx), "agx_mesh_set_levels")
 
     def mg_step(self, name, *args):
         """One per-level multigrid step: agx_mg_<name>(mesh, *args)."""
@@ -713,25 +738,34 @@ class SMRBoundary:
     major: u, w (nb, nhydro, n, n, n), bcc (nb, 3, n, n, n), b = (x1f, x2f, x3f)
     (nb, n, n, n+1) ..., coarse buffers cu, cw, cbcc, cb with nc = nx/2 + 2 ng, edges
     e1 (nb, n+1, n+1, n) ...  Cubic blocks of nx^3 cells, ng ghosts, the adiabatic EOS.
-    Launches are asynchronous on the driver's own stream (agx_stream_create) or, with
-    own_stream=False, on the default stream; sync() waits.  capture(...) records one
-    boundary step (and optionally the EMF correction) with fixed arrays into a hipGraph;
-    replay() launches it (about 100 kernels in one launch)."""
+    Launches are asynchronous on the driver's own stream (agx_stream_create), on
+    `stream` (for instance a mesh's, agx_mesh_stream) or, with own_stream=False, on the
+    default stream; sync() waits.  layout: AGX_REF_LAYOUT_DENSE (0) for the dense arrays
+    above, AGX_REF_LAYOUT_MESH (1) for the block arrays of an agx_mesh (agx_array_ptr).
+    tables: the blocks' and coarse buffers' coordinate tables (fine, coarse), by default
+    smr.block_coords over [0, box).  capture(...) records one boundary step (and
+    optionally the EMF correction) with fixed arrays into a hipGraph; replay() launches it
+    (about 100 kernels in one launch)."""
 
-    def __init__(self, tree, nx, ng, gamma, dfloor, pfloor, box=(1.0, 1.0, 1.0), own_stream=True):
+    def __init__(self, tree, nx, ng, gamma, dfloor, pfloor, box=(1.0, 1.0, 1.0), own_stream=True, stream=None,
+                 layout=0, tables=None):
         smr = _load_smr()
-        self.stream, self.graph = None, None
-        if own_stream:
+        self.stream, self.graph, self.own = None, None, False
+        if stream is not None:
+            self.stream = stream
+        elif own_stream:
             st = ctypes.c_void_p()
             _check(lib().agx_stream_create(ctypes.byref(st)), "agx_stream_create")
-            self.stream = st.value
+            self.stream, self.own = st.value, True
         self.tree, self.nx, self.ng = tree, nx, ng
         self.gamma, self.dfloor, self.pfloor = gamma, dfloor, pfloor
         nb = self.nb = len(tree.leaves)
-        self.geom = RefGeom(nx, nx, nx, ng, ng, 0)
+        self.geom = RefGeom(nx, nx, nx, ng, ng, layout)
         self._keep = []
-        self.fc, a1 = device_coords(smr.block_coords(tree, nx, ng, box=box))
-        self.cc, a2 = device_coords(smr.block_coords(tree, nx, ng, coarse=True, box=box))
+        if tables is None:
+            tables = (smr.block_coords(tree, nx, ng, box=box), smr.block_coords(tree, nx, ng, coarse=True, box=box))
+        self.fc, a1 = device_coords(tables[0])
+        self.cc, a2 = device_coords(tables[1])
         self._keep += a1 + a2
         self._lists = []
         L = self._list
@@ -859,9 +893,120 @@ class SMRBoundary:
         for a in self._keep:
             a.close()
         self._lists, self._keep = [], []
-        if self.stream:
+        if self.stream and self.own:
             lib().agx_stream_destroy(self.stream)
-            self.stream = None
+        self.stream = None
+
+
+class SMRMesh:
+    """MHD on a static-refinement mesh on the device: the leaves of `tree` (smr.SMRTree)
+    as the MeshBlocks of one agx_mesh (agx_mesh_set_levels), cubic blocks of nx^3 cells,
+    ng ghosts, FP64, adiabatic EOS, periodic or block boundaries only.  Each VL2 stage
+    runs the task order of TimeIntegratorTaskList with refinement (time_integrator.cpp:
+    1460-1558):
+      CalculateHydroFlux              agx_calculate_fluxes (donor cell in stage 1)
+      Send/ReceiveAndCorrectHydroFlux flux_from_finer (flux_correction_cc.cpp:77-283)
+      CalculateEMF                    agx_compute_corner_e
+      Send/ReceiveAndCorrectEMF       SMRBoundary.correct_emf (flux_correction_fc.cpp)
+      IntegrateHydro, IntegrateField  agx_integrate
+      Send/Receive/SetBoundaries, Prolongation
+                                      SMRBoundary.boundary_step on the mesh's own arrays
+      Primitives                      agx_cons2prim (interior and ghosts)
+    then time += dt and Mesh::NewTimeStep (main.cpp:459-489, mesh.cpp:1078-1119).  Every
+    launch runs on the mesh's stream.  Coordinates follow the reference's Coordinates on
+    a uniform mesh (smr.mesh_coords)."""
+
+    def __init__(self, tree, nx, ng=2, box=((0.0, 1.0),) * 3, gamma=5.0 / 3.0, cfl=0.3, xorder=2, rsolver=0):
+        smr = _load_smr()
+        self.tree, self.nx, self.ng, self.xorder = tree, nx, ng, xorder
+        nb = self.nb = len(tree.leaves)
+        desc = make_desc(nx=(nb * nx, nx, nx), bnx=(nx, nx, nx), box=((0.0, float(nb)), (0.0, 1.0), (0.0, 1.0)),
+                         nghost=ng, xorder=xorder, rsolver=rsolver, gamma=gamma, cfl=cfl)
+        self.desc = desc
+        self.m = Mesh(desc)
+        self._keep = []
+        try:
+            self.m.set_levels(tree.nrb, box, tree.leaves)
+            tabs = (smr.mesh_coords(tree, nx, ng, box), smr.mesh_coords(tree, nx, ng, box, coarse=True))
+            self.tables = tabs
+            self.bnd = SMRBoundary(tree, nx, ng, gamma, desc.dfloor, desc.pfloor, stream=self.m.stream(), layout=1,
+                                   tables=tabs)
+            plan = smr.flux_plan(tree)
+            self.flux = RefList(self.bnd.geom, "flux_from_finer", [RefItem(**d) for d in plan], nb, nb,
+                                stream=self.m.stream()) if plan else None
+            nc = nx // 2 + 2 * ng
+            self.cu = DeviceArray(np.zeros((nb, 5, nc, nc, nc)))
+            self.cw = DeviceArray(np.zeros((nb, 5, nc, nc, nc)))
+            self.cbcc = DeviceArray(np.zeros((nb, 3, nc, nc, nc)))
+            self.cb = [DeviceArray(np.zeros(sh)) for sh in ((nb, nc, nc, nc + 1), (nb, nc, nc + 1, nc),
+                                                              (nb, nc + 1, nc, nc))]
+            self._keep += [self.cu, self.cw, self.cbcc] + self.cb
+        except Exception:
+            self.close()
+            raise
+
+    def set(self, name, arr):
+        self.m.set(name, arr)
+
+    def get(self, name):
+        return self.m.get(name)
+
+    def time(self):
+        return self.m.time()
+
+    def sync(self):
+        self.m.sync()
+
+    def boundary(self):
+        """ghost zones of u and b across levels, then ProlongateBoundaries"""
+        p = self.m.ptr
+        self.bnd.boundary_step(p("u"), [p("b1f"), p("b2f"), p("b3f")], p("w"), p("bcc"), self.cu.ptr,
+                               [x.ptr for x in self.cb], self.cw.ptr, self.cbcc.ptr)
+
+    def initialize(self, tlim=1e300):
+        """Mesh::Initialize after the problem generator: ghost zones, primitives, first dt"""
+        self.boundary()
+        self.m.cons2prim()
+        t, _, n = self.m.time()
+        self.m.set_time(t, 1.7976931348623157e308, n)   # a fresh Mesh starts from dt = DBL_MAX
+        self.m.new_time_step(tlim)
+
+    def stage(self, s):
+        m, p = self.m, self.m.ptr
+        m.calculate_fluxes(1 if s == 1 else self.xorder)
+        if self.flux is not None:
+            f = [p("fx1"), p("fx2"), p("fx3")]
+            self.flux.flux_from_finer(5, *f, self.bnd.fc, *f)
+        m.corner_e()
+        self.bnd.correct_emf(p("e1"), p("e2"), p("e3"))
+        m.integrate(s)
+        self.boundary()
+        m.cons2prim()
+
+    def cycle(self, tlim, n=1):
+        """up to n cycles while time < tlim; returns the number run"""
+        done = 0
+        while done < n:
+            t, dt, nc = self.m.time()
+            if t >= tlim:
+                break
+            self.stage(1)
+            self.stage(2)
+            self.m.set_time(t + dt, dt, nc + 1)
+            self.m.new_time_step(tlim)
+            done += 1
+        return done
+
+    def close(self):
+        for x in (getattr(self, "flux", None), getattr(self, "bnd", None)):
+            if x is not None:
+                x.close()
+        for a in getattr(self, "_keep", []):
+            a.close()
+        self._keep, self.flux, self.bnd = [], None, None
+        if getattr(self, "m", None) is not None:
+            self.m.close()
+            self.m = None
 
 
 def _box_size(d):

@@ -163,6 +163,7 @@ struct agx_mesh {
   double time = 0.0, dt = 0.0;
   int32_t ncycle = 0;
   bool restarted = false;            // dt came from a restart file: agx_initialize keeps it
+  bool tree = false;                 // blocks on refinement levels (agx_mesh_set_levels)
   // multigrid
   int mg_on = 0;
   int nrootlevel = 0, nmblevel = 0, ntotal = 0, current = 0, fmglevel = 0, coffset = 0;
@@ -216,6 +217,9 @@ void block_coords(int nmesh, int nblk, int nbx, int ng, int lx, double xmin, dou
 std::vector<int64_t> gid_lattice_order(int nb1, int nb2, int nb3);
 
 void set_error(const char *fmt, ...);
+// entry points built on the lattice of one level (its ghost exchange, file layouts) refuse
+// a mesh with refinement levels
+int refuse_tree(const agx_mesh *m, const char *what);
 int hip_fail(hipError_t e, const char *what);
 
 // history sums over dense AthenaArray inputs (agx_diag.hip)

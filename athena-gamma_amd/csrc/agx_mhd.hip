@@ -35,6 +35,13 @@ void set_error(const char *fmt, ...) {
   g_err = buf;
 }
 
+int refuse_tree(const agx_mesh *m, const char *what) {
+  if (!m->tree) return 0;
+  set_error("%s: the mesh has refinement levels; it runs through the task-level entry points and the "
+            "refinement operators", what);
+  return -60;
+}
+
 int hip_fail(hipError_t e, const char *what) {
   set_error("HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
   return (int)e > 0 ? (int)e : 1;
@@ -2582,6 +2589,28 @@ static int64_t arr_bytes(int nv, const Geo &g, int esz) {
   return (b + 255) / 256 * 256;
 }
 
+// Host tables of block b at logical location lx of a lattice of nblk blocks over nmesh
+// cells between xmin and xmax per direction (Mesh::SetBlockSizeAndBoundaries on a uniform
+// mesh, mesh.cpp:1668-1690): its cell width and the PLM face-offset factors
+// (reconstruct/plm.cpp:116-117,224-225,330-331).
+static void set_block_geometry(agx_mesh *m, int b, const int nmesh[3], const int nblk[3], const int lx[3],
+                               const double xmin[3], const double xmax[3]) {
+  const Geo &g = m->g;
+  const int nbx[3] = {g.bx1, g.bx2, g.bx3};
+  for (int dd = 0; dd < 3; ++dd) {
+    std::vector<double> xf, xv;
+    double dx;
+    block_coords(nmesh[dd], nblk[dd], nbx[dd], g.ng, lx[dd], xmin[dd], xmax[dd], xf, xv, dx);
+    m->coef_h[b].dx[dd] = dx;
+    double *pp = &m->plm_h[((size_t)b * 3 + dd) * 2 * g.PL];
+    const int n = nbx[dd] + 2 * g.ng;
+    for (int i = 0; i < n; ++i) {
+      pp[i] = (xf[i + 1] - xv[i]) / dx;
+      pp[g.PL + i] = (xv[i] - xf[i]) / dx;
+    }
+  }
+}
+
 int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   if (!out) { set_error("null output pointer"); return -1; }
   *out = nullptr;
@@ -2645,25 +2674,13 @@ int agx_mesh_create(const agx_mesh_desc *desc, agx_mesh **out) {
   // per-block geometry
   m->coef_h.resize(g.nb);
   m->plm_h.assign((size_t)g.nb * 3 * 2 * g.PL, 0.0);
-  for (int b = 0; b < g.nb; ++b) {
-    int lx[3] = {b % g.nb1, (b / g.nb1) % g.nb2, b / (g.nb1 * g.nb2)};
-    int nmesh[3] = {desc->nx1, desc->nx2, desc->nx3};
-    int nblk[3] = {g.nb1, g.nb2, g.nb3};
-    int nbx[3] = {g.bx1, g.bx2, g.bx3};
-    double xmin[3] = {desc->x1min, desc->x2min, desc->x3min};
-    double xmax[3] = {desc->x1max, desc->x2max, desc->x3max};
-    for (int dd = 0; dd < 3; ++dd) {
-      std::vector<double> xf, xv;
-      double dx;
-      block_coords(nmesh[dd], nblk[dd], nbx[dd], g.ng, lx[dd], xmin[dd], xmax[dd], xf, xv, dx);
-      m->coef_h[b].dx[dd] = dx;
-      double *pp = &m->plm_h[((size_t)b * 3 + dd) * 2 * g.PL];
-      int n = nbx[dd] + 2 * g.ng;
-      // PLM face-offset factors, reconstruct/plm.cpp:116-117,224-225,330-331
-      for (int i = 0; i < n; ++i) {
-        pp[i] = (xf[i + 1] - xv[i]) / dx;
-        pp[g.PL + i] = (xv[i] - xf[i]) / dx;
-      }
+  {
+    const int nmesh[3] = {desc->nx1, desc->nx2, desc->nx3}, nblk[3] = {g.nb1, g.nb2, g.nb3};
+    const double xmin[3] = {desc->x1min, desc->x2min, desc->x3min};
+    const double xmax[3] = {desc->x1max, desc->x2max, desc->x3max};
+    for (int b = 0; b < g.nb; ++b) {
+      const int lx[3] = {b % g.nb1, (b / g.nb1) % g.nb2, b / (g.nb1 * g.nb2)};
+      set_block_geometry(m, b, nmesh, nblk, lx, xmin, xmax);
     }
   }
   e = hipMalloc(&m->coef, sizeof(BlockCoef) * g.nb);
@@ -2767,6 +2784,57 @@ int agx_array_dims(agx_mesh *m, int arr, int64_t *nblocks, int64_t dims[4]) {
 }
 
 int64_t agx_mesh_bytes(agx_mesh *m) { return m ? m->pool_bytes + m->mg_bytes : 0; }
+
+int agx_array_ptr(agx_mesh *m, int arr, void **ptr) {
+  if (!m || arr < 0 || arr >= AGX_ARR_COUNT || !ptr) { set_error("agx_array_ptr: bad argument"); return -1; }
+  *ptr = m->arr.a[arr];
+  return 0;
+}
+
+int agx_mesh_stream(agx_mesh *m, void **stream) {
+  if (!m || !stream) { set_error("agx_mesh_stream: bad argument"); return -1; }
+  *stream = m->stream;
+  return 0;
+}
+
+int agx_mesh_set_levels(agx_mesh *m, const int32_t nrb[3], const double box[6], const int32_t *level,
+                        const int64_t *lx) {
+  if (!m || !nrb || !box || !level || !lx) { set_error("agx_mesh_set_levels: null argument"); return -1; }
+  if (m->mixed) { set_error("agx_mesh_set_levels: the refined mesh runs the task-level FP64 steps only"); return -2; }
+  if (m->d.self_gravity) { set_error("agx_mesh_set_levels: multigrid on refined meshes is not built"); return -3; }
+  const Geo &g = m->g;
+  const int nbx[3] = {g.bx1, g.bx2, g.bx3};
+  for (int d = 0; d < 3; ++d) {
+    if (nrb[d] < 1 || nbx[d] % 2) { set_error("agx_mesh_set_levels: root blocks >= 1 and even block sizes"); return -4; }
+    if (!(box[2 * d + 1] > box[2 * d])) { set_error("agx_mesh_set_levels: empty box"); return -5; }
+  }
+  for (int b = 0; b < g.nb; ++b) {
+    if (level[b] < 0 || level[b] > 20) { set_error("agx_mesh_set_levels: block %d: level %d", b, level[b]); return -6; }
+    for (int d = 0; d < 3; ++d)
+      if (lx[3 * b + d] < 0 || lx[3 * b + d] >= ((int64_t)nrb[d] << level[b])) {
+        set_error("agx_mesh_set_levels: block %d: location outside the level's lattice", b);
+        return -7;
+      }
+  }
+  const double xmin[3] = {box[0], box[2], box[4]}, xmax[3] = {box[1], box[3], box[5]};
+  for (int b = 0; b < g.nb; ++b) {
+    const int L = level[b];
+    const int nmesh[3] = {(nrb[0] * nbx[0]) << L, (nrb[1] * nbx[1]) << L, (nrb[2] * nbx[2]) << L};
+    const int nblk[3] = {nrb[0] << L, nrb[1] << L, nrb[2] << L};
+    const int lxb[3] = {(int)lx[3 * b], (int)lx[3 * b + 1], (int)lx[3 * b + 2]};
+    set_block_geometry(m, b, nmesh, nblk, lxb, xmin, xmax);
+  }
+  std::vector<double> vol_h(g.nb);
+  for (int b = 0; b < g.nb; ++b) vol_h[b] = m->coef_h[b].dx[0] * m->coef_h[b].dx[1] * m->coef_h[b].dx[2];
+  AGX_HIP(hipMemcpyAsync(m->coef, m->coef_h.data(), sizeof(BlockCoef) * g.nb, hipMemcpyHostToDevice, m->stream));
+  AGX_HIP(hipMemcpyAsync(m->plm, m->plm_h.data(), sizeof(double) * m->plm_h.size(), hipMemcpyHostToDevice,
+                         m->stream));
+  AGX_HIP(hipMemcpyAsync(m->cellvol, vol_h.data(), sizeof(double) * g.nb, hipMemcpyHostToDevice, m->stream));
+  AGX_HIP(hipStreamSynchronize(m->stream));
+  m->tree = true;
+  graph_reset(m);
+  return 0;
+}
 
 // Strided 3D copies between the tight host layout and the padded device layout.  FP32
 // arrays of the mixed-precision form are staged through a host buffer and converted.
@@ -3139,6 +3207,7 @@ int agx_integrate(agx_mesh *m, int stage) {
 
 int agx_bvals_exchange(agx_mesh *m, int which) {
   if (!m) { set_error("null mesh"); return -1; }
+  if (int rc = refuse_tree(m, "agx_bvals_exchange")) return rc;
   const Geo &g = m->g;
   double **A = m->arr.a;
   timer_begin(m, AGX_K_BVALS);
@@ -3248,6 +3317,7 @@ int agx_new_time_step(agx_mesh *m, double tlim) {
 
 int agx_initialize(agx_mesh *m, double tlim) {
   if (!m) { set_error("null mesh"); return -1; }
+  if (int rc = refuse_tree(m, "agx_initialize")) return rc;
   int rc = 0;
   if (m->d.self_gravity) rc = mg_solve(m, 0);          // mesh.cpp:1401-1402
   if (!rc) rc = agx_bvals_exchange(m, 3);               // mesh.cpp:1404-1453
@@ -3535,6 +3605,7 @@ int agx_set_graph(agx_mesh *m, int32_t enable) {
 
 int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
   if (!m) { set_error("null mesh"); return -1; }
+  if (int rc = refuse_tree(m, "agx_cycle")) return rc;
   if (m->graph_on && !m->timer.enabled && !m->d.self_gravity) return cycle_graphs(m, tlim, ncycles, done);
   int32_t n = 0;
   int rc = 0;
@@ -3598,6 +3669,7 @@ int agx_mg_vcycles(agx_mesh *m, int32_t n) {
 // fastest.  Call between cycles.
 int agx_autotune(agx_mesh *m, int32_t *choice) {
   if (!m) { set_error("null mesh"); return -1; }
+  if (int rc = refuse_tree(m, "agx_autotune")) return rc;
   hipEvent_t e0, e1;
   AGX_HIP(hipEventCreate(&e0));
   AGX_HIP(hipEventCreate(&e1));

@@ -45,6 +45,8 @@ struct RG {            // derived geometry (see agx_ref_geom)
   int is, js, ks, ie, je, ke;
   int cis, cjs, cks;
   int h1, h2, h3;      // nx / 2
+  int64_t bS;          // block arrays in the mesh's padded layout: slab per variable (0: dense)
+  int bJ, bP;          // rows per plane and row pitch of that layout
 };
 
 RG make_rg(const agx_ref_geom &g) {
@@ -55,12 +57,37 @@ RG make_rg(const agx_ref_geom &g) {
   r.ie = g.ng + g.nx1 - 1; r.je = g.ng + g.nx2 - 1; r.ke = g.ng + g.nx3 - 1;
   r.cis = r.cjs = r.cks = g.cng;
   r.h1 = g.nx1 / 2; r.h2 = g.nx2 / 2; r.h3 = g.nx3 / 2;
+  r.bS = 0;
+  r.bJ = r.bP = 0;
+  if (g.layout == AGX_REF_LAYOUT_MESH) {   // as agx_mesh_create lays out its arrays
+    r.bP = ((r.n1 + 1) + 31) / 32 * 32;
+    r.bJ = r.n2 + 1;
+    r.bS = (int64_t)(r.n3 + 1) * r.bJ * r.bP;
+  }
   return r;
 }
 
 // element (b, n, k, j, i) of a block-major array of nvar variables of d3 x d2 x d1
 __device__ __forceinline__ int64_t ix(int d3, int d2, int d1, int nvar, int b, int n, int k, int j, int i) {
   return ((((int64_t)b * nvar + n) * d3 + k) * d2 + j) * (int64_t)d1 + i;
+}
+// the same element of a block-side array (MeshBlock data, not a coarse buffer): the dense
+// layout above, or the mesh's padded one, where every cell, face and edge array of a
+// block has nvar slabs of bS elements, bJ rows of bP elements per plane
+__device__ __forceinline__ int64_t bx(const RG &r, int d3, int d2, int d1, int nvar, int b, int n, int k, int j,
+                                      int i) {
+  if (r.bS) return ((int64_t)b * nvar + n) * r.bS + ((int64_t)k * r.bJ + j) * r.bP + i;
+  return ix(d3, d2, d1, nvar, b, n, k, j, i);
+}
+// an array of either side, for the operators that move data both ways
+struct Lay {
+  int d1, d2, d3;      // dense dims
+  int64_t S;           // > 0: padded layout (slab S, J rows of P per plane)
+  int J, P;
+};
+__device__ __forceinline__ int64_t lx(const Lay &L, int nvar, int b, int n, int k, int j, int i) {
+  if (L.S) return ((int64_t)b * nvar + n) * L.S + ((int64_t)k * L.J + j) * L.P + i;
+  return ix(L.d3, L.d2, L.d1, nvar, b, n, k, j, i);
 }
 
 __device__ __forceinline__ double ref_sign(double x) { return (x < 0.0) ? -1.0 : 1.0; }   // SIGN
@@ -106,7 +133,7 @@ b_ref_restrict_cc(RG r, int nvar, const double *__restrict__ fine, agx_ref_coord
         for (int c = 0; c < 2; ++c) v[a][b][c] = d1[i + c] * d2[j + b] * d3[k + a];
     const double tvol = ((v[0][0][0] + v[0][1][0]) + (v[0][0][1] + v[0][1][1])) +
                         ((v[1][0][0] + v[1][1][0]) + (v[1][0][1] + v[1][1][1]));
-    auto F = [&](int a, int b, int c) { return fine[ix(r.n3, r.n2, r.n1, nvar, it.fb, n, k + a, j + b, i + c)]; };
+    auto F = [&](int a, int b, int c) { return fine[bx(r, r.n3, r.n2, r.n1, nvar, it.fb, n, k + a, j + b, i + c)]; };
     const double sum = (((F(0, 0, 0) * v[0][0][0] + F(0, 1, 0) * v[0][1][0]) +
                          (F(0, 0, 1) * v[0][0][1] + F(0, 1, 1) * v[0][1][1])) +
                         ((F(1, 0, 0) * v[1][0][0] + F(1, 1, 0) * v[1][1][0]) +
@@ -137,7 +164,7 @@ b_ref_restrict_fc(RG r, const double *__restrict__ fine, agx_ref_coords fc, doub
     const unsigned s = t / (unsigned)q.w1;
     const int cj = q.sj + (int)(s % (unsigned)q.w2), ck = q.sk + (int)(s / (unsigned)q.w2);
     const int i = (ci - r.cis) * 2 + r.is, j = (cj - r.cjs) * 2 + r.js, k = (ck - r.cks) * 2 + r.ks;
-    auto F = [&](int a, int b, int c) { return fine[ix(f3, f2, f1, 1, it.fb, 0, k + a, j + b, i + c)]; };
+    auto F = [&](int a, int b, int c) { return fine[bx(r, f3, f2, f1, 1, it.fb, 0, k + a, j + b, i + c)]; };
     double val;
     if (DIR == 0) {   // Face1Area = dx2f(j)*dx3f(k)
       const double a00 = d2[j] * d3[k], a01 = d2[j + 1] * d3[k];
@@ -196,7 +223,7 @@ b_ref_prolong_cc(RG r, int nvar, const double *__restrict__ coarse, agx_ref_coor
     const double g2 = minmod_slope((cv - C(0, -1, 0)) / dx2m, (C(0, 1, 0) - cv) / dx2p);
     const double g3 = minmod_slope((cv - C(-1, 0, 0)) / dx3m, (C(1, 0, 0) - cv) / dx3p);
     auto FO = [&](int a, int b, int c) -> double & {
-      return fine[ix(r.n3, r.n2, r.n1, nvar, it.fb, n, fk + a, fj + b, fi + c)];
+      return fine[bx(r, r.n3, r.n2, r.n1, nvar, it.fb, n, fk + a, fj + b, fi + c)];
     };
     FO(0, 0, 0) = cv - (g1 * dx1fm + g2 * dx2fm + g3 * dx3fm);
     FO(0, 0, 1) = cv + (g1 * dx1fp - g2 * dx2fm - g3 * dx3fm);
@@ -234,7 +261,7 @@ b_ref_prolong_fc_shared(RG r, const double *__restrict__ coarse, agx_ref_coords 
     const int j = q.sj + (int)(s % (unsigned)q.w2), k = q.sk + (int)(s / (unsigned)q.w2);
     const int fi = (i - r.cis) * 2 + r.is, fj = (j - r.cjs) * 2 + r.js, fk = (k - r.cks) * 2 + r.ks;
     auto C = [&](int a, int b, int c) { return coarse[ix(q3, q2, q1, 1, it.cb, 0, k + a, j + b, i + c)]; };
-    auto FO = [&](int a, int b, int c) -> double & { return fine[ix(f3, f2, f1, 1, it.fb, 0, fk + a, fj + b, fi + c)]; };
+    auto FO = [&](int a, int b, int c) -> double & { return fine[bx(r, f3, f2, f1, 1, it.fb, 0, fk + a, fj + b, fi + c)]; };
     const double cv = C(0, 0, 0);
     if (DIR == 0) {
       const double g2 = minmod_slope((cv - C(0, -1, 0)) / (X2[j] - X2[j - 1]), (C(0, 1, 0) - cv) / (X2[j + 1] - X2[j]));
@@ -277,9 +304,9 @@ internal_group(const RG &r, double *__restrict__ b1, double *__restrict__ b2, do
                const double *__restrict__ d1, const double *__restrict__ d2, const double *__restrict__ d3, int fb,
                int i, int j, int k) {
     const int fi = (i - r.cis) * 2 + r.is, fj = (j - r.cjs) * 2 + r.js, fk = (k - r.cks) * 2 + r.ks;
-    auto B1 = [&](int kk, int jj, int ii) -> double & { return b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, kk, jj, ii)]; };
-    auto B2 = [&](int kk, int jj, int ii) -> double & { return b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, kk, jj, ii)]; };
-    auto B3 = [&](int kk, int jj, int ii) -> double & { return b3[ix(r.n3 + 1, r.n2, r.n1, 1, fb, 0, kk, jj, ii)]; };
+    auto B1 = [&](int kk, int jj, int ii) -> double & { return b1[bx(r, r.n3, r.n2, r.n1 + 1, 1, fb, 0, kk, jj, ii)]; };
+    auto B2 = [&](int kk, int jj, int ii) -> double & { return b2[bx(r, r.n3, r.n2 + 1, r.n1, 1, fb, 0, kk, jj, ii)]; };
+    auto B3 = [&](int kk, int jj, int ii) -> double & { return b3[bx(r, r.n3 + 1, r.n2, r.n1, 1, fb, 0, kk, jj, ii)]; };
     // Face?Area(fk+a, fj+b, ii)
     auto A1 = [&](int a, int b) { return d2[fj + b] * d3[fk + a]; };
     auto A2 = [&](int a, int ii) { return d1[ii] * d3[fk + a]; };
@@ -370,26 +397,26 @@ p2c_cell(const RG &r, const double *__restrict__ w, const double *__restrict__ b
          const double *__restrict__ b3, double *__restrict__ bcc, double *__restrict__ u, double igm1, int fb, int i,
          int j, int k) {
   constexpr int NH = ISO ? 4 : 5;
-    const double bc1 = 0.5 * b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, k, j, i)] +
-                       0.5 * b1[ix(r.n3, r.n2, r.n1 + 1, 1, fb, 0, k, j, i + 1)];
-    const double bc2 = 0.5 * b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, k, j, i)] +
-                       0.5 * b2[ix(r.n3, r.n2 + 1, r.n1, 1, fb, 0, k, j + 1, i)];
-    const double bc3 = 0.5 * b3[ix(r.n3 + 1, r.n2, r.n1, 1, fb, 0, k, j, i)] +
-                       0.5 * b3[ix(r.n3 + 1, r.n2, r.n1, 1, fb, 0, k + 1, j, i)];
-    bcc[ix(r.n3, r.n2, r.n1, 3, fb, 0, k, j, i)] = bc1;
-    bcc[ix(r.n3, r.n2, r.n1, 3, fb, 1, k, j, i)] = bc2;
-    bcc[ix(r.n3, r.n2, r.n1, 3, fb, 2, k, j, i)] = bc3;
-    const double w_d = w[ix(r.n3, r.n2, r.n1, NH, fb, 0, k, j, i)];
-    const double w_vx = w[ix(r.n3, r.n2, r.n1, NH, fb, 1, k, j, i)];
-    const double w_vy = w[ix(r.n3, r.n2, r.n1, NH, fb, 2, k, j, i)];
-    const double w_vz = w[ix(r.n3, r.n2, r.n1, NH, fb, 3, k, j, i)];
-    u[ix(r.n3, r.n2, r.n1, NH, fb, 0, k, j, i)] = w_d;
-    u[ix(r.n3, r.n2, r.n1, NH, fb, 1, k, j, i)] = w_vx * w_d;
-    u[ix(r.n3, r.n2, r.n1, NH, fb, 2, k, j, i)] = w_vy * w_d;
-    u[ix(r.n3, r.n2, r.n1, NH, fb, 3, k, j, i)] = w_vz * w_d;
+    const double bc1 = 0.5 * b1[bx(r, r.n3, r.n2, r.n1 + 1, 1, fb, 0, k, j, i)] +
+                       0.5 * b1[bx(r, r.n3, r.n2, r.n1 + 1, 1, fb, 0, k, j, i + 1)];
+    const double bc2 = 0.5 * b2[bx(r, r.n3, r.n2 + 1, r.n1, 1, fb, 0, k, j, i)] +
+                       0.5 * b2[bx(r, r.n3, r.n2 + 1, r.n1, 1, fb, 0, k, j + 1, i)];
+    const double bc3 = 0.5 * b3[bx(r, r.n3 + 1, r.n2, r.n1, 1, fb, 0, k, j, i)] +
+                       0.5 * b3[bx(r, r.n3 + 1, r.n2, r.n1, 1, fb, 0, k + 1, j, i)];
+    bcc[bx(r, r.n3, r.n2, r.n1, 3, fb, 0, k, j, i)] = bc1;
+    bcc[bx(r, r.n3, r.n2, r.n1, 3, fb, 1, k, j, i)] = bc2;
+    bcc[bx(r, r.n3, r.n2, r.n1, 3, fb, 2, k, j, i)] = bc3;
+    const double w_d = w[bx(r, r.n3, r.n2, r.n1, NH, fb, 0, k, j, i)];
+    const double w_vx = w[bx(r, r.n3, r.n2, r.n1, NH, fb, 1, k, j, i)];
+    const double w_vy = w[bx(r, r.n3, r.n2, r.n1, NH, fb, 2, k, j, i)];
+    const double w_vz = w[bx(r, r.n3, r.n2, r.n1, NH, fb, 3, k, j, i)];
+    u[bx(r, r.n3, r.n2, r.n1, NH, fb, 0, k, j, i)] = w_d;
+    u[bx(r, r.n3, r.n2, r.n1, NH, fb, 1, k, j, i)] = w_vx * w_d;
+    u[bx(r, r.n3, r.n2, r.n1, NH, fb, 2, k, j, i)] = w_vy * w_d;
+    u[bx(r, r.n3, r.n2, r.n1, NH, fb, 3, k, j, i)] = w_vz * w_d;
     if (!ISO) {
-      const double w_p = w[ix(r.n3, r.n2, r.n1, NH, fb, 4, k, j, i)];
-      u[ix(r.n3, r.n2, r.n1, NH, fb, 4, k, j, i)] =
+      const double w_p = w[bx(r, r.n3, r.n2, r.n1, NH, fb, 4, k, j, i)];
+      u[bx(r, r.n3, r.n2, r.n1, NH, fb, 4, k, j, i)] =
           w_p * igm1 + 0.5 * (w_d * (w_vx * w_vx + w_vy * w_vy + w_vz * w_vz) + (bc1 * bc1 + bc2 * bc2 + bc3 * bc3));
     }
 }
@@ -554,25 +581,25 @@ k_ref_flux_from_finer(RG r, int nvar, const double *__restrict__ ff1, const doub
       const int k = r.ks + 2 * a, j = r.js + 2 * b;
       const double amm = d2[j] * d3[k], amp = d2[j + 1] * d3[k], apm = d2[j] * d3[k + 1], app = d2[j + 1] * d3[k + 1];
       const double tarea = amm + amp + apm + app;
-      auto FX = [&](int p, int q) { return ff1[ix(r.n3, r.n2, r.n1 + 1, nvar, it.fb, n, k + p, j + q, i)]; };
+      auto FX = [&](int p, int q) { return ff1[bx(r, r.n3, r.n2, r.n1 + 1, nvar, it.fb, n, k + p, j + q, i)]; };
       const double v = (FX(0, 0) * amm + FX(0, 1) * amp + FX(1, 0) * apm + FX(1, 1) * app) / tarea;
-      cf1[ix(r.n3, r.n2, r.n1 + 1, nvar, it.cb, n, r.ks + (it.fi2 ? r.h3 : 0) + a, r.js + (it.fi1 ? r.h2 : 0) + b, ic)] = v;
+      cf1[bx(r, r.n3, r.n2, r.n1 + 1, nvar, it.cb, n, r.ks + (it.fi2 ? r.h3 : 0) + a, r.js + (it.fi1 ? r.h2 : 0) + b, ic)] = v;
     } else if (dir == 1) {
       const int jc = side ? r.je + 1 : r.js, j = side ? r.js : r.je + 1;
       const int k = r.ks + 2 * a, i = r.is + 2 * b;
       const double s00 = d1[i] * d3[k], s01 = d1[i + 1] * d3[k], s10 = d1[i] * d3[k + 1], s11 = d1[i + 1] * d3[k + 1];
       const double tarea = s00 + s01 + s10 + s11;
-      auto FX = [&](int p, int q) { return ff2[ix(r.n3, r.n2 + 1, r.n1, nvar, it.fb, n, k + p, j, i + q)]; };
+      auto FX = [&](int p, int q) { return ff2[bx(r, r.n3, r.n2 + 1, r.n1, nvar, it.fb, n, k + p, j, i + q)]; };
       const double v = (FX(0, 0) * s00 + FX(0, 1) * s01 + FX(1, 0) * s10 + FX(1, 1) * s11) / tarea;
-      cf2[ix(r.n3, r.n2 + 1, r.n1, nvar, it.cb, n, r.ks + (it.fi2 ? r.h3 : 0) + a, jc, r.is + (it.fi1 ? r.h1 : 0) + b)] = v;
+      cf2[bx(r, r.n3, r.n2 + 1, r.n1, nvar, it.cb, n, r.ks + (it.fi2 ? r.h3 : 0) + a, jc, r.is + (it.fi1 ? r.h1 : 0) + b)] = v;
     } else {
       const int kc = side ? r.ke + 1 : r.ks, k = side ? r.ks : r.ke + 1;
       const int j = r.js + 2 * a, i = r.is + 2 * b;
       const double s00 = d1[i] * d2[j], s01 = d1[i + 1] * d2[j], s10 = d1[i] * d2[j + 1], s11 = d1[i + 1] * d2[j + 1];
       const double tarea = s00 + s01 + s10 + s11;
-      auto FX = [&](int p, int q) { return ff3[ix(r.n3 + 1, r.n2, r.n1, nvar, it.fb, n, k, j + p, i + q)]; };
+      auto FX = [&](int p, int q) { return ff3[bx(r, r.n3 + 1, r.n2, r.n1, nvar, it.fb, n, k, j + p, i + q)]; };
       const double v = (FX(0, 0) * s00 + FX(0, 1) * s01 + FX(1, 0) * s10 + FX(1, 1) * s11) / tarea;
-      cf3[ix(r.n3 + 1, r.n2, r.n1, nvar, it.cb, n, kc, r.js + (it.fi2 ? r.h2 : 0) + a, r.is + (it.fi1 ? r.h1 : 0) + b)] = v;
+      cf3[bx(r, r.n3 + 1, r.n2, r.n1, nvar, it.cb, n, kc, r.js + (it.fi2 ? r.h2 : 0) + a, r.is + (it.fi1 ? r.h1 : 0) + b)] = v;
     }
   }
 }
@@ -601,9 +628,9 @@ k_ref_emf_to_coarser(RG r, const double *__restrict__ e1, const double *__restri
   const int dir = it.fid >> 1, side = it.fid & 1, fb = it.fb;
   const double *d1 = fc.dx1f + (int64_t)fb * r.n1, *d2 = fc.dx2f + (int64_t)fb * r.n2,
                *d3 = fc.dx3f + (int64_t)fb * r.n3;
-  auto E1 = [&](int k, int j, int i) { return e1[ix(r.n3 + 1, r.n2 + 1, r.n1, 1, fb, 0, k, j, i)]; };
-  auto E2 = [&](int k, int j, int i) { return e2[ix(r.n3 + 1, r.n2, r.n1 + 1, 1, fb, 0, k, j, i)]; };
-  auto E3 = [&](int k, int j, int i) { return e3[ix(r.n3, r.n2 + 1, r.n1 + 1, 1, fb, 0, k, j, i)]; };
+  auto E1 = [&](int k, int j, int i) { return e1[bx(r, r.n3 + 1, r.n2 + 1, r.n1, 1, fb, 0, k, j, i)]; };
+  auto E2 = [&](int k, int j, int i) { return e2[bx(r, r.n3 + 1, r.n2, r.n1 + 1, 1, fb, 0, k, j, i)]; };
+  auto E3 = [&](int k, int j, int i) { return e3[bx(r, r.n3, r.n2 + 1, r.n1 + 1, 1, fb, 0, k, j, i)]; };
   double *out = buf + (int64_t)blockIdx.y * stride;
   if (it.fid >= 6) {   // edge neighbours, flux_correction_fc.cpp:481-546: one line, pairs along it
     const int eid = it.fid - 6, lo = eid & 1, hi = (eid >> 1) & 1;
@@ -668,8 +695,8 @@ k_ref_emf_to_coarser(RG r, const double *__restrict__ e1, const double *__restri
 // coarse buffer) fb to the box of the same shape starting at (fid, fi1, fi2) = (i, j, k)
 // of block (or coarse buffer) cb.  S = source dims, D = destination dims.
 __global__ void __launch_bounds__(RBS)
-k_ref_copy_cc(int s1, int s2, int s3, int d1, int d2, int d3, int nvar, const double *__restrict__ src,
-              double *__restrict__ dst, const agx_ref_item *__restrict__ items) {
+k_ref_copy_cc(Lay sl, Lay dl, int nvar, const double *__restrict__ src, double *__restrict__ dst,
+              const agx_ref_item *__restrict__ items) {
   const agx_ref_item it = items[blockIdx.y];
   const Range q = item_range(it);
   const unsigned w1 = q.w1, w2 = q.w2, per = w1 * w2 * q.w3, cnt = per * nvar;
@@ -679,8 +706,7 @@ k_ref_copy_cc(int s1, int s2, int s3, int d1, int d2, int d3, int nvar, const do
     const int a = (int)(s % w1);
     s /= w1;
     const int b = (int)(s % w2), c = (int)(s / w2);
-    dst[ix(d3, d2, d1, nvar, it.cb, n, it.fi2 + c, it.fi1 + b, it.fid + a)] =
-        src[ix(s3, s2, s1, nvar, it.fb, n, q.sk + c, q.sj + b, q.si + a)];
+    dst[lx(dl, nvar, it.cb, n, it.fi2 + c, it.fi1 + b, it.fid + a)] = src[lx(sl, nvar, it.fb, n, q.sk + c, q.sj + b, q.si + a)];
   }
 }
 
@@ -694,13 +720,15 @@ k_ref_copy_cc(int s1, int s2, int s3, int d1, int d2, int d3, int nvar, const do
 // UNPACK = false: pack.  COARSE: the face arrays are coarse buffers.
 template <bool UNPACK>
 __global__ void __launch_bounds__(RBS)
-k_ref_fc_move(int n1, int n2, int n3, double *__restrict__ f1, double *__restrict__ f2, double *__restrict__ f3,
+k_ref_fc_move(Lay L, double *__restrict__ f1, double *__restrict__ f2, double *__restrict__ f3,
               double *__restrict__ buf, int64_t stride, const agx_ref_item *__restrict__ items) {
   const agx_ref_item it = items[blockIdx.y];
   const Range q = item_range(it);
   const int c = (it.aux & 3) - 1;
   double *f = c == 0 ? f1 : (c == 1 ? f2 : f3);
-  const int d1 = n1 + (c == 0), d2 = n2 + (c == 1), d3 = n3 + (c == 2);
+  L.d1 += (c == 0);
+  L.d2 += (c == 1);
+  L.d3 += (c == 2);
   const int b = UNPACK ? it.cb : it.fb;
   double *row = buf + (UNPACK ? (int64_t)it.fb : (int64_t)blockIdx.y) * stride;
   const unsigned w1 = q.w1, w2 = q.w2, cnt = w1 * w2 * q.w3;
@@ -708,7 +736,7 @@ k_ref_fc_move(int n1, int n2, int n3, double *__restrict__ f1, double *__restric
     const int i = q.si + (int)(t % w1);
     const unsigned s = t / w1;
     const int j = q.sj + (int)(s % w2), k = q.sk + (int)(s / w2);
-    const int64_t o = ix(d3, d2, d1, 1, b, 0, k, j, i);
+    const int64_t o = lx(L, 1, b, 0, k, j, i);
     if (UNPACK) f[o] = row[t];
     else row[t] = f[o];
   }
@@ -845,9 +873,9 @@ k_ref_emf_op(RG r, int kind, double *__restrict__ e1, double *__restrict__ e2, d
     const int j = x.j0 + (int)(s % (unsigned)x.nj), k = x.k0 + (int)(s / (unsigned)x.nj);
     double *e;
     int64_t off;
-    if (x.arr == 0) { e = e1; off = ix(r.n3 + 1, r.n2 + 1, r.n1, 1, b, 0, k, j, i); }
-    else if (x.arr == 1) { e = e2; off = ix(r.n3 + 1, r.n2, r.n1 + 1, 1, b, 0, k, j, i); }
-    else { e = e3; off = ix(r.n3, r.n2 + 1, r.n1 + 1, 1, b, 0, k, j, i); }
+    if (x.arr == 0) { e = e1; off = bx(r, r.n3 + 1, r.n2 + 1, r.n1, 1, b, 0, k, j, i); }
+    else if (x.arr == 1) { e = e2; off = bx(r, r.n3 + 1, r.n2, r.n1 + 1, 1, b, 0, k, j, i); }
+    else { e = e3; off = bx(r, r.n3, r.n2 + 1, r.n1 + 1, 1, b, 0, k, j, i); }
     if (OP == EOP_PACK) bb[t] = e[off];
     else if (OP == EOP_ADD) e[off] += bb[t];
     else if (OP == EOP_ZERO) e[off] = 0.0;
@@ -856,6 +884,12 @@ k_ref_emf_op(RG r, int kind, double *__restrict__ e1, double *__restrict__ e2, d
 }
 
 // ---------------------------------------------------------------- host side
+// layout of the coarse buffers (always dense) or of the blocks
+Lay side_lay(const RG &r, bool coarse) {
+  if (coarse) return Lay{r.c1, r.c2, r.c3, 0, 0, 0};
+  return Lay{r.n1, r.n2, r.n3, r.bS, r.bJ, r.bP};
+}
+
 bool in(int lo, int hi, int mn, int mx) { return lo <= hi && lo >= mn && hi <= mx; }
 
 // fine index range of coarse range [lo, hi] (f = (c - cs) * 2 + fs), plus `extra` cells
@@ -1034,6 +1068,9 @@ int agx_ref_list_create(const agx_ref_geom *g, int32_t kind, const agx_ref_item 
     set_error("refinement operators need 3D blocks with even nx1, nx2, nx3 >= 2"); return -44;
   }
   if (g->ng < 1 || g->cng < 1) { set_error("ng and cng must be >= 1"); return -45; }
+  if (g->layout != AGX_REF_LAYOUT_DENSE && g->layout != AGX_REF_LAYOUT_MESH) {
+    set_error("agx_ref_geom.layout must be AGX_REF_LAYOUT_DENSE or AGX_REF_LAYOUT_MESH"); return -51;
+  }
   if (nitems > 65535) { set_error("at most 65535 items per work list"); return -46; }
   if ((int64_t)(g->nx1 + 2 * g->ng) * (g->nx2 + 2 * g->ng) * (g->nx3 + 2 * g->ng) > (1 << 24)) {
     set_error("refinement operators take blocks of at most 2^24 cells"); return -49;
@@ -1403,9 +1440,8 @@ int agx_ref_copy_cc(const agx_ref_list *l, int32_t nvar, const double *src, doub
   if (l->nitems == 0) return 0;
   const RG r = make_rg(l->g);
   const bool sc = l->kind == AGX_REF_COPY_CC_CB, dc = l->kind == AGX_REF_COPY_CC_BC;
-  hipLaunchKernelGGL(k_ref_copy_cc, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream, sc ? r.c1 : r.n1,
-                     sc ? r.c2 : r.n2, sc ? r.c3 : r.n3, dc ? r.c1 : r.n1, dc ? r.c2 : r.n2, dc ? r.c3 : r.n3, nvar,
-                     src, dst, l->items);
+  hipLaunchKernelGGL(k_ref_copy_cc, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream, side_lay(r, sc),
+                     side_lay(r, dc), nvar, src, dst, l->items);
   return launch_status("agx_ref_copy_cc");
 }
 
@@ -1421,7 +1457,7 @@ int agx_ref_pack_fc(const agx_ref_list *l, const double *x1f, const double *x2f,
   const RG r = make_rg(l->g);
   const bool c = l->kind == AGX_REF_PACK_FC_C;
   hipLaunchKernelGGL(k_ref_fc_move<false>, dim3(l->G, l->nitems), dim3(RBS), 0, (hipStream_t)stream,
-                     c ? r.c1 : r.n1, c ? r.c2 : r.n2, c ? r.c3 : r.n3, const_cast<double *>(x1f),
+                     side_lay(r, c), const_cast<double *>(x1f),
                      const_cast<double *>(x2f), const_cast<double *>(x3f), buf, stride, l->items);
   return launch_status("agx_ref_pack_fc");
 }
@@ -1439,8 +1475,8 @@ int agx_ref_unpack_fc(const agx_ref_list *l, double *x1f, double *x2f, double *x
   for (int p = 0; p < l->nphase; ++p) {
     const int a = l->phase_off[p], n = l->phase_off[p + 1] - a;
     if (n == 0) continue;
-    hipLaunchKernelGGL(k_ref_fc_move<true>, dim3(l->G, n), dim3(RBS), 0, (hipStream_t)stream, c ? r.c1 : r.n1,
-                       c ? r.c2 : r.n2, c ? r.c3 : r.n3, x1f, x2f, x3f, const_cast<double *>(buf), stride,
+    hipLaunchKernelGGL(k_ref_fc_move<true>, dim3(l->G, n), dim3(RBS), 0, (hipStream_t)stream, side_lay(r, c),
+                       x1f, x2f, x3f, const_cast<double *>(buf), stride,
                        l->items + a);
     if (int rc = launch_status("agx_ref_unpack_fc")) return rc;
   }

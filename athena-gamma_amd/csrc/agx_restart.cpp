@@ -299,6 +299,7 @@ int agx_restart_read_host(const char *path, const agx_mesh_desc *d, double *time
 
 int agx_restart_write(agx_mesh *m, const char *path, const char *params) {
   if (!m) { set_error("agx_restart_write: null mesh"); return -1; }
+  if (int rc = refuse_tree(m, "agx_restart_write")) return rc;
   Layout L;
   int rc = layout_of(&m->d, L, "agx_restart_write");
   if (rc) return rc;
@@ -313,6 +314,7 @@ int agx_restart_write(agx_mesh *m, const char *path, const char *params) {
 
 int agx_restart_read(agx_mesh *m, const char *path) {
   if (!m) { set_error("agx_restart_read: null mesh"); return -1; }
+  if (int rc = refuse_tree(m, "agx_restart_read")) return rc;
   Layout L;
   int rc = layout_of(&m->d, L, "agx_restart_read");
   if (rc) return rc;

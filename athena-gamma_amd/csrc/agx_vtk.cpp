@@ -201,6 +201,7 @@ int agx_vtk_write_host(const char *basename, const char *file_id, int32_t file_n
 int agx_vtk_write(agx_mesh *m, const char *basename, const char *file_id, int32_t file_number,
                   const char *variable, int32_t ghost) {
   if (!m) { set_error("agx_vtk_write: null mesh"); return -1; }
+  if (int rc = refuse_tree(m, "agx_vtk_write")) return rc;
   int64_t nb, du[4], db[4], dp[4];
   int rc = agx_array_dims(m, AGX_ARR_U, &nb, du);
   if (!rc) rc = agx_array_dims(m, AGX_ARR_BCC, &nb, db);

@@ -469,6 +469,65 @@ def block_coords(tree, nx, ng, coarse=False, box=(1.0, 1.0, 1.0)):
     return {k: np.ascontiguousarray(np.array(v)) for k, v in tab.items()}
 
 
+def _meshgen_x(index, nrange):
+    """ComputeMeshGeneratorX with the symmetric form of the uniform generator
+    (mesh/mesh.hpp:389-405)"""
+    noffset = index - nrange // 2
+    noffset_ceil = index - (nrange + 1) // 2
+    return float(noffset + noffset_ceil) / (2.0 * nrange)
+
+
+def _uniform_gen(x, xmin, xmax):
+    """UniformMeshGeneratorX? (mesh/mesh.hpp:465-488)"""
+    return 0.5 * (xmin + xmax) + (x * xmax - x * xmin)
+
+
+def mesh_coords(tree, nx, ng, box, coarse=False):
+    """Per-block x?v / dx?f tables (dict of (nb, n) float64 arrays) as the reference's
+    Coordinates builds them for each MeshBlock of a uniform Cartesian mesh over
+    box = ((x1min, x1max), (x2min, x2max), (x3min, x3max)): block extents from
+    Mesh::SetBlockSizeAndBoundaries (mesh/mesh.cpp:1668-1730), faces from the uniform
+    generator at each face's offset on the block's level, the block's own first and last
+    faces set to its extents, dx?f = the block width over its cells, x?v the face
+    midpoints (coordinates/coordinates.cpp:86-145, cartesian.cpp:26-36).  coarse: the
+    coarse buffer (nx/2 cells, every other fine face, coarse_flag of the same
+    constructor)."""
+    tab = {k: [] for k in ("x1v", "x2v", "x3v", "dx1f", "dx2f", "dx3f")}
+    nc = nx // 2 if coarse else nx
+    for (lev, *lx) in tree.leaves:
+        for d in range(3):
+            xmin, xmax = float(box[d][0]), float(box[d][1])
+            nblk = tree.nrb[d] << lev
+            nroot = (tree.nrb[d] * nx) << lev
+            bmin = xmin if lx[d] == 0 else _uniform_gen(_meshgen_x(lx[d], nblk), xmin, xmax)
+            bmax = xmax if lx[d] == nblk - 1 else _uniform_gen(_meshgen_x(lx[d] + 1, nblk), xmin, xmax)
+            dx = (bmax - bmin) / nc
+            n = nc + 2 * ng
+            xf = []
+            for i in range(n + 1):
+                off = ((i - ng) * 2 if coarse else (i - ng)) + lx[d] * nx
+                xf.append(_uniform_gen(_meshgen_x(off, nroot), xmin, xmax))
+            xf[ng] = bmin
+            xf[ng + nc] = bmax
+            tab[f"x{d + 1}v"].append([0.5 * (xf[i + 1] + xf[i]) for i in range(n)])
+            tab[f"dx{d + 1}f"].append([dx] * n)
+    return {k: np.ascontiguousarray(np.array(v, dtype=np.float64)) for k, v in tab.items()}
+
+
+def flux_plan(tree):
+    """Work list of the hydro flux correction (flux_correction_cc.cpp:77-169,240-283): one
+    AGX_REF_FLUX_FROM_FINER item per finer face neighbour of a block, fb = the finer
+    block, cb = the block, fid = its face, fi1/fi2 = the quadrant, in the order of
+    ReceiveFluxCorrection over blocks and their neighbours.  The four items of a face
+    write disjoint quadrants."""
+    out = []
+    for g in range(len(tree.leaves)):
+        for nb in tree.neighbors(g, corners=False):
+            if nb.kind == "face" and nb.rel == 1:
+                out.append(_item(fb=nb.gid, cb=g, fid=nb.fid, fi1=nb.fi1, fi2=nb.fi2))
+    return out
+
+
 # ---------------------------------------------------------------- face-field ghosts
 @dataclass
 class FCPlan:

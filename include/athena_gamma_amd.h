@@ -125,6 +125,28 @@ const char *agx_last_error(void);
 int agx_array_dims(agx_mesh *m, int arr, int64_t *nblocks, int64_t dims[4]);
 /* device bytes held by the mesh */
 int64_t agx_mesh_bytes(agx_mesh *m);
+/* Device address of registered array `arr` (no counterpart in the reference, where each
+ * MeshBlock owns its AthenaArrays): the refinement operators below run on a mesh's own
+ * arrays through it, with agx_ref_geom.layout = AGX_REF_LAYOUT_MESH.  agx_integrate and
+ * agx_cycle rotate the u/u1 and b/b1 registers, so query the address again after them. */
+int agx_array_ptr(agx_mesh *m, int arr, void **ptr);
+/* The HIP stream every launch on the mesh runs on: agx_ref_* calls on the mesh's arrays
+ * pass it so that they are ordered with the task-level entry points. */
+int agx_mesh_stream(agx_mesh *m, void **stream);
+/* Static mesh refinement: block b of the mesh becomes the MeshBlock at LogicalLocation
+ * (level[b], lx[3b..3b+2]) of a tree whose root grid has nrb[0..2] blocks of desc.bnx
+ * cells over box = {x1min, x1max, x2min, x2max, x3min, x3max}; its cell widths and PLM
+ * factors follow Mesh::SetBlockSizeAndBoundaries on the uniform mesh generator
+ * (mesh/mesh.cpp:1668-1730, coordinates/coordinates.cpp:127-145).  The mesh is then
+ * advanced with the task-level FP64 steps (agx_calculate_fluxes, agx_compute_corner_e,
+ * agx_integrate, agx_cons2prim, agx_new_time_step) and the refinement operators for flux
+ * and EMF correction and for the ghost zones across levels, in the order of
+ * TimeIntegratorTaskList (time_integrator.cpp:1460-1558); the entry points built on the
+ * single-level lattice (agx_cycle, agx_initialize, agx_bvals_exchange, agx_autotune,
+ * restart and VTK output) return -60.  Refused for the mixed form and with self-gravity
+ * (multigrid on refined meshes is not built). */
+int agx_mesh_set_levels(agx_mesh *m, const int32_t nrb[3], const double box[6], const int32_t *level,
+                        const int64_t *lx);
 
 /* ---- host <-> device (AthenaArray copies at init/output only) */
 int agx_upload(agx_mesh *m, int arr, const double *host);
@@ -375,8 +397,13 @@ typedef struct {
   int32_t nx1, nx2, nx3;   /* interior cells of a block (even) */
   int32_t ng, cng;         /* ghost width of the blocks and of the coarse buffers (= ng in
                               the reference) */
-  int32_t pad;
+  int32_t layout;          /* layout of the block-side arrays (MeshBlock data; coarse
+                              buffers are always dense): AGX_REF_LAYOUT_DENSE, or
+                              AGX_REF_LAYOUT_MESH for the arrays of an agx_mesh with the
+                              same block size and ghost width (agx_array_dims), so that the
+                              operators work on a mesh's own arrays */
 } agx_ref_geom;
+enum { AGX_REF_LAYOUT_DENSE = 0, AGX_REF_LAYOUT_MESH = 1 };
 typedef struct {
   const double *x1v, *x2v, *x3v, *dx1f, *dx2f, *dx3f;
 } agx_ref_coords;

@@ -312,6 +312,24 @@ void orc_get_time(orc_ctx *c, double *out3) {
   out3[0] = c->time; out3[1] = c->dt; out3[2] = c->ncycle;
 }
 
+/* Static mesh refinement: block g becomes the MeshBlock at LogicalLocation (level[g],
+ * lx[3g..3g+2]) of a root grid of nrb blocks over box (x1min, x1max, ...): its
+ * coordinates as Mesh::SetBlockSizeAndBoundaries (mesh.cpp:1668-1730) and the
+ * Coordinates constructor (coordinates.cpp:86-145) set them on the uniform mesh of the
+ * block's level. */
+void orc_set_levels(orc_ctx *c, const int *nrb, const double *box, const int *level, const int64_t *lx) {
+  const orc_desc *d = &c->d;
+  for (int g = 0; g < c->nblocks; ++g) {
+    const int L = level[g];
+    block_coords((nrb[0] * d->bnx1) << L, nrb[0] << L, d->bnx1, c->ng, (int)lx[3 * g], box[0], box[1],
+                 c->x1f + g * (c->n1 + 1), c->x1v + g * (c->n1 + 1), c->dx1f + g * (c->n1 + 1));
+    block_coords((nrb[1] * d->bnx2) << L, nrb[1] << L, d->bnx2, c->ng, (int)lx[3 * g + 1], box[2], box[3],
+                 c->x2f + g * (c->n2 + 1), c->x2v + g * (c->n2 + 1), c->dx2f + g * (c->n2 + 1));
+    block_coords((nrb[2] * d->bnx3) << L, nrb[2] << L, d->bnx3, c->ng, (int)lx[3 * g + 2], box[4], box[5],
+                 c->x3f + g * (c->n3 + 1), c->x3v + g * (c->n3 + 1), c->dx3f + g * (c->n3 + 1));
+  }
+}
+
 void orc_coords(orc_ctx *c, int gid, int dir, double *xf, double *xv, double *dxf) {
   int n = (dir == 0) ? c->n1 : (dir == 1 ? c->n2 : c->n3);
   double *sf = (dir == 0) ? c->x1f : (dir == 1 ? c->x2f : c->x3f);

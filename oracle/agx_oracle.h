@@ -65,6 +65,8 @@ void orc_get_time(orc_ctx *c, double *out3);
 
 /* coordinates of block gid: fills x1f (n1+1), x1v (n1), etc. dir 0..2 */
 void orc_coords(orc_ctx *c, int gid, int dir, double *xf, double *xv, double *dxf);
+/* per-block coordinates of a static-refinement tree (mesh.cpp:1668-1730) */
+void orc_set_levels(orc_ctx *c, const int *nrb, const double *box, const int *level, const int64_t *lx);
 
 /* reference steps, all blocks */
 void orc_bvals_cc(orc_ctx *c, int which_arr);           /* bvals_cc.cpp:201-336 */

@@ -69,6 +69,8 @@ def _lib():
     lib.orc_set_time.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int]
     lib.orc_get_time.argtypes = [P, D]
     lib.orc_coords.argtypes = [P, ctypes.c_int, ctypes.c_int, D, D, D]
+    lib.orc_set_levels.argtypes = [P, ctypes.POINTER(ctypes.c_int), D, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int64)]
     for f in ("orc_bvals_cc", "orc_bvals_fc", "orc_calculate_fluxes", "orc_integrate",
               "orc_mg_solve"):
         getattr(lib, f).argtypes = [P, ctypes.c_int]
@@ -193,6 +195,16 @@ class Oracle:
         self.L.orc_coords(self.h, gid, d, xf.ctypes.data_as(D), xv.ctypes.data_as(D),
                           dx.ctypes.data_as(D))
         return xf, xv, dx
+
+    def set_levels(self, nrb, box, leaves):
+        """block g becomes the MeshBlock at LogicalLocation leaves[g] = (level, lx1, lx2,
+        lx3) of a root grid of nrb blocks over box = ((x1min, x1max), ...)"""
+        n = len(leaves)
+        lev = (ctypes.c_int * n)(*[int(x[0]) for x in leaves])
+        lx = (ctypes.c_int64 * (3 * n))(*[int(v) for x in leaves for v in x[1:]])
+        nr = (ctypes.c_int * 3)(*[int(x) for x in nrb])
+        bx = (ctypes.c_double * 6)(*[float(v) for b in box for v in b])
+        self.L.orc_set_levels(self.h, nr, bx, lev, lx)
 
     def time(self):
         out = np.zeros(3)

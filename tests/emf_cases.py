@@ -118,11 +118,14 @@ def items_of(cls, dicts):
 
 def run_oracle(orc, t, E, nx, ng, tab, skip=(), plan=None):
     """One EMF correction on copies of E with the oracle, in the reference's order;
-    `skip` leaves out steps ('clear', 'average') to plant a mistake."""
+    `skip` leaves out steps ('clear', 'average') to plant a mistake, or all of them
+    ('all')."""
     smr = load_smr()
     plan = plan or smr.emf_plan(t)
     g = orc.RefGeom(nx, nx, nx, ng, ng, 0)
     e1, e2, e3 = [np.ascontiguousarray(a.copy()) for a in E]
+    if "all" in skip:
+        return [e1, e2, e3], None
     ss = max(1, max((orc_count_same(nx, d["fid"]) for d in plan.same_level), default=1))
     sc = max(1, max((orc_count_coarser(nx, d["fid"]) for d in plan.to_coarser), default=1))
     buf_s = np.zeros((max(1, len(plan.same_level)), ss))

@@ -262,7 +262,7 @@ def mhd_state(t, nx, ng, seed, floors=True):
             for k, v in arrs.items()}
 
 
-def run_oracle_mhd(orc, t, A, nx, ng, ftab, ctab, order="reference"):
+def run_oracle_mhd(orc, t, A, nx, ng, ftab, ctab, order="reference", gamma=GAMMA, dfloor=DFLOOR, pfloor=PFLOOR):
     """ghost exchange of u (cells) and b (faces) across levels, then ProlongateBoundaries
     for MHD, on copies of the arrays in A"""
     smr = load_smr()
@@ -282,12 +282,12 @@ def run_oracle_mhd(orc, t, A, nx, ng, ftab, ctab, order="reference"):
         orc.ref_restrict_cc(g, 5, u, ftab, cu, I(st["restrict_cc"]))
         for c in range(3):
             orc.ref_restrict_fc(g, c, A["b"][c], ftab, A["cb"][c], I(st["restrict_fc"][c]))
-        orc.ref_cons_to_prim(g, cu, A["cb"], A["cw"], A["cbcc"], GAMMA, DFLOOR, PFLOOR, 0, I(st["c2p"]))
+        orc.ref_cons_to_prim(g, cu, A["cb"], A["cw"], A["cbcc"], gamma, dfloor, pfloor, 0, I(st["c2p"]))
         orc.ref_prolong_cc(g, 5, A["cw"], ctab, A["w"], ftab, I(st["prolong_cc"]))
         for c in range(3):
             orc.ref_prolong_fc_shared(g, c, A["cb"][c], ctab, A["b"][c], ftab, I(st["prolong_fc"][c]))
         orc.ref_prolong_fc_internal(g, A["b"][0], A["b"][1], A["b"][2], ftab, I(st["prolong_int"]))
-        orc.ref_prim_to_cons(g, A["w"], A["b"][0], A["b"][1], A["b"][2], A["bcc"], u, GAMMA, 0, I(st["p2c"]))
+        orc.ref_prim_to_cons(g, A["w"], A["b"][0], A["b"][1], A["b"][2], A["bcc"], u, gamma, 0, I(st["p2c"]))
     return A
 
 

@@ -1,0 +1,130 @@
+"""MHD on a static-refinement mesh on the device (SMRMesh: agx_mesh_set_levels, the
+task-level steps and the refinement operators on the mesh's own arrays, in the reference's
+task order with flux and EMF correction, time_integrator.cpp:1460-1558) against:
+  - the oracle's composition of the same cycle (tests/smr_cases.py, pinned by
+    test_oracle_smr_cycle.py) on one, two and three levels: identical bit for bit;
+  - the single-level lattice cycle (agx_cycle) on the same blocks: within round-off (the
+    lattice cycle leaves out the same-level EMF average, which moves shared edges by a
+    few ulps);
+  - what the corrections guarantee on a periodic mesh: mass, momentum and energy summed
+    over all blocks conserved to round-off (flux correction), the field through every
+    refinement boundary equal on both sides (EMF correction), div B at round-off."""
+import numpy as np
+import pytest
+
+from tests import smr_cases as sc
+
+pytestmark = pytest.mark.gpu
+BOX = ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0))
+FIELDS = ("u", "w", "bcc", "b1f", "b2f", "b3f")
+
+
+def gpu_run(agx, tree, nx, state, ncyc, xorder=2, rsolver=0):
+    m = agx.SMRMesh(tree, nx, 2, BOX, cfl=0.3, xorder=xorder, rsolver=rsolver)
+    try:
+        for name, a in zip(("u", "b1f", "b2f", "b3f"), state):
+            m.set(name, a)
+        m.initialize()
+        assert m.cycle(1e9, ncyc) == ncyc
+        out = {f: m.get(f) for f in FIELDS}
+        out["time"] = m.time()
+        return out
+    finally:
+        m.close()
+
+
+def oracle_run(tree, nx, state, ncyc, xorder=2, rsolver=0):
+    o = sc.OracleSMR(tree, nx, 2, BOX, cfl=0.3, xorder=xorder, rsolver=rsolver)
+    try:
+        for name, a in zip(("u", "b1f", "b2f", "b3f"), state):
+            o.set(name, a)
+        o.initialize()
+        assert o.cycle(1e9, ncyc) == ncyc
+        out = {f: o.get(f) for f in FIELDS}
+        out["time"] = o.time()
+        return out
+    finally:
+        o.close()
+
+
+def same(a, b, what):
+    ok = np.array_equal(a, b)
+    if not ok:
+        d = np.abs(a - b)
+        print(f"{what}: max |diff| {np.nanmax(d):.3e} at {np.unravel_index(np.nanargmax(d), d.shape)}", flush=True)
+    assert ok, what
+
+
+def test_single_level_equals_oracle_and_lattice(agx):
+    tree = sc.periodic_tree((2, 2, 2))
+    nx = 8
+    state = sc.smooth_state(tree, nx, 2, BOX)
+    got = gpu_run(agx, tree, nx, state, 3)
+    ref = oracle_run(tree, nx, state, 3)
+    assert got["time"] == ref["time"]
+    for f in FIELDS:
+        same(got[f], ref[f], f"one level: {f}")
+    desc = agx.make_desc(nx=(16, 16, 16), bnx=(8, 8, 8), box=BOX, cfl=0.3, nghost=2, gamma=sc.GAMMA)
+    m = agx.Mesh(desc)
+    try:
+        for name, a in zip(("u", "b1f", "b2f", "b3f"), state):
+            m.set(name, a)
+        m.initialize()
+        assert m.cycle(1e9, 3) == 3
+        for f in FIELDS:
+            lat = m.get(f)
+            assert np.abs(got[f] - lat).max() <= 1e-13 * np.abs(lat).max(), f
+    finally:
+        m.close()
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_refined_mesh_equals_oracle(agx, levels):
+    if levels == 2:
+        tree, nx = sc.periodic_tree((2, 2, 2), refine=((0, 0, 0),)), 8
+    else:
+        inner = tuple((x, y, z) for z in (1, 2) for y in (1, 2) for x in (1, 2))
+        tree, nx = sc.periodic_tree((4, 4, 4), refine=inner, refine2=((3, 3, 3),)), 4
+    print(f"{levels} levels: {len(tree.leaves)} blocks of {nx}^3", flush=True)
+    state = sc.smooth_state(tree, nx, 2, BOX)
+    got = gpu_run(agx, tree, nx, state, 3)
+    ref = oracle_run(tree, nx, state, 3)
+    assert got["time"] == ref["time"]
+    for f in FIELDS:
+        same(got[f], ref[f], f"{levels} levels: {f}")
+
+
+@pytest.mark.parametrize("xorder,rsolver", [(2, 0), (2, 1), (1, 0)])
+def test_refined_mesh_conserves_and_keeps_interfaces(agx, xorder, rsolver):
+    tree, nx, ng = sc.periodic_tree((2, 2, 2), refine=((0, 0, 0), (1, 1, 1))), 8, 2
+    state = sc.smooth_state(tree, nx, ng, BOX, amp=0.3)
+    m = agx.SMRMesh(tree, nx, ng, BOX, cfl=0.3, xorder=xorder, rsolver=rsolver)
+    try:
+        for name, a in zip(("u", "b1f", "b2f", "b3f"), state):
+            m.set(name, a)
+        m.initialize()
+        t0 = sc.totals(tree, nx, ng, BOX, m.get("u"))
+        assert m.cycle(1e9, 10) == 10
+        t1 = sc.totals(tree, nx, ng, BOX, m.get("u"))
+        b = [m.get(n) for n in ("b1f", "b2f", "b3f")]
+    finally:
+        m.close()
+    rel = float((np.abs(t1 - t0) / np.maximum(np.abs(t0), 1.0)).max())
+    divb, mis = sc.max_divb(tree, nx, ng, BOX, *b), sc.interface_mismatch(tree, nx, ng, *b)
+    print(f"xorder {xorder} rsolver {rsolver}: conservation {rel:.2e}, div B {divb:.2e}, interface {mis:.2e}",
+          flush=True)
+    assert rel < 1e-14
+    assert divb < 1e-14
+    assert mis < 1e-14
+
+
+def test_lattice_entry_points_refuse_a_refined_mesh(agx):
+    tree = sc.periodic_tree((2, 2, 2), refine=((0, 0, 0),))
+    m = agx.SMRMesh(tree, 8, 2, BOX)
+    try:
+        for call in (lambda: m.m.cycle(1e9, 1), lambda: m.m.initialize(), lambda: m.m.bvals(3),
+                     lambda: m.m.autotune()):
+            with pytest.raises(agx.AgxError, match="refinement levels"):
+                call()
+    finally:
+        m.close()

@@ -1,0 +1,95 @@
+"""Pins of the oracle's static-refinement MHD cycle (tests/smr_cases.py: the reference's
+task order with refinement, time_integrator.cpp:1460-1558, composed from the oracle's
+task-level steps and refinement operators), which the device's SMRMesh is held to bit for
+bit:
+  - on one level, with the EMF correction left out, it is the oracle's lattice cycle bit
+    for bit (every other refinement step is then a same-level copy); with the correction,
+    which the reference runs on every MHD mesh (SEND_FLDFLX / RECV_FLDFLX are added
+    unconditionally, time_integrator.cpp:985-994), it differs from the lattice cycle only
+    at round-off: neighbouring blocks' copies of a shared edge EMF differ by a few ulps
+    (their copies of the shared faces are never exchanged) and the correction averages
+    them;
+  - on two and three levels of a periodic mesh: mass, momentum and energy summed over all
+    blocks are conserved to round-off, and the field through every refinement boundary
+    agrees between the coarse face and the fine faces that cover it.  Leaving out the flux
+    correction breaks the first, leaving out the EMF correction the second; div B stays
+    at round-off in every block either way (CT)."""
+import numpy as np
+import pytest
+
+from oracle import orc
+from tests import smr_cases as sc
+
+BOX = ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0))
+
+
+def load(o, st):
+    for n, a in zip(("u", "b1f", "b2f", "b3f"), st):
+        o.set(n, a)
+
+
+@pytest.mark.parametrize("xorder", [1, 2])
+def test_single_level_is_the_lattice_cycle(xorder):
+    tree = sc.periodic_tree((2, 2, 2))
+    st = sc.smooth_state(tree, 8, 2, BOX)
+    desc = orc.make_desc(nx=(16, 16, 16), bnx=(8, 8, 8), box=BOX, cfl=0.3, nghost=2, gamma=sc.GAMMA, xorder=xorder)
+    r = orc.Oracle(desc)
+    load(r, st)
+    r.initialize()
+    for _ in range(4):
+        r.cycle(1e9)
+    for skip, exact in ((("all",), True), ((), False)):
+        o = sc.OracleSMR(tree, 8, 2, BOX, xorder=xorder, emf_skip=skip)
+        load(o, st)
+        o.initialize()
+        assert o.cycle(1e9, 4) == 4
+        for n in ("u", "w", "b1f", "b2f", "b3f"):
+            a, b = o.get(n), r.get(n)
+            if exact:
+                assert np.array_equal(a, b), n
+            else:
+                assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max(), n
+        o.close()
+
+
+def run(tree, nx, st, ncyc, **kw):
+    o = sc.OracleSMR(tree, nx, 2, BOX, **kw)
+    load(o, st)
+    o.initialize()
+    t0 = sc.totals(tree, nx, 2, BOX, o.get("u"))
+    assert o.cycle(1e9, ncyc) == ncyc
+    t1 = sc.totals(tree, nx, 2, BOX, o.get("u"))
+    b = [o.get(n) for n in ("b1f", "b2f", "b3f")]
+    o.close()
+    cons = float((np.abs(t1 - t0) / np.maximum(np.abs(t0), 1.0)).max())
+    return cons, sc.max_divb(tree, nx, 2, BOX, *b), sc.interface_mismatch(tree, nx, 2, *b)
+
+
+def two_levels():
+    return sc.periodic_tree((2, 2, 2), refine=((0, 0, 0), (1, 1, 1))), 8
+
+
+def three_levels():
+    inner = tuple((x, y, z) for z in (1, 2) for y in (1, 2) for x in (1, 2))
+    return sc.periodic_tree((4, 4, 4), refine=inner, refine2=((3, 3, 3),)), 4
+
+
+@pytest.mark.parametrize("mesh", [two_levels, three_levels])
+def test_refined_cycle_conserves_and_keeps_interfaces(mesh):
+    tree, nx = mesh()
+    st = sc.smooth_state(tree, nx, 2, BOX, amp=0.3)
+    assert sc.interface_mismatch(tree, nx, 2, *st[1:]) < 1e-15
+    cons, divb, mis = run(tree, nx, st, 4)
+    print(f"{len(tree.leaves)} blocks: conservation {cons:.2e}, div B {divb:.2e}, interface {mis:.2e}", flush=True)
+    assert cons < 1e-14
+    assert divb < 1e-14
+    assert mis < 1e-14
+
+
+def test_planted_omissions_fail_the_pins():
+    tree, nx = two_levels()
+    st = sc.smooth_state(tree, nx, 2, BOX, amp=0.3)
+    cons, divb, mis = run(tree, nx, st, 4, flux_skip=True)
+    assert cons > 1e-8 and divb < 1e-14
+    cons, divb, mis = run(tree, nx, st, 4, emf_skip=("all",))
+    assert mis > 1e-6 and divb < 1e-14
