@@ -127,3 +127,56 @@ def scatter_faces(desc, b1, b2, b3):
         o2[g, ng:ng + bx3, ng:ng + bx2 + 1, ng:ng + bx1] = b2[np.ix_(K[:-1], J, I[:-1])]
         o3[g, ng:ng + bx3 + 1, ng:ng + bx2, ng:ng + bx1] = b3[np.ix_(K, J[:-1], I[:-1])]
     return o1, o2, o3
+
+
+def divfree_blocks(faces, box, gamma=5.0 / 3.0, amp=0.2, b0=(0.6, 0.0, 0.0), seed=0):
+    """Smooth periodic MHD state on blocks of any refinement level, given each block's face
+    positions faces[g] = (x1f, x2f, x3f) (n + 1 each, ghosts included): density, pressure
+    and velocity sinusoids at the cell centres; the face field B = curl A + b0 with
+    A = (Ax(y, z), Ay(x, z), Az(x, y)) sinusoids and each face value the exact line
+    integral of A around the face over its area, so the flux through a coarse face equals
+    the sum over the fine faces that cover it and every cell's divergence vanishes up to
+    round-off.  Returns (u (nb, 5, n, n, n), b1f, b2f, b3f) in the reference's layouts."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = [b[1] - b[0] for b in box]
+    k = [2 * np.pi / x for x in L]
+    ph = rng.uniform(0, 2 * np.pi, 12)
+
+    def Ax(y, z):
+        return 0.1 * np.sin(k[1] * y + ph[0]) * np.cos(k[2] * z + ph[1])
+
+    def Ay(x, z):
+        return 0.1 * np.sin(k[2] * z + ph[2]) * np.cos(k[0] * x + ph[3])
+
+    def Az(x, y):
+        return 0.1 * np.sin(k[0] * x + ph[4]) * np.cos(k[1] * y + ph[5])
+    nb, n = len(faces), len(faces[0][0]) - 1
+    u = np.zeros((nb, 5, n, n, n))
+    b1 = np.zeros((nb, n, n, n + 1))
+    b2 = np.zeros((nb, n, n + 1, n))
+    b3 = np.zeros((nb, n + 1, n, n))
+    for g in range(nb):
+        f1, f2, f3 = faces[g]
+        c1, c2, c3 = 0.5 * (f1[1:] + f1[:-1]), 0.5 * (f2[1:] + f2[:-1]), 0.5 * (f3[1:] + f3[:-1])
+        Z, Y, X = np.meshgrid(c3, c2, c1, indexing="ij")
+        rho = 1.0 + amp * np.sin(k[0] * X + ph[6]) * np.cos(k[1] * Y + ph[7]) * np.sin(k[2] * Z + ph[8])
+        vx = amp * np.sin(k[1] * Y + ph[9])
+        vy = amp * np.sin(k[2] * Z + ph[10])
+        vz = amp * np.sin(k[0] * X + ph[11])
+        p = 1.0 + 0.5 * amp * np.cos(k[0] * X + ph[8]) * np.cos(k[2] * Z + ph[9])
+        Zf, Yf, Xf = np.meshgrid(f3, f2, f1, indexing="ij")
+        ax, ay, az = Ax(Yf, Zf), Ay(Xf, Zf), Az(Xf, Yf)
+        dx, dy, dz = np.diff(f1), np.diff(f2), np.diff(f3)
+        # x1 face (k, j, i): Bx = dAz/dy - dAy/dz as circulation over area
+        b1[g] = ((ay[:-1, :-1, :] - ay[1:, :-1, :]) / dz[:, None, None] +
+                 (az[:-1, 1:, :] - az[:-1, :-1, :]) / dy[None, :, None]) + b0[0]
+        b2[g] = ((az[:-1, :, :-1] - az[:-1, :, 1:]) / dx[None, None, :] +
+                 (ax[1:, :, :-1] - ax[:-1, :, :-1]) / dz[:, None, None]) + b0[1]
+        b3[g] = ((ax[:, :-1, :-1] - ax[:, 1:, :-1]) / dy[None, :, None] +
+                 (ay[:, :-1, 1:] - ay[:, :-1, :-1]) / dx[None, None, :]) + b0[2]
+        bc = [0.5 * (b1[g][..., 1:] + b1[g][..., :-1]), 0.5 * (b2[g][:, 1:] + b2[g][:, :-1]),
+              0.5 * (b3[g][1:] + b3[g][:-1])]
+        u[g, 0] = rho
+        u[g, 1], u[g, 2], u[g, 3] = rho * vx, rho * vy, rho * vz
+        u[g, 4] = p / (gamma - 1.0) + 0.5 * rho * (vx * vx + vy * vy + vz * vz) + 0.5 * sum(x * x for x in bc)
+    return u, b1, b2, b3

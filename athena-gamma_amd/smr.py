@@ -514,6 +514,25 @@ def mesh_coords(tree, nx, ng, box, coarse=False):
     return {k: np.ascontiguousarray(np.array(v, dtype=np.float64)) for k, v in tab.items()}
 
 
+def face_positions(tree, nx, ng, box):
+    """per block the three arrays of its n + 1 face positions (n = nx + 2 ng) along x1,
+    x2, x3: the uniform generator at each face's offset on the block's level, the block's
+    own first and last faces set to its extents (as mesh_coords)"""
+    out = []
+    for (lev, *lx) in tree.leaves:
+        per = []
+        for d in range(3):
+            xmin, xmax = float(box[d][0]), float(box[d][1])
+            nblk, nroot = tree.nrb[d] << lev, (tree.nrb[d] * nx) << lev
+            bmin = xmin if lx[d] == 0 else _uniform_gen(_meshgen_x(lx[d], nblk), xmin, xmax)
+            bmax = xmax if lx[d] == nblk - 1 else _uniform_gen(_meshgen_x(lx[d] + 1, nblk), xmin, xmax)
+            xf = [_uniform_gen(_meshgen_x(i - ng + lx[d] * nx, nroot), xmin, xmax) for i in range(nx + 2 * ng + 1)]
+            xf[ng], xf[ng + nx] = bmin, bmax
+            per.append(np.array(xf))
+        out.append(per)
+    return out
+
+
 def flux_plan(tree):
     """Work list of the hydro flux correction (flux_correction_cc.cpp:77-169,240-283): one
     AGX_REF_FLUX_FROM_FINER item per finer face neighbour of a block, fb = the finer

@@ -424,6 +424,65 @@ def large_bench(agx, pgen, args):
     print(json.dumps(out), flush=True)
 
 
+def smr_cycle_bench(agx, pgen, args):
+    """Static mesh refinement, three levels: a 4^3 root grid of 32^3 MeshBlocks on the
+    periodic unit cube, the central 2^3 root blocks refined and one inner level-1 block
+    refined again (127 blocks: 56 + 63 + 8, 4.16e6 cells), a smooth divergence-free state
+    (pgen.divfree_blocks), VL2+PLM+HLLD+CT in FP64 through SMRMesh: the task-level launches
+    plus flux correction, EMF correction and the ghost exchange with prolongation across
+    levels, every step on the mesh's stream.  Zone-cycles/s over K cycles; the per-class
+    kernel timers of two more cycles split the cycle into the MHD kernels and the rest
+    (refinement operators and launch gaps)."""
+    smr = agx._load_smr()
+    nx, ng, box = args.smr_block, 2, ((0.0, 1.0),) * 3
+    inner = [(x, y, z) for z in (1, 2) for y in (1, 2) for x in (1, 2)]
+    leaves = [(0, x, y, z) for z in range(4) for y in range(4) for x in range(4) if (x, y, z) not in inner]
+    for (x, y, z) in inner:
+        leaves += [(1, 2 * x + a, 2 * y + b, 2 * z + c) for c in range(2) for b in range(2) for a in range(2)
+                   if (2 * x + a, 2 * y + b, 2 * z + c) != (3, 3, 3)]
+    leaves += [(2, 6 + a, 6 + b, 6 + c) for c in range(2) for b in range(2) for a in range(2)]
+    tree = smr.SMRTree((4, 4, 4), leaves, periodic=(True, True, True))
+    t0 = time.time()
+    u, b1, b2, b3 = pgen.divfree_blocks(smr.face_positions(tree, nx, ng, box), box, amp=0.2)
+    m = agx.SMRMesh(tree, nx, ng, box, cfl=0.3)
+    try:
+        for name, a in zip(("u", "b1f", "b2f", "b3f"), (u, b1, b2, b3)):
+            m.set(name, a)
+        del u, b1, b2, b3
+        m.initialize()
+        m.cycle(1e30, 2)
+        m.sync()
+        log(f"[smr] {len(tree.leaves)} blocks of {nx}^3 on 3 levels set up in {time.time() - t0:.1f}s")
+        steps = args.smr_steps
+        t1 = time.perf_counter()
+        m.cycle(1e30, steps)
+        m.sync()
+        sec = time.perf_counter() - t1
+        m.m.timing(True)
+        m.m.timing_reset()
+        t2 = time.perf_counter()
+        m.cycle(1e30, 2)
+        m.sync()
+        sec2 = (time.perf_counter() - t2) / 2
+        per = {c: m.m.timing_get(c)[0] / 2 for c in KCLASSES}
+        m.m.timing(False)
+    finally:
+        m.close()
+    ncell = len(tree.leaves) * nx ** 3
+    kern = sum(per.values())
+    out = {"workload": f"{len(tree.leaves)} MeshBlocks of {nx}^3 on 3 refinement levels (4^3 root grid, central "
+                       f"2^3 refined, one inner block refined again), periodic, smooth MHD state, "
+                       f"VL2+PLM+HLLD+CT FP64, flux and EMF correction",
+           "value": ncell * steps / sec, "unit": "zone-cycles/s", "ms_per_step": sec * 1e3 / steps,
+           "cells": ncell, "steps": steps,
+           "mhd_kernel_ms_per_cycle": {k: v for k, v in per.items() if v > 0},
+           "refinement_and_gaps_ms_per_cycle": sec2 * 1e3 - kern}
+    log(f"[smr] {steps} cycles in {sec:.3f}s -> {out['value']:.4e} zone-cycles/s; per cycle (timed run) "
+        f"{sec2 * 1e3:.2f} ms: MHD kernels {kern:.2f} ms, refinement operators and gaps "
+        f"{out['refinement_and_gaps_ms_per_cycle']:.2f} ms")
+    return out
+
+
 def ref_bench(agx, args):
     """Refinement operators (static mesh refinement) on 512 blocks of 32^3 (ng 2, 5
     variables, coarse buffers 16^3 + 2 ng): whole-block cell restriction and prolongation,
@@ -724,6 +783,9 @@ def main():
                     help="skip the 256^3 MHD + self-gravity leg (FP64 and mixed)")
     ap.add_argument("--sg-steps", type=int, default=10)
     ap.add_argument("--no-ref", action="store_true", help="skip the refinement-operator leg")
+    ap.add_argument("--no-smr", action="store_true", help="skip the three-level static-refinement MHD leg")
+    ap.add_argument("--smr-block", type=int, default=32)
+    ap.add_argument("--smr-steps", type=int, default=10)
     ap.add_argument("--no-compare", dest="compare", action="store_false",
                     help="skip downloading w and B after both runs (mixed-vs-FP64 difference, div B)")
     ap.add_argument("--large", action="store_true",
@@ -829,6 +891,8 @@ def main():
             "kernel_ms_per_cycle": {k: v[0] / 3 for k, v in mqper.items()}}
     if not args.no_ref:
         out["refinement"] = ref_bench(agx, args)
+    if not args.no_smr:
+        out["smr_cycle"] = smr_cycle_bench(agx, pgen, args)
     if not args.no_sg:
         out["self_gravity"] = sg_run(agx, pgen, args.n, args.block, 2, 2.0, args.sg_steps, "sg", bx=0.1)
     if not args.no_cpu:

@@ -1,13 +1,9 @@
-"""Static-refinement MHD cycles for the tests: periodic block trees, a smooth
-divergence-free initial state sampled on every block of every level, and the oracle's
-composition of the cycle (the reference's task order with refinement,
+"""Static-refinement MHD cycles for the tests: periodic block trees, the package's smooth
+divergence-free state on every block of every level (pgen.divfree_blocks), diagnostics
+(conservation, div B, agreement of the field through refinement boundaries) and the
+oracle's composition of the cycle (the reference's task order with refinement,
 time_integrator.cpp:1460-1558) from the oracle's task-level steps and refinement
-operators, on its dense arrays.
-
-The face field is the curl of A = (Ax(y, z), Ay(x, z), Az(x, y)) + a uniform field, each
-face value the exact line integral of A around the face over its area, so the flux
-through a coarse face equals the sum over the fine faces that cover it and every cell's
-divergence vanishes up to round-off on every level."""
+operators, on its dense arrays."""
 import itertools
 
 import numpy as np
@@ -15,7 +11,7 @@ import numpy as np
 from oracle import orc
 from tests import emf_cases as ec
 from tests import ghost_cases as gc
-from tests.conftest import load_smr
+from tests.conftest import load_pgen, load_smr
 
 GAMMA = 5.0 / 3.0
 
@@ -34,69 +30,12 @@ def periodic_tree(nrb, refine=(), refine2=()):
 
 
 def face_coords(tree, nx, ng, box):
-    """per block and direction the n + 1 face positions of the block's cells (the uniform
-    generator, as smr.mesh_coords)"""
-    smr = load_smr()
-    out = []
-    for (lev, *lx) in tree.leaves:
-        per = []
-        for d in range(3):
-            xmin, xmax = float(box[d][0]), float(box[d][1])
-            nblk, nroot = tree.nrb[d] << lev, (tree.nrb[d] * nx) << lev
-            bmin = xmin if lx[d] == 0 else smr._uniform_gen(smr._meshgen_x(lx[d], nblk), xmin, xmax)
-            bmax = xmax if lx[d] == nblk - 1 else smr._uniform_gen(smr._meshgen_x(lx[d] + 1, nblk), xmin, xmax)
-            xf = [smr._uniform_gen(smr._meshgen_x(i - ng + lx[d] * nx, nroot), xmin, xmax)
-                  for i in range(nx + 2 * ng + 1)]
-            xf[ng], xf[ng + nx] = bmin, bmax
-            per.append(np.array(xf))
-        out.append(per)
-    return out
+    return load_smr().face_positions(tree, nx, ng, box)
 
 
 def smooth_state(tree, nx, ng, box, amp=0.2, b0=(0.6, 0.0, 0.0), seed=0):
-    """(u, b1f, b2f, b3f) on every block: density, pressure and velocity sinusoids at the
-    cell centres; the face field from A (exact face fluxes) plus b0"""
-    rng = np.random.default_rng(seed)
-    L = [b[1] - b[0] for b in box]
-    k = [2 * np.pi / l for l in L]
-    ph = rng.uniform(0, 2 * np.pi, 12)
-    Ax = lambda y, z: 0.1 * np.sin(k[1] * y + ph[0]) * np.cos(k[2] * z + ph[1])   # noqa: E731
-    Ay = lambda x, z: 0.1 * np.sin(k[2] * z + ph[2]) * np.cos(k[0] * x + ph[3])   # noqa: E731
-    Az = lambda x, y: 0.1 * np.sin(k[0] * x + ph[4]) * np.cos(k[1] * y + ph[5])   # noqa: E731
-    xf = face_coords(tree, nx, ng, box)
-    nb, n = len(tree.leaves), nx + 2 * ng
-    u = np.zeros((nb, 5, n, n, n))
-    b1 = np.zeros((nb, n, n, n + 1))
-    b2 = np.zeros((nb, n, n + 1, n))
-    b3 = np.zeros((nb, n + 1, n, n))
-    for g in range(nb):
-        f1, f2, f3 = xf[g]
-        c1, c2, c3 = 0.5 * (f1[1:] + f1[:-1]), 0.5 * (f2[1:] + f2[:-1]), 0.5 * (f3[1:] + f3[:-1])
-        Z, Y, X = np.meshgrid(c3, c2, c1, indexing="ij")
-        rho = 1.0 + amp * np.sin(k[0] * X + ph[6]) * np.cos(k[1] * Y + ph[7]) * np.sin(k[2] * Z + ph[8])
-        vx = amp * np.sin(k[1] * Y + ph[9])
-        vy = amp * np.sin(k[2] * Z + ph[10])
-        vz = amp * np.sin(k[0] * X + ph[11])
-        p = 1.0 + 0.5 * amp * np.cos(k[0] * X + ph[8]) * np.cos(k[2] * Z + ph[9])
-        # x1 faces at f1[i], spanning [f2[j], f2[j+1]] x [f3[k], f3[k+1]]
-        Zc, Yf, Xf = np.meshgrid(f3, f2, f1, indexing="ij")
-        ay = Ay(Xf, Zc)            # (n+1, n+1, n+1) at (z_k, ., x_i)
-        az = Az(Xf, Yf)
-        dy, dz = np.diff(f2), np.diff(f3)
-        b1[g] = ((ay[:-1, :-1, :] - ay[1:, :-1, :]) / dz[:, None, None] +
-                 (az[:-1, 1:, :] - az[:-1, :-1, :]) / dy[None, :, None]) + b0[0]
-        ax = Ax(Yf, Zc)
-        dx = np.diff(f1)
-        b2[g] = ((az[:-1, :, :-1] - az[:-1, :, 1:]) / dx[None, None, :] +
-                 (ax[1:, :, :-1] - ax[:-1, :, :-1]) / dz[:, None, None]) + b0[1]
-        b3[g] = ((ax[:, :-1, :-1] - ax[:, 1:, :-1]) / dy[None, :, None] +
-                 (ay[:, :-1, 1:] - ay[:, :-1, :-1]) / dx[None, None, :]) + b0[2]
-        bc = [0.5 * (b1[g][..., 1:] + b1[g][..., :-1]), 0.5 * (b2[g][:, 1:] + b2[g][:, :-1]),
-              0.5 * (b3[g][1:] + b3[g][:-1])]
-        u[g, 0] = rho
-        u[g, 1], u[g, 2], u[g, 3] = rho * vx, rho * vy, rho * vz
-        u[g, 4] = p / (GAMMA - 1.0) + 0.5 * rho * (vx * vx + vy * vy + vz * vz) + 0.5 * sum(x * x for x in bc)
-    return u, b1, b2, b3
+    """pgen.divfree_blocks on the tree's blocks"""
+    return load_pgen().divfree_blocks(face_coords(tree, nx, ng, box), box, GAMMA, amp, b0, seed)
 
 
 def max_divb(tree, nx, ng, box, b1, b2, b3):
