@@ -2803,6 +2803,10 @@ int agx_mesh_set_levels(agx_mesh *m, const int32_t nrb[3], const double box[6], 
   if (m->mixed) { set_error("agx_mesh_set_levels: the refined mesh runs the task-level FP64 steps only"); return -2; }
   if (m->d.self_gravity) { set_error("agx_mesh_set_levels: multigrid on refined meshes is not built"); return -3; }
   const Geo &g = m->g;
+  if (g.ng % 2) {   // MeshRefinement constructor, mesh_refinement.cpp:56-62
+    set_error("agx_mesh_set_levels: an odd ghost width (%d) is incompatible with mesh refinement", g.ng);
+    return -8;
+  }
   const int nbx[3] = {g.bx1, g.bx2, g.bx3};
   for (int d = 0; d < 3; ++d) {
     if (nrb[d] < 1 || nbx[d] % 2) { set_error("agx_mesh_set_levels: root blocks >= 1 and even block sizes"); return -4; }

@@ -143,8 +143,9 @@ int agx_mesh_stream(agx_mesh *m, void **stream);
  * and EMF correction and for the ghost zones across levels, in the order of
  * TimeIntegratorTaskList (time_integrator.cpp:1460-1558); the entry points built on the
  * single-level lattice (agx_cycle, agx_initialize, agx_bvals_exchange, agx_autotune,
- * restart and VTK output) return -60.  Refused for the mixed form and with self-gravity
- * (multigrid on refined meshes is not built). */
+ * restart and VTK output) return -60.  Refused for the mixed form, with self-gravity
+ * (multigrid on refined meshes is not built) and for an odd nghost (as the
+ * MeshRefinement constructor refuses it, mesh_refinement.cpp:56-62; PPM takes 4). */
 int agx_mesh_set_levels(agx_mesh *m, const int32_t nrb[3], const double box[6], const int32_t *level,
                         const int64_t *lx);
 

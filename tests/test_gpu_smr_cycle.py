@@ -19,8 +19,8 @@ BOX = ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0))
 FIELDS = ("u", "w", "bcc", "b1f", "b2f", "b3f")
 
 
-def gpu_run(agx, tree, nx, state, ncyc, xorder=2, rsolver=0):
-    m = agx.SMRMesh(tree, nx, 2, BOX, cfl=0.3, xorder=xorder, rsolver=rsolver)
+def gpu_run(agx, tree, nx, state, ncyc, xorder=2, rsolver=0, ng=2):
+    m = agx.SMRMesh(tree, nx, ng, BOX, cfl=0.3, xorder=xorder, rsolver=rsolver)
     try:
         for name, a in zip(("u", "b1f", "b2f", "b3f"), state):
             m.set(name, a)
@@ -33,8 +33,8 @@ def gpu_run(agx, tree, nx, state, ncyc, xorder=2, rsolver=0):
         m.close()
 
 
-def oracle_run(tree, nx, state, ncyc, xorder=2, rsolver=0):
-    o = sc.OracleSMR(tree, nx, 2, BOX, cfl=0.3, xorder=xorder, rsolver=rsolver)
+def oracle_run(tree, nx, state, ncyc, xorder=2, rsolver=0, ng=2):
+    o = sc.OracleSMR(tree, nx, ng, BOX, cfl=0.3, xorder=xorder, rsolver=rsolver)
     try:
         for name, a in zip(("u", "b1f", "b2f", "b3f"), state):
             o.set(name, a)
@@ -92,6 +92,24 @@ def test_refined_mesh_equals_oracle(agx, levels):
     assert got["time"] == ref["time"]
     for f in FIELDS:
         same(got[f], ref[f], f"{levels} levels: {f}")
+
+
+@pytest.mark.parametrize("rsolver", [0, 2, 3])
+def test_refined_ppm_equals_oracle(agx, rsolver):
+    """PPM (NGHOST = 4, the even width mesh refinement needs) with HLLD, Roe and LHLLD"""
+    tree, nx, ng = sc.periodic_tree((2, 2, 2), refine=((0, 0, 0),)), 8, 4
+    state = sc.smooth_state(tree, nx, ng, BOX, amp=0.3)
+    got = gpu_run(agx, tree, nx, state, 3, xorder=3, rsolver=rsolver, ng=ng)
+    ref = oracle_run(tree, nx, state, 3, xorder=3, rsolver=rsolver, ng=ng)
+    assert got["time"] == ref["time"]
+    for f in FIELDS:
+        same(got[f], ref[f], f"PPM rsolver {rsolver}: {f}")
+
+
+def test_odd_ghost_width_refused(agx):
+    tree = sc.periodic_tree((2, 2, 2), refine=((0, 0, 0),))
+    with pytest.raises(agx.AgxError, match="odd ghost width"):
+        agx.SMRMesh(tree, 8, 3, BOX, xorder=3)
 
 
 @pytest.mark.parametrize("xorder,rsolver", [(2, 0), (2, 1), (1, 0)])

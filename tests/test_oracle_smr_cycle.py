@@ -86,6 +86,24 @@ def test_refined_cycle_conserves_and_keeps_interfaces(mesh):
     assert mis < 1e-14
 
 
+@pytest.mark.parametrize("rsolver", [0, 3])
+def test_refined_ppm_cycle_conserves_and_keeps_interfaces(rsolver):
+    """PPM on a refined mesh needs an even ghost width of at least 3: NGHOST = 4"""
+    tree, nx, ng = sc.periodic_tree((2, 2, 2), refine=((0, 0, 0), (1, 1, 1))), 8, 4
+    st = sc.smooth_state(tree, nx, ng, BOX, amp=0.3)
+    o = sc.OracleSMR(tree, nx, ng, BOX, xorder=3, rsolver=rsolver)
+    load(o, st)
+    o.initialize()
+    t0 = sc.totals(tree, nx, ng, BOX, o.get("u"))
+    assert o.cycle(1e9, 3) == 3
+    t1 = sc.totals(tree, nx, ng, BOX, o.get("u"))
+    b = [o.get(n) for n in ("b1f", "b2f", "b3f")]
+    o.close()
+    assert float((np.abs(t1 - t0) / np.maximum(np.abs(t0), 1.0)).max()) < 1e-14
+    assert sc.max_divb(tree, nx, ng, BOX, *b) < 1e-14
+    assert sc.interface_mismatch(tree, nx, ng, *b) < 1e-14
+
+
 def test_planted_omissions_fail_the_pins():
     tree, nx = two_levels()
     st = sc.smooth_state(tree, nx, 2, BOX, amp=0.3)
