@@ -2,13 +2,12 @@
 task order with refinement, time_integrator.cpp:1460-1558, composed from the oracle's
 task-level steps and refinement operators), which the device's SMRMesh is held to bit for
 bit:
-  - on one level, with the EMF correction left out, it is the oracle's lattice cycle bit
-    for bit (every other refinement step is then a same-level copy); with the correction,
-    which the reference runs on every MHD mesh (SEND_FLDFLX / RECV_FLDFLX are added
-    unconditionally, time_integrator.cpp:985-994), it differs from the lattice cycle only
-    at round-off: neighbouring blocks' copies of a shared edge EMF differ by a few ulps
-    (their copies of the shared faces are never exchanged) and the correction averages
-    them;
+  - on one level it is the oracle's lattice cycle bit for bit when the state is given as
+    global arrays (the EMF correction, which the reference runs on every MHD mesh,
+    time_integrator.cpp:985-994, then averages equal values, exactly); with blocks built
+    one by one, whose two copies of a periodic-boundary face differ by rounding, it
+    differs only at round-off, and equals the lattice cycle bit for bit with the EMF
+    correction left out (every other refinement step is a same-level copy there);
   - on two and three levels of a periodic mesh: mass, momentum and energy summed over all
     blocks are conserved to round-off, and the field through every refinement boundary
     agrees between the coarse face and the fine faces that cover it.  Leaving out the flux
@@ -19,6 +18,7 @@ import pytest
 
 from oracle import orc
 from tests import smr_cases as sc
+from tests.states import load_state, turbulent_state
 
 BOX = ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0))
 
@@ -29,7 +29,40 @@ def load(o, st):
 
 
 @pytest.mark.parametrize("xorder", [1, 2])
+def test_single_level_with_one_face_array_is_the_lattice_cycle(xorder):
+    """a state given as global arrays (each shared face stored twice with one value, as
+    tests.states and the benchmark generators build it): every block computes the same
+    value for a shared edge, the EMF correction averages equal values, which is exact
+    ((e + e) * 0.5 and (((e + e) + e) + e) * 0.25 both give e), and the whole reference
+    sequence equals the lattice cycle bit for bit"""
+    tree = sc.periodic_tree((2, 2, 2))
+    desc = orc.make_desc(nx=(16, 16, 16), bnx=(8, 8, 8), box=BOX, cfl=0.3, nghost=2, gamma=sc.GAMMA, xorder=xorder)
+    st = turbulent_state(desc, seed=3, mach=1.5)
+    r = orc.Oracle(desc)
+    load_state(r, st)
+    r.initialize()
+    o = sc.OracleSMR(tree, 8, 2, BOX, xorder=xorder)
+    load_state(o.o, st)
+    o.initialize()
+    for _ in range(4):
+        r.cycle(1e9)
+    assert o.cycle(1e9, 4) == 4
+    for n in ("u", "w", "b1f", "b2f", "b3f"):
+        assert np.array_equal(o.get(n), r.get(n)), n
+    o.close()
+
+
+def test_equal_averages_are_exact():
+    e = np.concatenate([np.random.default_rng(1).standard_normal(200000),
+                        1.0 + np.arange(1, 200001) * 2.0 ** -52, 4.0 / 3.0 + np.arange(1, 200001) * 2.0 ** -52])
+    assert np.array_equal((e + e) * 0.5, e)
+    assert np.array_equal((((e + e) + e) + e) * 0.25, e)
+
+
+@pytest.mark.parametrize("xorder", [1, 2])
 def test_single_level_is_the_lattice_cycle(xorder):
+    """blocks built one by one (pgen.divfree_blocks): the two copies of a face on a
+    periodic boundary differ by the rounding of their coordinates"""
     tree = sc.periodic_tree((2, 2, 2))
     st = sc.smooth_state(tree, 8, 2, BOX)
     desc = orc.make_desc(nx=(16, 16, 16), bnx=(8, 8, 8), box=BOX, cfl=0.3, nghost=2, gamma=sc.GAMMA, xorder=xorder)
