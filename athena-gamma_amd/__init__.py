@@ -748,7 +748,7 @@ class SMRBoundary:
     (about 100 kernels in one launch)."""
 
     def __init__(self, tree, nx, ng, gamma, dfloor, pfloor, box=(1.0, 1.0, 1.0), own_stream=True, stream=None,
-                 layout=0, tables=None):
+                 layout=0, tables=None, iso=False):
         smr = _load_smr()
         self.stream, self.graph, self.own = None, None, False
         if stream is not None:
@@ -759,6 +759,7 @@ class SMRBoundary:
             self.stream, self.own = st.value, True
         self.tree, self.nx, self.ng = tree, nx, ng
         self.gamma, self.dfloor, self.pfloor = gamma, dfloor, pfloor
+        self.iso, self.nh = bool(iso), 4 if iso else 5   # hydro variables (NHYDRO)
         nb = self.nb = len(tree.leaves)
         self.geom = RefGeom(nx, nx, nx, ng, ng, layout)
         self._keep = []
@@ -808,7 +809,8 @@ class SMRBoundary:
         self._lists.append(lst)
         return lst
 
-    def exchange_cells(self, u, cu, nvar=5):
+    def exchange_cells(self, u, cu, nvar=None):
+        nvar = self.nh if nvar is None else nvar
         self.c_same.copy_cc(nvar, u, u)
         self.c_send.restrict_cc(nvar, u, self.fc, cu)
         self.c_from_finer.copy_cc(nvar, cu, u)
@@ -827,24 +829,24 @@ class SMRBoundary:
         if fused:   # four launches per slot
             L = lib()
             for s in self.slots:
-                _check(L.agx_ref_restrict_all(s["rcc"].h, *[x.h for x in s["rfc"]], 5, u, *b, self.fc, cu, *cb,
-                                              self.stream), "agx_ref_restrict_all")
-                s["c2p"].cons_to_prim(cu, *cb, cw, cbcc, self.gamma, self.dfloor, self.pfloor)
-                _check(L.agx_ref_prolong_all(s["pcc"].h, *[x.h for x in s["pfc"]], 5, cw, *cb, self.cc, w, *b,
-                                             self.fc, self.stream), "agx_ref_prolong_all")
-                _check(L.agx_ref_internal_to_cons(s["pint"].h, *b, self.fc, w, bcc, u, self.gamma, 0, self.stream),
-                       "agx_ref_internal_to_cons")
+                _check(L.agx_ref_restrict_all(s["rcc"].h, *[x.h for x in s["rfc"]], self.nh, u, *b, self.fc, cu,
+                                              *cb, self.stream), "agx_ref_restrict_all")
+                s["c2p"].cons_to_prim(cu, *cb, cw, cbcc, self.gamma, self.dfloor, self.pfloor, self.iso)
+                _check(L.agx_ref_prolong_all(s["pcc"].h, *[x.h for x in s["pfc"]], self.nh, cw, *cb, self.cc, w,
+                                             *b, self.fc, self.stream), "agx_ref_prolong_all")
+                _check(L.agx_ref_internal_to_cons(s["pint"].h, *b, self.fc, w, bcc, u, self.gamma, int(self.iso),
+                                                  self.stream), "agx_ref_internal_to_cons")
             return
         for s in self.slots:
-            s["rcc"].restrict_cc(5, u, self.fc, cu)
+            s["rcc"].restrict_cc(self.nh, u, self.fc, cu)
             for c in range(3):
                 s["rfc"][c].restrict_fc(b[c], self.fc, cb[c])
-            s["c2p"].cons_to_prim(cu, *cb, cw, cbcc, self.gamma, self.dfloor, self.pfloor)
-            s["pcc"].prolong_cc(5, cw, self.cc, w, self.fc)
+            s["c2p"].cons_to_prim(cu, *cb, cw, cbcc, self.gamma, self.dfloor, self.pfloor, self.iso)
+            s["pcc"].prolong_cc(self.nh, cw, self.cc, w, self.fc)
             for c in range(3):
                 s["pfc"][c].prolong_fc_shared(cb[c], self.cc, b[c], self.fc)
             s["pint"].prolong_fc_internal(*b, self.fc)
-            s["p2c"].prim_to_cons(w, *b, bcc, u, self.gamma)
+            s["p2c"].prim_to_cons(w, *b, bcc, u, self.gamma, self.iso)
 
     def correct_emf(self, e1, e2, e3):
         self.e_same.emf_same_level(e1, e2, e3, self.e_buf_s.ptr, self.e_ss)
@@ -901,9 +903,9 @@ class SMRBoundary:
 class SMRMesh:
     """MHD on a static-refinement mesh on the device: the leaves of `tree` (smr.SMRTree)
     as the MeshBlocks of one agx_mesh (agx_mesh_set_levels), cubic blocks of nx^3 cells,
-    ng ghosts, FP64, adiabatic EOS, periodic or block boundaries only.  Each VL2 stage
-    runs the task order of TimeIntegratorTaskList with refinement (time_integrator.cpp:
-    1460-1558):
+    ng ghosts (even), FP64, adiabatic EOS or isothermal (eos = 1, NHYDRO = 4, sound speed
+    iso_cs), periodic or block boundaries only.  Each VL2 stage runs the task order of
+    TimeIntegratorTaskList with refinement (time_integrator.cpp:1460-1558):
       CalculateHydroFlux              agx_calculate_fluxes (donor cell in stage 1)
       Send/ReceiveAndCorrectHydroFlux flux_from_finer (flux_correction_cc.cpp:77-283)
       CalculateEMF                    agx_compute_corner_e
@@ -916,12 +918,14 @@ class SMRMesh:
     launch runs on the mesh's stream.  Coordinates follow the reference's Coordinates on
     a uniform mesh (smr.mesh_coords)."""
 
-    def __init__(self, tree, nx, ng=2, box=((0.0, 1.0),) * 3, gamma=5.0 / 3.0, cfl=0.3, xorder=2, rsolver=0):
+    def __init__(self, tree, nx, ng=2, box=((0.0, 1.0),) * 3, gamma=5.0 / 3.0, cfl=0.3, xorder=2, rsolver=0, eos=0,
+                 iso_cs=1.0):
         smr = _load_smr()
         self.tree, self.nx, self.ng, self.xorder = tree, nx, ng, xorder
         nb = self.nb = len(tree.leaves)
+        self.nh = 4 if eos == 1 else 5
         desc = make_desc(nx=(nb * nx, nx, nx), bnx=(nx, nx, nx), box=((0.0, float(nb)), (0.0, 1.0), (0.0, 1.0)),
-                         nghost=ng, xorder=xorder, rsolver=rsolver, gamma=gamma, cfl=cfl)
+                         nghost=ng, xorder=xorder, rsolver=rsolver, gamma=gamma, cfl=cfl, eos=eos, iso_cs=iso_cs)
         self.desc = desc
         self.m = Mesh(desc)
         self._keep = []
@@ -930,13 +934,13 @@ class SMRMesh:
             tabs = (smr.mesh_coords(tree, nx, ng, box), smr.mesh_coords(tree, nx, ng, box, coarse=True))
             self.tables = tabs
             self.bnd = SMRBoundary(tree, nx, ng, gamma, desc.dfloor, desc.pfloor, stream=self.m.stream(), layout=1,
-                                   tables=tabs)
+                                   tables=tabs, iso=eos == 1)
             plan = smr.flux_plan(tree)
             self.flux = RefList(self.bnd.geom, "flux_from_finer", [RefItem(**d) for d in plan], nb, nb,
                                 stream=self.m.stream()) if plan else None
             nc = nx // 2 + 2 * ng
-            self.cu = DeviceArray(np.zeros((nb, 5, nc, nc, nc)))
-            self.cw = DeviceArray(np.zeros((nb, 5, nc, nc, nc)))
+            self.cu = DeviceArray(np.zeros((nb, self.nh, nc, nc, nc)))
+            self.cw = DeviceArray(np.zeros((nb, self.nh, nc, nc, nc)))
             self.cbcc = DeviceArray(np.zeros((nb, 3, nc, nc, nc)))
             self.cb = [DeviceArray(np.zeros(sh)) for sh in ((nb, nc, nc, nc + 1), (nb, nc, nc + 1, nc),
                                                               (nb, nc + 1, nc, nc))]
@@ -976,7 +980,7 @@ class SMRMesh:
         m.calculate_fluxes(1 if s == 1 else self.xorder)
         if self.flux is not None:
             f = [p("fx1"), p("fx2"), p("fx3")]
-            self.flux.flux_from_finer(5, *f, self.bnd.fc, *f)
+            self.flux.flux_from_finer(self.nh, *f, self.bnd.fc, *f)
         m.corner_e()
         self.bnd.correct_emf(p("e1"), p("e2"), p("e3"))
         m.integrate(s)

@@ -71,12 +71,15 @@ RG make_rg(const agx_ref_geom &g) {
 __device__ __forceinline__ int64_t ix(int d3, int d2, int d1, int nvar, int b, int n, int k, int j, int i) {
   return ((((int64_t)b * nvar + n) * d3 + k) * d2 + j) * (int64_t)d1 + i;
 }
+// slabs per block of a mesh array of nvar variables: the mesh's hydro arrays (u, w, the
+// fluxes) hold 5 slabs for either EOS, the isothermal one using the first 4
+__device__ __forceinline__ int mesh_slabs(int nvar) { return nvar == 4 ? 5 : nvar; }
 // the same element of a block-side array (MeshBlock data, not a coarse buffer): the dense
 // layout above, or the mesh's padded one, where every cell, face and edge array of a
-// block has nvar slabs of bS elements, bJ rows of bP elements per plane
+// block has its slabs of bS elements, bJ rows of bP elements per plane
 __device__ __forceinline__ int64_t bx(const RG &r, int d3, int d2, int d1, int nvar, int b, int n, int k, int j,
                                       int i) {
-  if (r.bS) return ((int64_t)b * nvar + n) * r.bS + ((int64_t)k * r.bJ + j) * r.bP + i;
+  if (r.bS) return ((int64_t)b * mesh_slabs(nvar) + n) * r.bS + ((int64_t)k * r.bJ + j) * r.bP + i;
   return ix(d3, d2, d1, nvar, b, n, k, j, i);
 }
 // an array of either side, for the operators that move data both ways
@@ -86,7 +89,7 @@ struct Lay {
   int J, P;
 };
 __device__ __forceinline__ int64_t lx(const Lay &L, int nvar, int b, int n, int k, int j, int i) {
-  if (L.S) return ((int64_t)b * nvar + n) * L.S + ((int64_t)k * L.J + j) * L.P + i;
+  if (L.S) return ((int64_t)b * mesh_slabs(nvar) + n) * L.S + ((int64_t)k * L.J + j) * L.P + i;
   return ix(L.d3, L.d2, L.d1, nvar, b, n, k, j, i);
 }
 

@@ -402,7 +402,8 @@ typedef struct {
                               buffers are always dense): AGX_REF_LAYOUT_DENSE, or
                               AGX_REF_LAYOUT_MESH for the arrays of an agx_mesh with the
                               same block size and ghost width (agx_array_dims), so that the
-                              operators work on a mesh's own arrays */
+                              operators work on a mesh's own arrays (whose hydro arrays
+                              hold 5 slabs per block for either EOS) */
 } agx_ref_geom;
 enum { AGX_REF_LAYOUT_DENSE = 0, AGX_REF_LAYOUT_MESH = 1 };
 typedef struct {

@@ -262,32 +262,35 @@ def mhd_state(t, nx, ng, seed, floors=True):
             for k, v in arrs.items()}
 
 
-def run_oracle_mhd(orc, t, A, nx, ng, ftab, ctab, order="reference", gamma=GAMMA, dfloor=DFLOOR, pfloor=PFLOOR):
+def run_oracle_mhd(orc, t, A, nx, ng, ftab, ctab, order="reference", gamma=GAMMA, dfloor=DFLOOR, pfloor=PFLOOR,
+                   iso=False):
     """ghost exchange of u (cells) and b (faces) across levels, then ProlongateBoundaries
-    for MHD, on copies of the arrays in A"""
+    for MHD, on copies of the arrays in A (u, w, cu, cw with NHYDRO = 4 variables for the
+    isothermal EOS)"""
     smr = load_smr()
     cc, fc = smr.cc_plan(t, nx, ng), smr.fc_plan(t, nx, ng)
     slots, seq = smr.mhd_prolong_plan(t, nx, ng, cc, fc)
     A = {k: (v.copy() if not isinstance(v, list) else [x.copy() for x in v]) for k, v in A.items()}
     g = geom(orc, nx, ng)
     I = lambda ds: [orc.RefItem(**d) for d in ds]   # noqa: E731
+    nh = 4 if iso else 5
     u, cu = A["u"], A["cu"]
-    orc.ref_copy_cc(g, 5, u, 0, u, 0, I(cc.copy_same))
-    orc.ref_restrict_cc(g, 5, u, ftab, cu, I(cc.restrict_send))
-    orc.ref_copy_cc(g, 5, cu, 1, u, 0, I(cc.copy_from_finer))
-    orc.ref_copy_cc(g, 5, u, 0, cu, 1, I(cc.copy_to_finer))
+    orc.ref_copy_cc(g, nh, u, 0, u, 0, I(cc.copy_same))
+    orc.ref_restrict_cc(g, nh, u, ftab, cu, I(cc.restrict_send))
+    orc.ref_copy_cc(g, nh, cu, 1, u, 0, I(cc.copy_from_finer))
+    orc.ref_copy_cc(g, nh, u, 0, cu, 1, I(cc.copy_to_finer))
     A["b"], A["cb"] = run_oracle_fc(orc, t, A["b"], A["cb"], nx, ng, ftab, ctab, plan=_exchange_only(fc),
                                     order=order)
     for st in (seq if order == "reference" else slots):
-        orc.ref_restrict_cc(g, 5, u, ftab, cu, I(st["restrict_cc"]))
+        orc.ref_restrict_cc(g, nh, u, ftab, cu, I(st["restrict_cc"]))
         for c in range(3):
             orc.ref_restrict_fc(g, c, A["b"][c], ftab, A["cb"][c], I(st["restrict_fc"][c]))
-        orc.ref_cons_to_prim(g, cu, A["cb"], A["cw"], A["cbcc"], gamma, dfloor, pfloor, 0, I(st["c2p"]))
-        orc.ref_prolong_cc(g, 5, A["cw"], ctab, A["w"], ftab, I(st["prolong_cc"]))
+        orc.ref_cons_to_prim(g, cu, A["cb"], A["cw"], A["cbcc"], gamma, dfloor, pfloor, int(iso), I(st["c2p"]))
+        orc.ref_prolong_cc(g, nh, A["cw"], ctab, A["w"], ftab, I(st["prolong_cc"]))
         for c in range(3):
             orc.ref_prolong_fc_shared(g, c, A["cb"][c], ctab, A["b"][c], ftab, I(st["prolong_fc"][c]))
         orc.ref_prolong_fc_internal(g, A["b"][0], A["b"][1], A["b"][2], ftab, I(st["prolong_int"]))
-        orc.ref_prim_to_cons(g, A["w"], A["b"][0], A["b"][1], A["b"][2], A["bcc"], u, gamma, 0, I(st["p2c"]))
+        orc.ref_prim_to_cons(g, A["w"], A["b"][0], A["b"][1], A["b"][2], A["bcc"], u, gamma, int(iso), I(st["p2c"]))
     return A
 
 

@@ -33,9 +33,12 @@ def face_coords(tree, nx, ng, box):
     return load_smr().face_positions(tree, nx, ng, box)
 
 
-def smooth_state(tree, nx, ng, box, amp=0.2, b0=(0.6, 0.0, 0.0), seed=0):
-    """pgen.divfree_blocks on the tree's blocks"""
-    return load_pgen().divfree_blocks(face_coords(tree, nx, ng, box), box, GAMMA, amp, b0, seed)
+def smooth_state(tree, nx, ng, box, amp=0.2, b0=(0.6, 0.0, 0.0), seed=0, iso=False):
+    """pgen.divfree_blocks on the tree's blocks (iso: the energy slot zero)"""
+    u, b1, b2, b3 = load_pgen().divfree_blocks(face_coords(tree, nx, ng, box), box, GAMMA, amp, b0, seed)
+    if iso:
+        u[:, 4] = 0.0
+    return u, b1, b2, b3
 
 
 def max_divb(tree, nx, ng, box, b1, b2, b3):
@@ -88,13 +91,14 @@ def interface_mismatch(tree, nx, ng, b1, b2, b3):
 
 
 def totals(tree, nx, ng, box, u):
-    """volume sums over every block's interior of the five conserved variables"""
+    """volume sums over every block's interior of the conserved variables"""
     xf = face_coords(tree, nx, ng, box)
     s = slice(ng, ng + nx)
-    out = np.zeros(5)
+    nv = u.shape[1]
+    out = np.zeros(nv)
     for g in range(len(tree.leaves)):
         vol = float(np.diff(xf[g][0])[ng]) * float(np.diff(xf[g][1])[ng]) * float(np.diff(xf[g][2])[ng])
-        out += u[g][:, s, s, s].reshape(5, -1).sum(1) * vol
+        out += u[g][:, s, s, s].reshape(nv, -1).sum(1) * vol
     return out
 
 
@@ -105,14 +109,15 @@ class OracleSMR:
     arrays; the prolongation in the reference's per-block neighbour order."""
 
     def __init__(self, tree, nx, ng=2, box=((0.0, 1.0),) * 3, cfl=0.3, xorder=2, rsolver=0, emf_skip=(),
-                 flux_skip=False):
+                 flux_skip=False, eos=0, iso_cs=1.0):
         smr = load_smr()
         self.tree, self.nx, self.ng, self.xorder = tree, nx, ng, xorder
         self.emf_skip = emf_skip   # steps of the EMF correction left out (emf_cases.run_oracle)
         self.flux_skip = flux_skip   # leave out the hydro flux correction
         nb = len(tree.leaves)
         desc = orc.make_desc(nx=(nb * nx, nx, nx), bnx=(nx, nx, nx), box=((0.0, float(nb)), (0.0, 1.0), (0.0, 1.0)),
-                             nghost=ng, xorder=xorder, rsolver=rsolver, gamma=GAMMA, cfl=cfl)
+                             nghost=ng, xorder=xorder, rsolver=rsolver, gamma=GAMMA, cfl=cfl, eos=eos, iso_cs=iso_cs)
+        self.iso, self.nh = eos == 1, 4 if eos == 1 else 5
         self.desc = desc
         self.o = orc.Oracle(desc)
         self.o.set_levels(tree.nrb, box, tree.leaves)
@@ -121,7 +126,7 @@ class OracleSMR:
         self.g = orc.RefGeom(nx, nx, nx, ng, ng, 0)
         self.flux = [orc.RefItem(**d) for d in smr.flux_plan(tree)]
         nc = nx // 2 + 2 * ng
-        self.A = dict(cu=np.zeros((nb, 5, nc, nc, nc)), cw=np.zeros((nb, 5, nc, nc, nc)),
+        self.A = dict(cu=np.zeros((nb, self.nh, nc, nc, nc)), cw=np.zeros((nb, self.nh, nc, nc, nc)),
                       cbcc=np.zeros((nb, 3, nc, nc, nc)),
                       cb=[np.zeros(s) for s in ((nb, nc, nc, nc + 1), (nb, nc, nc + 1, nc), (nb, nc + 1, nc, nc))])
 
@@ -135,12 +140,15 @@ class OracleSMR:
         return self.o.time()
 
     def boundary(self):
-        o = self.o
-        A = dict(self.A, u=o.view("u"), w=o.view("w"), bcc=o.view("bcc"), b=[o.view(n) for n in ("b1f", "b2f", "b3f")])
+        o, nh = self.o, self.nh
+        # the oracle's hydro arrays hold 5 slots; the isothermal operators take 4
+        A = dict(self.A, u=np.ascontiguousarray(o.view("u")[:, :nh]), w=np.ascontiguousarray(o.view("w")[:, :nh]),
+                 bcc=o.view("bcc"), b=[o.view(n) for n in ("b1f", "b2f", "b3f")])
         R = gc.run_oracle_mhd(orc, self.tree, A, self.nx, self.ng, self.ftab, self.ctab, order="reference",
-                              gamma=GAMMA, dfloor=self.desc.dfloor, pfloor=self.desc.pfloor)
-        for k in ("u", "w", "bcc"):
-            o.view(k)[...] = R[k]
+                              gamma=GAMMA, dfloor=self.desc.dfloor, pfloor=self.desc.pfloor, iso=self.iso)
+        for k in ("u", "w"):
+            o.view(k)[:, :nh] = R[k]
+        o.view("bcc")[...] = R["bcc"]
         for c, n in enumerate(("b1f", "b2f", "b3f")):
             o.view(n)[...] = R["b"][c]
         self.A = {k: R[k] for k in ("cu", "cw", "cbcc", "cb")}
@@ -156,8 +164,10 @@ class OracleSMR:
         o = self.o
         o.L.orc_calculate_fluxes(o.h, 1 if s == 1 else self.xorder)
         if self.flux and not self.flux_skip:
-            f = [o.view(n) for n in ("fx1", "fx2", "fx3")]
-            orc.ref_flux_from_finer(self.g, 5, *f, self.ftab, *f, self.flux)
+            f = [np.ascontiguousarray(o.view(n)[:, :self.nh]) for n in ("fx1", "fx2", "fx3")]
+            orc.ref_flux_from_finer(self.g, self.nh, *f, self.ftab, *f, self.flux)
+            for n, x in zip(("fx1", "fx2", "fx3"), f):
+                o.view(n)[:, :self.nh] = x
         o.L.orc_corner_e(o.h)
         E, _ = ec.run_oracle(orc, self.tree, [o.view(n) for n in ("e1", "e2", "e3")], self.nx, self.ng, self.ftab,
                              skip=self.emf_skip)

@@ -132,6 +132,31 @@ def test_refined_ppm_equals_oracle(agx, rsolver):
         same(got[f], ref[f], f"PPM rsolver {rsolver}: {f}")
 
 
+def test_refined_isothermal_equals_oracle(agx):
+    tree, nx = sc.periodic_tree((2, 2, 2), refine=((0, 0, 0),)), 8
+    u, b1, b2, b3 = sc.smooth_state(tree, nx, 2, BOX, amp=0.3, iso=True)
+    m = agx.SMRMesh(tree, nx, 2, BOX, cfl=0.3, eos=1, iso_cs=0.8)
+    o = sc.OracleSMR(tree, nx, 2, BOX, cfl=0.3, eos=1, iso_cs=0.8)
+    try:
+        m.set("u", u[:, :4])
+        o.set("u", u)
+        for name, a in zip(("b1f", "b2f", "b3f"), (b1, b2, b3)):
+            m.set(name, a)
+            o.set(name, a)
+        m.initialize()
+        o.initialize()
+        assert m.cycle(1e9, 3) == 3
+        assert o.cycle(1e9, 3) == 3
+        assert m.time() == o.time()
+        same(m.get("u"), o.get("u")[:, :4], "isothermal: u")
+        same(m.get("w")[:, :4], o.get("w")[:, :4], "isothermal: w")
+        for f in ("bcc", "b1f", "b2f", "b3f"):
+            same(m.get(f), o.get(f), f"isothermal: {f}")
+    finally:
+        m.close()
+        o.close()
+
+
 def test_odd_ghost_width_refused(agx):
     tree = sc.periodic_tree((2, 2, 2), refine=((0, 0, 0),))
     with pytest.raises(agx.AgxError, match="odd ghost width"):

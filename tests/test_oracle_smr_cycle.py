@@ -137,6 +137,23 @@ def test_refined_ppm_cycle_conserves_and_keeps_interfaces(rsolver):
     assert sc.interface_mismatch(tree, nx, ng, *b) < 1e-14
 
 
+def test_refined_isothermal_cycle_conserves_and_keeps_interfaces():
+    tree, nx = two_levels()
+    st = sc.smooth_state(tree, nx, 2, BOX, amp=0.3, iso=True)
+    o = sc.OracleSMR(tree, nx, 2, BOX, eos=1, iso_cs=0.8)
+    load(o, st)
+    o.initialize()
+    t0 = sc.totals(tree, nx, 2, BOX, o.get("u")[:, :4])
+    assert o.cycle(1e9, 4) == 4
+    t1 = sc.totals(tree, nx, 2, BOX, o.get("u")[:, :4])
+    b = [o.get(n) for n in ("b1f", "b2f", "b3f")]
+    assert np.all(o.get("u")[:, 4] == 0.0)
+    o.close()
+    assert float((np.abs(t1 - t0) / np.maximum(np.abs(t0), 1.0)).max()) < 1e-14
+    assert sc.max_divb(tree, nx, 2, BOX, *b) < 1e-14
+    assert sc.interface_mismatch(tree, nx, 2, *b) < 1e-14
+
+
 def test_planted_omissions_fail_the_pins():
     tree, nx = two_levels()
     st = sc.smooth_state(tree, nx, 2, BOX, amp=0.3)
