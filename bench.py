@@ -786,6 +786,8 @@ def main():
     ap.add_argument("--no-smr", action="store_true", help="skip the three-level static-refinement MHD leg")
     ap.add_argument("--smr-block", type=int, default=32)
     ap.add_argument("--smr-steps", type=int, default=10)
+    ap.add_argument("--smr-only", action="store_true",
+                    help="run only the static-refinement leg and print its JSON object (for profiling)")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
                     help="skip downloading w and B after both runs (mixed-vs-FP64 difference, div B)")
     ap.add_argument("--large", action="store_true",
@@ -809,6 +811,9 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     agx, pgen = load_pkg()
+    if args.smr_only:
+        print(json.dumps(smr_cycle_bench(agx, pgen, args)), flush=True)
+        return
     if args.large:
         large_bench(agx, pgen, args)
         return
