@@ -199,7 +199,11 @@ struct agx_mesh {
     hipGraphExec_t exec = nullptr;
     double *before[agx::ARR_TOTAL];
     double *after[agx::ARR_TOTAL];
+    int coffset_before = 0, coffset_after = 0;   // multigrid red-black offset (self-gravity)
+    int gen = 0;                                  // graph_gen when recorded
   };
+  bool capturing = false;            // a cycle is being recorded into a graph
+  int graph_gen = 0;                 // bumped by every setter that changes the recorded work
   std::vector<CycleGraph> graphs;
 };
 

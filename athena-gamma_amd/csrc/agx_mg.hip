@@ -2253,7 +2253,9 @@ int mg_solve(agx_mesh *m, int stage) {
       iters++;
     }
     subtract_average(m, 1);
-    m->last_norm = defect_norm(m);
+    // the defect norm is only reported (agx_mg_last); a cycle being recorded into a graph
+    // cannot read it back, so it is left out there
+    m->last_norm = m->capturing ? NAN : defect_norm(m);
   }
   m->last_iters = iters;
   int64_t nret = (int64_t)(g.bx1 + 2) * (g.bx2 + 2) * (g.bx3 + 2) * g.nb;
@@ -2292,6 +2294,7 @@ int mg_autotune(agx_mesh *m) {
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   m->mg_fuse = best_f;
+  m->graph_gen++;
   return rc;
 }
 
@@ -2541,11 +2544,13 @@ extern "C" int agx_mg_set_source_mask_sphere(agx_mesh *m, double xc, double yc, 
   m->srcmask = radius > 0.0 ? 1 : 0;
   m->srcmask_c[0] = xc; m->srcmask_c[1] = yc; m->srcmask_c[2] = zc;
   m->srcmask_r2 = radius * radius;   // collapse.cpp:68: rc2 = rc*rc
+  m->graph_gen++;
   return 0;
 }
 
 extern "C" int agx_set_mg_coarse(agx_mesh *m, int32_t on) {
   if (!m) { set_error("agx_set_mg_coarse: null mesh"); return -1; }
   m->mg_coarse = on ? 1 : 0;
+  m->graph_gen++;
   return 0;
 }

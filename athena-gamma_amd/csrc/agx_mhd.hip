@@ -3534,7 +3534,9 @@ static void graph_reset(agx_mesh *m) {
 static int cycle_device(agx_mesh *m) {
   const dim3 sg = stage_grid(m->g, m->tile);
   int rc = stage_fused(m, 1, false);
+  if (!rc && m->d.self_gravity) rc = mg_solve(m, 1);   // main.cpp:464-465
   if (!rc) rc = stage_fused(m, 2, true);
+  if (!rc && m->d.self_gravity) rc = mg_solve(m, 2);
   if (rc) return rc;
   hipLaunchKernelGGL(k_min_final, dim3(1), dim3(256), 0, m->stream, m->dtpart, (int64_t)sg.x * sg.y, m->scal);
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, m->stream, m->scal, m->d.cfl);
@@ -3546,21 +3548,28 @@ static int cycle_device(agx_mesh *m) {
 // captured on first use; the host's register table is left as the cycle leaves it.
 static int graph_for_cycle(agx_mesh *m, hipGraphExec_t *exec) {
   double **A = m->arr.a;
+  if (!m->graphs.empty() && m->graphs[0].gen != m->graph_gen) graph_reset(m);   // settings changed
   for (auto &cg : m->graphs)
-    if (std::memcmp(cg.before, A, sizeof(cg.before)) == 0) {
+    if (std::memcmp(cg.before, A, sizeof(cg.before)) == 0 && cg.coffset_before == m->coffset) {
       std::memcpy(A, cg.after, sizeof(cg.after));
+      m->coffset = cg.coffset_after;
       *exec = cg.exec;
       return 0;
     }
   if (m->graphs.size() >= 4) graph_reset(m);
   agx_mesh::CycleGraph cg;
   std::memcpy(cg.before, A, sizeof(cg.before));
+  cg.coffset_before = m->coffset;
+  cg.gen = m->graph_gen;
   hipGraph_t graph = nullptr;
   AGX_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+  m->capturing = true;
   int rc = cycle_device(m);
+  m->capturing = false;
   hipError_t e = hipStreamEndCapture(m->stream, &graph);
   if (rc || e != hipSuccess) {
     std::memcpy(A, cg.before, sizeof(cg.before));
+    m->coffset = cg.coffset_before;
     if (graph) hipGraphDestroy(graph);
     return rc ? rc : hip_fail(e, "hipStreamEndCapture");
   }
@@ -3568,9 +3577,11 @@ static int graph_for_cycle(agx_mesh *m, hipGraphExec_t *exec) {
   hipGraphDestroy(graph);
   if (e != hipSuccess) {
     std::memcpy(A, cg.before, sizeof(cg.before));
+    m->coffset = cg.coffset_before;
     return hip_fail(e, "hipGraphInstantiate");
   }
   std::memcpy(cg.after, A, sizeof(cg.after));
+  cg.coffset_after = m->coffset;
   m->graphs.push_back(cg);
   *exec = cg.exec;
   return 0;
@@ -3594,6 +3605,10 @@ static int cycle_graphs(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done
     m->dt = m->scal_host[SC_DT];
     m->time = m->scal_host[SC_TIME];
     m->ncycle++;
+    if (m->d.self_gravity) {   // the recorded solves: a fixed V-cycle count, no norm read back
+      m->last_norm = NAN;
+      m->last_iters = m->mg_niter;
+    }
     ++n;
   }
   if (done) *done = n;
@@ -3610,7 +3625,11 @@ int agx_set_graph(agx_mesh *m, int32_t enable) {
 int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
   if (!m) { set_error("null mesh"); return -1; }
   if (int rc = refuse_tree(m, "agx_cycle")) return rc;
-  if (m->graph_on && !m->timer.enabled && !m->d.self_gravity) return cycle_graphs(m, tlim, ncycles, done);
+  // with self-gravity only the fixed-count solve (niteration, SolveIterativeFixedTimes) is
+  // recorded: the threshold form decides its V-cycle count from norms read on the host
+  const bool sg_fixed = m->d.self_gravity && m->mg_on && m->mg_eps < 0.0 && !m->mp_on;
+  if (m->graph_on && !m->timer.enabled && (!m->d.self_gravity || sg_fixed))
+    return cycle_graphs(m, tlim, ncycles, done);
   int32_t n = 0;
   int rc = 0;
   const dim3 sg = stage_grid(m->g, m->tile);
@@ -3650,6 +3669,7 @@ int agx_mg_set_mode(agx_mesh *m, int mode, double threshold, int32_t niter) {
     set_error("either threshold or niteration must be set (mg_gravity.cpp:55-63)"); return -3;
   }
   m->mg_mode = mode; m->mg_eps = threshold; m->mg_niter = niter;
+  m->graph_gen++;
   return 0;
 }
 
@@ -3722,6 +3742,7 @@ int agx_set_mg_fuse(agx_mesh *m, int32_t mode) {
   if (!m) { set_error("null mesh"); return -1; }
   if (mode < -1 || mode > 1) { set_error("red-black pass form %d not in {-1, 0, 1}", mode); return -2; }
   m->mg_fuse = mode;
+  m->graph_gen++;
   return 0;
 }
 

@@ -110,3 +110,35 @@ def test_graph_rebuilt_after_tile_change(agx):
             o.cycle(1e9)
         for f in FIELDS:
             same(m.get(f), o.get(f), f"{f} tile {tile}")
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graph_with_fixed_count_gravity_equals_eager(agx, precision):
+    """Self-gravity with a fixed V-cycle count (niteration): both multigrid solves of a
+    cycle are recorded with the MHD stages.  Graph replays, interleaved with eager cycles
+    and downloads, equal the eager path bit for bit, potential included."""
+    desc = turb_desc(self_gravity=1, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=3, precision=precision)
+    st = turbulent_state(desc, seed=9, mach=1.5, drho=0.5)
+    runs = []
+    for graph in (False, True):
+        m = agx.Mesh(desc)
+        load_state(m, st)
+        m.initialize()
+        snaps = []
+        for n, g in ((2, graph), (3, graph), (1, False), (2, graph)):
+            m.set_graph(g)
+            assert m.cycle(1e9, n) == n
+            snap = {f: m.get(f) for f in FIELDS + ("phi",)}
+            snap["time"] = m.time()
+            snap["iters"] = m.mg_last()[1]
+            snaps.append(snap)
+        if graph:
+            norm, iters = m.mg_last()
+            assert iters == 3 and np.isnan(norm)
+        runs.append(snaps)
+        m.close()
+    for k, (a, b) in enumerate(zip(*runs)):
+        assert a["time"] == b["time"], k
+        assert a["iters"] == b["iters"] == 3
+        for f in FIELDS + ("phi",):
+            same(b[f], a[f], f"segment {k}: {f} (precision {precision})")
