@@ -398,6 +398,47 @@ def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None, sensitivity=False
     return out
 
 
+def sg_graph_bench(agx, pgen, N, B, steps, niter, bx=0.1, mach=2.0):
+    """The self-gravity workload of sg_run (lognormal density, rms Mach `mach`, Bx, PLM +
+    HLLD + CT, FP64) with a fixed V-cycle count per solve (niteration = `niter`, FMG
+    first): `steps` cycles eager, then `steps` cycles replayed as one hipGraph per cycle
+    (both multigrid solves recorded with the MHD stages).  Returns both rates."""
+    base = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4, gamma=5.0 / 3.0,
+                         xorder=2, nghost=2, rsolver=0)
+    U, b1, b2, b3 = pgen.turbulence_lognormal(base, seed=5, mach=mach, sigma_lnrho=1.0, beta=1.0, bx=bx)
+    ub = agx.scatter_cells(base, U)
+    del U
+    faces = pgen.scatter_faces(base, b1, b2, b3)
+    del b1, b2, b3
+    desc = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4, gamma=5.0 / 3.0, xorder=2,
+                         nghost=2, rsolver=0, self_gravity=1, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0,
+                         mg_niter=niter)
+    out = {}
+    for graph in (False, True):
+        m = agx.Mesh(desc)
+        m.set("u", ub)
+        for n, f in zip(("b1f", "b2f", "b3f"), faces):
+            m.set(n, f)
+        m.initialize()
+        m.autotune()
+        m.set_graph(graph)
+        m.cycle(1e30, 2)
+        m.sync()
+        t0 = time.perf_counter()
+        assert m.cycle(1e30, steps) == steps
+        m.sync()
+        sec = time.perf_counter() - t0
+        out["graph" if graph else "eager"] = float(N) ** 3 * steps / sec
+        m.close()
+    res = {"workload": f"{N}^3 self-gravitating MHD (lognormal rho, rms Mach {mach:g}, Bx={bx:g}), "
+                       f"{(N // B) ** 3} MeshBlocks of {B}^3, VL2+PLM+HLLD+CT FP64, FMG + {niter} V-cycles per solve",
+           "eager": out["eager"], "graph": out["graph"], "graph_vs_eager": out["graph"] / out["eager"],
+           "unit": "zone-cycles/s", "steps": steps}
+    log(f"[sg-graph] fixed {niter} V-cycles per solve: eager {out['eager']:.4e}, graph {out['graph']:.4e} "
+        f"zone-cycles/s ({res['graph_vs_eager']:.3f}x)")
+    return res
+
+
 def large_bench(agx, pgen, args):
     """The large single-GPU run: 512^3 periodic MHD + multigrid self-gravity in 512
     MeshBlocks of 64^3, VL2+PPM+HLLD+CT, lognormal density (sigma_lnrho = 1), rms Mach 5,
@@ -900,6 +941,8 @@ def main():
         out["smr_cycle"] = smr_cycle_bench(agx, pgen, args)
     if not args.no_sg:
         out["self_gravity"] = sg_run(agx, pgen, args.n, args.block, 2, 2.0, args.sg_steps, "sg", bx=0.1)
+        nv = max(1, int(np.ceil(out["self_gravity"]["fp64"]["vcycles_per_solve"])))
+        out["self_gravity_graph"] = sg_graph_bench(agx, pgen, args.n, args.block, args.sg_steps, nv)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)
