@@ -1770,7 +1770,7 @@ static int mg_solve_b(agx_mesh *m) {
       else vcycle_b(m);
       iters++;
     }
-    m->last_norm = defect_norm_b(m);
+    m->last_norm = m->capturing ? NAN : defect_norm_b(m);   // diagnostic only, see mg_solve
   }
   m->last_iters = iters;
   const int64_t nret = (int64_t)(g.bx1 + 2) * (g.bx2 + 2) * (g.bx3 + 2) * g.nb;

@@ -3626,8 +3626,9 @@ int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
   if (!m) { set_error("null mesh"); return -1; }
   if (int rc = refuse_tree(m, "agx_cycle")) return rc;
   // with self-gravity only the fixed-count solve (niteration, SolveIterativeFixedTimes) is
-  // recorded: the threshold form decides its V-cycle count from norms read on the host
-  const bool sg_fixed = m->d.self_gravity && m->mg_on && m->mg_eps < 0.0 && !m->mp_on;
+  // recorded: the threshold form decides its V-cycle count from norms read on the host; a
+  // fixed multipole origin is copied from host memory at every solve
+  const bool sg_fixed = m->d.self_gravity && m->mg_on && m->mg_eps < 0.0 && (!m->mp_on || m->d.mp_auto_origin);
   if (m->graph_on && !m->timer.enabled && (!m->d.self_gravity || sg_fixed))
     return cycle_graphs(m, tlim, ncycles, done);
   int32_t n = 0;

@@ -364,11 +364,11 @@ int agx_mg_set_source_mask_sphere(agx_mesh *m, double xc, double yc, double zc, 
  * read back once per cycle for the tlim test.  Results are identical to the eager path.
  * With self-gravity the graph also holds both multigrid solves of the cycle when they run
  * a fixed number of V-cycles (threshold < 0, niteration; SolveIterativeFixedTimes,
- * multigrid_driver.cpp:762-771) with periodic boundaries; agx_mg_last then reports the
- * V-cycle count and a NaN norm (the norm is only diagnostic and is not read back).  A
- * threshold solve decides its V-cycle count from norms read on the host
- * (multigrid_driver.cpp:721-755), and multipole boundaries are not recorded: those, and
- * cycles with the built-in timers on, run eagerly.  Every setter that changes the
+ * multigrid_driver.cpp:762-771), periodic or multipole (with the automatic origin);
+ * agx_mg_last then reports the V-cycle count and a NaN norm (the norm is only diagnostic
+ * and is not read back).  A threshold solve decides its V-cycle count from norms read on
+ * the host (multigrid_driver.cpp:721-755), and a fixed multipole origin is copied from
+ * host memory at each solve: those, and cycles with the built-in timers on, run eagerly.  Every setter that changes the
  * recorded work (multigrid mode, pass form, coarse kernel, source mask, stage form)
  * discards the recorded graphs.  On by default; enable = 0 selects the eager path. */
 int agx_set_graph(agx_mesh *m, int32_t enable);

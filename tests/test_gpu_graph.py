@@ -112,12 +112,14 @@ def test_graph_rebuilt_after_tile_change(agx):
             same(m.get(f), o.get(f), f"{f} tile {tile}")
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_graph_with_fixed_count_gravity_equals_eager(agx, precision):
-    """Self-gravity with a fixed V-cycle count (niteration): both multigrid solves of a
-    cycle are recorded with the MHD stages.  Graph replays, interleaved with eager cycles
-    and downloads, equal the eager path bit for bit, potential included."""
-    desc = turb_desc(self_gravity=1, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=3, precision=precision)
+@pytest.mark.parametrize("precision,mg_bc", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_graph_with_fixed_count_gravity_equals_eager(agx, precision, mg_bc):
+    """Self-gravity with a fixed V-cycle count (niteration), periodic or multipole
+    boundaries: both multigrid solves of a cycle are recorded with the MHD stages.  Graph
+    replays, interleaved with eager cycles and downloads, equal the eager path bit for
+    bit, potential included."""
+    desc = turb_desc(self_gravity=1, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=3, precision=precision,
+                     mg_bc=mg_bc)
     st = turbulent_state(desc, seed=9, mach=1.5, drho=0.5)
     runs = []
     for graph in (False, True):
@@ -141,4 +143,4 @@ def test_graph_with_fixed_count_gravity_equals_eager(agx, precision):
         assert a["time"] == b["time"], k
         assert a["iters"] == b["iters"] == 3
         for f in FIELDS + ("phi",):
-            same(b[f], a[f], f"segment {k}: {f} (precision {precision})")
+            same(b[f], a[f], f"segment {k}: {f} (precision {precision}, mg_bc {mg_bc})")
