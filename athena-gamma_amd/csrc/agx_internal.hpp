@@ -164,6 +164,8 @@ struct agx_mesh {
   int32_t ncycle = 0;
   bool restarted = false;            // dt came from a restart file: agx_initialize keeps it
   bool tree = false;                 // blocks on refinement levels (agx_mesh_set_levels)
+  int copies_state = -1;             // the two copies of every shared face agree: 1 yes, 0 no,
+                                     // -1 not checked since the last upload of a face array
   // multigrid
   int mg_on = 0;
   int nrootlevel = 0, nmblevel = 0, ntotal = 0, current = 0, fmglevel = 0, coffset = 0;

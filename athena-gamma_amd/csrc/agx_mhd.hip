@@ -1716,7 +1716,10 @@ struct StageArgs {
   double *dtpart;                    // per-workgroup min of the dt bound, or null
 };
 
-template <int TI, int TJ, typename TW, bool ISO>
+// PUSH: the two copies of every shared face agree (agx_mesh::copies_state), so the
+// ghosts receive this cell's floored u, w and bcc (the same bits a ghost C2P would give);
+// otherwise they receive the un-floored u and are converted by k_ghost_c2p.
+template <int TI, int TJ, typename TW, bool ISO, bool PUSH>
 __global__ void __launch_bounds__(TI * TJ, 4)   // >= 4 waves per SIMD: <= 128 VGPRs
 k_stage(Geo g, StageArgs<TW> a) {
   static_assert(TI * TJ == 256 || TI * TJ == 512, "k_stage runs 256- or 512-thread workgroups");
@@ -1915,17 +1918,16 @@ k_stage(Geo g, StageArgs<TW> a) {
         a.bccn[bo + 2 * S] = TW(bcc3);
         // ghost copies of the un-floored conserved state (bvals_cc.cpp:201-336 sends u
         // before ConservedToPrimitive; the receiver converts its ghosts itself)
-        {
-          int rc[6];
-          push_range(i, g.is, g.ie, g.bx1, g.ng, rc[0], rc[1]);
-          push_range(j, g.js, g.je, g.bx2, g.ng, rc[2], rc[3]);
-          push_range(c - 1, g.ks, g.ke, g.bx3, g.ng, rc[4], rc[5]);
+        int rc[6];
+        push_range(i, g.is, g.ie, g.bx1, g.ng, rc[0], rc[1]);
+        push_range(j, g.js, g.je, g.bx2, g.ng, rc[2], rc[3]);
+        push_range(c - 1, g.ks, g.ke, g.bx3, g.ng, rc[4], rc[5]);
+        if constexpr (!PUSH)
           for_each_ghost(g, b, c - 1, j, i, rc, [&](int nb, int64_t id2) {
             const int64_t c2 = (int64_t)nb * NHYDRO * S + id2;
 #pragma unroll
             for (int n = 0; n < NH; ++n) a.dst[c2 + n * S] = un[n];
           });
-        }
         // C2P writes its floors back into u (adiabatic_mhd.cpp:66-84)
         double wv[5];
         c2p_eos<ISO>(un[IDN], un[IVX], un[IVY], un[IVZ], un[IEN], bcc1, bcc2, bcc3, a.gm1, a.dfl, a.pfl,
@@ -1934,6 +1936,19 @@ k_stage(Geo g, StageArgs<TW> a) {
         for (int n = 0; n < NH; ++n) a.dst[cb0 + n * S] = un[n];
 #pragma unroll
         for (int n = 0; n < NH; ++n) a.wn[cb0 + n * S] = TW(wv[n]);
+        // PUSH: a ghost's bcc and its C2P are this cell's, bit for bit
+        if constexpr (PUSH)
+          for_each_ghost(g, b, c - 1, j, i, rc, [&](int nb, int64_t id2) {
+            const int64_t c2 = (int64_t)nb * NHYDRO * S + id2, b2 = (int64_t)nb * 3 * S + id2;
+#pragma unroll
+            for (int n = 0; n < NH; ++n) {
+              a.dst[c2 + n * S] = un[n];
+              a.wn[c2 + n * S] = TW(wv[n]);
+            }
+            a.bccn[b2] = TW(bcc1);
+            a.bccn[b2 + S] = TW(bcc2);
+            a.bccn[b2 + 2 * S] = TW(bcc3);
+          });
         if (a.dtpart)
           dtmin = dmin(dtmin, cell_dt<ISO>(a.gamma, a.cs, wv, bcc1, bcc2, bcc3, b1l, b2l, b3_lo, dx));
       }
@@ -2006,7 +2021,7 @@ __device__ __forceinline__ void flux_to_mesh(int dir, const T *f, T *out) {
   else { out[1] = f[2]; out[2] = f[3]; out[3] = f[1]; }
 }
 
-template <int TI, int TJ, int ORDER, int RS, typename TW, bool ISO>
+template <int TI, int TJ, int ORDER, int RS, typename TW, bool ISO, bool PUSH>
 __global__ void __launch_bounds__(TI * TJ, 2)   // >= 2 waves per SIMD
 k_vl2(Geo g, VL2Args<TW> a) {
   static_assert(64 % TI == 0 && TI * TJ == 256, "k_vl2 tiles: 256 threads, rows within a wavefront");
@@ -2280,23 +2295,34 @@ k_vl2(Geo g, VL2Args<TW> a) {
         a.bccn[bo] = TW(bcc1);
         a.bccn[bo + S] = TW(bcc2);
         a.bccn[bo + 2 * S] = TW(bcc3);
-        {
-          int rc[6];
-          push_range(i, g.is, g.ie, g.bx1, g.ng, rc[0], rc[1]);
-          push_range(j, g.js, g.je, g.bx2, g.ng, rc[2], rc[3]);
-          push_range(c - 1, g.ks, g.ke, g.bx3, g.ng, rc[4], rc[5]);
+        int rc[6];
+        push_range(i, g.is, g.ie, g.bx1, g.ng, rc[0], rc[1]);
+        push_range(j, g.js, g.je, g.bx2, g.ng, rc[2], rc[3]);
+        push_range(c - 1, g.ks, g.ke, g.bx3, g.ng, rc[4], rc[5]);
+        if constexpr (!PUSH)
           for_each_ghost(g, b, c - 1, j, i, rc, [&](int nb, int64_t id2) {
             const int64_t c2 = (int64_t)nb * NHYDRO * S + id2;
 #pragma unroll
             for (int n = 0; n < NH; ++n) a.dst[c2 + n * S] = un[n];
           });
-        }
         double wv[5];
         c2p_eos<ISO>(un[IDN], un[IVX], un[IVY], un[IVZ], un[IEN], bcc1, bcc2, bcc3, a.gm1, a.dfl, a.pfl, wv);
 #pragma unroll
         for (int n = 0; n < NH; ++n) a.dst[cb0 + n * S] = un[n];
 #pragma unroll
         for (int n = 0; n < NH; ++n) a.wn[cb0 + n * S] = TW(wv[n]);
+        if constexpr (PUSH)   // as in k_stage
+          for_each_ghost(g, b, c - 1, j, i, rc, [&](int nb, int64_t id2) {
+            const int64_t c2 = (int64_t)nb * NHYDRO * S + id2, b2 = (int64_t)nb * 3 * S + id2;
+#pragma unroll
+            for (int n = 0; n < NH; ++n) {
+              a.dst[c2 + n * S] = un[n];
+              a.wn[c2 + n * S] = TW(wv[n]);
+            }
+            a.bccn[b2] = TW(bcc1);
+            a.bccn[b2 + S] = TW(bcc2);
+            a.bccn[b2 + 2 * S] = TW(bcc3);
+          });
         if (a.dtpart)
           dtmin = dmin(dtmin, cell_dt<ISO>(a.gamma, a.cs, wv, bcc1, bcc2, bcc3, b1l, b2l, b3_lo, dx));
       }
@@ -2925,6 +2951,7 @@ static int copy_array(agx_mesh *m, int arr, double *host, bool to_device) {
 }
 
 int agx_upload(agx_mesh *m, int arr, const double *host) {
+  if (m && (arr == AGX_ARR_B1F || arr == AGX_ARR_B2F || arr == AGX_ARR_B3F)) m->copies_state = -1;
   return copy_array(m, arr, const_cast<double *>(host), true);
 }
 int agx_download(agx_mesh *m, int arr, double *host) { return copy_array(m, arr, host, false); }
@@ -3013,6 +3040,7 @@ static int copy_blocks(agx_mesh *m, int arr, int32_t nblocks, const int32_t *ori
 
 int agx_upload_blocks(agx_mesh *m, int arr, int32_t nblocks, const int32_t *origin, const int32_t *size,
                       const double *const *host) {
+  if (m && (arr == AGX_ARR_B1F || arr == AGX_ARR_B2F || arr == AGX_ARR_B3F)) m->copies_state = -1;
   return copy_blocks(m, arr, nblocks, origin, size, const_cast<double *const *>(host), true);
 }
 
@@ -3069,6 +3097,62 @@ __global__ void k_advance(double *__restrict__ s, double cfl) {
     s[SC_TIME] = time;
     s[SC_DT] = ndt;
   }
+}
+
+// The two copies of every shared face (the upper boundary face of a block and the lower
+// boundary face of its lattice neighbour, periodic), compared bit for bit over the
+// face's interior; any difference clears *flag.  One thread per face.
+extern "C++" {
+template <int DIR>
+__global__ void k_face_copies(Geo g, const double *__restrict__ f, double *__restrict__ flag) {
+  PlaneIdx p;
+  if (DIR == 0 && !plane_index(g.js, g.je, g.ks, g.ke, 0, p)) return;
+  if (DIR == 1 && !plane_index(g.is, g.ie, g.ks, g.ke, 0, p)) return;
+  if (DIR == 2 && !plane_index(g.is, g.ie, g.js, g.je, 0, p)) return;
+  int64_t up, lo;
+  int nb;
+  if (DIR == 0) {
+    up = IDX(g, p.j, p.i, g.ie + 1); lo = IDX(g, p.j, p.i, g.is); nb = nbr_block(g, p.b, 1, 0, 0);
+  } else if (DIR == 1) {
+    up = IDX(g, p.j, g.je + 1, p.i); lo = IDX(g, p.j, g.js, p.i); nb = nbr_block(g, p.b, 0, 1, 0);
+  } else {
+    up = IDX(g, g.ke + 1, p.j, p.i); lo = IDX(g, g.ks, p.j, p.i); nb = nbr_block(g, p.b, 0, 0, 1);
+  }
+  const long long x = __double_as_longlong(f[(int64_t)p.b * g.S + up]);
+  const long long y = __double_as_longlong(f[(int64_t)nb * g.S + lo]);
+  if (x != y) *flag = 0.0;
+}
+}  // extern "C++"
+
+__global__ void k_set_value(double *__restrict__ p, double v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
+// Whether the two copies of every shared face agree (m->copies_state), checked once after
+// each upload of a face array.  While they agree, every block computes the same value for
+// a shared edge, CT changes both copies alike and they keep agreeing, so the stage kernels
+// push each cell's floored u, w and bcc to the ghosts that copy it instead of converting
+// the ghosts afterwards from the block's own faces (the same bits).
+constexpr int SC_COPIES = 48;
+static int ensure_copies_checked(agx_mesh *m) {
+  if (m->copies_state >= 0) return 0;
+  if (m->tree) { m->copies_state = 0; return 0; }
+  const Geo &g = m->g;
+  double **A = m->arr.a;
+  double *flag = m->scal + SC_COPIES;
+  hipLaunchKernelGGL(k_set_value, dim3(1), dim3(64), 0, m->stream, flag, 1.0);
+  hipLaunchKernelGGL(k_face_copies<0>, plane_grid(g.js, g.je, g.ks, g.ke, 0, 0, g.nb), dim3(256), 0, m->stream, g,
+                     A[AGX_ARR_B1F], flag);
+  hipLaunchKernelGGL(k_face_copies<1>, plane_grid(g.is, g.ie, g.ks, g.ke, 0, 0, g.nb), dim3(256), 0, m->stream, g,
+                     A[AGX_ARR_B2F], flag);
+  hipLaunchKernelGGL(k_face_copies<2>, plane_grid(g.is, g.ie, g.js, g.je, 0, 0, g.nb), dim3(256), 0, m->stream, g,
+                     A[AGX_ARR_B3F], flag);
+  LAUNCH_CHECK("k_face_copies");
+  AGX_HIP(hipMemcpyAsync(m->scal_host + SC_COPIES, flag, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  AGX_HIP(hipStreamSynchronize(m->stream));
+  m->copies_state = m->scal_host[SC_COPIES] == 1.0 ? 1 : 0;
+  m->graph_gen++;
+  return 0;
 }
 
 static int push_scalars(agx_mesh *m, double tlim) {
@@ -3346,6 +3430,22 @@ int agx_initialize(agx_mesh *m, double tlim) {
 // fluxes of the half-step state and writes the new faces to the third register; the
 // rotation leaves b = new faces and b1 = half-step faces, as the reference's swap does.
 extern "C++" {
+template <typename TW, bool PUSH>
+static void launch_stage_t(agx_mesh *m, int tile, const StageArgs<TW> &a, dim3 grid) {
+  const Geo &g = m->g;
+  if (m->iso) {
+    if (tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, true, PUSH>), grid, dim3(256), 0, m->stream, g, a);
+    else if (tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, true, PUSH>), grid, dim3(256), 0, m->stream, g, a);
+    else if (tile == 3) hipLaunchKernelGGL((k_stage<64, 8, TW, true, PUSH>), grid, dim3(512), 0, m->stream, g, a);
+    else hipLaunchKernelGGL((k_stage<16, 16, TW, true, PUSH>), grid, dim3(256), 0, m->stream, g, a);
+  } else {
+    if (tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, false, PUSH>), grid, dim3(256), 0, m->stream, g, a);
+    else if (tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, false, PUSH>), grid, dim3(256), 0, m->stream, g, a);
+    else if (tile == 3) hipLaunchKernelGGL((k_stage<64, 8, TW, false, PUSH>), grid, dim3(512), 0, m->stream, g, a);
+    else hipLaunchKernelGGL((k_stage<16, 16, TW, false, PUSH>), grid, dim3(256), 0, m->stream, g, a);
+  }
+}
+
 template <typename TW>
 static void launch_stage(agx_mesh *m, int tile, int stage, double wfac, double *dtpart) {
   const Geo &g = m->g;
@@ -3377,47 +3477,44 @@ static void launch_stage(agx_mesh *m, int tile, int stage, double wfac, double *
   a.grav = m->d.self_gravity ? 1 : 0;
   a.dtpart = dtpart;
   const dim3 grid = stage_grid(g, tile);
-  if (m->iso) {
-    if (tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, true>), grid, dim3(256), 0, m->stream, g, a);
-    else if (tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, true>), grid, dim3(256), 0, m->stream, g, a);
-    else if (tile == 3) hipLaunchKernelGGL((k_stage<64, 8, TW, true>), grid, dim3(512), 0, m->stream, g, a);
-    else hipLaunchKernelGGL((k_stage<16, 16, TW, true>), grid, dim3(256), 0, m->stream, g, a);
-  } else {
-    if (tile == 1) hipLaunchKernelGGL((k_stage<32, 8, TW, false>), grid, dim3(256), 0, m->stream, g, a);
-    else if (tile == 2) hipLaunchKernelGGL((k_stage<64, 4, TW, false>), grid, dim3(256), 0, m->stream, g, a);
-    else if (tile == 3) hipLaunchKernelGGL((k_stage<64, 8, TW, false>), grid, dim3(512), 0, m->stream, g, a);
-    else hipLaunchKernelGGL((k_stage<16, 16, TW, false>), grid, dim3(256), 0, m->stream, g, a);
-  }
+  if (m->copies_state == 1) launch_stage_t<TW, true>(m, tile, a, grid);
+  else launch_stage_t<TW, false>(m, tile, a, grid);
 }
 }  // extern "C++"
 
 
 // The whole stage as one k_vl2 launch (fused form); tile 0 = 16 x 16, 1 = 32 x 8 columns.
 extern "C++" {
-template <int TI, int TJ, int ORDER, typename TW, bool ISO>
+template <int TI, int TJ, int ORDER, typename TW, bool ISO, bool PUSH>
 static void launch_vl2_rs(agx_mesh *m, int rs, const VL2Args<TW> &a, dim3 grid) {
   if (ISO) {
     if (rs == AGX_RSOLVER_HLLE)
-      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLE, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLE, TW, ISO, PUSH>), grid, dim3(256), 0, m->stream, m->g, a);
     else
-      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLD, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLD, TW, ISO, PUSH>), grid, dim3(256), 0, m->stream, m->g, a);
   } else {
     if (rs == AGX_RSOLVER_HLLE)
-      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLE, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLE, TW, ISO, PUSH>), grid, dim3(256), 0, m->stream, m->g, a);
     else if (rs == AGX_RSOLVER_ROE)
-      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_ROE, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_ROE, TW, ISO, PUSH>), grid, dim3(256), 0, m->stream, m->g, a);
     else if (rs == AGX_RSOLVER_LHLLD)
-      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_LHLLD, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_LHLLD, TW, ISO, PUSH>), grid, dim3(256), 0, m->stream, m->g, a);
     else
-      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLD, TW, ISO>), grid, dim3(256), 0, m->stream, m->g, a);
+      hipLaunchKernelGGL((k_vl2<TI, TJ, ORDER, AGX_RSOLVER_HLLD, TW, ISO, PUSH>), grid, dim3(256), 0, m->stream, m->g, a);
   }
 }
 
 template <int TI, int TJ, typename TW, bool ISO>
 static void launch_vl2_order(agx_mesh *m, int order, const VL2Args<TW> &a, dim3 grid) {
-  if (order == 1) launch_vl2_rs<TI, TJ, 1, TW, ISO>(m, m->d.rsolver, a, grid);
-  else if (order == 2) launch_vl2_rs<TI, TJ, 2, TW, ISO>(m, m->d.rsolver, a, grid);
-  else launch_vl2_rs<TI, TJ, 3, TW, ISO>(m, m->d.rsolver, a, grid);
+  if (m->copies_state == 1) {
+    if (order == 1) launch_vl2_rs<TI, TJ, 1, TW, ISO, true>(m, m->d.rsolver, a, grid);
+    else if (order == 2) launch_vl2_rs<TI, TJ, 2, TW, ISO, true>(m, m->d.rsolver, a, grid);
+    else launch_vl2_rs<TI, TJ, 3, TW, ISO, true>(m, m->d.rsolver, a, grid);
+  } else {
+    if (order == 1) launch_vl2_rs<TI, TJ, 1, TW, ISO, false>(m, m->d.rsolver, a, grid);
+    else if (order == 2) launch_vl2_rs<TI, TJ, 2, TW, ISO, false>(m, m->d.rsolver, a, grid);
+    else launch_vl2_rs<TI, TJ, 3, TW, ISO, false>(m, m->d.rsolver, a, grid);
+  }
 }
 
 template <typename TW>
@@ -3515,6 +3612,7 @@ static int stage_fused(agx_mesh *m, int stage, bool want_dt) {
       A[ARR_Q1F + d] = start;
     }
   }
+  if (m->copies_state == 1) return 0;   // the stage kernel pushed the ghosts' u, w and bcc
   timer_begin(m, AGX_K_BVALS);
   if (m->mixed) launch_ghost_c2p<float>(m);
   else launch_ghost_c2p<double>(m);
@@ -3625,6 +3723,7 @@ int agx_set_graph(agx_mesh *m, int32_t enable) {
 int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
   if (!m) { set_error("null mesh"); return -1; }
   if (int rc = refuse_tree(m, "agx_cycle")) return rc;
+  if (int rc = ensure_copies_checked(m)) return rc;
   // with self-gravity only the fixed-count solve (niteration, SolveIterativeFixedTimes) is
   // recorded: the threshold form decides its V-cycle count from norms read on the host; a
   // fixed multipole origin is copied from host memory at every solve
@@ -3695,6 +3794,7 @@ int agx_mg_vcycles(agx_mesh *m, int32_t n) {
 int agx_autotune(agx_mesh *m, int32_t *choice) {
   if (!m) { set_error("null mesh"); return -1; }
   if (int rc = refuse_tree(m, "agx_autotune")) return rc;
+  if (int rc = ensure_copies_checked(m)) return rc;
   hipEvent_t e0, e1;
   AGX_HIP(hipEventCreate(&e0));
   AGX_HIP(hipEventCreate(&e1));
