@@ -170,13 +170,16 @@ def mhd_bench(agx, pgen, args, precision):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "mhd_turb256_plm_hlld_fp64_mixed_hbm_traffic.txt")
 
 
-def kernel_name(precision, tile, order=2):
-    """rocprof name of the stage kernel of this form (adiabatic, HLLD)."""
+def kernel_name(precision, tile, order=2, push=True):
+    """rocprof name of the stage kernel of this form (adiabatic, HLLD).  push: the form
+    that writes ghost u, w and bcc itself, which runs while the two copies of every shared
+    face agree (always the case for a state uploaded as global arrays, as here)."""
     t = "double" if precision == "fp64" else "float"
+    p = "true" if push else "false"
     if tile >= VL2_FORM:
         ti = "16, 16" if tile == 4 else "32, 8"
-        return f"k_vl2<{ti}, {order}, 0, {t}, false>"
-    return f"k_stage<{('16, 16', '32, 8', '64, 4', '64, 8')[tile]}, {t}, false>"
+        return f"k_vl2<{ti}, {order}, 0, {t}, false, {p}>"
+    return f"k_stage<{('16, 16', '32, 8', '64, 4', '64, 8')[tile]}, {t}, false, {p}>"
 
 
 def measured_traffic(desc, precision, tile):
