@@ -118,6 +118,23 @@ struct MGLevelB32 {
   float *u, *src, *uold;
 };
 
+// Device state of a threshold solve whose V-cycle loop is recorded for a fixed number of
+// passes (the graph replay of agx_cycle).  While closed != 0 every multigrid kernel of the
+// loop returns at once; k_mg_gate_check closes it where SolveIterative's loop stops.
+struct MGGate {
+  int closed;
+  int iters;     // V-cycles the loop has run
+  double def;    // defect norm after the last of them (before the loop: the first norm)
+};
+
+// Host-side multigrid state that a V-cycle changes: the red-black offset, the level
+// cursor and the arrays the fused pass and the mixed correction swap.
+struct MGSnap {
+  std::vector<MGLevel> lev;
+  std::vector<MGLevel32> lev32;
+  int coffset = 0, current = 0, fmglevel = 0;
+};
+
 // History sums (agx_diag.hip): at most HIST_NW_MAX workgroups, each writing HIST_NV
 // partial sums (5 hydro + 3 kinetic + 1 gravitational + 3 magnetic at most).
 constexpr int HIST_NW_MAX = 2048;
@@ -178,6 +195,10 @@ struct agx_mesh {
   double srcmask_r2 = 0.0;                  // red-black pass form: -1 by level size, 0 two sweeps, 1 fused
   double mg_eps = -1.0, last_norm = 0.0, last_ave = 0.0;
   int32_t last_iters = 0;
+  agx::MGGate *gatedev = nullptr;    // gate of a recorded threshold loop (device)
+  agx::MGGate *gatehost = nullptr;   // its pinned host mirror
+  const int *gate = nullptr;         // &gatedev->closed while a gated loop is enqueued, else null
+  int mg_kpred[2] = {-1, -1};        // V-cycles of the last threshold solve of each stage
   std::vector<agx::MGLevel> lev;
   std::vector<agx::MGLevel32> lev32;   // mixed precision only
   double *mgpool = nullptr;
@@ -204,6 +225,17 @@ struct agx_mesh {
     int coffset_before = 0, coffset_after = 0;   // multigrid red-black offset (self-gravity)
     int gen = 0;                                  // graph_gen when recorded
   };
+  // threshold solves: the cycle is recorded in three segments, each ending where the
+  // host reads how many of the recorded V-cycles ran (mg_solve_settle)
+  struct SegGraph {
+    hipGraphExec_t exec = nullptr;
+    int seg = 0, k = 0, gen = 0;
+    double *before[agx::ARR_TOTAL];
+    double *after[agx::ARR_TOTAL];
+    std::vector<uintptr_t> mgkey;          // multigrid state at the start (mg_key)
+    std::vector<agx::MGSnap> snaps;        // multigrid state after 0..k recorded passes
+  };
+  std::vector<SegGraph> seggraphs;
   bool capturing = false;            // a cycle is being recorded into a graph
   int graph_gen = 0;                 // bumped by every setter that changes the recorded work
   std::vector<CycleGraph> graphs;
@@ -253,6 +285,19 @@ inline unsigned grid1d(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs)
 int mg_create(agx_mesh *m);
 void mg_destroy(agx_mesh *m);
 int mg_solve(agx_mesh *m, int stage);
+// A threshold solve in parts that enqueue no host read-back, for graph recording:
+// mg_solve_head (load, FMG, the first defect norm, the gate), mg_solve_gated (k V-cycles
+// gated on the device by the stopping rule; the host state after 0..k of them into snaps),
+// then, after the stream has run them, mg_solve_settle (the host state of the V-cycles
+// that ran; the loop continued eagerly if it had not stopped) and mg_solve_tail (mean
+// subtraction, retrieve).
+int mg_solve_head(agx_mesh *m);
+void mg_solve_gated(agx_mesh *m, int k, std::vector<agx::MGSnap> &snaps);
+int mg_solve_settle(agx_mesh *m, int stage, const std::vector<agx::MGSnap> &snaps);
+int mg_solve_tail(agx_mesh *m);
+agx::MGSnap mg_snap(const agx_mesh *m);
+void mg_restore(agx_mesh *m, const agx::MGSnap &s);
+std::vector<uintptr_t> mg_key(const agx_mesh *m);
 int mg_vcycles(agx_mesh *m, int n);
 int mg_autotune(agx_mesh *m);
 

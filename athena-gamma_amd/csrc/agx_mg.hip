@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cfloat>
 #include <cmath>
 
@@ -47,8 +48,9 @@ __device__ __forceinline__ T gs_value(T uc, T ukp, T ujp, T uip, T ukm, T ujm, T
 // of global cells with (I + J + K + color) even.  One thread per updated cell.
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_smooth(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3,
+k_mg_smooth(const int *__restrict__ gate, T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3,
             T dx2, T isix, int color) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   int nh = (n1 + 1) / 2;
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   int64_t tot = (int64_t)nh * n2 * n3;
@@ -79,8 +81,9 @@ template <> struct Vec2<float> { using type = float2; };
 
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_smooth_pairs(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3,
+k_mg_smooth_pairs(const int *__restrict__ gate, T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3,
                   T dx2, T isix, int color) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   using V = typename Vec2<T>::type;
   const unsigned nh = (unsigned)n1 >> 1;
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 pairs per level
@@ -122,8 +125,9 @@ k_mg_smooth_pairs(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, 
 constexpr int RB_TI = 64, RB_TJ = 8, RB_KC = 16;
 template <typename T>
 __global__ void __launch_bounds__(RB_TI * RB_TJ)
-k_mg_smooth_rb(const T *__restrict__ uin, T *__restrict__ uout, const T *__restrict__ src, int n1, int n2,
+k_mg_smooth_rb(const int *__restrict__ gate, const T *__restrict__ uin, T *__restrict__ uout, const T *__restrict__ src, int n1, int n2,
                int n3, T dx2, T isix, int c0) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   constexpr int TI = RB_TI, TJ = RB_TJ, NT = TI * TJ, OI = TI + 4, OJ = TJ + 4, RI = TI + 2, RJ = TJ + 2;
   __shared__ T s_old[3][OJ][OI];
   __shared__ T s_red[3][RJ][RI];
@@ -285,9 +289,10 @@ __device__ __forceinline__ T restricted_defect_pairs(const T *u, const T *src, i
 // (ZeroClearData, non-FAS).  One thread per coarse cell.
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_defect_restrict(const T *__restrict__ u, const T *__restrict__ src,
+k_mg_defect_restrict(const int *__restrict__ gate, const T *__restrict__ u, const T *__restrict__ src,
                      T *__restrict__ csrc, T *__restrict__ cu, int n1, int n2, int n3,
                      T idx2) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)c1 * c2 * c3) return;
@@ -339,7 +344,8 @@ __device__ __forceinline__ T prolong_corr(const T *cu, int c1, int c2, int c3, i
 // high child along each axis.
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_prolong(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3) {
+k_mg_prolong(const int *__restrict__ gate, const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t >= (int64_t)n1 * n2 * n3) return;
   int fi = (int)(t % (unsigned)n1);
@@ -355,7 +361,8 @@ k_mg_prolong(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3
 // prolong_corr.
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_prolong_pairs(const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3) {
+k_mg_prolong_pairs(const int *__restrict__ gate, const T *__restrict__ cu, T *__restrict__ u, int n1, int n2, int n3) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   using V = typename Vec2<T>::type;
   const unsigned nh = (unsigned)n1 >> 1;
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 pairs per level
@@ -461,7 +468,8 @@ __device__ __forceinline__ void coarse_smooth(const CoarseLevels<T> &L, int l, i
 
 template <typename T>
 __global__ void __launch_bounds__(COARSE_BS)
-k_mg_coarse(CoarseLevels<T> L, int top, int coffset, T isix, int first_sweep_done) {
+k_mg_coarse(const int *__restrict__ gate, CoarseLevels<T> L, int top, int coffset, T isix, int first_sweep_done) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   for (int l = top; l > 0; --l) {
     if (!(first_sweep_done && l == top)) coarse_smooth(L, l, 0 ^ coffset, isix);
     coarse_smooth(L, l, 1 ^ coffset, isix);
@@ -504,8 +512,9 @@ __device__ __forceinline__ void wg_range(unsigned n, unsigned &t0, unsigned &t1)
 // block-partial sums in FP64: mode 0 sum(a), mode 1 sum(def^2)
 template <int MODE, typename T>
 __global__ void __launch_bounds__(BS)
-k_mg_partial(const T *__restrict__ a, const T *__restrict__ src, int n1, int n2, int n3,
+k_mg_partial(const int *__restrict__ gate, const T *__restrict__ a, const T *__restrict__ src, int n1, int n2, int n3,
              T idx2, double *__restrict__ part) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   __shared__ double sh[BS];
   const unsigned ncell = (unsigned)n1 * n2 * n3;   // < 2^32 cells per level
   unsigned t0, t1;
@@ -532,8 +541,9 @@ k_mg_partial(const T *__restrict__ a, const T *__restrict__ src, int n1, int n2,
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ void __launch_bounds__(BS) k_mg_final(const double *__restrict__ part, int64_t n,
+__global__ void __launch_bounds__(BS) k_mg_final(const int *__restrict__ gate, const double *__restrict__ part, int64_t n,
                                                  double *__restrict__ out) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   __shared__ double sh[BS];
   double v = 0.0;
 #pragma unroll 8
@@ -548,14 +558,16 @@ __global__ void __launch_bounds__(BS) k_mg_final(const double *__restrict__ part
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BS) k_mg_add(T *__restrict__ a, int64_t n,
+__global__ void __launch_bounds__(BS) k_mg_add(const int *__restrict__ gate, T *__restrict__ a, int64_t n,
                                                const double *__restrict__ val, double scale) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t < n) a[t] -= T((*val) * scale);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BS) k_mg_zero(T *__restrict__ a, int64_t n) {
+__global__ void __launch_bounds__(BS) k_mg_zero(const int *__restrict__ gate, T *__restrict__ a, int64_t n) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;   // < 2^32 cells per level
   if (t < n) a[t] = T(0.0);
 }
@@ -569,9 +581,10 @@ __global__ void __launch_bounds__(BS) k_mg_zero(T *__restrict__ a, int64_t n) {
 // sweep (colour cred) from e = 0 gives gs_value(0, ..., 0, d) at those cells and 0
 // elsewhere, so that cycle starts with its second sweep.
 __global__ void __launch_bounds__(BS)
-k_mg_defect_to32(const double *__restrict__ u, const float *__restrict__ ein, double *__restrict__ uout,
+k_mg_defect_to32(const int *__restrict__ gate, const double *__restrict__ u, const float *__restrict__ ein, double *__restrict__ uout,
                  const double *__restrict__ src, float *__restrict__ d32, float *__restrict__ e0, int n1, int n2,
                  int n3, double idx2, float dx2f, float isixf, int cred, double *__restrict__ part) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   __shared__ double sh[BS];
   const unsigned ncell = (unsigned)n1 * n2 * n3;   // < 2^32 cells per level
   unsigned t0, t1;
@@ -613,9 +626,10 @@ k_mg_defect_to32(const double *__restrict__ u, const float *__restrict__ ein, do
 constexpr int DT_TI = 64, DT_TJ = 8, DT_KC = 16;
 template <bool CORR>
 __global__ void __launch_bounds__(DT_TI * DT_TJ)
-k_mg_defect_to32_t(const double *__restrict__ u, const float *__restrict__ ein, double *__restrict__ uout,
+k_mg_defect_to32_t(const int *__restrict__ gate, const double *__restrict__ u, const float *__restrict__ ein, double *__restrict__ uout,
                    const double *__restrict__ src, float *__restrict__ d32, float *__restrict__ e0, int n1,
                    int n2, int n3, double idx2, float dx2f, float isixf, int cred, double *__restrict__ part) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   constexpr int TI = DT_TI, TJ = DT_TJ, NT = TI * TJ, RI = TI + 2, RJ = TJ + 2;
   constexpr int NPF = (RI * RJ + NT - 1) / NT;
   __shared__ double s_phi[3][RJ][RI];
@@ -880,7 +894,8 @@ k_mb_bc_fill(double *__restrict__ bcv, MPGeom G, const double *__restrict__ mp, 
 // one launch: ghost = 2 phi_s(p) - (the value inside it), along the chain, with phi_s from
 // bcv (the same doubles k_mb_bc_fill computed).
 __global__ void __launch_bounds__(BS)
-k_mb_bc(double *__restrict__ u, MPGeom G, const double *__restrict__ bcv) {
+k_mb_bc(const int *__restrict__ gate, double *__restrict__ u, MPGeom G, const double *__restrict__ bcv) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned q = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   GhostChain c;
   if (!ghost_chain(G, q, c)) return;
@@ -893,7 +908,8 @@ k_mb_bc(double *__restrict__ u, MPGeom G, const double *__restrict__ bcv) {
 // The same chain for a correction of the potential: the boundary values are those of
 // the potential itself, so the correction's ghosts take 2 * 0 - (the value inside).
 __global__ void __launch_bounds__(BS)
-k_mb_bc_hom(float *__restrict__ u, MPGeom G) {
+k_mb_bc_hom(const int *__restrict__ gate, float *__restrict__ u, MPGeom G) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned q = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   GhostChain c;
   if (!ghost_chain(G, q, c)) return;
@@ -906,7 +922,8 @@ k_mb_bc_hom(float *__restrict__ u, MPGeom G) {
 // red-black update of the framed level (MGGravity::Smooth, mg_gravity.cpp:220-273)
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mb_smooth(T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3, T dx2, T isix, int color) {
+k_mb_smooth(const int *__restrict__ gate, T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3, T dx2, T isix, int color) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const int nh = (n1 + 1) / 2;
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (t >= (unsigned)nh * n2 * n3) return;
@@ -931,7 +948,8 @@ __device__ __forceinline__ T mb_defect(const T *u, const T *src, unsigned id, un
 // mg_gravity.cpp:324-351)
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mb_fas(const T *__restrict__ u, T *__restrict__ src, T *__restrict__ uold, int n1, int n2, int n3, T idx2) {
+k_mb_fas(const int *__restrict__ gate, const T *__restrict__ u, T *__restrict__ src, T *__restrict__ uold, int n1, int n2, int n3, T idx2) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   const unsigned w1 = n1 + 2, w2 = n2 + 2;
   if (t >= w1 * w2 * (unsigned)(n3 + 2)) return;
@@ -946,8 +964,9 @@ k_mb_fas(const T *__restrict__ u, T *__restrict__ src, T *__restrict__ uold, int
 // FAS, the average of u into the coarse u.  One thread per coarse interior cell.
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mb_restrict(const T *__restrict__ u, const T *__restrict__ src, T *__restrict__ csrc, T *__restrict__ cu,
+k_mb_restrict(const int *__restrict__ gate, const T *__restrict__ u, const T *__restrict__ src, T *__restrict__ csrc, T *__restrict__ cu,
               int n1, int n2, int n3, T idx2) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (t >= (unsigned)c1 * c2 * c3) return;
@@ -970,7 +989,8 @@ k_mb_restrict(const T *__restrict__ u, const T *__restrict__ src, T *__restrict_
 // gains the trilinear interpolation of (u - uold) of the framed coarse level.
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mb_prolong(const T *__restrict__ cu, const T *__restrict__ cuold, T *__restrict__ fu, int n1, int n2, int n3) {
+k_mb_prolong(const int *__restrict__ gate, const T *__restrict__ cu, const T *__restrict__ cuold, T *__restrict__ fu, int n1, int n2, int n3) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (t >= (unsigned)n1 * n2 * n3) return;
   const int fi = 1 + t % n1, r = t / n1, fj = 1 + r % n2, fk = 1 + r / n2;
@@ -989,8 +1009,9 @@ k_mb_prolong(const T *__restrict__ cu, const T *__restrict__ cuold, T *__restric
 // (ghosts filled) becomes the FP32 source of the correction cycle, whose correction
 // starts at zero over the framed level.
 __global__ void __launch_bounds__(BS)
-k_mb_defect32(const double *__restrict__ u, const double *__restrict__ src, float *__restrict__ src32,
+k_mb_defect32(const int *__restrict__ gate, const double *__restrict__ u, const double *__restrict__ src, float *__restrict__ src32,
               float *__restrict__ e32, int n1, int n2, int n3, double idx2) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   const unsigned w1 = n1 + 2, w2 = n2 + 2;
   if (t >= w1 * w2 * (unsigned)(n3 + 2)) return;
@@ -1002,7 +1023,8 @@ k_mb_defect32(const double *__restrict__ u, const double *__restrict__ src, floa
 
 // phi += e on the interior of the framed finest level
 __global__ void __launch_bounds__(BS)
-k_mb_add32(double *__restrict__ u, const float *__restrict__ e, int n1, int n2, int n3) {
+k_mb_add32(const int *__restrict__ gate, double *__restrict__ u, const float *__restrict__ e, int n1, int n2, int n3) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   const unsigned w1 = n1 + 2, w2 = n2 + 2;
   if (t >= w1 * w2 * (unsigned)(n3 + 2)) return;
@@ -1119,7 +1141,8 @@ __device__ __forceinline__ void mbc_prolong(const CoarseB<T> &L, int l) {   // l
 
 template <typename T, bool HOM>
 __global__ void __launch_bounds__(COARSE_BS)
-k_mb_coarse(CoarseB<T> L, int top, int fmglevel, int coffset, int ni) {
+k_mb_coarse(const int *__restrict__ gate, CoarseB<T> L, int top, int fmglevel, int coffset, int ni) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   for (int l = top; l > 0; --l) {
     mbc_bc<T, HOM>(L, l);
     if (l < fmglevel) mbc_fas(L, l);
@@ -1190,8 +1213,9 @@ k_mb_restrict_src(const double *__restrict__ src, double *__restrict__ csrc, int
 
 // squared defect of the framed finest level, partial sums per workgroup
 __global__ void __launch_bounds__(BS)
-k_mb_defect_partial(const double *__restrict__ u, const double *__restrict__ src, int n1, int n2, int n3,
+k_mb_defect_partial(const int *__restrict__ gate, const double *__restrict__ u, const double *__restrict__ src, int n1, int n2, int n3,
                     double idx2, double *__restrict__ part) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   __shared__ double sh[BS];
   unsigned t0, t1;
   wg_range((unsigned)n1 * n2 * n3, t0, t1);
@@ -1365,6 +1389,8 @@ static int64_t pad32(int64_t n) { return (n + 63) / 64 * 64; }   // 64 elements:
 static int64_t npart(int64_t n) { return std::min<int64_t>(grid1d(n, BS), NPART); }
 
 static double finish_norm(agx_mesh *m);
+static void first_defect_sum(agx_mesh *m);
+static void loop_step(agx_mesh *m);
 
 // ---------------------------------------------------------------- multipole hierarchy
 static int64_t bcells(const MGLevelB &L) { return (int64_t)(L.n1 + 2) * (L.n2 + 2) * (L.n3 + 2); }
@@ -1426,21 +1452,21 @@ static MPGeom mp_geom(const agx_mesh *m, const MGLevelB &B) {
 static void bc_b(agx_mesh *m, int l) {
   MGLevelB &B = m->levb[l];
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mb_bc, dim3(grid1d(bghosts(B), BS)), dim3(BS), 0, m->stream, B.u, mp_geom(m, B), B.bcv);
+  hipLaunchKernelGGL(k_mb_bc, dim3(grid1d(bghosts(B), BS)), dim3(BS), 0, m->stream, m->gate, B.u, mp_geom(m, B), B.bcv);
   timer_end(m);
 }
 static void smooth_b(agx_mesh *m, int l, int color) {
   MGLevelB &B = m->levb[l];
   const int64_t n = (int64_t)((B.n1 + 1) / 2) * B.n2 * B.n3;
   timer_begin(m, AGX_K_MG_SMOOTH);
-  hipLaunchKernelGGL(k_mb_smooth<double>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, B.u, B.src, B.n1, B.n2, B.n3,
+  hipLaunchKernelGGL(k_mb_smooth<double>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, m->gate, B.u, B.src, B.n1, B.n2, B.n3,
                      B.dx * B.dx, 1.15 / 6.0, color ^ m->coffset);
   timer_end(m);
 }
 static void fas_b(agx_mesh *m, int l) {
   MGLevelB &B = m->levb[l];
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mb_fas<double>, dim3(grid1d(bcells(B), BS)), dim3(BS), 0, m->stream, B.u, B.src, B.uold, B.n1,
+  hipLaunchKernelGGL(k_mb_fas<double>, dim3(grid1d(bcells(B), BS)), dim3(BS), 0, m->stream, m->gate, B.u, B.src, B.uold, B.n1,
                      B.n2, B.n3, 1.0 / (B.dx * B.dx));
   timer_end(m);
 }
@@ -1470,7 +1496,7 @@ static void coarse_b(agx_mesh *m, int top, int ni) {
     c.dx[l] = B.dx;
   }
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL((k_mb_coarse<double, false>), dim3(1), dim3(COARSE_BS), 0, m->stream, c, top, m->fmglevel,
+  hipLaunchKernelGGL((k_mb_coarse<double, false>), dim3(1), dim3(COARSE_BS), 0, m->stream, m->gate, c, top, m->fmglevel,
                      m->coffset, ni);
   timer_end(m);
 }
@@ -1485,7 +1511,7 @@ static void coarse_b32(agx_mesh *m, int top, int ni) {
     c.dx[l] = B.dx;
   }
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL((k_mb_coarse<float, true>), dim3(1), dim3(COARSE_BS), 0, m->stream, c, top, m->fmglevel,
+  hipLaunchKernelGGL((k_mb_coarse<float, true>), dim3(1), dim3(COARSE_BS), 0, m->stream, m->gate, c, top, m->fmglevel,
                      m->coffset, ni);
   timer_end(m);
 }
@@ -1508,7 +1534,7 @@ static void vcycle_b(agx_mesh *m) {
       smooth_b(m, l, 1);
       bc_b(m, l);
       timer_begin(m, AGX_K_MG_RESTRICT);
-      hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream,
+      hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream, m->gate,
                          F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
       timer_end(m);
       m->current--;
@@ -1520,7 +1546,7 @@ static void vcycle_b(agx_mesh *m) {
       MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
       bc_b(m, l - 1);
       timer_begin(m, AGX_K_MG_PROLONG);
-      hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, C.u,
+      hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, m->gate, C.u,
                          C.uold, F.u, F.n1, F.n2, F.n3);
       timer_end(m);
       bc_b(m, l);
@@ -1542,7 +1568,7 @@ static void vcycle_b(agx_mesh *m) {
     smooth_b(m, l, 1);
     bc_b(m, l);
     timer_begin(m, AGX_K_MG_RESTRICT);
-    hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream,
+    hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream, m->gate,
                        F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
     timer_end(m);
     m->current--;
@@ -1563,7 +1589,7 @@ static void vcycle_b(agx_mesh *m) {
     MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
     bc_b(m, l - 1);
     timer_begin(m, AGX_K_MG_PROLONG);
-    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, C.u,
+    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, m->gate, C.u,
                        C.uold, F.u, F.n1, F.n2, F.n3);
     timer_end(m);
     bc_b(m, l);
@@ -1604,7 +1630,7 @@ static void fmg_b(agx_mesh *m) {
 // correction (ghosts by the homogeneous chain), inside the FP64 defect correction
 static void bc_b32(agx_mesh *m, int l) {
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mb_bc_hom, dim3(grid1d(bghosts(m->levb[l]), BS)), dim3(BS), 0, m->stream, m->levb32[l].u,
+  hipLaunchKernelGGL(k_mb_bc_hom, dim3(grid1d(bghosts(m->levb[l]), BS)), dim3(BS), 0, m->stream, m->gate, m->levb32[l].u,
                      mp_geom(m, m->levb[l]));
   timer_end(m);
 }
@@ -1612,14 +1638,14 @@ static void smooth_b32(agx_mesh *m, int l, int color) {
   MGLevelB32 &B = m->levb32[l];
   const int64_t n = (int64_t)((B.n1 + 1) / 2) * B.n2 * B.n3;
   timer_begin(m, AGX_K_MG_SMOOTH);
-  hipLaunchKernelGGL(k_mb_smooth<float>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, B.u, B.src, B.n1, B.n2,
+  hipLaunchKernelGGL(k_mb_smooth<float>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, m->gate, B.u, B.src, B.n1, B.n2,
                      B.n3, float(B.dx * B.dx), float(1.15 / 6.0), color ^ m->coffset);
   timer_end(m);
 }
 static void fas_b32(agx_mesh *m, int l) {
   MGLevelB32 &B = m->levb32[l];
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mb_fas<float>, dim3(grid1d(bcells(m->levb[l]), BS)), dim3(BS), 0, m->stream, B.u, B.src,
+  hipLaunchKernelGGL(k_mb_fas<float>, dim3(grid1d(bcells(m->levb[l]), BS)), dim3(BS), 0, m->stream, m->gate, B.u, B.src,
                      B.uold, B.n1, B.n2, B.n3, float(1.0 / (B.dx * B.dx)));
   timer_end(m);
 }
@@ -1638,7 +1664,7 @@ static void vcycle_b32(agx_mesh *m) {   // the per-level sequence of vcycle_b
     bc_b32(m, l);
     timer_begin(m, AGX_K_MG_RESTRICT);
     hipLaunchKernelGGL(k_mb_restrict<float>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0,
-                       m->stream, F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, float(1.0 / (F.dx * F.dx)));
+                       m->stream, m->gate, F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, float(1.0 / (F.dx * F.dx)));
     timer_end(m);
     m->current--;
   }
@@ -1662,7 +1688,7 @@ static void vcycle_b32(agx_mesh *m) {   // the per-level sequence of vcycle_b
     bc_b32(m, l - 1);
     timer_begin(m, AGX_K_MG_PROLONG);
     hipLaunchKernelGGL(k_mb_prolong<float>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0,
-                       m->stream, C.u, C.uold, F.u, F.n1, F.n2, F.n3);
+                       m->stream, m->gate, C.u, C.uold, F.u, F.n1, F.n2, F.n3);
     timer_end(m);
     bc_b32(m, l);
     smooth_b32(m, l, 0);
@@ -1682,7 +1708,7 @@ static void correction_step_b(agx_mesh *m) {
   MGLevelB32 &E = m->levb32[L - 1];
   bc_b(m, L - 1);
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mb_defect32, dim3(grid1d(bcells(F), BS)), dim3(BS), 0, m->stream, F.u, F.src, E.src,
+  hipLaunchKernelGGL(k_mb_defect32, dim3(grid1d(bcells(F), BS)), dim3(BS), 0, m->stream, m->gate, F.u, F.src, E.src,
                      E.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
   timer_end(m);
   const int keep = m->fmglevel;
@@ -1691,19 +1717,23 @@ static void correction_step_b(agx_mesh *m) {
   vcycle_b32(m);
   m->fmglevel = keep;
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mb_add32, dim3(grid1d(bcells(F), BS)), dim3(BS), 0, m->stream, F.u, E.u, F.n1, F.n2,
+  hipLaunchKernelGGL(k_mb_add32, dim3(grid1d(bcells(F), BS)), dim3(BS), 0, m->stream, m->gate, F.u, E.u, F.n1, F.n2,
                      F.n3);
   timer_end(m);
   bc_b(m, L - 1);
 }
 
-// defect norm of the framed finest level (reads its ghost frame as it stands)
-static double defect_norm_b(agx_mesh *m) {
+// squared sum of the framed finest level's defect into scal[0] (reads its ghost frame as
+// it stands), and the defect norm from it
+static void defect_sum_b(agx_mesh *m) {
   MGLevelB &F = m->levb[m->ntotal - 1];
   const int64_t nb = npart((int64_t)F.n1 * F.n2 * F.n3);
-  hipLaunchKernelGGL(k_mb_defect_partial, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.u, F.src, F.n1, F.n2,
-                     F.n3, 1.0 / (F.dx * F.dx), m->red);
-  hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->red, nb, m->scal);
+  hipLaunchKernelGGL(k_mb_defect_partial, dim3((unsigned)nb), dim3(BS), 0, m->stream, m->gate, F.u, F.src, F.n1,
+                     F.n2, F.n3, 1.0 / (F.dx * F.dx), m->red);
+  hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->gate, m->red, nb, m->scal);
+}
+static double defect_norm_b(agx_mesh *m) {
+  defect_sum_b(m);
   return finish_norm(m);
 }
 
@@ -1733,8 +1763,9 @@ static void mp_coefficients(agx_mesh *m) {
   }
 }
 
-// MGGravityDriver::Solve (mg_gravity.cpp:166-211) with multipole boundaries
-static int mg_solve_b(agx_mesh *m) {
+// MGGravityDriver::Solve (mg_gravity.cpp:166-211) with multipole boundaries, up to the
+// iteration: load, multipole coefficients, FMG
+static void solve_load_b(agx_mesh *m) {
   const Geo &g = m->g;
   const int L = m->ntotal;
   MGLevelB &F = m->levb[L - 1];
@@ -1747,16 +1778,30 @@ static int mg_solve_b(agx_mesh *m) {
   m->mp_solved = true;
   if (m->mg_mode == 0) fmg_b(m);
   m->current = L - 1;
+}
+
+static int retrieve_b(agx_mesh *m) {
+  const Geo &g = m->g;
+  MGLevelB &F = m->levb[m->ntotal - 1];
+  const int64_t nret = (int64_t)(g.bx1 + 2) * (g.bx2 + 2) * (g.bx3 + 2) * g.nb;
+  hipLaunchKernelGGL(k_mb_retrieve, dim3(grid1d(nret, BS)), dim3(BS), 0, m->stream, g, F.u, m->arr.a[AGX_ARR_PHI]);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "multipole mg_solve launches");
+  return 0;
+}
+
+static int mg_solve_b(agx_mesh *m, int stage) {
+  solve_load_b(m);
   int iters = 0;
   if (m->mg_eps >= 0.0) {   // SolveIterative, multigrid_driver.cpp:721-755
-    double def = defect_norm_b(m);
+    first_defect_sum(m);
+    double def = finish_norm(m);
     int n = 0;
     while (def > m->mg_eps) {
-      if (m->mixed) correction_step_b(m);
-      else vcycle_b(m);
+      loop_step(m);   // correction_step_b (mixed) or vcycle_b, then defect_sum_b
       iters++;
       double olddef = def;
-      def = defect_norm_b(m);
+      def = finish_norm(m);
       if (def / olddef > 0.8) {
         if (m->mg_eps == 0.0) break;
       }
@@ -1764,6 +1809,7 @@ static int mg_solve_b(agx_mesh *m) {
       n++;
     }
     m->last_norm = def;
+    if (stage == 1 || stage == 2) m->mg_kpred[stage - 1] = iters;
   } else {   // SolveIterativeFixedTimes, multigrid_driver.cpp:762-771
     for (int n = 0; n < m->mg_niter; ++n) {
       if (m->mixed) correction_step_b(m);
@@ -1773,11 +1819,7 @@ static int mg_solve_b(agx_mesh *m) {
     m->last_norm = m->capturing ? NAN : defect_norm_b(m);   // diagnostic only, see mg_solve
   }
   m->last_iters = iters;
-  const int64_t nret = (int64_t)(g.bx1 + 2) * (g.bx2 + 2) * (g.bx3 + 2) * g.nb;
-  hipLaunchKernelGGL(k_mb_retrieve, dim3(grid1d(nret, BS)), dim3(BS), 0, m->stream, g, F.u, A[AGX_ARR_PHI]);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(e, "multipole mg_solve launches");
-  return 0;
+  return retrieve_b(m);
 }
 
 int mg_create(agx_mesh *m) {
@@ -1884,6 +1926,8 @@ int mg_create(agx_mesh *m) {
       }
     }
   }
+  if (!m->gatedev) AGX_HIP(hipMalloc(&m->gatedev, sizeof(MGGate)));
+  if (!m->gatehost) AGX_HIP(hipHostMalloc(&m->gatehost, sizeof(MGGate)));
   AGX_HIP(hipStreamSynchronize(m->stream));
   m->mg_on = 1;
   return 0;
@@ -1905,6 +1949,10 @@ void mg_destroy(agx_mesh *m) {
   m->mgpool32 = nullptr;
   m->lev.clear();
   m->lev32.clear();
+  if (m->gatedev) hipFree(m->gatedev);
+  if (m->gatehost) hipHostFree(m->gatehost);
+  m->gatedev = nullptr;
+  m->gatehost = nullptr;
   m->mg_on = 0;
 }
 
@@ -1919,10 +1967,10 @@ static void smooth(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l, int color) {
   int64_t n = (int64_t)((L.n1 + 1) / 2) * L.n2 * L.n3;
   timer_begin(m, AGX_K_MG_SMOOTH);
   if ((L.n1 & 1) == 0)
-    hipLaunchKernelGGL(k_mg_smooth_pairs<T>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, L.u, L.src, L.n1,
+    hipLaunchKernelGGL(k_mg_smooth_pairs<T>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, m->gate, L.u, L.src, L.n1,
                        L.n2, L.n3, T(dx2), T(isix), color ^ m->coffset);
   else
-    hipLaunchKernelGGL(k_mg_smooth<T>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, L.u, L.src, L.n1, L.n2,
+    hipLaunchKernelGGL(k_mg_smooth<T>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, m->gate, L.u, L.src, L.n1, L.n2,
                        L.n3, T(dx2), T(isix), color ^ m->coffset);
   timer_end(m);
 }
@@ -1948,7 +1996,7 @@ static void smooth_pass(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {
   }
   const int64_t nwg = (int64_t)(L.n1 / RB_TI) * (L.n2 / RB_TJ) * ((L.n3 + RB_KC - 1) / RB_KC);
   timer_begin(m, AGX_K_MG_SMOOTH);
-  hipLaunchKernelGGL(k_mg_smooth_rb<T>, dim3((unsigned)nwg), dim3(RB_TI * RB_TJ), 0, m->stream, L.u, L.def,
+  hipLaunchKernelGGL(k_mg_smooth_rb<T>, dim3((unsigned)nwg), dim3(RB_TI * RB_TJ), 0, m->stream, m->gate, L.u, L.def,
                      L.src, L.n1, L.n2, L.n3, T(L.dx * L.dx), T(1.15 / 6.0), m->coffset);
   timer_end(m);
   std::swap(L.u, L.def);
@@ -1959,7 +2007,7 @@ static void defect_restrict(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {
   MGLevelT<T> &F = H[l], &C = H[l - 1];
   double idx2 = 1.0 / (F.dx * F.dx);
   timer_begin(m, AGX_K_MG_RESTRICT);
-  hipLaunchKernelGGL(k_mg_defect_restrict<T>, dim3(grid1d(lcells(C), BS)), dim3(BS), 0, m->stream, F.u,
+  hipLaunchKernelGGL(k_mg_defect_restrict<T>, dim3(grid1d(lcells(C), BS)), dim3(BS), 0, m->stream, m->gate, F.u,
                      F.src, C.src, C.u, F.n1, F.n2, F.n3, T(idx2));
   timer_end(m);
 }
@@ -1969,10 +2017,10 @@ static void prolong(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {   // l-1 
   MGLevelT<T> &F = H[l], &C = H[l - 1];
   timer_begin(m, AGX_K_MG_PROLONG);
   if ((F.n1 & 1) == 0)   // every level above the root: n1 = 2 * (coarse n1)
-    hipLaunchKernelGGL(k_mg_prolong_pairs<T>, dim3(grid1d(lcells(F) / 2, BS)), dim3(BS), 0, m->stream, C.u, F.u,
+    hipLaunchKernelGGL(k_mg_prolong_pairs<T>, dim3(grid1d(lcells(F) / 2, BS)), dim3(BS), 0, m->stream, m->gate, C.u, F.u,
                        F.n1, F.n2, F.n3);
   else
-    hipLaunchKernelGGL(k_mg_prolong<T>, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, C.u, F.u, F.n1,
+    hipLaunchKernelGGL(k_mg_prolong<T>, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, m->gate, C.u, F.u, F.n1,
                        F.n2, F.n3);
   timer_end(m);
 }
@@ -1994,21 +2042,21 @@ static void level_sum(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l, int mode,
   double idx2 = 1.0 / (L.dx * L.dx);
   timer_begin(m, AGX_K_MG_OTHER);
   if (mode == 0)
-    hipLaunchKernelGGL((k_mg_partial<0, T>), dim3((unsigned)nb), dim3(BS), 0, m->stream, L.u, L.src, L.n1,
+    hipLaunchKernelGGL((k_mg_partial<0, T>), dim3((unsigned)nb), dim3(BS), 0, m->stream, m->gate, L.u, L.src, L.n1,
                        L.n2, L.n3, T(idx2), m->red);
   else
-    hipLaunchKernelGGL((k_mg_partial<1, T>), dim3((unsigned)nb), dim3(BS), 0, m->stream, L.u, L.src, L.n1,
+    hipLaunchKernelGGL((k_mg_partial<1, T>), dim3((unsigned)nb), dim3(BS), 0, m->stream, m->gate, L.u, L.src, L.n1,
                        L.n2, L.n3, T(idx2), m->red);
-  hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->red, nb, m->scal + slot);
+  hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->gate, m->red, nb, m->scal + slot);
   timer_end(m);
 }
 
 static void level_sum_src(agx_mesh *m, int l, int slot) {
   MGLevel &L = m->lev[l];
   int64_t nb = npart(lcells(L));
-  hipLaunchKernelGGL((k_mg_partial<0, double>), dim3((unsigned)nb), dim3(BS), 0, m->stream, L.src, L.src,
+  hipLaunchKernelGGL((k_mg_partial<0, double>), dim3((unsigned)nb), dim3(BS), 0, m->stream, m->gate, L.src, L.src,
                      L.n1, L.n2, L.n3, 0.0, m->red);
-  hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->red, nb, m->scal + slot);
+  hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->gate, m->red, nb, m->scal + slot);
 }
 
 static double mesh_volume(const agx_mesh_desc &d) {
@@ -2026,14 +2074,14 @@ static void subtract_average(agx_mesh *m, int type) {
   if (type == 0) level_sum_src(m, l, 1);
   else level_sum(m, m->lev, l, 0, 1);
   double scale = dv / mesh_volume(d);
-  hipLaunchKernelGGL(k_mg_add<double>, dim3(grid1d(lcells(L), BS)), dim3(BS), 0, m->stream,
+  hipLaunchKernelGGL(k_mg_add<double>, dim3(grid1d(lcells(L), BS)), dim3(BS), 0, m->stream, m->gate,
                      type ? L.u : L.src, lcells(L), m->scal + 1, scale);
 }
 
 template <typename T>
 static void zero_level(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {
   MGLevelT<T> &L = H[l];
-  hipLaunchKernelGGL(k_mg_zero<T>, dim3(grid1d(lcells(L), BS)), dim3(BS), 0, m->stream, L.u, lcells(L));
+  hipLaunchKernelGGL(k_mg_zero<T>, dim3(grid1d(lcells(L), BS)), dim3(BS), 0, m->stream, m->gate, L.u, lcells(L));
 }
 
 // SolveCoarsestGrid, multigrid_driver.cpp:778-819 (periodic: fsubtract_average_)
@@ -2060,7 +2108,7 @@ static void solve_coarsest(agx_mesh *m, std::vector<MGLevelT<T>> &H) {
       for (int it = 0; it < ni; ++it) { smooth(m, H, 0, 0); smooth(m, H, 0, 1); }
     }
     level_sum(m, H, 0, 0, 2);
-    hipLaunchKernelGGL(k_mg_add<T>, dim3(grid1d(lcells(LR), BS)), dim3(BS), 0, m->stream, LR.u, lcells(LR),
+    hipLaunchKernelGGL(k_mg_add<T>, dim3(grid1d(lcells(LR), BS)), dim3(BS), 0, m->stream, m->gate, LR.u, lcells(LR),
                        m->scal + 2, scale);
   }
 }
@@ -2090,7 +2138,7 @@ static void coarse_cycle(agx_mesh *m, std::vector<MGLevelT<T>> &H, int top, bool
     c.idx2[l] = T(1.0 / (H[l].dx * H[l].dx));
   }
   timer_begin(m, AGX_K_MG_SMOOTH);
-  hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(COARSE_BS), 0, m->stream, c, top, m->coffset,
+  hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(COARSE_BS), 0, m->stream, m->gate, c, top, m->coffset,
                      T(1.15 / 6.0), first_sweep_done ? 1 : 0);
   timer_end(m);
 }
@@ -2171,17 +2219,17 @@ static void defect_to32(agx_mesh *m, bool correct) {
   // e of the last cycle is read at neighbour cells, so the next cycle's start goes to the
   // spare FP32 array (then swapped), as the corrected phi goes to the FP64 one
   if (tiled && correct)
-    hipLaunchKernelGGL(k_mg_defect_to32_t<true>, dim3((unsigned)nb), dim3(DT_TI * DT_TJ), 0, m->stream, F.u,
+    hipLaunchKernelGGL(k_mg_defect_to32_t<true>, dim3((unsigned)nb), dim3(DT_TI * DT_TJ), 0, m->stream, m->gate, F.u,
                        F32.u, F.def, F.src, F32.src, F32.def, F.n1, F.n2, F.n3, idx2, dx2f, isixf, cred, m->red);
   else if (tiled)
-    hipLaunchKernelGGL(k_mg_defect_to32_t<false>, dim3((unsigned)nb), dim3(DT_TI * DT_TJ), 0, m->stream, F.u,
+    hipLaunchKernelGGL(k_mg_defect_to32_t<false>, dim3((unsigned)nb), dim3(DT_TI * DT_TJ), 0, m->stream, m->gate, F.u,
                        (const float *)nullptr, (double *)nullptr, F.src, F32.src, F32.def, F.n1, F.n2, F.n3, idx2,
                        dx2f, isixf, cred, m->red);
   else
-    hipLaunchKernelGGL(k_mg_defect_to32, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.u,
+    hipLaunchKernelGGL(k_mg_defect_to32, dim3((unsigned)nb), dim3(BS), 0, m->stream, m->gate, F.u,
                        correct ? F32.u : (const float *)nullptr, correct ? F.def : (double *)nullptr, F.src,
                        F32.src, F32.def, F.n1, F.n2, F.n3, idx2, dx2f, isixf, cred, m->red);
-  hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->red, nb, m->scal);
+  hipLaunchKernelGGL(k_mg_final, dim3(1), dim3(BS), 0, m->stream, m->gate, m->red, nb, m->scal);
   timer_end(m);
   if (correct) std::swap(F.u, F.def);
   std::swap(F32.u, F32.def);
@@ -2196,21 +2244,20 @@ static void correction_step(agx_mesh *m) {
   defect_to32(m, true);
 }
 
-int mg_solve(agx_mesh *m, int stage) {
-  (void)stage;
-  if (m->mp_on) return mg_solve_b(m);
+// MGGravityDriver::Solve (mg_gravity.cpp:166-211) on the periodic hierarchy, up to the
+// iteration: load, mean subtraction, FMG
+static void solve_load(agx_mesh *m) {
   const Geo &g = m->g;
   int L = m->ntotal;
   MGLevel &F = m->lev[L - 1];
   double **A = m->arr.a;
-  const bool mixed = m->mixed;
   hipLaunchKernelGGL(k_mg_load, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, g, A[AGX_ARR_U],
                      A[AGX_ARR_PHI], F.src, F.u, m->d.four_pi_G, m->mg_mode == 1 ? 1 : 0, src_mask(m));
   m->current = L - 1;
   m->fmglevel = L - 1;
   subtract_average(m, 0);   // SetupMultigrid, multigrid_driver.cpp:438-439
   if (m->mg_mode == 0) {
-    if (!mixed) {
+    if (!m->mixed) {
       fmg(m, m->lev);
     } else {
       // FMG in FP32 from the FP64 source; the result seeds the FP64 iterate
@@ -2224,19 +2271,53 @@ int mg_solve(agx_mesh *m, int stage) {
     m->current = L - 1;
     subtract_average(m, 1);
   }
+}
+
+static int retrieve(agx_mesh *m) {
+  const Geo &g = m->g;
+  int64_t nret = (int64_t)(g.bx1 + 2) * (g.bx2 + 2) * (g.bx3 + 2) * g.nb;
+  hipLaunchKernelGGL(k_mg_retrieve, dim3(grid1d(nret, BS)), dim3(BS), 0, m->stream, g, m->lev[m->ntotal - 1].u,
+                     m->arr.a[AGX_ARR_PHI]);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "mg_solve launches");
+  return 0;
+}
+
+// SolveIterative (multigrid_driver.cpp:721-755): the finest defect's squared sum into
+// scal[0] before the loop (the mixed form also writes the first correction cycle's
+// inputs), and one pass of the loop with the new defect's squared sum into scal[0]
+static void first_defect_sum(agx_mesh *m) {
+  if (m->mp_on) defect_sum_b(m);
+  else if (m->mixed) defect_to32(m, false);
+  else level_sum(m, m->lev, m->ntotal - 1, 1, 0);
+}
+static void loop_step(agx_mesh *m) {
+  if (m->mp_on) {
+    if (m->mixed) correction_step_b(m);
+    else vcycle_b(m);
+    defect_sum_b(m);
+  } else if (m->mixed) {
+    correction_step(m);   // the defect comes with the correction
+  } else {
+    vcycle(m, m->lev);
+    level_sum(m, m->lev, m->ntotal - 1, 1, 0);
+  }
+}
+
+int mg_solve(agx_mesh *m, int stage) {
+  if (m->mp_on) return mg_solve_b(m, stage);
+  const bool mixed = m->mixed;
+  solve_load(m);
   int iters = 0;
   if (m->mg_eps >= 0.0) {   // SolveIterative, multigrid_driver.cpp:721-755
-    double def;
-    if (mixed) { defect_to32(m, false); def = finish_norm(m); }
-    else def = defect_norm(m);
+    first_defect_sum(m);
+    double def = finish_norm(m);
     int n = 0;
     while (def > m->mg_eps) {
-      if (mixed) correction_step(m);
-      else vcycle(m, m->lev);
+      loop_step(m);
       iters++;
       double olddef = def;
-      if (mixed) def = finish_norm(m);   // the defect came with the correction
-      else def = defect_norm(m);
+      def = finish_norm(m);
       if (def / olddef > 0.8) {
         if (m->mg_eps == 0.0) break;
       }
@@ -2244,6 +2325,7 @@ int mg_solve(agx_mesh *m, int stage) {
       n++;
     }
     m->last_norm = def;
+    if (stage == 1 || stage == 2) m->mg_kpred[stage - 1] = iters;
     subtract_average(m, 1);
   } else {   // SolveIterativeFixedTimes, multigrid_driver.cpp:762-771
     if (mixed && m->mg_niter > 0) defect_to32(m, false);
@@ -2258,11 +2340,130 @@ int mg_solve(agx_mesh *m, int stage) {
     m->last_norm = m->capturing ? NAN : defect_norm(m);
   }
   m->last_iters = iters;
-  int64_t nret = (int64_t)(g.bx1 + 2) * (g.bx2 + 2) * (g.bx3 + 2) * g.nb;
-  hipLaunchKernelGGL(k_mg_retrieve, dim3(grid1d(nret, BS)), dim3(BS), 0, m->stream, g, F.u, A[AGX_ARR_PHI]);
+  return retrieve(m);
+}
+
+// ---- a threshold solve recorded into a graph
+//
+// The loop's V-cycle count is only known on the device, so the recorded loop runs k
+// passes whose kernels all take the gate: after each pass k_mg_gate_check applies the
+// loop's stopping rule to the new norm, and once it closes the gate the remaining passes
+// return at once.  The host state a pass changes (red-black offset, swapped arrays) is
+// captured after every recorded pass; mg_solve_settle reads how many passes ran and
+// restores the state after that many, then runs any further passes eagerly as mg_solve
+// would.  Every pass that runs is the same kernel sequence as mg_solve's, so the result
+// is the eager one bit for bit.
+
+// finish_norm on the device: sqrt(sum * dv / volume)
+__global__ void k_mg_gate_open(MGGate *g, const double *__restrict__ scal, double dv, double vol, double eps) {
+  if (threadIdx.x) return;
+  const double def = sqrt(scal[0] * dv / vol);
+  g->iters = 0;
+  g->def = def;
+  g->closed = (def > eps) ? 0 : 1;
+}
+
+// SolveIterative's tests after a V-cycle (multigrid_driver.cpp:733-753); n is the
+// loop's count before its increment
+__global__ void k_mg_gate_check(MGGate *g, const double *__restrict__ scal, double dv, double vol, double eps) {
+  if (threadIdx.x || g->closed) return;
+  const double olddef = g->def;
+  const double def = sqrt(scal[0] * dv / vol);
+  const int n = g->iters;
+  g->iters = n + 1;
+  g->def = def;
+  if ((def / olddef > 0.8 && eps == 0.0) || n > 100 || !(def > eps)) g->closed = 1;
+}
+
+MGSnap mg_snap(const agx_mesh *m) {
+  MGSnap s;
+  s.lev = m->lev;
+  s.lev32 = m->lev32;
+  s.coffset = m->coffset;
+  s.current = m->current;
+  s.fmglevel = m->fmglevel;
+  return s;
+}
+
+void mg_restore(agx_mesh *m, const MGSnap &s) {
+  m->lev = s.lev;
+  m->lev32 = s.lev32;
+  m->coffset = s.coffset;
+  m->current = s.current;
+  m->fmglevel = s.fmglevel;
+}
+
+std::vector<uintptr_t> mg_key(const agx_mesh *m) {
+  std::vector<uintptr_t> k;
+  for (const MGLevel &L : m->lev) { k.push_back((uintptr_t)L.u); k.push_back((uintptr_t)L.def); }
+  for (const MGLevel32 &L : m->lev32) { k.push_back((uintptr_t)L.u); k.push_back((uintptr_t)L.def); }
+  k.push_back((uintptr_t)m->coffset);
+  k.push_back((uintptr_t)m->current);
+  k.push_back((uintptr_t)m->fmglevel);
+  return k;
+}
+
+static double norm_dv(const agx_mesh *m) { return m->rdx_blk * m->rdy_blk * m->rdz_blk; }
+
+int mg_solve_head(agx_mesh *m) {
+  if (m->mp_on) solve_load_b(m);
+  else solve_load(m);
+  first_defect_sum(m);
+  hipLaunchKernelGGL(k_mg_gate_open, dim3(1), dim3(64), 0, m->stream, m->gatedev, m->scal, norm_dv(m),
+                     mesh_volume(m->d), m->mg_eps);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(e, "mg_solve launches");
+  return e == hipSuccess ? 0 : hip_fail(e, "mg_solve_head launches");
+}
+
+void mg_solve_gated(agx_mesh *m, int k, std::vector<MGSnap> &snaps) {
+  snaps.clear();
+  snaps.push_back(mg_snap(m));
+  m->gate = reinterpret_cast<const int *>(reinterpret_cast<const char *>(m->gatedev) + offsetof(MGGate, closed));
+  for (int i = 0; i < k; ++i) {
+    loop_step(m);
+    hipLaunchKernelGGL(k_mg_gate_check, dim3(1), dim3(64), 0, m->stream, m->gatedev, m->scal, norm_dv(m),
+                       mesh_volume(m->d), m->mg_eps);
+    snaps.push_back(mg_snap(m));
+  }
+  m->gate = nullptr;
+}
+
+int mg_solve_settle(agx_mesh *m, int stage, const std::vector<MGSnap> &snaps) {
+  hipError_t e = hipMemcpyAsync(m->gatehost, m->gatedev, sizeof(MGGate), hipMemcpyDeviceToHost, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e != hipSuccess) return hip_fail(e, "mg_solve_settle");
+  const MGGate gs = *m->gatehost;
+  if (gs.iters < 0 || gs.iters >= (int)snaps.size()) {
+    set_error("recorded multigrid loop reports %d V-cycles of %d", gs.iters, (int)snaps.size() - 1);
+    return -70;
+  }
+  mg_restore(m, snaps[gs.iters]);
+  double def = gs.def;
+  int iters = gs.iters;
+  if (!gs.closed) {   // the recorded passes ended before the loop did: go on as mg_solve
+    int n = iters;
+    while (def > m->mg_eps) {
+      loop_step(m);
+      iters++;
+      double olddef = def;
+      def = finish_norm(m);
+      if (def / olddef > 0.8) {
+        if (m->mg_eps == 0.0) break;
+      }
+      if (n > 100) break;
+      n++;
+    }
+  }
+  m->last_norm = def;
+  m->last_iters = iters;
+  if (stage == 1 || stage == 2) m->mg_kpred[stage - 1] = iters;
   return 0;
+}
+
+int mg_solve_tail(agx_mesh *m) {
+  if (m->mp_on) return retrieve_b(m);
+  subtract_average(m, 1);
+  return retrieve(m);
 }
 
 // Times V-cycles (on whatever the finest level holds; the next solve reloads it) with
@@ -2430,7 +2631,7 @@ extern "C" int agx_mg_restrict(agx_mesh *m, int32_t level) {
   if (m->mp_on) {
     MGLevelB &F = m->levb[level], &C = m->levb[level - 1];
     hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0,
-                       m->stream, F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
+                       m->stream, m->gate, F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
   } else {
     defect_restrict(m, m->lev, level);
   }
@@ -2443,7 +2644,7 @@ extern "C" int agx_mg_prolong(agx_mesh *m, int32_t level) {
   if (rc) return rc;
   if (m->mp_on) {
     MGLevelB &F = m->levb[level], &C = m->levb[level - 1];
-    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream,
+    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, m->gate,
                        C.u, C.uold, F.u, F.n1, F.n2, F.n3);
   } else {
     prolong(m, m->lev, level);

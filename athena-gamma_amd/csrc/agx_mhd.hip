@@ -3625,6 +3625,9 @@ static void graph_reset(agx_mesh *m) {
   for (auto &cg : m->graphs)
     if (cg.exec) hipGraphExecDestroy(cg.exec);
   m->graphs.clear();
+  for (auto &sg : m->seggraphs)
+    if (sg.exec) hipGraphExecDestroy(sg.exec);
+  m->seggraphs.clear();
 }
 
 // One fused cycle with everything on the device (flux and stage kernels read dt from
@@ -3713,6 +3716,106 @@ static int cycle_graphs(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done
   return rc;
 }
 
+// A cycle with threshold solves, in three segments that each end where the host learns
+// how many V-cycles a solve ran (mg_solve_settle): 0 = stage 1 and the first solve up to
+// the end of its recorded loop; 1 = that solve's tail, stage 2 and the second solve's
+// loop; 2 = the second solve's tail and the next time step.
+static int cycle_segment(agx_mesh *m, int seg, int k, std::vector<MGSnap> &snaps) {
+  int rc = 0;
+  if (seg > 0) rc = mg_solve_tail(m);
+  if (!rc && seg < 2) rc = stage_fused(m, seg + 1, seg == 1);
+  if (!rc && seg < 2) rc = mg_solve_head(m);
+  if (rc) return rc;
+  if (seg < 2) {
+    mg_solve_gated(m, k, snaps);
+    return 0;
+  }
+  const dim3 sg = stage_grid(m->g, m->tile);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(256), 0, m->stream, m->dtpart, (int64_t)sg.x * sg.y, m->scal);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, m->stream, m->scal, m->d.cfl);
+  LAUNCH_CHECK("k_advance");
+  snaps.assign(1, mg_snap(m));
+  return 0;
+}
+
+// Index into m->seggraphs of the graph of segment seg (k recorded V-cycles) that starts
+// from the current register table and multigrid state, recorded on first use; the host
+// state is left as the segment leaves it when all k V-cycles run.
+static int graph_for_segment(agx_mesh *m, int seg, int k, int *index) {
+  double **A = m->arr.a;
+  if (!m->seggraphs.empty() && m->seggraphs[0].gen != m->graph_gen) graph_reset(m);   // settings changed
+  const std::vector<uintptr_t> key = mg_key(m);
+  for (size_t i = 0; i < m->seggraphs.size(); ++i) {
+    agx_mesh::SegGraph &sgr = m->seggraphs[i];
+    if (sgr.seg == seg && sgr.k == k && sgr.mgkey == key && std::memcmp(sgr.before, A, sizeof(sgr.before)) == 0) {
+      std::memcpy(A, sgr.after, sizeof(sgr.after));
+      mg_restore(m, sgr.snaps.back());
+      *index = (int)i;
+      return 0;
+    }
+  }
+  if (m->seggraphs.size() >= 32) graph_reset(m);
+  agx_mesh::SegGraph sgr;
+  sgr.seg = seg;
+  sgr.k = k;
+  sgr.gen = m->graph_gen;
+  sgr.mgkey = key;
+  std::memcpy(sgr.before, A, sizeof(sgr.before));
+  const MGSnap start = mg_snap(m);
+  hipGraph_t graph = nullptr;
+  AGX_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+  m->capturing = true;
+  int rc = cycle_segment(m, seg, k, sgr.snaps);
+  m->capturing = false;
+  hipError_t e = hipStreamEndCapture(m->stream, &graph);
+  if (!rc && e == hipSuccess) {
+    e = hipGraphInstantiate(&sgr.exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) sgr.exec = nullptr;
+  }
+  if (graph) hipGraphDestroy(graph);
+  if (rc || e != hipSuccess) {
+    std::memcpy(A, sgr.before, sizeof(sgr.before));
+    mg_restore(m, start);
+    m->gate = nullptr;
+    return rc ? rc : hip_fail(e, "segment graph");
+  }
+  std::memcpy(sgr.after, A, sizeof(sgr.after));
+  m->seggraphs.push_back(std::move(sgr));
+  *index = (int)m->seggraphs.size() - 1;
+  return 0;
+}
+
+// agx_cycle with threshold solves: three graph launches per cycle.  The loop of each solve
+// is recorded with as many V-cycles as the same stage's solve ran last time (none before
+// the first cycle); passes beyond the stop return at once, and a solve that needs more
+// continues eagerly, so every cycle equals the eager one bit for bit.
+static int cycle_graphs_threshold(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
+  int32_t n = 0;
+  int rc = push_scalars(m, tlim);
+  while (!rc && n < ncycles && m->time < tlim) {
+    for (int seg = 0; seg < 3 && !rc; ++seg) {
+      const int k = seg < 2 ? std::max(m->mg_kpred[seg], 0) : 0;
+      int idx = -1;
+      rc = graph_for_segment(m, seg, k, &idx);
+      if (rc) break;
+      hipError_t e = hipGraphLaunch(m->seggraphs[idx].exec, m->stream);
+      if (e != hipSuccess) { rc = hip_fail(e, "cycle segment graph"); break; }
+      if (seg < 2) rc = mg_solve_settle(m, seg + 1, m->seggraphs[idx].snaps);
+    }
+    if (rc) break;
+    hipError_t e = hipMemcpyAsync(m->scal_host + SC_DT, m->scal + SC_DT, 2 * sizeof(double),
+                                  hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "cycle graph"); break; }
+    m->dt = m->scal_host[SC_DT];
+    m->time = m->scal_host[SC_TIME];
+    m->ncycle++;
+    ++n;
+  }
+  if (done) *done = n;
+  return rc;
+}
+
 int agx_set_graph(agx_mesh *m, int32_t enable) {
   if (!m) { set_error("null mesh"); return -1; }
   m->graph_on = enable != 0;
@@ -3724,12 +3827,14 @@ int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
   if (!m) { set_error("null mesh"); return -1; }
   if (int rc = refuse_tree(m, "agx_cycle")) return rc;
   if (int rc = ensure_copies_checked(m)) return rc;
-  // with self-gravity only the fixed-count solve (niteration, SolveIterativeFixedTimes) is
-  // recorded: the threshold form decides its V-cycle count from norms read on the host; a
-  // fixed multipole origin is copied from host memory at every solve
-  const bool sg_fixed = m->d.self_gravity && m->mg_on && m->mg_eps < 0.0 && (!m->mp_on || m->d.mp_auto_origin);
-  if (m->graph_on && !m->timer.enabled && (!m->d.self_gravity || sg_fixed))
+  // with self-gravity the fixed-count solve (niteration, SolveIterativeFixedTimes) is
+  // recorded whole; the threshold form (SolveIterative) in segments that end where the
+  // host reads how many V-cycles ran.  A fixed multipole origin is copied from host
+  // memory at every solve, so it runs eagerly.
+  const bool sg_rec = m->d.self_gravity && m->mg_on && (!m->mp_on || m->d.mp_auto_origin);
+  if (m->graph_on && !m->timer.enabled && (!m->d.self_gravity || (sg_rec && m->mg_eps < 0.0)))
     return cycle_graphs(m, tlim, ncycles, done);
+  if (m->graph_on && !m->timer.enabled && sg_rec) return cycle_graphs_threshold(m, tlim, ncycles, done);
   int32_t n = 0;
   int rc = 0;
   const dim3 sg = stage_grid(m->g, m->tile);

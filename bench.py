@@ -403,42 +403,56 @@ def sg_run(agx, pgen, N, B, xorder, mach, steps, tag, bx=None, sensitivity=False
 
 def sg_graph_bench(agx, pgen, N, B, steps, niter, bx=0.1, mach=2.0):
     """The self-gravity workload of sg_run (lognormal density, rms Mach `mach`, Bx, PLM +
-    HLLD + CT, FP64) with a fixed V-cycle count per solve (niteration = `niter`, FMG
-    first): `steps` cycles eager, then `steps` cycles replayed as one hipGraph per cycle
-    (both multigrid solves recorded with the MHD stages).  Returns both rates."""
+    HLLD + CT, FP64), `steps` cycles eager and `steps` cycles replayed from hipGraphs, in
+    two solver forms: a fixed V-cycle count per solve (niteration = `niter`, FMG first; one
+    graph per cycle with both solves recorded) and sg_run's threshold, 1e-10 x rms(rho -
+    <rho>) (three graph segments per cycle, each solve's loop recorded with the last
+    count and gated on the device by the stopping rule).  Returns the rates."""
     base = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4, gamma=5.0 / 3.0,
                          xorder=2, nghost=2, rsolver=0)
     U, b1, b2, b3 = pgen.turbulence_lognormal(base, seed=5, mach=mach, sigma_lnrho=1.0, beta=1.0, bx=bx)
+    src_norm = float(np.sqrt(np.mean((U[0] - U[0].mean()) ** 2)))
     ub = agx.scatter_cells(base, U)
     del U
     faces = pgen.scatter_faces(base, b1, b2, b3)
     del b1, b2, b3
-    desc = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4, gamma=5.0 / 3.0, xorder=2,
-                         nghost=2, rsolver=0, self_gravity=1, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0,
-                         mg_niter=niter)
     out = {}
-    for graph in (False, True):
-        m = agx.Mesh(desc)
-        m.set("u", ub)
-        for n, f in zip(("b1f", "b2f", "b3f"), faces):
-            m.set(n, f)
-        m.initialize()
-        m.autotune()
-        m.set_graph(graph)
-        m.cycle(1e30, 2)
-        m.sync()
-        t0 = time.perf_counter()
-        assert m.cycle(1e30, steps) == steps
-        m.sync()
-        sec = time.perf_counter() - t0
-        out["graph" if graph else "eager"] = float(N) ** 3 * steps / sec
-        m.close()
+    for form, eps, nit in (("fixed", -1.0, niter), ("threshold", 1e-10 * src_norm, -1)):
+        desc = agx.make_desc(nx=(N, N, N), bnx=(B, B, B), box=((0.0, 1.0),) * 3, cfl=0.4, gamma=5.0 / 3.0,
+                             xorder=2, nghost=2, rsolver=0, self_gravity=1, four_pi_G=1.0, mg_mode=0,
+                             mg_threshold=eps, mg_niter=nit)
+        for graph in (False, True):
+            m = agx.Mesh(desc)
+            m.set("u", ub)
+            for n, f in zip(("b1f", "b2f", "b3f"), faces):
+                m.set(n, f)
+            m.initialize()
+            m.autotune()
+            m.set_graph(graph)
+            m.cycle(1e30, 2)
+            m.sync()
+            t0 = time.perf_counter()
+            assert m.cycle(1e30, steps) == steps
+            m.sync()
+            sec = time.perf_counter() - t0
+            out[(form, graph)] = float(N) ** 3 * steps / sec
+            if form == "threshold":
+                out["iters"] = m.mg_last()[1]
+            m.close()
     res = {"workload": f"{N}^3 self-gravitating MHD (lognormal rho, rms Mach {mach:g}, Bx={bx:g}), "
                        f"{(N // B) ** 3} MeshBlocks of {B}^3, VL2+PLM+HLLD+CT FP64, FMG + {niter} V-cycles per solve",
-           "eager": out["eager"], "graph": out["graph"], "graph_vs_eager": out["graph"] / out["eager"],
+           "eager": out[("fixed", False)], "graph": out[("fixed", True)],
+           "graph_vs_eager": out[("fixed", True)] / out[("fixed", False)],
+           "threshold": {"workload": "the same with FMG + V-cycles to 1e-10 x rms(rho - <rho>) per solve",
+                         "eager": out[("threshold", False)], "graph": out[("threshold", True)],
+                         "graph_vs_eager": out[("threshold", True)] / out[("threshold", False)],
+                         "vcycles_last_solve": out["iters"]},
            "unit": "zone-cycles/s", "steps": steps}
-    log(f"[sg-graph] fixed {niter} V-cycles per solve: eager {out['eager']:.4e}, graph {out['graph']:.4e} "
+    log(f"[sg-graph] fixed {niter} V-cycles per solve: eager {res['eager']:.4e}, graph {res['graph']:.4e} "
         f"zone-cycles/s ({res['graph_vs_eager']:.3f}x)")
+    th = res["threshold"]
+    log(f"[sg-graph] threshold ({th['vcycles_last_solve']} V-cycles in the last solve): eager {th['eager']:.4e}, "
+        f"graph {th['graph']:.4e} zone-cycles/s ({th['graph_vs_eager']:.3f}x)")
     return res
 
 

@@ -144,3 +144,43 @@ def test_graph_with_fixed_count_gravity_equals_eager(agx, precision, mg_bc):
         assert a["iters"] == b["iters"] == 3
         for f in FIELDS + ("phi",):
             same(b[f], a[f], f"segment {k}: {f} (precision {precision}, mg_bc {mg_bc})")
+
+
+@pytest.mark.parametrize("precision,mg_bc", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_graph_with_threshold_gravity_equals_eager(agx, precision, mg_bc):
+    """Self-gravity solved to a threshold (SolveIterative): each solve's V-cycle loop is
+    recorded with the count the same stage's last solve needed, gated on the device by the
+    stopping rule, and continued eagerly when it needs more.  The thresholds change between
+    segments so that every case runs: no recorded pass (first cycle), exactly as many as
+    needed, more than needed (gated off) and fewer (continued).  Fields, potential, time,
+    the last defect norm and V-cycle count equal the eager path bit for bit."""
+    desc = turb_desc(self_gravity=1, four_pi_G=1.0, mg_mode=0, mg_threshold=1e-8, mg_niter=-1,
+                     precision=precision, mg_bc=mg_bc)
+    st = turbulent_state(desc, seed=9, mach=1.5, drho=0.5)
+    plan = ((2, True, 1e-8), (3, True, 1e-8), (1, False, 1e-8), (2, True, 1e-10), (2, True, 1e-6),
+            (1, True, 1e-9))
+    runs = []
+    for graph in (False, True):
+        m = agx.Mesh(desc)
+        load_state(m, st)
+        m.initialize()
+        snaps = []
+        for n, g, eps in plan:
+            m.mg_set_mode(0, eps, -1)
+            m.set_graph(g and graph)
+            assert m.cycle(1e9, n) == n
+            snap = {f: m.get(f) for f in FIELDS + ("phi",)}
+            snap["time"] = m.time()
+            snap["last"] = m.mg_last()
+            snaps.append(snap)
+            print(f"graph={graph} eps={eps:g}: last norm {snap['last'][0]:.6e}, {snap['last'][1]} V-cycles",
+                  flush=True)
+        runs.append(snaps)
+        m.close()
+    for k, (a, b) in enumerate(zip(*runs)):
+        assert a["time"] == b["time"], k
+        assert a["last"][1] == b["last"][1], (k, a["last"], b["last"])
+        assert a["last"][0] == b["last"][0], (k, a["last"], b["last"])
+        for f in FIELDS + ("phi",):
+            same(b[f], a[f], f"segment {k}: {f} (precision {precision}, mg_bc {mg_bc})")
+    assert len({s["last"][1] for s in runs[0]}) > 1   # the V-cycle count changed along the plan
