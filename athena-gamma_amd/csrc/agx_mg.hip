@@ -1737,6 +1737,14 @@ static double defect_norm_b(agx_mesh *m) {
   return finish_norm(m);
 }
 
+// origin of the expansion given in the input (<gravity>/mporigin_x1..x3, mg_gravity.cpp:116-118)
+__global__ void k_mb_set_origin(double *mp, double x1, double x2, double x3) {
+  if (threadIdx.x) return;
+  mp[0] = x1;
+  mp[1] = x2;
+  mp[2] = x3;
+}
+
 // MultipoleCoefficients of the finest source (SetupMultigrid, multigrid_driver.cpp:441-445)
 static void mp_coefficients(agx_mesh *m) {
   MGLevelB &F = m->levb[m->ntotal - 1];
@@ -1748,8 +1756,10 @@ static void mp_coefficients(agx_mesh *m) {
                        m->d.mporder, 1, part);
     hipLaunchKernelGGL(k_mb_moments_final, dim3(1), dim3(BS), 0, m->stream, part, (int)nb, m->d.mporder, 1, 0,
                        m->mpdev);
-  } else {
-    (void)hipMemcpyAsync(m->mpdev, m->d.mp_origin, 3 * sizeof(double), hipMemcpyHostToDevice, m->stream);
+  } else {   // the fixed origin is part of the mesh description: written by a kernel, so
+             // that a recorded solve holds it
+    hipLaunchKernelGGL(k_mb_set_origin, dim3(1), dim3(64), 0, m->stream, m->mpdev, m->d.mp_origin[0],
+                       m->d.mp_origin[1], m->d.mp_origin[2]);
   }
   hipLaunchKernelGGL(k_mb_moments, dim3((unsigned)nb), dim3(BS), 0, m->stream, F.src, G, m->mpdev, m->d.mporder,
                      0, part);

@@ -3829,9 +3829,8 @@ int agx_cycle(agx_mesh *m, double tlim, int32_t ncycles, int32_t *done) {
   if (int rc = ensure_copies_checked(m)) return rc;
   // with self-gravity the fixed-count solve (niteration, SolveIterativeFixedTimes) is
   // recorded whole; the threshold form (SolveIterative) in segments that end where the
-  // host reads how many V-cycles ran.  A fixed multipole origin is copied from host
-  // memory at every solve, so it runs eagerly.
-  const bool sg_rec = m->d.self_gravity && m->mg_on && (!m->mp_on || m->d.mp_auto_origin);
+  // host reads how many V-cycles ran
+  const bool sg_rec = m->d.self_gravity && m->mg_on;
   if (m->graph_on && !m->timer.enabled && (!m->d.self_gravity || (sg_rec && m->mg_eps < 0.0)))
     return cycle_graphs(m, tlim, ncycles, done);
   if (m->graph_on && !m->timer.enabled && sg_rec) return cycle_graphs_threshold(m, tlim, ncycles, done);

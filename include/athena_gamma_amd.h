@@ -364,16 +364,15 @@ int agx_mg_set_source_mask_sphere(agx_mesh *m, double xc, double yc, double zc, 
  * read back once per cycle for the tlim test.  Results are identical to the eager path.
  * With self-gravity the graph also holds both multigrid solves of the cycle when they run
  * a fixed number of V-cycles (threshold < 0, niteration; SolveIterativeFixedTimes,
- * multigrid_driver.cpp:762-771), periodic or multipole (with the automatic origin);
+ * multigrid_driver.cpp:762-771), periodic or multipole;
  * agx_mg_last then reports the V-cycle count and a NaN norm (the norm is only diagnostic
  * and is not read back).  Solves to a threshold (SolveIterative, multigrid_driver.cpp:
  * 721-755) replay as three graphs per cycle, split where the host reads how many V-cycles
  * each solve ran: each solve's loop is recorded with the count the same stage's previous
  * solve needed, a device-side check applies the loop's stopping rule after every V-cycle
  * and the recorded V-cycles after the stop return at once, and a solve that needs more
- * goes on eagerly; agx_mg_last reports the norm and count as the eager path does.  A
- * fixed multipole origin is copied from host memory at each solve: such solves, and
- * cycles with the built-in timers on, run eagerly.  Every setter that changes the
+ * goes on eagerly; agx_mg_last reports the norm and count as the eager path does.
+ * Cycles with the built-in timers on run eagerly.  Every setter that changes the
  * recorded work (multigrid mode, pass form, coarse kernel, source mask, stage form)
  * discards the recorded graphs.  On by default; enable = 0 selects the eager path. */
 int agx_set_graph(agx_mesh *m, int32_t enable);

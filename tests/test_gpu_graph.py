@@ -112,14 +112,23 @@ def test_graph_rebuilt_after_tile_change(agx):
             same(m.get(f), o.get(f), f"{f} tile {tile}")
 
 
-@pytest.mark.parametrize("precision,mg_bc", [(0, 0), (1, 0), (0, 1), (1, 1)])
-def test_graph_with_fixed_count_gravity_equals_eager(agx, precision, mg_bc):
+GRAV_FORMS = [(0, 0, 1), (1, 0, 1), (0, 1, 1), (1, 1, 1), (0, 1, 0), (1, 1, 0)]
+
+
+def grav_desc(precision, mg_bc, auto_origin, **kw):
+    """32^3 turbulence with multigrid self-gravity: periodic (mg_bc 0) or multipole
+    boundaries about the centre of mass (auto_origin 1) or about a fixed origin"""
+    mp = {} if auto_origin else dict(mp_auto_origin=0, mp_origin=(0.45, 0.5, 0.55))
+    return turb_desc(self_gravity=1, four_pi_G=1.0, mg_mode=0, precision=precision, mg_bc=mg_bc, **mp, **kw)
+
+
+@pytest.mark.parametrize("precision,mg_bc,auto_origin", GRAV_FORMS)
+def test_graph_with_fixed_count_gravity_equals_eager(agx, precision, mg_bc, auto_origin):
     """Self-gravity with a fixed V-cycle count (niteration), periodic or multipole
     boundaries: both multigrid solves of a cycle are recorded with the MHD stages.  Graph
     replays, interleaved with eager cycles and downloads, equal the eager path bit for
     bit, potential included."""
-    desc = turb_desc(self_gravity=1, four_pi_G=1.0, mg_mode=0, mg_threshold=-1.0, mg_niter=3, precision=precision,
-                     mg_bc=mg_bc)
+    desc = grav_desc(precision, mg_bc, auto_origin, mg_threshold=-1.0, mg_niter=3)
     st = turbulent_state(desc, seed=9, mach=1.5, drho=0.5)
     runs = []
     for graph in (False, True):
@@ -143,19 +152,18 @@ def test_graph_with_fixed_count_gravity_equals_eager(agx, precision, mg_bc):
         assert a["time"] == b["time"], k
         assert a["iters"] == b["iters"] == 3
         for f in FIELDS + ("phi",):
-            same(b[f], a[f], f"segment {k}: {f} (precision {precision}, mg_bc {mg_bc})")
+            same(b[f], a[f], f"segment {k}: {f} (precision {precision}, mg_bc {mg_bc}, auto origin {auto_origin})")
 
 
-@pytest.mark.parametrize("precision,mg_bc", [(0, 0), (1, 0), (0, 1), (1, 1)])
-def test_graph_with_threshold_gravity_equals_eager(agx, precision, mg_bc):
+@pytest.mark.parametrize("precision,mg_bc,auto_origin", GRAV_FORMS)
+def test_graph_with_threshold_gravity_equals_eager(agx, precision, mg_bc, auto_origin):
     """Self-gravity solved to a threshold (SolveIterative): each solve's V-cycle loop is
     recorded with the count the same stage's last solve needed, gated on the device by the
     stopping rule, and continued eagerly when it needs more.  The thresholds change between
     segments so that every case runs: no recorded pass (first cycle), exactly as many as
     needed, more than needed (gated off) and fewer (continued).  Fields, potential, time,
     the last defect norm and V-cycle count equal the eager path bit for bit."""
-    desc = turb_desc(self_gravity=1, four_pi_G=1.0, mg_mode=0, mg_threshold=1e-8, mg_niter=-1,
-                     precision=precision, mg_bc=mg_bc)
+    desc = grav_desc(precision, mg_bc, auto_origin, mg_threshold=1e-8, mg_niter=-1)
     st = turbulent_state(desc, seed=9, mach=1.5, drho=0.5)
     plan = ((2, True, 1e-8), (3, True, 1e-8), (1, False, 1e-8), (2, True, 1e-10), (2, True, 1e-6),
             (1, True, 1e-9))
@@ -182,5 +190,5 @@ def test_graph_with_threshold_gravity_equals_eager(agx, precision, mg_bc):
         assert a["last"][1] == b["last"][1], (k, a["last"], b["last"])
         assert a["last"][0] == b["last"][0], (k, a["last"], b["last"])
         for f in FIELDS + ("phi",):
-            same(b[f], a[f], f"segment {k}: {f} (precision {precision}, mg_bc {mg_bc})")
+            same(b[f], a[f], f"segment {k}: {f} (precision {precision}, mg_bc {mg_bc}, auto origin {auto_origin})")
     assert len({s["last"][1] for s in runs[0]}) > 1   # the V-cycle count changed along the plan
