@@ -871,3 +871,72 @@ def mhd_prolong_plan(tree, nx, ng, cc=None, fc=None):
                 sl["prolong_fc"][c] += step["prolong_fc"][c]
             k += 1
     return slots, order
+
+
+# ---- MeshBlocks of uneven sizes
+#
+# The reference's MeshBlocks all have the same cell count.  A mesh whose blocks have
+# 2^p times that many cells a side on any level is run as the tree of equal blocks it
+# divides into: block (level, x, y, z, p) has its lower corner at the level-`level`
+# position (x, y, z) counted in blocks of the base size nx (each a multiple of 2^p) and
+# becomes the 8^p leaves (level, x + a, y + b, z + c), 0 <= a, b, c < 2^p.  A block's
+# update does not depend on where the mesh is cut into blocks, so one device mesh of the
+# leaves runs every block of every size and level in the same launches.
+
+def uneven_leaves(blocks):
+    """the leaves of uneven `blocks` and, per leaf, (block index, a, b, c)"""
+    leaves, owner = [], []
+    for bi, (lev, x, y, z, p) in enumerate(blocks):
+        s = 1 << p
+        if x % s or y % s or z % s:
+            raise ValueError(f"block {bi}: corner ({x}, {y}, {z}) is not a multiple of {s}")
+        for c in range(s):
+            for b in range(s):
+                for a in range(s):
+                    leaves.append((lev, x + a, y + b, z + c))
+                    owner.append((bi, a, b, c))
+    return leaves, owner
+
+
+def _window(name, a, b, c, nx, ng):
+    """index window (k, j, i) of leaf (a, b, c) in its block's array, ghosts included"""
+    n = nx + 2 * ng
+    e = {"b1f": (0, 0, 1), "b2f": (0, 1, 0), "b3f": (1, 0, 0)}.get(name, (0, 0, 0))
+    return (slice(c * nx, c * nx + n + e[0]), slice(b * nx, b * nx + n + e[1]), slice(a * nx, a * nx + n + e[2]))
+
+
+def _interior(name, nx, ng):
+    """a leaf's own cells (or faces, both bounding faces included) within its window"""
+    e = {"b1f": (0, 0, 1), "b2f": (0, 1, 0), "b3f": (1, 0, 0)}.get(name, (0, 0, 0))
+    return tuple(slice(ng, ng + nx + e[d]) for d in range(3))
+
+
+def split_uneven(blocks, owner, name, arrays, nx, ng):
+    """leaf arrays of variable `name` ("u", "w", "bcc": (nvar, n, n, n); "b1f", "b2f",
+    "b3f": the face layouts) from one array per uneven block, ghosts included (a leaf's
+    ghosts inside its block are its neighbours' cells)"""
+    out = []
+    for bi, a, b, c in owner:
+        w = _window(name, a, b, c, nx, ng)
+        src = arrays[bi]
+        out.append(np.array(src[(Ellipsis,) + w]))
+    return np.stack(out)
+
+
+def merge_uneven(blocks, owner, name, leaves_arr, nx, ng):
+    """one array per uneven block from the leaf arrays: cells and faces from the leaf that
+    owns them, the block's ghost zones from its boundary leaves"""
+    out = []
+    for lev, x, y, z, p in blocks:
+        n = (nx << p) + 2 * ng
+        e = {"b1f": (0, 0, 1), "b2f": (0, 1, 0), "b3f": (1, 0, 0)}.get(name, (0, 0, 0))
+        lead = leaves_arr.shape[1:-3]
+        out.append(np.zeros(lead + (n + e[0], n + e[1], n + e[2])))
+    for g, (bi, a, b, c) in enumerate(owner):   # whole windows: the block's ghost zones
+        out[bi][(Ellipsis,) + _window(name, a, b, c, nx, ng)] = leaves_arr[g]
+    inner = _interior(name, nx, ng)
+    for g, (bi, a, b, c) in enumerate(owner):   # then every leaf's own cells and faces
+        w = _window(name, a, b, c, nx, ng)
+        dst = tuple(slice(w[d].start + inner[d].start, w[d].start + inner[d].stop) for d in range(3))
+        out[bi][(Ellipsis,) + dst] = leaves_arr[g][(Ellipsis,) + inner]
+    return out

@@ -228,3 +228,35 @@ def test_lattice_entry_points_refuse_a_refined_mesh(agx):
                 call()
     finally:
         m.close()
+
+
+def test_uneven_blocks_on_three_levels_equal_oracle(agx):
+    """MeshBlocks of 16^3 and 32^3 cells on three levels (smr.uneven_leaves: 7 root blocks
+    of 32^3, 7 level-1 blocks of 32^3 and 7 of 16^3, 8 level-2 blocks of 16^3).  The state
+    is given per uneven block, split into the 127 leaves of 16^3 that one device mesh holds,
+    run 3 cycles and merged back per block: equal bit for bit to the oracle's run of the
+    same mesh as 16^3 blocks."""
+    from tests.conftest import load_smr
+    from tests.test_smr_uneven import three_level_blocks
+    smr = load_smr()
+    nx, ng = 16, 2
+    blocks = three_level_blocks()
+    leaves, owner = smr.uneven_leaves(blocks)
+    tree = smr.SMRTree((4, 4, 4), leaves)
+    names = ("u", "b1f", "b2f", "b3f")
+    leaf_state = sc.smooth_state(tree, nx, ng, BOX, amp=0.3)
+    per_block = [smr.merge_uneven(blocks, owner, n, a, nx, ng) for n, a in zip(names, leaf_state)]
+    assert {(a.shape[-1] - 2 * ng) for a in per_block[0]} == {16, 32}
+    state = [smr.split_uneven(blocks, owner, n, arrs, nx, ng) for n, arrs in zip(names, per_block)]
+    for n, a, b in zip(names, state, leaf_state):
+        inner = (slice(None),) * (a.ndim - 3) + smr._interior(n, nx, ng)
+        assert np.array_equal(a[inner], b[inner]), n
+    got = gpu_run(agx, tree, nx, state, 3)
+    ref = oracle_run(tree, nx, state, 3)
+    assert got["time"] == ref["time"]
+    for f in FIELDS:
+        same(got[f], ref[f], f"uneven blocks: {f}")
+        mg = smr.merge_uneven(blocks, owner, f, got[f], nx, ng)
+        mo = smr.merge_uneven(blocks, owner, f, ref[f], nx, ng)
+        for bi, (a, b) in enumerate(zip(mg, mo)):
+            same(a, b, f"uneven block {bi}: {f}")
