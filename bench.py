@@ -844,6 +844,8 @@ def main():
     ap.add_argument("--no-smr", action="store_true", help="skip the three-level static-refinement MHD leg")
     ap.add_argument("--smr-block", type=int, default=32)
     ap.add_argument("--smr-steps", type=int, default=10)
+    ap.add_argument("--mg-only", action="store_true",
+                    help="only the 256^3 multigrid legs (periodic and multipole, FP64 and mixed)")
     ap.add_argument("--smr-only", action="store_true",
                     help="run only the static-refinement leg and print its JSON object (for profiling)")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
@@ -871,6 +873,14 @@ def main():
     agx, pgen = load_pkg()
     if args.smr_only:
         print(json.dumps(smr_cycle_bench(agx, pgen, args)), flush=True)
+        return
+    if args.mg_only:
+        res = {}
+        for form in ("fp64", "mixed", "multipole", "multipole_mixed"):
+            vps, iters, norm, t_solve, per = mg_bench(agx, pgen, args, form)
+            res[form] = {"vcycles_per_s": vps, "solve_ms_to_1e-10": t_solve * 1e3, "cycles_after_fmg": iters,
+                         "final_defect": norm, "kernel_ms_per_cycle": {k: v[0] / 3 for k, v in per.items()}}
+        print(json.dumps({"mg": res}), flush=True)
         return
     if args.large:
         large_bench(agx, pgen, args)
