@@ -125,7 +125,8 @@ k_mg_smooth_pairs(const int *__restrict__ gate, T *__restrict__ u, const T *__re
 constexpr int RB_TI = 64, RB_TJ = 8, RB_KC = 16;
 template <typename T>
 __global__ void __launch_bounds__(RB_TI * RB_TJ)
-k_mg_smooth_rb(const int *__restrict__ gate, const T *__restrict__ uin, T *__restrict__ uout, const T *__restrict__ src, int n1, int n2,
+k_mg_smooth_rb(const int *__restrict__ gate,
+               const T *__restrict__ uin, T *__restrict__ uout, const T *__restrict__ src, int n1, int n2,
                int n3, T dx2, T isix, int c0) {
   if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   constexpr int TI = RB_TI, TJ = RB_TJ, NT = TI * TJ, OI = TI + 4, OJ = TJ + 4, RI = TI + 2, RJ = TJ + 2;
@@ -541,7 +542,8 @@ k_mg_partial(const int *__restrict__ gate, const T *__restrict__ a, const T *__r
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ void __launch_bounds__(BS) k_mg_final(const int *__restrict__ gate, const double *__restrict__ part, int64_t n,
+__global__ void __launch_bounds__(BS) k_mg_final(const int *__restrict__ gate,
+                                  const double *__restrict__ part, int64_t n,
                                                  double *__restrict__ out) {
   if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   __shared__ double sh[BS];
@@ -581,7 +583,8 @@ __global__ void __launch_bounds__(BS) k_mg_zero(const int *__restrict__ gate, T 
 // sweep (colour cred) from e = 0 gives gs_value(0, ..., 0, d) at those cells and 0
 // elsewhere, so that cycle starts with its second sweep.
 __global__ void __launch_bounds__(BS)
-k_mg_defect_to32(const int *__restrict__ gate, const double *__restrict__ u, const float *__restrict__ ein, double *__restrict__ uout,
+k_mg_defect_to32(const int *__restrict__ gate,
+                 const double *__restrict__ u, const float *__restrict__ ein, double *__restrict__ uout,
                  const double *__restrict__ src, float *__restrict__ d32, float *__restrict__ e0, int n1, int n2,
                  int n3, double idx2, float dx2f, float isixf, int cred, double *__restrict__ part) {
   if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
@@ -626,7 +629,8 @@ k_mg_defect_to32(const int *__restrict__ gate, const double *__restrict__ u, con
 constexpr int DT_TI = 64, DT_TJ = 8, DT_KC = 16;
 template <bool CORR>
 __global__ void __launch_bounds__(DT_TI * DT_TJ)
-k_mg_defect_to32_t(const int *__restrict__ gate, const double *__restrict__ u, const float *__restrict__ ein, double *__restrict__ uout,
+k_mg_defect_to32_t(const int *__restrict__ gate,
+                   const double *__restrict__ u, const float *__restrict__ ein, double *__restrict__ uout,
                    const double *__restrict__ src, float *__restrict__ d32, float *__restrict__ e0, int n1,
                    int n2, int n3, double idx2, float dx2f, float isixf, int cred, double *__restrict__ part) {
   if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
@@ -922,7 +926,8 @@ k_mb_bc_hom(const int *__restrict__ gate, float *__restrict__ u, MPGeom G) {
 // red-black update of the framed level (MGGravity::Smooth, mg_gravity.cpp:220-273)
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mb_smooth(const int *__restrict__ gate, T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3, T dx2, T isix, int color) {
+k_mb_smooth(const int *__restrict__ gate,
+            T *__restrict__ u, const T *__restrict__ src, int n1, int n2, int n3, T dx2, T isix, int color) {
   if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const int nh = (n1 + 1) / 2;
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
@@ -948,7 +953,8 @@ __device__ __forceinline__ T mb_defect(const T *u, const T *src, unsigned id, un
 // mg_gravity.cpp:324-351)
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mb_fas(const int *__restrict__ gate, const T *__restrict__ u, T *__restrict__ src, T *__restrict__ uold, int n1, int n2, int n3, T idx2) {
+k_mb_fas(const int *__restrict__ gate,
+         const T *__restrict__ u, T *__restrict__ src, T *__restrict__ uold, int n1, int n2, int n3, T idx2) {
   if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   const unsigned w1 = n1 + 2, w2 = n2 + 2;
@@ -964,7 +970,8 @@ k_mb_fas(const int *__restrict__ gate, const T *__restrict__ u, T *__restrict__ 
 // FAS, the average of u into the coarse u.  One thread per coarse interior cell.
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mb_restrict(const int *__restrict__ gate, const T *__restrict__ u, const T *__restrict__ src, T *__restrict__ csrc, T *__restrict__ cu,
+k_mb_restrict(const int *__restrict__ gate,
+              const T *__restrict__ u, const T *__restrict__ src, T *__restrict__ csrc, T *__restrict__ cu,
               int n1, int n2, int n3, T idx2) {
   if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const int c1 = n1 >> 1, c2 = n2 >> 1, c3 = n3 >> 1;
@@ -989,7 +996,8 @@ k_mb_restrict(const int *__restrict__ gate, const T *__restrict__ u, const T *__
 // gains the trilinear interpolation of (u - uold) of the framed coarse level.
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_mb_prolong(const int *__restrict__ gate, const T *__restrict__ cu, const T *__restrict__ cuold, T *__restrict__ fu, int n1, int n2, int n3) {
+k_mb_prolong(const int *__restrict__ gate,
+             const T *__restrict__ cu, const T *__restrict__ cuold, T *__restrict__ fu, int n1, int n2, int n3) {
   if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (t >= (unsigned)n1 * n2 * n3) return;
@@ -1009,7 +1017,8 @@ k_mb_prolong(const int *__restrict__ gate, const T *__restrict__ cu, const T *__
 // (ghosts filled) becomes the FP32 source of the correction cycle, whose correction
 // starts at zero over the framed level.
 __global__ void __launch_bounds__(BS)
-k_mb_defect32(const int *__restrict__ gate, const double *__restrict__ u, const double *__restrict__ src, float *__restrict__ src32,
+k_mb_defect32(const int *__restrict__ gate,
+              const double *__restrict__ u, const double *__restrict__ src, float *__restrict__ src32,
               float *__restrict__ e32, int n1, int n2, int n3, double idx2) {
   if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
@@ -1213,7 +1222,8 @@ k_mb_restrict_src(const double *__restrict__ src, double *__restrict__ csrc, int
 
 // squared defect of the framed finest level, partial sums per workgroup
 __global__ void __launch_bounds__(BS)
-k_mb_defect_partial(const int *__restrict__ gate, const double *__restrict__ u, const double *__restrict__ src, int n1, int n2, int n3,
+k_mb_defect_partial(const int *__restrict__ gate,
+                    const double *__restrict__ u, const double *__restrict__ src, int n1, int n2, int n3,
                     double idx2, double *__restrict__ part) {
   if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
   __shared__ double sh[BS];
@@ -1459,14 +1469,16 @@ static void smooth_b(agx_mesh *m, int l, int color) {
   MGLevelB &B = m->levb[l];
   const int64_t n = (int64_t)((B.n1 + 1) / 2) * B.n2 * B.n3;
   timer_begin(m, AGX_K_MG_SMOOTH);
-  hipLaunchKernelGGL(k_mb_smooth<double>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, m->gate, B.u, B.src, B.n1, B.n2, B.n3,
+  hipLaunchKernelGGL(k_mb_smooth<double>, dim3(grid1d(n, BS)), dim3(BS), 0, m->stream, m->gate,
+                     B.u, B.src, B.n1, B.n2, B.n3,
                      B.dx * B.dx, 1.15 / 6.0, color ^ m->coffset);
   timer_end(m);
 }
 static void fas_b(agx_mesh *m, int l) {
   MGLevelB &B = m->levb[l];
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mb_fas<double>, dim3(grid1d(bcells(B), BS)), dim3(BS), 0, m->stream, m->gate, B.u, B.src, B.uold, B.n1,
+  hipLaunchKernelGGL(k_mb_fas<double>, dim3(grid1d(bcells(B), BS)), dim3(BS), 0, m->stream, m->gate,
+                     B.u, B.src, B.uold, B.n1,
                      B.n2, B.n3, 1.0 / (B.dx * B.dx));
   timer_end(m);
 }
@@ -1534,8 +1546,8 @@ static void vcycle_b(agx_mesh *m) {
       smooth_b(m, l, 1);
       bc_b(m, l);
       timer_begin(m, AGX_K_MG_RESTRICT);
-      hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream, m->gate,
-                         F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
+      hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream,
+                         m->gate, F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
       timer_end(m);
       m->current--;
     }
@@ -1546,7 +1558,8 @@ static void vcycle_b(agx_mesh *m) {
       MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
       bc_b(m, l - 1);
       timer_begin(m, AGX_K_MG_PROLONG);
-      hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, m->gate, C.u,
+      hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream,
+                         m->gate, C.u,
                          C.uold, F.u, F.n1, F.n2, F.n3);
       timer_end(m);
       bc_b(m, l);
@@ -1568,8 +1581,8 @@ static void vcycle_b(agx_mesh *m) {
     smooth_b(m, l, 1);
     bc_b(m, l);
     timer_begin(m, AGX_K_MG_RESTRICT);
-    hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream, m->gate,
-                       F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
+    hipLaunchKernelGGL(k_mb_restrict<double>, dim3(grid1d((int64_t)C.n1 * C.n2 * C.n3, BS)), dim3(BS), 0, m->stream,
+                       m->gate, F.u, F.src, C.src, C.u, F.n1, F.n2, F.n3, 1.0 / (F.dx * F.dx));
     timer_end(m);
     m->current--;
   }
@@ -1589,7 +1602,8 @@ static void vcycle_b(agx_mesh *m) {
     MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
     bc_b(m, l - 1);
     timer_begin(m, AGX_K_MG_PROLONG);
-    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, m->gate, C.u,
+    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream,
+                       m->gate, C.u,
                        C.uold, F.u, F.n1, F.n2, F.n3);
     timer_end(m);
     bc_b(m, l);
@@ -1630,7 +1644,8 @@ static void fmg_b(agx_mesh *m) {
 // correction (ghosts by the homogeneous chain), inside the FP64 defect correction
 static void bc_b32(agx_mesh *m, int l) {
   timer_begin(m, AGX_K_MG_OTHER);
-  hipLaunchKernelGGL(k_mb_bc_hom, dim3(grid1d(bghosts(m->levb[l]), BS)), dim3(BS), 0, m->stream, m->gate, m->levb32[l].u,
+  hipLaunchKernelGGL(k_mb_bc_hom, dim3(grid1d(bghosts(m->levb[l]), BS)), dim3(BS), 0, m->stream, m->gate,
+                     m->levb32[l].u,
                      mp_geom(m, m->levb[l]));
   timer_end(m);
 }
@@ -2027,7 +2042,8 @@ static void prolong(agx_mesh *m, std::vector<MGLevelT<T>> &H, int l) {   // l-1 
   MGLevelT<T> &F = H[l], &C = H[l - 1];
   timer_begin(m, AGX_K_MG_PROLONG);
   if ((F.n1 & 1) == 0)   // every level above the root: n1 = 2 * (coarse n1)
-    hipLaunchKernelGGL(k_mg_prolong_pairs<T>, dim3(grid1d(lcells(F) / 2, BS)), dim3(BS), 0, m->stream, m->gate, C.u, F.u,
+    hipLaunchKernelGGL(k_mg_prolong_pairs<T>, dim3(grid1d(lcells(F) / 2, BS)), dim3(BS), 0, m->stream, m->gate,
+                       C.u, F.u,
                        F.n1, F.n2, F.n3);
   else
     hipLaunchKernelGGL(k_mg_prolong<T>, dim3(grid1d(lcells(F), BS)), dim3(BS), 0, m->stream, m->gate, C.u, F.u, F.n1,
@@ -2654,8 +2670,8 @@ extern "C" int agx_mg_prolong(agx_mesh *m, int32_t level) {
   if (rc) return rc;
   if (m->mp_on) {
     MGLevelB &F = m->levb[level], &C = m->levb[level - 1];
-    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream, m->gate,
-                       C.u, C.uold, F.u, F.n1, F.n2, F.n3);
+    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream,
+                       m->gate, C.u, C.uold, F.u, F.n1, F.n2, F.n3);
   } else {
     prolong(m, m->lev, level);
   }
