@@ -1013,6 +1013,33 @@ k_mb_prolong(const int *__restrict__ gate,
 #undef C
 }
 
+// k_mb_prolong for the two x1 children (2i-1, 2i) of each coarse cell i, which share the
+// coarse (u - uold) values at i-1, i, i+1; every fine cell's sum is k_mb_prolong's, term
+// for term.  n1 even (every level above the root grid).
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_mb_prolong_pairs(const int *__restrict__ gate, const T *__restrict__ cu, const T *__restrict__ cuold,
+                   T *__restrict__ fu, int n1, int n2, int n3) {
+  if (gate && *gate) return;   // a closed gate: the recorded V-cycle loop has ended
+  const int c1 = n1 >> 1, c2 = n2 >> 1;
+  const unsigned t = xcd_slot(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  if (t >= (unsigned)c1 * n2 * n3) return;
+  const int i = 1 + t % c1, r = t / c1, fj = 1 + r % n2, fk = 1 + r / n2;
+  const int j = (fj - 1) / 2 + 1, k = (fk - 1) / 2 + 1;
+  const int dj = ((fj - 1) & 1) ? 1 : -1, dk = ((fk - 1) & 1) ? 1 : -1;
+#define C(kk, jj, ii) (cu[BX(c1, c2, kk, jj, ii)] - cuold[BX(c1, c2, kk, jj, ii)])
+  const T c0 = C(k, j, i), cj = C(k, j + dj, i), ck = C(k + dk, j, i), cjk = C(k + dk, j + dj, i);
+  const unsigned f = BX(n1, n2, fk, fj, 2 * i - 1);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int di = h ? 1 : -1;
+    const T ci = C(k, j, i + di), cij = C(k, j + dj, i + di), cik = C(k + dk, j, i + di),
+            cijk = C(k + dk, j + dj, i + di);
+    fu[f + h] += T(0.015625) * (T(27.0) * c0 + cijk + T(9.0) * (ci + cj + ck) + T(3.0) * (cjk + cik + cij));
+  }
+#undef C
+}
+
 // Mixed precision with multipole boundaries: the FP64 defect of the framed finest level
 // (ghosts filled) becomes the FP32 source of the correction cycle, whose correction
 // starts at zero over the framed level.
@@ -1459,6 +1486,18 @@ static MPGeom mp_geom(const agx_mesh *m, const MGLevelB &B) {
   return G;
 }
 
+// ProlongateAndCorrect with FAS from the framed coarse level: the paired form where the
+// fine rows have even length
+template <typename T>
+static void mb_prolong(agx_mesh *m, const T *cu, const T *cuold, T *fu, int n1, int n2, int n3) {
+  if ((n1 & 1) == 0)
+    hipLaunchKernelGGL(k_mb_prolong_pairs<T>, dim3(grid1d((int64_t)(n1 / 2) * n2 * n3, BS)), dim3(BS), 0,
+                       m->stream, m->gate, cu, cuold, fu, n1, n2, n3);
+  else
+    hipLaunchKernelGGL(k_mb_prolong<T>, dim3(grid1d((int64_t)n1 * n2 * n3, BS)), dim3(BS), 0, m->stream,
+                       m->gate, cu, cuold, fu, n1, n2, n3);
+}
+
 static void bc_b(agx_mesh *m, int l) {
   MGLevelB &B = m->levb[l];
   timer_begin(m, AGX_K_MG_OTHER);
@@ -1558,9 +1597,7 @@ static void vcycle_b(agx_mesh *m) {
       MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
       bc_b(m, l - 1);
       timer_begin(m, AGX_K_MG_PROLONG);
-      hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream,
-                         m->gate, C.u,
-                         C.uold, F.u, F.n1, F.n2, F.n3);
+      mb_prolong(m, C.u, C.uold, F.u, F.n1, F.n2, F.n3);
       timer_end(m);
       bc_b(m, l);
       smooth_b(m, l, 0);
@@ -1602,9 +1639,7 @@ static void vcycle_b(agx_mesh *m) {
     MGLevelB &F = m->levb[l], &C = m->levb[l - 1];
     bc_b(m, l - 1);
     timer_begin(m, AGX_K_MG_PROLONG);
-    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream,
-                       m->gate, C.u,
-                       C.uold, F.u, F.n1, F.n2, F.n3);
+    mb_prolong(m, C.u, C.uold, F.u, F.n1, F.n2, F.n3);
     timer_end(m);
     bc_b(m, l);
     smooth_b(m, l, 0);
@@ -1702,8 +1737,7 @@ static void vcycle_b32(agx_mesh *m) {   // the per-level sequence of vcycle_b
     MGLevelB32 &F = m->levb32[l], &C = m->levb32[l - 1];
     bc_b32(m, l - 1);
     timer_begin(m, AGX_K_MG_PROLONG);
-    hipLaunchKernelGGL(k_mb_prolong<float>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0,
-                       m->stream, m->gate, C.u, C.uold, F.u, F.n1, F.n2, F.n3);
+    mb_prolong(m, C.u, C.uold, F.u, F.n1, F.n2, F.n3);
     timer_end(m);
     bc_b32(m, l);
     smooth_b32(m, l, 0);
@@ -2670,8 +2704,7 @@ extern "C" int agx_mg_prolong(agx_mesh *m, int32_t level) {
   if (rc) return rc;
   if (m->mp_on) {
     MGLevelB &F = m->levb[level], &C = m->levb[level - 1];
-    hipLaunchKernelGGL(k_mb_prolong<double>, dim3(grid1d((int64_t)F.n1 * F.n2 * F.n3, BS)), dim3(BS), 0, m->stream,
-                       m->gate, C.u, C.uold, F.u, F.n1, F.n2, F.n3);
+    mb_prolong(m, C.u, C.uold, F.u, F.n1, F.n2, F.n3);
   } else {
     prolong(m, m->lev, level);
   }
